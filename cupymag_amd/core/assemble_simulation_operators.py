@@ -1,0 +1,82 @@
+"""All operators of the GSPM step (cupymag/core/assemble_simulation_operators.py:27-117).
+
+The reference assembles COO triplets per operator with Numba and converts
+with SciPy.  Here one call of the native host assembler builds every
+operator straight into CSR (in parallel over DOF rows, memory proportional
+to the final nnz), and the operators are uploaded once.  Fx, Fy, Fz are
+uploaded as three value sets of one device pattern so that the derivative
+kernels read the pattern once.
+"""
+import numpy as np
+
+from ..device import DeviceCSR, context
+from ..mesh.setup_FEM_mesh import FEMMesh
+from ..physics.host_ops import HostOps, voigt_C
+from ..utils.compute_derivatives import ComputeDerivatives, DerivView
+from ..utils.volume_average import VolumeAverage
+
+
+class SimulationOperators(FEMMesh):
+    def __init__(self, P, mesh_arrays=None, ctx=None, keep_host=None, nthreads=None):
+        super().__init__(P, mesh_arrays)
+        self.alpha1 = P.alpha
+        self.alpha2 = P.alpha * P.alpha_damp
+        self.ctx = ctx
+        self.keep_host = keep_host
+        self.nthreads = nthreads
+        self._dev = None
+
+    def _upload(self):
+        if self._dev is not None:
+            return self._dev
+        P = self.P
+        ctx = self.ctx or context()
+        Cv = voigt_C(P.c11, P.c12, P.c44, bool(P.rot111)) if P.ME else None
+        H = HostOps(self.node_coords_np, self.elements_np, self.global_id_np, self.n_dof, self.DefDOF,
+                    self.alpha1, self.alpha2, Cv, self.nthreads)
+        keep = self.keep_host if self.keep_host is not None else self.n_dof <= 300_000
+
+        def up(name, vals=None):
+            ip, ix, v, shape = H.csr(name)
+            if vals is not None:
+                v = vals
+            M = DeviceCSR(ip, ix, v, shape, ctx)
+            if not keep:
+                M._host = None
+            else:
+                M._host = (ip.copy(), ix.copy(), np.array(v, copy=True))
+            return M
+
+        d = {}
+        d["A_demag"] = up("A_demag")
+        F3v = np.stack([H.array("FX"), H.array("FY"), H.array("FZ")])
+        d["F3"] = up("Fx", F3v)
+        del F3v
+        for k in ("A1", "A2", "F1"):
+            d[k] = up(k)
+        if P.ME:
+            d["A_el"] = up("A_el")
+            d["F_el"] = up("F_el")
+        H.free()
+        self._dev = d
+        return d
+
+    def csr_demag_and_deriv(self):
+        d = self._upload()
+        Fx, Fy, Fz = (DerivView(d["F3"], k) for k in range(3))
+        return d["A_demag"], Fx, Fy, Fz, ComputeDerivatives(Fx, Fy, Fz)
+
+    def csr_GS(self):
+        d = self._upload()
+        return d["A1"], d["A2"], d["F1"]
+
+    def csr_elasticity(self):
+        if not self.P.ME:
+            return None, None
+        d = self._upload()
+        return d["A_el"], d["F_el"]
+
+    def volume_average(self):
+        if getattr(self, "_avg", None) is None:
+            self._avg = VolumeAverage(self.node_coords_np, self.elements_np, self.global_id_np)
+        return self._avg

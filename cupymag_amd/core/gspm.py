@@ -1,0 +1,113 @@
+"""The device-resident GSPM LLG step (Micromagnetics.py:110-307) behind one
+C call per step (cpm_sim_step).
+
+``GSPMSimulation`` owns the device state of one simulation: m, the CG warm
+starts of all ten solves, Htilde, E, U, u.  ``step`` returns the report
+the reference's loop needs on the host (sumsq, avg_m) plus the CG
+iteration count of every solve.
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from .. import _lib
+from .._lib import check, lib, ptr
+from ..utils.rot_111_matrices import get_M_matrix, get_R_matrix
+from ..utils.sigma_matrices import get_Ebar_sigma
+
+SYSTEMS = ("elasticity", "demag", "Gauss-Seidel g1n", "Gauss-Seidel g2n", "Gauss-Seidel g3n",
+           "Gauss-Seidel g1star", "Gauss-Seidel g2star", "Gauss-Seidel m1starstar",
+           "Gauss-Seidel m2starstar", "Gauss-Seidel m3starstar")
+
+
+class StepReport:
+    def __init__(self, rep):
+        self.sumsq = rep.sumsq
+        self.avg_m = np.array(rep.avg_m[:], dtype=np.float64)
+        self.iters = list(rep.iters)
+        self.failed = rep.failed
+        self.resid = rep.resid
+
+
+class GSPMSimulation:
+    def __init__(self, SimOp, P, mode="fp64"):
+        self.P = P
+        self.SimOp = SimOp
+        A_demag, Fx, Fy, Fz, self.Deriv = SimOp.csr_demag_and_deriv()
+        A1, A2, F1 = SimOp.csr_GS()
+        A_el, F_el = SimOp.csr_elasticity()
+        self.Avg = SimOp.volume_average()
+        F3 = Fx.base
+        self.ctx = F3.ctx
+        self.n = n = SimOp.n_dof
+        self._keep = (A_demag, F3, A1, A2, F1, A_el, F_el)
+        ops = _lib.cpm_sim_ops(A_demag.h, F3.h, A1.h, A2.h, F1.h, A_el.h if A_el is not None else None,
+                               F_el.h if F_el is not None else None)
+        pr = _lib.cpm_sim_params()
+        pr.n = n
+        pr.me = int(bool(P.ME))
+        pr.rot111 = int(bool(P.rot111))
+        pr.maxiter = int(P.maxiter)
+        pr.tol = float(P.tol)
+        pr.dt = float(P.dt)
+        pr.alpha_damp = float(P.alpha_damp)
+        pr.K1 = float(P.K1)
+        ec = (P.elasC1, P.elasC2, P.elasC3) if P.ME else (0.0, 0.0, 0.0)
+        for k in range(3):
+            pr.elasC[k] = float(ec[k])
+        pr.lambda100 = float(P.lambda100)
+        pr.lambda111 = float(P.lambda111)
+        for k, v in enumerate((P.N_x, P.N_y, P.N_z)):
+            pr.Nd[k] = float(v)
+        for k, v in enumerate(get_Ebar_sigma(P)):
+            pr.Ebar_sigma[k] = float(v)
+        R, M = get_R_matrix(), get_M_matrix()
+        MinvT = np.linalg.inv(M).T
+        for k, v in enumerate(R.ravel()):
+            pr.R[k] = float(v)
+        for k, v in enumerate(M.ravel()):
+            pr.Mv[k] = float(v)
+        for k, v in enumerate(MinvT.ravel()):
+            pr.MvinvT[k] = float(v)
+        pr.mode = {"fp64": _lib.MODE_FP64}[mode]
+        W = self.Avg.weights_device(self.ctx)
+        dmask = np.zeros(n, dtype=np.uint8)
+        dmask[np.asarray(SimOp.DefDOF, dtype=np.int64)] = 1
+        h = C.c_void_p()
+        check(lib().cpm_sim_create(self.ctx.h, C.byref(ops), C.byref(pr), ptr(W), float(self.Avg.volume),
+                                   ptr(dmask), C.byref(h)), "cpm_sim_create")
+        self.h = h
+        self.avg_m = None
+
+    def set_m(self, m):
+        from ..device import as_device
+        md = as_device(np.asarray(m, dtype=np.float64).reshape(-1, 3) if not isinstance(m, torch.Tensor) else m,
+                       self.ctx).reshape(-1, 3).contiguous()
+        out = np.zeros(3)
+        check(lib().cpm_sim_set_m(self.h, ptr(md), ptr(out)), "cpm_sim_set_m")
+        self.avg_m = out
+        return out
+
+    def step(self, H, use_init):
+        Hh = np.ascontiguousarray(H, dtype=np.float64)
+        rep = _lib.cpm_step_report()
+        info = check(lib().cpm_sim_step(self.h, ptr(Hh), int(bool(use_init)), C.byref(rep)), "cpm_sim_step")
+        r = StepReport(rep)
+        if info != 0:
+            sysname = SYSTEMS[rep.failed - 1] if rep.failed else "?"
+            raise RuntimeError(f"Error! CG for {sysname} did not converge. info={info}, residual={rep.resid}.")
+        self.avg_m = r.avg_m
+        return r
+
+    def get(self, field):
+        n = self.n
+        shape = {"M": (n, 3), "HTILDE": (3, n), "E": (n, 6), "U": (n,), "UEL": (3 * n,)}[field]
+        out = torch.empty(shape, dtype=torch.float64, device=self.ctx.device)
+        check(lib().cpm_sim_get(self.h, _lib.FIELD[field], ptr(out)), "cpm_sim_get")
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib._lib is not None:
+            _lib._lib.cpm_sim_destroy(self.h)
+            self.h = None

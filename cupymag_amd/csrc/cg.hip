@@ -1,0 +1,224 @@
+// CSR SpMV and the FP64 conjugate-gradient solver with CuPy's stopping rule.
+//
+// Replaces cuSPARSE `A @ x` and cupyx.scipy.sparse.linalg.cg as called by
+// solve_cg (cupymag/solvers/linear_solvers.py:59-62).  One CG iteration is
+// three kernels on the context stream:
+//   pupd : p = r (first iteration) or p = r + (rho/rho_old) p
+//   spmv : q = A p, fused p.q reduction; the last workgroup sets alpha
+//   xr   : x += alpha p, r -= alpha q, fused r.r reduction; the last
+//          workgroup advances the iteration count and tests ||r|| <= atol
+// The CG scalars live in device memory (CgState); every kernel returns
+// immediately once `done` is set, so the host enqueues batches of iterations
+// and reads the state back only between batches instead of once per
+// iteration.  All reductions are deterministic (fixed block order).
+#include <algorithm>
+#include <cmath>
+
+#include "device.h"
+
+namespace cpm {
+
+// y = A x with G lanes per row.
+template <int G>
+__global__ void __launch_bounds__(kBlock) k_spmv(int32_t nrows, const int32_t* __restrict__ ptr,
+                                                 const int32_t* __restrict__ idx, const double* __restrict__ val,
+                                                 const double* __restrict__ x, double* __restrict__ y) {
+  const int lane = threadIdx.x & (G - 1);
+  const int64_t stride = (int64_t)gridDim.x * (kBlock / G);
+  for (int64_t row = (int64_t)blockIdx.x * (kBlock / G) + threadIdx.x / G; row < nrows; row += stride) {
+    const int32_t b = ptr[row], e = ptr[row + 1];
+    double s = 0.0;
+    for (int32_t j = b + lane; j < e; j += G) s += val[j] * x[idx[j]];
+    s = group_sum<G>(s);
+    if (lane == 0) y[row] = s;
+  }
+}
+
+// q = A p and p.q; sets alpha = rho / (p.q).
+template <int G>
+__global__ void __launch_bounds__(kBlock) k_cg_spmv(int32_t nrows, const int32_t* __restrict__ ptr,
+                                                    const int32_t* __restrict__ idx, const double* __restrict__ val,
+                                                    const double* __restrict__ p, double* __restrict__ q,
+                                                    CgState* st, double* partials, unsigned* counter) {
+  if (st->done) return;
+  const int lane = threadIdx.x & (G - 1);
+  const int64_t stride = (int64_t)gridDim.x * (kBlock / G);
+  double acc[1] = {0.0};
+  for (int64_t row = (int64_t)blockIdx.x * (kBlock / G) + threadIdx.x / G; row < nrows; row += stride) {
+    const int32_t b = ptr[row], e = ptr[row + 1];
+    double s = 0.0;
+    for (int32_t j = b + lane; j < e; j += G) s += val[j] * p[idx[j]];
+    s = group_sum<G>(s);
+    if (lane == 0) {
+      q[row] = s;
+      acc[0] += p[row] * s;
+    }
+  }
+  if (grid_sum<1>(acc, partials, counter) && threadIdx.x == 0) {
+    st->pq = acc[0];
+    st->alpha = st->rho / acc[0];
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_cg_pupd(int64_t n, const double* __restrict__ r, double* __restrict__ p,
+                                                    const CgState* st) {
+  if (st->done) return;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  if (st->it == 0) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) p[i] = r[i];
+  } else {
+    const double beta = st->rho / st->rho_old;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) p[i] = r[i] + beta * p[i];
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_cg_xr(int64_t n, double* __restrict__ x, double* __restrict__ r,
+                                                  const double* __restrict__ p, const double* __restrict__ q,
+                                                  CgState* st, double* partials, unsigned* counter) {
+  if (st->done) return;
+  const double a = st->alpha;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  double acc[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    x[i] = x[i] + a * p[i];
+    const double ri = r[i] - a * q[i];
+    r[i] = ri;
+    acc[0] += ri * ri;
+  }
+  if (grid_sum<1>(acc, partials, counter) && threadIdx.x == 0) {
+    st->rho_old = st->rho;
+    st->rho = acc[0];
+    st->it += 1;
+    st->resid = sqrt(acc[0]);
+    if (st->resid <= st->atol) {
+      st->done = 1;
+    } else if (st->it >= st->maxiter) {
+      st->done = 1;
+      st->fail = 1;
+    }
+  }
+}
+
+// r = b - A x (use_x0, q holds A x) or x = 0, r = b; ||b||^2, r.r and atol.
+__global__ void __launch_bounds__(kBlock) k_cg_init(int64_t n, const double* __restrict__ b, double* __restrict__ x,
+                                                    double* __restrict__ r, const double* __restrict__ q, int use_x0,
+                                                    double tol, int32_t maxiter, CgState* st, double* partials,
+                                                    unsigned* counter) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  double acc[2] = {0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const double bi = b[i];
+    double ri;
+    if (use_x0) {
+      ri = bi - q[i];
+    } else {
+      x[i] = 0.0;
+      ri = bi;
+    }
+    r[i] = ri;
+    acc[0] += bi * bi;
+    acc[1] += ri * ri;
+  }
+  if (grid_sum<2>(acc, partials, counter) && threadIdx.x == 0) {
+    st->bnorm2 = acc[0];
+    st->rho = acc[1];
+    st->rho_old = 0.0;
+    st->tol = tol;
+    st->atol = tol * sqrt(acc[0]);
+    st->it = 0;
+    st->fail = 0;
+    st->maxiter = maxiter;
+    st->resid = sqrt(acc[1]);
+    // CuPy returns b itself when ||b|| == 0, and never iterates when maxiter <= 0
+    st->done = (acc[0] == 0.0 || maxiter <= 0) ? 1 : 0;
+    if (acc[0] != 0.0 && maxiter <= 0) st->fail = 0;
+  }
+}
+
+// When ||b|| == 0 the result is b (all zeros), also when an x0 was given.
+__global__ void __launch_bounds__(kBlock) k_cg_zero_b(int64_t n, double* __restrict__ x, const CgState* st) {
+  if (st->bnorm2 != 0.0) return;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) x[i] = 0.0;
+}
+
+template <int G>
+static void launch_spmv_g(const cpm_mat* A, const double* val, const double* x, double* y, hipStream_t s) {
+  const int grid = grid_for(A->nrows, kBlock / G);
+  k_spmv<G><<<grid, kBlock, 0, s>>>(A->nrows, A->ptr, A->idx, val, x, y);
+}
+
+template <int G>
+static void launch_cg_spmv_g(cpm_ctx* ctx, const cpm_mat* A, CgState* st) {
+  const int grid = grid_for(A->nrows, kBlock / G);
+  k_cg_spmv<G><<<grid, kBlock, 0, ctx->stream>>>(A->nrows, A->ptr, A->idx, A->val, ctx->p, ctx->q, st,
+                                                   ctx->partials, ctx->counter);
+}
+
+#define CPM_LANES_DISPATCH(L, F, ...) \
+  switch (L) {                        \
+    case 1: F<1>(__VA_ARGS__); break;   \
+    case 2: F<2>(__VA_ARGS__); break;   \
+    case 4: F<4>(__VA_ARGS__); break;   \
+    case 8: F<8>(__VA_ARGS__); break;   \
+    case 16: F<16>(__VA_ARGS__); break; \
+    case 32: F<32>(__VA_ARGS__); break; \
+    default: F<64>(__VA_ARGS__); break; \
+  }
+
+int launch_spmv(cpm_ctx* ctx, const cpm_mat* A, int valset, const double* x, double* y) {
+  if (A->nrows == 0) return CPM_OK;
+  const double* val = A->val + (int64_t)valset * A->nnz;
+  CPM_LANES_DISPATCH(A->lanes, launch_spmv_g, A, val, x, y, ctx->stream);
+  CPM_HIP(hipGetLastError());
+  return CPM_OK;
+}
+
+int run_cg(cpm_ctx* ctx, const cpm_mat* A, const double* b, double* x, int use_x0, double tol, int maxiter, int hint,
+           int32_t* iters, double* resid) {
+  const int64_t n = A->nrows;
+  if (A->ncols != A->nrows) return fail(CPM_EINVAL, "CG needs a square matrix");
+  if (int rc = ensure_workspace(ctx, n)) return rc;
+  CgState* st = ctx->cg;
+  hipStream_t s = ctx->stream;
+  const int gv = grid_for(n);
+  if (use_x0)
+    if (int rc = launch_spmv(ctx, A, 0, x, ctx->q)) return rc;
+  k_cg_init<<<gv, kBlock, 0, s>>>(n, b, x, ctx->r, ctx->q, use_x0, tol, maxiter, st, ctx->partials, ctx->counter);
+  k_cg_zero_b<<<gv, kBlock, 0, s>>>(n, x, st);
+  CPM_HIP(hipGetLastError());
+  CgState h{};
+  int batch = hint > 2 ? hint - 1 : 8;
+  int issued = 0;
+  for (;;) {
+    const int todo = std::max(1, std::min(batch, maxiter - issued));
+    for (int k = 0; k < todo; ++k) {
+      k_cg_pupd<<<gv, kBlock, 0, s>>>(n, ctx->r, ctx->p, st);
+      CPM_LANES_DISPATCH(A->lanes, launch_cg_spmv_g, ctx, A, st);
+      k_cg_xr<<<gv, kBlock, 0, s>>>(n, x, ctx->r, ctx->p, ctx->q, st, ctx->partials, ctx->counter);
+    }
+    issued += todo;
+    CPM_HIP(hipGetLastError());
+    if (int rc = read_cg_state(ctx, 0, &h)) return rc;
+    if (h.done || issued >= maxiter) break;
+    batch = std::max(2, std::min(64, h.it / 4 + 2));
+  }
+  if (iters) *iters = h.it;
+  if (resid) *resid = h.resid;
+  if (h.fail || (!h.done && h.it >= maxiter)) return h.it > 0 ? h.it : 1;
+  return CPM_OK;
+}
+
+}  // namespace cpm
+
+extern "C" int cpm_spmv(cpm_ctx* ctx, const cpm_mat* A, int valset, const double* x, double* y) {
+  if (!ctx || !A || !x || !y) return cpm::fail(CPM_EINVAL, "cpm_spmv: null argument");
+  if (valset < 0 || valset >= A->nvals) return cpm::fail(CPM_EINVAL, "cpm_spmv: valset out of range");
+  return cpm::launch_spmv(ctx, A, valset, x, y);
+}
+
+extern "C" int cpm_cg(cpm_ctx* ctx, const cpm_mat* A, const double* b, double* x, int use_x0, double tol, int maxiter,
+                      int32_t* iters, double* resid) {
+  if (!ctx || !A || !b || !x) return cpm::fail(CPM_EINVAL, "cpm_cg: null argument");
+  return cpm::run_cg(ctx, A, b, x, use_x0, tol, maxiter, 0, iters, resid);
+}

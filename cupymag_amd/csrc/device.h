@@ -1,0 +1,148 @@
+// Internal declarations shared by the HIP translation units of
+// libcupymag_hip.so: context / operator structs, launch shapes, and the
+// deterministic wave64 + LDS reductions used by every dot product.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "cupymag_hip.h"
+#include "errors.h"
+
+#define CPM_HIP(expr)                                                                      \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return cpm::fail(CPM_EHIP, std::string(#expr) + " failed: " + hipGetErrorString(e_)); \
+  } while (0)
+
+namespace cpm {
+
+constexpr int kBlock = 256;      // 4 wave64 per workgroup
+constexpr int kMaxGrid = 2048;   // 8 workgroups per CU on 256 CUs; grid-stride beyond
+constexpr int kMaxRed = 12;      // max values reduced together
+
+// Per-solve CG scalars, device resident.
+struct CgState {
+  double rho, rho_old, alpha, atol, bnorm2, resid, pq, tol;
+  int32_t it, done, fail, maxiter;
+};
+
+}  // namespace cpm
+
+struct cpm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  double* partials = nullptr;     // kMaxRed * kMaxGrid doubles
+  unsigned* counter = nullptr;    // grid-reduction arrival counter
+  double* scal = nullptr;         // device scalar scratch (256 doubles)
+  double* pinned = nullptr;       // pinned host mirror (256 doubles)
+  cpm::CgState* cg = nullptr;     // device CG state (16 slots)
+  // CG workspace (grown on demand)
+  double* r = nullptr;
+  double* p = nullptr;
+  double* q = nullptr;
+  int64_t ws_len = 0;
+  int ncu = 256;
+};
+
+struct cpm_mat {
+  cpm_ctx* ctx = nullptr;
+  int32_t nrows = 0, ncols = 0;
+  int64_t nnz = 0;
+  int nvals = 1;
+  int32_t* ptr = nullptr;
+  int32_t* idx = nullptr;
+  double* val = nullptr;   // nvals * nnz
+  int lanes = 8;           // threads per row of the vector-CSR kernels
+};
+
+namespace cpm {
+
+inline int grid_for(int64_t work_items, int items_per_block = kBlock) {
+  int64_t g = (work_items + items_per_block - 1) / items_per_block;
+  if (g < 1) g = 1;
+  if (g > kMaxGrid) g = kMaxGrid;
+  return (int)g;
+}
+
+int ensure_workspace(cpm_ctx* ctx, int64_t n);
+int read_cg_state(cpm_ctx* ctx, int slot, CgState* out);
+
+// y = A_valset x on the context stream (no sync).
+int launch_spmv(cpm_ctx* ctx, const cpm_mat* A, int valset, const double* x, double* y);
+// CG on the context stream; syncs to test convergence.  `hint` = expected
+// iteration count (0 = unknown), used only to size launch batches.
+int run_cg(cpm_ctx* ctx, const cpm_mat* A, const double* b, double* x, int use_x0, double tol, int maxiter,
+           int hint, int32_t* iters, double* resid);
+
+// ---- device reductions (blockDim.x == kBlock) -------------------------------
+
+template <int NV>
+__device__ inline void wave_sum(double (&v)[NV]) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] += __shfl_xor(v[k], o, 64);
+}
+
+// Sum over the workgroup; the result is valid in thread 0.
+template <int NV>
+__device__ inline void block_sum(double (&v)[NV], double* lds /* NV*4 */) {
+  wave_sum<NV>(v);
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) lds[k * 4 + wid] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = ((lds[k * 4 + 0] + lds[k * 4 + 1]) + lds[k * 4 + 2]) + lds[k * 4 + 3];
+  __syncthreads();
+}
+
+// Deterministic grid-wide sum: every workgroup stores its partial, the last
+// one to arrive adds the partials in block order.  Returns true in the last
+// workgroup, where thread 0 holds the totals in `v`.
+template <int NV>
+__device__ inline bool grid_sum(double (&v)[NV], double* partials, unsigned* counter) {
+  __shared__ double lds[NV * 4];
+  __shared__ int am_last;
+  block_sum<NV>(v, lds);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) partials[k * gridDim.x + blockIdx.x] = v[k];
+    __threadfence();
+    unsigned t = atomicAdd(counter, 1u);
+    am_last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!am_last) return false;
+  __threadfence();
+  double s[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) s[k] = 0.0;
+  for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) s[k] += __hip_atomic_load(&partials[k * gridDim.x + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  block_sum<NV>(s, lds);
+  if (threadIdx.x == 0) {
+    *counter = 0u;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = s[k];
+  }
+  return true;
+}
+
+// Row sum of a CSR row with G lanes (G a power of two <= 64), lanes of a
+// row contiguous within the wavefront; the sum is valid in every lane.
+template <int G>
+__device__ inline double group_sum(double s) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
+  return s;
+}
+
+}  // namespace cpm

@@ -1,0 +1,752 @@
+// Field operators and the device-resident GSPM LLG step (FP64).
+//
+// Restates cupymag/core/Micromagnetics.py:110-307 as a sequence of fused
+// HIP kernels on one stream.  The only host synchronisations are the CG
+// batch boundaries and the final report (sumsq, avg_m), which the reference
+// also pulls to the host every step (:299-309).
+//
+// Kernel map (reference line -> kernel):
+//   :112-121 m@R, build_E0_from_m, E0@M^T             -> k_e0
+//   :123     F_el @ E0                                -> launch_spmv
+//   :127-129 compute_E_from_u (6 or 9 SpMVs + mixing) -> k_strain (one pass
+//            over the shared Fx/Fy/Fz pattern, 9 row sums)
+//   :132,143 volume averages of E and m100            -> k_strain_avg
+//   :135-162 E -= avg, E += Ebar, E @ M^-T            -> k_efinal
+//   :166-168 -(Fx m~x + Fy m~y + Fz m~z), b[0] = 0    -> k_demag_rhs
+//   :171     Htilde = -F U                            -> k_hd
+//   :175-215 effective field, f*dt + m                -> k_field1
+//   :217-219,227,233,280-288 F1 @ .                   -> launch_spmv
+//   :225-227 / :231-233 m1*, m2* and their RHS        -> k_gs1 / k_gs2
+//   :237-278 m3*, stage-2 field, f*dt2 + m*           -> k_gs3_field2
+//   :292-306 normalise, defect -> 0, sumsq, avg_m     -> k_project
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "device.h"
+
+namespace cpm {
+
+struct FieldPrm {
+  double K1, c1, c2, c3, l100, dt, dt2;
+  double Nd[3], H[3];
+  double R[9];
+  int rot;
+};
+
+// Device scalar slots of a simulation (double offsets in sim->dscal).
+enum : int { S_AVGM = 0, S_AVGM100 = 3, S_AVGE = 6, S_EBAR = 12, S_SUMSQ = 18, S_NSCAL = 32 };
+
+__device__ inline void rot_apply(const double* R, double a0, double a1, double a2, double& b0, double& b1,
+                                 double& b2) {
+  // (a0,a1,a2) @ R  (row vector times R), in j order
+  b0 = a0 * R[0] + a1 * R[3] + a2 * R[6];
+  b1 = a0 * R[1] + a1 * R[4] + a2 * R[7];
+  b2 = a0 * R[2] + a1 * R[5] + a2 * R[8];
+}
+
+// ---- E0 = build_E0_from_m(m or m@R) [@ Mv^T] ------------------------------
+__global__ void __launch_bounds__(kBlock) k_e0(int64_t n, const double* __restrict__ m, double l100, double l111,
+                                               int rot, const double* __restrict__ R /*9, device*/,
+                                               const double* __restrict__ Mv /*36, device*/,
+                                               double* __restrict__ E0) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    double mx = m[3 * i], my = m[3 * i + 1], mz = m[3 * i + 2];
+    if (rot) {
+      double a, b, c;
+      rot_apply(R, mx, my, mz, a, b, c);
+      mx = a, my = b, mz = c;
+    }
+    const double t = 1.0 / 3.0;
+    double e[6];
+    e[0] = (1.5 * l100) * (mx * mx - t);
+    e[1] = (1.5 * l100) * (my * my - t);
+    e[2] = (1.5 * l100) * (mz * mz - t);
+    e[3] = 2.0 * ((1.5 * l111) * (mx * my));
+    e[4] = 2.0 * ((1.5 * l111) * (my * mz));
+    e[5] = 2.0 * ((1.5 * l111) * (mx * mz));
+    if (rot) {
+      double o[6];
+      for (int r = 0; r < 6; ++r) {
+        double s = 0.0;
+        for (int j = 0; j < 6; ++j) s += e[j] * Mv[r * 6 + j];
+        o[r] = s;
+      }
+      for (int r = 0; r < 6; ++r) e[r] = o[r];
+    }
+    for (int v = 0; v < 6; ++v) E0[6 * i + v] = e[v];
+  }
+}
+
+// ---- strain: 9 weak-form derivatives of u on the Fx/Fy/Fz pattern ------------
+template <int G>
+__global__ void __launch_bounds__(kBlock) k_strain(int32_t n, const int32_t* __restrict__ ptr,
+                                                   const int32_t* __restrict__ idx, const double* __restrict__ val,
+                                                   int64_t nnz, const double* __restrict__ u, int rot,
+                                                   const double* __restrict__ R, double* __restrict__ E) {
+  const int lane = threadIdx.x & (G - 1);
+  const int64_t stride = (int64_t)gridDim.x * (kBlock / G);
+  const double* Fx = val;
+  const double* Fy = val + nnz;
+  const double* Fz = val + 2 * nnz;
+  for (int64_t row = (int64_t)blockIdx.x * (kBlock / G) + threadIdx.x / G; row < n; row += stride) {
+    double d[3][3] = {};  // d[k][j] = (D_k u_j)_row
+    for (int32_t p = ptr[row] + lane; p < ptr[row + 1]; p += G) {
+      const int32_t c = idx[p];
+      const double uj[3] = {u[c], u[n + c], u[2 * (int64_t)n + c]};
+      const double f[3] = {Fx[p], Fy[p], Fz[p]};
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) d[k][j] += f[k] * uj[j];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) d[k][j] = group_sum<G>(d[k][j]);
+    if (lane == 0) {
+      double e[6];
+      if (!rot) {
+        e[0] = d[0][0];
+        e[1] = d[1][1];
+        e[2] = d[2][2];
+        e[3] = d[1][0] + d[0][1];
+        e[4] = d[2][1] + d[1][2];
+        e[5] = d[2][0] + d[0][2];
+      } else {
+        // compute_derivatives.py:94-123; d[k][j]: derivative k of component j
+        auto mix = [&](int col, int j) { return R[0 * 3 + col] * d[0][j] + R[1 * 3 + col] * d[1][j] + R[2 * 3 + col] * d[2][j]; };
+        e[0] = mix(0, 0);
+        e[1] = mix(1, 1);
+        e[2] = mix(2, 2);
+        e[3] = mix(1, 0) + mix(0, 1);
+        e[4] = mix(2, 1) + mix(1, 2);
+        e[5] = mix(2, 0) + mix(0, 2);
+      }
+#pragma unroll
+      for (int v = 0; v < 6; ++v) E[6 * row + v] = e[v];
+    }
+  }
+}
+
+// ---- volume averages of E (6) and m@R (3); Ebar from the averaged m --------
+__global__ void __launch_bounds__(kBlock) k_strain_avg(int64_t n, const double* __restrict__ W, double vol,
+                                                       const double* __restrict__ E, const double* __restrict__ m,
+                                                       int rot, const double* __restrict__ R, double l100,
+                                                       double l111, const double* __restrict__ Es /*6*/,
+                                                       const double* __restrict__ Mv, double* dscal, double* partials,
+                                                       unsigned* counter) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  double acc[9] = {};
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const double w = W[i];
+#pragma unroll
+    for (int v = 0; v < 6; ++v) acc[v] += w * E[6 * i + v];
+    if (rot) {
+      double a, b, c;
+      rot_apply(R, m[3 * i], m[3 * i + 1], m[3 * i + 2], a, b, c);
+      acc[6] += w * a;
+      acc[7] += w * b;
+      acc[8] += w * c;
+    }
+  }
+  if (grid_sum<9>(acc, partials, counter) && threadIdx.x == 0) {
+    for (int v = 0; v < 6; ++v) dscal[S_AVGE + v] = acc[v] / vol;
+    double am[3];
+    for (int k = 0; k < 3; ++k) {
+      if (rot) dscal[S_AVGM100 + k] = acc[6 + k] / vol;
+      am[k] = rot ? dscal[S_AVGM100 + k] : dscal[S_AVGM + k];
+    }
+    // Micromagnetics.py:135-151
+    double eb[6];
+    eb[0] = 1.5 * l100 * (am[0] * am[0] - 1.0 / 3.0) + Es[0];
+    eb[1] = 1.5 * l100 * (am[1] * am[1] - 1.0 / 3.0) + Es[1];
+    eb[2] = 1.5 * l100 * (am[2] * am[2] - 1.0 / 3.0) + Es[2];
+    eb[3] = 3.0 * l111 * am[0] * am[1] + Es[3];
+    eb[4] = 3.0 * l111 * am[1] * am[2] + Es[4];
+    eb[5] = 3.0 * l111 * am[0] * am[2] + Es[5];
+    if (rot) {
+      double o[6];
+      for (int r = 0; r < 6; ++r) {
+        double s = 0.0;
+        for (int j = 0; j < 6; ++j) s += Mv[r * 6 + j] * eb[j];
+        o[r] = s;
+      }
+      for (int r = 0; r < 6; ++r) eb[r] = o[r];
+    }
+    for (int v = 0; v < 6; ++v) dscal[S_EBAR + v] = eb[v];
+  }
+}
+
+// E = (E - avg(E)) + Ebar, then E @ M^-T when rot111 (Micromagnetics.py:132,154-162)
+__global__ void __launch_bounds__(kBlock) k_efinal(int64_t n, double* __restrict__ E, int rot,
+                                                   const double* __restrict__ MvinvT, const double* __restrict__ dscal) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  double av[6], eb[6];
+  for (int v = 0; v < 6; ++v) {
+    av[v] = dscal[S_AVGE + v];
+    eb[v] = dscal[S_EBAR + v];
+  }
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    double e[6];
+#pragma unroll
+    for (int v = 0; v < 6; ++v) e[v] = (E[6 * i + v] - av[v]) + eb[v];
+    if (rot) {
+      double o[6];
+      for (int c = 0; c < 6; ++c) {
+        double s = 0.0;
+        for (int k = 0; k < 6; ++k) s += e[k] * MvinvT[k * 6 + c];
+        o[c] = s;
+      }
+      for (int c = 0; c < 6; ++c) e[c] = o[c];
+    }
+#pragma unroll
+    for (int v = 0; v < 6; ++v) E[6 * i + v] = e[v];
+  }
+}
+
+// ---- demag RHS: b = -(Fx m~x + Fy m~y + Fz m~z), b[0] = 0 ------------------
+template <int G>
+__global__ void __launch_bounds__(kBlock) k_demag_rhs(int32_t n, const int32_t* __restrict__ ptr,
+                                                      const int32_t* __restrict__ idx, const double* __restrict__ val,
+                                                      int64_t nnz, const double* __restrict__ m,
+                                                      const double* __restrict__ avg, double* __restrict__ b) {
+  const int lane = threadIdx.x & (G - 1);
+  const int64_t stride = (int64_t)gridDim.x * (kBlock / G);
+  const double a0 = avg[0], a1 = avg[1], a2 = avg[2];
+  for (int64_t row = (int64_t)blockIdx.x * (kBlock / G) + threadIdx.x / G; row < n; row += stride) {
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int32_t p = ptr[row] + lane; p < ptr[row + 1]; p += G) {
+      const int64_t c = idx[p];
+      sx += val[p] * (m[3 * c] - a0);
+      sy += val[nnz + p] * (m[3 * c + 1] - a1);
+      sz += val[2 * nnz + p] * (m[3 * c + 2] - a2);
+    }
+    sx = group_sum<G>(sx);
+    sy = group_sum<G>(sy);
+    sz = group_sum<G>(sz);
+    if (lane == 0) b[row] = (row == 0) ? 0.0 : -((sx + sy) + sz);
+  }
+}
+
+// ---- Htilde_k = -F_k U ------------------------------------------------------
+template <int G>
+__global__ void __launch_bounds__(kBlock) k_hd(int32_t n, const int32_t* __restrict__ ptr,
+                                               const int32_t* __restrict__ idx, const double* __restrict__ val,
+                                               int64_t nnz, const double* __restrict__ U, double* __restrict__ H) {
+  const int lane = threadIdx.x & (G - 1);
+  const int64_t stride = (int64_t)gridDim.x * (kBlock / G);
+  for (int64_t row = (int64_t)blockIdx.x * (kBlock / G) + threadIdx.x / G; row < n; row += stride) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int32_t p = ptr[row] + lane; p < ptr[row + 1]; p += G) {
+      const double uc = U[idx[p]];
+      s0 += val[p] * uc;
+      s1 += val[nnz + p] * uc;
+      s2 += val[2 * nnz + p] * uc;
+    }
+    s0 = group_sum<G>(s0);
+    s1 = group_sum<G>(s1);
+    s2 = group_sum<G>(s2);
+    if (lane == 0) {
+      H[row] = -s0;
+      H[n + row] = -s1;
+      H[2 * (int64_t)n + row] = -s2;
+    }
+  }
+}
+
+// Effective field at one node (Micromagnetics.py:181-211 / :246-274), field
+// terms included, in the reference's left-to-right order.
+__device__ inline void field_at(const FieldPrm& P, double m1, double m2, double m3, const double* e /*6*/,
+                                const double Ht[3], const double Hb[3], double f[3]) {
+  if (!P.rot) {
+    const double s1 = m1 * m1, s2 = m2 * m2, s3 = m3 * m3;
+    f[0] = -2.0 * P.K1 * m1 * (s2 + s3) + 0.5 * Hb[0] + 0.5 * Ht[0] + P.H[0] - 2.0 * P.c1 * m1 * (s2 + s3) -
+           2.0 * P.c2 * (e[0] - P.l100) * m1 - P.c3 * (e[3] * m2 + e[5] * m3);
+    f[1] = -2.0 * P.K1 * m2 * (s3 + s1) + 0.5 * Hb[1] + 0.5 * Ht[1] + P.H[1] - 2.0 * P.c1 * m2 * (s3 + s1) -
+           2.0 * P.c2 * (e[1] - P.l100) * m2 - P.c3 * (e[4] * m3 + e[3] * m1);
+    f[2] = -2.0 * P.K1 * m3 * (s1 + s2) + 0.5 * Hb[2] + 0.5 * Ht[2] + P.H[2] - 2.0 * P.c1 * m3 * (s1 + s2) -
+           2.0 * P.c2 * (e[2] - P.l100) * m3 - P.c3 * (e[5] * m1 + e[4] * m2);
+  } else {
+    double a1, a2, a3;
+    rot_apply(P.R, m1, m2, m3, a1, a2, a3);
+    const double s1 = a1 * a1, s2 = a2 * a2, s3 = a3 * a3;
+    const double v1 = -2.0 * P.K1 * a1 * (s2 + s3) - 2.0 * P.c1 * a1 * (s2 + s3) -
+                      2.0 * P.c2 * (e[0] - P.l100) * a1 - P.c3 * (e[3] * a2 + e[5] * a3);
+    const double v2 = -2.0 * P.K1 * a2 * (s3 + s1) - 2.0 * P.c1 * a2 * (s3 + s1) -
+                      2.0 * P.c2 * (e[1] - P.l100) * a2 - P.c3 * (e[4] * a3 + e[3] * a1);
+    const double v3 = -2.0 * P.K1 * a3 * (s1 + s2) - 2.0 * P.c1 * a3 * (s1 + s2) -
+                      2.0 * P.c2 * (e[2] - P.l100) * a3 - P.c3 * (e[5] * a1 + e[4] * a2);
+    // v @ R^T: out_k = sum_j v_j R[k][j]
+    const double* R = P.R;
+    const double o1 = v1 * R[0] + v2 * R[1] + v3 * R[2];
+    const double o2 = v1 * R[3] + v2 * R[4] + v3 * R[5];
+    const double o3 = v1 * R[6] + v2 * R[7] + v3 * R[8];
+    f[0] = o1 + 0.5 * Hb[0] + 0.5 * Ht[0] + P.H[0];
+    f[1] = o2 + 0.5 * Hb[1] + 0.5 * Ht[1] + P.H[1];
+    f[2] = o3 + 0.5 * Hb[2] + 0.5 * Ht[2] + P.H[2];
+  }
+}
+
+__device__ inline void hbar_of(const FieldPrm& P, const double* avg, double Hb[3]) {
+  Hb[0] = -P.Nd[0] * avg[0];
+  Hb[1] = -P.Nd[1] * avg[1];
+  Hb[2] = -P.Nd[2] * avg[2];
+}
+
+// stage 1: f (3,n) and ftmp = f*dt + m (3,n)
+__global__ void __launch_bounds__(kBlock) k_field1(int64_t n, FieldPrm P, const double* __restrict__ m,
+                                                   const double* __restrict__ E, int me, const double* __restrict__ Ht,
+                                                   const double* __restrict__ avg, double* __restrict__ f,
+                                                   double* __restrict__ ftmp) {
+  double Hb[3];
+  hbar_of(P, avg, Hb);
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    double e[6] = {0, 0, 0, 0, 0, 0};
+    if (me)
+      for (int v = 0; v < 6; ++v) e[v] = E[6 * i + v];
+    const double ht[3] = {Ht[i], Ht[n + i], Ht[2 * n + i]};
+    const double m1 = m[3 * i], m2 = m[3 * i + 1], m3 = m[3 * i + 2];
+    double fv[3];
+    field_at(P, m1, m2, m3, e, ht, Hb, fv);
+    f[i] = fv[0];
+    f[n + i] = fv[1];
+    f[2 * n + i] = fv[2];
+    ftmp[i] = fv[0] * P.dt + m1;
+    ftmp[n + i] = fv[1] * P.dt + m2;
+    ftmp[2 * n + i] = fv[2] * P.dt + m3;
+  }
+}
+
+// m1* = m1 + g2n m3 - g3n m2 ; rhs = f1*dt + m1*
+__global__ void __launch_bounds__(kBlock) k_gs1(int64_t n, double dt, const double* __restrict__ m,
+                                                const double* __restrict__ gn, const double* __restrict__ f,
+                                                double* __restrict__ mstar, double* __restrict__ rhs) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const double m1s = m[3 * i] + gn[n + i] * m[3 * i + 2] - gn[2 * n + i] * m[3 * i + 1];
+    mstar[i] = m1s;
+    rhs[i] = f[i] * dt + m1s;
+  }
+}
+
+// m2* = m2 + g3n m1* - g1* m3 ; rhs = f2*dt + m2*
+__global__ void __launch_bounds__(kBlock) k_gs2(int64_t n, double dt, const double* __restrict__ m,
+                                                const double* __restrict__ gn, const double* __restrict__ g1s,
+                                                const double* __restrict__ f, double* __restrict__ mstar,
+                                                double* __restrict__ rhs) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const double m2s = m[3 * i + 1] + gn[2 * n + i] * mstar[i] - g1s[i] * m[3 * i + 2];
+    mstar[n + i] = m2s;
+    rhs[i] = f[n + i] * dt + m2s;
+  }
+}
+
+// m3* = m3 + g1* m2* - g2* m1* ; stage-2 RHS f(m*)*dt2 + m* (3,n)
+__global__ void __launch_bounds__(kBlock) k_gs3_field2(int64_t n, FieldPrm P, const double* __restrict__ m,
+                                                       const double* __restrict__ gs, double* __restrict__ mstar,
+                                                       const double* __restrict__ E, int me,
+                                                       const double* __restrict__ Ht, const double* __restrict__ avg,
+                                                       double* __restrict__ ftmp) {
+  double Hb[3];
+  hbar_of(P, avg, Hb);
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const double m1s = mstar[i], m2s = mstar[n + i];
+    const double m3s = m[3 * i + 2] + gs[i] * m2s - gs[n + i] * m1s;
+    mstar[2 * n + i] = m3s;
+    double e[6] = {0, 0, 0, 0, 0, 0};
+    if (me)
+      for (int v = 0; v < 6; ++v) e[v] = E[6 * i + v];
+    const double ht[3] = {Ht[i], Ht[n + i], Ht[2 * n + i]};
+    double fv[3];
+    field_at(P, m1s, m2s, m3s, e, ht, Hb, fv);
+    ftmp[i] = fv[0] * P.dt2 + m1s;
+    ftmp[n + i] = fv[1] * P.dt2 + m2s;
+    ftmp[2 * n + i] = fv[2] * P.dt2 + m3s;
+  }
+}
+
+// normalise m**, zero defects, sumsq = sum (m - m_prev)^2, m_prev = m, avg_m
+__global__ void __launch_bounds__(kBlock) k_project(int64_t n, const double* __restrict__ mss,
+                                                    const uint8_t* __restrict__ defect, const double* __restrict__ W,
+                                                    double vol, double* __restrict__ m, double* __restrict__ mprev,
+                                                    double* dscal, double* partials, unsigned* counter) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const double a = mss[i], b = mss[n + i], c = mss[2 * n + i];
+    const double mag = sqrt(a * a + b * b + c * c);
+    double o[3];
+    if (fabs(mag) < 1e-14 || defect[i]) {
+      o[0] = o[1] = o[2] = 0.0;
+    } else {
+      o[0] = a / mag;
+      o[1] = b / mag;
+      o[2] = c / mag;
+    }
+    const double w = W[i];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double d = o[k] - mprev[3 * i + k];
+      acc[0] += d * d;
+      m[3 * i + k] = o[k];
+      mprev[3 * i + k] = o[k];
+      acc[1 + k] += w * o[k];
+    }
+  }
+  if (grid_sum<4>(acc, partials, counter) && threadIdx.x == 0) {
+    dscal[S_SUMSQ] = acc[0];
+    for (int k = 0; k < 3; ++k) dscal[S_AVGM + k] = acc[1 + k] / vol;
+  }
+}
+
+// m[defect] = 0, m_prev = m, avg_m (initialize_m + Micromagnetics.py:72)
+__global__ void __launch_bounds__(kBlock) k_setm(int64_t n, const double* __restrict__ src,
+                                                 const uint8_t* __restrict__ defect, const double* __restrict__ W,
+                                                 double vol, double* __restrict__ m, double* __restrict__ mprev,
+                                                 double* dscal, double* partials, unsigned* counter) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const double w = W[i];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double v = defect[i] ? 0.0 : src[3 * i + k];
+      m[3 * i + k] = v;
+      mprev[3 * i + k] = v;
+      acc[k] += w * v;
+    }
+  }
+  if (grid_sum<3>(acc, partials, counter) && threadIdx.x == 0)
+    for (int k = 0; k < 3; ++k) dscal[S_AVGM + k] = acc[k] / vol;
+}
+
+// out[c] = sum_i W[i] f[i*nc + c]
+template <int NC>
+__global__ void __launch_bounds__(kBlock) k_wsum(int64_t n, const double* __restrict__ W, const double* __restrict__ f,
+                                                 double* out, double* partials, unsigned* counter) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  double acc[NC] = {};
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const double w = W[i];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc[c] += w * f[i * NC + c];
+  }
+  if (grid_sum<NC>(acc, partials, counter) && threadIdx.x == 0)
+#pragma unroll
+    for (int c = 0; c < NC; ++c) out[c] = acc[c];
+}
+
+#define CPM_G_DISPATCH(L, KERNEL, GRIDROWS, S, ...)                                         \
+  switch (L) {                                                                                \
+    case 1: KERNEL<1><<<grid_for(GRIDROWS, kBlock / 1), kBlock, 0, S>>>(__VA_ARGS__); break;    \
+    case 2: KERNEL<2><<<grid_for(GRIDROWS, kBlock / 2), kBlock, 0, S>>>(__VA_ARGS__); break;    \
+    case 4: KERNEL<4><<<grid_for(GRIDROWS, kBlock / 4), kBlock, 0, S>>>(__VA_ARGS__); break;    \
+    case 8: KERNEL<8><<<grid_for(GRIDROWS, kBlock / 8), kBlock, 0, S>>>(__VA_ARGS__); break;    \
+    case 16: KERNEL<16><<<grid_for(GRIDROWS, kBlock / 16), kBlock, 0, S>>>(__VA_ARGS__); break; \
+    case 32: KERNEL<32><<<grid_for(GRIDROWS, kBlock / 32), kBlock, 0, S>>>(__VA_ARGS__); break; \
+    default: KERNEL<64><<<grid_for(GRIDROWS, kBlock / 64), kBlock, 0, S>>>(__VA_ARGS__); break; \
+  }
+
+static int check_f3(const cpm_mat* F3) {
+  if (!F3 || F3->nvals != 3 || F3->nrows != F3->ncols)
+    return fail(CPM_EINVAL, "derivative operator must be square with 3 value sets (Fx, Fy, Fz)");
+  return CPM_OK;
+}
+
+}  // namespace cpm
+
+using namespace cpm;
+
+struct cpm_sim {
+  cpm_ctx* ctx = nullptr;
+  cpm_sim_ops ops{};
+  cpm_sim_params prm{};
+  double vol = 1.0;
+  int64_t n = 0;
+  double* W = nullptr;
+  uint8_t* defect = nullptr;
+  double* consts = nullptr;  // R(9) Mv(36) MvinvT(36) Ebar_sigma(6)
+  double* dscal = nullptr;
+  double *m = nullptr, *mprev = nullptr, *f = nullptr, *ftmp = nullptr, *b = nullptr, *gn = nullptr, *gs = nullptr;
+  double *mstar = nullptr, *mss = nullptr, *U = nullptr, *Ht = nullptr, *E = nullptr, *bd = nullptr;
+  double *E0 = nullptr, *bel = nullptr, *u = nullptr;
+  std::vector<void*> allocs;
+  int hint[10] = {};
+};
+
+static int sim_alloc(cpm_sim* S, double** p, int64_t count) {
+  hipError_t e = hipMalloc(p, sizeof(double) * std::max<int64_t>(count, 1));
+  if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? CPM_ENOMEM : CPM_EHIP, hipGetErrorString(e));
+  S->allocs.push_back(*p);
+  e = hipMemsetAsync(*p, 0, sizeof(double) * std::max<int64_t>(count, 1), S->ctx->stream);
+  if (e != hipSuccess) return fail(CPM_EHIP, hipGetErrorString(e));
+  return CPM_OK;
+}
+
+extern "C" int cpm_sim_destroy(cpm_sim* S) {
+  if (!S) return CPM_OK;
+  if (S->ctx) {
+    hipSetDevice(S->ctx->device);
+    hipStreamSynchronize(S->ctx->stream);
+  }
+  for (void* p : S->allocs) hipFree(p);
+  delete S;
+  return CPM_OK;
+}
+
+extern "C" int cpm_sim_create(cpm_ctx* ctx, const cpm_sim_ops* ops, const cpm_sim_params* prm, const double* W,
+                              double volume, const uint8_t* defect, cpm_sim** out) {
+  if (!ctx || !ops || !prm || !W || !out) return fail(CPM_EINVAL, "cpm_sim_create: null argument");
+  const int64_t n = prm->n;
+  if (n <= 0) return fail(CPM_EINVAL, "cpm_sim_create: n must be positive");
+  if (prm->mode != CPM_MODE_FP64) return fail(CPM_EINVAL, "cpm_sim_create: unknown mode");
+  auto sq = [&](const cpm_mat* A, int64_t rows) { return A && A->nrows == rows && A->ncols == rows; };
+  if (!sq(ops->A_demag, n) || !sq(ops->A1, n) || !sq(ops->A2, n) || !sq(ops->F1, n))
+    return fail(CPM_EINVAL, "cpm_sim_create: scalar operators must be n x n");
+  if (int rc = check_f3(ops->F3)) return rc;
+  if (ops->F3->nrows != n) return fail(CPM_EINVAL, "cpm_sim_create: F3 must be n x n");
+  if (prm->me) {
+    if (!sq(ops->A_el, 3 * n)) return fail(CPM_EINVAL, "cpm_sim_create: A_el must be 3n x 3n");
+    if (!ops->F_el || ops->F_el->nrows != 3 * n || ops->F_el->ncols != 6 * n)
+      return fail(CPM_EINVAL, "cpm_sim_create: F_el must be 3n x 6n");
+  }
+  CPM_HIP(hipSetDevice(ctx->device));
+  auto* S = new cpm_sim();
+  S->ctx = ctx;
+  S->ops = *ops;
+  S->prm = *prm;
+  S->vol = volume;
+  S->n = n;
+  int rc = CPM_OK;
+  auto A = [&](double** p, int64_t cnt) {
+    if (rc == CPM_OK) rc = sim_alloc(S, p, cnt);
+  };
+  A(&S->W, n);
+  A(&S->consts, 96);
+  A(&S->dscal, S_NSCAL);
+  A(&S->m, 3 * n);
+  A(&S->mprev, 3 * n);
+  A(&S->f, 3 * n);
+  A(&S->ftmp, 3 * n);
+  A(&S->b, 3 * n);
+  A(&S->gn, 3 * n);
+  A(&S->gs, 2 * n);
+  A(&S->mstar, 3 * n);
+  A(&S->mss, 3 * n);
+  A(&S->U, n);
+  A(&S->Ht, 3 * n);
+  A(&S->E, 6 * n);
+  A(&S->bd, n);
+  if (prm->me) {
+    A(&S->E0, 6 * n);
+    A(&S->bel, 3 * n);
+    A(&S->u, 3 * n);
+  }
+  if (rc == CPM_OK) {
+    hipError_t e = hipMalloc(&S->defect, std::max<int64_t>(n, 1));
+    if (e != hipSuccess) rc = fail(CPM_ENOMEM, "defect mask allocation");
+    else S->allocs.push_back(S->defect);
+  }
+  if (rc == CPM_OK) rc = ensure_workspace(ctx, prm->me ? 3 * n : n);
+  if (rc != CPM_OK) {
+    cpm_sim_destroy(S);
+    return rc;
+  }
+  std::vector<uint8_t> dm(n, 0);
+  if (defect) std::memcpy(dm.data(), defect, n);
+  std::vector<double> cst(96, 0.0);
+  std::memcpy(cst.data(), prm->R, 9 * sizeof(double));
+  std::memcpy(cst.data() + 9, prm->Mv, 36 * sizeof(double));
+  std::memcpy(cst.data() + 45, prm->MvinvT, 36 * sizeof(double));
+  std::memcpy(cst.data() + 81, prm->Ebar_sigma, 6 * sizeof(double));
+  hipError_t e = hipMemcpyAsync(S->defect, dm.data(), n, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(S->consts, cst.data(), 96 * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(S->W, W, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) {
+    cpm_sim_destroy(S);
+    return fail(CPM_EHIP, std::string("cpm_sim_create upload: ") + hipGetErrorString(e));
+  }
+  *out = S;
+  return CPM_OK;
+}
+
+extern "C" int cpm_sim_set_m(cpm_sim* S, const double* m, double* avg_m) {
+  if (!S || !m) return fail(CPM_EINVAL, "cpm_sim_set_m: null argument");
+  cpm_ctx* ctx = S->ctx;
+  k_setm<<<grid_for(S->n), kBlock, 0, ctx->stream>>>(S->n, m, S->defect, S->W, S->vol, S->m, S->mprev, S->dscal,
+                                                      ctx->partials, ctx->counter);
+  CPM_HIP(hipGetLastError());
+  CPM_HIP(hipMemcpyAsync(ctx->pinned, S->dscal + S_AVGM, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  CPM_HIP(hipStreamSynchronize(ctx->stream));
+  if (avg_m) std::memcpy(avg_m, ctx->pinned, 3 * sizeof(double));
+  return CPM_OK;
+}
+
+extern "C" int cpm_sim_get(cpm_sim* S, int field, double* dst) {
+  if (!S || !dst) return fail(CPM_EINVAL, "cpm_sim_get: null argument");
+  const double* src = nullptr;
+  int64_t cnt = 0;
+  switch (field) {
+    case CPM_FIELD_M: src = S->m, cnt = 3 * S->n; break;
+    case CPM_FIELD_HTILDE: src = S->Ht, cnt = 3 * S->n; break;
+    case CPM_FIELD_E: src = S->E, cnt = 6 * S->n; break;
+    case CPM_FIELD_U: src = S->U, cnt = S->n; break;
+    case CPM_FIELD_UEL:
+      if (!S->u) return fail(CPM_EINVAL, "cpm_sim_get: no displacement without magnetoelastic coupling");
+      src = S->u, cnt = 3 * S->n;
+      break;
+    default: return fail(CPM_EINVAL, "cpm_sim_get: unknown field");
+  }
+  CPM_HIP(hipMemcpyAsync(dst, src, cnt * sizeof(double), hipMemcpyDeviceToDevice, S->ctx->stream));
+  CPM_HIP(hipStreamSynchronize(S->ctx->stream));
+  return CPM_OK;
+}
+
+extern "C" int cpm_sim_step(cpm_sim* S, const double* H, int use_init, cpm_step_report* rep) {
+  if (!S || !H || !rep) return fail(CPM_EINVAL, "cpm_sim_step: null argument");
+  cpm_ctx* ctx = S->ctx;
+  hipStream_t s = ctx->stream;
+  const cpm_sim_params& p = S->prm;
+  const int64_t n = S->n;
+  std::memset(rep, 0, sizeof(*rep));
+  FieldPrm P{};
+  P.K1 = p.K1;
+  P.c1 = p.elasC[0];
+  P.c2 = p.elasC[1];
+  P.c3 = p.elasC[2];
+  P.l100 = p.lambda100;
+  P.dt = p.dt;
+  P.dt2 = p.dt * p.alpha_damp;
+  for (int k = 0; k < 3; ++k) {
+    P.Nd[k] = p.Nd[k];
+    P.H[k] = H[k];
+  }
+  std::memcpy(P.R, p.R, sizeof(P.R));
+  P.rot = p.rot111;
+  const double* dR = S->consts;
+  const double* dMv = S->consts + 9;
+  const double* dMvinvT = S->consts + 45;
+  const double* dEs = S->consts + 81;
+  const int gv = grid_for(n);
+  auto solve = [&](int slot, const cpm_mat* A, const double* b, double* x) -> int {
+    int32_t it = 0;
+    double res = 0.0;
+    int rc = run_cg(ctx, A, b, x, use_init, p.tol, p.maxiter, S->hint[slot], &it, &res);
+    rep->iters[slot] = it;
+    S->hint[slot] = it;
+    if (rc > 0) {
+      rep->failed = slot + 1;
+      rep->resid = res;
+    }
+    return rc;
+  };
+  int rc;
+  const cpm_mat* F3 = S->ops.F3;
+  if (p.me) {
+    k_e0<<<gv, kBlock, 0, s>>>(n, S->m, p.lambda100, p.lambda111, p.rot111, dR, dMv, S->E0);
+    if ((rc = launch_spmv(ctx, S->ops.F_el, 0, S->E0, S->bel))) return rc;
+    if ((rc = solve(0, S->ops.A_el, S->bel, S->u))) return rc;
+    CPM_G_DISPATCH(F3->lanes, k_strain, n, s, (int32_t)n, F3->ptr, F3->idx, F3->val, F3->nnz, S->u, p.rot111, dR,
+                   S->E);
+    k_strain_avg<<<gv, kBlock, 0, s>>>(n, S->W, S->vol, S->E, S->m, p.rot111, dR, p.lambda100, p.lambda111, dEs, dMv,
+                                       S->dscal, ctx->partials, ctx->counter);
+    k_efinal<<<gv, kBlock, 0, s>>>(n, S->E, p.rot111, dMvinvT, S->dscal);
+  }
+  CPM_G_DISPATCH(F3->lanes, k_demag_rhs, n, s, (int32_t)n, F3->ptr, F3->idx, F3->val, F3->nnz, S->m,
+                 S->dscal + S_AVGM, S->bd);
+  if ((rc = solve(1, S->ops.A_demag, S->bd, S->U))) return rc;
+  CPM_G_DISPATCH(F3->lanes, k_hd, n, s, (int32_t)n, F3->ptr, F3->idx, F3->val, F3->nnz, S->U, S->Ht);
+  k_field1<<<gv, kBlock, 0, s>>>(n, P, S->m, S->E, p.me, S->Ht, S->dscal + S_AVGM, S->f, S->ftmp);
+  for (int c = 0; c < 3; ++c)
+    if ((rc = launch_spmv(ctx, S->ops.F1, 0, S->ftmp + c * n, S->b + c * n))) return rc;
+  for (int c = 0; c < 3; ++c)
+    if ((rc = solve(2 + c, S->ops.A1, S->b + c * n, S->gn + c * n))) return rc;
+  k_gs1<<<gv, kBlock, 0, s>>>(n, p.dt, S->m, S->gn, S->f, S->mstar, S->ftmp);
+  if ((rc = launch_spmv(ctx, S->ops.F1, 0, S->ftmp, S->b))) return rc;
+  if ((rc = solve(5, S->ops.A1, S->b, S->gs))) return rc;
+  k_gs2<<<gv, kBlock, 0, s>>>(n, p.dt, S->m, S->gn, S->gs, S->f, S->mstar, S->ftmp);
+  if ((rc = launch_spmv(ctx, S->ops.F1, 0, S->ftmp, S->b))) return rc;
+  if ((rc = solve(6, S->ops.A1, S->b, S->gs + n))) return rc;
+  k_gs3_field2<<<gv, kBlock, 0, s>>>(n, P, S->m, S->gs, S->mstar, S->E, p.me, S->Ht, S->dscal + S_AVGM, S->ftmp);
+  for (int c = 0; c < 3; ++c)
+    if ((rc = launch_spmv(ctx, S->ops.F1, 0, S->ftmp + c * n, S->b + c * n))) return rc;
+  for (int c = 0; c < 3; ++c)
+    if ((rc = solve(7 + c, S->ops.A2, S->b + c * n, S->mss + c * n))) return rc;
+  k_project<<<gv, kBlock, 0, s>>>(n, S->mss, S->defect, S->W, S->vol, S->m, S->mprev, S->dscal, ctx->partials,
+                                  ctx->counter);
+  CPM_HIP(hipGetLastError());
+  CPM_HIP(hipMemcpyAsync(ctx->pinned, S->dscal, S_NSCAL * sizeof(double), hipMemcpyDeviceToHost, s));
+  CPM_HIP(hipStreamSynchronize(s));
+  rep->sumsq = ctx->pinned[S_SUMSQ];
+  for (int k = 0; k < 3; ++k) rep->avg_m[k] = ctx->pinned[S_AVGM + k];
+  return CPM_OK;
+}
+
+// ---- standalone field operators -------------------------------------------
+
+extern "C" int cpm_hd_from_u(cpm_ctx* ctx, const cpm_mat* F3, const double* U, double* H) {
+  if (!ctx || !U || !H) return fail(CPM_EINVAL, "cpm_hd_from_u: null argument");
+  if (int rc = check_f3(F3)) return rc;
+  const int64_t n = F3->nrows;
+  CPM_G_DISPATCH(F3->lanes, k_hd, n, ctx->stream, (int32_t)n, F3->ptr, F3->idx, F3->val, F3->nnz, U, H);
+  CPM_HIP(hipGetLastError());
+  return CPM_OK;
+}
+
+extern "C" int cpm_strain_from_u(cpm_ctx* ctx, const cpm_mat* F3, const double* u, const double* R, double* E) {
+  if (!ctx || !u || !E) return fail(CPM_EINVAL, "cpm_strain_from_u: null argument");
+  if (int rc = check_f3(F3)) return rc;
+  const int64_t n = F3->nrows;
+  double* dR = ctx->scal;  // first 9 slots of the context scratch
+  if (R) CPM_HIP(hipMemcpyAsync(dR, R, 9 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  CPM_G_DISPATCH(F3->lanes, k_strain, n, ctx->stream, (int32_t)n, F3->ptr, F3->idx, F3->val, F3->nnz, u, R ? 1 : 0,
+                 dR, E);
+  CPM_HIP(hipGetLastError());
+  CPM_HIP(hipStreamSynchronize(ctx->stream));
+  return CPM_OK;
+}
+
+extern "C" int cpm_build_e0(cpm_ctx* ctx, int32_t n, double l100, double l111, const double* m, const double* R,
+                            const double* Mv, double* E0) {
+  if (!ctx || !m || !E0 || n < 0) return fail(CPM_EINVAL, "cpm_build_e0: bad argument");
+  if ((R == nullptr) != (Mv == nullptr)) return fail(CPM_EINVAL, "cpm_build_e0: give both R and Mv or neither");
+  double* d = ctx->scal + 16;  // 9 + 36 doubles of context scratch
+  if (R) {
+    CPM_HIP(hipMemcpyAsync(d, R, 9 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    CPM_HIP(hipMemcpyAsync(d + 9, Mv, 36 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  }
+  k_e0<<<grid_for(n), kBlock, 0, ctx->stream>>>(n, m, l100, l111, R ? 1 : 0, d, d + 9, E0);
+  CPM_HIP(hipGetLastError());
+  CPM_HIP(hipStreamSynchronize(ctx->stream));
+  return CPM_OK;
+}
+
+extern "C" int cpm_weighted_sum(cpm_ctx* ctx, int32_t n, int ncomp, const double* W, const double* f, double* out) {
+  if (!ctx || !W || !f || !out || n < 0) return fail(CPM_EINVAL, "cpm_weighted_sum: bad argument");
+  double* d = ctx->scal + 128;
+  const int g = grid_for(n);
+  hipStream_t s = ctx->stream;
+  switch (ncomp) {
+    case 1: k_wsum<1><<<g, kBlock, 0, s>>>(n, W, f, d, ctx->partials, ctx->counter); break;
+    case 2: k_wsum<2><<<g, kBlock, 0, s>>>(n, W, f, d, ctx->partials, ctx->counter); break;
+    case 3: k_wsum<3><<<g, kBlock, 0, s>>>(n, W, f, d, ctx->partials, ctx->counter); break;
+    case 4: k_wsum<4><<<g, kBlock, 0, s>>>(n, W, f, d, ctx->partials, ctx->counter); break;
+    case 5: k_wsum<5><<<g, kBlock, 0, s>>>(n, W, f, d, ctx->partials, ctx->counter); break;
+    case 6: k_wsum<6><<<g, kBlock, 0, s>>>(n, W, f, d, ctx->partials, ctx->counter); break;
+    case 7: k_wsum<7><<<g, kBlock, 0, s>>>(n, W, f, d, ctx->partials, ctx->counter); break;
+    case 8: k_wsum<8><<<g, kBlock, 0, s>>>(n, W, f, d, ctx->partials, ctx->counter); break;
+    default: return fail(CPM_EINVAL, "cpm_weighted_sum: ncomp must be 1..8");
+  }
+  CPM_HIP(hipGetLastError());
+  CPM_HIP(hipMemcpyAsync(ctx->pinned, d, ncomp * sizeof(double), hipMemcpyDeviceToHost, s));
+  CPM_HIP(hipStreamSynchronize(s));
+  std::memcpy(out, ctx->pinned, ncomp * sizeof(double));
+  return CPM_OK;
+}

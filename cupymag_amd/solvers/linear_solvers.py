@@ -1,0 +1,38 @@
+"""CG wrapper (cupymag/solvers/linear_solvers.py:20-74) on the device CG of
+libcupymag_hip.so (cpm_cg), which follows CuPy's cg: x0 = 0 unless
+use_init; b returned when ||b|| = 0; atol = tol*||b||; at least one
+iteration; convergence tested after each update.  Non-convergence raises
+RuntimeError with the reference's message, including the true residual
+||b - A x||."""
+import ctypes as C
+
+import torch
+
+from .._lib import check, lib, ptr
+from ..device import as_device
+
+
+def solve_cg(A, b, M=None, x0=None, tol=1e-7, maxiter=5000, use_init=False, system=None, return_info=False):
+    if M is not None:
+        raise NotImplementedError("preconditioned CG is not part of the GSPM path (the reference passes M=None)")
+    base = getattr(A, "base", A)
+    if getattr(A, "valset", 0) != 0 or base.nsets != 1:
+        raise ValueError("solve_cg needs a single-valued square operator")
+    ctx = base.ctx
+    b = as_device(b, ctx)
+    if use_init and x0 is not None:
+        x = as_device(x0, ctx).clone()
+    else:
+        x = torch.empty_like(b)
+        use_init = False
+    it = C.c_int32()
+    res = C.c_double()
+    info = check(lib().cpm_cg(ctx.h, base.h, ptr(b), ptr(x), int(bool(use_init)), float(tol), int(maxiter),
+                              C.byref(it), C.byref(res)), "cpm_cg")
+    if info != 0:
+        rnorm = torch.linalg.norm(b - base.matvec(x)).item()
+        who = f"CG for {system}" if system is not None else "CG"
+        raise RuntimeError(f"Error! {who} did not converge. info={info}, residual={rnorm}.")
+    if return_info:
+        return x, int(it.value), float(res.value)
+    return x

@@ -1,0 +1,257 @@
+/*
+ * cupymag_hip.h -- C ABI of libcupymag_hip.so, the MI355X (gfx950) engine
+ * behind the GSPM LLG time step of CuPyMag.
+ *
+ * The reference (CuPyMag v0.9.1) is pure Python on CuPy; it has no FFI.  Its
+ * hot path is reached through a handful of Python call signatures, and each
+ * entry point below replaces the CuPy/cuSPARSE work behind one of them
+ * (citations are reference file:line).  The Python package `cupymag_amd`
+ * keeps those signatures and calls this library through ctypes.
+ *
+ * Conventions
+ *   - Every function returns int: CPM_OK (0) on success, a negative CPM_E*
+ *     code on argument/HIP/memory errors, and for the CG solvers a positive
+ *     value = CuPy's `info` (iterations done without convergence).
+ *     cpm_last_error() returns the message of the last failure on the
+ *     calling thread.
+ *   - Vectors are device pointers to contiguous float64 arrays owned by the
+ *     caller (PyTorch-ROCm tensors); the library never keeps a caller pointer
+ *     past the call, except buffers handed to cpm_sim_* which belong to the
+ *     simulation object.  Matrices and simulations are opaque handles owned
+ *     by the library and must be destroyed explicitly.
+ *   - All device work is enqueued on the context's stream (the caller's
+ *     hipStream_t, or a library stream at default priority).  Calls that
+ *     return host scalars (CG iteration counts, sums, step reports)
+ *     synchronise that stream, as the reference synchronises on them.
+ *   - One context per host thread; a context is not re-entrant.
+ *   - Layouts follow the reference: m is (n,3) row-major, u is [ux;uy;uz]
+ *     (assemble_elasticity.py:391), E0 is 6n interleaved [6*i+v]
+ *     (final_assembly.py:259-264), E is (n,6) row-major
+ *     (compute_derivatives.py:125).
+ */
+#ifndef CUPYMAG_HIP_H
+#define CUPYMAG_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CPM_ABI_VERSION 1
+
+#define CPM_OK 0
+#define CPM_EINVAL (-1)
+#define CPM_ENOMEM (-2)
+#define CPM_EHIP (-3)
+#define CPM_EINTERNAL (-4)
+#define CPM_ENODEV (-5)
+
+/* ------------------------------------------------------------------ */
+/* library / errors                                                    */
+/* ------------------------------------------------------------------ */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* cpm_last_error(void);
+/* CPM_ABI_VERSION of the built library. */
+int cpm_abi_version(void);
+
+/* ------------------------------------------------------------------ */
+/* context                                                             */
+/* ------------------------------------------------------------------ */
+typedef struct cpm_ctx cpm_ctx;
+
+/* Replaces CuPy's implicit current-device/default-stream state
+ * (cupymag/__init__.py:21-24).  `stream` is a hipStream_t to enqueue on, or
+ * NULL for a new library-owned stream at default priority. */
+int cpm_ctx_create(int device, void* stream, cpm_ctx** out);
+int cpm_ctx_destroy(cpm_ctx* ctx);
+int cpm_ctx_synchronize(cpm_ctx* ctx);
+
+/* ------------------------------------------------------------------ */
+/* sparse operators                                                    */
+/* ------------------------------------------------------------------ */
+typedef struct cpm_mat cpm_mat;
+
+/* Upload a CSR operator from host arrays.  Replaces
+ * cpx.csr_matrix(A_scipy, dtype=float_cp) in
+ * cupymag/utils/final_assembly.py:176-177,200-201.  `values` holds
+ * `nvalsets` arrays of length nnz back to back that share the pattern
+ * (1, or 3 for the Fx/Fy/Fz derivative triple of
+ * core/assemble_simulation_operators.py:56-63).  indptr is int64 on the
+ * host; the device copy is int32 (nnz must fit). */
+int cpm_mat_create_csr(cpm_ctx* ctx, int32_t nrows, int32_t ncols, int64_t nnz, const int64_t* indptr,
+                       const int32_t* indices, const double* values, int nvalsets, cpm_mat** out);
+int cpm_mat_destroy(cpm_mat* A);
+int cpm_mat_shape(const cpm_mat* A, int32_t* nrows, int32_t* ncols, int64_t* nnz);
+
+/* y = A_valset x.  Replaces `A @ x` on cupyx.scipy.sparse.csr_matrix
+ * (cuSPARSE SpMV), e.g. cupymag/core/Micromagnetics.py:123,217-219. */
+int cpm_spmv(cpm_ctx* ctx, const cpm_mat* A, int valset, const double* x, double* y);
+
+/* Unpreconditioned CG with CuPy's stopping rule.  Replaces
+ * cupyx.scipy.sparse.linalg.cg called from solve_cg,
+ * cupymag/solvers/linear_solvers.py:59-62: x0 = 0 unless use_x0; returns b
+ * (x = 0) when ||b|| = 0; atol = tol*||b||; at least one iteration; stop when
+ * ||r|| <= atol after the update.  x is input (x0, read iff use_x0) and
+ * output.  Returns 0 when converged, else CuPy's info (= iterations).
+ * *iters and *resid (= ||r||, the recurrence residual) are always set. */
+int cpm_cg(cpm_ctx* ctx, const cpm_mat* A, const double* b, double* x, int use_x0, double tol, int maxiter,
+           int32_t* iters, double* resid);
+
+/* ------------------------------------------------------------------ */
+/* field operators                                                     */
+/* ------------------------------------------------------------------ */
+
+/* H[k*n + i] = -(F_k U)_i, k = x,y,z on the 3-valset operator F3.
+ * Replaces ComputeDerivatives.compute_Hd_from_U,
+ * cupymag/utils/compute_derivatives.py:36-58. */
+int cpm_hd_from_u(cpm_ctx* ctx, const cpm_mat* F3, const double* U, double* H);
+
+/* Strain (n,6) [E11,E22,E33,E12,E23,E13] from u = [ux;uy;uz] by the weak-form
+ * derivative triple F3, optionally mixed by the 3x3 rotation R (host array,
+ * row-major; NULL = no rotation).  Replaces
+ * ComputeDerivatives.compute_E_from_u, compute_derivatives.py:60-125. */
+int cpm_strain_from_u(cpm_ctx* ctx, const cpm_mat* F3, const double* u, const double* R, double* E);
+
+/* Spontaneous strain E0 (6n, interleaved) of m (n,3).  With R and Mv (host,
+ * row-major 3x3 and 6x6) non-NULL, m is first rotated m100 = m R and the
+ * Voigt vector mapped E0 <- E0 Mv^T (Micromagnetics.py:112-119).  Replaces
+ * build_E0_from_m, cupymag/utils/final_assembly.py:208-266. */
+int cpm_build_e0(cpm_ctx* ctx, int32_t n, double lambda100, double lambda111, const double* m, const double* R,
+                 const double* Mv, double* E0);
+
+/* out[c] = sum_i W[i] f[i*ncomp + c] (host out, ncomp <= 8), FP64
+ * accumulation.  With W the precomputed per-DOF quadrature weights, out/V
+ * (V = total weight) is VolumeAverage.compute_average_field_gpu,
+ * volume_average.py:155-216. */
+int cpm_weighted_sum(cpm_ctx* ctx, int32_t n, int ncomp, const double* W, const double* f, double* out);
+
+/* ------------------------------------------------------------------ */
+/* device-resident GSPM LLG simulation (the hot loop)                  */
+/* ------------------------------------------------------------------ */
+typedef struct cpm_sim cpm_sim;
+
+typedef struct cpm_sim_ops {
+  const cpm_mat* A_demag; /* n x n, anchored at DOF 0 (assemble_demag.py:307-341) */
+  const cpm_mat* F3;      /* Fx,Fy,Fz on one pattern (assemble_demag.py:235-305) */
+  const cpm_mat* A1;      /* M + alpha1 K, defect surgery (assemble_simulation_operators.py:81) */
+  const cpm_mat* A2;      /* M + alpha2 K (:82) */
+  const cpm_mat* F1;      /* M, defect rows removed (:83) */
+  const cpm_mat* A_el;    /* 3n x 3n or NULL when magnetoelastic coupling is off */
+  const cpm_mat* F_el;    /* 3n x 6n or NULL */
+} cpm_sim_ops;
+
+typedef struct cpm_sim_params {
+  int32_t n;                  /* number of DOFs */
+  int32_t me;                 /* magnetoelastic coupling on */
+  int32_t rot111;             /* crystal x axis along [111] */
+  int32_t maxiter;            /* CG max iterations */
+  double tol;                 /* CG relative tolerance */
+  double dt;                  /* reduced time step */
+  double alpha_damp;          /* Gilbert damping */
+  double K1;                  /* reduced cubic anisotropy */
+  double elasC[3];            /* reduced magnetoelastic constants (zero when me == 0) */
+  double lambda100, lambda111;
+  double Nd[3];               /* demagnetisation factors */
+  double Ebar_sigma[6];       /* strain of the external stress (sigma_matrices.py:25-45) */
+  double R[9];                /* rot_111_matrices.get_R_matrix (row-major) */
+  double Mv[36];              /* rot_111_matrices.get_M_matrix (row-major) */
+  double MvinvT[36];          /* inv(Mv)^T (Micromagnetics.py:54) */
+  int32_t mode;               /* CPM_MODE_* */
+  int32_t reserved[7];
+} cpm_sim_params;
+
+#define CPM_MODE_FP64 0  /* plain FP64 path, the reference's sequence of solves */
+
+/* Report of one step (cpm_sim_step). */
+typedef struct cpm_step_report {
+  double sumsq;       /* sum (m - m_prev)^2, Micromagnetics.py:299 */
+  double avg_m[3];    /* volume average of the new m, :306 */
+  int32_t iters[10];  /* CG iterations: elasticity, demag, g1n, g2n, g3n, g1*, g2*, m1**, m2**, m3** */
+  int32_t failed;     /* index+1 of the solve that did not converge, 0 if all converged */
+  double resid;       /* recurrence residual of the failed solve */
+} cpm_step_report;
+
+/* Create a simulation on `ops` (borrowed; they must outlive the sim).
+ * W: device pointer to the n per-DOF quadrature weights and `volume` their
+ * total, so that avg = (W . f) / volume restates
+ * VolumeAverage.compute_average_field_gpu (volume_average.py:155-216) as one
+ * dot product; W is copied.  defect: host array of n bytes, 1 on defect
+ * DOFs (extract_defect_dofs, final_assembly.py:27-57). */
+int cpm_sim_create(cpm_ctx* ctx, const cpm_sim_ops* ops, const cpm_sim_params* prm, const double* W,
+                   double volume, const uint8_t* defect, cpm_sim** out);
+int cpm_sim_destroy(cpm_sim* sim);
+
+#define CPM_FIELD_M 0       /* (n,3) */
+#define CPM_FIELD_HTILDE 1  /* (3,n) component-major: Htilde1, Htilde2, Htilde3 */
+#define CPM_FIELD_E 2       /* (n,6) */
+#define CPM_FIELD_U 3       /* demag potential (n) */
+#define CPM_FIELD_UEL 4     /* displacement [ux;uy;uz] (3n) */
+
+/* Set m (device pointer, (n,3)); zeroes defect DOFs, computes avg_m
+ * (initialize_m + the first average, Micromagnetics.py:70-73) and returns
+ * it in avg_m (host, 3). */
+int cpm_sim_set_m(cpm_sim* sim, const double* m, double* avg_m);
+/* Copy a field (CPM_FIELD_*) into a caller device buffer. */
+int cpm_sim_get(cpm_sim* sim, int field, double* dst);
+/* One GSPM LLG step at external field H (reduced units), with the
+ * warm-start flag use_init (Micromagnetics.py:110-307).  Returns 0, or the
+ * positive CG info of the first solve that did not converge (the state is
+ * then that of the failing solve, as when the reference raises). */
+int cpm_sim_step(cpm_sim* sim, const double* H, int use_init, cpm_step_report* rep);
+
+/* ------------------------------------------------------------------ */
+/* host-side assembly (CPU, once per mesh)                             */
+/* ------------------------------------------------------------------ */
+typedef struct cpm_host_ops cpm_host_ops;
+
+/* Direct-to-CSR FEM assembly on the host.  Replaces the Numba assemblers
+ * AssembleDemag / AssembleGaussSeidel / AssembleElasticity
+ * (cupymag/physics/assemble_*.py) plus the COO->CSR and defect surgery of
+ * cupymag/utils/final_assembly.py:59-205 and
+ * core/assemble_simulation_operators.py:42-107.
+ *   elems: (n_elems, elem_stride) int32, first npe columns are node ids.
+ *   defect_dof: n bytes (1 = defect DOF) or NULL.
+ *   coef: [alpha1, alpha2, C_voigt (36, row-major, already rotated)].
+ * Arrays are read back with cpm_host_ops_array. */
+int cpm_host_assemble(const double* coords, int64_t n_nodes, const int32_t* elems, int64_t n_elems, int npe,
+                      int elem_stride, const int32_t* gid, int32_t ndof, const uint8_t* defect_dof,
+                      const double* coef, int want_elastic, int nthreads, cpm_host_ops** out);
+
+#define CPM_HA_PTR 0      /* int64 [n+1] pattern of A_demag, Fx, Fy, Fz */
+#define CPM_HA_IDX 1      /* int32 */
+#define CPM_HA_ADEMAG 2   /* double */
+#define CPM_HA_FX 3
+#define CPM_HA_FY 4
+#define CPM_HA_FZ 5
+#define CPM_HA_GS_PTR 6   /* pattern of A1/A2 after defect surgery */
+#define CPM_HA_GS_IDX 7
+#define CPM_HA_A1 8
+#define CPM_HA_A2 9
+#define CPM_HA_F1_PTR 10  /* pattern of F1 (defect rows removed) */
+#define CPM_HA_F1_IDX 11
+#define CPM_HA_F1 12
+#define CPM_HA_EL_PTR 13  /* A_el, 3n x 3n */
+#define CPM_HA_EL_IDX 14
+#define CPM_HA_AEL 15
+#define CPM_HA_FE_PTR 16  /* F_el, 3n x 6n */
+#define CPM_HA_FE_IDX 17
+#define CPM_HA_FEL 18
+int cpm_host_ops_array(cpm_host_ops* h, int which, void** ptr, int64_t* len);
+int cpm_host_ops_free(cpm_host_ops* h);
+
+/* Dense element matrices (any output may be NULL):
+ *   K, M: (n_elems, npe, npe); F: (3, n_elems, npe, npe);
+ *   Kel: (n_elems, 3npe, 3npe); Fel: (n_elems, 3npe, 6npe) using C36.
+ * The per-element integrals of assemble_demag.py:95-184,
+ * assemble_Gauss_Seidel.py:88-160, assemble_elasticity.py:194-348; the
+ * building block of the reference's COO API (build_coo_matrix*_numba). */
+int cpm_host_element_matrices(const double* coords, int64_t n_nodes, const int32_t* elems, int64_t n_elems, int npe,
+                              int elem_stride, const double* C36, double* K, double* M, double* F, double* Kel,
+                              double* Fel);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CUPYMAG_HIP_H */

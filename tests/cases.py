@@ -1,0 +1,75 @@
+"""Shared test inputs: YAML-equivalent configs and small meshes."""
+import copy
+
+import numpy as np
+
+BASE = {
+    "physics": {"magnetic": {"ms": 1.2e6, "exchange_stiffness": 1.5e-11, "anisotropy_k1": -1e4},
+                "elastic": {"c11": 2.4e11, "c12": 1.4e11, "c44": 1.2e11},
+                "magnetostriction": {"lambda100": 2.6e-4, "lambda111": -3e-5},
+                "demagnetization": {"n_x": 1 / 3, "n_y": 1 / 3, "n_z": 1 / 3},
+                "external_field": {"hext1": 5e4, "dhext1": 1e4, "hext2": 0.0, "hext3": 0.0},
+                "external_stress": {"sigma11": 0.0},
+                "rotation_111": False, "magnetoelastic_coupling": False,
+                "characteristic_length": 2e-9},
+    "grid": {"type": "Hex", "cubic_mesh": {"nx": 8, "ny": 8, "nz": 8, "ndx": 2, "ndy": 2, "ndz": 2}},
+    "simulation": {"precision": "DP",
+                   "llg": {"timestep": 1e-12, "alpha_damping": 0.5, "llg_accuracy_factor": 1e-30},
+                   "solver": {"tolerance": 1e-10, "max_iterations": 5000, "initial_factor": 100.0},
+                   "stop_condition": {"direction": [1.0, 0.0, 0.0], "value": -0.95}},
+    "output": {"write_magnetization": False},
+}
+
+
+def config(grid="Hex", n=8, me=False, rot=False, dt=1e-12, alpha=0.5, sigma11=0.0, ndef=2, tol=1e-10,
+           hext=(5e4, 0.0, 0.0), K1=-1e4, defect="Sphere", defect_params=2.0):
+    c = copy.deepcopy(BASE)
+    c["physics"]["magnetoelastic_coupling"] = me
+    c["physics"]["rotation_111"] = rot
+    c["physics"]["external_stress"]["sigma11"] = sigma11
+    c["physics"]["magnetic"]["anisotropy_k1"] = K1
+    c["physics"]["external_field"].update(hext1=hext[0], hext2=hext[1], hext3=hext[2])
+    c["simulation"]["llg"].update(timestep=dt, alpha_damping=alpha)
+    c["simulation"]["solver"]["tolerance"] = tol
+    if grid == "Hex":
+        c["grid"] = {"type": "Hex", "cubic_mesh": {"nx": n, "ny": n, "nz": n, "ndx": ndef, "ndy": ndef,
+                                                   "ndz": ndef}}
+    else:
+        c["grid"] = {"type": "Tet", "tet_mesh": {"mesh_file": None, "defect_shape": defect,
+                                                 "defect_parameters": defect_params}}
+    return c
+
+
+def tilted_m(n, seed=0, max_deg=5.0):
+    """+x with per-DOF random tilts <= max_deg (seeded)."""
+    rng = np.random.default_rng(seed)
+    th = np.deg2rad(max_deg) * rng.random(n)
+    ph = 2 * np.pi * rng.random(n)
+    return np.stack([np.cos(th), np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph)], axis=1)
+
+
+def oracle_mesh(cfg, tet_arrays=None):
+    """(coords, elements, gid) as the oracle builds them."""
+    from oracle import mesh as om
+    g = cfg["grid"]
+    if g["type"] == "Hex":
+        cm = g["cubic_mesh"]
+        coords, el, b = om.hex_grid(cm["nx"], cm["ny"], cm["nz"], cm["ndx"], cm["ndy"], cm["ndz"])
+        return coords, el, om.periodic_map(coords, b)
+    coords, tets = tet_arrays
+    tm = g["tet_mesh"]
+    p = tm["defect_parameters"]
+    shape = om.SHAPES[tm["defect_shape"]](p if tm["defect_shape"] != "RotatedEllipsoid" else
+                                          (p, np.array(tm["rotation_matrix"])))
+    flags = om.tet_defect_flags(coords, tets, shape)
+    el = np.concatenate([tets, flags[:, None]], axis=1).astype(np.int32)
+    return coords, el, om.periodic_map(coords, om.tet_bounds(coords))
+
+
+def oracle_setup(cfg, tet_arrays=None):
+    from oracle import fem, params
+    p = params.reduced(cfg)
+    coords, el, gid = oracle_mesh(cfg, tet_arrays)
+    ops = fem.build_operators(coords, el, gid, p)
+    Avg = fem.VolumeAverage(coords, el, gid)
+    return p, ops, Avg, (coords, el, gid)

@@ -1,0 +1,201 @@
+"""GPU parity of the HIP engine against the CPU oracle (same seeded inputs).
+
+Tolerances (FP64 path): SpMV 1e-14 relative; CG solutions 1e-10 relative
+with iteration counts within +-1 of the oracle's CuPy-rule CG; field
+operators 1e-12 relative; full GSPM steps: magnetization per node within
+1e-10, displacement within 1e-10 relative, CG iteration counts within +-1.
+The operators themselves come from two independent assemblies (native
+host assembler vs the oracle's SciPy COO path), so these checks also cover
+the assembly.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from tests.cases import config, oracle_setup, tilted_m  # noqa: E402
+
+
+def _product(cfg, tet_arrays=None):
+    from cupymag_amd.core.assemble_simulation_operators import SimulationOperators
+    from cupymag_amd.core.parameters import from_dict
+    P = from_dict(cfg)
+    return P, SimulationOperators(P, mesh_arrays=tet_arrays, keep_host=True)
+
+
+def _kuhn(n, jitter=0.15, seed=3):
+    from cupymag_amd.mesh.kuhn import kuhn_mesh
+    return kuhn_mesh(n, jitter=jitter, seed=seed)
+
+
+def _rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def test_spmv_matches_scipy():
+    cfg = config("Tet", me=True, defect_params=1.5)
+    arr = _kuhn(6)
+    P, S = _product(cfg, arr)
+    A_el, F_el = S.csr_elasticity()
+    A1, A2, F1 = S.csr_GS()
+    rng = np.random.default_rng(0)
+    for A in (A_el, F_el, A1, F1):
+        x = rng.standard_normal(A.shape[1])
+        y = (A @ torch.as_tensor(x, device="cuda")).cpu().numpy()
+        ref = A.to_scipy() @ x
+        assert _rel(y, ref) < 1e-14
+
+
+@pytest.mark.parametrize("which", ["A1", "A_demag", "A_el"])
+def test_cg_matches_oracle(which):
+    from cupymag_amd.solvers.linear_solvers import solve_cg
+    from oracle import gspm as og
+    cfg = config("Tet", me=True, defect_params=1.8)
+    arr = _kuhn(6)
+    P, S = _product(cfg, arr)
+    p, ops, Avg, _ = oracle_setup(cfg, arr)
+    A_dev = {"A1": S.csr_GS()[0], "A_demag": S.csr_demag_and_deriv()[0], "A_el": S.csr_elasticity()[0]}[which]
+    A_ref = ops[which]
+    rng = np.random.default_rng(1)
+    b = rng.standard_normal(A_ref.shape[0])
+    if which == "A_demag":
+        b[0] = 0.0
+    x, it, _ = solve_cg(A_dev, torch.as_tensor(b, device="cuda"), tol=1e-10, maxiter=5000, return_info=True)
+    xr, info, itr = og.cg(A_ref, b, None, 1e-10, 5000)
+    assert info == 0
+    assert abs(it - itr) <= 1
+    assert _rel(x.cpu().numpy(), xr) < 1e-10
+    # warm start from a perturbed solution
+    x0 = xr + 1e-6 * rng.standard_normal(len(b))
+    x2, it2, _ = solve_cg(A_dev, torch.as_tensor(b, device="cuda"), x0=torch.as_tensor(x0, device="cuda"),
+                          use_init=True, tol=1e-10, return_info=True)
+    xr2, _, itr2 = og.cg(A_ref, b, x0, 1e-10, 5000)
+    assert abs(it2 - itr2) <= 1
+    assert _rel(x2.cpu().numpy(), xr2) < 1e-10
+
+
+def test_cg_edge_cases():
+    from cupymag_amd.solvers.linear_solvers import solve_cg
+    cfg = config("Hex", n=6)
+    P, S = _product(cfg)
+    A1 = S.csr_GS()[0]
+    n = A1.shape[0]
+    # ||b|| == 0 returns b (zeros), also with a warm start
+    z = torch.zeros(n, dtype=torch.float64, device="cuda")
+    x, it, _ = solve_cg(A1, z, x0=torch.ones(n, dtype=torch.float64, device="cuda"), use_init=True,
+                        return_info=True)
+    assert it == 0 and float(x.abs().max()) == 0.0
+    # non-convergence raises the reference's RuntimeError
+    b = torch.as_tensor(np.random.default_rng(2).standard_normal(n), device="cuda")
+    with pytest.raises(RuntimeError, match=r"Error! CG for test did not converge. info=2, residual="):
+        solve_cg(A1, b, tol=1e-14, maxiter=2, system="test")
+
+
+@pytest.mark.parametrize("rot", [False, True])
+def test_field_operators(rot):
+    from cupymag_amd.utils.final_assembly import build_E0_from_m
+    from oracle import gspm as og
+    from oracle.params import rot111_M, rot111_R
+    cfg = config("Tet", me=True, rot=rot, defect_params=1.5)
+    arr = _kuhn(5)
+    P, S = _product(cfg, arr)
+    p, ops, Avg, _ = oracle_setup(cfg, arr)
+    n = ops["n"]
+    rng = np.random.default_rng(5)
+    _, Fx, Fy, Fz, Deriv = S.csr_demag_and_deriv()
+    U = rng.standard_normal(n)
+    H = Deriv.compute_Hd_from_U(torch.as_tensor(U, device="cuda"))
+    Hr = og.hd_from_U(ops, U)
+    for k in range(3):
+        assert _rel(H[k].cpu().numpy(), Hr[k]) < 1e-12
+    u = rng.standard_normal(3 * n)
+    R = rot111_R() if rot else None
+    E = Deriv.compute_E_from_u(n, torch.as_tensor(u, device="cuda"), R).cpu().numpy()
+    assert _rel(E, og.strain_from_u(ops, n, u, R)) < 1e-12
+    m = tilted_m(n, 1, 40.0)
+    if rot:
+        E0 = build_E0_from_m(p["lambda100"], p["lambda111"], m, rot111_R(), rot111_M()).cpu().numpy()
+        ref = (og.build_E0(p["lambda100"], p["lambda111"], m @ rot111_R()).reshape(n, 6) @ rot111_M().T).ravel()
+    else:
+        E0 = build_E0_from_m(p["lambda100"], p["lambda111"], m).cpu().numpy()
+        ref = og.build_E0(p["lambda100"], p["lambda111"], m)
+    assert _rel(E0, ref) < 1e-13
+    Avgp = S.volume_average()
+    f = rng.standard_normal((n, 3))
+    assert _rel(Avgp.compute_average_field_gpu(torch.as_tensor(f, device="cuda")), Avg.average(f)) < 1e-11
+    g = rng.standard_normal(n)
+    assert abs(Avgp.compute_average_field_gpu(g) - Avg.average(g)) < 1e-12 * max(1.0, abs(Avg.average(g)))
+
+
+STEP_CASES = [
+    dict(grid="Hex", n=8, me=False, rot=False),
+    dict(grid="Hex", n=6, me=True, rot=False, sigma11=1e8),
+    dict(grid="Tet", n=6, me=True, rot=True, dt=5e-12),
+    dict(grid="Tet", n=7, me=False, rot=True, dt=1e-11, alpha=0.1),
+    dict(grid="Tet", n=6, me=True, rot=False, dt=5e-12, K1=2e4, hext=(2e4, 3e4, -1e4)),
+]
+
+
+@pytest.mark.parametrize("case", STEP_CASES, ids=lambda c: "-".join(f"{k}{v}" for k, v in c.items()))
+def test_gspm_steps_match_oracle(case):
+    from cupymag_amd.core.gspm import GSPMSimulation
+    from oracle import gspm as og
+    case = dict(case)
+    grid, n = case.pop("grid"), case.pop("n")
+    cfg = config(grid, n=n, defect_params=1.6, **case)
+    arr = _kuhn(n) if grid == "Tet" else None
+    P, S = _product(cfg, arr)
+    p, ops, Avg, _ = oracle_setup(cfg, arr)
+    nd = ops["n"]
+    m0 = tilted_m(nd, seed=11)
+    sim = GSPMSimulation(S, P)
+    am = sim.set_m(m0)
+    st = og.init_state(ops, Avg, m0)
+    assert _rel(am, st["avg_m"]) < 1e-12
+    p["H_now"] = p["H"]
+    H = [P.Hext1, P.Hext2, P.Hext3]
+    for k in range(4):
+        log = []
+        ui = st["use_init"]
+        og.step(st, ops, Avg, p, log)
+        rep = sim.step(H, ui)
+        m = sim.get("M").cpu().numpy()
+        assert np.max(np.abs(m - st["m"])) < 1e-10, f"step {k}"
+        assert abs(rep.sumsq - st["sumsq"]) <= 1e-9 * max(st["sumsq"], 1e-30) + 1e-22
+        ref_it = [it for _, it in log]
+        got = [it for it, s in zip(rep.iters, range(10)) if (p["ME"] or s > 0)]
+        assert all(abs(a - b) <= 1 for a, b in zip(got, ref_it)), (got, ref_it)
+        if p["ME"]:
+            u = sim.get("UEL").cpu().numpy()
+            assert _rel(u, st["u"]) < 1e-10
+            assert _rel(sim.get("E").cpu().numpy(), st["E"]) < 1e-10
+        Ht = sim.get("HTILDE").cpu().numpy()
+        assert _rel(Ht, np.stack(st["Ht"])) < 1e-10
+        assert _rel(rep.avg_m, st["avg_m"]) < 1e-10
+
+
+def test_warm_start_path_matches_oracle():
+    """use_init switches on after the first step (large initial_factor)."""
+    from cupymag_amd.core.gspm import GSPMSimulation
+    from oracle import gspm as og
+    cfg = config("Hex", n=6, me=True)
+    cfg["simulation"]["llg"]["llg_accuracy_factor"] = 1e-3
+    cfg["simulation"]["solver"]["initial_factor"] = 1e6
+    P, S = _product(cfg)
+    p, ops, Avg, _ = oracle_setup(cfg)
+    m0 = tilted_m(ops["n"], seed=4)
+    sim = GSPMSimulation(S, P)
+    sim.set_m(m0)
+    st = og.init_state(ops, Avg, m0)
+    p["H_now"] = p["H"]
+    H = [P.Hext1, P.Hext2, P.Hext3]
+    use_init = False
+    for k in range(3):
+        og.step(st, ops, Avg, p)
+        rep = sim.step(H, use_init)
+        use_init = rep.sumsq < P.use_init_factor * ops["n"] * P.LLG_accuracy_factor
+        assert use_init == st["use_init"]
+        assert np.max(np.abs(sim.get("M").cpu().numpy() - st["m"])) < 1e-10
