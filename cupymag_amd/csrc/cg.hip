@@ -189,17 +189,31 @@ int run_cg(cpm_ctx* ctx, const cpm_mat* A, const double* b, double* x, int use_x
   CPM_HIP(hipGetLastError());
   CgState h{};
   int batch = hint > 2 ? hint - 1 : 8;
+  const int max_batch = ctx->prof ? 256 : 4096;
   int issued = 0;
   for (;;) {
-    const int todo = std::max(1, std::min(batch, maxiter - issued));
+    const int todo = std::max(1, std::min(std::min(batch, max_batch), maxiter - issued));
+    const int it_before = h.it;
     for (int k = 0; k < todo; ++k) {
       k_cg_pupd<<<gv, kBlock, 0, s>>>(n, ctx->r, ctx->p, st);
+      if (ctx->prof) CPM_HIP(hipEventRecord(ctx->ev[2 * k], s));
       CPM_LANES_DISPATCH(A->lanes, launch_cg_spmv_g, ctx, A, st);
+      if (ctx->prof) CPM_HIP(hipEventRecord(ctx->ev[2 * k + 1], s));
       k_cg_xr<<<gv, kBlock, 0, s>>>(n, x, ctx->r, ctx->p, ctx->q, st, ctx->partials, ctx->counter);
     }
     issued += todo;
     CPM_HIP(hipGetLastError());
     if (int rc = read_cg_state(ctx, 0, &h)) return rc;
+    if (ctx->prof) {
+      // only launches that did work (iterations it_before .. h.it-1 of this batch)
+      cpm_mat* Am = const_cast<cpm_mat*>(A);
+      for (int k = 0; k < h.it - it_before && k < todo; ++k) {
+        float ms = 0.f;
+        CPM_HIP(hipEventElapsedTime(&ms, ctx->ev[2 * k], ctx->ev[2 * k + 1]));
+        Am->prof_ms += ms;
+        Am->prof_launches += 1;
+      }
+    }
     if (h.done || issued >= maxiter) break;
     batch = std::max(2, std::min(64, h.it / 4 + 2));
   }
@@ -221,4 +235,23 @@ extern "C" int cpm_cg(cpm_ctx* ctx, const cpm_mat* A, const double* b, double* x
                       int32_t* iters, double* resid) {
   if (!ctx || !A || !b || !x) return cpm::fail(CPM_EINVAL, "cpm_cg: null argument");
   return cpm::run_cg(ctx, A, b, x, use_x0, tol, maxiter, 0, iters, resid);
+}
+
+extern "C" int cpm_prof_enable(cpm_ctx* ctx, int on) {
+  if (!ctx) return cpm::fail(CPM_EINVAL, "cpm_prof_enable: null context");
+  if (on && !ctx->ev[0])
+    for (auto& e : ctx->ev) CPM_HIP(hipEventCreate(&e));
+  ctx->prof = on ? 1 : 0;
+  return CPM_OK;
+}
+
+extern "C" int cpm_mat_prof(cpm_mat* A, double* total_ms, int64_t* launches, int reset) {
+  if (!A) return cpm::fail(CPM_EINVAL, "cpm_mat_prof: null matrix");
+  if (total_ms) *total_ms = A->prof_ms;
+  if (launches) *launches = A->prof_launches;
+  if (reset) {
+    A->prof_ms = 0.0;
+    A->prof_launches = 0;
+  }
+  return CPM_OK;
 }

@@ -45,16 +45,9 @@ extern "C" int cpm_ctx_create(int device, void* stream, cpm_ctx** out) {
   CPM_HIP(hipSetDevice(device));
   auto* c = new cpm_ctx();
   c->device = device;
-  if (stream) {
-    c->stream = (hipStream_t)stream;
-  } else {
-    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) {
-      delete c;
-      return cpm::fail(CPM_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
-    }
-    c->own_stream = true;
-  }
+  // NULL is the device's null stream (the stream torch uses by default)
+  c->stream = (hipStream_t)stream;
+  c->own_stream = false;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->ncu = prop.multiProcessorCount;
   hipError_t e = hipSuccess;
@@ -77,13 +70,15 @@ extern "C" int cpm_ctx_create(int device, void* stream, cpm_ctx** out) {
 
 extern "C" int cpm_ctx_destroy(cpm_ctx* c) {
   if (!c) return CPM_OK;
-  hipSetDevice(c->device);
-  if (c->stream) hipStreamSynchronize(c->stream);
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
   for (void* p : {(void*)c->partials, (void*)c->counter, (void*)c->scal, (void*)c->cg, (void*)c->r, (void*)c->p,
                   (void*)c->q})
-    if (p) hipFree(p);
-  if (c->pinned) hipHostFree(c->pinned);
-  if (c->own_stream) hipStreamDestroy(c->stream);
+    if (p) (void)hipFree(p);
+  if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->ev[0])
+    for (auto& e : c->ev) (void)hipEventDestroy(e);
+  if (c->own_stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return CPM_OK;
 }
@@ -146,11 +141,11 @@ extern "C" int cpm_mat_create_csr(cpm_ctx* ctx, int32_t nrows, int32_t ncols, in
 extern "C" int cpm_mat_destroy(cpm_mat* A) {
   if (!A) return CPM_OK;
   if (A->ctx) {
-    hipSetDevice(A->ctx->device);
-    hipStreamSynchronize(A->ctx->stream);
+    (void)hipSetDevice(A->ctx->device);
+    (void)hipStreamSynchronize(A->ctx->stream);
   }
   for (void* p : {(void*)A->ptr, (void*)A->idx, (void*)A->val})
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   delete A;
   return CPM_OK;
 }

@@ -46,6 +46,9 @@ struct cpm_ctx {
   double* q = nullptr;
   int64_t ws_len = 0;
   int ncu = 256;
+  // optional timing of every CG SpMV launch with HIP events (cpm_prof_enable)
+  int prof = 0;
+  hipEvent_t ev[2 * 256] = {};
 };
 
 struct cpm_mat {
@@ -57,6 +60,8 @@ struct cpm_mat {
   int32_t* idx = nullptr;
   double* val = nullptr;   // nvals * nnz
   int lanes = 8;           // threads per row of the vector-CSR kernels
+  double prof_ms = 0.0;    // accumulated CG SpMV kernel time (profiling on)
+  int64_t prof_launches = 0;
 };
 
 namespace cpm {

@@ -492,10 +492,10 @@ static int sim_alloc(cpm_sim* S, double** p, int64_t count) {
 extern "C" int cpm_sim_destroy(cpm_sim* S) {
   if (!S) return CPM_OK;
   if (S->ctx) {
-    hipSetDevice(S->ctx->device);
-    hipStreamSynchronize(S->ctx->stream);
+    (void)hipSetDevice(S->ctx->device);
+    (void)hipStreamSynchronize(S->ctx->stream);
   }
-  for (void* p : S->allocs) hipFree(p);
+  for (void* p : S->allocs) (void)hipFree(p);
   delete S;
   return CPM_OK;
 }

@@ -20,7 +20,7 @@
  *     simulation object.  Matrices and simulations are opaque handles owned
  *     by the library and must be destroyed explicitly.
  *   - All device work is enqueued on the context's stream (the caller's
- *     hipStream_t, or a library stream at default priority).  Calls that
+ *     hipStream_t or the null stream).  Calls that
  *     return host scalars (CG iteration counts, sums, step reports)
  *     synchronise that stream, as the reference synchronises on them.
  *   - One context per host thread; a context is not re-entrant.
@@ -62,8 +62,9 @@ int cpm_abi_version(void);
 typedef struct cpm_ctx cpm_ctx;
 
 /* Replaces CuPy's implicit current-device/default-stream state
- * (cupymag/__init__.py:21-24).  `stream` is a hipStream_t to enqueue on, or
- * NULL for a new library-owned stream at default priority. */
+ * (cupymag/__init__.py:21-24).  `stream` is the hipStream_t to enqueue on;
+ * NULL is the device's null stream (PyTorch's default stream), so work
+ * enqueued by the caller and by the library is ordered. */
 int cpm_ctx_create(int device, void* stream, cpm_ctx** out);
 int cpm_ctx_destroy(cpm_ctx* ctx);
 int cpm_ctx_synchronize(cpm_ctx* ctx);
@@ -98,6 +99,14 @@ int cpm_spmv(cpm_ctx* ctx, const cpm_mat* A, int valset, const double* x, double
  * *iters and *resid (= ||r||, the recurrence residual) are always set. */
 int cpm_cg(cpm_ctx* ctx, const cpm_mat* A, const double* b, double* x, int use_x0, double tol, int maxiter,
            int32_t* iters, double* resid);
+
+/* Kernel timing (no counterpart in the reference, which has only wall-clock
+ * prints, Micromagnetics.py:49,106,348-351): with `on`, every CG SpMV launch
+ * is bracketed by HIP events on the context stream and its duration is
+ * accumulated on the matrix; cpm_mat_prof reads (and optionally resets) the
+ * total milliseconds and the number of launches that did work. */
+int cpm_prof_enable(cpm_ctx* ctx, int on);
+int cpm_mat_prof(cpm_mat* A, double* total_ms, int64_t* launches, int reset);
 
 /* ------------------------------------------------------------------ */
 /* field operators                                                     */
