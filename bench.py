@@ -69,7 +69,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_gpu(n, steps, warmup):
+def run_gpu(n, steps, warmup, steps64, modes):
+    """Run the plain FP64 path and the fast paths on the same operators and
+    initial state; returns per-mode timings, CG iteration counts, the SpMV
+    kernel timing of the elasticity operator and the magnetization after the
+    same number of steps."""
     import ctypes as C
 
     import torch
@@ -89,42 +93,68 @@ def run_gpu(n, steps, warmup):
     log(f"[bench] mesh {n}^3: {S.n_nodes} nodes, {S.n_elements} tets, {S.n_dof} DOFs, "
         f"{int(S.elements_np[:, -1].sum())} defect tets ({time.time() - t0:.1f}s)")
     t0 = time.time()
-    sim = GSPMSimulation(S, P)
-    log(f"[bench] host assembly + upload + weights {time.time() - t0:.1f}s")
+    S.csr_demag_and_deriv()
+    log(f"[bench] host assembly + upload (CSR and SELL-64 forms) {time.time() - t0:.1f}s")
     A_el = S.csr_elasticity()[0]
-    ctx = sim.ctx
-    sim.set_m(initial_m(S.n_dof))
+    m0 = initial_m(S.n_dof)
     H = [P.Hext1, P.Hext2, P.Hext3]
-    for k in range(warmup):
-        t = time.time()
-        rep = sim.step(H, False)
-        log(f"[bench] warmup step {k}: {time.time() - t:.3f}s iters={rep.iters}")
-    check(lib().cpm_prof_enable(ctx.h, 1))
-    check(lib().cpm_mat_prof(A_el.h, None, None, 1))
-    torch.cuda.synchronize()
-    lat = []
-    iters = []
-    t_start = time.perf_counter()
-    for k in range(steps):
-        t = time.perf_counter()
-        rep = sim.step(H, False)
-        lat.append(time.perf_counter() - t)
-        iters.append(rep.iters)
-        log(f"[bench] step {k}: {lat[-1]:.3f}s iters={rep.iters}")
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t_start
-    ms = C.c_double()
-    nl = C.c_int64()
-    check(lib().cpm_mat_prof(A_el.h, C.byref(ms), C.byref(nl), 0))
-    check(lib().cpm_prof_enable(ctx.h, 0))
-    nrows, nnz = A_el.shape[0], A_el.nnz
-    # compulsory bytes of one CG SpMV launch: values + column indices + row
-    # pointers + p read once + q written once
-    bytes_launch = 12.0 * nnz + 4.0 * (nrows + 1) + 16.0 * nrows
-    avg_s = ms.value / 1e3 / max(nl.value, 1)
-    return dict(n_dof=S.n_dof, n_nodes=S.n_nodes, n_elements=S.n_elements, elapsed=elapsed, lat=lat, iters=iters,
-                spmv_launches=nl.value, spmv_avg_us=avg_s * 1e6, spmv_bytes=bytes_launch,
-                spmv_gbs=bytes_launch / avg_s / 1e9, A_el_nnz=nnz)
+    out = dict(n_dof=S.n_dof, n_nodes=S.n_nodes, n_elements=S.n_elements, modes={})
+    snap_at = warmup + steps64
+    for mode in modes:
+        sim = GSPMSimulation(S, P, mode=mode)
+        ctx = sim.ctx
+        sim.set_m(m0)
+        k_timed = steps64 if mode == "fp64" else steps
+        snap = None
+        done = 0
+        for k in range(warmup):
+            t = time.time()
+            rep = sim.step(H, False)
+            done += 1
+            log(f"[bench] {mode} warmup step {k}: {time.time() - t:.3f}s iters={rep.iters}")
+        check(lib().cpm_prof_enable(ctx.h, 1))
+        check(lib().cpm_mat_prof(A_el.h, None, None, 1))
+        torch.cuda.synchronize()
+        lat, iters = [], []
+        t_start = time.perf_counter()
+        for k in range(k_timed):
+            t = time.perf_counter()
+            rep = sim.step(H, False)
+            lat.append(time.perf_counter() - t)
+            iters.append(rep.iters)
+            done += 1
+            if done == snap_at:
+                snap = sim.get("M")
+            log(f"[bench] {mode} step {k}: {lat[-1]:.3f}s iters={rep.iters}")
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t_start
+        ms = C.c_double()
+        nl = C.c_int64()
+        check(lib().cpm_mat_prof(A_el.h, C.byref(ms), C.byref(nl), 0))
+        check(lib().cpm_prof_enable(ctx.h, 0))
+        while done < snap_at:
+            sim.step(H, False)
+            done += 1
+            if done == snap_at:
+                snap = sim.get("M")
+        rows, nnz = A_el.shape[0], A_el.nnz
+        if mode == "fp64":
+            # CSR FP64: values + column indices + row pointers + p read + q written
+            kernel = "k_cg_spmv (A_el, CSR FP64)"
+            bytes_launch = 12.0 * nnz + 4.0 * (rows + 1) + 16.0 * rows
+        else:
+            # SELL-64 3x3 blocks: off-diagonal blocks 9 values + 1 column index,
+            # diagonal blocks 9 FP64 values, p read, q written
+            s_ = 8 if mode == "fast64" else 4
+            kernel = f"k_sell_spmv<3,1,{'double' if s_ == 8 else 'float'}> (A_el, SELL-64 3x3 blocks)"
+            nb = rows // 3
+            bytes_launch = (nnz / 9.0 - nb) * (9 * s_ + 4) + nb * 72.0 + 2.0 * rows * s_
+        avg_s = ms.value / 1e3 / max(nl.value, 1)
+        out["modes"][mode] = dict(elapsed=elapsed, steps=k_timed, lat=lat, iters=iters, m=snap,
+                                  spmv=dict(kernel=kernel, launches=nl.value, avg_us=avg_s * 1e6,
+                                            bytes=bytes_launch, gbs=bytes_launch / avg_s / 1e9 if nl.value else 0.0))
+        del sim
+    return out
 
 
 def cpu_baseline(n_cpu, steps=1):
@@ -157,8 +187,10 @@ def cpu_baseline(n_cpu, steps=1):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=4, help="timed steps of the headline (mixed) path")
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps-fp64", type=int, default=2, help="timed steps of the plain FP64 path")
+    ap.add_argument("--modes", default="fp64,fast64,mixed")
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--cpu-n", type=int, default=40)
     ap.add_argument("--cpu-steps", type=int, default=1)
@@ -178,34 +210,48 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    g = run_gpu(args.n, args.steps, args.warmup)
+    modes = [m for m in args.modes.split(",") if m]
+    steps64 = min(args.steps_fp64, args.steps) if "fp64" in modes else args.steps_fp64
+    g = run_gpu(args.n, args.steps, args.warmup, steps64, modes)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n, args.cpu_steps)
-    value = g["n_dof"] * args.steps / g["elapsed"]
-    lat = np.array(g["lat"]) * 1e3
+    head = "mixed" if "mixed" in modes else modes[-1]
+    hm = g["modes"][head]
+    value = g["n_dof"] * hm["steps"] / hm["elapsed"]
+    lat = np.array(hm["lat"]) * 1e3
+    per_mode = {}
+    ref_m = g["modes"]["fp64"]["m"] if "fp64" in modes else None
+    for mode, r in g["modes"].items():
+        d = dict(node_steps_per_s=g["n_dof"] * r["steps"] / r["elapsed"], ms_per_step=r["elapsed"] * 1e3 / r["steps"],
+                 cg_iters_last_step=r["iters"][-1], spmv_A_el_GBs=r["spmv"]["gbs"],
+                 spmv_A_el_avg_us=r["spmv"]["avg_us"])
+        if ref_m is not None and r["m"] is not None:
+            d["max_abs_m_err_vs_fp64"] = float((r["m"] - ref_m).abs().max().item())
+            d["err_after_steps"] = args.warmup + steps64
+        per_mode[mode] = d
+    sp = hm["spmv"]
     out = {
         "metric": "LLG node-steps/s",
         "value": value,
         "unit": "node-steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": hm["steps"],
         "warmup": args.warmup,
-        "ms_per_step": g["elapsed"] * 1e3 / args.steps,
+        "ms_per_step": hm["elapsed"] * 1e3 / hm["steps"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": {"fp64": "f64", "fast64": "f64", "mixed": "f32 CG in f64 refinement"}[head],
         "data": "synthetic",
         "config": {"workload": f"GSPM LLG step, {args.n}^3 Kuhn tet mesh, rotated-ellipsoid defect, "
                                f"magnetoelastic on, rot111, dt 10 ps, alpha 0.1, tol 1e-10",
-                   "path": "plain FP64 (reference sequence of solves)",
-                   "n_dof": g["n_dof"], "nodes": g["n_nodes"], "elements": g["n_elements"]},
+                   "path": head, "n_dof": g["n_dof"], "nodes": g["n_nodes"], "elements": g["n_elements"]},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
-        "cg_iters_last_step": g["iters"][-1],
-        "roofline": {"bound": "hbm", "kernel": "k_cg_spmv (A_el, CSR FP64)",
-                     "achieved": g["spmv_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": g["spmv_gbs"] / HBM_PEAK_GBS, "traffic": None,
-                     "avg_launch_us": g["spmv_avg_us"], "bytes_per_launch": g["spmv_bytes"],
-                     "launches": g["spmv_launches"]},
+        "speedup_vs_fp64": (per_mode[head]["node_steps_per_s"] / per_mode["fp64"]["node_steps_per_s"]
+                            if "fp64" in per_mode else None),
+        "modes": per_mode,
+        "roofline": {"bound": "hbm", "kernel": sp["kernel"], "achieved": sp["gbs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": sp["gbs"] / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_us": sp["avg_us"], "bytes_per_launch": sp["bytes"], "launches": sp["launches"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

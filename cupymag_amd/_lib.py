@@ -18,7 +18,8 @@ CPM_EINVAL, CPM_ENOMEM, CPM_EHIP, CPM_EINTERNAL, CPM_ENODEV = -1, -2, -3, -4, -5
 HA = dict(PTR=0, IDX=1, ADEMAG=2, FX=3, FY=4, FZ=5, GS_PTR=6, GS_IDX=7, A1=8, A2=9,
           F1_PTR=10, F1_IDX=11, F1=12, EL_PTR=13, EL_IDX=14, AEL=15, FE_PTR=16, FE_IDX=17, FEL=18)
 FIELD = dict(M=0, HTILDE=1, E=2, U=3, UEL=4)
-MODE_FP64 = 0
+MODE_FP64, MODE_FAST64, MODE_MIXED = 0, 1, 2
+MODES = {"fp64": MODE_FP64, "fast64": MODE_FAST64, "mixed": MODE_MIXED}
 
 
 class CpmError(RuntimeError):
@@ -36,7 +37,7 @@ class cpm_sim_params(C.Structure):
                 ("elasC", C.c_double * 3), ("lambda100", C.c_double), ("lambda111", C.c_double),
                 ("Nd", C.c_double * 3), ("Ebar_sigma", C.c_double * 6), ("R", C.c_double * 9),
                 ("Mv", C.c_double * 36), ("MvinvT", C.c_double * 36), ("mode", C.c_int32),
-                ("reserved", C.c_int32 * 7)]
+                ("reserved0", C.c_int32), ("rel_delta", C.c_double), ("reserved", C.c_int32 * 4)]
 
 
 class cpm_step_report(C.Structure):
@@ -59,6 +60,10 @@ _SIGS = {
     "cpm_cg": (C.c_int, [_P, _P, _P, _P, C.c_int, _D, C.c_int, C.POINTER(_I32), C.POINTER(_D)]),
     "cpm_prof_enable": (C.c_int, [_P, C.c_int]),
     "cpm_mat_prof": (C.c_int, [_P, C.POINTER(_D), C.POINTER(_I64), C.c_int]),
+    "cpm_mat_add_sell": (C.c_int, [_P, C.c_int, _P, _P, _P]),
+    "cpm_cg_sell": (C.c_int, [_P, _P, C.c_int, _P, _P, C.c_int, _D, C.c_int, C.c_int, _D, C.POINTER(_I32),
+                              C.POINTER(_D)]),
+    "cpm_spmv_sell": (C.c_int, [_P, _P, C.c_int, _P, _P]),
     "cpm_hd_from_u": (C.c_int, [_P, _P, _P, _P]),
     "cpm_strain_from_u": (C.c_int, [_P, _P, _P, _P, _P]),
     "cpm_build_e0": (C.c_int, [_P, _I32, _D, _D, _P, _P, _P, _P]),

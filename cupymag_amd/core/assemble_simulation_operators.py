@@ -17,8 +17,11 @@ from ..utils.volume_average import VolumeAverage
 
 
 class SimulationOperators(FEMMesh):
-    def __init__(self, P, mesh_arrays=None, ctx=None, keep_host=None, nthreads=None):
+    def __init__(self, P, mesh_arrays=None, ctx=None, keep_host=None, nthreads=None, sell=True):
+        """``sell``: also attach the SELL-64 forms the fast / mixed-precision
+        solvers run on (A_demag, A1, A2 scalar; A_el in 3x3 blocks)."""
         super().__init__(P, mesh_arrays)
+        self.sell = sell
         self.alpha1 = P.alpha
         self.alpha2 = P.alpha * P.alpha_damp
         self.ctx = ctx
@@ -36,11 +39,13 @@ class SimulationOperators(FEMMesh):
                     self.alpha1, self.alpha2, Cv, self.nthreads)
         keep = self.keep_host if self.keep_host is not None else self.n_dof <= 300_000
 
-        def up(name, vals=None):
+        def up(name, vals=None, sell_bs=0):
             ip, ix, v, shape = H.csr(name)
             if vals is not None:
                 v = vals
             M = DeviceCSR(ip, ix, v, shape, ctx)
+            if sell_bs and self.sell:
+                M.add_sell(sell_bs, ip, ix, v)
             if not keep:
                 M._host = None
             else:
@@ -48,14 +53,15 @@ class SimulationOperators(FEMMesh):
             return M
 
         d = {}
-        d["A_demag"] = up("A_demag")
+        d["A_demag"] = up("A_demag", sell_bs=1)
         F3v = np.stack([H.array("FX"), H.array("FY"), H.array("FZ")])
         d["F3"] = up("Fx", F3v)
         del F3v
-        for k in ("A1", "A2", "F1"):
-            d[k] = up(k)
+        for k in ("A1", "A2"):
+            d[k] = up(k, sell_bs=1)
+        d["F1"] = up("F1")
         if P.ME:
-            d["A_el"] = up("A_el")
+            d["A_el"] = up("A_el", sell_bs=3)
             d["F_el"] = up("F_el")
         H.free()
         self._dev = d

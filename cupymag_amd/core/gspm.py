@@ -31,7 +31,11 @@ class StepReport:
 
 
 class GSPMSimulation:
-    def __init__(self, SimOp, P, mode="fp64"):
+    def __init__(self, SimOp, P, mode="fp64", rel_delta=0.1):
+        """mode: "fp64" (CSR operators, the reference's sequence of solves),
+        "fast64" (SELL-64 operators, 3-RHS solves) or "mixed" (as fast64 with
+        FP32 CG refined in FP64 by reliable updates: an FP64 residual update
+        whenever the FP32 recurrence residual has dropped by ``rel_delta``)."""
         self.P = P
         self.SimOp = SimOp
         A_demag, Fx, Fy, Fz, self.Deriv = SimOp.csr_demag_and_deriv()
@@ -70,7 +74,9 @@ class GSPMSimulation:
             pr.Mv[k] = float(v)
         for k, v in enumerate(MinvT.ravel()):
             pr.MvinvT[k] = float(v)
-        pr.mode = {"fp64": _lib.MODE_FP64}[mode]
+        pr.mode = _lib.MODES[mode]
+        pr.rel_delta = float(rel_delta)
+        self.mode = mode
         W = self.Avg.weights_device(self.ctx)
         dmask = np.zeros(n, dtype=np.uint8)
         dmask[np.asarray(SimOp.DefDOF, dtype=np.int64)] = 1

@@ -73,7 +73,8 @@ extern "C" int cpm_ctx_destroy(cpm_ctx* c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   for (void* p : {(void*)c->partials, (void*)c->counter, (void*)c->scal, (void*)c->cg, (void*)c->r, (void*)c->p,
-                  (void*)c->q})
+                  (void*)c->q, (void*)c->r32, (void*)c->p32, (void*)c->q32, (void*)c->x32, (void*)c->b32,
+                  (void*)c->y64})
     if (p) (void)hipFree(p);
   if (c->pinned) (void)hipHostFree(c->pinned);
   if (c->ev[0])
@@ -146,6 +147,7 @@ extern "C" int cpm_mat_destroy(cpm_mat* A) {
   }
   for (void* p : {(void*)A->ptr, (void*)A->idx, (void*)A->val})
     if (p) (void)hipFree(p);
+  cpm::sell_free(A->sell);
   delete A;
   return CPM_OK;
 }

@@ -9,6 +9,7 @@
 
 #include "cupymag_hip.h"
 #include "errors.h"
+#include "sell.h"
 
 #define CPM_HIP(expr)                                                                      \
   do {                                                                                     \
@@ -27,6 +28,8 @@ constexpr int kMaxRed = 12;      // max values reduced together
 struct CgState {
   double rho, rho_old, alpha, atol, bnorm2, resid, pq, tol;
   int32_t it, done, fail, maxiter;
+  double maxr;      // mixed precision: residual norm at the last FP64 update
+  int32_t upd, pad; // mixed precision: FP64 residual update requested
 };
 
 }  // namespace cpm
@@ -49,6 +52,10 @@ struct cpm_ctx {
   // optional timing of every CG SpMV launch with HIP events (cpm_prof_enable)
   int prof = 0;
   hipEvent_t ev[2 * 256] = {};
+  // FP32 workspace of the mixed-precision solver (+ one FP64 scratch vector)
+  float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr, *b32 = nullptr;
+  double* y64 = nullptr;
+  int64_t ws32_len = 0;
 };
 
 struct cpm_mat {
@@ -60,6 +67,7 @@ struct cpm_mat {
   int32_t* idx = nullptr;
   double* val = nullptr;   // nvals * nnz
   int lanes = 8;           // threads per row of the vector-CSR kernels
+  cpm::Sell sell;          // optional SELL-64 form (cpm_mat_add_sell)
   double prof_ms = 0.0;    // accumulated CG SpMV kernel time (profiling on)
   int64_t prof_launches = 0;
 };

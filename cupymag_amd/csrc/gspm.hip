@@ -476,6 +476,8 @@ struct cpm_sim {
   double *m = nullptr, *mprev = nullptr, *f = nullptr, *ftmp = nullptr, *b = nullptr, *gn = nullptr, *gs = nullptr;
   double *mstar = nullptr, *mss = nullptr, *U = nullptr, *Ht = nullptr, *E = nullptr, *bd = nullptr;
   double *E0 = nullptr, *bel = nullptr, *u = nullptr;
+  // fast modes: interleaved right-hand side and warm-start copies of the solutions
+  double *bint = nullptr, *uint = nullptr, *gnint = nullptr, *mssint = nullptr;
   std::vector<void*> allocs;
   int hint[10] = {};
 };
@@ -505,7 +507,17 @@ extern "C" int cpm_sim_create(cpm_ctx* ctx, const cpm_sim_ops* ops, const cpm_si
   if (!ctx || !ops || !prm || !W || !out) return fail(CPM_EINVAL, "cpm_sim_create: null argument");
   const int64_t n = prm->n;
   if (n <= 0) return fail(CPM_EINVAL, "cpm_sim_create: n must be positive");
-  if (prm->mode != CPM_MODE_FP64) return fail(CPM_EINVAL, "cpm_sim_create: unknown mode");
+  if (prm->mode != CPM_MODE_FP64 && prm->mode != CPM_MODE_FAST64 && prm->mode != CPM_MODE_MIXED)
+    return fail(CPM_EINVAL, "cpm_sim_create: unknown mode");
+  if (prm->mode != CPM_MODE_FP64) {
+    for (const cpm_mat* A : {ops->A_demag, ops->A1, ops->A2})
+      if (!A->sell.sptr || A->sell.bs != 1)
+        return fail(CPM_EINVAL, "cpm_sim_create: fast modes need SELL forms (bs 1) of A_demag, A1, A2");
+    if (prm->me && (!ops->A_el->sell.sptr || ops->A_el->sell.bs != 3))
+      return fail(CPM_EINVAL, "cpm_sim_create: fast modes need the bs-3 SELL form of A_el");
+    if (prm->mode == CPM_MODE_MIXED && !(prm->rel_delta > 0.0 && prm->rel_delta < 1.0))
+      return fail(CPM_EINVAL, "cpm_sim_create: rel_delta must be in (0, 1)");
+  }
   auto sq = [&](const cpm_mat* A, int64_t rows) { return A && A->nrows == rows && A->ncols == rows; };
   if (!sq(ops->A_demag, n) || !sq(ops->A1, n) || !sq(ops->A2, n) || !sq(ops->F1, n))
     return fail(CPM_EINVAL, "cpm_sim_create: scalar operators must be n x n");
@@ -547,6 +559,12 @@ extern "C" int cpm_sim_create(cpm_ctx* ctx, const cpm_sim_ops* ops, const cpm_si
     A(&S->E0, 6 * n);
     A(&S->bel, 3 * n);
     A(&S->u, 3 * n);
+  }
+  if (prm->mode != CPM_MODE_FP64) {
+    A(&S->bint, 3 * n);
+    A(&S->gnint, 3 * n);
+    A(&S->mssint, 3 * n);
+    if (prm->me) A(&S->uint, 3 * n);
   }
   if (rc == CPM_OK) {
     hipError_t e = hipMalloc(&S->defect, std::max<int64_t>(n, 1));
@@ -647,12 +665,54 @@ extern "C" int cpm_sim_step(cpm_sim* S, const double* H, int use_init, cpm_step_
     }
     return rc;
   };
+  // fast modes: K right-hand sides (K = 3, or the 3 components of one bs-3
+  // solve) gathered into an interleaved vector, solved on the SELL form, and
+  // scattered back into the reference's SoA layout; `xint` keeps the
+  // interleaved solution as the next warm start.
+  auto solve_sell = [&](int slot, const cpm_mat* A, int K, const double* b, double* x, double* xint) -> int {
+    cpm_mat* Am = const_cast<cpm_mat*>(A);
+    const int blocks = (A->sell.bs == 3 || K == 3) ? 3 : 1;
+    const double* bb = b;
+    double* xx = x;
+    if (blocks == 3) {
+      if (int rc = interleave(ctx, n, 3, b, S->bint, 0)) return rc;
+      bb = S->bint;
+      xx = xint;
+    }
+    int32_t it[3] = {0, 0, 0};
+    double res[3] = {0, 0, 0};
+    int failed = 0;
+    int hint = 0;
+    for (int j = 0; j < K; ++j) hint = std::max(hint, S->hint[slot + j]);
+    int rc = (p.mode == CPM_MODE_MIXED)
+                 ? sell_cg_mixed(ctx, Am, K, bb, xx, use_init, p.tol, p.maxiter, p.rel_delta, 0, it, res, &failed,
+                                 nullptr)
+                 : sell_cg64(ctx, Am, K, bb, xx, use_init, p.tol, p.maxiter, hint, it, res, &failed);
+    if (rc) return rc;
+    for (int j = 0; j < K; ++j) {
+      rep->iters[slot + j] = it[j];
+      S->hint[slot + j] = it[j];
+    }
+    if (failed) {
+      int jf = 0;
+      for (int j = 1; j < K; ++j)
+        if (it[j] > it[jf]) jf = j;
+      rep->failed = slot + jf + 1;
+      rep->resid = res[jf];
+      return std::max(1, (int)it[jf]);
+    }
+    if (blocks == 3)
+      if (int rc2 = interleave(ctx, n, 3, xint, x, 1)) return rc2;
+    return CPM_OK;
+  };
+  const bool fast = p.mode != CPM_MODE_FP64;
   int rc;
   const cpm_mat* F3 = S->ops.F3;
   if (p.me) {
     k_e0<<<gv, kBlock, 0, s>>>(n, S->m, p.lambda100, p.lambda111, p.rot111, dR, dMv, S->E0);
     if ((rc = launch_spmv(ctx, S->ops.F_el, 0, S->E0, S->bel))) return rc;
-    if ((rc = solve(0, S->ops.A_el, S->bel, S->u))) return rc;
+    if ((rc = fast ? solve_sell(0, S->ops.A_el, 1, S->bel, S->u, S->uint) : solve(0, S->ops.A_el, S->bel, S->u)))
+      return rc;
     CPM_G_DISPATCH(F3->lanes, k_strain, n, s, (int32_t)n, F3->ptr, F3->idx, F3->val, F3->nnz, S->u, p.rot111, dR,
                    S->E);
     k_strain_avg<<<gv, kBlock, 0, s>>>(n, S->W, S->vol, S->E, S->m, p.rot111, dR, p.lambda100, p.lambda111, dEs, dMv,
@@ -661,24 +721,34 @@ extern "C" int cpm_sim_step(cpm_sim* S, const double* H, int use_init, cpm_step_
   }
   CPM_G_DISPATCH(F3->lanes, k_demag_rhs, n, s, (int32_t)n, F3->ptr, F3->idx, F3->val, F3->nnz, S->m,
                  S->dscal + S_AVGM, S->bd);
-  if ((rc = solve(1, S->ops.A_demag, S->bd, S->U))) return rc;
+  if ((rc = fast ? solve_sell(1, S->ops.A_demag, 1, S->bd, S->U, nullptr) : solve(1, S->ops.A_demag, S->bd, S->U)))
+    return rc;
   CPM_G_DISPATCH(F3->lanes, k_hd, n, s, (int32_t)n, F3->ptr, F3->idx, F3->val, F3->nnz, S->U, S->Ht);
   k_field1<<<gv, kBlock, 0, s>>>(n, P, S->m, S->E, p.me, S->Ht, S->dscal + S_AVGM, S->f, S->ftmp);
   for (int c = 0; c < 3; ++c)
     if ((rc = launch_spmv(ctx, S->ops.F1, 0, S->ftmp + c * n, S->b + c * n))) return rc;
-  for (int c = 0; c < 3; ++c)
-    if ((rc = solve(2 + c, S->ops.A1, S->b + c * n, S->gn + c * n))) return rc;
+  if (fast) {
+    if ((rc = solve_sell(2, S->ops.A1, 3, S->b, S->gn, S->gnint))) return rc;
+  } else {
+    for (int c = 0; c < 3; ++c)
+      if ((rc = solve(2 + c, S->ops.A1, S->b + c * n, S->gn + c * n))) return rc;
+  }
   k_gs1<<<gv, kBlock, 0, s>>>(n, p.dt, S->m, S->gn, S->f, S->mstar, S->ftmp);
   if ((rc = launch_spmv(ctx, S->ops.F1, 0, S->ftmp, S->b))) return rc;
-  if ((rc = solve(5, S->ops.A1, S->b, S->gs))) return rc;
+  if ((rc = fast ? solve_sell(5, S->ops.A1, 1, S->b, S->gs, nullptr) : solve(5, S->ops.A1, S->b, S->gs))) return rc;
   k_gs2<<<gv, kBlock, 0, s>>>(n, p.dt, S->m, S->gn, S->gs, S->f, S->mstar, S->ftmp);
   if ((rc = launch_spmv(ctx, S->ops.F1, 0, S->ftmp, S->b))) return rc;
-  if ((rc = solve(6, S->ops.A1, S->b, S->gs + n))) return rc;
+  if ((rc = fast ? solve_sell(6, S->ops.A1, 1, S->b, S->gs + n, nullptr) : solve(6, S->ops.A1, S->b, S->gs + n)))
+    return rc;
   k_gs3_field2<<<gv, kBlock, 0, s>>>(n, P, S->m, S->gs, S->mstar, S->E, p.me, S->Ht, S->dscal + S_AVGM, S->ftmp);
   for (int c = 0; c < 3; ++c)
     if ((rc = launch_spmv(ctx, S->ops.F1, 0, S->ftmp + c * n, S->b + c * n))) return rc;
-  for (int c = 0; c < 3; ++c)
-    if ((rc = solve(7 + c, S->ops.A2, S->b + c * n, S->mss + c * n))) return rc;
+  if (fast) {
+    if ((rc = solve_sell(7, S->ops.A2, 3, S->b, S->mss, S->mssint))) return rc;
+  } else {
+    for (int c = 0; c < 3; ++c)
+      if ((rc = solve(7 + c, S->ops.A2, S->b + c * n, S->mss + c * n))) return rc;
+  }
   k_project<<<gv, kBlock, 0, s>>>(n, S->mss, S->defect, S->W, S->vol, S->m, S->mprev, S->dscal, ctx->partials,
                                   ctx->counter);
   CPM_HIP(hipGetLastError());

@@ -1,0 +1,31 @@
+// Sliced-ELL (SELL-64) operator form and its solvers; see sell.hip.
+#pragma once
+#include <cstdint>
+
+struct cpm_ctx;
+struct cpm_mat;
+
+namespace cpm {
+
+struct Sell {
+  int bs = 0;               // block size (1 scalar, 3 for the elasticity operator)
+  int32_t nbrows = 0;       // block rows
+  int32_t nslices = 0;      // ceil(nbrows / 64)
+  int64_t nslot = 0;        // total slots; slice s owns slots [sptr[s], sptr[s+1])
+  int64_t* sptr = nullptr;  // device
+  int32_t* col = nullptr;   // device, [slot][64]
+  double* v64 = nullptr;    // device, [slot][bs*bs][64], off-diagonal blocks
+  float* v32 = nullptr;     // device, FP32 copy of v64
+  double* dg64 = nullptr;   // device, [brow][bs*bs] diagonal blocks
+  double* dgc = nullptr;    // device, diagonal blocks of the FP32 form, compensated (see sell.hip)
+};
+
+void sell_free(Sell& S);
+int sell_spmv64(cpm_ctx* ctx, const cpm_mat* M, int K, const double* x, double* y);
+int sell_cg64(cpm_ctx* ctx, cpm_mat* M, int K, const double* b, double* x, int use_x0, double tol, int maxiter,
+              int hint, int32_t* iters, double* resid, int* failed);
+int sell_cg_mixed(cpm_ctx* ctx, cpm_mat* M, int K, const double* b, double* x, int use_x0, double tol, int maxiter,
+                  double delta, int hint, int32_t* iters, double* resid, int* failed, int32_t* sweeps);
+int interleave(cpm_ctx* ctx, int64_t n, int K, const double* src, double* dst, int inverse);
+
+}  // namespace cpm

@@ -1,0 +1,798 @@
+// Sliced-ELL operators and the multi-RHS / mixed-precision CG.
+//
+// SELL-64: rows are grouped in slices of 64 (one wavefront); within a slice
+// every row owns `w` slots (w = longest row of the slice, short rows padded
+// with zero values pointing at column 0), stored slot-major with the 64
+// lanes contiguous, so each load of a wavefront is one 512 B (FP64) or
+// 256 B (FP32) coalesced segment.  One thread per row, no shuffles.
+//
+// The diagonal block of every row is kept apart, in FP64, for both the FP64
+// and the FP32 form.  In the FP32 form it is compensated so that the row sums
+// of each displacement component (the action on rigid translations, resp.
+// on constants for scalar operators) equal those of the FP64 operator: the
+// rounding of the off-diagonal values to FP32 is added back to the diagonal.
+// The operators are anchored at one node only, so their smallest eigenvalue
+// (the near-translation mode) scales like 1/n^3; without the compensation
+// the FP32 rounding of the row sums is of that size at 128^3 and the FP32
+// solves no longer improve that mode.
+//
+// Block size BS = 3 stores the 3x3 blocks of the elasticity operator
+// (assemble_elasticity.py:389-398) with one column index per block and the
+// vectors interleaved node-major (3i + d) instead of the reference's
+// [ux; uy; uz]; the solver permutes in and out around the solve.
+//
+// The CG here is the same recurrence as cg.hip (CuPy's cupyx cg: rho = r.r,
+// p = r + (rho/rho_old) p, alpha = rho/(p.q), stop when ||r|| <= tol*||b||
+// after the update) for K right-hand sides that share the operator: every
+// column keeps its own scalars and stops independently, and the operator is
+// streamed from HBM once per iteration for all of them (the three m
+// components of Micromagnetics.py:221-223 and :281-290 share A1 / A2).
+//
+// The mixed-precision solver runs that CG with FP32 values and vectors (dot
+// products accumulated in FP64) and refines in FP64 by reliable updates: the
+// FP32 correction is periodically folded into the FP64 solution and the
+// residual replaced by the FP64 b - A x, until that residual meets tol*||b||.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "device.h"
+#include "sell.h"
+
+namespace cpm {
+
+// ---------------------------------------------------------------- building
+
+int sell_build(cpm_ctx* ctx, Sell& S, int bs, int32_t nbrows, const int64_t* indptr, const int32_t* indices,
+               const double* values, int64_t n_csr_rows) {
+  // Block-row b covers CSR rows {d*nbrows + b : d < bs} (bs = 3: the SoA DOF
+  // order of the elasticity operator).  Its block columns are the columns
+  // < nbrows of CSR row b; entry (d, e) of block (b, c) sits at CSR row
+  // d*nbrows + b, column e*nbrows + c.
+  if (bs != 1 && bs != 3) return fail(CPM_EINVAL, "SELL block size must be 1 or 3");
+  if ((int64_t)bs * nbrows != n_csr_rows) return fail(CPM_EINVAL, "SELL: rows do not match the block size");
+  const int64_t nsl = (nbrows + 63) / 64;
+  std::vector<int64_t> sptr(nsl + 1, 0);
+  std::vector<int32_t> blen(nbrows);
+  std::vector<int32_t> slots(nbrows);
+  for (int32_t b = 0; b < nbrows; ++b) {
+    int64_t len = 0, diag = 0;
+    for (int64_t p = indptr[b]; p < indptr[b + 1]; ++p) {
+      len += indices[p] < nbrows;
+      diag += indices[p] == b;
+    }
+    blen[b] = (int32_t)len;
+    slots[b] = (int32_t)(len - diag);
+    if (bs == 3)
+      for (int d = 0; d < 3; ++d)
+        if (indptr[d * nbrows + b + 1] - indptr[d * nbrows + b] != 3 * len)
+          return fail(CPM_EINVAL, "SELL: block row is not a full 3x3 block pattern");
+  }
+  for (int64_t s = 0; s < nsl; ++s) {
+    int32_t w = 0;
+    for (int64_t b = s * 64; b < std::min<int64_t>(nbrows, s * 64 + 64); ++b) w = std::max(w, slots[b]);
+    sptr[s + 1] = sptr[s] + w;
+  }
+  const int64_t nslot = sptr[nsl];
+  const int bb = bs * bs;
+  std::vector<int32_t> col(nslot * 64, 0);
+  std::vector<double> val(nslot * 64 * bb, 0.0);
+  std::vector<double> dg(std::max<int64_t>(1, (int64_t)nbrows * bb), 0.0), dgc(dg.size(), 0.0);
+  for (int32_t b = 0; b < nbrows; ++b) {
+    const int64_t s = b / 64, l = b % 64;
+    const int64_t len = blen[b];
+    double comp[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // sum over off-diagonal blocks of (a - fp32(a))
+    int64_t j = 0;
+    for (int64_t p = indptr[b]; p < indptr[b + 1]; ++p) {
+      const int32_t c = indices[p];
+      if (c >= nbrows) continue;
+      const int64_t k = p - indptr[b];  // position of c inside the row segment
+      double blk[9];
+      for (int d = 0; d < bs; ++d)
+        for (int e = 0; e < bs; ++e) {
+          const int64_t q = (bs == 1) ? p : indptr[d * nbrows + b] + e * len + k;
+          if (indices[q] != e * nbrows + c) return fail(CPM_EINVAL, "SELL: unexpected 3x3 block layout");
+          blk[d * bs + e] = values[q];
+        }
+      if (c == b) {
+        for (int t = 0; t < bb; ++t) dg[(int64_t)b * bb + t] = blk[t];
+        continue;
+      }
+      col[(sptr[s] + j) * 64 + l] = c;
+      for (int t = 0; t < bb; ++t) {
+        val[((sptr[s] + j) * bb + t) * 64 + l] = blk[t];
+        comp[t] += blk[t] - (double)(float)blk[t];
+      }
+      ++j;
+    }
+    for (int t = 0; t < bb; ++t) dgc[(int64_t)b * bb + t] = dg[(int64_t)b * bb + t] + comp[t];
+  }
+  S.bs = bs;
+  S.nbrows = nbrows;
+  S.nslices = (int32_t)nsl;
+  S.nslot = nslot;
+  std::vector<float> v32(val.size());
+  for (size_t i = 0; i < val.size(); ++i) v32[i] = (float)val[i];
+  hipStream_t st = ctx->stream;
+  CPM_HIP(hipMalloc(&S.sptr, sizeof(int64_t) * (nsl + 1)));
+  CPM_HIP(hipMalloc(&S.col, sizeof(int32_t) * std::max<int64_t>(1, col.size())));
+  CPM_HIP(hipMalloc(&S.v64, sizeof(double) * std::max<size_t>(1, val.size())));
+  CPM_HIP(hipMalloc(&S.v32, sizeof(float) * std::max<size_t>(1, val.size())));
+  CPM_HIP(hipMalloc(&S.dg64, sizeof(double) * dg.size()));
+  CPM_HIP(hipMalloc(&S.dgc, sizeof(double) * dgc.size()));
+  CPM_HIP(hipMemcpyAsync(S.dg64, dg.data(), sizeof(double) * dg.size(), hipMemcpyHostToDevice, st));
+  CPM_HIP(hipMemcpyAsync(S.dgc, dgc.data(), sizeof(double) * dgc.size(), hipMemcpyHostToDevice, st));
+  CPM_HIP(hipMemcpyAsync(S.sptr, sptr.data(), sizeof(int64_t) * (nsl + 1), hipMemcpyHostToDevice, st));
+  CPM_HIP(hipMemcpyAsync(S.col, col.data(), sizeof(int32_t) * col.size(), hipMemcpyHostToDevice, st));
+  CPM_HIP(hipMemcpyAsync(S.v64, val.data(), sizeof(double) * val.size(), hipMemcpyHostToDevice, st));
+  CPM_HIP(hipMemcpyAsync(S.v32, v32.data(), sizeof(float) * v32.size(), hipMemcpyHostToDevice, st));
+  CPM_HIP(hipStreamSynchronize(st));
+  return CPM_OK;
+}
+
+void sell_free(Sell& S) {
+  for (void* p : {(void*)S.sptr, (void*)S.col, (void*)S.v64, (void*)S.v32, (void*)S.dg64, (void*)S.dgc})
+    if (p) (void)hipFree(p);
+  S = Sell{};
+}
+
+// ---------------------------------------------------------------- kernels
+
+// y = A x (+ fused per-column dot x.y when Dot) for BS x BS blocks and K
+// interleaved right-hand sides: x[(c*BS + e)*K + j].
+template <int BS, int K, class T, bool CG, bool GATE = false>
+__global__ void __launch_bounds__(kBlock) k_sell_spmv(int32_t nbrows, const int64_t* __restrict__ sptr,
+                                                      const int32_t* __restrict__ col, const T* __restrict__ val,
+                                                      const double* __restrict__ dg, const T* __restrict__ x,
+                                                      T* __restrict__ y, CgState* st, double* partials,
+                                                      unsigned* counter) {
+  constexpr int BB = BS * BS;
+  constexpr int NV = BS * K;
+  bool active[K];
+  bool any = false;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    active[j] = CG ? !st[j].done : (GATE ? st[j].upd != 0 : true);
+    any |= active[j];
+  }
+  if (!any) return;
+  double acc[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) acc[j] = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t row = (int64_t)blockIdx.x * kBlock + threadIdx.x; row < (int64_t)((nbrows + 63) & ~63); row += stride) {
+    const int64_t s = row >> 6, l = row & 63;
+    const int64_t s0 = sptr[s], s1 = sptr[s + 1];
+    double sum[NV];  // FP64 accumulation for both value types
+#pragma unroll
+    for (int v = 0; v < NV; ++v) sum[v] = 0.0;
+    for (int64_t slot = s0; slot < s1; ++slot) {
+      const int64_t c = col[slot * 64 + l];
+      T xv[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) xv[v] = x[c * NV + v];
+#pragma unroll
+      for (int d = 0; d < BS; ++d)
+#pragma unroll
+        for (int e = 0; e < BS; ++e) {
+          const double a = (double)val[(slot * BB + d * BS + e) * 64 + l];
+#pragma unroll
+          for (int j = 0; j < K; ++j) sum[d * K + j] += a * (double)xv[e * K + j];
+        }
+    }
+    if (row < nbrows) {
+      T xo[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) xo[v] = x[row * NV + v];
+#pragma unroll
+      for (int d = 0; d < BS; ++d)
+#pragma unroll
+        for (int e = 0; e < BS; ++e) {
+          const double a = dg[row * BB + d * BS + e];
+#pragma unroll
+          for (int j = 0; j < K; ++j) sum[d * K + j] += a * (double)xo[e * K + j];
+        }
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        const T yv = (T)sum[v];
+        y[row * NV + v] = yv;
+        if (CG) acc[v % K] += (double)xo[v] * (double)yv;
+      }
+    }
+  }
+  if (!CG) return;
+  if (grid_sum<K>(acc, partials, counter) && threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+      if (active[j]) {
+        st[j].pq = acc[j];
+        st[j].alpha = st[j].rho / acc[j];
+      }
+  }
+}
+
+template <int K, class T>
+__global__ void __launch_bounds__(kBlock) k_mcg_pupd(int64_t n, const T* __restrict__ r, T* __restrict__ p,
+                                                     const CgState* st) {
+  double beta[K];
+  bool live[K], first[K];
+  bool any = false;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    live[j] = !st[j].done;
+    first[j] = st[j].it == 0;
+    beta[j] = (live[j] && !first[j]) ? st[j].rho / st[j].rho_old : 0.0;
+    any |= live[j];
+  }
+  if (!any) return;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const int j = (int)(i % K);
+    if (!live[j]) continue;
+    p[i] = first[j] ? r[i] : (T)(r[i] + (T)beta[j] * p[i]);
+  }
+}
+
+template <int K, class T>
+__global__ void __launch_bounds__(kBlock) k_mcg_xr(int64_t n, T* __restrict__ x, T* __restrict__ r,
+                                                   const T* __restrict__ p, const T* __restrict__ q, CgState* st,
+                                                   double* partials, unsigned* counter) {
+  double a[K];
+  bool live[K];
+  bool any = false;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    live[j] = !st[j].done;
+    a[j] = live[j] ? st[j].alpha : 0.0;
+    any |= live[j];
+  }
+  if (!any) return;
+  double acc[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) acc[j] = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const int j = (int)(i % K);
+    if (!live[j]) continue;
+    const T aj = (T)a[j];
+    x[i] = x[i] + aj * p[i];
+    const T ri = r[i] - aj * q[i];
+    r[i] = ri;
+    acc[j] += (double)ri * (double)ri;
+  }
+  if (grid_sum<K>(acc, partials, counter) && threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      if (!live[j]) continue;
+      CgState& s = st[j];
+      s.rho_old = s.rho;
+      s.rho = acc[j];
+      s.it += 1;
+      s.resid = sqrt(acc[j]);
+      if (s.resid <= s.atol) {
+        s.done = 1;
+      } else if (s.it >= s.maxiter) {
+        s.done = 1;
+        s.fail = 1;
+      }
+    }
+  }
+}
+
+// CG start: x = 0, r = b (or r = b - q with q = A x0); per-column ||b||^2, r.r.
+template <int K, class T>
+__global__ void __launch_bounds__(kBlock) k_mcg_init(int64_t n, const T* __restrict__ b, T* __restrict__ x,
+                                                     T* __restrict__ r, const T* __restrict__ q, int use_x0,
+                                                     double tol, int32_t maxiter, CgState* st, double* partials,
+                                                     unsigned* counter) {
+  double acc[2 * K];
+#pragma unroll
+  for (int j = 0; j < 2 * K; ++j) acc[j] = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const int j = (int)(i % K);
+    const T bi = b[i];
+    T ri;
+    if (use_x0) {
+      ri = bi - q[i];
+    } else {
+      x[i] = 0;
+      ri = bi;
+    }
+    r[i] = ri;
+    acc[j] += (double)bi * (double)bi;
+    acc[K + j] += (double)ri * (double)ri;
+  }
+  if (grid_sum<2 * K>(acc, partials, counter) && threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      CgState& s = st[j];
+      s.bnorm2 = acc[j];
+      s.rho = acc[K + j];
+      s.rho_old = 0.0;
+      s.tol = tol;
+      s.atol = tol * sqrt(acc[j]);
+      s.it = 0;
+      s.fail = 0;
+      s.maxiter = maxiter;
+      s.resid = sqrt(acc[K + j]);
+      s.done = (acc[j] == 0.0 || maxiter <= 0) ? 1 : 0;
+    }
+  }
+}
+
+template <int K, class T>
+__global__ void __launch_bounds__(kBlock) k_mcg_zero_b(int64_t n, T* __restrict__ x, const CgState* st) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+    if (st[i % K].bnorm2 == 0.0) x[i] = 0;
+}
+
+// ---- refinement helpers (FP64 outer loop) -----------------------------------
+
+
+
+
+// x[i] = 0 for every i (K == 0) or for the entries of column `col` of K
+__global__ void __launch_bounds__(kBlock) k_fill0(int64_t n, double* __restrict__ x, int K = 0, int col = 0) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+    if (K == 0 || i % K == col) x[i] = 0.0;
+}
+
+// ---------------------------------------------------------------- dispatch
+
+template <int BS, int K, class T, bool CG>
+static void launch_sell_t(cpm_ctx* ctx, const Sell& A, const T* val, const T* x, T* y, CgState* st) {
+  const int64_t rows = ((int64_t)A.nbrows + 63) & ~63;
+  const int grid = grid_for(rows);
+  k_sell_spmv<BS, K, T, CG><<<grid, kBlock, 0, ctx->stream>>>(A.nbrows, A.sptr, A.col, val,
+                                                             sizeof(T) == 8 ? A.dg64 : A.dgc, x, y, st,
+                                                             ctx->partials, ctx->counter);
+}
+
+template <class T>
+static int launch_sell(cpm_ctx* ctx, const Sell& A, int K, bool cg, const T* x, T* y, CgState* st) {
+  const T* val;
+  if constexpr (sizeof(T) == 8) val = A.v64; else val = A.v32;
+#define CPM_SELL_CASE(BS_, K_)                                         \
+  if (A.bs == BS_ && K == K_) {                                        \
+    if (cg) launch_sell_t<BS_, K_, T, true>(ctx, A, val, x, y, st);    \
+    else launch_sell_t<BS_, K_, T, false>(ctx, A, val, x, y, st);      \
+    return CPM_OK;                                                     \
+  }
+  CPM_SELL_CASE(1, 1)
+  CPM_SELL_CASE(1, 3)
+  CPM_SELL_CASE(3, 1)
+#undef CPM_SELL_CASE
+  return fail(CPM_EINVAL, "SELL: unsupported (block size, rhs count)");
+}
+
+template <int K, class T>
+static void launch_vec(cpm_ctx* ctx, int which, int64_t n, const T* b, T* x, T* r, T* p, T* q, int use_x0, double tol,
+                       int maxiter, CgState* st) {
+  const int g = grid_for(n);
+  hipStream_t s = ctx->stream;
+  switch (which) {
+    case 0:
+      k_mcg_init<K, T><<<g, kBlock, 0, s>>>(n, b, x, r, q, use_x0, tol, maxiter, st, ctx->partials, ctx->counter);
+      k_mcg_zero_b<K, T><<<g, kBlock, 0, s>>>(n, x, st);
+      break;
+    case 1: k_mcg_pupd<K, T><<<g, kBlock, 0, s>>>(n, r, p, st); break;
+    default: k_mcg_xr<K, T><<<g, kBlock, 0, s>>>(n, x, r, p, q, st, ctx->partials, ctx->counter); break;
+  }
+}
+
+template <class T>
+static void vec_op(cpm_ctx* ctx, int K, int which, int64_t n, const T* b, T* x, T* r, T* p, T* q, int use_x0,
+                   double tol, int maxiter, CgState* st) {
+  if (K == 1) launch_vec<1, T>(ctx, which, n, b, x, r, p, q, use_x0, tol, maxiter, st);
+  else launch_vec<3, T>(ctx, which, n, b, x, r, p, q, use_x0, tol, maxiter, st);
+}
+
+static int read_states(cpm_ctx* ctx, int K, CgState* h) {
+  CPM_HIP(hipMemcpyAsync(ctx->pinned, ctx->cg, K * sizeof(CgState), hipMemcpyDeviceToHost, ctx->stream));
+  CPM_HIP(hipStreamSynchronize(ctx->stream));
+  std::memcpy(h, ctx->pinned, K * sizeof(CgState));
+  return CPM_OK;
+}
+
+// Batched-launch CG on K interleaved columns (vectors of n = nbrows*BS*K).
+template <class T>
+static int mcg_core(cpm_ctx* ctx, const Sell& A, cpm_mat* prof, int K, const T* b, T* x, T* r, T* p, T* q,
+                    int use_x0, double tol, int maxiter, int hint, int32_t* iters, double* resid, int* failed) {
+  const int64_t n = (int64_t)A.nbrows * A.bs * K;
+  CgState* st = ctx->cg;
+  if (use_x0)
+    if (int rc = launch_sell<T>(ctx, A, K, false, x, q, st)) return rc;
+  vec_op<T>(ctx, K, 0, n, b, x, r, p, q, use_x0, tol, maxiter, st);
+  CPM_HIP(hipGetLastError());
+  CgState h[3] = {};
+  int batch = hint > 2 ? hint - 1 : 8;
+  const int max_batch = ctx->prof ? 256 : 4096;
+  int issued = 0;
+  for (;;) {
+    const int todo = std::max(1, std::min(std::min(batch, max_batch), maxiter - issued));
+    int it_before = 0;
+    for (int j = 0; j < K; ++j) it_before = std::max(it_before, h[j].it);
+    for (int k = 0; k < todo; ++k) {
+      vec_op<T>(ctx, K, 1, n, b, x, r, p, q, 0, 0.0, 0, st);
+      if (ctx->prof) CPM_HIP(hipEventRecord(ctx->ev[2 * k], ctx->stream));
+      if (int rc = launch_sell<T>(ctx, A, K, true, p, q, st)) return rc;
+      if (ctx->prof) CPM_HIP(hipEventRecord(ctx->ev[2 * k + 1], ctx->stream));
+      vec_op<T>(ctx, K, 2, n, b, x, r, p, q, 0, 0.0, 0, st);
+    }
+    issued += todo;
+    CPM_HIP(hipGetLastError());
+    if (int rc = read_states(ctx, K, h)) return rc;
+    int it_max = 0;
+    bool all_done = true;
+    for (int j = 0; j < K; ++j) {
+      it_max = std::max(it_max, h[j].it);
+      all_done &= h[j].done != 0;
+    }
+    if (ctx->prof && prof) {
+      for (int k = 0; k < it_max - it_before && k < todo; ++k) {
+        float ms = 0.f;
+        CPM_HIP(hipEventElapsedTime(&ms, ctx->ev[2 * k], ctx->ev[2 * k + 1]));
+        prof->prof_ms += ms;
+        prof->prof_launches += 1;
+      }
+    }
+    if (all_done || issued >= maxiter) break;
+    batch = std::max(2, std::min(64, it_max / 4 + 2));
+  }
+  *failed = 0;
+  for (int j = 0; j < K; ++j) {
+    iters[j] = h[j].it;
+    resid[j] = h[j].resid;
+    if (h[j].fail || !h[j].done) *failed = 1;
+  }
+  return CPM_OK;
+}
+
+int sell_cg64(cpm_ctx* ctx, cpm_mat* M, int K, const double* b, double* x, int use_x0, double tol, int maxiter,
+              int hint, int32_t* iters, double* resid, int* failed) {
+  const Sell& A = M->sell;
+  const int64_t n = (int64_t)A.nbrows * A.bs * K;
+  if (int rc = ensure_workspace(ctx, n)) return rc;
+  return mcg_core<double>(ctx, A, M, K, b, x, ctx->r, ctx->p, ctx->q, use_x0, tol, maxiter, hint, iters, resid,
+                          failed);
+}
+
+int ensure_workspace32(cpm_ctx* ctx, int64_t n) {
+  if (n <= ctx->ws32_len) return CPM_OK;
+  for (void** b : {(void**)&ctx->r32, (void**)&ctx->p32, (void**)&ctx->q32, (void**)&ctx->x32, (void**)&ctx->b32,
+                   (void**)&ctx->y64})
+    if (*b) {
+      CPM_HIP(hipFree(*b));
+      *b = nullptr;
+    }
+  CPM_HIP(hipMalloc(&ctx->r32, sizeof(float) * n));
+  CPM_HIP(hipMalloc(&ctx->p32, sizeof(float) * n));
+  CPM_HIP(hipMalloc(&ctx->q32, sizeof(float) * n));
+  CPM_HIP(hipMalloc(&ctx->x32, sizeof(float) * n));
+  CPM_HIP(hipMalloc(&ctx->b32, sizeof(float) * n));
+  CPM_HIP(hipMalloc(&ctx->y64, sizeof(double) * n));
+  ctx->ws32_len = n;
+  return CPM_OK;
+}
+
+// ---- mixed precision with reliable updates ----------------------------------
+//
+// FP32 CG recurrence (x32, r32, p32) on the FP32 operator; whenever the
+// recurrence residual has dropped by `delta` since the last update (or meets
+// the tolerance, or the iteration budget ends) the FP32 correction is folded
+// into the FP64 solution and the residual is replaced by the true FP64
+// residual b - A x64.  The search direction is kept across updates, so the
+// iteration count stays close to that of FP64 CG; convergence is declared
+// on the FP64 residual only.
+
+template <int K>
+__global__ void __launch_bounds__(kBlock) k_rel_init(int64_t n, const double* __restrict__ b, double* __restrict__ x,
+                                                     const double* __restrict__ y, double* __restrict__ r64,
+                                                     float* __restrict__ r32, float* __restrict__ x32, int use_x0,
+                                                     double tol, int32_t maxiter, CgState* st, double* partials,
+                                                     unsigned* counter) {
+  double acc[2 * K];
+#pragma unroll
+  for (int j = 0; j < 2 * K; ++j) acc[j] = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const int j = (int)(i % K);
+    const double bi = b[i];
+    double ri;
+    if (use_x0) {
+      ri = bi - y[i];
+    } else {
+      x[i] = 0.0;
+      ri = bi;
+    }
+    r64[i] = ri;
+    r32[i] = (float)ri;
+    x32[i] = 0.0f;
+    acc[j] += bi * bi;
+    acc[K + j] += ri * ri;
+  }
+  if (grid_sum<2 * K>(acc, partials, counter) && threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      CgState& s = st[j];
+      s.bnorm2 = acc[j];
+      s.rho = acc[K + j];
+      s.rho_old = 0.0;
+      s.tol = tol;
+      s.atol = tol * sqrt(acc[j]);
+      s.it = 0;
+      s.fail = 0;
+      s.maxiter = maxiter;
+      s.resid = sqrt(acc[K + j]);
+      s.maxr = s.resid;
+      s.upd = 0;
+      s.done = (acc[j] == 0.0 || s.resid <= s.atol || maxiter <= 0) ? 1 : 0;
+    }
+  }
+}
+
+template <int K>
+__global__ void __launch_bounds__(kBlock) k_rel_xr(int64_t n, float* __restrict__ x, float* __restrict__ r,
+                                                   const float* __restrict__ p, const float* __restrict__ q,
+                                                   double delta, CgState* st, double* partials, unsigned* counter) {
+  double a[K];
+  bool live[K];
+  bool any = false;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    live[j] = !st[j].done;
+    a[j] = live[j] ? st[j].alpha : 0.0;
+    any |= live[j];
+  }
+  if (!any) return;
+  double acc[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) acc[j] = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const int j = (int)(i % K);
+    if (!live[j]) continue;
+    const float aj = (float)a[j];
+    x[i] = x[i] + aj * p[i];
+    const float ri = r[i] - aj * q[i];
+    r[i] = ri;
+    acc[j] += (double)ri * (double)ri;
+  }
+  if (grid_sum<K>(acc, partials, counter) && threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      if (!live[j]) continue;
+      CgState& s = st[j];
+      s.rho_old = s.rho;
+      s.rho = acc[j];
+      s.it += 1;
+      s.resid = sqrt(acc[j]);
+      if (s.resid < delta * s.maxr || s.resid <= s.atol || s.it >= s.maxiter) s.upd = 1;
+    }
+  }
+}
+
+// x64 += x32, x32 = 0 on the columns that requested an FP64 update
+template <int K>
+__global__ void __launch_bounds__(kBlock) k_rel_x(int64_t n, double* __restrict__ x, float* __restrict__ x32,
+                                                  const CgState* st) {
+  bool upd[K];
+  bool any = false;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    upd[j] = st[j].upd != 0;
+    any |= upd[j];
+  }
+  if (!any) return;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const int j = (int)(i % K);
+    if (!upd[j]) continue;
+    x[i] += (double)x32[i];
+    x32[i] = 0.0f;
+  }
+}
+
+// r64 = b - y (y = A64 x64), r32 = r64, convergence on the FP64 residual
+template <int K>
+__global__ void __launch_bounds__(kBlock) k_rel_r(int64_t n, const double* __restrict__ b, const double* __restrict__ y,
+                                                  double* __restrict__ r64, float* __restrict__ r32, CgState* st,
+                                                  double* partials, unsigned* counter) {
+  bool upd[K];
+  bool any = false;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    upd[j] = st[j].upd != 0;
+    any |= upd[j];
+  }
+  if (!any) return;
+  double acc[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) acc[j] = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const int j = (int)(i % K);
+    if (!upd[j]) continue;
+    const double ri = b[i] - y[i];
+    r64[i] = ri;
+    r32[i] = (float)ri;
+    acc[j] += ri * ri;
+  }
+  if (grid_sum<K>(acc, partials, counter) && threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      if (!upd[j]) continue;
+      CgState& s = st[j];
+      s.upd = 0;
+      s.rho = acc[j];
+      s.resid = sqrt(acc[j]);
+      s.maxr = s.resid;
+      if (s.resid <= s.atol) {
+        s.done = 1;
+      } else if (s.it >= s.maxiter) {
+        s.done = 1;
+        s.fail = 1;
+      }
+    }
+  }
+}
+
+template <int K>
+static int rel_loop(cpm_ctx* ctx, cpm_mat* M, const double* b, double* x, int use_x0, double tol, int maxiter,
+                    double delta, int hint, int32_t* iters, double* resid, int* failed) {
+  const Sell& A = M->sell;
+  const int64_t n = (int64_t)A.nbrows * A.bs * K;
+  const int g = grid_for(n);
+  const int64_t rows = ((int64_t)A.nbrows + 63) & ~63;
+  const int gs = grid_for(rows);
+  hipStream_t s = ctx->stream;
+  CgState* st = ctx->cg;
+  double* y64 = ctx->y64;
+  constexpr int BS1 = 1;
+  if (use_x0) {
+    if (A.bs == 3) k_sell_spmv<3, 1, double, false><<<gs, kBlock, 0, s>>>(A.nbrows, A.sptr, A.col, A.v64, A.dg64, x, y64, st, ctx->partials, ctx->counter);
+    else k_sell_spmv<BS1, K, double, false><<<gs, kBlock, 0, s>>>(A.nbrows, A.sptr, A.col, A.v64, A.dg64, x, y64, st, ctx->partials, ctx->counter);
+  }
+  k_rel_init<K><<<g, kBlock, 0, s>>>(n, b, x, y64, ctx->r, ctx->r32, ctx->x32, use_x0, tol, maxiter, st, ctx->partials,
+                                     ctx->counter);
+  k_mcg_zero_b<K, double><<<g, kBlock, 0, s>>>(n, x, st);
+  CPM_HIP(hipGetLastError());
+  CgState h[3] = {};
+  int batch = hint > 2 ? hint - 1 : 8;
+  const int max_batch = ctx->prof ? 256 : 4096;
+  int issued = 0;
+  for (;;) {
+    const int todo = std::max(1, std::min(std::min(batch, max_batch), maxiter - issued + 1));
+    int it_before = 0;
+    for (int j = 0; j < K; ++j) it_before = std::max(it_before, h[j].it);
+    for (int k = 0; k < todo; ++k) {
+      k_mcg_pupd<K, float><<<g, kBlock, 0, s>>>(n, ctx->r32, ctx->p32, st);
+      if (ctx->prof) CPM_HIP(hipEventRecord(ctx->ev[2 * k], s));
+      if (A.bs == 3)
+        k_sell_spmv<3, 1, float, true><<<gs, kBlock, 0, s>>>(A.nbrows, A.sptr, A.col, A.v32, A.dgc, ctx->p32, ctx->q32,
+                                                              st, ctx->partials, ctx->counter);
+      else
+        k_sell_spmv<BS1, K, float, true><<<gs, kBlock, 0, s>>>(A.nbrows, A.sptr, A.col, A.v32, A.dgc, ctx->p32,
+                                                                ctx->q32, st, ctx->partials, ctx->counter);
+      if (ctx->prof) CPM_HIP(hipEventRecord(ctx->ev[2 * k + 1], s));
+      k_rel_xr<K><<<g, kBlock, 0, s>>>(n, ctx->x32, ctx->r32, ctx->p32, ctx->q32, delta, st, ctx->partials,
+                                       ctx->counter);
+      k_rel_x<K><<<g, kBlock, 0, s>>>(n, x, ctx->x32, st);
+      if (A.bs == 3)
+        k_sell_spmv<3, 1, double, false, true><<<gs, kBlock, 0, s>>>(A.nbrows, A.sptr, A.col, A.v64, A.dg64, x, y64,
+                                                                      st, ctx->partials, ctx->counter);
+      else
+        k_sell_spmv<BS1, K, double, false, true><<<gs, kBlock, 0, s>>>(A.nbrows, A.sptr, A.col, A.v64, A.dg64, x, y64,
+                                                                        st, ctx->partials, ctx->counter);
+      k_rel_r<K><<<g, kBlock, 0, s>>>(n, b, y64, ctx->r, ctx->r32, st, ctx->partials, ctx->counter);
+    }
+    issued += todo;
+    CPM_HIP(hipGetLastError());
+    if (int rc = read_states(ctx, K, h)) return rc;
+    int it_max = 0;
+    bool all_done = true;
+    for (int j = 0; j < K; ++j) {
+      it_max = std::max(it_max, h[j].it);
+      all_done &= h[j].done != 0;
+    }
+    if (ctx->prof) {
+      for (int k = 0; k < it_max - it_before && k < todo; ++k) {
+        float ms = 0.f;
+        CPM_HIP(hipEventElapsedTime(&ms, ctx->ev[2 * k], ctx->ev[2 * k + 1]));
+        M->prof_ms += ms;
+        M->prof_launches += 1;
+      }
+    }
+    if (all_done || issued > maxiter) break;
+    batch = std::max(2, std::min(64, it_max / 4 + 2));
+  }
+  *failed = 0;
+  for (int j = 0; j < K; ++j) {
+    iters[j] = h[j].it;
+    resid[j] = h[j].resid;
+    if (h[j].fail || !h[j].done) *failed = 1;
+  }
+  return CPM_OK;
+}
+
+int sell_cg_mixed(cpm_ctx* ctx, cpm_mat* M, int K, const double* b, double* x, int use_x0, double tol, int maxiter,
+                  double delta, int hint, int32_t* iters, double* resid, int* failed, int32_t* sweeps) {
+  const Sell& A = M->sell;
+  const int64_t n = (int64_t)A.nbrows * A.bs * K;
+  if (int rc = ensure_workspace(ctx, n)) return rc;
+  if (int rc = ensure_workspace32(ctx, n)) return rc;
+  if (sweeps) *sweeps = 0;
+  if (K == 1) return rel_loop<1>(ctx, M, b, x, use_x0, tol, maxiter, delta, hint, iters, resid, failed);
+  if (K == 3 && A.bs == 1) return rel_loop<3>(ctx, M, b, x, use_x0, tol, maxiter, delta, hint, iters, resid, failed);
+  return fail(CPM_EINVAL, "mixed CG: unsupported right-hand-side count");
+}
+
+// interleave K columns (col-major blocks of n) <-> [i*K + j]; BS = 3 SoA <-> node-major
+__global__ void __launch_bounds__(kBlock) k_interleave(int64_t n, int K, const double* __restrict__ src,
+                                                       double* __restrict__ dst, int inverse) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n * K; i += stride) {
+    const int64_t node = i / K, j = i % K;
+    if (!inverse) dst[i] = src[j * n + node];
+    else dst[j * n + node] = src[i];
+  }
+}
+
+int interleave(cpm_ctx* ctx, int64_t n, int K, const double* src, double* dst, int inverse) {
+  k_interleave<<<grid_for(n * K), kBlock, 0, ctx->stream>>>(n, K, src, dst, inverse);
+  CPM_HIP(hipGetLastError());
+  return CPM_OK;
+}
+
+int sell_spmv64(cpm_ctx* ctx, const cpm_mat* M, int K, const double* x, double* y) {
+  return launch_sell<double>(ctx, M->sell, K, false, x, y, ctx->cg);
+}
+
+}  // namespace cpm
+
+// ---------------------------------------------------------------- C ABI
+
+extern "C" int cpm_mat_add_sell(cpm_mat* A, int bs, const int64_t* indptr, const int32_t* indices,
+                                const double* values) {
+  if (!A || !indptr || !indices || !values) return cpm::fail(CPM_EINVAL, "cpm_mat_add_sell: null argument");
+  if (A->nrows != A->ncols) return cpm::fail(CPM_EINVAL, "cpm_mat_add_sell: operator must be square");
+  if (bs != 1 && bs != 3) return cpm::fail(CPM_EINVAL, "cpm_mat_add_sell: block size must be 1 or 3");
+  if (A->nrows % bs) return cpm::fail(CPM_EINVAL, "cpm_mat_add_sell: rows not divisible by the block size");
+  if (indptr[A->nrows] != A->nnz) return cpm::fail(CPM_EINVAL, "cpm_mat_add_sell: pattern differs from the operator");
+  cpm::sell_free(A->sell);
+  return cpm::sell_build(A->ctx, A->sell, bs, A->nrows / bs, indptr, indices, values, A->nrows);
+}
+
+extern "C" int cpm_cg_sell(cpm_ctx* ctx, cpm_mat* A, int nrhs, const double* b, double* x, int use_x0, double tol,
+                           int maxiter, int precision, double delta, int32_t* iters, double* resid) {
+  if (!ctx || !A || !b || !x || !iters || !resid) return cpm::fail(CPM_EINVAL, "cpm_cg_sell: null argument");
+  if (!A->sell.sptr) return cpm::fail(CPM_EINVAL, "cpm_cg_sell: operator has no SELL form (cpm_mat_add_sell)");
+  if (nrhs != 1 && !(nrhs == 3 && A->sell.bs == 1))
+    return cpm::fail(CPM_EINVAL, "cpm_cg_sell: nrhs must be 1 (or 3 for scalar operators)");
+  int failed = 0;
+  int rc;
+  if (precision == 64)
+    rc = cpm::sell_cg64(ctx, A, nrhs, b, x, use_x0, tol, maxiter, 0, iters, resid, &failed);
+  else if (precision == 32)
+    rc = cpm::sell_cg_mixed(ctx, A, nrhs, b, x, use_x0, tol, maxiter, delta, 0, iters, resid, &failed, nullptr);
+  else
+    return cpm::fail(CPM_EINVAL, "cpm_cg_sell: precision must be 64 or 32");
+  if (rc) return rc;
+  if (failed) {
+    int m = 1;
+    for (int j = 0; j < nrhs; ++j) m = std::max(m, (int)iters[j]);
+    return m;
+  }
+  return CPM_OK;
+}
+
+extern "C" int cpm_spmv_sell(cpm_ctx* ctx, const cpm_mat* A, int nrhs, const double* x, double* y) {
+  if (!ctx || !A || !x || !y) return cpm::fail(CPM_EINVAL, "cpm_spmv_sell: null argument");
+  if (!A->sell.sptr) return cpm::fail(CPM_EINVAL, "cpm_spmv_sell: operator has no SELL form");
+  if (nrhs != 1 && !(nrhs == 3 && A->sell.bs == 1)) return cpm::fail(CPM_EINVAL, "cpm_spmv_sell: bad nrhs");
+  return cpm::sell_spmv64(ctx, A, nrhs, x, y);
+}

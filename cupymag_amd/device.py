@@ -94,6 +94,18 @@ class DeviceCSR:
         A = A.tocsr()
         return cls(A.indptr, A.indices, A.data, A.shape, ctx)
 
+    def add_sell(self, bs, indptr=None, indices=None, values=None):
+        """Attach the SELL-64 form (FP64 + FP32 values) used by the fast
+        solvers; bs = 3 for the 3x3-block elasticity operator."""
+        if indptr is None:
+            indptr, indices, values = self.to_arrays()
+        indptr = np.ascontiguousarray(indptr, dtype=np.int64)
+        indices = np.ascontiguousarray(indices, dtype=np.int32)
+        values = np.ascontiguousarray(values, dtype=np.float64)
+        check(lib().cpm_mat_add_sell(self.h, int(bs), ptr(indptr), ptr(indices), ptr(values)), "cpm_mat_add_sell")
+        self.sell_bs = int(bs)
+        return self
+
     def matvec(self, x, out=None, valset=0):
         x = as_device(x, self.ctx)
         if x.numel() != self.shape[1]:

@@ -36,3 +36,22 @@ def solve_cg(A, b, M=None, x0=None, tol=1e-7, maxiter=5000, use_init=False, syst
     if return_info:
         return x, int(it.value), float(res.value)
     return x
+
+
+def solve_cg_sell(A, b, x0=None, tol=1e-7, maxiter=5000, precision=64, rel_delta=0.1, nrhs=1):
+    """CG on the SELL-64 form of ``A`` (see DeviceCSR.add_sell) for ``nrhs``
+    interleaved right-hand sides (b[i*nrhs + j]); bs-3 operators take
+    node-major interleaved vectors.  precision 64: CuPy's recurrence per
+    column; 32: FP32 CG refined in FP64 (reliable updates).  Returns
+    (x, iters[nrhs], resid[nrhs]); raises RuntimeError on non-convergence."""
+    ctx = A.ctx
+    b = as_device(b, ctx)
+    use_init = x0 is not None
+    x = as_device(x0, ctx).clone() if use_init else torch.empty_like(b)
+    it = (C.c_int32 * nrhs)()
+    res = (C.c_double * nrhs)()
+    info = check(lib().cpm_cg_sell(ctx.h, A.h, int(nrhs), ptr(b), ptr(x), int(use_init), float(tol), int(maxiter),
+                                   int(precision), float(rel_delta), it, res), "cpm_cg_sell")
+    if info != 0:
+        raise RuntimeError(f"Error! CG did not converge. info={info}, residual={max(res)}.")
+    return x, list(it), list(res)

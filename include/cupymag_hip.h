@@ -108,6 +108,26 @@ int cpm_cg(cpm_ctx* ctx, const cpm_mat* A, const double* b, double* x, int use_x
 int cpm_prof_enable(cpm_ctx* ctx, int on);
 int cpm_mat_prof(cpm_mat* A, double* total_ms, int64_t* launches, int reset);
 
+/* Attach a SELL-64 form (sliced ELL, slices of 64 rows, FP64 and FP32 values)
+ * to a square operator from the same host CSR arrays it was created from.
+ * bs = 3 groups rows/columns {i, n+i, 2n+i} into 3x3 blocks (the elasticity
+ * operator, DOF order of assemble_elasticity.py:391) and its solver works on
+ * node-major interleaved vectors. */
+int cpm_mat_add_sell(cpm_mat* A, int bs, const int64_t* indptr, const int32_t* indices, const double* values);
+/* CG on the SELL form for nrhs interleaved right-hand sides (b[i*nrhs + j];
+ * nrhs = 3 only for bs = 1; bs = 3 vectors are node-major interleaved).
+ * precision 64: CuPy's CG recurrence per column (as cpm_cg).  precision 32:
+ * FP32 CG (FP64 dot accumulation) refined in FP64 by reliable updates: when
+ * the FP32 recurrence residual has dropped by `delta` the correction is added
+ * to the FP64 solution and the residual replaced by the FP64 b - A x; stops
+ * when that FP64 residual is <= tol*||b||.
+ * iters/resid get nrhs entries.  Returns 0 or a positive count when a
+ * column did not converge within maxiter. */
+int cpm_cg_sell(cpm_ctx* ctx, cpm_mat* A, int nrhs, const double* b, double* x, int use_x0, double tol, int maxiter,
+                int precision, double delta, int32_t* iters, double* resid);
+/* y = A x on the SELL form (FP64), nrhs interleaved right-hand sides. */
+int cpm_spmv_sell(cpm_ctx* ctx, const cpm_mat* A, int nrhs, const double* x, double* y);
+
 /* ------------------------------------------------------------------ */
 /* field operators                                                     */
 /* ------------------------------------------------------------------ */
@@ -168,10 +188,15 @@ typedef struct cpm_sim_params {
   double Mv[36];              /* rot_111_matrices.get_M_matrix (row-major) */
   double MvinvT[36];          /* inv(Mv)^T (Micromagnetics.py:54) */
   int32_t mode;               /* CPM_MODE_* */
-  int32_t reserved[7];
+  int32_t reserved0;
+  double rel_delta;           /* CPM_MODE_MIXED: FP64 residual update when the FP32 recurrence
+                                 residual drops by this factor (reliable updates) */
+  int32_t reserved[4];
 } cpm_sim_params;
 
-#define CPM_MODE_FP64 0  /* plain FP64 path, the reference's sequence of solves */
+#define CPM_MODE_FP64 0   /* plain FP64 path: CSR operators, the reference's sequence of solves */
+#define CPM_MODE_FAST64 1 /* FP64, SELL-64 operators, 3-RHS solves for g1n..g3n and m1**..m3** */
+#define CPM_MODE_MIXED 2  /* as FAST64 with FP32 CG refined in FP64 (reliable updates) */
 
 /* Report of one step (cpm_sim_step). */
 typedef struct cpm_step_report {

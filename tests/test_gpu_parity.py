@@ -199,3 +199,106 @@ def test_warm_start_path_matches_oracle():
         use_init = rep.sumsq < P.use_init_factor * ops["n"] * P.LLG_accuracy_factor
         assert use_init == st["use_init"]
         assert np.max(np.abs(sim.get("M").cpu().numpy() - st["m"])) < 1e-10
+
+
+# ---------------------------------------------------------------- fast paths
+
+def _interleave(cols):
+    return np.stack(cols, axis=1).ravel()
+
+
+def test_sell_spmv_matches_scipy():
+    from cupymag_amd._lib import check, lib, ptr
+    cfg = config("Tet", me=True, defect_params=1.5)
+    P, S = _product(cfg, _kuhn(6))
+    A_el = S.csr_elasticity()[0]
+    A1 = S.csr_GS()[0]
+    n = A1.shape[0]
+    rng = np.random.default_rng(7)
+    # scalar, 3 interleaved right-hand sides
+    X = rng.standard_normal((n, 3))
+    y = torch.empty(3 * n, dtype=torch.float64, device="cuda")
+    check(lib().cpm_spmv_sell(A1.ctx.h, A1.h, 3, ptr(torch.as_tensor(X.ravel(), device="cuda")), ptr(y)))
+    ref = A1.to_scipy() @ X
+    assert _rel(y.cpu().numpy().reshape(n, 3), ref) < 1e-14
+    # 3x3 blocks on node-major vectors
+    u = rng.standard_normal(3 * n)
+    y = torch.empty(3 * n, dtype=torch.float64, device="cuda")
+    ui = _interleave([u[:n], u[n:2 * n], u[2 * n:]])
+    check(lib().cpm_spmv_sell(A_el.ctx.h, A_el.h, 1, ptr(torch.as_tensor(ui, device="cuda")), ptr(y)))
+    ref = A_el.to_scipy() @ u
+    assert _rel(y.cpu().numpy(), _interleave([ref[:n], ref[n:2 * n], ref[2 * n:]])) < 1e-14
+
+
+@pytest.mark.parametrize("which", ["A1x3", "A_el"])
+def test_sell_cg_matches_oracle(which):
+    from cupymag_amd.solvers.linear_solvers import solve_cg_sell
+    from oracle import gspm as og
+    cfg = config("Tet", me=True, defect_params=1.8)
+    arr = _kuhn(6)
+    P, S = _product(cfg, arr)
+    p, ops, Avg, _ = oracle_setup(cfg, arr)
+    n = ops["n"]
+    rng = np.random.default_rng(8)
+    if which == "A1x3":
+        A = S.csr_GS()[0]
+        B = rng.standard_normal((n, 3))
+        x, it, _ = solve_cg_sell(A, B.ravel(), tol=1e-10, nrhs=3)
+        X = x.cpu().numpy().reshape(n, 3)
+        for j in range(3):
+            xr, _, itr = og.cg(ops["A1"], B[:, j], None, 1e-10, 5000)
+            assert abs(it[j] - itr) <= 1 and _rel(X[:, j], xr) < 1e-10
+        # mixed precision: FP64 residual meets the tolerance, error small
+        x32, it32, res32 = solve_cg_sell(A, B.ravel(), tol=1e-10, nrhs=3, precision=32)
+        X32 = x32.cpu().numpy().reshape(n, 3)
+        for j in range(3):
+            xr, _, _ = og.cg(ops["A1"], B[:, j], None, 1e-10, 5000)
+            assert np.linalg.norm(B[:, j] - ops["A1"] @ X32[:, j]) <= 1e-10 * np.linalg.norm(B[:, j])
+            assert _rel(X32[:, j], xr) < 1e-8
+    else:
+        A = S.csr_elasticity()[0]
+        b = rng.standard_normal(3 * n)
+        b[[0, n, 2 * n]] = 0.0
+        bi = _interleave([b[:n], b[n:2 * n], b[2 * n:]])
+        x, it, _ = solve_cg_sell(A, bi, tol=1e-10)
+        xr, _, itr = og.cg(ops["A_el"], b, None, 1e-10, 5000)
+        assert abs(it[0] - itr) <= 1
+        assert _rel(x.cpu().numpy(), _interleave([xr[:n], xr[n:2 * n], xr[2 * n:]])) < 1e-10
+        x32, _, _ = solve_cg_sell(A, bi, tol=1e-10, precision=32)
+        assert _rel(x32.cpu().numpy(), _interleave([xr[:n], xr[n:2 * n], xr[2 * n:]])) < 1e-8
+
+
+@pytest.mark.parametrize("mode", ["fast64", "mixed"])
+@pytest.mark.parametrize("case", [dict(grid="Tet", n=6, me=True, rot=True, dt=5e-12),
+                                  dict(grid="Hex", n=6, me=True, rot=False, sigma11=1e8)],
+                         ids=["tet-me-rot", "hex-me"])
+def test_fast_modes_match_oracle(mode, case):
+    """fast64: same tolerances as the plain FP64 path (m 1e-10, CG iterations
+    +-1).  mixed: magnetization within 1e-7 of the FP64 oracle."""
+    from cupymag_amd.core.gspm import GSPMSimulation
+    from oracle import gspm as og
+    case = dict(case)
+    grid, n = case.pop("grid"), case.pop("n")
+    cfg = config(grid, n=n, defect_params=1.6, **case)
+    arr = _kuhn(n) if grid == "Tet" else None
+    P, S = _product(cfg, arr)
+    p, ops, Avg, _ = oracle_setup(cfg, arr)
+    m0 = tilted_m(ops["n"], seed=12)
+    sim = GSPMSimulation(S, P, mode=mode)
+    sim.set_m(m0)
+    st = og.init_state(ops, Avg, m0)
+    p["H_now"] = p["H"]
+    H = [P.Hext1, P.Hext2, P.Hext3]
+    for k in range(3):
+        log = []
+        ui = st["use_init"]
+        og.step(st, ops, Avg, p, log)
+        rep = sim.step(H, ui)
+        err = np.max(np.abs(sim.get("M").cpu().numpy() - st["m"]))
+        if mode == "fast64":
+            assert err < 1e-10, f"step {k}: {err}"
+            ref_it = [it for _, it in log]
+            assert all(abs(a - b) <= 1 for a, b in zip(rep.iters, ref_it)), (rep.iters, ref_it)
+            assert _rel(sim.get("UEL").cpu().numpy(), st["u"]) < 1e-10
+        else:
+            assert err < 1e-7, f"step {k}: {err}"
