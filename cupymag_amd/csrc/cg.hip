@@ -4,13 +4,15 @@
 // solve_cg (cupymag/solvers/linear_solvers.py:59-62).  One CG iteration is
 // three kernels on the context stream:
 //   pupd : p = r (first iteration) or p = r + (rho/rho_old) p
-//   spmv : q = A p, fused p.q reduction; the last workgroup sets alpha
-//   xr   : x += alpha p, r -= alpha q, fused r.r reduction; the last
-//          workgroup advances the iteration count and tests ||r|| <= atol
+//   spmv : q = A p, fused p.q partials; k_red sets alpha
+//   xr   : x += alpha p, r -= alpha q, fused r.r partials; k_red advances
+//          the iteration count and tests ||r|| <= atol
 // The CG scalars live in device memory (CgState); every kernel returns
 // immediately once `done` is set, so the host enqueues batches of iterations
 // and reads the state back only between batches instead of once per
-// iteration.  All reductions are deterministic (fixed block order).
+// iteration.  Both dot products are written as per-workgroup partials and
+// added by a one-workgroup kernel (k_red) in fixed order: deterministic, and
+// without a device-scope fence per workgroup.
 #include <algorithm>
 #include <cmath>
 
@@ -34,12 +36,12 @@ __global__ void __launch_bounds__(kBlock) k_spmv(int32_t nrows, const int32_t* _
   }
 }
 
-// q = A p and p.q; sets alpha = rho / (p.q).
+// q = A p and the partials of p.q (k_red then sets alpha = rho / (p.q)).
 template <int G>
 __global__ void __launch_bounds__(kBlock) k_cg_spmv(int32_t nrows, const int32_t* __restrict__ ptr,
                                                     const int32_t* __restrict__ idx, const double* __restrict__ val,
                                                     const double* __restrict__ p, double* __restrict__ q,
-                                                    CgState* st, double* partials, unsigned* counter) {
+                                                    const CgState* st, double* partials) {
   if (st->done) return;
   const int lane = threadIdx.x & (G - 1);
   const int64_t stride = (int64_t)gridDim.x * (kBlock / G);
@@ -54,10 +56,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_spmv(int32_t nrows, const int32_t
       acc[0] += p[row] * s;
     }
   }
-  if (grid_sum<1>(acc, partials, counter) && threadIdx.x == 0) {
-    st->pq = acc[0];
-    st->alpha = st->rho / acc[0];
-  }
+  block_partial<1>(acc, partials);
 }
 
 __global__ void __launch_bounds__(kBlock) k_cg_pupd(int64_t n, const double* __restrict__ r, double* __restrict__ p,
@@ -74,7 +73,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_pupd(int64_t n, const double* __r
 
 __global__ void __launch_bounds__(kBlock) k_cg_xr(int64_t n, double* __restrict__ x, double* __restrict__ r,
                                                   const double* __restrict__ p, const double* __restrict__ q,
-                                                  CgState* st, double* partials, unsigned* counter) {
+                                                  const CgState* st, double* partials) {
   if (st->done) return;
   const double a = st->alpha;
   const int64_t stride = (int64_t)gridDim.x * kBlock;
@@ -85,18 +84,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_xr(int64_t n, double* __restrict_
     r[i] = ri;
     acc[0] += ri * ri;
   }
-  if (grid_sum<1>(acc, partials, counter) && threadIdx.x == 0) {
-    st->rho_old = st->rho;
-    st->rho = acc[0];
-    st->it += 1;
-    st->resid = sqrt(acc[0]);
-    if (st->resid <= st->atol) {
-      st->done = 1;
-    } else if (st->it >= st->maxiter) {
-      st->done = 1;
-      st->fail = 1;
-    }
-  }
+  block_partial<1>(acc, partials);
 }
 
 // r = b - A x (use_x0, q holds A x) or x = 0, r = b; ||b||^2, r.r and atol.
@@ -149,10 +137,11 @@ static void launch_spmv_g(const cpm_mat* A, const double* val, const double* x, 
 }
 
 template <int G>
-static void launch_cg_spmv_g(cpm_ctx* ctx, const cpm_mat* A, CgState* st) {
+static void launch_cg_spmv_g(cpm_ctx* ctx, const cpm_mat* A, CgState* st, int* grid_out) {
   const int grid = grid_for(A->nrows, kBlock / G);
   k_cg_spmv<G><<<grid, kBlock, 0, ctx->stream>>>(A->nrows, A->ptr, A->idx, A->val, ctx->p, ctx->q, st,
-                                                   ctx->partials, ctx->counter);
+                                                   ctx->partials);
+  *grid_out = grid;
 }
 
 #define CPM_LANES_DISPATCH(L, F, ...) \
@@ -197,9 +186,12 @@ int run_cg(cpm_ctx* ctx, const cpm_mat* A, const double* b, double* x, int use_x
     for (int k = 0; k < todo; ++k) {
       k_cg_pupd<<<gv, kBlock, 0, s>>>(n, ctx->r, ctx->p, st);
       if (ctx->prof) CPM_HIP(hipEventRecord(ctx->ev[2 * k], s));
-      CPM_LANES_DISPATCH(A->lanes, launch_cg_spmv_g, ctx, A, st);
+      int gsp = 0;
+      CPM_LANES_DISPATCH(A->lanes, launch_cg_spmv_g, ctx, A, st, &gsp);
       if (ctx->prof) CPM_HIP(hipEventRecord(ctx->ev[2 * k + 1], s));
-      k_cg_xr<<<gv, kBlock, 0, s>>>(n, x, ctx->r, ctx->p, ctx->q, st, ctx->partials, ctx->counter);
+      k_red<1, RED_ALPHA><<<1, kBlock, 0, s>>>(gsp, ctx->partials, st, 0.0);
+      k_cg_xr<<<gv, kBlock, 0, s>>>(n, x, ctx->r, ctx->p, ctx->q, st, ctx->partials);
+      k_red<1, RED_RR><<<1, kBlock, 0, s>>>(gv, ctx->partials, st, 0.0);
     }
     issued += todo;
     CPM_HIP(hipGetLastError());

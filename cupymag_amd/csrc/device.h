@@ -149,6 +149,19 @@ __device__ inline bool grid_sum(double (&v)[NV], double* partials, unsigned* cou
   return true;
 }
 
+// Workgroup partial sum stored at partials[k * gridDim.x + blockIdx.x] with a
+// plain store.  A separate one-workgroup kernel adds the partials after the
+// producing kernel has finished: this avoids a device-scope fence and an
+// atomic per workgroup, which on a multi-XCD GPU cost an L2 write-back each.
+template <int NV>
+__device__ inline void block_partial(double (&v)[NV], double* partials) {
+  __shared__ double lds[NV * 4];
+  block_sum<NV>(v, lds);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) partials[k * gridDim.x + blockIdx.x] = v[k];
+}
+
 // Row sum of a CSR row with G lanes (G a power of two <= 64), lanes of a
 // row contiguous within the wavefront; the sum is valid in every lane.
 template <int G>
@@ -156,6 +169,70 @@ __device__ inline double group_sum(double s) {
 #pragma unroll
   for (int o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, G);
   return s;
+}
+
+// One workgroup: add the partials of the preceding kernel (deterministic
+// order) and advance the CG scalars.
+enum : int { RED_ALPHA = 0, RED_RR = 1, RED_RR_REL = 2, RED_RR_TRUE = 3 };
+
+template <int K, int MODE>
+__global__ void __launch_bounds__(kBlock) k_red(int nblk, const double* __restrict__ partials, CgState* st,
+                                                double delta) {
+  bool use[K];
+  bool any = false;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    use[j] = (MODE == RED_RR_TRUE) ? st[j].upd != 0 : !st[j].done;
+    any |= use[j];
+  }
+  if (!any) return;
+  __shared__ double lds[K * 4];
+  double v[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) v[j] = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += kBlock)
+#pragma unroll
+    for (int j = 0; j < K; ++j) v[j] += partials[j * nblk + i];
+  block_sum<K>(v, lds);
+  if (threadIdx.x != 0) return;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    if (!use[j]) continue;
+    CgState& c = st[j];
+    if (MODE == RED_ALPHA) {
+      c.pq = v[j];
+      c.alpha = c.rho / v[j];
+    } else if (MODE == RED_RR) {
+      // CuPy: rho = r.r; stop when ||r|| <= atol after the update
+      c.rho_old = c.rho;
+      c.rho = v[j];
+      c.it += 1;
+      c.resid = sqrt(v[j]);
+      if (c.resid <= c.atol) {
+        c.done = 1;
+      } else if (c.it >= c.maxiter) {
+        c.done = 1;
+        c.fail = 1;
+      }
+    } else if (MODE == RED_RR_REL) {
+      c.rho_old = c.rho;
+      c.rho = v[j];
+      c.it += 1;
+      c.resid = sqrt(v[j]);
+      if (c.resid < delta * c.maxr || c.resid <= c.atol || c.it >= c.maxiter) c.upd = 1;
+    } else {  // RED_RR_TRUE: FP64 residual replaced the FP32 recurrence residual
+      c.upd = 0;
+      c.rho = v[j];
+      c.resid = sqrt(v[j]);
+      c.maxr = c.resid;
+      if (c.resid <= c.atol) {
+        c.done = 1;
+      } else if (c.it >= c.maxiter) {
+        c.done = 1;
+        c.fail = 1;
+      }
+    }
+  }
 }
 
 }  // namespace cpm

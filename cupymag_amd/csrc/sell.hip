@@ -141,21 +141,19 @@ void sell_free(Sell& S) {
 
 // y = A x (+ fused per-column dot x.y when Dot) for BS x BS blocks and K
 // interleaved right-hand sides: x[(c*BS + e)*K + j].
-template <int BS, int K, class T, bool CG, bool GATE = false>
+template <int BS, int K, class T, bool CG, bool GATE = false, bool ADDX32 = false>
 __global__ void __launch_bounds__(kBlock) k_sell_spmv(int32_t nbrows, const int64_t* __restrict__ sptr,
                                                       const int32_t* __restrict__ col, const T* __restrict__ val,
                                                       const double* __restrict__ dg, const T* __restrict__ x,
-                                                      T* __restrict__ y, CgState* st, double* partials,
-                                                      unsigned* counter) {
+                                                      const float* __restrict__ x32, T* __restrict__ y,
+                                                      const CgState* st, double* partials) {
+  // y = A x (ADDX32: A (x + x32)); CG: per-column partials of x.y.  GATE:
+  // run only when some column requested an FP64 residual update.
   constexpr int BB = BS * BS;
   constexpr int NV = BS * K;
-  bool active[K];
   bool any = false;
 #pragma unroll
-  for (int j = 0; j < K; ++j) {
-    active[j] = CG ? !st[j].done : (GATE ? st[j].upd != 0 : true);
-    any |= active[j];
-  }
+  for (int j = 0; j < K; ++j) any |= CG ? !st[j].done : (GATE ? st[j].upd != 0 : true);
   if (!any) return;
   double acc[K];
 #pragma unroll
@@ -169,47 +167,45 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(int32_t nbrows, const int6
     for (int v = 0; v < NV; ++v) sum[v] = 0.0;
     for (int64_t slot = s0; slot < s1; ++slot) {
       const int64_t c = col[slot * 64 + l];
-      T xv[NV];
+      double xv[NV];
 #pragma unroll
-      for (int v = 0; v < NV; ++v) xv[v] = x[c * NV + v];
+      for (int v = 0; v < NV; ++v) {
+        xv[v] = (double)x[c * NV + v];
+        if (ADDX32) xv[v] += (double)x32[c * NV + v];
+      }
 #pragma unroll
       for (int d = 0; d < BS; ++d)
 #pragma unroll
         for (int e = 0; e < BS; ++e) {
           const double a = (double)val[(slot * BB + d * BS + e) * 64 + l];
 #pragma unroll
-          for (int j = 0; j < K; ++j) sum[d * K + j] += a * (double)xv[e * K + j];
+          for (int j = 0; j < K; ++j) sum[d * K + j] += a * xv[e * K + j];
         }
     }
     if (row < nbrows) {
-      T xo[NV];
+      double xo[NV];
 #pragma unroll
-      for (int v = 0; v < NV; ++v) xo[v] = x[row * NV + v];
+      for (int v = 0; v < NV; ++v) {
+        xo[v] = (double)x[row * NV + v];
+        if (ADDX32) xo[v] += (double)x32[row * NV + v];
+      }
 #pragma unroll
       for (int d = 0; d < BS; ++d)
 #pragma unroll
         for (int e = 0; e < BS; ++e) {
           const double a = dg[row * BB + d * BS + e];
 #pragma unroll
-          for (int j = 0; j < K; ++j) sum[d * K + j] += a * (double)xo[e * K + j];
+          for (int j = 0; j < K; ++j) sum[d * K + j] += a * xo[e * K + j];
         }
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
         const T yv = (T)sum[v];
         y[row * NV + v] = yv;
-        if (CG) acc[v % K] += (double)xo[v] * (double)yv;
+        if (CG) acc[v % K] += xo[v] * (double)yv;
       }
     }
   }
-  if (!CG) return;
-  if (grid_sum<K>(acc, partials, counter) && threadIdx.x == 0) {
-#pragma unroll
-    for (int j = 0; j < K; ++j)
-      if (active[j]) {
-        st[j].pq = acc[j];
-        st[j].alpha = st[j].rho / acc[j];
-      }
-  }
+  if (CG) block_partial<K>(acc, partials);
 }
 
 template <int K, class T>
@@ -236,8 +232,8 @@ __global__ void __launch_bounds__(kBlock) k_mcg_pupd(int64_t n, const T* __restr
 
 template <int K, class T>
 __global__ void __launch_bounds__(kBlock) k_mcg_xr(int64_t n, T* __restrict__ x, T* __restrict__ r,
-                                                   const T* __restrict__ p, const T* __restrict__ q, CgState* st,
-                                                   double* partials, unsigned* counter) {
+                                                   const T* __restrict__ p, const T* __restrict__ q, const CgState* st,
+                                                   double* partials) {
   double a[K];
   bool live[K];
   bool any = false;
@@ -261,23 +257,7 @@ __global__ void __launch_bounds__(kBlock) k_mcg_xr(int64_t n, T* __restrict__ x,
     r[i] = ri;
     acc[j] += (double)ri * (double)ri;
   }
-  if (grid_sum<K>(acc, partials, counter) && threadIdx.x == 0) {
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
-      if (!live[j]) continue;
-      CgState& s = st[j];
-      s.rho_old = s.rho;
-      s.rho = acc[j];
-      s.it += 1;
-      s.resid = sqrt(acc[j]);
-      if (s.resid <= s.atol) {
-        s.done = 1;
-      } else if (s.it >= s.maxiter) {
-        s.done = 1;
-        s.fail = 1;
-      }
-    }
-  }
+  block_partial<K>(acc, partials);
 }
 
 // CG start: x = 0, r = b (or r = b - q with q = A x0); per-column ||b||^2, r.r.
@@ -343,52 +323,36 @@ __global__ void __launch_bounds__(kBlock) k_fill0(int64_t n, double* __restrict_
 
 // ---------------------------------------------------------------- dispatch
 
-template <int BS, int K, class T, bool CG>
-static void launch_sell_t(cpm_ctx* ctx, const Sell& A, const T* val, const T* x, T* y, CgState* st) {
-  const int64_t rows = ((int64_t)A.nbrows + 63) & ~63;
-  const int grid = grid_for(rows);
-  k_sell_spmv<BS, K, T, CG><<<grid, kBlock, 0, ctx->stream>>>(A.nbrows, A.sptr, A.col, val,
-                                                             sizeof(T) == 8 ? A.dg64 : A.dgc, x, y, st,
-                                                             ctx->partials, ctx->counter);
+static inline int sell_grid(const Sell& A) { return grid_for(((int64_t)A.nbrows + 63) & ~63); }
+
+template <int BS, int K, class T, bool CG, bool GATE = false, bool ADDX32 = false>
+static int spmv_t(cpm_ctx* ctx, const Sell& A, const T* x, const float* x32, T* y) {
+  const T* val;
+  const double* dg;
+  if constexpr (sizeof(T) == 8) {
+    val = A.v64;
+    dg = A.dg64;
+  } else {
+    val = A.v32;
+    dg = A.dgc;
+  }
+  k_sell_spmv<BS, K, T, CG, GATE, ADDX32><<<sell_grid(A), kBlock, 0, ctx->stream>>>(A.nbrows, A.sptr, A.col, val, dg,
+                                                                                    x, x32, y, ctx->cg, ctx->partials);
+  CPM_HIP(hipGetLastError());
+  return CPM_OK;
 }
 
 template <class T>
-static int launch_sell(cpm_ctx* ctx, const Sell& A, int K, bool cg, const T* x, T* y, CgState* st) {
-  const T* val;
-  if constexpr (sizeof(T) == 8) val = A.v64; else val = A.v32;
-#define CPM_SELL_CASE(BS_, K_)                                         \
-  if (A.bs == BS_ && K == K_) {                                        \
-    if (cg) launch_sell_t<BS_, K_, T, true>(ctx, A, val, x, y, st);    \
-    else launch_sell_t<BS_, K_, T, false>(ctx, A, val, x, y, st);      \
-    return CPM_OK;                                                     \
-  }
-  CPM_SELL_CASE(1, 1)
-  CPM_SELL_CASE(1, 3)
-  CPM_SELL_CASE(3, 1)
-#undef CPM_SELL_CASE
+static int launch_sell(cpm_ctx* ctx, const Sell& A, int K, const T* x, T* y) {
+  if (A.bs == 1 && K == 1) return spmv_t<1, 1, T, false>(ctx, A, x, nullptr, y);
+  if (A.bs == 1 && K == 3) return spmv_t<1, 3, T, false>(ctx, A, x, nullptr, y);
+  if (A.bs == 3 && K == 1) return spmv_t<3, 1, T, false>(ctx, A, x, nullptr, y);
   return fail(CPM_EINVAL, "SELL: unsupported (block size, rhs count)");
 }
 
-template <int K, class T>
-static void launch_vec(cpm_ctx* ctx, int which, int64_t n, const T* b, T* x, T* r, T* p, T* q, int use_x0, double tol,
-                       int maxiter, CgState* st) {
-  const int g = grid_for(n);
-  hipStream_t s = ctx->stream;
-  switch (which) {
-    case 0:
-      k_mcg_init<K, T><<<g, kBlock, 0, s>>>(n, b, x, r, q, use_x0, tol, maxiter, st, ctx->partials, ctx->counter);
-      k_mcg_zero_b<K, T><<<g, kBlock, 0, s>>>(n, x, st);
-      break;
-    case 1: k_mcg_pupd<K, T><<<g, kBlock, 0, s>>>(n, r, p, st); break;
-    default: k_mcg_xr<K, T><<<g, kBlock, 0, s>>>(n, x, r, p, q, st, ctx->partials, ctx->counter); break;
-  }
-}
-
-template <class T>
-static void vec_op(cpm_ctx* ctx, int K, int which, int64_t n, const T* b, T* x, T* r, T* p, T* q, int use_x0,
-                   double tol, int maxiter, CgState* st) {
-  if (K == 1) launch_vec<1, T>(ctx, which, n, b, x, r, p, q, use_x0, tol, maxiter, st);
-  else launch_vec<3, T>(ctx, which, n, b, x, r, p, q, use_x0, tol, maxiter, st);
+template <int K, int MODE>
+static void reduce_t(cpm_ctx* ctx, int nblk, double delta = 0.0) {
+  k_red<K, MODE><<<1, kBlock, 0, ctx->stream>>>(nblk, ctx->partials, ctx->cg, delta);
 }
 
 static int read_states(cpm_ctx* ctx, int K, CgState* h) {
@@ -398,49 +362,40 @@ static int read_states(cpm_ctx* ctx, int K, CgState* h) {
   return CPM_OK;
 }
 
-// Batched-launch CG on K interleaved columns (vectors of n = nbrows*BS*K).
-template <class T>
-static int mcg_core(cpm_ctx* ctx, const Sell& A, cpm_mat* prof, int K, const T* b, T* x, T* r, T* p, T* q,
-                    int use_x0, double tol, int maxiter, int hint, int32_t* iters, double* resid, int* failed) {
-  const int64_t n = (int64_t)A.nbrows * A.bs * K;
-  CgState* st = ctx->cg;
-  if (use_x0)
-    if (int rc = launch_sell<T>(ctx, A, K, false, x, q, st)) return rc;
-  vec_op<T>(ctx, K, 0, n, b, x, r, p, q, use_x0, tol, maxiter, st);
-  CPM_HIP(hipGetLastError());
-  CgState h[3] = {};
+// Host side of a CG run: enqueue `issue(todo)` iteration batches, read the
+// column states back between batches, time the SpMV launches when
+// profiling is on.  Returns after every column is done or maxiter is spent.
+template <int K, class Issue>
+static int drive(cpm_ctx* ctx, cpm_mat* M, int maxiter, int hint, Issue&& issue, int32_t* iters, double* resid,
+                 int* failed) {
+  CgState h[K] = {};
+  if (int rc = read_states(ctx, K, h)) return rc;
   int batch = hint > 2 ? hint - 1 : 8;
   const int max_batch = ctx->prof ? 256 : 4096;
   int issued = 0;
-  for (;;) {
-    const int todo = std::max(1, std::min(std::min(batch, max_batch), maxiter - issued));
+  bool all_done = true;
+  for (int j = 0; j < K; ++j) all_done &= h[j].done != 0;
+  while (!all_done && issued <= maxiter) {
+    const int todo = std::max(1, std::min(std::min(batch, max_batch), maxiter - issued + 1));
     int it_before = 0;
     for (int j = 0; j < K; ++j) it_before = std::max(it_before, h[j].it);
-    for (int k = 0; k < todo; ++k) {
-      vec_op<T>(ctx, K, 1, n, b, x, r, p, q, 0, 0.0, 0, st);
-      if (ctx->prof) CPM_HIP(hipEventRecord(ctx->ev[2 * k], ctx->stream));
-      if (int rc = launch_sell<T>(ctx, A, K, true, p, q, st)) return rc;
-      if (ctx->prof) CPM_HIP(hipEventRecord(ctx->ev[2 * k + 1], ctx->stream));
-      vec_op<T>(ctx, K, 2, n, b, x, r, p, q, 0, 0.0, 0, st);
-    }
+    if (int rc = issue(todo)) return rc;
     issued += todo;
-    CPM_HIP(hipGetLastError());
     if (int rc = read_states(ctx, K, h)) return rc;
     int it_max = 0;
-    bool all_done = true;
+    all_done = true;
     for (int j = 0; j < K; ++j) {
       it_max = std::max(it_max, h[j].it);
       all_done &= h[j].done != 0;
     }
-    if (ctx->prof && prof) {
+    if (ctx->prof && M) {
       for (int k = 0; k < it_max - it_before && k < todo; ++k) {
         float ms = 0.f;
         CPM_HIP(hipEventElapsedTime(&ms, ctx->ev[2 * k], ctx->ev[2 * k + 1]));
-        prof->prof_ms += ms;
-        prof->prof_launches += 1;
+        M->prof_ms += ms;
+        M->prof_launches += 1;
       }
     }
-    if (all_done || issued >= maxiter) break;
     batch = std::max(2, std::min(64, it_max / 4 + 2));
   }
   *failed = 0;
@@ -452,13 +407,46 @@ static int mcg_core(cpm_ctx* ctx, const Sell& A, cpm_mat* prof, int K, const T* 
   return CPM_OK;
 }
 
+// FP64 CG on the SELL form, K interleaved columns (the cupyx cg recurrence).
+template <int BS, int K>
+static int cg64_loop(cpm_ctx* ctx, cpm_mat* M, const double* b, double* x, int use_x0, double tol, int maxiter,
+                     int hint, int32_t* iters, double* resid, int* failed) {
+  const Sell& A = M->sell;
+  const int64_t n = (int64_t)A.nbrows * BS * K;
+  const int g = grid_for(n), gs = sell_grid(A);
+  hipStream_t s = ctx->stream;
+  CgState* st = ctx->cg;
+  if (use_x0)
+    if (int rc = spmv_t<BS, K, double, false>(ctx, A, x, nullptr, ctx->q)) return rc;
+  k_mcg_init<K, double><<<g, kBlock, 0, s>>>(n, b, x, ctx->r, ctx->q, use_x0, tol, maxiter, st, ctx->partials,
+                                             ctx->counter);
+  k_mcg_zero_b<K, double><<<g, kBlock, 0, s>>>(n, x, st);
+  CPM_HIP(hipGetLastError());
+  auto issue = [&](int todo) -> int {
+    for (int k = 0; k < todo; ++k) {
+      k_mcg_pupd<K, double><<<g, kBlock, 0, s>>>(n, ctx->r, ctx->p, st);
+      if (ctx->prof) CPM_HIP(hipEventRecord(ctx->ev[2 * k], s));
+      if (int rc = spmv_t<BS, K, double, true>(ctx, A, ctx->p, nullptr, ctx->q)) return rc;
+      if (ctx->prof) CPM_HIP(hipEventRecord(ctx->ev[2 * k + 1], s));
+      reduce_t<K, RED_ALPHA>(ctx, gs);
+      k_mcg_xr<K, double><<<g, kBlock, 0, s>>>(n, x, ctx->r, ctx->p, ctx->q, st, ctx->partials);
+      reduce_t<K, RED_RR>(ctx, g);
+    }
+    CPM_HIP(hipGetLastError());
+    return CPM_OK;
+  };
+  return drive<K>(ctx, M, maxiter, hint, issue, iters, resid, failed);
+}
+
 int sell_cg64(cpm_ctx* ctx, cpm_mat* M, int K, const double* b, double* x, int use_x0, double tol, int maxiter,
               int hint, int32_t* iters, double* resid, int* failed) {
   const Sell& A = M->sell;
   const int64_t n = (int64_t)A.nbrows * A.bs * K;
   if (int rc = ensure_workspace(ctx, n)) return rc;
-  return mcg_core<double>(ctx, A, M, K, b, x, ctx->r, ctx->p, ctx->q, use_x0, tol, maxiter, hint, iters, resid,
-                          failed);
+  if (A.bs == 1 && K == 1) return cg64_loop<1, 1>(ctx, M, b, x, use_x0, tol, maxiter, hint, iters, resid, failed);
+  if (A.bs == 1 && K == 3) return cg64_loop<1, 3>(ctx, M, b, x, use_x0, tol, maxiter, hint, iters, resid, failed);
+  if (A.bs == 3 && K == 1) return cg64_loop<3, 1>(ctx, M, b, x, use_x0, tol, maxiter, hint, iters, resid, failed);
+  return fail(CPM_EINVAL, "SELL CG: unsupported (block size, rhs count)");
 }
 
 int ensure_workspace32(cpm_ctx* ctx, int64_t n) {
@@ -538,7 +526,7 @@ __global__ void __launch_bounds__(kBlock) k_rel_init(int64_t n, const double* __
 template <int K>
 __global__ void __launch_bounds__(kBlock) k_rel_xr(int64_t n, float* __restrict__ x, float* __restrict__ r,
                                                    const float* __restrict__ p, const float* __restrict__ q,
-                                                   double delta, CgState* st, double* partials, unsigned* counter) {
+                                                   const CgState* st, double* partials) {
   double a[K];
   bool live[K];
   bool any = false;
@@ -562,46 +550,16 @@ __global__ void __launch_bounds__(kBlock) k_rel_xr(int64_t n, float* __restrict_
     r[i] = ri;
     acc[j] += (double)ri * (double)ri;
   }
-  if (grid_sum<K>(acc, partials, counter) && threadIdx.x == 0) {
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
-      if (!live[j]) continue;
-      CgState& s = st[j];
-      s.rho_old = s.rho;
-      s.rho = acc[j];
-      s.it += 1;
-      s.resid = sqrt(acc[j]);
-      if (s.resid < delta * s.maxr || s.resid <= s.atol || s.it >= s.maxiter) s.upd = 1;
-    }
-  }
+  block_partial<K>(acc, partials);
 }
 
-// x64 += x32, x32 = 0 on the columns that requested an FP64 update
-template <int K>
-__global__ void __launch_bounds__(kBlock) k_rel_x(int64_t n, double* __restrict__ x, float* __restrict__ x32,
-                                                  const CgState* st) {
-  bool upd[K];
-  bool any = false;
-#pragma unroll
-  for (int j = 0; j < K; ++j) {
-    upd[j] = st[j].upd != 0;
-    any |= upd[j];
-  }
-  if (!any) return;
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-    const int j = (int)(i % K);
-    if (!upd[j]) continue;
-    x[i] += (double)x32[i];
-    x32[i] = 0.0f;
-  }
-}
-
-// r64 = b - y (y = A64 x64), r32 = r64, convergence on the FP64 residual
+// On the columns that requested an update (y = A (x64 + x32) computed just
+// before): x64 += x32, x32 = 0, r64 = b - y, r32 = r64, partials of r64.r64.
 template <int K>
 __global__ void __launch_bounds__(kBlock) k_rel_r(int64_t n, const double* __restrict__ b, const double* __restrict__ y,
-                                                  double* __restrict__ r64, float* __restrict__ r32, CgState* st,
-                                                  double* partials, unsigned* counter) {
+                                                  double* __restrict__ x, float* __restrict__ x32,
+                                                  double* __restrict__ r64, float* __restrict__ r32,
+                                                  const CgState* st, double* partials) {
   bool upd[K];
   bool any = false;
 #pragma unroll
@@ -617,106 +575,49 @@ __global__ void __launch_bounds__(kBlock) k_rel_r(int64_t n, const double* __res
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
     const int j = (int)(i % K);
     if (!upd[j]) continue;
+    x[i] += (double)x32[i];
+    x32[i] = 0.0f;
     const double ri = b[i] - y[i];
     r64[i] = ri;
     r32[i] = (float)ri;
     acc[j] += ri * ri;
   }
-  if (grid_sum<K>(acc, partials, counter) && threadIdx.x == 0) {
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
-      if (!upd[j]) continue;
-      CgState& s = st[j];
-      s.upd = 0;
-      s.rho = acc[j];
-      s.resid = sqrt(acc[j]);
-      s.maxr = s.resid;
-      if (s.resid <= s.atol) {
-        s.done = 1;
-      } else if (s.it >= s.maxiter) {
-        s.done = 1;
-        s.fail = 1;
-      }
-    }
-  }
+  block_partial<K>(acc, partials);
 }
 
-template <int K>
+template <int BS, int K>
 static int rel_loop(cpm_ctx* ctx, cpm_mat* M, const double* b, double* x, int use_x0, double tol, int maxiter,
                     double delta, int hint, int32_t* iters, double* resid, int* failed) {
   const Sell& A = M->sell;
-  const int64_t n = (int64_t)A.nbrows * A.bs * K;
-  const int g = grid_for(n);
-  const int64_t rows = ((int64_t)A.nbrows + 63) & ~63;
-  const int gs = grid_for(rows);
+  const int64_t n = (int64_t)A.nbrows * BS * K;
+  const int g = grid_for(n), gs = sell_grid(A);
   hipStream_t s = ctx->stream;
   CgState* st = ctx->cg;
   double* y64 = ctx->y64;
-  constexpr int BS1 = 1;
-  if (use_x0) {
-    if (A.bs == 3) k_sell_spmv<3, 1, double, false><<<gs, kBlock, 0, s>>>(A.nbrows, A.sptr, A.col, A.v64, A.dg64, x, y64, st, ctx->partials, ctx->counter);
-    else k_sell_spmv<BS1, K, double, false><<<gs, kBlock, 0, s>>>(A.nbrows, A.sptr, A.col, A.v64, A.dg64, x, y64, st, ctx->partials, ctx->counter);
-  }
+  if (use_x0)
+    if (int rc = spmv_t<BS, K, double, false>(ctx, A, x, nullptr, y64)) return rc;
   k_rel_init<K><<<g, kBlock, 0, s>>>(n, b, x, y64, ctx->r, ctx->r32, ctx->x32, use_x0, tol, maxiter, st, ctx->partials,
                                      ctx->counter);
   k_mcg_zero_b<K, double><<<g, kBlock, 0, s>>>(n, x, st);
   CPM_HIP(hipGetLastError());
-  CgState h[3] = {};
-  int batch = hint > 2 ? hint - 1 : 8;
-  const int max_batch = ctx->prof ? 256 : 4096;
-  int issued = 0;
-  for (;;) {
-    const int todo = std::max(1, std::min(std::min(batch, max_batch), maxiter - issued + 1));
-    int it_before = 0;
-    for (int j = 0; j < K; ++j) it_before = std::max(it_before, h[j].it);
+  auto issue = [&](int todo) -> int {
     for (int k = 0; k < todo; ++k) {
       k_mcg_pupd<K, float><<<g, kBlock, 0, s>>>(n, ctx->r32, ctx->p32, st);
       if (ctx->prof) CPM_HIP(hipEventRecord(ctx->ev[2 * k], s));
-      if (A.bs == 3)
-        k_sell_spmv<3, 1, float, true><<<gs, kBlock, 0, s>>>(A.nbrows, A.sptr, A.col, A.v32, A.dgc, ctx->p32, ctx->q32,
-                                                              st, ctx->partials, ctx->counter);
-      else
-        k_sell_spmv<BS1, K, float, true><<<gs, kBlock, 0, s>>>(A.nbrows, A.sptr, A.col, A.v32, A.dgc, ctx->p32,
-                                                                ctx->q32, st, ctx->partials, ctx->counter);
+      if (int rc = spmv_t<BS, K, float, true>(ctx, A, ctx->p32, nullptr, ctx->q32)) return rc;
       if (ctx->prof) CPM_HIP(hipEventRecord(ctx->ev[2 * k + 1], s));
-      k_rel_xr<K><<<g, kBlock, 0, s>>>(n, ctx->x32, ctx->r32, ctx->p32, ctx->q32, delta, st, ctx->partials,
-                                       ctx->counter);
-      k_rel_x<K><<<g, kBlock, 0, s>>>(n, x, ctx->x32, st);
-      if (A.bs == 3)
-        k_sell_spmv<3, 1, double, false, true><<<gs, kBlock, 0, s>>>(A.nbrows, A.sptr, A.col, A.v64, A.dg64, x, y64,
-                                                                      st, ctx->partials, ctx->counter);
-      else
-        k_sell_spmv<BS1, K, double, false, true><<<gs, kBlock, 0, s>>>(A.nbrows, A.sptr, A.col, A.v64, A.dg64, x, y64,
-                                                                        st, ctx->partials, ctx->counter);
-      k_rel_r<K><<<g, kBlock, 0, s>>>(n, b, y64, ctx->r, ctx->r32, st, ctx->partials, ctx->counter);
+      reduce_t<K, RED_ALPHA>(ctx, gs);
+      k_rel_xr<K><<<g, kBlock, 0, s>>>(n, ctx->x32, ctx->r32, ctx->p32, ctx->q32, st, ctx->partials);
+      reduce_t<K, RED_RR_REL>(ctx, g, delta);
+      // reliable update (runs only when a column asked for it): y = A (x64 + x32)
+      if (int rc = spmv_t<BS, K, double, false, true, true>(ctx, A, x, ctx->x32, y64)) return rc;
+      k_rel_r<K><<<g, kBlock, 0, s>>>(n, b, y64, x, ctx->x32, ctx->r, ctx->r32, st, ctx->partials);
+      reduce_t<K, RED_RR_TRUE>(ctx, g);
     }
-    issued += todo;
     CPM_HIP(hipGetLastError());
-    if (int rc = read_states(ctx, K, h)) return rc;
-    int it_max = 0;
-    bool all_done = true;
-    for (int j = 0; j < K; ++j) {
-      it_max = std::max(it_max, h[j].it);
-      all_done &= h[j].done != 0;
-    }
-    if (ctx->prof) {
-      for (int k = 0; k < it_max - it_before && k < todo; ++k) {
-        float ms = 0.f;
-        CPM_HIP(hipEventElapsedTime(&ms, ctx->ev[2 * k], ctx->ev[2 * k + 1]));
-        M->prof_ms += ms;
-        M->prof_launches += 1;
-      }
-    }
-    if (all_done || issued > maxiter) break;
-    batch = std::max(2, std::min(64, it_max / 4 + 2));
-  }
-  *failed = 0;
-  for (int j = 0; j < K; ++j) {
-    iters[j] = h[j].it;
-    resid[j] = h[j].resid;
-    if (h[j].fail || !h[j].done) *failed = 1;
-  }
-  return CPM_OK;
+    return CPM_OK;
+  };
+  return drive<K>(ctx, M, maxiter, hint, issue, iters, resid, failed);
 }
 
 int sell_cg_mixed(cpm_ctx* ctx, cpm_mat* M, int K, const double* b, double* x, int use_x0, double tol, int maxiter,
@@ -726,9 +627,10 @@ int sell_cg_mixed(cpm_ctx* ctx, cpm_mat* M, int K, const double* b, double* x, i
   if (int rc = ensure_workspace(ctx, n)) return rc;
   if (int rc = ensure_workspace32(ctx, n)) return rc;
   if (sweeps) *sweeps = 0;
-  if (K == 1) return rel_loop<1>(ctx, M, b, x, use_x0, tol, maxiter, delta, hint, iters, resid, failed);
-  if (K == 3 && A.bs == 1) return rel_loop<3>(ctx, M, b, x, use_x0, tol, maxiter, delta, hint, iters, resid, failed);
-  return fail(CPM_EINVAL, "mixed CG: unsupported right-hand-side count");
+  if (A.bs == 1 && K == 1) return rel_loop<1, 1>(ctx, M, b, x, use_x0, tol, maxiter, delta, hint, iters, resid, failed);
+  if (A.bs == 1 && K == 3) return rel_loop<1, 3>(ctx, M, b, x, use_x0, tol, maxiter, delta, hint, iters, resid, failed);
+  if (A.bs == 3 && K == 1) return rel_loop<3, 1>(ctx, M, b, x, use_x0, tol, maxiter, delta, hint, iters, resid, failed);
+  return fail(CPM_EINVAL, "mixed CG: unsupported (block size, rhs count)");
 }
 
 // interleave K columns (col-major blocks of n) <-> [i*K + j]; BS = 3 SoA <-> node-major
@@ -749,7 +651,7 @@ int interleave(cpm_ctx* ctx, int64_t n, int K, const double* src, double* dst, i
 }
 
 int sell_spmv64(cpm_ctx* ctx, const cpm_mat* M, int K, const double* x, double* y) {
-  return launch_sell<double>(ctx, M->sell, K, false, x, y, ctx->cg);
+  return launch_sell<double>(ctx, M->sell, K, x, y);
 }
 
 }  // namespace cpm
