@@ -1,0 +1,152 @@
+"""CPU tests of the product's host side: native assembly against the oracle,
+periodic map, Kuhn meshes, parameters, and the C-ABI exports."""
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from tests.cases import config, oracle_setup
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def _lib_or_skip():
+    from cupymag_amd import _lib
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("libcupymag_hip.so not built")
+    return _lib.lib()
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib_or_skip()
+    header = (ROOT / "include" / "cupymag_hip.h").read_text()
+    names = set(re.findall(r"^\s*(?:int|const char\*)\s+(cpm_\w+)\s*\(", header, re.M))
+    assert len(names) > 20
+    for name in sorted(names):
+        assert hasattr(L, name), name
+    assert L.cpm_abi_version() == int(re.search(r"CPM_ABI_VERSION (\d+)", header).group(1))
+
+
+@pytest.mark.parametrize("grid,rot", [("Hex", False), ("Tet", True)])
+def test_native_assembly_matches_oracle(grid, rot):
+    _lib_or_skip()
+    from cupymag_amd.mesh.kuhn import kuhn_mesh
+    from cupymag_amd.physics.host_ops import HostOps, voigt_C
+    arr = kuhn_mesh(5, jitter=0.15, seed=1) if grid == "Tet" else None
+    cfg = config(grid, n=6, me=True, rot=rot, defect_params=1.6)
+    p, ops, Avg, (coords, el, gid) = oracle_setup(cfg, arr)
+    H = HostOps(coords, el, gid, ops["n"], ops["DefDOF"], p["alpha1"], p["alpha2"],
+                voigt_C(p["c11"], p["c12"], p["c44"], p["rot111"]))
+    for name in ("A_demag", "Fx", "Fy", "Fz", "A1", "A2", "F1", "A_el", "F_el"):
+        A, B = ops[name], H.scipy(name)
+        A.sort_indices()
+        assert np.array_equal(A.indptr, B.indptr) and np.array_equal(A.indices, B.indices), name
+        assert abs(A - B).max() <= 2e-15 * abs(A).max(), name
+    H.free()
+
+
+def test_vectorised_periodic_map_matches_reference_loop():
+    from cupymag_amd.mesh.gridHex import HexGrid
+    from cupymag_amd.mesh.kuhn import kuhn_mesh
+    from cupymag_amd.mesh.periodic import periodic_node_map
+    from oracle import mesh as om
+    coords, tets = kuhn_mesh(4, jitter=0.2, seed=9)
+    ref = om.periodic_map(coords, om.tet_bounds(coords))
+    assert np.array_equal(periodic_node_map(coords, coords.min(0), coords.max(0)), ref)
+    g = HexGrid(3, 4, 5, 1, 1, 1)
+    c, el = g.std_fem_mesh()
+    rc, rel_, b = om.hex_grid(3, 4, 5, 1, 1, 1)
+    assert np.array_equal(c, rc) and np.array_equal(el, rel_)
+    assert np.array_equal(g.build_periodic_node_map(c), om.periodic_map(rc, b))
+
+
+def test_kuhn_mesh_orientation_and_dof_numbering():
+    from cupymag_amd.mesh.kuhn import kuhn_mesh
+    from cupymag_amd.mesh.periodic import periodic_node_map
+    n = 6
+    coords, tets = kuhn_mesh(n, jitter=0.15, seed=4)
+    x = coords[tets]
+    det = np.linalg.det(x[:, 1:] - x[:, :1])
+    assert (det > 0).all()
+    assert np.allclose(det.sum() / 6.0, n ** 3)
+    gid = periodic_node_map(coords, coords.min(0), coords.max(0))
+    i, j, k = np.meshgrid(np.arange(n + 1), np.arange(n + 1), np.arange(n + 1), indexing="ij")
+    expect = ((i % n) * n * n + (j % n) * n + (k % n)).ravel()
+    assert np.array_equal(gid, expect)
+
+
+def test_parameters_match_oracle():
+    from cupymag_amd.core.parameters import from_dict
+    from oracle import params
+    cfg = config("Tet", me=True, rot=True, sigma11=3e7, dt=5e-12)
+    P = from_dict(cfg)
+    p = params.reduced(cfg)
+    for a, b in (("dt", "dt"), ("alpha", "alpha1"), ("K1", "K1"), ("elasC1", None), ("s11", "s11")):
+        if b:
+            assert getattr(P, a) == p[b]
+    assert (P.elasC1, P.elasC2, P.elasC3) == p["elasC_raw"]
+    from cupymag_amd.utils.sigma_matrices import get_Ebar_sigma
+    assert np.array_equal(get_Ebar_sigma(P), p["Ebar_sigma"])
+
+
+def test_config_loader_and_cli_errors(tmp_path):
+    import yaml
+
+    from cupymag_amd.cli import main
+    from cupymag_amd.core.parameters import load_parameters
+    assert main([str(tmp_path / "missing.yaml")]) == 1
+    cfg = config("Hex")
+    cfg["output"]["directory"] = "out"
+    path = tmp_path / "c.yaml"
+    path.write_text(yaml.safe_dump(cfg))
+    P = load_parameters(path)
+    assert P.output_dir == (tmp_path / "out").resolve()
+    cfg["grid"]["type"] = "Prism"
+    path.write_text(yaml.safe_dump(cfg))
+    with pytest.raises(NotImplementedError):
+        load_parameters(path)
+
+
+def test_coo_api_matches_native_csr():
+    _lib_or_skip()
+    from scipy.sparse import coo_matrix
+
+    from cupymag_amd.mesh.kuhn import kuhn_mesh
+    from cupymag_amd.physics.assemble_coo import AssembleDemag, AssembleElasticity
+    from oracle import mesh as om
+    cfg = config("Tet", me=True, defect_params=1.2)
+    arr = kuhn_mesh(4, jitter=0.1, seed=2)
+    p, ops, Avg, (coords, el, gid) = oracle_setup(cfg, arr)
+    n = ops["n"]
+    r, c, v = AssembleDemag(coords, el, gid).build_coo_matrix_A_numba()
+    A = coo_matrix((v, (r, c)), shape=(n, n)).tocsr()
+    assert abs(A - ops["A_demag"]).max() < 1e-14 * abs(A).max()
+    E = AssembleElasticity(coords, el, gid, p["c11"], p["c12"], p["c44"], p["lambda100"], p["lambda111"])
+    r, c, v = E.build_coo_matrix_A_numba()
+    A = coo_matrix((v, (r, c)), shape=(3 * n, 3 * n)).tocsr()
+    assert abs(A - ops["A_el"]).max() < 1e-13 * abs(A).max()
+    r, c, v = E.build_coo_matrices_F_numba()
+    F = coo_matrix((v, (r, c)), shape=(3 * n, 6 * n)).tocsr()
+    assert abs(F - ops["F_el"]).max() < 1e-13 * abs(F).max()
+
+
+def test_vtu_writer_roundtrip(tmp_path):
+    import base64
+    import struct
+    import xml.etree.ElementTree as ET
+
+    from cupymag_amd.utils.volume_average import write_vtu
+    pts = np.random.default_rng(0).standard_normal((9, 3))
+    cells = np.array([[0, 1, 2, 3], [4, 5, 6, 7]])
+    write_vtu(tmp_path / "a.vtu", pts, cells, {"m1": np.arange(9.0)}, {"defect_flag": np.array([0, 1], np.int32)})
+    root = ET.parse(tmp_path / "a.vtu").getroot()
+    arrays = {a.get("Name"): a for a in root.iter("DataArray")}
+
+    def dec(a, dt):
+        raw = base64.b64decode(a.text)
+        nbytes = struct.unpack("<I", raw[:4])[0]
+        return np.frombuffer(raw[4:4 + nbytes], dtype=dt)
+    assert np.array_equal(dec(arrays["m1"], "<f8"), np.arange(9.0))
+    assert np.array_equal(dec(arrays["connectivity"], "<i8").reshape(2, 4), cells)
+    assert np.array_equal(dec(arrays["defect_flag"], "<i4"), [0, 1])
