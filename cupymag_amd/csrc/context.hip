@@ -51,7 +51,7 @@ extern "C" int cpm_ctx_create(int device, void* stream, cpm_ctx** out) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->ncu = prop.multiProcessorCount;
   hipError_t e = hipSuccess;
-  if (e == hipSuccess) e = hipMalloc(&c->partials, sizeof(double) * cpm::kMaxRed * cpm::kMaxGrid);
+  if (e == hipSuccess) e = hipMalloc(&c->partials, sizeof(double) * cpm::kMaxPartials);
   if (e == hipSuccess) e = hipMalloc(&c->counter, sizeof(unsigned) * 64);
   if (e == hipSuccess) e = hipMalloc(&c->scal, sizeof(double) * 256);
   if (e == hipSuccess) e = hipMalloc(&c->cg, sizeof(cpm::CgState) * 16);

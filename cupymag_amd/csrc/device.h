@@ -23,6 +23,8 @@ namespace cpm {
 constexpr int kBlock = 256;      // 4 wave64 per workgroup
 constexpr int kMaxGrid = 2048;   // 8 workgroups per CU on 256 CUs; grid-stride beyond
 constexpr int kMaxRed = 12;      // max values reduced together
+constexpr int kMaxPartials = kMaxRed * kMaxGrid;  // doubles in ctx->partials
+constexpr int kSellMaxGrid = 8192;  // SELL SpMV: up to K = 3 partials per workgroup still fit
 
 // Per-solve CG scalars, device resident.
 struct CgState {

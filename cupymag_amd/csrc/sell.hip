@@ -165,8 +165,11 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(int32_t nbrows, const int6
     double sum[NV];  // FP64 accumulation for both value types
 #pragma unroll
     for (int v = 0; v < NV; ++v) sum[v] = 0.0;
-    for (int64_t slot = s0; slot < s1; ++slot) {
-      const int64_t c = col[slot * 64 + l];
+    // matrix planes are read once per launch: streaming (non-temporal) loads
+    // keep them from evicting the gathered vector entries; two slots per
+    // trip keep more loads in flight
+    auto slot_fma = [&](int64_t slot) {
+      const int64_t c = __builtin_nontemporal_load(&col[slot * 64 + l]);
       double xv[NV];
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
@@ -177,11 +180,17 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(int32_t nbrows, const int6
       for (int d = 0; d < BS; ++d)
 #pragma unroll
         for (int e = 0; e < BS; ++e) {
-          const double a = (double)val[(slot * BB + d * BS + e) * 64 + l];
+          const double a = (double)__builtin_nontemporal_load(&val[(slot * BB + d * BS + e) * 64 + l]);
 #pragma unroll
           for (int j = 0; j < K; ++j) sum[d * K + j] += a * xv[e * K + j];
         }
+    };
+    int64_t slot = s0;
+    for (; slot + 1 < s1; slot += 2) {
+      slot_fma(slot);
+      slot_fma(slot + 1);
     }
+    if (slot < s1) slot_fma(slot);
     if (row < nbrows) {
       double xo[NV];
 #pragma unroll
@@ -323,7 +332,13 @@ __global__ void __launch_bounds__(kBlock) k_fill0(int64_t n, double* __restrict_
 
 // ---------------------------------------------------------------- dispatch
 
-static inline int sell_grid(const Sell& A) { return grid_for(((int64_t)A.nbrows + 63) & ~63); }
+// one thread per (block) row, up to kSellMaxGrid workgroups (3 partials each
+// fit the partials buffer)
+static inline int sell_grid(const Sell& A) {
+  int64_t g = ((((int64_t)A.nbrows + 63) & ~63) + kBlock - 1) / kBlock;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(g, kSellMaxGrid));
+}
+static_assert(3 * kSellMaxGrid <= kMaxPartials, "partials buffer too small for the SELL grid");
 
 template <int BS, int K, class T, bool CG, bool GATE = false, bool ADDX32 = false>
 static int spmv_t(cpm_ctx* ctx, const Sell& A, const T* x, const float* x32, T* y) {
