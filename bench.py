@@ -69,7 +69,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_gpu(n, steps, warmup, steps64, modes):
+def run_gpu(n, steps, warmup, steps64, modes, rel_delta=0.3):
     """Run the plain FP64 path and the fast paths on the same operators and
     initial state; returns per-mode timings, CG iteration counts, the SpMV
     kernel timing of the elasticity operator and the magnetization after the
@@ -101,10 +101,11 @@ def run_gpu(n, steps, warmup, steps64, modes):
     out = dict(n_dof=S.n_dof, n_nodes=S.n_nodes, n_elements=S.n_elements, modes={})
     snap_at = warmup + steps64
     for mode in modes:
-        sim = GSPMSimulation(S, P, mode=mode)
+        base, _, opt = mode.partition("+")
+        sim = GSPMSimulation(S, P, mode=base, warm_start=(opt == "warm"), rel_delta=rel_delta)
         ctx = sim.ctx
         sim.set_m(m0)
-        k_timed = steps64 if mode == "fp64" else steps
+        k_timed = steps64 if base == "fp64" else steps
         snap = None
         done = 0
         for k in range(warmup):
@@ -138,17 +139,21 @@ def run_gpu(n, steps, warmup, steps64, modes):
             if done == snap_at:
                 snap = sim.get("M")
         rows, nnz = A_el.shape[0], A_el.nnz
-        if mode == "fp64":
+        if base == "fp64":
             # CSR FP64: values + column indices + row pointers + p read + q written
             kernel = "k_cg_spmv (A_el, CSR FP64)"
             bytes_launch = 12.0 * nnz + 4.0 * (rows + 1) + 16.0 * rows
         else:
-            # SELL-64 3x3 blocks: off-diagonal blocks 9 values + 1 column index,
-            # diagonal blocks 9 FP64 values, p read, q written
-            s_ = 8 if mode == "fast64" else 4
-            kernel = f"k_sell_spmv<3,1,{'double' if s_ == 8 else 'float'}> (A_el, SELL-64 3x3 blocks)"
+            # SELL-64 3x3 blocks: off-diagonal blocks 9 values (+ 1 column index
+            # unless the index-free stencil form is in use), diagonal blocks 9
+            # FP64 values, p read, q written
+            s_ = 8 if base == "fast64" else 4
+            info = A_el.sell_info()
+            idx = 0 if info["stencil"] else 4
+            form = "stencil" if info["stencil"] else "indexed"
+            kernel = f"k_sell_spmv<3,1,{'double' if s_ == 8 else 'float'}> (A_el, SELL-64 3x3 blocks, {form})"
             nb = rows // 3
-            bytes_launch = (nnz / 9.0 - nb) * (9 * s_ + 4) + nb * 72.0 + 2.0 * rows * s_
+            bytes_launch = (nnz / 9.0 - nb) * (9 * s_ + idx) + nb * 72.0 + 2.0 * rows * s_
         avg_s = ms.value / 1e3 / max(nl.value, 1)
         out["modes"][mode] = dict(elapsed=elapsed, steps=k_timed, lat=lat, iters=iters, m=snap,
                                   spmv=dict(kernel=kernel, launches=nl.value, avg_us=avg_s * 1e6,
@@ -190,7 +195,10 @@ def main():
     ap.add_argument("--steps", type=int, default=4, help="timed steps of the headline (mixed) path")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--steps-fp64", type=int, default=2, help="timed steps of the plain FP64 path")
-    ap.add_argument("--modes", default="fp64,fast64,mixed")
+    ap.add_argument("--modes", default="fp64,fast64,mixed",
+                    help="comma list of fp64, fast64, mixed; '+warm' appended to a fast mode starts every CG "
+                         "from the previous step's solution")
+    ap.add_argument("--rel-delta", type=float, default=0.3)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--cpu-n", type=int, default=40)
     ap.add_argument("--cpu-steps", type=int, default=1)
@@ -212,7 +220,7 @@ def main():
         del src, dst
     modes = [m for m in args.modes.split(",") if m]
     steps64 = min(args.steps_fp64, args.steps) if "fp64" in modes else args.steps_fp64
-    g = run_gpu(args.n, args.steps, args.warmup, steps64, modes)
+    g = run_gpu(args.n, args.steps, args.warmup, steps64, modes, args.rel_delta)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n, args.cpu_steps)
     head = "mixed" if "mixed" in modes else modes[-1]
     hm = g["modes"][head]
@@ -240,7 +248,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fast64": "f64", "mixed": "f32 CG in f64 refinement"}[head],
+        "dtype": {"fp64": "f64", "fast64": "f64", "mixed": "f32 CG in f64 refinement"}[head.split("+")[0]],
         "data": "synthetic",
         "config": {"workload": f"GSPM LLG step, {args.n}^3 Kuhn tet mesh, rotated-ellipsoid defect, "
                                f"magnetoelastic on, rot111, dt 10 ps, alpha 0.1, tol 1e-10",

@@ -37,7 +37,7 @@ class cpm_sim_params(C.Structure):
                 ("elasC", C.c_double * 3), ("lambda100", C.c_double), ("lambda111", C.c_double),
                 ("Nd", C.c_double * 3), ("Ebar_sigma", C.c_double * 6), ("R", C.c_double * 9),
                 ("Mv", C.c_double * 36), ("MvinvT", C.c_double * 36), ("mode", C.c_int32),
-                ("reserved0", C.c_int32), ("rel_delta", C.c_double), ("reserved", C.c_int32 * 4)]
+                ("warm_start", C.c_int32), ("rel_delta", C.c_double), ("reserved", C.c_int32 * 4)]
 
 
 class cpm_step_report(C.Structure):
@@ -60,7 +60,8 @@ _SIGS = {
     "cpm_cg": (C.c_int, [_P, _P, _P, _P, C.c_int, _D, C.c_int, C.POINTER(_I32), C.POINTER(_D)]),
     "cpm_prof_enable": (C.c_int, [_P, C.c_int]),
     "cpm_mat_prof": (C.c_int, [_P, C.POINTER(_D), C.POINTER(_I64), C.c_int]),
-    "cpm_mat_add_sell": (C.c_int, [_P, C.c_int, _P, _P, _P]),
+    "cpm_mat_add_sell": (C.c_int, [_P, C.c_int, _P, _P, _P, _P]),
+    "cpm_mat_sell_info": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(_I64)]),
     "cpm_cg_sell": (C.c_int, [_P, _P, C.c_int, _P, _P, C.c_int, _D, C.c_int, C.c_int, _D, C.POINTER(_I32),
                               C.POINTER(_D)]),
     "cpm_spmv_sell": (C.c_int, [_P, _P, C.c_int, _P, _P]),

@@ -45,7 +45,7 @@ class SimulationOperators(FEMMesh):
                 v = vals
             M = DeviceCSR(ip, ix, v, shape, ctx)
             if sell_bs and self.sell:
-                M.add_sell(sell_bs, ip, ix, v)
+                M.add_sell(sell_bs, ip, ix, v, self.grid_dims)
             if not keep:
                 M._host = None
             else:

@@ -31,11 +31,14 @@ class StepReport:
 
 
 class GSPMSimulation:
-    def __init__(self, SimOp, P, mode="fp64", rel_delta=0.1):
+    def __init__(self, SimOp, P, mode="fp64", rel_delta=0.3, warm_start=False):
         """mode: "fp64" (CSR operators, the reference's sequence of solves),
         "fast64" (SELL-64 operators, 3-RHS solves) or "mixed" (as fast64 with
         FP32 CG refined in FP64 by reliable updates: an FP64 residual update
-        whenever the FP32 recurrence residual has dropped by ``rel_delta``)."""
+        whenever the FP32 recurrence residual has dropped by ``rel_delta``).
+        ``warm_start`` (fast modes): every CG starts from the previous step's
+        solution instead of zero; the reference does so only when use_init
+        is set, so results differ at the level of the CG tolerance."""
         self.P = P
         self.SimOp = SimOp
         A_demag, Fx, Fy, Fz, self.Deriv = SimOp.csr_demag_and_deriv()
@@ -76,6 +79,9 @@ class GSPMSimulation:
             pr.MvinvT[k] = float(v)
         pr.mode = _lib.MODES[mode]
         pr.rel_delta = float(rel_delta)
+        if warm_start and mode == "fp64":
+            raise ValueError("warm_start is a fast-mode option")
+        pr.warm_start = int(bool(warm_start))
         self.mode = mode
         W = self.Avg.weights_device(self.ctx)
         dmask = np.zeros(n, dtype=np.uint8)

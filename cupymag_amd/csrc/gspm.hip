@@ -480,6 +480,7 @@ struct cpm_sim {
   double *bint = nullptr, *uint = nullptr, *gnint = nullptr, *mssint = nullptr;
   std::vector<void*> allocs;
   int hint[10] = {};
+  bool stepped = false;  // the warm-start buffers hold a previous solution
 };
 
 static int sim_alloc(cpm_sim* S, double** p, int64_t count) {
@@ -669,6 +670,7 @@ extern "C" int cpm_sim_step(cpm_sim* S, const double* H, int use_init, cpm_step_
   // solve) gathered into an interleaved vector, solved on the SELL form, and
   // scattered back into the reference's SoA layout; `xint` keeps the
   // interleaved solution as the next warm start.
+  const int use_x0_fast = (use_init || (p.warm_start && S->stepped)) ? 1 : 0;
   auto solve_sell = [&](int slot, const cpm_mat* A, int K, const double* b, double* x, double* xint) -> int {
     cpm_mat* Am = const_cast<cpm_mat*>(A);
     const int blocks = (A->sell.bs == 3 || K == 3) ? 3 : 1;
@@ -685,9 +687,9 @@ extern "C" int cpm_sim_step(cpm_sim* S, const double* H, int use_init, cpm_step_
     int hint = 0;
     for (int j = 0; j < K; ++j) hint = std::max(hint, S->hint[slot + j]);
     int rc = (p.mode == CPM_MODE_MIXED)
-                 ? sell_cg_mixed(ctx, Am, K, bb, xx, use_init, p.tol, p.maxiter, p.rel_delta, 0, it, res, &failed,
-                                 nullptr)
-                 : sell_cg64(ctx, Am, K, bb, xx, use_init, p.tol, p.maxiter, hint, it, res, &failed);
+                 ? sell_cg_mixed(ctx, Am, K, bb, xx, use_x0_fast, p.tol, p.maxiter, p.rel_delta, hint, it, res,
+                                 &failed, nullptr)
+                 : sell_cg64(ctx, Am, K, bb, xx, use_x0_fast, p.tol, p.maxiter, hint, it, res, &failed);
     if (rc) return rc;
     for (int j = 0; j < K; ++j) {
       rep->iters[slot + j] = it[j];
@@ -754,6 +756,7 @@ extern "C" int cpm_sim_step(cpm_sim* S, const double* H, int use_init, cpm_step_
   CPM_HIP(hipGetLastError());
   CPM_HIP(hipMemcpyAsync(ctx->pinned, S->dscal, S_NSCAL * sizeof(double), hipMemcpyDeviceToHost, s));
   CPM_HIP(hipStreamSynchronize(s));
+  S->stepped = true;
   rep->sumsq = ctx->pinned[S_SUMSQ];
   for (int k = 0; k < 3; ++k) rep->avg_m[k] = ctx->pinned[S_AVGM + k];
   return CPM_OK;

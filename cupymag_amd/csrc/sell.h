@@ -18,6 +18,12 @@ struct Sell {
   float* v32 = nullptr;     // device, FP32 copy of v64
   double* dg64 = nullptr;   // device, [brow][bs*bs] diagonal blocks
   double* dgc = nullptr;    // device, diagonal blocks of the FP32 form, compensated (see sell.hip)
+  // index-free form for DOFs numbered i*ny*nz + j*nz + k on a periodic grid:
+  // slot s of every row holds the neighbour at offset soff[s], col is unused
+  int stencil = 0;
+  int32_t nst = 0;           // offsets per row (14 Kuhn, 26 hex)
+  int32_t dims[3] = {0, 0, 0};
+  int32_t* soff = nullptr;   // device, packed (di+1) | (dj+1) << 2 | (dk+1) << 4
 };
 
 void sell_free(Sell& S);

@@ -45,7 +45,7 @@ namespace cpm {
 // ---------------------------------------------------------------- building
 
 int sell_build(cpm_ctx* ctx, Sell& S, int bs, int32_t nbrows, const int64_t* indptr, const int32_t* indices,
-               const double* values, int64_t n_csr_rows) {
+               const double* values, int64_t n_csr_rows, const int32_t* dims) {
   // Block-row b covers CSR rows {d*nbrows + b : d < bs} (bs = 3: the SoA DOF
   // order of the elasticity operator).  Its block columns are the columns
   // < nbrows of CSR row b; entry (d, e) of block (b, c) sits at CSR row
@@ -69,9 +69,49 @@ int sell_build(cpm_ctx* ctx, Sell& S, int bs, int32_t nbrows, const int64_t* ind
         if (indptr[d * nbrows + b + 1] - indptr[d * nbrows + b] != 3 * len)
           return fail(CPM_EINVAL, "SELL: block row is not a full 3x3 block pattern");
   }
+  // Index-free form: every off-diagonal block column is a neighbour of the
+  // row on the periodic dims[0] x dims[1] x dims[2] grid at one of <= 32
+  // fixed offsets in {-1,0,1}^3.  Otherwise (or without dims) keep indices.
+  std::vector<int32_t> codes;  // packed offsets, sorted
+  std::vector<int8_t> slot_of;  // per CSR entry of row segment 0: stencil slot (-1 diagonal / skipped)
+  bool stencil = dims && dims[0] >= 3 && dims[1] >= 3 && dims[2] >= 3 &&
+                 (int64_t)dims[0] * dims[1] * dims[2] == nbrows;
+  auto code_of = [&](int32_t b, int32_t c, int32_t* code) {
+    const int32_t nyz = dims[1] * dims[2];
+    const int32_t bi = b / nyz, bj = (b / dims[2]) % dims[1], bk = b % dims[2];
+    const int32_t ci = c / nyz, cj = (c / dims[2]) % dims[1], ck = c % dims[2];
+    int32_t d[3] = {ci - bi, cj - bj, ck - bk};
+    for (int a = 0; a < 3; ++a) {
+      if (d[a] > dims[a] / 2) d[a] -= dims[a];
+      if (d[a] < -(dims[a] / 2)) d[a] += dims[a];
+      if (d[a] < -1 || d[a] > 1) return false;
+    }
+    *code = (d[0] + 1) | ((d[1] + 1) << 2) | ((d[2] + 1) << 4);
+    return true;
+  };
+  if (stencil) {
+    std::vector<char> seen(64, 0);
+    for (int32_t b = 0; b < nbrows && stencil; ++b)
+      for (int64_t p = indptr[b]; p < indptr[b + 1]; ++p) {
+        const int32_t c = indices[p];
+        if (c >= nbrows || c == b) continue;
+        int32_t code;
+        if (!code_of(b, c, &code)) {
+          stencil = false;
+          break;
+        }
+        seen[code] = 1;
+      }
+    for (int k = 0; k < 64 && stencil; ++k)
+      if (seen[k]) codes.push_back(k);
+    if (codes.size() > 32) stencil = false;
+  }
+  std::vector<int32_t> slot_index(64, -1);
+  for (size_t k = 0; k < codes.size(); ++k) slot_index[codes[k]] = (int32_t)k;
   for (int64_t s = 0; s < nsl; ++s) {
     int32_t w = 0;
     for (int64_t b = s * 64; b < std::min<int64_t>(nbrows, s * 64 + 64); ++b) w = std::max(w, slots[b]);
+    if (stencil) w = (int32_t)codes.size();
     sptr[s + 1] = sptr[s] + w;
   }
   const int64_t nslot = sptr[nsl];
@@ -99,9 +139,15 @@ int sell_build(cpm_ctx* ctx, Sell& S, int bs, int32_t nbrows, const int64_t* ind
         for (int t = 0; t < bb; ++t) dg[(int64_t)b * bb + t] = blk[t];
         continue;
       }
-      col[(sptr[s] + j) * 64 + l] = c;
+      int64_t slot = j;
+      if (stencil) {
+        int32_t code = 0;
+        code_of(b, c, &code);
+        slot = slot_index[code];
+      }
+      col[(sptr[s] + slot) * 64 + l] = c;
       for (int t = 0; t < bb; ++t) {
-        val[((sptr[s] + j) * bb + t) * 64 + l] = blk[t];
+        val[((sptr[s] + slot) * bb + t) * 64 + l] = blk[t];
         comp[t] += blk[t] - (double)(float)blk[t];
       }
       ++j;
@@ -110,13 +156,21 @@ int sell_build(cpm_ctx* ctx, Sell& S, int bs, int32_t nbrows, const int64_t* ind
   }
   S.bs = bs;
   S.nbrows = nbrows;
+  S.stencil = stencil ? 1 : 0;
+  S.nst = stencil ? (int32_t)codes.size() : 0;
+  if (stencil)
+    for (int a = 0; a < 3; ++a) S.dims[a] = dims[a];
   S.nslices = (int32_t)nsl;
   S.nslot = nslot;
   std::vector<float> v32(val.size());
   for (size_t i = 0; i < val.size(); ++i) v32[i] = (float)val[i];
   hipStream_t st = ctx->stream;
   CPM_HIP(hipMalloc(&S.sptr, sizeof(int64_t) * (nsl + 1)));
-  CPM_HIP(hipMalloc(&S.col, sizeof(int32_t) * std::max<int64_t>(1, col.size())));
+  if (stencil) {
+    CPM_HIP(hipMalloc(&S.soff, sizeof(int32_t) * std::max<size_t>(1, codes.size())));
+    CPM_HIP(hipMemcpyAsync(S.soff, codes.data(), sizeof(int32_t) * codes.size(), hipMemcpyHostToDevice, st));
+  }
+  CPM_HIP(hipMalloc(&S.col, sizeof(int32_t) * (stencil ? 1 : std::max<int64_t>(1, col.size()))));
   CPM_HIP(hipMalloc(&S.v64, sizeof(double) * std::max<size_t>(1, val.size())));
   CPM_HIP(hipMalloc(&S.v32, sizeof(float) * std::max<size_t>(1, val.size())));
   CPM_HIP(hipMalloc(&S.dg64, sizeof(double) * dg.size()));
@@ -124,7 +178,7 @@ int sell_build(cpm_ctx* ctx, Sell& S, int bs, int32_t nbrows, const int64_t* ind
   CPM_HIP(hipMemcpyAsync(S.dg64, dg.data(), sizeof(double) * dg.size(), hipMemcpyHostToDevice, st));
   CPM_HIP(hipMemcpyAsync(S.dgc, dgc.data(), sizeof(double) * dgc.size(), hipMemcpyHostToDevice, st));
   CPM_HIP(hipMemcpyAsync(S.sptr, sptr.data(), sizeof(int64_t) * (nsl + 1), hipMemcpyHostToDevice, st));
-  CPM_HIP(hipMemcpyAsync(S.col, col.data(), sizeof(int32_t) * col.size(), hipMemcpyHostToDevice, st));
+  if (!stencil) CPM_HIP(hipMemcpyAsync(S.col, col.data(), sizeof(int32_t) * col.size(), hipMemcpyHostToDevice, st));
   CPM_HIP(hipMemcpyAsync(S.v64, val.data(), sizeof(double) * val.size(), hipMemcpyHostToDevice, st));
   CPM_HIP(hipMemcpyAsync(S.v32, v32.data(), sizeof(float) * v32.size(), hipMemcpyHostToDevice, st));
   CPM_HIP(hipStreamSynchronize(st));
@@ -132,7 +186,7 @@ int sell_build(cpm_ctx* ctx, Sell& S, int bs, int32_t nbrows, const int64_t* ind
 }
 
 void sell_free(Sell& S) {
-  for (void* p : {(void*)S.sptr, (void*)S.col, (void*)S.v64, (void*)S.v32, (void*)S.dg64, (void*)S.dgc})
+  for (void* p : {(void*)S.sptr, (void*)S.col, (void*)S.v64, (void*)S.v32, (void*)S.dg64, (void*)S.dgc, (void*)S.soff})
     if (p) (void)hipFree(p);
   S = Sell{};
 }
@@ -141,12 +195,17 @@ void sell_free(Sell& S) {
 
 // y = A x (+ fused per-column dot x.y when Dot) for BS x BS blocks and K
 // interleaved right-hand sides: x[(c*BS + e)*K + j].
-template <int BS, int K, class T, bool CG, bool GATE = false, bool ADDX32 = false>
+struct GridArg {
+  int32_t nx, ny, nz, nst;
+  const int32_t* soff;
+};
+
+template <int BS, int K, class T, bool CG, bool GATE = false, bool ADDX32 = false, bool ST = false>
 __global__ void __launch_bounds__(kBlock) k_sell_spmv(int32_t nbrows, const int64_t* __restrict__ sptr,
                                                       const int32_t* __restrict__ col, const T* __restrict__ val,
                                                       const double* __restrict__ dg, const T* __restrict__ x,
                                                       const float* __restrict__ x32, T* __restrict__ y,
-                                                      const CgState* st, double* partials) {
+                                                      const CgState* st, double* partials, GridArg ga) {
   // y = A x (ADDX32: A (x + x32)); CG: per-column partials of x.y.  GATE:
   // run only when some column requested an FP64 residual update.
   constexpr int BB = BS * BS;
@@ -161,7 +220,15 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(int32_t nbrows, const int6
   const int64_t stride = (int64_t)gridDim.x * kBlock;
   for (int64_t row = (int64_t)blockIdx.x * kBlock + threadIdx.x; row < (int64_t)((nbrows + 63) & ~63); row += stride) {
     const int64_t s = row >> 6, l = row & 63;
-    const int64_t s0 = sptr[s], s1 = sptr[s + 1];
+    const int64_t s0 = ST ? s * ga.nst : sptr[s];
+    const int64_t s1 = ST ? s0 + ga.nst : sptr[s + 1];
+    int32_t ri = 0, rj = 0, rk = 0;
+    if (ST && row < nbrows) {
+      const int32_t r = (int32_t)row, nyz = ga.ny * ga.nz;
+      ri = r / nyz;
+      rj = (r / ga.nz) % ga.ny;
+      rk = r % ga.nz;
+    }
     double sum[NV];  // FP64 accumulation for both value types
 #pragma unroll
     for (int v = 0; v < NV; ++v) sum[v] = 0.0;
@@ -169,7 +236,21 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(int32_t nbrows, const int6
     // keep them from evicting the gathered vector entries; two slots per
     // trip keep more loads in flight
     auto slot_fma = [&](int64_t slot) {
-      const int64_t c = __builtin_nontemporal_load(&col[slot * 64 + l]);
+      int64_t c;
+      if (ST) {
+        // neighbour on the periodic grid (padding rows gather row 0)
+        const int32_t code = ga.soff[slot - s0];
+        int32_t i2 = ri + (code & 3) - 1, j2 = rj + ((code >> 2) & 3) - 1, k2 = rk + ((code >> 4) & 3) - 1;
+        i2 += (i2 < 0) ? ga.nx : 0;
+        i2 -= (i2 >= ga.nx) ? ga.nx : 0;
+        j2 += (j2 < 0) ? ga.ny : 0;
+        j2 -= (j2 >= ga.ny) ? ga.ny : 0;
+        k2 += (k2 < 0) ? ga.nz : 0;
+        k2 -= (k2 >= ga.nz) ? ga.nz : 0;
+        c = (row < nbrows) ? ((int64_t)i2 * ga.ny + j2) * ga.nz + k2 : 0;
+      } else {
+        c = __builtin_nontemporal_load(&col[slot * 64 + l]);
+      }
       double xv[NV];
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
@@ -351,8 +432,13 @@ static int spmv_t(cpm_ctx* ctx, const Sell& A, const T* x, const float* x32, T* 
     val = A.v32;
     dg = A.dgc;
   }
-  k_sell_spmv<BS, K, T, CG, GATE, ADDX32><<<sell_grid(A), kBlock, 0, ctx->stream>>>(A.nbrows, A.sptr, A.col, val, dg,
-                                                                                    x, x32, y, ctx->cg, ctx->partials);
+  const GridArg ga{A.dims[0], A.dims[1], A.dims[2], A.nst, A.soff};
+  if (A.stencil)
+    k_sell_spmv<BS, K, T, CG, GATE, ADDX32, true><<<sell_grid(A), kBlock, 0, ctx->stream>>>(
+        A.nbrows, A.sptr, A.col, val, dg, x, x32, y, ctx->cg, ctx->partials, ga);
+  else
+    k_sell_spmv<BS, K, T, CG, GATE, ADDX32, false><<<sell_grid(A), kBlock, 0, ctx->stream>>>(
+        A.nbrows, A.sptr, A.col, val, dg, x, x32, y, ctx->cg, ctx->partials, ga);
   CPM_HIP(hipGetLastError());
   return CPM_OK;
 }
@@ -674,14 +760,22 @@ int sell_spmv64(cpm_ctx* ctx, const cpm_mat* M, int K, const double* x, double* 
 // ---------------------------------------------------------------- C ABI
 
 extern "C" int cpm_mat_add_sell(cpm_mat* A, int bs, const int64_t* indptr, const int32_t* indices,
-                                const double* values) {
+                                const double* values, const int32_t* grid_dims) {
   if (!A || !indptr || !indices || !values) return cpm::fail(CPM_EINVAL, "cpm_mat_add_sell: null argument");
   if (A->nrows != A->ncols) return cpm::fail(CPM_EINVAL, "cpm_mat_add_sell: operator must be square");
   if (bs != 1 && bs != 3) return cpm::fail(CPM_EINVAL, "cpm_mat_add_sell: block size must be 1 or 3");
   if (A->nrows % bs) return cpm::fail(CPM_EINVAL, "cpm_mat_add_sell: rows not divisible by the block size");
   if (indptr[A->nrows] != A->nnz) return cpm::fail(CPM_EINVAL, "cpm_mat_add_sell: pattern differs from the operator");
   cpm::sell_free(A->sell);
-  return cpm::sell_build(A->ctx, A->sell, bs, A->nrows / bs, indptr, indices, values, A->nrows);
+  return cpm::sell_build(A->ctx, A->sell, bs, A->nrows / bs, indptr, indices, values, A->nrows, grid_dims);
+}
+
+extern "C" int cpm_mat_sell_info(const cpm_mat* A, int* bs, int* stencil, int64_t* slots) {
+  if (!A) return cpm::fail(CPM_EINVAL, "cpm_mat_sell_info: null matrix");
+  if (bs) *bs = A->sell.bs;
+  if (stencil) *stencil = A->sell.stencil;
+  if (slots) *slots = A->sell.nslot;
+  return CPM_OK;
 }
 
 extern "C" int cpm_cg_sell(cpm_ctx* ctx, cpm_mat* A, int nrhs, const double* b, double* x, int use_x0, double tol,

@@ -94,17 +94,27 @@ class DeviceCSR:
         A = A.tocsr()
         return cls(A.indptr, A.indices, A.data, A.shape, ctx)
 
-    def add_sell(self, bs, indptr=None, indices=None, values=None):
+    def add_sell(self, bs, indptr=None, indices=None, values=None, grid_dims=None):
         """Attach the SELL-64 form (FP64 + FP32 values) used by the fast
-        solvers; bs = 3 for the 3x3-block elasticity operator."""
+        solvers; bs = 3 for the 3x3-block elasticity operator.  grid_dims:
+        DOF grid (nx, ny, nz) of a structured periodic mesh numbered
+        (i*ny + j)*nz + k; the library then stores no column indices when the
+        pattern is a neighbour stencil (and keeps them otherwise)."""
         if indptr is None:
             indptr, indices, values = self.to_arrays()
         indptr = np.ascontiguousarray(indptr, dtype=np.int64)
         indices = np.ascontiguousarray(indices, dtype=np.int32)
         values = np.ascontiguousarray(values, dtype=np.float64)
-        check(lib().cpm_mat_add_sell(self.h, int(bs), ptr(indptr), ptr(indices), ptr(values)), "cpm_mat_add_sell")
+        dims = None if grid_dims is None else np.ascontiguousarray(grid_dims, dtype=np.int32)
+        check(lib().cpm_mat_add_sell(self.h, int(bs), ptr(indptr), ptr(indices), ptr(values), ptr(dims)),
+              "cpm_mat_add_sell")
         self.sell_bs = int(bs)
         return self
+
+    def sell_info(self):
+        bs, st, sl = C.c_int(), C.c_int(), C.c_int64()
+        check(lib().cpm_mat_sell_info(self.h, C.byref(bs), C.byref(st), C.byref(sl)), "cpm_mat_sell_info")
+        return dict(bs=bs.value, stencil=bool(st.value), slots=sl.value)
 
     def matvec(self, x, out=None, valset=0):
         x = as_device(x, self.ctx)

@@ -12,6 +12,7 @@ from ..utils.defect_shapes import get_defect_shape_function
 from ..utils.final_assembly import extract_defect_dofs
 from .gridHex import HexGrid
 from .gridTet import TetraGrid
+from .periodic import structured_dims
 
 
 class FEMMesh:
@@ -34,6 +35,7 @@ class FEMMesh:
         self.n_nodes = self.node_coords_np.shape[0]
         self.n_elements = self.elements_np.shape[0]
         self.DefDOF = extract_defect_dofs(self.elements_np, self.global_id_np)
+        self.grid_dims = structured_dims(self.node_coords_np, self.global_id_np)
 
     def get_mesh_data(self):
         return {"node_coords_np": self.node_coords_np, "elements_np": self.elements_np,

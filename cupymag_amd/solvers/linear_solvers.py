@@ -38,7 +38,7 @@ def solve_cg(A, b, M=None, x0=None, tol=1e-7, maxiter=5000, use_init=False, syst
     return x
 
 
-def solve_cg_sell(A, b, x0=None, tol=1e-7, maxiter=5000, precision=64, rel_delta=0.1, nrhs=1):
+def solve_cg_sell(A, b, x0=None, tol=1e-7, maxiter=5000, precision=64, rel_delta=0.3, nrhs=1):
     """CG on the SELL-64 form of ``A`` (see DeviceCSR.add_sell) for ``nrhs``
     interleaved right-hand sides (b[i*nrhs + j]); bs-3 operators take
     node-major interleaved vectors.  precision 64: CuPy's recurrence per

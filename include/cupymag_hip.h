@@ -112,8 +112,16 @@ int cpm_mat_prof(cpm_mat* A, double* total_ms, int64_t* launches, int reset);
  * to a square operator from the same host CSR arrays it was created from.
  * bs = 3 groups rows/columns {i, n+i, 2n+i} into 3x3 blocks (the elasticity
  * operator, DOF order of assemble_elasticity.py:391) and its solver works on
- * node-major interleaved vectors. */
-int cpm_mat_add_sell(cpm_mat* A, int bs, const int64_t* indptr, const int32_t* indices, const double* values);
+ * node-major interleaved vectors.  grid_dims (3 ints, may be NULL): the DOF
+ * grid of a structured periodic mesh; when the pattern is a fixed
+ * neighbour stencil of that grid the column indices are not stored. */
+int cpm_mat_add_sell(cpm_mat* A, int bs, const int64_t* indptr, const int32_t* indices, const double* values,
+                     const int32_t* grid_dims);
+/* SELL form in use: block size, 1 if index-free (grid_dims given to
+ * cpm_mat_add_sell, DOF (i,j,k) numbered (i*ny + j)*nz + k on a periodic
+ * grid, and every block column a neighbour at an offset in {-1,0,1}^3), and
+ * the number of 64-row slots. */
+int cpm_mat_sell_info(const cpm_mat* A, int* bs, int* stencil, int64_t* slots);
 /* CG on the SELL form for nrhs interleaved right-hand sides (b[i*nrhs + j];
  * nrhs = 3 only for bs = 1; bs = 3 vectors are node-major interleaved).
  * precision 64: CuPy's CG recurrence per column (as cpm_cg).  precision 32:
@@ -188,7 +196,8 @@ typedef struct cpm_sim_params {
   double Mv[36];              /* rot_111_matrices.get_M_matrix (row-major) */
   double MvinvT[36];          /* inv(Mv)^T (Micromagnetics.py:54) */
   int32_t mode;               /* CPM_MODE_* */
-  int32_t reserved0;
+  int32_t warm_start;         /* fast modes: start every CG from the previous step's solution
+                                 (the reference does so only when use_init is set) */
   double rel_delta;           /* CPM_MODE_MIXED: FP64 residual update when the FP32 recurrence
                                  residual drops by this factor (reliable updates) */
   int32_t reserved[4];

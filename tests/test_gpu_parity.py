@@ -268,13 +268,14 @@ def test_sell_cg_matches_oracle(which):
         assert _rel(x32.cpu().numpy(), _interleave([xr[:n], xr[n:2 * n], xr[2 * n:]])) < 1e-8
 
 
-@pytest.mark.parametrize("mode", ["fast64", "mixed"])
+@pytest.mark.parametrize("mode", ["fast64", "mixed", "mixed+warm"])
 @pytest.mark.parametrize("case", [dict(grid="Tet", n=6, me=True, rot=True, dt=5e-12),
                                   dict(grid="Hex", n=6, me=True, rot=False, sigma11=1e8)],
                          ids=["tet-me-rot", "hex-me"])
 def test_fast_modes_match_oracle(mode, case):
     """fast64: same tolerances as the plain FP64 path (m 1e-10, CG iterations
-    +-1).  mixed: magnetization within 1e-7 of the FP64 oracle."""
+    +-1).  mixed (and mixed with warm starts): magnetization within 1e-7 of
+    the FP64 oracle."""
     from cupymag_amd.core.gspm import GSPMSimulation
     from oracle import gspm as og
     case = dict(case)
@@ -284,7 +285,8 @@ def test_fast_modes_match_oracle(mode, case):
     P, S = _product(cfg, arr)
     p, ops, Avg, _ = oracle_setup(cfg, arr)
     m0 = tilted_m(ops["n"], seed=12)
-    sim = GSPMSimulation(S, P, mode=mode)
+    base, _, opt = mode.partition("+")
+    sim = GSPMSimulation(S, P, mode=base, warm_start=(opt == "warm"))
     sim.set_m(m0)
     st = og.init_state(ops, Avg, m0)
     p["H_now"] = p["H"]
@@ -302,3 +304,88 @@ def test_fast_modes_match_oracle(mode, case):
             assert _rel(sim.get("UEL").cpu().numpy(), st["u"]) < 1e-10
         else:
             assert err < 1e-7, f"step {k}: {err}"
+
+
+def test_cli_hysteresis_loop_matches_oracle(tmp_path):
+    """`cupymag config.yaml` end to end on a small hex mesh: field points of
+    hysteresis.txt (LLG convergence -> Hext -= dHext) match the oracle's loop,
+    VTU snapshots and last_m are written."""
+    import yaml
+
+    from cupymag_amd.cli import main
+    from oracle import gspm as og
+    cfg = config("Hex", n=6, me=False, ndef=2, hext=(-2e4, 0.0, 0.0))
+    cfg["physics"]["external_field"]["dhext1"] = 2e4
+    cfg["simulation"]["llg"].update(llg_accuracy_factor=1e-6, timestep=5e-12)
+    cfg["simulation"]["stop_condition"] = {"component": 0, "value": -0.5}
+    cfg["output"] = {"directory": "out", "write_magnetization": True, "save_frequency": 100000,
+                     "vtu_blend_alpha": 0.5}
+    path = tmp_path / "cfg.yaml"
+    path.write_text(yaml.safe_dump(cfg))
+    assert main([str(path), "--max-steps", "400"]) == 0
+    out = tmp_path / "out"
+    lines = (out / "hysteresis.txt").read_text().splitlines()
+    assert lines[0] == "Hext1\tHext2\tHext3\tavg_m1\tavg_m2\tavg_m3"
+    assert list(out.glob("field_*.vtu"))
+    assert (out / "last_m.h5").exists() or (out / "last_m.npy").exists()
+    # oracle: same control flow
+    p, ops, Avg, _ = oracle_setup(cfg)
+    st = og.init_state(ops, Avg, og.uniform_m(ops["n"]))
+    pts = og.run(st, ops, Avg, p, 400)
+    ms = cfg["physics"]["magnetic"]["ms"]
+    got = [list(map(float, l.split("\t"))) for l in lines[1:]]
+    # one line per LLG convergence, then the final state
+    assert len(pts) >= 1 and len(got) == len(pts) + 1
+    for (H, am), g in zip(pts, got):
+        assert abs(H[0] * ms - g[0]) < 0.1
+        assert np.allclose(am, g[3:], atol=1e-9)
+    # the loop stopped on the stop condition (m1 < -0.5) at the next field point
+    assert got[-1][3] <= -0.5 and abs(got[-1][0] - (pts[-1][0][0] - p["dH"][0]) * ms) < 0.1
+
+
+@pytest.mark.parametrize("grid", ["Tet", "Hex"])
+def test_stencil_and_indexed_sell_agree(grid):
+    """The index-free SELL form (structured periodic grid) and the indexed
+    form give the same products; a mesh with permuted node numbering falls
+    back to the indexed form."""
+    from cupymag_amd._lib import check, lib, ptr
+    from cupymag_amd.device import DeviceCSR
+    cfg = config(grid, n=6, me=True, defect_params=1.5)
+    P, S = _product(cfg, _kuhn(6) if grid == "Tet" else None)
+    assert S.grid_dims == (6, 6, 6)
+    A_el = S.csr_elasticity()[0]
+    A1 = S.csr_GS()[0]
+    assert A_el.sell_info()["stencil"] and A1.sell_info()["stencil"]
+    rng = np.random.default_rng(3)
+    for A, bs, K in ((A_el, 3, 1), (A1, 1, 3)):
+        ip, ix, v = A.to_arrays()
+        B = DeviceCSR(ip, ix, v, A.shape).add_sell(bs)
+        assert not B.sell_info()["stencil"]
+        x = torch.as_tensor(rng.standard_normal(A.shape[0] * (K if bs == 1 else 1)), device="cuda")
+        y1 = torch.empty_like(x)
+        y2 = torch.empty_like(x)
+        check(lib().cpm_spmv_sell(A.ctx.h, A.h, K, ptr(x), ptr(y1)))
+        check(lib().cpm_spmv_sell(B.ctx.h, B.h, K, ptr(x), ptr(y2)))
+        assert _rel(y1.cpu().numpy(), y2.cpu().numpy()) < 1e-14
+
+
+def test_unstructured_numbering_uses_indexed_sell():
+    from oracle import gspm as og
+    from cupymag_amd.core.gspm import GSPMSimulation
+    coords, tets = _kuhn(5)
+    perm = np.random.default_rng(1).permutation(len(coords))
+    inv = np.argsort(perm)
+    arr = (coords[perm], inv[tets].astype(np.int32))
+    cfg = config("Tet", n=5, me=True, rot=True, defect_params=1.4, dt=5e-12)
+    P, S = _product(cfg, arr)
+    assert S.grid_dims is None
+    assert not S.csr_elasticity()[0].sell_info()["stencil"]
+    p, ops, Avg, _ = oracle_setup(cfg, arr)
+    m0 = tilted_m(ops["n"], seed=2)
+    sim = GSPMSimulation(S, P, mode="fast64")
+    sim.set_m(m0)
+    st = og.init_state(ops, Avg, m0)
+    p["H_now"] = p["H"]
+    og.step(st, ops, Avg, p)
+    sim.step([P.Hext1, P.Hext2, P.Hext3], False)
+    assert np.max(np.abs(sim.get("M").cpu().numpy() - st["m"])) < 1e-10
