@@ -189,8 +189,155 @@ def cpu_baseline(n_cpu, steps=1):
                        f"({ops['n']} DOFs), {steps} step(s), {dt:.1f}s")
 
 
+def _build_instances(descs, ctx, workers=4):
+    """SimulationOperators of every instance description (mesh, host
+    assembly, upload), built on a few host threads."""
+    import concurrent.futures as cf
+
+    import torch
+
+    from cupymag_amd.core.assemble_simulation_operators import SimulationOperators
+    from cupymag_amd.core.parameters import from_dict
+    from cupymag_amd.mesh.kuhn import kuhn_mesh
+
+    def one(d):
+        torch.cuda.set_device(ctx.device)
+        P = from_dict(d["cfg"])
+        S = SimulationOperators(P, mesh_arrays=kuhn_mesh(d["n"]), ctx=ctx, keep_host=False)
+        S.csr_demag_and_deriv()
+        return P, S
+
+    with cf.ThreadPoolExecutor(max_workers=workers) as ex:
+        return list(ex.map(one, descs))
+
+
+def run_batch(args):
+    """Many independent simulations of different sizes: stepped one after
+    another with the single-simulation path, then all together as one
+    batch (each solve one batched CG), eager and with graph replay."""
+    import torch
+
+    from cupymag_amd.core.batch import GSPMBatch
+    from cupymag_amd.core.gspm import GSPMSimulation
+    from cupymag_amd.device import context
+    from cupymag_amd.serving.workloads import batch_workload, initial_m as init_m
+
+    torch.cuda.set_device(0)
+    ctx = context(0)
+    descs = batch_workload(args.count, seed=args.seed, nmin=args.nmin, nmax=args.nmax)
+    t0 = time.time()
+    built = _build_instances(descs, ctx)
+    ndof = [S.n_dof for _, S in built]
+    log(f"[bench] {len(built)} instances, n = {min(d['n'] for d in descs)}..{max(d['n'] for d in descs)}, "
+        f"{sum(ndof)} DOFs in total, set up in {time.time() - t0:.1f}s")
+    total = float(sum(ndof))
+    res = {}
+
+    def fresh():
+        sims = []
+        for (P, S), d in zip(built, descs):
+            sim = GSPMSimulation(S, P, mode="mixed", rel_delta=args.rel_delta)
+            sim.set_m(init_m(S.n_dof, d["m_seed"]))
+            sims.append(sim)
+        return sims
+
+    H = [[P.Hext1, P.Hext2, P.Hext3] for P, _ in built]
+    # one after another (single-simulation path)
+    sims = fresh()
+    for k in range(args.warmup):
+        for sim, h in zip(sims, H):
+            sim.step(h, False)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for k in range(args.steps):
+        for sim, h in zip(sims, H):
+            sim.step(h, False)
+        log(f"[bench] sequential step {k} done")
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t
+    m_seq = [s.get("M") for s in sims]
+    res["sequential"] = dict(node_steps_per_s=total * args.steps / el, ms_per_step=el * 1e3 / args.steps)
+    del sims
+    for graphs in (False, True):
+        name = "batched+graph" if graphs else "batched"
+        sims = fresh()
+        b = GSPMBatch(ctx, graphs=graphs)
+        for s in sims:
+            b.add(s)
+        for k in range(args.warmup):
+            b.step(H, [False] * len(sims))
+        torch.cuda.synchronize()
+        lat = []
+        t = time.perf_counter()
+        for k in range(args.steps):
+            t1 = time.perf_counter()
+            reps = b.step(H, [False] * len(sims))
+            lat.append(time.perf_counter() - t1)
+            log(f"[bench] {name} step {k}: {lat[-1]:.3f}s max iters {max(max(r.iters) for r in reps)}")
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t
+        err = max(float((s.get("M") - m).abs().max().item()) for s, m in zip(sims, m_seq))
+        res[name] = dict(node_steps_per_s=total * args.steps / el, ms_per_step=el * 1e3 / args.steps,
+                         latency_ms=dict(p50=float(np.percentile(np.array(lat) * 1e3, 50)),
+                                         p99=float(np.percentile(np.array(lat) * 1e3, 99))),
+                         max_abs_m_diff_vs_sequential=err)
+        del b, sims
+    head = max(("batched", "batched+graph"), key=lambda k: res[k]["node_steps_per_s"])
+    out = {
+        "metric": "LLG node-steps/s", "value": res[head]["node_steps_per_s"], "unit": "node-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": res[head]["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 CG in f64 refinement",
+        "data": "synthetic",
+        "config": {"workload": f"batch of {len(built)} independent GSPM simulations, Kuhn n^3 meshes with n "
+                               f"log-uniform in [{args.nmin},{args.nmax}], sphere defects, random Hext, "
+                               f"magnetoelastic on in every other one, mixed precision, dt 10 ps, alpha 0.1",
+                   "path": head, "instances": len(built), "n_dof_total": int(total),
+                   "n": [d["n"] for d in descs]},
+        "speedup_vs_sequential": res[head]["node_steps_per_s"] / res["sequential"]["node_steps_per_s"],
+        "paths": res,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_serve(args):
+    """Hysteresis-sweep requests arriving as a Poisson stream, served by
+    the continuous-batching server; p50/p99 per-point and per-loop latency
+    and aggregate node-steps/s."""
+    import torch
+
+    from cupymag_amd.device import context
+    from cupymag_amd.serving.server import SweepServer
+    from cupymag_amd.serving.workloads import sweep_requests
+
+    torch.cuda.set_device(0)
+    ctx = context(0)
+    reqs = sweep_requests(args.count, rate=args.rate, seed=args.seed, nmin=args.nmin, nmax=args.nmax,
+                          points=args.points, steps_per_point=args.steps_per_point)
+    srv = SweepServer(ctx, hbm_budget=args.hbm_budget_gb * 1e9, mode="mixed", graphs=args.graphs)
+    results, st = srv.run(reqs, log=log)
+    out = {
+        "metric": "LLG node-steps/s", "value": st["node_steps_per_s"], "unit": "node-steps/s", "n_gpus": 1,
+        "steps": st["batch_steps"], "warmup": 0, "ms_per_step": st["wall_s"] * 1e3 / max(st["batch_steps"], 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 CG in f64 refinement",
+        "data": "synthetic",
+        "config": {"workload": f"hysteresis-sweep serving: {args.count} requests, Poisson arrivals "
+                               f"{args.rate}/s, Kuhn n^3 with n log-uniform in [{args.nmin},{args.nmax}], "
+                               f"random rotated-ellipsoid defect, magnetoelastic on, {args.points} field points "
+                               f"in [-1e5, 1e5] A/m x {args.steps_per_point} steps, HBM budget "
+                               f"{args.hbm_budget_gb} GB, mixed precision",
+                   "requests": args.count, "n": [r["n"] for r in reqs]},
+        "latency_s": {"per_point": st["point_latency_s"], "per_loop": st["loop_latency_s"],
+                      "queue_delay": st["queue_delay_s"]},
+        "serving": st,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=("gspm", "batch", "serve"), default="gspm",
+                    help="gspm: one 128^3 simulation (the headline); batch: many independent simulations; "
+                         "serve: the hysteresis-sweep server")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4, help="timed steps of the headline (mixed) path")
     ap.add_argument("--warmup", type=int, default=1)
@@ -205,6 +352,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain device copy of this size (calibrates rocprof byte counters)")
+    ap.add_argument("--count", type=int, default=64, help="batch / serve: instances or requests")
+    ap.add_argument("--seed", type=int, default=4)
+    ap.add_argument("--nmin", type=int, default=16)
+    ap.add_argument("--nmax", type=int, default=96)
+    ap.add_argument("--rate", type=float, default=20.0, help="serve: request arrivals per second")
+    ap.add_argument("--points", type=int, default=41, help="serve: field points per sweep")
+    ap.add_argument("--steps-per-point", type=int, default=200)
+    ap.add_argument("--hbm-budget-gb", type=float, default=240.0)
+    ap.add_argument("--graphs", action="store_true", help="serve: graph replay of the CG iterations")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the GSPM step of one simulation is not sharded across GPUs "
@@ -218,6 +374,10 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if args.workload == "batch":
+        return run_batch(args)
+    if args.workload == "serve":
+        return run_serve(args)
     modes = [m for m in args.modes.split(",") if m]
     steps64 = min(args.steps_fp64, args.steps) if "fp64" in modes else args.steps_fp64
     g = run_gpu(args.n, args.steps, args.warmup, steps64, modes, args.rel_delta)

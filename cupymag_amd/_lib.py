@@ -20,6 +20,7 @@ HA = dict(PTR=0, IDX=1, ADEMAG=2, FX=3, FY=4, FZ=5, GS_PTR=6, GS_IDX=7, A1=8, A2
 FIELD = dict(M=0, HTILDE=1, E=2, U=3, UEL=4)
 MODE_FP64, MODE_FAST64, MODE_MIXED = 0, 1, 2
 MODES = {"fp64": MODE_FP64, "fast64": MODE_FAST64, "mixed": MODE_MIXED}
+BATCH_GRAPH = 1
 
 
 class CpmError(RuntimeError):
@@ -74,6 +75,12 @@ _SIGS = {
     "cpm_sim_set_m": (C.c_int, [_P, _P, _P]),
     "cpm_sim_get": (C.c_int, [_P, C.c_int, _P]),
     "cpm_sim_step": (C.c_int, [_P, _P, C.c_int, C.POINTER(cpm_step_report)]),
+    "cpm_batch_create": (C.c_int, [_P, C.c_int, C.POINTER(_P)]),
+    "cpm_batch_destroy": (C.c_int, [_P]),
+    "cpm_batch_add": (C.c_int, [_P, _P]),
+    "cpm_batch_remove": (C.c_int, [_P, _P]),
+    "cpm_batch_size": (C.c_int, [_P, C.POINTER(_I32)]),
+    "cpm_batch_step": (C.c_int, [_P, _P, _P, C.POINTER(cpm_step_report)]),
     "cpm_host_assemble": (C.c_int, [_P, _I64, _P, _I64, C.c_int, C.c_int, _P, _I32, _P, _P, C.c_int, C.c_int,
                                     C.POINTER(_P)]),
     "cpm_host_ops_array": (C.c_int, [_P, C.c_int, C.POINTER(_P), C.POINTER(_I64)]),

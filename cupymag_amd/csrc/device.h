@@ -173,34 +173,22 @@ __device__ inline double group_sum(double s) {
   return s;
 }
 
-// One workgroup: add the partials of the preceding kernel (deterministic
-// order) and advance the CG scalars.
+// CG scalars after a reduction (thread 0).  v[j]: the reduced value of
+// column j; only the columns in use (live, or with an update requested for
+// RED_RR_TRUE) are touched.
 enum : int { RED_ALPHA = 0, RED_RR = 1, RED_RR_REL = 2, RED_RR_TRUE = 3 };
 
+template <int MODE>
+__device__ inline bool cg_state_uses(const CgState& c) {
+  return (MODE == RED_RR_TRUE) ? c.upd != 0 : !c.done;
+}
+
 template <int K, int MODE>
-__global__ void __launch_bounds__(kBlock) k_red(int nblk, const double* __restrict__ partials, CgState* st,
-                                                double delta) {
-  bool use[K];
-  bool any = false;
+__device__ inline void cg_state_update(CgState* st, const double (&v)[K], double delta) {
 #pragma unroll
   for (int j = 0; j < K; ++j) {
-    use[j] = (MODE == RED_RR_TRUE) ? st[j].upd != 0 : !st[j].done;
-    any |= use[j];
-  }
-  if (!any) return;
-  __shared__ double lds[K * 4];
-  double v[K];
-#pragma unroll
-  for (int j = 0; j < K; ++j) v[j] = 0.0;
-  for (int i = threadIdx.x; i < nblk; i += kBlock)
-#pragma unroll
-    for (int j = 0; j < K; ++j) v[j] += partials[j * nblk + i];
-  block_sum<K>(v, lds);
-  if (threadIdx.x != 0) return;
-#pragma unroll
-  for (int j = 0; j < K; ++j) {
-    if (!use[j]) continue;
     CgState& c = st[j];
+    if (!cg_state_uses<MODE>(c)) continue;
     if (MODE == RED_ALPHA) {
       c.pq = v[j];
       c.alpha = c.rho / v[j];
@@ -235,6 +223,26 @@ __global__ void __launch_bounds__(kBlock) k_red(int nblk, const double* __restri
       }
     }
   }
+}
+
+// One workgroup: add the partials of the preceding kernel (deterministic
+// order) and advance the CG scalars.
+template <int K, int MODE>
+__global__ void __launch_bounds__(kBlock) k_red(int nblk, const double* __restrict__ partials, CgState* st,
+                                                double delta) {
+  bool any = false;
+#pragma unroll
+  for (int j = 0; j < K; ++j) any |= cg_state_uses<MODE>(st[j]);
+  if (!any) return;
+  __shared__ double lds[K * 4];
+  double v[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) v[j] = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += kBlock)
+#pragma unroll
+    for (int j = 0; j < K; ++j) v[j] += partials[j * nblk + i];
+  block_sum<K>(v, lds);
+  if (threadIdx.x == 0) cg_state_update<K, MODE>(st, v, delta);
 }
 
 }  // namespace cpm

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "device.h"
+#include "multi.h"
 
 namespace cpm {
 
@@ -479,6 +480,10 @@ struct cpm_sim {
   // fast modes: interleaved right-hand side and warm-start copies of the solutions
   double *bint = nullptr, *uint = nullptr, *gnint = nullptr, *mssint = nullptr;
   std::vector<void*> allocs;
+  // CG workspace of the batched solver (allocated when added to a batch)
+  double *wr = nullptr, *wp = nullptr, *wq = nullptr, *wy = nullptr;
+  float *wr32 = nullptr, *wp32 = nullptr, *wq32 = nullptr, *wx32 = nullptr;
+  cpm_batch* batch = nullptr;
   int hint[10] = {};
   bool stepped = false;  // the warm-start buffers hold a previous solution
 };
@@ -494,6 +499,7 @@ static int sim_alloc(cpm_sim* S, double** p, int64_t count) {
 
 extern "C" int cpm_sim_destroy(cpm_sim* S) {
   if (!S) return CPM_OK;
+  if (S->batch) (void)cpm_batch_remove(S->batch, S);
   if (S->ctx) {
     (void)hipSetDevice(S->ctx->device);
     (void)hipStreamSynchronize(S->ctx->stream);
@@ -628,13 +634,29 @@ extern "C" int cpm_sim_get(cpm_sim* S, int field, double* dst) {
   return CPM_OK;
 }
 
-extern "C" int cpm_sim_step(cpm_sim* S, const double* H, int use_init, cpm_step_report* rep) {
-  if (!S || !H || !rep) return fail(CPM_EINVAL, "cpm_sim_step: null argument");
-  cpm_ctx* ctx = S->ctx;
-  hipStream_t s = ctx->stream;
+// ---- the step as stages -------------------------------------------------------
+//
+// A step is six stages, each a few field kernels followed by one solve
+// (stage 0, the elasticity solve, only with magnetoelastic coupling), then
+// the projection.  cpm_sim_step runs them for one simulation; cpm_batch_step
+// runs every stage for all simulations of a batch and their solves as one
+// batched CG.
+
+namespace cpm {
+
+struct SolveSpec {
+  int slot = 0;                // first report slot of the solve
+  int K = 0;                   // right-hand sides sharing the operator
+  const cpm_mat* A = nullptr;  // nullptr: the stage has no solve
+  const double* b = nullptr;   // right-hand sides in the reference layout (K blocks of n, or 3n)
+  double* x = nullptr;         // solutions, same layout
+  // fast modes: interleaved copies (bs 3 or K = 3), nullptr when b / x are used directly
+  double* bint = nullptr;
+  double* xint = nullptr;
+};
+
+static FieldPrm field_prm(const cpm_sim* S, const double* H) {
   const cpm_sim_params& p = S->prm;
-  const int64_t n = S->n;
-  std::memset(rep, 0, sizeof(*rep));
   FieldPrm P{};
   P.K1 = p.K1;
   P.c1 = p.elasC[0];
@@ -649,116 +671,366 @@ extern "C" int cpm_sim_step(cpm_sim* S, const double* H, int use_init, cpm_step_
   }
   std::memcpy(P.R, p.R, sizeof(P.R));
   P.rot = p.rot111;
+  return P;
+}
+
+static inline bool fast_mode(const cpm_sim* S) { return S->prm.mode != CPM_MODE_FP64; }
+
+// Field kernels of stage `stage` (0..5) and the description of its solve.
+static int stage_pre(cpm_sim* S, int stage, const FieldPrm& P, SolveSpec& sp) {
+  cpm_ctx* ctx = S->ctx;
+  hipStream_t s = ctx->stream;
+  const cpm_sim_params& p = S->prm;
+  const int64_t n = S->n;
+  const int gv = grid_for(n);
   const double* dR = S->consts;
   const double* dMv = S->consts + 9;
   const double* dMvinvT = S->consts + 45;
   const double* dEs = S->consts + 81;
-  const int gv = grid_for(n);
-  auto solve = [&](int slot, const cpm_mat* A, const double* b, double* x) -> int {
-    int32_t it = 0;
-    double res = 0.0;
-    int rc = run_cg(ctx, A, b, x, use_init, p.tol, p.maxiter, S->hint[slot], &it, &res);
-    rep->iters[slot] = it;
-    S->hint[slot] = it;
-    if (rc > 0) {
-      rep->failed = slot + 1;
-      rep->resid = res;
-    }
-    return rc;
-  };
-  // fast modes: K right-hand sides (K = 3, or the 3 components of one bs-3
-  // solve) gathered into an interleaved vector, solved on the SELL form, and
-  // scattered back into the reference's SoA layout; `xint` keeps the
-  // interleaved solution as the next warm start.
-  const int use_x0_fast = (use_init || (p.warm_start && S->stepped)) ? 1 : 0;
-  auto solve_sell = [&](int slot, const cpm_mat* A, int K, const double* b, double* x, double* xint) -> int {
-    cpm_mat* Am = const_cast<cpm_mat*>(A);
-    const int blocks = (A->sell.bs == 3 || K == 3) ? 3 : 1;
-    const double* bb = b;
-    double* xx = x;
-    if (blocks == 3) {
-      if (int rc = interleave(ctx, n, 3, b, S->bint, 0)) return rc;
-      bb = S->bint;
-      xx = xint;
-    }
-    int32_t it[3] = {0, 0, 0};
-    double res[3] = {0, 0, 0};
-    int failed = 0;
-    int hint = 0;
-    for (int j = 0; j < K; ++j) hint = std::max(hint, S->hint[slot + j]);
-    int rc = (p.mode == CPM_MODE_MIXED)
-                 ? sell_cg_mixed(ctx, Am, K, bb, xx, use_x0_fast, p.tol, p.maxiter, p.rel_delta, hint, it, res,
-                                 &failed, nullptr)
-                 : sell_cg64(ctx, Am, K, bb, xx, use_x0_fast, p.tol, p.maxiter, hint, it, res, &failed);
-    if (rc) return rc;
-    for (int j = 0; j < K; ++j) {
-      rep->iters[slot + j] = it[j];
-      S->hint[slot + j] = it[j];
-    }
-    if (failed) {
-      int jf = 0;
-      for (int j = 1; j < K; ++j)
-        if (it[j] > it[jf]) jf = j;
-      rep->failed = slot + jf + 1;
-      rep->resid = res[jf];
-      return std::max(1, (int)it[jf]);
-    }
-    if (blocks == 3)
-      if (int rc2 = interleave(ctx, n, 3, xint, x, 1)) return rc2;
-    return CPM_OK;
-  };
-  const bool fast = p.mode != CPM_MODE_FP64;
-  int rc;
   const cpm_mat* F3 = S->ops.F3;
-  if (p.me) {
-    k_e0<<<gv, kBlock, 0, s>>>(n, S->m, p.lambda100, p.lambda111, p.rot111, dR, dMv, S->E0);
-    if ((rc = launch_spmv(ctx, S->ops.F_el, 0, S->E0, S->bel))) return rc;
-    if ((rc = fast ? solve_sell(0, S->ops.A_el, 1, S->bel, S->u, S->uint) : solve(0, S->ops.A_el, S->bel, S->u)))
-      return rc;
-    CPM_G_DISPATCH(F3->lanes, k_strain, n, s, (int32_t)n, F3->ptr, F3->idx, F3->val, F3->nnz, S->u, p.rot111, dR,
-                   S->E);
-    k_strain_avg<<<gv, kBlock, 0, s>>>(n, S->W, S->vol, S->E, S->m, p.rot111, dR, p.lambda100, p.lambda111, dEs, dMv,
-                                       S->dscal, ctx->partials, ctx->counter);
-    k_efinal<<<gv, kBlock, 0, s>>>(n, S->E, p.rot111, dMvinvT, S->dscal);
+  const bool fast = fast_mode(S);
+  sp = SolveSpec{};
+  int rc = CPM_OK;
+  switch (stage) {
+    case 0:  // Micromagnetics.py:112-125
+      if (!p.me) return CPM_OK;
+      k_e0<<<gv, kBlock, 0, s>>>(n, S->m, p.lambda100, p.lambda111, p.rot111, dR, dMv, S->E0);
+      if ((rc = launch_spmv(ctx, S->ops.F_el, 0, S->E0, S->bel))) return rc;
+      sp = SolveSpec{0, 1, S->ops.A_el, S->bel, S->u, fast ? S->bint : nullptr, fast ? S->uint : nullptr};
+      break;
+    case 1:  // :127-169
+      if (p.me) {
+        CPM_G_DISPATCH(F3->lanes, k_strain, n, s, (int32_t)n, F3->ptr, F3->idx, F3->val, F3->nnz, S->u, p.rot111,
+                       dR, S->E);
+        k_strain_avg<<<gv, kBlock, 0, s>>>(n, S->W, S->vol, S->E, S->m, p.rot111, dR, p.lambda100, p.lambda111,
+                                           dEs, dMv, S->dscal, ctx->partials, ctx->counter);
+        k_efinal<<<gv, kBlock, 0, s>>>(n, S->E, p.rot111, dMvinvT, S->dscal);
+      }
+      CPM_G_DISPATCH(F3->lanes, k_demag_rhs, n, s, (int32_t)n, F3->ptr, F3->idx, F3->val, F3->nnz, S->m,
+                     S->dscal + S_AVGM, S->bd);
+      sp = SolveSpec{1, 1, S->ops.A_demag, S->bd, S->U};
+      break;
+    case 2:  // :171-223
+      CPM_G_DISPATCH(F3->lanes, k_hd, n, s, (int32_t)n, F3->ptr, F3->idx, F3->val, F3->nnz, S->U, S->Ht);
+      k_field1<<<gv, kBlock, 0, s>>>(n, P, S->m, S->E, p.me, S->Ht, S->dscal + S_AVGM, S->f, S->ftmp);
+      for (int c = 0; c < 3; ++c)
+        if ((rc = launch_spmv(ctx, S->ops.F1, 0, S->ftmp + c * n, S->b + c * n))) return rc;
+      sp = SolveSpec{2, 3, S->ops.A1, S->b, S->gn, fast ? S->bint : nullptr, fast ? S->gnint : nullptr};
+      break;
+    case 3:  // :225-229
+      k_gs1<<<gv, kBlock, 0, s>>>(n, p.dt, S->m, S->gn, S->f, S->mstar, S->ftmp);
+      if ((rc = launch_spmv(ctx, S->ops.F1, 0, S->ftmp, S->b))) return rc;
+      sp = SolveSpec{5, 1, S->ops.A1, S->b, S->gs};
+      break;
+    case 4:  // :231-235
+      k_gs2<<<gv, kBlock, 0, s>>>(n, p.dt, S->m, S->gn, S->gs, S->f, S->mstar, S->ftmp);
+      if ((rc = launch_spmv(ctx, S->ops.F1, 0, S->ftmp, S->b))) return rc;
+      sp = SolveSpec{6, 1, S->ops.A1, S->b, S->gs + n};
+      break;
+    case 5:  // :237-290
+      k_gs3_field2<<<gv, kBlock, 0, s>>>(n, P, S->m, S->gs, S->mstar, S->E, p.me, S->Ht, S->dscal + S_AVGM,
+                                         S->ftmp);
+      for (int c = 0; c < 3; ++c)
+        if ((rc = launch_spmv(ctx, S->ops.F1, 0, S->ftmp + c * n, S->b + c * n))) return rc;
+      sp = SolveSpec{7, 3, S->ops.A2, S->b, S->mss, fast ? S->bint : nullptr, fast ? S->mssint : nullptr};
+      break;
+    default: return fail(CPM_EINVAL, "stage out of range");
   }
-  CPM_G_DISPATCH(F3->lanes, k_demag_rhs, n, s, (int32_t)n, F3->ptr, F3->idx, F3->val, F3->nnz, S->m,
-                 S->dscal + S_AVGM, S->bd);
-  if ((rc = fast ? solve_sell(1, S->ops.A_demag, 1, S->bd, S->U, nullptr) : solve(1, S->ops.A_demag, S->bd, S->U)))
-    return rc;
-  CPM_G_DISPATCH(F3->lanes, k_hd, n, s, (int32_t)n, F3->ptr, F3->idx, F3->val, F3->nnz, S->U, S->Ht);
-  k_field1<<<gv, kBlock, 0, s>>>(n, P, S->m, S->E, p.me, S->Ht, S->dscal + S_AVGM, S->f, S->ftmp);
-  for (int c = 0; c < 3; ++c)
-    if ((rc = launch_spmv(ctx, S->ops.F1, 0, S->ftmp + c * n, S->b + c * n))) return rc;
-  if (fast) {
-    if ((rc = solve_sell(2, S->ops.A1, 3, S->b, S->gn, S->gnint))) return rc;
-  } else {
-    for (int c = 0; c < 3; ++c)
-      if ((rc = solve(2 + c, S->ops.A1, S->b + c * n, S->gn + c * n))) return rc;
-  }
-  k_gs1<<<gv, kBlock, 0, s>>>(n, p.dt, S->m, S->gn, S->f, S->mstar, S->ftmp);
-  if ((rc = launch_spmv(ctx, S->ops.F1, 0, S->ftmp, S->b))) return rc;
-  if ((rc = fast ? solve_sell(5, S->ops.A1, 1, S->b, S->gs, nullptr) : solve(5, S->ops.A1, S->b, S->gs))) return rc;
-  k_gs2<<<gv, kBlock, 0, s>>>(n, p.dt, S->m, S->gn, S->gs, S->f, S->mstar, S->ftmp);
-  if ((rc = launch_spmv(ctx, S->ops.F1, 0, S->ftmp, S->b))) return rc;
-  if ((rc = fast ? solve_sell(6, S->ops.A1, 1, S->b, S->gs + n, nullptr) : solve(6, S->ops.A1, S->b, S->gs + n)))
-    return rc;
-  k_gs3_field2<<<gv, kBlock, 0, s>>>(n, P, S->m, S->gs, S->mstar, S->E, p.me, S->Ht, S->dscal + S_AVGM, S->ftmp);
-  for (int c = 0; c < 3; ++c)
-    if ((rc = launch_spmv(ctx, S->ops.F1, 0, S->ftmp + c * n, S->b + c * n))) return rc;
-  if (fast) {
-    if ((rc = solve_sell(7, S->ops.A2, 3, S->b, S->mss, S->mssint))) return rc;
-  } else {
-    for (int c = 0; c < 3; ++c)
-      if ((rc = solve(7 + c, S->ops.A2, S->b + c * n, S->mss + c * n))) return rc;
-  }
-  k_project<<<gv, kBlock, 0, s>>>(n, S->mss, S->defect, S->W, S->vol, S->m, S->mprev, S->dscal, ctx->partials,
-                                  ctx->counter);
   CPM_HIP(hipGetLastError());
-  CPM_HIP(hipMemcpyAsync(ctx->pinned, S->dscal, S_NSCAL * sizeof(double), hipMemcpyDeviceToHost, s));
-  CPM_HIP(hipStreamSynchronize(s));
+  // fast modes: K right-hand sides (K = 3, or the 3 components of one bs-3
+  // solve) are gathered into one interleaved vector for the SELL solvers
+  if (sp.bint) return interleave(ctx, n, 3, sp.b, sp.bint, 0);
+  return CPM_OK;
+}
+
+// After the solve of a stage: back to the reference layout.  The
+// interleaved solution stays in place as the next warm start.
+static int stage_post(cpm_sim* S, const SolveSpec& sp) {
+  if (sp.A && sp.xint) return interleave(S->ctx, S->n, 3, sp.xint, sp.x, 1);
+  return CPM_OK;
+}
+
+// :292-307 normalise / project, then the step scalars to `dst` (pinned).
+static int stage_final(cpm_sim* S, double* dst) {
+  cpm_ctx* ctx = S->ctx;
+  k_project<<<grid_for(S->n), kBlock, 0, ctx->stream>>>(S->n, S->mss, S->defect, S->W, S->vol, S->m, S->mprev,
+                                                        S->dscal, ctx->partials, ctx->counter);
+  CPM_HIP(hipGetLastError());
+  CPM_HIP(hipMemcpyAsync(dst, S->dscal, S_NSCAL * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  return CPM_OK;
+}
+
+static void report_scalars(cpm_sim* S, const double* scal, cpm_step_report* rep) {
   S->stepped = true;
-  rep->sumsq = ctx->pinned[S_SUMSQ];
-  for (int k = 0; k < 3; ++k) rep->avg_m[k] = ctx->pinned[S_AVGM + k];
+  rep->sumsq = scal[S_SUMSQ];
+  for (int k = 0; k < 3; ++k) rep->avg_m[k] = scal[S_AVGM + k];
+}
+
+static inline int use_x0_fast(const cpm_sim* S, int use_init) {
+  return (use_init || (S->prm.warm_start && S->stepped)) ? 1 : 0;
+}
+
+// One solve of a single simulation: CSR CG per column (plain FP64 path, the
+// reference's sequence) or the SELL solvers (fast modes).
+static int solve_one(cpm_sim* S, const SolveSpec& sp, int use_init, cpm_step_report* rep) {
+  cpm_ctx* ctx = S->ctx;
+  const cpm_sim_params& p = S->prm;
+  const int64_t n = S->n;
+  if (!fast_mode(S)) {
+    const int cols = sp.K;  // K = 3: three scalar solves on one operator
+    const int64_t len = sp.A->nrows;
+    for (int c = 0; c < cols; ++c) {
+      int32_t it = 0;
+      double res = 0.0;
+      const int slot = sp.slot + c;
+      int rc = run_cg(ctx, sp.A, sp.b + c * len, sp.x + c * len, use_init, p.tol, p.maxiter, S->hint[slot], &it,
+                      &res);
+      rep->iters[slot] = it;
+      S->hint[slot] = it;
+      if (rc > 0) {
+        rep->failed = slot + 1;
+        rep->resid = res;
+      }
+      if (rc) return rc;
+    }
+    return CPM_OK;
+  }
+  cpm_mat* Am = const_cast<cpm_mat*>(sp.A);
+  const double* bb = sp.bint ? sp.bint : sp.b;
+  double* xx = sp.xint ? sp.xint : sp.x;
+  int32_t it[3] = {0, 0, 0};
+  double res[3] = {0, 0, 0};
+  int failed = 0;
+  int hint = 0;
+  for (int j = 0; j < sp.K; ++j) hint = std::max(hint, S->hint[sp.slot + j]);
+  const int x0 = use_x0_fast(S, use_init);
+  int rc = (p.mode == CPM_MODE_MIXED)
+               ? sell_cg_mixed(ctx, Am, sp.K, bb, xx, x0, p.tol, p.maxiter, p.rel_delta, hint, it, res, &failed,
+                               nullptr)
+               : sell_cg64(ctx, Am, sp.K, bb, xx, x0, p.tol, p.maxiter, hint, it, res, &failed);
+  if (rc) return rc;
+  (void)n;
+  for (int j = 0; j < sp.K; ++j) {
+    rep->iters[sp.slot + j] = it[j];
+    S->hint[sp.slot + j] = it[j];
+  }
+  if (failed) {
+    int jf = 0;
+    for (int j = 1; j < sp.K; ++j)
+      if (it[j] > it[jf]) jf = j;
+    rep->failed = sp.slot + jf + 1;
+    rep->resid = res[jf];
+    return std::max(1, (int)it[jf]);
+  }
+  return CPM_OK;
+}
+
+}  // namespace cpm
+
+extern "C" int cpm_sim_step(cpm_sim* S, const double* H, int use_init, cpm_step_report* rep) {
+  if (!S || !H || !rep) return fail(CPM_EINVAL, "cpm_sim_step: null argument");
+  if (S->batch) return fail(CPM_EINVAL, "cpm_sim_step: the simulation belongs to a batch (cpm_batch_step)");
+  cpm_ctx* ctx = S->ctx;
+  std::memset(rep, 0, sizeof(*rep));
+  const FieldPrm P = field_prm(S, H);
+  for (int stage = 0; stage < 6; ++stage) {
+    SolveSpec sp;
+    if (int rc = stage_pre(S, stage, P, sp)) return rc;
+    if (!sp.A) continue;
+    if (int rc = solve_one(S, sp, use_init, rep)) return rc;
+    if (int rc = stage_post(S, sp)) return rc;
+  }
+  if (int rc = stage_final(S, ctx->pinned)) return rc;
+  CPM_HIP(hipStreamSynchronize(ctx->stream));
+  report_scalars(S, ctx->pinned, rep);
+  return CPM_OK;
+}
+
+// ---- batches of simulations --------------------------------------------------
+
+struct cpm_batch {
+  cpm_ctx* ctx = nullptr;
+  int flags = 0;
+  int mode = -1;
+  std::vector<cpm_sim*> sims;
+  cpm::MultiSolver ms[6];  // one per stage
+  double* scal = nullptr;  // pinned, S_NSCAL per simulation
+  int cap = 0;
+};
+
+static int batch_workspace(cpm_sim* S) {
+  // per-simulation CG workspace of the batched solver: the longest system
+  // of a step has 3n entries (elasticity, or the 3-RHS solves)
+  if (S->wr) return CPM_OK;
+  const int64_t len = 3 * S->n;
+  int rc = CPM_OK;
+  auto D = [&](double** p) {
+    if (rc == CPM_OK) rc = sim_alloc(S, p, len);
+  };
+  auto F = [&](float** p) {
+    if (rc != CPM_OK) return;
+    hipError_t e = hipMalloc(p, sizeof(float) * len);
+    if (e != hipSuccess) {
+      rc = fail(e == hipErrorOutOfMemory ? CPM_ENOMEM : CPM_EHIP, hipGetErrorString(e));
+      return;
+    }
+    S->allocs.push_back(*p);
+  };
+  D(&S->wr);
+  if (S->prm.mode == CPM_MODE_MIXED) {
+    D(&S->wy);
+    F(&S->wr32);
+    F(&S->wp32);
+    F(&S->wq32);
+    F(&S->wx32);
+  } else {
+    D(&S->wp);
+    D(&S->wq);
+  }
+  return rc;
+}
+
+extern "C" int cpm_batch_create(cpm_ctx* ctx, int flags, cpm_batch** out) {
+  if (!ctx || !out) return fail(CPM_EINVAL, "cpm_batch_create: null argument");
+  if (flags & ~CPM_BATCH_GRAPH) return fail(CPM_EINVAL, "cpm_batch_create: unknown flags");
+  auto* B = new cpm_batch();
+  B->ctx = ctx;
+  B->flags = flags;
+  *out = B;
+  return CPM_OK;
+}
+
+extern "C" int cpm_batch_destroy(cpm_batch* B) {
+  if (!B) return CPM_OK;
+  (void)hipSetDevice(B->ctx->device);
+  (void)hipStreamSynchronize(B->ctx->stream);
+  for (cpm_sim* S : B->sims) S->batch = nullptr;
+  if (B->scal) (void)hipHostFree(B->scal);
+  delete B;
+  return CPM_OK;
+}
+
+extern "C" int cpm_batch_add(cpm_batch* B, cpm_sim* S) {
+  if (!B || !S) return fail(CPM_EINVAL, "cpm_batch_add: null argument");
+  if (S->batch) return fail(CPM_EINVAL, "cpm_batch_add: the simulation already belongs to a batch");
+  if (S->ctx != B->ctx) return fail(CPM_EINVAL, "cpm_batch_add: the simulation uses another context");
+  if (S->prm.mode == CPM_MODE_FP64)
+    return fail(CPM_EINVAL, "cpm_batch_add: batches run the SELL paths (CPM_MODE_FAST64 or CPM_MODE_MIXED)");
+  if (!B->sims.empty() && S->prm.mode != B->mode)
+    return fail(CPM_EINVAL, "cpm_batch_add: every simulation of a batch uses the same mode");
+  CPM_HIP(hipSetDevice(B->ctx->device));
+  if (int rc = batch_workspace(S)) return rc;
+  if ((int)B->sims.size() + 1 > B->cap) {
+    const int cap = std::max(8, 2 * B->cap);
+    double* p = nullptr;
+    CPM_HIP(hipHostMalloc(&p, sizeof(double) * S_NSCAL * cap, hipHostMallocDefault));
+    if (B->scal) CPM_HIP(hipHostFree(B->scal));
+    B->scal = p;
+    B->cap = cap;
+  }
+  B->mode = S->prm.mode;
+  B->sims.push_back(S);
+  S->batch = B;
+  return CPM_OK;
+}
+
+extern "C" int cpm_batch_remove(cpm_batch* B, cpm_sim* S) {
+  if (!B || !S) return fail(CPM_EINVAL, "cpm_batch_remove: null argument");
+  auto it = std::find(B->sims.begin(), B->sims.end(), S);
+  if (it == B->sims.end()) return fail(CPM_EINVAL, "cpm_batch_remove: the simulation is not in this batch");
+  (void)hipSetDevice(B->ctx->device);
+  CPM_HIP(hipStreamSynchronize(B->ctx->stream));
+  B->sims.erase(it);
+  S->batch = nullptr;
+  return CPM_OK;
+}
+
+extern "C" int cpm_batch_size(const cpm_batch* B, int32_t* n) {
+  if (!B || !n) return fail(CPM_EINVAL, "cpm_batch_size: null argument");
+  *n = (int32_t)B->sims.size();
+  return CPM_OK;
+}
+
+extern "C" int cpm_batch_step(cpm_batch* B, const double* H, const int32_t* use_init, cpm_step_report* reps) {
+  if (!B || !H || !use_init || !reps) return fail(CPM_EINVAL, "cpm_batch_step: null argument");
+  const int nb = (int)B->sims.size();
+  if (nb == 0) return CPM_OK;
+  cpm_ctx* ctx = B->ctx;
+  CPM_HIP(hipSetDevice(ctx->device));
+  std::vector<FieldPrm> P(nb);
+  for (int j = 0; j < nb; ++j) {
+    std::memset(&reps[j], 0, sizeof(cpm_step_report));
+    P[j] = field_prm(B->sims[j], H + 3 * j);
+  }
+  const bool mixed = B->mode == CPM_MODE_MIXED;
+  std::vector<SolveSpec> sp(nb);
+  std::vector<MultiSys> sys;
+  std::vector<int> who;
+  std::vector<int32_t> iters;
+  std::vector<double> resid;
+  std::vector<int> failed;
+  for (int stage = 0; stage < 6; ++stage) {
+    sys.clear();
+    who.clear();
+    int hint = 0, bs = 1, K = 1;
+    for (int j = 0; j < nb; ++j) {
+      cpm_sim* S = B->sims[j];
+      if (int rc = stage_pre(S, stage, P[j], sp[j])) return rc;
+      if (!sp[j].A) continue;
+      const cpm_sim_params& p = S->prm;
+      MultiSys m;
+      m.A = sp[j].A;
+      bs = sp[j].A->sell.bs;
+      K = sp[j].K;
+      m.n = (int64_t)sp[j].A->sell.nbrows * bs * K;
+      m.b = sp[j].bint ? sp[j].bint : sp[j].b;
+      m.x = sp[j].xint ? sp[j].xint : sp[j].x;
+      m.use_x0 = use_x0_fast(S, use_init[j]);
+      m.tol = p.tol;
+      m.delta = p.rel_delta;
+      m.maxiter = p.maxiter;
+      m.r = S->wr;
+      m.p = S->wp;
+      m.q = S->wq;
+      m.y64 = S->wy;
+      m.r32 = S->wr32;
+      m.p32 = S->wp32;
+      m.q32 = S->wq32;
+      m.x32 = S->wx32;
+      sys.push_back(m);
+      who.push_back(j);
+      for (int k = 0; k < K; ++k) hint = std::max(hint, S->hint[sp[j].slot + k]);
+    }
+    if (sys.empty()) continue;
+    const int ns = (int)sys.size();
+    iters.assign(ns * K, 0);
+    resid.assign(ns * K, 0.0);
+    failed.assign(ns, 0);
+    if (int rc = multi_cg(ctx, B->ms[stage], bs, K, mixed ? 1 : 0, sys, hint, (B->flags & CPM_BATCH_GRAPH) != 0,
+                          iters.data(), resid.data(), failed.data()))
+      return rc;
+    for (int i = 0; i < ns; ++i) {
+      const int j = who[i];
+      cpm_sim* S = B->sims[j];
+      const SolveSpec& s = sp[j];
+      for (int k = 0; k < K; ++k) {
+        reps[j].iters[s.slot + k] = iters[i * K + k];
+        S->hint[s.slot + k] = iters[i * K + k];
+      }
+      if (failed[i] && !reps[j].failed) {
+        int kf = 0;
+        for (int k = 1; k < K; ++k)
+          if (iters[i * K + k] > iters[i * K + kf]) kf = k;
+        reps[j].failed = s.slot + kf + 1;
+        reps[j].resid = resid[i * K + kf];
+      }
+      if (int rc = stage_post(S, s)) return rc;
+    }
+  }
+  for (int j = 0; j < nb; ++j)
+    if (int rc = stage_final(B->sims[j], B->scal + (int64_t)j * S_NSCAL)) return rc;
+  CPM_HIP(hipStreamSynchronize(ctx->stream));
+  for (int j = 0; j < nb; ++j) report_scalars(B->sims[j], B->scal + (int64_t)j * S_NSCAL, &reps[j]);
   return CPM_OK;
 }
 

@@ -39,6 +39,7 @@
 
 #include "device.h"
 #include "sell.h"
+#include "sell_dev.h"
 
 namespace cpm {
 
@@ -193,13 +194,7 @@ void sell_free(Sell& S) {
 
 // ---------------------------------------------------------------- kernels
 
-// y = A x (+ fused per-column dot x.y when Dot) for BS x BS blocks and K
-// interleaved right-hand sides: x[(c*BS + e)*K + j].
-struct GridArg {
-  int32_t nx, ny, nz, nst;
-  const int32_t* soff;
-};
-
+// One system per launch: the bodies in sell_dev.h over the whole grid.
 template <int BS, int K, class T, bool CG, bool GATE = false, bool ADDX32 = false, bool ST = false>
 __global__ void __launch_bounds__(kBlock) k_sell_spmv(int32_t nbrows, const int64_t* __restrict__ sptr,
                                                       const int32_t* __restrict__ col, const T* __restrict__ val,
@@ -208,8 +203,6 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(int32_t nbrows, const int6
                                                       const CgState* st, double* partials, GridArg ga) {
   // y = A x (ADDX32: A (x + x32)); CG: per-column partials of x.y.  GATE:
   // run only when some column requested an FP64 residual update.
-  constexpr int BB = BS * BS;
-  constexpr int NV = BS * K;
   bool any = false;
 #pragma unroll
   for (int j = 0; j < K; ++j) any |= CG ? !st[j].done : (GATE ? st[j].upd != 0 : true);
@@ -217,198 +210,49 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(int32_t nbrows, const int6
   double acc[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) acc[j] = 0.0;
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t row = (int64_t)blockIdx.x * kBlock + threadIdx.x; row < (int64_t)((nbrows + 63) & ~63); row += stride) {
-    const int64_t s = row >> 6, l = row & 63;
-    const int64_t s0 = ST ? s * ga.nst : sptr[s];
-    const int64_t s1 = ST ? s0 + ga.nst : sptr[s + 1];
-    int32_t ri = 0, rj = 0, rk = 0;
-    if (ST && row < nbrows) {
-      const int32_t r = (int32_t)row, nyz = ga.ny * ga.nz;
-      ri = r / nyz;
-      rj = (r / ga.nz) % ga.ny;
-      rk = r % ga.nz;
-    }
-    double sum[NV];  // FP64 accumulation for both value types
-#pragma unroll
-    for (int v = 0; v < NV; ++v) sum[v] = 0.0;
-    // matrix planes are read once per launch: streaming (non-temporal) loads
-    // keep them from evicting the gathered vector entries; two slots per
-    // trip keep more loads in flight
-    auto slot_fma = [&](int64_t slot) {
-      int64_t c;
-      if (ST) {
-        // neighbour on the periodic grid (padding rows gather row 0)
-        const int32_t code = ga.soff[slot - s0];
-        int32_t i2 = ri + (code & 3) - 1, j2 = rj + ((code >> 2) & 3) - 1, k2 = rk + ((code >> 4) & 3) - 1;
-        i2 += (i2 < 0) ? ga.nx : 0;
-        i2 -= (i2 >= ga.nx) ? ga.nx : 0;
-        j2 += (j2 < 0) ? ga.ny : 0;
-        j2 -= (j2 >= ga.ny) ? ga.ny : 0;
-        k2 += (k2 < 0) ? ga.nz : 0;
-        k2 -= (k2 >= ga.nz) ? ga.nz : 0;
-        c = (row < nbrows) ? ((int64_t)i2 * ga.ny + j2) * ga.nz + k2 : 0;
-      } else {
-        c = __builtin_nontemporal_load(&col[slot * 64 + l]);
-      }
-      double xv[NV];
-#pragma unroll
-      for (int v = 0; v < NV; ++v) {
-        xv[v] = (double)x[c * NV + v];
-        if (ADDX32) xv[v] += (double)x32[c * NV + v];
-      }
-#pragma unroll
-      for (int d = 0; d < BS; ++d)
-#pragma unroll
-        for (int e = 0; e < BS; ++e) {
-          const double a = (double)__builtin_nontemporal_load(&val[(slot * BB + d * BS + e) * 64 + l]);
-#pragma unroll
-          for (int j = 0; j < K; ++j) sum[d * K + j] += a * xv[e * K + j];
-        }
-    };
-    int64_t slot = s0;
-    for (; slot + 1 < s1; slot += 2) {
-      slot_fma(slot);
-      slot_fma(slot + 1);
-    }
-    if (slot < s1) slot_fma(slot);
-    if (row < nbrows) {
-      double xo[NV];
-#pragma unroll
-      for (int v = 0; v < NV; ++v) {
-        xo[v] = (double)x[row * NV + v];
-        if (ADDX32) xo[v] += (double)x32[row * NV + v];
-      }
-#pragma unroll
-      for (int d = 0; d < BS; ++d)
-#pragma unroll
-        for (int e = 0; e < BS; ++e) {
-          const double a = dg[row * BB + d * BS + e];
-#pragma unroll
-          for (int j = 0; j < K; ++j) sum[d * K + j] += a * xo[e * K + j];
-        }
-#pragma unroll
-      for (int v = 0; v < NV; ++v) {
-        const T yv = (T)sum[v];
-        y[row * NV + v] = yv;
-        if (CG) acc[v % K] += xo[v] * (double)yv;
-      }
-    }
-  }
+  sell_rows<BS, K, T, CG, ADDX32, ST>(blockIdx.x, gridDim.x, nbrows, sptr, col, val, dg, x, x32, y, ga, acc);
   if (CG) block_partial<K>(acc, partials);
 }
 
 template <int K, class T>
 __global__ void __launch_bounds__(kBlock) k_mcg_pupd(int64_t n, const T* __restrict__ r, T* __restrict__ p,
                                                      const CgState* st) {
-  double beta[K];
-  bool live[K], first[K];
-  bool any = false;
-#pragma unroll
-  for (int j = 0; j < K; ++j) {
-    live[j] = !st[j].done;
-    first[j] = st[j].it == 0;
-    beta[j] = (live[j] && !first[j]) ? st[j].rho / st[j].rho_old : 0.0;
-    any |= live[j];
-  }
-  if (!any) return;
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-    const int j = (int)(i % K);
-    if (!live[j]) continue;
-    p[i] = first[j] ? r[i] : (T)(r[i] + (T)beta[j] * p[i]);
-  }
+  pupd_rows<K, T>(blockIdx.x, gridDim.x, n, r, p, st);
 }
 
 template <int K, class T>
 __global__ void __launch_bounds__(kBlock) k_mcg_xr(int64_t n, T* __restrict__ x, T* __restrict__ r,
                                                    const T* __restrict__ p, const T* __restrict__ q, const CgState* st,
                                                    double* partials) {
-  double a[K];
-  bool live[K];
-  bool any = false;
-#pragma unroll
-  for (int j = 0; j < K; ++j) {
-    live[j] = !st[j].done;
-    a[j] = live[j] ? st[j].alpha : 0.0;
-    any |= live[j];
-  }
-  if (!any) return;
-  double acc[K];
-#pragma unroll
-  for (int j = 0; j < K; ++j) acc[j] = 0.0;
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-    const int j = (int)(i % K);
-    if (!live[j]) continue;
-    const T aj = (T)a[j];
-    x[i] = x[i] + aj * p[i];
-    const T ri = r[i] - aj * q[i];
-    r[i] = ri;
-    acc[j] += (double)ri * (double)ri;
-  }
-  block_partial<K>(acc, partials);
+  xr_rows<K, T>(blockIdx.x, gridDim.x, n, x, r, p, q, st, partials);
 }
 
-// CG start: x = 0, r = b (or r = b - q with q = A x0); per-column ||b||^2, r.r.
-template <int K, class T>
-__global__ void __launch_bounds__(kBlock) k_mcg_init(int64_t n, const T* __restrict__ b, T* __restrict__ x,
-                                                     T* __restrict__ r, const T* __restrict__ q, int use_x0,
+// CG start (FP64 CG, or REL: the mixed solver's FP64 residual + FP32
+// copies); the column states are set by the last workgroup.
+template <int K, bool REL>
+__global__ void __launch_bounds__(kBlock) k_mcg_init(int64_t n, const double* __restrict__ b, double* __restrict__ x,
+                                                     const double* __restrict__ y, double* __restrict__ r,
+                                                     float* __restrict__ r32, float* __restrict__ x32, int use_x0,
                                                      double tol, int32_t maxiter, CgState* st, double* partials,
                                                      unsigned* counter) {
   double acc[2 * K];
 #pragma unroll
   for (int j = 0; j < 2 * K; ++j) acc[j] = 0.0;
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-    const int j = (int)(i % K);
-    const T bi = b[i];
-    T ri;
-    if (use_x0) {
-      ri = bi - q[i];
-    } else {
-      x[i] = 0;
-      ri = bi;
-    }
-    r[i] = ri;
-    acc[j] += (double)bi * (double)bi;
-    acc[K + j] += (double)ri * (double)ri;
-  }
-  if (grid_sum<2 * K>(acc, partials, counter) && threadIdx.x == 0) {
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
-      CgState& s = st[j];
-      s.bnorm2 = acc[j];
-      s.rho = acc[K + j];
-      s.rho_old = 0.0;
-      s.tol = tol;
-      s.atol = tol * sqrt(acc[j]);
-      s.it = 0;
-      s.fail = 0;
-      s.maxiter = maxiter;
-      s.resid = sqrt(acc[K + j]);
-      s.done = (acc[j] == 0.0 || maxiter <= 0) ? 1 : 0;
-    }
-  }
+  init_rows<K, REL>(blockIdx.x, gridDim.x, n, b, x, y, r, r32, x32, use_x0, acc);
+  if (grid_sum<2 * K>(acc, partials, counter) && threadIdx.x == 0) cg_state_init<K, REL>(st, acc, tol, maxiter);
 }
 
-template <int K, class T>
-__global__ void __launch_bounds__(kBlock) k_mcg_zero_b(int64_t n, T* __restrict__ x, const CgState* st) {
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
-    if (st[i % K].bnorm2 == 0.0) x[i] = 0;
+template <int K>
+__global__ void __launch_bounds__(kBlock) k_mcg_zero_b(int64_t n, double* __restrict__ x, const CgState* st) {
+  zero_b_rows<K>(blockIdx.x, gridDim.x, n, x, st);
 }
 
-// ---- refinement helpers (FP64 outer loop) -----------------------------------
-
-
-
-
-// x[i] = 0 for every i (K == 0) or for the entries of column `col` of K
-__global__ void __launch_bounds__(kBlock) k_fill0(int64_t n, double* __restrict__ x, int K = 0, int col = 0) {
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
-    if (K == 0 || i % K == col) x[i] = 0.0;
+template <int K>
+__global__ void __launch_bounds__(kBlock) k_rel_r(int64_t n, const double* __restrict__ b, const double* __restrict__ y,
+                                                  double* __restrict__ x, float* __restrict__ x32,
+                                                  double* __restrict__ r64, float* __restrict__ r32,
+                                                  const CgState* st, double* partials) {
+  rel_r_rows<K>(blockIdx.x, gridDim.x, n, b, y, x, x32, r64, r32, st, partials);
 }
 
 // ---------------------------------------------------------------- dispatch
@@ -519,9 +363,9 @@ static int cg64_loop(cpm_ctx* ctx, cpm_mat* M, const double* b, double* x, int u
   CgState* st = ctx->cg;
   if (use_x0)
     if (int rc = spmv_t<BS, K, double, false>(ctx, A, x, nullptr, ctx->q)) return rc;
-  k_mcg_init<K, double><<<g, kBlock, 0, s>>>(n, b, x, ctx->r, ctx->q, use_x0, tol, maxiter, st, ctx->partials,
-                                             ctx->counter);
-  k_mcg_zero_b<K, double><<<g, kBlock, 0, s>>>(n, x, st);
+  k_mcg_init<K, false><<<g, kBlock, 0, s>>>(n, b, x, ctx->q, ctx->r, nullptr, nullptr, use_x0, tol, maxiter, st,
+                                            ctx->partials, ctx->counter);
+  k_mcg_zero_b<K><<<g, kBlock, 0, s>>>(n, x, st);
   CPM_HIP(hipGetLastError());
   auto issue = [&](int todo) -> int {
     for (int k = 0; k < todo; ++k) {
@@ -578,114 +422,6 @@ int ensure_workspace32(cpm_ctx* ctx, int64_t n) {
 // iteration count stays close to that of FP64 CG; convergence is declared
 // on the FP64 residual only.
 
-template <int K>
-__global__ void __launch_bounds__(kBlock) k_rel_init(int64_t n, const double* __restrict__ b, double* __restrict__ x,
-                                                     const double* __restrict__ y, double* __restrict__ r64,
-                                                     float* __restrict__ r32, float* __restrict__ x32, int use_x0,
-                                                     double tol, int32_t maxiter, CgState* st, double* partials,
-                                                     unsigned* counter) {
-  double acc[2 * K];
-#pragma unroll
-  for (int j = 0; j < 2 * K; ++j) acc[j] = 0.0;
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-    const int j = (int)(i % K);
-    const double bi = b[i];
-    double ri;
-    if (use_x0) {
-      ri = bi - y[i];
-    } else {
-      x[i] = 0.0;
-      ri = bi;
-    }
-    r64[i] = ri;
-    r32[i] = (float)ri;
-    x32[i] = 0.0f;
-    acc[j] += bi * bi;
-    acc[K + j] += ri * ri;
-  }
-  if (grid_sum<2 * K>(acc, partials, counter) && threadIdx.x == 0) {
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
-      CgState& s = st[j];
-      s.bnorm2 = acc[j];
-      s.rho = acc[K + j];
-      s.rho_old = 0.0;
-      s.tol = tol;
-      s.atol = tol * sqrt(acc[j]);
-      s.it = 0;
-      s.fail = 0;
-      s.maxiter = maxiter;
-      s.resid = sqrt(acc[K + j]);
-      s.maxr = s.resid;
-      s.upd = 0;
-      s.done = (acc[j] == 0.0 || s.resid <= s.atol || maxiter <= 0) ? 1 : 0;
-    }
-  }
-}
-
-template <int K>
-__global__ void __launch_bounds__(kBlock) k_rel_xr(int64_t n, float* __restrict__ x, float* __restrict__ r,
-                                                   const float* __restrict__ p, const float* __restrict__ q,
-                                                   const CgState* st, double* partials) {
-  double a[K];
-  bool live[K];
-  bool any = false;
-#pragma unroll
-  for (int j = 0; j < K; ++j) {
-    live[j] = !st[j].done;
-    a[j] = live[j] ? st[j].alpha : 0.0;
-    any |= live[j];
-  }
-  if (!any) return;
-  double acc[K];
-#pragma unroll
-  for (int j = 0; j < K; ++j) acc[j] = 0.0;
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-    const int j = (int)(i % K);
-    if (!live[j]) continue;
-    const float aj = (float)a[j];
-    x[i] = x[i] + aj * p[i];
-    const float ri = r[i] - aj * q[i];
-    r[i] = ri;
-    acc[j] += (double)ri * (double)ri;
-  }
-  block_partial<K>(acc, partials);
-}
-
-// On the columns that requested an update (y = A (x64 + x32) computed just
-// before): x64 += x32, x32 = 0, r64 = b - y, r32 = r64, partials of r64.r64.
-template <int K>
-__global__ void __launch_bounds__(kBlock) k_rel_r(int64_t n, const double* __restrict__ b, const double* __restrict__ y,
-                                                  double* __restrict__ x, float* __restrict__ x32,
-                                                  double* __restrict__ r64, float* __restrict__ r32,
-                                                  const CgState* st, double* partials) {
-  bool upd[K];
-  bool any = false;
-#pragma unroll
-  for (int j = 0; j < K; ++j) {
-    upd[j] = st[j].upd != 0;
-    any |= upd[j];
-  }
-  if (!any) return;
-  double acc[K];
-#pragma unroll
-  for (int j = 0; j < K; ++j) acc[j] = 0.0;
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-    const int j = (int)(i % K);
-    if (!upd[j]) continue;
-    x[i] += (double)x32[i];
-    x32[i] = 0.0f;
-    const double ri = b[i] - y[i];
-    r64[i] = ri;
-    r32[i] = (float)ri;
-    acc[j] += ri * ri;
-  }
-  block_partial<K>(acc, partials);
-}
-
 template <int BS, int K>
 static int rel_loop(cpm_ctx* ctx, cpm_mat* M, const double* b, double* x, int use_x0, double tol, int maxiter,
                     double delta, int hint, int32_t* iters, double* resid, int* failed) {
@@ -697,9 +433,9 @@ static int rel_loop(cpm_ctx* ctx, cpm_mat* M, const double* b, double* x, int us
   double* y64 = ctx->y64;
   if (use_x0)
     if (int rc = spmv_t<BS, K, double, false>(ctx, A, x, nullptr, y64)) return rc;
-  k_rel_init<K><<<g, kBlock, 0, s>>>(n, b, x, y64, ctx->r, ctx->r32, ctx->x32, use_x0, tol, maxiter, st, ctx->partials,
-                                     ctx->counter);
-  k_mcg_zero_b<K, double><<<g, kBlock, 0, s>>>(n, x, st);
+  k_mcg_init<K, true><<<g, kBlock, 0, s>>>(n, b, x, y64, ctx->r, ctx->r32, ctx->x32, use_x0, tol, maxiter, st,
+                                           ctx->partials, ctx->counter);
+  k_mcg_zero_b<K><<<g, kBlock, 0, s>>>(n, x, st);
   CPM_HIP(hipGetLastError());
   auto issue = [&](int todo) -> int {
     for (int k = 0; k < todo; ++k) {
@@ -708,7 +444,7 @@ static int rel_loop(cpm_ctx* ctx, cpm_mat* M, const double* b, double* x, int us
       if (int rc = spmv_t<BS, K, float, true>(ctx, A, ctx->p32, nullptr, ctx->q32)) return rc;
       if (ctx->prof) CPM_HIP(hipEventRecord(ctx->ev[2 * k + 1], s));
       reduce_t<K, RED_ALPHA>(ctx, gs);
-      k_rel_xr<K><<<g, kBlock, 0, s>>>(n, ctx->x32, ctx->r32, ctx->p32, ctx->q32, st, ctx->partials);
+      k_mcg_xr<K, float><<<g, kBlock, 0, s>>>(n, ctx->x32, ctx->r32, ctx->p32, ctx->q32, st, ctx->partials);
       reduce_t<K, RED_RR_REL>(ctx, g, delta);
       // reliable update (runs only when a column asked for it): y = A (x64 + x32)
       if (int rc = spmv_t<BS, K, double, false, true, true>(ctx, A, x, ctx->x32, y64)) return rc;

@@ -1,0 +1,262 @@
+// Device bodies of the SELL-64 SpMV and of the CG vector updates, written
+// over a workgroup range (lb = this workgroup's index inside the range,
+// lg = workgroups in the range) so that one body serves both a launch that
+// covers a single system (lb = blockIdx.x, lg = gridDim.x) and a segmented
+// launch in which every system of a batch owns its own range of workgroups.
+// Partial sums are stored at partials[k * gridDim.x + blockIdx.x] in both
+// cases; the reducer of a system adds the entries of its own range.
+#pragma once
+#include "device.h"
+
+namespace cpm {
+
+// Periodic structured grid of an index-free (stencil) SELL operator.
+struct GridArg {
+  int32_t nx, ny, nz, nst;
+  const int32_t* soff;
+};
+
+// y = A x (ADDX32: A (x + x32)) over block rows of BS x BS blocks and K
+// interleaved right-hand sides, x[(c*BS + e)*K + j]; CG: acc[j] += x.y of
+// column j over the rows this thread owns.
+template <int BS, int K, class T, bool CG, bool ADDX32, bool ST>
+__device__ __forceinline__ void sell_rows(int64_t lb, int64_t lg, int32_t nbrows, const int64_t* __restrict__ sptr,
+                                          const int32_t* __restrict__ col, const T* __restrict__ val,
+                                          const double* __restrict__ dg, const T* __restrict__ x,
+                                          const float* __restrict__ x32, T* __restrict__ y, const GridArg& ga,
+                                          double (&acc)[K]) {
+  constexpr int BB = BS * BS;
+  constexpr int NV = BS * K;
+  const int64_t stride = lg * kBlock;
+  for (int64_t row = lb * kBlock + threadIdx.x; row < (int64_t)((nbrows + 63) & ~63); row += stride) {
+    const int64_t s = row >> 6, l = row & 63;
+    const int64_t s0 = ST ? s * ga.nst : sptr[s];
+    const int64_t s1 = ST ? s0 + ga.nst : sptr[s + 1];
+    int32_t ri = 0, rj = 0, rk = 0;
+    if (ST && row < nbrows) {
+      const int32_t r = (int32_t)row, nyz = ga.ny * ga.nz;
+      ri = r / nyz;
+      rj = (r / ga.nz) % ga.ny;
+      rk = r % ga.nz;
+    }
+    double sum[NV];  // FP64 accumulation for both value types
+#pragma unroll
+    for (int v = 0; v < NV; ++v) sum[v] = 0.0;
+    // matrix planes are read once per launch: streaming (non-temporal) loads
+    // keep them from evicting the gathered vector entries; two slots per
+    // trip keep more loads in flight
+    auto slot_fma = [&](int64_t slot) {
+      int64_t c;
+      if (ST) {
+        // neighbour on the periodic grid (padding rows gather row 0)
+        const int32_t code = ga.soff[slot - s0];
+        int32_t i2 = ri + (code & 3) - 1, j2 = rj + ((code >> 2) & 3) - 1, k2 = rk + ((code >> 4) & 3) - 1;
+        i2 += (i2 < 0) ? ga.nx : 0;
+        i2 -= (i2 >= ga.nx) ? ga.nx : 0;
+        j2 += (j2 < 0) ? ga.ny : 0;
+        j2 -= (j2 >= ga.ny) ? ga.ny : 0;
+        k2 += (k2 < 0) ? ga.nz : 0;
+        k2 -= (k2 >= ga.nz) ? ga.nz : 0;
+        c = (row < nbrows) ? ((int64_t)i2 * ga.ny + j2) * ga.nz + k2 : 0;
+      } else {
+        c = __builtin_nontemporal_load(&col[slot * 64 + l]);
+      }
+      double xv[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        xv[v] = (double)x[c * NV + v];
+        if (ADDX32) xv[v] += (double)x32[c * NV + v];
+      }
+#pragma unroll
+      for (int d = 0; d < BS; ++d)
+#pragma unroll
+        for (int e = 0; e < BS; ++e) {
+          const double a = (double)__builtin_nontemporal_load(&val[(slot * BB + d * BS + e) * 64 + l]);
+#pragma unroll
+          for (int j = 0; j < K; ++j) sum[d * K + j] += a * xv[e * K + j];
+        }
+    };
+    int64_t slot = s0;
+    for (; slot + 1 < s1; slot += 2) {
+      slot_fma(slot);
+      slot_fma(slot + 1);
+    }
+    if (slot < s1) slot_fma(slot);
+    if (row < nbrows) {
+      double xo[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        xo[v] = (double)x[row * NV + v];
+        if (ADDX32) xo[v] += (double)x32[row * NV + v];
+      }
+#pragma unroll
+      for (int d = 0; d < BS; ++d)
+#pragma unroll
+        for (int e = 0; e < BS; ++e) {
+          const double a = dg[row * BB + d * BS + e];
+#pragma unroll
+          for (int j = 0; j < K; ++j) sum[d * K + j] += a * xo[e * K + j];
+        }
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        const T yv = (T)sum[v];
+        y[row * NV + v] = yv;
+        if (CG) acc[v % K] += xo[v] * (double)yv;
+      }
+    }
+  }
+}
+
+// p = r (first iteration) or r + beta p, on the live columns.
+template <int K, class T>
+__device__ __forceinline__ void pupd_rows(int64_t lb, int64_t lg, int64_t n, const T* __restrict__ r,
+                                          T* __restrict__ p, const CgState* st) {
+  double beta[K];
+  bool live[K], first[K];
+  bool any = false;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    live[j] = !st[j].done;
+    first[j] = st[j].it == 0;
+    beta[j] = (live[j] && !first[j]) ? st[j].rho / st[j].rho_old : 0.0;
+    any |= live[j];
+  }
+  if (!any) return;
+  const int64_t stride = lg * kBlock;
+  for (int64_t i = lb * kBlock + threadIdx.x; i < n; i += stride) {
+    const int j = (int)(i % K);
+    if (!live[j]) continue;
+    p[i] = first[j] ? r[i] : (T)(r[i] + (T)beta[j] * p[i]);
+  }
+}
+
+// x += alpha p, r -= alpha q on the live columns; partials of r.r.  The
+// whole workgroup returns when no column is live (no partials then: the
+// reducer skips such systems too).
+template <int K, class T>
+__device__ __forceinline__ void xr_rows(int64_t lb, int64_t lg, int64_t n, T* __restrict__ x, T* __restrict__ r,
+                                        const T* __restrict__ p, const T* __restrict__ q, const CgState* st,
+                                        double* partials) {
+  double a[K];
+  bool live[K];
+  bool any = false;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    live[j] = !st[j].done;
+    a[j] = live[j] ? st[j].alpha : 0.0;
+    any |= live[j];
+  }
+  if (!any) return;
+  double acc[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) acc[j] = 0.0;
+  const int64_t stride = lg * kBlock;
+  for (int64_t i = lb * kBlock + threadIdx.x; i < n; i += stride) {
+    const int j = (int)(i % K);
+    if (!live[j]) continue;
+    const T aj = (T)a[j];
+    x[i] = x[i] + aj * p[i];
+    const T ri = r[i] - aj * q[i];
+    r[i] = ri;
+    acc[j] += (double)ri * (double)ri;
+  }
+  block_partial<K>(acc, partials);
+}
+
+// Mixed precision, on the columns that requested an update (y = A (x64 +
+// x32) computed just before): x64 += x32, x32 = 0, r64 = b - y, r32 = r64,
+// partials of r64.r64.
+template <int K>
+__device__ __forceinline__ void rel_r_rows(int64_t lb, int64_t lg, int64_t n, const double* __restrict__ b,
+                                           const double* __restrict__ y, double* __restrict__ x,
+                                           float* __restrict__ x32, double* __restrict__ r64,
+                                           float* __restrict__ r32, const CgState* st, double* partials) {
+  bool upd[K];
+  bool any = false;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    upd[j] = st[j].upd != 0;
+    any |= upd[j];
+  }
+  if (!any) return;
+  double acc[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) acc[j] = 0.0;
+  const int64_t stride = lg * kBlock;
+  for (int64_t i = lb * kBlock + threadIdx.x; i < n; i += stride) {
+    const int j = (int)(i % K);
+    if (!upd[j]) continue;
+    x[i] += (double)x32[i];
+    x32[i] = 0.0f;
+    const double ri = b[i] - y[i];
+    r64[i] = ri;
+    r32[i] = (float)ri;
+    acc[j] += ri * ri;
+  }
+  block_partial<K>(acc, partials);
+}
+
+// CG start: x = 0, r = b (or r = b - y with y = A x0); acc[j] = ||b_j||^2,
+// acc[K + j] = r_j.r_j.  REL (mixed precision): also r32 = r, x32 = 0.
+template <int K, bool REL>
+__device__ __forceinline__ void init_rows(int64_t lb, int64_t lg, int64_t n, const double* __restrict__ b,
+                                          double* __restrict__ x, const double* __restrict__ y,
+                                          double* __restrict__ r, float* __restrict__ r32,
+                                          float* __restrict__ x32, int use_x0, double (&acc)[2 * K]) {
+  const int64_t stride = lg * kBlock;
+  for (int64_t i = lb * kBlock + threadIdx.x; i < n; i += stride) {
+    const int j = (int)(i % K);
+    const double bi = b[i];
+    double ri;
+    if (use_x0) {
+      ri = bi - y[i];
+    } else {
+      x[i] = 0.0;
+      ri = bi;
+    }
+    r[i] = ri;
+    if (REL) {
+      r32[i] = (float)ri;
+      x32[i] = 0.0f;
+    }
+    acc[j] += bi * bi;
+    acc[K + j] += ri * ri;
+  }
+}
+
+// State of K columns from the start sums (thread 0).  FP64 CG keeps CuPy's
+// rule (return b when ||b|| = 0, otherwise at least one iteration); the
+// mixed solver also stops at once when r0 already meets the tolerance.
+template <int K, bool REL>
+__device__ inline void cg_state_init(CgState* st, const double (&acc)[2 * K], double tol, int32_t maxiter) {
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    CgState& s = st[j];
+    s.bnorm2 = acc[j];
+    s.rho = acc[K + j];
+    s.rho_old = 0.0;
+    s.tol = tol;
+    s.atol = tol * sqrt(acc[j]);
+    s.it = 0;
+    s.fail = 0;
+    s.maxiter = maxiter;
+    s.resid = sqrt(acc[K + j]);
+    s.maxr = s.resid;
+    s.upd = 0;
+    if (REL)
+      s.done = (acc[j] == 0.0 || s.resid <= s.atol || maxiter <= 0) ? 1 : 0;
+    else
+      s.done = (acc[j] == 0.0 || maxiter <= 0) ? 1 : 0;
+  }
+}
+
+// x = 0 on the columns whose right-hand side is zero (CuPy returns b there).
+template <int K>
+__device__ __forceinline__ void zero_b_rows(int64_t lb, int64_t lg, int64_t n, double* __restrict__ x,
+                                            const CgState* st) {
+  const int64_t stride = lg * kBlock;
+  for (int64_t i = lb * kBlock + threadIdx.x; i < n; i += stride)
+    if (st[i % K].bnorm2 == 0.0) x[i] = 0.0;
+}
+
+}  // namespace cpm

@@ -1,0 +1,205 @@
+"""Continuous-batching server for hysteresis-sweep requests.
+
+A request is one hysteresis sweep on its own mesh: a list of external-field
+points, at each of which the LLG step runs until the reference's
+convergence rule fires (sumsq < LLG_accuracy after llg_min_step steps,
+Micromagnetics.py:309-335) or a per-point step budget is spent.  The
+server
+
+* admits arrived requests in arrival order while their estimated device
+  memory fits the HBM budget (and a member limit),
+* prepares an admitted request (mesh, host assembly, upload, initial m) on
+  a worker thread while the GPU keeps stepping the others,
+* adds it to the running batch at the next step boundary, so requests that
+  arrive while others are on the GPU join as memory frees up,
+* steps every member of the batch with one ``GSPMBatch.step`` call per
+  step (each solve a batched CG over all members), keeping m, u and the
+  CG state of every member resident on the device,
+* retires a request when its last point is done and frees its memory.
+
+A member's arithmetic does not depend on the batch (``GSPMBatch``), so a
+request's results are the same alone and in any mix.  ``run`` returns the
+per-request results and the latency / throughput statistics: per-point
+latency (wall time of a field point), per-loop latency (arrival to the end
+of the sweep), queueing delay and aggregate node-steps/s.
+"""
+import concurrent.futures as cf
+import time
+from collections import deque
+
+import numpy as np
+
+from .workloads import initial_m
+
+# device bytes per DOF of a member (operators in CSR and SELL-64 forms, the
+# step's state vectors, the batched-CG workspace of the mixed mode)
+_BYTES_PER_DOF = {False: 2200.0, True: 8900.0}
+
+
+def estimate_bytes(n_dof, me):
+    """Device memory of one member with ``n_dof`` DOFs (upper estimate)."""
+    return int(1.15 * _BYTES_PER_DOF[bool(me)] * n_dof) + (64 << 20)
+
+
+class SweepJob:
+    """An admitted request: its simulation and the hysteresis control."""
+
+    def __init__(self, req, sim, P, n_dof, est, t_arrival, t_admit):
+        self.req = req
+        self.rid = req["rid"]
+        self.sim = sim
+        self.P = P
+        self.n_dof = n_dof
+        self.est = est
+        self.fields = np.asarray(req["fields"], dtype=np.float64)
+        self.fields_red = self.fields / P.ms
+        self.steps_per_point = int(req["steps_per_point"])
+        self.llg_accuracy = n_dof * P.LLG_accuracy_factor
+        self.llg_min_step = 10 if P.ME else 100
+        self.point = 0
+        self.nstep = 0
+        self.steps = 0
+        self.use_init = False
+        self.points = []        # (H [A/m], avg_m, steps) per finished point
+        self.point_lat = []
+        self.t_arrival = t_arrival
+        self.t_admit = t_admit
+        self.t_point = None
+        self.t_done = None
+
+    def field(self):
+        return self.fields_red[self.point]
+
+    def advance(self, rep, now):
+        """Account one step; returns True when the sweep is finished."""
+        self.nstep += 1
+        self.steps += 1
+        sumsq = rep.sumsq
+        self.use_init = bool(sumsq < self.P.use_init_factor * self.llg_accuracy)
+        converged = sumsq < self.llg_accuracy and self.nstep > self.llg_min_step
+        if converged or self.nstep >= self.steps_per_point:
+            self.points.append((self.fields[self.point].copy(), np.array(rep.avg_m), self.nstep))
+            self.point_lat.append(now - self.t_point)
+            self.t_point = now
+            self.point += 1
+            self.nstep = 0
+        return self.point >= len(self.fields)
+
+
+class SweepServer:
+    def __init__(self, ctx=None, hbm_budget=240e9, mode="mixed", graphs=False, max_members=256, prep_workers=2,
+                 rel_delta=0.3, keep_m=True):
+        from ..device import context
+        self.ctx = ctx or context()
+        self.hbm_budget = float(hbm_budget)
+        self.mode = mode
+        self.graphs = graphs
+        self.max_members = int(max_members)
+        self.prep_workers = int(prep_workers)
+        self.rel_delta = rel_delta
+        self.keep_m = keep_m
+
+    def _prepare(self, req):
+        import torch
+
+        from ..core.assemble_simulation_operators import SimulationOperators
+        from ..core.gspm import GSPMSimulation
+        from ..core.parameters import from_dict
+        from ..mesh.kuhn import kuhn_mesh
+        torch.cuda.set_device(self.ctx.device)
+        P = from_dict(req["cfg"])
+        arr = kuhn_mesh(req["n"])
+        S = SimulationOperators(P, mesh_arrays=arr, ctx=self.ctx, keep_host=False)
+        sim = GSPMSimulation(S, P, mode=self.mode, rel_delta=self.rel_delta)
+        f0 = np.asarray(req["fields"][0], dtype=np.float64)
+        direction = f0 if np.linalg.norm(f0) > 0 else (1.0, 0.0, 0.0)
+        sim.set_m(initial_m(S.n_dof, req["m_seed"], direction=direction))
+        self.ctx.sync()
+        return sim, P, S.n_dof
+
+    def run(self, requests, log=None):
+        """Serve ``requests`` (dicts of workloads.sweep_requests, arrival
+        times in seconds from the start); returns (results, stats)."""
+        from ..core.batch import GSPMBatch
+        say = log or (lambda *a: None)
+        pending = deque(sorted(requests, key=lambda r: r["arrival"]))
+        waiting = deque()
+        preparing = {}
+        active = []
+        used = 0
+        results = {}
+        batch = GSPMBatch(self.ctx, graphs=self.graphs)
+        node_steps = 0
+        max_members = 0
+        steps_run = 0
+        pool = cf.ThreadPoolExecutor(max_workers=self.prep_workers)
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0  # noqa: E731
+        try:
+            while pending or waiting or preparing or active:
+                t = now()
+                while pending and pending[0]["arrival"] <= t:
+                    waiting.append(pending.popleft())
+                # admission: arrival order, HBM budget, member limit
+                while waiting:
+                    req = waiting[0]
+                    me = bool(req["cfg"]["physics"]["magnetoelastic_coupling"])
+                    est = estimate_bytes(req["n"] ** 3, me)
+                    if est > self.hbm_budget:
+                        raise MemoryError(f"request {req['rid']} needs ~{est / 1e9:.1f} GB, over the HBM budget")
+                    if used + est > self.hbm_budget or len(active) + len(preparing) >= self.max_members:
+                        break
+                    waiting.popleft()
+                    used += est
+                    preparing[pool.submit(self._prepare, req)] = (req, est, t)
+                    say(f"[serve] t={t:.2f}s admit request {req['rid']} (n={req['n']}, ~{est / 1e9:.2f} GB)")
+                # prepared requests join the batch at this step boundary
+                for fut in [f for f in preparing if f.done()]:
+                    req, est, t_adm = preparing.pop(fut)
+                    sim, P, n_dof = fut.result()
+                    job = SweepJob(req, sim, P, n_dof, est, req["arrival"], t_adm)
+                    job.t_point = now()
+                    batch.add(sim)
+                    active.append(job)
+                    say(f"[serve] t={now():.2f}s request {job.rid} joins ({len(active)} members)")
+                if not active:
+                    if preparing:
+                        cf.wait(list(preparing), timeout=0.05, return_when=cf.FIRST_COMPLETED)
+                    elif pending:
+                        time.sleep(max(0.0, min(0.05, pending[0]["arrival"] - now())))
+                    continue
+                max_members = max(max_members, len(active))
+                reps = batch.step([j.field() for j in active], [j.use_init for j in active])
+                steps_run += 1
+                t = now()
+                finished = []
+                for job, rep in zip(active, reps):
+                    node_steps += job.n_dof
+                    if job.advance(rep, t):
+                        finished.append(job)
+                for job in finished:
+                    job.t_done = t
+                    batch.remove(job.sim)
+                    active.remove(job)
+                    used -= job.est
+                    results[job.rid] = dict(
+                        rid=job.rid, n=job.req["n"], n_dof=job.n_dof, points=job.points, steps=job.steps,
+                        m=job.sim.get("M").cpu().numpy() if self.keep_m else None,
+                        loop_latency=job.t_done - job.t_arrival, queue_delay=job.t_admit - job.t_arrival,
+                        point_latency=list(job.point_lat))
+                    say(f"[serve] t={t:.2f}s request {job.rid} done ({job.steps} steps, "
+                        f"{len(active)} members left)")
+                    job.sim = None
+        finally:
+            pool.shutdown(wait=True)
+        wall = now()
+        pl = np.concatenate([np.asarray(r["point_latency"]) for r in results.values()]) if results else np.zeros(0)
+        ll = np.asarray([r["loop_latency"] for r in results.values()])
+        qd = np.asarray([r["queue_delay"] for r in results.values()])
+        pct = lambda a, q: float(np.percentile(a, q)) if a.size else None  # noqa: E731
+        stats = dict(requests=len(results), wall_s=wall, batch_steps=steps_run, node_steps=node_steps,
+                     node_steps_per_s=node_steps / wall if wall > 0 else 0.0, max_members=max_members,
+                     point_latency_s=dict(p50=pct(pl, 50), p99=pct(pl, 99)),
+                     loop_latency_s=dict(p50=pct(ll, 50), p99=pct(ll, 99)),
+                     queue_delay_s=dict(p50=pct(qd, 50), p99=pct(qd, 99)))
+        return results, stats

@@ -245,6 +245,36 @@ int cpm_sim_get(cpm_sim* sim, int field, double* dst);
 int cpm_sim_step(cpm_sim* sim, const double* H, int use_init, cpm_step_report* rep);
 
 /* ------------------------------------------------------------------ */
+/* batches: many simulations (any mesh sizes) stepped together          */
+/* ------------------------------------------------------------------ */
+/* No counterpart in the reference, which runs one simulation per process
+ * (Micromagnetics.py:34).  A batch steps all of its simulations through
+ * the same stages of the step; each of the step's solves runs as one
+ * batched CG over every member (one launch per CG phase for the whole
+ * batch), so small meshes share the launches of the large ones.  Members
+ * must use the batch's context and one of the SELL modes (all the same).
+ * A member's arithmetic does not depend on the other members: its results
+ * are bit-identical alone and in any batch.  Members can be added and
+ * removed between steps (continuous batching); a member cannot be stepped
+ * with cpm_sim_step while in a batch.  Destroying a member removes it. */
+typedef struct cpm_batch cpm_batch;
+
+#define CPM_BATCH_GRAPH 1 /* replay captured chunks of CG iterations (hipGraph) */
+
+int cpm_batch_create(cpm_ctx* ctx, int flags, cpm_batch** out);
+int cpm_batch_destroy(cpm_batch* batch);
+/* Allocates the member's batched-CG workspace (3n FP64 + 4 x 3n FP32 for
+ * the mixed mode, 3 x 3n FP64 for FAST64) on first use. */
+int cpm_batch_add(cpm_batch* batch, cpm_sim* sim);
+int cpm_batch_remove(cpm_batch* batch, cpm_sim* sim);
+int cpm_batch_size(const cpm_batch* batch, int32_t* n);
+/* One step of every member, in the order they were added: H (host, 3 per
+ * member, reduced units), use_init (host, 1 per member), reps (host, 1 per
+ * member).  A member whose solve did not converge has reps[j].failed set
+ * and its step carried on; the call still returns 0 (errors < 0 only). */
+int cpm_batch_step(cpm_batch* batch, const double* H, const int32_t* use_init, cpm_step_report* reps);
+
+/* ------------------------------------------------------------------ */
 /* host-side assembly (CPU, once per mesh)                             */
 /* ------------------------------------------------------------------ */
 typedef struct cpm_host_ops cpm_host_ops;

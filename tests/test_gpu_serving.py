@@ -1,0 +1,81 @@
+"""The continuous-batching sweep server on the GPU.
+
+Each request's results (final m and the average magnetization at every
+field point) are bit-identical whether it is served alone, in a mix that
+arrives while others are running, or under an HBM budget that admits one
+request at a time; the admission estimate bounds the device memory a member
+actually takes.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def _requests(count=3, seed=9, me=True):
+    from cupymag_amd.serving.workloads import sweep_requests
+    reqs = sweep_requests(count, rate=200.0, seed=seed, nmin=8, nmax=12, points=3, steps_per_point=3, me=me,
+                          dt=5e-12, maxiter=5000)
+    for k, r in enumerate(reqs):  # all arrive within a few ms: they overlap on the device
+        r["arrival"] = 0.002 * k
+    return reqs
+
+
+def _same(a, b):
+    np.testing.assert_array_equal(a["m"], b["m"])
+    assert len(a["points"]) == len(b["points"])
+    for (h1, m1, s1), (h2, m2, s2) in zip(a["points"], b["points"]):
+        np.testing.assert_array_equal(h1, h2)
+        np.testing.assert_array_equal(m1, m2)
+        assert s1 == s2
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_results_same_alone_and_in_the_mix(graphs):
+    from cupymag_amd.serving.server import SweepServer
+    reqs = _requests()
+    srv = SweepServer(mode="mixed", graphs=graphs)
+    mixed, stats = srv.run(reqs)
+    assert stats["requests"] == len(reqs)
+    assert stats["max_members"] >= 2, stats  # they did overlap on the device
+    for r in reqs:
+        alone, _ = SweepServer(mode="mixed", graphs=graphs).run([dict(r, arrival=0.0)])
+        _same(mixed[r["rid"]], alone[r["rid"]])
+        assert len(mixed[r["rid"]]["points"]) == 3
+        m = mixed[r["rid"]]["m"]
+        nrm = np.linalg.norm(m, axis=1)
+        assert np.all((np.abs(nrm - 1) < 1e-12) | (nrm == 0))
+
+
+def test_hbm_budget_serialises_admission():
+    from cupymag_amd.serving.server import SweepServer, estimate_bytes
+    reqs = _requests(seed=10)
+    ests = [estimate_bytes(r["n"] ** 3, True) for r in reqs]
+    free, _ = SweepServer(mode="mixed").run(reqs)
+    tight, stats = SweepServer(mode="mixed", hbm_budget=max(ests) * 1.01).run(reqs)
+    assert stats["max_members"] == 1
+    for r in reqs:
+        _same(free[r["rid"]], tight[r["rid"]])
+    with pytest.raises(MemoryError):
+        SweepServer(mode="mixed", hbm_budget=min(ests) * 0.5).run(reqs[:1])
+
+
+@pytest.mark.parametrize("me", [False, True])
+def test_admission_estimate_bounds_member_memory(me):
+    from cupymag_amd.core.batch import GSPMBatch
+    from cupymag_amd.serving.server import SweepServer, estimate_bytes
+    from cupymag_amd.serving.workloads import sweep_requests
+    req = sweep_requests(1, seed=11, nmin=24, nmax=24, points=2, steps_per_point=1, me=me)[0]
+    srv = SweepServer(mode="mixed")
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    free0, _ = torch.cuda.mem_get_info()
+    sim, P, n_dof = srv._prepare(req)
+    b = GSPMBatch(srv.ctx)
+    b.add(sim)  # batched-CG workspace
+    torch.cuda.synchronize()
+    free1, _ = torch.cuda.mem_get_info()
+    used = free0 - free1
+    assert n_dof == 24 ** 3
+    assert 0 < used <= estimate_bytes(n_dof, me), (used, estimate_bytes(n_dof, me))
