@@ -142,6 +142,7 @@ def run_gpu(n, steps, warmup, steps64, modes, rel_delta=0.3):
         if base == "fp64":
             # CSR FP64: values + column indices + row pointers + p read + q written
             kernel = "k_cg_spmv (A_el, CSR FP64)"
+            rocname = None
             bytes_launch = 12.0 * nnz + 4.0 * (rows + 1) + 16.0 * rows
         else:
             # SELL-64 3x3 blocks: off-diagonal blocks 9 values (+ 1 column index
@@ -152,14 +153,33 @@ def run_gpu(n, steps, warmup, steps64, modes, rel_delta=0.3):
             idx = 0 if info["stencil"] else 4
             form = "stencil" if info["stencil"] else "indexed"
             kernel = f"k_sell_spmv<3,1,{'double' if s_ == 8 else 'float'}> (A_el, SELL-64 3x3 blocks, {form})"
+            rocname = (f"k_sell_spmv<3, 1, {'double' if s_ == 8 else 'float'}, true, false, false, "
+                       f"{'true' if info['stencil'] else 'false'}>")
             nb = rows // 3
             bytes_launch = (nnz / 9.0 - nb) * (9 * s_ + idx) + nb * 72.0 + 2.0 * rows * s_
         avg_s = ms.value / 1e3 / max(nl.value, 1)
         out["modes"][mode] = dict(elapsed=elapsed, steps=k_timed, lat=lat, iters=iters, m=snap,
-                                  spmv=dict(kernel=kernel, launches=nl.value, avg_us=avg_s * 1e6,
+                                  spmv=dict(kernel=kernel, rocname=rocname, launches=nl.value, avg_us=avg_s * 1e6,
                                             bytes=bytes_launch, gbs=bytes_launch / avg_s / 1e9 if nl.value else 0.0))
         del sim
     return out
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "cg_spmv_A_el_128cubed_hbm_traffic.md")
+
+
+def profiled_traffic(kernel, n):
+    """HBM bytes per launch of `kernel` at 128^3 from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel
+    calibration there); None for other kernels or sizes."""
+    if n != 128 or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    for line in open(TRAFFIC_PROFILE):
+        if line.startswith("traffic_bytes_per_launch "):
+            name, _, val = line[len("traffic_bytes_per_launch "):].strip().rpartition(" ")
+            if name == kernel:
+                return dict(bytes=float(val), source=os.path.relpath(TRAFFIC_PROFILE, ROOT))
+    return None
 
 
 def cpu_baseline(n_cpu, steps=1):
@@ -397,6 +417,7 @@ def main():
             d["err_after_steps"] = args.warmup + steps64
         per_mode[mode] = d
     sp = hm["spmv"]
+    traffic = profiled_traffic(sp["rocname"], args.n) if sp.get("rocname") else None
     out = {
         "metric": "LLG node-steps/s",
         "value": value,
@@ -418,7 +439,9 @@ def main():
                             if "fp64" in per_mode else None),
         "modes": per_mode,
         "roofline": {"bound": "hbm", "kernel": sp["kernel"], "achieved": sp["gbs"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": sp["gbs"] / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": sp["gbs"] / HBM_PEAK_GBS,
+                     "traffic": traffic["bytes"] if traffic else None,
+                     "traffic_source": traffic["source"] if traffic else None,
                      "avg_launch_us": sp["avg_us"], "bytes_per_launch": sp["bytes"], "launches": sp["launches"]},
         "cpu_baseline": cpu,
     }
