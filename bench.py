@@ -69,7 +69,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_gpu(n, steps, warmup, steps64, modes, rel_delta=0.3):
+def run_gpu(n, steps, warmup, steps64, modes, rel_delta=0.3, autotune=True):
     """Run the plain FP64 path and the fast paths on the same operators and
     initial state; returns per-mode timings, CG iteration counts, the SpMV
     kernel timing of the elasticity operator and the magnetization after the
@@ -89,16 +89,21 @@ def run_gpu(n, steps, warmup, steps64, modes, rel_delta=0.3):
     P = from_dict(cfg)
     t0 = time.time()
     arrays = kuhn_mesh(n)
-    S = SimulationOperators(P, mesh_arrays=arrays, keep_host=False)
+    S = SimulationOperators(P, mesh_arrays=arrays, keep_host=False, autotune=autotune)
     log(f"[bench] mesh {n}^3: {S.n_nodes} nodes, {S.n_elements} tets, {S.n_dof} DOFs, "
         f"{int(S.elements_np[:, -1].sum())} defect tets ({time.time() - t0:.1f}s)")
     t0 = time.time()
     S.csr_demag_and_deriv()
-    log(f"[bench] host assembly + upload (CSR and SELL-64 forms) {time.time() - t0:.1f}s")
+    log(f"[bench] host assembly + upload (CSR and SELL-64 forms){' + autotuning' if autotune else ''} "
+        f"{time.time() - t0:.1f}s")
+    for name, (sched, timings) in S.tuning.items():
+        log(f"[bench] schedule {name}: {sched}" + ("" if timings is None else
+            f" (best of {len(timings)}: {min(t['us'] for t in timings):.1f} us)"))
     A_el = S.csr_elasticity()[0]
     m0 = initial_m(S.n_dof)
     H = [P.Hext1, P.Hext2, P.Hext3]
-    out = dict(n_dof=S.n_dof, n_nodes=S.n_nodes, n_elements=S.n_elements, modes={})
+    out = dict(n_dof=S.n_dof, n_nodes=S.n_nodes, n_elements=S.n_elements, modes={},
+               schedules={k: v[0] for k, v in S.tuning.items()})
     snap_at = warmup + steps64
     for mode in modes:
         base, _, opt = mode.partition("+")
@@ -366,6 +371,7 @@ def main():
                     help="comma list of fp64, fast64, mixed; '+warm' appended to a fast mode starts every CG "
                          "from the previous step's solution")
     ap.add_argument("--rel-delta", type=float, default=0.3)
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default SpMV launch schedules")
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--cpu-n", type=int, default=40)
     ap.add_argument("--cpu-steps", type=int, default=1)
@@ -400,7 +406,7 @@ def main():
         return run_serve(args)
     modes = [m for m in args.modes.split(",") if m]
     steps64 = min(args.steps_fp64, args.steps) if "fp64" in modes else args.steps_fp64
-    g = run_gpu(args.n, args.steps, args.warmup, steps64, modes, args.rel_delta)
+    g = run_gpu(args.n, args.steps, args.warmup, steps64, modes, args.rel_delta, autotune=not args.no_autotune)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n, args.cpu_steps)
     head = "mixed" if "mixed" in modes else modes[-1]
     hm = g["modes"][head]
@@ -433,7 +439,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"GSPM LLG step, {args.n}^3 Kuhn tet mesh, rotated-ellipsoid defect, "
                                f"magnetoelastic on, rot111, dt 10 ps, alpha 0.1, tol 1e-10",
-                   "path": head, "n_dof": g["n_dof"], "nodes": g["n_nodes"], "elements": g["n_elements"]},
+                   "path": head, "n_dof": g["n_dof"], "nodes": g["n_nodes"], "elements": g["n_elements"],
+                   "spmv_schedules": g["schedules"]},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "speedup_vs_fp64": (per_mode[head]["node_steps_per_s"] / per_mode["fp64"]["node_steps_per_s"]
                             if "fp64" in per_mode else None),

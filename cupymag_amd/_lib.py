@@ -5,6 +5,7 @@ The library is built in-tree by ``cupymag_amd.build`` (``__graft_entry__``
 calls it).  Loading fails loudly when the library is missing: there is no
 CPU fallback for any device operation.
 """
+import atexit
 import ctypes as C
 import os
 from pathlib import Path
@@ -21,6 +22,7 @@ FIELD = dict(M=0, HTILDE=1, E=2, U=3, UEL=4)
 MODE_FP64, MODE_FAST64, MODE_MIXED = 0, 1, 2
 MODES = {"fp64": MODE_FP64, "fast64": MODE_FAST64, "mixed": MODE_MIXED}
 BATCH_GRAPH = 1
+FMT = {0: "csr", 1: "sell64", 2: "sell32"}
 
 
 class CpmError(RuntimeError):
@@ -44,6 +46,15 @@ class cpm_sim_params(C.Structure):
 class cpm_step_report(C.Structure):
     _fields_ = [("sumsq", C.c_double), ("avg_m", C.c_double * 3), ("iters", C.c_int32 * 10),
                 ("failed", C.c_int32), ("resid", C.c_double)]
+
+
+class cpm_schedule(C.Structure):
+    _fields_ = [("lanes", C.c_int32), ("grid_cap", C.c_int32), ("sell_grid_cap", C.c_int32),
+                ("spmv_sell", C.c_int32)]
+
+
+class cpm_tune_entry(C.Structure):
+    _fields_ = [("format", C.c_int32), ("lanes", C.c_int32), ("grid_cap", C.c_int32), ("us", C.c_float)]
 
 
 _P = C.c_void_p
@@ -75,6 +86,9 @@ _SIGS = {
     "cpm_sim_set_m": (C.c_int, [_P, _P, _P]),
     "cpm_sim_get": (C.c_int, [_P, C.c_int, _P]),
     "cpm_sim_step": (C.c_int, [_P, _P, C.c_int, C.POINTER(cpm_step_report)]),
+    "cpm_mat_autotune": (C.c_int, [_P, _P, C.c_int, C.POINTER(C.c_int)]),
+    "cpm_mat_set_schedule": (C.c_int, [_P, C.POINTER(cpm_schedule)]),
+    "cpm_mat_get_schedule": (C.c_int, [_P, C.POINTER(cpm_schedule)]),
     "cpm_batch_create": (C.c_int, [_P, C.c_int, C.POINTER(_P)]),
     "cpm_batch_destroy": (C.c_int, [_P]),
     "cpm_batch_add": (C.c_int, [_P, _P]),
@@ -89,6 +103,22 @@ _SIGS = {
 }
 
 _lib = None
+_finalizing = False
+
+
+@atexit.register
+def _at_exit():
+    # Objects still alive when the interpreter finalizes (for instance kept
+    # by a traceback) are not destroyed through the library: the device
+    # runtime may already be shutting down, and process exit releases
+    # everything anyway.
+    global _finalizing
+    _finalizing = True
+
+
+def alive():
+    """True while handles may still be destroyed through the library."""
+    return _lib is not None and not _finalizing
 
 
 def lib():

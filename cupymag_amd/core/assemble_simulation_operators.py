@@ -17,11 +17,17 @@ from ..utils.volume_average import VolumeAverage
 
 
 class SimulationOperators(FEMMesh):
-    def __init__(self, P, mesh_arrays=None, ctx=None, keep_host=None, nthreads=None, sell=True):
+    def __init__(self, P, mesh_arrays=None, ctx=None, keep_host=None, nthreads=None, sell=True, autotune=False):
         """``sell``: also attach the SELL-64 forms the fast / mixed-precision
-        solvers run on (A_demag, A1, A2 scalar; A_el in 3x3 blocks)."""
+        solvers run on (A_demag, A1, A2 scalar; A_el in 3x3 blocks).
+        ``autotune``: time the launch schedules of every operator after the
+        upload and keep the fastest (cupymag_amd.autotune, cached per size
+        class); ``self.tuning`` then maps operator names to their schedule
+        and timings."""
         super().__init__(P, mesh_arrays)
         self.sell = sell
+        self.autotune = autotune
+        self.tuning = {}
         self.alpha1 = P.alpha
         self.alpha2 = P.alpha * P.alpha_damp
         self.ctx = ctx
@@ -64,6 +70,10 @@ class SimulationOperators(FEMMesh):
             d["A_el"] = up("A_el", sell_bs=3)
             d["F_el"] = up("F_el")
         H.free()
+        if self.autotune:
+            from ..autotune import tune
+            for name, M in d.items():
+                self.tuning[name] = tune(M)
         self._dev = d
         return d
 

@@ -68,6 +68,6 @@ class GSPMBatch:
         return out
 
     def __del__(self):
-        if getattr(self, "h", None) and _lib._lib is not None:
+        if getattr(self, "h", None) and _lib.alive():
             _lib._lib.cpm_batch_destroy(self.h)
             self.h = None

@@ -120,6 +120,6 @@ class GSPMSimulation:
         return out
 
     def __del__(self):
-        if getattr(self, "h", None) and _lib._lib is not None:
+        if getattr(self, "h", None) and _lib.alive():
             _lib._lib.cpm_sim_destroy(self.h)
             self.h = None

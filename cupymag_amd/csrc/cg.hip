@@ -130,15 +130,22 @@ __global__ void __launch_bounds__(kBlock) k_cg_zero_b(int64_t n, double* __restr
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) x[i] = 0.0;
 }
 
+// workgroups of a CSR SpMV launch with G lanes per row, at most A->grid_cap
+template <int G>
+static int csr_grid(const cpm_mat* A) {
+  const int64_t g = ((int64_t)A->nrows + kBlock / G - 1) / (kBlock / G);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(g, A->grid_cap));
+}
+
 template <int G>
 static void launch_spmv_g(const cpm_mat* A, const double* val, const double* x, double* y, hipStream_t s) {
-  const int grid = grid_for(A->nrows, kBlock / G);
+  const int grid = csr_grid<G>(A);
   k_spmv<G><<<grid, kBlock, 0, s>>>(A->nrows, A->ptr, A->idx, val, x, y);
 }
 
 template <int G>
 static void launch_cg_spmv_g(cpm_ctx* ctx, const cpm_mat* A, CgState* st, int* grid_out) {
-  const int grid = grid_for(A->nrows, kBlock / G);
+  const int grid = csr_grid<G>(A);
   k_cg_spmv<G><<<grid, kBlock, 0, ctx->stream>>>(A->nrows, A->ptr, A->idx, A->val, ctx->p, ctx->q, st,
                                                    ctx->partials);
   *grid_out = grid;
@@ -157,6 +164,7 @@ static void launch_cg_spmv_g(cpm_ctx* ctx, const cpm_mat* A, CgState* st, int* g
 
 int launch_spmv(cpm_ctx* ctx, const cpm_mat* A, int valset, const double* x, double* y) {
   if (A->nrows == 0) return CPM_OK;
+  if (valset == 0 && A->spmv_sell && A->sell.sptr && A->sell.bs == 1) return sell_spmv64(ctx, A, 1, x, y);
   const double* val = A->val + (int64_t)valset * A->nnz;
   CPM_LANES_DISPATCH(A->lanes, launch_spmv_g, A, val, x, y, ctx->stream);
   CPM_HIP(hipGetLastError());
@@ -189,9 +197,9 @@ int run_cg(cpm_ctx* ctx, const cpm_mat* A, const double* b, double* x, int use_x
       int gsp = 0;
       CPM_LANES_DISPATCH(A->lanes, launch_cg_spmv_g, ctx, A, st, &gsp);
       if (ctx->prof) CPM_HIP(hipEventRecord(ctx->ev[2 * k + 1], s));
-      k_red<1, RED_ALPHA><<<1, kBlock, 0, s>>>(gsp, ctx->partials, st, 0.0);
+      k_red<1, RED_ALPHA><<<1, kRedBlock, 0, s>>>(gsp, ctx->partials, st, 0.0);
       k_cg_xr<<<gv, kBlock, 0, s>>>(n, x, ctx->r, ctx->p, ctx->q, st, ctx->partials);
-      k_red<1, RED_RR><<<1, kBlock, 0, s>>>(gv, ctx->partials, st, 0.0);
+      k_red<1, RED_RR><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st, 0.0);
     }
     issued += todo;
     CPM_HIP(hipGetLastError());

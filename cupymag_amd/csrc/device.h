@@ -23,8 +23,12 @@ namespace cpm {
 constexpr int kBlock = 256;      // 4 wave64 per workgroup
 constexpr int kMaxGrid = 2048;   // 8 workgroups per CU on 256 CUs; grid-stride beyond
 constexpr int kMaxRed = 12;      // max values reduced together
-constexpr int kMaxPartials = kMaxRed * kMaxGrid;  // doubles in ctx->partials
-constexpr int kSellMaxGrid = 8192;  // SELL SpMV: up to K = 3 partials per workgroup still fit
+constexpr int kSellMaxGrid = 8192;   // default SELL SpMV grid cap
+constexpr int kMaxSpmvGrid = 32768;  // largest SpMV grid cap an operator can be tuned to
+// doubles in ctx->partials: kMaxRed per workgroup of a vector kernel, or up
+// to K = 3 per workgroup of a tuned SpMV
+constexpr int kMaxPartials = (kMaxRed * kMaxGrid > 3 * kMaxSpmvGrid) ? kMaxRed * kMaxGrid : 3 * kMaxSpmvGrid;
+constexpr int kRedBlock = 1024;  // threads of the one-workgroup reducers
 
 // Per-solve CG scalars, device resident.
 struct CgState {
@@ -69,6 +73,8 @@ struct cpm_mat {
   int32_t* idx = nullptr;
   double* val = nullptr;   // nvals * nnz
   int lanes = 8;           // threads per row of the vector-CSR kernels
+  int grid_cap = cpm::kMaxGrid;  // workgroup cap of the CSR SpMV launches (grid-stride beyond)
+  int spmv_sell = 0;       // cpm_spmv on value set 0 uses the FP64 SELL form (scalar operators)
   cpm::Sell sell;          // optional SELL-64 form (cpm_mat_add_sell)
   double prof_ms = 0.0;    // accumulated CG SpMV kernel time (profiling on)
   int64_t prof_launches = 0;
@@ -115,6 +121,27 @@ __device__ inline void block_sum(double (&v)[NV], double* lds /* NV*4 */) {
   if (threadIdx.x == 0)
 #pragma unroll
     for (int k = 0; k < NV; ++k) v[k] = ((lds[k * 4 + 0] + lds[k * 4 + 1]) + lds[k * 4 + 2]) + lds[k * 4 + 3];
+  __syncthreads();
+}
+
+// Sum over a workgroup of NT threads; the result is valid in thread 0.  The
+// waves' sums are added in wave order.
+template <int NV, int NT>
+__device__ inline void block_sum_t(double (&v)[NV], double* lds /* NV*NT/64 */) {
+  constexpr int NW = NT / 64;
+  wave_sum<NV>(v);
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) lds[k * NW + wid] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double t = lds[k * NW];
+      for (int w = 1; w < NW; ++w) t += lds[k * NW + w];
+      v[k] = t;
+    }
   __syncthreads();
 }
 
@@ -227,21 +254,22 @@ __device__ inline void cg_state_update(CgState* st, const double (&v)[K], double
 
 // One workgroup: add the partials of the preceding kernel (deterministic
 // order) and advance the CG scalars.
+// Launched with kRedBlock threads.
 template <int K, int MODE>
-__global__ void __launch_bounds__(kBlock) k_red(int nblk, const double* __restrict__ partials, CgState* st,
-                                                double delta) {
+__global__ void __launch_bounds__(kRedBlock) k_red(int nblk, const double* __restrict__ partials, CgState* st,
+                                                   double delta) {
   bool any = false;
 #pragma unroll
   for (int j = 0; j < K; ++j) any |= cg_state_uses<MODE>(st[j]);
   if (!any) return;
-  __shared__ double lds[K * 4];
+  __shared__ double lds[K * (kRedBlock / 64)];
   double v[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) v[j] = 0.0;
-  for (int i = threadIdx.x; i < nblk; i += kBlock)
+  for (int i = threadIdx.x; i < nblk; i += kRedBlock)
 #pragma unroll
     for (int j = 0; j < K; ++j) v[j] += partials[j * nblk + i];
-  block_sum<K>(v, lds);
+  block_sum_t<K, kRedBlock>(v, lds);
   if (threadIdx.x == 0) cg_state_update<K, MODE>(st, v, delta);
 }
 

@@ -13,6 +13,7 @@ namespace cpm {
 
 struct CgState;
 struct BSys;
+struct Sell;
 
 // One system of a batched solve.  Vectors are interleaved as the SELL
 // solvers expect (length n = block rows * bs * K); the workspace vectors
@@ -58,7 +59,7 @@ struct MultiSolver {
 int multi_cg(cpm_ctx* ctx, MultiSolver& M, int bs, int K, int mixed, const std::vector<MultiSys>& sys, int hint,
              bool graph, int32_t* iters, double* resid, int* failed);
 
-// workgroups a system of `nbrows` block rows is given in batched launches
-int multi_blocks(int32_t nbrows);
+// workgroups a system is given in batched launches (its SELL grid)
+int multi_blocks(const Sell& A);
 
 }  // namespace cpm

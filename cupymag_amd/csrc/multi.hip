@@ -148,8 +148,8 @@ __global__ void __launch_bounds__(kBlock) kb_zero_b(const BSys* __restrict__ sys
 enum : int { RED_INIT64 = 10, RED_INIT_REL = 11 };
 
 template <int K, int MODE>
-__global__ void __launch_bounds__(kBlock) kb_red(const BSys* __restrict__ sys, int G,
-                                                 const double* __restrict__ partials) {
+__global__ void __launch_bounds__(kRedBlock) kb_red(const BSys* __restrict__ sys, int G,
+                                                    const double* __restrict__ partials) {
   constexpr bool INIT = MODE == RED_INIT64 || MODE == RED_INIT_REL;
   constexpr int NV = INIT ? 2 * K : K;
   const BSys& d = sys[blockIdx.x];
@@ -159,14 +159,14 @@ __global__ void __launch_bounds__(kBlock) kb_red(const BSys* __restrict__ sys, i
     for (int j = 0; j < K; ++j) any |= cg_state_uses<MODE>(d.st[j]);
     if (!any) return;
   }
-  __shared__ double lds[NV * 4];
+  __shared__ double lds[NV * (kRedBlock / 64)];
   double v[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) v[k] = 0.0;
-  for (int i = threadIdx.x; i < d.nblk; i += kBlock)
+  for (int i = threadIdx.x; i < d.nblk; i += kRedBlock)
 #pragma unroll
     for (int k = 0; k < NV; ++k) v[k] += partials[(int64_t)k * G + d.blk0 + i];
-  block_sum<NV>(v, lds);
+  block_sum_t<NV, kRedBlock>(v, lds);
   if (threadIdx.x != 0) return;
   if constexpr (MODE == RED_INIT64)
     cg_state_init<K, false>(d.st, v, d.tol, d.maxiter);
@@ -178,10 +178,7 @@ __global__ void __launch_bounds__(kBlock) kb_red(const BSys* __restrict__ sys, i
 
 // ---------------------------------------------------------------- host
 
-int multi_blocks(int32_t nbrows) {
-  const int64_t rows = ((int64_t)nbrows + 63) & ~(int64_t)63;
-  return (int)std::max<int64_t>(1, std::min<int64_t>((rows + kBlock - 1) / kBlock, kSellMaxGrid));
-}
+int multi_blocks(const Sell& A) { return sell_grid(A); }
 
 MultiSolver::~MultiSolver() {
   if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -229,18 +226,18 @@ static int issue_iters(const MultiSolver& M, int nsys, int G, int todo, hipStrea
     if (MIXED) {
       kb_pupd<K, float><<<G, kBlock, 0, s>>>(sys, b2s);
       kb_spmv<BS, K, float, OP_CG, true><<<G, kBlock, 0, s>>>(sys, b2s, pt);
-      kb_red<K, RED_ALPHA><<<nsys, kBlock, 0, s>>>(sys, G, pt);
+      kb_red<K, RED_ALPHA><<<nsys, kRedBlock, 0, s>>>(sys, G, pt);
       kb_xr<K, float><<<G, kBlock, 0, s>>>(sys, b2s, pt);
-      kb_red<K, RED_RR_REL><<<nsys, kBlock, 0, s>>>(sys, G, pt);
+      kb_red<K, RED_RR_REL><<<nsys, kRedBlock, 0, s>>>(sys, G, pt);
       kb_spmv<BS, K, double, OP_REL, true><<<G, kBlock, 0, s>>>(sys, b2s, pt);
       kb_rel_r<K><<<G, kBlock, 0, s>>>(sys, b2s, pt);
-      kb_red<K, RED_RR_TRUE><<<nsys, kBlock, 0, s>>>(sys, G, pt);
+      kb_red<K, RED_RR_TRUE><<<nsys, kRedBlock, 0, s>>>(sys, G, pt);
     } else {
       kb_pupd<K, double><<<G, kBlock, 0, s>>>(sys, b2s);
       kb_spmv<BS, K, double, OP_CG, false><<<G, kBlock, 0, s>>>(sys, b2s, pt);
-      kb_red<K, RED_ALPHA><<<nsys, kBlock, 0, s>>>(sys, G, pt);
+      kb_red<K, RED_ALPHA><<<nsys, kRedBlock, 0, s>>>(sys, G, pt);
       kb_xr<K, double><<<G, kBlock, 0, s>>>(sys, b2s, pt);
-      kb_red<K, RED_RR><<<nsys, kBlock, 0, s>>>(sys, G, pt);
+      kb_red<K, RED_RR><<<nsys, kRedBlock, 0, s>>>(sys, G, pt);
     }
   }
   CPM_HIP(hipGetLastError());
@@ -252,7 +249,7 @@ static int multi_loop(cpm_ctx* ctx, MultiSolver& M, const std::vector<MultiSys>&
                       int32_t* iters, double* resid, int* failed) {
   const int nsys = (int)sys.size();
   int G = 0;
-  for (const MultiSys& m : sys) G += multi_blocks(m.A->sell.nbrows);
+  for (const MultiSys& m : sys) G += multi_blocks(m.A->sell);
   if (int rc = multi_reserve(M, nsys, G)) return rc;
   int maxit = 0;
   bool any_x0 = false;
@@ -283,7 +280,7 @@ static int multi_loop(cpm_ctx* ctx, MultiSolver& M, const std::vector<MultiSys>&
     d.x32 = m.x32;
     d.st = M.d_st + 3 * j;
     d.blk0 = blk;
-    d.nblk = multi_blocks(A.nbrows);
+    d.nblk = multi_blocks(A);
     d.use_x0 = m.use_x0;
     d.maxiter = m.maxiter;
     d.tol = m.tol;
@@ -298,7 +295,7 @@ static int multi_loop(cpm_ctx* ctx, MultiSolver& M, const std::vector<MultiSys>&
   CPM_HIP(hipMemcpyAsync(M.d_b2s, M.h_b2s, sizeof(int32_t) * G, hipMemcpyHostToDevice, s));
   if (any_x0) kb_spmv<BS, K, double, OP_X0, MIXED><<<G, kBlock, 0, s>>>(M.d_sys, M.d_b2s, M.partials);
   kb_init<K, MIXED><<<G, kBlock, 0, s>>>(M.d_sys, M.d_b2s, M.partials);
-  kb_red<K, MIXED ? RED_INIT_REL : RED_INIT64><<<nsys, kBlock, 0, s>>>(M.d_sys, G, M.partials);
+  kb_red<K, MIXED ? RED_INIT_REL : RED_INIT64><<<nsys, kRedBlock, 0, s>>>(M.d_sys, G, M.partials);
   if (any_x0) kb_zero_b<K><<<G, kBlock, 0, s>>>(M.d_sys, M.d_b2s);
   CPM_HIP(hipGetLastError());
 

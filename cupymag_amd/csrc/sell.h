@@ -24,9 +24,11 @@ struct Sell {
   int32_t nst = 0;           // offsets per row (14 Kuhn, 26 hex)
   int32_t dims[3] = {0, 0, 0};
   int32_t* soff = nullptr;   // device, packed (di+1) | (dj+1) << 2 | (dk+1) << 4
+  int grid_cap = 8192;       // workgroup cap of the SpMV launches (tunable, grid-stride beyond)
 };
 
 void sell_free(Sell& S);
+int sell_grid(const Sell& A);
 int sell_spmv64(cpm_ctx* ctx, const cpm_mat* M, int K, const double* x, double* y);
 int sell_cg64(cpm_ctx* ctx, cpm_mat* M, int K, const double* b, double* x, int use_x0, double tol, int maxiter,
               int hint, int32_t* iters, double* resid, int* failed);

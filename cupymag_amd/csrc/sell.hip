@@ -257,13 +257,13 @@ __global__ void __launch_bounds__(kBlock) k_rel_r(int64_t n, const double* __res
 
 // ---------------------------------------------------------------- dispatch
 
-// one thread per (block) row, up to kSellMaxGrid workgroups (3 partials each
-// fit the partials buffer)
-static inline int sell_grid(const Sell& A) {
+// one thread per (block) row, up to A.grid_cap (<= kMaxSpmvGrid) workgroups
+// (3 partials each fit the partials buffer)
+int sell_grid(const Sell& A) {
   int64_t g = ((((int64_t)A.nbrows + 63) & ~63) + kBlock - 1) / kBlock;
-  return (int)std::max<int64_t>(1, std::min<int64_t>(g, kSellMaxGrid));
+  return (int)std::max<int64_t>(1, std::min<int64_t>(g, A.grid_cap));
 }
-static_assert(3 * kSellMaxGrid <= kMaxPartials, "partials buffer too small for the SELL grid");
+static_assert(3 * kMaxSpmvGrid <= kMaxPartials, "partials buffer too small for the SELL grid");
 
 template <int BS, int K, class T, bool CG, bool GATE = false, bool ADDX32 = false>
 static int spmv_t(cpm_ctx* ctx, const Sell& A, const T* x, const float* x32, T* y) {
@@ -297,7 +297,7 @@ static int launch_sell(cpm_ctx* ctx, const Sell& A, int K, const T* x, T* y) {
 
 template <int K, int MODE>
 static void reduce_t(cpm_ctx* ctx, int nblk, double delta = 0.0) {
-  k_red<K, MODE><<<1, kBlock, 0, ctx->stream>>>(nblk, ctx->partials, ctx->cg, delta);
+  k_red<K, MODE><<<1, kRedBlock, 0, ctx->stream>>>(nblk, ctx->partials, ctx->cg, delta);
 }
 
 static int read_states(cpm_ctx* ctx, int K, CgState* h) {

@@ -36,7 +36,7 @@ class Context:
         check(lib().cpm_ctx_synchronize(self.h), "cpm_ctx_synchronize")
 
     def __del__(self):
-        if getattr(self, "h", None) and _lib._lib is not None:
+        if getattr(self, "h", None) and _lib.alive():
             _lib._lib.cpm_ctx_destroy(self.h)
             self.h = None
 
@@ -143,6 +143,6 @@ class DeviceCSR:
         return sp.csr_matrix((vv, ix, ip), shape=self.shape)
 
     def __del__(self):
-        if getattr(self, "h", None) and _lib._lib is not None:
+        if getattr(self, "h", None) and _lib.alive():
             _lib._lib.cpm_mat_destroy(self.h)
             self.h = None

@@ -137,6 +137,43 @@ int cpm_cg_sell(cpm_ctx* ctx, cpm_mat* A, int nrhs, const double* b, double* x, 
 int cpm_spmv_sell(cpm_ctx* ctx, const cpm_mat* A, int nrhs, const double* x, double* y);
 
 /* ------------------------------------------------------------------ */
+/* SpMV autotuning                                                     */
+/* ------------------------------------------------------------------ */
+/* No counterpart in the reference (cuSPARSE picks its own kernel behind
+ * `A @ x`, final_assembly.py:176-177).  The launch schedule of an operator:
+ * CSR lanes per row (64/lanes rows per wavefront) and workgroup cap, SELL
+ * workgroup cap, and the format of the standalone product y = A x of a
+ * scalar operator (spmv_sell = 1: the FP64 SELL form; results equal to
+ * rounding).  Every solver and product reads it. */
+typedef struct cpm_schedule {
+  int32_t lanes;          /* CSR: threads per row, power of 2 in [1, 64] */
+  int32_t grid_cap;       /* CSR: at most this many workgroups per launch */
+  int32_t sell_grid_cap;  /* SELL: at most this many workgroups per launch (0: no SELL form) */
+  int32_t spmv_sell;      /* cpm_spmv on value set 0 runs the SELL form */
+} cpm_schedule;
+
+#define CPM_FMT_CSR 0
+#define CPM_FMT_SELL64 1
+#define CPM_FMT_SELL32 2
+
+typedef struct cpm_tune_entry {
+  int32_t format;   /* CPM_FMT_* */
+  int32_t lanes;    /* CSR lanes per row (64 for SELL: one row per lane) */
+  int32_t grid_cap; /* workgroup cap */
+  float us;         /* average microseconds of the SpMV + its dot reduction */
+} cpm_tune_entry;
+
+/* Time every candidate schedule of A on its context's stream (SpMV with its
+ * fused dot partials + the reduction of those partials, as one CG
+ * iteration runs it) and keep the fastest: CSR lanes x cap, SELL cap (FP32
+ * form, the mixed solvers' hot kernel), and the product format.  Writes up
+ * to max_out timings to out (may be NULL) and their count to n_out.
+ * Synchronises the stream. */
+int cpm_mat_autotune(cpm_mat* A, cpm_tune_entry* out, int max_out, int* n_out);
+int cpm_mat_set_schedule(cpm_mat* A, const cpm_schedule* s);
+int cpm_mat_get_schedule(const cpm_mat* A, cpm_schedule* s);
+
+/* ------------------------------------------------------------------ */
 /* field operators                                                     */
 /* ------------------------------------------------------------------ */
 
