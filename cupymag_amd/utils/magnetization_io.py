@@ -2,14 +2,16 @@
 (cupymag/utils/magnetization_io.py:21-124).
 
 ``write_array`` / ``read_array`` keep the reference's HDF5 layout (dataset
-``data``, gzip level 6, attributes ``shape`` and ``dtype``) when h5py is
-importable.  Without h5py the same array is stored as ``<name>.npy`` next to
-the requested path and read back from there.
+``data``, gzip level 6, root attributes ``shape`` and ``dtype``): through
+h5py when it is importable, otherwise through the module's own HDF5 writer
+and reader (``h5lite``), so ``last_m.h5`` is an HDF5 file either way.
 """
 from pathlib import Path
 
 import numpy as np
 import torch
+
+from . import h5lite
 
 try:  # optional dependency of the output format
     import h5py
@@ -28,7 +30,7 @@ def write_array(array, filename):
             f.attrs["shape"] = data.shape
             f.attrs["dtype"] = str(data.dtype)
     else:
-        np.save(filename.with_suffix(".npy"), data)
+        h5lite.write(filename, data, attrs={"shape": data.shape, "dtype": str(data.dtype)}, level=6)
 
 
 def read_array(filename, nDOF):
@@ -38,11 +40,11 @@ def read_array(filename, nDOF):
     filename = Path(filename)
     if filename.suffix != ".h5":
         filename = filename.with_suffix(".h5")
-    if h5py is not None and filename.exists():
+    if h5py is not None:
         with h5py.File(filename, "r") as f:
             data = f["data"][:]
     else:
-        data = np.load(filename.with_suffix(".npy"), allow_pickle=False)
+        data = h5lite.read(filename, "data")
     if data.shape != (nDOF, 3):
         print(f"Initial magnetization shape mismatch: expected ({nDOF}, 3), got {data.shape}.")
         print("Start with uniform magnetization.")
