@@ -106,8 +106,13 @@ def run_gpu(n, steps, warmup, steps64, modes, rel_delta=0.3, autotune=True):
                schedules={k: v[0] for k, v in S.tuning.items()})
     snap_at = warmup + steps64
     for mode in modes:
-        base, _, opt = mode.partition("+")
-        sim = GSPMSimulation(S, P, mode=base, warm_start=(opt == "warm"), rel_delta=rel_delta)
+        base, *opts = mode.split("+")
+        if "mg" in opts:
+            t0 = time.time()
+            S.multigrid(precision=32 if base == "mixed" else 64)
+            log(f"[bench] multigrid hierarchies for {mode}: {time.time() - t0:.1f}s")
+        sim = GSPMSimulation(S, P, mode=base, warm_start=("warm" in opts), rel_delta=rel_delta,
+                             precond="mg" if "mg" in opts else None)
         ctx = sim.ctx
         sim.set_m(m0)
         k_timed = steps64 if base == "fp64" else steps
@@ -369,7 +374,8 @@ def main():
     ap.add_argument("--steps-fp64", type=int, default=2, help="timed steps of the plain FP64 path")
     ap.add_argument("--modes", default="fp64,fast64,mixed",
                     help="comma list of fp64, fast64, mixed; '+warm' appended to a fast mode starts every CG "
-                         "from the previous step's solution")
+                         "from the previous step's solution, '+mg' preconditions the elasticity and demag solves "
+                         "with the geometric multigrid")
     ap.add_argument("--rel-delta", type=float, default=0.3)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default SpMV launch schedules")
     ap.add_argument("--n", type=int, default=128)

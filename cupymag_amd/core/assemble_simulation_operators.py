@@ -77,6 +77,16 @@ class SimulationOperators(FEMMesh):
         self._dev = d
         return d
 
+    def multigrid(self, precision=32, degree=2, ratio=20.0):
+        """Multigrid hierarchies of A_demag and A_el (cached per precision),
+        or None when the mesh has none (cupymag_amd.solvers.multigrid)."""
+        cache = self.__dict__.setdefault("_mg", {})
+        key = (precision, degree, ratio)
+        if key not in cache:
+            from ..solvers.multigrid import build_hierarchies
+            cache[key] = build_hierarchies(self, precision, degree, ratio, nthreads=self.nthreads)
+        return cache[key]
+
     def csr_demag_and_deriv(self):
         d = self._upload()
         Fx, Fy, Fz = (DerivView(d["F3"], k) for k in range(3))

@@ -36,6 +36,7 @@ struct CgState {
   int32_t it, done, fail, maxiter;
   double maxr;      // mixed precision: residual norm at the last FP64 update
   int32_t upd, pad; // mixed precision: FP64 residual update requested
+  double beta;      // preconditioned CG: (r - r_old).z / rho_old
 };
 
 }  // namespace cpm
@@ -203,7 +204,7 @@ __device__ inline double group_sum(double s) {
 // CG scalars after a reduction (thread 0).  v[j]: the reduced value of
 // column j; only the columns in use (live, or with an update requested for
 // RED_RR_TRUE) are touched.
-enum : int { RED_ALPHA = 0, RED_RR = 1, RED_RR_REL = 2, RED_RR_TRUE = 3 };
+enum : int { RED_ALPHA = 0, RED_RR = 1, RED_RR_REL = 2, RED_RR_TRUE = 3, RED_RR_PCG = 4 };
 
 template <int MODE>
 __device__ inline bool cg_state_uses(const CgState& c) {
@@ -237,6 +238,16 @@ __device__ inline void cg_state_update(CgState* st, const double (&v)[K], double
       c.it += 1;
       c.resid = sqrt(v[j]);
       if (c.resid < delta * c.maxr || c.resid <= c.atol || c.it >= c.maxiter) c.upd = 1;
+    } else if (MODE == RED_RR_PCG) {
+      // preconditioned CG: the residual norm only (rho = r.z comes later)
+      c.it += 1;
+      c.resid = sqrt(v[j]);
+      if (c.resid <= c.atol) {
+        c.done = 1;
+      } else if (c.it >= c.maxiter) {
+        c.done = 1;
+        c.fail = 1;
+      }
     } else {  // RED_RR_TRUE: FP64 residual replaced the FP32 recurrence residual
       c.upd = 0;
       c.rho = v[j];

@@ -26,6 +26,7 @@
 
 #include "device.h"
 #include "multi.h"
+#include "mg.h"
 
 namespace cpm {
 
@@ -524,6 +525,15 @@ extern "C" int cpm_sim_create(cpm_ctx* ctx, const cpm_sim_ops* ops, const cpm_si
       return fail(CPM_EINVAL, "cpm_sim_create: fast modes need the bs-3 SELL form of A_el");
     if (prm->mode == CPM_MODE_MIXED && !(prm->rel_delta > 0.0 && prm->rel_delta < 1.0))
       return fail(CPM_EINVAL, "cpm_sim_create: rel_delta must be in (0, 1)");
+    if (prm->precond && !ops->mg_demag)
+      return fail(CPM_EINVAL, "cpm_sim_create: precond needs the multigrid hierarchy of A_demag");
+    if (prm->precond && prm->me && !ops->mg_el)
+      return fail(CPM_EINVAL, "cpm_sim_create: precond needs the multigrid hierarchy of A_el");
+    if (prm->precond && (mg_operator(ops->mg_demag) != ops->A_demag ||
+                         (prm->me && mg_operator(ops->mg_el) != ops->A_el)))
+      return fail(CPM_EINVAL, "cpm_sim_create: a multigrid hierarchy does not belong to its operator");
+  } else if (prm->precond) {
+    return fail(CPM_EINVAL, "cpm_sim_create: precond is a fast-mode option");
   }
   auto sq = [&](const cpm_mat* A, int64_t rows) { return A && A->nrows == rows && A->ncols == rows; };
   if (!sq(ops->A_demag, n) || !sq(ops->A1, n) || !sq(ops->A2, n) || !sq(ops->F1, n))
@@ -804,7 +814,9 @@ static int solve_one(cpm_sim* S, const SolveSpec& sp, int use_init, cpm_step_rep
   int hint = 0;
   for (int j = 0; j < sp.K; ++j) hint = std::max(hint, S->hint[sp.slot + j]);
   const int x0 = use_x0_fast(S, use_init);
-  int rc = (p.mode == CPM_MODE_MIXED)
+  cpm_mg* mg = !p.precond ? nullptr : (sp.slot == 0 ? S->ops.mg_el : sp.slot == 1 ? S->ops.mg_demag : nullptr);
+  int rc = mg ? mg_pcg(mg, bb, xx, x0, p.tol, p.maxiter, hint, it, res, &failed)
+           : (p.mode == CPM_MODE_MIXED)
                ? sell_cg_mixed(ctx, Am, sp.K, bb, xx, x0, p.tol, p.maxiter, p.rel_delta, hint, it, res, &failed,
                                nullptr)
                : sell_cg64(ctx, Am, sp.K, bb, xx, x0, p.tol, p.maxiter, hint, it, res, &failed);
@@ -916,6 +928,8 @@ extern "C" int cpm_batch_add(cpm_batch* B, cpm_sim* S) {
   if (S->ctx != B->ctx) return fail(CPM_EINVAL, "cpm_batch_add: the simulation uses another context");
   if (S->prm.mode == CPM_MODE_FP64)
     return fail(CPM_EINVAL, "cpm_batch_add: batches run the SELL paths (CPM_MODE_FAST64 or CPM_MODE_MIXED)");
+  if (S->prm.precond)
+    return fail(CPM_EINVAL, "cpm_batch_add: batched solves run plain CG (create the simulation without precond)");
   if (!B->sims.empty() && S->prm.mode != B->mode)
     return fail(CPM_EINVAL, "cpm_batch_add: every simulation of a batch uses the same mode");
   CPM_HIP(hipSetDevice(B->ctx->device));

@@ -1,0 +1,841 @@
+// Geometric multigrid preconditioner and preconditioned CG for the
+// anchored stiffness operators (elasticity, bs = 3, and demag, bs = 1) on
+// structured periodic Kuhn meshes.
+//
+// Levels: level l+1 is the Kuhn mesh of the even nodes of level l (half the
+// nodes per dimension), its operator assembled on that mesh by the host
+// assembler (rediscretisation).  The Kuhn (Freudenthal) triangulation of
+// cell size h refines the one of cell size 2h, so continuous P1 functions of
+// the coarse mesh are P1 functions of the fine mesh and the prolongation is
+// their nodal interpolation: fine node 2I + d, d in {0,1}^3, takes the
+// coarse value at I (d = 0) or the mean of the values at I and I + d (the
+// coarse edge in direction d), periodic in every dimension.  Restriction is
+// its transpose.  Node 0 carries the operators' anchor (identity row and
+// column): it is left out of both transfers, so corrections never touch it.
+//
+// Smoother: Chebyshev iteration of degree m on D^-1 A (D = diagonal blocks,
+// inverted once), on the interval [lmax/ratio, lmax] with lmax = 1.1 x a
+// power-iteration estimate; the same polynomial before and after the coarse
+// correction, and an exact (dense inverse) solve on the coarsest level, so
+// the V-cycle is a symmetric positive definite preconditioner.  It runs in
+// FP32 (precision 32, on the FP32 SELL form) or FP64.
+//
+// The outer solver is CG preconditioned with one V-cycle per iteration, in
+// FP64 on the FP64 SELL form: x, r, p, q in FP64, z from the V-cycle;
+// beta = (r - r_old).z / rho_old (flexible CG, robust to a preconditioner
+// applied in FP32), and CuPy's stopping rule on the recurrence residual:
+// ||r|| <= tol ||b|| after the update, at least one iteration, b = 0 gives 0.
+// Every kernel returns at once when the solve is done, so the host enqueues
+// batches of iterations and reads the state between them, as the other
+// solvers do.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "device.h"
+#include "sell_dev.h"
+#include "mg.h"
+
+struct cpm_mg;
+
+namespace cpm {
+
+struct MgLevel {
+  cpm_mat* A = nullptr;
+  int32_t nb = 0;           // nodes (block rows)
+  int32_t dims[3] = {0, 0, 0};
+  int64_t n = 0;            // nb * bs
+  void *x = nullptr, *r = nullptr, *b = nullptr, *d0 = nullptr, *d1 = nullptr;
+  void* dinv = nullptr;     // bs*bs per node: inverse diagonal block
+  double lmax = 0.0;        // power-iteration estimate of lambda_max(D^-1 A)
+};
+
+}  // namespace cpm
+
+struct cpm_mg {
+  cpm_ctx* ctx = nullptr;
+  int bs = 1, prec = 32, degree = 2;
+  double ratio = 20.0;
+  std::vector<cpm::MgLevel> L;
+  void* cinv = nullptr;     // coarsest dense inverse (T), ncoarse x ncoarse
+  int64_t ncoarse = 0;
+  // Chebyshev coefficients: c0[l] = 1/theta; for step k of level l,
+  // c1[8l + k] = rho_k rho_(k-1) and c2[8l + k] = 2 rho_k / delta
+  std::vector<double> c0, c1, c2;
+  // FP64 PCG vectors (fine level)
+  double *r = nullptr, *p = nullptr, *q = nullptr, *rold = nullptr;
+  std::vector<void*> allocs;
+};
+
+namespace cpm {
+
+// ---------------------------------------------------------------- kernels
+
+template <int BS, class T>
+__device__ __forceinline__ void apply_dinv(const T* __restrict__ dinv, int64_t node, const double (&v)[BS],
+                                           double (&out)[BS]) {
+#pragma unroll
+  for (int d = 0; d < BS; ++d) {
+    double s = 0.0;
+#pragma unroll
+    for (int e = 0; e < BS; ++e) s += (double)dinv[node * BS * BS + d * BS + e] * v[e];
+    out[d] = s;
+  }
+}
+
+// First Chebyshev step.  ZERO: x0 = 0, so r = b (FROM64: b is the FP64
+// outer residual, also stored as this level's b), d = c0 D^-1 r, x = d.
+// Otherwise r is the current residual: d = c0 D^-1 r, x += d.
+template <int BS, class T, bool ZERO, bool FROM64>
+__global__ void __launch_bounds__(kBlock) k_mg_first(int32_t nb, const double* __restrict__ b64, T* __restrict__ b,
+                                                     T* __restrict__ r, T* __restrict__ d, T* __restrict__ x,
+                                                     const T* __restrict__ dinv, double c0, const CgState* st) {
+  if (st->done) return;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nb; i += stride) {
+    double rv[BS], dv[BS];
+#pragma unroll
+    for (int c = 0; c < BS; ++c) {
+      if (ZERO) {
+        const double bv = FROM64 ? b64[i * BS + c] : (double)b[i * BS + c];
+        if (FROM64) b[i * BS + c] = (T)bv;
+        r[i * BS + c] = (T)bv;
+        rv[c] = (double)(T)bv;
+      } else {
+        rv[c] = (double)r[i * BS + c];
+      }
+    }
+    apply_dinv<BS, T>(dinv, i, rv, dv);
+#pragma unroll
+    for (int c = 0; c < BS; ++c) {
+      const T dn = (T)(c0 * dv[c]);
+      d[i * BS + c] = dn;
+      x[i * BS + c] = ZERO ? dn : (T)((double)x[i * BS + c] + (double)dn);
+    }
+  }
+}
+
+// MODE 0: Chebyshev step r -= A d_in, d_out = c1 d_in + c2 D^-1 r, x += d_out.
+// MODE 1: r -= A d_in only (residual after the last step).
+// MODE 2: r = b - A x.
+template <int BS, class T, int MODE, bool ST>
+__global__ void __launch_bounds__(kBlock) k_mg_sweep(int32_t nb, const int64_t* __restrict__ sptr,
+                                                     const int32_t* __restrict__ col, const T* __restrict__ val,
+                                                     const double* __restrict__ dg, GridArg ga,
+                                                     const T* __restrict__ din, T* __restrict__ dout,
+                                                     T* __restrict__ x, T* __restrict__ r, const T* __restrict__ b,
+                                                     const T* __restrict__ dinv, double c1, double c2,
+                                                     const CgState* st) {
+  if (st->done) return;
+  const T* src = (MODE == 2) ? x : din;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t row = (int64_t)blockIdx.x * kBlock + threadIdx.x; row < (int64_t)((nb + 63) & ~63); row += stride) {
+    double sum[BS], xo[BS];
+    sell_row<BS, 1, T, false, ST>(row, nb, sptr, col, val, dg, src, nullptr, ga, sum, xo);
+    if (row >= nb) continue;
+    double rv[BS];
+#pragma unroll
+    for (int c = 0; c < BS; ++c) {
+      rv[c] = (MODE == 2) ? (double)b[row * BS + c] - sum[c] : (double)r[row * BS + c] - sum[c];
+      r[row * BS + c] = (T)rv[c];
+      rv[c] = (double)(T)rv[c];
+    }
+    if (MODE == 0) {
+      double dv[BS];
+      apply_dinv<BS, T>(dinv, row, rv, dv);
+#pragma unroll
+      for (int c = 0; c < BS; ++c) {
+        const T dn = (T)(c1 * xo[c] + c2 * dv[c]);
+        dout[row * BS + c] = dn;
+        x[row * BS + c] = (T)((double)x[row * BS + c] + (double)dn);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ int64_t wrap_idx(int32_t i, int32_t j, int32_t k, const int32_t* dims) {
+  i += (i < 0) ? dims[0] : 0;
+  i -= (i >= dims[0]) ? dims[0] : 0;
+  j += (j < 0) ? dims[1] : 0;
+  j -= (j >= dims[1]) ? dims[1] : 0;
+  k += (k < 0) ? dims[2] : 0;
+  k -= (k >= dims[2]) ? dims[2] : 0;
+  return ((int64_t)i * dims[1] + j) * dims[2] + k;
+}
+
+struct Dims3 {
+  int32_t d[3];
+};
+
+// b_c = R r_f (transpose of the nested P1 interpolation), anchor excluded.
+template <int BS, class T>
+__global__ void __launch_bounds__(kBlock) k_mg_restrict(Dims3 fine, Dims3 coarse, const T* __restrict__ rf,
+                                                        T* __restrict__ bc, const CgState* st) {
+  if (st->done) return;
+  const int64_t nc = (int64_t)coarse.d[0] * coarse.d[1] * coarse.d[2];
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t I = (int64_t)blockIdx.x * kBlock + threadIdx.x; I < nc; I += stride) {
+    double acc[BS];
+#pragma unroll
+    for (int c = 0; c < BS; ++c) acc[c] = 0.0;
+    if (I != 0) {
+      const int32_t i = (int32_t)(I / ((int64_t)coarse.d[1] * coarse.d[2]));
+      const int32_t j = (int32_t)((I / coarse.d[2]) % coarse.d[1]);
+      const int32_t k = (int32_t)(I % coarse.d[2]);
+      const int64_t f0 = wrap_idx(2 * i, 2 * j, 2 * k, fine.d);
+#pragma unroll
+      for (int c = 0; c < BS; ++c) acc[c] = (double)rf[f0 * BS + c];
+#pragma unroll
+      for (int dd = 1; dd < 8; ++dd) {
+        const int32_t a = dd & 1, bb = (dd >> 1) & 1, cc = (dd >> 2) & 1;
+        const int64_t fp = wrap_idx(2 * i + a, 2 * j + bb, 2 * k + cc, fine.d);
+        const int64_t fm = wrap_idx(2 * i - a, 2 * j - bb, 2 * k - cc, fine.d);
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+          if (fp != 0) acc[c] += 0.5 * (double)rf[fp * BS + c];
+          if (fm != 0) acc[c] += 0.5 * (double)rf[fm * BS + c];
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < BS; ++c) bc[I * BS + c] = (T)acc[c];
+  }
+}
+
+// x_f += P e_c (nested P1 interpolation), anchor excluded.
+template <int BS, class T>
+__global__ void __launch_bounds__(kBlock) k_mg_prolong(Dims3 fine, Dims3 coarse, const T* __restrict__ ec,
+                                                       T* __restrict__ xf, const CgState* st) {
+  if (st->done) return;
+  const int64_t nf = (int64_t)fine.d[0] * fine.d[1] * fine.d[2];
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t f = (int64_t)blockIdx.x * kBlock + threadIdx.x; f < nf; f += stride) {
+    if (f == 0) continue;
+    const int32_t i = (int32_t)(f / ((int64_t)fine.d[1] * fine.d[2]));
+    const int32_t j = (int32_t)((f / fine.d[2]) % fine.d[1]);
+    const int32_t k = (int32_t)(f % fine.d[2]);
+    const int32_t a = i & 1, bb = j & 1, cc = k & 1;
+    const int32_t I = i >> 1, J = j >> 1, K = k >> 1;
+    const int64_t c0 = wrap_idx(I, J, K, coarse.d);
+    double v[BS];
+    if ((a | bb | cc) == 0) {
+#pragma unroll
+      for (int c = 0; c < BS; ++c) v[c] = (c0 != 0) ? (double)ec[c0 * BS + c] : 0.0;
+    } else {
+      const int64_t c1 = wrap_idx(I + a, J + bb, K + cc, coarse.d);
+#pragma unroll
+      for (int c = 0; c < BS; ++c)
+        v[c] = 0.5 * (((c0 != 0) ? (double)ec[c0 * BS + c] : 0.0) + ((c1 != 0) ? (double)ec[c1 * BS + c] : 0.0));
+    }
+#pragma unroll
+    for (int c = 0; c < BS; ++c) xf[f * BS + c] = (T)((double)xf[f * BS + c] + v[c]);
+  }
+}
+
+// x = Cinv b on the coarsest level: one wavefront per row.
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_mg_dense(int64_t n, const T* __restrict__ cinv, const T* __restrict__ b,
+                                                     T* __restrict__ x, const CgState* st) {
+  if (st->done) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t waves = (int64_t)gridDim.x * (kBlock / 64);
+  for (int64_t row = (int64_t)blockIdx.x * (kBlock / 64) + threadIdx.x / 64; row < n; row += waves) {
+    double s = 0.0;
+    for (int64_t c = lane; c < n; c += 64) s += (double)cinv[row * n + c] * (double)b[c];
+    s = group_sum<64>(s);
+    if (lane == 0) x[row] = (T)s;
+  }
+}
+
+// y = D^-1 (A x) (FP64 form) and partials of y.y: power iteration.
+template <int BS, bool ST>
+__global__ void __launch_bounds__(kBlock) k_mg_power(int32_t nb, const int64_t* __restrict__ sptr,
+                                                     const int32_t* __restrict__ col, const double* __restrict__ val,
+                                                     const double* __restrict__ dg, GridArg ga,
+                                                     const double* __restrict__ x, double* __restrict__ y,
+                                                     const double* __restrict__ dinv, double* partials) {
+  double acc[1] = {0.0};
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t row = (int64_t)blockIdx.x * kBlock + threadIdx.x; row < (int64_t)((nb + 63) & ~63); row += stride) {
+    double sum[BS], xo[BS], dv[BS];
+    sell_row<BS, 1, double, false, ST>(row, nb, sptr, col, val, dg, x, nullptr, ga, sum, xo);
+    if (row >= nb) continue;
+    apply_dinv<BS, double>(dinv, row, sum, dv);
+#pragma unroll
+    for (int c = 0; c < BS; ++c) {
+      y[row * BS + c] = dv[c];
+      acc[0] += dv[c] * dv[c];
+    }
+  }
+  block_partial<1>(acc, partials);
+}
+
+__global__ void __launch_bounds__(kBlock) k_scale(int64_t n, const double* __restrict__ y, double* __restrict__ x,
+                                                  double s) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) x[i] = s * y[i];
+}
+
+// ---- outer preconditioned CG (FP64) ------------------------------------------
+
+// q = A p (FP64 SELL form) and partials of p.q; PLAIN: y = A x, no partials.
+template <int BS, bool ST, bool PLAIN>
+__global__ void __launch_bounds__(kBlock) k_pcg_spmv(int32_t nb, const int64_t* __restrict__ sptr,
+                                                     const int32_t* __restrict__ col, const double* __restrict__ val,
+                                                     const double* __restrict__ dg, GridArg ga,
+                                                     const double* __restrict__ p, double* __restrict__ q,
+                                                     const CgState* st, double* partials) {
+  if (!PLAIN && st->done) return;
+  double acc[1] = {0.0};
+  sell_rows<BS, 1, double, !PLAIN, false, ST>(blockIdx.x, gridDim.x, nb, sptr, col, val, dg, p, nullptr, q, ga, acc);
+  if (!PLAIN) block_partial<1>(acc, partials);
+}
+
+// r = b - y (use_x0) or x = 0, r = b; partials of ||b||^2, ||r||^2.
+__global__ void __launch_bounds__(kBlock) k_pcg_init(int64_t n, const double* __restrict__ b, double* __restrict__ x,
+                                                     const double* __restrict__ y, double* __restrict__ r,
+                                                     int use_x0, double* partials) {
+  double acc[2] = {0.0, 0.0};
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const double bi = b[i];
+    double ri;
+    if (use_x0) {
+      ri = bi - y[i];
+    } else {
+      x[i] = 0.0;
+      ri = bi;
+    }
+    r[i] = ri;
+    acc[0] += bi * bi;
+    acc[1] += ri * ri;
+  }
+  block_partial<2>(acc, partials);
+}
+
+__global__ void __launch_bounds__(kRedBlock) k_pcg_red_init(int nblk, const double* __restrict__ partials,
+                                                            CgState* st, double tol, int32_t maxiter) {
+  __shared__ double lds[2 * (kRedBlock / 64)];
+  double v[2] = {0.0, 0.0};
+  for (int i = threadIdx.x; i < nblk; i += kRedBlock) {
+    v[0] += partials[i];
+    v[1] += partials[nblk + i];
+  }
+  block_sum_t<2, kRedBlock>(v, lds);
+  if (threadIdx.x == 0) cg_state_init<1, false>(st, v, tol, maxiter);
+}
+
+__global__ void __launch_bounds__(kBlock) k_pcg_zero_b(int64_t n, double* __restrict__ x, const CgState* st) {
+  if (st->bnorm2 != 0.0) return;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) x[i] = 0.0;
+}
+
+// x += alpha p, r_old = r, r -= alpha q; partials of r.r.
+__global__ void __launch_bounds__(kBlock) k_pcg_xr(int64_t n, double* __restrict__ x, double* __restrict__ r,
+                                                   double* __restrict__ rold, const double* __restrict__ p,
+                                                   const double* __restrict__ q, const CgState* st,
+                                                   double* partials) {
+  if (st->done) return;
+  const double a = st->alpha;
+  double acc[1] = {0.0};
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    x[i] += a * p[i];
+    const double ro = r[i];
+    rold[i] = ro;
+    const double rn = ro - a * q[i];
+    r[i] = rn;
+    acc[0] += rn * rn;
+  }
+  block_partial<1>(acc, partials);
+}
+
+// partials of r.z and r_old.z
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_pcg_dot(int64_t n, const double* __restrict__ r,
+                                                    const double* __restrict__ rold, const T* __restrict__ z,
+                                                    const CgState* st, double* partials) {
+  if (st->done) return;
+  const bool first = st->it == 0;
+  double acc[2] = {0.0, 0.0};
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const double zi = (double)z[i];
+    acc[0] += r[i] * zi;
+    if (!first) acc[1] += rold[i] * zi;
+  }
+  block_partial<2>(acc, partials);
+}
+
+// rho = r.z; beta = (r.z - r_old.z) / rho_old (flexible CG)
+__global__ void __launch_bounds__(kRedBlock) k_pcg_red_rz(int nblk, const double* __restrict__ partials,
+                                                          CgState* st) {
+  if (st->done) return;
+  __shared__ double lds[2 * (kRedBlock / 64)];
+  double v[2] = {0.0, 0.0};
+  for (int i = threadIdx.x; i < nblk; i += kRedBlock) {
+    v[0] += partials[i];
+    v[1] += partials[nblk + i];
+  }
+  block_sum_t<2, kRedBlock>(v, lds);
+  if (threadIdx.x == 0) {
+    CgState& c = *st;
+    c.rho_old = c.rho;
+    c.rho = v[0];
+    c.beta = (c.it == 0) ? 0.0 : (v[0] - v[1]) / c.rho_old;
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) k_pcg_pupd(int64_t n, const T* __restrict__ z, double* __restrict__ p,
+                                                     const CgState* st) {
+  if (st->done) return;
+  const bool first = st->it == 0;
+  const double beta = st->beta;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+    p[i] = first ? (double)z[i] : (double)z[i] + beta * p[i];
+}
+
+// ---------------------------------------------------------------- host
+
+static GridArg grid_arg(const Sell& S) { return GridArg{S.dims[0], S.dims[1], S.dims[2], S.nst, S.soff}; }
+
+template <class T>
+static const T* vals(const Sell& S) {
+  if constexpr (sizeof(T) == 8)
+    return S.v64;
+  else
+    return S.v32;
+}
+template <class T>
+static const double* diag(const Sell& S) {
+  if constexpr (sizeof(T) == 8)
+    return S.dg64;
+  else
+    return S.dgc;
+}
+
+template <int BS, class T, int MODE>
+static void sweep(cpm_mg* M, const MgLevel& Lv, const void* din, void* dout, double c1, double c2,
+                  const CgState* st) {
+  const Sell& S = Lv.A->sell;
+  hipStream_t s = M->ctx->stream;
+  const int g = sell_grid(S);
+  if (S.stencil)
+    k_mg_sweep<BS, T, MODE, true><<<g, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, vals<T>(S), diag<T>(S), grid_arg(S),
+                                                       (const T*)din, (T*)dout, (T*)Lv.x, (T*)Lv.r, (const T*)Lv.b,
+                                                       (const T*)Lv.dinv, c1, c2, st);
+  else
+    k_mg_sweep<BS, T, MODE, false><<<g, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, vals<T>(S), diag<T>(S),
+                                                        grid_arg(S), (const T*)din, (T*)dout, (T*)Lv.x, (T*)Lv.r,
+                                                        (const T*)Lv.b, (const T*)Lv.dinv, c1, c2, st);
+}
+
+// One V-cycle from level l: b of level 0 read from the FP64 vector b64.
+template <int BS, class T>
+static void vcycle(cpm_mg* M, size_t l, const double* b64, const CgState* st) {
+  MgLevel& Lv = M->L[l];
+  hipStream_t s = M->ctx->stream;
+  const int gn = grid_for(Lv.nb);
+  if (l + 1 == M->L.size()) {
+    const int g = (int)std::min<int64_t>(kMaxGrid, (Lv.n + 3) / 4);
+    k_mg_dense<T><<<g, kBlock, 0, s>>>(Lv.n, (const T*)M->cinv, (const T*)Lv.b, (T*)Lv.x, st);
+    return;
+  }
+  const int m = M->degree;
+  const double c0 = M->c0[l];
+  const double* c1 = &M->c1[l * 8];
+  const double* c2 = &M->c2[l * 8];
+  void* d[2] = {Lv.d0, Lv.d1};
+  // pre-smoothing from x = 0
+  if (l == 0 && b64)
+    k_mg_first<BS, T, true, true><<<gn, kBlock, 0, s>>>(Lv.nb, b64, (T*)Lv.b, (T*)Lv.r, (T*)d[0], (T*)Lv.x,
+                                                        (const T*)Lv.dinv, c0, st);
+  else
+    k_mg_first<BS, T, true, false><<<gn, kBlock, 0, s>>>(Lv.nb, nullptr, (T*)Lv.b, (T*)Lv.r, (T*)d[0], (T*)Lv.x,
+                                                         (const T*)Lv.dinv, c0, st);
+  for (int k = 1; k < m; ++k) sweep<BS, T, 0>(M, Lv, d[(k - 1) & 1], d[k & 1], c1[k], c2[k], st);
+  sweep<BS, T, 1>(M, Lv, d[(m - 1) & 1], nullptr, 0.0, 0.0, st);
+  // coarse-grid correction
+  MgLevel& Lc = M->L[l + 1];
+  const Dims3 fd{{Lv.dims[0], Lv.dims[1], Lv.dims[2]}}, cd{{Lc.dims[0], Lc.dims[1], Lc.dims[2]}};
+  k_mg_restrict<BS, T><<<grid_for(Lc.nb), kBlock, 0, s>>>(fd, cd, (const T*)Lv.r, (T*)Lc.b, st);
+  vcycle<BS, T>(M, l + 1, nullptr, st);
+  k_mg_prolong<BS, T><<<gn, kBlock, 0, s>>>(fd, cd, (const T*)Lc.x, (T*)Lv.x, st);
+  // post-smoothing: the same polynomial on the new residual
+  sweep<BS, T, 2>(M, Lv, nullptr, nullptr, 0.0, 0.0, st);
+  k_mg_first<BS, T, false, false><<<gn, kBlock, 0, s>>>(Lv.nb, nullptr, (T*)Lv.b, (T*)Lv.r, (T*)d[0], (T*)Lv.x,
+                                                        (const T*)Lv.dinv, c0, st);
+  for (int k = 1; k < m; ++k) sweep<BS, T, 0>(M, Lv, d[(k - 1) & 1], d[k & 1], c1[k], c2[k], st);
+}
+
+static void vcycle_any(cpm_mg* M, const double* b64, const CgState* st) {
+  if (M->bs == 3) {
+    if (M->prec == 32) vcycle<3, float>(M, 0, b64, st);
+    else vcycle<3, double>(M, 0, b64, st);
+  } else {
+    if (M->prec == 32) vcycle<1, float>(M, 0, b64, st);
+    else vcycle<1, double>(M, 0, b64, st);
+  }
+}
+
+template <int BS>
+static int power_lmax(cpm_mg* M, MgLevel& Lv, double* x, double* y, const double* dinv64, double* lmax) {
+  cpm_ctx* ctx = M->ctx;
+  hipStream_t s = ctx->stream;
+  const Sell& S = Lv.A->sell;
+  const int g = sell_grid(S);
+  std::vector<double> h(Lv.n);
+  uint64_t st = 0x9E3779B97F4A7C15ull;
+  for (auto& v : h) {
+    st ^= st << 13;
+    st ^= st >> 7;
+    st ^= st << 17;
+    v = 0.5 + (double)(st >> 11) * (1.0 / 9007199254740992.0);
+  }
+  for (int c = 0; c < BS; ++c) h[c] = 0.0;  // anchor node
+  CPM_HIP(hipMemcpyAsync(x, h.data(), sizeof(double) * Lv.n, hipMemcpyHostToDevice, s));
+  double nrm = 0.0;
+  for (double v : h) nrm += v * v;
+  nrm = std::sqrt(nrm);
+  k_scale<<<grid_for(Lv.n), kBlock, 0, s>>>(Lv.n, x, x, 1.0 / nrm);
+  double lam = 0.0;
+  for (int it = 0; it < 20; ++it) {
+    if (S.stencil)
+      k_mg_power<BS, true><<<g, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), x, y, dinv64,
+                                                ctx->partials);
+    else
+      k_mg_power<BS, false><<<g, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), x, y, dinv64,
+                                                 ctx->partials);
+    CPM_HIP(hipGetLastError());
+    std::vector<double> part(g);
+    CPM_HIP(hipMemcpyAsync(part.data(), ctx->partials, sizeof(double) * g, hipMemcpyDeviceToHost, s));
+    CPM_HIP(hipStreamSynchronize(s));
+    double ss = 0.0;
+    for (double v : part) ss += v;
+    lam = std::sqrt(ss);
+    if (!(lam > 0.0)) return fail(CPM_EINVAL, "multigrid: operator annihilates the power-iteration vector");
+    k_scale<<<grid_for(Lv.n), kBlock, 0, s>>>(Lv.n, y, x, 1.0 / lam);
+  }
+  *lmax = lam;
+  return CPM_OK;
+}
+
+// inverse of every diagonal block (host, FP64)
+static bool invert_blocks(int bs, int64_t nb, const std::vector<double>& dg, std::vector<double>& out) {
+  out.assign(dg.size(), 0.0);
+  for (int64_t i = 0; i < nb; ++i) {
+    const double* a = &dg[i * bs * bs];
+    double* o = &out[i * bs * bs];
+    if (bs == 1) {
+      if (a[0] == 0.0) return false;
+      o[0] = 1.0 / a[0];
+      continue;
+    }
+    const double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+                       a[2] * (a[3] * a[7] - a[4] * a[6]);
+    if (det == 0.0 || !std::isfinite(det)) return false;
+    const double id = 1.0 / det;
+    o[0] = (a[4] * a[8] - a[5] * a[7]) * id;
+    o[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+    o[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+    o[3] = (a[5] * a[6] - a[3] * a[8]) * id;
+    o[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+    o[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+    o[6] = (a[3] * a[7] - a[4] * a[6]) * id;
+    o[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+    o[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+  }
+  return true;
+}
+
+static int mg_alloc(cpm_mg* M, void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 8));
+  if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? CPM_ENOMEM : CPM_EHIP, hipGetErrorString(e));
+  M->allocs.push_back(*p);
+  return CPM_OK;
+}
+
+// Preconditioned CG on level 0 (see the file comment).
+template <int BS, class T>
+static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol, int maxiter, int hint,
+                   int32_t* iters, double* resid, int* failed) {
+  cpm_ctx* ctx = M->ctx;
+  hipStream_t s = ctx->stream;
+  MgLevel& L0 = M->L[0];
+  const Sell& S = L0.A->sell;
+  const int gs = sell_grid(S), gv = grid_for(L0.n);
+  CgState* st = ctx->cg;
+  auto spmv = [&](const double* in, double* out, bool plain) {
+    if (plain) {
+      if (S.stencil)
+        k_pcg_spmv<BS, true, true><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), in, out,
+                                                         st, ctx->partials);
+      else
+        k_pcg_spmv<BS, false, true><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), in,
+                                                          out, st, ctx->partials);
+    } else {
+      if (S.stencil)
+        k_pcg_spmv<BS, true, false><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), in,
+                                                          out, st, ctx->partials);
+      else
+        k_pcg_spmv<BS, false, false><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), in,
+                                                           out, st, ctx->partials);
+    }
+  };
+  // start: r = b - A x0 (or b), state, z = M r, rho = r.z
+  if (use_x0) spmv(x, M->q, true);
+  k_pcg_init<<<gv, kBlock, 0, s>>>(L0.n, b, x, M->q, M->r, use_x0, ctx->partials);
+  k_pcg_red_init<<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st, tol, maxiter);
+  if (use_x0) k_pcg_zero_b<<<gv, kBlock, 0, s>>>(L0.n, x, st);
+  vcycle<BS, T>(M, 0, M->r, st);
+  k_pcg_dot<T><<<gv, kBlock, 0, s>>>(L0.n, M->r, M->rold, (const T*)L0.x, st, ctx->partials);
+  k_pcg_red_rz<<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st);
+  CPM_HIP(hipGetLastError());
+  auto issue = [&](int todo) -> int {
+    for (int k = 0; k < todo; ++k) {
+      k_pcg_pupd<T><<<gv, kBlock, 0, s>>>(L0.n, (const T*)L0.x, M->p, st);
+      spmv(M->p, M->q, false);
+      k_red<1, RED_ALPHA><<<1, kRedBlock, 0, s>>>(gs, ctx->partials, st, 0.0);
+      k_pcg_xr<<<gv, kBlock, 0, s>>>(L0.n, x, M->r, M->rold, M->p, M->q, st, ctx->partials);
+      k_red<1, RED_RR_PCG><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st, 0.0);
+      vcycle<BS, T>(M, 0, M->r, st);
+      k_pcg_dot<T><<<gv, kBlock, 0, s>>>(L0.n, M->r, M->rold, (const T*)L0.x, st, ctx->partials);
+      k_pcg_red_rz<<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st);
+    }
+    CPM_HIP(hipGetLastError());
+    return CPM_OK;
+  };
+  CgState h{};
+  auto read = [&]() -> int {
+    CPM_HIP(hipMemcpyAsync(ctx->pinned, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+    CPM_HIP(hipStreamSynchronize(s));
+    std::memcpy(&h, ctx->pinned, sizeof(CgState));
+    return CPM_OK;
+  };
+  if (int rc = read()) return rc;
+  int batch = hint > 2 ? hint - 1 : 4;
+  int issued = 0;
+  while (!h.done && issued <= maxiter) {
+    const int todo = std::max(1, std::min(std::min(batch, 1024), maxiter - issued + 1));
+    if (int rc = issue(todo)) return rc;
+    issued += todo;
+    if (int rc = read()) return rc;
+    batch = std::max(2, std::min(16, h.it / 4 + 2));
+  }
+  *iters = h.it;
+  *resid = h.resid;
+  *failed = (h.fail || !h.done) ? 1 : 0;
+  return CPM_OK;
+}
+
+const cpm_mat* mg_operator(const cpm_mg* M) { return M && !M->L.empty() ? M->L[0].A : nullptr; }
+
+int mg_pcg(cpm_mg* M, const double* b, double* x, int use_x0, double tol, int maxiter, int hint, int32_t* iters,
+           double* resid, int* failed) {
+  if (M->bs == 3)
+    return M->prec == 32 ? pcg_run<3, float>(M, b, x, use_x0, tol, maxiter, hint, iters, resid, failed)
+                         : pcg_run<3, double>(M, b, x, use_x0, tol, maxiter, hint, iters, resid, failed);
+  return M->prec == 32 ? pcg_run<1, float>(M, b, x, use_x0, tol, maxiter, hint, iters, resid, failed)
+                       : pcg_run<1, double>(M, b, x, use_x0, tol, maxiter, hint, iters, resid, failed);
+}
+
+}  // namespace cpm
+
+using namespace cpm;
+
+extern "C" int cpm_mg_destroy(cpm_mg* M) {
+  if (!M) return CPM_OK;
+  if (M->ctx) {
+    (void)hipSetDevice(M->ctx->device);
+    (void)hipStreamSynchronize(M->ctx->stream);
+  }
+  for (void* p : M->allocs) (void)hipFree(p);
+  delete M;
+  return CPM_OK;
+}
+
+extern "C" int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, const double* coarse_inv,
+                             int precision, int degree, double ratio, cpm_mg** out) {
+  if (!ctx || !levels || !coarse_inv || !out) return fail(CPM_EINVAL, "cpm_mg_create: null argument");
+  if (nlevels < 2 || nlevels > 16) return fail(CPM_EINVAL, "cpm_mg_create: 2..16 levels");
+  if (precision != 32 && precision != 64) return fail(CPM_EINVAL, "cpm_mg_create: precision must be 32 or 64");
+  if (degree < 1 || degree > 8) return fail(CPM_EINVAL, "cpm_mg_create: Chebyshev degree must be 1..8");
+  if (!(ratio > 1.0)) return fail(CPM_EINVAL, "cpm_mg_create: eigenvalue ratio must be > 1");
+  const int bs = levels[0] ? levels[0]->sell.bs : 0;
+  for (int l = 0; l < nlevels; ++l) {
+    const cpm_mat* A = levels[l];
+    if (!A || !A->sell.sptr || A->sell.bs != bs || (bs != 1 && bs != 3))
+      return fail(CPM_EINVAL, "cpm_mg_create: every level needs a SELL form of one block size (1 or 3)");
+    if (!A->sell.stencil) return fail(CPM_EINVAL, "cpm_mg_create: levels must be structured periodic grids");
+    if (l > 0) {
+      const Sell& F = levels[l - 1]->sell;
+      const Sell& C = A->sell;
+      for (int a = 0; a < 3; ++a)
+        if (F.dims[a] != 2 * C.dims[a])
+          return fail(CPM_EINVAL, "cpm_mg_create: each level must halve the grid in every dimension");
+    }
+  }
+  CPM_HIP(hipSetDevice(ctx->device));
+  auto* M = new cpm_mg();
+  M->ctx = ctx;
+  M->bs = bs;
+  M->prec = precision;
+  M->degree = degree;
+  M->ratio = ratio;
+  const size_t ts = precision == 32 ? 4 : 8;
+  int rc = CPM_OK;
+  auto A = [&](void** p, size_t bytes) {
+    if (rc == CPM_OK) rc = mg_alloc(M, p, bytes);
+  };
+  hipStream_t s = ctx->stream;
+  double *px = nullptr, *py = nullptr, *dinv64 = nullptr;
+  for (int l = 0; l < nlevels && rc == CPM_OK; ++l) {
+    MgLevel Lv;
+    Lv.A = levels[l];
+    const Sell& S = Lv.A->sell;
+    Lv.nb = S.nbrows;
+    for (int a = 0; a < 3; ++a) Lv.dims[a] = S.dims[a];
+    Lv.n = (int64_t)Lv.nb * bs;
+    for (void** p : {&Lv.x, &Lv.r, &Lv.b, &Lv.d0, &Lv.d1}) A(p, ts * Lv.n);
+    A(&Lv.dinv, ts * Lv.nb * bs * bs);
+    if (rc != CPM_OK) break;
+    // inverse diagonal blocks: of the FP64 form, and of the (compensated) FP32 form
+    const int64_t nbb = (int64_t)Lv.nb * bs * bs;
+    std::vector<double> dg(nbb), inv64, inv_t;
+    hipError_t e = hipMemcpy(dg.data(), S.dg64, sizeof(double) * nbb, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+      rc = fail(CPM_EHIP, hipGetErrorString(e));
+      break;
+    }
+    if (!invert_blocks(bs, Lv.nb, dg, inv64)) {
+      rc = fail(CPM_EINVAL, "cpm_mg_create: singular diagonal block");
+      break;
+    }
+    if (precision == 32) {
+      e = hipMemcpy(dg.data(), S.dgc, sizeof(double) * nbb, hipMemcpyDeviceToHost);
+      if (e != hipSuccess || !invert_blocks(bs, Lv.nb, dg, inv_t)) {
+        rc = fail(CPM_EINVAL, "cpm_mg_create: singular diagonal block (FP32 form)");
+        break;
+      }
+      std::vector<float> f(inv_t.begin(), inv_t.end());
+      e = hipMemcpy(Lv.dinv, f.data(), sizeof(float) * nbb, hipMemcpyHostToDevice);
+    } else {
+      e = hipMemcpy(Lv.dinv, inv64.data(), sizeof(double) * nbb, hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+      rc = fail(CPM_EHIP, hipGetErrorString(e));
+      break;
+    }
+    if (l + 1 < nlevels) {
+      // lambda_max(D^-1 A) by power iteration on the FP64 form
+      void *vx = nullptr, *vy = nullptr, *vd = nullptr;
+      A(&vx, 8 * Lv.n);
+      A(&vy, 8 * Lv.n);
+      A(&vd, 8 * nbb);
+      if (rc != CPM_OK) break;
+      px = (double*)vx, py = (double*)vy, dinv64 = (double*)vd;
+      e = hipMemcpy(dinv64, inv64.data(), sizeof(double) * nbb, hipMemcpyHostToDevice);
+      if (e != hipSuccess) {
+        rc = fail(CPM_EHIP, hipGetErrorString(e));
+        break;
+      }
+      rc = bs == 3 ? power_lmax<3>(M, Lv, px, py, dinv64, &Lv.lmax) : power_lmax<1>(M, Lv, px, py, dinv64, &Lv.lmax);
+      // the scratch of the estimate is not needed afterwards
+      for (void* p : {vx, vy, vd}) {
+        (void)hipFree(p);
+        M->allocs.erase(std::find(M->allocs.begin(), M->allocs.end(), p));
+      }
+    }
+    M->L.push_back(Lv);
+  }
+  if (rc == CPM_OK) {
+    // Chebyshev coefficients per level
+    M->c0.assign(nlevels, 0.0);
+    M->c1.assign(nlevels * 8, 0.0);
+    M->c2.assign(nlevels * 8, 0.0);
+    for (int l = 0; l + 1 < nlevels; ++l) {
+      const double hi = 1.1 * M->L[l].lmax, lo = hi / ratio;
+      const double th = 0.5 * (hi + lo), de = 0.5 * (hi - lo), sg = th / de;
+      M->c0[l] = 1.0 / th;
+      double rho = 1.0 / sg;
+      for (int k = 1; k < degree; ++k) {
+        const double rn = 1.0 / (2.0 * sg - rho);
+        M->c1[l * 8 + k] = rn * rho;
+        M->c2[l * 8 + k] = 2.0 * rn / de;
+        rho = rn;
+      }
+    }
+    // coarsest dense inverse
+    const int64_t nc = M->L.back().n;
+    M->ncoarse = nc;
+    A(&M->cinv, ts * nc * nc);
+    if (rc == CPM_OK) {
+      hipError_t e;
+      if (precision == 32) {
+        std::vector<float> f(coarse_inv, coarse_inv + nc * nc);
+        e = hipMemcpy(M->cinv, f.data(), sizeof(float) * nc * nc, hipMemcpyHostToDevice);
+      } else {
+        e = hipMemcpy(M->cinv, coarse_inv, sizeof(double) * nc * nc, hipMemcpyHostToDevice);
+      }
+      if (e != hipSuccess) rc = fail(CPM_EHIP, hipGetErrorString(e));
+    }
+    const int64_t n0 = M->L[0].n;
+    for (void** p : {(void**)&M->r, (void**)&M->p, (void**)&M->q, (void**)&M->rold}) A(p, 8 * n0);
+    if (rc == CPM_OK) {
+      hipError_t e = hipMemsetAsync(M->rold, 0, 8 * n0, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      if (e != hipSuccess) rc = fail(CPM_EHIP, hipGetErrorString(e));
+    }
+  }
+  if (rc != CPM_OK) {
+    cpm_mg_destroy(M);
+    return rc;
+  }
+  *out = M;
+  return CPM_OK;
+}
+
+extern "C" int cpm_mg_info(const cpm_mg* M, int* nlevels, double* lmax, int max_levels) {
+  if (!M || !nlevels) return fail(CPM_EINVAL, "cpm_mg_info: null argument");
+  *nlevels = (int)M->L.size();
+  if (lmax)
+    for (int l = 0; l < std::min<int>(max_levels, (int)M->L.size()); ++l) lmax[l] = M->L[l].lmax;
+  return CPM_OK;
+}
+
+extern "C" int cpm_mg_apply(cpm_mg* M, const double* r, double* z) {
+  if (!M || !r || !z) return fail(CPM_EINVAL, "cpm_mg_apply: null argument");
+  cpm_ctx* ctx = M->ctx;
+  CPM_HIP(hipSetDevice(ctx->device));
+  CgState* st = ctx->cg + 15;  // a live state so that the gated kernels run
+  CgState h{};
+  CPM_HIP(hipMemcpy(st, &h, sizeof(CgState), hipMemcpyHostToDevice));
+  vcycle_any(M, r, st);
+  CPM_HIP(hipGetLastError());
+  const MgLevel& L0 = M->L[0];
+  if (M->prec == 64) {
+    CPM_HIP(hipMemcpyAsync(z, L0.x, 8 * L0.n, hipMemcpyDeviceToDevice, ctx->stream));
+  } else {
+    std::vector<float> f(L0.n);
+    std::vector<double> d(L0.n);
+    CPM_HIP(hipMemcpyAsync(f.data(), L0.x, 4 * L0.n, hipMemcpyDeviceToHost, ctx->stream));
+    CPM_HIP(hipStreamSynchronize(ctx->stream));
+    for (int64_t i = 0; i < L0.n; ++i) d[i] = f[i];
+    CPM_HIP(hipMemcpyAsync(z, d.data(), 8 * L0.n, hipMemcpyHostToDevice, ctx->stream));
+  }
+  CPM_HIP(hipStreamSynchronize(ctx->stream));
+  return CPM_OK;
+}
+
+extern "C" int cpm_cg_mg(cpm_mg* M, const double* b, double* x, int use_x0, double tol, int maxiter, int32_t* iters,
+                         double* resid) {
+  if (!M || !b || !x || !iters || !resid) return fail(CPM_EINVAL, "cpm_cg_mg: null argument");
+  CPM_HIP(hipSetDevice(M->ctx->device));
+  int failed = 0;
+  if (int rc = mg_pcg(M, b, x, use_x0, tol, maxiter, 0, iters, resid, &failed)) return rc;
+  return failed ? std::max(1, (int)*iters) : CPM_OK;
+}

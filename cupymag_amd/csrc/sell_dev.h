@@ -16,87 +16,101 @@ struct GridArg {
   const int32_t* soff;
 };
 
-// y = A x (ADDX32: A (x + x32)) over block rows of BS x BS blocks and K
-// interleaved right-hand sides, x[(c*BS + e)*K + j]; CG: acc[j] += x.y of
-// column j over the rows this thread owns.
+// sum = row `row` of A x (ADDX32: A (x + x32)), FP64 accumulation, for
+// BS x BS blocks and K interleaved right-hand sides x[(c*BS + e)*K + j];
+// xo = the row's own entries of x (+ x32).  Padding rows (row >= nbrows)
+// only stream their (zero) slots and leave xo unset.
+template <int BS, int K, class T, bool ADDX32, bool ST>
+__device__ __forceinline__ void sell_row(int64_t row, int32_t nbrows, const int64_t* __restrict__ sptr,
+                                         const int32_t* __restrict__ col, const T* __restrict__ val,
+                                         const double* __restrict__ dg, const T* __restrict__ x,
+                                         const float* __restrict__ x32, const GridArg& ga, double (&sum)[BS * K],
+                                         double (&xo)[BS * K]) {
+  constexpr int BB = BS * BS;
+  constexpr int NV = BS * K;
+  const int64_t s = row >> 6, l = row & 63;
+  const int64_t s0 = ST ? s * ga.nst : sptr[s];
+  const int64_t s1 = ST ? s0 + ga.nst : sptr[s + 1];
+  int32_t ri = 0, rj = 0, rk = 0;
+  if (ST && row < nbrows) {
+    const int32_t r = (int32_t)row, nyz = ga.ny * ga.nz;
+    ri = r / nyz;
+    rj = (r / ga.nz) % ga.ny;
+    rk = r % ga.nz;
+  }
+#pragma unroll
+  for (int v = 0; v < NV; ++v) sum[v] = 0.0;
+  // matrix planes are read once per launch: streaming (non-temporal) loads
+  // keep them from evicting the gathered vector entries; two slots per
+  // trip keep more loads in flight
+  auto slot_fma = [&](int64_t slot) {
+    int64_t c;
+    if (ST) {
+      // neighbour on the periodic grid (padding rows gather row 0)
+      const int32_t code = ga.soff[slot - s0];
+      int32_t i2 = ri + (code & 3) - 1, j2 = rj + ((code >> 2) & 3) - 1, k2 = rk + ((code >> 4) & 3) - 1;
+      i2 += (i2 < 0) ? ga.nx : 0;
+      i2 -= (i2 >= ga.nx) ? ga.nx : 0;
+      j2 += (j2 < 0) ? ga.ny : 0;
+      j2 -= (j2 >= ga.ny) ? ga.ny : 0;
+      k2 += (k2 < 0) ? ga.nz : 0;
+      k2 -= (k2 >= ga.nz) ? ga.nz : 0;
+      c = (row < nbrows) ? ((int64_t)i2 * ga.ny + j2) * ga.nz + k2 : 0;
+    } else {
+      c = __builtin_nontemporal_load(&col[slot * 64 + l]);
+    }
+    double xv[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      xv[v] = (double)x[c * NV + v];
+      if (ADDX32) xv[v] += (double)x32[c * NV + v];
+    }
+#pragma unroll
+    for (int d = 0; d < BS; ++d)
+#pragma unroll
+      for (int e = 0; e < BS; ++e) {
+        const double a = (double)__builtin_nontemporal_load(&val[(slot * BB + d * BS + e) * 64 + l]);
+#pragma unroll
+        for (int j = 0; j < K; ++j) sum[d * K + j] += a * xv[e * K + j];
+      }
+  };
+  int64_t slot = s0;
+  for (; slot + 1 < s1; slot += 2) {
+    slot_fma(slot);
+    slot_fma(slot + 1);
+  }
+  if (slot < s1) slot_fma(slot);
+  if (row < nbrows) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      xo[v] = (double)x[row * NV + v];
+      if (ADDX32) xo[v] += (double)x32[row * NV + v];
+    }
+#pragma unroll
+    for (int d = 0; d < BS; ++d)
+#pragma unroll
+      for (int e = 0; e < BS; ++e) {
+        const double a = dg[row * BB + d * BS + e];
+#pragma unroll
+        for (int j = 0; j < K; ++j) sum[d * K + j] += a * xo[e * K + j];
+      }
+  }
+}
+
+// y = A x (ADDX32: A (x + x32)) over the block rows of a workgroup range; CG:
+// acc[j] += x.y of column j over the rows this thread owns.
 template <int BS, int K, class T, bool CG, bool ADDX32, bool ST>
 __device__ __forceinline__ void sell_rows(int64_t lb, int64_t lg, int32_t nbrows, const int64_t* __restrict__ sptr,
                                           const int32_t* __restrict__ col, const T* __restrict__ val,
                                           const double* __restrict__ dg, const T* __restrict__ x,
                                           const float* __restrict__ x32, T* __restrict__ y, const GridArg& ga,
                                           double (&acc)[K]) {
-  constexpr int BB = BS * BS;
   constexpr int NV = BS * K;
   const int64_t stride = lg * kBlock;
   for (int64_t row = lb * kBlock + threadIdx.x; row < (int64_t)((nbrows + 63) & ~63); row += stride) {
-    const int64_t s = row >> 6, l = row & 63;
-    const int64_t s0 = ST ? s * ga.nst : sptr[s];
-    const int64_t s1 = ST ? s0 + ga.nst : sptr[s + 1];
-    int32_t ri = 0, rj = 0, rk = 0;
-    if (ST && row < nbrows) {
-      const int32_t r = (int32_t)row, nyz = ga.ny * ga.nz;
-      ri = r / nyz;
-      rj = (r / ga.nz) % ga.ny;
-      rk = r % ga.nz;
-    }
-    double sum[NV];  // FP64 accumulation for both value types
-#pragma unroll
-    for (int v = 0; v < NV; ++v) sum[v] = 0.0;
-    // matrix planes are read once per launch: streaming (non-temporal) loads
-    // keep them from evicting the gathered vector entries; two slots per
-    // trip keep more loads in flight
-    auto slot_fma = [&](int64_t slot) {
-      int64_t c;
-      if (ST) {
-        // neighbour on the periodic grid (padding rows gather row 0)
-        const int32_t code = ga.soff[slot - s0];
-        int32_t i2 = ri + (code & 3) - 1, j2 = rj + ((code >> 2) & 3) - 1, k2 = rk + ((code >> 4) & 3) - 1;
-        i2 += (i2 < 0) ? ga.nx : 0;
-        i2 -= (i2 >= ga.nx) ? ga.nx : 0;
-        j2 += (j2 < 0) ? ga.ny : 0;
-        j2 -= (j2 >= ga.ny) ? ga.ny : 0;
-        k2 += (k2 < 0) ? ga.nz : 0;
-        k2 -= (k2 >= ga.nz) ? ga.nz : 0;
-        c = (row < nbrows) ? ((int64_t)i2 * ga.ny + j2) * ga.nz + k2 : 0;
-      } else {
-        c = __builtin_nontemporal_load(&col[slot * 64 + l]);
-      }
-      double xv[NV];
-#pragma unroll
-      for (int v = 0; v < NV; ++v) {
-        xv[v] = (double)x[c * NV + v];
-        if (ADDX32) xv[v] += (double)x32[c * NV + v];
-      }
-#pragma unroll
-      for (int d = 0; d < BS; ++d)
-#pragma unroll
-        for (int e = 0; e < BS; ++e) {
-          const double a = (double)__builtin_nontemporal_load(&val[(slot * BB + d * BS + e) * 64 + l]);
-#pragma unroll
-          for (int j = 0; j < K; ++j) sum[d * K + j] += a * xv[e * K + j];
-        }
-    };
-    int64_t slot = s0;
-    for (; slot + 1 < s1; slot += 2) {
-      slot_fma(slot);
-      slot_fma(slot + 1);
-    }
-    if (slot < s1) slot_fma(slot);
+    double sum[NV], xo[NV];
+    sell_row<BS, K, T, ADDX32, ST>(row, nbrows, sptr, col, val, dg, x, x32, ga, sum, xo);
     if (row < nbrows) {
-      double xo[NV];
-#pragma unroll
-      for (int v = 0; v < NV; ++v) {
-        xo[v] = (double)x[row * NV + v];
-        if (ADDX32) xo[v] += (double)x32[row * NV + v];
-      }
-#pragma unroll
-      for (int d = 0; d < BS; ++d)
-#pragma unroll
-        for (int e = 0; e < BS; ++e) {
-          const double a = dg[row * BB + d * BS + e];
-#pragma unroll
-          for (int j = 0; j < K; ++j) sum[d * K + j] += a * xo[e * K + j];
-        }
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
         const T yv = (T)sum[v];

@@ -1,0 +1,127 @@
+"""Geometric multigrid for the elasticity and demag solves (cpm_mg_*).
+
+The reference solves both systems with unpreconditioned CG
+(solvers/linear_solvers.py:20-74, ``M=None``); their iteration counts grow
+with the mesh size (about 2 600 for the elasticity system and 700 for the
+demag system per step at 128^3).  On a structured periodic Kuhn mesh the
+operators have a natural hierarchy: the Kuhn mesh of the even nodes,
+assembled again by the host assembler (rediscretisation), down to <= 8
+cells per dimension, where a dense inverse finishes the V-cycle.
+``build_hierarchies`` builds it for a ``SimulationOperators`` object;
+``KuhnMultigrid.solve`` runs the preconditioned CG (same stopping rule as
+``solve_cg``).
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from .. import _lib
+from .._lib import check, lib, ptr
+from ..device import DeviceCSR, as_device
+from ..mesh.coarsen import coarse_kuhn, hierarchy_sizes, kuhn_cells
+from ..mesh.periodic import periodic_node_map, structured_dims
+
+
+class KuhnMultigrid:
+    """A multigrid hierarchy (levels[0] is the operator being solved)."""
+
+    def __init__(self, levels, coarse_dense, precision=32, degree=2, ratio=20.0):
+        self.levels = list(levels)
+        self.ctx = levels[0].ctx
+        self.precision = int(precision)
+        self.degree = int(degree)
+        self.ratio = float(ratio)
+        arr = (C.c_void_p * len(levels))(*[L.h.value for L in levels])
+        cinv = np.ascontiguousarray(coarse_dense, dtype=np.float64)
+        h = C.c_void_p()
+        check(lib().cpm_mg_create(self.ctx.h, len(levels), arr, ptr(cinv), self.precision, self.degree,
+                                  self.ratio, C.byref(h)), "cpm_mg_create")
+        self.h = h
+
+    def info(self):
+        n = C.c_int()
+        lm = (C.c_double * 16)()
+        check(lib().cpm_mg_info(self.h, C.byref(n), lm, 16), "cpm_mg_info")
+        return dict(levels=n.value, lmax=list(lm[:n.value]), dims=[getattr(L, "grid_dims", None) for L in self.levels])
+
+    def apply(self, r):
+        """One V-cycle: z = M^-1 r (vectors in the SELL order)."""
+        r = as_device(r, self.ctx)
+        z = torch.empty_like(r)
+        check(lib().cpm_mg_apply(self.h, ptr(r), ptr(z)), "cpm_mg_apply")
+        return z
+
+    def solve(self, b, x0=None, tol=1e-7, maxiter=5000, use_init=False):
+        """(x, iterations, residual) of the preconditioned CG; raises the
+        reference's error when maxiter is reached."""
+        b = as_device(b, self.ctx)
+        x = x0.clone() if (use_init and x0 is not None) else torch.empty_like(b)
+        it, res = C.c_int32(), C.c_double()
+        info = check(lib().cpm_cg_mg(self.h, ptr(b), ptr(x), int(bool(use_init and x0 is not None)), float(tol),
+                                     int(maxiter), C.byref(it), C.byref(res)), "cpm_cg_mg")
+        if info > 0:
+            raise RuntimeError(f"Error! CG did not converge. info={info}, residual={res.value}.")
+        return x, it.value, res.value
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib.alive():
+            _lib._lib.cpm_mg_destroy(self.h)
+            self.h = None
+
+
+def _interleave_perm(n, bs):
+    """perm[i*bs + d] = d*n + i: SoA (reference) order -> node-major order."""
+    i = np.arange(n)
+    return (np.arange(bs)[None, :] * n + i[:, None]).ravel()
+
+
+def build_hierarchies(SimOp, precision=32, degree=2, ratio=20.0, coarsest_max=8, nthreads=None):
+    """{"A_demag": KuhnMultigrid, "A_el": KuhnMultigrid (with coupling)} for
+    a SimulationOperators object on an n^3 Kuhn mesh with n even, or None
+    when the mesh has no such hierarchy (not a Kuhn mesh in kuhn_mesh
+    order, or n odd / too small to coarsen)."""
+    from ..physics.host_ops import HostOps, voigt_C
+    n = kuhn_cells(SimOp.node_coords_np, SimOp.elements_np)
+    if n is None or SimOp.grid_dims != (n, n, n):
+        return None
+    sizes = hierarchy_sizes(n, coarsest_max)
+    if len(sizes) < 2 or sizes[-1] > 10:
+        return None
+    P = SimOp.P
+    ctx = SimOp.csr_demag_and_deriv()[0].ctx
+    fine = {"A_demag": SimOp.csr_demag_and_deriv()[0]}
+    if P.ME:
+        fine["A_el"] = SimOp.csr_elasticity()[0]
+    Cv = voigt_C(P.c11, P.c12, P.c44, bool(P.rot111)) if P.ME else None
+    levels = {k: [v] for k, v in fine.items()}
+    dense = {}
+    coords = SimOp.node_coords_np
+    for li, m in enumerate(sizes[1:]):
+        coords, tets = coarse_kuhn(coords, 2 * m)
+        el = np.zeros((tets.shape[0], 5), dtype=np.int32)
+        el[:, :4] = tets
+        gid = periodic_node_map(coords, coords.min(axis=0), coords.max(axis=0))
+        ndof = int(gid.max()) + 1
+        dims = structured_dims(coords, gid)
+        if dims != (m, m, m):
+            return None
+        H = HostOps(coords, el, gid, ndof, None, SimOp.alpha1, SimOp.alpha2, Cv, nthreads)
+        last = li == len(sizes) - 2
+        for name, bs in (("A_demag", 1), ("A_el", 3)):
+            if name not in fine:
+                continue
+            ip, ix, v, shape = H.csr(name)
+            M = DeviceCSR(ip, ix, v, shape, ctx)
+            # the host arrays are views of the assembler's buffers, freed below
+            M._host = (ip.copy(), ix.copy(), np.array(v, copy=True))
+            M.grid_dims = dims
+            M.add_sell(bs, ip, ix, v, dims)
+            levels[name].append(M)
+            if last:
+                import scipy.sparse as sps
+                A = sps.csr_matrix((np.array(v), np.array(ix), np.array(ip)), shape=shape).toarray()
+                perm = _interleave_perm(ndof, bs)
+                dense[name] = np.linalg.inv(A[np.ix_(perm, perm)])
+        H.free()
+    return {name: KuhnMultigrid(levels[name], dense[name], precision, degree, ratio) for name in fine}

@@ -202,6 +202,37 @@ int cpm_build_e0(cpm_ctx* ctx, int32_t n, double lambda100, double lambda111, co
 int cpm_weighted_sum(cpm_ctx* ctx, int32_t n, int ncomp, const double* W, const double* f, double* out);
 
 /* ------------------------------------------------------------------ */
+/* multigrid-preconditioned CG (structured periodic Kuhn meshes)        */
+/* ------------------------------------------------------------------ */
+/* No counterpart in the reference, whose solve_cg passes M=None
+ * (linear_solvers.py:20,59-62): an optional preconditioner for the two
+ * slowly converging solves (elasticity, demag).  levels[0] is the operator
+ * (its index-free SELL form), levels[l+1] the same operator assembled on
+ * the Kuhn mesh of the even nodes of level l (half the nodes per
+ * dimension); the last level is solved with coarse_inv (dense n x n,
+ * row-major, in the SELL vector order: node-major interleaved for bs 3).
+ * Transfers: nested P1 interpolation and its transpose, node 0 (the
+ * anchor) excluded.  Smoother: Chebyshev of `degree` on D^-1 A over
+ * [lmax/ratio, lmax], lmax = 1.1 x a power-iteration estimate.  precision:
+ * 32 or 64, the arithmetic of the V-cycle (the outer CG is FP64).  The
+ * levels are borrowed and must outlive the hierarchy. */
+typedef struct cpm_mg cpm_mg;
+int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, const double* coarse_inv, int precision,
+                  int degree, double ratio, cpm_mg** out);
+int cpm_mg_destroy(cpm_mg* mg);
+/* levels and the lambda_max estimate of each level (host, up to max_levels) */
+int cpm_mg_info(const cpm_mg* mg, int* nlevels, double* lmax, int max_levels);
+/* z = one V-cycle applied to r (device pointers, SELL vector order); synchronises */
+int cpm_mg_apply(cpm_mg* mg, const double* r, double* z);
+/* CG on levels[0] preconditioned by one V-cycle per iteration, FP64, on
+ * the context's stream: CuPy's stopping rule on the recurrence residual
+ * (||r|| <= tol*||b|| after the update, at least one iteration, b = 0 ->
+ * x = 0), flexible beta = (r - r_old).z / rho_old.  Returns 0, or a
+ * positive iteration count when maxiter was reached. */
+int cpm_cg_mg(cpm_mg* mg, const double* b, double* x, int use_x0, double tol, int maxiter, int32_t* iters,
+              double* resid);
+
+/* ------------------------------------------------------------------ */
 /* device-resident GSPM LLG simulation (the hot loop)                  */
 /* ------------------------------------------------------------------ */
 typedef struct cpm_sim cpm_sim;
@@ -214,6 +245,8 @@ typedef struct cpm_sim_ops {
   const cpm_mat* F1;      /* M, defect rows removed (:83) */
   const cpm_mat* A_el;    /* 3n x 3n or NULL when magnetoelastic coupling is off */
   const cpm_mat* F_el;    /* 3n x 6n or NULL */
+  cpm_mg* mg_el;          /* multigrid hierarchy of A_el (used when params.precond) or NULL */
+  cpm_mg* mg_demag;       /* multigrid hierarchy of A_demag (used when params.precond) or NULL */
 } cpm_sim_ops;
 
 typedef struct cpm_sim_params {
@@ -237,7 +270,9 @@ typedef struct cpm_sim_params {
                                  (the reference does so only when use_init is set) */
   double rel_delta;           /* CPM_MODE_MIXED: FP64 residual update when the FP32 recurrence
                                  residual drops by this factor (reliable updates) */
-  int32_t reserved[4];
+  int32_t precond;            /* fast modes: 1 = the elasticity and demag solves run multigrid-
+                                 preconditioned CG (cpm_cg_mg) on the hierarchies in cpm_sim_ops */
+  int32_t reserved[3];
 } cpm_sim_params;
 
 #define CPM_MODE_FP64 0   /* plain FP64 path: CSR operators, the reference's sequence of solves */

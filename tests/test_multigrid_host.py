@@ -1,0 +1,95 @@
+"""Host side of the multigrid hierarchy: Kuhn-mesh detection, level sizes,
+and the transfer operator pinned by the Galerkin identity of nested P1
+spaces: for an undistorted Kuhn mesh, P^T K_fine P equals the operator
+assembled on the coarse mesh (scalar stiffness and elasticity, operators
+from the oracle's element matrices), which fails for a wrong stencil,
+weight or wrap-around."""
+import numpy as np
+import scipy.sparse as sp
+
+from cupymag_amd.mesh.coarsen import coarse_kuhn, hierarchy_sizes, kuhn_cells, prolongation
+from cupymag_amd.mesh.kuhn import kuhn_mesh
+from cupymag_amd.mesh.periodic import structured_dims
+from oracle import fem
+from oracle import mesh as om
+
+
+def _gid(coords):
+    return om.periodic_map(coords, om.tet_bounds(coords))
+
+
+def _elements(tets):
+    el = np.zeros((tets.shape[0], 5), dtype=np.int32)
+    el[:, :4] = tets
+    return el
+
+
+def _K(coords, tets):
+    el = _elements(tets)
+    gid = _gid(coords)
+    n = int(gid.max()) + 1
+    assert structured_dims(coords, gid) is not None
+    K, _, _ = fem.element_matrices(coords, el, want=("K",))
+    r, c, v = fem.scalar_coo(el, gid, K)
+    return sp.coo_matrix((v, (r, c)), shape=(n, n)).tocsr()
+
+
+def _Kel(coords, tets, Cv):
+    """elasticity stiffness, node-major interleaved (3i + d)"""
+    el = _elements(tets)
+    gid = _gid(coords)
+    n = int(gid.max()) + 1
+    Ke, _ = fem.elastic_element_matrices(coords, el, Cv)
+    g = gid[tets]                                   # (ne, 4)
+    # element DOF 3a + d (the oracle's B matrix order) -> global 3 gid_a + d
+    dof = (3 * g[:, :, None] + np.arange(3)[None, None, :]).reshape(len(tets), 12)
+    Kn = Ke
+    rows = np.repeat(dof[:, :, None], 12, axis=2).ravel()
+    cols = np.repeat(dof[:, None, :], 12, axis=1).ravel()
+    return sp.coo_matrix((Kn.ravel(), (rows, cols)), shape=(3 * n, 3 * n)).tocsr()
+
+
+def test_kuhn_detection_and_sizes():
+    coords, tets = kuhn_mesh(8)
+    assert kuhn_cells(coords, _elements(tets)) == 8
+    cj, tj = kuhn_mesh(8, jitter=0.15, seed=1)
+    assert kuhn_cells(cj, _elements(tj)) == 8
+    perm = np.random.default_rng(0).permutation(len(coords))
+    assert kuhn_cells(coords[perm], _elements(tets)) is None
+    assert hierarchy_sizes(128) == [128, 64, 32, 16, 8]
+    assert hierarchy_sizes(96) == [96, 48, 24, 12, 6]
+    assert hierarchy_sizes(90) == [90, 45]
+    C, T = coarse_kuhn(coords, 8)
+    assert np.array_equal(T, kuhn_mesh(4)[1])
+    np.testing.assert_array_equal(C, kuhn_mesh(4, h=2.0)[0])
+
+
+def test_prolongation_basics():
+    P = prolongation((8, 8, 8), anchor=False)
+    np.testing.assert_allclose(P @ np.ones(64), 1.0)
+    Pa = prolongation((8, 8, 8), anchor=True)
+    assert Pa[0].nnz == 0 and Pa[:, 0].nnz == 0
+    assert (P - Pa)[1:, 1:].nnz == 0
+
+
+def test_galerkin_identity_scalar():
+    cf, tf = kuhn_mesh(8)
+    cc, tc = coarse_kuhn(cf, 8)
+    Kf, Kc = _K(cf, tf), _K(cc, tc)
+    P = prolongation((8, 8, 8), anchor=False)
+    G = (P.T @ Kf @ P).toarray()
+    assert np.abs(G - Kc.toarray()).max() < 1e-12 * np.abs(Kc.toarray()).max()
+    # the identity is specific: a jittered fine mesh breaks it
+    cj, _ = kuhn_mesh(8, jitter=0.15, seed=2)
+    Gj = (P.T @ _K(cj, tf) @ P).toarray()
+    assert np.abs(Gj - _K(*coarse_kuhn(cj, 8)).toarray()).max() > 1e-6
+
+
+def test_galerkin_identity_elasticity():
+    Cv = fem.voigt_C(2.4e11, 1.4e11, 1.2e11) / 1e11
+    cf, tf = kuhn_mesh(8)
+    cc, tc = coarse_kuhn(cf, 8)
+    Kf, Kc = _Kel(cf, tf, Cv), _Kel(cc, tc, Cv)
+    P3 = prolongation((8, 8, 8), bs=3, anchor=False)
+    G = (P3.T @ Kf @ P3).toarray()
+    assert np.abs(G - Kc.toarray()).max() < 1e-12 * np.abs(Kc.toarray()).max()
