@@ -149,11 +149,21 @@ def run_gpu(n, steps, warmup, steps64, modes, rel_delta=0.3, autotune=True):
             if done == snap_at:
                 snap = sim.get("M")
         rows, nnz = A_el.shape[0], A_el.nnz
+        nb = rows // 3
         if base == "fp64":
             # CSR FP64: values + column indices + row pointers + p read + q written
             kernel = "k_cg_spmv (A_el, CSR FP64)"
             rocname = None
             bytes_launch = 12.0 * nnz + 4.0 * (rows + 1) + 16.0 * rows
+        elif "mg" in opts:
+            # level-0 Chebyshev step of the V-cycle (r -= A d, d = c1 d + c2 D^-1 r,
+            # x += d): the operator's planes (stencil form) and diagonal blocks,
+            # d gathered, r, x read and written, D^-1 (9 values) read, d written
+            s_ = 4 if base == "mixed" else 8
+            t_ = "float" if s_ == 4 else "double"
+            kernel = f"k_mg_sweep<3,{t_},0> (A_el level 0, Chebyshev step, SELL-64 3x3 blocks, stencil)"
+            rocname = f"k_mg_sweep<3, {t_}, 0, true>"
+            bytes_launch = (nnz / 9.0 - nb) * 9 * s_ + nb * 72.0 + nb * 27.0 * s_
         else:
             # SELL-64 3x3 blocks: off-diagonal blocks 9 values (+ 1 column index
             # unless the index-free stencil form is in use), diagonal blocks 9
@@ -165,7 +175,6 @@ def run_gpu(n, steps, warmup, steps64, modes, rel_delta=0.3, autotune=True):
             kernel = f"k_sell_spmv<3,1,{'double' if s_ == 8 else 'float'}> (A_el, SELL-64 3x3 blocks, {form})"
             rocname = (f"k_sell_spmv<3, 1, {'double' if s_ == 8 else 'float'}, true, false, false, "
                        f"{'true' if info['stencil'] else 'false'}>")
-            nb = rows // 3
             bytes_launch = (nnz / 9.0 - nb) * (9 * s_ + idx) + nb * 72.0 + 2.0 * rows * s_
         avg_s = ms.value / 1e3 / max(nl.value, 1)
         out["modes"][mode] = dict(elapsed=elapsed, steps=k_timed, lat=lat, iters=iters, m=snap,
@@ -175,20 +184,24 @@ def run_gpu(n, steps, warmup, steps64, modes, rel_delta=0.3, autotune=True):
     return out
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "cg_spmv_A_el_128cubed_hbm_traffic.md")
+PROFILES = os.path.join(ROOT, "profiles")
 
 
 def profiled_traffic(kernel, n):
-    """HBM bytes per launch of `kernel` at 128^3 from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel
-    calibration there); None for other kernels or sizes."""
-    if n != 128 or not os.path.exists(TRAFFIC_PROFILE):
+    """HBM bytes per launch of `kernel` at 128^3 from a committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary under profiles/ (scaled by the
+    copy-kernel calibration there); None for other kernels or sizes."""
+    if n != 128 or not os.path.isdir(PROFILES):
         return None
-    for line in open(TRAFFIC_PROFILE):
-        if line.startswith("traffic_bytes_per_launch "):
-            name, _, val = line[len("traffic_bytes_per_launch "):].strip().rpartition(" ")
-            if name == kernel:
-                return dict(bytes=float(val), source=os.path.relpath(TRAFFIC_PROFILE, ROOT))
+    for fn in sorted(os.listdir(PROFILES)):
+        if not fn.endswith(".md"):
+            continue
+        path = os.path.join(PROFILES, fn)
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch "):
+                name, _, val = line[len("traffic_bytes_per_launch "):].strip().rpartition(" ")
+                if name == kernel:
+                    return dict(bytes=float(val), source=os.path.relpath(path, ROOT))
     return None
 
 
@@ -372,7 +385,7 @@ def main():
     ap.add_argument("--steps", type=int, default=4, help="timed steps of the headline (mixed) path")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--steps-fp64", type=int, default=2, help="timed steps of the plain FP64 path")
-    ap.add_argument("--modes", default="fp64,fast64,mixed",
+    ap.add_argument("--modes", default="fp64,fast64,mixed,fast64+mg,mixed+mg",
                     help="comma list of fp64, fast64, mixed; '+warm' appended to a fast mode starts every CG "
                          "from the previous step's solution, '+mg' preconditions the elasticity and demag solves "
                          "with the geometric multigrid")
@@ -414,20 +427,27 @@ def main():
     steps64 = min(args.steps_fp64, args.steps) if "fp64" in modes else args.steps_fp64
     g = run_gpu(args.n, args.steps, args.warmup, steps64, modes, args.rel_delta, autotune=not args.no_autotune)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n, args.cpu_steps)
-    head = "mixed" if "mixed" in modes else modes[-1]
-    hm = g["modes"][head]
-    value = g["n_dof"] * hm["steps"] / hm["elapsed"]
-    lat = np.array(hm["lat"]) * 1e3
     per_mode = {}
     ref_m = g["modes"]["fp64"]["m"] if "fp64" in modes else None
     for mode, r in g["modes"].items():
         d = dict(node_steps_per_s=g["n_dof"] * r["steps"] / r["elapsed"], ms_per_step=r["elapsed"] * 1e3 / r["steps"],
-                 cg_iters_last_step=r["iters"][-1], spmv_A_el_GBs=r["spmv"]["gbs"],
-                 spmv_A_el_avg_us=r["spmv"]["avg_us"])
+                 cg_iters_last_step=r["iters"][-1], kernel=r["spmv"]["kernel"], kernel_GBs=r["spmv"]["gbs"],
+                 kernel_frac_of_hbm_peak=r["spmv"]["gbs"] / HBM_PEAK_GBS, kernel_avg_us=r["spmv"]["avg_us"])
         if ref_m is not None and r["m"] is not None:
             d["max_abs_m_err_vs_fp64"] = float((r["m"] - ref_m).abs().max().item())
             d["err_after_steps"] = args.warmup + steps64
         per_mode[mode] = d
+    if "fp64" in per_mode:
+        for d in per_mode.values():
+            d["speedup_vs_fp64"] = d["node_steps_per_s"] / per_mode["fp64"]["node_steps_per_s"]
+    # headline: the fastest mixed-precision path whose magnetization stays
+    # within 1e-7 of the plain FP64 path after the same steps
+    mixed = [m for m in modes if m.split("+")[0] == "mixed"]
+    ok = [m for m in mixed if per_mode[m].get("max_abs_m_err_vs_fp64", 1.0) <= 1e-7] if ref_m is not None else mixed
+    head = max(ok or mixed or modes, key=lambda m: per_mode[m]["node_steps_per_s"])
+    hm = g["modes"][head]
+    value = g["n_dof"] * hm["steps"] / hm["elapsed"]
+    lat = np.array(hm["lat"]) * 1e3
     sp = hm["spmv"]
     traffic = profiled_traffic(sp["rocname"], args.n) if sp.get("rocname") else None
     out = {
@@ -441,7 +461,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fast64": "f64", "mixed": "f32 CG in f64 refinement"}[head.split("+")[0]],
+        "dtype": ("f32 V-cycle/CG in f64 PCG/refinement" if head == "mixed+mg" else
+                  {"fp64": "f64", "fast64": "f64", "mixed": "f32 CG in f64 refinement"}[head.split("+")[0]]),
         "data": "synthetic",
         "config": {"workload": f"GSPM LLG step, {args.n}^3 Kuhn tet mesh, rotated-ellipsoid defect, "
                                f"magnetoelastic on, rot111, dt 10 ps, alpha 0.1, tol 1e-10",

@@ -435,8 +435,10 @@ static void sweep(cpm_mg* M, const MgLevel& Lv, const void* din, void* dout, dou
 }
 
 // One V-cycle from level l: b of level 0 read from the FP64 vector b64.
+// ev (profiling, may be null): events recorded around the first Chebyshev
+// step of level 0.
 template <int BS, class T>
-static void vcycle(cpm_mg* M, size_t l, const double* b64, const CgState* st) {
+static void vcycle(cpm_mg* M, size_t l, const double* b64, const CgState* st, hipEvent_t* ev = nullptr) {
   MgLevel& Lv = M->L[l];
   hipStream_t s = M->ctx->stream;
   const int gn = grid_for(Lv.nb);
@@ -457,7 +459,11 @@ static void vcycle(cpm_mg* M, size_t l, const double* b64, const CgState* st) {
   else
     k_mg_first<BS, T, true, false><<<gn, kBlock, 0, s>>>(Lv.nb, nullptr, (T*)Lv.b, (T*)Lv.r, (T*)d[0], (T*)Lv.x,
                                                          (const T*)Lv.dinv, c0, st);
-  for (int k = 1; k < m; ++k) sweep<BS, T, 0>(M, Lv, d[(k - 1) & 1], d[k & 1], c1[k], c2[k], st);
+  for (int k = 1; k < m; ++k) {
+    if (k == 1 && ev) (void)hipEventRecord(ev[0], s);
+    sweep<BS, T, 0>(M, Lv, d[(k - 1) & 1], d[k & 1], c1[k], c2[k], st);
+    if (k == 1 && ev) (void)hipEventRecord(ev[1], s);
+  }
   sweep<BS, T, 1>(M, Lv, d[(m - 1) & 1], nullptr, 0.0, 0.0, st);
   // coarse-grid correction
   MgLevel& Lc = M->L[l + 1];
@@ -595,6 +601,9 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
   k_pcg_dot<T><<<gv, kBlock, 0, s>>>(L0.n, M->r, M->rold, (const T*)L0.x, st, ctx->partials);
   k_pcg_red_rz<<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st);
   CPM_HIP(hipGetLastError());
+  // profiling: the first Chebyshev step of level 0 of every iteration is
+  // timed with HIP events and accumulated on the level-0 operator
+  const bool prof = ctx->prof && M->degree >= 2;
   auto issue = [&](int todo) -> int {
     for (int k = 0; k < todo; ++k) {
       k_pcg_pupd<T><<<gv, kBlock, 0, s>>>(L0.n, (const T*)L0.x, M->p, st);
@@ -602,7 +611,7 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
       k_red<1, RED_ALPHA><<<1, kRedBlock, 0, s>>>(gs, ctx->partials, st, 0.0);
       k_pcg_xr<<<gv, kBlock, 0, s>>>(L0.n, x, M->r, M->rold, M->p, M->q, st, ctx->partials);
       k_red<1, RED_RR_PCG><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st, 0.0);
-      vcycle<BS, T>(M, 0, M->r, st);
+      vcycle<BS, T>(M, 0, M->r, st, prof ? &ctx->ev[2 * k] : nullptr);
       k_pcg_dot<T><<<gv, kBlock, 0, s>>>(L0.n, M->r, M->rold, (const T*)L0.x, st, ctx->partials);
       k_pcg_red_rz<<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st);
     }
@@ -619,11 +628,24 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
   if (int rc = read()) return rc;
   int batch = hint > 2 ? hint - 1 : 4;
   int issued = 0;
+  cpm_mat* A0 = L0.A;
   while (!h.done && issued <= maxiter) {
-    const int todo = std::max(1, std::min(std::min(batch, 1024), maxiter - issued + 1));
+    const int todo = std::max(1, std::min(std::min(batch, prof ? 256 : 1024), maxiter - issued + 1));
+    const int it_before = h.it;
     if (int rc = issue(todo)) return rc;
     issued += todo;
     if (int rc = read()) return rc;
+    if (prof) {
+      // V-cycles that did work: every iteration of this batch except a
+      // converged last one
+      const int worked = std::min(todo, h.it - it_before - ((h.done && !h.fail) ? 1 : 0));
+      for (int k = 0; k < worked; ++k) {
+        float ms = 0.f;
+        CPM_HIP(hipEventElapsedTime(&ms, ctx->ev[2 * k], ctx->ev[2 * k + 1]));
+        A0->prof_ms += ms;
+        A0->prof_launches += 1;
+      }
+    }
     batch = std::max(2, std::min(16, h.it / 4 + 2));
   }
   *iters = h.it;
