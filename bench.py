@@ -342,6 +342,72 @@ def run_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def accuracy_config(seed=2025):
+    """The 64^3 accuracy workload: Kuhn tets with node jitter <= 0.15 h, a
+    spherical defect of radius 8 h at the centre, magnetoelastic coupling on,
+    dt 5 ps, alpha 0.5, tol 1e-10."""
+    from cupymag_amd.serving.workloads import instance_config
+    cfg = instance_config((5e4, 0.0, 0.0), me=True, defect_shape="Sphere", defect_params=8.0, dt=5e-12, alpha=0.5)
+    return cfg
+
+
+def run_accuracy(args):
+    """Per-field error of the fast paths against the plain FP64 path over a
+    long run (default 100 steps): m, u, E and Htilde after every 10 steps."""
+    import torch
+
+    from cupymag_amd.core.assemble_simulation_operators import SimulationOperators
+    from cupymag_amd.core.gspm import GSPMSimulation
+    from cupymag_amd.core.parameters import from_dict
+    from cupymag_amd.mesh.kuhn import kuhn_mesh
+
+    torch.cuda.set_device(0)
+    n = args.n if args.n != 128 else 64
+    P = from_dict(accuracy_config())
+    S = SimulationOperators(P, mesh_arrays=kuhn_mesh(n, jitter=0.15, seed=3), keep_host=False, autotune=True)
+    S.csr_demag_and_deriv()
+    m0 = initial_m(S.n_dof)
+    H = [P.Hext1, P.Hext2, P.Hext3]
+    every = 10
+    modes = [m for m in args.modes.split(",") if m] if args.modes != "fp64,fast64,mixed,fast64+mg,mixed+mg" else \
+        ["fp64", "fast64", "mixed", "mixed+mg"]
+    snaps, timing, iters = {}, {}, {}
+    for mode in modes:
+        base, *opts = mode.split("+")
+        sim = GSPMSimulation(S, P, mode=base, precond="mg" if "mg" in opts else None)
+        sim.set_m(m0)
+        snaps[mode] = []
+        t = time.perf_counter()
+        use_init = False
+        for k in range(args.steps):
+            rep = sim.step(H, use_init)
+            use_init = rep.sumsq < P.use_init_factor * S.n_dof * P.LLG_accuracy_factor
+            if (k + 1) % every == 0:
+                snaps[mode].append({f: sim.get(f).cpu() for f in ("M", "UEL", "E", "HTILDE")})
+        torch.cuda.synchronize()
+        timing[mode] = (time.perf_counter() - t) / args.steps
+        iters[mode] = rep.iters
+        log(f"[bench] accuracy {mode}: {args.steps} steps, {timing[mode] * 1e3:.1f} ms/step, last iters {rep.iters}")
+        del sim
+    ref = snaps["fp64"]
+    errors = {}
+    for mode in modes:
+        if mode == "fp64":
+            continue
+        rows = []
+        for k, (a, b) in enumerate(zip(snaps[mode], ref)):
+            rel = {f: float((a[f] - b[f]).abs().max() / b[f].abs().max().clamp_min(1e-300)) for f in ("UEL", "E", "HTILDE")}
+            rows.append(dict(step=(k + 1) * every, m_max_abs=float((a["M"] - b["M"]).abs().max()),
+                             u_rel=rel["UEL"], E_rel=rel["E"], Htilde_rel=rel["HTILDE"]))
+        errors[mode] = rows
+    out = {"metric": "max per-field error vs plain FP64", "unit": "abs (m) / rel (u, E, Htilde)",
+           "workload": f"{n}^3 Kuhn tets, node jitter 0.15 h, sphere defect r = 8 h, magnetoelastic on, "
+                       f"dt 5 ps, alpha 0.5, tol 1e-10, {args.steps} steps",
+           "n_dof": S.n_dof, "ms_per_step": {k: v * 1e3 for k, v in timing.items()}, "last_iters": iters,
+           "errors": errors}
+    print(json.dumps(out), flush=True)
+
+
 def run_serve(args):
     """Hysteresis-sweep requests arriving as a Poisson stream, served by
     the continuous-batching server; p50/p99 per-point and per-loop latency
@@ -378,9 +444,10 @@ def run_serve(args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=("gspm", "batch", "serve"), default="gspm",
+    ap.add_argument("--workload", choices=("gspm", "batch", "serve", "accuracy"), default="gspm",
                     help="gspm: one 128^3 simulation (the headline); batch: many independent simulations; "
-                         "serve: the hysteresis-sweep server")
+                         "serve: the hysteresis-sweep server; accuracy: per-field error of the fast paths "
+                         "against plain FP64 over a long run (64^3 jittered, --steps 100)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4, help="timed steps of the headline (mixed) path")
     ap.add_argument("--warmup", type=int, default=1)
@@ -423,6 +490,8 @@ def main():
         return run_batch(args)
     if args.workload == "serve":
         return run_serve(args)
+    if args.workload == "accuracy":
+        return run_accuracy(args)
     modes = [m for m in args.modes.split(",") if m]
     steps64 = min(args.steps_fp64, args.steps) if "fp64" in modes else args.steps_fp64
     g = run_gpu(args.n, args.steps, args.warmup, steps64, modes, args.rel_delta, autotune=not args.no_autotune)
