@@ -69,7 +69,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_gpu(n, steps, warmup, steps64, modes, rel_delta=0.3, autotune=True):
+def run_gpu(n, steps, warmup, steps64, modes, rel_delta=0.3, autotune=True, rel_delta_gs=None):
     """Run the plain FP64 path and the fast paths on the same operators and
     initial state; returns per-mode timings, CG iteration counts, the SpMV
     kernel timing of the elasticity operator and the magnetization after the
@@ -89,7 +89,7 @@ def run_gpu(n, steps, warmup, steps64, modes, rel_delta=0.3, autotune=True):
     P = from_dict(cfg)
     t0 = time.time()
     arrays = kuhn_mesh(n)
-    S = SimulationOperators(P, mesh_arrays=arrays, keep_host=False, autotune=autotune)
+    S = SimulationOperators(P, mesh_arrays=arrays, keep_host=False, autotune=autotune, matrix_free_fel=True)
     log(f"[bench] mesh {n}^3: {S.n_nodes} nodes, {S.n_elements} tets, {S.n_dof} DOFs, "
         f"{int(S.elements_np[:, -1].sum())} defect tets ({time.time() - t0:.1f}s)")
     t0 = time.time()
@@ -111,8 +111,9 @@ def run_gpu(n, steps, warmup, steps64, modes, rel_delta=0.3, autotune=True):
             t0 = time.time()
             S.multigrid(precision=32 if base == "mixed" else 64)
             log(f"[bench] multigrid hierarchies for {mode}: {time.time() - t0:.1f}s")
+        kw = {} if rel_delta_gs is None else dict(rel_delta_gs=rel_delta_gs)
         sim = GSPMSimulation(S, P, mode=base, warm_start=("warm" in opts), rel_delta=rel_delta,
-                             precond="mg" if "mg" in opts else None)
+                             precond="mg" if "mg" in opts else None, **kw)
         ctx = sim.ctx
         sim.set_m(m0)
         k_timed = steps64 if base == "fp64" else steps
@@ -246,7 +247,7 @@ def _build_instances(descs, ctx, workers=4):
     def one(d):
         torch.cuda.set_device(ctx.device)
         P = from_dict(d["cfg"])
-        S = SimulationOperators(P, mesh_arrays=kuhn_mesh(d["n"]), ctx=ctx, keep_host=False)
+        S = SimulationOperators(P, mesh_arrays=kuhn_mesh(d["n"]), ctx=ctx, keep_host=False, matrix_free_fel=True)
         S.csr_demag_and_deriv()
         return P, S
 
@@ -364,7 +365,8 @@ def run_accuracy(args):
     torch.cuda.set_device(0)
     n = args.n if args.n != 128 else 64
     P = from_dict(accuracy_config())
-    S = SimulationOperators(P, mesh_arrays=kuhn_mesh(n, jitter=0.15, seed=3), keep_host=False, autotune=True)
+    S = SimulationOperators(P, mesh_arrays=kuhn_mesh(n, jitter=0.15, seed=3), keep_host=False, autotune=True,
+                            matrix_free_fel=True)
     S.csr_demag_and_deriv()
     m0 = initial_m(S.n_dof)
     H = [P.Hext1, P.Hext2, P.Hext3]
@@ -457,6 +459,8 @@ def main():
                          "from the previous step's solution, '+mg' preconditions the elasticity and demag solves "
                          "with the geometric multigrid")
     ap.add_argument("--rel-delta", type=float, default=0.3)
+    ap.add_argument("--rel-delta-gs", type=float, default=None,
+                    help="reliable-update factor of the Gauss-Seidel solves (default 0.01)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default SpMV launch schedules")
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--cpu-n", type=int, default=40)
@@ -494,7 +498,8 @@ def main():
         return run_accuracy(args)
     modes = [m for m in args.modes.split(",") if m]
     steps64 = min(args.steps_fp64, args.steps) if "fp64" in modes else args.steps_fp64
-    g = run_gpu(args.n, args.steps, args.warmup, steps64, modes, args.rel_delta, autotune=not args.no_autotune)
+    g = run_gpu(args.n, args.steps, args.warmup, steps64, modes, args.rel_delta, autotune=not args.no_autotune,
+                rel_delta_gs=args.rel_delta_gs)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n, args.cpu_steps)
     per_mode = {}
     ref_m = g["modes"]["fp64"]["m"] if "fp64" in modes else None

@@ -32,7 +32,7 @@ class CpmError(RuntimeError):
 class cpm_sim_ops(C.Structure):
     _fields_ = [("A_demag", C.c_void_p), ("F3", C.c_void_p), ("A1", C.c_void_p), ("A2", C.c_void_p),
                 ("F1", C.c_void_p), ("A_el", C.c_void_p), ("F_el", C.c_void_p), ("mg_el", C.c_void_p),
-                ("mg_demag", C.c_void_p)]
+                ("mg_demag", C.c_void_p), ("fel", C.c_void_p)]
 
 
 class cpm_sim_params(C.Structure):
@@ -42,7 +42,7 @@ class cpm_sim_params(C.Structure):
                 ("Nd", C.c_double * 3), ("Ebar_sigma", C.c_double * 6), ("R", C.c_double * 9),
                 ("Mv", C.c_double * 36), ("MvinvT", C.c_double * 36), ("mode", C.c_int32),
                 ("warm_start", C.c_int32), ("rel_delta", C.c_double), ("precond", C.c_int32),
-                ("reserved", C.c_int32 * 3)]
+                ("reserved", C.c_int32 * 3), ("rel_delta_gs", C.c_double)]
 
 
 class cpm_step_report(C.Structure):
@@ -96,6 +96,9 @@ _SIGS = {
     "cpm_mg_info": (C.c_int, [_P, C.POINTER(C.c_int), _P, C.c_int]),
     "cpm_mg_apply": (C.c_int, [_P, _P, _P]),
     "cpm_cg_mg": (C.c_int, [_P, _P, _P, C.c_int, _D, C.c_int, C.POINTER(_I32), C.POINTER(_D)]),
+    "cpm_fel_create": (C.c_int, [_P, _I32, _P, _P, C.POINTER(_P)]),
+    "cpm_fel_destroy": (C.c_int, [_P]),
+    "cpm_fel_apply": (C.c_int, [_P, _P, _P]),
     "cpm_batch_create": (C.c_int, [_P, C.c_int, C.POINTER(_P)]),
     "cpm_batch_destroy": (C.c_int, [_P]),
     "cpm_batch_add": (C.c_int, [_P, _P]),

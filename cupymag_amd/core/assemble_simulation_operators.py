@@ -17,17 +17,21 @@ from ..utils.volume_average import VolumeAverage
 
 
 class SimulationOperators(FEMMesh):
-    def __init__(self, P, mesh_arrays=None, ctx=None, keep_host=None, nthreads=None, sell=True, autotune=False):
+    def __init__(self, P, mesh_arrays=None, ctx=None, keep_host=None, nthreads=None, sell=True, autotune=False,
+                 matrix_free_fel=False):
         """``sell``: also attach the SELL-64 forms the fast / mixed-precision
         solvers run on (A_demag, A1, A2 scalar; A_el in 3x3 blocks).
         ``autotune``: time the launch schedules of every operator after the
         upload and keep the fastest (cupymag_amd.autotune, cached per size
         class); ``self.tuning`` then maps operator names to their schedule
-        and timings."""
+        and timings.  ``matrix_free_fel``: on a structured Kuhn mesh, F_el is
+        not uploaded; ``csr_elasticity`` returns a matrix-free operator
+        (cupymag_amd.physics.fel) in its place."""
         super().__init__(P, mesh_arrays)
         self.sell = sell
         self.autotune = autotune
         self.tuning = {}
+        self.matrix_free_fel = matrix_free_fel
         self.alpha1 = P.alpha
         self.alpha2 = P.alpha * P.alpha_damp
         self.ctx = ctx
@@ -68,14 +72,25 @@ class SimulationOperators(FEMMesh):
         d["F1"] = up("F1")
         if P.ME:
             d["A_el"] = up("A_el", sell_bs=3)
-            d["F_el"] = up("F_el")
+            mf = self._matrix_free_fel(ctx, Cv) if self.matrix_free_fel else None
+            d["F_el"] = mf if mf is not None else up("F_el")
         H.free()
         if self.autotune:
             from ..autotune import tune
             for name, M in d.items():
-                self.tuning[name] = tune(M)
+                if isinstance(M, DeviceCSR):
+                    self.tuning[name] = tune(M)
         self._dev = d
         return d
+
+    def _matrix_free_fel(self, ctx, Cv):
+        """A MatrixFreeFel for an n^3 Kuhn mesh in generator order, else None."""
+        from ..mesh.coarsen import kuhn_cells
+        from ..physics.fel import MatrixFreeFel
+        n = kuhn_cells(self.node_coords_np, self.elements_np)
+        if n is None or self.grid_dims != (n, n, n):
+            return None
+        return MatrixFreeFel(self.node_coords_np, n, Cv, ctx)
 
     def multigrid(self, precision=32, degree=2, ratio=20.0):
         """Multigrid hierarchies of A_demag and A_el (cached per precision),

@@ -31,11 +31,13 @@ class StepReport:
 
 
 class GSPMSimulation:
-    def __init__(self, SimOp, P, mode="fp64", rel_delta=0.3, warm_start=False, precond=None):
+    def __init__(self, SimOp, P, mode="fp64", rel_delta=0.3, warm_start=False, precond=None, rel_delta_gs=0.01):
         """mode: "fp64" (CSR operators, the reference's sequence of solves),
         "fast64" (SELL-64 operators, 3-RHS solves) or "mixed" (as fast64 with
         FP32 CG refined in FP64 by reliable updates: an FP64 residual update
-        whenever the FP32 recurrence residual has dropped by ``rel_delta``).
+        whenever the FP32 recurrence residual has dropped by ``rel_delta``;
+        ``rel_delta_gs`` sets it for the Gauss-Seidel solves alone: these
+        converge fast, so fewer FP64 updates suffice).
         ``warm_start`` (fast modes): every CG starts from the previous step's
         solution instead of zero; the reference does so only when use_init
         is set, so results differ at the level of the CG tolerance.
@@ -65,8 +67,11 @@ class GSPMSimulation:
                                  "dimension")
         mg_el = self.mg["A_el"].h if (self.mg and "A_el" in self.mg) else None
         mg_dm = self.mg["A_demag"].h if self.mg else None
+        from ..physics.fel import MatrixFreeFel
+        mf = F_el if isinstance(F_el, MatrixFreeFel) else None
         ops = _lib.cpm_sim_ops(A_demag.h, F3.h, A1.h, A2.h, F1.h, A_el.h if A_el is not None else None,
-                               F_el.h if F_el is not None else None, mg_el, mg_dm)
+                               F_el.h if (F_el is not None and mf is None) else None, mg_el, mg_dm,
+                               mf.h if mf is not None else None)
         pr = _lib.cpm_sim_params()
         pr.n = n
         pr.me = int(bool(P.ME))
@@ -95,6 +100,7 @@ class GSPMSimulation:
             pr.MvinvT[k] = float(v)
         pr.mode = _lib.MODES[mode]
         pr.rel_delta = float(rel_delta)
+        pr.rel_delta_gs = float(rel_delta_gs or 0.0)
         if warm_start and mode == "fp64":
             raise ValueError("warm_start is a fast-mode option")
         pr.warm_start = int(bool(warm_start))

@@ -1,0 +1,213 @@
+// Matrix-free magnetoelastic right-hand side b = F_el E0 on structured
+// periodic Kuhn meshes.
+//
+// F_el is the operator of assemble_elasticity.py:268-348,507-527: for every
+// tetrahedron e and local nodes a, b, F_e[3a+d][6b+v] = sum_q (B_a^T C)[d][v]
+// N_q[b] detJ w_q, rows gid_a + d*n, columns 6 gid_b + v, rows {0, n, 2n}
+// zeroed.  Its CSR form holds 270 entries per row triple (6.8 GB at 128^3)
+// and is read once per step; here each DOF thread walks the 24 tetrahedra
+// around its vertex (8 cubes of the periodic grid, 6 Kuhn tetrahedra each),
+// recomputes the element geometry from the node coordinates exactly as the
+// host assembler does (tet Jacobian rows = edge vectors, signed detJ, zero
+// element when |detJ| < 1e-14, the 4-point table N_q with weights 1/24), and
+// accumulates its three rows.  No atomics: every output is written by one
+// thread in a fixed order, so the result is deterministic; it differs from
+// the CSR product only in summation order.
+//
+// Mesh: kuhn_mesh order, node (i,j,k) of the (n+1)^3 grid at index
+// (i*(n+1) + j)*(n+1) + k; cube c has the tetrahedra of the 6 axis
+// permutations pi: vertices c, c + e_pi0, c + e_pi0 + e_pi1, c + (1,1,1),
+// the last two swapped when that order is negatively oriented; DOF of a
+// vertex = its indices mod n.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "device.h"
+
+struct cpm_fel {
+  cpm_ctx* ctx = nullptr;
+  int32_t n = 0;             // cells per dimension
+  double* coords = nullptr;  // (n+1)^3 x 3
+  double C[36];
+};
+
+namespace cpm {
+
+struct FelArgs {
+  int32_t n;
+  const double* coords;
+  double C[36];
+};
+
+// local vertex offsets (bit 0: x, bit 1: y, bit 2: z) of the 6 Kuhn
+// tetrahedra of a cube, in kuhn_mesh order, orientation fixed
+struct KuhnTets {
+  int8_t v[6][4];
+};
+
+static KuhnTets kuhn_tets() {
+  KuhnTets K{};
+  const int perms[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+  for (int p = 0; p < 6; ++p) {
+    int off[4][3] = {{0, 0, 0}};
+    for (int s = 0; s < 3; ++s) {
+      for (int a = 0; a < 3; ++a) off[s + 1][a] = off[s][a];
+      off[s + 1][perms[p][s]] += 1;
+    }
+    // orientation: det of the rows off[1..3] - off[0]
+    const int(*o)[3] = off;
+    const int det = o[1][0] * (o[2][1] * o[3][2] - o[2][2] * o[3][1]) - o[1][1] * (o[2][0] * o[3][2] - o[2][2] * o[3][0]) +
+                    o[1][2] * (o[2][0] * o[3][1] - o[2][1] * o[3][0]);
+    int order[4] = {0, 1, 2, 3};
+    if (det < 0) order[2] = 3, order[3] = 2;
+    for (int a = 0; a < 4; ++a) {
+      const int* q = off[order[a]];
+      K.v[p][a] = (int8_t)(q[0] | (q[1] << 1) | (q[2] << 2));
+    }
+  }
+  return K;
+}
+
+__global__ void __launch_bounds__(kBlock) k_fel_mf(FelArgs fa, KuhnTets kt, const double* __restrict__ E0,
+                                                   double* __restrict__ b) {
+  const int32_t n = fa.n;
+  const int64_t N = (int64_t)n * n * n;
+  const int64_t n1 = n + 1;
+  const double a7 = 0.5854102, b7 = 0.1381966;  // N_q[b] of the 4-point rule (7-digit table)
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t dof = (int64_t)blockIdx.x * kBlock + threadIdx.x; dof < N; dof += stride) {
+    const int32_t i = (int32_t)(dof / ((int64_t)n * n)), j = (int32_t)((dof / n) % n), k = (int32_t)(dof % n);
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int o = 0; o < 8; ++o) {
+      // cube whose local vertex l is (i,j,k): l = 1 - o (per axis)
+      const int lx = 1 - (o & 1), ly = 1 - ((o >> 1) & 1), lz = 1 - ((o >> 2) & 1);
+      int32_t ci = i - lx, cj = j - ly, ck = k - lz;
+      ci += (ci < 0) ? n : 0;
+      cj += (cj < 0) ? n : 0;
+      ck += (ck < 0) ? n : 0;
+      const int lcode = lx | (ly << 1) | (lz << 2);
+      for (int t = 0; t < 6; ++t) {
+        int a = -1;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (kt.v[t][q] == lcode) a = q;
+        if (a < 0) continue;
+        double x[4][3];
+        int64_t gdof[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int c = kt.v[t][q];
+          const int64_t vi = ci + (c & 1), vj = cj + ((c >> 1) & 1), vk = ck + ((c >> 2) & 1);
+          const int64_t node = (vi * n1 + vj) * n1 + vk;
+#pragma unroll
+          for (int d = 0; d < 3; ++d) x[q][d] = fa.coords[node * 3 + d];
+          gdof[q] = ((vi % n) * n + (vj % n)) * n + (vk % n);
+        }
+        double J[3][3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) J[r][c] = x[r + 1][c] - x[0][c];
+        const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+        const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+        const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+        const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+        if (fabs(det) < 1e-14) continue;
+        const double id = 1.0 / det;
+        const double inv[3][3] = {
+            {c00 * id, (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id, (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id},
+            {c01 * id, (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id, (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id},
+            {c02 * id, (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id, (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id}};
+        // reference gradient of local node a: (-1,-1,-1) for a = 0, e_(a-1) otherwise
+        double dref[3] = {a == 0 ? -1.0 : (a == 1 ? 1.0 : 0.0), a == 0 ? -1.0 : (a == 2 ? 1.0 : 0.0),
+                          a == 0 ? -1.0 : (a == 3 ? 1.0 : 0.0)};
+        double g[3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) g[r] = inv[r][0] * dref[0] + inv[r][1] * dref[1] + inv[r][2] * dref[2];
+        // B_a^T C (3 x 6)
+        const double dx = g[0], dy = g[1], dz = g[2];
+        double BtC[3][6];
+#pragma unroll
+        for (int v = 0; v < 6; ++v) {
+          const double* Cc = fa.C;
+          BtC[0][v] = dx * Cc[0 * 6 + v] + dy * Cc[3 * 6 + v] + dz * Cc[5 * 6 + v];
+          BtC[1][v] = dy * Cc[1 * 6 + v] + dx * Cc[3 * 6 + v] + dz * Cc[4 * 6 + v];
+          BtC[2][v] = dz * Cc[2 * 6 + v] + dy * Cc[4 * 6 + v] + dx * Cc[5 * 6 + v];
+        }
+        // sum_q sum_b N_q[b] E0_b, then x detJ / 24
+        double Eb[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int bb = 0; bb < 4; ++bb) {
+            const double w = (q == bb) ? a7 : b7;
+#pragma unroll
+            for (int v = 0; v < 6; ++v) Eb[v] += w * E0[gdof[bb] * 6 + v];
+          }
+        const double dw = det * (1.0 / 24.0);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          double s = 0.0;
+#pragma unroll
+          for (int v = 0; v < 6; ++v) s += BtC[d][v] * Eb[v];
+          acc[d] += s * dw;
+        }
+      }
+    }
+    for (int d = 0; d < 3; ++d) b[d * N + dof] = (dof == 0) ? 0.0 : acc[d];
+  }
+}
+
+int fel_apply(const cpm_fel* F, const double* E0, double* b) {
+  FelArgs fa;
+  fa.n = F->n;
+  fa.coords = F->coords;
+  std::copy(F->C, F->C + 36, fa.C);
+  static const KuhnTets kt = kuhn_tets();
+  const int64_t N = (int64_t)F->n * F->n * F->n;
+  k_fel_mf<<<grid_for(N), kBlock, 0, F->ctx->stream>>>(fa, kt, E0, b);
+  CPM_HIP(hipGetLastError());
+  return CPM_OK;
+}
+
+int64_t fel_rows(const cpm_fel* F) { return 3 * (int64_t)F->n * F->n * F->n; }
+
+}  // namespace cpm
+
+using namespace cpm;
+
+extern "C" int cpm_fel_create(cpm_ctx* ctx, int32_t n, const double* coords, const double* C, cpm_fel** out) {
+  if (!ctx || !coords || !C || !out) return fail(CPM_EINVAL, "cpm_fel_create: null argument");
+  if (n < 2) return fail(CPM_EINVAL, "cpm_fel_create: at least 2 cells per dimension");
+  CPM_HIP(hipSetDevice(ctx->device));
+  auto* F = new cpm_fel();
+  F->ctx = ctx;
+  F->n = n;
+  std::copy(C, C + 36, F->C);
+  const size_t bytes = sizeof(double) * 3 * (size_t)(n + 1) * (n + 1) * (n + 1);
+  hipError_t e = hipMalloc(&F->coords, bytes);
+  if (e == hipSuccess) e = hipMemcpy(F->coords, coords, bytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    if (F->coords) (void)hipFree(F->coords);
+    delete F;
+    return fail(e == hipErrorOutOfMemory ? CPM_ENOMEM : CPM_EHIP, hipGetErrorString(e));
+  }
+  *out = F;
+  return CPM_OK;
+}
+
+extern "C" int cpm_fel_destroy(cpm_fel* F) {
+  if (!F) return CPM_OK;
+  (void)hipSetDevice(F->ctx->device);
+  (void)hipStreamSynchronize(F->ctx->stream);
+  if (F->coords) (void)hipFree(F->coords);
+  delete F;
+  return CPM_OK;
+}
+
+extern "C" int cpm_fel_apply(cpm_fel* F, const double* E0, double* b) {
+  if (!F || !E0 || !b) return fail(CPM_EINVAL, "cpm_fel_apply: null argument");
+  CPM_HIP(hipSetDevice(F->ctx->device));
+  return fel_apply(F, E0, b);
+}

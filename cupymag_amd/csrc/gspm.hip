@@ -28,6 +28,12 @@
 #include "multi.h"
 #include "mg.h"
 
+struct cpm_fel;
+namespace cpm {
+int fel_apply(const cpm_fel* F, const double* E0, double* b);
+int64_t fel_rows(const cpm_fel* F);
+}  // namespace cpm
+
 namespace cpm {
 
 struct FieldPrm {
@@ -525,6 +531,8 @@ extern "C" int cpm_sim_create(cpm_ctx* ctx, const cpm_sim_ops* ops, const cpm_si
       return fail(CPM_EINVAL, "cpm_sim_create: fast modes need the bs-3 SELL form of A_el");
     if (prm->mode == CPM_MODE_MIXED && !(prm->rel_delta > 0.0 && prm->rel_delta < 1.0))
       return fail(CPM_EINVAL, "cpm_sim_create: rel_delta must be in (0, 1)");
+    if (prm->mode == CPM_MODE_MIXED && !(prm->rel_delta_gs >= 0.0 && prm->rel_delta_gs < 1.0))
+      return fail(CPM_EINVAL, "cpm_sim_create: rel_delta_gs must be in [0, 1)");
     if (prm->precond && !ops->mg_demag)
       return fail(CPM_EINVAL, "cpm_sim_create: precond needs the multigrid hierarchy of A_demag");
     if (prm->precond && prm->me && !ops->mg_el)
@@ -542,8 +550,11 @@ extern "C" int cpm_sim_create(cpm_ctx* ctx, const cpm_sim_ops* ops, const cpm_si
   if (ops->F3->nrows != n) return fail(CPM_EINVAL, "cpm_sim_create: F3 must be n x n");
   if (prm->me) {
     if (!sq(ops->A_el, 3 * n)) return fail(CPM_EINVAL, "cpm_sim_create: A_el must be 3n x 3n");
-    if (!ops->F_el || ops->F_el->nrows != 3 * n || ops->F_el->ncols != 6 * n)
+    if (ops->fel) {
+      if (fel_rows(ops->fel) != 3 * n) return fail(CPM_EINVAL, "cpm_sim_create: matrix-free F_el of another mesh");
+    } else if (!ops->F_el || ops->F_el->nrows != 3 * n || ops->F_el->ncols != 6 * n) {
       return fail(CPM_EINVAL, "cpm_sim_create: F_el must be 3n x 6n");
+    }
   }
   CPM_HIP(hipSetDevice(ctx->device));
   auto* S = new cpm_sim();
@@ -705,7 +716,11 @@ static int stage_pre(cpm_sim* S, int stage, const FieldPrm& P, SolveSpec& sp) {
     case 0:  // Micromagnetics.py:112-125
       if (!p.me) return CPM_OK;
       k_e0<<<gv, kBlock, 0, s>>>(n, S->m, p.lambda100, p.lambda111, p.rot111, dR, dMv, S->E0);
-      if ((rc = launch_spmv(ctx, S->ops.F_el, 0, S->E0, S->bel))) return rc;
+      if (S->ops.fel) {
+        if ((rc = fel_apply(S->ops.fel, S->E0, S->bel))) return rc;
+      } else if ((rc = launch_spmv(ctx, S->ops.F_el, 0, S->E0, S->bel))) {
+        return rc;
+      }
       sp = SolveSpec{0, 1, S->ops.A_el, S->bel, S->u, fast ? S->bint : nullptr, fast ? S->uint : nullptr};
       break;
     case 1:  // :127-169
@@ -776,6 +791,12 @@ static void report_scalars(cpm_sim* S, const double* scal, cpm_step_report* rep)
   for (int k = 0; k < 3; ++k) rep->avg_m[k] = scal[S_AVGM + k];
 }
 
+// reliable-update factor of a solve: slots 0 and 1 are the elasticity and
+// demag solves, slots 2..9 the Gauss-Seidel solves
+static inline double solve_delta(const cpm_sim_params& p, int slot) {
+  return (slot >= 2 && p.rel_delta_gs > 0.0) ? p.rel_delta_gs : p.rel_delta;
+}
+
 static inline int use_x0_fast(const cpm_sim* S, int use_init) {
   return (use_init || (S->prm.warm_start && S->stepped)) ? 1 : 0;
 }
@@ -817,7 +838,8 @@ static int solve_one(cpm_sim* S, const SolveSpec& sp, int use_init, cpm_step_rep
   cpm_mg* mg = !p.precond ? nullptr : (sp.slot == 0 ? S->ops.mg_el : sp.slot == 1 ? S->ops.mg_demag : nullptr);
   int rc = mg ? mg_pcg(mg, bb, xx, x0, p.tol, p.maxiter, hint, it, res, &failed)
            : (p.mode == CPM_MODE_MIXED)
-               ? sell_cg_mixed(ctx, Am, sp.K, bb, xx, x0, p.tol, p.maxiter, p.rel_delta, hint, it, res, &failed,
+               ? sell_cg_mixed(ctx, Am, sp.K, bb, xx, x0, p.tol, p.maxiter, solve_delta(p, sp.slot), hint, it, res,
+                               &failed,
                                nullptr)
                : sell_cg64(ctx, Am, sp.K, bb, xx, x0, p.tol, p.maxiter, hint, it, res, &failed);
   if (rc) return rc;
@@ -1001,7 +1023,7 @@ extern "C" int cpm_batch_step(cpm_batch* B, const double* H, const int32_t* use_
       m.x = sp[j].xint ? sp[j].xint : sp[j].x;
       m.use_x0 = use_x0_fast(S, use_init[j]);
       m.tol = p.tol;
-      m.delta = p.rel_delta;
+      m.delta = solve_delta(p, sp[j].slot);
       m.maxiter = p.maxiter;
       m.r = S->wr;
       m.p = S->wp;

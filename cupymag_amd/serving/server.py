@@ -31,9 +31,10 @@ import numpy as np
 
 from .workloads import initial_m
 
-# device bytes per DOF of a member (operators in CSR and SELL-64 forms, the
-# step's state vectors, the batched-CG workspace of the mixed mode)
-_BYTES_PER_DOF = {False: 2200.0, True: 8900.0}
+# device bytes per DOF of a member (operators in CSR and SELL-64 forms with
+# the matrix-free F_el, the step's state vectors, the batched-CG workspace
+# of the mixed mode)
+_BYTES_PER_DOF = {False: 2200.0, True: 5700.0}
 
 
 def estimate_bytes(n_dof, me):
@@ -109,7 +110,7 @@ class SweepServer:
         torch.cuda.set_device(self.ctx.device)
         P = from_dict(req["cfg"])
         arr = kuhn_mesh(req["n"])
-        S = SimulationOperators(P, mesh_arrays=arr, ctx=self.ctx, keep_host=False)
+        S = SimulationOperators(P, mesh_arrays=arr, ctx=self.ctx, keep_host=False, matrix_free_fel=True)
         sim = GSPMSimulation(S, P, mode=self.mode, rel_delta=self.rel_delta)
         f0 = np.asarray(req["fields"][0], dtype=np.float64)
         direction = f0 if np.linalg.norm(f0) > 0 else (1.0, 0.0, 0.0)

@@ -233,6 +233,23 @@ int cpm_cg_mg(cpm_mg* mg, const double* b, double* x, int use_x0, double tol, in
               double* resid);
 
 /* ------------------------------------------------------------------ */
+/* matrix-free magnetoelastic right-hand side                           */
+/* ------------------------------------------------------------------ */
+/* b = F_el E0 without the F_el matrix (assemble_elasticity.py:268-348,
+ * 507-527, used at Micromagnetics.py:123), for an n^3 periodic Kuhn mesh
+ * whose (n+1)^3 node coordinates are in kuhn_mesh order: every element's
+ * geometry is recomputed from the coordinates with the host assembler's
+ * formulas (signed detJ, 7-digit 4-point table, zero element when
+ * |detJ| < 1e-14).  C: the 6x6 stiffness (rotated when rot111), row-major.
+ * E0: 6n interleaved; b: 3n [bx; by; bz], rows 0, n, 2n zero.  Results equal
+ * the CSR product up to summation order.  Saves the 270 entries per row
+ * triple of F_el (6.8 GB at 128^3). */
+typedef struct cpm_fel cpm_fel;
+int cpm_fel_create(cpm_ctx* ctx, int32_t n, const double* coords, const double* C, cpm_fel** out);
+int cpm_fel_destroy(cpm_fel* fel);
+int cpm_fel_apply(cpm_fel* fel, const double* E0, double* b);
+
+/* ------------------------------------------------------------------ */
 /* device-resident GSPM LLG simulation (the hot loop)                  */
 /* ------------------------------------------------------------------ */
 typedef struct cpm_sim cpm_sim;
@@ -247,6 +264,7 @@ typedef struct cpm_sim_ops {
   const cpm_mat* F_el;    /* 3n x 6n or NULL */
   cpm_mg* mg_el;          /* multigrid hierarchy of A_el (used when params.precond) or NULL */
   cpm_mg* mg_demag;       /* multigrid hierarchy of A_demag (used when params.precond) or NULL */
+  cpm_fel* fel;           /* matrix-free F_el (cpm_fel_create); used instead of F_el when not NULL */
 } cpm_sim_ops;
 
 typedef struct cpm_sim_params {
@@ -273,6 +291,8 @@ typedef struct cpm_sim_params {
   int32_t precond;            /* fast modes: 1 = the elasticity and demag solves run multigrid-
                                  preconditioned CG (cpm_cg_mg) on the hierarchies in cpm_sim_ops */
   int32_t reserved[3];
+  double rel_delta_gs;        /* CPM_MODE_MIXED: rel_delta of the Gauss-Seidel solves (A1, A2);
+                                 0 = rel_delta */
 } cpm_sim_params;
 
 #define CPM_MODE_FP64 0   /* plain FP64 path: CSR operators, the reference's sequence of solves */
