@@ -4,8 +4,10 @@ The reference runs one simulation per process (Micromagnetics.py:34).  A
 ``GSPMBatch`` holds any number of ``GSPMSimulation`` objects of any mesh
 sizes (fast modes, all the same mode) and advances them by one LLG step
 each per ``step`` call.  Each of the step's ten solves runs as one batched
-CG over every member (one launch per CG phase for the whole batch); the
-field kernels run per member.  Members can join and leave between steps,
+CG over every member (one launch per CG phase for the whole batch); members
+created with ``precond="mg"`` run their elasticity and demag solves as one
+batched multigrid-preconditioned CG instead.  The field kernels run per
+member.  Members can join and leave between steps,
 which is what the serving loop uses for continuous batching.  A member's
 results are bit-identical alone and in any batch.
 """

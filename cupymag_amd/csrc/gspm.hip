@@ -22,11 +22,13 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <vector>
 
 #include "device.h"
-#include "multi.h"
 #include "mg.h"
+#include "multi.h"
+#include "multi_mg.h"
 
 struct cpm_fel;
 namespace cpm {
@@ -888,6 +890,8 @@ struct cpm_batch {
   int mode = -1;
   std::vector<cpm_sim*> sims;
   cpm::MultiSolver ms[6];  // one per stage
+  // multigrid-preconditioned members of stages 0 and 1, per (precision, degree)
+  std::map<int, cpm::MultiMgSolver> mgs[2];
   double* scal = nullptr;  // pinned, S_NSCAL per simulation
   int cap = 0;
 };
@@ -950,8 +954,6 @@ extern "C" int cpm_batch_add(cpm_batch* B, cpm_sim* S) {
   if (S->ctx != B->ctx) return fail(CPM_EINVAL, "cpm_batch_add: the simulation uses another context");
   if (S->prm.mode == CPM_MODE_FP64)
     return fail(CPM_EINVAL, "cpm_batch_add: batches run the SELL paths (CPM_MODE_FAST64 or CPM_MODE_MIXED)");
-  if (S->prm.precond)
-    return fail(CPM_EINVAL, "cpm_batch_add: batched solves run plain CG (create the simulation without precond)");
   if (!B->sims.empty() && S->prm.mode != B->mode)
     return fail(CPM_EINVAL, "cpm_batch_add: every simulation of a batch uses the same mode");
   CPM_HIP(hipSetDevice(B->ctx->device));
@@ -1005,15 +1007,36 @@ extern "C" int cpm_batch_step(cpm_batch* B, const double* H, const int32_t* use_
   std::vector<int32_t> iters;
   std::vector<double> resid;
   std::vector<int> failed;
+  std::vector<std::pair<int, MultiMgSys>> mgq;  // (member, system) still to solve
+  std::vector<MultiMgSys> mgsys;
+  std::vector<int> mgwho;
+  std::vector<int32_t> done_it(nb);
+  std::vector<double> done_res(nb);
+  std::vector<int> done_fail(nb), solved(nb);
   for (int stage = 0; stage < 6; ++stage) {
     sys.clear();
     who.clear();
-    int hint = 0, bs = 1, K = 1;
+    mgq.clear();
+    int hint = 0, mghint = 0, bs = 1, K = 1;
     for (int j = 0; j < nb; ++j) {
       cpm_sim* S = B->sims[j];
+      solved[j] = 0;
       if (int rc = stage_pre(S, stage, P[j], sp[j])) return rc;
       if (!sp[j].A) continue;
       const cpm_sim_params& p = S->prm;
+      cpm_mg* mg = !p.precond ? nullptr : (sp[j].slot == 0 ? S->ops.mg_el : sp[j].slot == 1 ? S->ops.mg_demag : nullptr);
+      if (mg) {
+        MultiMgSys y;
+        y.M = mg;
+        y.b = sp[j].bint ? sp[j].bint : sp[j].b;
+        y.x = sp[j].xint ? sp[j].xint : sp[j].x;
+        y.use_x0 = use_x0_fast(S, use_init[j]);
+        y.tol = p.tol;
+        y.maxiter = p.maxiter;
+        mgq.emplace_back(j, y);
+        mghint = std::max(mghint, S->hint[sp[j].slot]);
+        continue;
+      }
       MultiSys m;
       m.A = sp[j].A;
       bs = sp[j].A->sell.bs;
@@ -1037,14 +1060,44 @@ extern "C" int cpm_batch_step(cpm_batch* B, const double* H, const int32_t* use_
       who.push_back(j);
       for (int k = 0; k < K; ++k) hint = std::max(hint, S->hint[sp[j].slot + k]);
     }
-    if (sys.empty()) continue;
+    if (sys.empty() && mgq.empty()) continue;
     const int ns = (int)sys.size();
     iters.assign(ns * K, 0);
     resid.assign(ns * K, 0.0);
     failed.assign(ns, 0);
-    if (int rc = multi_cg(ctx, B->ms[stage], bs, K, mixed ? 1 : 0, sys, hint, (B->flags & CPM_BATCH_GRAPH) != 0,
-                          iters.data(), resid.data(), failed.data()))
-      return rc;
+    if (ns)
+      if (int rc = multi_cg(ctx, B->ms[stage], bs, K, mixed ? 1 : 0, sys, hint, (B->flags & CPM_BATCH_GRAPH) != 0,
+                            iters.data(), resid.data(), failed.data()))
+        return rc;
+    // multigrid members: one batched solve per (precision, degree) with each
+    // hierarchy at most once (members sharing operators go to later rounds)
+    while (!mgq.empty()) {
+      const cpm_mg* M0 = mgq[0].second.M;
+      const int key = M0->prec * 16 + M0->degree;
+      mgsys.clear();
+      mgwho.clear();
+      std::vector<std::pair<int, MultiMgSys>> rest;
+      for (auto& e : mgq) {
+        const cpm_mg* M = e.second.M;
+        bool dup = false;
+        for (const auto& y : mgsys) dup |= (y.M == M);
+        if (dup || M->prec * 16 + M->degree != key)
+          rest.push_back(e);
+        else
+          mgsys.push_back(e.second), mgwho.push_back(e.first);
+      }
+      const int nm = (int)mgsys.size();
+      std::vector<int32_t> mit(nm);
+      std::vector<double> mres(nm);
+      std::vector<int> mfail(nm);
+      if (int rc = multi_mg_pcg(ctx, B->mgs[stage][key], mgsys, mghint, mit.data(), mres.data(), mfail.data()))
+        return rc;
+      for (int i = 0; i < nm; ++i) {
+        const int j = mgwho[i];
+        done_it[j] = mit[i], done_res[j] = mres[i], done_fail[j] = mfail[i], solved[j] = 1;
+      }
+      mgq.swap(rest);
+    }
     for (int i = 0; i < ns; ++i) {
       const int j = who[i];
       cpm_sim* S = B->sims[j];
@@ -1059,6 +1112,18 @@ extern "C" int cpm_batch_step(cpm_batch* B, const double* H, const int32_t* use_
           if (iters[i * K + k] > iters[i * K + kf]) kf = k;
         reps[j].failed = s.slot + kf + 1;
         reps[j].resid = resid[i * K + kf];
+      }
+      if (int rc = stage_post(S, s)) return rc;
+    }
+    for (int j = 0; j < nb; ++j) {
+      if (!solved[j]) continue;
+      cpm_sim* S = B->sims[j];
+      const SolveSpec& s = sp[j];
+      reps[j].iters[s.slot] = done_it[j];
+      S->hint[s.slot] = done_it[j];
+      if (done_fail[j] && !reps[j].failed) {
+        reps[j].failed = s.slot + 1;
+        reps[j].resid = done_res[j];
       }
       if (int rc = stage_post(S, s)) return rc;
     }

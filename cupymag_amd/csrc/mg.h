@@ -1,9 +1,41 @@
 // Multigrid-preconditioned CG (see mg.hip).
 #pragma once
 #include <cstdint>
+#include <vector>
 
-struct cpm_mg;
+struct cpm_ctx;
 struct cpm_mat;
+
+namespace cpm {
+
+struct MgLevel {
+  cpm_mat* A = nullptr;
+  int32_t nb = 0;           // nodes (block rows)
+  int32_t dims[3] = {0, 0, 0};
+  int64_t n = 0;            // nb * bs
+  void *x = nullptr, *r = nullptr, *b = nullptr, *d0 = nullptr, *d1 = nullptr;
+  void* dinv = nullptr;     // bs*bs per node: inverse diagonal block
+  double lmax = 0.0;        // power-iteration estimate of lambda_max(D^-1 A)
+};
+
+}  // namespace cpm
+
+struct cpm_mg {
+  cpm_ctx* ctx = nullptr;
+  int bs = 1, prec = 32, degree = 2;
+  double ratio = 20.0;
+  std::vector<cpm::MgLevel> L;
+  void* cinv = nullptr;     // coarsest dense inverse (T), ncoarse x ncoarse
+  int64_t ncoarse = 0;
+  // Chebyshev coefficients: c0[l] = 1/theta; for step k of level l,
+  // c1[8l + k] = rho_k rho_(k-1) and c2[8l + k] = 2 rho_k / delta
+  std::vector<double> c0, c1, c2;
+  // FP64 PCG vectors (fine level)
+  double *r = nullptr, *p = nullptr, *q = nullptr, *rold = nullptr;
+  std::vector<void*> allocs;
+};
+
+
 
 namespace cpm {
 

@@ -34,91 +34,21 @@
 #include <vector>
 
 #include "device.h"
-#include "sell_dev.h"
 #include "mg.h"
-
-struct cpm_mg;
-
-namespace cpm {
-
-struct MgLevel {
-  cpm_mat* A = nullptr;
-  int32_t nb = 0;           // nodes (block rows)
-  int32_t dims[3] = {0, 0, 0};
-  int64_t n = 0;            // nb * bs
-  void *x = nullptr, *r = nullptr, *b = nullptr, *d0 = nullptr, *d1 = nullptr;
-  void* dinv = nullptr;     // bs*bs per node: inverse diagonal block
-  double lmax = 0.0;        // power-iteration estimate of lambda_max(D^-1 A)
-};
-
-}  // namespace cpm
-
-struct cpm_mg {
-  cpm_ctx* ctx = nullptr;
-  int bs = 1, prec = 32, degree = 2;
-  double ratio = 20.0;
-  std::vector<cpm::MgLevel> L;
-  void* cinv = nullptr;     // coarsest dense inverse (T), ncoarse x ncoarse
-  int64_t ncoarse = 0;
-  // Chebyshev coefficients: c0[l] = 1/theta; for step k of level l,
-  // c1[8l + k] = rho_k rho_(k-1) and c2[8l + k] = 2 rho_k / delta
-  std::vector<double> c0, c1, c2;
-  // FP64 PCG vectors (fine level)
-  double *r = nullptr, *p = nullptr, *q = nullptr, *rold = nullptr;
-  std::vector<void*> allocs;
-};
+#include "mg_dev.h"
 
 namespace cpm {
 
 // ---------------------------------------------------------------- kernels
 
-template <int BS, class T>
-__device__ __forceinline__ void apply_dinv(const T* __restrict__ dinv, int64_t node, const double (&v)[BS],
-                                           double (&out)[BS]) {
-#pragma unroll
-  for (int d = 0; d < BS; ++d) {
-    double s = 0.0;
-#pragma unroll
-    for (int e = 0; e < BS; ++e) s += (double)dinv[node * BS * BS + d * BS + e] * v[e];
-    out[d] = s;
-  }
-}
-
-// First Chebyshev step.  ZERO: x0 = 0, so r = b (FROM64: b is the FP64
-// outer residual, also stored as this level's b), d = c0 D^-1 r, x = d.
-// Otherwise r is the current residual: d = c0 D^-1 r, x += d.
 template <int BS, class T, bool ZERO, bool FROM64>
 __global__ void __launch_bounds__(kBlock) k_mg_first(int32_t nb, const double* __restrict__ b64, T* __restrict__ b,
                                                      T* __restrict__ r, T* __restrict__ d, T* __restrict__ x,
                                                      const T* __restrict__ dinv, double c0, const CgState* st) {
   if (st->done) return;
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nb; i += stride) {
-    double rv[BS], dv[BS];
-#pragma unroll
-    for (int c = 0; c < BS; ++c) {
-      if (ZERO) {
-        const double bv = FROM64 ? b64[i * BS + c] : (double)b[i * BS + c];
-        if (FROM64) b[i * BS + c] = (T)bv;
-        r[i * BS + c] = (T)bv;
-        rv[c] = (double)(T)bv;
-      } else {
-        rv[c] = (double)r[i * BS + c];
-      }
-    }
-    apply_dinv<BS, T>(dinv, i, rv, dv);
-#pragma unroll
-    for (int c = 0; c < BS; ++c) {
-      const T dn = (T)(c0 * dv[c]);
-      d[i * BS + c] = dn;
-      x[i * BS + c] = ZERO ? dn : (T)((double)x[i * BS + c] + (double)dn);
-    }
-  }
+  mg_first_rows<BS, T, ZERO, FROM64>(blockIdx.x, gridDim.x, nb, b64, b, r, d, x, dinv, c0);
 }
 
-// MODE 0: Chebyshev step r -= A d_in, d_out = c1 d_in + c2 D^-1 r, x += d_out.
-// MODE 1: r -= A d_in only (residual after the last step).
-// MODE 2: r = b - A x.
 template <int BS, class T, int MODE, bool ST>
 __global__ void __launch_bounds__(kBlock) k_mg_sweep(int32_t nb, const int64_t* __restrict__ sptr,
                                                      const int32_t* __restrict__ col, const T* __restrict__ val,
@@ -128,124 +58,28 @@ __global__ void __launch_bounds__(kBlock) k_mg_sweep(int32_t nb, const int64_t* 
                                                      const T* __restrict__ dinv, double c1, double c2,
                                                      const CgState* st) {
   if (st->done) return;
-  const T* src = (MODE == 2) ? x : din;
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t row = (int64_t)blockIdx.x * kBlock + threadIdx.x; row < (int64_t)((nb + 63) & ~63); row += stride) {
-    double sum[BS], xo[BS];
-    sell_row<BS, 1, T, false, ST>(row, nb, sptr, col, val, dg, src, nullptr, ga, sum, xo);
-    if (row >= nb) continue;
-    double rv[BS];
-#pragma unroll
-    for (int c = 0; c < BS; ++c) {
-      rv[c] = (MODE == 2) ? (double)b[row * BS + c] - sum[c] : (double)r[row * BS + c] - sum[c];
-      r[row * BS + c] = (T)rv[c];
-      rv[c] = (double)(T)rv[c];
-    }
-    if (MODE == 0) {
-      double dv[BS];
-      apply_dinv<BS, T>(dinv, row, rv, dv);
-#pragma unroll
-      for (int c = 0; c < BS; ++c) {
-        const T dn = (T)(c1 * xo[c] + c2 * dv[c]);
-        dout[row * BS + c] = dn;
-        x[row * BS + c] = (T)((double)x[row * BS + c] + (double)dn);
-      }
-    }
-  }
+  mg_sweep_rows<BS, T, MODE, ST>(blockIdx.x, gridDim.x, nb, sptr, col, val, dg, ga, din, dout, x, r, b, dinv, c1, c2);
 }
 
-__device__ __forceinline__ int64_t wrap_idx(int32_t i, int32_t j, int32_t k, const int32_t* dims) {
-  i += (i < 0) ? dims[0] : 0;
-  i -= (i >= dims[0]) ? dims[0] : 0;
-  j += (j < 0) ? dims[1] : 0;
-  j -= (j >= dims[1]) ? dims[1] : 0;
-  k += (k < 0) ? dims[2] : 0;
-  k -= (k >= dims[2]) ? dims[2] : 0;
-  return ((int64_t)i * dims[1] + j) * dims[2] + k;
-}
-
-struct Dims3 {
-  int32_t d[3];
-};
-
-// b_c = R r_f (transpose of the nested P1 interpolation), anchor excluded.
 template <int BS, class T>
 __global__ void __launch_bounds__(kBlock) k_mg_restrict(Dims3 fine, Dims3 coarse, const T* __restrict__ rf,
                                                         T* __restrict__ bc, const CgState* st) {
   if (st->done) return;
-  const int64_t nc = (int64_t)coarse.d[0] * coarse.d[1] * coarse.d[2];
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t I = (int64_t)blockIdx.x * kBlock + threadIdx.x; I < nc; I += stride) {
-    double acc[BS];
-#pragma unroll
-    for (int c = 0; c < BS; ++c) acc[c] = 0.0;
-    if (I != 0) {
-      const int32_t i = (int32_t)(I / ((int64_t)coarse.d[1] * coarse.d[2]));
-      const int32_t j = (int32_t)((I / coarse.d[2]) % coarse.d[1]);
-      const int32_t k = (int32_t)(I % coarse.d[2]);
-      const int64_t f0 = wrap_idx(2 * i, 2 * j, 2 * k, fine.d);
-#pragma unroll
-      for (int c = 0; c < BS; ++c) acc[c] = (double)rf[f0 * BS + c];
-#pragma unroll
-      for (int dd = 1; dd < 8; ++dd) {
-        const int32_t a = dd & 1, bb = (dd >> 1) & 1, cc = (dd >> 2) & 1;
-        const int64_t fp = wrap_idx(2 * i + a, 2 * j + bb, 2 * k + cc, fine.d);
-        const int64_t fm = wrap_idx(2 * i - a, 2 * j - bb, 2 * k - cc, fine.d);
-#pragma unroll
-        for (int c = 0; c < BS; ++c) {
-          if (fp != 0) acc[c] += 0.5 * (double)rf[fp * BS + c];
-          if (fm != 0) acc[c] += 0.5 * (double)rf[fm * BS + c];
-        }
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < BS; ++c) bc[I * BS + c] = (T)acc[c];
-  }
+  mg_restrict_rows<BS, T>(blockIdx.x, gridDim.x, fine, coarse, rf, bc);
 }
 
-// x_f += P e_c (nested P1 interpolation), anchor excluded.
 template <int BS, class T>
 __global__ void __launch_bounds__(kBlock) k_mg_prolong(Dims3 fine, Dims3 coarse, const T* __restrict__ ec,
                                                        T* __restrict__ xf, const CgState* st) {
   if (st->done) return;
-  const int64_t nf = (int64_t)fine.d[0] * fine.d[1] * fine.d[2];
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t f = (int64_t)blockIdx.x * kBlock + threadIdx.x; f < nf; f += stride) {
-    if (f == 0) continue;
-    const int32_t i = (int32_t)(f / ((int64_t)fine.d[1] * fine.d[2]));
-    const int32_t j = (int32_t)((f / fine.d[2]) % fine.d[1]);
-    const int32_t k = (int32_t)(f % fine.d[2]);
-    const int32_t a = i & 1, bb = j & 1, cc = k & 1;
-    const int32_t I = i >> 1, J = j >> 1, K = k >> 1;
-    const int64_t c0 = wrap_idx(I, J, K, coarse.d);
-    double v[BS];
-    if ((a | bb | cc) == 0) {
-#pragma unroll
-      for (int c = 0; c < BS; ++c) v[c] = (c0 != 0) ? (double)ec[c0 * BS + c] : 0.0;
-    } else {
-      const int64_t c1 = wrap_idx(I + a, J + bb, K + cc, coarse.d);
-#pragma unroll
-      for (int c = 0; c < BS; ++c)
-        v[c] = 0.5 * (((c0 != 0) ? (double)ec[c0 * BS + c] : 0.0) + ((c1 != 0) ? (double)ec[c1 * BS + c] : 0.0));
-    }
-#pragma unroll
-    for (int c = 0; c < BS; ++c) xf[f * BS + c] = (T)((double)xf[f * BS + c] + v[c]);
-  }
+  mg_prolong_rows<BS, T>(blockIdx.x, gridDim.x, fine, coarse, ec, xf);
 }
 
-// x = Cinv b on the coarsest level: one wavefront per row.
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_mg_dense(int64_t n, const T* __restrict__ cinv, const T* __restrict__ b,
                                                      T* __restrict__ x, const CgState* st) {
   if (st->done) return;
-  const int lane = threadIdx.x & 63;
-  const int64_t waves = (int64_t)gridDim.x * (kBlock / 64);
-  for (int64_t row = (int64_t)blockIdx.x * (kBlock / 64) + threadIdx.x / 64; row < n; row += waves) {
-    double s = 0.0;
-    for (int64_t c = lane; c < n; c += 64) s += (double)cinv[row * n + c] * (double)b[c];
-    s = group_sum<64>(s);
-    if (lane == 0) x[row] = (T)s;
-  }
+  mg_dense_rows<T>(blockIdx.x, gridDim.x, n, cinv, b, x);
 }
 
 // y = D^-1 (A x) (FP64 form) and partials of y.y: power iteration.
@@ -297,20 +131,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_init(int64_t n, const double* __
                                                      const double* __restrict__ y, double* __restrict__ r,
                                                      int use_x0, double* partials) {
   double acc[2] = {0.0, 0.0};
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-    const double bi = b[i];
-    double ri;
-    if (use_x0) {
-      ri = bi - y[i];
-    } else {
-      x[i] = 0.0;
-      ri = bi;
-    }
-    r[i] = ri;
-    acc[0] += bi * bi;
-    acc[1] += ri * ri;
-  }
+  pcg_init_rows(blockIdx.x, gridDim.x, n, b, x, y, r, use_x0, acc);
   block_partial<2>(acc, partials);
 }
 
@@ -338,17 +159,8 @@ __global__ void __launch_bounds__(kBlock) k_pcg_xr(int64_t n, double* __restrict
                                                    const double* __restrict__ q, const CgState* st,
                                                    double* partials) {
   if (st->done) return;
-  const double a = st->alpha;
   double acc[1] = {0.0};
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-    x[i] += a * p[i];
-    const double ro = r[i];
-    rold[i] = ro;
-    const double rn = ro - a * q[i];
-    r[i] = rn;
-    acc[0] += rn * rn;
-  }
+  pcg_xr_rows(blockIdx.x, gridDim.x, n, st->alpha, x, r, rold, p, q, acc);
   block_partial<1>(acc, partials);
 }
 
@@ -358,14 +170,8 @@ __global__ void __launch_bounds__(kBlock) k_pcg_dot(int64_t n, const double* __r
                                                     const double* __restrict__ rold, const T* __restrict__ z,
                                                     const CgState* st, double* partials) {
   if (st->done) return;
-  const bool first = st->it == 0;
   double acc[2] = {0.0, 0.0};
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-    const double zi = (double)z[i];
-    acc[0] += r[i] * zi;
-    if (!first) acc[1] += rold[i] * zi;
-  }
+  pcg_dot_rows<T>(blockIdx.x, gridDim.x, n, r, rold, z, st->it == 0, acc);
   block_partial<2>(acc, partials);
 }
 
@@ -380,23 +186,14 @@ __global__ void __launch_bounds__(kRedBlock) k_pcg_red_rz(int nblk, const double
     v[1] += partials[nblk + i];
   }
   block_sum_t<2, kRedBlock>(v, lds);
-  if (threadIdx.x == 0) {
-    CgState& c = *st;
-    c.rho_old = c.rho;
-    c.rho = v[0];
-    c.beta = (c.it == 0) ? 0.0 : (v[0] - v[1]) / c.rho_old;
-  }
+  if (threadIdx.x == 0) pcg_state_rz(*st, v);
 }
 
 template <class T>
 __global__ void __launch_bounds__(kBlock) k_pcg_pupd(int64_t n, const T* __restrict__ z, double* __restrict__ p,
                                                      const CgState* st) {
   if (st->done) return;
-  const bool first = st->it == 0;
-  const double beta = st->beta;
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
-    p[i] = first ? (double)z[i] : (double)z[i] + beta * p[i];
+  pcg_pupd_rows<T>(blockIdx.x, gridDim.x, n, z, p, st->it == 0, st->beta);
 }
 
 // ---------------------------------------------------------------- host

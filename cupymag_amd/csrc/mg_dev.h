@@ -1,0 +1,247 @@
+// Device bodies of the multigrid V-cycle and of the preconditioned CG,
+// written over a workgroup range (lb, lg) like sell_dev.h, so that one
+// system per launch (mg.hip) and many systems per segmented launch
+// (multi_mg.hip) share them.
+#pragma once
+#include "sell_dev.h"
+
+namespace cpm {
+
+template <int BS, class T>
+__device__ __forceinline__ void apply_dinv(const T* __restrict__ dinv, int64_t node, const double (&v)[BS],
+                                           double (&out)[BS]) {
+#pragma unroll
+  for (int d = 0; d < BS; ++d) {
+    double s = 0.0;
+#pragma unroll
+    for (int e = 0; e < BS; ++e) s += (double)dinv[node * BS * BS + d * BS + e] * v[e];
+    out[d] = s;
+  }
+}
+
+// First Chebyshev step.  ZERO: x0 = 0, so r = b (FROM64: b is the FP64
+// outer residual, also stored as this level's b), d = c0 D^-1 r, x = d.
+// Otherwise r is the current residual: d = c0 D^-1 r, x += d.
+template <int BS, class T, bool ZERO, bool FROM64>
+__device__ __forceinline__ void mg_first_rows(int64_t lb, int64_t lg, int32_t nb, const double* __restrict__ b64,
+                                              T* __restrict__ b, T* __restrict__ r, T* __restrict__ d,
+                                              T* __restrict__ x, const T* __restrict__ dinv, double c0) {
+  const int64_t stride = lg * kBlock;
+  for (int64_t i = lb * kBlock + threadIdx.x; i < nb; i += stride) {
+    double rv[BS], dv[BS];
+#pragma unroll
+    for (int c = 0; c < BS; ++c) {
+      if (ZERO) {
+        const double bv = FROM64 ? b64[i * BS + c] : (double)b[i * BS + c];
+        if (FROM64) b[i * BS + c] = (T)bv;
+        r[i * BS + c] = (T)bv;
+        rv[c] = (double)(T)bv;
+      } else {
+        rv[c] = (double)r[i * BS + c];
+      }
+    }
+    apply_dinv<BS, T>(dinv, i, rv, dv);
+#pragma unroll
+    for (int c = 0; c < BS; ++c) {
+      const T dn = (T)(c0 * dv[c]);
+      d[i * BS + c] = dn;
+      x[i * BS + c] = ZERO ? dn : (T)((double)x[i * BS + c] + (double)dn);
+    }
+  }
+}
+
+// MODE 0: Chebyshev step r -= A d_in, d_out = c1 d_in + c2 D^-1 r, x += d_out.
+// MODE 1: r -= A d_in only (residual after the last step).
+// MODE 2: r = b - A x.
+template <int BS, class T, int MODE, bool ST>
+__device__ __forceinline__ void mg_sweep_rows(int64_t lb, int64_t lg, int32_t nb, const int64_t* __restrict__ sptr,
+                                              const int32_t* __restrict__ col, const T* __restrict__ val,
+                                              const double* __restrict__ dg, const GridArg& ga,
+                                              const T* __restrict__ din, T* __restrict__ dout, T* __restrict__ x,
+                                              T* __restrict__ r, const T* __restrict__ b,
+                                              const T* __restrict__ dinv, double c1, double c2) {
+  const T* src = (MODE == 2) ? x : din;
+  const int64_t stride = lg * kBlock;
+  for (int64_t row = lb * kBlock + threadIdx.x; row < (int64_t)((nb + 63) & ~63); row += stride) {
+    double sum[BS], xo[BS];
+    sell_row<BS, 1, T, false, ST>(row, nb, sptr, col, val, dg, src, nullptr, ga, sum, xo);
+    if (row >= nb) continue;
+    double rv[BS];
+#pragma unroll
+    for (int c = 0; c < BS; ++c) {
+      rv[c] = (MODE == 2) ? (double)b[row * BS + c] - sum[c] : (double)r[row * BS + c] - sum[c];
+      r[row * BS + c] = (T)rv[c];
+      rv[c] = (double)(T)rv[c];
+    }
+    if (MODE == 0) {
+      double dv[BS];
+      apply_dinv<BS, T>(dinv, row, rv, dv);
+#pragma unroll
+      for (int c = 0; c < BS; ++c) {
+        const T dn = (T)(c1 * xo[c] + c2 * dv[c]);
+        dout[row * BS + c] = dn;
+        x[row * BS + c] = (T)((double)x[row * BS + c] + (double)dn);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ int64_t wrap_idx(int32_t i, int32_t j, int32_t k, const int32_t* dims) {
+  i += (i < 0) ? dims[0] : 0;
+  i -= (i >= dims[0]) ? dims[0] : 0;
+  j += (j < 0) ? dims[1] : 0;
+  j -= (j >= dims[1]) ? dims[1] : 0;
+  k += (k < 0) ? dims[2] : 0;
+  k -= (k >= dims[2]) ? dims[2] : 0;
+  return ((int64_t)i * dims[1] + j) * dims[2] + k;
+}
+
+struct Dims3 {
+  int32_t d[3];
+};
+
+// b_c = R r_f (transpose of the nested P1 interpolation), anchor excluded.
+template <int BS, class T>
+__device__ __forceinline__ void mg_restrict_rows(int64_t lb, int64_t lg, const Dims3& fine, const Dims3& coarse,
+                                                 const T* __restrict__ rf, T* __restrict__ bc) {
+  const int64_t nc = (int64_t)coarse.d[0] * coarse.d[1] * coarse.d[2];
+  const int64_t stride = lg * kBlock;
+  for (int64_t I = lb * kBlock + threadIdx.x; I < nc; I += stride) {
+    double acc[BS];
+#pragma unroll
+    for (int c = 0; c < BS; ++c) acc[c] = 0.0;
+    if (I != 0) {
+      const int32_t i = (int32_t)(I / ((int64_t)coarse.d[1] * coarse.d[2]));
+      const int32_t j = (int32_t)((I / coarse.d[2]) % coarse.d[1]);
+      const int32_t k = (int32_t)(I % coarse.d[2]);
+      const int64_t f0 = wrap_idx(2 * i, 2 * j, 2 * k, fine.d);
+#pragma unroll
+      for (int c = 0; c < BS; ++c) acc[c] = (double)rf[f0 * BS + c];
+#pragma unroll
+      for (int dd = 1; dd < 8; ++dd) {
+        const int32_t a = dd & 1, bb = (dd >> 1) & 1, cc = (dd >> 2) & 1;
+        const int64_t fp = wrap_idx(2 * i + a, 2 * j + bb, 2 * k + cc, fine.d);
+        const int64_t fm = wrap_idx(2 * i - a, 2 * j - bb, 2 * k - cc, fine.d);
+#pragma unroll
+        for (int c = 0; c < BS; ++c) {
+          if (fp != 0) acc[c] += 0.5 * (double)rf[fp * BS + c];
+          if (fm != 0) acc[c] += 0.5 * (double)rf[fm * BS + c];
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < BS; ++c) bc[I * BS + c] = (T)acc[c];
+  }
+}
+
+// x_f += P e_c (nested P1 interpolation), anchor excluded.
+template <int BS, class T>
+__device__ __forceinline__ void mg_prolong_rows(int64_t lb, int64_t lg, const Dims3& fine, const Dims3& coarse,
+                                                const T* __restrict__ ec, T* __restrict__ xf) {
+  const int64_t nf = (int64_t)fine.d[0] * fine.d[1] * fine.d[2];
+  const int64_t stride = lg * kBlock;
+  for (int64_t f = lb * kBlock + threadIdx.x; f < nf; f += stride) {
+    if (f == 0) continue;
+    const int32_t i = (int32_t)(f / ((int64_t)fine.d[1] * fine.d[2]));
+    const int32_t j = (int32_t)((f / fine.d[2]) % fine.d[1]);
+    const int32_t k = (int32_t)(f % fine.d[2]);
+    const int32_t a = i & 1, bb = j & 1, cc = k & 1;
+    const int32_t I = i >> 1, J = j >> 1, K = k >> 1;
+    const int64_t c0 = wrap_idx(I, J, K, coarse.d);
+    double v[BS];
+    if ((a | bb | cc) == 0) {
+#pragma unroll
+      for (int c = 0; c < BS; ++c) v[c] = (c0 != 0) ? (double)ec[c0 * BS + c] : 0.0;
+    } else {
+      const int64_t c1 = wrap_idx(I + a, J + bb, K + cc, coarse.d);
+#pragma unroll
+      for (int c = 0; c < BS; ++c)
+        v[c] = 0.5 * (((c0 != 0) ? (double)ec[c0 * BS + c] : 0.0) + ((c1 != 0) ? (double)ec[c1 * BS + c] : 0.0));
+    }
+#pragma unroll
+    for (int c = 0; c < BS; ++c) xf[f * BS + c] = (T)((double)xf[f * BS + c] + v[c]);
+  }
+}
+
+// x = Cinv b on the coarsest level: one wavefront per row.
+template <class T>
+__device__ __forceinline__ void mg_dense_rows(int64_t lb, int64_t lg, int64_t n, const T* __restrict__ cinv,
+                                              const T* __restrict__ b, T* __restrict__ x) {
+  const int lane = threadIdx.x & 63;
+  const int64_t waves = lg * (kBlock / 64);
+  for (int64_t row = lb * (kBlock / 64) + threadIdx.x / 64; row < n; row += waves) {
+    double s = 0.0;
+    for (int64_t c = lane; c < n; c += 64) s += (double)cinv[row * n + c] * (double)b[c];
+    s = group_sum<64>(s);
+    if (lane == 0) x[row] = (T)s;
+  }
+}
+
+// ---- preconditioned CG vector bodies ------------------------------------------
+
+// r = b - y (use_x0) or x = 0, r = b; acc = (||b||^2, ||r||^2).
+__device__ __forceinline__ void pcg_init_rows(int64_t lb, int64_t lg, int64_t n, const double* __restrict__ b,
+                                              double* __restrict__ x, const double* __restrict__ y,
+                                              double* __restrict__ r, int use_x0, double (&acc)[2]) {
+  const int64_t stride = lg * kBlock;
+  for (int64_t i = lb * kBlock + threadIdx.x; i < n; i += stride) {
+    const double bi = b[i];
+    double ri;
+    if (use_x0) {
+      ri = bi - y[i];
+    } else {
+      x[i] = 0.0;
+      ri = bi;
+    }
+    r[i] = ri;
+    acc[0] += bi * bi;
+    acc[1] += ri * ri;
+  }
+}
+
+// x += alpha p, r_old = r, r -= alpha q; acc += r.r.
+__device__ __forceinline__ void pcg_xr_rows(int64_t lb, int64_t lg, int64_t n, double alpha, double* __restrict__ x,
+                                            double* __restrict__ r, double* __restrict__ rold,
+                                            const double* __restrict__ p, const double* __restrict__ q,
+                                            double (&acc)[1]) {
+  const int64_t stride = lg * kBlock;
+  for (int64_t i = lb * kBlock + threadIdx.x; i < n; i += stride) {
+    x[i] += alpha * p[i];
+    const double ro = r[i];
+    rold[i] = ro;
+    const double rn = ro - alpha * q[i];
+    r[i] = rn;
+    acc[0] += rn * rn;
+  }
+}
+
+// acc = (r.z, r_old.z)  (r_old.z only after the first iteration)
+template <class T>
+__device__ __forceinline__ void pcg_dot_rows(int64_t lb, int64_t lg, int64_t n, const double* __restrict__ r,
+                                             const double* __restrict__ rold, const T* __restrict__ z, bool first,
+                                             double (&acc)[2]) {
+  const int64_t stride = lg * kBlock;
+  for (int64_t i = lb * kBlock + threadIdx.x; i < n; i += stride) {
+    const double zi = (double)z[i];
+    acc[0] += r[i] * zi;
+    if (!first) acc[1] += rold[i] * zi;
+  }
+}
+
+// p = z (first iteration) or z + beta p
+template <class T>
+__device__ __forceinline__ void pcg_pupd_rows(int64_t lb, int64_t lg, int64_t n, const T* __restrict__ z,
+                                              double* __restrict__ p, bool first, double beta) {
+  const int64_t stride = lg * kBlock;
+  for (int64_t i = lb * kBlock + threadIdx.x; i < n; i += stride)
+    p[i] = first ? (double)z[i] : (double)z[i] + beta * p[i];
+}
+
+// rho = r.z, beta = (r.z - r_old.z) / rho_old (flexible CG), from the sums v
+__device__ inline void pcg_state_rz(CgState& c, const double (&v)[2]) {
+  c.rho_old = c.rho;
+  c.rho = v[0];
+  c.beta = (c.it == 0) ? 0.0 : (v[0] - v[1]) / c.rho_old;
+}
+
+}  // namespace cpm

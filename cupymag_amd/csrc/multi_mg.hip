@@ -1,0 +1,511 @@
+// Batched multigrid-preconditioned CG: many independent systems, each with
+// its own multigrid hierarchy (any mesh sizes, any depths), advanced
+// together with one launch per phase for the whole batch.
+//
+// As in multi.hip, every system owns contiguous workgroup ranges sized from
+// its own rows only (its outer operator for the CG vectors, each level's
+// operator for that level's V-cycle work), so its arithmetic does not depend
+// on the batch.  The V-cycle runs level by level: at depth d every system
+// that still has a coarser level smooths, forms its residual and restricts
+// it; systems whose coarsest level is d + 1 solve it densely right after;
+// on the way up every system prolongs and post-smooths at its own depths.
+// The kernel count per iteration is that of one hierarchy of the deepest
+// member, whatever the batch size.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "device.h"
+#include "mg.h"
+#include "mg_dev.h"
+#include "multi_mg.h"
+#include "sell.h"
+
+namespace cpm {
+
+struct OuterSeg {
+  const int64_t* sptr;
+  const int32_t* col;
+  const double* v64;
+  const double* dg64;
+  GridArg ga;
+  int32_t nb, stencil;
+  int64_t n;
+  const double* b;
+  double* x;
+  double *r, *p, *q, *rold;
+  const void* z;
+  CgState* st;
+  int32_t blk0, nblk;    // vector phases: grid_for(n) workgroups
+  int32_t sblk0, snblk;  // SpMV: the operator's SELL grid
+  int32_t use_x0, maxiter;
+  double tol;
+};
+
+struct SmoothSeg {
+  const int64_t* sptr;
+  const int32_t* col;
+  const void* val;
+  const double* dg;
+  GridArg ga;
+  int32_t nb, stencil;
+  void *x, *r, *b, *d0, *d1;
+  const void* dinv;
+  const double* b64;  // depth 0: the system's FP64 outer residual
+  double c0, c1[8], c2[8];
+  Dims3 fd, cd;
+  void *bc, *xc;      // next level's b and x
+  const CgState* st;
+  int32_t blk0, nblk;
+};
+
+struct DenseSeg {
+  int64_t n;
+  const void* cinv;
+  const void* b;
+  void* x;
+  const CgState* st;
+  int32_t blk0, nblk;
+};
+
+// ---------------------------------------------------------------- kernels
+
+template <int BS, class T, bool ZERO, bool FROM64>
+__global__ void __launch_bounds__(kBlock) kbm_first(const SmoothSeg* __restrict__ seg, const int32_t* __restrict__ map) {
+  const SmoothSeg& d = seg[map[blockIdx.x]];
+  if (d.st->done) return;
+  mg_first_rows<BS, T, ZERO, FROM64>((int64_t)blockIdx.x - d.blk0, d.nblk, d.nb, d.b64, (T*)d.b, (T*)d.r,
+                                     (T*)d.d0, (T*)d.x, (const T*)d.dinv, d.c0);
+}
+
+// MODE 0: Chebyshev step k (d_in = d[(k-1)&1], d_out = d[k&1]); MODE 1: the
+// residual after step `k` (d_in = d[k&1]); MODE 2: r = b - A x.
+template <int BS, class T, int MODE>
+__global__ void __launch_bounds__(kBlock) kbm_sweep(const SmoothSeg* __restrict__ seg, const int32_t* __restrict__ map,
+                                                    int k) {
+  const SmoothSeg& d = seg[map[blockIdx.x]];
+  if (d.st->done) return;
+  T* dv[2] = {(T*)d.d0, (T*)d.d1};
+  const T* din = (MODE == 0) ? dv[(k - 1) & 1] : dv[k & 1];
+  T* dout = dv[k & 1];
+  const double c1 = (MODE == 0) ? d.c1[k] : 0.0, c2 = (MODE == 0) ? d.c2[k] : 0.0;
+  const int64_t lb = (int64_t)blockIdx.x - d.blk0;
+  if (d.stencil)
+    mg_sweep_rows<BS, T, MODE, true>(lb, d.nblk, d.nb, d.sptr, d.col, (const T*)d.val, d.dg, d.ga, din, dout,
+                                     (T*)d.x, (T*)d.r, (const T*)d.b, (const T*)d.dinv, c1, c2);
+  else
+    mg_sweep_rows<BS, T, MODE, false>(lb, d.nblk, d.nb, d.sptr, d.col, (const T*)d.val, d.dg, d.ga, din, dout,
+                                      (T*)d.x, (T*)d.r, (const T*)d.b, (const T*)d.dinv, c1, c2);
+}
+
+template <int BS, class T>
+__global__ void __launch_bounds__(kBlock) kbm_restrict(const SmoothSeg* __restrict__ seg,
+                                                       const int32_t* __restrict__ map) {
+  const SmoothSeg& d = seg[map[blockIdx.x]];
+  if (d.st->done) return;
+  mg_restrict_rows<BS, T>((int64_t)blockIdx.x - d.blk0, d.nblk, d.fd, d.cd, (const T*)d.r, (T*)d.bc);
+}
+
+template <int BS, class T>
+__global__ void __launch_bounds__(kBlock) kbm_prolong(const SmoothSeg* __restrict__ seg,
+                                                      const int32_t* __restrict__ map) {
+  const SmoothSeg& d = seg[map[blockIdx.x]];
+  if (d.st->done) return;
+  mg_prolong_rows<BS, T>((int64_t)blockIdx.x - d.blk0, d.nblk, d.fd, d.cd, (const T*)d.xc, (T*)d.x);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) kbm_dense(const DenseSeg* __restrict__ seg, const int32_t* __restrict__ map) {
+  const DenseSeg& d = seg[map[blockIdx.x]];
+  if (d.st->done) return;
+  mg_dense_rows<T>((int64_t)blockIdx.x - d.blk0, d.nblk, d.n, (const T*)d.cinv, (const T*)d.b, (T*)d.x);
+}
+
+// q = A p and partials of p.q (PLAIN: y = A x for the start, no partials)
+template <int BS, bool PLAIN>
+__global__ void __launch_bounds__(kBlock) kbm_spmv(const OuterSeg* __restrict__ seg, const int32_t* __restrict__ map,
+                                                   double* partials) {
+  const OuterSeg& d = seg[map[blockIdx.x]];
+  if (PLAIN ? !d.use_x0 : d.st->done) return;
+  double acc[1] = {0.0};
+  const int64_t lb = (int64_t)blockIdx.x - d.sblk0;
+  const double* in = PLAIN ? d.x : d.p;
+  if (d.stencil)
+    sell_rows<BS, 1, double, !PLAIN, false, true>(lb, d.snblk, d.nb, d.sptr, d.col, d.v64, d.dg64, in, nullptr, d.q,
+                                                  d.ga, acc);
+  else
+    sell_rows<BS, 1, double, !PLAIN, false, false>(lb, d.snblk, d.nb, d.sptr, d.col, d.v64, d.dg64, in, nullptr, d.q,
+                                                   d.ga, acc);
+  if (!PLAIN) block_partial<1>(acc, partials);
+}
+
+__global__ void __launch_bounds__(kBlock) kbm_init(const OuterSeg* __restrict__ seg, const int32_t* __restrict__ map,
+                                                   double* partials) {
+  const OuterSeg& d = seg[map[blockIdx.x]];
+  double acc[2] = {0.0, 0.0};
+  pcg_init_rows((int64_t)blockIdx.x - d.blk0, d.nblk, d.n, d.b, d.x, d.q, d.r, d.use_x0, acc);
+  block_partial<2>(acc, partials);
+}
+
+__global__ void __launch_bounds__(kBlock) kbm_zero_b(const OuterSeg* __restrict__ seg,
+                                                     const int32_t* __restrict__ map) {
+  const OuterSeg& d = seg[map[blockIdx.x]];
+  if (!d.use_x0 || d.st->bnorm2 != 0.0) return;
+  const int64_t stride = (int64_t)d.nblk * kBlock;
+  for (int64_t i = ((int64_t)blockIdx.x - d.blk0) * kBlock + threadIdx.x; i < d.n; i += stride) d.x[i] = 0.0;
+}
+
+__global__ void __launch_bounds__(kBlock) kbm_xr(const OuterSeg* __restrict__ seg, const int32_t* __restrict__ map,
+                                                 double* partials) {
+  const OuterSeg& d = seg[map[blockIdx.x]];
+  if (d.st->done) return;
+  double acc[1] = {0.0};
+  pcg_xr_rows((int64_t)blockIdx.x - d.blk0, d.nblk, d.n, d.st->alpha, d.x, d.r, d.rold, d.p, d.q, acc);
+  block_partial<1>(acc, partials);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) kbm_dot(const OuterSeg* __restrict__ seg, const int32_t* __restrict__ map,
+                                                  double* partials) {
+  const OuterSeg& d = seg[map[blockIdx.x]];
+  if (d.st->done) return;
+  double acc[2] = {0.0, 0.0};
+  pcg_dot_rows<T>((int64_t)blockIdx.x - d.blk0, d.nblk, d.n, d.r, d.rold, (const T*)d.z, d.st->it == 0, acc);
+  block_partial<2>(acc, partials);
+}
+
+template <class T>
+__global__ void __launch_bounds__(kBlock) kbm_pupd(const OuterSeg* __restrict__ seg, const int32_t* __restrict__ map) {
+  const OuterSeg& d = seg[map[blockIdx.x]];
+  if (d.st->done) return;
+  pcg_pupd_rows<T>((int64_t)blockIdx.x - d.blk0, d.nblk, d.n, (const T*)d.z, d.p, d.st->it == 0, d.st->beta);
+}
+
+enum : int { MRED_INIT = 0, MRED_ALPHA = 1, MRED_RR = 2, MRED_RZ = 3 };
+
+// One workgroup per system: the system's partials in a fixed order (those
+// of its SpMV workgroups for MRED_ALPHA, of its vector workgroups otherwise),
+// as the one-system reducers of mg.hip sum them.
+template <int MODE>
+__global__ void __launch_bounds__(kRedBlock) kbm_red(const OuterSeg* __restrict__ seg, int G,
+                                                     const double* __restrict__ partials) {
+  constexpr int NV = (MODE == MRED_INIT || MODE == MRED_RZ) ? 2 : 1;
+  const OuterSeg& d = seg[blockIdx.x];
+  if (MODE != MRED_INIT && d.st->done) return;
+  __shared__ double lds[NV * (kRedBlock / 64)];
+  double v[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = 0.0;
+  const int32_t b0 = (MODE == MRED_ALPHA) ? d.sblk0 : d.blk0, nbk = (MODE == MRED_ALPHA) ? d.snblk : d.nblk;
+  for (int i = threadIdx.x; i < nbk; i += kRedBlock)
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] += partials[(int64_t)k * G + b0 + i];
+  block_sum_t<NV, kRedBlock>(v, lds);
+  if (threadIdx.x != 0) return;
+  if constexpr (MODE == MRED_INIT) {
+    cg_state_init<1, false>(d.st, v, d.tol, d.maxiter);
+  } else if constexpr (MODE == MRED_ALPHA) {
+    cg_state_update<1, RED_ALPHA>(d.st, v, 0.0);
+  } else if constexpr (MODE == MRED_RR) {
+    cg_state_update<1, RED_RR_PCG>(d.st, v, 0.0);
+  } else {
+    pcg_state_rz(*d.st, v);
+  }
+}
+
+// ---------------------------------------------------------------- host
+
+namespace {
+
+template <class Seg>
+struct Table {
+  std::vector<Seg> h;
+  std::vector<int32_t> map;
+  Seg* d = nullptr;
+  int32_t* dmap = nullptr;
+  size_t cap = 0, capmap = 0;
+  int G = 0;
+};
+
+template <class Seg>
+int upload(Table<Seg>& t, hipStream_t s) {
+  if (t.h.size() > t.cap) {
+    if (t.d) CPM_HIP(hipFree(t.d));
+    t.cap = std::max(t.h.size(), 2 * t.cap);
+    CPM_HIP(hipMalloc(&t.d, sizeof(Seg) * t.cap));
+  }
+  if (t.map.size() > t.capmap) {
+    if (t.dmap) CPM_HIP(hipFree(t.dmap));
+    t.capmap = std::max(t.map.size(), t.capmap + t.capmap / 2);
+    CPM_HIP(hipMalloc(&t.dmap, sizeof(int32_t) * t.capmap));
+  }
+  if (!t.h.empty()) CPM_HIP(hipMemcpyAsync(t.d, t.h.data(), sizeof(Seg) * t.h.size(), hipMemcpyHostToDevice, s));
+  if (!t.map.empty())
+    CPM_HIP(hipMemcpyAsync(t.dmap, t.map.data(), sizeof(int32_t) * t.map.size(), hipMemcpyHostToDevice, s));
+  t.G = (int)t.map.size();
+  return CPM_OK;
+}
+
+template <class Seg>
+void free_table(Table<Seg>& t) {
+  if (t.d) (void)hipFree(t.d);
+  if (t.dmap) (void)hipFree(t.dmap);
+  t.d = nullptr, t.dmap = nullptr, t.cap = t.capmap = 0;
+}
+
+}  // namespace
+
+struct MultiMgTables {
+  Table<OuterSeg> outer;               // map: vector workgroups
+  std::vector<int32_t> smap;           // SpMV workgroups
+  int32_t* dsmap = nullptr;
+  size_t capsmap = 0;
+  std::vector<Table<SmoothSeg>> smooth;  // per depth
+  std::vector<Table<DenseSeg>> dense;    // per depth
+  CgState* st = nullptr;
+  CgState* hst = nullptr;
+  double* partials = nullptr;
+  size_t cap_sys = 0, cap_part = 0;
+  ~MultiMgTables() {
+    free_table(outer);
+    for (auto& t : smooth) free_table(t);
+    for (auto& t : dense) free_table(t);
+    if (st) (void)hipFree(st);
+    if (hst) (void)hipHostFree(hst);
+    if (partials) (void)hipFree(partials);
+    if (dsmap) (void)hipFree(dsmap);
+  }
+};
+
+MultiMgSolver::MultiMgSolver() : T(new MultiMgTables()) {}
+MultiMgSolver::~MultiMgSolver() { delete T; }
+
+template <int BS, class T>
+static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sys, int hint, int32_t* iters,
+               double* resid, int* failed) {
+  hipStream_t s = ctx->stream;
+  const int ns = (int)sys.size();
+  const int m = sys[0].M->degree;
+  int depth = 0;  // deepest level index over the batch
+  for (const auto& y : sys) depth = std::max(depth, (int)y.M->L.size() - 1);
+  if (ns > (int)W.cap_sys) {
+    if (W.st) CPM_HIP(hipFree(W.st));
+    if (W.hst) CPM_HIP(hipHostFree(W.hst));
+    W.cap_sys = std::max((size_t)ns, 2 * W.cap_sys);
+    CPM_HIP(hipMalloc(&W.st, sizeof(CgState) * W.cap_sys));
+    CPM_HIP(hipHostMalloc(&W.hst, sizeof(CgState) * W.cap_sys, hipHostMallocDefault));
+  }
+  // outer (fine level, FP64) segments
+  auto& O = W.outer;
+  O.h.clear();
+  O.map.clear();
+  W.smap.clear();
+  for (int j = 0; j < ns; ++j) {
+    const MultiMgSys& y = sys[j];
+    cpm_mg* M = y.M;
+    const Sell& S = M->L[0].A->sell;
+    OuterSeg o{};
+    o.sptr = S.sptr, o.col = S.col, o.v64 = S.v64, o.dg64 = S.dg64;
+    o.ga = GridArg{S.dims[0], S.dims[1], S.dims[2], S.nst, S.soff};
+    o.nb = S.nbrows, o.stencil = S.stencil, o.n = M->L[0].n;
+    o.b = y.b, o.x = y.x, o.r = M->r, o.p = M->p, o.q = M->q, o.rold = M->rold, o.z = M->L[0].x;
+    o.st = W.st + j;
+    o.blk0 = (int32_t)O.map.size();
+    o.nblk = grid_for(o.n);
+    o.sblk0 = (int32_t)W.smap.size();
+    o.snblk = sell_grid(S);
+    o.use_x0 = y.use_x0, o.maxiter = y.maxiter, o.tol = y.tol;
+    for (int b = 0; b < o.nblk; ++b) O.map.push_back(j);
+    for (int b = 0; b < o.snblk; ++b) W.smap.push_back(j);
+    O.h.push_back(o);
+  }
+  // V-cycle segments per depth
+  if ((int)W.smooth.size() < depth + 1) W.smooth.resize(depth + 1);
+  if ((int)W.dense.size() < depth + 1) W.dense.resize(depth + 1);
+  for (int d = 0; d <= depth; ++d) {
+    auto& Sm = W.smooth[d];
+    auto& Dn = W.dense[d];
+    Sm.h.clear(), Sm.map.clear(), Dn.h.clear(), Dn.map.clear();
+    for (int j = 0; j < ns; ++j) {
+      cpm_mg* M = sys[j].M;
+      const int last = (int)M->L.size() - 1;
+      if (d < last) {
+        const MgLevel& Lv = M->L[d];
+        const MgLevel& Lc = M->L[d + 1];
+        const Sell& S = Lv.A->sell;
+        SmoothSeg g{};
+        g.sptr = S.sptr, g.col = S.col;
+        g.val = (M->prec == 32) ? (const void*)S.v32 : (const void*)S.v64;
+        g.dg = (M->prec == 32) ? S.dgc : S.dg64;
+        g.ga = GridArg{S.dims[0], S.dims[1], S.dims[2], S.nst, S.soff};
+        g.nb = S.nbrows, g.stencil = S.stencil;
+        g.x = Lv.x, g.r = Lv.r, g.b = Lv.b, g.d0 = Lv.d0, g.d1 = Lv.d1, g.dinv = Lv.dinv;
+        g.b64 = (d == 0) ? M->r : nullptr;
+        g.c0 = M->c0[d];
+        for (int k = 0; k < 8; ++k) g.c1[k] = M->c1[d * 8 + k], g.c2[k] = M->c2[d * 8 + k];
+        g.fd = Dims3{{Lv.dims[0], Lv.dims[1], Lv.dims[2]}};
+        g.cd = Dims3{{Lc.dims[0], Lc.dims[1], Lc.dims[2]}};
+        g.bc = Lc.b, g.xc = Lc.x;
+        g.st = W.st + j;
+        g.blk0 = (int32_t)Sm.map.size();
+        g.nblk = sell_grid(S);
+        for (int b = 0; b < g.nblk; ++b) Sm.map.push_back((int32_t)Sm.h.size());
+        Sm.h.push_back(g);
+      } else if (d == last) {
+        DenseSeg e{};
+        e.n = M->L[d].n, e.cinv = M->cinv, e.b = M->L[d].b, e.x = M->L[d].x, e.st = W.st + j;
+        e.blk0 = (int32_t)Dn.map.size();
+        e.nblk = (int32_t)std::max<int64_t>(1, std::min<int64_t>(kMaxGrid, (e.n + 3) / 4));
+        for (int b = 0; b < e.nblk; ++b) Dn.map.push_back((int32_t)Dn.h.size());
+        Dn.h.push_back(e);
+      }
+    }
+    if (int rc = upload(Sm, s)) return rc;
+    if (int rc = upload(Dn, s)) return rc;
+  }
+  if (int rc = upload(O, s)) return rc;
+  if (W.smap.size() > W.capsmap) {
+    if (W.dsmap) CPM_HIP(hipFree(W.dsmap));
+    W.capsmap = std::max(W.smap.size(), W.capsmap + W.capsmap / 2);
+    CPM_HIP(hipMalloc(&W.dsmap, sizeof(int32_t) * W.capsmap));
+  }
+  CPM_HIP(hipMemcpyAsync(W.dsmap, W.smap.data(), sizeof(int32_t) * W.smap.size(), hipMemcpyHostToDevice, s));
+  const int G = O.G, Gs = (int)W.smap.size();
+  const size_t need = 2 * (size_t)std::max(G, Gs);
+  if (need > W.cap_part) {
+    if (W.partials) CPM_HIP(hipFree(W.partials));
+    W.cap_part = std::max(need, W.cap_part + W.cap_part / 2);
+    CPM_HIP(hipMalloc(&W.partials, sizeof(double) * W.cap_part));
+  }
+  double* pt = W.partials;
+  bool any_x0 = false;
+  int maxit = 0;
+  for (const auto& y : sys) any_x0 |= y.use_x0 != 0, maxit = std::max(maxit, (int)y.maxiter);
+
+  auto vcycle = [&]() {
+    for (int d = 0; d < depth; ++d) {
+      auto& Sm = W.smooth[d];
+      if (Sm.G) {
+        if (d == 0)
+          kbm_first<BS, T, true, true><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap);
+        else
+          kbm_first<BS, T, true, false><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap);
+        for (int k = 1; k < m; ++k) kbm_sweep<BS, T, 0><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap, k);
+        kbm_sweep<BS, T, 1><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap, m - 1);
+        kbm_restrict<BS, T><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap);
+      }
+      auto& Dn = W.dense[d + 1];
+      if (Dn.G) kbm_dense<T><<<Dn.G, kBlock, 0, s>>>(Dn.d, Dn.dmap);
+    }
+    for (int d = depth - 1; d >= 0; --d) {
+      auto& Sm = W.smooth[d];
+      if (!Sm.G) continue;
+      kbm_prolong<BS, T><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap);
+      kbm_sweep<BS, T, 2><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap, 0);
+      kbm_first<BS, T, false, false><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap);
+      for (int k = 1; k < m; ++k) kbm_sweep<BS, T, 0><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap, k);
+    }
+  };
+
+  // start
+  if (any_x0) kbm_spmv<BS, true><<<Gs, kBlock, 0, s>>>(O.d, W.dsmap, pt);
+  kbm_init<<<G, kBlock, 0, s>>>(O.d, O.dmap, pt);
+  kbm_red<MRED_INIT><<<ns, kRedBlock, 0, s>>>(O.d, G, pt);
+  if (any_x0) kbm_zero_b<<<G, kBlock, 0, s>>>(O.d, O.dmap);
+  vcycle();
+  kbm_dot<T><<<G, kBlock, 0, s>>>(O.d, O.dmap, pt);
+  kbm_red<MRED_RZ><<<ns, kRedBlock, 0, s>>>(O.d, G, pt);
+  CPM_HIP(hipGetLastError());
+  auto issue = [&](int todo) -> int {
+    for (int k = 0; k < todo; ++k) {
+      kbm_pupd<T><<<G, kBlock, 0, s>>>(O.d, O.dmap);
+      kbm_spmv<BS, false><<<Gs, kBlock, 0, s>>>(O.d, W.dsmap, pt);
+      kbm_red<MRED_ALPHA><<<ns, kRedBlock, 0, s>>>(O.d, Gs, pt);
+      kbm_xr<<<G, kBlock, 0, s>>>(O.d, O.dmap, pt);
+      kbm_red<MRED_RR><<<ns, kRedBlock, 0, s>>>(O.d, G, pt);
+      vcycle();
+      kbm_dot<T><<<G, kBlock, 0, s>>>(O.d, O.dmap, pt);
+      kbm_red<MRED_RZ><<<ns, kRedBlock, 0, s>>>(O.d, G, pt);
+    }
+    CPM_HIP(hipGetLastError());
+    return CPM_OK;
+  };
+  auto read = [&]() -> int {
+    CPM_HIP(hipMemcpyAsync(W.hst, W.st, sizeof(CgState) * ns, hipMemcpyDeviceToHost, s));
+    CPM_HIP(hipStreamSynchronize(s));
+    return CPM_OK;
+  };
+  auto all_done = [&]() {
+    for (int j = 0; j < ns; ++j)
+      if (!W.hst[j].done) return false;
+    return true;
+  };
+  if (int rc = read()) return rc;
+  int batch = hint > 2 ? hint - 1 : 4;
+  int issued = 0;
+  while (!all_done() && issued <= maxit) {
+    const int todo = std::max(1, std::min(std::min(batch, 1024), maxit - issued + 1));
+    if (int rc = issue(todo)) return rc;
+    issued += todo;
+    if (int rc = read()) return rc;
+    int it_max = 0;
+    for (int j = 0; j < ns; ++j)
+      if (!W.hst[j].done) it_max = std::max(it_max, (int)W.hst[j].it);
+    batch = std::max(2, std::min(16, it_max / 4 + 2));
+  }
+  for (int j = 0; j < ns; ++j) {
+    iters[j] = W.hst[j].it;
+    resid[j] = W.hst[j].resid;
+    failed[j] = (W.hst[j].fail || !W.hst[j].done) ? 1 : 0;
+  }
+  return CPM_OK;
+}
+
+int multi_mg_pcg(cpm_ctx* ctx, MultiMgSolver& W, const std::vector<MultiMgSys>& sys, int hint, int32_t* iters,
+                 double* resid, int* failed) {
+  if (sys.empty()) return CPM_OK;
+  const cpm_mg* M0 = sys[0].M;
+  for (const auto& y : sys) {
+    if (!y.M || !y.b || !y.x) return fail(CPM_EINVAL, "batched multigrid CG: null system");
+    if (y.M->bs != M0->bs || y.M->prec != M0->prec || y.M->degree != M0->degree)
+      return fail(CPM_EINVAL, "batched multigrid CG: hierarchies differ in block size, precision or degree");
+    if (y.M->L.size() > 16) return fail(CPM_EINVAL, "batched multigrid CG: too many levels");
+  }
+  for (size_t i = 0; i < sys.size(); ++i)
+    for (size_t j = i + 1; j < sys.size(); ++j)
+      if (sys[i].M == sys[j].M)
+        return fail(CPM_EINVAL, "batched multigrid CG: two systems share one hierarchy (and its workspace)");
+  if (M0->bs == 3)
+    return M0->prec == 32 ? run<3, float>(ctx, *W.T, sys, hint, iters, resid, failed)
+                          : run<3, double>(ctx, *W.T, sys, hint, iters, resid, failed);
+  return M0->prec == 32 ? run<1, float>(ctx, *W.T, sys, hint, iters, resid, failed)
+                        : run<1, double>(ctx, *W.T, sys, hint, iters, resid, failed);
+}
+
+}  // namespace cpm
+
+using namespace cpm;
+
+extern "C" int cpm_cg_mg_batch(cpm_ctx* ctx, int count, cpm_mg* const* mgs, const double* const* b, double* const* x,
+                               const int32_t* use_x0, double tol, int maxiter, int32_t* iters, double* resid,
+                               int32_t* failed) {
+  if (!ctx || count < 0 || (count > 0 && (!mgs || !b || !x || !iters || !resid || !failed)))
+    return fail(CPM_EINVAL, "cpm_cg_mg_batch: null argument");
+  if (count == 0) return CPM_OK;
+  CPM_HIP(hipSetDevice(ctx->device));
+  std::vector<MultiMgSys> sys(count);
+  for (int j = 0; j < count; ++j) {
+    if (mgs[j] && mgs[j]->ctx != ctx) return fail(CPM_EINVAL, "cpm_cg_mg_batch: a hierarchy uses another context");
+    sys[j].M = mgs[j];
+    sys[j].b = b[j];
+    sys[j].x = x[j];
+    sys[j].use_x0 = use_x0 ? use_x0[j] : 0;
+    sys[j].tol = tol;
+    sys[j].maxiter = maxiter;
+  }
+  MultiMgSolver W;
+  std::vector<int> f(count);
+  if (int rc = multi_mg_pcg(ctx, W, sys, 0, iters, resid, f.data())) return rc;
+  for (int j = 0; j < count; ++j) failed[j] = f[j];
+  return CPM_OK;
+}

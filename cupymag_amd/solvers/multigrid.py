@@ -70,6 +70,32 @@ class KuhnMultigrid:
             self.h = None
 
 
+def solve_batch(mgs, rhs, x0s=None, tol=1e-7, maxiter=5000):
+    """``KuhnMultigrid.solve`` on many independent systems at once
+    (cpm_cg_mg_batch): one launch per phase for the whole batch.  mgs:
+    hierarchies (one block size, precision and degree; distinct), rhs: their
+    right-hand sides, x0s: optional initial guesses (used where not None).
+    Each system's (x, iterations, residual) equals ``solve``'s bit for bit.
+    Raises the reference's error when a system reaches maxiter."""
+    if not mgs:
+        return []
+    ctx = mgs[0].ctx
+    rhs = [as_device(b, ctx) for b in rhs]
+    x0s = list(x0s) if x0s is not None else [None] * len(mgs)
+    xs = [x0.clone() if x0 is not None else torch.empty_like(b) for b, x0 in zip(rhs, x0s)]
+    n = len(mgs)
+    h = (C.c_void_p * n)(*[m.h.value for m in mgs])
+    bp = (C.c_void_p * n)(*[b.data_ptr() for b in rhs])
+    xp = (C.c_void_p * n)(*[x.data_ptr() for x in xs])
+    ux = (C.c_int32 * n)(*[int(x0 is not None) for x0 in x0s])
+    it, res, fl = (C.c_int32 * n)(), (C.c_double * n)(), (C.c_int32 * n)()
+    check(lib().cpm_cg_mg_batch(ctx.h, n, h, bp, xp, ux, float(tol), int(maxiter), it, res, fl), "cpm_cg_mg_batch")
+    for j in range(n):
+        if fl[j]:
+            raise RuntimeError(f"Error! CG did not converge. info={it[j]}, residual={res[j]}.")
+    return [(x, it[j], res[j]) for j, x in enumerate(xs)]
+
+
 def _interleave_perm(n, bs):
     """perm[i*bs + d] = d*n + i: SoA (reference) order -> node-major order."""
     i = np.arange(n)

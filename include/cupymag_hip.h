@@ -231,6 +231,15 @@ int cpm_mg_apply(cpm_mg* mg, const double* r, double* z);
  * positive iteration count when maxiter was reached. */
 int cpm_cg_mg(cpm_mg* mg, const double* b, double* x, int use_x0, double tol, int maxiter, int32_t* iters,
               double* resid);
+/* No counterpart in the reference.  cpm_cg_mg on `count` independent
+ * systems at once (hierarchies of one block size, V-cycle precision and
+ * smoother degree, each at most once, all on ctx): one launch per phase for
+ * the whole batch, V-cycles level by level whatever the depths.  Each
+ * system's result, iteration count and residual equal cpm_cg_mg's on the
+ * same input bit for bit.  failed[j] = 1 when system j reached maxiter;
+ * returns 0 or an error < 0. */
+int cpm_cg_mg_batch(cpm_ctx* ctx, int count, cpm_mg* const* mgs, const double* const* b, double* const* x,
+                    const int32_t* use_x0, double tol, int maxiter, int32_t* iters, double* resid, int32_t* failed);
 
 /* ------------------------------------------------------------------ */
 /* matrix-free magnetoelastic right-hand side                           */
@@ -345,6 +354,10 @@ int cpm_sim_step(cpm_sim* sim, const double* H, int use_init, cpm_step_report* r
  * batched CG over every member (one launch per CG phase for the whole
  * batch), so small meshes share the launches of the large ones.  Members
  * must use the batch's context and one of the SELL modes (all the same).
+ * Members created with params.precond run their elasticity and demag
+ * solves as one batched multigrid-preconditioned CG (V-cycles of all
+ * hierarchies level by level, any depths) and the others as batched plain
+ * CG; members that share a hierarchy are solved in turn.
  * A member's arithmetic does not depend on the other members: its results
  * are bit-identical alone and in any batch.  Members can be added and
  * removed between steps (continuous batching); a member cannot be stepped

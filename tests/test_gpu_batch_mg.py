@@ -1,0 +1,153 @@
+"""Batched multigrid-preconditioned CG (cpm_cg_mg_batch, and the batched
+elasticity/demag solves of cpm_batch_step for members created with
+precond="mg").
+
+* Hierarchies of different depths (10^3 -> 5^3, 20^3 -> 10^3 -> 5^3,
+  12^3 -> 6^3 Kuhn meshes), scalar and 3x3-block operators, FP32 and FP64
+  V-cycles: every system's solution, iteration count and residual equal
+  the one-system solve (cpm_cg_mg) bit for bit, with and without initial
+  guesses, for a zero right-hand side too; maxiter reached raises.
+* GSPM batches mixing multigrid members of different depths, a member
+  without the preconditioner and two members on one set of operators (one
+  hierarchy, solved in turn): each member's m, u, iteration counts and
+  sumsq equal its run alone in a batch bit for bit, and fast64 members stay
+  within 1e-8 of the FP64 oracle (the single multigrid path's tolerance).
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from tests.cases import config, oracle_setup, tilted_m  # noqa: E402
+
+
+def _ops(n, me=True, jitter=0.1, seed=3):
+    from cupymag_amd.core.assemble_simulation_operators import SimulationOperators
+    from cupymag_amd.core.parameters import from_dict
+    from cupymag_amd.mesh.kuhn import kuhn_mesh
+    cfg = config("Tet", n=n, me=me, rot=True, dt=5e-12, defect_params=2.5)
+    arr = kuhn_mesh(n, jitter=jitter, seed=seed)
+    P = from_dict(cfg)
+    return cfg, arr, P, SimulationOperators(P, mesh_arrays=arr, keep_host=True)
+
+
+@pytest.fixture(scope="module")
+def meshes():
+    return [_ops(n) for n in (10, 20, 12)]
+
+
+@pytest.mark.parametrize("name", ["A_demag", "A_el"])
+@pytest.mark.parametrize("prec", [32, 64])
+def test_solve_batch_equals_single_solves(meshes, name, prec):
+    from cupymag_amd.solvers.multigrid import solve_batch
+    mgs = [S.multigrid(precision=prec)[name] for _, _, _, S in meshes]
+    assert [m.info()["levels"] for m in mgs] == [2, 3, 2]
+    bs = 3 if name == "A_el" else 1
+    rng = np.random.default_rng(11)
+    rhs, x0s = [], []
+    for k, m in enumerate(mgs):
+        n = m.levels[0].shape[0]
+        b = rng.standard_normal(n)
+        b[:bs] = 0.0
+        rhs.append(torch.as_tensor(b, device="cuda"))
+        x0s.append(torch.as_tensor(rng.standard_normal(n) * 0.1, device="cuda") if k == 1 else None)
+    got = solve_batch(mgs, rhs, x0s, tol=1e-10, maxiter=500)
+    for m, b, x0, (x, it, res) in zip(mgs, rhs, x0s, got):
+        xs, its, ress = m.solve(b, x0=x0, tol=1e-10, maxiter=500, use_init=x0 is not None)
+        assert it == its and 0 < it < 60
+        assert res == ress
+        np.testing.assert_array_equal(x.cpu().numpy(), xs.cpu().numpy())
+    # a zero right-hand side among others: x = 0 after no iteration
+    rhs[0] = torch.zeros_like(rhs[0])
+    got = solve_batch(mgs, rhs, [None, x0s[1], None], tol=1e-10, maxiter=500)
+    assert got[0][1] == 0 and float(got[0][0].abs().max()) == 0.0
+    for m, b, x0, (x, it, res) in list(zip(mgs, rhs, x0s, got))[1:]:
+        xs, its, _ = m.solve(b, x0=x0, tol=1e-10, maxiter=500, use_init=x0 is not None)
+        assert it == its
+        np.testing.assert_array_equal(x.cpu().numpy(), xs.cpu().numpy())
+    with pytest.raises(RuntimeError, match="did not converge"):
+        solve_batch(mgs, rhs, tol=1e-14, maxiter=2)
+
+
+def test_solve_batch_rejects_shared_or_mismatched_hierarchies(meshes):
+    from cupymag_amd.solvers.multigrid import solve_batch
+    S0, S1 = meshes[0][3], meshes[1][3]
+    m0 = S0.multigrid(precision=32)["A_demag"]
+    b = torch.zeros(m0.levels[0].shape[0], dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError, match="share one hierarchy"):
+        solve_batch([m0, m0], [b, b])
+    m1 = S1.multigrid(precision=64)["A_demag"]
+    b1 = torch.zeros(m1.levels[0].shape[0], dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError, match="differ"):
+        solve_batch([m0, m1], [b, b1])
+
+
+def _H(P):
+    return [P.Hext1, P.Hext2, P.Hext3]
+
+
+def _state(sim):
+    out = [sim.get("M").cpu().numpy()]
+    if sim.P.ME:
+        out.append(sim.get("UEL").cpu().numpy())
+    return out
+
+
+@pytest.mark.parametrize("mode", ["fast64", "mixed"])
+def test_batch_step_with_multigrid_members(meshes, mode):
+    """Members: 10^3 (ME, multigrid), 20^3 (no ME, multigrid, 3 levels),
+    12^3 (ME, multigrid), 12^3 again on the same operators (one hierarchy,
+    solved in a second round), 8^3 without the preconditioner."""
+    from cupymag_amd.core.batch import GSPMBatch
+    from cupymag_amd.core.gspm import GSPMSimulation
+    from oracle import gspm as og
+    _, _, P20, S20 = _ops(20, me=False, seed=4)
+    _, _, P8, S8 = _ops(8)
+    specs = [(meshes[0][2], meshes[0][3], "mg"), (P20, S20, "mg"), (meshes[2][2], meshes[2][3], "mg"),
+             (meshes[2][2], meshes[2][3], "mg"), (P8, S8, None)]
+    nsteps = 3
+
+    def fresh(k):
+        P, S, pc = specs[k]
+        sim = GSPMSimulation(S, P, mode=mode, precond=pc)
+        sim.set_m(tilted_m(S.n_dof, seed=50 + k))
+        return sim
+
+    solo = []
+    for k in range(len(specs)):
+        sim = fresh(k)
+        b = GSPMBatch(sim.ctx)
+        b.add(sim)
+        hist = []
+        for s in range(nsteps):
+            rep = b.step([_H(specs[k][0])], [s > 0])[0]
+            hist.append((_state(sim), rep.iters, rep.sumsq))
+        solo.append(hist)
+        b.remove(sim)
+        del b, sim
+
+    sims = [fresh(k) for k in range(len(specs))]
+    b = GSPMBatch(sims[0].ctx)
+    for sim in sims:
+        b.add(sim)
+    for s in range(nsteps):
+        reps = b.step([_H(P) for P, _, _ in specs], [s > 0] * len(specs))
+        for k, rep in enumerate(reps):
+            st, iters, sumsq = solo[k][s]
+            for a, r in zip(_state(sims[k]), st):
+                np.testing.assert_array_equal(a, r)
+            assert rep.iters == iters and rep.sumsq == sumsq
+            if specs[k][2] == "mg":
+                assert 0 < rep.iters[1] < 40
+    if mode != "fast64":
+        return
+    # oracle check of the 12^3 ME multigrid member (from its seed, same steps)
+    cfg, arr, P, S = meshes[2]
+    p, ops, Avg, _ = oracle_setup(cfg, arr)
+    st = og.init_state(ops, Avg, tilted_m(ops["n"], seed=52))
+    p["H_now"] = p["H"]
+    for s in range(nsteps):
+        st["use_init"] = s > 0
+        og.step(st, ops, Avg, p, [])
+    assert np.max(np.abs(sims[2].get("M").cpu().numpy() - st["m"])) < 1e-8

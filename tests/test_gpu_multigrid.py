@@ -174,8 +174,9 @@ def test_multigrid_rejections():
         GSPMSimulation(S, P, mode="fp64", precond="mg")
     sim = GSPMSimulation(S, P, mode="mixed", precond="mg")
     b = GSPMBatch(sim.ctx)
-    with pytest.raises(ValueError, match="plain CG"):
-        b.add(sim)
+    b.add(sim)  # batched multigrid solves (test_gpu_batch_mg.py)
+    with pytest.raises(ValueError, match="same mode"):
+        b.add(GSPMSimulation(S, P, mode="fast64", precond="mg"))
     _, _, P7, S7 = _setup(n=7, me=False)
     with pytest.raises(ValueError, match="multigrid needs"):
         GSPMSimulation(S7, P7, mode="mixed", precond="mg")
