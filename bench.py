@@ -233,9 +233,10 @@ def cpu_baseline(n_cpu, steps=1):
                        f"({ops['n']} DOFs), {steps} step(s), {dt:.1f}s")
 
 
-def _build_instances(descs, ctx, workers=4):
+def _build_instances(descs, ctx, workers=4, multigrid=False):
     """SimulationOperators of every instance description (mesh, host
-    assembly, upload), built on a few host threads."""
+    assembly, upload), built on a few host threads; multigrid: also the
+    FP32 multigrid hierarchies where the mesh has them."""
     import concurrent.futures as cf
 
     import torch
@@ -249,6 +250,8 @@ def _build_instances(descs, ctx, workers=4):
         P = from_dict(d["cfg"])
         S = SimulationOperators(P, mesh_arrays=kuhn_mesh(d["n"]), ctx=ctx, keep_host=False, matrix_free_fel=True)
         S.csr_demag_and_deriv()
+        if multigrid:
+            S.multigrid(precision=32)
         return P, S
 
     with cf.ThreadPoolExecutor(max_workers=workers) as ex:
@@ -270,17 +273,21 @@ def run_batch(args):
     ctx = context(0)
     descs = batch_workload(args.count, seed=args.seed, nmin=args.nmin, nmax=args.nmax)
     t0 = time.time()
-    built = _build_instances(descs, ctx)
+    use_mg = args.precond == "mg"
+    built = _build_instances(descs, ctx, multigrid=use_mg)
     ndof = [S.n_dof for _, S in built]
+    # members whose mesh has a multigrid hierarchy use it (n even, coarsest <= 10^3)
+    pc = ["mg" if use_mg and S.multigrid(precision=32) is not None else None for _, S in built]
     log(f"[bench] {len(built)} instances, n = {min(d['n'] for d in descs)}..{max(d['n'] for d in descs)}, "
-        f"{sum(ndof)} DOFs in total, set up in {time.time() - t0:.1f}s")
+        f"{sum(ndof)} DOFs in total, {sum(p is not None for p in pc)} with multigrid, "
+        f"set up in {time.time() - t0:.1f}s")
     total = float(sum(ndof))
     res = {}
 
     def fresh():
         sims = []
-        for (P, S), d in zip(built, descs):
-            sim = GSPMSimulation(S, P, mode="mixed", rel_delta=args.rel_delta)
+        for (P, S), d, prc in zip(built, descs, pc):
+            sim = GSPMSimulation(S, P, mode="mixed", rel_delta=args.rel_delta, precond=prc)
             sim.set_m(init_m(S.n_dof, d["m_seed"]))
             sims.append(sim)
         return sims
@@ -334,9 +341,12 @@ def run_batch(args):
         "data": "synthetic",
         "config": {"workload": f"batch of {len(built)} independent GSPM simulations, Kuhn n^3 meshes with n "
                                f"log-uniform in [{args.nmin},{args.nmax}], sphere defects, random Hext, "
-                               f"magnetoelastic on in every other one, mixed precision, dt 10 ps, alpha 0.1",
+                               f"magnetoelastic on in every other one, mixed precision, dt 10 ps, alpha 0.1"
+                               + (", multigrid-preconditioned elasticity/demag solves where the mesh has a "
+                                  "hierarchy" if use_mg else ""),
                    "path": head, "instances": len(built), "n_dof_total": int(total),
-                   "n": [d["n"] for d in descs]},
+                   "n": [d["n"] for d in descs], "precond": args.precond,
+                   "members_with_multigrid": sum(p is not None for p in pc)},
         "speedup_vs_sequential": res[head]["node_steps_per_s"] / res["sequential"]["node_steps_per_s"],
         "paths": res,
     }
@@ -424,7 +434,8 @@ def run_serve(args):
     ctx = context(0)
     reqs = sweep_requests(args.count, rate=args.rate, seed=args.seed, nmin=args.nmin, nmax=args.nmax,
                           points=args.points, steps_per_point=args.steps_per_point)
-    srv = SweepServer(ctx, hbm_budget=args.hbm_budget_gb * 1e9, mode="mixed", graphs=args.graphs)
+    srv = SweepServer(ctx, hbm_budget=args.hbm_budget_gb * 1e9, mode="mixed", graphs=args.graphs,
+                      precond=args.precond if args.precond != "none" else None)
     results, st = srv.run(reqs, log=log)
     out = {
         "metric": "LLG node-steps/s", "value": st["node_steps_per_s"], "unit": "node-steps/s", "n_gpus": 1,
@@ -435,8 +446,9 @@ def run_serve(args):
                                f"{args.rate}/s, Kuhn n^3 with n log-uniform in [{args.nmin},{args.nmax}], "
                                f"random rotated-ellipsoid defect, magnetoelastic on, {args.points} field points "
                                f"in [-1e5, 1e5] A/m x {args.steps_per_point} steps, HBM budget "
-                               f"{args.hbm_budget_gb} GB, mixed precision",
-                   "requests": args.count, "n": [r["n"] for r in reqs]},
+                               f"{args.hbm_budget_gb} GB, mixed precision"
+                               + (", multigrid where the mesh has a hierarchy" if args.precond == "mg" else ""),
+                   "requests": args.count, "n": [r["n"] for r in reqs], "precond": args.precond},
         "latency_s": {"per_point": st["point_latency_s"], "per_loop": st["loop_latency_s"],
                       "queue_delay": st["queue_delay_s"]},
         "serving": st,
@@ -477,6 +489,9 @@ def main():
     ap.add_argument("--steps-per-point", type=int, default=200)
     ap.add_argument("--hbm-budget-gb", type=float, default=240.0)
     ap.add_argument("--graphs", action="store_true", help="serve: graph replay of the CG iterations")
+    ap.add_argument("--precond", choices=("none", "mg"), default="none",
+                    help="batch / serve: multigrid-preconditioned elasticity and demag solves for the members "
+                         "whose mesh has a hierarchy")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the GSPM step of one simulation is not sharded across GPUs "

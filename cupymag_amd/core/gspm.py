@@ -63,8 +63,7 @@ class GSPMSimulation:
                 raise ValueError("precond is a fast-mode option")
             self.mg = SimOp.multigrid(precision=32 if mode == "mixed" else 64)
             if self.mg is None:
-                raise ValueError("multigrid needs a structured Kuhn mesh with an even number of cells per "
-                                 "dimension")
+                raise ValueError("multigrid needs a structured Kuhn mesh of more than 8 cells per dimension")
         mg_el = self.mg["A_el"].h if (self.mg and "A_el" in self.mg) else None
         mg_dm = self.mg["A_demag"].h if self.mg else None
         from ..physics.fel import MatrixFreeFel

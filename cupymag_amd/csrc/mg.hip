@@ -2,16 +2,21 @@
 // anchored stiffness operators (elasticity, bs = 3, and demag, bs = 1) on
 // structured periodic Kuhn meshes.
 //
-// Levels: level l+1 is the Kuhn mesh of the even nodes of level l (half the
-// nodes per dimension), its operator assembled on that mesh by the host
-// assembler (rediscretisation).  The Kuhn (Freudenthal) triangulation of
-// cell size h refines the one of cell size 2h, so continuous P1 functions of
-// the coarse mesh are P1 functions of the fine mesh and the prolongation is
-// their nodal interpolation: fine node 2I + d, d in {0,1}^3, takes the
-// coarse value at I (d = 0) or the mean of the values at I and I + d (the
-// coarse edge in direction d), periodic in every dimension.  Restriction is
-// its transpose.  Node 0 carries the operators' anchor (identity row and
-// column): it is left out of both transfers, so corrections never touch it.
+// Levels: a level of n cells per dimension has a coarser one of ceil(n/2)
+// cells, its operator assembled on that mesh by the host assembler
+// (rediscretisation).  For even n the coarse mesh is the Kuhn mesh of the
+// even nodes: the Kuhn (Freudenthal) triangulation of cell size h refines
+// the one of cell size 2h, so continuous P1 functions of the coarse mesh are
+// P1 functions of the fine mesh and the prolongation is their nodal
+// interpolation: fine node 2I + d, d in {0,1}^3, takes the coarse value at I
+// (d = 0) or the mean of the values at I and I + d (the coarse edge in
+// direction d), periodic in every dimension.  For odd n the coarse mesh is
+// the regular Kuhn mesh of (n+1)/2 cells of the same periodic box and the
+// prolongation is the same P1 interpolation at the fine nodes' nominal
+// positions (non-nested: barycentric weights in the coarse Kuhn
+// tetrahedron that holds the point).  Restriction is its transpose.  Node 0
+// carries the operators' anchor (identity row and column): it is left out
+// of both transfers, so corrections never touch it.
 //
 // Smoother: Chebyshev iteration of degree m on D^-1 A (D = diagonal blocks,
 // inverted once), on the interval [lmax/ratio, lmax] with lmax = 1.1 x a
@@ -494,8 +499,8 @@ extern "C" int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, 
       const Sell& F = levels[l - 1]->sell;
       const Sell& C = A->sell;
       for (int a = 0; a < 3; ++a)
-        if (F.dims[a] != 2 * C.dims[a])
-          return fail(CPM_EINVAL, "cpm_mg_create: each level must halve the grid in every dimension");
+        if (C.dims[a] != (F.dims[a] + 1) / 2)
+          return fail(CPM_EINVAL, "cpm_mg_create: each level must halve the grid (rounding up) in every dimension");
     }
   }
   CPM_HIP(hipSetDevice(ctx->device));

@@ -100,12 +100,54 @@ struct Dims3 {
   int32_t d[3];
 };
 
-// b_c = R r_f (transpose of the nested P1 interpolation), anchor excluded.
+__device__ __forceinline__ bool nested_dims(const Dims3& fine, const Dims3& coarse) {
+  return fine.d[0] == 2 * coarse.d[0] && fine.d[1] == 2 * coarse.d[1] && fine.d[2] == 2 * coarse.d[2];
+}
+
+// P1 interpolation weights of fine node (i,j,k) on the coarse Kuhn mesh of
+// the same periodic box (fine node i lies at coarse coordinate i*nc/nf per
+// axis): cell c, fractions t sorted t_p0 >= t_p1 >= t_p2, vertices c,
+// c + e_p0, c + e_p0 + e_p1, c + (1,1,1) with weights 1 - t_p0, t_p0 - t_p1,
+// t_p1 - t_p2, t_p2 (barycentric coordinates in the Kuhn tetrahedron that
+// holds the point).  For nf = 2 nc these are the nested weights (0.5 pairs).
+__device__ __forceinline__ void kuhn_weights(int32_t i, int32_t j, int32_t k, const Dims3& fine,
+                                             const Dims3& coarse, int64_t (&v)[4], double (&w)[4]) {
+  const int32_t idx[3] = {i, j, k};
+  int32_t cell[3];
+  double t[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int64_t q = (int64_t)idx[a] * coarse.d[a];
+    cell[a] = (int32_t)(q / fine.d[a]);
+    t[a] = (double)(q - (int64_t)cell[a] * fine.d[a]) / (double)fine.d[a];
+  }
+  int p0 = 0, p1 = 1, p2 = 2, tmp;
+  if (t[p1] > t[p0]) tmp = p0, p0 = p1, p1 = tmp;
+  if (t[p2] > t[p1]) tmp = p1, p1 = p2, p2 = tmp;
+  if (t[p1] > t[p0]) tmp = p0, p0 = p1, p1 = tmp;
+  int32_t o[3] = {0, 0, 0};
+  v[0] = wrap_idx(cell[0], cell[1], cell[2], coarse.d);
+  w[0] = 1.0 - t[p0];
+  o[p0] = 1;
+  v[1] = wrap_idx(cell[0] + o[0], cell[1] + o[1], cell[2] + o[2], coarse.d);
+  w[1] = t[p0] - t[p1];
+  o[p1] = 1;
+  v[2] = wrap_idx(cell[0] + o[0], cell[1] + o[1], cell[2] + o[2], coarse.d);
+  w[2] = t[p1] - t[p2];
+  v[3] = wrap_idx(cell[0] + 1, cell[1] + 1, cell[2] + 1, coarse.d);
+  w[3] = t[p2];
+}
+
+// b_c = R r_f (R = P^T of the P1 interpolation), anchor node 0 excluded on
+// both grids.  Nested grids (fine = 2 x coarse) use the fixed stencil; other
+// size pairs gather, for every coarse node, over the fine nodes whose cell
+// is the node's own or the one below it in each dimension.
 template <int BS, class T>
 __device__ __forceinline__ void mg_restrict_rows(int64_t lb, int64_t lg, const Dims3& fine, const Dims3& coarse,
                                                  const T* __restrict__ rf, T* __restrict__ bc) {
   const int64_t nc = (int64_t)coarse.d[0] * coarse.d[1] * coarse.d[2];
   const int64_t stride = lg * kBlock;
+  const bool nested = nested_dims(fine, coarse);
   for (int64_t I = lb * kBlock + threadIdx.x; I < nc; I += stride) {
     double acc[BS];
 #pragma unroll
@@ -114,19 +156,54 @@ __device__ __forceinline__ void mg_restrict_rows(int64_t lb, int64_t lg, const D
       const int32_t i = (int32_t)(I / ((int64_t)coarse.d[1] * coarse.d[2]));
       const int32_t j = (int32_t)((I / coarse.d[2]) % coarse.d[1]);
       const int32_t k = (int32_t)(I % coarse.d[2]);
-      const int64_t f0 = wrap_idx(2 * i, 2 * j, 2 * k, fine.d);
+      if (nested) {
+        const int64_t f0 = wrap_idx(2 * i, 2 * j, 2 * k, fine.d);
 #pragma unroll
-      for (int c = 0; c < BS; ++c) acc[c] = (double)rf[f0 * BS + c];
+        for (int c = 0; c < BS; ++c) acc[c] = (double)rf[f0 * BS + c];
 #pragma unroll
-      for (int dd = 1; dd < 8; ++dd) {
-        const int32_t a = dd & 1, bb = (dd >> 1) & 1, cc = (dd >> 2) & 1;
-        const int64_t fp = wrap_idx(2 * i + a, 2 * j + bb, 2 * k + cc, fine.d);
-        const int64_t fm = wrap_idx(2 * i - a, 2 * j - bb, 2 * k - cc, fine.d);
+        for (int dd = 1; dd < 8; ++dd) {
+          const int32_t a = dd & 1, bb = (dd >> 1) & 1, cc = (dd >> 2) & 1;
+          const int64_t fp = wrap_idx(2 * i + a, 2 * j + bb, 2 * k + cc, fine.d);
+          const int64_t fm = wrap_idx(2 * i - a, 2 * j - bb, 2 * k - cc, fine.d);
 #pragma unroll
-        for (int c = 0; c < BS; ++c) {
-          if (fp != 0) acc[c] += 0.5 * (double)rf[fp * BS + c];
-          if (fm != 0) acc[c] += 0.5 * (double)rf[fm * BS + c];
+          for (int c = 0; c < BS; ++c) {
+            if (fp != 0) acc[c] += 0.5 * (double)rf[fp * BS + c];
+            if (fm != 0) acc[c] += 0.5 * (double)rf[fm * BS + c];
+          }
         }
+      } else {
+        // per dimension: fine indices with cell in {C - 1, C}, i.e.
+        // ceil((C-1) nf / nc) .. ceil((C+1) nf / nc) - 1 (unwrapped)
+        const int32_t C3[3] = {i, j, k};
+        int32_t lo[3], hi[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const int64_t nf = fine.d[a], ncd = coarse.d[a];
+          auto ceil_div = [](int64_t num, int64_t den) -> int64_t {
+            return num >= 0 ? (num + den - 1) / den : -((-num) / den);
+          };
+          lo[a] = (int32_t)ceil_div((int64_t)(C3[a] - 1) * nf, ncd);
+          hi[a] = (int32_t)ceil_div((int64_t)(C3[a] + 1) * nf, ncd) - 1;
+          if (hi[a] - lo[a] + 1 > nf) hi[a] = lo[a] + (int32_t)nf - 1;
+        }
+        for (int32_t fi = lo[0]; fi <= hi[0]; ++fi)
+          for (int32_t fj = lo[1]; fj <= hi[1]; ++fj)
+            for (int32_t fk = lo[2]; fk <= hi[2]; ++fk) {
+              const int32_t wi = fi < 0 ? fi + fine.d[0] : (fi >= fine.d[0] ? fi - fine.d[0] : fi);
+              const int32_t wj = fj < 0 ? fj + fine.d[1] : (fj >= fine.d[1] ? fj - fine.d[1] : fj);
+              const int32_t wk = fk < 0 ? fk + fine.d[2] : (fk >= fine.d[2] ? fk - fine.d[2] : fk);
+              const int64_t f = ((int64_t)wi * fine.d[1] + wj) * fine.d[2] + wk;
+              if (f == 0) continue;
+              int64_t v[4];
+              double w[4];
+              kuhn_weights(wi, wj, wk, fine, coarse, v, w);
+              double wt = 0.0;
+#pragma unroll
+              for (int q = 0; q < 4; ++q) wt += (v[q] == I) ? w[q] : 0.0;
+              if (wt == 0.0) continue;
+#pragma unroll
+              for (int c = 0; c < BS; ++c) acc[c] += wt * (double)rf[f * BS + c];
+            }
       }
     }
 #pragma unroll
@@ -134,29 +211,44 @@ __device__ __forceinline__ void mg_restrict_rows(int64_t lb, int64_t lg, const D
   }
 }
 
-// x_f += P e_c (nested P1 interpolation), anchor excluded.
+// x_f += P e_c (P1 interpolation), anchor excluded on both grids.
 template <int BS, class T>
 __device__ __forceinline__ void mg_prolong_rows(int64_t lb, int64_t lg, const Dims3& fine, const Dims3& coarse,
                                                 const T* __restrict__ ec, T* __restrict__ xf) {
   const int64_t nf = (int64_t)fine.d[0] * fine.d[1] * fine.d[2];
   const int64_t stride = lg * kBlock;
+  const bool nested = nested_dims(fine, coarse);
   for (int64_t f = lb * kBlock + threadIdx.x; f < nf; f += stride) {
     if (f == 0) continue;
     const int32_t i = (int32_t)(f / ((int64_t)fine.d[1] * fine.d[2]));
     const int32_t j = (int32_t)((f / fine.d[2]) % fine.d[1]);
     const int32_t k = (int32_t)(f % fine.d[2]);
-    const int32_t a = i & 1, bb = j & 1, cc = k & 1;
-    const int32_t I = i >> 1, J = j >> 1, K = k >> 1;
-    const int64_t c0 = wrap_idx(I, J, K, coarse.d);
     double v[BS];
-    if ((a | bb | cc) == 0) {
+    if (nested) {
+      const int32_t a = i & 1, bb = j & 1, cc = k & 1;
+      const int32_t I = i >> 1, J = j >> 1, K = k >> 1;
+      const int64_t c0 = wrap_idx(I, J, K, coarse.d);
+      if ((a | bb | cc) == 0) {
 #pragma unroll
-      for (int c = 0; c < BS; ++c) v[c] = (c0 != 0) ? (double)ec[c0 * BS + c] : 0.0;
+        for (int c = 0; c < BS; ++c) v[c] = (c0 != 0) ? (double)ec[c0 * BS + c] : 0.0;
+      } else {
+        const int64_t c1 = wrap_idx(I + a, J + bb, K + cc, coarse.d);
+#pragma unroll
+        for (int c = 0; c < BS; ++c)
+          v[c] = 0.5 * (((c0 != 0) ? (double)ec[c0 * BS + c] : 0.0) + ((c1 != 0) ? (double)ec[c1 * BS + c] : 0.0));
+      }
     } else {
-      const int64_t c1 = wrap_idx(I + a, J + bb, K + cc, coarse.d);
+      int64_t cv[4];
+      double w[4];
+      kuhn_weights(i, j, k, fine, coarse, cv, w);
 #pragma unroll
-      for (int c = 0; c < BS; ++c)
-        v[c] = 0.5 * (((c0 != 0) ? (double)ec[c0 * BS + c] : 0.0) + ((c1 != 0) ? (double)ec[c1 * BS + c] : 0.0));
+      for (int c = 0; c < BS; ++c) v[c] = 0.0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (cv[q] == 0 || w[q] == 0.0) continue;
+#pragma unroll
+        for (int c = 0; c < BS; ++c) v[c] += w[q] * (double)ec[cv[q] * BS + c];
+      }
     }
 #pragma unroll
     for (int c = 0; c < BS; ++c) xf[f * BS + c] = (T)((double)xf[f * BS + c] + v[c]);

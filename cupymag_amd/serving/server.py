@@ -35,11 +35,27 @@ from .workloads import initial_m
 # the matrix-free F_el, the step's state vectors, the batched-CG workspace
 # of the mixed mode)
 _BYTES_PER_DOF = {False: 2200.0, True: 5700.0}
+# FP32 multigrid hierarchies per fine DOF: coarse operators (1/8 + 1/64 + ...
+# of the fine SELL forms), level vectors, the FP64 outer CG vectors
+_MG_BYTES_PER_DOF = {False: 250.0, True: 900.0}
 
 
-def estimate_bytes(n_dof, me):
-    """Device memory of one member with ``n_dof`` DOFs (upper estimate)."""
-    return int(1.15 * _BYTES_PER_DOF[bool(me)] * n_dof) + (64 << 20)
+def estimate_bytes(n_dof, me, multigrid_n=None):
+    """Device memory of one member with ``n_dof`` DOFs (upper estimate);
+    multigrid_n: cells per dimension when the member runs the multigrid
+    solves (adds the hierarchies and their dense FP32 coarsest inverses)."""
+    b = 1.15 * _BYTES_PER_DOF[bool(me)] * n_dof
+    if multigrid_n:
+        from ..mesh.coarsen import hierarchy_sizes
+        nc = hierarchy_sizes(int(multigrid_n))[-1] ** 3
+        b += _MG_BYTES_PER_DOF[bool(me)] * n_dof + 4.0 * (nc ** 2 + (9 * nc ** 2 if me else 0))
+    return int(b) + (64 << 20)
+
+
+def multigrid_ok(n, coarsest_max=8):
+    """Whether an n^3 Kuhn mesh has a multigrid hierarchy (the rule of
+    solvers.multigrid.build_hierarchies)."""
+    return int(n) > coarsest_max
 
 
 class SweepJob:
@@ -89,7 +105,12 @@ class SweepJob:
 
 class SweepServer:
     def __init__(self, ctx=None, hbm_budget=240e9, mode="mixed", graphs=False, max_members=256, prep_workers=2,
-                 rel_delta=0.3, keep_m=True):
+                 rel_delta=0.3, keep_m=True, precond=None):
+        """precond="mg": requests whose mesh has a multigrid hierarchy run
+        multigrid-preconditioned elasticity and demag solves (batched over
+        those members); the others run plain CG."""
+        if precond not in (None, "mg"):
+            raise ValueError(f"unknown preconditioner {precond!r} (None or 'mg')")
         from ..device import context
         self.ctx = ctx or context()
         self.hbm_budget = float(hbm_budget)
@@ -99,6 +120,10 @@ class SweepServer:
         self.prep_workers = int(prep_workers)
         self.rel_delta = rel_delta
         self.keep_m = keep_m
+        self.precond = precond
+
+    def _uses_mg(self, n):
+        return self.precond == "mg" and multigrid_ok(n)
 
     def _prepare(self, req):
         import torch
@@ -111,7 +136,8 @@ class SweepServer:
         P = from_dict(req["cfg"])
         arr = kuhn_mesh(req["n"])
         S = SimulationOperators(P, mesh_arrays=arr, ctx=self.ctx, keep_host=False, matrix_free_fel=True)
-        sim = GSPMSimulation(S, P, mode=self.mode, rel_delta=self.rel_delta)
+        sim = GSPMSimulation(S, P, mode=self.mode, rel_delta=self.rel_delta,
+                             precond="mg" if self._uses_mg(req["n"]) else None)
         f0 = np.asarray(req["fields"][0], dtype=np.float64)
         direction = f0 if np.linalg.norm(f0) > 0 else (1.0, 0.0, 0.0)
         sim.set_m(initial_m(S.n_dof, req["m_seed"], direction=direction))
@@ -145,7 +171,7 @@ class SweepServer:
                 while waiting:
                     req = waiting[0]
                     me = bool(req["cfg"]["physics"]["magnetoelastic_coupling"])
-                    est = estimate_bytes(req["n"] ** 3, me)
+                    est = estimate_bytes(req["n"] ** 3, me, req["n"] if self._uses_mg(req["n"]) else None)
                     if est > self.hbm_budget:
                         raise MemoryError(f"request {req['rid']} needs ~{est / 1e9:.1f} GB, over the HBM budget")
                     if used + est > self.hbm_budget or len(active) + len(preparing) >= self.max_members:

@@ -4,9 +4,11 @@ The reference solves both systems with unpreconditioned CG
 (solvers/linear_solvers.py:20-74, ``M=None``); their iteration counts grow
 with the mesh size (about 2 600 for the elasticity system and 700 for the
 demag system per step at 128^3).  On a structured periodic Kuhn mesh the
-operators have a natural hierarchy: the Kuhn mesh of the even nodes,
-assembled again by the host assembler (rediscretisation), down to <= 8
-cells per dimension, where a dense inverse finishes the V-cycle.
+operators have a natural hierarchy: the Kuhn mesh of the even nodes (or,
+for an odd number of cells, the regular Kuhn mesh of half as many cells,
+rounded up, over the same box), assembled again by the host assembler
+(rediscretisation), down to <= 8 cells per dimension, where a dense inverse
+finishes the V-cycle.
 ``build_hierarchies`` builds it for a ``SimulationOperators`` object;
 ``KuhnMultigrid.solve`` runs the preconditioned CG (same stopping rule as
 ``solve_cg``).
@@ -104,15 +106,15 @@ def _interleave_perm(n, bs):
 
 def build_hierarchies(SimOp, precision=32, degree=2, ratio=20.0, coarsest_max=8, nthreads=None):
     """{"A_demag": KuhnMultigrid, "A_el": KuhnMultigrid (with coupling)} for
-    a SimulationOperators object on an n^3 Kuhn mesh with n even, or None
-    when the mesh has no such hierarchy (not a Kuhn mesh in kuhn_mesh
-    order, or n odd / too small to coarsen)."""
+    a SimulationOperators object on an n^3 Kuhn mesh with n > coarsest_max,
+    or None when the mesh has no such hierarchy (not a Kuhn mesh in
+    kuhn_mesh order, or too small to coarsen)."""
     from ..physics.host_ops import HostOps, voigt_C
     n = kuhn_cells(SimOp.node_coords_np, SimOp.elements_np)
     if n is None or SimOp.grid_dims != (n, n, n):
         return None
     sizes = hierarchy_sizes(n, coarsest_max)
-    if len(sizes) < 2 or sizes[-1] > 10:
+    if len(sizes) < 2:
         return None
     P = SimOp.P
     ctx = SimOp.csr_demag_and_deriv()[0].ctx
@@ -124,7 +126,7 @@ def build_hierarchies(SimOp, precision=32, degree=2, ratio=20.0, coarsest_max=8,
     dense = {}
     coords = SimOp.node_coords_np
     for li, m in enumerate(sizes[1:]):
-        coords, tets = coarse_kuhn(coords, 2 * m)
+        coords, tets = coarse_kuhn(coords, sizes[li])
         el = np.zeros((tets.shape[0], 5), dtype=np.int32)
         el[:, :4] = tets
         gid = periodic_node_map(coords, coords.min(axis=0), coords.max(axis=0))

@@ -3,7 +3,7 @@ elasticity/demag solves of cpm_batch_step for members created with
 precond="mg").
 
 * Hierarchies of different depths (10^3 -> 5^3, 20^3 -> 10^3 -> 5^3,
-  12^3 -> 6^3 Kuhn meshes), scalar and 3x3-block operators, FP32 and FP64
+  13^3 -> 7^3 non-nested), scalar and 3x3-block operators, FP32 and FP64
   V-cycles: every system's solution, iteration count and residual equal
   the one-system solve (cpm_cg_mg) bit for bit, with and without initial
   guesses, for a zero right-hand side too; maxiter reached raises.
@@ -34,7 +34,7 @@ def _ops(n, me=True, jitter=0.1, seed=3):
 
 @pytest.fixture(scope="module")
 def meshes():
-    return [_ops(n) for n in (10, 20, 12)]
+    return [_ops(n) for n in (10, 20, 13)]
 
 
 @pytest.mark.parametrize("name", ["A_demag", "A_el"])
@@ -97,8 +97,8 @@ def _state(sim):
 @pytest.mark.parametrize("mode", ["fast64", "mixed"])
 def test_batch_step_with_multigrid_members(meshes, mode):
     """Members: 10^3 (ME, multigrid), 20^3 (no ME, multigrid, 3 levels),
-    12^3 (ME, multigrid), 12^3 again on the same operators (one hierarchy,
-    solved in a second round), 8^3 without the preconditioner."""
+    13^3 (ME, multigrid, non-nested), 13^3 again on the same operators (one
+    hierarchy, solved in a second round), 8^3 without the preconditioner."""
     from cupymag_amd.core.batch import GSPMBatch
     from cupymag_amd.core.gspm import GSPMSimulation
     from oracle import gspm as og
@@ -142,7 +142,7 @@ def test_batch_step_with_multigrid_members(meshes, mode):
                 assert 0 < rep.iters[1] < 40
     if mode != "fast64":
         return
-    # oracle check of the 12^3 ME multigrid member (from its seed, same steps)
+    # oracle check of the 13^3 ME multigrid member (from its seed, same steps)
     cfg, arr, P, S = meshes[2]
     p, ops, Avg, _ = oracle_setup(cfg, arr)
     st = og.init_state(ops, Avg, tilted_m(ops["n"], seed=52))

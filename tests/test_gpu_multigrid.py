@@ -94,16 +94,21 @@ def _host_vcycle(mats, dims, lmax, cinv, bs, degree, ratio, b):
     return V(0, b)
 
 
+@pytest.mark.parametrize("n", [16, 18, 9])
 @pytest.mark.parametrize("name,bs", [("A_demag", 1), ("A_el", 3)])
-def test_vcycle_matches_host_reference(name, bs):
-    _, _, _, S = _setup(n=16)
+def test_vcycle_matches_host_reference(name, bs, n):
+    """16 -> 8 (nested), 18 -> 9 -> 5 (nested, then non-nested), 9 -> 5
+    (non-nested at the top): the device transfers equal the host P."""
+    from cupymag_amd.mesh.coarsen import hierarchy_sizes
+    _, _, _, S = _setup(n=n)
+    sizes = hierarchy_sizes(n)
     for prec, tol in ((64, 1e-11), (32, 1e-5)):
         mg = S.multigrid(precision=prec)[name]
         info = mg.info()
-        assert info["levels"] == 2 and info["dims"][1] == (8, 8, 8)
+        assert info["levels"] == len(sizes) and info["dims"][-1] == (sizes[-1],) * 3
         assert all(l > 0 for l in info["lmax"][:-1])
         mats = [_interleaved(L, bs) for L in mg.levels]
-        dims = [(16, 16, 16), (8, 8, 8)]
+        dims = [(m, m, m) for m in sizes]
         from cupymag_amd.solvers.multigrid import _interleave_perm
         Ac = mg.levels[-1].to_scipy().toarray()
         perm = _interleave_perm(Ac.shape[0] // bs, bs)
@@ -118,10 +123,10 @@ def test_vcycle_matches_host_reference(name, bs):
         assert np.all(z[:bs] == 0.0)
 
 
-@pytest.mark.parametrize("jitter", [0.0, 0.15])
+@pytest.mark.parametrize("n,jitter", [(16, 0.0), (16, 0.15), (21, 0.15)])
 @pytest.mark.parametrize("name,bs", [("A_demag", 1), ("A_el", 3)])
-def test_mg_pcg_matches_direct_solve(name, bs, jitter):
-    _, _, _, S = _setup(n=16, jitter=jitter)
+def test_mg_pcg_matches_direct_solve(name, bs, n, jitter):
+    _, _, _, S = _setup(n=n, jitter=jitter)
     for prec in (32, 64):
         mg = S.multigrid(precision=prec)[name]
         A = _interleaved(mg.levels[0], bs)
@@ -177,6 +182,6 @@ def test_multigrid_rejections():
     b.add(sim)  # batched multigrid solves (test_gpu_batch_mg.py)
     with pytest.raises(ValueError, match="same mode"):
         b.add(GSPMSimulation(S, P, mode="fast64", precond="mg"))
-    _, _, P7, S7 = _setup(n=7, me=False)
+    _, _, P7, S7 = _setup(n=8, me=False)
     with pytest.raises(ValueError, match="multigrid needs"):
         GSPMSimulation(S7, P7, mode="mixed", precond="mg")

@@ -31,16 +31,21 @@ def _same(a, b):
         assert s1 == s2
 
 
-@pytest.mark.parametrize("graphs", [False, True])
-def test_results_same_alone_and_in_the_mix(graphs):
-    from cupymag_amd.serving.server import SweepServer
-    reqs = _requests()
-    srv = SweepServer(mode="mixed", graphs=graphs)
+@pytest.mark.parametrize("graphs,precond", [(False, None), (True, None), (False, "mg")])
+def test_results_same_alone_and_in_the_mix(graphs, precond):
+    """precond="mg": the 12^3 and 9^3 requests run the multigrid solves, the
+    8^3 one (no hierarchy) plain CG, in the same batch."""
+    from cupymag_amd.serving.server import SweepServer, multigrid_ok
+    reqs = _requests(count=4, seed=12) if precond else _requests()
+    if precond:
+        assert sorted(r["n"] for r in reqs) == [8, 9, 12, 12]
+        assert [multigrid_ok(r["n"]) for r in reqs].count(True) == 3
+    srv = SweepServer(mode="mixed", graphs=graphs, precond=precond)
     mixed, stats = srv.run(reqs)
     assert stats["requests"] == len(reqs)
     assert stats["max_members"] >= 2, stats  # they did overlap on the device
     for r in reqs:
-        alone, _ = SweepServer(mode="mixed", graphs=graphs).run([dict(r, arrival=0.0)])
+        alone, _ = SweepServer(mode="mixed", graphs=graphs, precond=precond).run([dict(r, arrival=0.0)])
         _same(mixed[r["rid"]], alone[r["rid"]])
         assert len(mixed[r["rid"]]["points"]) == 3
         m = mixed[r["rid"]]["m"]
@@ -61,21 +66,24 @@ def test_hbm_budget_serialises_admission():
         SweepServer(mode="mixed", hbm_budget=min(ests) * 0.5).run(reqs[:1])
 
 
-@pytest.mark.parametrize("me", [False, True])
-def test_admission_estimate_bounds_member_memory(me):
+@pytest.mark.parametrize("me,precond", [(False, None), (True, None), (False, "mg"), (True, "mg")])
+def test_admission_estimate_bounds_member_memory(me, precond):
     from cupymag_amd.core.batch import GSPMBatch
     from cupymag_amd.serving.server import SweepServer, estimate_bytes
     from cupymag_amd.serving.workloads import sweep_requests
     req = sweep_requests(1, seed=11, nmin=24, nmax=24, points=2, steps_per_point=1, me=me)[0]
-    srv = SweepServer(mode="mixed")
+    srv = SweepServer(mode="mixed", precond=precond)
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
     free0, _ = torch.cuda.mem_get_info()
     sim, P, n_dof = srv._prepare(req)
     b = GSPMBatch(srv.ctx)
     b.add(sim)  # batched-CG workspace
+    b.step([sim.P.Hext1, sim.P.Hext2, sim.P.Hext3], [False])  # batched-solver tables
     torch.cuda.synchronize()
     free1, _ = torch.cuda.mem_get_info()
     used = free0 - free1
     assert n_dof == 24 ** 3
-    assert 0 < used <= estimate_bytes(n_dof, me), (used, estimate_bytes(n_dof, me))
+    assert (sim.mg is not None) == (precond == "mg")
+    est = estimate_bytes(n_dof, me, 24 if precond else None)
+    assert 0 < used <= est, (used, est)

@@ -58,10 +58,17 @@ def test_kuhn_detection_and_sizes():
     assert kuhn_cells(coords[perm], _elements(tets)) is None
     assert hierarchy_sizes(128) == [128, 64, 32, 16, 8]
     assert hierarchy_sizes(96) == [96, 48, 24, 12, 6]
-    assert hierarchy_sizes(90) == [90, 45]
+    assert hierarchy_sizes(90) == [90, 45, 23, 12, 6]
+    assert hierarchy_sizes(87) == [87, 44, 22, 11, 6]
+    assert hierarchy_sizes(9) == [9, 5] and hierarchy_sizes(8) == [8]
     C, T = coarse_kuhn(coords, 8)
     assert np.array_equal(T, kuhn_mesh(4)[1])
     np.testing.assert_array_equal(C, kuhn_mesh(4, h=2.0)[0])
+    # odd: the regular mesh of (n+1)/2 cells over the same box
+    c9, _ = kuhn_mesh(9, h=1.5, jitter=0.15, seed=4)
+    C, T = coarse_kuhn(c9, 9)
+    assert np.array_equal(T, kuhn_mesh(5)[1])
+    np.testing.assert_allclose(C, kuhn_mesh(5, h=1.5 * 9 / 5)[0], rtol=0, atol=1e-12)
 
 
 def test_prolongation_basics():
@@ -93,3 +100,46 @@ def test_galerkin_identity_elasticity():
     P3 = prolongation((8, 8, 8), bs=3, anchor=False)
     G = (P3.T @ Kf @ P3).toarray()
     assert np.abs(G - Kc.toarray()).max() < 1e-12 * np.abs(Kc.toarray()).max()
+
+
+def _kuhn_eval(uc, nc, x):
+    """Brute force: the P1 function of the periodic coarse Kuhn mesh with
+    nodal values uc (nc cells per dimension, kuhn_mesh tetrahedra) at point
+    x (coarse cell units): find a tetrahedron of the containing cube whose
+    barycentric coordinates are all >= 0 and interpolate."""
+    nc = np.asarray(nc)
+    cell = np.minimum(np.floor(x).astype(int), nc - 1)
+    grid, tets = kuhn_mesh(1)
+    for t in tets:
+        V = grid[t]                                  # 4 x 3 local vertex offsets
+        A = np.vstack([V.T, np.ones(4)])
+        lam = np.linalg.solve(A, np.append(x - cell, 1.0))
+        if lam.min() >= -1e-12:
+            val = 0.0
+            for a in range(4):
+                v = (cell + V[a].astype(int)) % nc
+                val += lam[a] * uc[(v[0] * nc[1] + v[1]) * nc[2] + v[2]]
+            return val
+    raise AssertionError("point in no tetrahedron")
+
+
+def test_non_nested_prolongation_interpolates_coarse_p1_functions():
+    """P of a non-nested pair (9 -> 5, and a box of 9 x 7 x 5 -> 5 x 4 x 3
+    cells) applied to a random coarse nodal vector gives the coarse P1
+    function at every fine node's nominal position, found by brute-force
+    search of the coarse Kuhn tetrahedra; for even sizes the same code is the
+    nested P (pinned by the Galerkin identities above)."""
+    rng = np.random.default_rng(8)
+    for nf in ((9, 9, 9), (9, 7, 5)):
+        nf = np.asarray(nf)
+        nc = (nf + 1) // 2
+        P = prolongation(tuple(nf), anchor=False)
+        uc = rng.standard_normal(int(np.prod(nc)))
+        uf = P @ uc
+        for f in rng.choice(int(np.prod(nf)), size=150, replace=False):
+            i, j, k = f // (nf[1] * nf[2]), (f // nf[2]) % nf[1], f % nf[2]
+            x = np.array([i, j, k]) * nc / nf
+            assert abs(uf[f] - _kuhn_eval(uc, nc, x)) < 1e-12
+        np.testing.assert_allclose(P @ np.ones(len(uc)), 1.0, rtol=0, atol=1e-14)
+        Pa = prolongation(tuple(nf), anchor=True)
+        assert Pa[0].nnz == 0 and Pa[:, 0].nnz == 0 and (P - Pa)[1:, 1:].nnz == 0

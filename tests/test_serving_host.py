@@ -2,7 +2,7 @@
 workload generators and the per-request hysteresis control."""
 import numpy as np
 
-from cupymag_amd.serving.server import SweepJob, estimate_bytes
+from cupymag_amd.serving.server import SweepJob, estimate_bytes, multigrid_ok
 from cupymag_amd.serving.workloads import batch_workload, initial_m, sweep_requests
 
 
@@ -91,3 +91,12 @@ def test_sweep_job_points_and_use_init():
 def test_estimate_grows_with_size_and_coupling():
     assert estimate_bytes(1000, True) > estimate_bytes(1000, False)
     assert estimate_bytes(128 ** 3, True) < 30e9  # a 128^3 member with coupling fits many times in 288 GB
+
+
+def test_multigrid_rule_and_estimate():
+    assert multigrid_ok(9) and multigrid_ok(22) and multigrid_ok(87) and multigrid_ok(128)
+    assert not multigrid_ok(8) and not multigrid_ok(7)
+    for me in (False, True):
+        assert estimate_bytes(64 ** 3, me, 64) > estimate_bytes(64 ** 3, me)
+    # 16^3 -> 8^3 with coupling: the dense FP32 inverses take 4 (512^2 + 1536^2) bytes
+    assert estimate_bytes(16 ** 3, True, 16) - estimate_bytes(16 ** 3, True) >= 4 * (512 ** 2 + 1536 ** 2)
