@@ -107,13 +107,14 @@ def run_gpu(n, steps, warmup, steps64, modes, rel_delta=0.3, autotune=True, rel_
     snap_at = warmup + steps64
     for mode in modes:
         base, *opts = mode.split("+")
-        if "mg" in opts:
+        mg = "mg" in opts or "mged" in opts
+        if mg:
             t0 = time.time()
             S.multigrid(precision=32 if base == "mixed" else 64)
             log(f"[bench] multigrid hierarchies for {mode}: {time.time() - t0:.1f}s")
         kw = {} if rel_delta_gs is None else dict(rel_delta_gs=rel_delta_gs)
         sim = GSPMSimulation(S, P, mode=base, warm_start=("warm" in opts), rel_delta=rel_delta,
-                             precond="mg" if "mg" in opts else None, **kw)
+                             precond="mg" if mg else None, mg_gs="mged" not in opts, **kw)
         ctx = sim.ctx
         sim.set_m(m0)
         k_timed = steps64 if base == "fp64" else steps
@@ -156,7 +157,7 @@ def run_gpu(n, steps, warmup, steps64, modes, rel_delta=0.3, autotune=True, rel_
             kernel = "k_cg_spmv (A_el, CSR FP64)"
             rocname = None
             bytes_launch = 12.0 * nnz + 4.0 * (rows + 1) + 16.0 * rows
-        elif "mg" in opts:
+        elif mg:
             # level-0 Chebyshev step of the V-cycle (r -= A d, d = c1 d + c2 D^-1 r,
             # x += d): the operator's planes (stencil form) and diagonal blocks,
             # d gathered, r, x read and written, D^-1 (9 values) read, d written
@@ -468,8 +469,9 @@ def main():
     ap.add_argument("--steps-fp64", type=int, default=2, help="timed steps of the plain FP64 path")
     ap.add_argument("--modes", default="fp64,fast64,mixed,fast64+mg,mixed+mg",
                     help="comma list of fp64, fast64, mixed; '+warm' appended to a fast mode starts every CG "
-                         "from the previous step's solution, '+mg' preconditions the elasticity and demag solves "
-                         "with the geometric multigrid")
+                         "from the previous step's solution, '+mg' preconditions the elasticity, demag and "
+                         "Gauss-Seidel (A1) solves with the geometric multigrid, '+mged' only the elasticity and "
+                         "demag solves")
     ap.add_argument("--rel-delta", type=float, default=0.3)
     ap.add_argument("--rel-delta-gs", type=float, default=None,
                     help="reliable-update factor of the Gauss-Seidel solves (default 0.01)")

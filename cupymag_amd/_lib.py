@@ -22,6 +22,8 @@ FIELD = dict(M=0, HTILDE=1, E=2, U=3, UEL=4)
 MODE_FP64, MODE_FAST64, MODE_MIXED = 0, 1, 2
 MODES = {"fp64": MODE_FP64, "fast64": MODE_FAST64, "mixed": MODE_MIXED}
 BATCH_GRAPH = 1
+PRECOND_EL_DEMAG = 1  # elasticity and demag solves: multigrid-preconditioned CG
+PRECOND_GS = 2        # Gauss-Seidel solves on A1: multigrid-preconditioned CG
 FMT = {0: "csr", 1: "sell64", 2: "sell32"}
 
 
@@ -32,7 +34,7 @@ class CpmError(RuntimeError):
 class cpm_sim_ops(C.Structure):
     _fields_ = [("A_demag", C.c_void_p), ("F3", C.c_void_p), ("A1", C.c_void_p), ("A2", C.c_void_p),
                 ("F1", C.c_void_p), ("A_el", C.c_void_p), ("F_el", C.c_void_p), ("mg_el", C.c_void_p),
-                ("mg_demag", C.c_void_p), ("fel", C.c_void_p)]
+                ("mg_demag", C.c_void_p), ("fel", C.c_void_p), ("mg_a1", C.c_void_p)]
 
 
 class cpm_sim_params(C.Structure):
@@ -95,8 +97,8 @@ _SIGS = {
     "cpm_mg_destroy": (C.c_int, [_P]),
     "cpm_mg_info": (C.c_int, [_P, C.POINTER(C.c_int), _P, C.c_int]),
     "cpm_mg_apply": (C.c_int, [_P, _P, _P]),
-    "cpm_cg_mg": (C.c_int, [_P, _P, _P, C.c_int, _D, C.c_int, C.POINTER(_I32), C.POINTER(_D)]),
-    "cpm_cg_mg_batch": (C.c_int, [_P, C.c_int, _P, _P, _P, _P, _D, C.c_int, _P, _P, _P]),
+    "cpm_cg_mg": (C.c_int, [_P, C.c_int, _P, _P, C.c_int, _D, C.c_int, _P, _P]),
+    "cpm_cg_mg_batch": (C.c_int, [_P, C.c_int, C.c_int, _P, _P, _P, _P, _D, C.c_int, _P, _P, _P]),
     "cpm_fel_create": (C.c_int, [_P, _I32, _P, _P, C.POINTER(_P)]),
     "cpm_fel_destroy": (C.c_int, [_P]),
     "cpm_fel_apply": (C.c_int, [_P, _P, _P]),

@@ -31,7 +31,8 @@ class StepReport:
 
 
 class GSPMSimulation:
-    def __init__(self, SimOp, P, mode="fp64", rel_delta=0.3, warm_start=False, precond=None, rel_delta_gs=0.01):
+    def __init__(self, SimOp, P, mode="fp64", rel_delta=0.3, warm_start=False, precond=None, rel_delta_gs=0.01,
+                 mg_gs=True):
         """mode: "fp64" (CSR operators, the reference's sequence of solves),
         "fast64" (SELL-64 operators, 3-RHS solves) or "mixed" (as fast64 with
         FP32 CG refined in FP64 by reliable updates: an FP64 residual update
@@ -44,7 +45,9 @@ class GSPMSimulation:
         ``precond="mg"`` (fast modes, structured Kuhn meshes): the
         elasticity and demag solves run CG preconditioned by a geometric
         multigrid V-cycle (FP32 V-cycle in the mixed mode, FP64 in fast64;
-        cupymag_amd.solvers.multigrid), converged to the same tolerance."""
+        cupymag_amd.solvers.multigrid), converged to the same tolerance;
+        with ``mg_gs`` also the Gauss-Seidel solves on A1 = M + alpha1 K
+        (g1n..g3n as one 3-column solve, g1*, g2*)."""
         self.P = P
         self.SimOp = SimOp
         A_demag, Fx, Fy, Fz, self.Deriv = SimOp.csr_demag_and_deriv()
@@ -66,11 +69,12 @@ class GSPMSimulation:
                 raise ValueError("multigrid needs a structured Kuhn mesh of more than 8 cells per dimension")
         mg_el = self.mg["A_el"].h if (self.mg and "A_el" in self.mg) else None
         mg_dm = self.mg["A_demag"].h if self.mg else None
+        mg_a1 = self.mg["A1"].h if (self.mg and mg_gs and "A1" in self.mg) else None
         from ..physics.fel import MatrixFreeFel
         mf = F_el if isinstance(F_el, MatrixFreeFel) else None
         ops = _lib.cpm_sim_ops(A_demag.h, F3.h, A1.h, A2.h, F1.h, A_el.h if A_el is not None else None,
                                F_el.h if (F_el is not None and mf is None) else None, mg_el, mg_dm,
-                               mf.h if mf is not None else None)
+                               mf.h if mf is not None else None, mg_a1)
         pr = _lib.cpm_sim_params()
         pr.n = n
         pr.me = int(bool(P.ME))
@@ -103,7 +107,7 @@ class GSPMSimulation:
         if warm_start and mode == "fp64":
             raise ValueError("warm_start is a fast-mode option")
         pr.warm_start = int(bool(warm_start))
-        pr.precond = int(precond == "mg")
+        pr.precond = ((_lib.PRECOND_EL_DEMAG | (_lib.PRECOND_GS if mg_a1 else 0)) if precond == "mg" else 0)
         self.mode = mode
         W = self.Avg.weights_device(self.ctx)
         dmask = np.zeros(n, dtype=np.uint8)

@@ -535,12 +535,16 @@ extern "C" int cpm_sim_create(cpm_ctx* ctx, const cpm_sim_ops* ops, const cpm_si
       return fail(CPM_EINVAL, "cpm_sim_create: rel_delta must be in (0, 1)");
     if (prm->mode == CPM_MODE_MIXED && !(prm->rel_delta_gs >= 0.0 && prm->rel_delta_gs < 1.0))
       return fail(CPM_EINVAL, "cpm_sim_create: rel_delta_gs must be in [0, 1)");
-    if (prm->precond && !ops->mg_demag)
+    if (prm->precond & ~(CPM_PRECOND_EL_DEMAG | CPM_PRECOND_GS))
+      return fail(CPM_EINVAL, "cpm_sim_create: unknown precond bits");
+    const bool pe = (prm->precond & CPM_PRECOND_EL_DEMAG) != 0, pg = (prm->precond & CPM_PRECOND_GS) != 0;
+    if (pe && !ops->mg_demag)
       return fail(CPM_EINVAL, "cpm_sim_create: precond needs the multigrid hierarchy of A_demag");
-    if (prm->precond && prm->me && !ops->mg_el)
+    if (pe && prm->me && !ops->mg_el)
       return fail(CPM_EINVAL, "cpm_sim_create: precond needs the multigrid hierarchy of A_el");
-    if (prm->precond && (mg_operator(ops->mg_demag) != ops->A_demag ||
-                         (prm->me && mg_operator(ops->mg_el) != ops->A_el)))
+    if (pg && !ops->mg_a1) return fail(CPM_EINVAL, "cpm_sim_create: precond needs the multigrid hierarchy of A1");
+    if ((pe && (mg_operator(ops->mg_demag) != ops->A_demag || (prm->me && mg_operator(ops->mg_el) != ops->A_el))) ||
+        (pg && mg_operator(ops->mg_a1) != ops->A1))
       return fail(CPM_EINVAL, "cpm_sim_create: a multigrid hierarchy does not belong to its operator");
   } else if (prm->precond) {
     return fail(CPM_EINVAL, "cpm_sim_create: precond is a fast-mode option");
@@ -803,6 +807,17 @@ static inline int use_x0_fast(const cpm_sim* S, int use_init) {
   return (use_init || (S->prm.warm_start && S->stepped)) ? 1 : 0;
 }
 
+// The multigrid hierarchy a solve runs on, or null (plain CG): elasticity
+// and demag (CPM_PRECOND_EL_DEMAG), the Gauss-Seidel solves on A1
+// (CPM_PRECOND_GS).
+static cpm_mg* solve_mg(const cpm_sim* S, const SolveSpec& sp) {
+  const int pc = S->prm.precond;
+  if ((pc & CPM_PRECOND_EL_DEMAG) && sp.slot == 0) return S->ops.mg_el;
+  if ((pc & CPM_PRECOND_EL_DEMAG) && sp.slot == 1) return S->ops.mg_demag;
+  if ((pc & CPM_PRECOND_GS) && sp.A == S->ops.A1) return S->ops.mg_a1;
+  return nullptr;
+}
+
 // One solve of a single simulation: CSR CG per column (plain FP64 path, the
 // reference's sequence) or the SELL solvers (fast modes).
 static int solve_one(cpm_sim* S, const SolveSpec& sp, int use_init, cpm_step_report* rep) {
@@ -837,8 +852,8 @@ static int solve_one(cpm_sim* S, const SolveSpec& sp, int use_init, cpm_step_rep
   int hint = 0;
   for (int j = 0; j < sp.K; ++j) hint = std::max(hint, S->hint[sp.slot + j]);
   const int x0 = use_x0_fast(S, use_init);
-  cpm_mg* mg = !p.precond ? nullptr : (sp.slot == 0 ? S->ops.mg_el : sp.slot == 1 ? S->ops.mg_demag : nullptr);
-  int rc = mg ? mg_pcg(mg, bb, xx, x0, p.tol, p.maxiter, hint, it, res, &failed)
+  cpm_mg* mg = solve_mg(S, sp);
+  int rc = mg ? mg_pcg(mg, sp.K, bb, xx, x0, p.tol, p.maxiter, hint, it, res, &failed)
            : (p.mode == CPM_MODE_MIXED)
                ? sell_cg_mixed(ctx, Am, sp.K, bb, xx, x0, p.tol, p.maxiter, solve_delta(p, sp.slot), hint, it, res,
                                &failed,
@@ -890,8 +905,8 @@ struct cpm_batch {
   int mode = -1;
   std::vector<cpm_sim*> sims;
   cpm::MultiSolver ms[6];  // one per stage
-  // multigrid-preconditioned members of stages 0 and 1, per (precision, degree)
-  std::map<int, cpm::MultiMgSolver> mgs[2];
+  // multigrid-preconditioned members of each stage, per (precision, degree)
+  std::map<int, cpm::MultiMgSolver> mgs[6];
   double* scal = nullptr;  // pinned, S_NSCAL per simulation
   int cap = 0;
 };
@@ -1010,21 +1025,21 @@ extern "C" int cpm_batch_step(cpm_batch* B, const double* H, const int32_t* use_
   std::vector<std::pair<int, MultiMgSys>> mgq;  // (member, system) still to solve
   std::vector<MultiMgSys> mgsys;
   std::vector<int> mgwho;
-  std::vector<int32_t> done_it(nb);
-  std::vector<double> done_res(nb);
+  std::vector<int32_t> done_it(3 * nb);
+  std::vector<double> done_res(3 * nb);
   std::vector<int> done_fail(nb), solved(nb);
   for (int stage = 0; stage < 6; ++stage) {
     sys.clear();
     who.clear();
     mgq.clear();
-    int hint = 0, mghint = 0, bs = 1, K = 1;
+    int hint = 0, mghint = 0, bs = 1, K = 1, mgK = 1;
     for (int j = 0; j < nb; ++j) {
       cpm_sim* S = B->sims[j];
       solved[j] = 0;
       if (int rc = stage_pre(S, stage, P[j], sp[j])) return rc;
       if (!sp[j].A) continue;
       const cpm_sim_params& p = S->prm;
-      cpm_mg* mg = !p.precond ? nullptr : (sp[j].slot == 0 ? S->ops.mg_el : sp[j].slot == 1 ? S->ops.mg_demag : nullptr);
+      cpm_mg* mg = solve_mg(S, sp[j]);
       if (mg) {
         MultiMgSys y;
         y.M = mg;
@@ -1034,7 +1049,8 @@ extern "C" int cpm_batch_step(cpm_batch* B, const double* H, const int32_t* use_
         y.tol = p.tol;
         y.maxiter = p.maxiter;
         mgq.emplace_back(j, y);
-        mghint = std::max(mghint, S->hint[sp[j].slot]);
+        mgK = sp[j].K;
+        for (int k = 0; k < mgK; ++k) mghint = std::max(mghint, S->hint[sp[j].slot + k]);
         continue;
       }
       MultiSys m;
@@ -1087,14 +1103,15 @@ extern "C" int cpm_batch_step(cpm_batch* B, const double* H, const int32_t* use_
           mgsys.push_back(e.second), mgwho.push_back(e.first);
       }
       const int nm = (int)mgsys.size();
-      std::vector<int32_t> mit(nm);
-      std::vector<double> mres(nm);
+      std::vector<int32_t> mit(nm * mgK);
+      std::vector<double> mres(nm * mgK);
       std::vector<int> mfail(nm);
-      if (int rc = multi_mg_pcg(ctx, B->mgs[stage][key], mgsys, mghint, mit.data(), mres.data(), mfail.data()))
+      if (int rc = multi_mg_pcg(ctx, B->mgs[stage][key], mgsys, mgK, mghint, mit.data(), mres.data(), mfail.data()))
         return rc;
       for (int i = 0; i < nm; ++i) {
         const int j = mgwho[i];
-        done_it[j] = mit[i], done_res[j] = mres[i], done_fail[j] = mfail[i], solved[j] = 1;
+        for (int k = 0; k < mgK; ++k) done_it[3 * j + k] = mit[i * mgK + k], done_res[3 * j + k] = mres[i * mgK + k];
+        done_fail[j] = mfail[i], solved[j] = 1;
       }
       mgq.swap(rest);
     }
@@ -1119,11 +1136,16 @@ extern "C" int cpm_batch_step(cpm_batch* B, const double* H, const int32_t* use_
       if (!solved[j]) continue;
       cpm_sim* S = B->sims[j];
       const SolveSpec& s = sp[j];
-      reps[j].iters[s.slot] = done_it[j];
-      S->hint[s.slot] = done_it[j];
+      for (int k = 0; k < s.K; ++k) {
+        reps[j].iters[s.slot + k] = done_it[3 * j + k];
+        S->hint[s.slot + k] = done_it[3 * j + k];
+      }
       if (done_fail[j] && !reps[j].failed) {
-        reps[j].failed = s.slot + 1;
-        reps[j].resid = done_res[j];
+        int kf = 0;
+        for (int k = 1; k < s.K; ++k)
+          if (done_it[3 * j + k] > done_it[3 * j + kf]) kf = k;
+        reps[j].failed = s.slot + kf + 1;
+        reps[j].resid = done_res[3 * j + kf];
       }
       if (int rc = stage_post(S, s)) return rc;
     }

@@ -23,6 +23,7 @@ struct MgLevel {
 struct cpm_mg {
   cpm_ctx* ctx = nullptr;
   int bs = 1, prec = 32, degree = 2;
+  int anchor = 1;           // row 0 of the fine operator is an identity row: node 0 left out of the transfers
   double ratio = 20.0;
   std::vector<cpm::MgLevel> L;
   void* cinv = nullptr;     // coarsest dense inverse (T), ncoarse x ncoarse
@@ -30,8 +31,9 @@ struct cpm_mg {
   // Chebyshev coefficients: c0[l] = 1/theta; for step k of level l,
   // c1[8l + k] = rho_k rho_(k-1) and c2[8l + k] = 2 rho_k / delta
   std::vector<double> c0, c1, c2;
-  // FP64 PCG vectors (fine level)
+  // FP64 PCG vectors (fine level); level and PCG vectors hold kcap columns
   double *r = nullptr, *p = nullptr, *q = nullptr, *rold = nullptr;
+  int kcap = 1;
   std::vector<void*> allocs;
 };
 
@@ -39,9 +41,13 @@ struct cpm_mg {
 
 namespace cpm {
 
-// PCG on the hierarchy's fine operator, on its context's stream.
-int mg_pcg(cpm_mg* M, const double* b, double* x, int use_x0, double tol, int maxiter, int hint, int32_t* iters,
-           double* resid, int* failed);
+// PCG on the hierarchy's fine operator for K interleaved columns (K = 1, or
+// 3 for a scalar operator), on its context's stream; iters and resid hold K
+// entries.
+int mg_pcg(cpm_mg* M, int K, const double* b, double* x, int use_x0, double tol, int maxiter, int hint,
+           int32_t* iters, double* resid, int* failed);
+// grow the level and PCG vectors of M to K columns (synchronises)
+int mg_reserve(cpm_mg* M, int K);
 // the fine operator a hierarchy was built on
 const cpm_mat* mg_operator(const cpm_mg* M);
 

@@ -45,16 +45,17 @@
 namespace cpm {
 
 // ---------------------------------------------------------------- kernels
+// V-cycle kernels run while any of the solve's K columns is live.
 
-template <int BS, class T, bool ZERO, bool FROM64>
+template <int BS, int K, class T, bool ZERO, bool FROM64>
 __global__ void __launch_bounds__(kBlock) k_mg_first(int32_t nb, const double* __restrict__ b64, T* __restrict__ b,
                                                      T* __restrict__ r, T* __restrict__ d, T* __restrict__ x,
                                                      const T* __restrict__ dinv, double c0, const CgState* st) {
-  if (st->done) return;
-  mg_first_rows<BS, T, ZERO, FROM64>(blockIdx.x, gridDim.x, nb, b64, b, r, d, x, dinv, c0);
+  if (all_done<K>(st)) return;
+  mg_first_rows<BS, K, T, ZERO, FROM64>(blockIdx.x, gridDim.x, nb, b64, b, r, d, x, dinv, c0);
 }
 
-template <int BS, class T, int MODE, bool ST>
+template <int BS, int K, class T, int MODE, bool ST>
 __global__ void __launch_bounds__(kBlock) k_mg_sweep(int32_t nb, const int64_t* __restrict__ sptr,
                                                      const int32_t* __restrict__ col, const T* __restrict__ val,
                                                      const double* __restrict__ dg, GridArg ga,
@@ -62,32 +63,34 @@ __global__ void __launch_bounds__(kBlock) k_mg_sweep(int32_t nb, const int64_t* 
                                                      T* __restrict__ x, T* __restrict__ r, const T* __restrict__ b,
                                                      const T* __restrict__ dinv, double c1, double c2,
                                                      const CgState* st) {
-  if (st->done) return;
-  mg_sweep_rows<BS, T, MODE, ST>(blockIdx.x, gridDim.x, nb, sptr, col, val, dg, ga, din, dout, x, r, b, dinv, c1, c2);
+  if (all_done<K>(st)) return;
+  mg_sweep_rows<BS, K, T, MODE, ST>(blockIdx.x, gridDim.x, nb, sptr, col, val, dg, ga, din, dout, x, r, b, dinv, c1,
+                                    c2);
 }
 
-template <int BS, class T>
-__global__ void __launch_bounds__(kBlock) k_mg_restrict(Dims3 fine, Dims3 coarse, const T* __restrict__ rf,
-                                                        T* __restrict__ bc, const CgState* st) {
-  if (st->done) return;
-  mg_restrict_rows<BS, T>(blockIdx.x, gridDim.x, fine, coarse, rf, bc);
+template <int W, int K, class T>
+__global__ void __launch_bounds__(kBlock) k_mg_restrict(Dims3 fine, Dims3 coarse, int anchor,
+                                                        const T* __restrict__ rf, T* __restrict__ bc,
+                                                        const CgState* st) {
+  if (all_done<K>(st)) return;
+  mg_restrict_rows<W, T>(blockIdx.x, gridDim.x, fine, coarse, anchor != 0, rf, bc);
 }
 
-template <int BS, class T>
-__global__ void __launch_bounds__(kBlock) k_mg_prolong(Dims3 fine, Dims3 coarse, const T* __restrict__ ec,
-                                                       T* __restrict__ xf, const CgState* st) {
-  if (st->done) return;
-  mg_prolong_rows<BS, T>(blockIdx.x, gridDim.x, fine, coarse, ec, xf);
+template <int W, int K, class T>
+__global__ void __launch_bounds__(kBlock) k_mg_prolong(Dims3 fine, Dims3 coarse, int anchor,
+                                                       const T* __restrict__ ec, T* __restrict__ xf,
+                                                       const CgState* st) {
+  if (all_done<K>(st)) return;
+  mg_prolong_rows<W, T>(blockIdx.x, gridDim.x, fine, coarse, anchor != 0, ec, xf);
 }
 
-template <class T>
+template <int K, class T>
 __global__ void __launch_bounds__(kBlock) k_mg_dense(int64_t n, const T* __restrict__ cinv, const T* __restrict__ b,
                                                      T* __restrict__ x, const CgState* st) {
-  if (st->done) return;
-  mg_dense_rows<T>(blockIdx.x, gridDim.x, n, cinv, b, x);
+  if (all_done<K>(st)) return;
+  mg_dense_rows<K, T>(blockIdx.x, gridDim.x, n, cinv, b, x);
 }
 
-// y = D^-1 (A x) (FP64 form) and partials of y.y: power iteration.
 template <int BS, bool ST>
 __global__ void __launch_bounds__(kBlock) k_mg_power(int32_t nb, const int64_t* __restrict__ sptr,
                                                      const int32_t* __restrict__ col, const double* __restrict__ val,
@@ -100,7 +103,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_power(int32_t nb, const int64_t* 
     double sum[BS], xo[BS], dv[BS];
     sell_row<BS, 1, double, false, ST>(row, nb, sptr, col, val, dg, x, nullptr, ga, sum, xo);
     if (row >= nb) continue;
-    apply_dinv<BS, double>(dinv, row, sum, dv);
+    apply_dinv<BS, 1, double>(dinv, row, sum, dv);
 #pragma unroll
     for (int c = 0; c < BS; ++c) {
       y[row * BS + c] = dv[c];
@@ -117,88 +120,105 @@ __global__ void __launch_bounds__(kBlock) k_scale(int64_t n, const double* __res
 }
 
 // ---- outer preconditioned CG (FP64) ------------------------------------------
+// K columns interleaved (entry i belongs to column i % K, state st[i % K]).
 
-// q = A p (FP64 SELL form) and partials of p.q; PLAIN: y = A x, no partials.
-template <int BS, bool ST, bool PLAIN>
+// q = A p (FP64 SELL form) and partials of p_j.q_j; PLAIN: y = A x, no partials.
+template <int BS, int K, bool ST, bool PLAIN>
 __global__ void __launch_bounds__(kBlock) k_pcg_spmv(int32_t nb, const int64_t* __restrict__ sptr,
                                                      const int32_t* __restrict__ col, const double* __restrict__ val,
                                                      const double* __restrict__ dg, GridArg ga,
                                                      const double* __restrict__ p, double* __restrict__ q,
                                                      const CgState* st, double* partials) {
-  if (!PLAIN && st->done) return;
-  double acc[1] = {0.0};
-  sell_rows<BS, 1, double, !PLAIN, false, ST>(blockIdx.x, gridDim.x, nb, sptr, col, val, dg, p, nullptr, q, ga, acc);
-  if (!PLAIN) block_partial<1>(acc, partials);
+  if (!PLAIN && all_done<K>(st)) return;
+  double acc[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) acc[j] = 0.0;
+  sell_rows<BS, K, double, !PLAIN, false, ST>(blockIdx.x, gridDim.x, nb, sptr, col, val, dg, p, nullptr, q, ga, acc);
+  if (!PLAIN) block_partial<K>(acc, partials);
 }
 
-// r = b - y (use_x0) or x = 0, r = b; partials of ||b||^2, ||r||^2.
+// r = b - y (use_x0) or x = 0, r = b; partials of ||b_j||^2, ||r_j||^2.
+template <int K>
 __global__ void __launch_bounds__(kBlock) k_pcg_init(int64_t n, const double* __restrict__ b, double* __restrict__ x,
                                                      const double* __restrict__ y, double* __restrict__ r,
                                                      int use_x0, double* partials) {
-  double acc[2] = {0.0, 0.0};
-  pcg_init_rows(blockIdx.x, gridDim.x, n, b, x, y, r, use_x0, acc);
-  block_partial<2>(acc, partials);
+  double acc[2 * K];
+#pragma unroll
+  for (int j = 0; j < 2 * K; ++j) acc[j] = 0.0;
+  pcg_init_rows<K>(blockIdx.x, gridDim.x, n, b, x, y, r, use_x0, acc);
+  block_partial<2 * K>(acc, partials);
 }
 
+template <int K>
 __global__ void __launch_bounds__(kRedBlock) k_pcg_red_init(int nblk, const double* __restrict__ partials,
                                                             CgState* st, double tol, int32_t maxiter) {
-  __shared__ double lds[2 * (kRedBlock / 64)];
-  double v[2] = {0.0, 0.0};
-  for (int i = threadIdx.x; i < nblk; i += kRedBlock) {
-    v[0] += partials[i];
-    v[1] += partials[nblk + i];
-  }
-  block_sum_t<2, kRedBlock>(v, lds);
-  if (threadIdx.x == 0) cg_state_init<1, false>(st, v, tol, maxiter);
+  __shared__ double lds[2 * K * (kRedBlock / 64)];
+  double v[2 * K];
+#pragma unroll
+  for (int j = 0; j < 2 * K; ++j) v[j] = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += kRedBlock)
+#pragma unroll
+    for (int j = 0; j < 2 * K; ++j) v[j] += partials[j * nblk + i];
+  block_sum_t<2 * K, kRedBlock>(v, lds);
+  if (threadIdx.x == 0) cg_state_init<K, false>(st, v, tol, maxiter);
 }
 
+template <int K>
 __global__ void __launch_bounds__(kBlock) k_pcg_zero_b(int64_t n, double* __restrict__ x, const CgState* st) {
-  if (st->bnorm2 != 0.0) return;
-  const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) x[i] = 0.0;
+  zero_b_rows<K>(blockIdx.x, gridDim.x, n, x, st);
 }
 
-// x += alpha p, r_old = r, r -= alpha q; partials of r.r.
+// live columns: x += alpha p, r_old = r, r -= alpha q; partials of r_j.r_j.
+template <int K>
 __global__ void __launch_bounds__(kBlock) k_pcg_xr(int64_t n, double* __restrict__ x, double* __restrict__ r,
                                                    double* __restrict__ rold, const double* __restrict__ p,
                                                    const double* __restrict__ q, const CgState* st,
                                                    double* partials) {
-  if (st->done) return;
-  double acc[1] = {0.0};
-  pcg_xr_rows(blockIdx.x, gridDim.x, n, st->alpha, x, r, rold, p, q, acc);
-  block_partial<1>(acc, partials);
+  if (all_done<K>(st)) return;
+  double acc[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) acc[j] = 0.0;
+  pcg_xr_rows<K>(blockIdx.x, gridDim.x, n, st, x, r, rold, p, q, acc);
+  block_partial<K>(acc, partials);
 }
 
-// partials of r.z and r_old.z
-template <class T>
+// partials of r_j.z_j and r_old_j.z_j
+template <int K, class T>
 __global__ void __launch_bounds__(kBlock) k_pcg_dot(int64_t n, const double* __restrict__ r,
                                                     const double* __restrict__ rold, const T* __restrict__ z,
                                                     const CgState* st, double* partials) {
-  if (st->done) return;
-  double acc[2] = {0.0, 0.0};
-  pcg_dot_rows<T>(blockIdx.x, gridDim.x, n, r, rold, z, st->it == 0, acc);
-  block_partial<2>(acc, partials);
+  if (all_done<K>(st)) return;
+  double acc[2 * K];
+#pragma unroll
+  for (int j = 0; j < 2 * K; ++j) acc[j] = 0.0;
+  pcg_dot_rows<K, T>(blockIdx.x, gridDim.x, n, st, r, rold, z, acc);
+  block_partial<2 * K>(acc, partials);
 }
 
-// rho = r.z; beta = (r.z - r_old.z) / rho_old (flexible CG)
+// live columns: rho = r.z; beta = (r.z - r_old.z) / rho_old (flexible CG)
+template <int K>
 __global__ void __launch_bounds__(kRedBlock) k_pcg_red_rz(int nblk, const double* __restrict__ partials,
                                                           CgState* st) {
-  if (st->done) return;
-  __shared__ double lds[2 * (kRedBlock / 64)];
-  double v[2] = {0.0, 0.0};
-  for (int i = threadIdx.x; i < nblk; i += kRedBlock) {
-    v[0] += partials[i];
-    v[1] += partials[nblk + i];
-  }
-  block_sum_t<2, kRedBlock>(v, lds);
-  if (threadIdx.x == 0) pcg_state_rz(*st, v);
+  if (all_done<K>(st)) return;
+  __shared__ double lds[2 * K * (kRedBlock / 64)];
+  double v[2 * K];
+#pragma unroll
+  for (int j = 0; j < 2 * K; ++j) v[j] = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += kRedBlock)
+#pragma unroll
+    for (int j = 0; j < 2 * K; ++j) v[j] += partials[j * nblk + i];
+  block_sum_t<2 * K, kRedBlock>(v, lds);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+      if (!st[j].done) pcg_state_rz(st[j], v[j], v[K + j]);
 }
 
-template <class T>
+template <int K, class T>
 __global__ void __launch_bounds__(kBlock) k_pcg_pupd(int64_t n, const T* __restrict__ z, double* __restrict__ p,
                                                      const CgState* st) {
-  if (st->done) return;
-  pcg_pupd_rows<T>(blockIdx.x, gridDim.x, n, z, p, st->it == 0, st->beta);
+  if (all_done<K>(st)) return;
+  pcg_pupd_rows<K, T>(blockIdx.x, gridDim.x, n, st, z, p);
 }
 
 // ---------------------------------------------------------------- host
@@ -220,33 +240,33 @@ static const double* diag(const Sell& S) {
     return S.dgc;
 }
 
-template <int BS, class T, int MODE>
+template <int BS, int K, class T, int MODE>
 static void sweep(cpm_mg* M, const MgLevel& Lv, const void* din, void* dout, double c1, double c2,
                   const CgState* st) {
   const Sell& S = Lv.A->sell;
   hipStream_t s = M->ctx->stream;
   const int g = sell_grid(S);
   if (S.stencil)
-    k_mg_sweep<BS, T, MODE, true><<<g, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, vals<T>(S), diag<T>(S), grid_arg(S),
-                                                       (const T*)din, (T*)dout, (T*)Lv.x, (T*)Lv.r, (const T*)Lv.b,
-                                                       (const T*)Lv.dinv, c1, c2, st);
+    k_mg_sweep<BS, K, T, MODE, true><<<g, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, vals<T>(S), diag<T>(S),
+                                                          grid_arg(S), (const T*)din, (T*)dout, (T*)Lv.x, (T*)Lv.r,
+                                                          (const T*)Lv.b, (const T*)Lv.dinv, c1, c2, st);
   else
-    k_mg_sweep<BS, T, MODE, false><<<g, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, vals<T>(S), diag<T>(S),
-                                                        grid_arg(S), (const T*)din, (T*)dout, (T*)Lv.x, (T*)Lv.r,
-                                                        (const T*)Lv.b, (const T*)Lv.dinv, c1, c2, st);
+    k_mg_sweep<BS, K, T, MODE, false><<<g, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, vals<T>(S), diag<T>(S),
+                                                           grid_arg(S), (const T*)din, (T*)dout, (T*)Lv.x, (T*)Lv.r,
+                                                           (const T*)Lv.b, (const T*)Lv.dinv, c1, c2, st);
 }
 
-// One V-cycle from level l: b of level 0 read from the FP64 vector b64.
-// ev (profiling, may be null): events recorded around the first Chebyshev
-// step of level 0.
-template <int BS, class T>
+// One V-cycle from level l for K columns: b of level 0 read from the FP64
+// vector b64.  ev (profiling, may be null): events recorded around the first
+// Chebyshev step of level 0.
+template <int BS, int K, class T>
 static void vcycle(cpm_mg* M, size_t l, const double* b64, const CgState* st, hipEvent_t* ev = nullptr) {
   MgLevel& Lv = M->L[l];
   hipStream_t s = M->ctx->stream;
   const int gn = grid_for(Lv.nb);
   if (l + 1 == M->L.size()) {
     const int g = (int)std::min<int64_t>(kMaxGrid, (Lv.n + 3) / 4);
-    k_mg_dense<T><<<g, kBlock, 0, s>>>(Lv.n, (const T*)M->cinv, (const T*)Lv.b, (T*)Lv.x, st);
+    k_mg_dense<K, T><<<g, kBlock, 0, s>>>(Lv.n, (const T*)M->cinv, (const T*)Lv.b, (T*)Lv.x, st);
     return;
   }
   const int m = M->degree;
@@ -256,37 +276,37 @@ static void vcycle(cpm_mg* M, size_t l, const double* b64, const CgState* st, hi
   void* d[2] = {Lv.d0, Lv.d1};
   // pre-smoothing from x = 0
   if (l == 0 && b64)
-    k_mg_first<BS, T, true, true><<<gn, kBlock, 0, s>>>(Lv.nb, b64, (T*)Lv.b, (T*)Lv.r, (T*)d[0], (T*)Lv.x,
-                                                        (const T*)Lv.dinv, c0, st);
+    k_mg_first<BS, K, T, true, true><<<gn, kBlock, 0, s>>>(Lv.nb, b64, (T*)Lv.b, (T*)Lv.r, (T*)d[0], (T*)Lv.x,
+                                                           (const T*)Lv.dinv, c0, st);
   else
-    k_mg_first<BS, T, true, false><<<gn, kBlock, 0, s>>>(Lv.nb, nullptr, (T*)Lv.b, (T*)Lv.r, (T*)d[0], (T*)Lv.x,
-                                                         (const T*)Lv.dinv, c0, st);
+    k_mg_first<BS, K, T, true, false><<<gn, kBlock, 0, s>>>(Lv.nb, nullptr, (T*)Lv.b, (T*)Lv.r, (T*)d[0],
+                                                            (T*)Lv.x, (const T*)Lv.dinv, c0, st);
   for (int k = 1; k < m; ++k) {
     if (k == 1 && ev) (void)hipEventRecord(ev[0], s);
-    sweep<BS, T, 0>(M, Lv, d[(k - 1) & 1], d[k & 1], c1[k], c2[k], st);
+    sweep<BS, K, T, 0>(M, Lv, d[(k - 1) & 1], d[k & 1], c1[k], c2[k], st);
     if (k == 1 && ev) (void)hipEventRecord(ev[1], s);
   }
-  sweep<BS, T, 1>(M, Lv, d[(m - 1) & 1], nullptr, 0.0, 0.0, st);
+  sweep<BS, K, T, 1>(M, Lv, d[(m - 1) & 1], nullptr, 0.0, 0.0, st);
   // coarse-grid correction
   MgLevel& Lc = M->L[l + 1];
   const Dims3 fd{{Lv.dims[0], Lv.dims[1], Lv.dims[2]}}, cd{{Lc.dims[0], Lc.dims[1], Lc.dims[2]}};
-  k_mg_restrict<BS, T><<<grid_for(Lc.nb), kBlock, 0, s>>>(fd, cd, (const T*)Lv.r, (T*)Lc.b, st);
-  vcycle<BS, T>(M, l + 1, nullptr, st);
-  k_mg_prolong<BS, T><<<gn, kBlock, 0, s>>>(fd, cd, (const T*)Lc.x, (T*)Lv.x, st);
+  k_mg_restrict<BS * K, K, T><<<grid_for(Lc.nb), kBlock, 0, s>>>(fd, cd, M->anchor, (const T*)Lv.r, (T*)Lc.b, st);
+  vcycle<BS, K, T>(M, l + 1, nullptr, st);
+  k_mg_prolong<BS * K, K, T><<<gn, kBlock, 0, s>>>(fd, cd, M->anchor, (const T*)Lc.x, (T*)Lv.x, st);
   // post-smoothing: the same polynomial on the new residual
-  sweep<BS, T, 2>(M, Lv, nullptr, nullptr, 0.0, 0.0, st);
-  k_mg_first<BS, T, false, false><<<gn, kBlock, 0, s>>>(Lv.nb, nullptr, (T*)Lv.b, (T*)Lv.r, (T*)d[0], (T*)Lv.x,
-                                                        (const T*)Lv.dinv, c0, st);
-  for (int k = 1; k < m; ++k) sweep<BS, T, 0>(M, Lv, d[(k - 1) & 1], d[k & 1], c1[k], c2[k], st);
+  sweep<BS, K, T, 2>(M, Lv, nullptr, nullptr, 0.0, 0.0, st);
+  k_mg_first<BS, K, T, false, false><<<gn, kBlock, 0, s>>>(Lv.nb, nullptr, (T*)Lv.b, (T*)Lv.r, (T*)d[0], (T*)Lv.x,
+                                                           (const T*)Lv.dinv, c0, st);
+  for (int k = 1; k < m; ++k) sweep<BS, K, T, 0>(M, Lv, d[(k - 1) & 1], d[k & 1], c1[k], c2[k], st);
 }
 
 static void vcycle_any(cpm_mg* M, const double* b64, const CgState* st) {
   if (M->bs == 3) {
-    if (M->prec == 32) vcycle<3, float>(M, 0, b64, st);
-    else vcycle<3, double>(M, 0, b64, st);
+    if (M->prec == 32) vcycle<3, 1, float>(M, 0, b64, st);
+    else vcycle<3, 1, double>(M, 0, b64, st);
   } else {
-    if (M->prec == 32) vcycle<1, float>(M, 0, b64, st);
-    else vcycle<1, double>(M, 0, b64, st);
+    if (M->prec == 32) vcycle<1, 1, float>(M, 0, b64, st);
+    else vcycle<1, 1, double>(M, 0, b64, st);
   }
 }
 
@@ -367,80 +387,118 @@ static int mg_alloc(cpm_mg* M, void** p, size_t bytes) {
   return CPM_OK;
 }
 
-// Preconditioned CG on level 0 (see the file comment).
-template <int BS, class T>
+// Level and outer vectors for K columns (allocated for one column at
+// creation, grown on the first solve with more).
+int mg_reserve(cpm_mg* M, int K) {
+  if (K <= M->kcap) return CPM_OK;
+  cpm_ctx* ctx = M->ctx;
+  CPM_HIP(hipStreamSynchronize(ctx->stream));
+  const size_t ts = M->prec == 32 ? 4 : 8;
+  auto regrow = [&](void** p, size_t bytes) -> int {
+    auto it = std::find(M->allocs.begin(), M->allocs.end(), *p);
+    if (it != M->allocs.end()) M->allocs.erase(it);
+    if (*p) CPM_HIP(hipFree(*p));
+    *p = nullptr;
+    return mg_alloc(M, p, bytes);
+  };
+  for (MgLevel& Lv : M->L)
+    for (void** p : {&Lv.x, &Lv.r, &Lv.b, &Lv.d0, &Lv.d1})
+      if (int rc = regrow(p, ts * Lv.n * K)) return rc;
+  const int64_t n0 = M->L[0].n;
+  for (void** p : {(void**)&M->r, (void**)&M->p, (void**)&M->q, (void**)&M->rold})
+    if (int rc = regrow(p, 8 * n0 * K)) return rc;
+  CPM_HIP(hipMemsetAsync(M->rold, 0, 8 * n0 * K, ctx->stream));
+  CPM_HIP(hipStreamSynchronize(ctx->stream));
+  M->kcap = K;
+  return CPM_OK;
+}
+
+// Preconditioned CG on level 0 for K columns (see the file comment); column
+// j of b and x is entry i*K + j, with its own state ctx->cg[j].
+template <int BS, int K, class T>
 static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol, int maxiter, int hint,
                    int32_t* iters, double* resid, int* failed) {
   cpm_ctx* ctx = M->ctx;
   hipStream_t s = ctx->stream;
   MgLevel& L0 = M->L[0];
   const Sell& S = L0.A->sell;
-  const int gs = sell_grid(S), gv = grid_for(L0.n);
+  const int64_t n = L0.n * K;
+  const int gs = sell_grid(S), gv = grid_for(n);
   CgState* st = ctx->cg;
   auto spmv = [&](const double* in, double* out, bool plain) {
     if (plain) {
       if (S.stencil)
-        k_pcg_spmv<BS, true, true><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), in, out,
-                                                         st, ctx->partials);
+        k_pcg_spmv<BS, K, true, true><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), in,
+                                                            out, st, ctx->partials);
       else
-        k_pcg_spmv<BS, false, true><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), in,
-                                                          out, st, ctx->partials);
+        k_pcg_spmv<BS, K, false, true><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), in,
+                                                             out, st, ctx->partials);
     } else {
       if (S.stencil)
-        k_pcg_spmv<BS, true, false><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), in,
-                                                          out, st, ctx->partials);
+        k_pcg_spmv<BS, K, true, false><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), in,
+                                                             out, st, ctx->partials);
       else
-        k_pcg_spmv<BS, false, false><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), in,
-                                                           out, st, ctx->partials);
+        k_pcg_spmv<BS, K, false, false><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S),
+                                                              in, out, st, ctx->partials);
     }
   };
   // start: r = b - A x0 (or b), state, z = M r, rho = r.z
   if (use_x0) spmv(x, M->q, true);
-  k_pcg_init<<<gv, kBlock, 0, s>>>(L0.n, b, x, M->q, M->r, use_x0, ctx->partials);
-  k_pcg_red_init<<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st, tol, maxiter);
-  if (use_x0) k_pcg_zero_b<<<gv, kBlock, 0, s>>>(L0.n, x, st);
-  vcycle<BS, T>(M, 0, M->r, st);
-  k_pcg_dot<T><<<gv, kBlock, 0, s>>>(L0.n, M->r, M->rold, (const T*)L0.x, st, ctx->partials);
-  k_pcg_red_rz<<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st);
+  k_pcg_init<K><<<gv, kBlock, 0, s>>>(n, b, x, M->q, M->r, use_x0, ctx->partials);
+  k_pcg_red_init<K><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st, tol, maxiter);
+  if (use_x0) k_pcg_zero_b<K><<<gv, kBlock, 0, s>>>(n, x, st);
+  vcycle<BS, K, T>(M, 0, M->r, st);
+  k_pcg_dot<K, T><<<gv, kBlock, 0, s>>>(n, M->r, M->rold, (const T*)L0.x, st, ctx->partials);
+  k_pcg_red_rz<K><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st);
   CPM_HIP(hipGetLastError());
   // profiling: the first Chebyshev step of level 0 of every iteration is
   // timed with HIP events and accumulated on the level-0 operator
-  const bool prof = ctx->prof && M->degree >= 2;
+  const bool prof = ctx->prof && M->degree >= 2 && K == 1;
   auto issue = [&](int todo) -> int {
     for (int k = 0; k < todo; ++k) {
-      k_pcg_pupd<T><<<gv, kBlock, 0, s>>>(L0.n, (const T*)L0.x, M->p, st);
+      k_pcg_pupd<K, T><<<gv, kBlock, 0, s>>>(n, (const T*)L0.x, M->p, st);
       spmv(M->p, M->q, false);
-      k_red<1, RED_ALPHA><<<1, kRedBlock, 0, s>>>(gs, ctx->partials, st, 0.0);
-      k_pcg_xr<<<gv, kBlock, 0, s>>>(L0.n, x, M->r, M->rold, M->p, M->q, st, ctx->partials);
-      k_red<1, RED_RR_PCG><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st, 0.0);
-      vcycle<BS, T>(M, 0, M->r, st, prof ? &ctx->ev[2 * k] : nullptr);
-      k_pcg_dot<T><<<gv, kBlock, 0, s>>>(L0.n, M->r, M->rold, (const T*)L0.x, st, ctx->partials);
-      k_pcg_red_rz<<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st);
+      k_red<K, RED_ALPHA><<<1, kRedBlock, 0, s>>>(gs, ctx->partials, st, 0.0);
+      k_pcg_xr<K><<<gv, kBlock, 0, s>>>(n, x, M->r, M->rold, M->p, M->q, st, ctx->partials);
+      k_red<K, RED_RR_PCG><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st, 0.0);
+      vcycle<BS, K, T>(M, 0, M->r, st, prof ? &ctx->ev[2 * k] : nullptr);
+      k_pcg_dot<K, T><<<gv, kBlock, 0, s>>>(n, M->r, M->rold, (const T*)L0.x, st, ctx->partials);
+      k_pcg_red_rz<K><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st);
     }
     CPM_HIP(hipGetLastError());
     return CPM_OK;
   };
-  CgState h{};
+  CgState h[K];
   auto read = [&]() -> int {
-    CPM_HIP(hipMemcpyAsync(ctx->pinned, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+    CPM_HIP(hipMemcpyAsync(ctx->pinned, st, sizeof(CgState) * K, hipMemcpyDeviceToHost, s));
     CPM_HIP(hipStreamSynchronize(s));
-    std::memcpy(&h, ctx->pinned, sizeof(CgState));
+    std::memcpy(h, ctx->pinned, sizeof(CgState) * K);
     return CPM_OK;
+  };
+  auto all = [&]() {
+    for (int j = 0; j < K; ++j)
+      if (!h[j].done) return false;
+    return true;
+  };
+  auto it_max = [&]() {
+    int m = 0;
+    for (int j = 0; j < K; ++j) m = std::max(m, (int)h[j].it);
+    return m;
   };
   if (int rc = read()) return rc;
   int batch = hint > 2 ? hint - 1 : 4;
   int issued = 0;
   cpm_mat* A0 = L0.A;
-  while (!h.done && issued <= maxiter) {
+  while (!all() && issued <= maxiter) {
     const int todo = std::max(1, std::min(std::min(batch, prof ? 256 : 1024), maxiter - issued + 1));
-    const int it_before = h.it;
+    const int it_before = h[0].it;
     if (int rc = issue(todo)) return rc;
     issued += todo;
     if (int rc = read()) return rc;
     if (prof) {
       // V-cycles that did work: every iteration of this batch except a
       // converged last one
-      const int worked = std::min(todo, h.it - it_before - ((h.done && !h.fail) ? 1 : 0));
+      const int worked = std::min(todo, h[0].it - it_before - ((h[0].done && !h[0].fail) ? 1 : 0));
       for (int k = 0; k < worked; ++k) {
         float ms = 0.f;
         CPM_HIP(hipEventElapsedTime(&ms, ctx->ev[2 * k], ctx->ev[2 * k + 1]));
@@ -448,23 +506,32 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
         A0->prof_launches += 1;
       }
     }
-    batch = std::max(2, std::min(16, h.it / 4 + 2));
+    batch = std::max(2, std::min(16, it_max() / 4 + 2));
   }
-  *iters = h.it;
-  *resid = h.resid;
-  *failed = (h.fail || !h.done) ? 1 : 0;
+  *failed = 0;
+  for (int j = 0; j < K; ++j) {
+    iters[j] = h[j].it;
+    resid[j] = h[j].resid;
+    if (h[j].fail || !h[j].done) *failed = 1;
+  }
   return CPM_OK;
 }
 
 const cpm_mat* mg_operator(const cpm_mg* M) { return M && !M->L.empty() ? M->L[0].A : nullptr; }
 
-int mg_pcg(cpm_mg* M, const double* b, double* x, int use_x0, double tol, int maxiter, int hint, int32_t* iters,
-           double* resid, int* failed) {
+int mg_pcg(cpm_mg* M, int K, const double* b, double* x, int use_x0, double tol, int maxiter, int hint,
+           int32_t* iters, double* resid, int* failed) {
+  if (K != 1 && !(K == 3 && M->bs == 1))
+    return fail(CPM_EINVAL, "multigrid CG: 1 right-hand side, or 3 for a scalar operator");
+  if (int rc = mg_reserve(M, K)) return rc;
   if (M->bs == 3)
-    return M->prec == 32 ? pcg_run<3, float>(M, b, x, use_x0, tol, maxiter, hint, iters, resid, failed)
-                         : pcg_run<3, double>(M, b, x, use_x0, tol, maxiter, hint, iters, resid, failed);
-  return M->prec == 32 ? pcg_run<1, float>(M, b, x, use_x0, tol, maxiter, hint, iters, resid, failed)
-                       : pcg_run<1, double>(M, b, x, use_x0, tol, maxiter, hint, iters, resid, failed);
+    return M->prec == 32 ? pcg_run<3, 1, float>(M, b, x, use_x0, tol, maxiter, hint, iters, resid, failed)
+                         : pcg_run<3, 1, double>(M, b, x, use_x0, tol, maxiter, hint, iters, resid, failed);
+  if (K == 3)
+    return M->prec == 32 ? pcg_run<1, 3, float>(M, b, x, use_x0, tol, maxiter, hint, iters, resid, failed)
+                         : pcg_run<1, 3, double>(M, b, x, use_x0, tol, maxiter, hint, iters, resid, failed);
+  return M->prec == 32 ? pcg_run<1, 1, float>(M, b, x, use_x0, tol, maxiter, hint, iters, resid, failed)
+                       : pcg_run<1, 1, double>(M, b, x, use_x0, tol, maxiter, hint, iters, resid, failed);
 }
 
 }  // namespace cpm
@@ -510,6 +577,23 @@ extern "C" int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, 
   M->prec = precision;
   M->degree = degree;
   M->ratio = ratio;
+  {
+    // anchored operator (row 0 an identity row, as A_demag and A_el): node 0
+    // stays out of the transfers; otherwise (A1) it takes part
+    const Sell& F = levels[0]->sell;
+    const int64_t cnt = (int64_t)F.nst * bs * bs * 64;
+    std::vector<double> v0(std::max<int64_t>(cnt, 1), 0.0);
+    if (cnt > 0) {
+      const hipError_t e = hipMemcpy(v0.data(), F.v64, sizeof(double) * cnt, hipMemcpyDeviceToHost);
+      if (e != hipSuccess) {
+        delete M;
+        return fail(CPM_EHIP, hipGetErrorString(e));
+      }
+    }
+    bool off = false;
+    for (int64_t q = 0; q < cnt; q += 64) off |= v0[q] != 0.0;
+    M->anchor = off ? 0 : 1;
+  }
   const size_t ts = precision == 32 ? 4 : 8;
   int rc = CPM_OK;
   auto A = [&](void** p, size_t bytes) {
@@ -655,11 +739,13 @@ extern "C" int cpm_mg_apply(cpm_mg* M, const double* r, double* z) {
   return CPM_OK;
 }
 
-extern "C" int cpm_cg_mg(cpm_mg* M, const double* b, double* x, int use_x0, double tol, int maxiter, int32_t* iters,
-                         double* resid) {
+extern "C" int cpm_cg_mg(cpm_mg* M, int nrhs, const double* b, double* x, int use_x0, double tol, int maxiter,
+                         int32_t* iters, double* resid) {
   if (!M || !b || !x || !iters || !resid) return fail(CPM_EINVAL, "cpm_cg_mg: null argument");
   CPM_HIP(hipSetDevice(M->ctx->device));
   int failed = 0;
-  if (int rc = mg_pcg(M, b, x, use_x0, tol, maxiter, 0, iters, resid, &failed)) return rc;
-  return failed ? std::max(1, (int)*iters) : CPM_OK;
+  if (int rc = mg_pcg(M, nrhs, b, x, use_x0, tol, maxiter, 0, iters, resid, &failed)) return rc;
+  int worst = 0;
+  for (int j = 0; j < nrhs; ++j) worst = std::max(worst, (int)iters[j]);
+  return failed ? std::max(1, worst) : CPM_OK;
 }

@@ -7,45 +7,62 @@
 
 namespace cpm {
 
-template <int BS, class T>
-__device__ __forceinline__ void apply_dinv(const T* __restrict__ dinv, int64_t node, const double (&v)[BS],
-                                           double (&out)[BS]) {
+// Vectors of a level hold K right-hand sides interleaved per node:
+// entry (c*BS + e)*K + j for block row c, component e, column j (the layout
+// of the K-column SELL solvers); NV = BS*K entries per node.
+
+// out = D^-1 v at `node` (BS x BS block applied to each of the K columns)
+template <int BS, int K, class T>
+__device__ __forceinline__ void apply_dinv(const T* __restrict__ dinv, int64_t node, const double (&v)[BS * K],
+                                           double (&out)[BS * K]) {
 #pragma unroll
-  for (int d = 0; d < BS; ++d) {
-    double s = 0.0;
+  for (int d = 0; d < BS; ++d)
 #pragma unroll
-    for (int e = 0; e < BS; ++e) s += (double)dinv[node * BS * BS + d * BS + e] * v[e];
-    out[d] = s;
-  }
+    for (int j = 0; j < K; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int e = 0; e < BS; ++e) s += (double)dinv[node * BS * BS + d * BS + e] * v[e * K + j];
+      out[d * K + j] = s;
+    }
+}
+
+// all K columns of a solve have converged
+template <int K>
+__device__ __forceinline__ bool all_done(const CgState* st) {
+  bool d = true;
+#pragma unroll
+  for (int j = 0; j < K; ++j) d = d && st[j].done;
+  return d;
 }
 
 // First Chebyshev step.  ZERO: x0 = 0, so r = b (FROM64: b is the FP64
 // outer residual, also stored as this level's b), d = c0 D^-1 r, x = d.
 // Otherwise r is the current residual: d = c0 D^-1 r, x += d.
-template <int BS, class T, bool ZERO, bool FROM64>
+template <int BS, int K, class T, bool ZERO, bool FROM64>
 __device__ __forceinline__ void mg_first_rows(int64_t lb, int64_t lg, int32_t nb, const double* __restrict__ b64,
                                               T* __restrict__ b, T* __restrict__ r, T* __restrict__ d,
                                               T* __restrict__ x, const T* __restrict__ dinv, double c0) {
+  constexpr int NV = BS * K;
   const int64_t stride = lg * kBlock;
   for (int64_t i = lb * kBlock + threadIdx.x; i < nb; i += stride) {
-    double rv[BS], dv[BS];
+    double rv[NV], dv[NV];
 #pragma unroll
-    for (int c = 0; c < BS; ++c) {
+    for (int c = 0; c < NV; ++c) {
       if (ZERO) {
-        const double bv = FROM64 ? b64[i * BS + c] : (double)b[i * BS + c];
-        if (FROM64) b[i * BS + c] = (T)bv;
-        r[i * BS + c] = (T)bv;
+        const double bv = FROM64 ? b64[i * NV + c] : (double)b[i * NV + c];
+        if (FROM64) b[i * NV + c] = (T)bv;
+        r[i * NV + c] = (T)bv;
         rv[c] = (double)(T)bv;
       } else {
-        rv[c] = (double)r[i * BS + c];
+        rv[c] = (double)r[i * NV + c];
       }
     }
-    apply_dinv<BS, T>(dinv, i, rv, dv);
+    apply_dinv<BS, K, T>(dinv, i, rv, dv);
 #pragma unroll
-    for (int c = 0; c < BS; ++c) {
+    for (int c = 0; c < NV; ++c) {
       const T dn = (T)(c0 * dv[c]);
-      d[i * BS + c] = dn;
-      x[i * BS + c] = ZERO ? dn : (T)((double)x[i * BS + c] + (double)dn);
+      d[i * NV + c] = dn;
+      x[i * NV + c] = ZERO ? dn : (T)((double)x[i * NV + c] + (double)dn);
     }
   }
 }
@@ -53,34 +70,35 @@ __device__ __forceinline__ void mg_first_rows(int64_t lb, int64_t lg, int32_t nb
 // MODE 0: Chebyshev step r -= A d_in, d_out = c1 d_in + c2 D^-1 r, x += d_out.
 // MODE 1: r -= A d_in only (residual after the last step).
 // MODE 2: r = b - A x.
-template <int BS, class T, int MODE, bool ST>
+template <int BS, int K, class T, int MODE, bool ST>
 __device__ __forceinline__ void mg_sweep_rows(int64_t lb, int64_t lg, int32_t nb, const int64_t* __restrict__ sptr,
                                               const int32_t* __restrict__ col, const T* __restrict__ val,
                                               const double* __restrict__ dg, const GridArg& ga,
                                               const T* __restrict__ din, T* __restrict__ dout, T* __restrict__ x,
                                               T* __restrict__ r, const T* __restrict__ b,
                                               const T* __restrict__ dinv, double c1, double c2) {
+  constexpr int NV = BS * K;
   const T* src = (MODE == 2) ? x : din;
   const int64_t stride = lg * kBlock;
   for (int64_t row = lb * kBlock + threadIdx.x; row < (int64_t)((nb + 63) & ~63); row += stride) {
-    double sum[BS], xo[BS];
-    sell_row<BS, 1, T, false, ST>(row, nb, sptr, col, val, dg, src, nullptr, ga, sum, xo);
+    double sum[NV], xo[NV];
+    sell_row<BS, K, T, false, ST>(row, nb, sptr, col, val, dg, src, nullptr, ga, sum, xo);
     if (row >= nb) continue;
-    double rv[BS];
+    double rv[NV];
 #pragma unroll
-    for (int c = 0; c < BS; ++c) {
-      rv[c] = (MODE == 2) ? (double)b[row * BS + c] - sum[c] : (double)r[row * BS + c] - sum[c];
-      r[row * BS + c] = (T)rv[c];
+    for (int c = 0; c < NV; ++c) {
+      rv[c] = (MODE == 2) ? (double)b[row * NV + c] - sum[c] : (double)r[row * NV + c] - sum[c];
+      r[row * NV + c] = (T)rv[c];
       rv[c] = (double)(T)rv[c];
     }
     if (MODE == 0) {
-      double dv[BS];
-      apply_dinv<BS, T>(dinv, row, rv, dv);
+      double dv[NV];
+      apply_dinv<BS, K, T>(dinv, row, rv, dv);
 #pragma unroll
-      for (int c = 0; c < BS; ++c) {
+      for (int c = 0; c < NV; ++c) {
         const T dn = (T)(c1 * xo[c] + c2 * dv[c]);
-        dout[row * BS + c] = dn;
-        x[row * BS + c] = (T)((double)x[row * BS + c] + (double)dn);
+        dout[row * NV + c] = dn;
+        x[row * NV + c] = (T)((double)x[row * NV + c] + (double)dn);
       }
     }
   }
@@ -138,13 +156,14 @@ __device__ __forceinline__ void kuhn_weights(int32_t i, int32_t j, int32_t k, co
   w[3] = t[p2];
 }
 
-// b_c = R r_f (R = P^T of the P1 interpolation), anchor node 0 excluded on
-// both grids.  Nested grids (fine = 2 x coarse) use the fixed stencil; other
-// size pairs gather, for every coarse node, over the fine nodes whose cell
-// is the node's own or the one below it in each dimension.
+// b_c = R r_f (R = P^T of the P1 interpolation); anchor: node 0 (the
+// operators' anchor) excluded on both grids.  Nested grids (fine = 2 x
+// coarse) use the fixed stencil; other size pairs gather, for every coarse
+// node, over the fine nodes whose cell is the node's own or the one below it
+// in each dimension.
 template <int BS, class T>
 __device__ __forceinline__ void mg_restrict_rows(int64_t lb, int64_t lg, const Dims3& fine, const Dims3& coarse,
-                                                 const T* __restrict__ rf, T* __restrict__ bc) {
+                                                 bool anchor, const T* __restrict__ rf, T* __restrict__ bc) {
   const int64_t nc = (int64_t)coarse.d[0] * coarse.d[1] * coarse.d[2];
   const int64_t stride = lg * kBlock;
   const bool nested = nested_dims(fine, coarse);
@@ -152,14 +171,14 @@ __device__ __forceinline__ void mg_restrict_rows(int64_t lb, int64_t lg, const D
     double acc[BS];
 #pragma unroll
     for (int c = 0; c < BS; ++c) acc[c] = 0.0;
-    if (I != 0) {
+    if (!anchor || I != 0) {
       const int32_t i = (int32_t)(I / ((int64_t)coarse.d[1] * coarse.d[2]));
       const int32_t j = (int32_t)((I / coarse.d[2]) % coarse.d[1]);
       const int32_t k = (int32_t)(I % coarse.d[2]);
       if (nested) {
         const int64_t f0 = wrap_idx(2 * i, 2 * j, 2 * k, fine.d);
 #pragma unroll
-        for (int c = 0; c < BS; ++c) acc[c] = (double)rf[f0 * BS + c];
+        for (int c = 0; c < BS; ++c) acc[c] = (anchor && f0 == 0) ? 0.0 : (double)rf[f0 * BS + c];
 #pragma unroll
         for (int dd = 1; dd < 8; ++dd) {
           const int32_t a = dd & 1, bb = (dd >> 1) & 1, cc = (dd >> 2) & 1;
@@ -167,8 +186,8 @@ __device__ __forceinline__ void mg_restrict_rows(int64_t lb, int64_t lg, const D
           const int64_t fm = wrap_idx(2 * i - a, 2 * j - bb, 2 * k - cc, fine.d);
 #pragma unroll
           for (int c = 0; c < BS; ++c) {
-            if (fp != 0) acc[c] += 0.5 * (double)rf[fp * BS + c];
-            if (fm != 0) acc[c] += 0.5 * (double)rf[fm * BS + c];
+            if (!anchor || fp != 0) acc[c] += 0.5 * (double)rf[fp * BS + c];
+            if (!anchor || fm != 0) acc[c] += 0.5 * (double)rf[fm * BS + c];
           }
         }
       } else {
@@ -193,7 +212,7 @@ __device__ __forceinline__ void mg_restrict_rows(int64_t lb, int64_t lg, const D
               const int32_t wj = fj < 0 ? fj + fine.d[1] : (fj >= fine.d[1] ? fj - fine.d[1] : fj);
               const int32_t wk = fk < 0 ? fk + fine.d[2] : (fk >= fine.d[2] ? fk - fine.d[2] : fk);
               const int64_t f = ((int64_t)wi * fine.d[1] + wj) * fine.d[2] + wk;
-              if (f == 0) continue;
+              if (anchor && f == 0) continue;
               int64_t v[4];
               double w[4];
               kuhn_weights(wi, wj, wk, fine, coarse, v, w);
@@ -211,15 +230,15 @@ __device__ __forceinline__ void mg_restrict_rows(int64_t lb, int64_t lg, const D
   }
 }
 
-// x_f += P e_c (P1 interpolation), anchor excluded on both grids.
+// x_f += P e_c (P1 interpolation); anchor: node 0 excluded on both grids.
 template <int BS, class T>
 __device__ __forceinline__ void mg_prolong_rows(int64_t lb, int64_t lg, const Dims3& fine, const Dims3& coarse,
-                                                const T* __restrict__ ec, T* __restrict__ xf) {
+                                                bool anchor, const T* __restrict__ ec, T* __restrict__ xf) {
   const int64_t nf = (int64_t)fine.d[0] * fine.d[1] * fine.d[2];
   const int64_t stride = lg * kBlock;
   const bool nested = nested_dims(fine, coarse);
   for (int64_t f = lb * kBlock + threadIdx.x; f < nf; f += stride) {
-    if (f == 0) continue;
+    if (anchor && f == 0) continue;
     const int32_t i = (int32_t)(f / ((int64_t)fine.d[1] * fine.d[2]));
     const int32_t j = (int32_t)((f / fine.d[2]) % fine.d[1]);
     const int32_t k = (int32_t)(f % fine.d[2]);
@@ -230,12 +249,13 @@ __device__ __forceinline__ void mg_prolong_rows(int64_t lb, int64_t lg, const Di
       const int64_t c0 = wrap_idx(I, J, K, coarse.d);
       if ((a | bb | cc) == 0) {
 #pragma unroll
-        for (int c = 0; c < BS; ++c) v[c] = (c0 != 0) ? (double)ec[c0 * BS + c] : 0.0;
+        for (int c = 0; c < BS; ++c) v[c] = (!anchor || c0 != 0) ? (double)ec[c0 * BS + c] : 0.0;
       } else {
         const int64_t c1 = wrap_idx(I + a, J + bb, K + cc, coarse.d);
 #pragma unroll
         for (int c = 0; c < BS; ++c)
-          v[c] = 0.5 * (((c0 != 0) ? (double)ec[c0 * BS + c] : 0.0) + ((c1 != 0) ? (double)ec[c1 * BS + c] : 0.0));
+          v[c] = 0.5 * (((!anchor || c0 != 0) ? (double)ec[c0 * BS + c] : 0.0) +
+                        ((!anchor || c1 != 0) ? (double)ec[c1 * BS + c] : 0.0));
       }
     } else {
       int64_t cv[4];
@@ -245,7 +265,7 @@ __device__ __forceinline__ void mg_prolong_rows(int64_t lb, int64_t lg, const Di
       for (int c = 0; c < BS; ++c) v[c] = 0.0;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        if (cv[q] == 0 || w[q] == 0.0) continue;
+        if ((anchor && cv[q] == 0) || w[q] == 0.0) continue;
 #pragma unroll
         for (int c = 0; c < BS; ++c) v[c] += w[q] * (double)ec[cv[q] * BS + c];
       }
@@ -255,28 +275,41 @@ __device__ __forceinline__ void mg_prolong_rows(int64_t lb, int64_t lg, const Di
   }
 }
 
-// x = Cinv b on the coarsest level: one wavefront per row.
-template <class T>
+// x = Cinv b on the coarsest level (n rows of Cinv, K columns of b and x):
+// one wavefront per row.
+template <int K, class T>
 __device__ __forceinline__ void mg_dense_rows(int64_t lb, int64_t lg, int64_t n, const T* __restrict__ cinv,
                                               const T* __restrict__ b, T* __restrict__ x) {
   const int lane = threadIdx.x & 63;
   const int64_t waves = lg * (kBlock / 64);
   for (int64_t row = lb * (kBlock / 64) + threadIdx.x / 64; row < n; row += waves) {
-    double s = 0.0;
-    for (int64_t c = lane; c < n; c += 64) s += (double)cinv[row * n + c] * (double)b[c];
-    s = group_sum<64>(s);
-    if (lane == 0) x[row] = (T)s;
+    double s[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) s[j] = 0.0;
+    for (int64_t c = lane; c < n; c += 64) {
+      const double a = (double)cinv[row * n + c];
+#pragma unroll
+      for (int j = 0; j < K; ++j) s[j] += a * (double)b[c * K + j];
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j) s[j] = group_sum<64>(s[j]);
+    if (lane == 0)
+#pragma unroll
+      for (int j = 0; j < K; ++j) x[row * K + j] = (T)s[j];
   }
 }
 
 // ---- preconditioned CG vector bodies ------------------------------------------
+// Vectors of length n = rows * K, column j = i % K; st[j] is column j's state.
 
-// r = b - y (use_x0) or x = 0, r = b; acc = (||b||^2, ||r||^2).
+// r = b - y (use_x0) or x = 0, r = b; acc[j] = ||b_j||^2, acc[K + j] = ||r_j||^2.
+template <int K>
 __device__ __forceinline__ void pcg_init_rows(int64_t lb, int64_t lg, int64_t n, const double* __restrict__ b,
                                               double* __restrict__ x, const double* __restrict__ y,
-                                              double* __restrict__ r, int use_x0, double (&acc)[2]) {
+                                              double* __restrict__ r, int use_x0, double (&acc)[2 * K]) {
   const int64_t stride = lg * kBlock;
   for (int64_t i = lb * kBlock + threadIdx.x; i < n; i += stride) {
+    const int j = (int)(i % K);
     const double bi = b[i];
     double ri;
     if (use_x0) {
@@ -286,54 +319,72 @@ __device__ __forceinline__ void pcg_init_rows(int64_t lb, int64_t lg, int64_t n,
       ri = bi;
     }
     r[i] = ri;
-    acc[0] += bi * bi;
-    acc[1] += ri * ri;
+    acc[j] += bi * bi;
+    acc[K + j] += ri * ri;
   }
 }
 
-// x += alpha p, r_old = r, r -= alpha q; acc += r.r.
-__device__ __forceinline__ void pcg_xr_rows(int64_t lb, int64_t lg, int64_t n, double alpha, double* __restrict__ x,
-                                            double* __restrict__ r, double* __restrict__ rold,
-                                            const double* __restrict__ p, const double* __restrict__ q,
-                                            double (&acc)[1]) {
+// live columns: x += alpha p, r_old = r, r -= alpha q; acc[j] += r_j.r_j.
+template <int K>
+__device__ __forceinline__ void pcg_xr_rows(int64_t lb, int64_t lg, int64_t n, const CgState* st,
+                                            double* __restrict__ x, double* __restrict__ r,
+                                            double* __restrict__ rold, const double* __restrict__ p,
+                                            const double* __restrict__ q, double (&acc)[K]) {
+  double alpha[K];
+  bool live[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) live[j] = !st[j].done, alpha[j] = st[j].alpha;
   const int64_t stride = lg * kBlock;
   for (int64_t i = lb * kBlock + threadIdx.x; i < n; i += stride) {
-    x[i] += alpha * p[i];
+    const int j = (int)(i % K);
+    if (!live[j]) continue;
+    x[i] += alpha[j] * p[i];
     const double ro = r[i];
     rold[i] = ro;
-    const double rn = ro - alpha * q[i];
+    const double rn = ro - alpha[j] * q[i];
     r[i] = rn;
-    acc[0] += rn * rn;
+    acc[j] += rn * rn;
   }
 }
 
-// acc = (r.z, r_old.z)  (r_old.z only after the first iteration)
-template <class T>
-__device__ __forceinline__ void pcg_dot_rows(int64_t lb, int64_t lg, int64_t n, const double* __restrict__ r,
-                                             const double* __restrict__ rold, const T* __restrict__ z, bool first,
-                                             double (&acc)[2]) {
+// acc[j] = r_j.z_j, acc[K + j] = r_old_j.z_j (the latter after the first iteration)
+template <int K, class T>
+__device__ __forceinline__ void pcg_dot_rows(int64_t lb, int64_t lg, int64_t n, const CgState* st,
+                                             const double* __restrict__ r, const double* __restrict__ rold,
+                                             const T* __restrict__ z, double (&acc)[2 * K]) {
+  bool first[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) first[j] = st[j].it == 0;
   const int64_t stride = lg * kBlock;
   for (int64_t i = lb * kBlock + threadIdx.x; i < n; i += stride) {
+    const int j = (int)(i % K);
     const double zi = (double)z[i];
-    acc[0] += r[i] * zi;
-    if (!first) acc[1] += rold[i] * zi;
+    acc[j] += r[i] * zi;
+    if (!first[j]) acc[K + j] += rold[i] * zi;
   }
 }
 
-// p = z (first iteration) or z + beta p
-template <class T>
-__device__ __forceinline__ void pcg_pupd_rows(int64_t lb, int64_t lg, int64_t n, const T* __restrict__ z,
-                                              double* __restrict__ p, bool first, double beta) {
+// live columns: p = z (first iteration) or z + beta p
+template <int K, class T>
+__device__ __forceinline__ void pcg_pupd_rows(int64_t lb, int64_t lg, int64_t n, const CgState* st,
+                                              const T* __restrict__ z, double* __restrict__ p) {
+  double beta[K];
+  bool live[K], first[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) live[j] = !st[j].done, first[j] = st[j].it == 0, beta[j] = st[j].beta;
   const int64_t stride = lg * kBlock;
-  for (int64_t i = lb * kBlock + threadIdx.x; i < n; i += stride)
-    p[i] = first ? (double)z[i] : (double)z[i] + beta * p[i];
+  for (int64_t i = lb * kBlock + threadIdx.x; i < n; i += stride) {
+    const int j = (int)(i % K);
+    if (!live[j]) continue;
+    p[i] = first[j] ? (double)z[i] : (double)z[i] + beta[j] * p[i];
+  }
 }
 
 // rho = r.z, beta = (r.z - r_old.z) / rho_old (flexible CG), from the sums v
-__device__ inline void pcg_state_rz(CgState& c, const double (&v)[2]) {
+__device__ inline void pcg_state_rz(CgState& c, double rz, double rold_z) {
   c.rho_old = c.rho;
-  c.rho = v[0];
-  c.beta = (c.it == 0) ? 0.0 : (v[0] - v[1]) / c.rho_old;
+  c.rho = rz;
+  c.beta = (c.it == 0) ? 0.0 : (rz - rold_z) / c.rho_old;
 }
 
 }  // namespace cpm

@@ -30,12 +30,13 @@ struct MultiMgSolver {
 };
 
 // Solve every system of `sys` (hierarchies of one block size, V-cycle
-// precision and smoother degree; each hierarchy at most once) with one
-// launch per phase for the whole batch.  Each system keeps its own
-// scalars, stops on its own tolerance and uses its own workgroups and
-// reduction order (those of mg_pcg), so its result equals mg_pcg's on the
-// same input bit for bit, whatever the other members.
-int multi_mg_pcg(cpm_ctx* ctx, MultiMgSolver& W, const std::vector<MultiMgSys>& sys, int hint, int32_t* iters,
-                 double* resid, int* failed);
+// precision and smoother degree; each hierarchy at most once; K interleaved
+// columns each, K = 1 or 3 for scalar operators) with one launch per phase
+// for the whole batch.  Each system keeps its own scalars, stops on its own
+// tolerance and uses its own workgroups and reduction order (those of
+// mg_pcg), so its result equals mg_pcg's on the same input bit for bit,
+// whatever the other members.  iters and resid: K entries per system.
+int multi_mg_pcg(cpm_ctx* ctx, MultiMgSolver& W, const std::vector<MultiMgSys>& sys, int K, int hint,
+                 int32_t* iters, double* resid, int* failed);
 
 }  // namespace cpm

@@ -54,6 +54,7 @@ struct SmoothSeg {
   const double* b64;  // depth 0: the system's FP64 outer residual
   double c0, c1[8], c2[8];
   Dims3 fd, cd;
+  int32_t anchor;     // node 0 left out of the transfers
   void *bc, *xc;      // next level's b and x
   const CgState* st;
   int32_t blk0, nblk;
@@ -69,116 +70,128 @@ struct DenseSeg {
 };
 
 // ---------------------------------------------------------------- kernels
+// K columns per system (interleaved as in mg.hip); a system's V-cycle work
+// runs while any of its columns is live, its CG updates per live column.
 
-template <int BS, class T, bool ZERO, bool FROM64>
+template <int BS, int K, class T, bool ZERO, bool FROM64>
 __global__ void __launch_bounds__(kBlock) kbm_first(const SmoothSeg* __restrict__ seg, const int32_t* __restrict__ map) {
   const SmoothSeg& d = seg[map[blockIdx.x]];
-  if (d.st->done) return;
-  mg_first_rows<BS, T, ZERO, FROM64>((int64_t)blockIdx.x - d.blk0, d.nblk, d.nb, d.b64, (T*)d.b, (T*)d.r,
-                                     (T*)d.d0, (T*)d.x, (const T*)d.dinv, d.c0);
+  if (all_done<K>(d.st)) return;
+  mg_first_rows<BS, K, T, ZERO, FROM64>((int64_t)blockIdx.x - d.blk0, d.nblk, d.nb, d.b64, (T*)d.b, (T*)d.r,
+                                        (T*)d.d0, (T*)d.x, (const T*)d.dinv, d.c0);
 }
 
 // MODE 0: Chebyshev step k (d_in = d[(k-1)&1], d_out = d[k&1]); MODE 1: the
 // residual after step `k` (d_in = d[k&1]); MODE 2: r = b - A x.
-template <int BS, class T, int MODE>
+template <int BS, int K, class T, int MODE>
 __global__ void __launch_bounds__(kBlock) kbm_sweep(const SmoothSeg* __restrict__ seg, const int32_t* __restrict__ map,
                                                     int k) {
   const SmoothSeg& d = seg[map[blockIdx.x]];
-  if (d.st->done) return;
+  if (all_done<K>(d.st)) return;
   T* dv[2] = {(T*)d.d0, (T*)d.d1};
   const T* din = (MODE == 0) ? dv[(k - 1) & 1] : dv[k & 1];
   T* dout = dv[k & 1];
   const double c1 = (MODE == 0) ? d.c1[k] : 0.0, c2 = (MODE == 0) ? d.c2[k] : 0.0;
   const int64_t lb = (int64_t)blockIdx.x - d.blk0;
   if (d.stencil)
-    mg_sweep_rows<BS, T, MODE, true>(lb, d.nblk, d.nb, d.sptr, d.col, (const T*)d.val, d.dg, d.ga, din, dout,
-                                     (T*)d.x, (T*)d.r, (const T*)d.b, (const T*)d.dinv, c1, c2);
+    mg_sweep_rows<BS, K, T, MODE, true>(lb, d.nblk, d.nb, d.sptr, d.col, (const T*)d.val, d.dg, d.ga, din, dout,
+                                        (T*)d.x, (T*)d.r, (const T*)d.b, (const T*)d.dinv, c1, c2);
   else
-    mg_sweep_rows<BS, T, MODE, false>(lb, d.nblk, d.nb, d.sptr, d.col, (const T*)d.val, d.dg, d.ga, din, dout,
-                                      (T*)d.x, (T*)d.r, (const T*)d.b, (const T*)d.dinv, c1, c2);
+    mg_sweep_rows<BS, K, T, MODE, false>(lb, d.nblk, d.nb, d.sptr, d.col, (const T*)d.val, d.dg, d.ga, din, dout,
+                                         (T*)d.x, (T*)d.r, (const T*)d.b, (const T*)d.dinv, c1, c2);
 }
 
-template <int BS, class T>
+template <int W, int K, class T>
 __global__ void __launch_bounds__(kBlock) kbm_restrict(const SmoothSeg* __restrict__ seg,
                                                        const int32_t* __restrict__ map) {
   const SmoothSeg& d = seg[map[blockIdx.x]];
-  if (d.st->done) return;
-  mg_restrict_rows<BS, T>((int64_t)blockIdx.x - d.blk0, d.nblk, d.fd, d.cd, (const T*)d.r, (T*)d.bc);
+  if (all_done<K>(d.st)) return;
+  mg_restrict_rows<W, T>((int64_t)blockIdx.x - d.blk0, d.nblk, d.fd, d.cd, d.anchor != 0, (const T*)d.r, (T*)d.bc);
 }
 
-template <int BS, class T>
+template <int W, int K, class T>
 __global__ void __launch_bounds__(kBlock) kbm_prolong(const SmoothSeg* __restrict__ seg,
                                                       const int32_t* __restrict__ map) {
   const SmoothSeg& d = seg[map[blockIdx.x]];
-  if (d.st->done) return;
-  mg_prolong_rows<BS, T>((int64_t)blockIdx.x - d.blk0, d.nblk, d.fd, d.cd, (const T*)d.xc, (T*)d.x);
+  if (all_done<K>(d.st)) return;
+  mg_prolong_rows<W, T>((int64_t)blockIdx.x - d.blk0, d.nblk, d.fd, d.cd, d.anchor != 0, (const T*)d.xc, (T*)d.x);
 }
 
-template <class T>
+template <int K, class T>
 __global__ void __launch_bounds__(kBlock) kbm_dense(const DenseSeg* __restrict__ seg, const int32_t* __restrict__ map) {
   const DenseSeg& d = seg[map[blockIdx.x]];
-  if (d.st->done) return;
-  mg_dense_rows<T>((int64_t)blockIdx.x - d.blk0, d.nblk, d.n, (const T*)d.cinv, (const T*)d.b, (T*)d.x);
+  if (all_done<K>(d.st)) return;
+  mg_dense_rows<K, T>((int64_t)blockIdx.x - d.blk0, d.nblk, d.n, (const T*)d.cinv, (const T*)d.b, (T*)d.x);
 }
 
-// q = A p and partials of p.q (PLAIN: y = A x for the start, no partials)
-template <int BS, bool PLAIN>
+// q = A p and partials of p_j.q_j (PLAIN: y = A x for the start, no partials)
+template <int BS, int K, bool PLAIN>
 __global__ void __launch_bounds__(kBlock) kbm_spmv(const OuterSeg* __restrict__ seg, const int32_t* __restrict__ map,
                                                    double* partials) {
   const OuterSeg& d = seg[map[blockIdx.x]];
-  if (PLAIN ? !d.use_x0 : d.st->done) return;
-  double acc[1] = {0.0};
+  if (PLAIN ? !d.use_x0 : all_done<K>(d.st)) return;
+  double acc[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) acc[j] = 0.0;
   const int64_t lb = (int64_t)blockIdx.x - d.sblk0;
   const double* in = PLAIN ? d.x : d.p;
   if (d.stencil)
-    sell_rows<BS, 1, double, !PLAIN, false, true>(lb, d.snblk, d.nb, d.sptr, d.col, d.v64, d.dg64, in, nullptr, d.q,
+    sell_rows<BS, K, double, !PLAIN, false, true>(lb, d.snblk, d.nb, d.sptr, d.col, d.v64, d.dg64, in, nullptr, d.q,
                                                   d.ga, acc);
   else
-    sell_rows<BS, 1, double, !PLAIN, false, false>(lb, d.snblk, d.nb, d.sptr, d.col, d.v64, d.dg64, in, nullptr, d.q,
+    sell_rows<BS, K, double, !PLAIN, false, false>(lb, d.snblk, d.nb, d.sptr, d.col, d.v64, d.dg64, in, nullptr, d.q,
                                                    d.ga, acc);
-  if (!PLAIN) block_partial<1>(acc, partials);
+  if (!PLAIN) block_partial<K>(acc, partials);
 }
 
+template <int K>
 __global__ void __launch_bounds__(kBlock) kbm_init(const OuterSeg* __restrict__ seg, const int32_t* __restrict__ map,
                                                    double* partials) {
   const OuterSeg& d = seg[map[blockIdx.x]];
-  double acc[2] = {0.0, 0.0};
-  pcg_init_rows((int64_t)blockIdx.x - d.blk0, d.nblk, d.n, d.b, d.x, d.q, d.r, d.use_x0, acc);
-  block_partial<2>(acc, partials);
+  double acc[2 * K];
+#pragma unroll
+  for (int j = 0; j < 2 * K; ++j) acc[j] = 0.0;
+  pcg_init_rows<K>((int64_t)blockIdx.x - d.blk0, d.nblk, d.n, d.b, d.x, d.q, d.r, d.use_x0, acc);
+  block_partial<2 * K>(acc, partials);
 }
 
+template <int K>
 __global__ void __launch_bounds__(kBlock) kbm_zero_b(const OuterSeg* __restrict__ seg,
                                                      const int32_t* __restrict__ map) {
   const OuterSeg& d = seg[map[blockIdx.x]];
-  if (!d.use_x0 || d.st->bnorm2 != 0.0) return;
-  const int64_t stride = (int64_t)d.nblk * kBlock;
-  for (int64_t i = ((int64_t)blockIdx.x - d.blk0) * kBlock + threadIdx.x; i < d.n; i += stride) d.x[i] = 0.0;
+  if (!d.use_x0) return;
+  zero_b_rows<K>((int64_t)blockIdx.x - d.blk0, d.nblk, d.n, d.x, d.st);
 }
 
+template <int K>
 __global__ void __launch_bounds__(kBlock) kbm_xr(const OuterSeg* __restrict__ seg, const int32_t* __restrict__ map,
                                                  double* partials) {
   const OuterSeg& d = seg[map[blockIdx.x]];
-  if (d.st->done) return;
-  double acc[1] = {0.0};
-  pcg_xr_rows((int64_t)blockIdx.x - d.blk0, d.nblk, d.n, d.st->alpha, d.x, d.r, d.rold, d.p, d.q, acc);
-  block_partial<1>(acc, partials);
+  if (all_done<K>(d.st)) return;
+  double acc[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) acc[j] = 0.0;
+  pcg_xr_rows<K>((int64_t)blockIdx.x - d.blk0, d.nblk, d.n, d.st, d.x, d.r, d.rold, d.p, d.q, acc);
+  block_partial<K>(acc, partials);
 }
 
-template <class T>
+template <int K, class T>
 __global__ void __launch_bounds__(kBlock) kbm_dot(const OuterSeg* __restrict__ seg, const int32_t* __restrict__ map,
                                                   double* partials) {
   const OuterSeg& d = seg[map[blockIdx.x]];
-  if (d.st->done) return;
-  double acc[2] = {0.0, 0.0};
-  pcg_dot_rows<T>((int64_t)blockIdx.x - d.blk0, d.nblk, d.n, d.r, d.rold, (const T*)d.z, d.st->it == 0, acc);
-  block_partial<2>(acc, partials);
+  if (all_done<K>(d.st)) return;
+  double acc[2 * K];
+#pragma unroll
+  for (int j = 0; j < 2 * K; ++j) acc[j] = 0.0;
+  pcg_dot_rows<K, T>((int64_t)blockIdx.x - d.blk0, d.nblk, d.n, d.st, d.r, d.rold, (const T*)d.z, acc);
+  block_partial<2 * K>(acc, partials);
 }
 
-template <class T>
+template <int K, class T>
 __global__ void __launch_bounds__(kBlock) kbm_pupd(const OuterSeg* __restrict__ seg, const int32_t* __restrict__ map) {
   const OuterSeg& d = seg[map[blockIdx.x]];
-  if (d.st->done) return;
-  pcg_pupd_rows<T>((int64_t)blockIdx.x - d.blk0, d.nblk, d.n, (const T*)d.z, d.p, d.st->it == 0, d.st->beta);
+  if (all_done<K>(d.st)) return;
+  pcg_pupd_rows<K, T>((int64_t)blockIdx.x - d.blk0, d.nblk, d.n, d.st, (const T*)d.z, d.p);
 }
 
 enum : int { MRED_INIT = 0, MRED_ALPHA = 1, MRED_RR = 2, MRED_RZ = 3 };
@@ -186,12 +199,12 @@ enum : int { MRED_INIT = 0, MRED_ALPHA = 1, MRED_RR = 2, MRED_RZ = 3 };
 // One workgroup per system: the system's partials in a fixed order (those
 // of its SpMV workgroups for MRED_ALPHA, of its vector workgroups otherwise),
 // as the one-system reducers of mg.hip sum them.
-template <int MODE>
+template <int K, int MODE>
 __global__ void __launch_bounds__(kRedBlock) kbm_red(const OuterSeg* __restrict__ seg, int G,
                                                      const double* __restrict__ partials) {
-  constexpr int NV = (MODE == MRED_INIT || MODE == MRED_RZ) ? 2 : 1;
+  constexpr int NV = (MODE == MRED_INIT || MODE == MRED_RZ) ? 2 * K : K;
   const OuterSeg& d = seg[blockIdx.x];
-  if (MODE != MRED_INIT && d.st->done) return;
+  if (MODE != MRED_INIT && all_done<K>(d.st)) return;
   __shared__ double lds[NV * (kRedBlock / 64)];
   double v[NV];
 #pragma unroll
@@ -203,13 +216,15 @@ __global__ void __launch_bounds__(kRedBlock) kbm_red(const OuterSeg* __restrict_
   block_sum_t<NV, kRedBlock>(v, lds);
   if (threadIdx.x != 0) return;
   if constexpr (MODE == MRED_INIT) {
-    cg_state_init<1, false>(d.st, v, d.tol, d.maxiter);
+    cg_state_init<K, false>(d.st, v, d.tol, d.maxiter);
   } else if constexpr (MODE == MRED_ALPHA) {
-    cg_state_update<1, RED_ALPHA>(d.st, v, 0.0);
+    cg_state_update<K, RED_ALPHA>(d.st, v, 0.0);
   } else if constexpr (MODE == MRED_RR) {
-    cg_state_update<1, RED_RR_PCG>(d.st, v, 0.0);
+    cg_state_update<K, RED_RR_PCG>(d.st, v, 0.0);
   } else {
-    pcg_state_rz(*d.st, v);
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+      if (!d.st[j].done) pcg_state_rz(d.st[j], v[j], v[K + j]);
   }
 }
 
@@ -280,7 +295,7 @@ struct MultiMgTables {
 MultiMgSolver::MultiMgSolver() : T(new MultiMgTables()) {}
 MultiMgSolver::~MultiMgSolver() { delete T; }
 
-template <int BS, class T>
+template <int BS, int K, class T>
 static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sys, int hint, int32_t* iters,
                double* resid, int* failed) {
   hipStream_t s = ctx->stream;
@@ -288,10 +303,10 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
   const int m = sys[0].M->degree;
   int depth = 0;  // deepest level index over the batch
   for (const auto& y : sys) depth = std::max(depth, (int)y.M->L.size() - 1);
-  if (ns > (int)W.cap_sys) {
+  if ((size_t)(ns * K) > W.cap_sys) {
     if (W.st) CPM_HIP(hipFree(W.st));
     if (W.hst) CPM_HIP(hipHostFree(W.hst));
-    W.cap_sys = std::max((size_t)ns, 2 * W.cap_sys);
+    W.cap_sys = std::max((size_t)(ns * K), 2 * W.cap_sys);
     CPM_HIP(hipMalloc(&W.st, sizeof(CgState) * W.cap_sys));
     CPM_HIP(hipHostMalloc(&W.hst, sizeof(CgState) * W.cap_sys, hipHostMallocDefault));
   }
@@ -307,9 +322,9 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
     OuterSeg o{};
     o.sptr = S.sptr, o.col = S.col, o.v64 = S.v64, o.dg64 = S.dg64;
     o.ga = GridArg{S.dims[0], S.dims[1], S.dims[2], S.nst, S.soff};
-    o.nb = S.nbrows, o.stencil = S.stencil, o.n = M->L[0].n;
+    o.nb = S.nbrows, o.stencil = S.stencil, o.n = M->L[0].n * K;
     o.b = y.b, o.x = y.x, o.r = M->r, o.p = M->p, o.q = M->q, o.rold = M->rold, o.z = M->L[0].x;
-    o.st = W.st + j;
+    o.st = W.st + j * K;
     o.blk0 = (int32_t)O.map.size();
     o.nblk = grid_for(o.n);
     o.sblk0 = (int32_t)W.smap.size();
@@ -346,14 +361,15 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
         g.fd = Dims3{{Lv.dims[0], Lv.dims[1], Lv.dims[2]}};
         g.cd = Dims3{{Lc.dims[0], Lc.dims[1], Lc.dims[2]}};
         g.bc = Lc.b, g.xc = Lc.x;
-        g.st = W.st + j;
+        g.anchor = M->anchor;
+        g.st = W.st + j * K;
         g.blk0 = (int32_t)Sm.map.size();
         g.nblk = sell_grid(S);
         for (int b = 0; b < g.nblk; ++b) Sm.map.push_back((int32_t)Sm.h.size());
         Sm.h.push_back(g);
       } else if (d == last) {
         DenseSeg e{};
-        e.n = M->L[d].n, e.cinv = M->cinv, e.b = M->L[d].b, e.x = M->L[d].x, e.st = W.st + j;
+        e.n = M->L[d].n, e.cinv = M->cinv, e.b = M->L[d].b, e.x = M->L[d].x, e.st = W.st + j * K;
         e.blk0 = (int32_t)Dn.map.size();
         e.nblk = (int32_t)std::max<int64_t>(1, std::min<int64_t>(kMaxGrid, (e.n + 3) / 4));
         for (int b = 0; b < e.nblk; ++b) Dn.map.push_back((int32_t)Dn.h.size());
@@ -371,7 +387,7 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
   }
   CPM_HIP(hipMemcpyAsync(W.dsmap, W.smap.data(), sizeof(int32_t) * W.smap.size(), hipMemcpyHostToDevice, s));
   const int G = O.G, Gs = (int)W.smap.size();
-  const size_t need = 2 * (size_t)std::max(G, Gs);
+  const size_t need = 2 * K * (size_t)std::max(G, Gs);
   if (need > W.cap_part) {
     if (W.partials) CPM_HIP(hipFree(W.partials));
     W.cap_part = std::max(need, W.cap_part + W.cap_part / 2);
@@ -387,115 +403,127 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
       auto& Sm = W.smooth[d];
       if (Sm.G) {
         if (d == 0)
-          kbm_first<BS, T, true, true><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap);
+          kbm_first<BS, K, T, true, true><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap);
         else
-          kbm_first<BS, T, true, false><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap);
-        for (int k = 1; k < m; ++k) kbm_sweep<BS, T, 0><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap, k);
-        kbm_sweep<BS, T, 1><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap, m - 1);
-        kbm_restrict<BS, T><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap);
+          kbm_first<BS, K, T, true, false><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap);
+        for (int k = 1; k < m; ++k) kbm_sweep<BS, K, T, 0><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap, k);
+        kbm_sweep<BS, K, T, 1><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap, m - 1);
+        kbm_restrict<BS * K, K, T><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap);
       }
       auto& Dn = W.dense[d + 1];
-      if (Dn.G) kbm_dense<T><<<Dn.G, kBlock, 0, s>>>(Dn.d, Dn.dmap);
+      if (Dn.G) kbm_dense<K, T><<<Dn.G, kBlock, 0, s>>>(Dn.d, Dn.dmap);
     }
     for (int d = depth - 1; d >= 0; --d) {
       auto& Sm = W.smooth[d];
       if (!Sm.G) continue;
-      kbm_prolong<BS, T><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap);
-      kbm_sweep<BS, T, 2><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap, 0);
-      kbm_first<BS, T, false, false><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap);
-      for (int k = 1; k < m; ++k) kbm_sweep<BS, T, 0><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap, k);
+      kbm_prolong<BS * K, K, T><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap);
+      kbm_sweep<BS, K, T, 2><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap, 0);
+      kbm_first<BS, K, T, false, false><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap);
+      for (int k = 1; k < m; ++k) kbm_sweep<BS, K, T, 0><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap, k);
     }
   };
 
   // start
-  if (any_x0) kbm_spmv<BS, true><<<Gs, kBlock, 0, s>>>(O.d, W.dsmap, pt);
-  kbm_init<<<G, kBlock, 0, s>>>(O.d, O.dmap, pt);
-  kbm_red<MRED_INIT><<<ns, kRedBlock, 0, s>>>(O.d, G, pt);
-  if (any_x0) kbm_zero_b<<<G, kBlock, 0, s>>>(O.d, O.dmap);
+  if (any_x0) kbm_spmv<BS, K, true><<<Gs, kBlock, 0, s>>>(O.d, W.dsmap, pt);
+  kbm_init<K><<<G, kBlock, 0, s>>>(O.d, O.dmap, pt);
+  kbm_red<K, MRED_INIT><<<ns, kRedBlock, 0, s>>>(O.d, G, pt);
+  if (any_x0) kbm_zero_b<K><<<G, kBlock, 0, s>>>(O.d, O.dmap);
   vcycle();
-  kbm_dot<T><<<G, kBlock, 0, s>>>(O.d, O.dmap, pt);
-  kbm_red<MRED_RZ><<<ns, kRedBlock, 0, s>>>(O.d, G, pt);
+  kbm_dot<K, T><<<G, kBlock, 0, s>>>(O.d, O.dmap, pt);
+  kbm_red<K, MRED_RZ><<<ns, kRedBlock, 0, s>>>(O.d, G, pt);
   CPM_HIP(hipGetLastError());
   auto issue = [&](int todo) -> int {
     for (int k = 0; k < todo; ++k) {
-      kbm_pupd<T><<<G, kBlock, 0, s>>>(O.d, O.dmap);
-      kbm_spmv<BS, false><<<Gs, kBlock, 0, s>>>(O.d, W.dsmap, pt);
-      kbm_red<MRED_ALPHA><<<ns, kRedBlock, 0, s>>>(O.d, Gs, pt);
-      kbm_xr<<<G, kBlock, 0, s>>>(O.d, O.dmap, pt);
-      kbm_red<MRED_RR><<<ns, kRedBlock, 0, s>>>(O.d, G, pt);
+      kbm_pupd<K, T><<<G, kBlock, 0, s>>>(O.d, O.dmap);
+      kbm_spmv<BS, K, false><<<Gs, kBlock, 0, s>>>(O.d, W.dsmap, pt);
+      kbm_red<K, MRED_ALPHA><<<ns, kRedBlock, 0, s>>>(O.d, Gs, pt);
+      kbm_xr<K><<<G, kBlock, 0, s>>>(O.d, O.dmap, pt);
+      kbm_red<K, MRED_RR><<<ns, kRedBlock, 0, s>>>(O.d, G, pt);
       vcycle();
-      kbm_dot<T><<<G, kBlock, 0, s>>>(O.d, O.dmap, pt);
-      kbm_red<MRED_RZ><<<ns, kRedBlock, 0, s>>>(O.d, G, pt);
+      kbm_dot<K, T><<<G, kBlock, 0, s>>>(O.d, O.dmap, pt);
+      kbm_red<K, MRED_RZ><<<ns, kRedBlock, 0, s>>>(O.d, G, pt);
     }
     CPM_HIP(hipGetLastError());
     return CPM_OK;
   };
+  const int nst = ns * K;
   auto read = [&]() -> int {
-    CPM_HIP(hipMemcpyAsync(W.hst, W.st, sizeof(CgState) * ns, hipMemcpyDeviceToHost, s));
+    CPM_HIP(hipMemcpyAsync(W.hst, W.st, sizeof(CgState) * nst, hipMemcpyDeviceToHost, s));
     CPM_HIP(hipStreamSynchronize(s));
     return CPM_OK;
   };
-  auto all_done = [&]() {
-    for (int j = 0; j < ns; ++j)
+  auto all_live_done = [&]() {
+    for (int j = 0; j < nst; ++j)
       if (!W.hst[j].done) return false;
     return true;
   };
   if (int rc = read()) return rc;
   int batch = hint > 2 ? hint - 1 : 4;
   int issued = 0;
-  while (!all_done() && issued <= maxit) {
+  while (!all_live_done() && issued <= maxit) {
     const int todo = std::max(1, std::min(std::min(batch, 1024), maxit - issued + 1));
     if (int rc = issue(todo)) return rc;
     issued += todo;
     if (int rc = read()) return rc;
     int it_max = 0;
-    for (int j = 0; j < ns; ++j)
+    for (int j = 0; j < nst; ++j)
       if (!W.hst[j].done) it_max = std::max(it_max, (int)W.hst[j].it);
     batch = std::max(2, std::min(16, it_max / 4 + 2));
   }
   for (int j = 0; j < ns; ++j) {
-    iters[j] = W.hst[j].it;
-    resid[j] = W.hst[j].resid;
-    failed[j] = (W.hst[j].fail || !W.hst[j].done) ? 1 : 0;
+    failed[j] = 0;
+    for (int c = 0; c < K; ++c) {
+      const CgState& h = W.hst[j * K + c];
+      iters[j * K + c] = h.it;
+      resid[j * K + c] = h.resid;
+      if (h.fail || !h.done) failed[j] = 1;
+    }
   }
   return CPM_OK;
 }
 
-int multi_mg_pcg(cpm_ctx* ctx, MultiMgSolver& W, const std::vector<MultiMgSys>& sys, int hint, int32_t* iters,
-                 double* resid, int* failed) {
+int multi_mg_pcg(cpm_ctx* ctx, MultiMgSolver& W, const std::vector<MultiMgSys>& sys, int K, int hint,
+                 int32_t* iters, double* resid, int* failed) {
   if (sys.empty()) return CPM_OK;
   const cpm_mg* M0 = sys[0].M;
   for (const auto& y : sys) {
     if (!y.M || !y.b || !y.x) return fail(CPM_EINVAL, "batched multigrid CG: null system");
+    if (y.M->ctx != ctx) return fail(CPM_EINVAL, "batched multigrid CG: a hierarchy uses another context");
     if (y.M->bs != M0->bs || y.M->prec != M0->prec || y.M->degree != M0->degree)
       return fail(CPM_EINVAL, "batched multigrid CG: hierarchies differ in block size, precision or degree");
     if (y.M->L.size() > 16) return fail(CPM_EINVAL, "batched multigrid CG: too many levels");
   }
+  if (K != 1 && !(K == 3 && M0->bs == 1))
+    return fail(CPM_EINVAL, "batched multigrid CG: 1 right-hand side, or 3 for scalar operators");
   for (size_t i = 0; i < sys.size(); ++i)
     for (size_t j = i + 1; j < sys.size(); ++j)
       if (sys[i].M == sys[j].M)
         return fail(CPM_EINVAL, "batched multigrid CG: two systems share one hierarchy (and its workspace)");
+  for (const auto& y : sys)
+    if (int rc = mg_reserve(y.M, K)) return rc;
   if (M0->bs == 3)
-    return M0->prec == 32 ? run<3, float>(ctx, *W.T, sys, hint, iters, resid, failed)
-                          : run<3, double>(ctx, *W.T, sys, hint, iters, resid, failed);
-  return M0->prec == 32 ? run<1, float>(ctx, *W.T, sys, hint, iters, resid, failed)
-                        : run<1, double>(ctx, *W.T, sys, hint, iters, resid, failed);
+    return M0->prec == 32 ? run<3, 1, float>(ctx, *W.T, sys, hint, iters, resid, failed)
+                          : run<3, 1, double>(ctx, *W.T, sys, hint, iters, resid, failed);
+  if (K == 3)
+    return M0->prec == 32 ? run<1, 3, float>(ctx, *W.T, sys, hint, iters, resid, failed)
+                          : run<1, 3, double>(ctx, *W.T, sys, hint, iters, resid, failed);
+  return M0->prec == 32 ? run<1, 1, float>(ctx, *W.T, sys, hint, iters, resid, failed)
+                        : run<1, 1, double>(ctx, *W.T, sys, hint, iters, resid, failed);
 }
 
 }  // namespace cpm
 
 using namespace cpm;
 
-extern "C" int cpm_cg_mg_batch(cpm_ctx* ctx, int count, cpm_mg* const* mgs, const double* const* b, double* const* x,
-                               const int32_t* use_x0, double tol, int maxiter, int32_t* iters, double* resid,
-                               int32_t* failed) {
+extern "C" int cpm_cg_mg_batch(cpm_ctx* ctx, int count, int nrhs, cpm_mg* const* mgs, const double* const* b,
+                               double* const* x, const int32_t* use_x0, double tol, int maxiter, int32_t* iters,
+                               double* resid, int32_t* failed) {
   if (!ctx || count < 0 || (count > 0 && (!mgs || !b || !x || !iters || !resid || !failed)))
     return fail(CPM_EINVAL, "cpm_cg_mg_batch: null argument");
   if (count == 0) return CPM_OK;
   CPM_HIP(hipSetDevice(ctx->device));
   std::vector<MultiMgSys> sys(count);
   for (int j = 0; j < count; ++j) {
-    if (mgs[j] && mgs[j]->ctx != ctx) return fail(CPM_EINVAL, "cpm_cg_mg_batch: a hierarchy uses another context");
     sys[j].M = mgs[j];
     sys[j].b = b[j];
     sys[j].x = x[j];
@@ -505,7 +533,7 @@ extern "C" int cpm_cg_mg_batch(cpm_ctx* ctx, int count, cpm_mg* const* mgs, cons
   }
   MultiMgSolver W;
   std::vector<int> f(count);
-  if (int rc = multi_mg_pcg(ctx, W, sys, 0, iters, resid, f.data())) return rc;
+  if (int rc = multi_mg_pcg(ctx, W, sys, nrhs, 0, iters, resid, f.data())) return rc;
   for (int j = 0; j < count; ++j) failed[j] = f[j];
   return CPM_OK;
 }

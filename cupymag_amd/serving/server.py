@@ -35,9 +35,10 @@ from .workloads import initial_m
 # the matrix-free F_el, the step's state vectors, the batched-CG workspace
 # of the mixed mode)
 _BYTES_PER_DOF = {False: 2200.0, True: 5700.0}
-# FP32 multigrid hierarchies per fine DOF: coarse operators (1/8 + 1/64 + ...
-# of the fine SELL forms), level vectors, the FP64 outer CG vectors
-_MG_BYTES_PER_DOF = {False: 250.0, True: 900.0}
+# FP32 multigrid hierarchies per fine DOF (demag, Gauss-Seidel A1 with three
+# columns, elasticity): coarse operators (1/8 + 1/64 + ... of the fine SELL
+# forms), level vectors, the FP64 outer CG vectors
+_MG_BYTES_PER_DOF = {False: 500.0, True: 1200.0}
 
 
 def estimate_bytes(n_dof, me, multigrid_n=None):
@@ -48,7 +49,7 @@ def estimate_bytes(n_dof, me, multigrid_n=None):
     if multigrid_n:
         from ..mesh.coarsen import hierarchy_sizes
         nc = hierarchy_sizes(int(multigrid_n))[-1] ** 3
-        b += _MG_BYTES_PER_DOF[bool(me)] * n_dof + 4.0 * (nc ** 2 + (9 * nc ** 2 if me else 0))
+        b += _MG_BYTES_PER_DOF[bool(me)] * n_dof + 4.0 * (2 * nc ** 2 + (9 * nc ** 2 if me else 0))
     return int(b) + (64 << 20)
 
 

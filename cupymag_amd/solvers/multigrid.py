@@ -54,17 +54,21 @@ class KuhnMultigrid:
         check(lib().cpm_mg_apply(self.h, ptr(r), ptr(z)), "cpm_mg_apply")
         return z
 
-    def solve(self, b, x0=None, tol=1e-7, maxiter=5000, use_init=False):
+    def solve(self, b, x0=None, tol=1e-7, maxiter=5000, use_init=False, nrhs=1):
         """(x, iterations, residual) of the preconditioned CG; raises the
-        reference's error when maxiter is reached."""
+        reference's error when maxiter is reached.  nrhs = 3 (scalar
+        operators): b holds three interleaved columns (entry i*3 + j) and
+        iterations / residual are lists of three."""
         b = as_device(b, self.ctx)
         x = x0.clone() if (use_init and x0 is not None) else torch.empty_like(b)
-        it, res = C.c_int32(), C.c_double()
-        info = check(lib().cpm_cg_mg(self.h, ptr(b), ptr(x), int(bool(use_init and x0 is not None)), float(tol),
-                                     int(maxiter), C.byref(it), C.byref(res)), "cpm_cg_mg")
+        it, res = (C.c_int32 * nrhs)(), (C.c_double * nrhs)()
+        info = check(lib().cpm_cg_mg(self.h, int(nrhs), ptr(b), ptr(x), int(bool(use_init and x0 is not None)),
+                                     float(tol), int(maxiter), it, res), "cpm_cg_mg")
         if info > 0:
-            raise RuntimeError(f"Error! CG did not converge. info={info}, residual={res.value}.")
-        return x, it.value, res.value
+            raise RuntimeError(f"Error! CG did not converge. info={info}, residual={max(res)}.")
+        if nrhs == 1:
+            return x, it[0], res[0]
+        return x, list(it), list(res)
 
     def __del__(self):
         if getattr(self, "h", None) and _lib.alive():
@@ -72,13 +76,14 @@ class KuhnMultigrid:
             self.h = None
 
 
-def solve_batch(mgs, rhs, x0s=None, tol=1e-7, maxiter=5000):
+def solve_batch(mgs, rhs, x0s=None, tol=1e-7, maxiter=5000, nrhs=1):
     """``KuhnMultigrid.solve`` on many independent systems at once
     (cpm_cg_mg_batch): one launch per phase for the whole batch.  mgs:
     hierarchies (one block size, precision and degree; distinct), rhs: their
-    right-hand sides, x0s: optional initial guesses (used where not None).
-    Each system's (x, iterations, residual) equals ``solve``'s bit for bit.
-    Raises the reference's error when a system reaches maxiter."""
+    right-hand sides (nrhs interleaved columns each), x0s: optional initial
+    guesses (used where not None).  Each system's (x, iterations, residual)
+    equals ``solve``'s bit for bit.  Raises the reference's error when a
+    system reaches maxiter."""
     if not mgs:
         return []
     ctx = mgs[0].ctx
@@ -90,12 +95,16 @@ def solve_batch(mgs, rhs, x0s=None, tol=1e-7, maxiter=5000):
     bp = (C.c_void_p * n)(*[b.data_ptr() for b in rhs])
     xp = (C.c_void_p * n)(*[x.data_ptr() for x in xs])
     ux = (C.c_int32 * n)(*[int(x0 is not None) for x0 in x0s])
-    it, res, fl = (C.c_int32 * n)(), (C.c_double * n)(), (C.c_int32 * n)()
-    check(lib().cpm_cg_mg_batch(ctx.h, n, h, bp, xp, ux, float(tol), int(maxiter), it, res, fl), "cpm_cg_mg_batch")
-    for j in range(n):
+    it, res, fl = (C.c_int32 * (n * nrhs))(), (C.c_double * (n * nrhs))(), (C.c_int32 * n)()
+    check(lib().cpm_cg_mg_batch(ctx.h, n, int(nrhs), h, bp, xp, ux, float(tol), int(maxiter), it, res, fl),
+          "cpm_cg_mg_batch")
+    out = []
+    for j, x in enumerate(xs):
+        its, rs = list(it[j * nrhs:(j + 1) * nrhs]), list(res[j * nrhs:(j + 1) * nrhs])
         if fl[j]:
-            raise RuntimeError(f"Error! CG did not converge. info={it[j]}, residual={res[j]}.")
-    return [(x, it[j], res[j]) for j, x in enumerate(xs)]
+            raise RuntimeError(f"Error! CG did not converge. info={max(its)}, residual={max(rs)}.")
+        out.append((x, its[0], rs[0]) if nrhs == 1 else (x, its, rs))
+    return out
 
 
 def _interleave_perm(n, bs):
@@ -105,8 +114,10 @@ def _interleave_perm(n, bs):
 
 
 def build_hierarchies(SimOp, precision=32, degree=2, ratio=20.0, coarsest_max=8, nthreads=None):
-    """{"A_demag": KuhnMultigrid, "A_el": KuhnMultigrid (with coupling)} for
-    a SimulationOperators object on an n^3 Kuhn mesh with n > coarsest_max,
+    """{"A_demag": KuhnMultigrid, "A1": KuhnMultigrid (Gauss-Seidel operator
+    M + alpha1 K; coarse levels without the defect surgery), "A_el":
+    KuhnMultigrid (with coupling)} for a SimulationOperators object on an
+    n^3 Kuhn mesh with n > coarsest_max,
     or None when the mesh has no such hierarchy (not a Kuhn mesh in
     kuhn_mesh order, or too small to coarsen)."""
     from ..physics.host_ops import HostOps, voigt_C
@@ -118,7 +129,7 @@ def build_hierarchies(SimOp, precision=32, degree=2, ratio=20.0, coarsest_max=8,
         return None
     P = SimOp.P
     ctx = SimOp.csr_demag_and_deriv()[0].ctx
-    fine = {"A_demag": SimOp.csr_demag_and_deriv()[0]}
+    fine = {"A_demag": SimOp.csr_demag_and_deriv()[0], "A1": SimOp.csr_GS()[0]}
     if P.ME:
         fine["A_el"] = SimOp.csr_elasticity()[0]
     Cv = voigt_C(P.c11, P.c12, P.c44, bool(P.rot111)) if P.ME else None
@@ -136,7 +147,7 @@ def build_hierarchies(SimOp, precision=32, degree=2, ratio=20.0, coarsest_max=8,
             return None
         H = HostOps(coords, el, gid, ndof, None, SimOp.alpha1, SimOp.alpha2, Cv, nthreads)
         last = li == len(sizes) - 2
-        for name, bs in (("A_demag", 1), ("A_el", 3)):
+        for name, bs in (("A_demag", 1), ("A_el", 3), ("A1", 1)):
             if name not in fine:
                 continue
             ip, ix, v, shape = H.csr(name)

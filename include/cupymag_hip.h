@@ -227,19 +227,24 @@ int cpm_mg_apply(cpm_mg* mg, const double* r, double* z);
 /* CG on levels[0] preconditioned by one V-cycle per iteration, FP64, on
  * the context's stream: CuPy's stopping rule on the recurrence residual
  * (||r|| <= tol*||b|| after the update, at least one iteration, b = 0 ->
- * x = 0), flexible beta = (r - r_old).z / rho_old.  Returns 0, or a
- * positive iteration count when maxiter was reached. */
-int cpm_cg_mg(cpm_mg* mg, const double* b, double* x, int use_x0, double tol, int maxiter, int32_t* iters,
-              double* resid);
+ * x = 0), flexible beta = (r - r_old).z / rho_old.  nrhs = 1, or 3 for a
+ * scalar operator: b and x hold nrhs interleaved columns (entry i*nrhs + j),
+ * each with its own scalars and stopping test; iters and resid hold nrhs
+ * entries.  Returns 0, or a positive iteration count when maxiter was
+ * reached. */
+int cpm_cg_mg(cpm_mg* mg, int nrhs, const double* b, double* x, int use_x0, double tol, int maxiter,
+              int32_t* iters, double* resid);
 /* No counterpart in the reference.  cpm_cg_mg on `count` independent
  * systems at once (hierarchies of one block size, V-cycle precision and
- * smoother degree, each at most once, all on ctx): one launch per phase for
- * the whole batch, V-cycles level by level whatever the depths.  Each
- * system's result, iteration count and residual equal cpm_cg_mg's on the
- * same input bit for bit.  failed[j] = 1 when system j reached maxiter;
- * returns 0 or an error < 0. */
-int cpm_cg_mg_batch(cpm_ctx* ctx, int count, cpm_mg* const* mgs, const double* const* b, double* const* x,
-                    const int32_t* use_x0, double tol, int maxiter, int32_t* iters, double* resid, int32_t* failed);
+ * smoother degree, each at most once, all on ctx) with nrhs columns each:
+ * one launch per phase for the whole batch, V-cycles level by level
+ * whatever the depths.  Each system's result, iteration counts and
+ * residuals (iters, resid: nrhs entries per system) equal cpm_cg_mg's on
+ * the same input bit for bit.  failed[j] = 1 when a column of system j
+ * reached maxiter; returns 0 or an error < 0. */
+int cpm_cg_mg_batch(cpm_ctx* ctx, int count, int nrhs, cpm_mg* const* mgs, const double* const* b,
+                    double* const* x, const int32_t* use_x0, double tol, int maxiter, int32_t* iters,
+                    double* resid, int32_t* failed);
 
 /* ------------------------------------------------------------------ */
 /* matrix-free magnetoelastic right-hand side                           */
@@ -274,7 +279,11 @@ typedef struct cpm_sim_ops {
   cpm_mg* mg_el;          /* multigrid hierarchy of A_el (used when params.precond) or NULL */
   cpm_mg* mg_demag;       /* multigrid hierarchy of A_demag (used when params.precond) or NULL */
   cpm_fel* fel;           /* matrix-free F_el (cpm_fel_create); used instead of F_el when not NULL */
+  cpm_mg* mg_a1;          /* multigrid hierarchy of A1 (used when params.precond has CPM_PRECOND_GS) or NULL */
 } cpm_sim_ops;
+
+#define CPM_PRECOND_EL_DEMAG 1 /* elasticity and demag solves: multigrid-preconditioned CG */
+#define CPM_PRECOND_GS 2       /* Gauss-Seidel solves on A1 (g1n..g3n, g1*, g2*): multigrid-preconditioned CG */
 
 typedef struct cpm_sim_params {
   int32_t n;                  /* number of DOFs */
@@ -297,7 +306,7 @@ typedef struct cpm_sim_params {
                                  (the reference does so only when use_init is set) */
   double rel_delta;           /* CPM_MODE_MIXED: FP64 residual update when the FP32 recurrence
                                  residual drops by this factor (reliable updates) */
-  int32_t precond;            /* fast modes: 1 = the elasticity and demag solves run multigrid-
+  int32_t precond;            /* fast modes: CPM_PRECOND_* bits; the solves named run multigrid-
                                  preconditioned CG (cpm_cg_mg) on the hierarchies in cpm_sim_ops */
   int32_t reserved[3];
   double rel_delta_gs;        /* CPM_MODE_MIXED: rel_delta of the Gauss-Seidel solves (A1, A2);

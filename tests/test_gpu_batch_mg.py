@@ -3,10 +3,11 @@ elasticity/demag solves of cpm_batch_step for members created with
 precond="mg").
 
 * Hierarchies of different depths (10^3 -> 5^3, 20^3 -> 10^3 -> 5^3,
-  13^3 -> 7^3 non-nested), scalar and 3x3-block operators, FP32 and FP64
-  V-cycles: every system's solution, iteration count and residual equal
-  the one-system solve (cpm_cg_mg) bit for bit, with and without initial
-  guesses, for a zero right-hand side too; maxiter reached raises.
+  13^3 -> 7^3 non-nested), scalar (demag, Gauss-Seidel A1) and 3x3-block
+  operators, FP32 and FP64 V-cycles, one column or three (A1): every
+  system's solution, iteration counts and residuals equal the one-system
+  solve (cpm_cg_mg) bit for bit, with and without initial guesses, for a
+  zero right-hand side too; maxiter reached raises.
 * GSPM batches mixing multigrid members of different depths, a member
   without the preconditioner and two members on one set of operators (one
   hierarchy, solved in turn): each member's m, u, iteration counts and
@@ -37,7 +38,7 @@ def meshes():
     return [_ops(n) for n in (10, 20, 13)]
 
 
-@pytest.mark.parametrize("name", ["A_demag", "A_el"])
+@pytest.mark.parametrize("name", ["A_demag", "A_el", "A1"])
 @pytest.mark.parametrize("prec", [32, 64])
 def test_solve_batch_equals_single_solves(meshes, name, prec):
     from cupymag_amd.solvers.multigrid import solve_batch
@@ -70,6 +71,29 @@ def test_solve_batch_equals_single_solves(meshes, name, prec):
         solve_batch(mgs, rhs, tol=1e-14, maxiter=2)
 
 
+@pytest.mark.parametrize("prec", [32, 64])
+def test_solve_batch_three_columns_equals_single_solves(meshes, prec):
+    """The Gauss-Seidel operator A1 with three interleaved columns (one of
+    them zero in one system, an initial guess in another)."""
+    from cupymag_amd.solvers.multigrid import solve_batch
+    mgs = [S.multigrid(precision=prec)["A1"] for _, _, _, S in meshes]
+    rng = np.random.default_rng(12)
+    rhs, x0s = [], []
+    for k, m in enumerate(mgs):
+        B = rng.standard_normal((m.levels[0].shape[0], 3))
+        if k == 0:
+            B[:, 2] = 0.0
+        rhs.append(torch.as_tensor(B.ravel(), device="cuda"))
+        x0s.append(torch.as_tensor(0.1 * rng.standard_normal(B.size), device="cuda") if k == 2 else None)
+    got = solve_batch(mgs, rhs, x0s, tol=1e-10, maxiter=500, nrhs=3)
+    for m, b, x0, (x, its, res) in zip(mgs, rhs, x0s, got):
+        xs, its1, res1 = m.solve(b, x0=x0, tol=1e-10, maxiter=500, use_init=x0 is not None, nrhs=3)
+        assert its == its1 and res == res1
+        np.testing.assert_array_equal(x.cpu().numpy(), xs.cpu().numpy())
+    assert got[0][1][2] == 0 and float(got[0][0].reshape(-1, 3)[:, 2].abs().max()) == 0.0
+    assert all(0 < it < 40 for it in got[1][1])
+
+
 def test_solve_batch_rejects_shared_or_mismatched_hierarchies(meshes):
     from cupymag_amd.solvers.multigrid import solve_batch
     S0, S1 = meshes[0][3], meshes[1][3]
@@ -81,6 +105,10 @@ def test_solve_batch_rejects_shared_or_mismatched_hierarchies(meshes):
     b1 = torch.zeros(m1.levels[0].shape[0], dtype=torch.float64, device="cuda")
     with pytest.raises(ValueError, match="differ"):
         solve_batch([m0, m1], [b, b1])
+    me = meshes[0][3].multigrid(precision=32)["A_el"]
+    be = torch.zeros(me.levels[0].shape[0] * 3, dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError, match="3 for scalar"):
+        solve_batch([me], [be], nrhs=3)
 
 
 def _H(P):
