@@ -69,7 +69,8 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_gpu(n, steps, warmup, steps64, modes, rel_delta=0.3, autotune=True, rel_delta_gs=None):
+def run_gpu(n, steps, warmup, steps64, modes, rel_delta=0.3, autotune=True, rel_delta_gs=None, mg_degree=2,
+            mg_ratio=10.0):
     """Run the plain FP64 path and the fast paths on the same operators and
     initial state; returns per-mode timings, CG iteration counts, the SpMV
     kernel timing of the elasticity operator and the magnetization after the
@@ -107,14 +108,16 @@ def run_gpu(n, steps, warmup, steps64, modes, rel_delta=0.3, autotune=True, rel_
     snap_at = warmup + steps64
     for mode in modes:
         base, *opts = mode.split("+")
-        mg = "mg" in opts or "mged" in opts
+        mg = "mg" in opts or "mged" in opts or "mg32" in opts
+        mgp = (32 if "mg32" in opts else 16) if base == "mixed" else 64
         if mg:
             t0 = time.time()
-            S.multigrid(precision=32 if base == "mixed" else 64)
+            S.multigrid(precision=mgp, degree=mg_degree, ratio=mg_ratio)
             log(f"[bench] multigrid hierarchies for {mode}: {time.time() - t0:.1f}s")
         kw = {} if rel_delta_gs is None else dict(rel_delta_gs=rel_delta_gs)
         sim = GSPMSimulation(S, P, mode=base, warm_start=("warm" in opts), rel_delta=rel_delta,
-                             precond="mg" if mg else None, mg_gs="mged" not in opts, **kw)
+                             precond="mg" if mg else None, mg_gs="mged" not in opts, mg_degree=mg_degree,
+                             mg_ratio=mg_ratio, mg_precision=mgp if mg else None, **kw)
         ctx = sim.ctx
         sim.set_m(m0)
         k_timed = steps64 if base == "fp64" else steps
@@ -163,9 +166,11 @@ def run_gpu(n, steps, warmup, steps64, modes, rel_delta=0.3, autotune=True, rel_
             # d gathered, r, x read and written, D^-1 (9 values) read, d written
             s_ = 4 if base == "mixed" else 8
             t_ = "float" if s_ == 4 else "double"
-            kernel = f"k_mg_sweep<3,{t_},0> (A_el level 0, Chebyshev step, SELL-64 3x3 blocks, stencil)"
-            rocname = f"k_mg_sweep<3, {t_}, 0, true>"
-            bytes_launch = (nnz / 9.0 - nb) * 9 * s_ + nb * 72.0 + nb * 27.0 * s_
+            v_, vs_ = ("_Float16", 2) if mgp == 16 else (t_, s_)
+            kernel = (f"k_mg_sweep<3,1,{t_},0,{v_}> (A_el level 0, Chebyshev step, SELL-64 3x3 blocks, stencil, "
+                      f"{v_} off-diagonal values)")
+            rocname = f"k_mg_sweep<3, 1, {t_}, 0, true, {v_}>"
+            bytes_launch = (nnz / 9.0 - nb) * 9 * vs_ + nb * 72.0 + nb * 27.0 * s_
         else:
             # SELL-64 3x3 blocks: off-diagonal blocks 9 values (+ 1 column index
             # unless the index-free stencil form is in use), diagonal blocks 9
@@ -237,7 +242,8 @@ def cpu_baseline(n_cpu, steps=1):
 def _build_instances(descs, ctx, workers=4, multigrid=False):
     """SimulationOperators of every instance description (mesh, host
     assembly, upload), built on a few host threads; multigrid: also the
-    FP32 multigrid hierarchies where the mesh has them."""
+    multigrid hierarchies of the mixed mode (FP16 level operator values)
+    where the mesh has them."""
     import concurrent.futures as cf
 
     import torch
@@ -252,7 +258,7 @@ def _build_instances(descs, ctx, workers=4, multigrid=False):
         S = SimulationOperators(P, mesh_arrays=kuhn_mesh(d["n"]), ctx=ctx, keep_host=False, matrix_free_fel=True)
         S.csr_demag_and_deriv()
         if multigrid:
-            S.multigrid(precision=32)
+            S.multigrid(precision=16)
         return P, S
 
     with cf.ThreadPoolExecutor(max_workers=workers) as ex:
@@ -278,7 +284,7 @@ def run_batch(args):
     built = _build_instances(descs, ctx, multigrid=use_mg)
     ndof = [S.n_dof for _, S in built]
     # members whose mesh has a multigrid hierarchy use it (n even, coarsest <= 10^3)
-    pc = ["mg" if use_mg and S.multigrid(precision=32) is not None else None for _, S in built]
+    pc = ["mg" if use_mg and S.multigrid(precision=16) is not None else None for _, S in built]
     log(f"[bench] {len(built)} instances, n = {min(d['n'] for d in descs)}..{max(d['n'] for d in descs)}, "
         f"{sum(ndof)} DOFs in total, {sum(p is not None for p in pc)} with multigrid, "
         f"set up in {time.time() - t0:.1f}s")
@@ -471,8 +477,11 @@ def main():
                     help="comma list of fp64, fast64, mixed; '+warm' appended to a fast mode starts every CG "
                          "from the previous step's solution, '+mg' preconditions the elasticity, demag and "
                          "Gauss-Seidel (A1) solves with the geometric multigrid, '+mged' only the elasticity and "
-                         "demag solves")
+                         "demag solves; the mixed mode's V-cycle stores the level operators' values in FP16, "
+                         "'+mg32' in FP32")
     ap.add_argument("--rel-delta", type=float, default=0.3)
+    ap.add_argument("--mg-degree", type=int, default=2, help="multigrid: Chebyshev smoother degree")
+    ap.add_argument("--mg-ratio", type=float, default=10.0, help="multigrid: Chebyshev eigenvalue ratio")
     ap.add_argument("--rel-delta-gs", type=float, default=None,
                     help="reliable-update factor of the Gauss-Seidel solves (default 0.01)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default SpMV launch schedules")
@@ -516,7 +525,7 @@ def main():
     modes = [m for m in args.modes.split(",") if m]
     steps64 = min(args.steps_fp64, args.steps) if "fp64" in modes else args.steps_fp64
     g = run_gpu(args.n, args.steps, args.warmup, steps64, modes, args.rel_delta, autotune=not args.no_autotune,
-                rel_delta_gs=args.rel_delta_gs)
+                rel_delta_gs=args.rel_delta_gs, mg_degree=args.mg_degree, mg_ratio=args.mg_ratio)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n, args.cpu_steps)
     per_mode = {}
     ref_m = g["modes"]["fp64"]["m"] if "fp64" in modes else None

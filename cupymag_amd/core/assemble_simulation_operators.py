@@ -92,7 +92,7 @@ class SimulationOperators(FEMMesh):
             return None
         return MatrixFreeFel(self.node_coords_np, n, Cv, ctx)
 
-    def multigrid(self, precision=32, degree=2, ratio=20.0):
+    def multigrid(self, precision=32, degree=2, ratio=10.0):
         """Multigrid hierarchies of A_demag and A_el (cached per precision),
         or None when the mesh has none (cupymag_amd.solvers.multigrid)."""
         cache = self.__dict__.setdefault("_mg", {})

@@ -32,7 +32,7 @@ class StepReport:
 
 class GSPMSimulation:
     def __init__(self, SimOp, P, mode="fp64", rel_delta=0.3, warm_start=False, precond=None, rel_delta_gs=0.01,
-                 mg_gs=True):
+                 mg_gs=True, mg_degree=2, mg_ratio=10.0, mg_precision=None):
         """mode: "fp64" (CSR operators, the reference's sequence of solves),
         "fast64" (SELL-64 operators, 3-RHS solves) or "mixed" (as fast64 with
         FP32 CG refined in FP64 by reliable updates: an FP64 residual update
@@ -47,7 +47,10 @@ class GSPMSimulation:
         multigrid V-cycle (FP32 V-cycle in the mixed mode, FP64 in fast64;
         cupymag_amd.solvers.multigrid), converged to the same tolerance;
         with ``mg_gs`` also the Gauss-Seidel solves on A1 = M + alpha1 K
-        (g1n..g3n as one 3-column solve, g1*, g2*)."""
+        (g1n..g3n as one 3-column solve, g1*, g2*); ``mg_degree`` and
+        ``mg_ratio``: Chebyshev smoother degree and eigenvalue ratio;
+        ``mg_precision``: V-cycle arithmetic (default 16 in the mixed mode:
+        FP32 vectors with FP16 operator values; 64 in fast64; 32 possible)."""
         self.P = P
         self.SimOp = SimOp
         A_demag, Fx, Fy, Fz, self.Deriv = SimOp.csr_demag_and_deriv()
@@ -64,7 +67,8 @@ class GSPMSimulation:
         if precond == "mg":
             if mode == "fp64":
                 raise ValueError("precond is a fast-mode option")
-            self.mg = SimOp.multigrid(precision=32 if mode == "mixed" else 64)
+            prec = mg_precision or (16 if mode == "mixed" else 64)
+            self.mg = SimOp.multigrid(precision=prec, degree=mg_degree, ratio=mg_ratio)
             if self.mg is None:
                 raise ValueError("multigrid needs a structured Kuhn mesh of more than 8 cells per dimension")
         mg_el = self.mg["A_el"].h if (self.mg and "A_el" in self.mg) else None

@@ -1089,7 +1089,7 @@ extern "C" int cpm_batch_step(cpm_batch* B, const double* H, const int32_t* use_
     // hierarchy at most once (members sharing operators go to later rounds)
     while (!mgq.empty()) {
       const cpm_mg* M0 = mgq[0].second.M;
-      const int key = M0->prec * 16 + M0->degree;
+      const int key = (M0->prec + M0->half) * 16 + M0->degree;
       mgsys.clear();
       mgwho.clear();
       std::vector<std::pair<int, MultiMgSys>> rest;
@@ -1097,7 +1097,7 @@ extern "C" int cpm_batch_step(cpm_batch* B, const double* H, const int32_t* use_
         const cpm_mg* M = e.second.M;
         bool dup = false;
         for (const auto& y : mgsys) dup |= (y.M == M);
-        if (dup || M->prec * 16 + M->degree != key)
+        if (dup || (M->prec + M->half) * 16 + M->degree != key)
           rest.push_back(e);
         else
           mgsys.push_back(e.second), mgwho.push_back(e.first);

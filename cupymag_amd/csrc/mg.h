@@ -16,6 +16,12 @@ struct MgLevel {
   void *x = nullptr, *r = nullptr, *b = nullptr, *d0 = nullptr, *d1 = nullptr;
   void* dinv = nullptr;     // bs*bs per node: inverse diagonal block
   double lmax = 0.0;        // power-iteration estimate of lambda_max(D^-1 A)
+  // FP16 form (precision 16): off-diagonal values times vscale^-1 (a power
+  // of two), in the SELL layout, and FP64 diagonal blocks compensated for
+  // their rounding
+  void* v16 = nullptr;
+  double* dgh = nullptr;
+  double vscale = 1.0;
 };
 
 }  // namespace cpm
@@ -24,6 +30,7 @@ struct cpm_mg {
   cpm_ctx* ctx = nullptr;
   int bs = 1, prec = 32, degree = 2;
   int anchor = 1;           // row 0 of the fine operator is an identity row: node 0 left out of the transfers
+  int half = 0;             // precision 16: FP32 vectors, FP16 off-diagonal values
   double ratio = 20.0;
   std::vector<cpm::MgLevel> L;
   void* cinv = nullptr;     // coarsest dense inverse (T), ncoarse x ncoarse

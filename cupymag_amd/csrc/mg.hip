@@ -55,17 +55,49 @@ __global__ void __launch_bounds__(kBlock) k_mg_first(int32_t nb, const double* _
   mg_first_rows<BS, K, T, ZERO, FROM64>(blockIdx.x, gridDim.x, nb, b64, b, r, d, x, dinv, c0);
 }
 
-template <int BS, int K, class T, int MODE, bool ST>
+template <int BS, int K, class T, int MODE, bool ST, class VT>
 __global__ void __launch_bounds__(kBlock) k_mg_sweep(int32_t nb, const int64_t* __restrict__ sptr,
-                                                     const int32_t* __restrict__ col, const T* __restrict__ val,
+                                                     const int32_t* __restrict__ col, const VT* __restrict__ val,
                                                      const double* __restrict__ dg, GridArg ga,
                                                      const T* __restrict__ din, T* __restrict__ dout,
                                                      T* __restrict__ x, T* __restrict__ r, const T* __restrict__ b,
                                                      const T* __restrict__ dinv, double c1, double c2,
-                                                     const CgState* st) {
+                                                     double vscale, const CgState* st) {
   if (all_done<K>(st)) return;
-  mg_sweep_rows<BS, K, T, MODE, ST>(blockIdx.x, gridDim.x, nb, sptr, col, val, dg, ga, din, dout, x, r, b, dinv, c1,
-                                    c2);
+  mg_sweep_rows<BS, K, T, MODE, ST, VT>(blockIdx.x, gridDim.x, nb, sptr, col, val, dg, ga, din, dout, x, r, b, dinv,
+                                        c1, c2, vscale);
+}
+
+// FP16 form of a level operator: v16 = fp16(v64 / vscale) in the SELL
+// layout (padding lanes included), dgh = dg64 + the rounding of the row's
+// off-diagonal values, so that the stored row sums equal the FP64 ones.
+template <int BS>
+__global__ void __launch_bounds__(kBlock) k_mg_make_half(int32_t nb, int64_t nrows_pad, const int64_t* __restrict__ sptr,
+                                                         int32_t nst, int stencil, const double* __restrict__ v64,
+                                                         const double* __restrict__ dg64, double vscale,
+                                                         _Float16* __restrict__ v16, double* __restrict__ dgh) {
+  constexpr int BB = BS * BS;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t row = (int64_t)blockIdx.x * kBlock + threadIdx.x; row < nrows_pad; row += stride) {
+    const int64_t s = row >> 6, l = row & 63;
+    const int64_t s0 = stencil ? s * nst : sptr[s];
+    const int64_t s1 = stencil ? s0 + nst : sptr[s + 1];
+    double comp[BB];
+#pragma unroll
+    for (int t = 0; t < BB; ++t) comp[t] = 0.0;
+    for (int64_t slot = s0; slot < s1; ++slot)
+#pragma unroll
+      for (int t = 0; t < BB; ++t) {
+        const int64_t i = (slot * BB + t) * 64 + l;
+        const double a = v64[i];
+        const _Float16 h = (_Float16)(a / vscale);
+        v16[i] = h;
+        comp[t] += a - (double)h * vscale;
+      }
+    if (row < nb)
+#pragma unroll
+      for (int t = 0; t < BB; ++t) dgh[row * BB + t] = dg64[row * BB + t] + comp[t];
+  }
 }
 
 template <int W, int K, class T>
@@ -246,14 +278,30 @@ static void sweep(cpm_mg* M, const MgLevel& Lv, const void* din, void* dout, dou
   const Sell& S = Lv.A->sell;
   hipStream_t s = M->ctx->stream;
   const int g = sell_grid(S);
+  if constexpr (sizeof(T) == 4) {
+    if (M->half) {
+      const _Float16* v16 = (const _Float16*)Lv.v16;
+      if (S.stencil)
+        k_mg_sweep<BS, K, T, MODE, true, _Float16><<<g, kBlock, 0, s>>>(
+            S.nbrows, S.sptr, S.col, v16, Lv.dgh, grid_arg(S), (const T*)din, (T*)dout, (T*)Lv.x, (T*)Lv.r,
+            (const T*)Lv.b, (const T*)Lv.dinv, c1, c2, Lv.vscale, st);
+      else
+        k_mg_sweep<BS, K, T, MODE, false, _Float16><<<g, kBlock, 0, s>>>(
+            S.nbrows, S.sptr, S.col, v16, Lv.dgh, grid_arg(S), (const T*)din, (T*)dout, (T*)Lv.x, (T*)Lv.r,
+            (const T*)Lv.b, (const T*)Lv.dinv, c1, c2, Lv.vscale, st);
+      return;
+    }
+  }
   if (S.stencil)
-    k_mg_sweep<BS, K, T, MODE, true><<<g, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, vals<T>(S), diag<T>(S),
-                                                          grid_arg(S), (const T*)din, (T*)dout, (T*)Lv.x, (T*)Lv.r,
-                                                          (const T*)Lv.b, (const T*)Lv.dinv, c1, c2, st);
+    k_mg_sweep<BS, K, T, MODE, true, T><<<g, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, vals<T>(S), diag<T>(S),
+                                                             grid_arg(S), (const T*)din, (T*)dout, (T*)Lv.x,
+                                                             (T*)Lv.r, (const T*)Lv.b, (const T*)Lv.dinv, c1, c2, 1.0,
+                                                             st);
   else
-    k_mg_sweep<BS, K, T, MODE, false><<<g, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, vals<T>(S), diag<T>(S),
-                                                           grid_arg(S), (const T*)din, (T*)dout, (T*)Lv.x, (T*)Lv.r,
-                                                           (const T*)Lv.b, (const T*)Lv.dinv, c1, c2, st);
+    k_mg_sweep<BS, K, T, MODE, false, T><<<g, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, vals<T>(S), diag<T>(S),
+                                                              grid_arg(S), (const T*)din, (T*)dout, (T*)Lv.x,
+                                                              (T*)Lv.r, (const T*)Lv.b, (const T*)Lv.dinv, c1, c2,
+                                                              1.0, st);
 }
 
 // One V-cycle from level l for K columns: b of level 0 read from the FP64
@@ -553,7 +601,10 @@ extern "C" int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, 
                              int precision, int degree, double ratio, cpm_mg** out) {
   if (!ctx || !levels || !coarse_inv || !out) return fail(CPM_EINVAL, "cpm_mg_create: null argument");
   if (nlevels < 2 || nlevels > 16) return fail(CPM_EINVAL, "cpm_mg_create: 2..16 levels");
-  if (precision != 32 && precision != 64) return fail(CPM_EINVAL, "cpm_mg_create: precision must be 32 or 64");
+  if (precision != 16 && precision != 32 && precision != 64)
+    return fail(CPM_EINVAL, "cpm_mg_create: precision must be 16, 32 or 64");
+  const bool half = precision == 16;  // FP32 vectors, FP16 off-diagonal values
+  if (half) precision = 32;
   if (degree < 1 || degree > 8) return fail(CPM_EINVAL, "cpm_mg_create: Chebyshev degree must be 1..8");
   if (!(ratio > 1.0)) return fail(CPM_EINVAL, "cpm_mg_create: eigenvalue ratio must be > 1");
   const int bs = levels[0] ? levels[0]->sell.bs : 0;
@@ -575,6 +626,7 @@ extern "C" int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, 
   M->ctx = ctx;
   M->bs = bs;
   M->prec = precision;
+  M->half = half ? 1 : 0;
   M->degree = degree;
   M->ratio = ratio;
   {
@@ -623,8 +675,31 @@ extern "C" int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, 
       rc = fail(CPM_EINVAL, "cpm_mg_create: singular diagonal block");
       break;
     }
+    if (half) {
+      // scale: a power of two that brings the largest diagonal entry (a
+      // bound of every entry of the SPD operator) to <= 2^14
+      double dmax = 0.0;
+      for (double v : dg) dmax = std::max(dmax, std::fabs(v));
+      Lv.vscale = std::ldexp(1.0, std::max(-60, (int)std::ceil(std::log2(std::max(dmax, 1e-300) / 16384.0))));
+      A(&Lv.v16, sizeof(_Float16) * std::max<int64_t>(1, S.nslot * bs * bs * 64));
+      A((void**)&Lv.dgh, sizeof(double) * nbb);
+      if (rc != CPM_OK) break;
+      const int64_t pad = (int64_t)S.nslices * 64;
+      if (bs == 3)
+        k_mg_make_half<3><<<grid_for(pad), kBlock, 0, s>>>(S.nbrows, pad, S.sptr, S.nst, S.stencil, S.v64, S.dg64,
+                                                           Lv.vscale, (_Float16*)Lv.v16, Lv.dgh);
+      else
+        k_mg_make_half<1><<<grid_for(pad), kBlock, 0, s>>>(S.nbrows, pad, S.sptr, S.nst, S.stencil, S.v64, S.dg64,
+                                                           Lv.vscale, (_Float16*)Lv.v16, Lv.dgh);
+      e = hipGetLastError();
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      if (e != hipSuccess) {
+        rc = fail(CPM_EHIP, hipGetErrorString(e));
+        break;
+      }
+    }
     if (precision == 32) {
-      e = hipMemcpy(dg.data(), S.dgc, sizeof(double) * nbb, hipMemcpyDeviceToHost);
+      e = hipMemcpy(dg.data(), half ? Lv.dgh : S.dgc, sizeof(double) * nbb, hipMemcpyDeviceToHost);
       if (e != hipSuccess || !invert_blocks(bs, Lv.nb, dg, inv_t)) {
         rc = fail(CPM_EINVAL, "cpm_mg_create: singular diagonal block (FP32 form)");
         break;

@@ -70,19 +70,21 @@ __device__ __forceinline__ void mg_first_rows(int64_t lb, int64_t lg, int32_t nb
 // MODE 0: Chebyshev step r -= A d_in, d_out = c1 d_in + c2 D^-1 r, x += d_out.
 // MODE 1: r -= A d_in only (residual after the last step).
 // MODE 2: r = b - A x.
-template <int BS, int K, class T, int MODE, bool ST>
+// Off-diagonal values of type VT (T, or _Float16 scaled by 1/vscale).
+template <int BS, int K, class T, int MODE, bool ST, class VT = T>
 __device__ __forceinline__ void mg_sweep_rows(int64_t lb, int64_t lg, int32_t nb, const int64_t* __restrict__ sptr,
-                                              const int32_t* __restrict__ col, const T* __restrict__ val,
+                                              const int32_t* __restrict__ col, const VT* __restrict__ val,
                                               const double* __restrict__ dg, const GridArg& ga,
                                               const T* __restrict__ din, T* __restrict__ dout, T* __restrict__ x,
                                               T* __restrict__ r, const T* __restrict__ b,
-                                              const T* __restrict__ dinv, double c1, double c2) {
+                                              const T* __restrict__ dinv, double c1, double c2,
+                                              double vscale = 1.0) {
   constexpr int NV = BS * K;
   const T* src = (MODE == 2) ? x : din;
   const int64_t stride = lg * kBlock;
   for (int64_t row = lb * kBlock + threadIdx.x; row < (int64_t)((nb + 63) & ~63); row += stride) {
     double sum[NV], xo[NV];
-    sell_row<BS, K, T, false, ST>(row, nb, sptr, col, val, dg, src, nullptr, ga, sum, xo);
+    sell_row<BS, K, T, false, ST, VT>(row, nb, sptr, col, val, dg, src, nullptr, ga, sum, xo, vscale);
     if (row >= nb) continue;
     double rv[NV];
 #pragma unroll

@@ -54,6 +54,7 @@ struct SmoothSeg {
   const double* b64;  // depth 0: the system's FP64 outer residual
   double c0, c1[8], c2[8];
   Dims3 fd, cd;
+  double vscale;      // FP16 values: their scale
   int32_t anchor;     // node 0 left out of the transfers
   void *bc, *xc;      // next level's b and x
   const CgState* st;
@@ -83,7 +84,7 @@ __global__ void __launch_bounds__(kBlock) kbm_first(const SmoothSeg* __restrict_
 
 // MODE 0: Chebyshev step k (d_in = d[(k-1)&1], d_out = d[k&1]); MODE 1: the
 // residual after step `k` (d_in = d[k&1]); MODE 2: r = b - A x.
-template <int BS, int K, class T, int MODE>
+template <int BS, int K, class T, int MODE, class VT>
 __global__ void __launch_bounds__(kBlock) kbm_sweep(const SmoothSeg* __restrict__ seg, const int32_t* __restrict__ map,
                                                     int k) {
   const SmoothSeg& d = seg[map[blockIdx.x]];
@@ -94,11 +95,11 @@ __global__ void __launch_bounds__(kBlock) kbm_sweep(const SmoothSeg* __restrict_
   const double c1 = (MODE == 0) ? d.c1[k] : 0.0, c2 = (MODE == 0) ? d.c2[k] : 0.0;
   const int64_t lb = (int64_t)blockIdx.x - d.blk0;
   if (d.stencil)
-    mg_sweep_rows<BS, K, T, MODE, true>(lb, d.nblk, d.nb, d.sptr, d.col, (const T*)d.val, d.dg, d.ga, din, dout,
-                                        (T*)d.x, (T*)d.r, (const T*)d.b, (const T*)d.dinv, c1, c2);
+    mg_sweep_rows<BS, K, T, MODE, true, VT>(lb, d.nblk, d.nb, d.sptr, d.col, (const VT*)d.val, d.dg, d.ga, din, dout,
+                                            (T*)d.x, (T*)d.r, (const T*)d.b, (const T*)d.dinv, c1, c2, d.vscale);
   else
-    mg_sweep_rows<BS, K, T, MODE, false>(lb, d.nblk, d.nb, d.sptr, d.col, (const T*)d.val, d.dg, d.ga, din, dout,
-                                         (T*)d.x, (T*)d.r, (const T*)d.b, (const T*)d.dinv, c1, c2);
+    mg_sweep_rows<BS, K, T, MODE, false, VT>(lb, d.nblk, d.nb, d.sptr, d.col, (const VT*)d.val, d.dg, d.ga, din, dout,
+                                             (T*)d.x, (T*)d.r, (const T*)d.b, (const T*)d.dinv, c1, c2, d.vscale);
 }
 
 template <int W, int K, class T>
@@ -295,7 +296,7 @@ struct MultiMgTables {
 MultiMgSolver::MultiMgSolver() : T(new MultiMgTables()) {}
 MultiMgSolver::~MultiMgSolver() { delete T; }
 
-template <int BS, int K, class T>
+template <int BS, int K, class T, class VT>
 static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sys, int hint, int32_t* iters,
                double* resid, int* failed) {
   hipStream_t s = ctx->stream;
@@ -350,8 +351,9 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
         const Sell& S = Lv.A->sell;
         SmoothSeg g{};
         g.sptr = S.sptr, g.col = S.col;
-        g.val = (M->prec == 32) ? (const void*)S.v32 : (const void*)S.v64;
-        g.dg = (M->prec == 32) ? S.dgc : S.dg64;
+        g.val = M->half ? (const void*)Lv.v16 : (M->prec == 32) ? (const void*)S.v32 : (const void*)S.v64;
+        g.dg = M->half ? Lv.dgh : (M->prec == 32) ? S.dgc : S.dg64;
+        g.vscale = M->half ? Lv.vscale : 1.0;
         g.ga = GridArg{S.dims[0], S.dims[1], S.dims[2], S.nst, S.soff};
         g.nb = S.nbrows, g.stencil = S.stencil;
         g.x = Lv.x, g.r = Lv.r, g.b = Lv.b, g.d0 = Lv.d0, g.d1 = Lv.d1, g.dinv = Lv.dinv;
@@ -406,8 +408,8 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
           kbm_first<BS, K, T, true, true><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap);
         else
           kbm_first<BS, K, T, true, false><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap);
-        for (int k = 1; k < m; ++k) kbm_sweep<BS, K, T, 0><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap, k);
-        kbm_sweep<BS, K, T, 1><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap, m - 1);
+        for (int k = 1; k < m; ++k) kbm_sweep<BS, K, T, 0, VT><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap, k);
+        kbm_sweep<BS, K, T, 1, VT><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap, m - 1);
         kbm_restrict<BS * K, K, T><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap);
       }
       auto& Dn = W.dense[d + 1];
@@ -417,9 +419,9 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
       auto& Sm = W.smooth[d];
       if (!Sm.G) continue;
       kbm_prolong<BS * K, K, T><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap);
-      kbm_sweep<BS, K, T, 2><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap, 0);
+      kbm_sweep<BS, K, T, 2, VT><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap, 0);
       kbm_first<BS, K, T, false, false><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap);
-      for (int k = 1; k < m; ++k) kbm_sweep<BS, K, T, 0><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap, k);
+      for (int k = 1; k < m; ++k) kbm_sweep<BS, K, T, 0, VT><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap, k);
     }
   };
 
@@ -489,7 +491,7 @@ int multi_mg_pcg(cpm_ctx* ctx, MultiMgSolver& W, const std::vector<MultiMgSys>& 
   for (const auto& y : sys) {
     if (!y.M || !y.b || !y.x) return fail(CPM_EINVAL, "batched multigrid CG: null system");
     if (y.M->ctx != ctx) return fail(CPM_EINVAL, "batched multigrid CG: a hierarchy uses another context");
-    if (y.M->bs != M0->bs || y.M->prec != M0->prec || y.M->degree != M0->degree)
+    if (y.M->bs != M0->bs || y.M->prec != M0->prec || y.M->half != M0->half || y.M->degree != M0->degree)
       return fail(CPM_EINVAL, "batched multigrid CG: hierarchies differ in block size, precision or degree");
     if (y.M->L.size() > 16) return fail(CPM_EINVAL, "batched multigrid CG: too many levels");
   }
@@ -501,14 +503,20 @@ int multi_mg_pcg(cpm_ctx* ctx, MultiMgSolver& W, const std::vector<MultiMgSys>& 
         return fail(CPM_EINVAL, "batched multigrid CG: two systems share one hierarchy (and its workspace)");
   for (const auto& y : sys)
     if (int rc = mg_reserve(y.M, K)) return rc;
+  MultiMgTables& T = *W.T;
+  if (M0->half) {
+    if (M0->bs == 3) return run<3, 1, float, _Float16>(ctx, T, sys, hint, iters, resid, failed);
+    if (K == 3) return run<1, 3, float, _Float16>(ctx, T, sys, hint, iters, resid, failed);
+    return run<1, 1, float, _Float16>(ctx, T, sys, hint, iters, resid, failed);
+  }
   if (M0->bs == 3)
-    return M0->prec == 32 ? run<3, 1, float>(ctx, *W.T, sys, hint, iters, resid, failed)
-                          : run<3, 1, double>(ctx, *W.T, sys, hint, iters, resid, failed);
+    return M0->prec == 32 ? run<3, 1, float, float>(ctx, T, sys, hint, iters, resid, failed)
+                          : run<3, 1, double, double>(ctx, T, sys, hint, iters, resid, failed);
   if (K == 3)
-    return M0->prec == 32 ? run<1, 3, float>(ctx, *W.T, sys, hint, iters, resid, failed)
-                          : run<1, 3, double>(ctx, *W.T, sys, hint, iters, resid, failed);
-  return M0->prec == 32 ? run<1, 1, float>(ctx, *W.T, sys, hint, iters, resid, failed)
-                        : run<1, 1, double>(ctx, *W.T, sys, hint, iters, resid, failed);
+    return M0->prec == 32 ? run<1, 3, float, float>(ctx, T, sys, hint, iters, resid, failed)
+                          : run<1, 3, double, double>(ctx, T, sys, hint, iters, resid, failed);
+  return M0->prec == 32 ? run<1, 1, float, float>(ctx, T, sys, hint, iters, resid, failed)
+                        : run<1, 1, double, double>(ctx, T, sys, hint, iters, resid, failed);
 }
 
 }  // namespace cpm

@@ -8,6 +8,10 @@
 #pragma once
 #include "device.h"
 
+#ifndef SELL_HALF_UNROLL
+#define SELL_HALF_UNROLL 7
+#endif
+
 namespace cpm {
 
 // Periodic structured grid of an index-free (stencil) SELL operator.
@@ -19,13 +23,15 @@ struct GridArg {
 // sum = row `row` of A x (ADDX32: A (x + x32)), FP64 accumulation, for
 // BS x BS blocks and K interleaved right-hand sides x[(c*BS + e)*K + j];
 // xo = the row's own entries of x (+ x32).  Padding rows (row >= nbrows)
-// only stream their (zero) slots and leave xo unset.
-template <int BS, int K, class T, bool ADDX32, bool ST>
+// only stream their (zero) slots and leave xo unset.  Off-diagonal values
+// of type VT (default T); for VT = _Float16 they are stored times a power of
+// two and their sum is scaled back by vscale.
+template <int BS, int K, class T, bool ADDX32, bool ST, class VT = T>
 __device__ __forceinline__ void sell_row(int64_t row, int32_t nbrows, const int64_t* __restrict__ sptr,
-                                         const int32_t* __restrict__ col, const T* __restrict__ val,
+                                         const int32_t* __restrict__ col, const VT* __restrict__ val,
                                          const double* __restrict__ dg, const T* __restrict__ x,
                                          const float* __restrict__ x32, const GridArg& ga, double (&sum)[BS * K],
-                                         double (&xo)[BS * K]) {
+                                         double (&xo)[BS * K], double vscale = 1.0) {
   constexpr int BB = BS * BS;
   constexpr int NV = BS * K;
   const int64_t s = row >> 6, l = row & 63;
@@ -74,12 +80,19 @@ __device__ __forceinline__ void sell_row(int64_t row, int32_t nbrows, const int6
         for (int j = 0; j < K; ++j) sum[d * K + j] += a * xv[e * K + j];
       }
   };
+  // FP16 values: more slots per trip keep as many bytes in flight as the
+  // wider types do
+  constexpr int U = (sizeof(VT) == 2) ? SELL_HALF_UNROLL : 2;
   int64_t slot = s0;
-  for (; slot + 1 < s1; slot += 2) {
-    slot_fma(slot);
-    slot_fma(slot + 1);
+  for (; slot + U - 1 < s1; slot += U) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) slot_fma(slot + u);
   }
-  if (slot < s1) slot_fma(slot);
+  for (; slot < s1; ++slot) slot_fma(slot);
+  if constexpr (sizeof(VT) == 2) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) sum[v] *= vscale;
+  }
   if (row < nbrows) {
 #pragma unroll
     for (int v = 0; v < NV; ++v) {

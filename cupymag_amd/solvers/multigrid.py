@@ -28,7 +28,7 @@ from ..mesh.periodic import periodic_node_map, structured_dims
 class KuhnMultigrid:
     """A multigrid hierarchy (levels[0] is the operator being solved)."""
 
-    def __init__(self, levels, coarse_dense, precision=32, degree=2, ratio=20.0):
+    def __init__(self, levels, coarse_dense, precision=32, degree=2, ratio=10.0):
         self.levels = list(levels)
         self.ctx = levels[0].ctx
         self.precision = int(precision)
@@ -113,7 +113,7 @@ def _interleave_perm(n, bs):
     return (np.arange(bs)[None, :] * n + i[:, None]).ravel()
 
 
-def build_hierarchies(SimOp, precision=32, degree=2, ratio=20.0, coarsest_max=8, nthreads=None):
+def build_hierarchies(SimOp, precision=32, degree=2, ratio=10.0, coarsest_max=8, nthreads=None):
     """{"A_demag": KuhnMultigrid, "A1": KuhnMultigrid (Gauss-Seidel operator
     M + alpha1 K; coarse levels without the defect surgery), "A_el":
     KuhnMultigrid (with coupling)} for a SimulationOperators object on an

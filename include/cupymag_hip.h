@@ -205,17 +205,22 @@ int cpm_weighted_sum(cpm_ctx* ctx, int32_t n, int ncomp, const double* W, const 
 /* multigrid-preconditioned CG (structured periodic Kuhn meshes)        */
 /* ------------------------------------------------------------------ */
 /* No counterpart in the reference, whose solve_cg passes M=None
- * (linear_solvers.py:20,59-62): an optional preconditioner for the two
- * slowly converging solves (elasticity, demag).  levels[0] is the operator
- * (its index-free SELL form), levels[l+1] the same operator assembled on
- * the Kuhn mesh of the even nodes of level l (half the nodes per
- * dimension); the last level is solved with coarse_inv (dense n x n,
- * row-major, in the SELL vector order: node-major interleaved for bs 3).
- * Transfers: nested P1 interpolation and its transpose, node 0 (the
- * anchor) excluded.  Smoother: Chebyshev of `degree` on D^-1 A over
- * [lmax/ratio, lmax], lmax = 1.1 x a power-iteration estimate.  precision:
- * 32 or 64, the arithmetic of the V-cycle (the outer CG is FP64).  The
- * levels are borrowed and must outlive the hierarchy. */
+ * (linear_solvers.py:20,59-62): an optional preconditioner for the slowly
+ * converging solves (elasticity, demag, the Gauss-Seidel operator A1).
+ * levels[0] is the operator (its index-free SELL form), levels[l+1] the
+ * same operator assembled on the coarse Kuhn mesh of level l: ceil(n/2)
+ * cells per dimension (the even nodes for even n, the regular mesh of the
+ * same box for odd n); the last level is solved with coarse_inv (dense
+ * n x n, row-major, in the SELL vector order: node-major interleaved for
+ * bs 3).  Transfers: P1 interpolation (nested for even n, at the nominal
+ * node positions for odd n) and its transpose; node 0 is left out of both
+ * when row 0 of levels[0] is an identity row (the anchor of A_demag and
+ * A_el).  Smoother: Chebyshev of `degree` on D^-1 A over [lmax/ratio, lmax],
+ * lmax = 1.1 x a power-iteration estimate.  precision: the arithmetic of
+ * the V-cycle (the outer CG is FP64): 64, 32, or 16 = FP32 vectors with the
+ * level operators' off-diagonal values stored in FP16 (scaled by a power of
+ * two, diagonal blocks in FP64 compensated for the rounding).  The levels
+ * are borrowed and must outlive the hierarchy. */
 typedef struct cpm_mg cpm_mg;
 int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, const double* coarse_inv, int precision,
                   int degree, double ratio, cpm_mg** out);

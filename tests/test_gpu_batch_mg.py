@@ -39,7 +39,7 @@ def meshes():
 
 
 @pytest.mark.parametrize("name", ["A_demag", "A_el", "A1"])
-@pytest.mark.parametrize("prec", [32, 64])
+@pytest.mark.parametrize("prec", [16, 32, 64])
 def test_solve_batch_equals_single_solves(meshes, name, prec):
     from cupymag_amd.solvers.multigrid import solve_batch
     mgs = [S.multigrid(precision=prec)[name] for _, _, _, S in meshes]
@@ -71,7 +71,7 @@ def test_solve_batch_equals_single_solves(meshes, name, prec):
         solve_batch(mgs, rhs, tol=1e-14, maxiter=2)
 
 
-@pytest.mark.parametrize("prec", [32, 64])
+@pytest.mark.parametrize("prec", [16, 32, 64])
 def test_solve_batch_three_columns_equals_single_solves(meshes, prec):
     """The Gauss-Seidel operator A1 with three interleaved columns (one of
     them zero in one system, an initial guess in another)."""

@@ -3,7 +3,8 @@
 * One V-cycle in FP64 equals a NumPy/SciPy V-cycle built from the same
   level operators, the host transfer matrix (whose stencil the Galerkin
   identity pins in test_multigrid_host.py) and the same Chebyshev interval,
-  to 1e-11 relative; the FP32 V-cycle to 1e-5.
+  to 1e-11 relative; the FP32 V-cycle to 1e-5; the V-cycle with FP16
+  operator values (precision 16) to 3e-3.
 * The preconditioned CG meets tol 1e-10 (CuPy's rule) and its solution
   matches a sparse direct solve to 1e-8 relative, for the scalar (demag)
   and 3x3-block (elasticity) operators, on regular and jittered meshes, in
@@ -103,7 +104,7 @@ def test_vcycle_matches_host_reference(name, bs, n):
     from cupymag_amd.mesh.coarsen import hierarchy_sizes
     _, _, _, S = _setup(n=n)
     sizes = hierarchy_sizes(n)
-    for prec, tol in ((64, 1e-11), (32, 1e-5)):
+    for prec, tol in ((64, 1e-11), (32, 1e-5), (16, 3e-3)):
         mg = S.multigrid(precision=prec)[name]
         info = mg.info()
         assert info["levels"] == len(sizes) and info["dims"][-1] == (sizes[-1],) * 3
@@ -136,7 +137,7 @@ def test_vcycle_matches_host_reference(name, bs, n):
 @pytest.mark.parametrize("name,bs", [("A_demag", 1), ("A_el", 3), ("A1", 1)])
 def test_mg_pcg_matches_direct_solve(name, bs, n, jitter):
     _, _, _, S = _setup(n=n, jitter=jitter)
-    for prec in (32, 64):
+    for prec in (16, 32, 64):
         mg = S.multigrid(precision=prec)[name]
         A = _interleaved(mg.levels[0], bs)
         rng = np.random.default_rng(6)
