@@ -69,6 +69,17 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def mode_kwargs(base, opts, mg_degree=2, mg_ratio=10.0):
+    """GSPMSimulation options of a bench mode: '+warm' (warm start), '+mg'
+    (multigrid for the elasticity, demag and Gauss-Seidel A1 solves; FP16
+    level values in the mixed mode), '+mged' (multigrid for elasticity and
+    demag only), '+mg32' (FP32 level values)."""
+    mg = "mg" in opts or "mged" in opts or "mg32" in opts
+    mgp = (32 if "mg32" in opts else 16) if base == "mixed" else 64
+    return dict(warm_start="warm" in opts, precond="mg" if mg else None, mg_gs="mged" not in opts,
+                mg_degree=mg_degree, mg_ratio=mg_ratio, mg_precision=mgp if mg else None)
+
+
 def run_gpu(n, steps, warmup, steps64, modes, rel_delta=0.3, autotune=True, rel_delta_gs=None, mg_degree=2,
             mg_ratio=10.0):
     """Run the plain FP64 path and the fast paths on the same operators and
@@ -108,16 +119,15 @@ def run_gpu(n, steps, warmup, steps64, modes, rel_delta=0.3, autotune=True, rel_
     snap_at = warmup + steps64
     for mode in modes:
         base, *opts = mode.split("+")
-        mg = "mg" in opts or "mged" in opts or "mg32" in opts
-        mgp = (32 if "mg32" in opts else 16) if base == "mixed" else 64
+        kw = mode_kwargs(base, opts, mg_degree, mg_ratio)
+        mg, mgp = kw["precond"] == "mg", kw["mg_precision"]
         if mg:
             t0 = time.time()
             S.multigrid(precision=mgp, degree=mg_degree, ratio=mg_ratio)
             log(f"[bench] multigrid hierarchies for {mode}: {time.time() - t0:.1f}s")
-        kw = {} if rel_delta_gs is None else dict(rel_delta_gs=rel_delta_gs)
-        sim = GSPMSimulation(S, P, mode=base, warm_start=("warm" in opts), rel_delta=rel_delta,
-                             precond="mg" if mg else None, mg_gs="mged" not in opts, mg_degree=mg_degree,
-                             mg_ratio=mg_ratio, mg_precision=mgp if mg else None, **kw)
+        if rel_delta_gs is not None:
+            kw["rel_delta_gs"] = rel_delta_gs
+        sim = GSPMSimulation(S, P, mode=base, rel_delta=rel_delta, **kw)
         ctx = sim.ctx
         sim.set_m(m0)
         k_timed = steps64 if base == "fp64" else steps
@@ -393,7 +403,7 @@ def run_accuracy(args):
     snaps, timing, iters = {}, {}, {}
     for mode in modes:
         base, *opts = mode.split("+")
-        sim = GSPMSimulation(S, P, mode=base, precond="mg" if "mg" in opts else None)
+        sim = GSPMSimulation(S, P, mode=base, **mode_kwargs(base, opts, args.mg_degree, args.mg_ratio))
         sim.set_m(m0)
         snaps[mode] = []
         t = time.perf_counter()

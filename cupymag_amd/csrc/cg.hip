@@ -239,8 +239,10 @@ extern "C" int cpm_cg(cpm_ctx* ctx, const cpm_mat* A, const double* b, double* x
 
 extern "C" int cpm_prof_enable(cpm_ctx* ctx, int on) {
   if (!ctx) return cpm::fail(CPM_EINVAL, "cpm_prof_enable: null context");
+  // timing-only events: no system-scope fence (no cache write-back) at
+  // record, so a pair around one launch times the kernel, not the flush
   if (on && !ctx->ev[0])
-    for (auto& e : ctx->ev) CPM_HIP(hipEventCreate(&e));
+    for (auto& e : ctx->ev) CPM_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
   ctx->prof = on ? 1 : 0;
   return CPM_OK;
 }

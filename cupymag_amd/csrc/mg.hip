@@ -38,6 +38,8 @@
 #include <cstring>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "device.h"
 #include "mg.h"
 #include "mg_dev.h"
@@ -272,9 +274,26 @@ static const double* diag(const Sell& S) {
     return S.dgc;
 }
 
+template <int BS, int K, class T, int MODE, bool ST, class VT>
+static void launch_sweep(hipStream_t s, int g, const Sell& S, const VT* val, const double* dg, const MgLevel& Lv,
+                         const void* din, void* dout, double c1, double c2, double vscale, const CgState* st,
+                         hipEvent_t* ev) {
+  if (ev)
+    hipExtLaunchKernelGGL(k_mg_sweep<BS, K, T, MODE, ST, VT>, dim3(g), dim3(kBlock), 0, s, ev[0], ev[1], 0,
+                          (int32_t)S.nbrows, (const int64_t*)S.sptr, (const int32_t*)S.col, val, dg, grid_arg(S),
+                          (const T*)din, (T*)dout, (T*)Lv.x, (T*)Lv.r, (const T*)Lv.b, (const T*)Lv.dinv, c1, c2,
+                          vscale, st);
+  else
+    k_mg_sweep<BS, K, T, MODE, ST, VT><<<g, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, val, dg, grid_arg(S),
+                                                           (const T*)din, (T*)dout, (T*)Lv.x, (T*)Lv.r,
+                                                           (const T*)Lv.b, (const T*)Lv.dinv, c1, c2, vscale, st);
+}
+
+// ev (profiling, may be null): start and stop events taken by the dispatch
+// itself (hipExtLaunchKernelGGL), so they time the kernel alone
 template <int BS, int K, class T, int MODE>
 static void sweep(cpm_mg* M, const MgLevel& Lv, const void* din, void* dout, double c1, double c2,
-                  const CgState* st) {
+                  const CgState* st, hipEvent_t* ev = nullptr) {
   const Sell& S = Lv.A->sell;
   hipStream_t s = M->ctx->stream;
   const int g = sell_grid(S);
@@ -282,31 +301,22 @@ static void sweep(cpm_mg* M, const MgLevel& Lv, const void* din, void* dout, dou
     if (M->half) {
       const _Float16* v16 = (const _Float16*)Lv.v16;
       if (S.stencil)
-        k_mg_sweep<BS, K, T, MODE, true, _Float16><<<g, kBlock, 0, s>>>(
-            S.nbrows, S.sptr, S.col, v16, Lv.dgh, grid_arg(S), (const T*)din, (T*)dout, (T*)Lv.x, (T*)Lv.r,
-            (const T*)Lv.b, (const T*)Lv.dinv, c1, c2, Lv.vscale, st);
+        launch_sweep<BS, K, T, MODE, true>(s, g, S, v16, Lv.dgh, Lv, din, dout, c1, c2, Lv.vscale, st, ev);
       else
-        k_mg_sweep<BS, K, T, MODE, false, _Float16><<<g, kBlock, 0, s>>>(
-            S.nbrows, S.sptr, S.col, v16, Lv.dgh, grid_arg(S), (const T*)din, (T*)dout, (T*)Lv.x, (T*)Lv.r,
-            (const T*)Lv.b, (const T*)Lv.dinv, c1, c2, Lv.vscale, st);
+        launch_sweep<BS, K, T, MODE, false>(s, g, S, v16, Lv.dgh, Lv, din, dout, c1, c2, Lv.vscale, st, ev);
       return;
     }
   }
   if (S.stencil)
-    k_mg_sweep<BS, K, T, MODE, true, T><<<g, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, vals<T>(S), diag<T>(S),
-                                                             grid_arg(S), (const T*)din, (T*)dout, (T*)Lv.x,
-                                                             (T*)Lv.r, (const T*)Lv.b, (const T*)Lv.dinv, c1, c2, 1.0,
-                                                             st);
+    launch_sweep<BS, K, T, MODE, true>(s, g, S, vals<T>(S), diag<T>(S), Lv, din, dout, c1, c2, 1.0, st, ev);
   else
-    k_mg_sweep<BS, K, T, MODE, false, T><<<g, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, vals<T>(S), diag<T>(S),
-                                                              grid_arg(S), (const T*)din, (T*)dout, (T*)Lv.x,
-                                                              (T*)Lv.r, (const T*)Lv.b, (const T*)Lv.dinv, c1, c2,
-                                                              1.0, st);
+    launch_sweep<BS, K, T, MODE, false>(s, g, S, vals<T>(S), diag<T>(S), Lv, din, dout, c1, c2, 1.0, st, ev);
 }
 
 // One V-cycle from level l for K columns: b of level 0 read from the FP64
-// vector b64.  ev (profiling, may be null): events recorded around the first
-// Chebyshev step of level 0.
+// vector b64.  ev (profiling, may be null): two event pairs, taken by the
+// first Chebyshev step of level 0 in the pre-smoothing (ev[0], ev[1]) and in
+// the post-smoothing (ev[2], ev[3]).
 template <int BS, int K, class T>
 static void vcycle(cpm_mg* M, size_t l, const double* b64, const CgState* st, hipEvent_t* ev = nullptr) {
   MgLevel& Lv = M->L[l];
@@ -329,11 +339,8 @@ static void vcycle(cpm_mg* M, size_t l, const double* b64, const CgState* st, hi
   else
     k_mg_first<BS, K, T, true, false><<<gn, kBlock, 0, s>>>(Lv.nb, nullptr, (T*)Lv.b, (T*)Lv.r, (T*)d[0],
                                                             (T*)Lv.x, (const T*)Lv.dinv, c0, st);
-  for (int k = 1; k < m; ++k) {
-    if (k == 1 && ev) (void)hipEventRecord(ev[0], s);
-    sweep<BS, K, T, 0>(M, Lv, d[(k - 1) & 1], d[k & 1], c1[k], c2[k], st);
-    if (k == 1 && ev) (void)hipEventRecord(ev[1], s);
-  }
+  for (int k = 1; k < m; ++k)
+    sweep<BS, K, T, 0>(M, Lv, d[(k - 1) & 1], d[k & 1], c1[k], c2[k], st, (k == 1 && ev) ? ev : nullptr);
   sweep<BS, K, T, 1>(M, Lv, d[(m - 1) & 1], nullptr, 0.0, 0.0, st);
   // coarse-grid correction
   MgLevel& Lc = M->L[l + 1];
@@ -345,7 +352,8 @@ static void vcycle(cpm_mg* M, size_t l, const double* b64, const CgState* st, hi
   sweep<BS, K, T, 2>(M, Lv, nullptr, nullptr, 0.0, 0.0, st);
   k_mg_first<BS, K, T, false, false><<<gn, kBlock, 0, s>>>(Lv.nb, nullptr, (T*)Lv.b, (T*)Lv.r, (T*)d[0], (T*)Lv.x,
                                                            (const T*)Lv.dinv, c0, st);
-  for (int k = 1; k < m; ++k) sweep<BS, K, T, 0>(M, Lv, d[(k - 1) & 1], d[k & 1], c1[k], c2[k], st);
+  for (int k = 1; k < m; ++k)
+    sweep<BS, K, T, 0>(M, Lv, d[(k - 1) & 1], d[k & 1], c1[k], c2[k], st, (k == 1 && ev) ? ev + 2 : nullptr);
 }
 
 static void vcycle_any(cpm_mg* M, const double* b64, const CgState* st) {
@@ -499,8 +507,9 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
   k_pcg_dot<K, T><<<gv, kBlock, 0, s>>>(n, M->r, M->rold, (const T*)L0.x, st, ctx->partials);
   k_pcg_red_rz<K><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st);
   CPM_HIP(hipGetLastError());
-  // profiling: the first Chebyshev step of level 0 of every iteration is
-  // timed with HIP events and accumulated on the level-0 operator
+  // profiling: the first Chebyshev step of level 0 in the pre- and in the
+  // post-smoothing of every iteration is timed with HIP events and
+  // accumulated on the level-0 operator
   const bool prof = ctx->prof && M->degree >= 2 && K == 1;
   auto issue = [&](int todo) -> int {
     for (int k = 0; k < todo; ++k) {
@@ -509,7 +518,7 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
       k_red<K, RED_ALPHA><<<1, kRedBlock, 0, s>>>(gs, ctx->partials, st, 0.0);
       k_pcg_xr<K><<<gv, kBlock, 0, s>>>(n, x, M->r, M->rold, M->p, M->q, st, ctx->partials);
       k_red<K, RED_RR_PCG><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st, 0.0);
-      vcycle<BS, K, T>(M, 0, M->r, st, prof ? &ctx->ev[2 * k] : nullptr);
+      vcycle<BS, K, T>(M, 0, M->r, st, prof ? &ctx->ev[4 * k] : nullptr);
       k_pcg_dot<K, T><<<gv, kBlock, 0, s>>>(n, M->r, M->rold, (const T*)L0.x, st, ctx->partials);
       k_pcg_red_rz<K><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st);
     }
@@ -538,7 +547,7 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
   int issued = 0;
   cpm_mat* A0 = L0.A;
   while (!all() && issued <= maxiter) {
-    const int todo = std::max(1, std::min(std::min(batch, prof ? 256 : 1024), maxiter - issued + 1));
+    const int todo = std::max(1, std::min(std::min(batch, prof ? 128 : 1024), maxiter - issued + 1));
     const int it_before = h[0].it;
     if (int rc = issue(todo)) return rc;
     issued += todo;
@@ -547,12 +556,13 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
       // V-cycles that did work: every iteration of this batch except a
       // converged last one
       const int worked = std::min(todo, h[0].it - it_before - ((h[0].done && !h[0].fail) ? 1 : 0));
-      for (int k = 0; k < worked; ++k) {
-        float ms = 0.f;
-        CPM_HIP(hipEventElapsedTime(&ms, ctx->ev[2 * k], ctx->ev[2 * k + 1]));
-        A0->prof_ms += ms;
-        A0->prof_launches += 1;
-      }
+      for (int k = 0; k < worked; ++k)
+        for (int h2 = 0; h2 < 2; ++h2) {
+          float ms = 0.f;
+          CPM_HIP(hipEventElapsedTime(&ms, ctx->ev[4 * k + 2 * h2], ctx->ev[4 * k + 2 * h2 + 1]));
+          A0->prof_ms += ms;
+          A0->prof_launches += 1;
+        }
     }
     batch = std::max(2, std::min(16, it_max() / 4 + 2));
   }

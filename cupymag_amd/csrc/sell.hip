@@ -37,6 +37,8 @@
 #include <cstring>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "device.h"
 #include "sell.h"
 #include "sell_dev.h"
@@ -265,8 +267,10 @@ int sell_grid(const Sell& A) {
 }
 static_assert(3 * kMaxSpmvGrid <= kMaxPartials, "partials buffer too small for the SELL grid");
 
+// ev (profiling, may be null): start and stop events taken by the dispatch
+// itself (hipExtLaunchKernelGGL), so they time the kernel alone
 template <int BS, int K, class T, bool CG, bool GATE = false, bool ADDX32 = false>
-static int spmv_t(cpm_ctx* ctx, const Sell& A, const T* x, const float* x32, T* y) {
+static int spmv_t(cpm_ctx* ctx, const Sell& A, const T* x, const float* x32, T* y, hipEvent_t* ev = nullptr) {
   const T* val;
   const double* dg;
   if constexpr (sizeof(T) == 8) {
@@ -277,12 +281,23 @@ static int spmv_t(cpm_ctx* ctx, const Sell& A, const T* x, const float* x32, T* 
     dg = A.dgc;
   }
   const GridArg ga{A.dims[0], A.dims[1], A.dims[2], A.nst, A.soff};
-  if (A.stencil)
-    k_sell_spmv<BS, K, T, CG, GATE, ADDX32, true><<<sell_grid(A), kBlock, 0, ctx->stream>>>(
+  const dim3 grid(sell_grid(A)), block(kBlock);
+  if (ev) {
+    if (A.stencil)
+      hipExtLaunchKernelGGL(k_sell_spmv<BS, K, T, CG, GATE, ADDX32, true>, grid, block, 0, ctx->stream, ev[0], ev[1],
+                            0, A.nbrows, (const int64_t*)A.sptr, (const int32_t*)A.col, val, dg, x, x32, y,
+                            (CgState*)ctx->cg, ctx->partials, ga);
+    else
+      hipExtLaunchKernelGGL(k_sell_spmv<BS, K, T, CG, GATE, ADDX32, false>, grid, block, 0, ctx->stream, ev[0],
+                            ev[1], 0, A.nbrows, (const int64_t*)A.sptr, (const int32_t*)A.col, val, dg, x, x32, y,
+                            (CgState*)ctx->cg, ctx->partials, ga);
+  } else if (A.stencil) {
+    k_sell_spmv<BS, K, T, CG, GATE, ADDX32, true><<<grid, block, 0, ctx->stream>>>(
         A.nbrows, A.sptr, A.col, val, dg, x, x32, y, ctx->cg, ctx->partials, ga);
-  else
-    k_sell_spmv<BS, K, T, CG, GATE, ADDX32, false><<<sell_grid(A), kBlock, 0, ctx->stream>>>(
+  } else {
+    k_sell_spmv<BS, K, T, CG, GATE, ADDX32, false><<<grid, block, 0, ctx->stream>>>(
         A.nbrows, A.sptr, A.col, val, dg, x, x32, y, ctx->cg, ctx->partials, ga);
+  }
   CPM_HIP(hipGetLastError());
   return CPM_OK;
 }
@@ -370,9 +385,8 @@ static int cg64_loop(cpm_ctx* ctx, cpm_mat* M, const double* b, double* x, int u
   auto issue = [&](int todo) -> int {
     for (int k = 0; k < todo; ++k) {
       k_mcg_pupd<K, double><<<g, kBlock, 0, s>>>(n, ctx->r, ctx->p, st);
-      if (ctx->prof) CPM_HIP(hipEventRecord(ctx->ev[2 * k], s));
-      if (int rc = spmv_t<BS, K, double, true>(ctx, A, ctx->p, nullptr, ctx->q)) return rc;
-      if (ctx->prof) CPM_HIP(hipEventRecord(ctx->ev[2 * k + 1], s));
+      if (int rc = spmv_t<BS, K, double, true>(ctx, A, ctx->p, nullptr, ctx->q, ctx->prof ? &ctx->ev[2 * k] : nullptr))
+        return rc;
       reduce_t<K, RED_ALPHA>(ctx, gs);
       k_mcg_xr<K, double><<<g, kBlock, 0, s>>>(n, x, ctx->r, ctx->p, ctx->q, st, ctx->partials);
       reduce_t<K, RED_RR>(ctx, g);
@@ -440,9 +454,8 @@ static int rel_loop(cpm_ctx* ctx, cpm_mat* M, const double* b, double* x, int us
   auto issue = [&](int todo) -> int {
     for (int k = 0; k < todo; ++k) {
       k_mcg_pupd<K, float><<<g, kBlock, 0, s>>>(n, ctx->r32, ctx->p32, st);
-      if (ctx->prof) CPM_HIP(hipEventRecord(ctx->ev[2 * k], s));
-      if (int rc = spmv_t<BS, K, float, true>(ctx, A, ctx->p32, nullptr, ctx->q32)) return rc;
-      if (ctx->prof) CPM_HIP(hipEventRecord(ctx->ev[2 * k + 1], s));
+      if (int rc = spmv_t<BS, K, float, true>(ctx, A, ctx->p32, nullptr, ctx->q32, ctx->prof ? &ctx->ev[2 * k] : nullptr))
+        return rc;
       reduce_t<K, RED_ALPHA>(ctx, gs);
       k_mcg_xr<K, float><<<g, kBlock, 0, s>>>(n, ctx->x32, ctx->r32, ctx->p32, ctx->q32, st, ctx->partials);
       reduce_t<K, RED_RR_REL>(ctx, g, delta);
