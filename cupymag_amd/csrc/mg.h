@@ -3,6 +3,8 @@
 #include <cstdint>
 #include <vector>
 
+#include <hip/hip_runtime.h>
+
 struct cpm_ctx;
 struct cpm_mat;
 
@@ -41,6 +43,19 @@ struct cpm_mg {
   // FP64 PCG vectors (fine level); level and PCG vectors hold kcap columns
   double *r = nullptr, *p = nullptr, *q = nullptr, *rold = nullptr;
   int kcap = 1;
+  // stream the V-cycle kernels go to (null: the context's stream); set to
+  // the side stream while a chunk of iterations is captured
+  hipStream_t ls = nullptr;
+  // captured chunks of PCG iterations (hipGraph), replayed on the context
+  // stream; keyed by the vectors and states they were captured with
+  int graphs = 1;
+  hipStream_t side = nullptr;
+  struct Chunk {
+    uint64_t key = 0;
+    hipGraphExec_t exec = nullptr;
+    int iters = 0;
+  };
+  std::vector<Chunk> chunks;
   std::vector<void*> allocs;
 };
 

@@ -295,7 +295,7 @@ template <int BS, int K, class T, int MODE>
 static void sweep(cpm_mg* M, const MgLevel& Lv, const void* din, void* dout, double c1, double c2,
                   const CgState* st, hipEvent_t* ev = nullptr) {
   const Sell& S = Lv.A->sell;
-  hipStream_t s = M->ctx->stream;
+  hipStream_t s = M->ls ? M->ls : M->ctx->stream;
   const int g = sell_grid(S);
   if constexpr (sizeof(T) == 4) {
     if (M->half) {
@@ -320,7 +320,7 @@ static void sweep(cpm_mg* M, const MgLevel& Lv, const void* din, void* dout, dou
 template <int BS, int K, class T>
 static void vcycle(cpm_mg* M, size_t l, const double* b64, const CgState* st, hipEvent_t* ev = nullptr) {
   MgLevel& Lv = M->L[l];
-  hipStream_t s = M->ctx->stream;
+  hipStream_t s = M->ls ? M->ls : M->ctx->stream;
   const int gn = grid_for(Lv.nb);
   if (l + 1 == M->L.size()) {
     const int g = (int)std::min<int64_t>(kMaxGrid, (Lv.n + 3) / 4);
@@ -449,6 +449,9 @@ int mg_reserve(cpm_mg* M, int K) {
   if (K <= M->kcap) return CPM_OK;
   cpm_ctx* ctx = M->ctx;
   CPM_HIP(hipStreamSynchronize(ctx->stream));
+  // captured chunks refer to the vectors replaced below
+  for (auto& c : M->chunks) (void)hipGraphExecDestroy(c.exec);
+  M->chunks.clear();
   const size_t ts = M->prec == 32 ? 4 : 8;
   auto regrow = [&](void** p, size_t bytes) -> int {
     auto it = std::find(M->allocs.begin(), M->allocs.end(), *p);
@@ -475,7 +478,7 @@ template <int BS, int K, class T>
 static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol, int maxiter, int hint,
                    int32_t* iters, double* resid, int* failed) {
   cpm_ctx* ctx = M->ctx;
-  hipStream_t s = ctx->stream;
+  hipStream_t s = ctx->stream;  // launch stream of the lambdas below
   MgLevel& L0 = M->L[0];
   const Sell& S = L0.A->sell;
   const int64_t n = L0.n * K;
@@ -508,22 +511,73 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
   k_pcg_red_rz<K><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st);
   CPM_HIP(hipGetLastError());
   // profiling: the first Chebyshev step of level 0 in the pre- and in the
-  // post-smoothing of every iteration is timed with HIP events and
-  // accumulated on the level-0 operator
+  // post-smoothing of the first iteration of every batch (launched
+  // directly, the rest of the batch from the captured chunk) is timed with
+  // HIP events and accumulated on the level-0 operator
   const bool prof = ctx->prof && M->degree >= 2 && K == 1;
-  auto issue = [&](int todo) -> int {
+  auto iterate = [&](int todo, hipEvent_t* ev) -> int {
     for (int k = 0; k < todo; ++k) {
       k_pcg_pupd<K, T><<<gv, kBlock, 0, s>>>(n, (const T*)L0.x, M->p, st);
       spmv(M->p, M->q, false);
       k_red<K, RED_ALPHA><<<1, kRedBlock, 0, s>>>(gs, ctx->partials, st, 0.0);
       k_pcg_xr<K><<<gv, kBlock, 0, s>>>(n, x, M->r, M->rold, M->p, M->q, st, ctx->partials);
       k_red<K, RED_RR_PCG><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st, 0.0);
-      vcycle<BS, K, T>(M, 0, M->r, st, prof ? &ctx->ev[4 * k] : nullptr);
+      vcycle<BS, K, T>(M, 0, M->r, st, (k == 0) ? ev : nullptr);
       k_pcg_dot<K, T><<<gv, kBlock, 0, s>>>(n, M->r, M->rold, (const T*)L0.x, st, ctx->partials);
       k_pcg_red_rz<K><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st);
     }
     CPM_HIP(hipGetLastError());
     return CPM_OK;
+  };
+  // the captured chunk: `chunk` iterations with this solve's vectors and
+  // states (the kernels read every scalar from the states, so a replay is
+  // the same work as launching them)
+  constexpr int chunk = 2;
+  hipGraphExec_t gexec = nullptr;
+  if (M->graphs) {
+    uint64_t key = 1469598103934665603ull;
+    for (uint64_t v : {(uint64_t)(uintptr_t)x, (uint64_t)(uintptr_t)M->r, (uint64_t)(uintptr_t)L0.x,
+                       (uint64_t)(uintptr_t)st, (uint64_t)(uintptr_t)ctx->partials, (uint64_t)K,
+                       (uint64_t)sizeof(T), (uint64_t)chunk})
+      key = (key ^ v) * 1099511628211ull;
+    for (auto& c : M->chunks)
+      if (c.key == key) gexec = c.exec;
+    if (!gexec) {
+      if (!M->side) CPM_HIP(hipStreamCreateWithFlags(&M->side, hipStreamNonBlocking));
+      hipGraph_t g = nullptr;
+      CPM_HIP(hipStreamBeginCapture(M->side, hipStreamCaptureModeThreadLocal));
+      s = M->side;
+      M->ls = M->side;
+      const int rc = iterate(chunk, nullptr);
+      s = ctx->stream;
+      M->ls = nullptr;
+      hipError_t e = hipStreamEndCapture(M->side, &g);
+      if (rc) return rc;
+      if (e != hipSuccess) return fail(CPM_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+      e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) return fail(CPM_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+      if (M->chunks.size() >= 4) {
+        (void)hipGraphExecDestroy(M->chunks.front().exec);
+        M->chunks.erase(M->chunks.begin());
+      }
+      M->chunks.push_back({key, gexec, chunk});
+    }
+  }
+  auto issue = [&](int todo) -> int {
+    int done_direct = 0;
+    if (!gexec || prof) {
+      done_direct = gexec ? 1 : todo;
+      if (int rc = iterate(done_direct, prof ? &ctx->ev[0] : nullptr)) return rc;
+    }
+    if (gexec)
+      for (int k = done_direct; k < todo; k += chunk) CPM_HIP(hipGraphLaunch(gexec, s));
+    return CPM_OK;
+  };
+  auto issued_for = [&](int todo) {
+    if (!gexec) return todo;
+    const int direct = prof ? 1 : 0;
+    return direct + ((todo - direct + chunk - 1) / chunk) * chunk;
   };
   CgState h[K];
   auto read = [&]() -> int {
@@ -547,19 +601,19 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
   int issued = 0;
   cpm_mat* A0 = L0.A;
   while (!all() && issued <= maxiter) {
-    const int todo = std::max(1, std::min(std::min(batch, prof ? 128 : 1024), maxiter - issued + 1));
+    const int todo = std::max(1, std::min(std::min(batch, 1024), maxiter - issued + 1));
     const int it_before = h[0].it;
     if (int rc = issue(todo)) return rc;
-    issued += todo;
+    issued += issued_for(todo);
     if (int rc = read()) return rc;
     if (prof) {
-      // V-cycles that did work: every iteration of this batch except a
-      // converged last one
-      const int worked = std::min(todo, h[0].it - it_before - ((h[0].done && !h[0].fail) ? 1 : 0));
-      for (int k = 0; k < worked; ++k)
+      // the directly launched first iteration of the batch did V-cycle work
+      // unless the solve converged in it
+      const bool worked = h[0].it > it_before + 1 || (h[0].it == it_before + 1 && (!h[0].done || h[0].fail));
+      if (worked)
         for (int h2 = 0; h2 < 2; ++h2) {
           float ms = 0.f;
-          CPM_HIP(hipEventElapsedTime(&ms, ctx->ev[4 * k + 2 * h2], ctx->ev[4 * k + 2 * h2 + 1]));
+          CPM_HIP(hipEventElapsedTime(&ms, ctx->ev[2 * h2], ctx->ev[2 * h2 + 1]));
           A0->prof_ms += ms;
           A0->prof_launches += 1;
         }
@@ -602,6 +656,8 @@ extern "C" int cpm_mg_destroy(cpm_mg* M) {
     (void)hipSetDevice(M->ctx->device);
     (void)hipStreamSynchronize(M->ctx->stream);
   }
+  for (auto& c : M->chunks) (void)hipGraphExecDestroy(c.exec);
+  if (M->side) (void)hipStreamDestroy(M->side);
   for (void* p : M->allocs) (void)hipFree(p);
   delete M;
   return CPM_OK;
@@ -833,4 +889,10 @@ extern "C" int cpm_cg_mg(cpm_mg* M, int nrhs, const double* b, double* x, int us
   int worst = 0;
   for (int j = 0; j < nrhs; ++j) worst = std::max(worst, (int)iters[j]);
   return failed ? std::max(1, worst) : CPM_OK;
+}
+
+extern "C" int cpm_mg_set_graphs(cpm_mg* M, int on) {
+  if (!M) return fail(CPM_EINVAL, "cpm_mg_set_graphs: null hierarchy");
+  M->graphs = on ? 1 : 0;
+  return CPM_OK;
 }

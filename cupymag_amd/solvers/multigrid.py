@@ -47,6 +47,11 @@ class KuhnMultigrid:
         check(lib().cpm_mg_info(self.h, C.byref(n), lm, 16), "cpm_mg_info")
         return dict(levels=n.value, lmax=list(lm[:n.value]), dims=[getattr(L, "grid_dims", None) for L in self.levels])
 
+    def set_graphs(self, on):
+        """Replay the PCG iterations from captured hipGraph chunks (default)
+        or launch every kernel; the results are the same bit for bit."""
+        check(lib().cpm_mg_set_graphs(self.h, int(bool(on))), "cpm_mg_set_graphs")
+
     def apply(self, r):
         """One V-cycle: z = M^-1 r (vectors in the SELL order)."""
         r = as_device(r, self.ctx)

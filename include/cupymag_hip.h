@@ -237,6 +237,12 @@ int cpm_mg_apply(cpm_mg* mg, const double* r, double* z);
  * each with its own scalars and stopping test; iters and resid hold nrhs
  * entries.  Returns 0, or a positive iteration count when maxiter was
  * reached. */
+/* No counterpart in the reference.  Iterations of cpm_cg_mg (and of the
+ * multigrid solves of cpm_sim_step) are replayed from captured chunks
+ * (hipGraph, 2 iterations each, cached per hierarchy for the vectors they
+ * were captured with) unless turned off here; the results are the same
+ * bit for bit.  Default on. */
+int cpm_mg_set_graphs(cpm_mg* mg, int on);
 int cpm_cg_mg(cpm_mg* mg, int nrhs, const double* b, double* x, int use_x0, double tol, int maxiter,
               int32_t* iters, double* resid);
 /* No counterpart in the reference.  cpm_cg_mg on `count` independent

@@ -213,3 +213,50 @@ def test_multigrid_rejections():
     _, _, P7, S7 = _setup(n=8, me=False)
     with pytest.raises(ValueError, match="multigrid needs"):
         GSPMSimulation(S7, P7, mode="mixed", precond="mg")
+
+
+@pytest.mark.parametrize("prec", [16, 64])
+def test_graph_replay_matches_direct_launches(prec):
+    """Iterations replayed from captured hipGraph chunks equal launching
+    every kernel, bit for bit: several solves in a row per hierarchy (each
+    replaying the cached chunk many times, with the solve's own launches and
+    host reads of the state in between), one and three columns, and three
+    GSPM steps with every solve on the multigrid."""
+    from cupymag_amd.core.gspm import GSPMSimulation
+    cfg, arr, P, S = _setup(n=12, jitter=0.1)
+    hier = S.multigrid(precision=prec)
+    rng = np.random.default_rng(21)
+    cases = [("A_el", 1), ("A_demag", 1), ("A1", 1), ("A1", 3)]
+    rhs = {c: [rng.standard_normal(hier[c[0]].levels[0].shape[0] * c[1]) for _ in range(3)] for c in cases}
+    for c in cases:
+        if c[0] != "A1":
+            for b in rhs[c]:
+                b[:3 if c[0] == "A_el" else 1] = 0.0
+    out = {}
+    for graphs in (False, True):
+        for m in hier.values():
+            m.set_graphs(graphs)
+        res = []
+        for (name, k) in cases:
+            for b in rhs[(name, k)]:
+                x, it, r = hier[name].solve(torch.as_tensor(b, device="cuda"), tol=1e-10, maxiter=500, nrhs=k)
+                res.append((x.cpu().numpy(), it, r))
+        out[graphs] = res
+    for (x0, i0, r0), (x1, i1, r1) in zip(out[False], out[True]):
+        assert i0 == i1 and r0 == r1
+        np.testing.assert_array_equal(x0, x1)
+    runs = []
+    for graphs in (False, True):
+        sim = GSPMSimulation(S, P, mode="mixed" if prec == 16 else "fast64", precond="mg")
+        for m in sim.mg.values():
+            m.set_graphs(graphs)
+        sim.set_m(tilted_m(S.n_dof, seed=4))
+        hist = []
+        for k in range(3):
+            rep = sim.step([P.Hext1, P.Hext2, P.Hext3], k > 0)
+            hist.append((sim.get("M").cpu().numpy(), sim.get("UEL").cpu().numpy(), rep.iters))
+        runs.append(hist)
+    for (m0, u0, i0), (m1, u1, i1) in zip(*runs):
+        assert i0 == i1
+        np.testing.assert_array_equal(m0, m1)
+        np.testing.assert_array_equal(u0, u1)
