@@ -71,8 +71,10 @@ __global__ void __launch_bounds__(kBlock) k_mg_sweep(int32_t nb, const int64_t* 
 }
 
 // FP16 form of a level operator: v16 = fp16(v64 / vscale) in the SELL
-// layout (padding lanes included), dgh = dg64 + the rounding of the row's
-// off-diagonal values, so that the stored row sums equal the FP64 ones.
+// layout (padding lanes included; for 3x3 blocks entries 2q and 2q+1,
+// q < 4, interleaved per lane so that one 32-bit load fetches both, entry 8
+// after them), dgh = dg64 + the rounding of the row's off-diagonal values,
+// so that the stored row sums equal the FP64 ones.
 template <int BS>
 __global__ void __launch_bounds__(kBlock) k_mg_make_half(int32_t nb, int64_t nrows_pad, const int64_t* __restrict__ sptr,
                                                          int32_t nst, int stencil, const double* __restrict__ v64,
@@ -93,7 +95,8 @@ __global__ void __launch_bounds__(kBlock) k_mg_make_half(int32_t nb, int64_t nro
         const int64_t i = (slot * BB + t) * 64 + l;
         const double a = v64[i];
         const _Float16 h = (_Float16)(a / vscale);
-        v16[i] = h;
+        const int64_t o = (BB == 9) ? slot * 576 + (t < 8 ? (t >> 1) * 128 + 2 * l + (t & 1) : 512 + l) : i;
+        v16[o] = h;
         comp[t] += a - (double)h * vscale;
       }
     if (row < nb)

@@ -71,14 +71,34 @@ __device__ __forceinline__ void sell_row(int64_t row, int32_t nbrows, const int6
       xv[v] = (double)x[c * NV + v];
       if (ADDX32) xv[v] += (double)x32[c * NV + v];
     }
+    if constexpr (sizeof(VT) == 2 && BB == 9) {
+      // FP16 3x3 blocks: entries 2q, 2q+1 (q < 4) share a 32-bit word per
+      // lane, entry 8 on its own (5 loads per slot instead of 9)
+      double av[9];
+      const uint32_t* w = reinterpret_cast<const uint32_t*>(val + slot * 576);
 #pragma unroll
-    for (int d = 0; d < BS; ++d)
-#pragma unroll
-      for (int e = 0; e < BS; ++e) {
-        const double a = (double)__builtin_nontemporal_load(&val[(slot * BB + d * BS + e) * 64 + l]);
-#pragma unroll
-        for (int j = 0; j < K; ++j) sum[d * K + j] += a * xv[e * K + j];
+      for (int q = 0; q < 4; ++q) {
+        const uint32_t pw = __builtin_nontemporal_load(&w[q * 64 + l]);
+        av[2 * q] = (double)__builtin_bit_cast(_Float16, (uint16_t)(pw & 0xffffu));
+        av[2 * q + 1] = (double)__builtin_bit_cast(_Float16, (uint16_t)(pw >> 16));
       }
+      av[8] = (double)__builtin_nontemporal_load(&val[slot * 576 + 512 + l]);
+#pragma unroll
+      for (int d = 0; d < BS; ++d)
+#pragma unroll
+        for (int e = 0; e < BS; ++e)
+#pragma unroll
+          for (int j = 0; j < K; ++j) sum[d * K + j] += av[d * BS + e] * xv[e * K + j];
+    } else {
+#pragma unroll
+      for (int d = 0; d < BS; ++d)
+#pragma unroll
+        for (int e = 0; e < BS; ++e) {
+          const double a = (double)__builtin_nontemporal_load(&val[(slot * BB + d * BS + e) * 64 + l]);
+#pragma unroll
+          for (int j = 0; j < K; ++j) sum[d * K + j] += a * xv[e * K + j];
+        }
+    }
   };
   // FP16 values: more slots per trip keep as many bytes in flight as the
   // wider types do
