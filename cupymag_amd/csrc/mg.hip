@@ -450,6 +450,7 @@ static int mg_alloc(cpm_mg* M, void** p, size_t bytes) {
 // creation, grown on the first solve with more).
 int mg_reserve(cpm_mg* M, int K) {
   if (K <= M->kcap) return CPM_OK;
+  if (M->kcap == 0) return fail(CPM_ENOMEM, "multigrid: hierarchy unusable after a failed allocation");
   cpm_ctx* ctx = M->ctx;
   CPM_HIP(hipStreamSynchronize(ctx->stream));
   // captured chunks refer to the vectors replaced below
@@ -463,6 +464,9 @@ int mg_reserve(cpm_mg* M, int K) {
     *p = nullptr;
     return mg_alloc(M, p, bytes);
   };
+  // on a failure the vectors are left partly freed: the hierarchy refuses
+  // further solves (kcap = 0)
+  M->kcap = 0;
   for (MgLevel& Lv : M->L)
     for (void** p : {&Lv.x, &Lv.r, &Lv.b, &Lv.d0, &Lv.d1})
       if (int rc = regrow(p, ts * Lv.n * K)) return rc;
@@ -861,6 +865,7 @@ extern "C" int cpm_mg_info(const cpm_mg* M, int* nlevels, double* lmax, int max_
 
 extern "C" int cpm_mg_apply(cpm_mg* M, const double* r, double* z) {
   if (!M || !r || !z) return fail(CPM_EINVAL, "cpm_mg_apply: null argument");
+  if (M->kcap == 0) return fail(CPM_ENOMEM, "cpm_mg_apply: hierarchy unusable after a failed allocation");
   cpm_ctx* ctx = M->ctx;
   CPM_HIP(hipSetDevice(ctx->device));
   CgState* st = ctx->cg + 15;  // a live state so that the gated kernels run
