@@ -11,6 +11,12 @@
 #ifndef SELL_HALF_UNROLL
 #define SELL_HALF_UNROLL 7
 #endif
+#ifndef SELL_FLOAT_UNROLL
+#define SELL_FLOAT_UNROLL 7
+#endif
+#ifndef SELL_DOUBLE_UNROLL
+#define SELL_DOUBLE_UNROLL 7
+#endif
 
 namespace cpm {
 
@@ -100,9 +106,10 @@ __device__ __forceinline__ void sell_row(int64_t row, int32_t nbrows, const int6
         }
     }
   };
-  // FP16 values: more slots per trip keep as many bytes in flight as the
-  // wider types do
-  constexpr int U = (sizeof(VT) == 2) ? SELL_HALF_UNROLL : 2;
+  // slots per trip: loads of U slots in flight before their FMAs (7 = half
+  // the 14 neighbours of a Kuhn row; a sweep of 2..7 at 128^3 favoured 7 for
+  // every value type, most for FP16)
+  constexpr int U = (sizeof(VT) == 2) ? SELL_HALF_UNROLL : (sizeof(VT) == 4) ? SELL_FLOAT_UNROLL : SELL_DOUBLE_UNROLL;
   int64_t slot = s0;
   for (; slot + U - 1 < s1; slot += U) {
 #pragma unroll
