@@ -113,6 +113,16 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict(Dims3 fine, Dims3 coarse
   mg_restrict_rows<W, T>(blockIdx.x, gridDim.x, fine, coarse, anchor != 0, rf, bc);
 }
 
+template <int BS, int K, class T>
+__global__ void __launch_bounds__(kBlock) k_mg_restrict_first(Dims3 fine, Dims3 coarse, int anchor,
+                                                              const T* __restrict__ rf, T* __restrict__ bc,
+                                                              T* __restrict__ rc, T* __restrict__ dc,
+                                                              T* __restrict__ xc, const T* __restrict__ dinvc,
+                                                              double c0, const CgState* st) {
+  if (all_done<K>(st)) return;
+  mg_restrict_first_rows<BS, K, T>(blockIdx.x, gridDim.x, fine, coarse, anchor != 0, rf, bc, rc, dc, xc, dinvc, c0);
+}
+
 template <int W, int K, class T>
 __global__ void __launch_bounds__(kBlock) k_mg_prolong(Dims3 fine, Dims3 coarse, int anchor,
                                                        const T* __restrict__ ec, T* __restrict__ xf,
@@ -320,8 +330,11 @@ static void sweep(cpm_mg* M, const MgLevel& Lv, const void* din, void* dout, dou
 // vector b64.  ev (profiling, may be null): two event pairs, taken by the
 // first Chebyshev step of level 0 in the pre-smoothing (ev[0], ev[1]) and in
 // the post-smoothing (ev[2], ev[3]).
+// first_done: the first Chebyshev step of level l already ran (fused with
+// the restriction from level l-1).
 template <int BS, int K, class T>
-static void vcycle(cpm_mg* M, size_t l, const double* b64, const CgState* st, hipEvent_t* ev = nullptr) {
+static void vcycle(cpm_mg* M, size_t l, const double* b64, const CgState* st, hipEvent_t* ev = nullptr,
+                   bool first_done = false) {
   MgLevel& Lv = M->L[l];
   hipStream_t s = M->ls ? M->ls : M->ctx->stream;
   const int gn = grid_for(Lv.nb);
@@ -339,7 +352,7 @@ static void vcycle(cpm_mg* M, size_t l, const double* b64, const CgState* st, hi
   if (l == 0 && b64)
     k_mg_first<BS, K, T, true, true><<<gn, kBlock, 0, s>>>(Lv.nb, b64, (T*)Lv.b, (T*)Lv.r, (T*)d[0], (T*)Lv.x,
                                                            (const T*)Lv.dinv, c0, st);
-  else
+  else if (!first_done)
     k_mg_first<BS, K, T, true, false><<<gn, kBlock, 0, s>>>(Lv.nb, nullptr, (T*)Lv.b, (T*)Lv.r, (T*)d[0],
                                                             (T*)Lv.x, (const T*)Lv.dinv, c0, st);
   for (int k = 1; k < m; ++k)
@@ -348,8 +361,17 @@ static void vcycle(cpm_mg* M, size_t l, const double* b64, const CgState* st, hi
   // coarse-grid correction
   MgLevel& Lc = M->L[l + 1];
   const Dims3 fd{{Lv.dims[0], Lv.dims[1], Lv.dims[2]}}, cd{{Lc.dims[0], Lc.dims[1], Lc.dims[2]}};
-  k_mg_restrict<BS * K, K, T><<<grid_for(Lc.nb), kBlock, 0, s>>>(fd, cd, M->anchor, (const T*)Lv.r, (T*)Lc.b, st);
-  vcycle<BS, K, T>(M, l + 1, nullptr, st);
+  if (l + 2 < M->L.size()) {
+    // restriction fused with the coarse level's first step
+    k_mg_restrict_first<BS, K, T><<<grid_for(Lc.nb), kBlock, 0, s>>>(fd, cd, M->anchor, (const T*)Lv.r, (T*)Lc.b,
+                                                                    (T*)Lc.r, (T*)Lc.d0, (T*)Lc.x,
+                                                                    (const T*)Lc.dinv, M->c0[l + 1], st);
+    vcycle<BS, K, T>(M, l + 1, nullptr, st, nullptr, true);
+  } else {
+    k_mg_restrict<BS * K, K, T><<<grid_for(Lc.nb), kBlock, 0, s>>>(fd, cd, M->anchor, (const T*)Lv.r, (T*)Lc.b,
+                                                                  st);
+    vcycle<BS, K, T>(M, l + 1, nullptr, st);
+  }
   k_mg_prolong<BS * K, K, T><<<gn, kBlock, 0, s>>>(fd, cd, M->anchor, (const T*)Lc.x, (T*)Lv.x, st);
   // post-smoothing: the same polynomial on the new residual
   sweep<BS, K, T, 2>(M, Lv, nullptr, nullptr, 0.0, 0.0, st);

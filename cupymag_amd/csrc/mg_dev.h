@@ -158,11 +158,72 @@ __device__ __forceinline__ void kuhn_weights(int32_t i, int32_t j, int32_t k, co
   w[3] = t[p2];
 }
 
-// b_c = R r_f (R = P^T of the P1 interpolation); anchor: node 0 (the
-// operators' anchor) excluded on both grids.  Nested grids (fine = 2 x
-// coarse) use the fixed stencil; other size pairs gather, for every coarse
-// node, over the fine nodes whose cell is the node's own or the one below it
-// in each dimension.
+// (R r_f) at coarse node I (R = P^T of the P1 interpolation, W values per
+// node); anchor: node 0 (the operators' anchor) excluded on both grids.
+// Nested grids (fine = 2 x coarse) use the fixed stencil; other size pairs
+// gather over the fine nodes whose cell is the node's own or the one below
+// it in each dimension.
+template <int W, class T>
+__device__ __forceinline__ void restrict_node(int64_t I, const Dims3& fine, const Dims3& coarse, bool nested,
+                                              bool anchor, const T* __restrict__ rf, double (&acc)[W]) {
+#pragma unroll
+  for (int c = 0; c < W; ++c) acc[c] = 0.0;
+  if (anchor && I == 0) return;
+  const int32_t i = (int32_t)(I / ((int64_t)coarse.d[1] * coarse.d[2]));
+  const int32_t j = (int32_t)((I / coarse.d[2]) % coarse.d[1]);
+  const int32_t k = (int32_t)(I % coarse.d[2]);
+  if (nested) {
+    const int64_t f0 = wrap_idx(2 * i, 2 * j, 2 * k, fine.d);
+#pragma unroll
+    for (int c = 0; c < W; ++c) acc[c] = (anchor && f0 == 0) ? 0.0 : (double)rf[f0 * W + c];
+#pragma unroll
+    for (int dd = 1; dd < 8; ++dd) {
+      const int32_t a = dd & 1, bb = (dd >> 1) & 1, cc = (dd >> 2) & 1;
+      const int64_t fp = wrap_idx(2 * i + a, 2 * j + bb, 2 * k + cc, fine.d);
+      const int64_t fm = wrap_idx(2 * i - a, 2 * j - bb, 2 * k - cc, fine.d);
+#pragma unroll
+      for (int c = 0; c < W; ++c) {
+        if (!anchor || fp != 0) acc[c] += 0.5 * (double)rf[fp * W + c];
+        if (!anchor || fm != 0) acc[c] += 0.5 * (double)rf[fm * W + c];
+      }
+    }
+    return;
+  }
+  // per dimension: fine indices with cell in {C - 1, C}, i.e.
+  // ceil((C-1) nf / nc) .. ceil((C+1) nf / nc) - 1 (unwrapped)
+  const int32_t C3[3] = {i, j, k};
+  int32_t lo[3], hi[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int64_t nf = fine.d[a], ncd = coarse.d[a];
+    auto ceil_div = [](int64_t num, int64_t den) -> int64_t {
+      return num >= 0 ? (num + den - 1) / den : -((-num) / den);
+    };
+    lo[a] = (int32_t)ceil_div((int64_t)(C3[a] - 1) * nf, ncd);
+    hi[a] = (int32_t)ceil_div((int64_t)(C3[a] + 1) * nf, ncd) - 1;
+    if (hi[a] - lo[a] + 1 > nf) hi[a] = lo[a] + (int32_t)nf - 1;
+  }
+  for (int32_t fi = lo[0]; fi <= hi[0]; ++fi)
+    for (int32_t fj = lo[1]; fj <= hi[1]; ++fj)
+      for (int32_t fk = lo[2]; fk <= hi[2]; ++fk) {
+        const int32_t wi = fi < 0 ? fi + fine.d[0] : (fi >= fine.d[0] ? fi - fine.d[0] : fi);
+        const int32_t wj = fj < 0 ? fj + fine.d[1] : (fj >= fine.d[1] ? fj - fine.d[1] : fj);
+        const int32_t wk = fk < 0 ? fk + fine.d[2] : (fk >= fine.d[2] ? fk - fine.d[2] : fk);
+        const int64_t f = ((int64_t)wi * fine.d[1] + wj) * fine.d[2] + wk;
+        if (anchor && f == 0) continue;
+        int64_t v[4];
+        double w[4];
+        kuhn_weights(wi, wj, wk, fine, coarse, v, w);
+        double wt = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) wt += (v[q] == I) ? w[q] : 0.0;
+        if (wt == 0.0) continue;
+#pragma unroll
+        for (int c = 0; c < W; ++c) acc[c] += wt * (double)rf[f * W + c];
+      }
+}
+
+// b_c = R r_f over the coarse nodes of a workgroup range.
 template <int BS, class T>
 __device__ __forceinline__ void mg_restrict_rows(int64_t lb, int64_t lg, const Dims3& fine, const Dims3& coarse,
                                                  bool anchor, const T* __restrict__ rf, T* __restrict__ bc) {
@@ -171,64 +232,41 @@ __device__ __forceinline__ void mg_restrict_rows(int64_t lb, int64_t lg, const D
   const bool nested = nested_dims(fine, coarse);
   for (int64_t I = lb * kBlock + threadIdx.x; I < nc; I += stride) {
     double acc[BS];
-#pragma unroll
-    for (int c = 0; c < BS; ++c) acc[c] = 0.0;
-    if (!anchor || I != 0) {
-      const int32_t i = (int32_t)(I / ((int64_t)coarse.d[1] * coarse.d[2]));
-      const int32_t j = (int32_t)((I / coarse.d[2]) % coarse.d[1]);
-      const int32_t k = (int32_t)(I % coarse.d[2]);
-      if (nested) {
-        const int64_t f0 = wrap_idx(2 * i, 2 * j, 2 * k, fine.d);
-#pragma unroll
-        for (int c = 0; c < BS; ++c) acc[c] = (anchor && f0 == 0) ? 0.0 : (double)rf[f0 * BS + c];
-#pragma unroll
-        for (int dd = 1; dd < 8; ++dd) {
-          const int32_t a = dd & 1, bb = (dd >> 1) & 1, cc = (dd >> 2) & 1;
-          const int64_t fp = wrap_idx(2 * i + a, 2 * j + bb, 2 * k + cc, fine.d);
-          const int64_t fm = wrap_idx(2 * i - a, 2 * j - bb, 2 * k - cc, fine.d);
-#pragma unroll
-          for (int c = 0; c < BS; ++c) {
-            if (!anchor || fp != 0) acc[c] += 0.5 * (double)rf[fp * BS + c];
-            if (!anchor || fm != 0) acc[c] += 0.5 * (double)rf[fm * BS + c];
-          }
-        }
-      } else {
-        // per dimension: fine indices with cell in {C - 1, C}, i.e.
-        // ceil((C-1) nf / nc) .. ceil((C+1) nf / nc) - 1 (unwrapped)
-        const int32_t C3[3] = {i, j, k};
-        int32_t lo[3], hi[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          const int64_t nf = fine.d[a], ncd = coarse.d[a];
-          auto ceil_div = [](int64_t num, int64_t den) -> int64_t {
-            return num >= 0 ? (num + den - 1) / den : -((-num) / den);
-          };
-          lo[a] = (int32_t)ceil_div((int64_t)(C3[a] - 1) * nf, ncd);
-          hi[a] = (int32_t)ceil_div((int64_t)(C3[a] + 1) * nf, ncd) - 1;
-          if (hi[a] - lo[a] + 1 > nf) hi[a] = lo[a] + (int32_t)nf - 1;
-        }
-        for (int32_t fi = lo[0]; fi <= hi[0]; ++fi)
-          for (int32_t fj = lo[1]; fj <= hi[1]; ++fj)
-            for (int32_t fk = lo[2]; fk <= hi[2]; ++fk) {
-              const int32_t wi = fi < 0 ? fi + fine.d[0] : (fi >= fine.d[0] ? fi - fine.d[0] : fi);
-              const int32_t wj = fj < 0 ? fj + fine.d[1] : (fj >= fine.d[1] ? fj - fine.d[1] : fj);
-              const int32_t wk = fk < 0 ? fk + fine.d[2] : (fk >= fine.d[2] ? fk - fine.d[2] : fk);
-              const int64_t f = ((int64_t)wi * fine.d[1] + wj) * fine.d[2] + wk;
-              if (anchor && f == 0) continue;
-              int64_t v[4];
-              double w[4];
-              kuhn_weights(wi, wj, wk, fine, coarse, v, w);
-              double wt = 0.0;
-#pragma unroll
-              for (int q = 0; q < 4; ++q) wt += (v[q] == I) ? w[q] : 0.0;
-              if (wt == 0.0) continue;
-#pragma unroll
-              for (int c = 0; c < BS; ++c) acc[c] += wt * (double)rf[f * BS + c];
-            }
-      }
-    }
+    restrict_node<BS, T>(I, fine, coarse, nested, anchor, rf, acc);
 #pragma unroll
     for (int c = 0; c < BS; ++c) bc[I * BS + c] = (T)acc[c];
+  }
+}
+
+// The restriction fused with the first (x = 0) Chebyshev step of the coarse
+// level: b_c = R r_f, r_c = b_c, d_c = c0 D_c^-1 r_c, x_c = d_c -- the same
+// arithmetic as mg_restrict_rows followed by mg_first_rows<ZERO>.
+template <int BS, int K, class T>
+__device__ __forceinline__ void mg_restrict_first_rows(int64_t lb, int64_t lg, const Dims3& fine,
+                                                       const Dims3& coarse, bool anchor, const T* __restrict__ rf,
+                                                       T* __restrict__ bc, T* __restrict__ rc, T* __restrict__ dc,
+                                                       T* __restrict__ xc, const T* __restrict__ dinvc, double c0) {
+  constexpr int NV = BS * K;
+  const int64_t nc = (int64_t)coarse.d[0] * coarse.d[1] * coarse.d[2];
+  const int64_t stride = lg * kBlock;
+  const bool nested = nested_dims(fine, coarse);
+  for (int64_t I = lb * kBlock + threadIdx.x; I < nc; I += stride) {
+    double acc[NV], rv[NV], dv[NV];
+    restrict_node<NV, T>(I, fine, coarse, nested, anchor, rf, acc);
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+      const T bt = (T)acc[c];
+      bc[I * NV + c] = bt;
+      rc[I * NV + c] = bt;
+      rv[c] = (double)bt;
+    }
+    apply_dinv<BS, K, T>(dinvc, I, rv, dv);
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+      const T dn = (T)(c0 * dv[c]);
+      dc[I * NV + c] = dn;
+      xc[I * NV + c] = dn;
+    }
   }
 }
 

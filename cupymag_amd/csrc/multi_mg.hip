@@ -57,6 +57,10 @@ struct SmoothSeg {
   double vscale;      // FP16 values: their scale
   int32_t anchor;     // node 0 left out of the transfers
   void *bc, *xc;      // next level's b and x
+  void *rc, *dc;      // next level's r and d0 (fused first step)
+  const void* dinvc;  // next level's D^-1
+  double c0c;         // next level's first Chebyshev coefficient
+  int32_t fuse;       // the next level is not the coarsest: restriction + its first step
   const CgState* st;
   int32_t blk0, nblk;
 };
@@ -102,12 +106,17 @@ __global__ void __launch_bounds__(kBlock) kbm_sweep(const SmoothSeg* __restrict_
                                              (T*)d.x, (T*)d.r, (const T*)d.b, (const T*)d.dinv, c1, c2, d.vscale);
 }
 
-template <int W, int K, class T>
+template <int BS, int K, class T>
 __global__ void __launch_bounds__(kBlock) kbm_restrict(const SmoothSeg* __restrict__ seg,
                                                        const int32_t* __restrict__ map) {
   const SmoothSeg& d = seg[map[blockIdx.x]];
   if (all_done<K>(d.st)) return;
-  mg_restrict_rows<W, T>((int64_t)blockIdx.x - d.blk0, d.nblk, d.fd, d.cd, d.anchor != 0, (const T*)d.r, (T*)d.bc);
+  const int64_t lb = (int64_t)blockIdx.x - d.blk0;
+  if (d.fuse)
+    mg_restrict_first_rows<BS, K, T>(lb, d.nblk, d.fd, d.cd, d.anchor != 0, (const T*)d.r, (T*)d.bc, (T*)d.rc,
+                                     (T*)d.dc, (T*)d.xc, (const T*)d.dinvc, d.c0c);
+  else
+    mg_restrict_rows<BS * K, T>(lb, d.nblk, d.fd, d.cd, d.anchor != 0, (const T*)d.r, (T*)d.bc);
 }
 
 template <int W, int K, class T>
@@ -364,6 +373,9 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
         g.cd = Dims3{{Lc.dims[0], Lc.dims[1], Lc.dims[2]}};
         g.bc = Lc.b, g.xc = Lc.x;
         g.anchor = M->anchor;
+        g.fuse = (d + 1 < last) ? 1 : 0;
+        g.rc = Lc.r, g.dc = Lc.d0, g.dinvc = Lc.dinv;
+        g.c0c = (d + 1 < last) ? M->c0[d + 1] : 0.0;
         g.st = W.st + j * K;
         g.blk0 = (int32_t)Sm.map.size();
         g.nblk = sell_grid(S);
@@ -404,13 +416,11 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
     for (int d = 0; d < depth; ++d) {
       auto& Sm = W.smooth[d];
       if (Sm.G) {
-        if (d == 0)
-          kbm_first<BS, K, T, true, true><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap);
-        else
-          kbm_first<BS, K, T, true, false><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap);
+        // depth > 0: the first step ran fused with the restriction
+        if (d == 0) kbm_first<BS, K, T, true, true><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap);
         for (int k = 1; k < m; ++k) kbm_sweep<BS, K, T, 0, VT><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap, k);
         kbm_sweep<BS, K, T, 1, VT><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap, m - 1);
-        kbm_restrict<BS * K, K, T><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap);
+        kbm_restrict<BS, K, T><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap);
       }
       auto& Dn = W.dense[d + 1];
       if (Dn.G) kbm_dense<K, T><<<Dn.G, kBlock, 0, s>>>(Dn.d, Dn.dmap);
