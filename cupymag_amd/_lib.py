@@ -44,7 +44,7 @@ class cpm_sim_params(C.Structure):
                 ("Nd", C.c_double * 3), ("Ebar_sigma", C.c_double * 6), ("R", C.c_double * 9),
                 ("Mv", C.c_double * 36), ("MvinvT", C.c_double * 36), ("mode", C.c_int32),
                 ("warm_start", C.c_int32), ("rel_delta", C.c_double), ("precond", C.c_int32),
-                ("reserved", C.c_int32 * 3), ("rel_delta_gs", C.c_double)]
+                ("reserved", C.c_int32 * 3), ("rel_delta_gs", C.c_double), ("rel_delta_mg", C.c_double)]
 
 
 class cpm_step_report(C.Structure):
@@ -97,7 +97,7 @@ _SIGS = {
     "cpm_mg_destroy": (C.c_int, [_P]),
     "cpm_mg_info": (C.c_int, [_P, C.POINTER(C.c_int), _P, C.c_int]),
     "cpm_mg_apply": (C.c_int, [_P, _P, _P]),
-    "cpm_cg_mg": (C.c_int, [_P, C.c_int, _P, _P, C.c_int, _D, C.c_int, _P, _P]),
+    "cpm_cg_mg": (C.c_int, [_P, C.c_int, _P, _P, C.c_int, _D, C.c_int, _D, _P, _P]),
     "cpm_mg_set_graphs": (C.c_int, [_P, C.c_int]),
     "cpm_cg_mg_batch": (C.c_int, [_P, C.c_int, C.c_int, _P, _P, _P, _P, _D, C.c_int, _P, _P, _P]),
     "cpm_fel_create": (C.c_int, [_P, _I32, _P, _P, C.POINTER(_P)]),

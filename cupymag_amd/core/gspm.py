@@ -32,7 +32,7 @@ class StepReport:
 
 class GSPMSimulation:
     def __init__(self, SimOp, P, mode="fp64", rel_delta=0.3, warm_start=False, precond=None, rel_delta_gs=0.01,
-                 mg_gs=True, mg_degree=2, mg_ratio=10.0, mg_precision=None):
+                 mg_gs=True, mg_degree=2, mg_ratio=10.0, mg_precision=None, rel_delta_mg=1e-3):
         """mode: "fp64" (CSR operators, the reference's sequence of solves),
         "fast64" (SELL-64 operators, 3-RHS solves) or "mixed" (as fast64 with
         FP32 CG refined in FP64 by reliable updates: an FP64 residual update
@@ -50,7 +50,10 @@ class GSPMSimulation:
         (g1n..g3n as one 3-column solve, g1*, g2*); ``mg_degree`` and
         ``mg_ratio``: Chebyshev smoother degree and eigenvalue ratio;
         ``mg_precision``: V-cycle arithmetic (default 16 in the mixed mode:
-        FP32 vectors with FP16 operator values; 64 in fast64; 32 possible)."""
+        FP32 vectors with FP16 operator values; 64 in fast64; 32 possible);
+        ``rel_delta_mg`` (mixed mode): the multigrid solves' outer products
+        use the FP32 operator with an FP64 residual replacement whenever the
+        recurrence residual drops by this factor (0: FP64 products)."""
         self.P = P
         self.SimOp = SimOp
         A_demag, Fx, Fy, Fz, self.Deriv = SimOp.csr_demag_and_deriv()
@@ -108,6 +111,7 @@ class GSPMSimulation:
         pr.mode = _lib.MODES[mode]
         pr.rel_delta = float(rel_delta)
         pr.rel_delta_gs = float(rel_delta_gs or 0.0)
+        pr.rel_delta_mg = float(rel_delta_mg or 0.0)
         if warm_start and mode == "fp64":
             raise ValueError("warm_start is a fast-mode option")
         pr.warm_start = int(bool(warm_start))

@@ -204,11 +204,19 @@ __device__ inline double group_sum(double s) {
 // CG scalars after a reduction (thread 0).  v[j]: the reduced value of
 // column j; only the columns in use (live, or with an update requested for
 // RED_RR_TRUE) are touched.
-enum : int { RED_ALPHA = 0, RED_RR = 1, RED_RR_REL = 2, RED_RR_TRUE = 3, RED_RR_PCG = 4 };
+enum : int {
+  RED_ALPHA = 0,
+  RED_RR = 1,
+  RED_RR_REL = 2,
+  RED_RR_TRUE = 3,
+  RED_RR_PCG = 4,
+  RED_RR_PCG_REL = 5,   // preconditioned CG with reliable updates: residual norm, update request
+  RED_RR_PCG_TRUE = 6   // ... the FP64 residual replaced the recurrence residual
+};
 
 template <int MODE>
 __device__ inline bool cg_state_uses(const CgState& c) {
-  return (MODE == RED_RR_TRUE) ? c.upd != 0 : !c.done;
+  return (MODE == RED_RR_TRUE || MODE == RED_RR_PCG_TRUE) ? c.upd != 0 : !c.done;
 }
 
 template <int K, int MODE>
@@ -242,6 +250,23 @@ __device__ inline void cg_state_update(CgState* st, const double (&v)[K], double
       // preconditioned CG: the residual norm only (rho = r.z comes later)
       c.it += 1;
       c.resid = sqrt(v[j]);
+      if (c.resid <= c.atol) {
+        c.done = 1;
+      } else if (c.it >= c.maxiter) {
+        c.done = 1;
+        c.fail = 1;
+      }
+    } else if (MODE == RED_RR_PCG_REL) {
+      // recurrence residual of a PCG whose operator products are inexact:
+      // an FP64 residual is requested when it has dropped by delta since
+      // the last one, or meets the tolerance, or at maxiter
+      c.it += 1;
+      c.resid = sqrt(v[j]);
+      if (c.resid < delta * c.maxr || c.resid <= c.atol || c.it >= c.maxiter) c.upd = 1;
+    } else if (MODE == RED_RR_PCG_TRUE) {
+      c.upd = 0;
+      c.resid = sqrt(v[j]);
+      c.maxr = c.resid;
       if (c.resid <= c.atol) {
         c.done = 1;
       } else if (c.it >= c.maxiter) {

@@ -535,6 +535,8 @@ extern "C" int cpm_sim_create(cpm_ctx* ctx, const cpm_sim_ops* ops, const cpm_si
       return fail(CPM_EINVAL, "cpm_sim_create: rel_delta must be in (0, 1)");
     if (prm->mode == CPM_MODE_MIXED && !(prm->rel_delta_gs >= 0.0 && prm->rel_delta_gs < 1.0))
       return fail(CPM_EINVAL, "cpm_sim_create: rel_delta_gs must be in [0, 1)");
+    if (prm->mode == CPM_MODE_MIXED && !(prm->rel_delta_mg >= 0.0 && prm->rel_delta_mg < 1.0))
+      return fail(CPM_EINVAL, "cpm_sim_create: rel_delta_mg must be in [0, 1)");
     if (prm->precond & ~(CPM_PRECOND_EL_DEMAG | CPM_PRECOND_GS))
       return fail(CPM_EINVAL, "cpm_sim_create: unknown precond bits");
     const bool pe = (prm->precond & CPM_PRECOND_EL_DEMAG) != 0, pg = (prm->precond & CPM_PRECOND_GS) != 0;
@@ -853,7 +855,8 @@ static int solve_one(cpm_sim* S, const SolveSpec& sp, int use_init, cpm_step_rep
   for (int j = 0; j < sp.K; ++j) hint = std::max(hint, S->hint[sp.slot + j]);
   const int x0 = use_x0_fast(S, use_init);
   cpm_mg* mg = solve_mg(S, sp);
-  int rc = mg ? mg_pcg(mg, sp.K, bb, xx, x0, p.tol, p.maxiter, hint, it, res, &failed)
+  const double dmg = (p.mode == CPM_MODE_MIXED) ? p.rel_delta_mg : 0.0;
+  int rc = mg ? mg_pcg(mg, sp.K, bb, xx, x0, p.tol, p.maxiter, dmg, hint, it, res, &failed)
            : (p.mode == CPM_MODE_MIXED)
                ? sell_cg_mixed(ctx, Am, sp.K, bb, xx, x0, p.tol, p.maxiter, solve_delta(p, sp.slot), hint, it, res,
                                &failed,
@@ -1048,6 +1051,7 @@ extern "C" int cpm_batch_step(cpm_batch* B, const double* H, const int32_t* use_
         y.use_x0 = use_x0_fast(S, use_init[j]);
         y.tol = p.tol;
         y.maxiter = p.maxiter;
+        y.delta = (p.mode == CPM_MODE_MIXED) ? p.rel_delta_mg : 0.0;
         mgq.emplace_back(j, y);
         mgK = sp[j].K;
         for (int k = 0; k < mgK; ++k) mghint = std::max(mghint, S->hint[sp[j].slot + k]);

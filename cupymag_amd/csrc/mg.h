@@ -66,8 +66,10 @@ namespace cpm {
 // PCG on the hierarchy's fine operator for K interleaved columns (K = 1, or
 // 3 for a scalar operator), on its context's stream; iters and resid hold K
 // entries.
-int mg_pcg(cpm_mg* M, int K, const double* b, double* x, int use_x0, double tol, int maxiter, int hint,
-           int32_t* iters, double* resid, int* failed);
+// delta > 0: FP32 operator products with FP64 residual replacements (the
+// stopping test on the FP64 residual); 0: FP64 products.
+int mg_pcg(cpm_mg* M, int K, const double* b, double* x, int use_x0, double tol, int maxiter, double delta,
+           int hint, int32_t* iters, double* resid, int* failed);
 // grow the level and PCG vectors of M to K columns (synchronises)
 int mg_reserve(cpm_mg* M, int K);
 // the fine operator a hierarchy was built on

@@ -169,10 +169,11 @@ __global__ void __launch_bounds__(kBlock) k_scale(int64_t n, const double* __res
 // ---- outer preconditioned CG (FP64) ------------------------------------------
 // K columns interleaved (entry i belongs to column i % K, state st[i % K]).
 
-// q = A p (FP64 SELL form) and partials of p_j.q_j; PLAIN: y = A x, no partials.
-template <int BS, int K, bool ST, bool PLAIN>
+// q = A p (SELL form with VT values: FP64, or FP32 with compensated FP64
+// diagonals) and partials of p_j.q_j; PLAIN: y = A x, no partials.
+template <int BS, int K, bool ST, bool PLAIN, class VT = double>
 __global__ void __launch_bounds__(kBlock) k_pcg_spmv(int32_t nb, const int64_t* __restrict__ sptr,
-                                                     const int32_t* __restrict__ col, const double* __restrict__ val,
+                                                     const int32_t* __restrict__ col, const VT* __restrict__ val,
                                                      const double* __restrict__ dg, GridArg ga,
                                                      const double* __restrict__ p, double* __restrict__ q,
                                                      const CgState* st, double* partials) {
@@ -180,8 +181,34 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(int32_t nb, const int64_t* 
   double acc[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) acc[j] = 0.0;
-  sell_rows<BS, K, double, !PLAIN, false, ST>(blockIdx.x, gridDim.x, nb, sptr, col, val, dg, p, nullptr, q, ga, acc);
+  sell_rows<BS, K, double, !PLAIN, false, ST, VT>(blockIdx.x, gridDim.x, nb, sptr, col, val, dg, p, nullptr, q, ga,
+                                                  acc);
   if (!PLAIN) block_partial<K>(acc, partials);
+}
+
+// reliable update, run when a column asked for one: y = A x (FP64 form)
+template <int BS, int K, bool ST>
+__global__ void __launch_bounds__(kBlock) k_pcg_true(int32_t nb, const int64_t* __restrict__ sptr,
+                                                     const int32_t* __restrict__ col, const double* __restrict__ val,
+                                                     const double* __restrict__ dg, GridArg ga,
+                                                     const double* __restrict__ x, double* __restrict__ y,
+                                                     const CgState* st) {
+  if (!any_upd<K>(st)) return;
+  double acc[K];
+  sell_rows<BS, K, double, false, false, ST>(blockIdx.x, gridDim.x, nb, sptr, col, val, dg, x, nullptr, y, ga, acc);
+}
+
+// ... then r = b - y on those columns and partials of their r.r
+template <int K>
+__global__ void __launch_bounds__(kBlock) k_pcg_rel_r(int64_t n, const double* __restrict__ b,
+                                                      const double* __restrict__ y, double* __restrict__ r,
+                                                      const CgState* st, double* partials) {
+  if (!any_upd<K>(st)) return;
+  double acc[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) acc[j] = 0.0;
+  pcg_rel_r_rows<K>(blockIdx.x, gridDim.x, n, st, b, y, r, acc);
+  block_partial<K>(acc, partials);
 }
 
 // r = b - y (use_x0) or x = 0, r = b; partials of ||b_j||^2, ||r_j||^2.
@@ -196,7 +223,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_init(int64_t n, const double* __
   block_partial<2 * K>(acc, partials);
 }
 
-template <int K>
+template <int K, bool REL>
 __global__ void __launch_bounds__(kRedBlock) k_pcg_red_init(int nblk, const double* __restrict__ partials,
                                                             CgState* st, double tol, int32_t maxiter) {
   __shared__ double lds[2 * K * (kRedBlock / 64)];
@@ -207,7 +234,7 @@ __global__ void __launch_bounds__(kRedBlock) k_pcg_red_init(int nblk, const doub
 #pragma unroll
     for (int j = 0; j < 2 * K; ++j) v[j] += partials[j * nblk + i];
   block_sum_t<2 * K, kRedBlock>(v, lds);
-  if (threadIdx.x == 0) cg_state_init<K, false>(st, v, tol, maxiter);
+  if (threadIdx.x == 0) cg_state_init<K, REL>(st, v, tol, maxiter);
 }
 
 template <int K>
@@ -503,9 +530,13 @@ int mg_reserve(cpm_mg* M, int K) {
 
 // Preconditioned CG on level 0 for K columns (see the file comment); column
 // j of b and x is entry i*K + j, with its own state ctx->cg[j].
+// delta > 0: the outer products use the FP32 form of the operator, with an
+// FP64 residual replacement whenever the recurrence residual has dropped by
+// delta since the last one (and before stopping: the stopping test is on
+// the FP64 residual); delta = 0: FP64 products.
 template <int BS, int K, class T>
-static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol, int maxiter, int hint,
-                   int32_t* iters, double* resid, int* failed) {
+static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol, int maxiter, double delta,
+                   int hint, int32_t* iters, double* resid, int* failed) {
   cpm_ctx* ctx = M->ctx;
   hipStream_t s = ctx->stream;  // launch stream of the lambdas below
   MgLevel& L0 = M->L[0];
@@ -513,7 +544,17 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
   const int64_t n = L0.n * K;
   const int gs = sell_grid(S), gv = grid_for(n);
   CgState* st = ctx->cg;
+  const bool rel = delta > 0.0;
   auto spmv = [&](const double* in, double* out, bool plain) {
+    if (!plain && rel) {
+      if (S.stencil)
+        k_pcg_spmv<BS, K, true, false, float><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v32, S.dgc,
+                                                                    grid_arg(S), in, out, st, ctx->partials);
+      else
+        k_pcg_spmv<BS, K, false, false, float><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v32, S.dgc,
+                                                                     grid_arg(S), in, out, st, ctx->partials);
+      return;
+    }
     if (plain) {
       if (S.stencil)
         k_pcg_spmv<BS, K, true, true><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), in,
@@ -533,7 +574,10 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
   // start: r = b - A x0 (or b), state, z = M r, rho = r.z
   if (use_x0) spmv(x, M->q, true);
   k_pcg_init<K><<<gv, kBlock, 0, s>>>(n, b, x, M->q, M->r, use_x0, ctx->partials);
-  k_pcg_red_init<K><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st, tol, maxiter);
+  if (rel)
+    k_pcg_red_init<K, true><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st, tol, maxiter);
+  else
+    k_pcg_red_init<K, false><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st, tol, maxiter);
   if (use_x0) k_pcg_zero_b<K><<<gv, kBlock, 0, s>>>(n, x, st);
   vcycle<BS, K, T>(M, 0, M->r, st);
   k_pcg_dot<K, T><<<gv, kBlock, 0, s>>>(n, M->r, M->rold, (const T*)L0.x, st, ctx->partials);
@@ -550,7 +594,19 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
       spmv(M->p, M->q, false);
       k_red<K, RED_ALPHA><<<1, kRedBlock, 0, s>>>(gs, ctx->partials, st, 0.0);
       k_pcg_xr<K><<<gv, kBlock, 0, s>>>(n, x, M->r, M->rold, M->p, M->q, st, ctx->partials);
-      k_red<K, RED_RR_PCG><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st, 0.0);
+      if (rel) {
+        k_red<K, RED_RR_PCG_REL><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st, delta);
+        if (S.stencil)
+          k_pcg_true<BS, K, true><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), x, M->q,
+                                                        st);
+        else
+          k_pcg_true<BS, K, false><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), x,
+                                                         M->q, st);
+        k_pcg_rel_r<K><<<gv, kBlock, 0, s>>>(n, b, M->q, M->r, st, ctx->partials);
+        k_red<K, RED_RR_PCG_TRUE><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st, 0.0);
+      } else {
+        k_red<K, RED_RR_PCG><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st, 0.0);
+      }
       vcycle<BS, K, T>(M, 0, M->r, st, (k == 0) ? ev : nullptr);
       k_pcg_dot<K, T><<<gv, kBlock, 0, s>>>(n, M->r, M->rold, (const T*)L0.x, st, ctx->partials);
       k_pcg_red_rz<K><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st);
@@ -565,9 +621,11 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
   hipGraphExec_t gexec = nullptr;
   if (M->graphs) {
     uint64_t key = 1469598103934665603ull;
-    for (uint64_t v : {(uint64_t)(uintptr_t)x, (uint64_t)(uintptr_t)M->r, (uint64_t)(uintptr_t)L0.x,
-                       (uint64_t)(uintptr_t)st, (uint64_t)(uintptr_t)ctx->partials, (uint64_t)K,
-                       (uint64_t)sizeof(T), (uint64_t)chunk})
+    uint64_t dbits;
+    std::memcpy(&dbits, &delta, sizeof dbits);
+    for (uint64_t v : {(uint64_t)(uintptr_t)x, (uint64_t)(uintptr_t)b, (uint64_t)(uintptr_t)M->r,
+                       (uint64_t)(uintptr_t)L0.x, (uint64_t)(uintptr_t)st, (uint64_t)(uintptr_t)ctx->partials,
+                       (uint64_t)K, (uint64_t)sizeof(T), (uint64_t)chunk, dbits})
       key = (key ^ v) * 1099511628211ull;
     for (auto& c : M->chunks)
       if (c.key == key) gexec = c.exec;
@@ -660,19 +718,21 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
 
 const cpm_mat* mg_operator(const cpm_mg* M) { return M && !M->L.empty() ? M->L[0].A : nullptr; }
 
-int mg_pcg(cpm_mg* M, int K, const double* b, double* x, int use_x0, double tol, int maxiter, int hint,
-           int32_t* iters, double* resid, int* failed) {
+int mg_pcg(cpm_mg* M, int K, const double* b, double* x, int use_x0, double tol, int maxiter, double delta,
+           int hint, int32_t* iters, double* resid, int* failed) {
   if (K != 1 && !(K == 3 && M->bs == 1))
     return fail(CPM_EINVAL, "multigrid CG: 1 right-hand side, or 3 for a scalar operator");
+  if (!(delta >= 0.0 && delta < 1.0)) return fail(CPM_EINVAL, "multigrid CG: delta must be in [0, 1)");
   if (int rc = mg_reserve(M, K)) return rc;
+  const double d = delta;
   if (M->bs == 3)
-    return M->prec == 32 ? pcg_run<3, 1, float>(M, b, x, use_x0, tol, maxiter, hint, iters, resid, failed)
-                         : pcg_run<3, 1, double>(M, b, x, use_x0, tol, maxiter, hint, iters, resid, failed);
+    return M->prec == 32 ? pcg_run<3, 1, float>(M, b, x, use_x0, tol, maxiter, d, hint, iters, resid, failed)
+                         : pcg_run<3, 1, double>(M, b, x, use_x0, tol, maxiter, d, hint, iters, resid, failed);
   if (K == 3)
-    return M->prec == 32 ? pcg_run<1, 3, float>(M, b, x, use_x0, tol, maxiter, hint, iters, resid, failed)
-                         : pcg_run<1, 3, double>(M, b, x, use_x0, tol, maxiter, hint, iters, resid, failed);
-  return M->prec == 32 ? pcg_run<1, 1, float>(M, b, x, use_x0, tol, maxiter, hint, iters, resid, failed)
-                       : pcg_run<1, 1, double>(M, b, x, use_x0, tol, maxiter, hint, iters, resid, failed);
+    return M->prec == 32 ? pcg_run<1, 3, float>(M, b, x, use_x0, tol, maxiter, d, hint, iters, resid, failed)
+                         : pcg_run<1, 3, double>(M, b, x, use_x0, tol, maxiter, d, hint, iters, resid, failed);
+  return M->prec == 32 ? pcg_run<1, 1, float>(M, b, x, use_x0, tol, maxiter, d, hint, iters, resid, failed)
+                       : pcg_run<1, 1, double>(M, b, x, use_x0, tol, maxiter, d, hint, iters, resid, failed);
 }
 
 }  // namespace cpm
@@ -911,11 +971,11 @@ extern "C" int cpm_mg_apply(cpm_mg* M, const double* r, double* z) {
 }
 
 extern "C" int cpm_cg_mg(cpm_mg* M, int nrhs, const double* b, double* x, int use_x0, double tol, int maxiter,
-                         int32_t* iters, double* resid) {
+                         double delta, int32_t* iters, double* resid) {
   if (!M || !b || !x || !iters || !resid) return fail(CPM_EINVAL, "cpm_cg_mg: null argument");
   CPM_HIP(hipSetDevice(M->ctx->device));
   int failed = 0;
-  if (int rc = mg_pcg(M, nrhs, b, x, use_x0, tol, maxiter, 0, iters, resid, &failed)) return rc;
+  if (int rc = mg_pcg(M, nrhs, b, x, use_x0, tol, maxiter, delta, 0, iters, resid, &failed)) return rc;
   int worst = 0;
   for (int j = 0; j < nrhs; ++j) worst = std::max(worst, (int)iters[j]);
   return failed ? std::max(1, worst) : CPM_OK;

@@ -420,6 +420,34 @@ __device__ __forceinline__ void pcg_pupd_rows(int64_t lb, int64_t lg, int64_t n,
   }
 }
 
+// FP64 residual replacement (reliable update) on the columns that asked
+// for it: r = b - y (y = A x with the FP64 operator); acc[j] += r_j.r_j.
+template <int K>
+__device__ __forceinline__ void pcg_rel_r_rows(int64_t lb, int64_t lg, int64_t n, const CgState* st,
+                                               const double* __restrict__ b, const double* __restrict__ y,
+                                               double* __restrict__ r, double (&acc)[K]) {
+  bool upd[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) upd[j] = st[j].upd != 0;
+  const int64_t stride = lg * kBlock;
+  for (int64_t i = lb * kBlock + threadIdx.x; i < n; i += stride) {
+    const int j = (int)(i % K);
+    if (!upd[j]) continue;
+    const double ri = b[i] - y[i];
+    r[i] = ri;
+    acc[j] += ri * ri;
+  }
+}
+
+// some column asked for a reliable update
+template <int K>
+__device__ __forceinline__ bool any_upd(const CgState* st) {
+  bool u = false;
+#pragma unroll
+  for (int j = 0; j < K; ++j) u = u || st[j].upd != 0;
+  return u;
+}
+
 // rho = r.z, beta = (r.z - r_old.z) / rho_old (flexible CG), from the sums v
 __device__ inline void pcg_state_rz(CgState& c, double rz, double rold_z) {
   c.rho_old = c.rho;

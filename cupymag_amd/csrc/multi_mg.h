@@ -16,6 +16,7 @@ struct MultiMgSys {
   int use_x0 = 0;
   double tol = 1e-7;
   int32_t maxiter = 5000;
+  double delta = 0.0;  // > 0: FP32 products with FP64 residual replacements (see mg_pcg)
 };
 
 struct MultiMgTables;

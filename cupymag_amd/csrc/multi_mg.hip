@@ -28,6 +28,8 @@ struct OuterSeg {
   const int32_t* col;
   const double* v64;
   const double* dg64;
+  const float* v32;      // FP32 form (reliable-update systems)
+  const double* dgc;
   GridArg ga;
   int32_t nb, stencil;
   int64_t n;
@@ -40,6 +42,7 @@ struct OuterSeg {
   int32_t sblk0, snblk;  // SpMV: the operator's SELL grid
   int32_t use_x0, maxiter;
   double tol;
+  double delta;          // > 0: FP32 products with FP64 residual replacements (mg.hip)
 };
 
 struct SmoothSeg {
@@ -145,13 +148,49 @@ __global__ void __launch_bounds__(kBlock) kbm_spmv(const OuterSeg* __restrict__ 
   for (int j = 0; j < K; ++j) acc[j] = 0.0;
   const int64_t lb = (int64_t)blockIdx.x - d.sblk0;
   const double* in = PLAIN ? d.x : d.p;
-  if (d.stencil)
+  if (!PLAIN && d.delta > 0.0) {
+    if (d.stencil)
+      sell_rows<BS, K, double, true, false, true, float>(lb, d.snblk, d.nb, d.sptr, d.col, d.v32, d.dgc, in, nullptr,
+                                                         d.q, d.ga, acc);
+    else
+      sell_rows<BS, K, double, true, false, false, float>(lb, d.snblk, d.nb, d.sptr, d.col, d.v32, d.dgc, in,
+                                                          nullptr, d.q, d.ga, acc);
+  } else if (d.stencil) {
     sell_rows<BS, K, double, !PLAIN, false, true>(lb, d.snblk, d.nb, d.sptr, d.col, d.v64, d.dg64, in, nullptr, d.q,
                                                   d.ga, acc);
-  else
+  } else {
     sell_rows<BS, K, double, !PLAIN, false, false>(lb, d.snblk, d.nb, d.sptr, d.col, d.v64, d.dg64, in, nullptr, d.q,
                                                    d.ga, acc);
+  }
   if (!PLAIN) block_partial<K>(acc, partials);
+}
+
+// reliable update of the systems that asked for one: q = A x (FP64 form)
+template <int BS, int K>
+__global__ void __launch_bounds__(kBlock) kbm_true(const OuterSeg* __restrict__ seg, const int32_t* __restrict__ map) {
+  const OuterSeg& d = seg[map[blockIdx.x]];
+  if (!any_upd<K>(d.st)) return;
+  double acc[K];
+  const int64_t lb = (int64_t)blockIdx.x - d.sblk0;
+  if (d.stencil)
+    sell_rows<BS, K, double, false, false, true>(lb, d.snblk, d.nb, d.sptr, d.col, d.v64, d.dg64, d.x, nullptr, d.q,
+                                                 d.ga, acc);
+  else
+    sell_rows<BS, K, double, false, false, false>(lb, d.snblk, d.nb, d.sptr, d.col, d.v64, d.dg64, d.x, nullptr, d.q,
+                                                  d.ga, acc);
+}
+
+// ... then r = b - q on their columns that asked, partials of r.r
+template <int K>
+__global__ void __launch_bounds__(kBlock) kbm_rel_r(const OuterSeg* __restrict__ seg, const int32_t* __restrict__ map,
+                                                    double* partials) {
+  const OuterSeg& d = seg[map[blockIdx.x]];
+  if (!any_upd<K>(d.st)) return;
+  double acc[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) acc[j] = 0.0;
+  pcg_rel_r_rows<K>((int64_t)blockIdx.x - d.blk0, d.nblk, d.n, d.st, d.b, d.q, d.r, acc);
+  block_partial<K>(acc, partials);
 }
 
 template <int K>
@@ -204,7 +243,7 @@ __global__ void __launch_bounds__(kBlock) kbm_pupd(const OuterSeg* __restrict__ 
   pcg_pupd_rows<K, T>((int64_t)blockIdx.x - d.blk0, d.nblk, d.n, d.st, (const T*)d.z, d.p);
 }
 
-enum : int { MRED_INIT = 0, MRED_ALPHA = 1, MRED_RR = 2, MRED_RZ = 3 };
+enum : int { MRED_INIT = 0, MRED_ALPHA = 1, MRED_RR = 2, MRED_RZ = 3, MRED_TRUE = 4 };
 
 // One workgroup per system: the system's partials in a fixed order (those
 // of its SpMV workgroups for MRED_ALPHA, of its vector workgroups otherwise),
@@ -214,7 +253,7 @@ __global__ void __launch_bounds__(kRedBlock) kbm_red(const OuterSeg* __restrict_
                                                      const double* __restrict__ partials) {
   constexpr int NV = (MODE == MRED_INIT || MODE == MRED_RZ) ? 2 * K : K;
   const OuterSeg& d = seg[blockIdx.x];
-  if (MODE != MRED_INIT && all_done<K>(d.st)) return;
+  if (MODE == MRED_TRUE ? !any_upd<K>(d.st) : (MODE != MRED_INIT && all_done<K>(d.st))) return;
   __shared__ double lds[NV * (kRedBlock / 64)];
   double v[NV];
 #pragma unroll
@@ -226,11 +265,19 @@ __global__ void __launch_bounds__(kRedBlock) kbm_red(const OuterSeg* __restrict_
   block_sum_t<NV, kRedBlock>(v, lds);
   if (threadIdx.x != 0) return;
   if constexpr (MODE == MRED_INIT) {
-    cg_state_init<K, false>(d.st, v, d.tol, d.maxiter);
+    if (d.delta > 0.0)
+      cg_state_init<K, true>(d.st, v, d.tol, d.maxiter);
+    else
+      cg_state_init<K, false>(d.st, v, d.tol, d.maxiter);
   } else if constexpr (MODE == MRED_ALPHA) {
     cg_state_update<K, RED_ALPHA>(d.st, v, 0.0);
   } else if constexpr (MODE == MRED_RR) {
-    cg_state_update<K, RED_RR_PCG>(d.st, v, 0.0);
+    if (d.delta > 0.0)
+      cg_state_update<K, RED_RR_PCG_REL>(d.st, v, d.delta);
+    else
+      cg_state_update<K, RED_RR_PCG>(d.st, v, 0.0);
+  } else if constexpr (MODE == MRED_TRUE) {
+    cg_state_update<K, RED_RR_PCG_TRUE>(d.st, v, 0.0);
   } else {
 #pragma unroll
     for (int j = 0; j < K; ++j)
@@ -330,7 +377,8 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
     cpm_mg* M = y.M;
     const Sell& S = M->L[0].A->sell;
     OuterSeg o{};
-    o.sptr = S.sptr, o.col = S.col, o.v64 = S.v64, o.dg64 = S.dg64;
+    o.sptr = S.sptr, o.col = S.col, o.v64 = S.v64, o.dg64 = S.dg64, o.v32 = S.v32, o.dgc = S.dgc;
+    o.delta = y.delta;
     o.ga = GridArg{S.dims[0], S.dims[1], S.dims[2], S.nst, S.soff};
     o.nb = S.nbrows, o.stencil = S.stencil, o.n = M->L[0].n * K;
     o.b = y.b, o.x = y.x, o.r = M->r, o.p = M->p, o.q = M->q, o.rold = M->rold, o.z = M->L[0].x;
@@ -408,9 +456,10 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
     CPM_HIP(hipMalloc(&W.partials, sizeof(double) * W.cap_part));
   }
   double* pt = W.partials;
-  bool any_x0 = false;
+  bool any_x0 = false, any_rel = false;
   int maxit = 0;
-  for (const auto& y : sys) any_x0 |= y.use_x0 != 0, maxit = std::max(maxit, (int)y.maxiter);
+  for (const auto& y : sys)
+    any_x0 |= y.use_x0 != 0, any_rel |= y.delta > 0.0, maxit = std::max(maxit, (int)y.maxiter);
 
   auto vcycle = [&]() {
     for (int d = 0; d < depth; ++d) {
@@ -451,6 +500,11 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
       kbm_red<K, MRED_ALPHA><<<ns, kRedBlock, 0, s>>>(O.d, Gs, pt);
       kbm_xr<K><<<G, kBlock, 0, s>>>(O.d, O.dmap, pt);
       kbm_red<K, MRED_RR><<<ns, kRedBlock, 0, s>>>(O.d, G, pt);
+      if (any_rel) {
+        kbm_true<BS, K><<<Gs, kBlock, 0, s>>>(O.d, W.dsmap);
+        kbm_rel_r<K><<<G, kBlock, 0, s>>>(O.d, O.dmap, pt);
+        kbm_red<K, MRED_TRUE><<<ns, kRedBlock, 0, s>>>(O.d, G, pt);
+      }
       vcycle();
       kbm_dot<K, T><<<G, kBlock, 0, s>>>(O.d, O.dmap, pt);
       kbm_red<K, MRED_RZ><<<ns, kRedBlock, 0, s>>>(O.d, G, pt);
@@ -504,6 +558,7 @@ int multi_mg_pcg(cpm_ctx* ctx, MultiMgSolver& W, const std::vector<MultiMgSys>& 
     if (y.M->bs != M0->bs || y.M->prec != M0->prec || y.M->half != M0->half || y.M->degree != M0->degree)
       return fail(CPM_EINVAL, "batched multigrid CG: hierarchies differ in block size, precision or degree");
     if (y.M->L.size() > 16) return fail(CPM_EINVAL, "batched multigrid CG: too many levels");
+    if (!(y.delta >= 0.0 && y.delta < 1.0)) return fail(CPM_EINVAL, "batched multigrid CG: delta must be in [0, 1)");
   }
   if (K != 1 && !(K == 3 && M0->bs == 1))
     return fail(CPM_EINVAL, "batched multigrid CG: 1 right-hand side, or 3 for scalar operators");

@@ -139,9 +139,9 @@ __device__ __forceinline__ void sell_row(int64_t row, int32_t nbrows, const int6
 
 // y = A x (ADDX32: A (x + x32)) over the block rows of a workgroup range; CG:
 // acc[j] += x.y of column j over the rows this thread owns.
-template <int BS, int K, class T, bool CG, bool ADDX32, bool ST>
+template <int BS, int K, class T, bool CG, bool ADDX32, bool ST, class VT = T>
 __device__ __forceinline__ void sell_rows(int64_t lb, int64_t lg, int32_t nbrows, const int64_t* __restrict__ sptr,
-                                          const int32_t* __restrict__ col, const T* __restrict__ val,
+                                          const int32_t* __restrict__ col, const VT* __restrict__ val,
                                           const double* __restrict__ dg, const T* __restrict__ x,
                                           const float* __restrict__ x32, T* __restrict__ y, const GridArg& ga,
                                           double (&acc)[K]) {
@@ -149,7 +149,7 @@ __device__ __forceinline__ void sell_rows(int64_t lb, int64_t lg, int32_t nbrows
   const int64_t stride = lg * kBlock;
   for (int64_t row = lb * kBlock + threadIdx.x; row < (int64_t)((nbrows + 63) & ~63); row += stride) {
     double sum[NV], xo[NV];
-    sell_row<BS, K, T, ADDX32, ST>(row, nbrows, sptr, col, val, dg, x, x32, ga, sum, xo);
+    sell_row<BS, K, T, ADDX32, ST, VT>(row, nbrows, sptr, col, val, dg, x, x32, ga, sum, xo);
     if (row < nbrows) {
 #pragma unroll
       for (int v = 0; v < NV; ++v) {

@@ -59,16 +59,18 @@ class KuhnMultigrid:
         check(lib().cpm_mg_apply(self.h, ptr(r), ptr(z)), "cpm_mg_apply")
         return z
 
-    def solve(self, b, x0=None, tol=1e-7, maxiter=5000, use_init=False, nrhs=1):
+    def solve(self, b, x0=None, tol=1e-7, maxiter=5000, use_init=False, nrhs=1, delta=0.0):
         """(x, iterations, residual) of the preconditioned CG; raises the
         reference's error when maxiter is reached.  nrhs = 3 (scalar
         operators): b holds three interleaved columns (entry i*3 + j) and
-        iterations / residual are lists of three."""
+        iterations / residual are lists of three.  delta > 0: FP32 operator
+        products with FP64 residual replacements (stopping test on the FP64
+        residual)."""
         b = as_device(b, self.ctx)
         x = x0.clone() if (use_init and x0 is not None) else torch.empty_like(b)
         it, res = (C.c_int32 * nrhs)(), (C.c_double * nrhs)()
         info = check(lib().cpm_cg_mg(self.h, int(nrhs), ptr(b), ptr(x), int(bool(use_init and x0 is not None)),
-                                     float(tol), int(maxiter), it, res), "cpm_cg_mg")
+                                     float(tol), int(maxiter), float(delta), it, res), "cpm_cg_mg")
         if info > 0:
             raise RuntimeError(f"Error! CG did not converge. info={info}, residual={max(res)}.")
         if nrhs == 1:

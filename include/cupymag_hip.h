@@ -235,8 +235,11 @@ int cpm_mg_apply(cpm_mg* mg, const double* r, double* z);
  * x = 0), flexible beta = (r - r_old).z / rho_old.  nrhs = 1, or 3 for a
  * scalar operator: b and x hold nrhs interleaved columns (entry i*nrhs + j),
  * each with its own scalars and stopping test; iters and resid hold nrhs
- * entries.  Returns 0, or a positive iteration count when maxiter was
- * reached. */
+ * entries.  delta > 0: the outer products use the FP32 form of the
+ * operator, with an FP64 residual replacement whenever the recurrence
+ * residual drops by delta and before stopping (the stopping test is then
+ * on the FP64 residual); 0: FP64 products.  Returns 0, or a positive
+ * iteration count when maxiter was reached. */
 /* No counterpart in the reference.  Iterations of cpm_cg_mg (and of the
  * multigrid solves of cpm_sim_step) are replayed from captured chunks
  * (hipGraph, 2 iterations each, cached per hierarchy for the vectors they
@@ -244,7 +247,7 @@ int cpm_mg_apply(cpm_mg* mg, const double* r, double* z);
  * bit for bit.  Default on. */
 int cpm_mg_set_graphs(cpm_mg* mg, int on);
 int cpm_cg_mg(cpm_mg* mg, int nrhs, const double* b, double* x, int use_x0, double tol, int maxiter,
-              int32_t* iters, double* resid);
+              double delta, int32_t* iters, double* resid);
 /* No counterpart in the reference.  cpm_cg_mg on `count` independent
  * systems at once (hierarchies of one block size, V-cycle precision and
  * smoother degree, each at most once, all on ctx) with nrhs columns each:
@@ -322,6 +325,10 @@ typedef struct cpm_sim_params {
   int32_t reserved[3];
   double rel_delta_gs;        /* CPM_MODE_MIXED: rel_delta of the Gauss-Seidel solves (A1, A2);
                                  0 = rel_delta */
+  double rel_delta_mg;        /* CPM_MODE_MIXED with precond: the multigrid solves' outer products use
+                                 the FP32 operator, with an FP64 residual replacement whenever the
+                                 recurrence residual drops by this factor and before stopping;
+                                 0 = FP64 products */
 } cpm_sim_params;
 
 #define CPM_MODE_FP64 0   /* plain FP64 path: CSR operators, the reference's sequence of solves */

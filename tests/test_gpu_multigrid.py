@@ -260,3 +260,29 @@ def test_graph_replay_matches_direct_launches(prec):
         assert i0 == i1
         np.testing.assert_array_equal(m0, m1)
         np.testing.assert_array_equal(u0, u1)
+
+
+@pytest.mark.parametrize("name,bs,nrhs", [("A_el", 3, 1), ("A_demag", 1, 1), ("A1", 1, 3)])
+def test_reliable_update_pcg_meets_the_fp64_tolerance(name, bs, nrhs):
+    """FP32 outer products with FP64 residual replacements (delta 1e-3): the
+    FP64 residual meets tol 1e-10 (the stopping test is on it), the solution
+    equals a sparse direct solve to 1e-8, within two iterations of the FP64
+    outer products."""
+    _, _, _, S = _setup(n=16, jitter=0.1)
+    mg = S.multigrid(precision=16)[name]
+    A = _interleaved(mg.levels[0], bs)
+    rng = np.random.default_rng(13)
+    B = rng.standard_normal((A.shape[0], nrhs))
+    if name != "A1":
+        B[:bs] = 0.0
+    b = torch.as_tensor(B.ravel(), device="cuda")
+    x64, it64, _ = mg.solve(b, tol=1e-10, maxiter=500, nrhs=nrhs)
+    x32, it32, res = mg.solve(b, tol=1e-10, maxiter=500, nrhs=nrhs, delta=1e-3)
+    X = x32.cpu().numpy().reshape(-1, nrhs)
+    it64 = it64 if nrhs > 1 else [it64]
+    it32 = it32 if nrhs > 1 else [it32]
+    for j in range(nrhs):
+        assert np.linalg.norm(B[:, j] - A @ X[:, j]) <= 1e-10 * np.linalg.norm(B[:, j])
+        xr = spla.spsolve(A.tocsc(), B[:, j])
+        assert np.abs(X[:, j] - xr).max() / np.abs(xr).max() < 1e-8
+        assert abs(it32[j] - it64[j]) <= 2, (it32, it64)
