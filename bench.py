@@ -497,7 +497,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default SpMV launch schedules")
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--cpu-n", type=int, default=40)
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain device copy of this size (calibrates rocprof byte counters)")
