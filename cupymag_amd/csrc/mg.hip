@@ -186,28 +186,19 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(int32_t nb, const int64_t* 
   if (!PLAIN) block_partial<K>(acc, partials);
 }
 
-// reliable update, run when a column asked for one: y = A x (FP64 form)
+// reliable update, run when a column asked for one: r = b - A x (FP64
+// form) on those columns, partials of their r.r (SpMV workgroup layout)
 template <int BS, int K, bool ST>
 __global__ void __launch_bounds__(kBlock) k_pcg_true(int32_t nb, const int64_t* __restrict__ sptr,
                                                      const int32_t* __restrict__ col, const double* __restrict__ val,
                                                      const double* __restrict__ dg, GridArg ga,
-                                                     const double* __restrict__ x, double* __restrict__ y,
-                                                     const CgState* st) {
-  if (!any_upd<K>(st)) return;
-  double acc[K];
-  sell_rows<BS, K, double, false, false, ST>(blockIdx.x, gridDim.x, nb, sptr, col, val, dg, x, nullptr, y, ga, acc);
-}
-
-// ... then r = b - y on those columns and partials of their r.r
-template <int K>
-__global__ void __launch_bounds__(kBlock) k_pcg_rel_r(int64_t n, const double* __restrict__ b,
-                                                      const double* __restrict__ y, double* __restrict__ r,
-                                                      const CgState* st, double* partials) {
+                                                     const double* __restrict__ x, const double* __restrict__ b,
+                                                     double* __restrict__ r, const CgState* st, double* partials) {
   if (!any_upd<K>(st)) return;
   double acc[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) acc[j] = 0.0;
-  pcg_rel_r_rows<K>(blockIdx.x, gridDim.x, n, st, b, y, r, acc);
+  pcg_true_rows<BS, K, ST>(blockIdx.x, gridDim.x, nb, sptr, col, val, dg, ga, x, b, r, st, acc);
   block_partial<K>(acc, partials);
 }
 
@@ -597,13 +588,12 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
       if (rel) {
         k_red<K, RED_RR_PCG_REL><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st, delta);
         if (S.stencil)
-          k_pcg_true<BS, K, true><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), x, M->q,
-                                                        st);
+          k_pcg_true<BS, K, true><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), x, b,
+                                                        M->r, st, ctx->partials);
         else
-          k_pcg_true<BS, K, false><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), x,
-                                                         M->q, st);
-        k_pcg_rel_r<K><<<gv, kBlock, 0, s>>>(n, b, M->q, M->r, st, ctx->partials);
-        k_red<K, RED_RR_PCG_TRUE><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st, 0.0);
+          k_pcg_true<BS, K, false><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), x, b,
+                                                         M->r, st, ctx->partials);
+        k_red<K, RED_RR_PCG_TRUE><<<1, kRedBlock, 0, s>>>(gs, ctx->partials, st, 0.0);
       } else {
         k_red<K, RED_RR_PCG><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st, 0.0);
       }

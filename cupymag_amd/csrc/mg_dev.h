@@ -420,22 +420,32 @@ __device__ __forceinline__ void pcg_pupd_rows(int64_t lb, int64_t lg, int64_t n,
   }
 }
 
-// FP64 residual replacement (reliable update) on the columns that asked
-// for it: r = b - y (y = A x with the FP64 operator); acc[j] += r_j.r_j.
-template <int K>
-__device__ __forceinline__ void pcg_rel_r_rows(int64_t lb, int64_t lg, int64_t n, const CgState* st,
-                                               const double* __restrict__ b, const double* __restrict__ y,
-                                               double* __restrict__ r, double (&acc)[K]) {
+// Reliable update fused with the FP64 product: for the rows of a
+// workgroup range, r = b - A x (FP64 form) on the columns that asked for an
+// update; acc[j] += r_j.r_j.
+template <int BS, int K, bool ST>
+__device__ __forceinline__ void pcg_true_rows(int64_t lb, int64_t lg, int32_t nb, const int64_t* __restrict__ sptr,
+                                              const int32_t* __restrict__ col, const double* __restrict__ val,
+                                              const double* __restrict__ dg, const GridArg& ga,
+                                              const double* __restrict__ x, const double* __restrict__ b,
+                                              double* __restrict__ r, const CgState* st, double (&acc)[K]) {
+  constexpr int NV = BS * K;
   bool upd[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) upd[j] = st[j].upd != 0;
   const int64_t stride = lg * kBlock;
-  for (int64_t i = lb * kBlock + threadIdx.x; i < n; i += stride) {
-    const int j = (int)(i % K);
-    if (!upd[j]) continue;
-    const double ri = b[i] - y[i];
-    r[i] = ri;
-    acc[j] += ri * ri;
+  for (int64_t row = lb * kBlock + threadIdx.x; row < (int64_t)((nb + 63) & ~63); row += stride) {
+    double sum[NV], xo[NV];
+    sell_row<BS, K, double, false, ST>(row, nb, sptr, col, val, dg, x, nullptr, ga, sum, xo);
+    if (row >= nb) continue;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const int j = v % K;
+      if (!upd[j]) continue;
+      const double ri = b[row * NV + v] - sum[v];
+      r[row * NV + v] = ri;
+      acc[j] += ri * ri;
+    }
   }
 }
 

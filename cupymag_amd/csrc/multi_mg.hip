@@ -165,31 +165,21 @@ __global__ void __launch_bounds__(kBlock) kbm_spmv(const OuterSeg* __restrict__ 
   if (!PLAIN) block_partial<K>(acc, partials);
 }
 
-// reliable update of the systems that asked for one: q = A x (FP64 form)
+// reliable update of the systems that asked for one: r = b - A x (FP64
+// form) on those columns, partials of their r.r (SpMV workgroup layout)
 template <int BS, int K>
-__global__ void __launch_bounds__(kBlock) kbm_true(const OuterSeg* __restrict__ seg, const int32_t* __restrict__ map) {
-  const OuterSeg& d = seg[map[blockIdx.x]];
-  if (!any_upd<K>(d.st)) return;
-  double acc[K];
-  const int64_t lb = (int64_t)blockIdx.x - d.sblk0;
-  if (d.stencil)
-    sell_rows<BS, K, double, false, false, true>(lb, d.snblk, d.nb, d.sptr, d.col, d.v64, d.dg64, d.x, nullptr, d.q,
-                                                 d.ga, acc);
-  else
-    sell_rows<BS, K, double, false, false, false>(lb, d.snblk, d.nb, d.sptr, d.col, d.v64, d.dg64, d.x, nullptr, d.q,
-                                                  d.ga, acc);
-}
-
-// ... then r = b - q on their columns that asked, partials of r.r
-template <int K>
-__global__ void __launch_bounds__(kBlock) kbm_rel_r(const OuterSeg* __restrict__ seg, const int32_t* __restrict__ map,
-                                                    double* partials) {
+__global__ void __launch_bounds__(kBlock) kbm_true(const OuterSeg* __restrict__ seg, const int32_t* __restrict__ map,
+                                                   double* partials) {
   const OuterSeg& d = seg[map[blockIdx.x]];
   if (!any_upd<K>(d.st)) return;
   double acc[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) acc[j] = 0.0;
-  pcg_rel_r_rows<K>((int64_t)blockIdx.x - d.blk0, d.nblk, d.n, d.st, d.b, d.q, d.r, acc);
+  const int64_t lb = (int64_t)blockIdx.x - d.sblk0;
+  if (d.stencil)
+    pcg_true_rows<BS, K, true>(lb, d.snblk, d.nb, d.sptr, d.col, d.v64, d.dg64, d.ga, d.x, d.b, d.r, d.st, acc);
+  else
+    pcg_true_rows<BS, K, false>(lb, d.snblk, d.nb, d.sptr, d.col, d.v64, d.dg64, d.ga, d.x, d.b, d.r, d.st, acc);
   block_partial<K>(acc, partials);
 }
 
@@ -246,7 +236,8 @@ __global__ void __launch_bounds__(kBlock) kbm_pupd(const OuterSeg* __restrict__ 
 enum : int { MRED_INIT = 0, MRED_ALPHA = 1, MRED_RR = 2, MRED_RZ = 3, MRED_TRUE = 4 };
 
 // One workgroup per system: the system's partials in a fixed order (those
-// of its SpMV workgroups for MRED_ALPHA, of its vector workgroups otherwise),
+// of its SpMV workgroups for MRED_ALPHA and MRED_TRUE, of its vector
+// workgroups otherwise),
 // as the one-system reducers of mg.hip sum them.
 template <int K, int MODE>
 __global__ void __launch_bounds__(kRedBlock) kbm_red(const OuterSeg* __restrict__ seg, int G,
@@ -258,7 +249,8 @@ __global__ void __launch_bounds__(kRedBlock) kbm_red(const OuterSeg* __restrict_
   double v[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) v[k] = 0.0;
-  const int32_t b0 = (MODE == MRED_ALPHA) ? d.sblk0 : d.blk0, nbk = (MODE == MRED_ALPHA) ? d.snblk : d.nblk;
+  constexpr bool SP = (MODE == MRED_ALPHA || MODE == MRED_TRUE);  // partials of SpMV workgroups
+  const int32_t b0 = SP ? d.sblk0 : d.blk0, nbk = SP ? d.snblk : d.nblk;
   for (int i = threadIdx.x; i < nbk; i += kRedBlock)
 #pragma unroll
     for (int k = 0; k < NV; ++k) v[k] += partials[(int64_t)k * G + b0 + i];
@@ -501,9 +493,8 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
       kbm_xr<K><<<G, kBlock, 0, s>>>(O.d, O.dmap, pt);
       kbm_red<K, MRED_RR><<<ns, kRedBlock, 0, s>>>(O.d, G, pt);
       if (any_rel) {
-        kbm_true<BS, K><<<Gs, kBlock, 0, s>>>(O.d, W.dsmap);
-        kbm_rel_r<K><<<G, kBlock, 0, s>>>(O.d, O.dmap, pt);
-        kbm_red<K, MRED_TRUE><<<ns, kRedBlock, 0, s>>>(O.d, G, pt);
+        kbm_true<BS, K><<<Gs, kBlock, 0, s>>>(O.d, W.dsmap, pt);
+        kbm_red<K, MRED_TRUE><<<ns, kRedBlock, 0, s>>>(O.d, Gs, pt);
       }
       vcycle();
       kbm_dot<K, T><<<G, kBlock, 0, s>>>(O.d, O.dmap, pt);
