@@ -1,6 +1,7 @@
 // Geometric multigrid preconditioner and preconditioned CG for the
-// anchored stiffness operators (elasticity, bs = 3, and demag, bs = 1) on
-// structured periodic Kuhn meshes.
+// stiffness-type operators of structured periodic Kuhn meshes: elasticity
+// (bs = 3), demag (bs = 1) and the Gauss-Seidel operator M + alpha1 K
+// (bs = 1, three right-hand-side columns at once).
 //
 // Levels: a level of n cells per dimension has a coarser one of ceil(n/2)
 // cells, its operator assembled on that mesh by the host assembler
@@ -14,25 +15,35 @@
 // the regular Kuhn mesh of (n+1)/2 cells of the same periodic box and the
 // prolongation is the same P1 interpolation at the fine nodes' nominal
 // positions (non-nested: barycentric weights in the coarse Kuhn
-// tetrahedron that holds the point).  Restriction is its transpose.  Node 0
-// carries the operators' anchor (identity row and column): it is left out
-// of both transfers, so corrections never touch it.
+// tetrahedron that holds the point).  Restriction is its transpose.  When
+// row 0 of the operator is an identity row (the anchor of A_demag and
+// A_el), node 0 is left out of both transfers, so corrections never touch
+// it.
 //
 // Smoother: Chebyshev iteration of degree m on D^-1 A (D = diagonal blocks,
 // inverted once), on the interval [lmax/ratio, lmax] with lmax = 1.1 x a
 // power-iteration estimate; the same polynomial before and after the coarse
 // correction, and an exact (dense inverse) solve on the coarsest level, so
 // the V-cycle is a symmetric positive definite preconditioner.  It runs in
-// FP32 (precision 32, on the FP32 SELL form) or FP64.
+// FP64, FP32 (precision 32, on the FP32 SELL form), or FP32 with the level
+// operators' off-diagonal values in FP16 (precision 16: scaled by a power of
+// two, FP64 diagonal blocks compensated for the rounding).  The restriction
+// into a level that is not the coarsest is fused with that level's first
+// Chebyshev step.
 //
-// The outer solver is CG preconditioned with one V-cycle per iteration, in
-// FP64 on the FP64 SELL form: x, r, p, q in FP64, z from the V-cycle;
-// beta = (r - r_old).z / rho_old (flexible CG, robust to a preconditioner
-// applied in FP32), and CuPy's stopping rule on the recurrence residual:
-// ||r|| <= tol ||b|| after the update, at least one iteration, b = 0 gives 0.
-// Every kernel returns at once when the solve is done, so the host enqueues
-// batches of iterations and reads the state between them, as the other
-// solvers do.
+// The outer solver is CG preconditioned with one V-cycle per iteration: x,
+// r, p, q in FP64, z from the V-cycle; beta = (r - r_old).z / rho_old
+// (flexible CG, robust to a preconditioner applied in lower precision), and
+// CuPy's stopping rule on the residual: ||r|| <= tol ||b|| after the update,
+// at least one iteration, b = 0 gives 0.  The products A p use the FP64
+// form, or (delta > 0) the FP32 form with an FP64 residual replacement
+// r = b - A x whenever the recurrence residual has dropped by delta and
+// before stopping, so that the stopping test is on the FP64 residual.
+// K columns (K = 3 for the Gauss-Seidel solve) share the products and the
+// V-cycle; each keeps its own scalars and stops on its own.  Every kernel
+// returns at once when the solve is done, so the host enqueues batches of
+// iterations and reads the state between them; after the first iteration
+// of a batch the iterations are replayed from captured chunks (hipGraph).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
