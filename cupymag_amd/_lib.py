@@ -135,6 +135,17 @@ def alive():
     return _lib is not None and not _finalizing
 
 
+def release(name, h):
+    """Destroy a native handle from a ``__del__`` (``name``: the library's
+    destroy function).  A no-op once the interpreter finalizes, including
+    when this module's globals have already been cleared."""
+    try:
+        if alive():
+            getattr(_lib, name)(h)
+    except (AttributeError, TypeError):
+        pass
+
+
 def lib():
     """Load (once) and return the shared library; raises if it is not built."""
     global _lib

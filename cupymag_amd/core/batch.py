@@ -69,7 +69,8 @@ class GSPMBatch:
             sim.avg_m = r.avg_m
         return out
 
-    def __del__(self):
-        if getattr(self, "h", None) and _lib.alive():
-            _lib._lib.cpm_batch_destroy(self.h)
-            self.h = None
+    def __del__(self, _release=_lib.release):
+        h = getattr(self, "h", None)
+        self.h = None
+        if h:
+            _release("cpm_batch_destroy", h)

@@ -153,7 +153,8 @@ class GSPMSimulation:
         check(lib().cpm_sim_get(self.h, _lib.FIELD[field], ptr(out)), "cpm_sim_get")
         return out
 
-    def __del__(self):
-        if getattr(self, "h", None) and _lib.alive():
-            _lib._lib.cpm_sim_destroy(self.h)
-            self.h = None
+    def __del__(self, _release=_lib.release):
+        h = getattr(self, "h", None)
+        self.h = None
+        if h:
+            _release("cpm_sim_destroy", h)

@@ -35,10 +35,11 @@ class Context:
     def sync(self):
         check(lib().cpm_ctx_synchronize(self.h), "cpm_ctx_synchronize")
 
-    def __del__(self):
-        if getattr(self, "h", None) and _lib.alive():
-            _lib._lib.cpm_ctx_destroy(self.h)
-            self.h = None
+    def __del__(self, _release=_lib.release):
+        h = getattr(self, "h", None)
+        self.h = None
+        if h:
+            _release("cpm_ctx_destroy", h)
 
 
 def context(device=None):
@@ -142,7 +143,8 @@ class DeviceCSR:
         vv = v if v.ndim == 1 else v[valset]
         return sp.csr_matrix((vv, ix, ip), shape=self.shape)
 
-    def __del__(self):
-        if getattr(self, "h", None) and _lib.alive():
-            _lib._lib.cpm_mat_destroy(self.h)
-            self.h = None
+    def __del__(self, _release=_lib.release):
+        h = getattr(self, "h", None)
+        self.h = None
+        if h:
+            _release("cpm_mat_destroy", h)

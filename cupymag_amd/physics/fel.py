@@ -39,7 +39,8 @@ class MatrixFreeFel:
     def __matmul__(self, x):
         return self.matvec(x)
 
-    def __del__(self):
-        if getattr(self, "h", None) and _lib.alive():
-            _lib._lib.cpm_fel_destroy(self.h)
-            self.h = None
+    def __del__(self, _release=_lib.release):
+        h = getattr(self, "h", None)
+        self.h = None
+        if h:
+            _release("cpm_fel_destroy", h)

@@ -77,10 +77,11 @@ class KuhnMultigrid:
             return x, it[0], res[0]
         return x, list(it), list(res)
 
-    def __del__(self):
-        if getattr(self, "h", None) and _lib.alive():
-            _lib._lib.cpm_mg_destroy(self.h)
-            self.h = None
+    def __del__(self, _release=_lib.release):
+        h = getattr(self, "h", None)
+        self.h = None
+        if h:
+            _release("cpm_mg_destroy", h)
 
 
 def solve_batch(mgs, rhs, x0s=None, tol=1e-7, maxiter=5000, nrhs=1):
