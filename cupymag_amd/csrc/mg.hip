@@ -177,6 +177,26 @@ __global__ void __launch_bounds__(kBlock) k_scale(int64_t n, const double* __res
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) x[i] = s * y[i];
 }
 
+// lam = sqrt(sum of the partials), one workgroup (power iteration)
+__global__ void __launch_bounds__(kRedBlock) k_power_norm(int nblk, const double* __restrict__ partials,
+                                                          double* __restrict__ lam) {
+  __shared__ double lds[kRedBlock / 64];
+  double v[1] = {0.0};
+  for (int i = threadIdx.x; i < nblk; i += kRedBlock) v[0] += partials[i];
+  block_sum_t<1, kRedBlock>(v, lds);
+  if (threadIdx.x == 0) *lam = sqrt(v[0]);
+}
+
+// x = y / lam (lam > 0; otherwise x is left alone and the host reports it)
+__global__ void __launch_bounds__(kBlock) k_scale_inv(int64_t n, const double* __restrict__ y, double* __restrict__ x,
+                                                      const double* __restrict__ lam) {
+  const double l = *lam;
+  if (!(l > 0.0)) return;
+  const double s = 1.0 / l;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) x[i] = s * y[i];
+}
+
 // ---- outer preconditioned CG (FP64) ------------------------------------------
 // K columns interleaved (entry i belongs to column i % K, state st[i % K]).
 
@@ -421,7 +441,10 @@ static void vcycle_any(cpm_mg* M, const double* b64, const CgState* st) {
 }
 
 template <int BS>
-static int power_lmax(cpm_mg* M, MgLevel& Lv, double* x, double* y, const double* dinv64, double* lmax) {
+// 20 power iterations on D^-1 A without host round trips (the norm stays on
+// the device, in lam_dev); one read at the end.
+static int power_lmax(cpm_mg* M, MgLevel& Lv, double* x, double* y, const double* dinv64, double* lam_dev,
+                      double* lmax) {
   cpm_ctx* ctx = M->ctx;
   hipStream_t s = ctx->stream;
   const Sell& S = Lv.A->sell;
@@ -440,7 +463,6 @@ static int power_lmax(cpm_mg* M, MgLevel& Lv, double* x, double* y, const double
   for (double v : h) nrm += v * v;
   nrm = std::sqrt(nrm);
   k_scale<<<grid_for(Lv.n), kBlock, 0, s>>>(Lv.n, x, x, 1.0 / nrm);
-  double lam = 0.0;
   for (int it = 0; it < 20; ++it) {
     if (S.stencil)
       k_mg_power<BS, true><<<g, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), x, y, dinv64,
@@ -448,16 +470,15 @@ static int power_lmax(cpm_mg* M, MgLevel& Lv, double* x, double* y, const double
     else
       k_mg_power<BS, false><<<g, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), x, y, dinv64,
                                                  ctx->partials);
-    CPM_HIP(hipGetLastError());
-    std::vector<double> part(g);
-    CPM_HIP(hipMemcpyAsync(part.data(), ctx->partials, sizeof(double) * g, hipMemcpyDeviceToHost, s));
-    CPM_HIP(hipStreamSynchronize(s));
-    double ss = 0.0;
-    for (double v : part) ss += v;
-    lam = std::sqrt(ss);
-    if (!(lam > 0.0)) return fail(CPM_EINVAL, "multigrid: operator annihilates the power-iteration vector");
-    k_scale<<<grid_for(Lv.n), kBlock, 0, s>>>(Lv.n, y, x, 1.0 / lam);
+    k_power_norm<<<1, kRedBlock, 0, s>>>(g, ctx->partials, lam_dev);
+    k_scale_inv<<<grid_for(Lv.n), kBlock, 0, s>>>(Lv.n, y, x, lam_dev);
   }
+  CPM_HIP(hipGetLastError());
+  double lam = 0.0;
+  CPM_HIP(hipMemcpyAsync(ctx->pinned, lam_dev, sizeof(double), hipMemcpyDeviceToHost, s));
+  CPM_HIP(hipStreamSynchronize(s));
+  std::memcpy(&lam, ctx->pinned, sizeof(double));
+  if (!(lam > 0.0)) return fail(CPM_EINVAL, "multigrid: operator annihilates the power-iteration vector");
   *lmax = lam;
   return CPM_OK;
 }
@@ -874,7 +895,7 @@ extern "C" int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, 
       void *vx = nullptr, *vy = nullptr, *vd = nullptr;
       A(&vx, 8 * Lv.n);
       A(&vy, 8 * Lv.n);
-      A(&vd, 8 * nbb);
+      A(&vd, 8 * (nbb + 1));  // D^-1 (FP64) and the norm of the iterate
       if (rc != CPM_OK) break;
       px = (double*)vx, py = (double*)vy, dinv64 = (double*)vd;
       e = hipMemcpy(dinv64, inv64.data(), sizeof(double) * nbb, hipMemcpyHostToDevice);
@@ -882,7 +903,9 @@ extern "C" int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, 
         rc = fail(CPM_EHIP, hipGetErrorString(e));
         break;
       }
-      rc = bs == 3 ? power_lmax<3>(M, Lv, px, py, dinv64, &Lv.lmax) : power_lmax<1>(M, Lv, px, py, dinv64, &Lv.lmax);
+      double* lam_dev = dinv64 + nbb;
+      rc = bs == 3 ? power_lmax<3>(M, Lv, px, py, dinv64, lam_dev, &Lv.lmax)
+                   : power_lmax<1>(M, Lv, px, py, dinv64, lam_dev, &Lv.lmax);
       // the scratch of the estimate is not needed afterwards
       for (void* p : {vx, vy, vd}) {
         (void)hipFree(p);

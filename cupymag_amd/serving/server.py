@@ -24,6 +24,8 @@ latency (wall time of a field point), per-loop latency (arrival to the end
 of the sweep), queueing delay and aggregate node-steps/s.
 """
 import concurrent.futures as cf
+import contextlib
+import os
 import time
 from collections import deque
 
@@ -51,6 +53,16 @@ def estimate_bytes(n_dof, me, multigrid_n=None):
         nc = hierarchy_sizes(int(multigrid_n))[-1] ** 3
         b += _MG_BYTES_PER_DOF[bool(me)] * n_dof + 4.0 * (2 * nc ** 2 + (9 * nc ** 2 if me else 0))
     return int(b) + (64 << 20)
+
+
+def _blas_threads(n):
+    """Limit the BLAS / OpenMP thread pools of this process to n threads
+    inside the block (threadpoolctl when importable, else no limit)."""
+    try:
+        from threadpoolctl import threadpool_limits
+    except ImportError:
+        return contextlib.nullcontext()
+    return threadpool_limits(limits=n)
 
 
 def multigrid_ok(n, coarsest_max=8):
@@ -134,15 +146,26 @@ class SweepServer:
         from ..core.parameters import from_dict
         from ..mesh.kuhn import kuhn_mesh
         torch.cuda.set_device(self.ctx.device)
+        t0 = time.perf_counter()
         P = from_dict(req["cfg"])
         arr = kuhn_mesh(req["n"])
-        S = SimulationOperators(P, mesh_arrays=arr, ctx=self.ctx, keep_host=False, matrix_free_fel=True)
-        sim = GSPMSimulation(S, P, mode=self.mode, rel_delta=self.rel_delta,
-                             precond="mg" if self._uses_mg(req["n"]) else None)
+        # the preparing threads share the host with each other and with the
+        # stepping loop: few native threads each (assembly, BLAS)
+        nth = max(1, (os.cpu_count() or 8) // (2 * self.prep_workers))
+        with _blas_threads(nth):
+            S = SimulationOperators(P, mesh_arrays=arr, ctx=self.ctx, keep_host=False, matrix_free_fel=True,
+                                    nthreads=nth)
+            S.csr_demag_and_deriv()
+            t1 = time.perf_counter()
+            sim = GSPMSimulation(S, P, mode=self.mode, rel_delta=self.rel_delta,
+                                 precond="mg" if self._uses_mg(req["n"]) else None)
+        t2 = time.perf_counter()
         f0 = np.asarray(req["fields"][0], dtype=np.float64)
         direction = f0 if np.linalg.norm(f0) > 0 else (1.0, 0.0, 0.0)
         sim.set_m(initial_m(S.n_dof, req["m_seed"], direction=direction))
         self.ctx.sync()
+        # host-side preparation times (operators, simulation incl. hierarchies, initial state)
+        sim.prep_s = (t1 - t0, t2 - t1, time.perf_counter() - t2)
         return sim, P, S.n_dof
 
     def run(self, requests, log=None):
@@ -189,7 +212,9 @@ class SweepServer:
                     job.t_point = now()
                     batch.add(sim)
                     active.append(job)
-                    say(f"[serve] t={now():.2f}s request {job.rid} joins ({len(active)} members)")
+                    pt = getattr(sim, "prep_s", (0.0, 0.0, 0.0))
+                    say(f"[serve] t={now():.2f}s request {job.rid} joins ({len(active)} members; prepared in "
+                        f"{sum(pt):.2f}s: operators {pt[0]:.2f}s, simulation {pt[1]:.2f}s, state {pt[2]:.2f}s)")
                 if not active:
                     if preparing:
                         cf.wait(list(preparing), timeout=0.05, return_when=cf.FIRST_COMPLETED)
