@@ -92,14 +92,16 @@ class SimulationOperators(FEMMesh):
             return None
         return MatrixFreeFel(self.node_coords_np, n, Cv, ctx)
 
-    def multigrid(self, precision=32, degree=2, ratio=10.0):
-        """Multigrid hierarchies of A_demag and A_el (cached per precision),
-        or None when the mesh has none (cupymag_amd.solvers.multigrid)."""
+    def multigrid(self, precision=32, degree=2, ratio=10.0, floating=True):
+        """Multigrid hierarchies of A_demag, A1 and A_el (cached per
+        settings), or None when the mesh has none
+        (cupymag_amd.solvers.multigrid.build_hierarchies)."""
         cache = self.__dict__.setdefault("_mg", {})
-        key = (precision, degree, ratio)
+        key = (precision, degree, ratio, bool(floating))
         if key not in cache:
             from ..solvers.multigrid import build_hierarchies
-            cache[key] = build_hierarchies(self, precision, degree, ratio, nthreads=self.nthreads)
+            cache[key] = build_hierarchies(self, precision, degree, ratio, nthreads=self.nthreads,
+                                           floating=floating)
         return cache[key]
 
     def csr_demag_and_deriv(self):

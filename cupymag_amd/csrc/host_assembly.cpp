@@ -246,8 +246,10 @@ struct cpm_host_ops {
 
 extern "C" int cpm_host_assemble(const double* coords, int64_t n_nodes, const int32_t* elems, int64_t n_elems,
                                  int npe, int elem_stride, const int32_t* gid, int32_t ndof,
-                                 const uint8_t* defect_dof, const double* coef, int want_elastic, int nthreads,
+                                 const uint8_t* defect_dof, const double* coef, int flags, int nthreads,
                                  cpm_host_ops** out) {
+  const bool want_elastic = (flags & CPM_HA_ELASTIC) != 0;
+  const bool anchors = (flags & CPM_HA_NO_ANCHORS) == 0;
   try {
     if (!coords || !elems || !gid || !coef || !out) return cpm::fail(CPM_EINVAL, "null argument");
     if (npe != 4 && npe != 8) return cpm::fail(CPM_EINVAL, "npe must be 4 (tet) or 8 (hex)");
@@ -359,7 +361,7 @@ extern "C" int cpm_host_assemble(const double* coords, int64_t n_nodes, const in
       }
     });
     // anchor of the demag potential at DOF 0 (assemble_demag.py:307-341)
-    for (int64_t d = 0; d < n; ++d)
+    for (int64_t d = 0; d < n && anchors; ++d)
       for (int64_t p = H->ptr[d]; p < H->ptr[d + 1]; ++p)
         if (d == 0 || H->idx[p] == 0) H->vK[p] = (d == 0 && H->idx[p] == 0) ? 1.0 : 0.0;
     // Gauss-Seidel surgery: defect rows/cols -> identity (final_assembly.py:59-112),
@@ -428,14 +430,14 @@ extern "C" int cpm_host_assemble(const double* coords, int64_t n_nodes, const in
       });
       // anchors {0, n, 2n} (assemble_elasticity.py:461-527)
       const int64_t anc[3] = {0, n, 2 * n};
-      for (int64_t r = 0; r < 3 * n; ++r)
+      for (int64_t r = 0; r < 3 * n && anchors; ++r)
         for (int64_t p = H->el_ptr[r]; p < H->el_ptr[r + 1]; ++p) {
           const int64_t c = H->el_idx[p];
           bool ra = (r == anc[0] || r == anc[1] || r == anc[2]);
           bool ca = (c == anc[0] || c == anc[1] || c == anc[2]);
           if (ra || ca) H->vAel[p] = (r == c) ? 1.0 : 0.0;
         }
-      for (int a = 0; a < 3; ++a)
+      for (int a = 0; a < 3 && anchors; ++a)
         for (int64_t p = H->fe_ptr[anc[a]]; p < H->fe_ptr[anc[a] + 1]; ++p) H->vFel[p] = 0.0;
     }
     *out = H;

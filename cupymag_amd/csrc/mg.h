@@ -24,6 +24,12 @@ struct MgLevel {
   void* v16 = nullptr;
   double* dgh = nullptr;
   double vscale = 1.0;
+  // floating hierarchy, level 0: the smoothing operator is the periodic
+  // stiffness recovered from the anchored operator (owned copies; FP64 for
+  // precision 64, the compensated FP32 form for precision 32); null: the
+  // level's own SELL form
+  double *kv64 = nullptr, *kdg64 = nullptr, *kdgc = nullptr;
+  float* kv32 = nullptr;
 };
 
 }  // namespace cpm
@@ -32,6 +38,9 @@ struct cpm_mg {
   cpm_ctx* ctx = nullptr;
   int bs = 1, prec = 32, degree = 2;
   int anchor = 1;           // row 0 of the fine operator is an identity row: node 0 left out of the transfers
+  // anchored fine operator over singular periodic levels: the V-cycle
+  // solves the periodic problem, with the anchor applied around it
+  int floating = 0;
   int half = 0;             // precision 16: FP32 vectors, FP16 off-diagonal values
   double ratio = 20.0;
   std::vector<cpm::MgLevel> L;

@@ -15,10 +15,23 @@
 // the regular Kuhn mesh of (n+1)/2 cells of the same periodic box and the
 // prolongation is the same P1 interpolation at the fine nodes' nominal
 // positions (non-nested: barycentric weights in the coarse Kuhn
-// tetrahedron that holds the point).  Restriction is its transpose.  When
-// row 0 of the operator is an identity row (the anchor of A_demag and
-// A_el), node 0 is left out of both transfers, so corrections never touch
-// it.
+// tetrahedron that holds the point).  Restriction is its transpose.
+//
+// Anchored operators (row 0 an identity row: the anchor of A_demag and
+// A_el) come in two kinds of hierarchy.  Floating (coarse levels given as
+// the singular periodic stiffness K, coarsest solve its pseudo-inverse):
+// the V-cycle V works on the periodic problem, where the anchor is no point
+// constraint for the coarse levels to approximate, and the preconditioner
+// is z = F^T V(F r) + e_0 r_0 with F = I - e_0 1^T per component -- node 0
+// takes minus the sum of the other entries of r (a right-hand side in the
+// range of K), the output is shifted by its node-0 value and node 0 gets
+// r_0 (the identity row).  With an exact V this is the inverse of the
+// anchored operator; it is symmetric, so CG keeps its theory.  Level 0
+// smooths with K, recovered on the device from the anchored operator (K
+// annihilates the translations: the block of node 0's column in row i is
+// minus the sum of the row's other blocks; row 0 by symmetry).  Anchored
+// coarse levels instead: node 0 is left out of both transfers, so
+// corrections never touch it.
 //
 // Smoother: Chebyshev iteration of degree m on D^-1 A (D = diagonal blocks,
 // inverted once), on the interval [lmax/ratio, lmax] with lmax = 1.1 x a
@@ -60,12 +73,88 @@ namespace cpm {
 // ---------------------------------------------------------------- kernels
 // V-cycle kernels run while any of the solve's K columns is live.
 
+// fpart (floating hierarchy, level 0): partials of the per-component sums
+// of b64, fn of them at stride fstride
 template <int BS, int K, class T, bool ZERO, bool FROM64>
 __global__ void __launch_bounds__(kBlock) k_mg_first(int32_t nb, const double* __restrict__ b64, T* __restrict__ b,
                                                      T* __restrict__ r, T* __restrict__ d, T* __restrict__ x,
-                                                     const T* __restrict__ dinv, double c0, const CgState* st) {
+                                                     const T* __restrict__ dinv, double c0, const CgState* st,
+                                                     const double* __restrict__ fpart = nullptr,
+                                                     int64_t fstride = 0, int32_t fn = 0) {
   if (all_done<K>(st)) return;
-  mg_first_rows<BS, K, T, ZERO, FROM64>(blockIdx.x, gridDim.x, nb, b64, b, r, d, x, dinv, c0);
+  mg_first_rows<BS, K, T, ZERO, FROM64>(blockIdx.x, gridDim.x, nb, b64, b, r, d, x, dinv, c0, fpart, fstride, fn);
+}
+
+// Periodic stiffness K from the anchored operator A of a floating
+// hierarchy (A: node 0's rows and columns replaced by the identity, every
+// other entry that of K; K annihilates the bs translations of a periodic
+// mesh).  Rows i != 0: the block in the column of node 0 is minus the sum
+// of the row's other blocks (diagonal block included), per component.
+// Stencil SELL form, FP64 values v and diagonal blocks dg (copies of A's,
+// changed in place).
+template <int BS>
+__global__ void __launch_bounds__(kBlock) k_mg_float_rows(int32_t nb, int32_t nst, GridArg ga, double* __restrict__ v,
+                                                          const double* __restrict__ dg) {
+  constexpr int BB = BS * BS;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t row = 1 + (int64_t)blockIdx.x * kBlock + threadIdx.x; row < nb; row += stride) {
+    const int64_t s0 = (row >> 6) * nst, l = row & 63;
+    const int32_t nyz = ga.ny * ga.nz;
+    const int32_t ri = (int32_t)row / nyz, rj = ((int32_t)row / ga.nz) % ga.ny, rk = (int32_t)row % ga.nz;
+    int64_t zslot = -1;
+    double sum[BB];
+#pragma unroll
+    for (int t = 0; t < BB; ++t) sum[t] = dg[row * BB + t];
+    for (int32_t k = 0; k < nst; ++k) {
+      const int32_t code = ga.soff[k];
+      int32_t i2 = ri + (code & 3) - 1, j2 = rj + ((code >> 2) & 3) - 1, k2 = rk + ((code >> 4) & 3) - 1;
+      i2 = (i2 + ga.nx) % ga.nx, j2 = (j2 + ga.ny) % ga.ny, k2 = (k2 + ga.nz) % ga.nz;
+      if (i2 == 0 && j2 == 0 && k2 == 0) {
+        zslot = s0 + k;
+        continue;
+      }
+#pragma unroll
+      for (int t = 0; t < BB; ++t) sum[t] += v[((s0 + k) * BB + t) * 64 + l];
+    }
+    if (zslot < 0) continue;
+#pragma unroll
+    for (int t = 0; t < BB; ++t) v[(zslot * BB + t) * 64 + l] = -sum[t];
+  }
+}
+
+// Row 0 of K (one thread, after k_mg_float_rows): the block in the column
+// of neighbour c is the transpose of K's block (c, 0) (K is symmetric), the
+// diagonal block minus the sum of them.
+template <int BS>
+__global__ void k_mg_float_row0(int32_t nst, GridArg ga, double* __restrict__ v, double* __restrict__ dg) {
+  constexpr int BB = BS * BS;
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double dsum[BB];
+#pragma unroll
+  for (int t = 0; t < BB; ++t) dsum[t] = 0.0;
+  for (int32_t k = 0; k < nst; ++k) {
+    const int32_t code = ga.soff[k];
+    const int32_t di = (code & 3) - 1, dj = ((code >> 2) & 3) - 1, dk = ((code >> 4) & 3) - 1;
+    const int32_t i2 = (di + ga.nx) % ga.nx, j2 = (dj + ga.ny) % ga.ny, k2 = (dk + ga.nz) % ga.nz;
+    const int64_t c = ((int64_t)i2 * ga.ny + j2) * ga.nz + k2;
+    // the slot of row c that points back to node 0: the opposite offset
+    const int32_t back = (1 - di) | ((1 - dj) << 2) | ((1 - dk) << 4);
+    int64_t cs = -1;
+    for (int32_t q = 0; q < nst; ++q)
+      if (ga.soff[q] == back) cs = (c >> 6) * nst + q;
+    if (cs < 0) continue;  // not a symmetric stencil: left as it is (the host checks the result)
+    const int64_t lc = c & 63;
+#pragma unroll
+    for (int d = 0; d < BS; ++d)
+#pragma unroll
+      for (int e = 0; e < BS; ++e) {
+        const double a = v[(cs * BB + e * BS + d) * 64 + lc];
+        v[(k * BB + d * BS + e) * 64] = a;  // row 0: slice 0, lane 0
+        dsum[d * BS + e] += a;
+      }
+  }
+#pragma unroll
+  for (int t = 0; t < BB; ++t) dg[t] = -dsum[t];
 }
 
 template <int BS, int K, class T, int MODE, bool ST, class VT>
@@ -81,16 +170,17 @@ __global__ void __launch_bounds__(kBlock) k_mg_sweep(int32_t nb, const int64_t* 
                                         c1, c2, vscale);
 }
 
-// FP16 form of a level operator: v16 = fp16(v64 / vscale) in the SELL
-// layout (padding lanes included; for 3x3 blocks entries 2q and 2q+1,
-// q < 4, interleaved per lane so that one 32-bit load fetches both, entry 8
-// after them), dgh = dg64 + the rounding of the row's off-diagonal values,
-// so that the stored row sums equal the FP64 ones.
-template <int BS>
-__global__ void __launch_bounds__(kBlock) k_mg_make_half(int32_t nb, int64_t nrows_pad, const int64_t* __restrict__ sptr,
-                                                         int32_t nst, int stencil, const double* __restrict__ v64,
-                                                         const double* __restrict__ dg64, double vscale,
-                                                         _Float16* __restrict__ v16, double* __restrict__ dgh) {
+// Low-precision form of a level operator: VT = _Float16: v16 =
+// fp16(v64 / vscale) in the SELL layout (padding lanes included; for 3x3
+// blocks entries 2q and 2q+1, q < 4, interleaved per lane so that one
+// 32-bit load fetches both, entry 8 after them); VT = float: fp32(v64)
+// (vscale 1) in the layout of v64.  dgh = dg64 + the rounding of the row's
+// off-diagonal values, so that the stored row sums equal the FP64 ones.
+template <int BS, class VT>
+__global__ void __launch_bounds__(kBlock) k_mg_make_low(int32_t nb, int64_t nrows_pad, const int64_t* __restrict__ sptr,
+                                                        int32_t nst, int stencil, const double* __restrict__ v64,
+                                                        const double* __restrict__ dg64, double vscale,
+                                                        VT* __restrict__ v16, double* __restrict__ dgh) {
   constexpr int BB = BS * BS;
   const int64_t stride = (int64_t)gridDim.x * kBlock;
   for (int64_t row = (int64_t)blockIdx.x * kBlock + threadIdx.x; row < nrows_pad; row += stride) {
@@ -105,8 +195,9 @@ __global__ void __launch_bounds__(kBlock) k_mg_make_half(int32_t nb, int64_t nro
       for (int t = 0; t < BB; ++t) {
         const int64_t i = (slot * BB + t) * 64 + l;
         const double a = v64[i];
-        const _Float16 h = (_Float16)(a / vscale);
-        const int64_t o = (BB == 9) ? slot * 576 + (t < 8 ? (t >> 1) * 128 + 2 * l + (t & 1) : 512 + l) : i;
+        const VT h = (VT)(a / vscale);
+        const int64_t o =
+            (sizeof(VT) == 2 && BB == 9) ? slot * 576 + (t < 8 ? (t >> 1) * 128 + 2 * l + (t & 1) : 512 + l) : i;
         v16[o] = h;
         comp[t] += a - (double)h * vscale;
       }
@@ -278,8 +369,8 @@ __global__ void __launch_bounds__(kBlock) k_pcg_xr(int64_t n, double* __restrict
   block_partial<K>(acc, partials);
 }
 
-// partials of r_j.z_j and r_old_j.z_j
-template <int K, class T>
+// partials of r_j.z_j and r_old_j.z_j (FB: see pcg_z)
+template <int K, class T, int FB>
 __global__ void __launch_bounds__(kBlock) k_pcg_dot(int64_t n, const double* __restrict__ r,
                                                     const double* __restrict__ rold, const T* __restrict__ z,
                                                     const CgState* st, double* partials) {
@@ -287,8 +378,30 @@ __global__ void __launch_bounds__(kBlock) k_pcg_dot(int64_t n, const double* __r
   double acc[2 * K];
 #pragma unroll
   for (int j = 0; j < 2 * K; ++j) acc[j] = 0.0;
-  pcg_dot_rows<K, T>(blockIdx.x, gridDim.x, n, st, r, rold, z, acc);
+  pcg_dot_rows<K, T, FB>(blockIdx.x, gridDim.x, n, st, r, rold, z, acc);
   block_partial<2 * K>(acc, partials);
+}
+
+// partials of the per-component sums of r (floating hierarchy: read by the
+// first Chebyshev step of level 0)
+template <int NV, int K>
+__global__ void __launch_bounds__(kBlock) k_pcg_rsum(int64_t n, const double* __restrict__ r, const CgState* st,
+                                                     double* partials) {
+  if (all_done<K>(st)) return;
+  double acc[NV];
+#pragma unroll
+  for (int c = 0; c < NV; ++c) acc[c] = 0.0;
+  pcg_rsum_rows<NV>(blockIdx.x, gridDim.x, n, r, acc);
+  block_partial<NV>(acc, partials);
+}
+
+// z = F^T y + e_0 r_0 in FP64 (the preconditioner of a floating hierarchy,
+// for cpm_mg_apply)
+template <int BS, class T>
+__global__ void __launch_bounds__(kBlock) k_mg_float_out(int64_t n, const T* __restrict__ y,
+                                                         const double* __restrict__ r, double* __restrict__ z) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) z[i] = pcg_z<BS, T>(i, y, r);
 }
 
 // live columns: rho = r.z; beta = (r.z - r_old.z) / rho_old (flexible CG)
@@ -310,30 +423,33 @@ __global__ void __launch_bounds__(kRedBlock) k_pcg_red_rz(int nblk, const double
       if (!st[j].done) pcg_state_rz(st[j], v[j], v[K + j]);
 }
 
-template <int K, class T>
+template <int K, class T, int FB>
 __global__ void __launch_bounds__(kBlock) k_pcg_pupd(int64_t n, const T* __restrict__ z, double* __restrict__ p,
-                                                     const CgState* st) {
+                                                     const double* __restrict__ r, const CgState* st) {
   if (all_done<K>(st)) return;
-  pcg_pupd_rows<K, T>(blockIdx.x, gridDim.x, n, st, z, p);
+  pcg_pupd_rows<K, T, FB>(blockIdx.x, gridDim.x, n, st, z, p, r);
 }
 
 // ---------------------------------------------------------------- host
 
 static GridArg grid_arg(const Sell& S) { return GridArg{S.dims[0], S.dims[1], S.dims[2], S.nst, S.soff}; }
 
+
+// smoothing values of a level (FP64 or FP32 form): the recovered periodic
+// stiffness on level 0 of a floating hierarchy, else the level's operator
 template <class T>
-static const T* vals(const Sell& S) {
+static const T* lvals(const MgLevel& Lv) {
   if constexpr (sizeof(T) == 8)
-    return S.v64;
+    return Lv.kv64 ? Lv.kv64 : Lv.A->sell.v64;
   else
-    return S.v32;
+    return Lv.kv32 ? Lv.kv32 : Lv.A->sell.v32;
 }
 template <class T>
-static const double* diag(const Sell& S) {
+static const double* ldiag(const MgLevel& Lv) {
   if constexpr (sizeof(T) == 8)
-    return S.dg64;
+    return Lv.kdg64 ? Lv.kdg64 : Lv.A->sell.dg64;
   else
-    return S.dgc;
+    return Lv.kdgc ? Lv.kdgc : Lv.A->sell.dgc;
 }
 
 template <int BS, int K, class T, int MODE, bool ST, class VT>
@@ -370,9 +486,9 @@ static void sweep(cpm_mg* M, const MgLevel& Lv, const void* din, void* dout, dou
     }
   }
   if (S.stencil)
-    launch_sweep<BS, K, T, MODE, true>(s, g, S, vals<T>(S), diag<T>(S), Lv, din, dout, c1, c2, 1.0, st, ev);
+    launch_sweep<BS, K, T, MODE, true>(s, g, S, lvals<T>(Lv), ldiag<T>(Lv), Lv, din, dout, c1, c2, 1.0, st, ev);
   else
-    launch_sweep<BS, K, T, MODE, false>(s, g, S, vals<T>(S), diag<T>(S), Lv, din, dout, c1, c2, 1.0, st, ev);
+    launch_sweep<BS, K, T, MODE, false>(s, g, S, lvals<T>(Lv), ldiag<T>(Lv), Lv, din, dout, c1, c2, 1.0, st, ev);
 }
 
 // One V-cycle from level l for K columns: b of level 0 read from the FP64
@@ -397,10 +513,14 @@ static void vcycle(cpm_mg* M, size_t l, const double* b64, const CgState* st, hi
   const double* c1 = &M->c1[l * 8];
   const double* c2 = &M->c2[l * 8];
   void* d[2] = {Lv.d0, Lv.d1};
-  // pre-smoothing from x = 0
-  if (l == 0 && b64)
+  // pre-smoothing from x = 0 (floating: b64's component sums are in the
+  // partials of k_pcg_rsum, launched just before over grid_for(n) workgroups)
+  if (l == 0 && b64) {
+    const int gf = grid_for(Lv.n * K);
     k_mg_first<BS, K, T, true, true><<<gn, kBlock, 0, s>>>(Lv.nb, b64, (T*)Lv.b, (T*)Lv.r, (T*)d[0], (T*)Lv.x,
-                                                           (const T*)Lv.dinv, c0, st);
+                                                           (const T*)Lv.dinv, c0, st,
+                                                           M->floating ? M->ctx->partials : nullptr, gf, gf);
+  }
   else if (!first_done)
     k_mg_first<BS, K, T, true, false><<<gn, kBlock, 0, s>>>(Lv.nb, nullptr, (T*)Lv.b, (T*)Lv.r, (T*)d[0],
                                                             (T*)Lv.x, (const T*)Lv.dinv, c0, st);
@@ -457,18 +577,19 @@ static int power_lmax(cpm_mg* M, MgLevel& Lv, double* x, double* y, const double
     st ^= st << 17;
     v = 0.5 + (double)(st >> 11) * (1.0 / 9007199254740992.0);
   }
-  for (int c = 0; c < BS; ++c) h[c] = 0.0;  // anchor node
+  for (int c = 0; c < BS; ++c) h[c] = 0.0;  // node 0 (the anchor of anchored operators)
   CPM_HIP(hipMemcpyAsync(x, h.data(), sizeof(double) * Lv.n, hipMemcpyHostToDevice, s));
   double nrm = 0.0;
   for (double v : h) nrm += v * v;
   nrm = std::sqrt(nrm);
   k_scale<<<grid_for(Lv.n), kBlock, 0, s>>>(Lv.n, x, x, 1.0 / nrm);
+  const double *v64 = lvals<double>(Lv), *dg64 = ldiag<double>(Lv);
   for (int it = 0; it < 20; ++it) {
     if (S.stencil)
-      k_mg_power<BS, true><<<g, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), x, y, dinv64,
+      k_mg_power<BS, true><<<g, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, v64, dg64, grid_arg(S), x, y, dinv64,
                                                 ctx->partials);
     else
-      k_mg_power<BS, false><<<g, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), x, y, dinv64,
+      k_mg_power<BS, false><<<g, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, v64, dg64, grid_arg(S), x, y, dinv64,
                                                  ctx->partials);
     k_power_norm<<<1, kRedBlock, 0, s>>>(g, ctx->partials, lam_dev);
     k_scale_inv<<<grid_for(Lv.n), kBlock, 0, s>>>(Lv.n, y, x, lam_dev);
@@ -594,6 +715,26 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
                                                               in, out, st, ctx->partials);
     }
   };
+  // floating hierarchy (one column): z = F^T V(F r) + e_0 r_0, the sums of
+  // r for F r just before the V-cycle, the shift in the consumers of z
+  const bool fl = M->floating != 0;
+  if (fl && K != 1) return fail(CPM_EINVAL, "multigrid CG: a floating hierarchy solves one column");
+  constexpr int FB = (K == 1) ? BS : 0;
+  auto precond = [&](hipEvent_t* ev) {
+    if (fl) k_pcg_rsum<BS * K, K><<<gv, kBlock, 0, s>>>(n, M->r, st, ctx->partials);
+    vcycle<BS, K, T>(M, 0, M->r, st, ev);
+    if (fl)
+      k_pcg_dot<K, T, FB><<<gv, kBlock, 0, s>>>(n, M->r, M->rold, (const T*)L0.x, st, ctx->partials);
+    else
+      k_pcg_dot<K, T, 0><<<gv, kBlock, 0, s>>>(n, M->r, M->rold, (const T*)L0.x, st, ctx->partials);
+    k_pcg_red_rz<K><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st);
+  };
+  auto pupd = [&]() {
+    if (fl)
+      k_pcg_pupd<K, T, FB><<<gv, kBlock, 0, s>>>(n, (const T*)L0.x, M->p, M->r, st);
+    else
+      k_pcg_pupd<K, T, 0><<<gv, kBlock, 0, s>>>(n, (const T*)L0.x, M->p, M->r, st);
+  };
   // start: r = b - A x0 (or b), state, z = M r, rho = r.z
   if (use_x0) spmv(x, M->q, true);
   k_pcg_init<K><<<gv, kBlock, 0, s>>>(n, b, x, M->q, M->r, use_x0, ctx->partials);
@@ -602,9 +743,7 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
   else
     k_pcg_red_init<K, false><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st, tol, maxiter);
   if (use_x0) k_pcg_zero_b<K><<<gv, kBlock, 0, s>>>(n, x, st);
-  vcycle<BS, K, T>(M, 0, M->r, st);
-  k_pcg_dot<K, T><<<gv, kBlock, 0, s>>>(n, M->r, M->rold, (const T*)L0.x, st, ctx->partials);
-  k_pcg_red_rz<K><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st);
+  precond(nullptr);
   CPM_HIP(hipGetLastError());
   // profiling: the first Chebyshev step of level 0 in the pre- and in the
   // post-smoothing of the first iteration of every batch (launched
@@ -613,7 +752,7 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
   const bool prof = ctx->prof && M->degree >= 2 && K == 1;
   auto iterate = [&](int todo, hipEvent_t* ev) -> int {
     for (int k = 0; k < todo; ++k) {
-      k_pcg_pupd<K, T><<<gv, kBlock, 0, s>>>(n, (const T*)L0.x, M->p, st);
+      pupd();
       spmv(M->p, M->q, false);
       k_red<K, RED_ALPHA><<<1, kRedBlock, 0, s>>>(gs, ctx->partials, st, 0.0);
       k_pcg_xr<K><<<gv, kBlock, 0, s>>>(n, x, M->r, M->rold, M->p, M->q, st, ctx->partials);
@@ -629,9 +768,7 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
       } else {
         k_red<K, RED_RR_PCG><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st, 0.0);
       }
-      vcycle<BS, K, T>(M, 0, M->r, st, (k == 0) ? ev : nullptr);
-      k_pcg_dot<K, T><<<gv, kBlock, 0, s>>>(n, M->r, M->rold, (const T*)L0.x, st, ctx->partials);
-      k_pcg_red_rz<K><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st);
+      precond((k == 0) ? ev : nullptr);
     }
     CPM_HIP(hipGetLastError());
     return CPM_OK;
@@ -807,21 +944,47 @@ extern "C" int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, 
   M->degree = degree;
   M->ratio = ratio;
   {
-    // anchored operator (row 0 an identity row, as A_demag and A_el): node 0
-    // stays out of the transfers; otherwise (A1) it takes part
-    const Sell& F = levels[0]->sell;
-    const int64_t cnt = (int64_t)F.nst * bs * bs * 64;
-    std::vector<double> v0(std::max<int64_t>(cnt, 1), 0.0);
-    if (cnt > 0) {
-      const hipError_t e = hipMemcpy(v0.data(), F.v64, sizeof(double) * cnt, hipMemcpyDeviceToHost);
+    // anchored operator (row 0 an identity row, as A_demag and A_el): with
+    // anchored coarse levels node 0 stays out of the transfers; with
+    // singular periodic coarse levels the hierarchy is floating (level 0
+    // smooths with the periodic stiffness recovered from the fine operator,
+    // node 0 takes part in the transfers).  Otherwise (A1) node 0 takes part.
+    auto anchored = [&](const Sell& F, bool* a) -> hipError_t {
+      const int64_t cnt = (int64_t)F.nst * bs * bs * 64;
+      std::vector<double> v0(std::max<int64_t>(cnt, 1), 0.0);
+      if (cnt > 0) {
+        const hipError_t e = hipMemcpy(v0.data(), F.v64, sizeof(double) * cnt, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return e;
+      }
+      bool off = false;
+      for (int64_t q = 0; q < cnt; q += 64) off |= v0[q] != 0.0;  // row 0 = lane 0 of slice 0
+      *a = !off;
+      return hipSuccess;
+    };
+    bool a[16];
+    for (int l = 0; l < nlevels; ++l) {
+      const hipError_t e = anchored(levels[l]->sell, &a[l]);
       if (e != hipSuccess) {
         delete M;
         return fail(CPM_EHIP, hipGetErrorString(e));
       }
     }
-    bool off = false;
-    for (int64_t q = 0; q < cnt; q += 64) off |= v0[q] != 0.0;
-    M->anchor = off ? 0 : 1;
+    for (int l = 2; l < nlevels; ++l)
+      if (a[l] != a[1]) {
+        delete M;
+        return fail(CPM_EINVAL, "cpm_mg_create: coarse levels must all be anchored or all periodic");
+      }
+    if (!a[0] && a[1]) {
+      delete M;
+      return fail(CPM_EINVAL, "cpm_mg_create: anchored coarse levels under a periodic fine operator");
+    }
+    M->floating = (a[0] && !a[1]) ? 1 : 0;
+    M->anchor = (a[0] && a[1]) ? 1 : 0;
+    const Sell& F = levels[0]->sell;
+    if (M->floating && (F.dims[0] < 3 || F.dims[1] < 3 || F.dims[2] < 3)) {
+      delete M;
+      return fail(CPM_EINVAL, "cpm_mg_create: a floating hierarchy needs >= 3 fine nodes per dimension");
+    }
   }
   const size_t ts = precision == 32 ? 4 : 8;
   int rc = CPM_OK;
@@ -840,10 +1003,38 @@ extern "C" int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, 
     for (void** p : {&Lv.x, &Lv.r, &Lv.b, &Lv.d0, &Lv.d1}) A(p, ts * Lv.n);
     A(&Lv.dinv, ts * Lv.nb * bs * bs);
     if (rc != CPM_OK) break;
-    // inverse diagonal blocks: of the FP64 form, and of the (compensated) FP32 form
     const int64_t nbb = (int64_t)Lv.nb * bs * bs;
+    const int64_t nval = S.nslot * bs * bs * 64;
+    hipError_t e = hipSuccess;
+    if (l == 0 && M->floating) {
+      // the periodic stiffness: FP64 copies of the fine operator's values
+      // and diagonal blocks, node 0's rows and columns recovered in place
+      A((void**)&Lv.kv64, sizeof(double) * std::max<int64_t>(1, nval));
+      A((void**)&Lv.kdg64, sizeof(double) * nbb);
+      if (rc != CPM_OK) break;
+      e = hipMemcpyAsync(Lv.kv64, S.v64, sizeof(double) * nval, hipMemcpyDeviceToDevice, s);
+      if (e == hipSuccess) e = hipMemcpyAsync(Lv.kdg64, S.dg64, sizeof(double) * nbb, hipMemcpyDeviceToDevice, s);
+      if (e == hipSuccess) {
+        if (bs == 3) {
+          k_mg_float_rows<3><<<grid_for(Lv.nb), kBlock, 0, s>>>(S.nbrows, S.nst, grid_arg(S), Lv.kv64, Lv.kdg64);
+          k_mg_float_row0<3><<<1, 64, 0, s>>>(S.nst, grid_arg(S), Lv.kv64, Lv.kdg64);
+        } else {
+          k_mg_float_rows<1><<<grid_for(Lv.nb), kBlock, 0, s>>>(S.nbrows, S.nst, grid_arg(S), Lv.kv64, Lv.kdg64);
+          k_mg_float_row0<1><<<1, 64, 0, s>>>(S.nst, grid_arg(S), Lv.kv64, Lv.kdg64);
+        }
+        e = hipGetLastError();
+      }
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      if (e != hipSuccess) {
+        rc = fail(CPM_EHIP, hipGetErrorString(e));
+        break;
+      }
+    }
+    const double* v64 = Lv.kv64 ? Lv.kv64 : S.v64;
+    const double* dg64 = Lv.kdg64 ? Lv.kdg64 : S.dg64;
+    // inverse diagonal blocks: of the FP64 form, and of the (compensated) FP32 form
     std::vector<double> dg(nbb), inv64, inv_t;
-    hipError_t e = hipMemcpy(dg.data(), S.dg64, sizeof(double) * nbb, hipMemcpyDeviceToHost);
+    e = hipMemcpy(dg.data(), dg64, sizeof(double) * nbb, hipMemcpyDeviceToHost);
     if (e != hipSuccess) {
       rc = fail(CPM_EHIP, hipGetErrorString(e));
       break;
@@ -863,11 +1054,29 @@ extern "C" int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, 
       if (rc != CPM_OK) break;
       const int64_t pad = (int64_t)S.nslices * 64;
       if (bs == 3)
-        k_mg_make_half<3><<<grid_for(pad), kBlock, 0, s>>>(S.nbrows, pad, S.sptr, S.nst, S.stencil, S.v64, S.dg64,
-                                                           Lv.vscale, (_Float16*)Lv.v16, Lv.dgh);
+        k_mg_make_low<3, _Float16><<<grid_for(pad), kBlock, 0, s>>>(S.nbrows, pad, S.sptr, S.nst, S.stencil, v64,
+                                                                    dg64, Lv.vscale, (_Float16*)Lv.v16, Lv.dgh);
       else
-        k_mg_make_half<1><<<grid_for(pad), kBlock, 0, s>>>(S.nbrows, pad, S.sptr, S.nst, S.stencil, S.v64, S.dg64,
-                                                           Lv.vscale, (_Float16*)Lv.v16, Lv.dgh);
+        k_mg_make_low<1, _Float16><<<grid_for(pad), kBlock, 0, s>>>(S.nbrows, pad, S.sptr, S.nst, S.stencil, v64,
+                                                                    dg64, Lv.vscale, (_Float16*)Lv.v16, Lv.dgh);
+      e = hipGetLastError();
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      if (e != hipSuccess) {
+        rc = fail(CPM_EHIP, hipGetErrorString(e));
+        break;
+      }
+    } else if (precision == 32 && Lv.kv64) {
+      // the compensated FP32 form of the recovered stiffness
+      A((void**)&Lv.kv32, sizeof(float) * std::max<int64_t>(1, nval));
+      A((void**)&Lv.kdgc, sizeof(double) * nbb);
+      if (rc != CPM_OK) break;
+      const int64_t pad = (int64_t)S.nslices * 64;
+      if (bs == 3)
+        k_mg_make_low<3, float><<<grid_for(pad), kBlock, 0, s>>>(S.nbrows, pad, S.sptr, S.nst, S.stencil, v64, dg64,
+                                                                 1.0, Lv.kv32, Lv.kdgc);
+      else
+        k_mg_make_low<1, float><<<grid_for(pad), kBlock, 0, s>>>(S.nbrows, pad, S.sptr, S.nst, S.stencil, v64, dg64,
+                                                                 1.0, Lv.kv32, Lv.kdgc);
       e = hipGetLastError();
       if (e == hipSuccess) e = hipStreamSynchronize(s);
       if (e != hipSuccess) {
@@ -876,7 +1085,7 @@ extern "C" int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, 
       }
     }
     if (precision == 32) {
-      e = hipMemcpy(dg.data(), half ? Lv.dgh : S.dgc, sizeof(double) * nbb, hipMemcpyDeviceToHost);
+      e = hipMemcpy(dg.data(), half ? Lv.dgh : ldiag<float>(Lv), sizeof(double) * nbb, hipMemcpyDeviceToHost);
       if (e != hipSuccess || !invert_blocks(bs, Lv.nb, dg, inv_t)) {
         rc = fail(CPM_EINVAL, "cpm_mg_create: singular diagonal block (FP32 form)");
         break;
@@ -911,6 +1120,14 @@ extern "C" int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, 
         (void)hipFree(p);
         M->allocs.erase(std::find(M->allocs.begin(), M->allocs.end(), p));
       }
+    }
+    if (Lv.kv64 && precision != 64) {
+      // the FP64 recovered stiffness served the conversions and the estimate
+      for (void* p : {(void*)Lv.kv64, (void*)Lv.kdg64}) {
+        (void)hipFree(p);
+        M->allocs.erase(std::find(M->allocs.begin(), M->allocs.end(), p));
+      }
+      Lv.kv64 = Lv.kdg64 = nullptr;
     }
     M->L.push_back(Lv);
   }
@@ -977,10 +1194,31 @@ extern "C" int cpm_mg_apply(cpm_mg* M, const double* r, double* z) {
   CgState* st = ctx->cg + 15;  // a live state so that the gated kernels run
   CgState h{};
   CPM_HIP(hipMemcpy(st, &h, sizeof(CgState), hipMemcpyHostToDevice));
+  const MgLevel& L0 = M->L[0];
+  const int gv = grid_for(L0.n);
+  if (M->floating) {
+    // the preconditioner of the floating hierarchy: F^T V(F r) + e_0 r_0
+    if (M->bs == 3)
+      k_pcg_rsum<3, 1><<<gv, kBlock, 0, ctx->stream>>>(L0.n, r, st, ctx->partials);
+    else
+      k_pcg_rsum<1, 1><<<gv, kBlock, 0, ctx->stream>>>(L0.n, r, st, ctx->partials);
+  }
   vcycle_any(M, r, st);
   CPM_HIP(hipGetLastError());
-  const MgLevel& L0 = M->L[0];
-  if (M->prec == 64) {
+  if (M->floating) {
+    if (M->bs == 3) {
+      if (M->prec == 32)
+        k_mg_float_out<3, float><<<gv, kBlock, 0, ctx->stream>>>(L0.n, (const float*)L0.x, r, z);
+      else
+        k_mg_float_out<3, double><<<gv, kBlock, 0, ctx->stream>>>(L0.n, (const double*)L0.x, r, z);
+    } else {
+      if (M->prec == 32)
+        k_mg_float_out<1, float><<<gv, kBlock, 0, ctx->stream>>>(L0.n, (const float*)L0.x, r, z);
+      else
+        k_mg_float_out<1, double><<<gv, kBlock, 0, ctx->stream>>>(L0.n, (const double*)L0.x, r, z);
+    }
+    CPM_HIP(hipGetLastError());
+  } else if (M->prec == 64) {
     CPM_HIP(hipMemcpyAsync(z, L0.x, 8 * L0.n, hipMemcpyDeviceToDevice, ctx->stream));
   } else {
     std::vector<float> f(L0.n);

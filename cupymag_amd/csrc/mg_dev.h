@@ -38,18 +38,36 @@ __device__ __forceinline__ bool all_done(const CgState* st) {
 // First Chebyshev step.  ZERO: x0 = 0, so r = b (FROM64: b is the FP64
 // outer residual, also stored as this level's b), d = c0 D^-1 r, x = d.
 // Otherwise r is the current residual: d = c0 D^-1 r, x += d.
+// Floating hierarchy (fpart non-null, level 0 from b64): node 0 takes
+// b64_0 - sum_i b64_i per component, so that the right-hand side of the
+// periodic problem sums to zero (F b, F = I - e_0 1^T); the sums are the
+// fn partials fpart[v * fstride + k] of pcg_rsum_rows, added up by the
+// first workgroup of the range, which holds node 0.
 template <int BS, int K, class T, bool ZERO, bool FROM64>
 __device__ __forceinline__ void mg_first_rows(int64_t lb, int64_t lg, int32_t nb, const double* __restrict__ b64,
                                               T* __restrict__ b, T* __restrict__ r, T* __restrict__ d,
-                                              T* __restrict__ x, const T* __restrict__ dinv, double c0) {
+                                              T* __restrict__ x, const T* __restrict__ dinv, double c0,
+                                              const double* __restrict__ fpart = nullptr, int64_t fstride = 0,
+                                              int32_t fn = 0) {
   constexpr int NV = BS * K;
+  double fsum[NV];
+#pragma unroll
+  for (int c = 0; c < NV; ++c) fsum[c] = 0.0;
+  if (FROM64 && fpart && lb == 0) {
+    __shared__ double lds[NV * (kBlock / 64)];
+    for (int k = threadIdx.x; k < fn; k += kBlock)
+#pragma unroll
+      for (int c = 0; c < NV; ++c) fsum[c] += fpart[c * fstride + k];
+    block_sum_t<NV, kBlock>(fsum, lds);  // totals in thread 0, the one that takes node 0
+  }
   const int64_t stride = lg * kBlock;
   for (int64_t i = lb * kBlock + threadIdx.x; i < nb; i += stride) {
     double rv[NV], dv[NV];
 #pragma unroll
     for (int c = 0; c < NV; ++c) {
       if (ZERO) {
-        const double bv = FROM64 ? b64[i * NV + c] : (double)b[i * NV + c];
+        double bv = FROM64 ? b64[i * NV + c] : (double)b[i * NV + c];
+        if (FROM64 && fpart && i == 0) bv -= fsum[c];
         if (FROM64) b[i * NV + c] = (T)bv;
         r[i * NV + c] = (T)bv;
         rv[c] = (double)(T)bv;
@@ -387,27 +405,51 @@ __device__ __forceinline__ void pcg_xr_rows(int64_t lb, int64_t lg, int64_t n, c
   }
 }
 
+// acc[NV] += per-component sums of r (vector entry i to component i % NV)
+template <int NV>
+__device__ __forceinline__ void pcg_rsum_rows(int64_t lb, int64_t lg, int64_t n, const double* __restrict__ r,
+                                              double (&acc)[NV]) {
+  const int64_t stride = lg * kBlock;
+  for (int64_t i = lb * kBlock + threadIdx.x; i < n; i += stride) acc[i % NV] += r[i];
+}
+
+// Entry i of the preconditioned residual.  FB = 0: the V-cycle's output y.
+// Floating hierarchy (FB = block size, one column): z = F^T y + e_0 r_0,
+// i.e. y shifted by its node-0 value (so that node 0 stays 0) and r_0 at
+// node 0 (the anchor's identity row).
+template <int FB, class T>
+__device__ __forceinline__ double pcg_z(int64_t i, const T* __restrict__ z, const double* __restrict__ r) {
+  if constexpr (FB == 0) {
+    return (double)z[i];
+  } else {
+    return (i < FB) ? r[i] : (double)z[i] - (double)z[i % FB];
+  }
+}
+
 // acc[j] = r_j.z_j, acc[K + j] = r_old_j.z_j (the latter after the first iteration)
-template <int K, class T>
+template <int K, class T, int FB = 0>
 __device__ __forceinline__ void pcg_dot_rows(int64_t lb, int64_t lg, int64_t n, const CgState* st,
                                              const double* __restrict__ r, const double* __restrict__ rold,
                                              const T* __restrict__ z, double (&acc)[2 * K]) {
+  static_assert(FB == 0 || K == 1, "floating hierarchies solve one column");
   bool first[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) first[j] = st[j].it == 0;
   const int64_t stride = lg * kBlock;
   for (int64_t i = lb * kBlock + threadIdx.x; i < n; i += stride) {
     const int j = (int)(i % K);
-    const double zi = (double)z[i];
+    const double zi = pcg_z<FB, T>(i, z, r);
     acc[j] += r[i] * zi;
     if (!first[j]) acc[K + j] += rold[i] * zi;
   }
 }
 
 // live columns: p = z (first iteration) or z + beta p
-template <int K, class T>
+template <int K, class T, int FB = 0>
 __device__ __forceinline__ void pcg_pupd_rows(int64_t lb, int64_t lg, int64_t n, const CgState* st,
-                                              const T* __restrict__ z, double* __restrict__ p) {
+                                              const T* __restrict__ z, double* __restrict__ p,
+                                              const double* __restrict__ r) {
+  static_assert(FB == 0 || K == 1, "floating hierarchies solve one column");
   double beta[K];
   bool live[K], first[K];
 #pragma unroll
@@ -416,7 +458,8 @@ __device__ __forceinline__ void pcg_pupd_rows(int64_t lb, int64_t lg, int64_t n,
   for (int64_t i = lb * kBlock + threadIdx.x; i < n; i += stride) {
     const int j = (int)(i % K);
     if (!live[j]) continue;
-    p[i] = first[j] ? (double)z[i] : (double)z[i] + beta[j] * p[i];
+    const double zi = pcg_z<FB, T>(i, z, r);
+    p[i] = first[j] ? zi : zi + beta[j] * p[i];
   }
 }
 

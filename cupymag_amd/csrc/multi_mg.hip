@@ -43,6 +43,7 @@ struct OuterSeg {
   int32_t use_x0, maxiter;
   double tol;
   double delta;          // > 0: FP32 products with FP64 residual replacements (mg.hip)
+  int32_t floating;      // floating hierarchy (mg.hip): F r into the V-cycle, F^T on its output
 };
 
 struct SmoothSeg {
@@ -64,6 +65,9 @@ struct SmoothSeg {
   const void* dinvc;  // next level's D^-1
   double c0c;         // next level's first Chebyshev coefficient
   int32_t fuse;       // the next level is not the coarsest: restriction + its first step
+  const double* fpart;  // depth 0 of a floating hierarchy: partials of the sums of b64
+  int64_t fstride;
+  int32_t fn;
   const CgState* st;
   int32_t blk0, nblk;
 };
@@ -86,7 +90,8 @@ __global__ void __launch_bounds__(kBlock) kbm_first(const SmoothSeg* __restrict_
   const SmoothSeg& d = seg[map[blockIdx.x]];
   if (all_done<K>(d.st)) return;
   mg_first_rows<BS, K, T, ZERO, FROM64>((int64_t)blockIdx.x - d.blk0, d.nblk, d.nb, d.b64, (T*)d.b, (T*)d.r,
-                                        (T*)d.d0, (T*)d.x, (const T*)d.dinv, d.c0);
+                                        (T*)d.d0, (T*)d.x, (const T*)d.dinv, d.c0, FROM64 ? d.fpart : nullptr,
+                                        d.fstride, d.fn);
 }
 
 // MODE 0: Chebyshev step k (d_in = d[(k-1)&1], d_out = d[k&1]); MODE 1: the
@@ -214,7 +219,21 @@ __global__ void __launch_bounds__(kBlock) kbm_xr(const OuterSeg* __restrict__ se
   block_partial<K>(acc, partials);
 }
 
-template <int K, class T>
+// floating systems: partials of the per-component sums of r (read by the
+// first Chebyshev step of depth 0)
+template <int BS, int K>
+__global__ void __launch_bounds__(kBlock) kbm_rsum(const OuterSeg* __restrict__ seg, const int32_t* __restrict__ map,
+                                                   double* partials) {
+  const OuterSeg& d = seg[map[blockIdx.x]];
+  if (!d.floating || all_done<K>(d.st)) return;
+  double acc[BS * K];
+#pragma unroll
+  for (int c = 0; c < BS * K; ++c) acc[c] = 0.0;
+  pcg_rsum_rows<BS * K>((int64_t)blockIdx.x - d.blk0, d.nblk, d.n, d.r, acc);
+  block_partial<BS * K>(acc, partials);
+}
+
+template <int BS, int K, class T>
 __global__ void __launch_bounds__(kBlock) kbm_dot(const OuterSeg* __restrict__ seg, const int32_t* __restrict__ map,
                                                   double* partials) {
   const OuterSeg& d = seg[map[blockIdx.x]];
@@ -222,15 +241,30 @@ __global__ void __launch_bounds__(kBlock) kbm_dot(const OuterSeg* __restrict__ s
   double acc[2 * K];
 #pragma unroll
   for (int j = 0; j < 2 * K; ++j) acc[j] = 0.0;
-  pcg_dot_rows<K, T>((int64_t)blockIdx.x - d.blk0, d.nblk, d.n, d.st, d.r, d.rold, (const T*)d.z, acc);
+  const int64_t lb = (int64_t)blockIdx.x - d.blk0;
+  bool done = false;
+  if constexpr (K == 1) {
+    if (d.floating) {
+      pcg_dot_rows<K, T, BS>(lb, d.nblk, d.n, d.st, d.r, d.rold, (const T*)d.z, acc);
+      done = true;
+    }
+  }
+  if (!done) pcg_dot_rows<K, T, 0>(lb, d.nblk, d.n, d.st, d.r, d.rold, (const T*)d.z, acc);
   block_partial<2 * K>(acc, partials);
 }
 
-template <int K, class T>
+template <int BS, int K, class T>
 __global__ void __launch_bounds__(kBlock) kbm_pupd(const OuterSeg* __restrict__ seg, const int32_t* __restrict__ map) {
   const OuterSeg& d = seg[map[blockIdx.x]];
   if (all_done<K>(d.st)) return;
-  pcg_pupd_rows<K, T>((int64_t)blockIdx.x - d.blk0, d.nblk, d.n, d.st, (const T*)d.z, d.p);
+  const int64_t lb = (int64_t)blockIdx.x - d.blk0;
+  if constexpr (K == 1) {
+    if (d.floating) {
+      pcg_pupd_rows<K, T, BS>(lb, d.nblk, d.n, d.st, (const T*)d.z, d.p, d.r);
+      return;
+    }
+  }
+  pcg_pupd_rows<K, T, 0>(lb, d.nblk, d.n, d.st, (const T*)d.z, d.p, d.r);
 }
 
 enum : int { MRED_INIT = 0, MRED_ALPHA = 1, MRED_RR = 2, MRED_RZ = 3, MRED_TRUE = 4 };
@@ -380,10 +414,22 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
     o.sblk0 = (int32_t)W.smap.size();
     o.snblk = sell_grid(S);
     o.use_x0 = y.use_x0, o.maxiter = y.maxiter, o.tol = y.tol;
+    o.floating = M->floating;
     for (int b = 0; b < o.nblk; ++b) O.map.push_back(j);
     for (int b = 0; b < o.snblk; ++b) W.smap.push_back(j);
     O.h.push_back(o);
   }
+  // partials: 2K per vector workgroup (BS*K for the sums of a floating
+  // system's residual), K per SpMV workgroup
+  {
+    const size_t need = (size_t)std::max(2 * K, BS * K) * std::max(O.map.size(), W.smap.size());
+    if (need > W.cap_part) {
+      if (W.partials) CPM_HIP(hipFree(W.partials));
+      W.cap_part = std::max(need, W.cap_part + W.cap_part / 2);
+      CPM_HIP(hipMalloc(&W.partials, sizeof(double) * W.cap_part));
+    }
+  }
+  double* pt = W.partials;
   // V-cycle segments per depth
   if ((int)W.smooth.size() < depth + 1) W.smooth.resize(depth + 1);
   if ((int)W.dense.size() < depth + 1) W.dense.resize(depth + 1);
@@ -400,8 +446,12 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
         const Sell& S = Lv.A->sell;
         SmoothSeg g{};
         g.sptr = S.sptr, g.col = S.col;
-        g.val = M->half ? (const void*)Lv.v16 : (M->prec == 32) ? (const void*)S.v32 : (const void*)S.v64;
-        g.dg = M->half ? Lv.dgh : (M->prec == 32) ? S.dgc : S.dg64;
+        // smoothing values: FP16 form, or (recovered stiffness of a
+        // floating level 0, else the level's operator) FP32 / FP64
+        g.val = M->half            ? (const void*)Lv.v16
+                : (M->prec == 32) ? (Lv.kv32 ? (const void*)Lv.kv32 : (const void*)S.v32)
+                                  : (Lv.kv64 ? (const void*)Lv.kv64 : (const void*)S.v64);
+        g.dg = M->half ? Lv.dgh : (M->prec == 32) ? (Lv.kdgc ? Lv.kdgc : S.dgc) : (Lv.kdg64 ? Lv.kdg64 : S.dg64);
         g.vscale = M->half ? Lv.vscale : 1.0;
         g.ga = GridArg{S.dims[0], S.dims[1], S.dims[2], S.nst, S.soff};
         g.nb = S.nbrows, g.stencil = S.stencil;
@@ -416,6 +466,12 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
         g.fuse = (d + 1 < last) ? 1 : 0;
         g.rc = Lc.r, g.dc = Lc.d0, g.dinvc = Lc.dinv;
         g.c0c = (d + 1 < last) ? M->c0[d + 1] : 0.0;
+        if (d == 0 && M->floating) {
+          // the partials kbm_rsum leaves for this system's vector workgroups
+          g.fpart = pt + O.h[j].blk0;
+          g.fstride = (int64_t)O.map.size();
+          g.fn = O.h[j].nblk;
+        }
         g.st = W.st + j * K;
         g.blk0 = (int32_t)Sm.map.size();
         g.nblk = sell_grid(S);
@@ -441,19 +497,15 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
   }
   CPM_HIP(hipMemcpyAsync(W.dsmap, W.smap.data(), sizeof(int32_t) * W.smap.size(), hipMemcpyHostToDevice, s));
   const int G = O.G, Gs = (int)W.smap.size();
-  const size_t need = 2 * K * (size_t)std::max(G, Gs);
-  if (need > W.cap_part) {
-    if (W.partials) CPM_HIP(hipFree(W.partials));
-    W.cap_part = std::max(need, W.cap_part + W.cap_part / 2);
-    CPM_HIP(hipMalloc(&W.partials, sizeof(double) * W.cap_part));
-  }
-  double* pt = W.partials;
   bool any_x0 = false, any_rel = false;
   int maxit = 0;
   for (const auto& y : sys)
     any_x0 |= y.use_x0 != 0, any_rel |= y.delta > 0.0, maxit = std::max(maxit, (int)y.maxiter);
 
+  bool any_fl = false;
+  for (const auto& y : sys) any_fl |= y.M->floating != 0;
   auto vcycle = [&]() {
+    if (any_fl) kbm_rsum<BS, K><<<G, kBlock, 0, s>>>(O.d, O.dmap, pt);
     for (int d = 0; d < depth; ++d) {
       auto& Sm = W.smooth[d];
       if (Sm.G) {
@@ -482,12 +534,12 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
   kbm_red<K, MRED_INIT><<<ns, kRedBlock, 0, s>>>(O.d, G, pt);
   if (any_x0) kbm_zero_b<K><<<G, kBlock, 0, s>>>(O.d, O.dmap);
   vcycle();
-  kbm_dot<K, T><<<G, kBlock, 0, s>>>(O.d, O.dmap, pt);
+  kbm_dot<BS, K, T><<<G, kBlock, 0, s>>>(O.d, O.dmap, pt);
   kbm_red<K, MRED_RZ><<<ns, kRedBlock, 0, s>>>(O.d, G, pt);
   CPM_HIP(hipGetLastError());
   auto issue = [&](int todo) -> int {
     for (int k = 0; k < todo; ++k) {
-      kbm_pupd<K, T><<<G, kBlock, 0, s>>>(O.d, O.dmap);
+      kbm_pupd<BS, K, T><<<G, kBlock, 0, s>>>(O.d, O.dmap);
       kbm_spmv<BS, K, false><<<Gs, kBlock, 0, s>>>(O.d, W.dsmap, pt);
       kbm_red<K, MRED_ALPHA><<<ns, kRedBlock, 0, s>>>(O.d, Gs, pt);
       kbm_xr<K><<<G, kBlock, 0, s>>>(O.d, O.dmap, pt);
@@ -497,7 +549,7 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
         kbm_red<K, MRED_TRUE><<<ns, kRedBlock, 0, s>>>(O.d, Gs, pt);
       }
       vcycle();
-      kbm_dot<K, T><<<G, kBlock, 0, s>>>(O.d, O.dmap, pt);
+      kbm_dot<BS, K, T><<<G, kBlock, 0, s>>>(O.d, O.dmap, pt);
       kbm_red<K, MRED_RZ><<<ns, kRedBlock, 0, s>>>(O.d, G, pt);
     }
     CPM_HIP(hipGetLastError());
@@ -553,6 +605,8 @@ int multi_mg_pcg(cpm_ctx* ctx, MultiMgSolver& W, const std::vector<MultiMgSys>& 
   }
   if (K != 1 && !(K == 3 && M0->bs == 1))
     return fail(CPM_EINVAL, "batched multigrid CG: 1 right-hand side, or 3 for scalar operators");
+  for (const auto& y : sys)
+    if (y.M->floating && K != 1) return fail(CPM_EINVAL, "batched multigrid CG: a floating hierarchy solves one column");
   for (size_t i = 0; i < sys.size(); ++i)
     for (size_t j = i + 1; j < sys.size(); ++j)
       if (sys[i].M == sys[j].M)

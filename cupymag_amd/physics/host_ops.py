@@ -44,7 +44,11 @@ def voigt_C(c11, c12, c44, rot111=False):
 
 
 class HostOps:
-    def __init__(self, coords, elements, gid, ndof, defect_dofs, alpha1, alpha2, Cvoigt=None, nthreads=None):
+    def __init__(self, coords, elements, gid, ndof, defect_dofs, alpha1, alpha2, Cvoigt=None, nthreads=None,
+                 anchors=True):
+        """anchors=False: A_demag and A_el stay the singular periodic
+        stiffness (no identity rows at DOF 0), as the floating multigrid
+        levels use them."""
         coords = np.ascontiguousarray(coords, dtype=np.float64)
         elements = np.ascontiguousarray(elements, dtype=np.int32)
         gid = np.ascontiguousarray(gid, dtype=np.int32)
@@ -61,7 +65,8 @@ class HostOps:
         h = C.c_void_p()
         check(lib().cpm_host_assemble(ptr(coords), coords.shape[0], ptr(elements), elements.shape[0], npe,
                                       elements.shape[1], ptr(gid), int(ndof), ptr(mask), ptr(coef),
-                                      int(self.elastic), int(nt), C.byref(h)), "cpm_host_assemble")
+                                      int(self.elastic) | (0 if anchors else 2), int(nt), C.byref(h)),
+              "cpm_host_assemble")
         self.h = h
         self.n = int(ndof)
 

@@ -28,7 +28,12 @@ from ..mesh.periodic import periodic_node_map, structured_dims
 class KuhnMultigrid:
     """A multigrid hierarchy (levels[0] is the operator being solved)."""
 
-    def __init__(self, levels, coarse_dense, precision=32, degree=2, ratio=10.0):
+    def __init__(self, levels, coarse_dense, precision=32, degree=2, ratio=10.0, floating=False):
+        """floating: levels[0] is anchored, levels[1:] are the periodic
+        stiffness and coarse_dense its pseudo-inverse (cpm_mg_create tells
+        the kinds apart from the operators themselves; the flag is kept for
+        info())."""
+        self.floating = bool(floating)
         self.levels = list(levels)
         self.ctx = levels[0].ctx
         self.precision = int(precision)
@@ -45,7 +50,8 @@ class KuhnMultigrid:
         n = C.c_int()
         lm = (C.c_double * 16)()
         check(lib().cpm_mg_info(self.h, C.byref(n), lm, 16), "cpm_mg_info")
-        return dict(levels=n.value, lmax=list(lm[:n.value]), dims=[getattr(L, "grid_dims", None) for L in self.levels])
+        return dict(levels=n.value, lmax=list(lm[:n.value]), dims=[getattr(L, "grid_dims", None) for L in self.levels],
+                    floating=self.floating)
 
     def set_graphs(self, on):
         """Replay the PCG iterations from captured hipGraph chunks (default)
@@ -121,13 +127,34 @@ def _interleave_perm(n, bs):
     return (np.arange(bs)[None, :] * n + i[:, None]).ravel()
 
 
-def build_hierarchies(SimOp, precision=32, degree=2, ratio=10.0, coarsest_max=8, nthreads=None):
+def _pinv_periodic(A, bs):
+    """Pseudo-inverse of a singular periodic stiffness whose null space is
+    the bs translations (node-major interleaved order): with Q their
+    orthonormal basis, A^+ = (A + s Q Q^T)^-1 - Q Q^T / s for any s > 0
+    (s = the mean diagonal entry, for conditioning)."""
+    n = A.shape[0] // bs
+    Q = np.zeros((A.shape[0], bs))
+    for d in range(bs):
+        Q[d::bs, d] = 1.0 / np.sqrt(n)
+    sc = float(np.mean(np.diag(A)))
+    QQ = Q @ Q.T
+    return np.linalg.inv(A + sc * QQ) - QQ / sc
+
+
+def build_hierarchies(SimOp, precision=32, degree=2, ratio=10.0, coarsest_max=8, nthreads=None, floating=True):
     """{"A_demag": KuhnMultigrid, "A1": KuhnMultigrid (Gauss-Seidel operator
     M + alpha1 K; coarse levels without the defect surgery), "A_el":
     KuhnMultigrid (with coupling)} for a SimulationOperators object on an
     n^3 Kuhn mesh with n > coarsest_max,
     or None when the mesh has no such hierarchy (not a Kuhn mesh in
-    kuhn_mesh order, or too small to coarsen)."""
+    kuhn_mesh order, or too small to coarsen).
+
+    floating (default): the coarse levels of the anchored operators (A_demag,
+    A_el) are the singular periodic stiffness, without the anchor, and the
+    coarsest solve is its pseudo-inverse; the V-cycle then solves the
+    periodic problem and the anchor is applied around it (cpm_mg_create),
+    which keeps the iteration counts flat as the mesh grows.  False: anchored
+    coarse levels (node 0 left out of the transfers)."""
     from ..physics.host_ops import HostOps, voigt_C
     n = kuhn_cells(SimOp.node_coords_np, SimOp.elements_np)
     if n is None or SimOp.grid_dims != (n, n, n):
@@ -153,7 +180,7 @@ def build_hierarchies(SimOp, precision=32, degree=2, ratio=10.0, coarsest_max=8,
         dims = structured_dims(coords, gid)
         if dims != (m, m, m):
             return None
-        H = HostOps(coords, el, gid, ndof, None, SimOp.alpha1, SimOp.alpha2, Cv, nthreads)
+        H = HostOps(coords, el, gid, ndof, None, SimOp.alpha1, SimOp.alpha2, Cv, nthreads, anchors=not floating)
         last = li == len(sizes) - 2
         for name, bs in (("A_demag", 1), ("A_el", 3), ("A1", 1)):
             if name not in fine:
@@ -169,6 +196,8 @@ def build_hierarchies(SimOp, precision=32, degree=2, ratio=10.0, coarsest_max=8,
                 import scipy.sparse as sps
                 A = sps.csr_matrix((np.array(v), np.array(ix), np.array(ip)), shape=shape).toarray()
                 perm = _interleave_perm(ndof, bs)
-                dense[name] = np.linalg.inv(A[np.ix_(perm, perm)])
+                Ai = A[np.ix_(perm, perm)]
+                dense[name] = _pinv_periodic(Ai, bs) if floating and name != "A1" else np.linalg.inv(Ai)
         H.free()
-    return {name: KuhnMultigrid(levels[name], dense[name], precision, degree, ratio) for name in fine}
+    return {name: KuhnMultigrid(levels[name], dense[name], precision, degree, ratio,
+                                floating=floating and name != "A1") for name in fine}

@@ -213,9 +213,15 @@ int cpm_weighted_sum(cpm_ctx* ctx, int32_t n, int ncomp, const double* W, const 
  * same box for odd n); the last level is solved with coarse_inv (dense
  * n x n, row-major, in the SELL vector order: node-major interleaved for
  * bs 3).  Transfers: P1 interpolation (nested for even n, at the nominal
- * node positions for odd n) and its transpose; node 0 is left out of both
- * when row 0 of levels[0] is an identity row (the anchor of A_demag and
- * A_el).  Smoother: Chebyshev of `degree` on D^-1 A over [lmax/ratio, lmax],
+ * node positions for odd n) and its transpose.  When row 0 of levels[0] is
+ * an identity row (the anchor of A_demag and A_el), the coarse levels are
+ * either all anchored the same way (node 0 is then left out of both
+ * transfers) or all the singular periodic stiffness K, with coarse_inv its
+ * pseudo-inverse: a floating hierarchy, whose V-cycle V solves the periodic
+ * problem (level 0 smooths with K, recovered on the device from levels[0]
+ * by K's annihilation of the translations) and whose preconditioner is
+ * F^T V(F r) + e_0 r_0, F = I - e_0 1^T per component (one column only).
+ * Smoother: Chebyshev of `degree` on D^-1 A over [lmax/ratio, lmax],
  * lmax = 1.1 x a power-iteration estimate.  precision: the arithmetic of
  * the V-cycle (the outer CG is FP64): 64, 32, or 16 = FP32 vectors with the
  * level operators' off-diagonal values stored in FP16 (scaled by a power of
@@ -227,7 +233,8 @@ int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, const doubl
 int cpm_mg_destroy(cpm_mg* mg);
 /* levels and the lambda_max estimate of each level (host, up to max_levels) */
 int cpm_mg_info(const cpm_mg* mg, int* nlevels, double* lmax, int max_levels);
-/* z = one V-cycle applied to r (device pointers, SELL vector order); synchronises */
+/* z = the preconditioner applied to r: one V-cycle (floating hierarchy:
+ * F^T V(F r) + e_0 r_0); device pointers, SELL vector order; synchronises */
 int cpm_mg_apply(cpm_mg* mg, const double* r, double* z);
 /* CG on levels[0] preconditioned by one V-cycle per iteration, FP64, on
  * the context's stream: CuPy's stopping rule on the recurrence residual
@@ -419,10 +426,17 @@ typedef struct cpm_host_ops cpm_host_ops;
  *   elems: (n_elems, elem_stride) int32, first npe columns are node ids.
  *   defect_dof: n bytes (1 = defect DOF) or NULL.
  *   coef: [alpha1, alpha2, C_voigt (36, row-major, already rotated)].
+ *   flags: CPM_HA_ELASTIC assembles A_el and F_el as well;
+ *   CPM_HA_NO_ANCHORS leaves out the anchors (A_demag row/column 0 and
+ *   A_el rows/columns {0, n, 2n} stay the periodic stiffness, F_el rows
+ *   {0, n, 2n} are kept): the singular operators that the floating
+ *   multigrid levels use (no counterpart in the reference).
  * Arrays are read back with cpm_host_ops_array. */
+#define CPM_HA_ELASTIC 1
+#define CPM_HA_NO_ANCHORS 2
 int cpm_host_assemble(const double* coords, int64_t n_nodes, const int32_t* elems, int64_t n_elems, int npe,
                       int elem_stride, const int32_t* gid, int32_t ndof, const uint8_t* defect_dof,
-                      const double* coef, int want_elastic, int nthreads, cpm_host_ops** out);
+                      const double* coef, int flags, int nthreads, cpm_host_ops** out);
 
 #define CPM_HA_PTR 0      /* int64 [n+1] pattern of A_demag, Fx, Fy, Fz */
 #define CPM_HA_IDX 1      /* int32 */

@@ -2,6 +2,7 @@
 import copy
 
 import numpy as np
+import scipy.sparse as sp
 
 BASE = {
     "physics": {"magnetic": {"ms": 1.2e6, "exchange_stiffness": 1.5e-11, "anisotropy_k1": -1e4},
@@ -73,3 +74,39 @@ def oracle_setup(cfg, tet_arrays=None):
     ops = fem.build_operators(coords, el, gid, p)
     Avg = fem.VolumeAverage(coords, el, gid)
     return p, ops, Avg, (coords, el, gid)
+
+
+def unanchor(A, bs):
+    """The periodic stiffness K of an anchored operator A (node 0's rows and
+    columns replaced by the identity): K annihilates the bs translations, so
+    K's block in node 0's column of row i != 0 is minus the sum of the
+    row's other blocks, row 0 is the transpose of that column (K symmetric)
+    and its diagonal block minus the sum of its other blocks.  Entries only
+    where A's pattern holds them."""
+    A = A.tocsr().astype(np.float64)
+    N = A.shape[0]
+    comp = np.arange(N) % bs
+    Ac = A.tocsc()
+    rows0 = set()
+    for e in range(bs):
+        rows0 |= set(Ac.indices[Ac.indptr[e]:Ac.indptr[e + 1]].tolist())
+    rows0 = np.array(sorted(r for r in rows0 if r >= bs), dtype=np.int64)
+    B = A.tolil()
+    B[:bs, :] = 0.0
+    B[:, :bs] = 0.0
+    B = B.tocsr()
+    col = np.zeros((N, bs))
+    for e in range(bs):
+        s_e = np.asarray(B[:, np.flatnonzero(comp == e)].sum(axis=1)).ravel()
+        col[rows0, e] = -s_e[rows0]
+    Cm = sp.lil_matrix((N, N))
+    for e in range(bs):
+        Cm[rows0, e] = col[rows0, e].reshape(-1, 1)
+    Cm = Cm.tocsr()
+    D0 = np.zeros((bs, bs))
+    for d in range(bs):
+        for e in range(bs):
+            D0[d, e] = -col[rows0[comp[rows0] == e], d].sum()
+    Dm = sp.lil_matrix((N, N))
+    Dm[:bs, :bs] = D0
+    return (B + Cm + Cm.T + Dm.tocsr()).tocsr()

@@ -20,7 +20,7 @@ import scipy.sparse.linalg as spla
 torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
-from tests.cases import config, oracle_setup, tilted_m  # noqa: E402
+from tests.cases import config, oracle_setup, tilted_m, unanchor  # noqa: E402
 
 
 def _setup(n=16, me=True, jitter=0.0):
@@ -95,17 +95,25 @@ def _host_vcycle(mats, dims, lmax, cinv, bs, degree, ratio, b, anchor=True):
     return V(0, b)
 
 
+@pytest.mark.parametrize("floating", [True, False])
 @pytest.mark.parametrize("n", [16, 18, 9])
 @pytest.mark.parametrize("name,bs", [("A_demag", 1), ("A_el", 3), ("A1", 1)])
-def test_vcycle_matches_host_reference(name, bs, n):
+def test_vcycle_matches_host_reference(name, bs, n, floating):
     """16 -> 8 (nested), 18 -> 9 -> 5 (nested, then non-nested), 9 -> 5
     (non-nested at the top): the device transfers equal the host P.  A1
-    (no anchor row) keeps node 0 in the transfers."""
+    (no anchor row) keeps node 0 in the transfers.  Floating hierarchies of
+    the anchored operators: the V-cycle on the periodic stiffness (level 0
+    recovered from the anchored operator on the device, here by unanchor),
+    node 0 in the transfers, the pseudo-inverse on the coarsest level,
+    applied as F^T V(F r) + e_0 r_0 with F = I - e_0 1^T; otherwise node 0
+    is left out of the transfers."""
     from cupymag_amd.mesh.coarsen import hierarchy_sizes
+    if floating and name == "A1":
+        pytest.skip("A1 has no anchor: one kind of hierarchy")
     _, _, _, S = _setup(n=n)
     sizes = hierarchy_sizes(n)
     for prec, tol in ((64, 1e-11), (32, 1e-5), (16, 3e-3)):
-        mg = S.multigrid(precision=prec)[name]
+        mg = S.multigrid(precision=prec, floating=floating)[name]
         info = mg.info()
         assert info["levels"] == len(sizes) and info["dims"][-1] == (sizes[-1],) * 3
         assert all(l > 0 for l in info["lmax"][:-1])
@@ -114,17 +122,34 @@ def test_vcycle_matches_host_reference(name, bs, n):
         from cupymag_amd.solvers.multigrid import _interleave_perm
         Ac = mg.levels[-1].to_scipy().toarray()
         perm = _interleave_perm(Ac.shape[0] // bs, bs)
-        cinv = np.linalg.inv(Ac[np.ix_(perm, perm)])
         anchored = name != "A1"
+        fl = floating and anchored
         rng = np.random.default_rng(5)
         r = rng.standard_normal(mats[0].shape[0])
-        if anchored:
+        if anchored and not fl:
             r[:bs] = 0.0  # anchor node
         z = mg.apply(torch.as_tensor(r, device="cuda")).cpu().numpy()
-        zr = _host_vcycle(mats, dims, info["lmax"], cinv, bs, mg.degree, mg.ratio, r, anchor=anchored)
+        if fl:
+            # singular periodic levels: row 0 of level 1 is no identity row
+            assert np.count_nonzero(mats[1][0].toarray()) > bs
+            mats[0] = unanchor(mats[0], bs)
+            cinv = np.linalg.pinv(Ac[np.ix_(perm, perm)])
+            Fr = r.copy()
+            for d in range(bs):
+                Fr[d] = r[d] - r[d::bs].sum()
+            y = _host_vcycle(mats, dims, info["lmax"], cinv, bs, mg.degree, mg.ratio, Fr, anchor=False)
+            zr = y.copy()
+            for d in range(bs):
+                zr[d::bs] -= y[d]
+            zr[:bs] = r[:bs]
+        else:
+            cinv = np.linalg.inv(Ac[np.ix_(perm, perm)])
+            zr = _host_vcycle(mats, dims, info["lmax"], cinv, bs, mg.degree, mg.ratio, r, anchor=anchored)
         err = np.abs(z - zr).max() / np.abs(zr).max()
         assert err < tol, (prec, err)
-        if anchored:
+        if fl:
+            np.testing.assert_array_equal(z[:bs], r[:bs])
+        elif anchored:
             assert np.all(z[:bs] == 0.0)
         elif prec == 64:
             # with node 0 left out of the transfers the V-cycle differs (so

@@ -143,3 +143,43 @@ def test_non_nested_prolongation_interpolates_coarse_p1_functions():
         np.testing.assert_allclose(P @ np.ones(len(uc)), 1.0, rtol=0, atol=1e-14)
         Pa = prolongation(tuple(nf), anchor=True)
         assert Pa[0].nnz == 0 and Pa[:, 0].nnz == 0 and (P - Pa)[1:, 1:].nnz == 0
+
+
+def test_unanchored_assembly_and_anchor_recovery():
+    """The floating levels' operators: the host assembler without anchors
+    gives the oracle's periodic stiffness (scalar) and the anchored
+    operators differ from it only in node 0's rows and columns; recovering
+    K from the anchored operator (tests.cases.unanchor, the rule the device
+    applies to level 0) gives it back, and K annihilates the translations."""
+    from cupymag_amd.mesh.periodic import periodic_node_map
+    from cupymag_amd.physics.host_ops import HostOps, voigt_C
+    from tests.cases import unanchor
+    coords, tets = kuhn_mesh(6, jitter=0.15, seed=2)
+    el = _elements(tets)
+    gid = periodic_node_map(coords, coords.min(axis=0), coords.max(axis=0))
+    n = int(gid.max()) + 1
+    Cv = voigt_C(2.4e11, 1.4e11, 1.2e11, True) / 1e11
+    ops = {}
+    for anchors in (True, False):
+        H = HostOps(coords, el, gid, n, None, 2.0, 1.0, Cv, 2, anchors=anchors)
+        ops[anchors] = {k: H.scipy(k) for k in ("A_demag", "A_el", "A1")}
+        H.free()
+    K = _K(coords, tets)
+    Ku = ops[False]["A_demag"]
+    assert abs(Ku - K).max() <= 1e-12 * abs(K).max()
+    np.testing.assert_array_equal(ops[True]["A1"].toarray(), ops[False]["A1"].toarray())
+    for name, bs in (("A_demag", 1), ("A_el", 3)):
+        perm = (np.arange(bs)[None, :] * n + np.arange(n)[:, None]).ravel()   # node-major interleaved
+        As = ops[True][name][perm][:, perm].tocsr()   # the pattern keeps the anchor's explicit zeros
+        Aa = As.toarray()
+        Au = ops[False][name][perm][:, perm].toarray()
+        keep = np.arange(bs, bs * n)
+        np.testing.assert_array_equal(Aa[np.ix_(keep, keep)], Au[np.ix_(keep, keep)])
+        np.testing.assert_array_equal(Aa[:bs, :bs], np.eye(bs))
+        assert np.all(Aa[:bs, bs:] == 0.0) and np.all(Aa[bs:, :bs] == 0.0)
+        Kr = unanchor(As, bs).toarray()
+        assert np.abs(Kr - Au).max() <= 1e-12 * np.abs(Au).max(), name
+        for d in range(bs):
+            t = np.zeros(bs * n)
+            t[d::bs] = 1.0
+            assert np.abs(Au @ t).max() <= 1e-12 * np.abs(Au).max()
