@@ -341,7 +341,9 @@ def run_batch(args):
             t1 = time.perf_counter()
             reps = b.step(H, [False] * len(sims))
             lat.append(time.perf_counter() - t1)
-            log(f"[bench] {name} step {k}: {lat[-1]:.3f}s max iters {max(max(r.iters) for r in reps)}")
+            worst = max(range(len(reps)), key=lambda j: max(reps[j].iters))
+            log(f"[bench] {name} step {k}: {lat[-1]:.3f}s max iters {max(reps[worst].iters)} "
+                f"(n = {descs[worst]['n']}, iterations {list(reps[worst].iters)})")
         torch.cuda.synchronize()
         el = time.perf_counter() - t
         err = max(float((s.get("M") - m).abs().max().item()) for s, m in zip(sims, m_seq))

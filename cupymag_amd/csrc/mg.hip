@@ -562,9 +562,12 @@ static void vcycle_any(cpm_mg* M, const double* b64, const CgState* st) {
 
 template <int BS>
 // 20 power iterations on D^-1 A without host round trips (the norm stays on
-// the device, in lam_dev); one read at the end.
+// the device, in lam_dev); one read at the end.  The partials buffer
+// (sell_grid(S) doubles) belongs to the hierarchy being built, not to the
+// context, so that hierarchies built from several host threads on one
+// context (their kernels interleaved on its stream) do not mix their sums.
 static int power_lmax(cpm_mg* M, MgLevel& Lv, double* x, double* y, const double* dinv64, double* lam_dev,
-                      double* lmax) {
+                      double* part, double* lmax) {
   cpm_ctx* ctx = M->ctx;
   hipStream_t s = ctx->stream;
   const Sell& S = Lv.A->sell;
@@ -587,18 +590,17 @@ static int power_lmax(cpm_mg* M, MgLevel& Lv, double* x, double* y, const double
   for (int it = 0; it < 20; ++it) {
     if (S.stencil)
       k_mg_power<BS, true><<<g, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, v64, dg64, grid_arg(S), x, y, dinv64,
-                                                ctx->partials);
+                                                part);
     else
       k_mg_power<BS, false><<<g, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, v64, dg64, grid_arg(S), x, y, dinv64,
-                                                 ctx->partials);
-    k_power_norm<<<1, kRedBlock, 0, s>>>(g, ctx->partials, lam_dev);
+                                                 part);
+    k_power_norm<<<1, kRedBlock, 0, s>>>(g, part, lam_dev);
     k_scale_inv<<<grid_for(Lv.n), kBlock, 0, s>>>(Lv.n, y, x, lam_dev);
   }
   CPM_HIP(hipGetLastError());
   double lam = 0.0;
-  CPM_HIP(hipMemcpyAsync(ctx->pinned, lam_dev, sizeof(double), hipMemcpyDeviceToHost, s));
+  CPM_HIP(hipMemcpyAsync(&lam, lam_dev, sizeof(double), hipMemcpyDeviceToHost, s));
   CPM_HIP(hipStreamSynchronize(s));
-  std::memcpy(&lam, ctx->pinned, sizeof(double));
   if (!(lam > 0.0)) return fail(CPM_EINVAL, "multigrid: operator annihilates the power-iteration vector");
   *lmax = lam;
   return CPM_OK;
@@ -1104,7 +1106,8 @@ extern "C" int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, 
       void *vx = nullptr, *vy = nullptr, *vd = nullptr;
       A(&vx, 8 * Lv.n);
       A(&vy, 8 * Lv.n);
-      A(&vd, 8 * (nbb + 1));  // D^-1 (FP64) and the norm of the iterate
+      const int gp = sell_grid(S);
+      A(&vd, 8 * (nbb + 1 + gp));  // D^-1 (FP64), the norm of the iterate, the partials
       if (rc != CPM_OK) break;
       px = (double*)vx, py = (double*)vy, dinv64 = (double*)vd;
       e = hipMemcpy(dinv64, inv64.data(), sizeof(double) * nbb, hipMemcpyHostToDevice);
@@ -1113,8 +1116,8 @@ extern "C" int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, 
         break;
       }
       double* lam_dev = dinv64 + nbb;
-      rc = bs == 3 ? power_lmax<3>(M, Lv, px, py, dinv64, lam_dev, &Lv.lmax)
-                   : power_lmax<1>(M, Lv, px, py, dinv64, lam_dev, &Lv.lmax);
+      rc = bs == 3 ? power_lmax<3>(M, Lv, px, py, dinv64, lam_dev, lam_dev + 1, &Lv.lmax)
+                   : power_lmax<1>(M, Lv, px, py, dinv64, lam_dev, lam_dev + 1, &Lv.lmax);
       // the scratch of the estimate is not needed afterwards
       for (void* p : {vx, vy, vd}) {
         (void)hipFree(p);

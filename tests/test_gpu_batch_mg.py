@@ -179,3 +179,39 @@ def test_batch_step_with_multigrid_members(meshes, mode):
         st["use_init"] = s > 0
         og.step(st, ops, Avg, p, [])
     assert np.max(np.abs(sims[2].get("M").cpu().numpy() - st["m"])) < 1e-8
+
+
+def test_hierarchies_built_concurrently_match_sequential():
+    """Hierarchies built from several host threads on one context (as the
+    batch and serving preparation do) equal those built one at a time: the
+    power-iteration estimates of every level, and a solve's iterations and
+    solution.  (The estimate once summed partials from the context's shared
+    buffer, which the interleaved builds overwrote.)"""
+    import concurrent.futures as cf
+
+    from cupymag_amd.core.assemble_simulation_operators import SimulationOperators
+    from cupymag_amd.core.parameters import from_dict
+    from cupymag_amd.device import context
+    from cupymag_amd.mesh.kuhn import kuhn_mesh
+    ctx = context(0)
+    sizes = [10, 12, 13, 16, 18, 11]
+
+    def build(n):
+        torch.cuda.set_device(ctx.device)
+        cfg = config("Tet", n=n, me=True, rot=True, dt=5e-12, defect_params=2.5)
+        S = SimulationOperators(from_dict(cfg), mesh_arrays=kuhn_mesh(n, jitter=0.1, seed=n), ctx=ctx)
+        return S, S.multigrid(precision=16)
+
+    seq = [build(n) for n in sizes]
+    with cf.ThreadPoolExecutor(max_workers=4) as ex:
+        par = list(ex.map(build, sizes))
+    rng = np.random.default_rng(3)
+    for (_, hs), (_, hp) in zip(seq, par):
+        for name in ("A_demag", "A_el", "A1"):
+            assert hs[name].info()["lmax"] == hp[name].info()["lmax"], name
+        b = rng.standard_normal(hs["A_el"].levels[0].shape[0])
+        b[:3] = 0.0
+        xs, its, _ = hs["A_el"].solve(torch.as_tensor(b, device="cuda"), tol=1e-10, maxiter=500)
+        xp, itp, _ = hp["A_el"].solve(torch.as_tensor(b, device="cuda"), tol=1e-10, maxiter=500)
+        assert its == itp and its < 40
+        np.testing.assert_array_equal(xs.cpu().numpy(), xp.cpu().numpy())
