@@ -177,7 +177,9 @@ def run_gpu(n, steps, warmup, steps64, modes, rel_delta=0.3, autotune=True, rel_
             s_ = 4 if base == "mixed" else 8
             t_ = "float" if s_ == 4 else "double"
             v_, vs_ = ("_Float16", 2) if mgp == 16 else (t_, s_)
-            kernel = (f"k_mg_sweep<3,1,{t_},0,{v_}> (A_el level 0, Chebyshev step, SELL-64 3x3 blocks, stencil, "
+            # (the pre-smoothing's step is MODE 0, the post-smoothing's MODE 4,
+            # which also adds the previous direction to x: the same bytes)
+            kernel = (f"k_mg_sweep<3,1,{t_},0|4,{v_}> (A_el level 0, Chebyshev step, SELL-64 3x3 blocks, stencil, "
                       f"{v_} off-diagonal values)")
             rocname = f"k_mg_sweep<3, 1, {t_}, 0, true, {v_}>"
             bytes_launch = (nnz / 9.0 - nb) * 9 * vs_ + nb * 72.0 + nb * 27.0 * s_

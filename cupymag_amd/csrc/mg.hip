@@ -542,12 +542,18 @@ static void vcycle(cpm_mg* M, size_t l, const double* b64, const CgState* st, hi
     vcycle<BS, K, T>(M, l + 1, nullptr, st);
   }
   k_mg_prolong<BS * K, K, T><<<gn, kBlock, 0, s>>>(fd, cd, M->anchor, (const T*)Lc.x, (T*)Lv.x, st);
-  // post-smoothing: the same polynomial on the new residual
-  sweep<BS, K, T, 2>(M, Lv, nullptr, nullptr, 0.0, 0.0, st);
-  k_mg_first<BS, K, T, false, false><<<gn, kBlock, 0, s>>>(Lv.nb, nullptr, (T*)Lv.b, (T*)Lv.r, (T*)d[0], (T*)Lv.x,
-                                                           (const T*)Lv.dinv, c0, st);
-  for (int k = 1; k < m; ++k)
-    sweep<BS, K, T, 0>(M, Lv, d[(k - 1) & 1], d[k & 1], c1[k], c2[k], st, (k == 1 && ev) ? ev + 2 : nullptr);
+  // post-smoothing: the same polynomial on the new residual (degree >= 2:
+  // the residual and the first step's direction in one pass, its x update
+  // in the second step's pass)
+  if (m >= 2) {
+    sweep<BS, K, T, 3>(M, Lv, nullptr, d[0], 0.0, c0, st);
+    sweep<BS, K, T, 4>(M, Lv, d[0], d[1], c1[1], c2[1], st, ev ? ev + 2 : nullptr);
+    for (int k = 2; k < m; ++k) sweep<BS, K, T, 0>(M, Lv, d[(k - 1) & 1], d[k & 1], c1[k], c2[k], st);
+  } else {
+    sweep<BS, K, T, 2>(M, Lv, nullptr, nullptr, 0.0, 0.0, st);
+    k_mg_first<BS, K, T, false, false><<<gn, kBlock, 0, s>>>(Lv.nb, nullptr, (T*)Lv.b, (T*)Lv.r, (T*)d[0],
+                                                             (T*)Lv.x, (const T*)Lv.dinv, c0, st);
+  }
 }
 
 static void vcycle_any(cpm_mg* M, const double* b64, const CgState* st) {

@@ -88,6 +88,11 @@ __device__ __forceinline__ void mg_first_rows(int64_t lb, int64_t lg, int32_t nb
 // MODE 0: Chebyshev step r -= A d_in, d_out = c1 d_in + c2 D^-1 r, x += d_out.
 // MODE 1: r -= A d_in only (residual after the last step).
 // MODE 2: r = b - A x.
+// MODE 3: r = b - A x and the first step's direction d_out = c2 D^-1 r
+//         (c1 = 0), x left alone: the post-smoothing's residual and first
+//         step in one pass (x can not change while neighbours read it).
+// MODE 4: MODE 0 that also adds the first step's d_in to x, rounded as the
+//         separate updates were: x = (x + d_in) + d_out.
 // Off-diagonal values of type VT (T, or _Float16 scaled by 1/vscale).
 template <int BS, int K, class T, int MODE, bool ST, class VT = T>
 __device__ __forceinline__ void mg_sweep_rows(int64_t lb, int64_t lg, int32_t nb, const int64_t* __restrict__ sptr,
@@ -98,7 +103,8 @@ __device__ __forceinline__ void mg_sweep_rows(int64_t lb, int64_t lg, int32_t nb
                                               const T* __restrict__ dinv, double c1, double c2,
                                               double vscale = 1.0) {
   constexpr int NV = BS * K;
-  const T* src = (MODE == 2) ? x : din;
+  constexpr bool RES = (MODE == 2 || MODE == 3);  // residual of x (else r -= A d_in)
+  const T* src = RES ? x : din;
   const int64_t stride = lg * kBlock;
   for (int64_t row = lb * kBlock + threadIdx.x; row < (int64_t)((nb + 63) & ~63); row += stride) {
     double sum[NV], xo[NV];
@@ -107,18 +113,23 @@ __device__ __forceinline__ void mg_sweep_rows(int64_t lb, int64_t lg, int32_t nb
     double rv[NV];
 #pragma unroll
     for (int c = 0; c < NV; ++c) {
-      rv[c] = (MODE == 2) ? (double)b[row * NV + c] - sum[c] : (double)r[row * NV + c] - sum[c];
+      rv[c] = RES ? (double)b[row * NV + c] - sum[c] : (double)r[row * NV + c] - sum[c];
       r[row * NV + c] = (T)rv[c];
       rv[c] = (double)(T)rv[c];
     }
-    if (MODE == 0) {
+    if (MODE == 0 || MODE == 3 || MODE == 4) {
       double dv[NV];
       apply_dinv<BS, K, T>(dinv, row, rv, dv);
 #pragma unroll
       for (int c = 0; c < NV; ++c) {
-        const T dn = (T)(c1 * xo[c] + c2 * dv[c]);
+        const T dn = (MODE == 3) ? (T)(c2 * dv[c]) : (T)(c1 * xo[c] + c2 * dv[c]);
         dout[row * NV + c] = dn;
-        x[row * NV + c] = (T)((double)x[row * NV + c] + (double)dn);
+        if (MODE == 0) {
+          x[row * NV + c] = (T)((double)x[row * NV + c] + (double)dn);
+        } else if (MODE == 4) {
+          const T x1 = (T)((double)x[row * NV + c] + xo[c]);
+          x[row * NV + c] = (T)((double)x1 + (double)dn);
+        }
       }
     }
   }

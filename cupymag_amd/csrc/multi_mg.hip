@@ -94,17 +94,19 @@ __global__ void __launch_bounds__(kBlock) kbm_first(const SmoothSeg* __restrict_
                                         d.fstride, d.fn);
 }
 
-// MODE 0: Chebyshev step k (d_in = d[(k-1)&1], d_out = d[k&1]); MODE 1: the
-// residual after step `k` (d_in = d[k&1]); MODE 2: r = b - A x.
+// MODE 0 / 4: Chebyshev step k (d_in = d[(k-1)&1], d_out = d[k&1]; 4: also
+// x += d_in); MODE 1: the residual after step `k` (d_in = d[k&1]); MODE 2:
+// r = b - A x; MODE 3: r = b - A x and d[0] = c0 D^-1 r (mg_dev.h).
 template <int BS, int K, class T, int MODE, class VT>
 __global__ void __launch_bounds__(kBlock) kbm_sweep(const SmoothSeg* __restrict__ seg, const int32_t* __restrict__ map,
                                                     int k) {
   const SmoothSeg& d = seg[map[blockIdx.x]];
   if (all_done<K>(d.st)) return;
   T* dv[2] = {(T*)d.d0, (T*)d.d1};
-  const T* din = (MODE == 0) ? dv[(k - 1) & 1] : dv[k & 1];
+  const T* din = (MODE == 0 || MODE == 4) ? dv[(k - 1) & 1] : dv[k & 1];
   T* dout = dv[k & 1];
-  const double c1 = (MODE == 0) ? d.c1[k] : 0.0, c2 = (MODE == 0) ? d.c2[k] : 0.0;
+  const double c1 = (MODE == 0 || MODE == 4) ? d.c1[k] : 0.0;
+  const double c2 = (MODE == 0 || MODE == 4) ? d.c2[k] : (MODE == 3) ? d.c0 : 0.0;
   const int64_t lb = (int64_t)blockIdx.x - d.blk0;
   if (d.stencil)
     mg_sweep_rows<BS, K, T, MODE, true, VT>(lb, d.nblk, d.nb, d.sptr, d.col, (const VT*)d.val, d.dg, d.ga, din, dout,
@@ -522,9 +524,15 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
       auto& Sm = W.smooth[d];
       if (!Sm.G) continue;
       kbm_prolong<BS * K, K, T><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap);
-      kbm_sweep<BS, K, T, 2, VT><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap, 0);
-      kbm_first<BS, K, T, false, false><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap);
-      for (int k = 1; k < m; ++k) kbm_sweep<BS, K, T, 0, VT><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap, k);
+      if (m >= 2) {
+        // the residual with the first step's direction, then x += d_0 with the second step
+        kbm_sweep<BS, K, T, 3, VT><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap, 0);
+        kbm_sweep<BS, K, T, 4, VT><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap, 1);
+        for (int k = 2; k < m; ++k) kbm_sweep<BS, K, T, 0, VT><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap, k);
+      } else {
+        kbm_sweep<BS, K, T, 2, VT><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap, 0);
+        kbm_first<BS, K, T, false, false><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap);
+      }
     }
   };
 
