@@ -432,7 +432,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_pupd(int64_t n, const T* __restr
 
 // ---------------------------------------------------------------- host
 
-static GridArg grid_arg(const Sell& S) { return GridArg{S.dims[0], S.dims[1], S.dims[2], S.nst, S.soff}; }
+static GridArg grid_arg(const Sell& S) { return CPM_GRID_ARG(S); }
 
 
 // smoothing values of a level (FP64 or FP32 form): the recovered periodic

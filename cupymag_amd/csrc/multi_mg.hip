@@ -407,7 +407,7 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
     OuterSeg o{};
     o.sptr = S.sptr, o.col = S.col, o.v64 = S.v64, o.dg64 = S.dg64, o.v32 = S.v32, o.dgc = S.dgc;
     o.delta = y.delta;
-    o.ga = GridArg{S.dims[0], S.dims[1], S.dims[2], S.nst, S.soff};
+    o.ga = CPM_GRID_ARG(S);
     o.nb = S.nbrows, o.stencil = S.stencil, o.n = M->L[0].n * K;
     o.b = y.b, o.x = y.x, o.r = M->r, o.p = M->p, o.q = M->q, o.rold = M->rold, o.z = M->L[0].x;
     o.st = W.st + j * K;
@@ -455,7 +455,7 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
                                   : (Lv.kv64 ? (const void*)Lv.kv64 : (const void*)S.v64);
         g.dg = M->half ? Lv.dgh : (M->prec == 32) ? (Lv.kdgc ? Lv.kdgc : S.dgc) : (Lv.kdg64 ? Lv.kdg64 : S.dg64);
         g.vscale = M->half ? Lv.vscale : 1.0;
-        g.ga = GridArg{S.dims[0], S.dims[1], S.dims[2], S.nst, S.soff};
+        g.ga = CPM_GRID_ARG(S);
         g.nb = S.nbrows, g.stencil = S.stencil;
         g.x = Lv.x, g.r = Lv.r, g.b = Lv.b, g.d0 = Lv.d0, g.d1 = Lv.d1, g.dinv = Lv.dinv;
         g.b64 = (d == 0) ? M->r : nullptr;

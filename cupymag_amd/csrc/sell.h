@@ -24,8 +24,12 @@ struct Sell {
   int32_t nst = 0;           // offsets per row (14 Kuhn, 26 hex)
   int32_t dims[3] = {0, 0, 0};
   int32_t* soff = nullptr;   // device, packed (di+1) | (dj+1) << 2 | (dk+1) << 4
+  int kuhn = 0;              // the offsets are the Kuhn stencil in kKuhnCodes order (sell_dev.h)
   int grid_cap = 8192;       // workgroup cap of the SpMV launches (tunable, grid-stride beyond)
 };
+
+// the grid argument of the kernels for a stencil-form operator
+#define CPM_GRID_ARG(S) (GridArg{(S).dims[0], (S).dims[1], (S).dims[2], (S).nst, (S).soff, (S).kuhn})
 
 void sell_free(Sell& S);
 int sell_grid(const Sell& A);

@@ -161,6 +161,11 @@ int sell_build(cpm_ctx* ctx, Sell& S, int bs, int32_t nbrows, const int64_t* ind
   S.nbrows = nbrows;
   S.stencil = stencil ? 1 : 0;
   S.nst = stencil ? (int32_t)codes.size() : 0;
+  S.kuhn = 0;
+  if (stencil && codes.size() == (size_t)kKuhnSlots) {
+    S.kuhn = 1;
+    for (int k = 0; k < kKuhnSlots; ++k) S.kuhn &= codes[k] == kKuhnCodes[k] ? 1 : 0;
+  }
   if (stencil)
     for (int a = 0; a < 3; ++a) S.dims[a] = dims[a];
   S.nslices = (int32_t)nsl;
@@ -280,7 +285,7 @@ static int spmv_t(cpm_ctx* ctx, const Sell& A, const T* x, const float* x32, T* 
     val = A.v32;
     dg = A.dgc;
   }
-  const GridArg ga{A.dims[0], A.dims[1], A.dims[2], A.nst, A.soff};
+  const GridArg ga = CPM_GRID_ARG(A);
   const dim3 grid(sell_grid(A)), block(kBlock);
   if (ev) {
     if (A.stencil)

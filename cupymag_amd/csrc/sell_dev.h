@@ -21,10 +21,21 @@
 namespace cpm {
 
 // Periodic structured grid of an index-free (stencil) SELL operator.
+// kuhn: the slots are the 14 neighbours of a Kuhn mesh in the order of
+// kKuhnCodes (the host checks soff), so the kernels decode them at compile
+// time.
 struct GridArg {
   int32_t nx, ny, nz, nst;
   const int32_t* soff;
+  int32_t kuhn;
 };
+
+// Slot codes (di+1) | (dj+1) << 2 | (dk+1) << 4 of the neighbours of a node
+// of a Kuhn mesh (cube diagonal (0,0,0)-(1,1,1)): +-e1, +-e2, +-e3,
+// +-(e1+e2), +-(e2+e3), +-(e1+e3), +-(e1+e2+e3), ascending (the order the
+// SELL builder gives them).
+constexpr int kKuhnSlots = 14;
+constexpr int32_t kKuhnCodes[kKuhnSlots] = {0, 1, 4, 5, 16, 17, 20, 22, 25, 26, 37, 38, 41, 42};
 
 // sum = row `row` of A x (ADDX32: A (x + x32)), FP64 accumulation, for
 // BS x BS blocks and K interleaved right-hand sides x[(c*BS + e)*K + j];
@@ -55,22 +66,7 @@ __device__ __forceinline__ void sell_row(int64_t row, int32_t nbrows, const int6
   // matrix planes are read once per launch: streaming (non-temporal) loads
   // keep them from evicting the gathered vector entries; two slots per
   // trip keep more loads in flight
-  auto slot_fma = [&](int64_t slot) {
-    int64_t c;
-    if (ST) {
-      // neighbour on the periodic grid (padding rows gather row 0)
-      const int32_t code = ga.soff[slot - s0];
-      int32_t i2 = ri + (code & 3) - 1, j2 = rj + ((code >> 2) & 3) - 1, k2 = rk + ((code >> 4) & 3) - 1;
-      i2 += (i2 < 0) ? ga.nx : 0;
-      i2 -= (i2 >= ga.nx) ? ga.nx : 0;
-      j2 += (j2 < 0) ? ga.ny : 0;
-      j2 -= (j2 >= ga.ny) ? ga.ny : 0;
-      k2 += (k2 < 0) ? ga.nz : 0;
-      k2 -= (k2 >= ga.nz) ? ga.nz : 0;
-      c = (row < nbrows) ? ((int64_t)i2 * ga.ny + j2) * ga.nz + k2 : 0;
-    } else {
-      c = __builtin_nontemporal_load(&col[slot * 64 + l]);
-    }
+  auto slot_fma_at = [&](int64_t slot, int64_t c) {
     double xv[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
@@ -106,16 +102,55 @@ __device__ __forceinline__ void sell_row(int64_t row, int32_t nbrows, const int6
         }
     }
   };
+  auto slot_fma = [&](int64_t slot) {
+    int64_t c;
+    if (ST) {
+      // neighbour on the periodic grid (padding rows gather row 0)
+      const int32_t code = ga.soff[slot - s0];
+      int32_t i2 = ri + (code & 3) - 1, j2 = rj + ((code >> 2) & 3) - 1, k2 = rk + ((code >> 4) & 3) - 1;
+      i2 += (i2 < 0) ? ga.nx : 0;
+      i2 -= (i2 >= ga.nx) ? ga.nx : 0;
+      j2 += (j2 < 0) ? ga.ny : 0;
+      j2 -= (j2 >= ga.ny) ? ga.ny : 0;
+      k2 += (k2 < 0) ? ga.nz : 0;
+      k2 -= (k2 >= ga.nz) ? ga.nz : 0;
+      c = (row < nbrows) ? ((int64_t)i2 * ga.ny + j2) * ga.nz + k2 : 0;
+    } else {
+      c = __builtin_nontemporal_load(&col[slot * 64 + l]);
+    }
+    slot_fma_at(slot, c);
+  };
   // slots per trip: loads of U slots in flight before their FMAs (7 = half
   // the 14 neighbours of a Kuhn row; a sweep of 2..7 at 128^3 favoured 7 for
   // every value type, most for FP16)
   constexpr int U = (sizeof(VT) == 2) ? SELL_HALF_UNROLL : (sizeof(VT) == 4) ? SELL_FLOAT_UNROLL : SELL_DOUBLE_UNROLL;
-  int64_t slot = s0;
-  for (; slot + U - 1 < s1; slot += U) {
+  if (ST && ga.kuhn) {
+    // Kuhn stencil: slot k's offset is known at compile time, so its
+    // neighbour is the row plus one wrap-aware step per dimension (computed
+    // once per row), in 32-bit arithmetic, with no per-slot loads of codes
+    const int32_t r = (row < nbrows) ? (int32_t)row : 0;  // padding rows gather row 0's neighbours
+    const int32_t nyz = ga.ny * ga.nz;
+    const int32_t stp[3][3] = {
+        {ri == 0 ? (ga.nx - 1) * nyz : -nyz, 0, ri == ga.nx - 1 ? (1 - ga.nx) * nyz : nyz},
+        {rj == 0 ? (ga.ny - 1) * ga.nz : -ga.nz, 0, rj == ga.ny - 1 ? (1 - ga.ny) * ga.nz : ga.nz},
+        {rk == 0 ? ga.nz - 1 : -1, 0, rk == ga.nz - 1 ? 1 - ga.nz : 1}};
 #pragma unroll
-    for (int u = 0; u < U; ++u) slot_fma(slot + u);
+    for (int t = 0; t < kKuhnSlots; t += 7) {
+#pragma unroll
+      for (int u = 0; u < 7; ++u) {
+        const int32_t code = kKuhnCodes[t + u];
+        const int32_t c = r + stp[0][code & 3] + stp[1][(code >> 2) & 3] + stp[2][(code >> 4) & 3];
+        slot_fma_at(s * kKuhnSlots + t + u, c);
+      }
+    }
+  } else {
+    int64_t slot = s0;
+    for (; slot + U - 1 < s1; slot += U) {
+#pragma unroll
+      for (int u = 0; u < U; ++u) slot_fma(slot + u);
+    }
+    for (; slot < s1; ++slot) slot_fma(slot);
   }
-  for (; slot < s1; ++slot) slot_fma(slot);
   if constexpr (sizeof(VT) == 2) {
 #pragma unroll
     for (int v = 0; v < NV; ++v) sum[v] *= vscale;
