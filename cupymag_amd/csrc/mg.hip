@@ -172,8 +172,8 @@ __global__ void __launch_bounds__(kBlock) k_mg_sweep(int32_t nb, const int64_t* 
 
 // Low-precision form of a level operator: VT = _Float16: v16 =
 // fp16(v64 / vscale) in the SELL layout (padding lanes included; for 3x3
-// blocks entries 2q and 2q+1, q < 4, interleaved per lane so that one
-// 32-bit load fetches both, entry 8 after them); VT = float: fp32(v64)
+// blocks a lane's entries 0..7 contiguous, 16 bytes, so that one 128-bit
+// load fetches them, entry 8 of the 64 lanes after them); VT = float: fp32(v64)
 // (vscale 1) in the layout of v64.  dgh = dg64 + the rounding of the row's
 // off-diagonal values, so that the stored row sums equal the FP64 ones.
 template <int BS, class VT>
@@ -197,7 +197,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_make_low(int32_t nb, int64_t nrow
         const double a = v64[i];
         const VT h = (VT)(a / vscale);
         const int64_t o =
-            (sizeof(VT) == 2 && BB == 9) ? slot * 576 + (t < 8 ? (t >> 1) * 128 + 2 * l + (t & 1) : 512 + l) : i;
+            (sizeof(VT) == 2 && BB == 9) ? slot * 576 + (t < 8 ? 8 * l + t : 512 + l) : i;
         v16[o] = h;
         comp[t] += a - (double)h * vscale;
       }

@@ -74,13 +74,16 @@ __device__ __forceinline__ void sell_row(int64_t row, int32_t nbrows, const int6
       if (ADDX32) xv[v] += (double)x32[c * NV + v];
     }
     if constexpr (sizeof(VT) == 2 && BB == 9) {
-      // FP16 3x3 blocks: entries 2q, 2q+1 (q < 4) share a 32-bit word per
-      // lane, entry 8 on its own (5 loads per slot instead of 9)
+      // FP16 3x3 blocks: entries 0..7 of a lane in one 16-byte word (one
+      // 128-bit load), entry 8 on its own (2 loads per slot instead of 9)
       double av[9];
-      const uint32_t* w = reinterpret_cast<const uint32_t*>(val + slot * 576);
+      typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+      const u32x4* w = reinterpret_cast<const u32x4*>(val + slot * 576);
+      const u32x4 w4 = __builtin_nontemporal_load(&w[l]);
+      const uint32_t pw4[4] = {w4.x, w4.y, w4.z, w4.w};
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const uint32_t pw = __builtin_nontemporal_load(&w[q * 64 + l]);
+        const uint32_t pw = pw4[q];
         av[2 * q] = (double)__builtin_bit_cast(_Float16, (uint16_t)(pw & 0xffffu));
         av[2 * q + 1] = (double)__builtin_bit_cast(_Float16, (uint16_t)(pw >> 16));
       }
