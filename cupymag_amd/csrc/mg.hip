@@ -142,7 +142,7 @@ __global__ void k_mg_float_row0(int32_t nst, GridArg ga, double* __restrict__ v,
     int64_t cs = -1;
     for (int32_t q = 0; q < nst; ++q)
       if (ga.soff[q] == back) cs = (c >> 6) * nst + q;
-    if (cs < 0) continue;  // not a symmetric stencil: left as it is (the host checks the result)
+    if (cs < 0) continue;  // no opposite offset: not a Kuhn stencil, which cpm_mg_create rejects
     const int64_t lc = c & 63;
 #pragma unroll
     for (int d = 0; d < BS; ++d)
@@ -989,9 +989,10 @@ extern "C" int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, 
     M->floating = (a[0] && !a[1]) ? 1 : 0;
     M->anchor = (a[0] && a[1]) ? 1 : 0;
     const Sell& F = levels[0]->sell;
-    if (M->floating && (F.dims[0] < 3 || F.dims[1] < 3 || F.dims[2] < 3)) {
+    if (M->floating && (F.dims[0] < 3 || F.dims[1] < 3 || F.dims[2] < 3 || !F.kuhn)) {
       delete M;
-      return fail(CPM_EINVAL, "cpm_mg_create: a floating hierarchy needs >= 3 fine nodes per dimension");
+      return fail(CPM_EINVAL, "cpm_mg_create: a floating hierarchy needs a Kuhn stencil with >= 3 fine nodes per "
+                              "dimension");
     }
   }
   const size_t ts = precision == 32 ? 4 : 8;

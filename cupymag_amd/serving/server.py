@@ -43,15 +43,26 @@ _BYTES_PER_DOF = {False: 2200.0, True: 5700.0}
 _MG_BYTES_PER_DOF = {False: 500.0, True: 1200.0}
 
 
-def estimate_bytes(n_dof, me, multigrid_n=None):
+# FP64 hierarchies (the fast64 mode) add per fine DOF: the FP64 level vectors
+# and coarse forms, and the FP64 copy of the level-0 periodic stiffness that
+# the floating hierarchies keep (demag: 14 off-diagonal + 1 diagonal value;
+# elasticity: 14 off-diagonal 3x3 blocks + the diagonal block)
+_MG64_EXTRA_PER_DOF = {False: 500.0 + 120.0, True: 1200.0 + 1080.0 + 120.0}
+
+
+def estimate_bytes(n_dof, me, multigrid_n=None, mode="mixed"):
     """Device memory of one member with ``n_dof`` DOFs (upper estimate);
     multigrid_n: cells per dimension when the member runs the multigrid
-    solves (adds the hierarchies and their dense FP32 coarsest inverses)."""
+    solves (adds the hierarchies and their dense coarsest inverses: FP32 in
+    the mixed mode, FP64 otherwise)."""
     b = 1.15 * _BYTES_PER_DOF[bool(me)] * n_dof
     if multigrid_n:
         from ..mesh.coarsen import hierarchy_sizes
         nc = hierarchy_sizes(int(multigrid_n))[-1] ** 3
-        b += _MG_BYTES_PER_DOF[bool(me)] * n_dof + 4.0 * (2 * nc ** 2 + (9 * nc ** 2 if me else 0))
+        ts = 4.0 if mode == "mixed" else 8.0
+        b += _MG_BYTES_PER_DOF[bool(me)] * n_dof + ts * (2 * nc ** 2 + (9 * nc ** 2 if me else 0))
+        if mode != "mixed":
+            b += _MG64_EXTRA_PER_DOF[bool(me)] * n_dof
     return int(b) + (64 << 20)
 
 
@@ -195,7 +206,8 @@ class SweepServer:
                 while waiting:
                     req = waiting[0]
                     me = bool(req["cfg"]["physics"]["magnetoelastic_coupling"])
-                    est = estimate_bytes(req["n"] ** 3, me, req["n"] if self._uses_mg(req["n"]) else None)
+                    est = estimate_bytes(req["n"] ** 3, me, req["n"] if self._uses_mg(req["n"]) else None,
+                                         self.mode)
                     if est > self.hbm_budget:
                         raise MemoryError(f"request {req['rid']} needs ~{est / 1e9:.1f} GB, over the HBM budget")
                     if used + est > self.hbm_budget or len(active) + len(preparing) >= self.max_members:

@@ -66,13 +66,14 @@ def test_hbm_budget_serialises_admission():
         SweepServer(mode="mixed", hbm_budget=min(ests) * 0.5).run(reqs[:1])
 
 
-@pytest.mark.parametrize("me,precond", [(False, None), (True, None), (False, "mg"), (True, "mg")])
-def test_admission_estimate_bounds_member_memory(me, precond):
+@pytest.mark.parametrize("me,precond,mode", [(False, None, "mixed"), (True, None, "mixed"), (False, "mg", "mixed"),
+                                             (True, "mg", "mixed"), (True, "mg", "fast64")])
+def test_admission_estimate_bounds_member_memory(me, precond, mode):
     from cupymag_amd.core.batch import GSPMBatch
     from cupymag_amd.serving.server import SweepServer, estimate_bytes
     from cupymag_amd.serving.workloads import sweep_requests
     req = sweep_requests(1, seed=11, nmin=24, nmax=24, points=2, steps_per_point=1, me=me)[0]
-    srv = SweepServer(mode="mixed", precond=precond)
+    srv = SweepServer(mode=mode, precond=precond)
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
     free0, _ = torch.cuda.mem_get_info()
@@ -85,5 +86,5 @@ def test_admission_estimate_bounds_member_memory(me, precond):
     used = free0 - free1
     assert n_dof == 24 ** 3
     assert (sim.mg is not None) == (precond == "mg")
-    est = estimate_bytes(n_dof, me, 24 if precond else None)
+    est = estimate_bytes(n_dof, me, 24 if precond else None, mode)
     assert 0 < used <= est, (used, est)
