@@ -153,6 +153,16 @@ __device__ __forceinline__ bool nested_dims(const Dims3& fine, const Dims3& coar
   return fine.d[0] == 2 * coarse.d[0] && fine.d[1] == 2 * coarse.d[1] && fine.d[2] == 2 * coarse.d[2];
 }
 
+// Axes by decreasing fraction (t[p0] >= t[p1] >= t[p2]; the comparison
+// order fixes the ties, the same for the prolongation and the restriction).
+__device__ __forceinline__ void kuhn_order(const double (&t)[3], int& p0, int& p1, int& p2) {
+  int tmp;
+  p0 = 0, p1 = 1, p2 = 2;
+  if (t[p1] > t[p0]) tmp = p0, p0 = p1, p1 = tmp;
+  if (t[p2] > t[p1]) tmp = p1, p1 = p2, p2 = tmp;
+  if (t[p1] > t[p0]) tmp = p0, p0 = p1, p1 = tmp;
+}
+
 // P1 interpolation weights of fine node (i,j,k) on the coarse Kuhn mesh of
 // the same periodic box (fine node i lies at coarse coordinate i*nc/nf per
 // axis): cell c, fractions t sorted t_p0 >= t_p1 >= t_p2, vertices c,
@@ -166,14 +176,14 @@ __device__ __forceinline__ void kuhn_weights(int32_t i, int32_t j, int32_t k, co
   double t[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const int64_t q = (int64_t)idx[a] * coarse.d[a];
-    cell[a] = (int32_t)(q / fine.d[a]);
-    t[a] = (double)(q - (int64_t)cell[a] * fine.d[a]) / (double)fine.d[a];
+    // 32-bit: fine index x coarse size stays far below 2^31 for any grid
+    // the operators fit
+    const int32_t q = idx[a] * coarse.d[a];
+    cell[a] = q / fine.d[a];
+    t[a] = (double)(q - cell[a] * fine.d[a]) / (double)fine.d[a];
   }
-  int p0 = 0, p1 = 1, p2 = 2, tmp;
-  if (t[p1] > t[p0]) tmp = p0, p0 = p1, p1 = tmp;
-  if (t[p2] > t[p1]) tmp = p1, p1 = p2, p2 = tmp;
-  if (t[p1] > t[p0]) tmp = p0, p0 = p1, p1 = tmp;
+  int p0, p1, p2;
+  kuhn_order(t, p0, p1, p2);
   int32_t o[3] = {0, 0, 0};
   v[0] = wrap_idx(cell[0], cell[1], cell[2], coarse.d);
   w[0] = 1.0 - t[p0];
@@ -231,6 +241,61 @@ __device__ __forceinline__ void restrict_node(int64_t I, const Dims3& fine, cons
     lo[a] = (int32_t)ceil_div((int64_t)(C3[a] - 1) * nf, ncd);
     hi[a] = (int32_t)ceil_div((int64_t)(C3[a] + 1) * nf, ncd) - 1;
     if (hi[a] - lo[a] + 1 > nf) hi[a] = lo[a] + (int32_t)nf - 1;
+  }
+  constexpr int kMaxCand = 6;  // fine indices per dimension (at most 5 when the grid halves, rounding up)
+  if (coarse.d[0] >= 2 && coarse.d[1] >= 2 && coarse.d[2] >= 2 && hi[0] - lo[0] < kMaxCand &&
+      hi[1] - lo[1] < kMaxCand && hi[2] - lo[2] < kMaxCand) {
+    // Candidates per dimension: wrapped fine index, fraction t and the
+    // offset o = C - cell in {0, 1} of this coarse node from the cell that
+    // holds the fine node.  With >= 2 coarse nodes per dimension, vertex
+    // cell + o_v of that cell's Kuhn tetrahedron is this node iff o_v = o,
+    // so the weight is that of vertex o: 1 - t_p0 (o = 0), t_p0 - t_p1
+    // (o = e_p0), t_p1 - t_p2 (o = e_p0 + e_p1), t_p2 (o = (1,1,1)) -- the
+    // values kuhn_weights gives, added in the same order as the generic
+    // loop below.
+    int32_t wf[3][kMaxCand], off[3][kMaxCand];
+    double tf[3][kMaxCand];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const int32_t nf = fine.d[a], ncd = coarse.d[a];
+#pragma unroll
+      for (int m = 0; m < kMaxCand; ++m) {
+        const int32_t f = lo[a] + m;
+        const int32_t wrapped = f < 0 ? f + nf : (f >= nf ? f - nf : f);
+        const int32_t q = wrapped * ncd;
+        const int32_t cell = q / nf;
+        tf[a][m] = (double)(q - cell * nf) / (double)nf;
+        // unwrapped cell: the wrapped one shifted by the wrap (f = wrapped - nf <-> cell - ncd)
+        const int32_t cu = cell + (f < 0 ? -ncd : (f >= nf ? ncd : 0));
+        off[a][m] = C3[a] - cu;
+        wf[a][m] = wrapped;
+      }
+    }
+    for (int32_t m0 = 0; m0 <= hi[0] - lo[0]; ++m0)
+      for (int32_t m1 = 0; m1 <= hi[1] - lo[1]; ++m1)
+        for (int32_t m2 = 0; m2 <= hi[2] - lo[2]; ++m2) {
+          const int64_t f = ((int64_t)wf[0][m0] * fine.d[1] + wf[1][m1]) * fine.d[2] + wf[2][m2];
+          if (anchor && f == 0) continue;
+          const double t[3] = {tf[0][m0], tf[1][m1], tf[2][m2]};
+          const int32_t o[3] = {off[0][m0], off[1][m1], off[2][m2]};
+          int p0, p1, p2;
+          kuhn_order(t, p0, p1, p2);
+          double wt;
+          if (o[0] == 0 && o[1] == 0 && o[2] == 0)
+            wt = 1.0 - t[p0];
+          else if (o[0] == 1 && o[1] == 1 && o[2] == 1)
+            wt = t[p2];
+          else if (o[p0] == 1 && o[p1] == 0 && o[p2] == 0)
+            wt = t[p0] - t[p1];
+          else if (o[p0] == 1 && o[p1] == 1 && o[p2] == 0)
+            wt = t[p1] - t[p2];
+          else
+            wt = 0.0;
+          if (wt == 0.0) continue;
+#pragma unroll
+          for (int c = 0; c < W; ++c) acc[c] += wt * (double)rf[f * W + c];
+        }
+    return;
   }
   for (int32_t fi = lo[0]; fi <= hi[0]; ++fi)
     for (int32_t fj = lo[1]; fj <= hi[1]; ++fj)
