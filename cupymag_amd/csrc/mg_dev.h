@@ -242,42 +242,44 @@ __device__ __forceinline__ void restrict_node(int64_t I, const Dims3& fine, cons
     hi[a] = (int32_t)ceil_div((int64_t)(C3[a] + 1) * nf, ncd) - 1;
     if (hi[a] - lo[a] + 1 > nf) hi[a] = lo[a] + (int32_t)nf - 1;
   }
-  constexpr int kMaxCand = 6;  // fine indices per dimension (at most 5 when the grid halves, rounding up)
-  if (coarse.d[0] >= 2 && coarse.d[1] >= 2 && coarse.d[2] >= 2 && hi[0] - lo[0] < kMaxCand &&
-      hi[1] - lo[1] < kMaxCand && hi[2] - lo[2] < kMaxCand) {
-    // Candidates per dimension: wrapped fine index, fraction t and the
-    // offset o = C - cell in {0, 1} of this coarse node from the cell that
-    // holds the fine node.  With >= 2 coarse nodes per dimension, vertex
-    // cell + o_v of that cell's Kuhn tetrahedron is this node iff o_v = o,
-    // so the weight is that of vertex o: 1 - t_p0 (o = 0), t_p0 - t_p1
-    // (o = e_p0), t_p1 - t_p2 (o = e_p0 + e_p1), t_p2 (o = (1,1,1)) -- the
-    // values kuhn_weights gives, added in the same order as the generic
-    // loop below.
-    int32_t wf[3][kMaxCand], off[3][kMaxCand];
-    double tf[3][kMaxCand];
+  if (coarse.d[0] >= 2 && coarse.d[1] >= 2 && coarse.d[2] >= 2) {
+    // Candidates walked per dimension with their cell (unwrapped) and
+    // remainder kept up to date by addition (fine index f lies at coarse
+    // coordinate f nc / nf = cell + rem / nf; nc < nf, so a step of one fine
+    // index advances the cell by at most one).  With >= 2 coarse nodes per
+    // dimension, vertex cell + o_v of the Kuhn tetrahedron that holds the
+    // fine node is this node iff o_v = o = C - cell, so the weight is that
+    // of vertex o: 1 - t_p0 (o = 0), t_p0 - t_p1 (o = e_p0), t_p1 - t_p2
+    // (o = e_p0 + e_p1), t_p2 (o = (1,1,1)) -- the values kuhn_weights
+    // gives, added in the same order as the generic loop below.
+    int32_t cell0[3], rem0[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       const int32_t nf = fine.d[a], ncd = coarse.d[a];
-#pragma unroll
-      for (int m = 0; m < kMaxCand; ++m) {
-        const int32_t f = lo[a] + m;
-        const int32_t wrapped = f < 0 ? f + nf : (f >= nf ? f - nf : f);
-        const int32_t q = wrapped * ncd;
-        const int32_t cell = q / nf;
-        tf[a][m] = (double)(q - cell * nf) / (double)nf;
-        // unwrapped cell: the wrapped one shifted by the wrap (f = wrapped - nf <-> cell - ncd)
-        const int32_t cu = cell + (f < 0 ? -ncd : (f >= nf ? ncd : 0));
-        off[a][m] = C3[a] - cu;
-        wf[a][m] = wrapped;
-      }
+      const int32_t q = lo[a] * ncd;
+      const int32_t c = q >= 0 ? q / nf : -((-q + nf - 1) / nf);  // floor
+      cell0[a] = c;
+      rem0[a] = q - c * nf;
     }
-    for (int32_t m0 = 0; m0 <= hi[0] - lo[0]; ++m0)
-      for (int32_t m1 = 0; m1 <= hi[1] - lo[1]; ++m1)
-        for (int32_t m2 = 0; m2 <= hi[2] - lo[2]; ++m2) {
-          const int64_t f = ((int64_t)wf[0][m0] * fine.d[1] + wf[1][m1]) * fine.d[2] + wf[2][m2];
+    int32_t c0 = cell0[0], r0 = rem0[0];
+    for (int32_t fi = lo[0]; fi <= hi[0]; ++fi) {
+      const int32_t wi = fi < 0 ? fi + fine.d[0] : (fi >= fine.d[0] ? fi - fine.d[0] : fi);
+      const double t0 = (double)r0 / (double)fine.d[0];
+      const int32_t o0 = C3[0] - c0;
+      int32_t c1 = cell0[1], r1 = rem0[1];
+      for (int32_t fj = lo[1]; fj <= hi[1]; ++fj) {
+        const int32_t wj = fj < 0 ? fj + fine.d[1] : (fj >= fine.d[1] ? fj - fine.d[1] : fj);
+        const double t1 = (double)r1 / (double)fine.d[1];
+        const int32_t o1 = C3[1] - c1;
+        int32_t c2 = cell0[2], r2 = rem0[2];
+        for (int32_t fk = lo[2]; fk <= hi[2]; ++fk) {
+          const int32_t wk = fk < 0 ? fk + fine.d[2] : (fk >= fine.d[2] ? fk - fine.d[2] : fk);
+          const double t[3] = {t0, t1, (double)r2 / (double)fine.d[2]};
+          const int32_t o[3] = {o0, o1, C3[2] - c2};
+          r2 += coarse.d[2];
+          if (r2 >= fine.d[2]) r2 -= fine.d[2], ++c2;
+          const int64_t f = ((int64_t)wi * fine.d[1] + wj) * fine.d[2] + wk;
           if (anchor && f == 0) continue;
-          const double t[3] = {tf[0][m0], tf[1][m1], tf[2][m2]};
-          const int32_t o[3] = {off[0][m0], off[1][m1], off[2][m2]};
           int p0, p1, p2;
           kuhn_order(t, p0, p1, p2);
           double wt;
@@ -295,6 +297,12 @@ __device__ __forceinline__ void restrict_node(int64_t I, const Dims3& fine, cons
 #pragma unroll
           for (int c = 0; c < W; ++c) acc[c] += wt * (double)rf[f * W + c];
         }
+        r1 += coarse.d[1];
+        if (r1 >= fine.d[1]) r1 -= fine.d[1], ++c1;
+      }
+      r0 += coarse.d[0];
+      if (r0 >= fine.d[0]) r0 -= fine.d[0], ++c0;
+    }
     return;
   }
   for (int32_t fi = lo[0]; fi <= hi[0]; ++fi)
