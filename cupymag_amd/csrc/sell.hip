@@ -527,7 +527,7 @@ extern "C" int cpm_mat_add_sell(cpm_mat* A, int bs, const int64_t* indptr, const
 extern "C" int cpm_mat_sell_info(const cpm_mat* A, int* bs, int* stencil, int64_t* slots) {
   if (!A) return cpm::fail(CPM_EINVAL, "cpm_mat_sell_info: null matrix");
   if (bs) *bs = A->sell.bs;
-  if (stencil) *stencil = A->sell.stencil;
+  if (stencil) *stencil = A->sell.stencil ? (A->sell.kuhn ? 2 : 1) : 0;
   if (slots) *slots = A->sell.nslot;
   return CPM_OK;
 }

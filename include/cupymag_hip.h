@@ -117,10 +117,11 @@ int cpm_mat_prof(cpm_mat* A, double* total_ms, int64_t* launches, int reset);
  * neighbour stencil of that grid the column indices are not stored. */
 int cpm_mat_add_sell(cpm_mat* A, int bs, const int64_t* indptr, const int32_t* indices, const double* values,
                      const int32_t* grid_dims);
-/* SELL form in use: block size, 1 if index-free (grid_dims given to
- * cpm_mat_add_sell, DOF (i,j,k) numbered (i*ny + j)*nz + k on a periodic
- * grid, and every block column a neighbour at an offset in {-1,0,1}^3), and
- * the number of 64-row slots. */
+/* SELL form in use: block size, stencil = 1 if index-free (grid_dims given
+ * to cpm_mat_add_sell, DOF (i,j,k) numbered (i*ny + j)*nz + k on a periodic
+ * grid, and every block column a neighbour at an offset in {-1,0,1}^3), 2
+ * if those offsets are the 14 neighbours of a Kuhn mesh (decoded at compile
+ * time by the kernels), 0 if indexed; and the number of 64-row slots. */
 int cpm_mat_sell_info(const cpm_mat* A, int* bs, int* stencil, int64_t* slots);
 /* CG on the SELL form for nrhs interleaved right-hand sides (b[i*nrhs + j];
  * nrhs = 3 only for bs = 1; bs = 3 vectors are node-major interleaved).

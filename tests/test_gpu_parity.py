@@ -348,9 +348,11 @@ def test_cli_hysteresis_loop_matches_oracle(tmp_path):
 
 @pytest.mark.parametrize("grid", ["Tet", "Hex"])
 def test_stencil_and_indexed_sell_agree(grid):
-    """The index-free SELL form (structured periodic grid) and the indexed
-    form give the same products; a mesh with permuted node numbering falls
-    back to the indexed form."""
+    """The index-free SELL form (structured periodic grid; on the Kuhn
+    mesh with the neighbour offsets decoded at compile time, on the hex grid
+    with the 26 offsets read per slot) and the indexed form give the same
+    products; a mesh with permuted node numbering falls back to the indexed
+    form."""
     from cupymag_amd._lib import check, lib, ptr
     from cupymag_amd.device import DeviceCSR
     cfg = config(grid, n=6, me=True, defect_params=1.5)
@@ -359,6 +361,7 @@ def test_stencil_and_indexed_sell_agree(grid):
     A_el = S.csr_elasticity()[0]
     A1 = S.csr_GS()[0]
     assert A_el.sell_info()["stencil"] and A1.sell_info()["stencil"]
+    assert A_el.sell_info()["kuhn"] == A1.sell_info()["kuhn"] == (grid == "Tet")
     rng = np.random.default_rng(3)
     for A, bs, K in ((A_el, 3, 1), (A1, 1, 3)):
         ip, ix, v = A.to_arrays()
