@@ -251,6 +251,26 @@ def cpu_baseline(n_cpu, steps=1):
                        f"({ops['n']} DOFs), {steps} step(s), {dt:.1f}s")
 
 
+def copy_rate(mib=1024, reps=10):
+    """Device-to-device copy rate of this GPU (GB/s, bytes read + written)
+    over `reps` copies of `mib` MiB, timed with events: the practical
+    streaming rate that the roofline kernel is also set against."""
+    import torch
+    src = torch.ones(mib << 17, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    del src, dst
+    return 2.0 * (mib << 20) * reps / (ms / 1e3) / 1e9
+
+
 def _build_instances(descs, ctx, workers=4, multigrid=False):
     """SimulationOperators of every instance description (mesh, host
     assembly, upload), built on a few host threads; multigrid: also the
@@ -564,6 +584,7 @@ def main():
     lat = np.array(hm["lat"]) * 1e3
     sp = hm["spmv"]
     traffic = profiled_traffic(sp["rocname"], args.n) if sp.get("rocname") else None
+    copy_gbs = copy_rate()
     out = {
         "metric": "LLG node-steps/s",
         "value": value,
@@ -590,7 +611,8 @@ def main():
                      "unit": "GB/s", "frac": sp["gbs"] / HBM_PEAK_GBS,
                      "traffic": traffic["bytes"] if traffic else None,
                      "traffic_source": traffic["source"] if traffic else None,
-                     "avg_launch_us": sp["avg_us"], "bytes_per_launch": sp["bytes"], "launches": sp["launches"]},
+                     "avg_launch_us": sp["avg_us"], "bytes_per_launch": sp["bytes"], "launches": sp["launches"],
+                     "copy_GBs": copy_gbs, "frac_of_copy": sp["gbs"] / copy_gbs},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
