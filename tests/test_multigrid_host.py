@@ -3,7 +3,10 @@ and the transfer operator pinned by the Galerkin identity of nested P1
 spaces: for an undistorted Kuhn mesh, P^T K_fine P equals the operator
 assembled on the coarse mesh (scalar stiffness and elasticity, operators
 from the oracle's element matrices), which fails for a wrong stencil,
-weight or wrap-around."""
+weight or wrap-around; the non-nested transfers against a brute-force
+search of the coarse tetrahedra; and the periodic stiffness of the floating
+levels (the host assembly without anchors against the oracle's stiffness,
+its recovery from the anchored operator)."""
 import numpy as np
 import scipy.sparse as sp
 
