@@ -173,7 +173,9 @@ def run_gpu(n, steps, warmup, steps64, modes, rel_delta=0.3, autotune=True, rel_
         elif mg:
             # level-0 Chebyshev step of the V-cycle (r -= A d, d = c1 d + c2 D^-1 r,
             # x += d): the operator's planes (stencil form) and diagonal blocks,
-            # d gathered, r, x read and written, D^-1 (9 values) read, d written
+            # d gathered, r, x read and written, D^-1 (9 values) read, d written;
+            # the FP16 forms keep FP32 diagonal blocks and invert them per node
+            # instead of reading D^-1
             s_ = 4 if base == "mixed" else 8
             t_ = "float" if s_ == 4 else "double"
             v_, vs_ = ("_Float16", 2) if mgp == 16 else (t_, s_)
@@ -182,7 +184,10 @@ def run_gpu(n, steps, warmup, steps64, modes, rel_delta=0.3, autotune=True, rel_
             kernel = (f"k_mg_sweep<3,1,{t_},0|4,{v_}> (A_el level 0, Chebyshev step, SELL-64 3x3 blocks, stencil, "
                       f"{v_} off-diagonal values)")
             rocname = f"k_mg_sweep<3, 1, {t_}, 0, true, {v_}>"
-            bytes_launch = (nnz / 9.0 - nb) * 9 * vs_ + nb * 72.0 + nb * 27.0 * s_
+            if mgp == 16:
+                bytes_launch = (nnz / 9.0 - nb) * 9 * vs_ + nb * 36.0 + nb * 18.0 * s_
+            else:
+                bytes_launch = (nnz / 9.0 - nb) * 9 * vs_ + nb * 72.0 + nb * 27.0 * s_
         else:
             # SELL-64 3x3 blocks: off-diagonal blocks 9 values (+ 1 column index
             # unless the index-free stencil form is in use), diagonal blocks 9

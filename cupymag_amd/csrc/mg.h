@@ -19,10 +19,12 @@ struct MgLevel {
   void* dinv = nullptr;     // bs*bs per node: inverse diagonal block
   double lmax = 0.0;        // power-iteration estimate of lambda_max(D^-1 A)
   // FP16 form (precision 16): off-diagonal values times vscale^-1 (a power
-  // of two), in the SELL layout, and FP64 diagonal blocks compensated for
-  // their rounding
+  // of two), in the SELL layout, and diagonal blocks compensated for their
+  // rounding: dgh in FP64 (during set-up only), dg32 its FP32 rounding, which
+  // the FP16 sweeps read (and invert per node)
   void* v16 = nullptr;
   double* dgh = nullptr;
+  float* dg32 = nullptr;
   double vscale = 1.0;
   // floating hierarchy, level 0: the smoothing operator is the periodic
   // stiffness recovered from the anchored operator (owned copies; FP64 for

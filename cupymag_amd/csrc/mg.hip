@@ -160,7 +160,7 @@ __global__ void k_mg_float_row0(int32_t nst, GridArg ga, double* __restrict__ v,
 template <int BS, int K, class T, int MODE, bool ST, class VT>
 __global__ void __launch_bounds__(kBlock) k_mg_sweep(int32_t nb, const int64_t* __restrict__ sptr,
                                                      const int32_t* __restrict__ col, const VT* __restrict__ val,
-                                                     const double* __restrict__ dg, GridArg ga,
+                                                     const sell_diag_t<VT>* __restrict__ dg, GridArg ga,
                                                      const T* __restrict__ din, T* __restrict__ dout,
                                                      T* __restrict__ x, T* __restrict__ r, const T* __restrict__ b,
                                                      const T* __restrict__ dinv, double c1, double c2,
@@ -453,7 +453,8 @@ static const double* ldiag(const MgLevel& Lv) {
 }
 
 template <int BS, int K, class T, int MODE, bool ST, class VT>
-static void launch_sweep(hipStream_t s, int g, const Sell& S, const VT* val, const double* dg, const MgLevel& Lv,
+static void launch_sweep(hipStream_t s, int g, const Sell& S, const VT* val, const sell_diag_t<VT>* dg,
+                         const MgLevel& Lv,
                          const void* din, void* dout, double c1, double c2, double vscale, const CgState* st,
                          hipEvent_t* ev) {
   if (ev)
@@ -479,9 +480,9 @@ static void sweep(cpm_mg* M, const MgLevel& Lv, const void* din, void* dout, dou
     if (M->half) {
       const _Float16* v16 = (const _Float16*)Lv.v16;
       if (S.stencil)
-        launch_sweep<BS, K, T, MODE, true>(s, g, S, v16, Lv.dgh, Lv, din, dout, c1, c2, Lv.vscale, st, ev);
+        launch_sweep<BS, K, T, MODE, true>(s, g, S, v16, Lv.dg32, Lv, din, dout, c1, c2, Lv.vscale, st, ev);
       else
-        launch_sweep<BS, K, T, MODE, false>(s, g, S, v16, Lv.dgh, Lv, din, dout, c1, c2, Lv.vscale, st, ev);
+        launch_sweep<BS, K, T, MODE, false>(s, g, S, v16, Lv.dg32, Lv, din, dout, c1, c2, Lv.vscale, st, ev);
       return;
     }
   }
@@ -1098,6 +1099,21 @@ extern "C" int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, 
       if (e != hipSuccess || !invert_blocks(bs, Lv.nb, dg, inv_t)) {
         rc = fail(CPM_EINVAL, "cpm_mg_create: singular diagonal block (FP32 form)");
         break;
+      }
+      if (half) {
+        // the FP32 diagonal blocks of the FP16 sweeps
+        A((void**)&Lv.dg32, sizeof(float) * nbb);
+        if (rc != CPM_OK) break;
+        std::vector<float> d32(dg.begin(), dg.end());
+        e = hipMemcpy(Lv.dg32, d32.data(), sizeof(float) * nbb, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+          rc = fail(CPM_EHIP, hipGetErrorString(e));
+          break;
+        }
+        // the FP64 compensated blocks served the inverse and the FP32 copy
+        (void)hipFree(Lv.dgh);
+        M->allocs.erase(std::find(M->allocs.begin(), M->allocs.end(), (void*)Lv.dgh));
+        Lv.dgh = nullptr;
       }
       std::vector<float> f(inv_t.begin(), inv_t.end());
       e = hipMemcpy(Lv.dinv, f.data(), sizeof(float) * nbb, hipMemcpyHostToDevice);

@@ -26,6 +26,45 @@ __device__ __forceinline__ void apply_dinv(const T* __restrict__ dinv, int64_t n
     }
 }
 
+// D^-1 v for a node from its diagonal block (FP32 in the FP16 level forms,
+// read back from the cache: the row's product has just loaded it), the
+// inverse by cofactors in FP64 -- the host's rule for the stored D^-1.
+template <int BS, int K, class DG>
+__device__ __forceinline__ void solve_diag(const DG* __restrict__ dg, int64_t node, const double (&v)[BS * K],
+                                           double (&out)[BS * K]) {
+  const DG* a = dg + node * BS * BS;
+  if constexpr (BS == 1) {
+    const double id = 1.0 / (double)a[0];
+#pragma unroll
+    for (int j = 0; j < K; ++j) out[j] = v[j] * id;
+  } else {
+    static_assert(BS == 3, "1x1 or 3x3 blocks");
+    double m[9], inv[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) m[t] = (double)a[t];
+    const double c0 = m[4] * m[8] - m[5] * m[7], c1 = m[5] * m[6] - m[3] * m[8], c2 = m[3] * m[7] - m[4] * m[6];
+    const double id = 1.0 / (m[0] * c0 + m[1] * c1 + m[2] * c2);
+    inv[0] = c0 * id;
+    inv[1] = (m[2] * m[7] - m[1] * m[8]) * id;
+    inv[2] = (m[1] * m[5] - m[2] * m[4]) * id;
+    inv[3] = c1 * id;
+    inv[4] = (m[0] * m[8] - m[2] * m[6]) * id;
+    inv[5] = (m[2] * m[3] - m[0] * m[5]) * id;
+    inv[6] = c2 * id;
+    inv[7] = (m[1] * m[6] - m[0] * m[7]) * id;
+    inv[8] = (m[0] * m[4] - m[1] * m[3]) * id;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        double s = 0.0;
+#pragma unroll
+        for (int e = 0; e < 3; ++e) s += inv[d * 3 + e] * v[e * K + j];
+        out[d * K + j] = s;
+      }
+  }
+}
+
 // all K columns of a solve have converged
 template <int K>
 __device__ __forceinline__ bool all_done(const CgState* st) {
@@ -93,11 +132,13 @@ __device__ __forceinline__ void mg_first_rows(int64_t lb, int64_t lg, int32_t nb
 //         step in one pass (x can not change while neighbours read it).
 // MODE 4: MODE 0 that also adds the first step's d_in to x, rounded as the
 //         separate updates were: x = (x + d_in) + d_out.
-// Off-diagonal values of type VT (T, or _Float16 scaled by 1/vscale).
+// Off-diagonal values of type VT (T, or _Float16 scaled by 1/vscale; the
+// FP16 forms keep FP32 diagonal blocks and take D^-1 r from them, so these
+// sweeps do not read the stored D^-1).
 template <int BS, int K, class T, int MODE, bool ST, class VT = T>
 __device__ __forceinline__ void mg_sweep_rows(int64_t lb, int64_t lg, int32_t nb, const int64_t* __restrict__ sptr,
                                               const int32_t* __restrict__ col, const VT* __restrict__ val,
-                                              const double* __restrict__ dg, const GridArg& ga,
+                                              const sell_diag_t<VT>* __restrict__ dg, const GridArg& ga,
                                               const T* __restrict__ din, T* __restrict__ dout, T* __restrict__ x,
                                               T* __restrict__ r, const T* __restrict__ b,
                                               const T* __restrict__ dinv, double c1, double c2,
@@ -119,7 +160,10 @@ __device__ __forceinline__ void mg_sweep_rows(int64_t lb, int64_t lg, int32_t nb
     }
     if (MODE == 0 || MODE == 3 || MODE == 4) {
       double dv[NV];
-      apply_dinv<BS, K, T>(dinv, row, rv, dv);
+      if constexpr (sizeof(VT) == 2)
+        solve_diag<BS, K>(dg, row, rv, dv);
+      else
+        apply_dinv<BS, K, T>(dinv, row, rv, dv);
 #pragma unroll
       for (int c = 0; c < NV; ++c) {
         const T dn = (MODE == 3) ? (T)(c2 * dv[c]) : (T)(c1 * xo[c] + c2 * dv[c]);

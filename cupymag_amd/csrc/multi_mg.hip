@@ -50,7 +50,7 @@ struct SmoothSeg {
   const int64_t* sptr;
   const int32_t* col;
   const void* val;
-  const double* dg;
+  const void* dg;     // diagonal blocks: sell_diag_t of the value type
   GridArg ga;
   int32_t nb, stencil;
   void *x, *r, *b, *d0, *d1;
@@ -109,10 +109,12 @@ __global__ void __launch_bounds__(kBlock) kbm_sweep(const SmoothSeg* __restrict_
   const double c2 = (MODE == 0 || MODE == 4) ? d.c2[k] : (MODE == 3) ? d.c0 : 0.0;
   const int64_t lb = (int64_t)blockIdx.x - d.blk0;
   if (d.stencil)
-    mg_sweep_rows<BS, K, T, MODE, true, VT>(lb, d.nblk, d.nb, d.sptr, d.col, (const VT*)d.val, d.dg, d.ga, din, dout,
+    mg_sweep_rows<BS, K, T, MODE, true, VT>(lb, d.nblk, d.nb, d.sptr, d.col, (const VT*)d.val,
+                                            (const sell_diag_t<VT>*)d.dg, d.ga, din, dout,
                                             (T*)d.x, (T*)d.r, (const T*)d.b, (const T*)d.dinv, c1, c2, d.vscale);
   else
-    mg_sweep_rows<BS, K, T, MODE, false, VT>(lb, d.nblk, d.nb, d.sptr, d.col, (const VT*)d.val, d.dg, d.ga, din, dout,
+    mg_sweep_rows<BS, K, T, MODE, false, VT>(lb, d.nblk, d.nb, d.sptr, d.col, (const VT*)d.val,
+                                             (const sell_diag_t<VT>*)d.dg, d.ga, din, dout,
                                              (T*)d.x, (T*)d.r, (const T*)d.b, (const T*)d.dinv, c1, c2, d.vscale);
 }
 
@@ -453,7 +455,9 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
         g.val = M->half            ? (const void*)Lv.v16
                 : (M->prec == 32) ? (Lv.kv32 ? (const void*)Lv.kv32 : (const void*)S.v32)
                                   : (Lv.kv64 ? (const void*)Lv.kv64 : (const void*)S.v64);
-        g.dg = M->half ? Lv.dgh : (M->prec == 32) ? (Lv.kdgc ? Lv.kdgc : S.dgc) : (Lv.kdg64 ? Lv.kdg64 : S.dg64);
+        g.dg = M->half            ? (const void*)Lv.dg32
+               : (M->prec == 32) ? (const void*)(Lv.kdgc ? Lv.kdgc : S.dgc)
+                                 : (const void*)(Lv.kdg64 ? Lv.kdg64 : S.dg64);
         g.vscale = M->half ? Lv.vscale : 1.0;
         g.ga = CPM_GRID_ARG(S);
         g.nb = S.nbrows, g.stencil = S.stencil;

@@ -6,6 +6,8 @@
 // Partial sums are stored at partials[k * gridDim.x + blockIdx.x] in both
 // cases; the reducer of a system adds the entries of its own range.
 #pragma once
+#include <type_traits>
+
 #include "device.h"
 
 #ifndef SELL_HALF_UNROLL
@@ -37,16 +39,22 @@ struct GridArg {
 constexpr int kKuhnSlots = 14;
 constexpr int32_t kKuhnCodes[kKuhnSlots] = {0, 1, 4, 5, 16, 17, 20, 22, 25, 26, 37, 38, 41, 42};
 
+// Diagonal blocks of a SELL form with off-diagonal values of type VT: FP64,
+// or FP32 for the FP16 multigrid level forms.
+template <class VT>
+using sell_diag_t = typename std::conditional<sizeof(VT) == 2, float, double>::type;
+
 // sum = row `row` of A x (ADDX32: A (x + x32)), FP64 accumulation, for
 // BS x BS blocks and K interleaved right-hand sides x[(c*BS + e)*K + j];
 // xo = the row's own entries of x (+ x32).  Padding rows (row >= nbrows)
 // only stream their (zero) slots and leave xo unset.  Off-diagonal values
 // of type VT (default T); for VT = _Float16 they are stored times a power of
-// two and their sum is scaled back by vscale.
+// two and their sum is scaled back by vscale, and the diagonal blocks are
+// FP32 (sell_diag_t).
 template <int BS, int K, class T, bool ADDX32, bool ST, class VT = T>
 __device__ __forceinline__ void sell_row(int64_t row, int32_t nbrows, const int64_t* __restrict__ sptr,
                                          const int32_t* __restrict__ col, const VT* __restrict__ val,
-                                         const double* __restrict__ dg, const T* __restrict__ x,
+                                         const sell_diag_t<VT>* __restrict__ dg, const T* __restrict__ x,
                                          const float* __restrict__ x32, const GridArg& ga, double (&sum)[BS * K],
                                          double (&xo)[BS * K], double vscale = 1.0) {
   constexpr int BB = BS * BS;
@@ -168,7 +176,7 @@ __device__ __forceinline__ void sell_row(int64_t row, int32_t nbrows, const int6
     for (int d = 0; d < BS; ++d)
 #pragma unroll
       for (int e = 0; e < BS; ++e) {
-        const double a = dg[row * BB + d * BS + e];
+        const double a = (double)dg[row * BB + d * BS + e];
 #pragma unroll
         for (int j = 0; j < K; ++j) sum[d * K + j] += a * xo[e * K + j];
       }
@@ -180,7 +188,7 @@ __device__ __forceinline__ void sell_row(int64_t row, int32_t nbrows, const int6
 template <int BS, int K, class T, bool CG, bool ADDX32, bool ST, class VT = T>
 __device__ __forceinline__ void sell_rows(int64_t lb, int64_t lg, int32_t nbrows, const int64_t* __restrict__ sptr,
                                           const int32_t* __restrict__ col, const VT* __restrict__ val,
-                                          const double* __restrict__ dg, const T* __restrict__ x,
+                                          const sell_diag_t<VT>* __restrict__ dg, const T* __restrict__ x,
                                           const float* __restrict__ x32, T* __restrict__ y, const GridArg& ga,
                                           double (&acc)[K]) {
   constexpr int NV = BS * K;
