@@ -186,3 +186,42 @@ def test_unanchored_assembly_and_anchor_recovery():
             t = np.zeros(bs * n)
             t[d::bs] = 1.0
             assert np.abs(Au @ t).max() <= 1e-12 * np.abs(Au).max()
+
+
+def test_floating_preconditioner_is_the_anchored_inverse():
+    """The floating hierarchy's preconditioner z = F^T V(F r) + e_0 r_0,
+    F = I - e_0 1^T per component, is the inverse of the anchored operator
+    when V is exact (V = K^+, the pseudo-inverse of the periodic stiffness,
+    as the coarsest solve computes it): checked densely on a jittered 4^3
+    mesh for the scalar and the elasticity operator, with the pseudo-inverse
+    from solvers.multigrid._pinv_periodic (pinned against numpy's pinv)."""
+    from cupymag_amd.mesh.periodic import periodic_node_map
+    from cupymag_amd.physics.host_ops import HostOps, voigt_C
+    from cupymag_amd.solvers.multigrid import _interleave_perm, _pinv_periodic
+    coords, tets = kuhn_mesh(4, jitter=0.1, seed=5)
+    el = _elements(tets)
+    gid = periodic_node_map(coords, coords.min(axis=0), coords.max(axis=0))
+    n = int(gid.max()) + 1
+    Cv = voigt_C(2.4e11, 1.4e11, 1.2e11, True) / 1e11
+    ops = {}
+    for anchors in (True, False):
+        H = HostOps(coords, el, gid, n, None, 2.0, 1.0, Cv, 2, anchors=anchors)
+        ops[anchors] = {k: H.scipy(k).toarray() for k in ("A_demag", "A_el")}
+        H.free()
+    for name, bs in (("A_demag", 1), ("A_el", 3)):
+        perm = _interleave_perm(n, bs)
+        A = ops[True][name][np.ix_(perm, perm)]
+        K = ops[False][name][np.ix_(perm, perm)]
+        Kp = _pinv_periodic(K, bs)
+        assert np.abs(Kp - np.linalg.pinv(K)).max() <= 1e-9 * np.abs(Kp).max()
+        N = bs * n
+        F = np.eye(N)
+        for d in range(bs):
+            F[d, d::bs] -= 1.0          # row of node 0, component d: r_0 - sum of that component
+        E0 = np.zeros((N, N))
+        E0[:bs, :bs] = np.eye(bs)
+        P = F.T @ Kp @ F + E0
+        Ainv = np.linalg.inv(A)
+        assert np.abs(P - Ainv).max() <= 1e-8 * np.abs(Ainv).max(), name
+        # and the plain shift without F (the V-cycle on r as it is) is not
+        assert np.abs(F.T @ Kp + E0 - Ainv).max() > 1e-3 * np.abs(Ainv).max()
