@@ -4,7 +4,10 @@
   level operators, the host transfer matrix (whose stencil the Galerkin
   identity pins in test_multigrid_host.py) and the same Chebyshev interval,
   to 1e-11 relative; the FP32 V-cycle to 1e-5; the V-cycle with FP16
-  operator values (precision 16) to 3e-3.
+  operator values (precision 16) to 3e-3.  For the floating hierarchies of
+  the anchored operators the reference is F^T V(F r) + e_0 r_0 on the
+  periodic stiffness (level 0 recovered from the anchored operator by
+  tests.cases.unanchor, the coarsest solve a pseudo-inverse).
 * The preconditioned CG meets tol 1e-10 (CuPy's rule) and its solution
   matches a sparse direct solve to 1e-8 relative, for the scalar (demag)
   and 3x3-block (elasticity) operators, on regular and jittered meshes, in
