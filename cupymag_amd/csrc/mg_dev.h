@@ -179,7 +179,7 @@ __device__ __forceinline__ void mg_sweep_rows(int64_t lb, int64_t lg, int32_t nb
   }
 }
 
-__device__ __forceinline__ int64_t wrap_idx(int32_t i, int32_t j, int32_t k, const int32_t* dims) {
+__host__ __device__ __forceinline__ int64_t wrap_idx(int32_t i, int32_t j, int32_t k, const int32_t* dims) {
   i += (i < 0) ? dims[0] : 0;
   i -= (i >= dims[0]) ? dims[0] : 0;
   j += (j < 0) ? dims[1] : 0;
@@ -193,13 +193,13 @@ struct Dims3 {
   int32_t d[3];
 };
 
-__device__ __forceinline__ bool nested_dims(const Dims3& fine, const Dims3& coarse) {
+__host__ __device__ __forceinline__ bool nested_dims(const Dims3& fine, const Dims3& coarse) {
   return fine.d[0] == 2 * coarse.d[0] && fine.d[1] == 2 * coarse.d[1] && fine.d[2] == 2 * coarse.d[2];
 }
 
 // Axes by decreasing fraction (t[p0] >= t[p1] >= t[p2]; the comparison
 // order fixes the ties, the same for the prolongation and the restriction).
-__device__ __forceinline__ void kuhn_order(const double (&t)[3], int& p0, int& p1, int& p2) {
+__host__ __device__ __forceinline__ void kuhn_order(const double (&t)[3], int& p0, int& p1, int& p2) {
   int tmp;
   p0 = 0, p1 = 1, p2 = 2;
   if (t[p1] > t[p0]) tmp = p0, p0 = p1, p1 = tmp;
@@ -213,7 +213,7 @@ __device__ __forceinline__ void kuhn_order(const double (&t)[3], int& p0, int& p
 // c + e_p0, c + e_p0 + e_p1, c + (1,1,1) with weights 1 - t_p0, t_p0 - t_p1,
 // t_p1 - t_p2, t_p2 (barycentric coordinates in the Kuhn tetrahedron that
 // holds the point).  For nf = 2 nc these are the nested weights (0.5 pairs).
-__device__ __forceinline__ void kuhn_weights(int32_t i, int32_t j, int32_t k, const Dims3& fine,
+__host__ __device__ __forceinline__ void kuhn_weights(int32_t i, int32_t j, int32_t k, const Dims3& fine,
                                              const Dims3& coarse, int64_t (&v)[4], double (&w)[4]) {
   const int32_t idx[3] = {i, j, k};
   int32_t cell[3];
@@ -247,7 +247,7 @@ __device__ __forceinline__ void kuhn_weights(int32_t i, int32_t j, int32_t k, co
 // gather over the fine nodes whose cell is the node's own or the one below
 // it in each dimension.
 template <int W, class T>
-__device__ __forceinline__ void restrict_node(int64_t I, const Dims3& fine, const Dims3& coarse, bool nested,
+__host__ __device__ __forceinline__ void restrict_node(int64_t I, const Dims3& fine, const Dims3& coarse, bool nested,
                                               bool anchor, const T* __restrict__ rf, double (&acc)[W]) {
 #pragma unroll
   for (int c = 0; c < W; ++c) acc[c] = 0.0;
@@ -416,6 +416,43 @@ __device__ __forceinline__ void mg_restrict_first_rows(int64_t lb, int64_t lg, c
   }
 }
 
+// (P e_c) at fine node f (W values per node); anchor: node 0 excluded on
+// both grids.
+template <int W, class T>
+__host__ __device__ __forceinline__ void prolong_node(int64_t f, const Dims3& fine, const Dims3& coarse, bool nested,
+                                                      bool anchor, const T* __restrict__ ec, double (&v)[W]) {
+  const int32_t i = (int32_t)(f / ((int64_t)fine.d[1] * fine.d[2]));
+  const int32_t j = (int32_t)((f / fine.d[2]) % fine.d[1]);
+  const int32_t k = (int32_t)(f % fine.d[2]);
+  if (nested) {
+    const int32_t a = i & 1, bb = j & 1, cc = k & 1;
+    const int32_t I = i >> 1, J = j >> 1, K = k >> 1;
+    const int64_t c0 = wrap_idx(I, J, K, coarse.d);
+    if ((a | bb | cc) == 0) {
+#pragma unroll
+      for (int c = 0; c < W; ++c) v[c] = (!anchor || c0 != 0) ? (double)ec[c0 * W + c] : 0.0;
+    } else {
+      const int64_t c1 = wrap_idx(I + a, J + bb, K + cc, coarse.d);
+#pragma unroll
+      for (int c = 0; c < W; ++c)
+        v[c] = 0.5 * (((!anchor || c0 != 0) ? (double)ec[c0 * W + c] : 0.0) +
+                      ((!anchor || c1 != 0) ? (double)ec[c1 * W + c] : 0.0));
+    }
+    return;
+  }
+  int64_t cv[4];
+  double w[4];
+  kuhn_weights(i, j, k, fine, coarse, cv, w);
+#pragma unroll
+  for (int c = 0; c < W; ++c) v[c] = 0.0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if ((anchor && cv[q] == 0) || w[q] == 0.0) continue;
+#pragma unroll
+    for (int c = 0; c < W; ++c) v[c] += w[q] * (double)ec[cv[q] * W + c];
+  }
+}
+
 // x_f += P e_c (P1 interpolation); anchor: node 0 excluded on both grids.
 template <int BS, class T>
 __device__ __forceinline__ void mg_prolong_rows(int64_t lb, int64_t lg, const Dims3& fine, const Dims3& coarse,
@@ -425,37 +462,8 @@ __device__ __forceinline__ void mg_prolong_rows(int64_t lb, int64_t lg, const Di
   const bool nested = nested_dims(fine, coarse);
   for (int64_t f = lb * kBlock + threadIdx.x; f < nf; f += stride) {
     if (anchor && f == 0) continue;
-    const int32_t i = (int32_t)(f / ((int64_t)fine.d[1] * fine.d[2]));
-    const int32_t j = (int32_t)((f / fine.d[2]) % fine.d[1]);
-    const int32_t k = (int32_t)(f % fine.d[2]);
     double v[BS];
-    if (nested) {
-      const int32_t a = i & 1, bb = j & 1, cc = k & 1;
-      const int32_t I = i >> 1, J = j >> 1, K = k >> 1;
-      const int64_t c0 = wrap_idx(I, J, K, coarse.d);
-      if ((a | bb | cc) == 0) {
-#pragma unroll
-        for (int c = 0; c < BS; ++c) v[c] = (!anchor || c0 != 0) ? (double)ec[c0 * BS + c] : 0.0;
-      } else {
-        const int64_t c1 = wrap_idx(I + a, J + bb, K + cc, coarse.d);
-#pragma unroll
-        for (int c = 0; c < BS; ++c)
-          v[c] = 0.5 * (((!anchor || c0 != 0) ? (double)ec[c0 * BS + c] : 0.0) +
-                        ((!anchor || c1 != 0) ? (double)ec[c1 * BS + c] : 0.0));
-      }
-    } else {
-      int64_t cv[4];
-      double w[4];
-      kuhn_weights(i, j, k, fine, coarse, cv, w);
-#pragma unroll
-      for (int c = 0; c < BS; ++c) v[c] = 0.0;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        if ((anchor && cv[q] == 0) || w[q] == 0.0) continue;
-#pragma unroll
-        for (int c = 0; c < BS; ++c) v[c] += w[q] * (double)ec[cv[q] * BS + c];
-      }
-    }
+    prolong_node<BS, T>(f, fine, coarse, nested, anchor, ec, v);
 #pragma unroll
     for (int c = 0; c < BS; ++c) xf[f * BS + c] = (T)((double)xf[f * BS + c] + v[c]);
   }

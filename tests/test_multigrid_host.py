@@ -225,3 +225,46 @@ def test_floating_preconditioner_is_the_anchored_inverse():
         assert np.abs(P - Ainv).max() <= 1e-8 * np.abs(Ainv).max(), name
         # and the plain shift without F (the V-cycle on r as it is) is not
         assert np.abs(F.T @ Kp + E0 - Ainv).max() > 1e-3 * np.abs(Ainv).max()
+
+
+def _transfer_host(tmp_path_factory):
+    """tests/transfer_host.hip built for the host (the device transfer rules
+    of mg_dev.h, run on the CPU), or None without hipcc."""
+    import os
+    import shutil
+    import subprocess
+    hipcc = shutil.which("hipcc") or ("/opt/rocm/bin/hipcc" if os.path.exists("/opt/rocm/bin/hipcc") else None)
+    if hipcc is None:
+        return None
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = str(tmp_path_factory.mktemp("transfer") / "transfer_host")
+    subprocess.run([hipcc, "--offload-host-only", "-O2", "-std=c++17", f"-I{root}/cupymag_amd/csrc",
+                    f"-I{root}/include", os.path.join(root, "tests", "transfer_host.hip"), "-o", exe],
+                   check=True, capture_output=True)
+    return exe
+
+
+def test_device_transfer_rules_equal_the_host_prolongation(tmp_path_factory):
+    """The per-node restriction and prolongation of mg_dev.h (compiled for
+    the host) equal P^T r and P e with the host P (which the Galerkin and
+    brute-force tests above pin): nested and non-nested size pairs, a box,
+    with and without the anchor node."""
+    import subprocess
+    import pytest
+    exe = _transfer_host(tmp_path_factory)
+    if exe is None:
+        pytest.skip("hipcc not available")
+    rng = np.random.default_rng(17)
+    for nf3, nc3 in (((16, 16, 16), (8, 8, 8)), ((9, 9, 9), (5, 5, 5)), ((13, 13, 13), (7, 7, 7)),
+                     ((17, 17, 17), (9, 9, 9)), ((9, 7, 5), (5, 4, 3)), ((10, 10, 10), (5, 5, 5))):
+        for anchor in (False, True):
+            P = prolongation(nf3, anchor=anchor, dims_c=nc3)
+            nf, nc = P.shape
+            r = rng.standard_normal(nf)
+            e = rng.standard_normal(nc)
+            out = subprocess.run([exe, *map(str, nf3), *map(str, nc3), str(int(anchor))],
+                                 input=r.tobytes() + e.tobytes(), capture_output=True, check=True).stdout
+            got = np.frombuffer(out, dtype=np.float64)
+            Rr, Pe = got[:nc], got[nc:]
+            np.testing.assert_allclose(Rr, P.T @ r, rtol=0, atol=1e-13 * np.abs(P.T @ r).max())
+            np.testing.assert_allclose(Pe, P @ e, rtol=0, atol=1e-13 * np.abs(P @ e).max())
