@@ -100,3 +100,15 @@ def test_multigrid_rule_and_estimate():
         assert estimate_bytes(64 ** 3, me, 64) > estimate_bytes(64 ** 3, me)
     # 16^3 -> 8^3 with coupling: the dense FP32 inverses take 4 (512^2 + 1536^2) bytes
     assert estimate_bytes(16 ** 3, True, 16) - estimate_bytes(16 ** 3, True) >= 4 * (512 ** 2 + 1536 ** 2)
+
+
+def test_multigrid_estimate_per_mode():
+    """FP64 hierarchies (fast64: FP64 level vectors and dense inverses, the
+    FP64 level-0 stiffness copy of the floating hierarchies) take more than
+    the mixed mode's FP32/FP16 ones; without multigrid the mode changes
+    nothing."""
+    for me in (False, True):
+        assert estimate_bytes(64 ** 3, me, 64, "fast64") > estimate_bytes(64 ** 3, me, 64, "mixed")
+        assert estimate_bytes(64 ** 3, me, None, "fast64") == estimate_bytes(64 ** 3, me, None, "mixed")
+    # 16^3 -> 8^3 with coupling in FP64: the dense inverses take 8 (512^2 + 1536^2) bytes
+    assert estimate_bytes(16 ** 3, True, 16, "fast64") - estimate_bytes(16 ** 3, True) >= 8 * (512 ** 2 + 1536 ** 2)
