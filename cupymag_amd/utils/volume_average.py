@@ -113,49 +113,78 @@ class VolumeAverage:
         return avg[0] if scalar else avg
 
     # ------------------------------------------------------------------ output
-    def write_to_paraview(self, field_dict, filename, alpha=0.5, eps=1e-14):
-        """VTU with nodal values blended with cell-centred averages
-        (volume_average.py:219-318): point data ``alpha*nodal +
-        (1-alpha)*cell-average``, the cell average made consistent across
-        periodic images, cell data ``defect_flag``."""
-        from scipy.sparse import coo_matrix
-        coords, cells, gid = self.coords, self.elements[:, :self.n_nodes_per_elem], self.original_global_id
-        n_nodes, n_elems, npe = coords.shape[0], cells.shape[0], self.n_nodes_per_elem
-        A = None
-        point_data = {}
+    def blend_operator(self, eps=1e-14):
+        """The cell-average part of the ParaView blend as one sparse DOF x DOF
+        operator R, built once per mesh (volume_average.py:219-318 evaluates
+        it per field with gathers and scatters).  For a field f given per
+        DOF, (R f)[d] is the value the reference's blend gives every node of
+        DOF d before mixing with the nodal value:
+
+        * element value: the quadrature mean sum_g |detJ|w_g sum_b N_gb f_b /
+          sum_g |detJ|w_g, or the plain corner mean when the element's
+          quadrature volume is <= eps;
+        * node value: the mean of the element values over the elements
+          incident to the node (a node in no element keeps f at its DOF);
+        * DOF value: the mean of the node values over the periodic images
+          of the DOF.
+
+        Each step is a sparse matrix with rows summing to one, so R is their
+        product: R = Pimg . Ninc . Eq (DOF <- node <- element <- DOF)."""
+        if getattr(self, "_blend", None) is not None and self._blend[0] == eps:
+            return self._blend[1]
+        from scipy.sparse import csr_matrix
+        npe = self.n_nodes_per_elem
+        ne = self.elements.shape[0]
+        nn = self.coords.shape[0]
+        nd = self.ndof
+        gid = self.original_global_id.astype(np.int64)
+        vol = self.vw.sum(axis=1)
+        ok = vol > eps
+        # element <- DOF: quadrature weights of each corner, or 1/npe
+        cw = np.where(ok[:, None], (self.vw @ self.N_gauss) / np.where(ok, vol, 1.0)[:, None], 1.0 / npe)
+        eids = np.repeat(np.arange(ne, dtype=np.int64), npe)
+        Eq = csr_matrix((cw.ravel(), (eids, self.corner_dofs.ravel().astype(np.int64))), shape=(ne, nd))
+        # node <- element: mean over incident elements; orphan nodes map to their DOF
+        nodes = self.elements[:, :npe].ravel().astype(np.int64)
+        inc = np.bincount(nodes, minlength=nn).astype(np.float64)
+        Ninc = csr_matrix((1.0 / inc[nodes], (nodes, eids)), shape=(nn, ne))
+        Rn = (Ninc @ Eq).tocsr()
+        orphan = np.flatnonzero(inc == 0)
+        if orphan.size:
+            Rn = Rn + csr_matrix((np.ones(orphan.size), (orphan, gid[orphan])), shape=(nn, nd))
+        # DOF <- node: mean over the periodic images of each DOF
+        img = np.bincount(gid, minlength=nd).astype(np.float64)
+        Pimg = csr_matrix((1.0 / img[gid], (gid, np.arange(nn, dtype=np.int64))), shape=(nd, nn))
+        R = (Pimg @ Rn).tocsr()
+        R.sum_duplicates()
+        self._blend = (eps, R)
+        return R
+
+    def blended_point_data(self, field_dict, alpha=0.5, eps=1e-14):
+        """Point data of the ParaView output: alpha * nodal value + (1 - alpha)
+        * the blend operator's value at the node's DOF (alpha >= 0.999: the
+        nodal value alone, as volume_average.py:267 skips the blend); vector
+        fields become ``name_1 .. name_C``."""
+        gid = self.original_global_id.astype(np.int64)
+        R = self.blend_operator(eps) if alpha < 0.999 else None
+        out = {}
         for name, f in field_dict.items():
             f = f.detach().cpu().numpy() if isinstance(f, torch.Tensor) else np.asarray(f)
+            cols = f.reshape(f.shape[0], -1)
+            per_dof = cols if R is None else alpha * cols + (1.0 - alpha) * np.asarray(R @ cols)
+            vals = per_dof[gid]
             if f.ndim == 1:
-                f = f[:, None]
-            nC = f.shape[1]
-            node_val = f[gid]
-            if alpha < 0.999:
-                if A is None:
-                    A = coo_matrix((np.ones(n_elems * npe, dtype=np.float32),
-                                    (cells.reshape(-1), np.repeat(np.arange(n_elems), npe))),
-                                   shape=(n_nodes, n_elems)).tocsr()
-                    A_sum = np.asarray(A.sum(axis=1)).ravel()
-                    uniq, inverse, counts = np.unique(gid.astype(np.int64), return_inverse=True, return_counts=True)
-                    dof_inv_cnt = (1.0 / counts)[inverse][:, None]
-                f_e = f[self.corner_dofs]
-                F_e_g = np.tensordot(f_e, self.N_gauss, axes=(1, 1)).transpose(0, 2, 1)
-                num = np.einsum("eg,egc->ec", self.vw, F_e_g)
-                den = np.sum(self.vw, axis=1, keepdims=True)
-                with np.errstate(divide="ignore", invalid="ignore"):
-                    elem_val = np.where(den > eps, num / den, np.mean(f_e, axis=1))
-                cc_val = np.asarray(A @ elem_val)
-                with np.errstate(divide="ignore", invalid="ignore"):
-                    cc_val = np.where(A_sum[:, None] > 0, cc_val / A_sum[:, None], node_val)
-                buf = np.zeros((uniq.size, nC), dtype=cc_val.dtype)
-                np.add.at(buf, inverse, cc_val)
-                cc_val = buf[inverse] * dof_inv_cnt
-                node_val = alpha * node_val + (1.0 - alpha) * cc_val
-            if nC == 1:
-                point_data[name] = node_val[:, 0]
+                out[name] = vals[:, 0]
             else:
-                for c in range(nC):
-                    point_data[f"{name}_{c + 1}"] = node_val[:, c]
-        write_vtu(filename, coords, cells, point_data, {"defect_flag": self.defect_flags})
+                out.update({f"{name}_{c + 1}": vals[:, c] for c in range(vals.shape[1])})
+        return out
+
+    def write_to_paraview(self, field_dict, filename, alpha=0.5, eps=1e-14):
+        """VTU of the mesh with the blended point data and the cell data
+        ``defect_flag`` (volume_average.py:219-318)."""
+        cells = self.elements[:, :self.n_nodes_per_elem]
+        write_vtu(filename, self.coords, cells, self.blended_point_data(field_dict, alpha, eps),
+                  {"defect_flag": self.defect_flags})
         print(f"Field written to {filename}.")
 
 
