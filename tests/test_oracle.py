@@ -361,3 +361,172 @@ def test_step_invariants_with_defect():
         assert not st["m"][dd].any()
         off = np.setdiff1d(np.arange(ops["n"]), dd)
         assert np.allclose(np.linalg.norm(st["m"][off], axis=1), 1.0, atol=1e-14)
+
+
+def _random_rotation(seed):
+    q, r = np.linalg.qr(np.random.default_rng(seed).standard_normal((3, 3)))
+    q = q @ np.diag(np.sign(np.diag(r)))
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q
+
+
+def test_rotated_ellipsoid_axes_are_the_rotation_columns():
+    """RotatedEllipsoid (defect_shapes.py:113-134): the semi-axis k lies
+    along column k of the rotation (x^T R diag(1/a^2) R^T x < 1).  Points at
+    0.999 and 1.001 of each semi-axis along +-R[:, k] fall inside and
+    outside; with the transposed rotation (axes along the rows) they do not,
+    so the check separates R from R^T.  Both the oracle and the product's
+    shape function are checked."""
+    from cupymag_amd.utils.defect_shapes import get_defect_shape_function
+    abc = np.array([5.0, 3.0, 2.0])
+    for seed in (0, 1, 2):
+        R = _random_rotation(seed)
+        assert np.abs(R - R.T).max() > 0.1
+        fns = [om.rotated_ellipsoid(abc, R), get_defect_shape_function("RotatedEllipsoid", (abc, R))]
+        pts, want = [], []
+        for k in range(3):
+            for sgn in (1.0, -1.0):
+                pts += [sgn * 0.999 * abc[k] * R[:, k], sgn * 1.001 * abc[k] * R[:, k]]
+                want += [True, False]
+        pts = np.array(pts)
+        for fn in fns:
+            assert list(fn(pts)) == want
+        assert all(bool(fns[1](p)) == w for p, w in zip(pts, want))     # the product's one-point form
+        assert list(om.rotated_ellipsoid(abc, R.T)(pts)) != want
+
+
+def test_strain_of_affine_displacement_rotated_frame():
+    """compute_E_from_u with R (compute_derivatives.py:94-123): for
+    u = G x the weak-form derivatives are D_i u_j = G[j, i] * lump, so the
+    R-mixed strain is the Voigt form of G R (engineering shears:
+    E12 = (GR)01 + (GR)10 ...) times the lumped mass.  With R^T in place of
+    R the result is the Voigt form of G R^T, which differs for the [111]
+    rotation and for a random one."""
+    from cupymag_amd.mesh.kuhn import kuhn_mesh
+    coords, tets = kuhn_mesh(3, jitter=0.1, seed=2)
+    el = np.concatenate([tets, np.zeros((len(tets), 1), np.int32)], axis=1)
+    gid = np.arange(len(coords), dtype=np.int32)
+    p = params.reduced(config("Tet"))
+    ops = fem.build_operators(coords, el, gid, p)
+    n = ops["n"]
+    G = np.random.default_rng(3).standard_normal((3, 3))
+    u = (coords @ G.T).T.ravel()
+    lump = np.asarray(ops["F1"].sum(axis=1)).ravel() / (0.5854102 + 3 * 0.1381966)
+    for R in (params.rot111_R(), _random_rotation(7)):
+        E = gspm.strain_from_u(ops, n, u, R=R)
+        GR = G @ R
+        ref = [GR[0, 0], GR[1, 1], GR[2, 2], GR[0, 1] + GR[1, 0], GR[1, 2] + GR[2, 1], GR[0, 2] + GR[2, 0]]
+        assert np.allclose(E, np.outer(lump, ref), rtol=1e-10, atol=1e-12)
+        GRt = G @ R.T
+        wrong = [GRt[0, 0], GRt[1, 1], GRt[2, 2], GRt[0, 1] + GRt[1, 0], GRt[1, 2] + GRt[2, 1], GRt[0, 2] + GRt[2, 0]]
+        assert not np.allclose(E, np.outer(lump, wrong), rtol=1e-6, atol=1e-9)
+
+
+def _dense_scatter(Ke, blocks):
+    """Dense global matrix from element matrices: entry (a, b) of element e
+    adds to (blocks[e][a], blocks[e][b])."""
+    N = int(max(b.max() for b in blocks)) + 1
+    D = np.zeros((N, N))
+    for e in range(Ke.shape[0]):
+        idx = blocks[e]
+        for a in range(len(idx)):
+            for b in range(len(idx)):
+                D[idx[a], idx[b]] += Ke[e, a, b]
+    return D
+
+
+@pytest.mark.parametrize("grid", ["Hex", "Tet"])
+def test_operator_surgery_dense(grid):
+    """Anchors and defect surgery, densely on a small periodic mesh with a
+    defect (final_assembly.py:59-158, assemble_demag.py:307-341,
+    assemble_elasticity.py:461-527): every operator equals its element
+    matrices scattered through the DOF map, then A_demag has row/column 0
+    replaced by the identity, A_el rows/columns 0, n, 2n, A1 and A2 the
+    rows/columns of the defect DOFs; F1 loses the defect rows and F_el the
+    anchor rows; Fx, Fy, Fz are untouched.  The identity diagonals must be
+    stored (CG never reads decoupled rows, so a missing one would go
+    unnoticed in a solve)."""
+    from cupymag_amd.mesh.kuhn import kuhn_mesh
+    cfg = config(grid, n=4, me=True, ndef=2, defect_params=0.9)
+    if grid == "Hex":
+        p, ops, _, (coords, el, gid) = oracle_setup(cfg)
+    else:
+        p, ops, _, (coords, el, gid) = oracle_setup(cfg, kuhn_mesh(3, jitter=0.1, seed=4))
+    n = ops["n"]
+    npe = el.shape[1] - 1
+    g = gid[el[:, :npe]].astype(np.int64)
+    dd = ops["DefDOF"]
+    # defect DOFs: every DOF of a node of a flagged element
+    want_dd = sorted({int(gid[v]) for e in range(len(el)) if el[e, -1] == 1 for v in el[e, :npe]})
+    assert list(dd) == want_dd and len(dd) > 0
+    K, M, F = fem.element_matrices(coords, el)
+    Kd, Md = _dense_scatter(K, g), _dense_scatter(M, g)
+
+    def identity_on(D, rows):
+        D = D.copy()
+        D[rows, :] = 0.0
+        D[:, rows] = 0.0
+        D[rows, rows] = 1.0
+        return D
+
+    def check(A, want):
+        A = A.tocsr()
+        assert np.abs(A.toarray() - want).max() <= 1e-13 * np.abs(want).max()
+        return A
+
+    A = check(ops["A_demag"], identity_on(Kd, [0]))
+    assert A[0, 0] == 1.0 and A.indptr[1] - A.indptr[0] >= 1
+    for name, a in (("A1", p["alpha1"]), ("A2", p["alpha2"])):
+        A = check(ops[name], identity_on(Md + a * Kd, dd))
+        assert all(A[d, d] == 1.0 for d in dd)
+    F1 = Md.copy()
+    F1[dd, :] = 0.0
+    check(ops["F1"], F1)
+    for k, name in enumerate(("Fx", "Fy", "Fz")):
+        check(ops[name], _dense_scatter(F[k], g))
+    C = fem.voigt_C(p["c11"], p["c12"], p["c44"], p["rot111"], params.rot111_M())
+    Ke, Fe = fem.elastic_element_matrices(coords, el, C)
+    # element DOF (3 a + d) is the global DOF g_a + d n (assemble_elasticity.py:389-398)
+    blk = np.stack([g[:, a] + d * n for a in range(npe) for d in range(3)], axis=1)
+    perm = np.array([3 * a + d for a in range(npe) for d in range(3)])
+    Kel = _dense_scatter(Ke[:, perm][:, :, perm], blk)
+    A = check(ops["A_el"], identity_on(Kel, [0, n, 2 * n]))
+    assert all(A[r, r] == 1.0 for r in (0, n, 2 * n))
+    Fw = np.zeros((3 * n, 6 * n))
+    for e in range(len(el)):
+        for a in range(npe):
+            for d in range(3):
+                for b in range(npe):
+                    Fw[g[e, a] + d * n, 6 * g[e, b]:6 * g[e, b] + 6] += Fe[e, 3 * a + d, 6 * b:6 * b + 6]
+    Fw[[0, n, 2 * n], :] = 0.0
+    check(ops["F_el"], Fw)
+
+
+def test_hex_defect_box_closed_form():
+    """gridHex.py:99-139: a cell is defective iff its centre lies in the
+    half-open box [c - Nd/2, c + Nd/2); extract_defect_dofs
+    (final_assembly.py:27-57) gives the DOFs of those cells' nodes.
+    32^3 with a centred 4^3 box: cells 14..17 per axis (64 cells) and nodes
+    14..18 (125 DOFs).  A box whose faces pass through cell centres takes
+    the lower face's cells and not the upper's; a box at the periodic face
+    shares DOFs with the opposite face."""
+    coords, el, b = om.hex_grid(32, 32, 32, 4, 4, 4)
+    gid = om.periodic_map(coords, b)
+    assert int(el[:, -1].sum()) == 64
+    assert len(om.defect_dofs(el, gid)) == 125
+    cells = np.argwhere(el[:, -1].reshape(32, 32, 32) == 1)
+    assert cells.min() == 14 and cells.max() == 17
+    # faces through cell centres: [2.5, 5.5) holds centres 2.5, 3.5, 4.5
+    coords, el, b = om.hex_grid(8, 8, 8, 3, 3, 3, defect_center=(4.0, 4.0, 4.0))
+    assert int(el[:, -1].sum()) == 27
+    cells = np.argwhere(el[:, -1].reshape(8, 8, 8) == 1)
+    assert cells.min() == 2 and cells.max() == 4
+    # a box reaching the upper face x = 8: cells 6, 7 in x; nodes x = 6, 7, 8 and 8 wraps to 0
+    coords, el, b = om.hex_grid(8, 8, 8, 3, 3, 3, defect_center=(8.0, 4.0, 4.0))
+    gid = om.periodic_map(coords, b)
+    assert int(el[:, -1].sum()) == 2 * 3 * 3
+    dd = om.defect_dofs(el, gid)
+    assert len(dd) == 3 * 4 * 4
+    x_of_dof = {int(gid[i]): coords[i, 0] for i in range(len(coords)) if coords[i, 0] < 8}
+    assert sorted({x_of_dof[int(d)] for d in dd}) == [0.0, 6.0, 7.0]
