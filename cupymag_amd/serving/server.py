@@ -15,7 +15,11 @@ server
 * steps every member of the batch with one ``GSPMBatch.step`` call per
   step (each solve a batched CG over all members), keeping m, u and the
   CG state of every member resident on the device,
-* retires a request when its last point is done and frees its memory.
+* retires a request when its last point is done and frees its memory,
+* contains failures to their request: a request over the budget is
+  rejected, and one whose preparation raises or whose CG does not converge
+  (the reference's RuntimeError, linear_solvers.py:64-74) is retired with
+  the error text while the others continue.
 
 A member's arithmetic does not depend on the batch (``GSPMBatch``), so a
 request's results are the same alone and in any mix.  ``run`` returns the
@@ -31,6 +35,7 @@ from collections import deque
 
 import numpy as np
 
+from ..core.gspm import SYSTEMS
 from .workloads import initial_m
 
 # device bytes per DOF of a member (operators in CSR and SELL-64 forms with
@@ -80,6 +85,14 @@ def multigrid_ok(n, coarsest_max=8):
     """Whether an n^3 Kuhn mesh has a multigrid hierarchy (the rule of
     solvers.multigrid.build_hierarchies)."""
     return int(n) > coarsest_max
+
+
+def _failed(req, t, error, t_admit=None, points=(), steps=0):
+    """Result record of a request that was rejected or failed: the error
+    text and whatever field points it finished."""
+    return dict(rid=req["rid"], n=req["n"], failed=True, error=error, points=list(points), steps=steps, m=None,
+                loop_latency=t - req["arrival"], queue_delay=None if t_admit is None else t_admit - req["arrival"],
+                point_latency=[])
 
 
 class SweepJob:
@@ -209,7 +222,12 @@ class SweepServer:
                     est = estimate_bytes(req["n"] ** 3, me, req["n"] if self._uses_mg(req["n"]) else None,
                                          self.mode)
                     if est > self.hbm_budget:
-                        raise MemoryError(f"request {req['rid']} needs ~{est / 1e9:.1f} GB, over the HBM budget")
+                        # rejected alone: the others keep being served
+                        waiting.popleft()
+                        results[req["rid"]] = _failed(req, t, f"needs ~{est / 1e9:.1f} GB, over the HBM budget "
+                                                              f"of {self.hbm_budget / 1e9:.1f} GB")
+                        say(f"[serve] t={t:.2f}s reject request {req['rid']}: {results[req['rid']]['error']}")
+                        continue
                     if used + est > self.hbm_budget or len(active) + len(preparing) >= self.max_members:
                         break
                     waiting.popleft()
@@ -219,7 +237,14 @@ class SweepServer:
                 # prepared requests join the batch at this step boundary
                 for fut in [f for f in preparing if f.done()]:
                     req, est, t_adm = preparing.pop(fut)
-                    sim, P, n_dof = fut.result()
+                    try:
+                        sim, P, n_dof = fut.result()
+                    except Exception as e:  # noqa: BLE001 -- any preparation error fails this request only
+                        used -= est
+                        results[req["rid"]] = _failed(req, now(), f"preparation failed: {type(e).__name__}: {e}",
+                                                      t_admit=t_adm)
+                        say(f"[serve] t={now():.2f}s request {req['rid']} failed: {results[req['rid']]['error']}")
+                        continue
                     job = SweepJob(req, sim, P, n_dof, est, req["arrival"], t_adm)
                     job.t_point = now()
                     batch.add(sim)
@@ -234,14 +259,27 @@ class SweepServer:
                         time.sleep(max(0.0, min(0.05, pending[0]["arrival"] - now())))
                     continue
                 max_members = max(max_members, len(active))
-                reps = batch.step([j.field() for j in active], [j.use_init for j in active])
+                reps = batch.step([j.field() for j in active], [j.use_init for j in active], raise_on_fail=False)
                 steps_run += 1
                 t = now()
-                finished = []
+                finished, broken = [], []
                 for job, rep in zip(active, reps):
                     node_steps += job.n_dof
-                    if job.advance(rep, t):
+                    if rep.failed:
+                        # the reference raises here (linear_solvers.py:64-74); the server
+                        # fails this request and keeps stepping the others
+                        broken.append((job, f"Error! CG for {SYSTEMS[rep.failed - 1]} did not converge. "
+                                            f"info={max(rep.iters)}, residual={rep.resid}."))
+                    elif job.advance(rep, t):
                         finished.append(job)
+                for job, err in broken:
+                    batch.remove(job.sim)
+                    active.remove(job)
+                    used -= job.est
+                    results[job.rid] = _failed(job.req, t, err, t_admit=job.t_admit, points=job.points,
+                                               steps=job.steps + 1)
+                    say(f"[serve] t={t:.2f}s request {job.rid} failed at point {job.point}: {err}")
+                    job.sim = None
                 for job in finished:
                     job.t_done = t
                     batch.remove(job.sim)
@@ -258,11 +296,12 @@ class SweepServer:
         finally:
             pool.shutdown(wait=True)
         wall = now()
-        pl = np.concatenate([np.asarray(r["point_latency"]) for r in results.values()]) if results else np.zeros(0)
-        ll = np.asarray([r["loop_latency"] for r in results.values()])
-        qd = np.asarray([r["queue_delay"] for r in results.values()])
+        ok = [r for r in results.values() if not r.get("failed")]
+        pl = np.concatenate([np.asarray(r["point_latency"]) for r in ok]) if ok else np.zeros(0)
+        ll = np.asarray([r["loop_latency"] for r in ok])
+        qd = np.asarray([r["queue_delay"] for r in ok])
         pct = lambda a, q: float(np.percentile(a, q)) if a.size else None  # noqa: E731
-        stats = dict(requests=len(results), wall_s=wall, batch_steps=steps_run, node_steps=node_steps,
+        stats = dict(requests=len(ok), failed=len(results) - len(ok), wall_s=wall, batch_steps=steps_run, node_steps=node_steps,
                      node_steps_per_s=node_steps / wall if wall > 0 else 0.0, max_members=max_members,
                      point_latency_s=dict(p50=pct(pl, 50), p99=pct(pl, 99)),
                      loop_latency_s=dict(p50=pct(ll, 50), p99=pct(ll, 99)),

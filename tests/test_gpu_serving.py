@@ -4,7 +4,7 @@ Each request's results (final m and the average magnetization at every
 field point) are bit-identical whether it is served alone, in a mix that
 arrives while others are running, or under an HBM budget that admits one
 request at a time; the admission estimate bounds the device memory a member
-actually takes.
+actually takes; a failing request leaves the others untouched.
 """
 import numpy as np
 import pytest
@@ -62,8 +62,42 @@ def test_hbm_budget_serialises_admission():
     assert stats["max_members"] == 1
     for r in reqs:
         _same(free[r["rid"]], tight[r["rid"]])
-    with pytest.raises(MemoryError):
-        SweepServer(mode="mixed", hbm_budget=min(ests) * 0.5).run(reqs[:1])
+    res, stats = SweepServer(mode="mixed", hbm_budget=min(ests) * 0.5).run(reqs[:1])
+    assert stats["requests"] == 0 and stats["failed"] == 1
+    assert res[reqs[0]["rid"]]["failed"] and "over the HBM budget" in res[reqs[0]["rid"]]["error"]
+
+
+@pytest.mark.parametrize("precond", [None, "mg"])
+def test_failures_stay_with_their_request(precond):
+    """A request whose CG does not converge (max_iterations 1: the
+    reference's RuntimeError), one whose preparation raises (an unknown
+    defect shape: the reference's bare KeyError) and one over the HBM
+    budget are retired with their error; the healthy requests in the same
+    mix finish with results bit-identical to their runs alone."""
+    import copy
+
+    from cupymag_amd.serving.server import SweepServer, estimate_bytes
+    from cupymag_amd.serving.workloads import sweep_requests
+    reqs = _requests(count=4, seed=12) if precond else _requests(count=4, seed=9)
+    big = sweep_requests(1, rate=200.0, seed=3, nmin=40, nmax=40, points=3, steps_per_point=3, me=True, dt=5e-12)[0]
+    big["rid"], big["arrival"] = 100, 0.001
+    budget = 0.5 * estimate_bytes(40 ** 3, True)
+    assert budget > sum(estimate_bytes(r["n"] ** 3, True, r["n"] if precond else None) for r in reqs)
+    bad_cg, bad_prep = copy.deepcopy(reqs[1]), copy.deepcopy(reqs[2])
+    bad_cg["cfg"]["simulation"]["solver"]["max_iterations"] = 1
+    bad_prep["cfg"]["grid"]["tet_mesh"]["defect_shape"] = "Cube"
+    mix = [reqs[0], bad_cg, bad_prep, reqs[3], big]
+    srv = SweepServer(mode="mixed", precond=precond, hbm_budget=budget)
+    res, stats = srv.run(mix)
+    assert stats["requests"] == 2 and stats["failed"] == 3, stats
+    assert res[bad_cg["rid"]]["failed"]
+    assert res[bad_cg["rid"]]["error"].startswith("Error! CG for ") and "did not converge" in res[bad_cg["rid"]]["error"]
+    assert res[bad_prep["rid"]]["failed"] and "KeyError" in res[bad_prep["rid"]]["error"]
+    assert res[100]["failed"] and "over the HBM budget" in res[100]["error"]
+    for r in (reqs[0], reqs[3]):
+        assert not res[r["rid"]].get("failed")
+        alone, _ = SweepServer(mode="mixed", precond=precond).run([dict(r, arrival=0.0)])
+        _same(res[r["rid"]], alone[r["rid"]])
 
 
 @pytest.mark.parametrize("me,precond,mode", [(False, None, "mixed"), (True, None, "mixed"), (False, "mg", "mixed"),
