@@ -35,9 +35,16 @@ struct CgState {
   double rho, rho_old, alpha, atol, bnorm2, resid, pq, tol;
   int32_t it, done, fail, maxiter;
   double maxr;      // mixed precision: residual norm at the last FP64 update
-  int32_t upd, pad; // mixed precision: FP64 residual update requested
+  int32_t upd;      // mixed precision: FP64 residual update requested
+  int32_t p64;      // mixed-precision PCG: the outer products have switched to FP64
   double beta;      // preconditioned CG: (r - r_old).z / rho_old
+  int32_t it_upd;   // mixed-precision PCG: iteration of the last FP64 residual
+  int32_t pad;
 };
+
+// mixed-precision PCG: longest run of iterations between two FP64
+// residuals (a refinement cycle takes 6-8 when the FP32 products carry it)
+constexpr int kRelCycle = 40;
 
 }  // namespace cpm
 
@@ -211,7 +218,8 @@ enum : int {
   RED_RR_TRUE = 3,
   RED_RR_PCG = 4,
   RED_RR_PCG_REL = 5,   // preconditioned CG with reliable updates: residual norm, update request
-  RED_RR_PCG_TRUE = 6   // ... the FP64 residual replaced the recurrence residual
+  RED_RR_PCG_TRUE = 6,  // ... the FP64 residual replaced the recurrence residual
+  RED_ALPHA_PCG_REL = 7 // alpha of the preconditioned CG whose products may be FP32
 };
 
 template <int MODE>
@@ -228,6 +236,18 @@ __device__ inline void cg_state_update(CgState* st, const double (&v)[K], double
     if (MODE == RED_ALPHA) {
       c.pq = v[j];
       c.alpha = c.rho / v[j];
+    } else if (MODE == RED_ALPHA_PCG_REL) {
+      // p.q of an SPD operator is positive; an FP32 product that has lost
+      // it (search direction in the operator's softest modes) gives no
+      // step: x and r stay, the FP64 residual is taken and the remaining
+      // products of the solve run in FP64
+      c.pq = v[j];
+      c.alpha = c.rho / v[j];
+      if (!(v[j] > 0.0) || !(c.rho > 0.0) || !isfinite(c.alpha)) {
+        c.alpha = 0.0;
+        c.upd = 1;
+        c.p64 = 1;
+      }
     } else if (MODE == RED_RR) {
       // CuPy: rho = r.r; stop when ||r|| <= atol after the update
       c.rho_old = c.rho;
@@ -252,7 +272,7 @@ __device__ inline void cg_state_update(CgState* st, const double (&v)[K], double
       c.resid = sqrt(v[j]);
       if (c.resid <= c.atol) {
         c.done = 1;
-      } else if (c.it >= c.maxiter) {
+      } else if (c.it >= c.maxiter || !isfinite(c.resid)) {
         c.done = 1;
         c.fail = 1;
       }
@@ -262,14 +282,35 @@ __device__ inline void cg_state_update(CgState* st, const double (&v)[K], double
       // the last one, or meets the tolerance, or at maxiter
       c.it += 1;
       c.resid = sqrt(v[j]);
-      if (c.resid < delta * c.maxr || c.resid <= c.atol || c.it >= c.maxiter) c.upd = 1;
+      if (c.resid < delta * c.maxr || c.resid <= c.atol || c.it >= c.maxiter || c.it - c.it_upd >= kRelCycle ||
+          !isfinite(c.resid))
+        c.upd = 1;
     } else if (MODE == RED_RR_PCG_TRUE) {
+      // c.resid still holds the recurrence residual and c.maxr the FP64
+      // residual of the last replacement.  An FP64 residual more than 10x
+      // above the recurrence's or not half the last one, or a cycle of
+      // kRelCycle iterations that did not bring the recurrence residual
+      // down by delta, means the FP32 products no longer carry the
+      // refinement (seen on the anchored operators at tolerances near
+      // 1e-13): the rest of the solve multiplies in FP64
+      //
+      // Once the products are FP64, a replacement that follows a recurrence
+      // residual <= atol and finds the FP64 residual no lower than half the
+      // last one has reached the attainable accuracy of FP64 products
+      // (about eps64 ||A|| ||x||, 1e-13 of ||b|| on the anchored elasticity
+      // operator at 64^3): the solve ends there, converged, as the
+      // reference's test on the recurrence residual would.
+      const double tr = sqrt(v[j]);
+      const bool stalled = tr > 0.5 * c.maxr;
+      const bool floor = c.p64 && stalled && c.resid <= c.atol;
+      if (tr > 10.0 * c.resid || stalled || c.it - c.it_upd >= kRelCycle) c.p64 = 1;
+      c.it_upd = c.it;
       c.upd = 0;
-      c.resid = sqrt(v[j]);
+      c.resid = tr;
       c.maxr = c.resid;
-      if (c.resid <= c.atol) {
+      if (c.resid <= c.atol || (floor && isfinite(tr))) {
         c.done = 1;
-      } else if (c.it >= c.maxiter) {
+      } else if (c.it >= c.maxiter || !isfinite(c.resid)) {
         c.done = 1;
         c.fail = 1;
       }

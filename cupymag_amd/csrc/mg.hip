@@ -52,6 +52,12 @@
 // form, or (delta > 0) the FP32 form with an FP64 residual replacement
 // r = b - A x whenever the recurrence residual has dropped by delta and
 // before stopping, so that the stopping test is on the FP64 residual.
+// When a replacement finds the FP64 residual more than 10x above the
+// recurrence's, or p.q is not positive, the FP32 products can no longer
+// carry the refinement (tight tolerances on the anchored operators, whose
+// condition number times the FP32 rounding nears 1): the rest of the solve
+// takes its products in FP64 (a flag in the CG state, read by the product
+// kernel, so captured iterations follow it too).
 // K columns (K = 3 for the Gauss-Seidel solve) share the products and the
 // V-cycle; each keeps its own scalars and stops on its own.  Every kernel
 // returns at once when the solve is done, so the host enqueues batches of
@@ -306,6 +312,30 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(int32_t nb, const int64_t* 
   sell_rows<BS, K, double, !PLAIN, false, ST, VT>(blockIdx.x, gridDim.x, nb, sptr, col, val, dg, p, nullptr, q, ga,
                                                   acc);
   if (!PLAIN) block_partial<K>(acc, partials);
+}
+
+// q = A p of the mixed-precision PCG: the FP32 form with compensated FP64
+// diagonals, or the FP64 form once the solve has switched its products to
+// FP64 (CgState::p64); partials of p_j.q_j
+template <int BS, int K, bool ST>
+__global__ void __launch_bounds__(kBlock) k_pcg_spmv_rel(int32_t nb, const int64_t* __restrict__ sptr,
+                                                         const int32_t* __restrict__ col, const float* __restrict__ v32,
+                                                         const double* __restrict__ dgc,
+                                                         const double* __restrict__ v64,
+                                                         const double* __restrict__ dg64, GridArg ga,
+                                                         const double* __restrict__ p, double* __restrict__ q,
+                                                         const CgState* st, double* partials) {
+  if (all_done<K>(st)) return;
+  double acc[K];
+#pragma unroll
+  for (int j = 0; j < K; ++j) acc[j] = 0.0;
+  if (any_p64<K>(st))
+    sell_rows<BS, K, double, true, false, ST>(blockIdx.x, gridDim.x, nb, sptr, col, v64, dg64, p, nullptr, q, ga,
+                                              acc);
+  else
+    sell_rows<BS, K, double, true, false, ST, float>(blockIdx.x, gridDim.x, nb, sptr, col, v32, dgc, p, nullptr, q,
+                                                     ga, acc);
+  block_partial<K>(acc, partials);
 }
 
 // reliable update, run when a column asked for one: r = b - A x (FP64
@@ -701,11 +731,11 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
   auto spmv = [&](const double* in, double* out, bool plain) {
     if (!plain && rel) {
       if (S.stencil)
-        k_pcg_spmv<BS, K, true, false, float><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v32, S.dgc,
-                                                                    grid_arg(S), in, out, st, ctx->partials);
+        k_pcg_spmv_rel<BS, K, true><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v32, S.dgc, S.v64, S.dg64,
+                                                          grid_arg(S), in, out, st, ctx->partials);
       else
-        k_pcg_spmv<BS, K, false, false, float><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v32, S.dgc,
-                                                                     grid_arg(S), in, out, st, ctx->partials);
+        k_pcg_spmv_rel<BS, K, false><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v32, S.dgc, S.v64, S.dg64,
+                                                           grid_arg(S), in, out, st, ctx->partials);
       return;
     }
     if (plain) {
@@ -763,7 +793,10 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
     for (int k = 0; k < todo; ++k) {
       pupd();
       spmv(M->p, M->q, false);
-      k_red<K, RED_ALPHA><<<1, kRedBlock, 0, s>>>(gs, ctx->partials, st, 0.0);
+      if (rel)
+        k_red<K, RED_ALPHA_PCG_REL><<<1, kRedBlock, 0, s>>>(gs, ctx->partials, st, 0.0);
+      else
+        k_red<K, RED_ALPHA><<<1, kRedBlock, 0, s>>>(gs, ctx->partials, st, 0.0);
       k_pcg_xr<K><<<gv, kBlock, 0, s>>>(n, x, M->r, M->rold, M->p, M->q, st, ctx->partials);
       if (rel) {
         k_red<K, RED_RR_PCG_REL><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st, delta);

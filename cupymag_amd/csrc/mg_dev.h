@@ -74,6 +74,15 @@ __device__ __forceinline__ bool all_done(const CgState* st) {
   return d;
 }
 
+// some column of a mixed-precision solve has switched its products to FP64
+template <int K>
+__device__ __forceinline__ bool any_p64(const CgState* st) {
+  bool h = false;
+#pragma unroll
+  for (int j = 0; j < K; ++j) h = h || st[j].p64 != 0;
+  return h;
+}
+
 // First Chebyshev step.  ZERO: x0 = 0, so r = b (FROM64: b is the FP64
 // outer residual, also stored as this level's b), d = c0 D^-1 r, x = d.
 // Otherwise r is the current residual: d = c0 D^-1 r, x += d.
@@ -637,11 +646,18 @@ __device__ __forceinline__ bool any_upd(const CgState* st) {
   return u;
 }
 
-// rho = r.z, beta = (r.z - r_old.z) / rho_old (flexible CG), from the sums v
+// rho = r.z, beta = (r.z - r_old.z) / rho_old (flexible CG), from the sums v.
+// r.z of an SPD preconditioner is positive; a preconditioner applied in
+// reduced precision to a residual at its rounding level can lose that, and
+// then the direction restarts (beta = 0) and the products go to FP64.
 __device__ inline void pcg_state_rz(CgState& c, double rz, double rold_z) {
   c.rho_old = c.rho;
   c.rho = rz;
   c.beta = (c.it == 0) ? 0.0 : (rz - rold_z) / c.rho_old;
+  if (!isfinite(c.beta) || !(rz > 0.0)) {
+    c.beta = 0.0;
+    c.p64 = 1;
+  }
 }
 
 }  // namespace cpm

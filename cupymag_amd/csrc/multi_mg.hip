@@ -157,7 +157,7 @@ __global__ void __launch_bounds__(kBlock) kbm_spmv(const OuterSeg* __restrict__ 
   for (int j = 0; j < K; ++j) acc[j] = 0.0;
   const int64_t lb = (int64_t)blockIdx.x - d.sblk0;
   const double* in = PLAIN ? d.x : d.p;
-  if (!PLAIN && d.delta > 0.0) {
+  if (!PLAIN && d.delta > 0.0 && !any_p64<K>(d.st)) {
     if (d.stencil)
       sell_rows<BS, K, double, true, false, true, float>(lb, d.snblk, d.nb, d.sptr, d.col, d.v32, d.dgc, in, nullptr,
                                                          d.q, d.ga, acc);
@@ -300,7 +300,10 @@ __global__ void __launch_bounds__(kRedBlock) kbm_red(const OuterSeg* __restrict_
     else
       cg_state_init<K, false>(d.st, v, d.tol, d.maxiter);
   } else if constexpr (MODE == MRED_ALPHA) {
-    cg_state_update<K, RED_ALPHA>(d.st, v, 0.0);
+    if (d.delta > 0.0)
+      cg_state_update<K, RED_ALPHA_PCG_REL>(d.st, v, 0.0);
+    else
+      cg_state_update<K, RED_ALPHA>(d.st, v, 0.0);
   } else if constexpr (MODE == MRED_RR) {
     if (d.delta > 0.0)
       cg_state_update<K, RED_RR_PCG_REL>(d.st, v, d.delta);

@@ -343,6 +343,8 @@ __device__ inline void cg_state_init(CgState* st, const double (&acc)[2 * K], do
     s.resid = sqrt(acc[K + j]);
     s.maxr = s.resid;
     s.upd = 0;
+    s.p64 = 0;
+    s.it_upd = 0;
     if (REL)
       s.done = (acc[j] == 0.0 || s.resid <= s.atol || maxiter <= 0) ? 1 : 0;
     else
