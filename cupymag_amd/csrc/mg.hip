@@ -40,7 +40,10 @@
 // the V-cycle is a symmetric positive definite preconditioner.  It runs in
 // FP64, FP32 (precision 32, on the FP32 SELL form), or FP32 with the level
 // operators' off-diagonal values in FP16 (precision 16: scaled by a power of
-// two, FP64 diagonal blocks compensated for the rounding).  The restriction
+// two, diagonal blocks compensated for the rounding in FP64 and then stored
+// in FP32, so the stored row sums equal the FP64 ones to the FP32 rounding of
+// the diagonal; every Chebyshev step uses the inverse of those FP32 blocks).
+// The restriction
 // into a level that is not the coarsest is fused with that level's first
 // Chebyshev step.
 //
@@ -181,7 +184,8 @@ __global__ void __launch_bounds__(kBlock) k_mg_sweep(int32_t nb, const int64_t* 
 // blocks a lane's entries 0..7 contiguous, 16 bytes, so that one 128-bit
 // load fetches them, entry 8 of the 64 lanes after them); VT = float: fp32(v64)
 // (vscale 1) in the layout of v64.  dgh = dg64 + the rounding of the row's
-// off-diagonal values, so that the stored row sums equal the FP64 ones.
+// off-diagonal values, so that the row sums with dgh equal the FP64 ones
+// (the FP16 forms keep fp32(dgh): equal to the FP32 rounding of the diagonal).
 template <int BS, class VT>
 __global__ void __launch_bounds__(kBlock) k_mg_make_low(int32_t nb, int64_t nrows_pad, const int64_t* __restrict__ sptr,
                                                         int32_t nst, int stencil, const double* __restrict__ v64,
@@ -1129,6 +1133,11 @@ extern "C" int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, 
     }
     if (precision == 32) {
       e = hipMemcpy(dg.data(), half ? Lv.dgh : ldiag<float>(Lv), sizeof(double) * nbb, hipMemcpyDeviceToHost);
+      // FP16 forms: the sweeps invert the FP32-rounded diagonal blocks per
+      // node, so the stored D^-1 of the first step is the inverse of the
+      // same rounded blocks (one D for every step of the polynomial)
+      if (half)
+        for (double& v : dg) v = (double)(float)v;
       if (e != hipSuccess || !invert_blocks(bs, Lv.nb, dg, inv_t)) {
         rc = fail(CPM_EINVAL, "cpm_mg_create: singular diagonal block (FP32 form)");
         break;
@@ -1143,7 +1152,7 @@ extern "C" int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, 
           rc = fail(CPM_EHIP, hipGetErrorString(e));
           break;
         }
-        // the FP64 compensated blocks served the inverse and the FP32 copy
+        // the FP64 compensated blocks served the FP32 copy (and, rounded, the inverse)
         (void)hipFree(Lv.dgh);
         M->allocs.erase(std::find(M->allocs.begin(), M->allocs.end(), (void*)Lv.dgh));
         Lv.dgh = nullptr;

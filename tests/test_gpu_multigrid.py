@@ -346,3 +346,24 @@ def test_reliable_update_pcg_at_tolerances_below_the_fp64_floor(name, bs, below)
     _, it_a, _ = mg.solve(b, tol=1e-10, maxiter=300)
     _, it_b, _ = mg.solve(b, tol=1e-10, maxiter=300, delta=1e-3)
     assert abs(it_a - it_b) <= 2
+
+
+@pytest.mark.parametrize("name,bs", [("A_demag", 1), ("A_el", 3), ("A1", 1)])
+def test_vcycle_is_symmetric_positive(name, bs):
+    """The V-cycle is a symmetric positive definite preconditioner: the same
+    Chebyshev polynomial in one D^-1 A before and after the coarse
+    correction (for the FP16 level forms, every step inverts the same
+    FP32-rounded diagonal blocks), R = P^T, an exact coarsest solve.  So
+    u.Mv = v.Mu up to the rounding of the V-cycle's vectors (FP64: 1e-12,
+    FP32: 1e-5 of |u| |Mv|) and u.Mu > 0."""
+    _, _, _, S = _setup(n=18, jitter=0.1)
+    rng = np.random.default_rng(8)
+    for prec, tol in ((64, 1e-12), (32, 1e-5), (16, 1e-5)):
+        mg = S.multigrid(precision=prec)[name]
+        N = mg.levels[0].shape[0]
+        for _ in range(3):
+            u, v = rng.standard_normal(N), rng.standard_normal(N)
+            Mu = mg.apply(torch.as_tensor(u, device="cuda")).cpu().numpy()
+            Mv = mg.apply(torch.as_tensor(v, device="cuda")).cpu().numpy()
+            assert abs(u @ Mv - v @ Mu) <= tol * np.linalg.norm(u) * np.linalg.norm(Mv), (prec, u @ Mv, v @ Mu)
+            assert u @ Mu > 0 and v @ Mv > 0
