@@ -483,6 +483,8 @@ struct cpm_sim {
   uint8_t* defect = nullptr;
   double* consts = nullptr;  // R(9) Mv(36) MvinvT(36) Ebar_sigma(6)
   double* dscal = nullptr;
+  double* hscal = nullptr;  // pinned host mirror of dscal (this simulation's own: set_m and the
+                            // step report of simulations sharing a context may run on several threads)
   double *m = nullptr, *mprev = nullptr, *f = nullptr, *ftmp = nullptr, *b = nullptr, *gn = nullptr, *gs = nullptr;
   double *mstar = nullptr, *mss = nullptr, *U = nullptr, *Ht = nullptr, *E = nullptr, *bd = nullptr;
   double *E0 = nullptr, *bel = nullptr, *u = nullptr;
@@ -514,6 +516,7 @@ extern "C" int cpm_sim_destroy(cpm_sim* S) {
     (void)hipStreamSynchronize(S->ctx->stream);
   }
   for (void* p : S->allocs) (void)hipFree(p);
+  if (S->hscal) (void)hipHostFree(S->hscal);
   delete S;
   return CPM_OK;
 }
@@ -578,6 +581,10 @@ extern "C" int cpm_sim_create(cpm_ctx* ctx, const cpm_sim_ops* ops, const cpm_si
   A(&S->W, n);
   A(&S->consts, 96);
   A(&S->dscal, S_NSCAL);
+  if (rc == CPM_OK) {
+    hipError_t e = hipHostMalloc(&S->hscal, sizeof(double) * S_NSCAL, hipHostMallocDefault);
+    if (e != hipSuccess) rc = fail(CPM_ENOMEM, "pinned scalar buffer");
+  }
   A(&S->m, 3 * n);
   A(&S->mprev, 3 * n);
   A(&S->f, 3 * n);
@@ -637,9 +644,9 @@ extern "C" int cpm_sim_set_m(cpm_sim* S, const double* m, double* avg_m) {
   k_setm<<<grid_for(S->n), kBlock, 0, ctx->stream>>>(S->n, m, S->defect, S->W, S->vol, S->m, S->mprev, S->dscal,
                                                       ctx->partials, ctx->counter);
   CPM_HIP(hipGetLastError());
-  CPM_HIP(hipMemcpyAsync(ctx->pinned, S->dscal + S_AVGM, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  CPM_HIP(hipMemcpyAsync(S->hscal, S->dscal + S_AVGM, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   CPM_HIP(hipStreamSynchronize(ctx->stream));
-  if (avg_m) std::memcpy(avg_m, ctx->pinned, 3 * sizeof(double));
+  if (avg_m) std::memcpy(avg_m, S->hscal, 3 * sizeof(double));
   return CPM_OK;
 }
 
@@ -894,9 +901,9 @@ extern "C" int cpm_sim_step(cpm_sim* S, const double* H, int use_init, cpm_step_
     if (int rc = solve_one(S, sp, use_init, rep)) return rc;
     if (int rc = stage_post(S, sp)) return rc;
   }
-  if (int rc = stage_final(S, ctx->pinned)) return rc;
+  if (int rc = stage_final(S, S->hscal)) return rc;
   CPM_HIP(hipStreamSynchronize(ctx->stream));
-  report_scalars(S, ctx->pinned, rep);
+  report_scalars(S, S->hscal, rep);
   return CPM_OK;
 }
 

@@ -215,3 +215,36 @@ def test_hierarchies_built_concurrently_match_sequential():
         xp, itp, _ = hp["A_el"].solve(torch.as_tensor(b, device="cuda"), tol=1e-10, maxiter=500)
         assert its == itp and its < 40
         np.testing.assert_array_equal(xs.cpu().numpy(), xp.cpu().numpy())
+
+
+def test_set_m_from_concurrent_threads_on_one_context():
+    """Simulations sharing one context take their initial m from several
+    host threads (as the serving preparation does): each returns its own
+    average magnetization, equal to a one-at-a-time set_m, because every
+    simulation copies its scalars through its own pinned buffer (a shared
+    context buffer once let one thread read another's average)."""
+    import concurrent.futures as cf
+
+    from cupymag_amd.core.assemble_simulation_operators import SimulationOperators
+    from cupymag_amd.core.gspm import GSPMSimulation
+    from cupymag_amd.core.parameters import from_dict
+    from cupymag_amd.device import context
+    from cupymag_amd.mesh.kuhn import kuhn_mesh
+    ctx = context(0)
+    cfg = config("Tet", n=8, me=True, rot=False, dt=5e-12, defect_params=2.0)
+    P = from_dict(cfg)
+    S = SimulationOperators(P, mesh_arrays=kuhn_mesh(8, jitter=0.1, seed=2), ctx=ctx)
+    sims = [GSPMSimulation(S, P, mode="mixed") for _ in range(8)]
+    m0 = [tilted_m(S.n_dof, seed=k, max_deg=10.0 + 8.0 * k) for k in range(8)]
+    want = [sim.set_m(m).copy() for sim, m in zip(sims, m0)]
+    assert len({tuple(w) for w in want}) == 8
+
+    def go(k):
+        torch.cuda.set_device(ctx.device)
+        return [sims[k].set_m(m0[k]).copy() for _ in range(25)]
+
+    with cf.ThreadPoolExecutor(max_workers=8) as ex:
+        got = list(ex.map(go, range(8)))
+    for k in range(8):
+        for g in got[k]:
+            np.testing.assert_array_equal(g, want[k])
