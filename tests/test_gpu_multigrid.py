@@ -339,7 +339,9 @@ def test_reliable_update_pcg_at_tolerances_below_the_fp64_floor(name, bs, below)
         assert floor > 1e-14                               # the recurrence went below what x attains
     x32, it32, res = mg.solve(b, tol=1e-14, maxiter=300, delta=1e-3)
     true = np.linalg.norm(B - A @ x32.cpu().numpy()) / nb
-    assert np.isfinite(res) and abs(res / nb - true) <= 0.1 * true
+    # at the floor b - A x is rounding noise itself: the device's and
+    # SciPy's summation orders agree to a small factor, not to digits
+    assert np.isfinite(res) and true / 3.0 <= res / nb <= 3.0 * true
     assert true <= max(4.0 * floor, 1e-14), (true, floor)
     assert it32 <= it64 + 12, (it32, it64)
     # at an attainable tolerance the products stay FP32 and the counts match

@@ -79,9 +79,9 @@ def test_failures_stay_with_their_request(precond):
     from cupymag_amd.serving.server import SweepServer, estimate_bytes
     from cupymag_amd.serving.workloads import sweep_requests
     reqs = _requests(count=4, seed=12) if precond else _requests(count=4, seed=9)
-    big = sweep_requests(1, rate=200.0, seed=3, nmin=40, nmax=40, points=3, steps_per_point=3, me=True, dt=5e-12)[0]
+    big = sweep_requests(1, rate=200.0, seed=3, nmin=56, nmax=56, points=3, steps_per_point=3, me=True, dt=5e-12)[0]
     big["rid"], big["arrival"] = 100, 0.001
-    budget = 0.5 * estimate_bytes(40 ** 3, True)
+    budget = 0.5 * estimate_bytes(56 ** 3, True)
     assert budget > sum(estimate_bytes(r["n"] ** 3, True, r["n"] if precond else None) for r in reqs)
     bad_cg, bad_prep = copy.deepcopy(reqs[1]), copy.deepcopy(reqs[2])
     bad_cg["cfg"]["simulation"]["solver"]["max_iterations"] = 1
