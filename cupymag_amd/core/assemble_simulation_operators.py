@@ -17,9 +17,13 @@ from ..utils.volume_average import VolumeAverage
 
 
 class SimulationOperators(FEMMesh):
-    def __init__(self, P, mesh_arrays=None, ctx=None, keep_host=None, nthreads=None, sell=True, autotune=False,
+    def __init__(self, P=None, mesh_arrays=None, ctx=None, keep_host=None, nthreads=None, sell=True, autotune=False,
                  matrix_free_fel=False):
-        """``sell``: also attach the SELL-64 forms the fast / mixed-precision
+        """``P``: the simulation parameters; ``SimulationOperators()`` reads
+        them, as the reference does at import (parameters.py:44-57), from
+        the config named by ``sys.argv[1]``, else ``$CUPYMAG_CONFIG_PATH``,
+        else ./config.yaml (core.parameters.load_parameters).
+        ``sell``: also attach the SELL-64 forms the fast / mixed-precision
         solvers run on (A_demag, A1, A2 scalar; A_el in 3x3 blocks).
         ``autotune``: time the launch schedules of every operator after the
         upload and keep the fastest (cupymag_amd.autotune, cached per size
@@ -27,6 +31,9 @@ class SimulationOperators(FEMMesh):
         and timings.  ``matrix_free_fel``: on a structured Kuhn mesh, F_el is
         not uploaded; ``csr_elasticity`` returns a matrix-free operator
         (cupymag_amd.physics.fel) in its place."""
+        if P is None:
+            from .parameters import load_parameters
+            P = load_parameters()
         super().__init__(P, mesh_arrays)
         self.sell = sell
         self.autotune = autotune

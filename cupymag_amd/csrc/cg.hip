@@ -87,6 +87,43 @@ __global__ void __launch_bounds__(kBlock) k_cg_xr(int64_t n, double* __restrict_
   block_partial<1>(acc, partials);
 }
 
+// Preconditioned CG (cupyx cg with M, which solve_cg forwards,
+// linear_solvers.py:59-62): z = M r with the partials of r.z (k_red then
+// sets rho_old = rho, rho = r.z), p = z or z + (rho/rho_old) p.
+template <int G>
+__global__ void __launch_bounds__(kBlock) k_cg_prec(int32_t nrows, const int32_t* __restrict__ ptr,
+                                                    const int32_t* __restrict__ idx, const double* __restrict__ val,
+                                                    const double* __restrict__ r, double* __restrict__ z,
+                                                    const CgState* st, double* partials) {
+  if (st->done) return;
+  const int lane = threadIdx.x & (G - 1);
+  const int64_t stride = (int64_t)gridDim.x * (kBlock / G);
+  double acc[1] = {0.0};
+  for (int64_t row = (int64_t)blockIdx.x * (kBlock / G) + threadIdx.x / G; row < nrows; row += stride) {
+    const int32_t b = ptr[row], e = ptr[row + 1];
+    double s = 0.0;
+    for (int32_t j = b + lane; j < e; j += G) s += val[j] * r[idx[j]];
+    s = group_sum<G>(s);
+    if (lane == 0) {
+      z[row] = s;
+      acc[0] += r[row] * s;
+    }
+  }
+  block_partial<1>(acc, partials);
+}
+
+__global__ void __launch_bounds__(kBlock) k_cg_pupd_z(int64_t n, const double* __restrict__ z, double* __restrict__ p,
+                                                      const CgState* st) {
+  if (st->done) return;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  if (st->it == 0) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) p[i] = z[i];
+  } else {
+    const double beta = st->rho / st->rho_old;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) p[i] = z[i] + beta * p[i];
+  }
+}
+
 // r = b - A x (use_x0, q holds A x) or x = 0, r = b; ||b||^2, r.r and atol.
 __global__ void __launch_bounds__(kBlock) k_cg_init(int64_t n, const double* __restrict__ b, double* __restrict__ x,
                                                     double* __restrict__ r, const double* __restrict__ q, int use_x0,
@@ -144,6 +181,14 @@ static void launch_spmv_g(const cpm_mat* A, const double* val, const double* x, 
 }
 
 template <int G>
+static void launch_cg_prec_g(cpm_ctx* ctx, const cpm_mat* Mp, CgState* st, int* grid_out) {
+  const int grid = csr_grid<G>(Mp);
+  k_cg_prec<G><<<grid, kBlock, 0, ctx->stream>>>(Mp->nrows, Mp->ptr, Mp->idx, Mp->val, ctx->r, ctx->z, st,
+                                                   ctx->partials);
+  *grid_out = grid;
+}
+
+template <int G>
 static void launch_cg_spmv_g(cpm_ctx* ctx, const cpm_mat* A, CgState* st, int* grid_out) {
   const int grid = csr_grid<G>(A);
   k_cg_spmv<G><<<grid, kBlock, 0, ctx->stream>>>(A->nrows, A->ptr, A->idx, A->val, ctx->p, ctx->q, st,
@@ -171,10 +216,13 @@ int launch_spmv(cpm_ctx* ctx, const cpm_mat* A, int valset, const double* x, dou
   return CPM_OK;
 }
 
+// Mp: the preconditioner (z = Mp r each iteration, CuPy's recurrence with
+// M), or nullptr for plain CG.
 int run_cg(cpm_ctx* ctx, const cpm_mat* A, const double* b, double* x, int use_x0, double tol, int maxiter, int hint,
-           int32_t* iters, double* resid) {
+           int32_t* iters, double* resid, const cpm_mat* Mp) {
   const int64_t n = A->nrows;
   if (A->ncols != A->nrows) return fail(CPM_EINVAL, "CG needs a square matrix");
+  if (Mp && (Mp->nrows != n || Mp->ncols != n)) return fail(CPM_EINVAL, "CG: the preconditioner must be n x n like A");
   if (int rc = ensure_workspace(ctx, n)) return rc;
   CgState* st = ctx->cg;
   hipStream_t s = ctx->stream;
@@ -192,14 +240,24 @@ int run_cg(cpm_ctx* ctx, const cpm_mat* A, const double* b, double* x, int use_x
     const int todo = std::max(1, std::min(std::min(batch, max_batch), maxiter - issued));
     const int it_before = h.it;
     for (int k = 0; k < todo; ++k) {
-      k_cg_pupd<<<gv, kBlock, 0, s>>>(n, ctx->r, ctx->p, st);
+      if (Mp) {
+        int gm = 0;
+        CPM_LANES_DISPATCH(Mp->lanes, launch_cg_prec_g, ctx, Mp, st, &gm);
+        k_red<1, RED_RHO><<<1, kRedBlock, 0, s>>>(gm, ctx->partials, st, 0.0);
+        k_cg_pupd_z<<<gv, kBlock, 0, s>>>(n, ctx->z, ctx->p, st);
+      } else {
+        k_cg_pupd<<<gv, kBlock, 0, s>>>(n, ctx->r, ctx->p, st);
+      }
       if (ctx->prof) CPM_HIP(hipEventRecord(ctx->ev[2 * k], s));
       int gsp = 0;
       CPM_LANES_DISPATCH(A->lanes, launch_cg_spmv_g, ctx, A, st, &gsp);
       if (ctx->prof) CPM_HIP(hipEventRecord(ctx->ev[2 * k + 1], s));
       k_red<1, RED_ALPHA><<<1, kRedBlock, 0, s>>>(gsp, ctx->partials, st, 0.0);
       k_cg_xr<<<gv, kBlock, 0, s>>>(n, x, ctx->r, ctx->p, ctx->q, st, ctx->partials);
-      k_red<1, RED_RR><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st, 0.0);
+      if (Mp)
+        k_red<1, RED_RR_PCG><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st, 0.0);
+      else
+        k_red<1, RED_RR><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st, 0.0);
     }
     issued += todo;
     CPM_HIP(hipGetLastError());
@@ -234,7 +292,13 @@ extern "C" int cpm_spmv(cpm_ctx* ctx, const cpm_mat* A, int valset, const double
 extern "C" int cpm_cg(cpm_ctx* ctx, const cpm_mat* A, const double* b, double* x, int use_x0, double tol, int maxiter,
                       int32_t* iters, double* resid) {
   if (!ctx || !A || !b || !x) return cpm::fail(CPM_EINVAL, "cpm_cg: null argument");
-  return cpm::run_cg(ctx, A, b, x, use_x0, tol, maxiter, 0, iters, resid);
+  return cpm::run_cg(ctx, A, b, x, use_x0, tol, maxiter, 0, iters, resid, nullptr);
+}
+
+extern "C" int cpm_cg_precond(cpm_ctx* ctx, const cpm_mat* A, const cpm_mat* M, const double* b, double* x,
+                              int use_x0, double tol, int maxiter, int32_t* iters, double* resid) {
+  if (!ctx || !A || !M || !b || !x) return cpm::fail(CPM_EINVAL, "cpm_cg_precond: null argument");
+  return cpm::run_cg(ctx, A, b, x, use_x0, tol, maxiter, 0, iters, resid, M);
 }
 
 extern "C" int cpm_prof_enable(cpm_ctx* ctx, int on) {

@@ -14,12 +14,12 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 
 int ensure_workspace(cpm_ctx* ctx, int64_t n) {
   if (n <= ctx->ws_len) return CPM_OK;
-  for (double** b : {&ctx->r, &ctx->p, &ctx->q})
+  for (double** b : {&ctx->r, &ctx->p, &ctx->q, &ctx->z})
     if (*b) {
       CPM_HIP(hipFree(*b));
       *b = nullptr;
     }
-  for (double** b : {&ctx->r, &ctx->p, &ctx->q}) CPM_HIP(hipMalloc(b, sizeof(double) * (size_t)n));
+  for (double** b : {&ctx->r, &ctx->p, &ctx->q, &ctx->z}) CPM_HIP(hipMalloc(b, sizeof(double) * (size_t)n));
   ctx->ws_len = n;
   return CPM_OK;
 }
@@ -73,7 +73,7 @@ extern "C" int cpm_ctx_destroy(cpm_ctx* c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   for (void* p : {(void*)c->partials, (void*)c->counter, (void*)c->scal, (void*)c->cg, (void*)c->r, (void*)c->p,
-                  (void*)c->q, (void*)c->r32, (void*)c->p32, (void*)c->q32, (void*)c->x32, (void*)c->b32,
+                  (void*)c->q, (void*)c->z, (void*)c->r32, (void*)c->p32, (void*)c->q32, (void*)c->x32, (void*)c->b32,
                   (void*)c->y64})
     if (p) (void)hipFree(p);
   if (c->pinned) (void)hipHostFree(c->pinned);

@@ -61,6 +61,7 @@ struct cpm_ctx {
   double* r = nullptr;
   double* p = nullptr;
   double* q = nullptr;
+  double* z = nullptr;  // preconditioned CG: z = M r
   int64_t ws_len = 0;
   int ncu = 256;
   // optional timing of every CG SpMV launch with HIP events (cpm_prof_enable)
@@ -104,8 +105,9 @@ int read_cg_state(cpm_ctx* ctx, int slot, CgState* out);
 int launch_spmv(cpm_ctx* ctx, const cpm_mat* A, int valset, const double* x, double* y);
 // CG on the context stream; syncs to test convergence.  `hint` = expected
 // iteration count (0 = unknown), used only to size launch batches.
+// Mp: preconditioner applied as z = Mp r (CuPy's cg with M), or nullptr.
 int run_cg(cpm_ctx* ctx, const cpm_mat* A, const double* b, double* x, int use_x0, double tol, int maxiter,
-           int hint, int32_t* iters, double* resid);
+           int hint, int32_t* iters, double* resid, const cpm_mat* Mp = nullptr);
 
 // ---- device reductions (blockDim.x == kBlock) -------------------------------
 
@@ -219,7 +221,8 @@ enum : int {
   RED_RR_PCG = 4,
   RED_RR_PCG_REL = 5,   // preconditioned CG with reliable updates: residual norm, update request
   RED_RR_PCG_TRUE = 6,  // ... the FP64 residual replaced the recurrence residual
-  RED_ALPHA_PCG_REL = 7 // alpha of the preconditioned CG whose products may be FP32
+  RED_ALPHA_PCG_REL = 7, // alpha of the preconditioned CG whose products may be FP32
+  RED_RHO = 8            // preconditioned CG (CuPy's recurrence): rho_old = rho, rho = r.z
 };
 
 template <int MODE>
@@ -236,6 +239,9 @@ __device__ inline void cg_state_update(CgState* st, const double (&v)[K], double
     if (MODE == RED_ALPHA) {
       c.pq = v[j];
       c.alpha = c.rho / v[j];
+    } else if (MODE == RED_RHO) {
+      c.rho_old = c.rho;
+      c.rho = v[j];
     } else if (MODE == RED_ALPHA_PCG_REL) {
       // p.q of an SPD operator is positive; an FP32 product that has lost
       // it (search direction in the operator's softest modes) gives no

@@ -100,6 +100,16 @@ int cpm_spmv(cpm_ctx* ctx, const cpm_mat* A, int valset, const double* x, double
 int cpm_cg(cpm_ctx* ctx, const cpm_mat* A, const double* b, double* x, int use_x0, double tol, int maxiter,
            int32_t* iters, double* resid);
 
+/* Preconditioned CG with CuPy's recurrence for a given M.  Replaces
+ * cupyx.scipy.sparse.linalg.cg(A, b, M=M, ...) as solve_cg forwards M,
+ * cupymag/solvers/linear_solvers.py:59-62: each iteration z = M r,
+ * rho = r.z, p = z (first) or z + (rho/rho_old) p, q = A p,
+ * alpha = rho/(p.q), x += alpha p, r -= alpha q, stop when ||r|| <= atol.
+ * M is an n x n operator uploaded like A (cpm_mat_create_csr).  Same
+ * x0, zero-b, return-value and *iters / *resid conventions as cpm_cg. */
+int cpm_cg_precond(cpm_ctx* ctx, const cpm_mat* A, const cpm_mat* M, const double* b, double* x, int use_x0,
+                   double tol, int maxiter, int32_t* iters, double* resid);
+
 /* Kernel timing (no counterpart in the reference, which has only wall-clock
  * prints, Micromagnetics.py:49,106,348-351): with `on`, every CG SpMV launch
  * is bracketed by HIP events on the context stream and its duration is

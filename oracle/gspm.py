@@ -36,8 +36,9 @@ class CGError(RuntimeError):
     pass
 
 
-def cg(A, b, x0=None, tol=1e-7, maxiter=5000):
-    """Return (x, info, iters)."""
+def cg(A, b, x0=None, tol=1e-7, maxiter=5000, M=None):
+    """Return (x, info, iters).  M (cupyx cg's preconditioner, applied as
+    z = M r): rho = r.z, p = z or z + (rho/rho1) p; M None is the identity."""
     n = A.shape[0]
     x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
     bn = np.sqrt(np.dot(b, b))
@@ -50,12 +51,13 @@ def cg(A, b, x0=None, tol=1e-7, maxiter=5000):
     p = None
     resid = np.inf
     while it < maxiter:
+        z = r if M is None else M @ r
         rho1 = rho
-        rho = np.dot(r, r)
+        rho = np.dot(r, z)
         if it == 0:
-            p = r.copy()
+            p = z.copy()
         else:
-            p = r + (rho / rho1) * p
+            p = z + (rho / rho1) * p
         q = A @ p
         alpha = rho / np.dot(p, q)
         x = x + alpha * p

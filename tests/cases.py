@@ -110,3 +110,36 @@ def unanchor(A, bs):
     Dm = sp.lil_matrix((N, N))
     Dm[:bs, :bs] = D0
     return (B + Cm + Cm.T + Dm.tocsr()).tocsr()
+
+
+def cfg2_case(n=64):
+    """The 64^3 magnetoelastic workload: Kuhn tets (6 per cube) with seeded
+    node jitter <= 0.15 h, a spherical non-magnetic defect of radius 8 h at
+    the centre, magnetoelastic coupling on (c11 2.4e11, c12 1.4e11, c44
+    1.2e11 Pa, lambda100 2.6e-4, lambda111 -3e-5), Hext (5e4, 0, 0) A/m,
+    dt 5 ps, alpha 0.5, CG tol 1e-10; m = +x tilted <= 5 deg per DOF (seed 7).
+    Returns (cfg, (coords, tets), m0 as a function of the DOF count)."""
+    from cupymag_amd.mesh.kuhn import kuhn_mesh
+    from cupymag_amd.serving.workloads import instance_config
+    cfg = instance_config((5e4, 0.0, 0.0), me=True, defect_shape="Sphere", defect_params=8.0 * n / 64.0,
+                          dt=5e-12, alpha=0.5)
+    return cfg, kuhn_mesh(n, jitter=0.15, seed=3), (lambda ndof: tilted_m(ndof, seed=7))
+
+
+CFG2_SAMPLES = 4096   # DOFs sampled per field in the golden file
+CFG2_BLOCK = 256      # DOFs per block sum
+
+
+def cfg2_summary(n_dof, fields):
+    """What the golden file keeps of full fields (dict name -> (n_dof, C)
+    array): the values at CFG2_SAMPLES seeded DOFs and the sums over blocks
+    of CFG2_BLOCK consecutive DOFs (a local error anywhere moves its block's
+    sum)."""
+    idx = np.sort(np.random.default_rng(99).choice(n_dof, CFG2_SAMPLES, replace=False))
+    out = {"idx": idx.astype(np.int32)}
+    nb = n_dof // CFG2_BLOCK
+    for k, v in fields.items():
+        v = np.asarray(v, dtype=np.float64).reshape(n_dof, -1)
+        out[f"{k}_samples"] = v[idx]
+        out[f"{k}_blocks"] = v[:nb * CFG2_BLOCK].reshape(nb, CFG2_BLOCK, -1).sum(axis=1)
+    return out

@@ -77,6 +77,52 @@ def test_cg_matches_oracle(which):
     assert _rel(x2.cpu().numpy(), xr2) < 1e-10
 
 
+@pytest.mark.parametrize("which", ["A1", "A_demag", "A_el"])
+def test_preconditioned_cg_matches_oracle(which):
+    """solve_cg(M=...) forwards the preconditioner as the reference does to
+    cupyx cg (linear_solvers.py:59-62): with a Jacobi M (SciPy sparse, and
+    as an uploaded device operator) solutions match the oracle's
+    preconditioned CG to 1e-10, iterations +-1, cold and warm; M = I gives
+    the unpreconditioned iteration count."""
+    import scipy.sparse as sps
+
+    from cupymag_amd.device import DeviceCSR
+    from cupymag_amd.solvers.linear_solvers import solve_cg
+    from oracle import gspm as og
+    cfg = config("Tet", me=True, defect_params=1.8)
+    arr = _kuhn(6)
+    P, S = _product(cfg, arr)
+    p, ops, Avg, _ = oracle_setup(cfg, arr)
+    A_dev = {"A1": S.csr_GS()[0], "A_demag": S.csr_demag_and_deriv()[0], "A_el": S.csr_elasticity()[0]}[which]
+    A_ref = ops[which]
+    n = A_ref.shape[0]
+    J = sps.diags(1.0 / A_ref.diagonal()).tocsr()
+    rng = np.random.default_rng(4)
+    b = rng.standard_normal(n)
+    if which == "A_demag":
+        b[0] = 0.0
+    bd = torch.as_tensor(b, device="cuda")
+    xr, info, itr = og.cg(A_ref, b, None, 1e-10, 5000, M=J)
+    assert info == 0
+    Jd = DeviceCSR(J.indptr, J.indices, J.data, J.shape)
+    for M in (J, Jd):
+        x, it, _ = solve_cg(A_dev, bd, M=M, tol=1e-10, maxiter=5000, return_info=True)
+        assert abs(it - itr) <= 1
+        assert _rel(x.cpu().numpy(), xr) < 1e-10
+    x0 = xr + 1e-6 * rng.standard_normal(n)
+    x2, it2, _ = solve_cg(A_dev, bd, M=Jd, x0=torch.as_tensor(x0, device="cuda"), use_init=True, tol=1e-10,
+                          return_info=True)
+    xr2, _, itr2 = og.cg(A_ref, b, x0, 1e-10, 5000, M=J)
+    assert abs(it2 - itr2) <= 1 and _rel(x2.cpu().numpy(), xr2) < 1e-10
+    _, itI, _ = solve_cg(A_dev, bd, M=sps.identity(n, format="csr"), tol=1e-10, return_info=True)
+    _, it0, _ = solve_cg(A_dev, bd, tol=1e-10, return_info=True)
+    assert itI == it0
+    with pytest.raises(TypeError, match="M must be a sparse or dense matrix"):
+        solve_cg(A_dev, bd, M=lambda r: r)
+    with pytest.raises(RuntimeError, match="did not converge"):
+        solve_cg(A_dev, bd, M=Jd, tol=1e-14, maxiter=2, system="demag")
+
+
 def test_cg_edge_cases():
     from cupymag_amd.solvers.linear_solvers import solve_cg
     cfg = config("Hex", n=6)

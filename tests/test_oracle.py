@@ -228,6 +228,32 @@ def test_cg_iteration_counts():
         gspm.solve_cg(A2, b, tol=1e-14, maxiter=2, system="demag")
 
 
+def test_preconditioned_cg_closed_forms():
+    """cupyx cg with M (what solve_cg forwards, linear_solvers.py:59-62):
+    M = I takes the unpreconditioned iterates exactly; M = A^-1 converges
+    in one iteration; for M = (A restricted to k eigenvalue clusters)^-1
+    ... PCG on M A with k distinct eigenvalues ends in k iterations; a
+    Jacobi M gives the direct solve."""
+    A, Q = _spd(40, 5)
+    b = np.random.default_rng(6).standard_normal(40)
+    x0_, _, it0 = gspm.cg(A, b, tol=1e-12, maxiter=200)
+    x1, _, it1 = gspm.cg(A, b, tol=1e-12, maxiter=200, M=sp.identity(40, format="csr"))
+    assert it0 == it1 and np.array_equal(x0_, x1)
+    Ainv = np.linalg.inv(A.toarray())
+    x2, info2, it2 = gspm.cg(A, b, tol=1e-10, maxiter=50, M=Ainv)
+    assert info2 == 0 and it2 == 1 and np.allclose(x2, spsolve(A.tocsc(), b), rtol=1e-10)
+    # M A has the eigenvalues 1, 2, 5 when M = Q diag(s_i / d_i) Q^T
+    d = np.linalg.eigvalsh(A.toarray())
+    Qd = np.linalg.eigh(A.toarray())[1]
+    s_ = np.repeat([1.0, 2.0, 5.0], [10, 15, 15])
+    Mk = Qd @ np.diag(s_ / d) @ Qd.T
+    _, info3, it3 = gspm.cg(A, b, tol=1e-10, maxiter=50, M=Mk)
+    assert info3 == 0 and it3 == 3
+    J = sp.diags(1.0 / A.diagonal()).tocsr()
+    x4, info4, _ = gspm.cg(A, b, tol=1e-12, maxiter=500, M=J)
+    assert info4 == 0 and np.allclose(x4, spsolve(A.tocsc(), b), rtol=1e-9)
+
+
 def test_build_E0_tensor_definition():
     m = tilted_m(5, 1, 60.0)
     l100, l111 = 2.6e-4, -3e-5
