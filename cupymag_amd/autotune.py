@@ -10,11 +10,16 @@ class) reuse it without timing again.
 """
 import ctypes as C
 import math
+import threading
 
 from . import _lib
 from ._lib import check, lib
 
 _cache = {}
+# one tuning at a time: operators prepared on several host threads (batch
+# and serving preparation) share the context stream, so concurrent timings
+# would include each other's kernels; the lock also tunes a class once
+_lock = threading.Lock()
 
 
 def size_class(A):
@@ -42,18 +47,20 @@ def tune(A, use_cache=True):
     (schedule, timings); timings is None when the size class was already
     tuned and its schedule was applied."""
     key = size_class(A)
-    if use_cache and key in _cache:
-        set_schedule(A, _cache[key])
-        return _cache[key], None
-    cap = 64
-    arr = (_lib.cpm_tune_entry * cap)()
-    n = C.c_int()
-    check(lib().cpm_mat_autotune(A.h, arr, cap, C.byref(n)), "cpm_mat_autotune")
-    timings = [dict(format=_lib.FMT[e.format], lanes=e.lanes, grid_cap=e.grid_cap, us=float(e.us))
-               for e in arr[:min(n.value, cap)]]
-    sched = get_schedule(A)
-    _cache[key] = sched
-    return sched, timings
+    with _lock:
+        if use_cache and key in _cache:
+            set_schedule(A, _cache[key])
+            return _cache[key], None
+        cap = 64
+        arr = (_lib.cpm_tune_entry * cap)()
+        n = C.c_int()
+        A.ctx.sync()
+        check(lib().cpm_mat_autotune(A.h, arr, cap, C.byref(n)), "cpm_mat_autotune")
+        timings = [dict(format=_lib.FMT[e.format], lanes=e.lanes, grid_cap=e.grid_cap, us=float(e.us))
+                   for e in arr[:min(n.value, cap)]]
+        sched = get_schedule(A)
+        _cache[key] = sched
+        return sched, timings
 
 
 def clear_cache():

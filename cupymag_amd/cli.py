@@ -16,6 +16,10 @@ def main(argv=None):
                         help="Path to YAML configuration file (default: config.yaml)")
     parser.add_argument("--version", action="version", version=f"CuPyMag v{__version__} (cupymag_amd)")
     parser.add_argument("--max-steps", type=int, default=None, help="cap on the total number of LLG steps")
+    parser.add_argument("--engine", choices=("auto", "fp64", "fast64", "mixed"), default="auto",
+                        help="step engine: auto follows simulation.precision (DP: fast64, FP64 CG on SELL operators; "
+                             "SP: mixed, FP32 CG in FP64 refinement, multigrid where the mesh has a hierarchy); "
+                             "fp64: CSR operators, the reference's sequence of solves")
     args = parser.parse_args(argv)
     config_path = Path(args.config_file).resolve()
     if not config_path.exists():
@@ -25,7 +29,7 @@ def main(argv=None):
     try:
         os.environ["CUPYMAG_CONFIG_PATH"] = str(config_path)
         from .core.Micromagnetics import main as run_simulation
-        run_simulation(config_path, max_steps=args.max_steps)
+        run_simulation(config_path, max_steps=args.max_steps, engine=args.engine)
         return 0
     except KeyboardInterrupt:
         print("\nSimulation interrupted by user.")

@@ -26,9 +26,33 @@ def _fields(sim, me):
             "E11": E[:, 0], "E22": E[:, 1], "E33": E[:, 2], "E12": E[:, 3], "E23": E[:, 4], "E13": E[:, 5]}
 
 
-def main(config_path=None, max_steps=None, mesh_arrays=None, verbose=True):
+ENGINES = ("auto", "fp64", "fast64", "mixed")
+
+
+def engine_for(P, SimOp, engine="auto"):
+    """GSPMSimulation options of an engine.  "auto" follows the config's
+    ``simulation.precision`` (precision_select.py:37-53): DP -> fast64 (FP64
+    CG with the reference's recurrence and stopping rule on the SELL
+    operators: the plain FP64 path's results to rounding, about twice its
+    speed); SP -> mixed (FP32 CG refined in FP64, magnetization within 1e-7
+    of FP64, where the reference computes everything in FP32), with the
+    multigrid-preconditioned solves when the mesh has a hierarchy (a
+    structured Kuhn mesh of more than 8 cells per dimension).  "fp64": CSR
+    operators, one solve per right-hand side, the reference's sequence."""
+    if engine not in ENGINES:
+        raise ValueError(f"unknown engine {engine!r} (one of {', '.join(ENGINES)})")
+    if engine == "auto":
+        engine = "mixed" if P.precision == "SP" else "fast64"
+    kw = {"mode": engine}
+    if engine == "mixed" and SimOp.multigrid(precision=16) is not None:
+        kw["precond"] = "mg"
+    return kw
+
+
+def main(config_path=None, max_steps=None, mesh_arrays=None, verbose=True, engine="auto"):
     """Run the simulation of ``config_path``; ``max_steps`` caps the total
-    number of LLG steps (the reference loop has no cap)."""
+    number of LLG steps (the reference loop has no cap); ``engine``: see
+    engine_for."""
     P = load_parameters(config_path)
     say = print if verbose else (lambda *a, **k: None)
     out_dir = str(P.output_dir)
@@ -39,7 +63,10 @@ def main(config_path=None, max_steps=None, mesh_arrays=None, verbose=True):
     SimOp = SimulationOperators(P, mesh_arrays=mesh_arrays)
     say(f"Initialized {P.grid_type} FEM mesh with #nodes={SimOp.n_nodes}, #elements={SimOp.n_elements}, "
         f"#defect elements={int(SimOp.elements_np[:, -1].sum())}.")
-    sim = GSPMSimulation(SimOp, P)
+    kw = engine_for(P, SimOp, engine)
+    sim = GSPMSimulation(SimOp, P, **kw)
+    say(f"Engine: {kw['mode']}{' with multigrid-preconditioned solves' if kw.get('precond') else ''} "
+        f"(simulation.precision {P.precision}).")
     Avg = sim.Avg
     nDOF = SimOp.n_dof
     m0 = initialize_m(P.restart_m, P.restart, P.initial_magnetization, nDOF, SimOp.DefDOF)

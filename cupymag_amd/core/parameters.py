@@ -101,6 +101,8 @@ def from_dict(config, base_dir=None):
 
     sim = config["simulation"]
     P.precision = sim["precision"]
+    from ..utils.precision_select import get_float_type
+    get_float_type(P.precision)  # the reference's ValueError for anything but SP / DP
     llg = sim["llg"]
     P.timestep = llg["timestep"]
     P.alpha_damp = llg["alpha_damping"]

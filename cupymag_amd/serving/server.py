@@ -142,10 +142,12 @@ class SweepJob:
 
 class SweepServer:
     def __init__(self, ctx=None, hbm_budget=240e9, mode="mixed", graphs=False, max_members=256, prep_workers=2,
-                 rel_delta=0.3, keep_m=True, precond=None):
+                 rel_delta=0.3, keep_m=True, precond=None, autotune=True):
         """precond="mg": requests whose mesh has a multigrid hierarchy run
         multigrid-preconditioned elasticity and demag solves (batched over
-        those members); the others run plain CG."""
+        those members); the others run plain CG.  autotune: each request's
+        operators take the SpMV schedule of their size class, timed on the
+        device the first time a class is seen (cupymag_amd.autotune)."""
         if precond not in (None, "mg"):
             raise ValueError(f"unknown preconditioner {precond!r} (None or 'mg')")
         from ..device import context
@@ -158,6 +160,7 @@ class SweepServer:
         self.rel_delta = rel_delta
         self.keep_m = keep_m
         self.precond = precond
+        self.autotune = bool(autotune)
 
     def _uses_mg(self, n):
         return self.precond == "mg" and multigrid_ok(n)
@@ -178,7 +181,7 @@ class SweepServer:
         nth = max(1, (os.cpu_count() or 8) // (2 * self.prep_workers))
         with _blas_threads(nth):
             S = SimulationOperators(P, mesh_arrays=arr, ctx=self.ctx, keep_host=False, matrix_free_fel=True,
-                                    nthreads=nth)
+                                    nthreads=nth, autotune=self.autotune)
             S.csr_demag_and_deriv()
             t1 = time.perf_counter()
             sim = GSPMSimulation(S, P, mode=self.mode, rel_delta=self.rel_delta,

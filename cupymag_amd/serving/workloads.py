@@ -52,6 +52,28 @@ def instance_config(hext, me, defect_shape, defect_params, rot111=False, dt=1e-1
     }
 
 
+def headline_config(n=128, tol=1e-12, seed=2024):
+    """The single large instance: a Kuhn mesh of n^3 cells (12.6M tets at
+    n = 128), a rotated-ellipsoid defect with semi-axes (20, 12, 8) h * n/128
+    and a seeded rotation (QR of a Gaussian 3x3, det +1), magnetoelastic
+    coupling on, the crystal x axis along [111], Hext (5e4, 0, 0) A/m,
+    dt 10 ps, alpha 0.1.  tol: CG tolerance -- at 1e-10 the trajectory of
+    this strongly driven state (m reverses within 50 steps) is itself
+    determined only to ~1e-6 after 50 steps by the FP64 solves (their
+    difference from the same solves at 1e-13), so 1e-12 is the default."""
+    rng = np.random.default_rng(seed)
+    q, r = np.linalg.qr(rng.standard_normal((3, 3)))
+    q = q @ np.diag(np.sign(np.diag(r)))
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    s = n / 128.0
+    cfg = instance_config((5e4, 0.0, 0.0), me=True, defect_shape="RotatedEllipsoid",
+                          defect_params=[20.0 * s, 12.0 * s, 8.0 * s], rot111=True, dt=1e-11, alpha=0.1, tol=tol)
+    cfg["grid"]["tet_mesh"]["rotation_matrix"] = q.tolist()
+    cfg["simulation"]["stop_condition"]["value"] = -0.95
+    return cfg
+
+
 def _unit(rng):
     v = rng.standard_normal(3)
     return v / np.linalg.norm(v)

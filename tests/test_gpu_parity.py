@@ -352,15 +352,29 @@ def test_fast_modes_match_oracle(mode, case):
             assert err < 1e-7, f"step {k}: {err}"
 
 
-def test_cli_hysteresis_loop_matches_oracle(tmp_path):
-    """`cupymag config.yaml` end to end on a small hex mesh: field points of
-    hysteresis.txt (LLG convergence -> Hext -= dHext) match the oracle's loop,
-    VTU snapshots and last_m are written."""
+@pytest.mark.parametrize("grid,precision,engine", [("Hex", "DP", "auto"), ("Hex", "DP", "fp64"),
+                                                   ("Hex", "SP", "auto"), ("Tet", "SP", "auto")])
+def test_cli_hysteresis_loop_matches_oracle(tmp_path, grid, precision, engine):
+    """`cupymag config.yaml` end to end: field points of hysteresis.txt (LLG
+    convergence -> Hext -= dHext) match the oracle's loop, VTU snapshots and
+    last_m are written.  simulation.precision picks the engine: DP runs
+    fast64 (FP64 CG, the oracle's iterates to rounding; --engine fp64 the
+    CSR path), SP the mixed-precision engine (within 1e-7), on a Kuhn .nas
+    mesh of 10^3 cells with the multigrid solves."""
     import yaml
 
     from cupymag_amd.cli import main
+    from cupymag_amd.mesh.kuhn import kuhn_mesh, write_nas
     from oracle import gspm as og
-    cfg = config("Hex", n=6, me=False, ndef=2, hext=(-2e4, 0.0, 0.0))
+    arrays = None
+    if grid == "Hex":
+        cfg = config("Hex", n=6, me=False, ndef=2, hext=(-2e4, 0.0, 0.0))
+    else:
+        arrays = kuhn_mesh(10, jitter=0.1, seed=6)
+        write_nas(tmp_path / "mesh.nas", *arrays)
+        cfg = config("Tet", n=10, me=False, hext=(-2e4, 0.0, 0.0), defect_params=2.0)
+        cfg["grid"]["tet_mesh"]["mesh_file"] = "mesh.nas"
+    cfg["simulation"]["precision"] = precision
     cfg["physics"]["external_field"]["dhext1"] = 2e4
     cfg["simulation"]["llg"].update(llg_accuracy_factor=1e-6, timestep=5e-12)
     cfg["simulation"]["stop_condition"] = {"component": 0, "value": -0.5}
@@ -368,26 +382,29 @@ def test_cli_hysteresis_loop_matches_oracle(tmp_path):
                      "vtu_blend_alpha": 0.5}
     path = tmp_path / "cfg.yaml"
     path.write_text(yaml.safe_dump(cfg))
-    assert main([str(path), "--max-steps", "400"]) == 0
+    assert main([str(path), "--max-steps", "400", "--engine", engine]) == 0
     out = tmp_path / "out"
     lines = (out / "hysteresis.txt").read_text().splitlines()
     assert lines[0] == "Hext1\tHext2\tHext3\tavg_m1\tavg_m2\tavg_m3"
     assert list(out.glob("field_*.vtu"))
     from cupymag_amd.utils.magnetization_io import read_array
-    last = read_array(out / "last_m.h5", 6 ** 3)  # an HDF5 file (h5py, or the built-in writer)
+    ndof = 6 ** 3 if grid == "Hex" else 10 ** 3
+    last = read_array(out / "last_m.h5", ndof)  # an HDF5 file (h5py, or the built-in writer)
     nrm = np.linalg.norm(last, axis=1)
     assert np.all((np.abs(nrm - 1) < 1e-12) | (nrm == 0))
-    # oracle: same control flow
-    p, ops, Avg, _ = oracle_setup(cfg)
+    # oracle: same control flow, in FP64
+    cfg_o = dict(cfg, simulation=dict(cfg["simulation"], precision="DP"))
+    p, ops, Avg, _ = oracle_setup(cfg_o, arrays)
     st = og.init_state(ops, Avg, og.uniform_m(ops["n"]))
     pts = og.run(st, ops, Avg, p, 400)
     ms = cfg["physics"]["magnetic"]["ms"]
     got = [list(map(float, l.split("\t"))) for l in lines[1:]]
     # one line per LLG convergence, then the final state
     assert len(pts) >= 1 and len(got) == len(pts) + 1
+    atol = 1e-9 if precision == "DP" else 1e-7
     for (H, am), g in zip(pts, got):
         assert abs(H[0] * ms - g[0]) < 0.1
-        assert np.allclose(am, g[3:], atol=1e-9)
+        assert np.allclose(am, g[3:], atol=atol)
     # the loop stopped on the stop condition (m1 < -0.5) at the next field point
     assert got[-1][3] <= -0.5 and abs(got[-1][0] - (pts[-1][0][0] - p["dH"][0]) * ms) < 0.1
 

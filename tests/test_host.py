@@ -222,3 +222,30 @@ def test_vtu_blend_matches_per_node_evaluation(case, tmp_path):
         assert not np.any(el[:, :8] == 0)
     V.write_to_paraview(fields, str(tmp_path / "f.vtu"), alpha=0.5)
     assert (tmp_path / "f.vtu").stat().st_size > 0
+
+
+def test_cfg2_golden_file_matches_its_case():
+    """The committed oracle file of the 64^3 magnetoelastic workload belongs
+    to tests.cases.cfg2_case (same config, DOF count, sample DOFs), and its
+    oracle magnetization is physical: |m| = 1 or 0 (defect) at the samples,
+    block sums of |m|-bounded entries, and CG iterations of a plausible
+    magnitude for this mesh (elasticity far above demag above Gauss-Seidel)."""
+    import json
+    import os
+
+    from tests.cases import CFG2_BLOCK, CFG2_SAMPLES, cfg2_case
+    fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "cfg2_fp64_oracle.npz"), allow_pickle=False)
+    cfg, (coords, tets), _ = cfg2_case()
+    assert json.loads(str(fx["cfg"])) == json.loads(json.dumps(cfg, sort_keys=True))
+    n = int(fx["n_dof"])
+    assert n == 64 ** 3 and len(tets) == 6 * n
+    idx = np.sort(np.random.default_rng(99).choice(n, CFG2_SAMPLES, replace=False))
+    for k in (1, int(fx["steps"])):
+        np.testing.assert_array_equal(fx[f"s{k}_idx"], idx)
+        nrm = np.linalg.norm(fx[f"s{k}_m_samples"], axis=1)
+        assert np.all((np.abs(nrm - 1) < 1e-12) | (nrm == 0)) and (nrm == 0).any()
+        assert np.abs(fx[f"s{k}_m_blocks"]).max() <= CFG2_BLOCK
+        assert fx[f"s{k}_u_blocks"].shape == (n // CFG2_BLOCK, 3)
+    it = np.asarray(fx["iters"])
+    assert it.shape == (int(fx["steps"]), 10)
+    assert np.all(it[:, 0] > it[:, 1]) and np.all(it[:, 1] > it[:, 2:].max(axis=1))
