@@ -98,6 +98,7 @@ def test_cfg2_mixed_paths_stay_within_1e7_of_plain_fp64_over_100_steps():
             s.step(H, False)
         if step % 20:
             continue
+        print(f"[cfg2] step {step}", flush=True)
         ref = _fields(sims["fp64"], n)
         for name in worst:
             got = _fields(sims[name], n)
@@ -139,6 +140,7 @@ def test_cfg3_mixed_multigrid_within_1e7_of_fp64_after_50_steps():
         assert max(rep.iters[:7]) < 60, rep.iters   # multigrid solves stay short at this tolerance
         if step % 10 == 0:
             err = float((fast.get("M") - ref.get("M")).abs().max())
+            print(f"[cfg3] step {step}: max |m_mixed+mg - m_fast64| = {err:.3e}", flush=True)
             assert err <= 1e-7, (step, err)
     moved = float((ref.get("M") - m_prev).abs().max())
     assert moved > 0.5    # the state really evolves (m reverses in places)
@@ -170,7 +172,11 @@ def test_cfg4_batch_of_64_equals_one_at_a_time():
         return P, S
 
     with cf.ThreadPoolExecutor(max_workers=4) as ex:
-        built = list(ex.map(build, descs))
+        built = []
+        for k, b in enumerate(ex.map(build, descs)):
+            built.append(b)
+            if k % 8 == 7:
+                print(f"[cfg4] {k + 1} instances built", flush=True)
     # every operator carries the schedule of its size class
     from cupymag_amd.autotune import get_schedule, size_class
     by_class = {}
@@ -190,9 +196,10 @@ def test_cfg4_batch_of_64_equals_one_at_a_time():
 
     H = [[P.Hext1, P.Hext2, P.Hext3] for P, _ in built]
     seq = fresh()
-    for _ in range(2):
+    for k in range(2):
         for s, h in zip(seq, H):
             s.step(h, False)
+        print(f"[cfg4] sequential step {k + 1} done", flush=True)
     bat = fresh()
     b = GSPMBatch(ctx)
     for s in bat:
@@ -218,11 +225,12 @@ def test_cfg5_requests_up_to_128_cubed_same_alone_and_in_the_mix():
     assert ns == [20, 26, 106, 125], ns
     for k, r in enumerate(reqs):
         r["arrival"] = 0.01 * k
-    mixed, stats = SweepServer(mode="mixed", precond="mg").run(reqs)
+    say = lambda *a: print(*a, flush=True)  # noqa: E731
+    mixed, stats = SweepServer(mode="mixed", precond="mg").run(reqs, log=say)
     assert stats["requests"] == len(reqs) and stats["failed"] == 0
     assert stats["max_members"] >= 2
     for r in reqs:
-        alone, _ = SweepServer(mode="mixed", precond="mg").run([dict(r, arrival=0.0)])
+        alone, _ = SweepServer(mode="mixed", precond="mg").run([dict(r, arrival=0.0)], log=say)
         a, b = mixed[r["rid"]], alone[r["rid"]]
         np.testing.assert_array_equal(a["m"], b["m"])
         for (h1, m1, s1), (h2, m2, s2) in zip(a["points"], b["points"]):
