@@ -311,6 +311,17 @@ extern "C" int cpm_prof_enable(cpm_ctx* ctx, int on) {
   return CPM_OK;
 }
 
+extern "C" int cpm_mat_prof_outer(cpm_mat* A, double* total_ms, int64_t* launches, int reset) {
+  if (!A) return cpm::fail(CPM_EINVAL, "cpm_mat_prof_outer: null matrix");
+  if (total_ms) *total_ms = A->prof2_ms;
+  if (launches) *launches = A->prof2_launches;
+  if (reset) {
+    A->prof2_ms = 0.0;
+    A->prof2_launches = 0;
+  }
+  return CPM_OK;
+}
+
 extern "C" int cpm_mat_prof(cpm_mat* A, double* total_ms, int64_t* launches, int reset) {
   if (!A) return cpm::fail(CPM_EINVAL, "cpm_mat_prof: null matrix");
   if (total_ms) *total_ms = A->prof_ms;

@@ -87,6 +87,8 @@ struct cpm_mat {
   cpm::Sell sell;          // optional SELL-64 form (cpm_mat_add_sell)
   double prof_ms = 0.0;    // accumulated CG SpMV kernel time (profiling on)
   int64_t prof_launches = 0;
+  double prof2_ms = 0.0;   // multigrid PCG: accumulated time of the outer FP32 product (profiling on)
+  int64_t prof2_launches = 0;
 };
 
 namespace cpm {

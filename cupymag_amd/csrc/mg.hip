@@ -732,9 +732,15 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
   const int gs = sell_grid(S), gv = grid_for(n);
   CgState* st = ctx->cg;
   const bool rel = delta > 0.0;
-  auto spmv = [&](const double* in, double* out, bool plain) {
+  // ev (profiling): the product is timed by its dispatch (mixed, stencil form)
+  auto spmv = [&](const double* in, double* out, bool plain, hipEvent_t* ev = nullptr) {
     if (!plain && rel) {
-      if (S.stencil)
+      if (S.stencil && ev)
+        hipExtLaunchKernelGGL(k_pcg_spmv_rel<BS, K, true>, dim3(gs), dim3(kBlock), 0, s, ev[0], ev[1], 0,
+                              (int32_t)S.nbrows, (const int64_t*)S.sptr, (const int32_t*)S.col,
+                              (const float*)S.v32, (const double*)S.dgc, (const double*)S.v64,
+                              (const double*)S.dg64, grid_arg(S), in, out, (const CgState*)st, ctx->partials);
+      else if (S.stencil)
         k_pcg_spmv_rel<BS, K, true><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v32, S.dgc, S.v64, S.dg64,
                                                           grid_arg(S), in, out, st, ctx->partials);
       else
@@ -796,7 +802,7 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
   auto iterate = [&](int todo, hipEvent_t* ev) -> int {
     for (int k = 0; k < todo; ++k) {
       pupd();
-      spmv(M->p, M->q, false);
+      spmv(M->p, M->q, false, (k == 0 && ev && rel && S.stencil) ? ev + 4 : nullptr);
       if (rel)
         k_red<K, RED_ALPHA_PCG_REL><<<1, kRedBlock, 0, s>>>(gs, ctx->partials, st, 0.0);
       else
@@ -909,6 +915,13 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
           A0->prof_ms += ms;
           A0->prof_launches += 1;
         }
+      // the outer product of that iteration ran whenever the iteration did
+      if (rel && S.stencil && h[0].it > it_before) {
+        float ms = 0.f;
+        CPM_HIP(hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]));
+        A0->prof2_ms += ms;
+        A0->prof2_launches += 1;
+      }
     }
     batch = std::max(2, std::min(16, it_max() / 4 + 2));
   }

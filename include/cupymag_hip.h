@@ -117,6 +117,11 @@ int cpm_cg_precond(cpm_ctx* ctx, const cpm_mat* A, const cpm_mat* M, const doubl
  * total milliseconds and the number of launches that did work. */
 int cpm_prof_enable(cpm_ctx* ctx, int on);
 int cpm_mat_prof(cpm_mat* A, double* total_ms, int64_t* launches, int reset);
+/* No counterpart in the reference (measurement).  With profiling on, the
+ * outer FP32 product of the mixed-precision multigrid PCG on this (level-0)
+ * operator is timed by its dispatch in the first iteration of every
+ * iteration batch; read (and optionally reset) like cpm_mat_prof. */
+int cpm_mat_prof_outer(cpm_mat* A, double* total_ms, int64_t* launches, int reset);
 
 /* Attach a SELL-64 form (sliced ELL, slices of 64 rows, FP64 and FP32 values)
  * to a square operator from the same host CSR arrays it was created from.
