@@ -15,6 +15,8 @@
 // without a device-scope fence per workgroup.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <mutex>
 
 #include "device.h"
 
@@ -85,6 +87,41 @@ __global__ void __launch_bounds__(kBlock) k_cg_xr(int64_t n, double* __restrict_
     acc[0] += ri * ri;
   }
   block_partial<1>(acc, partials);
+}
+
+// ---- CG without device allocation (cpm_cg_noalloc) ----------------------------
+// For callers that cannot allocate device memory (code under graph capture,
+// tight serving loops): the solve's vectors r, p, q, its reduction partials,
+// grid-sum counter and CG state live in one file-scope device array, held by
+// every GPU the library runs on until the process ends.  Calls on one GPU
+// share it: overlapping calls race.
+constexpr int64_t kNoAllocMaxRows = int64_t(1) << 20;
+constexpr int64_t kScratchHead = 64;  // counter (64 unsigned = 32 doubles), then the CgState
+constexpr int64_t kScratchDoubles = kScratchHead + kMaxPartials + 3 * kNoAllocMaxRows;
+static_assert(sizeof(CgState) <= 32 * sizeof(double), "CgState must fit the scratch head");
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[64] = {};
+
+// the device address of `scratch` on `device` (the current device), looked
+// up once per device
+int noalloc_scratch(int device, double** out) {
+  if (device < 0 || device >= 64) return fail(CPM_EINVAL, "no-allocation scratch: device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(CPM_EHIP, std::string("no-allocation scratch: hipGetSymbolAddress: ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = (double*)p;
+  }
+  *out = g_scratch_ptr[device];
+  return CPM_OK;
+}
+
+__global__ void k_zero_counter(unsigned* c) {
+  if (threadIdx.x < 64) c[threadIdx.x] = 0u;
 }
 
 // Preconditioned CG (cupyx cg with M, which solve_cg forwards,
@@ -181,18 +218,16 @@ static void launch_spmv_g(const cpm_mat* A, const double* val, const double* x, 
 }
 
 template <int G>
-static void launch_cg_prec_g(cpm_ctx* ctx, const cpm_mat* Mp, CgState* st, int* grid_out) {
+static void launch_cg_prec_g(cpm_ctx* ctx, const cpm_mat* Mp, const CgWork& w, int* grid_out) {
   const int grid = csr_grid<G>(Mp);
-  k_cg_prec<G><<<grid, kBlock, 0, ctx->stream>>>(Mp->nrows, Mp->ptr, Mp->idx, Mp->val, ctx->r, ctx->z, st,
-                                                   ctx->partials);
+  k_cg_prec<G><<<grid, kBlock, 0, ctx->stream>>>(Mp->nrows, Mp->ptr, Mp->idx, Mp->val, w.r, w.z, w.st, w.partials);
   *grid_out = grid;
 }
 
 template <int G>
-static void launch_cg_spmv_g(cpm_ctx* ctx, const cpm_mat* A, CgState* st, int* grid_out) {
+static void launch_cg_spmv_g(cpm_ctx* ctx, const cpm_mat* A, const CgWork& w, int* grid_out) {
   const int grid = csr_grid<G>(A);
-  k_cg_spmv<G><<<grid, kBlock, 0, ctx->stream>>>(A->nrows, A->ptr, A->idx, A->val, ctx->p, ctx->q, st,
-                                                   ctx->partials);
+  k_cg_spmv<G><<<grid, kBlock, 0, ctx->stream>>>(A->nrows, A->ptr, A->idx, A->val, w.p, w.q, w.st, w.partials);
   *grid_out = grid;
 }
 
@@ -218,18 +253,27 @@ int launch_spmv(cpm_ctx* ctx, const cpm_mat* A, int valset, const double* x, dou
 
 // Mp: the preconditioner (z = Mp r each iteration, CuPy's recurrence with
 // M), or nullptr for plain CG.
+// Wk: the vectors, partials, grid-sum counter and CG state to use (the
+// context's, grown on demand, when null).
 int run_cg(cpm_ctx* ctx, const cpm_mat* A, const double* b, double* x, int use_x0, double tol, int maxiter, int hint,
-           int32_t* iters, double* resid, const cpm_mat* Mp) {
+           int32_t* iters, double* resid, const cpm_mat* Mp, const CgWork* Wk) {
   const int64_t n = A->nrows;
   if (A->ncols != A->nrows) return fail(CPM_EINVAL, "CG needs a square matrix");
   if (Mp && (Mp->nrows != n || Mp->ncols != n)) return fail(CPM_EINVAL, "CG: the preconditioner must be n x n like A");
-  if (int rc = ensure_workspace(ctx, n)) return rc;
-  CgState* st = ctx->cg;
+  if (Wk && Mp) return fail(CPM_EINVAL, "CG: a given workspace has no room for the preconditioner's vector");
+  CgWork w;
+  if (Wk) {
+    w = *Wk;
+  } else {
+    if (int rc = ensure_workspace(ctx, n)) return rc;
+    w = CgWork{ctx->r, ctx->p, ctx->q, ctx->z, ctx->partials, ctx->counter, ctx->cg};
+  }
+  CgState* st = w.st;
   hipStream_t s = ctx->stream;
   const int gv = grid_for(n);
   if (use_x0)
-    if (int rc = launch_spmv(ctx, A, 0, x, ctx->q)) return rc;
-  k_cg_init<<<gv, kBlock, 0, s>>>(n, b, x, ctx->r, ctx->q, use_x0, tol, maxiter, st, ctx->partials, ctx->counter);
+    if (int rc = launch_spmv(ctx, A, 0, x, w.q)) return rc;
+  k_cg_init<<<gv, kBlock, 0, s>>>(n, b, x, w.r, w.q, use_x0, tol, maxiter, st, w.partials, w.counter);
   k_cg_zero_b<<<gv, kBlock, 0, s>>>(n, x, st);
   CPM_HIP(hipGetLastError());
   CgState h{};
@@ -242,26 +286,28 @@ int run_cg(cpm_ctx* ctx, const cpm_mat* A, const double* b, double* x, int use_x
     for (int k = 0; k < todo; ++k) {
       if (Mp) {
         int gm = 0;
-        CPM_LANES_DISPATCH(Mp->lanes, launch_cg_prec_g, ctx, Mp, st, &gm);
-        k_red<1, RED_RHO><<<1, kRedBlock, 0, s>>>(gm, ctx->partials, st, 0.0);
-        k_cg_pupd_z<<<gv, kBlock, 0, s>>>(n, ctx->z, ctx->p, st);
+        CPM_LANES_DISPATCH(Mp->lanes, launch_cg_prec_g, ctx, Mp, w, &gm);
+        k_red<1, RED_RHO><<<1, kRedBlock, 0, s>>>(gm, w.partials, st, 0.0);
+        k_cg_pupd_z<<<gv, kBlock, 0, s>>>(n, w.z, w.p, st);
       } else {
-        k_cg_pupd<<<gv, kBlock, 0, s>>>(n, ctx->r, ctx->p, st);
+        k_cg_pupd<<<gv, kBlock, 0, s>>>(n, w.r, w.p, st);
       }
       if (ctx->prof) CPM_HIP(hipEventRecord(ctx->ev[2 * k], s));
       int gsp = 0;
-      CPM_LANES_DISPATCH(A->lanes, launch_cg_spmv_g, ctx, A, st, &gsp);
+      CPM_LANES_DISPATCH(A->lanes, launch_cg_spmv_g, ctx, A, w, &gsp);
       if (ctx->prof) CPM_HIP(hipEventRecord(ctx->ev[2 * k + 1], s));
-      k_red<1, RED_ALPHA><<<1, kRedBlock, 0, s>>>(gsp, ctx->partials, st, 0.0);
-      k_cg_xr<<<gv, kBlock, 0, s>>>(n, x, ctx->r, ctx->p, ctx->q, st, ctx->partials);
+      k_red<1, RED_ALPHA><<<1, kRedBlock, 0, s>>>(gsp, w.partials, st, 0.0);
+      k_cg_xr<<<gv, kBlock, 0, s>>>(n, x, w.r, w.p, w.q, st, w.partials);
       if (Mp)
-        k_red<1, RED_RR_PCG><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st, 0.0);
+        k_red<1, RED_RR_PCG><<<1, kRedBlock, 0, s>>>(gv, w.partials, st, 0.0);
       else
-        k_red<1, RED_RR><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st, 0.0);
+        k_red<1, RED_RR><<<1, kRedBlock, 0, s>>>(gv, w.partials, st, 0.0);
     }
     issued += todo;
     CPM_HIP(hipGetLastError());
-    if (int rc = read_cg_state(ctx, 0, &h)) return rc;
+    CPM_HIP(hipMemcpyAsync(ctx->pinned, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+    CPM_HIP(hipStreamSynchronize(s));
+    std::memcpy(&h, ctx->pinned, sizeof(CgState));
     if (ctx->prof) {
       // only launches that did work (iterations it_before .. h.it-1 of this batch)
       cpm_mat* Am = const_cast<cpm_mat*>(A);
@@ -293,6 +339,29 @@ extern "C" int cpm_cg(cpm_ctx* ctx, const cpm_mat* A, const double* b, double* x
                       int32_t* iters, double* resid) {
   if (!ctx || !A || !b || !x) return cpm::fail(CPM_EINVAL, "cpm_cg: null argument");
   return cpm::run_cg(ctx, A, b, x, use_x0, tol, maxiter, 0, iters, resid, nullptr);
+}
+
+extern "C" int cpm_cg_noalloc(cpm_ctx* ctx, const cpm_mat* A, const double* b, double* x, int use_x0, double tol,
+                              int maxiter, int32_t* iters, double* resid) {
+  using namespace cpm;
+  if (!ctx || !A || !b || !x) return fail(CPM_EINVAL, "cpm_cg_noalloc: null argument");
+  if (A->nrows > kNoAllocMaxRows)
+    return fail(CPM_EINVAL, "cpm_cg_noalloc: " + std::to_string(A->nrows) + " rows exceed the maximum of " +
+                                std::to_string(kNoAllocMaxRows) + " (use cpm_cg)");
+  if (!ctx->noalloc) return fail(CPM_EHIP, "cpm_cg_noalloc: the context has no scratch address");
+  double* base = ctx->noalloc;
+  CgWork w;
+  w.counter = (unsigned*)base;
+  w.st = (CgState*)(base + 32);
+  w.partials = base + kScratchHead;
+  w.r = w.partials + kMaxPartials;
+  w.p = w.r + kNoAllocMaxRows;
+  w.q = w.p + kNoAllocMaxRows;
+  w.z = nullptr;
+  // the grid-sum counter must start at zero (a fill kernel, also under capture)
+  k_zero_counter<<<1, 64, 0, ctx->stream>>>(w.counter);
+  CPM_HIP(hipGetLastError());
+  return run_cg(ctx, A, b, x, use_x0, tol, maxiter, 0, iters, resid, nullptr, &w);
 }
 
 extern "C" int cpm_cg_precond(cpm_ctx* ctx, const cpm_mat* A, const cpm_mat* M, const double* b, double* x,

@@ -64,6 +64,11 @@ extern "C" int cpm_ctx_create(int device, void* stream, cpm_ctx** out) {
     cpm_ctx_destroy(c);
     return cpm::fail(CPM_EHIP, std::string("context allocation: ") + hipGetErrorString(e));
   }
+  // the no-allocation CG's device scratch on this device (cpm_cg_noalloc)
+  if (int rc = cpm::noalloc_scratch(device, &c->noalloc)) {
+    cpm_ctx_destroy(c);
+    return rc;
+  }
   *out = c;
   return CPM_OK;
 }

@@ -62,6 +62,7 @@ struct cpm_ctx {
   double* p = nullptr;
   double* q = nullptr;
   double* z = nullptr;  // preconditioned CG: z = M r
+  double* noalloc = nullptr;  // device address of the no-allocation CG scratch (cg.hip)
   int64_t ws_len = 0;
   int ncu = 256;
   // optional timing of every CG SpMV launch with HIP events (cpm_prof_enable)
@@ -107,9 +108,22 @@ int read_cg_state(cpm_ctx* ctx, int slot, CgState* out);
 int launch_spmv(cpm_ctx* ctx, const cpm_mat* A, int valset, const double* x, double* y);
 // CG on the context stream; syncs to test convergence.  `hint` = expected
 // iteration count (0 = unknown), used only to size launch batches.
+// Device memory of one CG solve: vectors r, p, q (z: preconditioned CG),
+// the reduction partials (kMaxPartials doubles), the grid-sum counter
+// (zero between uses) and the CG state.
+struct CgWork {
+  double *r, *p, *q, *z;
+  double* partials;
+  unsigned* counter;
+  CgState* st;
+};
 // Mp: preconditioner applied as z = Mp r (CuPy's cg with M), or nullptr.
+// Wk: the workspace to use (nullptr: the context's, grown on demand).
 int run_cg(cpm_ctx* ctx, const cpm_mat* A, const double* b, double* x, int use_x0, double tol, int maxiter,
-           int hint, int32_t* iters, double* resid, const cpm_mat* Mp = nullptr);
+           int hint, int32_t* iters, double* resid, const cpm_mat* Mp = nullptr, const CgWork* Wk = nullptr);
+// Device address of the no-allocation CG scratch on the current device
+// (looked up once per device; cg.hip).
+int noalloc_scratch(int device, double** out);
 
 // ---- device reductions (blockDim.x == kBlock) -------------------------------
 

@@ -110,6 +110,18 @@ int cpm_cg(cpm_ctx* ctx, const cpm_mat* A, const double* b, double* x, int use_x
 int cpm_cg_precond(cpm_ctx* ctx, const cpm_mat* A, const cpm_mat* M, const double* b, double* x, int use_x0,
                    double tol, int maxiter, int32_t* iters, double* resid);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  cpm_cg on the same arguments with the same kernels,
+ * results and return values, but the solve's vectors, reduction partials and
+ * CG state live in a fixed device array of the library (one per GPU, held
+ * until the process ends; its address is looked up when a context is
+ * created), so the call allocates nothing.  At most 1,048,576 rows (larger
+ * A: CPM_EINVAL, use cpm_cg).  Calls that overlap on one GPU share that
+ * array and race; each call starts from a cleared counter. */
+int cpm_cg_noalloc(cpm_ctx* ctx, const cpm_mat* A, const double* b, double* x, int use_x0, double tol, int maxiter,
+                   int32_t* iters, double* resid);
+
 /* Kernel timing (no counterpart in the reference, which has only wall-clock
  * prints, Micromagnetics.py:49,106,348-351): with `on`, every CG SpMV launch
  * is bracketed by HIP events on the context stream and its duration is
