@@ -34,7 +34,7 @@ class CpmError(RuntimeError):
 class cpm_sim_ops(C.Structure):
     _fields_ = [("A_demag", C.c_void_p), ("F3", C.c_void_p), ("A1", C.c_void_p), ("A2", C.c_void_p),
                 ("F1", C.c_void_p), ("A_el", C.c_void_p), ("F_el", C.c_void_p), ("mg_el", C.c_void_p),
-                ("mg_demag", C.c_void_p), ("fel", C.c_void_p), ("mg_a1", C.c_void_p)]
+                ("mg_demag", C.c_void_p), ("fel", C.c_void_p), ("mg_a1", C.c_void_p), ("fder", C.c_void_p)]
 
 
 class cpm_sim_params(C.Structure):
@@ -79,6 +79,9 @@ _SIGS = {
     "cpm_prof_enable": (C.c_int, [_P, C.c_int]),
     "cpm_mat_prof": (C.c_int, [_P, C.POINTER(_D), C.POINTER(_I64), C.c_int]),
     "cpm_mat_prof_outer": (C.c_int, [_P, C.POINTER(_D), C.POINTER(_I64), C.c_int]),
+    "cpm_fel_strain": (C.c_int, [_P, _P, _P, _P]),
+    "cpm_fel_hd": (C.c_int, [_P, _P, _P]),
+    "cpm_fel_dmul": (C.c_int, [_P, C.c_int, _P, _P]),
     "cpm_mat_add_sell": (C.c_int, [_P, C.c_int, _P, _P, _P, _P]),
     "cpm_mat_sell_info": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(_I64)]),
     "cpm_cg_sell": (C.c_int, [_P, _P, C.c_int, _P, _P, C.c_int, _D, C.c_int, C.c_int, _D, C.POINTER(_I32),

@@ -18,7 +18,7 @@ from ..utils.volume_average import VolumeAverage
 
 class SimulationOperators(FEMMesh):
     def __init__(self, P=None, mesh_arrays=None, ctx=None, keep_host=None, nthreads=None, sell=True, autotune=False,
-                 matrix_free_fel=False):
+                 matrix_free_fel=False, matrix_free_deriv=False):
         """``P``: the simulation parameters; ``SimulationOperators()`` reads
         them, as the reference does at import (parameters.py:44-57), from
         the config named by ``sys.argv[1]``, else ``$CUPYMAG_CONFIG_PATH``,
@@ -30,7 +30,11 @@ class SimulationOperators(FEMMesh):
         class); ``self.tuning`` then maps operator names to their schedule
         and timings.  ``matrix_free_fel``: on a structured Kuhn mesh, F_el is
         not uploaded; ``csr_elasticity`` returns a matrix-free operator
-        (cupymag_amd.physics.fel) in its place."""
+        (cupymag_amd.physics.fel) in its place.  ``matrix_free_deriv``: likewise
+        Fx, Fy, Fz (their 0.9 GB CSR at 128^3 is not uploaded;
+        ``csr_demag_and_deriv`` returns matrix-free operators, and the step
+        forms the strain, Htilde and the demag right-hand side by walking the
+        elements)."""
         if P is None:
             from .parameters import load_parameters
             P = load_parameters()
@@ -39,6 +43,7 @@ class SimulationOperators(FEMMesh):
         self.autotune = autotune
         self.tuning = {}
         self.matrix_free_fel = matrix_free_fel
+        self.matrix_free_deriv = matrix_free_deriv
         self.alpha1 = P.alpha
         self.alpha2 = P.alpha * P.alpha_damp
         self.ctx = ctx
@@ -71,16 +76,22 @@ class SimulationOperators(FEMMesh):
 
         d = {}
         d["A_demag"] = up("A_demag", sell_bs=1)
-        F3v = np.stack([H.array("FX"), H.array("FY"), H.array("FZ")])
-        d["F3"] = up("Fx", F3v)
-        del F3v
+        mf = None
+        if self.matrix_free_deriv or (P.ME and self.matrix_free_fel):
+            mf = self._matrix_free_fel(ctx, Cv if (P.ME and self.matrix_free_fel) else None)
+        d["fder"] = mf if (self.matrix_free_deriv and mf is not None) else None
+        if d["fder"] is None:
+            F3v = np.stack([H.array("FX"), H.array("FY"), H.array("FZ")])
+            d["F3"] = up("Fx", F3v)
+            del F3v
+        else:
+            d["F3"] = None
         for k in ("A1", "A2"):
             d[k] = up(k, sell_bs=1)
         d["F1"] = up("F1")
         if P.ME:
             d["A_el"] = up("A_el", sell_bs=3)
-            mf = self._matrix_free_fel(ctx, Cv) if self.matrix_free_fel else None
-            d["F_el"] = mf if mf is not None else up("F_el")
+            d["F_el"] = mf if (self.matrix_free_fel and mf is not None) else up("F_el")
         H.free()
         if self.autotune:
             from ..autotune import tune
@@ -113,6 +124,9 @@ class SimulationOperators(FEMMesh):
 
     def csr_demag_and_deriv(self):
         d = self._upload()
+        if d["F3"] is None:
+            Fx, Fy, Fz = d["fder"].derivs()
+            return d["A_demag"], Fx, Fy, Fz, ComputeDerivatives(Fx, Fy, Fz)
         Fx, Fy, Fz = (DerivView(d["F3"], k) for k in range(3))
         return d["A_demag"], Fx, Fy, Fz, ComputeDerivatives(Fx, Fy, Fz)
 

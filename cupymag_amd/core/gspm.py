@@ -60,8 +60,10 @@ class GSPMSimulation:
         A1, A2, F1 = SimOp.csr_GS()
         A_el, F_el = SimOp.csr_elasticity()
         self.Avg = SimOp.volume_average()
-        F3 = Fx.base
-        self.ctx = F3.ctx
+        from ..physics.fel import FelDerivView
+        fder = Fx.fel if isinstance(Fx, FelDerivView) else None
+        F3 = None if fder is not None else Fx.base
+        self.ctx = Fx.ctx if fder is not None else F3.ctx
         self.n = n = SimOp.n_dof
         self._keep = (A_demag, F3, A1, A2, F1, A_el, F_el)
         self.mg = None
@@ -79,9 +81,11 @@ class GSPMSimulation:
         mg_a1 = self.mg["A1"].h if (self.mg and mg_gs and "A1" in self.mg) else None
         from ..physics.fel import MatrixFreeFel
         mf = F_el if isinstance(F_el, MatrixFreeFel) else None
-        ops = _lib.cpm_sim_ops(A_demag.h, F3.h, A1.h, A2.h, F1.h, A_el.h if A_el is not None else None,
+        ops = _lib.cpm_sim_ops(A_demag.h, F3.h if F3 is not None else None, A1.h, A2.h, F1.h,
+                               A_el.h if A_el is not None else None,
                                F_el.h if (F_el is not None and mf is None) else None, mg_el, mg_dm,
-                               mf.h if mf is not None else None, mg_a1)
+                               mf.h if mf is not None else None, mg_a1, fder.h if fder is not None else None)
+        self._keep = self._keep + (fder,)
         pr = _lib.cpm_sim_params()
         pr.n = n
         pr.me = int(bool(P.ME))

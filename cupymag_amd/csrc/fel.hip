@@ -29,6 +29,7 @@ struct cpm_fel {
   cpm_ctx* ctx = nullptr;
   int32_t n = 0;             // cells per dimension
   double* coords = nullptr;  // (n+1)^3 x 3
+  double* dR = nullptr;      // 9 doubles: the rotation of cpm_fel_strain
   double C[36];
 };
 
@@ -159,6 +160,195 @@ __global__ void __launch_bounds__(kBlock) k_fel_mf(FelArgs fa, KuhnTets kt, cons
   }
 }
 
+// ---- matrix-free weak-form derivatives Fx, Fy, Fz -------------------------
+// F_k[a][b] = sum_q N_q[a] (dN_b/dx_k) detJ w_q (assemble_demag.py:235-305,
+// the host assembler's scalar_row with the same products in the same
+// order): per element the values of the assembled CSR, summed over the 24
+// tetrahedra of a vertex in a fixed order instead of along the CSR row.
+// Geometry of tetrahedron t of the cube at (ci, cj, ck): corner DOFs,
+// reference-to-physical gradients of the 4 shape functions and detJ; false
+// for the zero element (|detJ| < 1e-14).
+__device__ inline bool kuhn_tet_grads(const double* __restrict__ coords, int32_t n, const KuhnTets& kt, int t,
+                                      int32_t ci, int32_t cj, int32_t ck, int64_t (&gdof)[4], double (&g)[4][3],
+                                      double& det) {
+  const int64_t n1 = n + 1;
+  double x[4][3];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int c = kt.v[t][q];
+    const int64_t vi = ci + (c & 1), vj = cj + ((c >> 1) & 1), vk = ck + ((c >> 2) & 1);
+    const int64_t node = (vi * n1 + vj) * n1 + vk;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) x[q][d] = coords[node * 3 + d];
+    gdof[q] = ((vi % n) * n + (vj % n)) * n + (vk % n);
+  }
+  double J[3][3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) J[r][c] = x[r + 1][c] - x[0][c];
+  const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+  const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+  const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+  det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+  if (fabs(det) < 1e-14) return false;
+  const double id = 1.0 / det;
+  const double inv[3][3] = {
+      {c00 * id, (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id, (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id},
+      {c01 * id, (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id, (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id},
+      {c02 * id, (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id, (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id}};
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const double dref[3] = {b == 0 ? -1.0 : (b == 1 ? 1.0 : 0.0), b == 0 ? -1.0 : (b == 2 ? 1.0 : 0.0),
+                            b == 0 ? -1.0 : (b == 3 ? 1.0 : 0.0)};
+#pragma unroll
+    for (int r = 0; r < 3; ++r) g[b][r] = inv[r][0] * dref[0] + inv[r][1] * dref[1] + inv[r][2] * dref[2];
+  }
+  return true;
+}
+
+struct FderIn {
+  const double* u;    // strain: displacement [ux; uy; uz] (3n), or null
+  const double* m;    // demag RHS: m (n, 3), or null
+  const double* avg;  // demag RHS: device average magnetization (3)
+  const double* U;    // Htilde: the demag potential (n), or null
+  const double* x;    // one product F_k x (n), or null
+  int32_t k;          // ... its component
+  int32_t rot;        // strain in the rotated frame (R mixing)
+  const double* R;    // device 3x3 (rot)
+};
+struct FderOut {
+  double* E;   // (n, 6) strain
+  double* bd;  // (n) demag RHS -(Fx m~x + Fy m~y + Fz m~z), entry 0 = 0
+  double* H;   // (3, n) Htilde = -F U
+  double* y;   // (n) F_k x
+};
+
+// One thread per DOF walks the 24 tetrahedra of its vertex and forms the
+// products requested in `in` (each output written once: deterministic).
+__global__ void __launch_bounds__(kBlock) k_fder(int32_t n, const double* __restrict__ coords, KuhnTets kt,
+                                                 FderIn in, FderOut out) {
+  const int64_t N = (int64_t)n * n * n;
+  const double a7 = 0.5854102, b7 = 0.1381966;  // N_q[a] of the 4-point rule (7-digit table)
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  const double a0 = in.m ? in.avg[0] : 0.0, a1 = in.m ? in.avg[1] : 0.0, a2 = in.m ? in.avg[2] : 0.0;
+  for (int64_t dof = (int64_t)blockIdx.x * kBlock + threadIdx.x; dof < N; dof += stride) {
+    const int32_t i = (int32_t)(dof / ((int64_t)n * n)), j = (int32_t)((dof / n) % n), k = (int32_t)(dof % n);
+    double D[3][3] = {};       // D[k][j] = (F_k u_j)_dof
+    double sm[3] = {0, 0, 0};  // (F_k m~_k)_dof
+    double sh[3] = {0, 0, 0};  // (F_k U)_dof
+    double sx = 0.0;           // (F_k x)_dof
+    for (int o = 0; o < 8; ++o) {
+      const int lx = 1 - (o & 1), ly = 1 - ((o >> 1) & 1), lz = 1 - ((o >> 2) & 1);
+      int32_t ci = i - lx, cj = j - ly, ck = k - lz;
+      ci += (ci < 0) ? n : 0;
+      cj += (cj < 0) ? n : 0;
+      ck += (ck < 0) ? n : 0;
+      const int lcode = lx | (ly << 1) | (lz << 2);
+      for (int t = 0; t < 6; ++t) {
+        int a = -1;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (kt.v[t][q] == lcode) a = q;
+        if (a < 0) continue;
+        int64_t gd[4];
+        double g[4][3], det;
+        if (!kuhn_tet_grads(coords, n, kt, t, ci, cj, ck, gd, g, det)) continue;
+        const double dw = det * (1.0 / 24.0);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          double f[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const double nq = (q == a) ? a7 : b7;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) f[c] += nq * g[b][c] * dw;
+          }
+          const int64_t cb = gd[b];
+          if (in.u) {
+            const double uj[3] = {in.u[cb], in.u[N + cb], in.u[2 * N + cb]};
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+              for (int jj = 0; jj < 3; ++jj) D[c][jj] += f[c] * uj[jj];
+          }
+          if (in.m) {
+            sm[0] += f[0] * (in.m[3 * cb] - a0);
+            sm[1] += f[1] * (in.m[3 * cb + 1] - a1);
+            sm[2] += f[2] * (in.m[3 * cb + 2] - a2);
+          }
+          if (in.U) {
+            const double uc = in.U[cb];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) sh[c] += f[c] * uc;
+          }
+          if (in.x) sx += f[in.k] * in.x[cb];
+        }
+      }
+    }
+    if (in.u) {
+      double e[6];
+      if (!in.rot) {
+        e[0] = D[0][0];
+        e[1] = D[1][1];
+        e[2] = D[2][2];
+        e[3] = D[1][0] + D[0][1];
+        e[4] = D[2][1] + D[1][2];
+        e[5] = D[2][0] + D[0][2];
+      } else {
+        // compute_derivatives.py:94-123; D[k][j]: derivative k of component j
+        const double* R = in.R;
+        auto mix = [&](int col, int jj) { return R[col] * D[0][jj] + R[3 + col] * D[1][jj] + R[6 + col] * D[2][jj]; };
+        e[0] = mix(0, 0);
+        e[1] = mix(1, 1);
+        e[2] = mix(2, 2);
+        e[3] = mix(1, 0) + mix(0, 1);
+        e[4] = mix(2, 1) + mix(1, 2);
+        e[5] = mix(2, 0) + mix(0, 2);
+      }
+#pragma unroll
+      for (int v = 0; v < 6; ++v) out.E[6 * dof + v] = e[v];
+    }
+    if (in.m) out.bd[dof] = (dof == 0) ? 0.0 : -((sm[0] + sm[1]) + sm[2]);
+    if (in.U) {
+      out.H[dof] = -sh[0];
+      out.H[N + dof] = -sh[1];
+      out.H[2 * N + dof] = -sh[2];
+    }
+    if (in.x) out.y[dof] = sx;
+  }
+}
+
+int fder_apply(const cpm_fel* F, const FderIn& in, const FderOut& out) {
+  static const KuhnTets kt = kuhn_tets();
+  const int64_t N = (int64_t)F->n * F->n * F->n;
+  k_fder<<<grid_for(N), kBlock, 0, F->ctx->stream>>>(F->n, F->coords, kt, in, out);
+  CPM_HIP(hipGetLastError());
+  return CPM_OK;
+}
+
+int fder_strain_demag(const cpm_fel* F, const double* u, int rot, const double* dR, const double* m,
+                      const double* davg, double* E, double* bd) {
+  FderIn in{};
+  in.u = u;
+  in.rot = rot;
+  in.R = dR;
+  in.m = m;
+  in.avg = davg;
+  FderOut out{};
+  out.E = E;
+  out.bd = bd;
+  return fder_apply(F, in, out);
+}
+
+int fder_hd(const cpm_fel* F, const double* U, double* H) {
+  FderIn in{};
+  in.U = U;
+  FderOut out{};
+  out.H = H;
+  return fder_apply(F, in, out);
+}
+
 int fel_apply(const cpm_fel* F, const double* E0, double* b) {
   FelArgs fa;
   fa.n = F->n;
@@ -178,18 +368,23 @@ int64_t fel_rows(const cpm_fel* F) { return 3 * (int64_t)F->n * F->n * F->n; }
 using namespace cpm;
 
 extern "C" int cpm_fel_create(cpm_ctx* ctx, int32_t n, const double* coords, const double* C, cpm_fel** out) {
-  if (!ctx || !coords || !C || !out) return fail(CPM_EINVAL, "cpm_fel_create: null argument");
+  if (!ctx || !coords || !out) return fail(CPM_EINVAL, "cpm_fel_create: null argument");
   if (n < 2) return fail(CPM_EINVAL, "cpm_fel_create: at least 2 cells per dimension");
   CPM_HIP(hipSetDevice(ctx->device));
   auto* F = new cpm_fel();
   F->ctx = ctx;
   F->n = n;
-  std::copy(C, C + 36, F->C);
+  if (C)
+    std::copy(C, C + 36, F->C);
+  else
+    std::fill(F->C, F->C + 36, 0.0);
   const size_t bytes = sizeof(double) * 3 * (size_t)(n + 1) * (n + 1) * (n + 1);
   hipError_t e = hipMalloc(&F->coords, bytes);
+  if (e == hipSuccess) e = hipMalloc(&F->dR, sizeof(double) * 9);
   if (e == hipSuccess) e = hipMemcpy(F->coords, coords, bytes, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     if (F->coords) (void)hipFree(F->coords);
+    if (F->dR) (void)hipFree(F->dR);
     delete F;
     return fail(e == hipErrorOutOfMemory ? CPM_ENOMEM : CPM_EHIP, hipGetErrorString(e));
   }
@@ -202,6 +397,7 @@ extern "C" int cpm_fel_destroy(cpm_fel* F) {
   (void)hipSetDevice(F->ctx->device);
   (void)hipStreamSynchronize(F->ctx->stream);
   if (F->coords) (void)hipFree(F->coords);
+  if (F->dR) (void)hipFree(F->dR);
   delete F;
   return CPM_OK;
 }
@@ -210,4 +406,31 @@ extern "C" int cpm_fel_apply(cpm_fel* F, const double* E0, double* b) {
   if (!F || !E0 || !b) return fail(CPM_EINVAL, "cpm_fel_apply: null argument");
   CPM_HIP(hipSetDevice(F->ctx->device));
   return fel_apply(F, E0, b);
+}
+
+extern "C" int cpm_fel_strain(cpm_fel* F, const double* u, const double* R, double* E) {
+  if (!F || !u || !E) return fail(CPM_EINVAL, "cpm_fel_strain: null argument");
+  CPM_HIP(hipSetDevice(F->ctx->device));
+  if (R) CPM_HIP(hipMemcpy(F->dR, R, 9 * sizeof(double), hipMemcpyHostToDevice));
+  if (int rc = fder_strain_demag(F, u, R ? 1 : 0, F->dR, nullptr, nullptr, E, nullptr)) return rc;
+  CPM_HIP(hipStreamSynchronize(F->ctx->stream));
+  return CPM_OK;
+}
+
+extern "C" int cpm_fel_hd(cpm_fel* F, const double* U, double* H) {
+  if (!F || !U || !H) return fail(CPM_EINVAL, "cpm_fel_hd: null argument");
+  CPM_HIP(hipSetDevice(F->ctx->device));
+  return fder_hd(F, U, H);
+}
+
+extern "C" int cpm_fel_dmul(cpm_fel* F, int k, const double* x, double* y) {
+  if (!F || !x || !y) return fail(CPM_EINVAL, "cpm_fel_dmul: null argument");
+  if (k < 0 || k > 2) return fail(CPM_EINVAL, "cpm_fel_dmul: k must be 0, 1 or 2 (Fx, Fy, Fz)");
+  CPM_HIP(hipSetDevice(F->ctx->device));
+  FderIn in{};
+  in.x = x;
+  in.k = k;
+  FderOut out{};
+  out.y = y;
+  return fder_apply(F, in, out);
 }

@@ -34,6 +34,9 @@ struct cpm_fel;
 namespace cpm {
 int fel_apply(const cpm_fel* F, const double* E0, double* b);
 int64_t fel_rows(const cpm_fel* F);
+int fder_strain_demag(const cpm_fel* F, const double* u, int rot, const double* dR, const double* m,
+                      const double* davg, double* E, double* bd);
+int fder_hd(const cpm_fel* F, const double* U, double* H);
 }  // namespace cpm
 
 namespace cpm {
@@ -557,8 +560,12 @@ extern "C" int cpm_sim_create(cpm_ctx* ctx, const cpm_sim_ops* ops, const cpm_si
   auto sq = [&](const cpm_mat* A, int64_t rows) { return A && A->nrows == rows && A->ncols == rows; };
   if (!sq(ops->A_demag, n) || !sq(ops->A1, n) || !sq(ops->A2, n) || !sq(ops->F1, n))
     return fail(CPM_EINVAL, "cpm_sim_create: scalar operators must be n x n");
-  if (int rc = check_f3(ops->F3)) return rc;
-  if (ops->F3->nrows != n) return fail(CPM_EINVAL, "cpm_sim_create: F3 must be n x n");
+  if (ops->fder) {
+    if (fel_rows(ops->fder) != 3 * n) return fail(CPM_EINVAL, "cpm_sim_create: matrix-free Fx, Fy, Fz of another mesh");
+  } else {
+    if (int rc = check_f3(ops->F3)) return rc;
+    if (ops->F3->nrows != n) return fail(CPM_EINVAL, "cpm_sim_create: F3 must be n x n");
+  }
   if (prm->me) {
     if (!sq(ops->A_el, 3 * n)) return fail(CPM_EINVAL, "cpm_sim_create: A_el must be 3n x 3n");
     if (ops->fel) {
@@ -739,19 +746,31 @@ static int stage_pre(cpm_sim* S, int stage, const FieldPrm& P, SolveSpec& sp) {
       sp = SolveSpec{0, 1, S->ops.A_el, S->bel, S->u, fast ? S->bint : nullptr, fast ? S->uint : nullptr};
       break;
     case 1:  // :127-169
-      if (p.me) {
+      if (S->ops.fder) {
+        // strain (when coupled) and the demag right-hand side in one walk
+        if ((rc = fder_strain_demag(S->ops.fder, p.me ? S->u : nullptr, p.rot111, dR, S->m, S->dscal + S_AVGM,
+                                    S->E, S->bd)))
+          return rc;
+      } else if (p.me) {
         CPM_G_DISPATCH(F3->lanes, k_strain, n, s, (int32_t)n, F3->ptr, F3->idx, F3->val, F3->nnz, S->u, p.rot111,
                        dR, S->E);
+      }
+      if (p.me) {
         k_strain_avg<<<gv, kBlock, 0, s>>>(n, S->W, S->vol, S->E, S->m, p.rot111, dR, p.lambda100, p.lambda111,
                                            dEs, dMv, S->dscal, ctx->partials, ctx->counter);
         k_efinal<<<gv, kBlock, 0, s>>>(n, S->E, p.rot111, dMvinvT, S->dscal);
       }
-      CPM_G_DISPATCH(F3->lanes, k_demag_rhs, n, s, (int32_t)n, F3->ptr, F3->idx, F3->val, F3->nnz, S->m,
-                     S->dscal + S_AVGM, S->bd);
+      if (!S->ops.fder)
+        CPM_G_DISPATCH(F3->lanes, k_demag_rhs, n, s, (int32_t)n, F3->ptr, F3->idx, F3->val, F3->nnz, S->m,
+                       S->dscal + S_AVGM, S->bd);
       sp = SolveSpec{1, 1, S->ops.A_demag, S->bd, S->U};
       break;
     case 2:  // :171-223
-      CPM_G_DISPATCH(F3->lanes, k_hd, n, s, (int32_t)n, F3->ptr, F3->idx, F3->val, F3->nnz, S->U, S->Ht);
+      if (S->ops.fder) {
+        if ((rc = fder_hd(S->ops.fder, S->U, S->Ht))) return rc;
+      } else {
+        CPM_G_DISPATCH(F3->lanes, k_hd, n, s, (int32_t)n, F3->ptr, F3->idx, F3->val, F3->nnz, S->U, S->Ht);
+      }
       k_field1<<<gv, kBlock, 0, s>>>(n, P, S->m, S->E, p.me, S->Ht, S->dscal + S_AVGM, S->f, S->ftmp);
       for (int c = 0; c < 3; ++c)
         if ((rc = launch_spmv(ctx, S->ops.F1, 0, S->ftmp + c * n, S->b + c * n))) return rc;

@@ -3,7 +3,9 @@
 Fx, Fy, Fz share one sparsity pattern; when they are the three value sets
 of one device operator, ``compute_Hd_from_U`` (3 SpMVs) and
 ``compute_E_from_u`` (6 or 9 SpMVs plus the rotation mixing) each run as a
-single fused pass over the pattern (cpm_hd_from_u / cpm_strain_from_u).
+single fused pass over the pattern (cpm_hd_from_u / cpm_strain_from_u);
+matrix-free Fx, Fy, Fz (physics.fel.FelDerivView) walk the elements instead
+(cpm_fel_hd / cpm_fel_strain).
 """
 import numpy as np
 import torch
@@ -33,12 +35,19 @@ class DerivView:
 
 class ComputeDerivatives:
     def __init__(self, Fx, Fy, Fz):
+        from ..physics.fel import FelDerivView
         self.Dx, self.Dy, self.Dz = Fx, Fy, Fz
         fused = all(isinstance(F, DerivView) for F in (Fx, Fy, Fz)) and \
             Fx.base is Fy.base is Fz.base and (Fx.valset, Fy.valset, Fz.valset) == (0, 1, 2)
         self.F3 = Fx.base if fused else None
+        # matrix-free Fx, Fy, Fz of one mesh (cpm_fel_strain / cpm_fel_hd)
+        mf = all(isinstance(F, FelDerivView) for F in (Fx, Fy, Fz)) and Fx.fel is Fy.fel is Fz.fel and \
+            (Fx.k, Fy.k, Fz.k) == (0, 1, 2)
+        self.fel = Fx.fel if mf else None
 
     def compute_Hd_from_U(self, U):
+        if self.fel is not None:
+            return self.fel.hd(U)
         if self.F3 is None:
             return -(self.Dx @ U), -(self.Dy @ U), -(self.Dz @ U)
         ctx = self.F3.ctx
@@ -49,6 +58,10 @@ class ComputeDerivatives:
         return H[:n], H[n:2 * n], H[2 * n:]
 
     def compute_E_from_u(self, n, u, R=None):
+        if self.fel is not None:
+            if self.fel.n ** 3 != n:
+                raise ValueError("u must have 3*n entries")
+            return self.fel.strain(u, R)
         if self.F3 is None:
             raise NotImplementedError("compute_E_from_u needs Fx, Fy, Fz on one device pattern")
         ctx = self.F3.ctx

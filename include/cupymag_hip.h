@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define CPM_ABI_VERSION 1
+#define CPM_ABI_VERSION 2
 
 #define CPM_OK 0
 #define CPM_EINVAL (-1)
@@ -312,6 +312,18 @@ int cpm_fel_create(cpm_ctx* ctx, int32_t n, const double* coords, const double* 
 int cpm_fel_destroy(cpm_fel* fel);
 int cpm_fel_apply(cpm_fel* fel, const double* E0, double* b);
 
+/* The weak-form derivative operators Fx, Fy, Fz (assemble_demag.py:235-305)
+ * of the same mesh without their CSR form (0.9 GB at 128^3), element values
+ * formed as the host assembler forms them (C of cpm_fel_create may be NULL
+ * when only these are used).  Replace, in that order,
+ * ComputeDerivatives.compute_E_from_u (compute_derivatives.py:60-125: u 3n
+ * [ux;uy;uz], R host 3x3 or NULL, E (n,6) [E11,E22,E33,E12,E23,E13]),
+ * compute_Hd_from_U (:36-58: H (3,n) = -F U) and a product Fk @ x (k = 0,1,2
+ * for Fx, Fy, Fz).  Results equal the CSR products up to summation order. */
+int cpm_fel_strain(cpm_fel* fel, const double* u, const double* R, double* E);
+int cpm_fel_hd(cpm_fel* fel, const double* U, double* H);
+int cpm_fel_dmul(cpm_fel* fel, int k, const double* x, double* y);
+
 /* ------------------------------------------------------------------ */
 /* device-resident GSPM LLG simulation (the hot loop)                  */
 /* ------------------------------------------------------------------ */
@@ -329,6 +341,8 @@ typedef struct cpm_sim_ops {
   cpm_mg* mg_demag;       /* multigrid hierarchy of A_demag (used when params.precond) or NULL */
   cpm_fel* fel;           /* matrix-free F_el (cpm_fel_create); used instead of F_el when not NULL */
   cpm_mg* mg_a1;          /* multigrid hierarchy of A1 (used when params.precond has CPM_PRECOND_GS) or NULL */
+  cpm_fel* fder;          /* matrix-free Fx, Fy, Fz (cpm_fel of this mesh): strain, Htilde and the demag right-hand
+                             side without F3 (which may then be NULL), or NULL */
 } cpm_sim_ops;
 
 #define CPM_PRECOND_EL_DEMAG 1 /* elasticity and demag solves: multigrid-preconditioned CG */
