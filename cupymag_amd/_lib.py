@@ -49,7 +49,16 @@ class cpm_sim_params(C.Structure):
 
 class cpm_step_report(C.Structure):
     _fields_ = [("sumsq", C.c_double), ("avg_m", C.c_double * 3), ("iters", C.c_int32 * 10),
-                ("failed", C.c_int32), ("resid", C.c_double)]
+                ("failed", C.c_int32), ("resid", C.c_double), ("point_done", C.c_int32), ("state", C.c_int32),
+                ("point", C.c_int32), ("nstep", C.c_int32), ("use_init", C.c_int32), ("pad", C.c_int32),
+                ("H", C.c_double * 3), ("H_next", C.c_double * 3)]
+
+
+class cpm_sweep_cfg(C.Structure):
+    _fields_ = [("H0", C.c_double * 3), ("dH", C.c_double * 3), ("llg_accuracy", C.c_double),
+                ("initial_factor", C.c_double), ("stop_dir", C.c_double * 3), ("stop_value", C.c_double),
+                ("llg_min_step", C.c_int32), ("steps_per_point", C.c_int32), ("npoints", C.c_int32),
+                ("pad", C.c_int32)]
 
 
 class cpm_schedule(C.Structure):
@@ -96,6 +105,7 @@ _SIGS = {
     "cpm_sim_set_m": (C.c_int, [_P, _P, _P]),
     "cpm_sim_get": (C.c_int, [_P, C.c_int, _P]),
     "cpm_sim_step": (C.c_int, [_P, _P, C.c_int, C.POINTER(cpm_step_report)]),
+    "cpm_sim_control": (C.c_int, [_P, C.POINTER(cpm_sweep_cfg), _P]),
     "cpm_mat_autotune": (C.c_int, [_P, _P, C.c_int, C.POINTER(C.c_int)]),
     "cpm_mat_set_schedule": (C.c_int, [_P, C.POINTER(cpm_schedule)]),
     "cpm_mat_get_schedule": (C.c_int, [_P, C.POINTER(cpm_schedule)]),

@@ -5,7 +5,10 @@ then step the GSPM integrator until the average magnetization along
 ``stop_direction`` drops below ``stop_value``; on LLG convergence
 (``sumsq < LLG_accuracy`` after ``llg_min_step`` steps) write a field point
 to hysteresis.txt, a VTU snapshot and last_m, and step the external field
-by -dHext.  Each LLG step is one call into the device engine.
+by -dHext.  Each LLG step is one call into the device engine; the loop's
+control (use_init, LLG convergence, the field step, the stop condition) is
+evaluated on the device after each step (GSPMSimulation.set_control), and
+the host reads its outcome from the step report.
 """
 import os
 import time
@@ -71,13 +74,14 @@ def main(config_path=None, max_steps=None, mesh_arrays=None, verbose=True, engin
     nDOF = SimOp.n_dof
     m0 = initialize_m(P.restart_m, P.restart, P.initial_magnetization, nDOF, SimOp.DefDOF)
     avg_m = sim.set_m(m0)
-    use_init = False
     nstep = count = 0
     sumsq = np.inf
-    llg_min_step = 10 if P.ME else 100
-    LLG_accuracy = nDOF * P.LLG_accuracy_factor
     H = [P.Hext1, P.Hext2, P.Hext3]
     dH = [P.dHext1, P.dHext2, P.dHext3]
+    # the loop's control (use_init, LLG convergence, the field step, the stop
+    # condition) runs on the device after every step; the host reads its
+    # outcome from the step report
+    sim.set_control(H0=H, dH=dH)
     hyst = open(os.path.join(out_dir, "hysteresis.txt"), "w", buffering=1)
     hyst.write("Hext1\tHext2\tHext3\tavg_m1\tavg_m2\tavg_m3\n")
     t1 = time.time()
@@ -88,25 +92,25 @@ def main(config_path=None, max_steps=None, mesh_arrays=None, verbose=True, engin
             break
         nstep += 1
         total += 1
-        rep = sim.step(H, use_init)
+        rep = sim.step()
         sumsq = rep.sumsq
-        use_init = sumsq < P.use_init_factor * LLG_accuracy
         if nstep % 200 == 0:
             say(f"m diff L2 norm sq: {sumsq:.10f} at LLG step {nstep}")
         avg_m = rep.avg_m
-        if (sumsq < LLG_accuracy and nstep > llg_min_step) or nstep % P.save_frequency == 0:
+        Hs = rep.H  # the field of this step
+        if rep.point_done or nstep % P.save_frequency == 0:
             count += 1
-            Avg.write_to_paraview(_fields(sim, P.ME), os.path.join(out_dir, f"field_{H[0] * P.ms:.0f}_{count}.vtu"),
+            Avg.write_to_paraview(_fields(sim, P.ME), os.path.join(out_dir, f"field_{Hs[0] * P.ms:.0f}_{count}.vtu"),
                                   alpha=P.vtu_alpha)
-            hyst.write(f"{H[0] * P.ms:.1f}\t{H[1] * P.ms:.1f}\t{H[2] * P.ms:.1f}\t{avg_m[0]}\t{avg_m[1]}\t{avg_m[2]}\n")
+            hyst.write(f"{Hs[0] * P.ms:.1f}\t{Hs[1] * P.ms:.1f}\t{Hs[2] * P.ms:.1f}\t{avg_m[0]}\t{avg_m[1]}\t{avg_m[2]}\n")
             if P.write_m:
                 write_array(sim.get("M"), os.path.join(out_dir, "last_m.h5"))
             if nstep >= 5000:
-                say(f"LLG not converging at external field {H[0] * P.ms:.0f} A/m, at LLG step {nstep}.")
-        if sumsq < LLG_accuracy and nstep > llg_min_step:
+                say(f"LLG not converging at external field {Hs[0] * P.ms:.0f} A/m, at LLG step {nstep}.")
+        H = list(rep.H_next)
+        if rep.point_done:
             say(f"LLG converge at LLG step {nstep}.")
             say(f"Current average magnetization is m1 = {avg_m[0]}, m2 = {avg_m[1]}, m3 = {avg_m[2]}.")
-            H = [H[k] - dH[k] for k in range(3)]
             nstep = count = 0
             if avg_m @ P.stop_direction > P.stop_value:
                 say(f"Now calculating LLG with Hext1 = {H[0] * P.ms:.0f} A/m.")

@@ -46,17 +46,20 @@ class GSPMBatch:
     def __len__(self):
         return len(self.members)
 
-    def step(self, H, use_init, raise_on_fail=True):
+    def step(self, H=None, use_init=None, raise_on_fail=True):
         """One step of every member.  H: (len, 3) reduced external fields,
-        use_init: (len,) warm-start flags, both in member order.  Returns
+        use_init: (len,) warm-start flags, both in member order (ignored for
+        members with a control attached, GSPMSimulation.set_control; None:
+        zeros, when every member has one).  Returns
         one StepReport per member; a member whose solve did not converge
         raises the reference's error (raise_on_fail) or is reported with
         ``failed`` set."""
         nb = len(self.members)
         if nb == 0:
             return []
-        Hh = np.ascontiguousarray(np.asarray(H, dtype=np.float64).reshape(nb, 3))
-        ui = np.ascontiguousarray(np.asarray(use_init, dtype=np.int32).reshape(nb))
+        Hh = np.ascontiguousarray(np.zeros((nb, 3)) if H is None else np.asarray(H, dtype=np.float64).reshape(nb, 3))
+        ui = np.ascontiguousarray(np.zeros(nb, dtype=np.int32) if use_init is None else
+                                  np.asarray(use_init, dtype=np.int32).reshape(nb))
         reps = (_lib.cpm_step_report * nb)()
         check(lib().cpm_batch_step(self.h, ptr(Hh), ptr(ui), reps), "cpm_batch_step")
         out = [StepReport(r) for r in reps]

@@ -28,6 +28,14 @@ class StepReport:
         self.iters = list(rep.iters)
         self.failed = rep.failed
         self.resid = rep.resid
+        # hysteresis control (set_control): outcome of the step
+        self.point_done = bool(rep.point_done)
+        self.state = rep.state          # 0 running, 1 stop condition met, 2 field list done
+        self.point = rep.point
+        self.nstep = rep.nstep
+        self.use_init = bool(rep.use_init)
+        self.H = np.array(rep.H[:], dtype=np.float64)
+        self.H_next = np.array(rep.H_next[:], dtype=np.float64)
 
 
 class GSPMSimulation:
@@ -139,7 +147,47 @@ class GSPMSimulation:
         self.avg_m = out
         return out
 
-    def step(self, H, use_init):
+    def set_control(self, H0=(0.0, 0.0, 0.0), dH=(0.0, 0.0, 0.0), fields=None, steps_per_point=0,
+                    llg_accuracy=None, initial_factor=None, stop_dir=None, stop_value=None, llg_min_step=None):
+        """Keep the reference's hysteresis control on the device
+        (Micromagnetics.py:109,299-335, cpm_sim_control): from now on the
+        steps take their field and use_init from it, and each StepReport says
+        whether the step completed a field point (LLG convergence, or
+        ``steps_per_point``), the point index and the state (1: the stop
+        condition avg_m . stop_dir <= stop_value holds, 2: the ``fields``
+        list is done).  Defaults from the parameters: H0 and dH the config's
+        Hext and dHext (reduced), llg_accuracy = nDOF * llg_accuracy_factor,
+        the config's initial_factor, stop condition and llg_min_step."""
+        P = self.P
+        cfg = _lib.cpm_sweep_cfg()
+        cfg.llg_accuracy = float(self.n * P.LLG_accuracy_factor if llg_accuracy is None else llg_accuracy)
+        cfg.initial_factor = float(P.use_init_factor if initial_factor is None else initial_factor)
+        sd = np.asarray(P.stop_direction if stop_dir is None else stop_dir, dtype=np.float64)
+        for k in range(3):
+            cfg.H0[k] = float(H0[k])
+            cfg.dH[k] = float(dH[k])
+            cfg.stop_dir[k] = float(sd[k])
+        cfg.stop_value = float(P.stop_value if stop_value is None else stop_value)
+        cfg.llg_min_step = int((10 if P.ME else 100) if llg_min_step is None else llg_min_step)
+        cfg.steps_per_point = int(steps_per_point)
+        fl = None
+        if fields is not None:
+            fl = np.ascontiguousarray(np.asarray(fields, dtype=np.float64).reshape(-1, 3))
+            cfg.npoints = fl.shape[0]
+        check(lib().cpm_sim_control(self.h, C.byref(cfg), ptr(fl)), "cpm_sim_control")
+        self.controlled = True
+
+    def clear_control(self):
+        check(lib().cpm_sim_control(self.h, None, None), "cpm_sim_control")
+        self.controlled = False
+
+    def step(self, H=None, use_init=False):
+        """One LLG step at field H (reduced) with the reference's use_init;
+        under a control (set_control) both come from the device state."""
+        if H is None:
+            if not getattr(self, "controlled", False):
+                raise ValueError("step needs the field H unless a control is attached (set_control)")
+            H = (0.0, 0.0, 0.0)
         Hh = np.ascontiguousarray(H, dtype=np.float64)
         rep = _lib.cpm_step_report()
         info = check(lib().cpm_sim_step(self.h, ptr(Hh), int(bool(use_init)), C.byref(rep)), "cpm_sim_step")

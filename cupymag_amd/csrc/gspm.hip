@@ -29,6 +29,7 @@
 #include "mg.h"
 #include "multi.h"
 #include "multi_mg.h"
+#include "sweep_ctl.h"
 
 struct cpm_fel;
 namespace cpm {
@@ -46,10 +47,10 @@ struct FieldPrm {
   double Nd[3], H[3];
   double R[9];
   int rot;
+  const double* Hdev;  // the external field in device memory (sweep control), else H
 };
 
-// Device scalar slots of a simulation (double offsets in sim->dscal).
-enum : int { S_AVGM = 0, S_AVGM100 = 3, S_AVGE = 6, S_EBAR = 12, S_SUMSQ = 18, S_NSCAL = 32 };
+
 
 __device__ inline void rot_apply(const double* R, double a0, double a1, double a2, double& b0, double& b1,
                                  double& b2) {
@@ -310,10 +311,13 @@ __device__ inline void hbar_of(const FieldPrm& P, const double* avg, double Hb[3
 }
 
 // stage 1: f (3,n) and ftmp = f*dt + m (3,n)
-__global__ void __launch_bounds__(kBlock) k_field1(int64_t n, FieldPrm P, const double* __restrict__ m,
+__global__ void __launch_bounds__(kBlock) k_field1(int64_t n, FieldPrm P0, const double* __restrict__ m,
                                                    const double* __restrict__ E, int me, const double* __restrict__ Ht,
                                                    const double* __restrict__ avg, double* __restrict__ f,
                                                    double* __restrict__ ftmp) {
+  FieldPrm P = P0;
+  if (P0.Hdev)
+    for (int k = 0; k < 3; ++k) P.H[k] = P0.Hdev[k];
   double Hb[3];
   hbar_of(P, avg, Hb);
   const int64_t stride = (int64_t)gridDim.x * kBlock;
@@ -360,11 +364,14 @@ __global__ void __launch_bounds__(kBlock) k_gs2(int64_t n, double dt, const doub
 }
 
 // m3* = m3 + g1* m2* - g2* m1* ; stage-2 RHS f(m*)*dt2 + m* (3,n)
-__global__ void __launch_bounds__(kBlock) k_gs3_field2(int64_t n, FieldPrm P, const double* __restrict__ m,
+__global__ void __launch_bounds__(kBlock) k_gs3_field2(int64_t n, FieldPrm P0, const double* __restrict__ m,
                                                        const double* __restrict__ gs, double* __restrict__ mstar,
                                                        const double* __restrict__ E, int me,
                                                        const double* __restrict__ Ht, const double* __restrict__ avg,
                                                        double* __restrict__ ftmp) {
+  FieldPrm P = P0;
+  if (P0.Hdev)
+    for (int k = 0; k < 3; ++k) P.H[k] = P0.Hdev[k];
   double Hb[3];
   hbar_of(P, avg, Hb);
   const int64_t stride = (int64_t)gridDim.x * kBlock;
@@ -500,6 +507,10 @@ struct cpm_sim {
   cpm_batch* batch = nullptr;
   int hint[10] = {};
   bool stepped = false;  // the warm-start buffers hold a previous solution
+  // hysteresis control (cpm_sim_control): on, host copy of the next step's use_init, field list
+  int ctl_on = 0;
+  int ctl_use_init = 0;
+  double* ctl_fields = nullptr;
 };
 
 static int sim_alloc(cpm_sim* S, double** p, int64_t count) {
@@ -657,6 +668,50 @@ extern "C" int cpm_sim_set_m(cpm_sim* S, const double* m, double* avg_m) {
   return CPM_OK;
 }
 
+extern "C" int cpm_sim_control(cpm_sim* S, const cpm_sweep_cfg* cfg, const double* fields) {
+  if (!S) return fail(CPM_EINVAL, "cpm_sim_control: null simulation");
+  cpm_ctx* ctx = S->ctx;
+  CPM_HIP(hipSetDevice(ctx->device));
+  CPM_HIP(hipStreamSynchronize(ctx->stream));
+  double c[S_NSCAL - C_ON];
+  std::memset(c, 0, sizeof(c));
+  if (!cfg) {  // detach: the caller's H and use_init drive the steps again
+    CPM_HIP(hipMemcpy(S->dscal + C_ON, c, sizeof(c), hipMemcpyHostToDevice));
+    S->ctl_on = 0;
+    S->ctl_use_init = 0;
+    return CPM_OK;
+  }
+  if (cfg->npoints < 0 || (cfg->npoints > 0 && !fields))
+    return fail(CPM_EINVAL, "cpm_sim_control: npoints must be >= 0, with `fields` (npoints x 3) when > 0");
+  if (cfg->npoints > 0) {
+    if (S->ctl_fields) {
+      (void)hipFree(S->ctl_fields);
+      S->allocs.erase(std::find(S->allocs.begin(), S->allocs.end(), (void*)S->ctl_fields));
+      S->ctl_fields = nullptr;
+    }
+    if (int rc = sim_alloc(S, &S->ctl_fields, 3 * (int64_t)cfg->npoints)) return rc;
+    CPM_HIP(hipMemcpy(S->ctl_fields, fields, sizeof(double) * 3 * cfg->npoints, hipMemcpyHostToDevice));
+  }
+  auto at = [&](int slot) -> double& { return c[slot - C_ON]; };
+  at(C_ON) = 1.0;
+  for (int k = 0; k < 3; ++k) {
+    at(C_H + k) = cfg->npoints > 0 ? fields[k] : cfg->H0[k];
+    at(C_DH + k) = cfg->dH[k];
+    at(C_STOPDIR + k) = cfg->stop_dir[k];
+    at(C_PH + k) = at(C_H + k);
+  }
+  at(C_ACC) = cfg->llg_accuracy;
+  at(C_INITF) = cfg->initial_factor;
+  at(C_STOPVAL) = cfg->stop_value;
+  at(C_MINSTEP) = cfg->llg_min_step;
+  at(C_SPP) = cfg->steps_per_point;
+  at(C_NPTS) = cfg->npoints;
+  CPM_HIP(hipMemcpy(S->dscal + C_ON, c, sizeof(c), hipMemcpyHostToDevice));
+  S->ctl_on = 1;
+  S->ctl_use_init = 0;
+  return CPM_OK;
+}
+
 extern "C" int cpm_sim_get(cpm_sim* S, int field, double* dst) {
   if (!S || !dst) return fail(CPM_EINVAL, "cpm_sim_get: null argument");
   const double* src = nullptr;
@@ -698,6 +753,11 @@ struct SolveSpec {
   double* xint = nullptr;
 };
 
+// Hysteresis control after a step (one thread, sweep_ctl.h).
+__global__ void k_sweep_ctl(double* __restrict__ c, const double* __restrict__ fields) {
+  sweep_ctl_update(c, fields);
+}
+
 static FieldPrm field_prm(const cpm_sim* S, const double* H) {
   const cpm_sim_params& p = S->prm;
   FieldPrm P{};
@@ -710,10 +770,11 @@ static FieldPrm field_prm(const cpm_sim* S, const double* H) {
   P.dt2 = p.dt * p.alpha_damp;
   for (int k = 0; k < 3; ++k) {
     P.Nd[k] = p.Nd[k];
-    P.H[k] = H[k];
+    P.H[k] = H ? H[k] : 0.0;
   }
   std::memcpy(P.R, p.R, sizeof(P.R));
   P.rot = p.rot111;
+  P.Hdev = S->ctl_on ? S->dscal + C_H : nullptr;
   return P;
 }
 
@@ -814,6 +875,7 @@ static int stage_final(cpm_sim* S, double* dst) {
   cpm_ctx* ctx = S->ctx;
   k_project<<<grid_for(S->n), kBlock, 0, ctx->stream>>>(S->n, S->mss, S->defect, S->W, S->vol, S->m, S->mprev,
                                                         S->dscal, ctx->partials, ctx->counter);
+  if (S->ctl_on) k_sweep_ctl<<<1, 1, 0, ctx->stream>>>(S->dscal, S->ctl_fields);
   CPM_HIP(hipGetLastError());
   CPM_HIP(hipMemcpyAsync(dst, S->dscal, S_NSCAL * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   return CPM_OK;
@@ -823,7 +885,22 @@ static void report_scalars(cpm_sim* S, const double* scal, cpm_step_report* rep)
   S->stepped = true;
   rep->sumsq = scal[S_SUMSQ];
   for (int k = 0; k < 3; ++k) rep->avg_m[k] = scal[S_AVGM + k];
+  if (S->ctl_on) {
+    S->ctl_use_init = scal[C_USEINIT] != 0.0 ? 1 : 0;
+    rep->use_init = S->ctl_use_init;
+    rep->point_done = scal[C_PDONE] != 0.0 ? 1 : 0;
+    rep->state = (int32_t)scal[C_STATE];
+    rep->point = (int32_t)scal[C_POINT];
+    rep->nstep = (int32_t)scal[C_NSTEP];
+    for (int k = 0; k < 3; ++k) {
+      rep->H[k] = scal[C_PH + k];
+      rep->H_next[k] = scal[C_H + k];
+    }
+  }
 }
+
+// the use_init flag of a step: the control's when one is attached
+static inline int step_use_init(const cpm_sim* S, int given) { return S->ctl_on ? S->ctl_use_init : given; }
 
 // reliable-update factor of a solve: slots 0 and 1 are the elasticity and
 // demag solves, slots 2..9 the Gauss-Seidel solves
@@ -908,11 +985,12 @@ static int solve_one(cpm_sim* S, const SolveSpec& sp, int use_init, cpm_step_rep
 }  // namespace cpm
 
 extern "C" int cpm_sim_step(cpm_sim* S, const double* H, int use_init, cpm_step_report* rep) {
-  if (!S || !H || !rep) return fail(CPM_EINVAL, "cpm_sim_step: null argument");
+  if (!S || !rep || (!H && !S->ctl_on)) return fail(CPM_EINVAL, "cpm_sim_step: null argument");
   if (S->batch) return fail(CPM_EINVAL, "cpm_sim_step: the simulation belongs to a batch (cpm_batch_step)");
   cpm_ctx* ctx = S->ctx;
   std::memset(rep, 0, sizeof(*rep));
-  const FieldPrm P = field_prm(S, H);
+  use_init = step_use_init(S, use_init);
+  const FieldPrm P = field_prm(S, S->ctl_on ? nullptr : H);
   for (int stage = 0; stage < 6; ++stage) {
     SolveSpec sp;
     if (int rc = stage_pre(S, stage, P, sp)) return rc;
@@ -1033,17 +1111,21 @@ extern "C" int cpm_batch_size(const cpm_batch* B, int32_t* n) {
   return CPM_OK;
 }
 
-extern "C" int cpm_batch_step(cpm_batch* B, const double* H, const int32_t* use_init, cpm_step_report* reps) {
+extern "C" int cpm_batch_step(cpm_batch* B, const double* H, const int32_t* use_init_in, cpm_step_report* reps) {
+  const int32_t* use_init = use_init_in;
   if (!B || !H || !use_init || !reps) return fail(CPM_EINVAL, "cpm_batch_step: null argument");
   const int nb = (int)B->sims.size();
   if (nb == 0) return CPM_OK;
   cpm_ctx* ctx = B->ctx;
   CPM_HIP(hipSetDevice(ctx->device));
   std::vector<FieldPrm> P(nb);
+  std::vector<int32_t> ui(nb);
   for (int j = 0; j < nb; ++j) {
     std::memset(&reps[j], 0, sizeof(cpm_step_report));
-    P[j] = field_prm(B->sims[j], H + 3 * j);
+    P[j] = field_prm(B->sims[j], B->sims[j]->ctl_on ? nullptr : H + 3 * j);
+    ui[j] = step_use_init(B->sims[j], use_init[j]);
   }
+  use_init = ui.data();
   const bool mixed = B->mode == CPM_MODE_MIXED;
   std::vector<SolveSpec> sp(nb);
   std::vector<MultiSys> sys;

@@ -13,8 +13,10 @@ server
 * adds it to the running batch at the next step boundary, so requests that
   arrive while others are on the GPU join as memory frees up,
 * steps every member of the batch with one ``GSPMBatch.step`` call per
-  step (each solve a batched CG over all members), keeping m, u and the
-  CG state of every member resident on the device,
+  step (each solve a batched CG over all members), keeping m, u, the CG
+  state and the sweep's control (field points, the convergence rule,
+  use_init: GSPMSimulation.set_control) of every member resident on the
+  device,
 * retires a request when its last point is done and frees its memory,
 * contains failures to their request: a request over the budget is
   rejected, and one whose preparation raises or whose CG does not converge
@@ -121,23 +123,25 @@ class SweepJob:
         self.t_point = None
         self.t_done = None
 
-    def field(self):
-        return self.fields_red[self.point]
+    def attach_control(self):
+        """The sweep's control on the device: the field list, the per-point
+        step cap and the reference's convergence rule and use_init."""
+        self.sim.set_control(fields=self.fields_red, steps_per_point=self.steps_per_point,
+                             llg_accuracy=self.llg_accuracy, initial_factor=self.P.use_init_factor,
+                             llg_min_step=self.llg_min_step, stop_value=-2.0)
 
     def advance(self, rep, now):
-        """Account one step; returns True when the sweep is finished."""
+        """Account one step from its report; returns True when the sweep is
+        finished."""
         self.nstep += 1
         self.steps += 1
-        sumsq = rep.sumsq
-        self.use_init = bool(sumsq < self.P.use_init_factor * self.llg_accuracy)
-        converged = sumsq < self.llg_accuracy and self.nstep > self.llg_min_step
-        if converged or self.nstep >= self.steps_per_point:
+        if rep.point_done:
             self.points.append((self.fields[self.point].copy(), np.array(rep.avg_m), self.nstep))
             self.point_lat.append(now - self.t_point)
             self.t_point = now
             self.point += 1
             self.nstep = 0
-        return self.point >= len(self.fields)
+        return rep.state == 2
 
 
 class SweepServer:
@@ -249,6 +253,7 @@ class SweepServer:
                         say(f"[serve] t={now():.2f}s request {req['rid']} failed: {results[req['rid']]['error']}")
                         continue
                     job = SweepJob(req, sim, P, n_dof, est, req["arrival"], t_adm)
+                    job.attach_control()
                     job.t_point = now()
                     batch.add(sim)
                     active.append(job)
@@ -262,7 +267,8 @@ class SweepServer:
                         time.sleep(max(0.0, min(0.05, pending[0]["arrival"] - now())))
                     continue
                 max_members = max(max_members, len(active))
-                reps = batch.step([j.field() for j in active], [j.use_init for j in active], raise_on_fail=False)
+                # field, use_init and the point control of every member live on the device
+                reps = batch.step(raise_on_fail=False)
                 steps_run += 1
                 t = now()
                 finished, broken = [], []

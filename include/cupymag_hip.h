@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define CPM_ABI_VERSION 2
+#define CPM_ABI_VERSION 3
 
 #define CPM_OK 0
 #define CPM_EINVAL (-1)
@@ -391,7 +391,43 @@ typedef struct cpm_step_report {
   int32_t iters[10];  /* CG iterations: elasticity, demag, g1n, g2n, g3n, g1*, g2*, m1**, m2**, m3** */
   int32_t failed;     /* index+1 of the solve that did not converge, 0 if all converged */
   double resid;       /* recurrence residual of the failed solve */
+  /* with a hysteresis control attached (cpm_sim_control), filled from its
+   * device state in the step's one scalar transfer: */
+  int32_t point_done; /* the step completed a field point (LLG convergence, or the per-point cap) */
+  int32_t state;      /* 0 running, 1 stop condition met (Micromagnetics.py:109), 2 field list done */
+  int32_t point;      /* index of the current field point (points completed) */
+  int32_t nstep;      /* steps at the current point (0 right after a point) */
+  int32_t use_init;   /* use_init of the next step (:300) */
+  int32_t pad;
+  double H[3];        /* field of this step (reduced units) */
+  double H_next[3];   /* field of the next step */
 } cpm_step_report;
+
+/* Hysteresis control of a simulation (Micromagnetics.py:109,299-335), kept
+ * on the device and evaluated by a kernel after every step: use_init,
+ * LLG convergence (sumsq < llg_accuracy after more than llg_min_step steps
+ * at a point), the field step (H -= dH, or the next of `npoints` listed
+ * fields), the stop condition (avg_m . stop_dir <= stop_value) and an
+ * optional cap of steps per point.  Field and use_init of the steps then
+ * come from this state (the H and use_init arguments of cpm_sim_step /
+ * cpm_batch_step are ignored for the simulation), and the host learns the
+ * outcome from the step report without a further transfer. */
+typedef struct cpm_sweep_cfg {
+  double H0[3];          /* first field (npoints = 0) */
+  double dH[3];          /* field step on convergence */
+  double llg_accuracy;   /* nDOF * llg_accuracy_factor (:88) */
+  double initial_factor; /* use_init when sumsq < initial_factor * llg_accuracy (:300) */
+  double stop_dir[3];
+  double stop_value;
+  int32_t llg_min_step;  /* 10 with magnetoelastic coupling, else 100 (:83-86) */
+  int32_t steps_per_point; /* 0: no cap */
+  int32_t npoints;       /* > 0: fields (npoints x 3, reduced) instead of H0 and dH */
+  int32_t pad;
+} cpm_sweep_cfg;
+
+/* No counterpart as a call in the reference (its loop body, :109,299-335).
+ * Attach (cfg) or detach (cfg NULL) the control; the step counters start at 0. */
+int cpm_sim_control(cpm_sim* sim, const cpm_sweep_cfg* cfg, const double* fields);
 
 /* Create a simulation on `ops` (borrowed; they must outlive the sim).
  * W: device pointer to the n per-DOF quadrature weights and `volume` their

@@ -464,3 +464,40 @@ def test_unstructured_numbering_uses_indexed_sell():
     og.step(st, ops, Avg, p)
     sim.step([P.Hext1, P.Hext2, P.Hext3], False)
     assert np.max(np.abs(sim.get("M").cpu().numpy() - st["m"])) < 1e-10
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_device_control_equals_host_control(mode):
+    """The hysteresis control on the device (set_control) drives the same
+    trajectory as the host applying the reference's rules
+    (Micromagnetics.py:299-335) around step(H, use_init): the same field
+    points at the same steps, and m bit-identical after them."""
+    from cupymag_amd.core.gspm import GSPMSimulation
+    cfg = config("Hex", n=6, me=False, ndef=2, hext=(-2e4, 0.0, 0.0))
+    cfg["physics"]["external_field"]["dhext1"] = 2e4
+    cfg["simulation"]["llg"].update(llg_accuracy_factor=1e-6, timestep=5e-12)
+    P, S = _product(cfg)
+    m0 = tilted_m(S.n_dof, seed=12)
+    a, b = GSPMSimulation(S, P, mode=mode), GSPMSimulation(S, P, mode=mode)
+    a.set_m(m0)
+    b.set_m(m0)
+    b.set_control(H0=[P.Hext1, P.Hext2, P.Hext3], dH=[P.dHext1, P.dHext2, P.dHext3], stop_value=-2.0)
+    acc = S.n_dof * P.LLG_accuracy_factor
+    H = np.array([P.Hext1, P.Hext2, P.Hext3])
+    use_init, nstep, pts_a, pts_b = False, 0, [], []
+    for k in range(300):
+        nstep += 1
+        ra = a.step(H, use_init)
+        rb = b.step()
+        np.testing.assert_array_equal(rb.H, H)
+        use_init = ra.sumsq < P.use_init_factor * acc
+        assert rb.use_init == use_init and rb.sumsq == ra.sumsq
+        if ra.sumsq < acc and nstep > 100:
+            pts_a.append(k)
+            H = H - np.array([P.dHext1, P.dHext2, P.dHext3])
+            nstep = 0
+        if rb.point_done:
+            pts_b.append(k)
+        assert rb.nstep == nstep
+    assert pts_a == pts_b and len(pts_a) >= 1
+    assert torch.equal(a.get("M"), b.get("M"))

@@ -249,3 +249,93 @@ def test_cfg2_golden_file_matches_its_case():
     it = np.asarray(fx["iters"])
     assert it.shape == (int(fx["steps"]), 10)
     assert np.all(it[:, 0] > it[:, 1]) and np.all(it[:, 1] > it[:, 2:].max(axis=1))
+
+
+def _sweep_ctl_host(tmp_path):
+    import shutil
+    import subprocess
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    root = Path(__file__).resolve().parent.parent
+    exe = str(tmp_path / "sweep_ctl_host")
+    subprocess.run([hipcc, "--offload-host-only", "-O2", "-std=c++17", f"-I{root}/cupymag_amd/csrc",
+                    f"-I{root}/include", str(root / "tests" / "sweep_ctl_host.hip"), "-o", exe],
+                   check=True, capture_output=True)
+    return exe
+
+
+def _reference_loop(seq, H0, dH, acc, initf, minstep, stop_dir, stop_value, fields=None, spp=0):
+    """Micromagnetics.py:109,299-335 (and, with a field list and a per-point
+    cap, the sweep server's rule) step by step: per step use_init,
+    point_done, state, point, nstep, H of the step, H of the next step."""
+    H = np.array(fields[0] if fields is not None else H0, dtype=np.float64)
+    nstep = point = 0
+    state = 0
+    out = []
+    for sumsq, avg in seq:
+        if state:
+            break
+        nstep += 1
+        use_init = sumsq < initf * acc
+        Hs = H.copy()
+        conv = sumsq < acc and nstep > minstep
+        done = conv or (spp > 0 and nstep >= spp)
+        if done:
+            point += 1
+            nstep = 0
+            if fields is not None:
+                if point >= len(fields):
+                    state = 2
+                else:
+                    H = np.array(fields[point], dtype=np.float64)
+            elif conv:
+                H = H - np.asarray(dH)
+        if state == 0 and not (np.dot(avg, stop_dir) > stop_value):
+            state = 1
+        out.append([float(use_init), float(done), float(state), float(point), float(nstep), *Hs, *H])
+    return np.array(out)
+
+
+@pytest.mark.parametrize("kind", ["field_step", "field_list"])
+def test_device_sweep_control_rule_matches_the_reference_loop(kind, tmp_path):
+    """The hysteresis control the device applies after every step
+    (sweep_ctl.h, built for the host) gives, step by step, the reference
+    loop's use_init, field points, field steps and stop (Micromagnetics.py
+    :109,299-335) on a scripted sumsq / avg_m sequence that converges,
+    stalls, reverses and crosses the stop value; and the server's rule with
+    a field list and a per-point step cap."""
+    import subprocess
+    exe = _sweep_ctl_host(tmp_path)
+    rng = np.random.default_rng(4)
+    nst = 700
+    acc, initf, minstep = 1e-3, 100.0, 10
+    sumsq = np.where(rng.random(nst) < 0.7, 10.0 ** rng.uniform(-6, -2, nst), 10.0 ** rng.uniform(-1, 2, nst))
+    sumsq[:40] = 1e-9                      # converges as soon as nstep > minstep
+    avg = np.zeros((nst, 3))
+    avg[:, 0] = np.linspace(1.0, -1.0, nst)  # reverses: the stop value is crossed near the end
+    avg[:, 1] = 0.1
+    stop_dir, stop_value = np.array([1.0, 0.0, 0.0]), -0.9
+    H0, dH = np.array([0.05, 0.0, 0.01]), np.array([0.002, 0.0, 0.0])
+    fields, spp = None, 0
+    if kind == "field_list":
+        fields = np.linspace(-0.08, 0.08, 9)[:, None] * np.array([[0.6, 0.0, 0.8]])
+        spp, stop_value = 37, -2.0
+    c = np.zeros(64)
+    c[32] = 1.0
+    c[33:36] = fields[0] if fields is not None else H0
+    c[36:39] = dH
+    c[39], c[40] = acc, initf
+    c[41:44], c[44] = stop_dir, stop_value
+    c[45], c[46] = minstep, spp
+    c[47] = 0 if fields is None else len(fields)
+    c[53:56] = c[33:36]
+    blob = c.tobytes() + np.array([c[47]]).tobytes()
+    if fields is not None:
+        blob += np.ascontiguousarray(fields).tobytes()
+    blob += np.array([float(nst)]).tobytes() + np.concatenate([sumsq[:, None], avg], axis=1).tobytes()
+    got = np.frombuffer(subprocess.run([exe], input=blob, capture_output=True, check=True).stdout).reshape(nst, 11)
+    want = _reference_loop(zip(sumsq, avg), H0, dH, acc, initf, minstep, stop_dir, stop_value, fields, spp)
+    m = len(want)
+    assert m > 100 and want[:, 1].sum() >= 3              # several field points
+    assert want[-1, 2] == (1 if kind == "field_step" else 2)  # ends on the stop value / the list's end
+    np.testing.assert_array_equal(got[:m], want)
+    assert np.all(got[m:, :5] == got[m - 1, :5])           # once stopped, the state stays

@@ -64,28 +64,31 @@ class _Rep:
         self.avg_m = np.array([1.0, 0.0, 0.0])
 
 
-def test_sweep_job_points_and_use_init():
+class _CtlRep:
+    def __init__(self, point_done=False, state=0, avg=(1.0, 0.0, 0.0)):
+        self.point_done, self.state, self.avg_m = point_done, state, np.array(avg)
+
+
+def test_sweep_job_bookkeeping_from_the_device_control():
+    """The per-point rule runs on the device (sweep_ctl.h, pinned in
+    test_host.py); the job records each completed point with its field,
+    average magnetization, step count and latency, and ends on state 2."""
     req = dict(rid=0, fields=np.array([[-1e5, 0, 0], [0, 0, 0], [1e5, 0, 0]]), steps_per_point=150)
     job = SweepJob(req, None, _P, n_dof=1000, est=1, t_arrival=0.0, t_admit=0.0)
     job.t_point = 0.0
-    np.testing.assert_allclose(job.field(), [-1e5 / 1.2e6, 0, 0])
-    # point 0: converges (sumsq < 1000 * 1e-3) only after llg_min_step = 100 steps
     for k in range(100):
-        assert not job.advance(_Rep(1e-6), float(k))
-        assert job.point == 0
-        assert job.use_init  # sumsq < 100 * LLG_accuracy
-    assert not job.advance(_Rep(1e-6), 100.0)
-    assert job.point == 1 and job.points[0][2] == 101
-    # point 1: never converges -> the per-point budget ends it
+        assert not job.advance(_CtlRep(), float(k))
+    assert not job.advance(_CtlRep(True, 0, (0.9, 0, 0)), 100.0)
+    assert job.point == 1 and job.points[0][2] == 101 and job.point_lat == [100.0]
+    np.testing.assert_array_equal(job.points[0][0], [-1e5, 0, 0])
     for k in range(149):
-        job.advance(_Rep(1e9), 0.0)
-        assert not job.use_init
-    assert job.point == 1
-    job.advance(_Rep(1e9), 0.0)
+        job.advance(_CtlRep(), 0.0)
+    job.advance(_CtlRep(True), 0.0)
     assert job.point == 2 and job.points[1][2] == 150
-    for k in range(101):
-        done = job.advance(_Rep(0.0), 0.0)
-    assert done and len(job.points) == 3 and job.steps == 101 + 150 + 101
+    for k in range(100):
+        job.advance(_CtlRep(), 0.0)
+    assert job.advance(_CtlRep(True, 2), 0.0)
+    assert len(job.points) == 3 and job.steps == 101 + 150 + 101
 
 
 def test_estimate_grows_with_size_and_coupling():
