@@ -339,3 +339,72 @@ def test_device_sweep_control_rule_matches_the_reference_loop(kind, tmp_path):
     assert want[-1, 2] == (1 if kind == "field_step" else 2)  # ends on the stop value / the list's end
     np.testing.assert_array_equal(got[:m], want)
     assert np.all(got[m:, :5] == got[m - 1, :5])           # once stopped, the state stays
+
+
+@pytest.fixture(scope="module")
+def _asan_assembler(tmp_path_factory):
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    root = Path(__file__).resolve().parent.parent
+    exe = str(tmp_path_factory.mktemp("asan") / "assembly_sanitize")
+    subprocess.run([gxx, "-O1", "-g", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+                    "-fno-omit-frame-pointer", "-std=c++17", "-pthread", f"-I{root}/include",
+                    f"-I{root}/cupymag_amd/csrc", "-I/opt/rocm/include", "-D__HIP_PLATFORM_AMD__",
+                    str(root / "cupymag_amd" / "csrc" / "host_assembly.cpp"), str(root / "tests" / "assembly_sanitize.cpp"),
+                    "-o", exe], check=True, capture_output=True)
+    return exe
+
+
+@pytest.mark.parametrize("grid,elastic,anchors,threads", [("Hex", False, True, 1), ("Hex", True, True, 4),
+                                                          ("Tet", True, True, 4), ("Tet", True, False, 3)])
+def test_host_assembler_under_address_and_ub_sanitizers(_asan_assembler, grid, elastic, anchors, threads):
+    """The native host assembler built with AddressSanitizer and
+    UndefinedBehaviorSanitizer (no recovery: any finding aborts the run)
+    assembles a hex grid and a jittered Kuhn mesh with a defect, with and
+    without the elasticity operators and anchors, on several threads; every
+    array equals the library build's bit for bit."""
+    import os
+    import subprocess
+
+    from cupymag_amd.core.parameters import from_dict
+    from cupymag_amd.mesh.kuhn import kuhn_mesh
+    from cupymag_amd.mesh.setup_FEM_mesh import FEMMesh
+    from cupymag_amd.physics.host_ops import HostOps, voigt_C
+    cfg = config(grid, n=5, me=elastic, ndef=2, defect_params=1.3)
+    P = from_dict(cfg)
+    M = FEMMesh(P, kuhn_mesh(5, jitter=0.15, seed=8) if grid == "Tet" else None)
+    coords, el, gid, n = M.node_coords_np, M.elements_np, M.global_id_np, M.n_dof
+    Cv = voigt_C(P.c11, P.c12, P.c44, False) if elastic else None
+    npe = el.shape[1] - 1
+    mask = np.zeros(n)
+    mask[np.asarray(M.DefDOF)] = 1
+    coef = np.zeros(38)
+    coef[0], coef[1] = P.alpha, P.alpha * P.alpha_damp
+    if elastic:
+        coef[2:] = Cv.ravel()
+    flags = int(elastic) | (0 if anchors else 2)
+    blob = np.concatenate([[coords.shape[0]], coords.ravel(), [el.shape[0], npe, el.shape[1]], el.ravel(), [n],
+                           gid, mask, coef, [flags, threads]]).astype(np.float64).tobytes()
+    env = dict(os.environ, ASAN_OPTIONS="halt_on_error=1:detect_leaks=1:abort_on_error=0",
+               UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1")
+    r = subprocess.run([_asan_assembler], input=blob, capture_output=True, env=env)
+    assert r.returncode == 0, r.stderr.decode()[-3000:]
+    out = r.stdout
+    H = HostOps(coords, el, gid, n, M.DefDOF, coef[0], coef[1], Cv, nthreads=threads, anchors=anchors)
+    names = ["PTR", "IDX", "ADEMAG", "FX", "FY", "FZ", "GS_PTR", "GS_IDX", "A1", "A2", "F1_PTR", "F1_IDX", "F1",
+             "EL_PTR", "EL_IDX", "AEL", "FE_PTR", "FE_IDX", "FEL"]
+    pos = 0
+    for name in names:
+        ln = int(np.frombuffer(out[pos:pos + 8], dtype=np.int64)[0])
+        pos += 8
+        dt = np.int32 if name.endswith("IDX") else (np.int64 if name.endswith("PTR") else np.float64)
+        got = np.frombuffer(out[pos:pos + ln * np.dtype(dt).itemsize], dtype=dt)
+        pos += ln * np.dtype(dt).itemsize
+        if name.startswith(("EL", "FE", "AEL")) and not elastic:
+            assert ln == 0
+            continue
+        np.testing.assert_array_equal(got, np.asarray(H.array(name)), err_msg=name)
+    assert pos < len(out)  # the element matrices follow
