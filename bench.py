@@ -60,10 +60,12 @@ def mode_kwargs(base, opts, mg_degree=2, mg_ratio=10.0):
                 mg_degree=mg_degree, mg_ratio=mg_ratio, mg_precision=mgp if mg else None)
 
 
-def kernel_bytes(base, mg, mgp, nnz, rows, stencil=True):
+def kernel_bytes(base, mg, mgp, nnz, rows, stencil=True, half=False):
     """Algorithmic bytes per launch of a mode's roofline kernel and its name
     (rocprofv3 spelling).  nnz, rows: of the elasticity operator A_el (3x3
-    blocks, nb = rows / 3 block rows, nnz / 9 blocks of which nb diagonal)."""
+    blocks, nb = rows / 3 block rows, nnz / 9 blocks of which nb diagonal).
+    half: the level-0 FP16 form keeps half of the symmetric stencil (each
+    off-diagonal block stored once, for the row and its mirror)."""
     nb = rows // 3
     off = nnz / 9.0 - nb
     if base == "fp64":
@@ -83,7 +85,10 @@ def kernel_bytes(base, mg, mgp, nnz, rows, stencil=True):
                 f"{v_} off-diagonal values)")
         roc = f"k_mg_sweep<3, 1, {t_}, 0, true, {v_}>"
         if mgp == 16:
-            return name, roc, off * 9 * vs_ + nb * 36.0 + nb * 18.0 * s_
+            stored = off / 2 if half else off
+            if half:
+                name = name.replace("stencil,", "half-stored symmetric stencil,")
+            return name, roc, stored * 9 * vs_ + nb * 36.0 + nb * 18.0 * s_
         return name, roc, off * 9 * vs_ + nb * 72.0 + nb * 27.0 * s_
     # SELL-64 3x3 blocks: off-diagonal blocks 9 values (+ 1 column index
     # unless the index-free stencil form is in use), diagonal blocks 9 FP64
@@ -199,7 +204,9 @@ def run_gpu(n, steps, warmup, steps64, modes, tol, ref_steps=REF_STEPS, rel_delt
         while done < total:
             one("untimed")
         info = A_el.sell_info()
-        kname, rocname, bytes_launch = kernel_bytes(base, mg, mgp, nnz, rows, info["stencil"])
+        half = bool(mg and mgp == 16 and S.multigrid(precision=mgp, degree=mg_degree,
+                                                      ratio=mg_ratio)["A_el"].info()["half_stencil"][0])
+        kname, rocname, bytes_launch = kernel_bytes(base, mg, mgp, nnz, rows, info["stencil"], half)
         avg_s = ms.value / 1e3 / max(nl.value, 1)
         d = dict(elapsed=elapsed, steps=k_timed, lat=lat, iters=iters, snaps=snaps,
                  spmv=dict(kernel=kname, rocname=rocname, launches=nl.value, avg_us=avg_s * 1e6,

@@ -112,6 +112,7 @@ _SIGS = {
     "cpm_mg_create": (C.c_int, [_P, C.c_int, _P, _P, C.c_int, C.c_int, _D, C.POINTER(_P)]),
     "cpm_mg_destroy": (C.c_int, [_P]),
     "cpm_mg_info": (C.c_int, [_P, C.POINTER(C.c_int), _P, C.c_int]),
+    "cpm_mg_forms": (C.c_int, [_P, _P, C.c_int]),
     "cpm_mg_apply": (C.c_int, [_P, _P, _P]),
     "cpm_cg_mg": (C.c_int, [_P, C.c_int, _P, _P, C.c_int, _D, C.c_int, _D, _P, _P]),
     "cpm_mg_set_graphs": (C.c_int, [_P, C.c_int]),

@@ -23,6 +23,7 @@ struct MgLevel {
   // rounding: dgh in FP64 (during set-up only), dg32 its FP32 rounding, which
   // the FP16 sweeps read (and invert per node)
   void* v16 = nullptr;
+  int khalf = 0;            // v16 keeps half of a symmetric Kuhn stencil (GridArg::kuhn = 2)
   double* dgh = nullptr;
   float* dg32 = nullptr;
   double vscale = 1.0;

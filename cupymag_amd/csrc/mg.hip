@@ -217,6 +217,53 @@ __global__ void __launch_bounds__(kBlock) k_mg_make_low(int32_t nb, int64_t nrow
   }
 }
 
+// Symmetry of a Kuhn stencil operator, slot by slot: amax[0] = max over the
+// backward blocks of |A(i, i-d) - A(i-d, i)^T| (the mirrored forward block
+// of the neighbour row), amax[1] = max |A(i, i-d)|, as FP64 bit patterns
+// (non-negative values order like their bits).
+template <int BS>
+__global__ void __launch_bounds__(kBlock) k_mg_kuhn_asym(int32_t nb, GridArg ga, const double* __restrict__ v64,
+                                                         unsigned long long* __restrict__ amax) {
+  constexpr int BB = BS * BS;
+  double dm = 0.0, am = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t row = (int64_t)blockIdx.x * kBlock + threadIdx.x; row < nb; row += stride) {
+    const int64_t s = row >> 6, l = row & 63;
+    for (int t = 0; t < 7; ++t) {
+      const int64_t c = kuhn_neighbour(row, ga, t);
+      const int64_t cs = (c >> 6) * kKuhnSlots + (kKuhnSlots - 1 - t), lc = c & 63;
+#pragma unroll
+      for (int d = 0; d < BS; ++d)
+#pragma unroll
+        for (int e = 0; e < BS; ++e) {
+          const double a = v64[((s * kKuhnSlots + t) * BB + d * BS + e) * 64 + l];
+          const double an = v64[(cs * BB + e * BS + d) * 64 + lc];
+          dm = fmax(dm, fabs(a - an));
+          am = fmax(am, fabs(a));
+        }
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    dm = fmax(dm, __shfl_xor(dm, o));
+    am = fmax(am, __shfl_xor(am, o));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(&amax[0], (unsigned long long)__double_as_longlong(dm));
+    atomicMax(&amax[1], (unsigned long long)__double_as_longlong(am));
+  }
+}
+
+// FP16 half form of a symmetric Kuhn stencil operator (make_half_row).
+template <int BS>
+__global__ void __launch_bounds__(kBlock) k_mg_make_half(int32_t nb, int64_t nrows_pad, GridArg ga,
+                                                         const double* __restrict__ v64,
+                                                         const double* __restrict__ dg64, double vscale,
+                                                         _Float16* __restrict__ v16, double* __restrict__ dgh) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t row = (int64_t)blockIdx.x * kBlock + threadIdx.x; row < nrows_pad; row += stride)
+    make_half_row<BS>(row, nb, ga, v64, dg64, vscale, v16, dgh);
+}
+
 template <int W, int K, class T>
 __global__ void __launch_bounds__(kBlock) k_mg_restrict(Dims3 fine, Dims3 coarse, int anchor,
                                                         const T* __restrict__ rf, T* __restrict__ bc,
@@ -491,13 +538,15 @@ static void launch_sweep(hipStream_t s, int g, const Sell& S, const VT* val, con
                          const MgLevel& Lv,
                          const void* din, void* dout, double c1, double c2, double vscale, const CgState* st,
                          hipEvent_t* ev) {
+  GridArg ga = grid_arg(S);
+  if (sizeof(VT) == 2 && Lv.khalf) ga.kuhn = 2;
   if (ev)
     hipExtLaunchKernelGGL(k_mg_sweep<BS, K, T, MODE, ST, VT>, dim3(g), dim3(kBlock), 0, s, ev[0], ev[1], 0,
-                          (int32_t)S.nbrows, (const int64_t*)S.sptr, (const int32_t*)S.col, val, dg, grid_arg(S),
+                          (int32_t)S.nbrows, (const int64_t*)S.sptr, (const int32_t*)S.col, val, dg, ga,
                           (const T*)din, (T*)dout, (T*)Lv.x, (T*)Lv.r, (const T*)Lv.b, (const T*)Lv.dinv, c1, c2,
                           vscale, st);
   else
-    k_mg_sweep<BS, K, T, MODE, ST, VT><<<g, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, val, dg, grid_arg(S),
+    k_mg_sweep<BS, K, T, MODE, ST, VT><<<g, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, val, dg, ga,
                                                            (const T*)din, (T*)dout, (T*)Lv.x, (T*)Lv.r,
                                                            (const T*)Lv.b, (const T*)Lv.dinv, c1, c2, vscale, st);
 }
@@ -1109,11 +1158,48 @@ extern "C" int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, 
       double dmax = 0.0;
       for (double v : dg) dmax = std::max(dmax, std::fabs(v));
       Lv.vscale = std::ldexp(1.0, std::max(-60, (int)std::ceil(std::log2(std::max(dmax, 1e-300) / 16384.0))));
-      A(&Lv.v16, sizeof(_Float16) * std::max<int64_t>(1, S.nslot * bs * bs * 64));
+      // half storage when the Kuhn stencil is symmetric slot by slot (to
+      // 1e-12 of its largest entry) and every offset reaches its own node
+      if (S.kuhn && S.dims[0] >= 3 && S.dims[1] >= 3 && S.dims[2] >= 3) {
+        void* am = nullptr;
+        A(&am, 2 * sizeof(unsigned long long));
+        if (rc != CPM_OK) break;
+        unsigned long long hm[2] = {0, 0};
+        e = hipMemcpy(am, hm, sizeof(hm), hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+          if (bs == 3)
+            k_mg_kuhn_asym<3><<<grid_for(Lv.nb), kBlock, 0, s>>>(S.nbrows, grid_arg(S), v64,
+                                                                 (unsigned long long*)am);
+          else
+            k_mg_kuhn_asym<1><<<grid_for(Lv.nb), kBlock, 0, s>>>(S.nbrows, grid_arg(S), v64,
+                                                                 (unsigned long long*)am);
+          e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(hm, am, sizeof(hm), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        (void)hipFree(am);
+        M->allocs.erase(std::find(M->allocs.begin(), M->allocs.end(), am));
+        if (e != hipSuccess) {
+          rc = fail(CPM_EHIP, hipGetErrorString(e));
+          break;
+        }
+        double asym, amx;
+        std::memcpy(&asym, &hm[0], sizeof(double));
+        std::memcpy(&amx, &hm[1], sizeof(double));
+        Lv.khalf = (asym <= 1e-12 * amx) ? 1 : 0;
+      }
+      A(&Lv.v16, sizeof(_Float16) * std::max<int64_t>(1, (Lv.khalf ? S.nslot / 2 : S.nslot) * bs * bs * 64));
       A((void**)&Lv.dgh, sizeof(double) * nbb);
       if (rc != CPM_OK) break;
       const int64_t pad = (int64_t)S.nslices * 64;
-      if (bs == 3)
+      if (Lv.khalf) {
+        if (bs == 3)
+          k_mg_make_half<3><<<grid_for(pad), kBlock, 0, s>>>(S.nbrows, pad, grid_arg(S), v64, dg64, Lv.vscale,
+                                                             (_Float16*)Lv.v16, Lv.dgh);
+        else
+          k_mg_make_half<1><<<grid_for(pad), kBlock, 0, s>>>(S.nbrows, pad, grid_arg(S), v64, dg64, Lv.vscale,
+                                                             (_Float16*)Lv.v16, Lv.dgh);
+      } else if (bs == 3)
         k_mg_make_low<3, _Float16><<<grid_for(pad), kBlock, 0, s>>>(S.nbrows, pad, S.sptr, S.nst, S.stencil, v64,
                                                                     dg64, Lv.vscale, (_Float16*)Lv.v16, Lv.dgh);
       else
@@ -1264,6 +1350,13 @@ extern "C" int cpm_mg_info(const cpm_mg* M, int* nlevels, double* lmax, int max_
   *nlevels = (int)M->L.size();
   if (lmax)
     for (int l = 0; l < std::min<int>(max_levels, (int)M->L.size()); ++l) lmax[l] = M->L[l].lmax;
+  return CPM_OK;
+}
+
+extern "C" int cpm_mg_forms(const cpm_mg* M, int* half_stencil, int max_levels) {
+  if (!M || !half_stencil) return fail(CPM_EINVAL, "cpm_mg_forms: null argument");
+  for (int l = 0; l < std::min<int>(max_levels, (int)M->L.size()); ++l)
+    half_stencil[l] = (M->half && M->L[l].khalf) ? 1 : 0;
   return CPM_OK;
 }
 

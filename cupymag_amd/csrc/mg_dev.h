@@ -188,6 +188,62 @@ __device__ __forceinline__ void mg_sweep_rows(int64_t lb, int64_t lg, int32_t nb
   }
 }
 
+// Neighbour row of `row` at Kuhn slot t of a periodic grid (dims >= 3).
+__host__ __device__ __forceinline__ int64_t kuhn_neighbour(int64_t row, const GridArg& ga, int t) {
+  const int32_t code = kKuhnCodes[t];
+  const int32_t nyz = ga.ny * ga.nz;
+  int32_t i = (int32_t)(row / nyz) + (code & 3) - 1;
+  int32_t j = (int32_t)((row / ga.nz) % ga.ny) + ((code >> 2) & 3) - 1;
+  int32_t k = (int32_t)(row % ga.nz) + ((code >> 4) & 3) - 1;
+  i += (i < 0) ? ga.nx : 0;
+  i -= (i >= ga.nx) ? ga.nx : 0;
+  j += (j < 0) ? ga.ny : 0;
+  j -= (j >= ga.ny) ? ga.ny : 0;
+  k += (k < 0) ? ga.nz : 0;
+  k -= (k >= ga.nz) ? ga.nz : 0;
+  return ((int64_t)i * ga.ny + j) * ga.nz + k;
+}
+
+// Row `row` (< nrows_pad, padding rows included) of the FP16 half form of a
+// symmetric Kuhn stencil operator, read by sell_row with GridArg::kuhn = 2:
+// plane s*7 + u = fp16(v64 slot s*14 + 7 + u / vscale), a lane's 3x3 entries
+// 0..7 in one 16-byte word (8 halves at 8*l), entry 8 at 512 + l; for rows
+// < nb, dgh = dg64 + the rounding of the fourteen blocks the sweeps read for
+// the row (its forward blocks, and for its backward slots the transposed
+// forward blocks of the neighbours), so that the row sums with dgh equal the
+// FP64 ones.
+template <int BS>
+__host__ __device__ inline void make_half_row(int64_t row, int32_t nb, const GridArg& ga,
+                                              const double* __restrict__ v64, const double* __restrict__ dg64,
+                                              double vscale, _Float16* __restrict__ v16,
+                                              double* __restrict__ dgh) {
+  constexpr int BB = BS * BS;
+  const int64_t s = row >> 6, l = row & 63;
+  double comp[BB];
+  for (int t = 0; t < BB; ++t) comp[t] = 0.0;
+  for (int u = 0; u < 7; ++u) {
+    const int64_t slot = s * kKuhnSlots + 7 + u, plane = s * 7 + u;
+    for (int t = 0; t < BB; ++t) {
+      const double a = v64[(slot * BB + t) * 64 + l];
+      const _Float16 h = (_Float16)(a / vscale);
+      v16[BB == 9 ? plane * 576 + (t < 8 ? 8 * l + t : 512 + l) : plane * 64 + l] = h;
+      comp[t] += a - (double)h * vscale;
+    }
+  }
+  if (row >= nb) return;
+  for (int t = 0; t < 7; ++t) {
+    const int64_t c = kuhn_neighbour(row, ga, t);
+    const int64_t cs = (c >> 6) * kKuhnSlots + (kKuhnSlots - 1 - t), lc = c & 63;
+    for (int d = 0; d < BS; ++d)
+      for (int e = 0; e < BS; ++e) {
+        const double a = v64[((s * kKuhnSlots + t) * BB + d * BS + e) * 64 + l];
+        const _Float16 h = (_Float16)(v64[(cs * BB + e * BS + d) * 64 + lc] / vscale);
+        comp[d * BS + e] += a - (double)h * vscale;
+      }
+  }
+  for (int t = 0; t < BB; ++t) dgh[row * BB + t] = dg64[row * BB + t] + comp[t];
+}
+
 __host__ __device__ __forceinline__ int64_t wrap_idx(int32_t i, int32_t j, int32_t k, const int32_t* dims) {
   i += (i < 0) ? dims[0] : 0;
   i -= (i >= dims[0]) ? dims[0] : 0;

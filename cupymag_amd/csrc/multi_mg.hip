@@ -463,6 +463,7 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
                                  : (const void*)(Lv.kdg64 ? Lv.kdg64 : S.dg64);
         g.vscale = M->half ? Lv.vscale : 1.0;
         g.ga = CPM_GRID_ARG(S);
+        if (M->half && Lv.khalf) g.ga.kuhn = 2;  // half of the stencil kept (k_mg_make_half)
         g.nb = S.nbrows, g.stencil = S.stencil;
         g.x = Lv.x, g.r = Lv.r, g.b = Lv.b, g.d0 = Lv.d0, g.d1 = Lv.d1, g.dinv = Lv.dinv;
         g.b64 = (d == 0) ? M->r : nullptr;

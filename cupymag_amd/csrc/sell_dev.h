@@ -25,7 +25,8 @@ namespace cpm {
 // Periodic structured grid of an index-free (stencil) SELL operator.
 // kuhn: the slots are the 14 neighbours of a Kuhn mesh in the order of
 // kKuhnCodes (the host checks soff), so the kernels decode them at compile
-// time.
+// time; 2: besides, an FP16 form passed with it keeps half of the stencil
+// (k_mg_make_half: the forward slots 7..13, 7 planes per slice).
 struct GridArg {
   int32_t nx, ny, nz, nst;
   const int32_t* soff;
@@ -52,7 +53,7 @@ using sell_diag_t = typename std::conditional<sizeof(VT) == 2, float, double>::t
 // two and their sum is scaled back by vscale, and the diagonal blocks are
 // FP32 (sell_diag_t).
 template <int BS, int K, class T, bool ADDX32, bool ST, class VT = T>
-__device__ __forceinline__ void sell_row(int64_t row, int32_t nbrows, const int64_t* __restrict__ sptr,
+__host__ __device__ __forceinline__ void sell_row(int64_t row, int32_t nbrows, const int64_t* __restrict__ sptr,
                                          const int32_t* __restrict__ col, const VT* __restrict__ val,
                                          const sell_diag_t<VT>* __restrict__ dg, const T* __restrict__ x,
                                          const float* __restrict__ x32, const GridArg& ga, double (&sum)[BS * K],
@@ -145,14 +146,72 @@ __device__ __forceinline__ void sell_row(int64_t row, int32_t nbrows, const int6
         {ri == 0 ? (ga.nx - 1) * nyz : -nyz, 0, ri == ga.nx - 1 ? (1 - ga.nx) * nyz : nyz},
         {rj == 0 ? (ga.ny - 1) * ga.nz : -ga.nz, 0, rj == ga.ny - 1 ? (1 - ga.ny) * ga.nz : ga.nz},
         {rk == 0 ? ga.nz - 1 : -1, 0, rk == ga.nz - 1 ? 1 - ga.nz : 1}};
+    auto kuhn_full = [&]() {
 #pragma unroll
-    for (int t = 0; t < kKuhnSlots; t += 7) {
+      for (int t = 0; t < kKuhnSlots; t += 7) {
 #pragma unroll
-      for (int u = 0; u < 7; ++u) {
-        const int32_t code = kKuhnCodes[t + u];
-        const int32_t c = r + stp[0][code & 3] + stp[1][(code >> 2) & 3] + stp[2][(code >> 4) & 3];
-        slot_fma_at(s * kKuhnSlots + t + u, c);
+        for (int u = 0; u < 7; ++u) {
+          const int32_t code = kKuhnCodes[t + u];
+          const int32_t c = r + stp[0][code & 3] + stp[1][(code >> 2) & 3] + stp[2][(code >> 4) & 3];
+          slot_fma_at(s * kKuhnSlots + t + u, c);
+        }
       }
+    };
+    if constexpr (sizeof(VT) == 2) {
+      if (ga.kuhn == 2) {
+        // half storage (FP16 forms of symmetric level operators): slice s
+        // holds planes s*7 + u with the forward blocks of slots 7 + u
+        // (offsets +d); the backward block of slot t < 7 (offset -d) is
+        // A(i, i-d) = A(i-d, i)^T, the forward block of slot 13 - t (plane
+        // 6 - t) of the neighbour row, read transposed.  Plain (cached)
+        // loads: every block serves two rows.
+#pragma unroll
+        for (int t = 0; t < kKuhnSlots; t += 7) {
+#pragma unroll
+          for (int u = 0; u < 7; ++u) {
+            const int32_t code = kKuhnCodes[t + u];
+            const int32_t c = r + stp[0][code & 3] + stp[1][(code >> 2) & 3] + stp[2][(code >> 4) & 3];
+            const bool back = t == 0;
+            const int64_t plane = back ? (int64_t)(c >> 6) * 7 + (6 - u) : s * 7 + u;
+            const int64_t lane = back ? (int64_t)(c & 63) : l;
+            double xv[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+              xv[v] = (double)x[(int64_t)c * NV + v];
+              if (ADDX32) xv[v] += (double)x32[(int64_t)c * NV + v];
+            }
+            if constexpr (BB == 9) {
+              double av[9];
+              typedef uint32_t u32x4h __attribute__((ext_vector_type(4)));
+              const u32x4h w4 = reinterpret_cast<const u32x4h*>(val + plane * 576)[lane];
+              const uint32_t pw4[4] = {w4.x, w4.y, w4.z, w4.w};
+#pragma unroll
+              for (int q = 0; q < 4; ++q) {
+                av[2 * q] = (double)__builtin_bit_cast(_Float16, (uint16_t)(pw4[q] & 0xffffu));
+                av[2 * q + 1] = (double)__builtin_bit_cast(_Float16, (uint16_t)(pw4[q] >> 16));
+              }
+              av[8] = (double)val[plane * 576 + 512 + lane];
+#pragma unroll
+              for (int d = 0; d < BS; ++d)
+#pragma unroll
+                for (int e = 0; e < BS; ++e) {
+                  const double a = back ? av[e * BS + d] : av[d * BS + e];
+#pragma unroll
+                  for (int j = 0; j < K; ++j) sum[d * K + j] += a * xv[e * K + j];
+                }
+            } else {
+              static_assert(BB == 1, "1x1 or 3x3 blocks");
+              const double a = (double)val[plane * 64 + lane];
+#pragma unroll
+              for (int j = 0; j < K; ++j) sum[j] += a * xv[j];
+            }
+          }
+        }
+      } else {
+        kuhn_full();
+      }
+    } else {
+      kuhn_full();
     }
   } else {
     int64_t slot = s0;

@@ -50,8 +50,10 @@ class KuhnMultigrid:
         n = C.c_int()
         lm = (C.c_double * 16)()
         check(lib().cpm_mg_info(self.h, C.byref(n), lm, 16), "cpm_mg_info")
+        hs = (C.c_int * 16)()
+        check(lib().cpm_mg_forms(self.h, hs, 16), "cpm_mg_forms")
         return dict(levels=n.value, lmax=list(lm[:n.value]), dims=[getattr(L, "grid_dims", None) for L in self.levels],
-                    floating=self.floating)
+                    floating=self.floating, half_stencil=[bool(v) for v in hs[:n.value]])
 
     def set_graphs(self, on):
         """Replay the PCG iterations from captured hipGraph chunks (default)

@@ -261,6 +261,11 @@ int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, const doubl
 int cpm_mg_destroy(cpm_mg* mg);
 /* levels and the lambda_max estimate of each level (host, up to max_levels) */
 int cpm_mg_info(const cpm_mg* mg, int* nlevels, double* lmax, int max_levels);
+/* storage of each level's FP16 form (host, up to max_levels): 1 = half of a
+ * slot-symmetric Kuhn stencil (forward offsets only, the backward blocks read
+ * transposed from the neighbour rows), 0 = every slot, or no FP16 form.  No
+ * counterpart in the reference (its levels are CSR matrices). */
+int cpm_mg_forms(const cpm_mg* mg, int* half_stencil, int max_levels);
 /* z = the preconditioner applied to r: one V-cycle (floating hierarchy:
  * F^T V(F r) + e_0 r_0); device pointers, SELL vector order; synchronises */
 int cpm_mg_apply(cpm_mg* mg, const double* r, double* z);

@@ -120,6 +120,12 @@ def test_vcycle_matches_host_reference(name, bs, n, floating):
         info = mg.info()
         assert info["levels"] == len(sizes) and info["dims"][-1] == (sizes[-1],) * 3
         assert all(l > 0 for l in info["lmax"][:-1])
+        # FP16 forms of the (slot-symmetric) periodic Kuhn levels keep half
+        # of the stencil; the other precisions have no FP16 form
+        if prec == 16 and (floating or name == "A1"):
+            assert all(info["half_stencil"][:-1]), info["half_stencil"]
+        elif prec != 16:
+            assert not any(info["half_stencil"])
         mats = [_interleaved(L, bs) for L in mg.levels]
         dims = [(m, m, m) for m in sizes]
         from cupymag_amd.solvers.multigrid import _interleave_perm

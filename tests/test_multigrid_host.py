@@ -268,3 +268,30 @@ def test_device_transfer_rules_equal_the_host_prolongation(tmp_path_factory):
             Rr, Pe = got[:nc], got[nc:]
             np.testing.assert_allclose(Rr, P.T @ r, rtol=0, atol=1e-13 * np.abs(P.T @ r).max())
             np.testing.assert_allclose(Pe, P @ e, rtol=0, atol=1e-13 * np.abs(P @ e).max())
+
+
+def test_half_stored_kuhn_stencil_equals_the_full_form(tmp_path_factory):
+    """FP16 level forms of a symmetric Kuhn stencil: the half layout
+    (make_half_row of mg_dev.h: forward slots only, backward blocks read
+    transposed from the neighbour rows) gives the full layout's product bit
+    for bit, its compensated diagonal blocks agree to rounding, and both are
+    the FP64 product to FP16 precision (random slot-symmetric operators,
+    1x1 and 3x3 blocks, grids with padding rows and the smallest dims)."""
+    import os
+    import shutil
+    import subprocess
+    import pytest
+    hipcc = shutil.which("hipcc") or ("/opt/rocm/bin/hipcc" if os.path.exists("/opt/rocm/bin/hipcc") else None)
+    if hipcc is None:
+        pytest.skip("hipcc not available")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = str(tmp_path_factory.mktemp("kuhn_half") / "kuhn_half_host")
+    subprocess.run([hipcc, "--offload-host-only", "-O2", "-std=c++17", f"-I{root}/cupymag_amd/csrc",
+                    f"-I{root}/include", os.path.join(root, "tests", "kuhn_half_host.hip"), "-o", exe],
+                   check=True, capture_output=True)
+    for dims, bs in (((3, 3, 3), 3), ((5, 6, 7), 3), ((8, 8, 8), 1), ((4, 5, 64), 3), ((9, 3, 11), 1)):
+        out = subprocess.run([exe, *map(str, dims), str(bs), "7"], capture_output=True, check=True, text=True).stdout
+        dy, ym, dd, dm, e64, y64 = map(float, out.split())
+        assert dy == 0.0 and ym > 0.0, (dims, bs, dy)
+        assert dd <= 1e-14 * dm, (dims, bs, dd, dm)
+        assert 0.0 < e64 <= 1e-3 * y64, (dims, bs, e64, y64)
