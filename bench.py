@@ -60,12 +60,13 @@ def mode_kwargs(base, opts, mg_degree=2, mg_ratio=10.0):
                 mg_degree=mg_degree, mg_ratio=mg_ratio, mg_precision=mgp if mg else None)
 
 
-def kernel_bytes(base, mg, mgp, nnz, rows, stencil=True, half=False):
+def kernel_bytes(base, mg, mgp, nnz, rows, stencil=True, half=False, half32=False):
     """Algorithmic bytes per launch of a mode's roofline kernel and its name
     (rocprofv3 spelling).  nnz, rows: of the elasticity operator A_el (3x3
     blocks, nb = rows / 3 block rows, nnz / 9 blocks of which nb diagonal).
     half: the level-0 FP16 form keeps half of the symmetric stencil (each
-    off-diagonal block stored once, for the row and its mirror)."""
+    off-diagonal block stored once, for the row and its mirror); half32: the
+    FP32 form of A_el does."""
     nb = rows // 3
     off = nnz / 9.0 - nb
     if base == "fp64":
@@ -95,19 +96,22 @@ def kernel_bytes(base, mg, mgp, nnz, rows, stencil=True, half=False):
     # values, p read, q written
     s_ = 8 if base == "fast64" else 4
     idx = 0 if stencil else 4
-    form = "stencil" if stencil else "indexed"
+    hs = half32 and s_ == 4
+    form = ("half-stored symmetric stencil" if hs else "stencil") if stencil else "indexed"
     name = f"k_sell_spmv<3,1,{'double' if s_ == 8 else 'float'}> (A_el, SELL-64 3x3 blocks, {form})"
     roc = (f"k_sell_spmv<3, 1, {'double' if s_ == 8 else 'float'}, true, false, false, "
            f"{'true' if stencil else 'false'}>")
-    return name, roc, off * (9 * s_ + idx) + nb * 72.0 + 2.0 * rows * s_
+    return name, roc, (off / 2 if hs else off) * (9 * s_ + idx) + nb * 72.0 + 2.0 * rows * s_
 
 
-def outer_bytes(nnz, rows):
+def outer_bytes(nnz, rows, half32=False):
     """Algorithmic bytes of the mixed multigrid PCG's outer product q = A p
-    (k_pcg_spmv_rel, FP32 form): FP32 off-diagonal 3x3 blocks, FP64
+    (k_pcg_spmv_rel, FP32 form): FP32 off-diagonal 3x3 blocks (each stored
+    once when the FP32 form keeps half of the symmetric stencil), FP64
     compensated diagonal blocks, p read and q written in FP64."""
     nb = rows // 3
-    return (nnz / 9.0 - nb) * 36.0 + nb * 72.0 + 2.0 * rows * 8.0
+    off = nnz / 9.0 - nb
+    return (off / 2 if half32 else off) * 36.0 + nb * 72.0 + 2.0 * rows * 8.0
 
 
 def run_gpu(n, steps, warmup, steps64, modes, tol, ref_steps=REF_STEPS, rel_delta=0.3, autotune=True,
@@ -206,16 +210,18 @@ def run_gpu(n, steps, warmup, steps64, modes, tol, ref_steps=REF_STEPS, rel_delt
         info = A_el.sell_info()
         half = bool(mg and mgp == 16 and S.multigrid(precision=mgp, degree=mg_degree,
                                                       ratio=mg_ratio)["A_el"].info()["half_stencil"][0])
-        kname, rocname, bytes_launch = kernel_bytes(base, mg, mgp, nnz, rows, info["stencil"], half)
+        kname, rocname, bytes_launch = kernel_bytes(base, mg, mgp, nnz, rows, info["stencil"], half,
+                                                    info["half32"])
         avg_s = ms.value / 1e3 / max(nl.value, 1)
         d = dict(elapsed=elapsed, steps=k_timed, lat=lat, iters=iters, snaps=snaps,
                  spmv=dict(kernel=kname, rocname=rocname, launches=nl.value, avg_us=avg_s * 1e6,
                            bytes=bytes_launch, gbs=bytes_launch / avg_s / 1e9 if nl.value else 0.0))
         if nl2.value:
-            ob = outer_bytes(nnz, rows)
+            ob = outer_bytes(nnz, rows, info["half32"])
             avg2 = ms2.value / 1e3 / nl2.value
-            d["outer"] = dict(kernel="k_pcg_spmv_rel<3,1,true> (A_el level 0, outer PCG product, FP32 3x3 blocks, "
-                                     "FP64 compensated diagonal blocks and vectors)",
+            d["outer"] = dict(kernel="k_pcg_spmv_rel<3,1,true> (A_el level 0, outer PCG product, FP32 3x3 blocks"
+                                     + (", half-stored symmetric stencil" if info["half32"] else "")
+                                     + ", FP64 compensated diagonal blocks and vectors)",
                               rocname="k_pcg_spmv_rel<3, 1, true>", launches=nl2.value, avg_us=avg2 * 1e6,
                               bytes=ob, gbs=ob / avg2 / 1e9)
         out["modes"][mode] = d

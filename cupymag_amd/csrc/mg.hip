@@ -538,8 +538,10 @@ static void launch_sweep(hipStream_t s, int g, const Sell& S, const VT* val, con
                          const MgLevel& Lv,
                          const void* din, void* dout, double c1, double c2, double vscale, const CgState* st,
                          hipEvent_t* ev) {
+  // values keeping half of the stencil: the FP16 form (k_mg_make_half), or
+  // the level's own FP32 form
   GridArg ga = grid_arg(S);
-  if (sizeof(VT) == 2 && Lv.khalf) ga.kuhn = 2;
+  if (sizeof(VT) == 2 ? Lv.khalf != 0 : ((const void*)val == (const void*)S.v32 && S.half32)) ga.kuhn = 2;
   if (ev)
     hipExtLaunchKernelGGL(k_mg_sweep<BS, K, T, MODE, ST, VT>, dim3(g), dim3(kBlock), 0, s, ev[0], ev[1], 0,
                           (int32_t)S.nbrows, (const int64_t*)S.sptr, (const int32_t*)S.col, val, dg, ga,
@@ -788,13 +790,13 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
         hipExtLaunchKernelGGL(k_pcg_spmv_rel<BS, K, true>, dim3(gs), dim3(kBlock), 0, s, ev[0], ev[1], 0,
                               (int32_t)S.nbrows, (const int64_t*)S.sptr, (const int32_t*)S.col,
                               (const float*)S.v32, (const double*)S.dgc, (const double*)S.v64,
-                              (const double*)S.dg64, grid_arg(S), in, out, (const CgState*)st, ctx->partials);
+                              (const double*)S.dg64, CPM_GRID_ARG32(S), in, out, (const CgState*)st, ctx->partials);
       else if (S.stencil)
         k_pcg_spmv_rel<BS, K, true><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v32, S.dgc, S.v64, S.dg64,
-                                                          grid_arg(S), in, out, st, ctx->partials);
+                                                          CPM_GRID_ARG32(S), in, out, st, ctx->partials);
       else
         k_pcg_spmv_rel<BS, K, false><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v32, S.dgc, S.v64, S.dg64,
-                                                           grid_arg(S), in, out, st, ctx->partials);
+                                                           CPM_GRID_ARG32(S), in, out, st, ctx->partials);
       return;
     }
     if (plain) {

@@ -188,22 +188,6 @@ __device__ __forceinline__ void mg_sweep_rows(int64_t lb, int64_t lg, int32_t nb
   }
 }
 
-// Neighbour row of `row` at Kuhn slot t of a periodic grid (dims >= 3).
-__host__ __device__ __forceinline__ int64_t kuhn_neighbour(int64_t row, const GridArg& ga, int t) {
-  const int32_t code = kKuhnCodes[t];
-  const int32_t nyz = ga.ny * ga.nz;
-  int32_t i = (int32_t)(row / nyz) + (code & 3) - 1;
-  int32_t j = (int32_t)((row / ga.nz) % ga.ny) + ((code >> 2) & 3) - 1;
-  int32_t k = (int32_t)(row % ga.nz) + ((code >> 4) & 3) - 1;
-  i += (i < 0) ? ga.nx : 0;
-  i -= (i >= ga.nx) ? ga.nx : 0;
-  j += (j < 0) ? ga.ny : 0;
-  j -= (j >= ga.ny) ? ga.ny : 0;
-  k += (k < 0) ? ga.nz : 0;
-  k -= (k >= ga.nz) ? ga.nz : 0;
-  return ((int64_t)i * ga.ny + j) * ga.nz + k;
-}
-
 // Row `row` (< nrows_pad, padding rows included) of the FP16 half form of a
 // symmetric Kuhn stencil operator, read by sell_row with GridArg::kuhn = 2:
 // plane s*7 + u = fp16(v64 slot s*14 + 7 + u / vscale), a lane's 3x3 entries

@@ -264,7 +264,7 @@ static int multi_loop(cpm_ctx* ctx, MultiSolver& M, const std::vector<MultiSys>&
     d.v32 = A.v32;
     d.dg64 = A.dg64;
     d.dgc = A.dgc;
-    d.ga = CPM_GRID_ARG(A);
+    d.ga = CPM_GRID_ARG32(A);  // for v32; the FP64 products ignore the half-storage flag
     d.nbrows = A.nbrows;
     d.stencil = A.stencil;
     d.n = m.n;

@@ -412,7 +412,7 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
     OuterSeg o{};
     o.sptr = S.sptr, o.col = S.col, o.v64 = S.v64, o.dg64 = S.dg64, o.v32 = S.v32, o.dgc = S.dgc;
     o.delta = y.delta;
-    o.ga = CPM_GRID_ARG(S);
+    o.ga = CPM_GRID_ARG32(S);  // for v32; the FP64 products ignore the half-storage flag
     o.nb = S.nbrows, o.stencil = S.stencil, o.n = M->L[0].n * K;
     o.b = y.b, o.x = y.x, o.r = M->r, o.p = M->p, o.q = M->q, o.rold = M->rold, o.z = M->L[0].x;
     o.st = W.st + j * K;
@@ -463,7 +463,9 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
                                  : (const void*)(Lv.kdg64 ? Lv.kdg64 : S.dg64);
         g.vscale = M->half ? Lv.vscale : 1.0;
         g.ga = CPM_GRID_ARG(S);
-        if (M->half && Lv.khalf) g.ga.kuhn = 2;  // half of the stencil kept (k_mg_make_half)
+        // values keeping half of the stencil: the FP16 form (k_mg_make_half),
+        // or the level's own FP32 form
+        if (M->half ? Lv.khalf != 0 : (g.val == (const void*)S.v32 && S.half32)) g.ga.kuhn = 2;
         g.nb = S.nbrows, g.stencil = S.stencil;
         g.x = Lv.x, g.r = Lv.r, g.b = Lv.b, g.d0 = Lv.d0, g.d1 = Lv.d1, g.dinv = Lv.dinv;
         g.b64 = (d == 0) ? M->r : nullptr;

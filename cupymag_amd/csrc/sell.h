@@ -25,11 +25,20 @@ struct Sell {
   int32_t dims[3] = {0, 0, 0};
   int32_t* soff = nullptr;   // device, packed (di+1) | (dj+1) << 2 | (dk+1) << 4
   int kuhn = 0;              // the offsets are the Kuhn stencil in kKuhnCodes order (sell_dev.h)
+  // Kuhn stencil symmetric slot by slot (A(i, i-d) = A(i-d, i)^T to 1e-12 of
+  // its largest entry, every dim >= 3): v32 keeps the forward slots 7..13
+  // only, 7 planes per slice ([slice*7 + u][bs*bs][64]), read with
+  // GridArg::kuhn = 2 (CPM_GRID_ARG32)
+  int half32 = 0;
   int grid_cap = 8192;       // workgroup cap of the SpMV launches (tunable, grid-stride beyond)
 };
 
-// the grid argument of the kernels for a stencil-form operator
+// the grid argument of the kernels for a stencil-form operator (FP64 form)
 #define CPM_GRID_ARG(S) (GridArg{(S).dims[0], (S).dims[1], (S).dims[2], (S).nst, (S).soff, (S).kuhn})
+// ... for its FP32 form v32 (kuhn = 2 when v32 keeps half of the stencil;
+// the FP64 products of a kernel that reads both ignore the difference)
+#define CPM_GRID_ARG32(S) \
+  (GridArg{(S).dims[0], (S).dims[1], (S).dims[2], (S).nst, (S).soff, (S).kuhn ? ((S).half32 ? 2 : 1) : 0})
 
 void sell_free(Sell& S);
 int sell_grid(const Sell& A);

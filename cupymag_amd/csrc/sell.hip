@@ -170,8 +170,22 @@ int sell_build(cpm_ctx* ctx, Sell& S, int bs, int32_t nbrows, const int64_t* ind
     for (int a = 0; a < 3; ++a) S.dims[a] = dims[a];
   S.nslices = (int32_t)nsl;
   S.nslot = nslot;
-  std::vector<float> v32(val.size());
-  for (size_t i = 0; i < val.size(); ++i) v32[i] = (float)val[i];
+  // FP32 form: half of a slot-symmetric Kuhn stencil (every block once, the
+  // mirrored neighbour reads it transposed), else every slot
+  S.half32 = 0;
+  std::vector<float> v32;
+  if (S.kuhn) {  // a stencil form: dims given, each >= 3
+    const GridArg ga{dims[0], dims[1], dims[2], kKuhnSlots, nullptr, 1};
+    if (kuhn_slot_symmetric(bs, nbrows, ga, val.data())) {
+      S.half32 = 1;
+      v32.assign(nsl * 7 * bb * 64, 0.0f);
+      kuhn_half32(bs, nbrows, nsl, ga, val.data(), dg.data(), v32.data(), dgc.data());
+    }
+  }
+  if (!S.half32) {
+    v32.resize(val.size());
+    for (size_t i = 0; i < val.size(); ++i) v32[i] = (float)val[i];
+  }
   hipStream_t st = ctx->stream;
   CPM_HIP(hipMalloc(&S.sptr, sizeof(int64_t) * (nsl + 1)));
   if (stencil) {
@@ -180,7 +194,7 @@ int sell_build(cpm_ctx* ctx, Sell& S, int bs, int32_t nbrows, const int64_t* ind
   }
   CPM_HIP(hipMalloc(&S.col, sizeof(int32_t) * (stencil ? 1 : std::max<int64_t>(1, col.size()))));
   CPM_HIP(hipMalloc(&S.v64, sizeof(double) * std::max<size_t>(1, val.size())));
-  CPM_HIP(hipMalloc(&S.v32, sizeof(float) * std::max<size_t>(1, val.size())));
+  CPM_HIP(hipMalloc(&S.v32, sizeof(float) * std::max<size_t>(1, v32.size())));
   CPM_HIP(hipMalloc(&S.dg64, sizeof(double) * dg.size()));
   CPM_HIP(hipMalloc(&S.dgc, sizeof(double) * dgc.size()));
   CPM_HIP(hipMemcpyAsync(S.dg64, dg.data(), sizeof(double) * dg.size(), hipMemcpyHostToDevice, st));
@@ -285,7 +299,7 @@ static int spmv_t(cpm_ctx* ctx, const Sell& A, const T* x, const float* x32, T* 
     val = A.v32;
     dg = A.dgc;
   }
-  const GridArg ga = CPM_GRID_ARG(A);
+  const GridArg ga = sizeof(T) == 8 ? CPM_GRID_ARG(A) : CPM_GRID_ARG32(A);
   const dim3 grid(sell_grid(A)), block(kBlock);
   if (ev) {
     if (A.stencil)
@@ -527,7 +541,7 @@ extern "C" int cpm_mat_add_sell(cpm_mat* A, int bs, const int64_t* indptr, const
 extern "C" int cpm_mat_sell_info(const cpm_mat* A, int* bs, int* stencil, int64_t* slots) {
   if (!A) return cpm::fail(CPM_EINVAL, "cpm_mat_sell_info: null matrix");
   if (bs) *bs = A->sell.bs;
-  if (stencil) *stencil = A->sell.stencil ? (A->sell.kuhn ? 2 : 1) : 0;
+  if (stencil) *stencil = A->sell.stencil ? (A->sell.kuhn ? (A->sell.half32 ? 3 : 2) : 1) : 0;
   if (slots) *slots = A->sell.nslot;
   return CPM_OK;
 }

@@ -6,6 +6,7 @@
 // Partial sums are stored at partials[k * gridDim.x + blockIdx.x] in both
 // cases; the reducer of a system adds the entries of its own range.
 #pragma once
+#include <cmath>
 #include <type_traits>
 
 #include "device.h"
@@ -25,8 +26,9 @@ namespace cpm {
 // Periodic structured grid of an index-free (stencil) SELL operator.
 // kuhn: the slots are the 14 neighbours of a Kuhn mesh in the order of
 // kKuhnCodes (the host checks soff), so the kernels decode them at compile
-// time; 2: besides, an FP16 form passed with it keeps half of the stencil
-// (k_mg_make_half: the forward slots 7..13, 7 planes per slice).
+// time; 2: besides, the FP32 or FP16 values passed with it keep half of the
+// stencil (the forward slots 7..13, 7 planes per slice: Sell::half32,
+// k_mg_make_half); FP64 values always hold every slot.
 struct GridArg {
   int32_t nx, ny, nz, nst;
   const int32_t* soff;
@@ -39,6 +41,80 @@ struct GridArg {
 // SELL builder gives them).
 constexpr int kKuhnSlots = 14;
 constexpr int32_t kKuhnCodes[kKuhnSlots] = {0, 1, 4, 5, 16, 17, 20, 22, 25, 26, 37, 38, 41, 42};
+
+// Neighbour row of `row` at Kuhn slot t of a periodic grid (dims >= 3).
+__host__ __device__ __forceinline__ int64_t kuhn_neighbour(int64_t row, const GridArg& ga, int t) {
+  const int32_t code = kKuhnCodes[t];
+  const int32_t nyz = ga.ny * ga.nz;
+  int32_t i = (int32_t)(row / nyz) + (code & 3) - 1;
+  int32_t j = (int32_t)((row / ga.nz) % ga.ny) + ((code >> 2) & 3) - 1;
+  int32_t k = (int32_t)(row % ga.nz) + ((code >> 4) & 3) - 1;
+  i += (i < 0) ? ga.nx : 0;
+  i -= (i >= ga.nx) ? ga.nx : 0;
+  j += (j < 0) ? ga.ny : 0;
+  j -= (j >= ga.ny) ? ga.ny : 0;
+  k += (k < 0) ? ga.nz : 0;
+  k -= (k >= ga.nz) ? ga.nz : 0;
+  return ((int64_t)i * ga.ny + j) * ga.nz + k;
+}
+
+// Host, at build time, for a Kuhn stencil form with FP64 values val
+// ([slot][bs*bs][64], slot = slice*14 + t): whether A(i, i-d) = A(i-d, i)^T
+// for every row and backward slot, to 1e-12 of the largest entry.
+inline bool kuhn_slot_symmetric(int bs, int32_t nbrows, const GridArg& ga, const double* val) {
+  const int bb = bs * bs;
+  double amax = 0.0, dmax = 0.0;
+  for (int64_t b = 0; b < nbrows; ++b) {
+    const int64_t s = b >> 6, l = b & 63;
+    for (int t = 0; t < 7; ++t) {
+      const int64_t c = kuhn_neighbour(b, ga, t);
+      const int64_t cs = (c >> 6) * kKuhnSlots + (kKuhnSlots - 1 - t), lc = c & 63;
+      for (int d = 0; d < bs; ++d)
+        for (int e = 0; e < bs; ++e) {
+          const double a = val[((s * kKuhnSlots + t) * bb + d * bs + e) * 64 + l];
+          const double an = val[(cs * bb + e * bs + d) * 64 + lc];
+          dmax = dmax > std::fabs(a - an) ? dmax : std::fabs(a - an);
+          amax = amax > std::fabs(a) ? amax : std::fabs(a);
+        }
+    }
+  }
+  return dmax <= 1e-12 * amax;
+}
+
+// Host: the FP32 half form of a slot-symmetric Kuhn stencil operator,
+// v32[((slice*7 + u)*bs*bs + t)*64 + l] = fp32 of slot 7 + u (nsl*7*bs*bs*64
+// values, read with GridArg::kuhn = 2), and dgc = dg + the rounding of the
+// fourteen blocks as the FP32 product reads them (the row's forward blocks,
+// and for its backward slots the neighbours' forward blocks, transposed), so
+// that the row sums of the FP32 form equal the FP64 ones.
+inline void kuhn_half32(int bs, int32_t nbrows, int64_t nsl, const GridArg& ga, const double* val,
+                        const double* dg, float* v32, double* dgc) {
+  const int bb = bs * bs;
+  for (int64_t s = 0; s < nsl; ++s)
+    for (int u = 0; u < 7; ++u)
+      for (int t = 0; t < bb; ++t)
+        for (int l = 0; l < 64; ++l)
+          v32[((s * 7 + u) * bb + t) * 64 + l] = (float)val[((s * kKuhnSlots + 7 + u) * bb + t) * 64 + l];
+  for (int64_t b = 0; b < nbrows; ++b) {
+    const int64_t s = b >> 6, l = b & 63;
+    double comp[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int u = 0; u < 7; ++u)
+      for (int t = 0; t < bb; ++t) {
+        const double a = val[((s * kKuhnSlots + 7 + u) * bb + t) * 64 + l];
+        comp[t] += a - (double)(float)a;
+      }
+    for (int t = 0; t < 7; ++t) {
+      const int64_t c = kuhn_neighbour(b, ga, t);
+      const int64_t cs = (c >> 6) * kKuhnSlots + (kKuhnSlots - 1 - t), lc = c & 63;
+      for (int d = 0; d < bs; ++d)
+        for (int e = 0; e < bs; ++e) {
+          const double a = val[((s * kKuhnSlots + t) * bb + d * bs + e) * 64 + l];
+          comp[d * bs + e] += a - (double)(float)val[(cs * bb + e * bs + d) * 64 + lc];
+        }
+    }
+    for (int t = 0; t < bb; ++t) dgc[b * bb + t] = dg[b * bb + t] + comp[t];
+  }
+}
 
 // Diagonal blocks of a SELL form with off-diagonal values of type VT: FP64,
 // or FP32 for the FP16 multigrid level forms.
@@ -157,10 +233,10 @@ __host__ __device__ __forceinline__ void sell_row(int64_t row, int32_t nbrows, c
         }
       }
     };
-    if constexpr (sizeof(VT) == 2) {
+    if constexpr (sizeof(VT) <= 4) {
       if (ga.kuhn == 2) {
-        // half storage (FP16 forms of symmetric level operators): slice s
-        // holds planes s*7 + u with the forward blocks of slots 7 + u
+        // half storage (FP32 and FP16 forms of slot-symmetric operators):
+        // slice s holds planes s*7 + u with the forward blocks of slots 7 + u
         // (offsets +d); the backward block of slot t < 7 (offset -d) is
         // A(i, i-d) = A(i-d, i)^T, the forward block of slot 13 - t (plane
         // 6 - t) of the neighbour row, read transposed.  Plain (cached)
@@ -180,7 +256,19 @@ __host__ __device__ __forceinline__ void sell_row(int64_t row, int32_t nbrows, c
               xv[v] = (double)x[(int64_t)c * NV + v];
               if (ADDX32) xv[v] += (double)x32[(int64_t)c * NV + v];
             }
-            if constexpr (BB == 9) {
+            if constexpr (sizeof(VT) == 4) {
+              double av[BB];
+#pragma unroll
+              for (int q = 0; q < BB; ++q) av[q] = (double)val[(plane * BB + q) * 64 + lane];
+#pragma unroll
+              for (int d = 0; d < BS; ++d)
+#pragma unroll
+                for (int e = 0; e < BS; ++e) {
+                  const double a = back ? av[e * BS + d] : av[d * BS + e];
+#pragma unroll
+                  for (int j = 0; j < K; ++j) sum[d * K + j] += a * xv[e * K + j];
+                }
+            } else if constexpr (BB == 9) {
               double av[9];
               typedef uint32_t u32x4h __attribute__((ext_vector_type(4)));
               const u32x4h w4 = reinterpret_cast<const u32x4h*>(val + plane * 576)[lane];

@@ -1,7 +1,8 @@
-// Host build of the FP16 Kuhn stencil forms of the multigrid levels
-// (make_half_row of cupymag_amd/csrc/mg_dev.h and sell_row of sell_dev.h,
-// compiled for the host with hipcc --offload-host-only), for
-// tests/test_multigrid_host.py.
+// Host build of the half-stored Kuhn stencil forms: the FP16 forms of the
+// multigrid levels (make_half_row of cupymag_amd/csrc/mg_dev.h), the FP32
+// operator forms (kuhn_slot_symmetric and kuhn_half32 of sell_dev.h) and
+// the row products reading them (sell_row of sell_dev.h), compiled for the
+// host with hipcc --offload-host-only, for tests/test_multigrid_host.py.
 //
 // usage: kuhn_half_host nx ny nz bs seed
 // Builds a random operator on the periodic nx*ny*nz grid whose off-diagonal
@@ -10,6 +11,8 @@
 // (FP64, and FP16 in the 3x3 word layout) and in the half layout, and prints
 //   max |y_half - y_full|  max |y_full|  max |dgh_half - dgh_full|  max |dgh_full|
 //   max |y_full - y64|  max |y64|
+// and, for the FP32 forms (kuhn_half32 against every slot in FP32),
+//   max |y_half - y_full|  max |y_full|  max |dgc_half - dgc_full|  max |dgc_full|  symmetric  perturbed-symmetric
 // where y_* = the FP16 forms' products with one random vector (same FP32
 // diagonal blocks in both), y64 the FP64 product, and dgh_* the compensated
 // diagonal blocks (dg64 + the rounding of the row's fourteen blocks).
@@ -102,6 +105,45 @@ static int run(int nx, int ny, int nz, unsigned seed) {
     }
   }
   std::printf("%.17g %.17g %.17g %.17g %.17g %.17g\n", dy, ym, dd, dm, e64, y64m);
+  // FP32 forms: every slot (and its compensated diagonal blocks) against
+  // the half form of kuhn_half32, on the same operator
+  std::vector<float> f32(v64.size());
+  std::vector<double> dgc32(dg64);
+  for (int64_t row = 0; row < pad; ++row) {
+    const int64_t s = row >> 6, l = row & 63;
+    for (int q = 0; q < cpm::kKuhnSlots; ++q)
+      for (int t = 0; t < BB; ++t) {
+        const int64_t i = ((s * cpm::kKuhnSlots + q) * BB + t) * 64 + l;
+        f32[i] = (float)v64[i];
+        if (row < nb) dgc32[row * BB + t] += v64[i] - (double)f32[i];
+      }
+  }
+  std::vector<float> h32(nsl * 7 * BB * 64);
+  std::vector<double> dgh32((size_t)nb * BB);
+  const bool sym = cpm::kuhn_slot_symmetric(BS, nb, ga, v64.data());
+  cpm::kuhn_half32(BS, nb, nsl, ga, v64.data(), dg64.data(), h32.data(), dgh32.data());
+  double dy2 = 0, ym2 = 0, dd2 = 0, dm2 = 0;
+  for (int64_t row = 0; row < nb; ++row) {
+    double yf[BS], yh[BS], xo[BS];
+    cpm::GridArg gf = ga, gh = ga;
+    gh.kuhn = 2;
+    cpm::sell_row<BS, 1, double, false, true, float>(row, nb, nullptr, nullptr, f32.data(), dgc32.data(), x.data(),
+                                                     nullptr, gf, yf, xo);
+    cpm::sell_row<BS, 1, double, false, true, float>(row, nb, nullptr, nullptr, h32.data(), dgc32.data(), x.data(),
+                                                     nullptr, gh, yh, xo);
+    for (int d = 0; d < BS; ++d) {
+      dy2 = std::fmax(dy2, std::fabs(yh[d] - yf[d]));
+      ym2 = std::fmax(ym2, std::fabs(yf[d]));
+    }
+    for (int t = 0; t < BB; ++t) {
+      dd2 = std::fmax(dd2, std::fabs(dgh32[row * BB + t] - dgc32[row * BB + t]));
+      dm2 = std::fmax(dm2, std::fabs(dgc32[row * BB + t]));
+    }
+  }
+  // one backward entry off by 1e-9 of its size: no longer slot-symmetric
+  v64[((0 * cpm::kKuhnSlots + 3) * BB + 0) * 64 + 5] *= 1.0 + 1e-9;
+  const bool sym2 = cpm::kuhn_slot_symmetric(BS, nb, ga, v64.data());
+  std::printf("%.17g %.17g %.17g %.17g %d %d\n", dy2, ym2, dd2, dm2, (int)sym, (int)sym2);
   return 0;
 }
 

@@ -8,6 +8,8 @@ The operators themselves come from two independent assemblies (native
 host assembler vs the oracle's SciPy COO path), so these checks also cover
 the assembly.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -421,7 +423,10 @@ def test_stencil_and_indexed_sell_agree(grid):
     mesh with the neighbour offsets decoded at compile time, on the hex grid
     with the 26 offsets read per slot) and the indexed form give the same
     products; a mesh with permuted node numbering falls back to the indexed
-    form."""
+    form.  On the Kuhn mesh the FP32 form keeps half of the (slot-symmetric)
+    stencil: the mixed-precision CG on it and on the indexed form (every
+    slot in FP32) take about the same iterations (within 25%) to the same
+    solution."""
     from cupymag_amd._lib import check, lib, ptr
     from cupymag_amd.device import DeviceCSR
     cfg = config(grid, n=6, me=True, defect_params=1.5)
@@ -431,6 +436,7 @@ def test_stencil_and_indexed_sell_agree(grid):
     A1 = S.csr_GS()[0]
     assert A_el.sell_info()["stencil"] and A1.sell_info()["stencil"]
     assert A_el.sell_info()["kuhn"] == A1.sell_info()["kuhn"] == (grid == "Tet")
+    assert A_el.sell_info()["half32"] == A1.sell_info()["half32"] == (grid == "Tet")
     rng = np.random.default_rng(3)
     for A, bs, K in ((A_el, 3, 1), (A1, 1, 3)):
         ip, ix, v = A.to_arrays()
@@ -442,6 +448,17 @@ def test_stencil_and_indexed_sell_agree(grid):
         check(lib().cpm_spmv_sell(A.ctx.h, A.h, K, ptr(x), ptr(y1)))
         check(lib().cpm_spmv_sell(B.ctx.h, B.h, K, ptr(x), ptr(y2)))
         assert _rel(y1.cpu().numpy(), y2.cpu().numpy()) < 1e-14
+        sols = []
+        for M in (A, B):
+            xs = torch.zeros_like(x)
+            it, res = (C.c_int32 * K)(), (C.c_double * K)()
+            check(lib().cpm_cg_sell(M.ctx.h, M.h, K, ptr(x), ptr(xs), 0, 1e-10, 5000, 32, 0.3, it, res))
+            sols.append((list(it), xs.cpu().numpy()))
+        (it_a, x_a), (it_b, x_b) = sols
+        # the two forms sum in different orders, and the FP32 recurrence of
+        # the elasticity solve is sensitive to that rounding (Hex: 71 vs 78)
+        assert all(abs(p - q) <= max(2, 0.25 * q) for p, q in zip(it_a, it_b)), (it_a, it_b)
+        assert _rel(x_a, x_b) < 1e-8
 
 
 def test_unstructured_numbering_uses_indexed_sell():

@@ -271,7 +271,7 @@ def test_device_transfer_rules_equal_the_host_prolongation(tmp_path_factory):
 
 
 def test_half_stored_kuhn_stencil_equals_the_full_form(tmp_path_factory):
-    """FP16 level forms of a symmetric Kuhn stencil: the half layout
+    """FP16 level forms and FP32 operator forms of a symmetric Kuhn stencil: the half layout
     (make_half_row of mg_dev.h: forward slots only, backward blocks read
     transposed from the neighbour rows) gives the full layout's product bit
     for bit, its compensated diagonal blocks agree to rounding, and both are
@@ -291,7 +291,14 @@ def test_half_stored_kuhn_stencil_equals_the_full_form(tmp_path_factory):
                    check=True, capture_output=True)
     for dims, bs in (((3, 3, 3), 3), ((5, 6, 7), 3), ((8, 8, 8), 1), ((4, 5, 64), 3), ((9, 3, 11), 1)):
         out = subprocess.run([exe, *map(str, dims), str(bs), "7"], capture_output=True, check=True, text=True).stdout
-        dy, ym, dd, dm, e64, y64 = map(float, out.split())
+        l16, l32 = out.strip().split("\n")
+        dy, ym, dd, dm, e64, y64 = map(float, l16.split())
         assert dy == 0.0 and ym > 0.0, (dims, bs, dy)
         assert dd <= 1e-14 * dm, (dims, bs, dd, dm)
         assert 0.0 < e64 <= 1e-3 * y64, (dims, bs, e64, y64)
+        # FP32 forms (Sell::half32, kuhn_half32 of sell_dev.h): the same, and
+        # the symmetry check tells a perturbed operator apart
+        dy, ym, dd, dm, sym, sym2 = map(float, l32.split())
+        assert dy == 0.0 and ym > 0.0, (dims, bs, dy)
+        assert dd <= 1e-14 * dm, (dims, bs, dd, dm)
+        assert sym == 1.0 and sym2 == 0.0
