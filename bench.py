@@ -50,13 +50,15 @@ def log(*a):
 
 
 def mode_kwargs(base, opts, mg_degree=2, mg_ratio=10.0):
-    """GSPMSimulation options of a bench mode: '+warm' (warm start), '+mg'
+    """GSPMSimulation options of a bench mode: '+warm' (warm start), '+xwarm'
+    (warm start from the extrapolation of the last two solutions), '+mg'
     (multigrid for the elasticity, demag and Gauss-Seidel A1 solves; FP16
     level values in the mixed mode), '+mged' (multigrid for elasticity and
     demag only), '+mg32' (FP32 level values)."""
     mg = "mg" in opts or "mged" in opts or "mg32" in opts
     mgp = (32 if "mg32" in opts else 16) if base == "mixed" else 64
-    return dict(warm_start="warm" in opts, precond="mg" if mg else None, mg_gs="mged" not in opts,
+    warm = "extrapolate" if "xwarm" in opts else ("warm" in opts)
+    return dict(warm_start=warm, precond="mg" if mg else None, mg_gs="mged" not in opts,
                 mg_degree=mg_degree, mg_ratio=mg_ratio, mg_precision=mgp if mg else None)
 
 
@@ -562,9 +564,10 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-12, help="CG tolerance of every path (headline workload)")
     ap.add_argument("--ref-steps", type=int, default=REF_STEPS,
                     help="the fast paths step (untimed) to this many steps; errors against fast64 are taken there")
-    ap.add_argument("--modes", default="fp64,fast64,mixed,fast64+mg,mixed+mg",
+    ap.add_argument("--modes", default="fp64,fast64,mixed,fast64+mg,mixed+mg,mixed+mg+xwarm",
                     help="comma list of fp64, fast64, mixed; '+warm' appended to a fast mode starts every CG "
-                         "from the previous step's solution, '+mg' preconditions the elasticity, demag and "
+                         "from the previous step's solution, '+xwarm' from the linear extrapolation of the last "
+                         "two steps' solutions, '+mg' preconditions the elasticity, demag and "
                          "Gauss-Seidel (A1) solves with the geometric multigrid, '+mged' only the elasticity and "
                          "demag solves; the mixed mode's V-cycle stores the level operators' values in FP16, "
                          "'+mg32' in FP32")
@@ -667,6 +670,7 @@ def main():
               "mixed+mg": "f16 level-operator values and f32 vectors in the V-cycle, f32 outer products, "
                           "f64 PCG recurrence and residual replacements",
               "fast64+mg": "f64"}
+    dtypes["mixed+mg+xwarm"] = dtypes["mixed+mg"]
     out = {
         "metric": "LLG node-steps/s",
         "value": value,

@@ -48,8 +48,11 @@ class GSPMSimulation:
         ``rel_delta_gs`` sets it for the Gauss-Seidel solves alone: these
         converge fast, so fewer FP64 updates suffice).
         ``warm_start`` (fast modes): every CG starts from the previous step's
-        solution instead of zero; the reference does so only when use_init
-        is set, so results differ at the level of the CG tolerance.
+        solution instead of zero (True or 1), or from the linear
+        extrapolation in time of the last two steps' solutions, 2 x_n -
+        x_{n-1} ("extrapolate" or 2); the reference starts from the previous
+        solution only when use_init is set, so results differ at the level
+        of the CG tolerance.
         ``precond="mg"`` (fast modes, structured Kuhn meshes): the
         elasticity and demag solves run CG preconditioned by a geometric
         multigrid V-cycle (FP32 V-cycle in the mixed mode, FP64 in fast64;
@@ -126,7 +129,7 @@ class GSPMSimulation:
         pr.rel_delta_mg = float(rel_delta_mg or 0.0)
         if warm_start and mode == "fp64":
             raise ValueError("warm_start is a fast-mode option")
-        pr.warm_start = int(bool(warm_start))
+        pr.warm_start = 2 if warm_start in ("extrapolate", 2) else int(bool(warm_start))
         pr.precond = ((_lib.PRECOND_EL_DEMAG | (_lib.PRECOND_GS if mg_a1 else 0)) if precond == "mg" else 0)
         self.mode = mode
         W = self.Avg.weights_device(self.ctx)

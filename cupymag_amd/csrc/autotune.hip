@@ -48,7 +48,7 @@ __global__ void __launch_bounds__(kBlock) k_tune_sell(int32_t nbrows, const int6
                                                       const double* __restrict__ dg, const T* __restrict__ x,
                                                       T* __restrict__ y, double* partials, GridArg ga) {
   double acc[1] = {0.0};
-  sell_rows<BS, 1, T, true, false, ST>(blockIdx.x, gridDim.x, nbrows, sptr, col, val, dg, x, nullptr, y, ga, acc);
+  sell_rows<BS, 1, T, true, false, ST>(xcd_perm(blockIdx.x, gridDim.x, ga.xg), gridDim.x, nbrows, sptr, col, val, dg, x, nullptr, y, ga, acc);
   block_partial<1>(acc, partials);
 }
 

@@ -101,6 +101,31 @@ inline int grid_for(int64_t work_items, int items_per_block = kBlock) {
   return (int)g;
 }
 
+// Position w of a workgroup range of g workgroups, permuted so that the rows
+// a workgroup shares data with run on the same XCD.  The dispatcher deals
+// workgroups to the eight XCDs round-robin (workgroup w runs on XCD w % 8)
+// and every XCD has an L2 of its own.  With G > 1, each XCD takes G
+// consecutive positions of every run of 8 G (the tail of the range keeps
+// the dispatch order): when 8 G workgroups cover one plane of a periodic
+// grid, the previous grid line and the previous plane of a row (the
+// mirrored blocks of a half-stored stencil, the gathered vector entries)
+// were read by the same XCD a few workgroups earlier.  G < 0: each XCD one
+// contiguous run of the range.  A bijection of [0, g).  A workgroup's
+// partial sum stays at its dispatch position w, so launches that cover one
+// system with g workgroups, alone or as a segment of a batched launch, add
+// the same rows in the same order.
+__host__ __device__ __forceinline__ int64_t xcd_perm(int64_t w, int64_t g, int32_t G) {
+  constexpr int64_t kXcd = 8;
+  if (G < 0) {
+    const int64_t q = g / kXcd, rem = g % kXcd, x = w % kXcd;
+    return x * q + (x < rem ? x : rem) + w / kXcd;
+  }
+  if (G <= 1) return w;
+  const int64_t span = kXcd * G;
+  if (w >= (g / span) * span) return w;
+  return (w / span) * span + (w % kXcd) * G + (w / kXcd) % G;
+}
+
 int ensure_workspace(cpm_ctx* ctx, int64_t n);
 int read_cg_state(cpm_ctx* ctx, int slot, CgState* out);
 

@@ -507,6 +507,11 @@ struct cpm_sim {
   cpm_batch* batch = nullptr;
   int hint[10] = {};
   bool stepped = false;  // the warm-start buffers hold a previous solution
+  // warm_start 2: the solutions of the solve before the last, per solve
+  // (12 n: elasticity 3n, demag n, Gauss-Seidel 3n + n + n, m** 3n), and the
+  // solutions each solve has kept so far (0, 1, 2 or more)
+  double* xhist = nullptr;
+  int nwarm[10] = {};
   // hysteresis control (cpm_sim_control): on, host copy of the next step's use_init, field list
   int ctl_on = 0;
   int ctl_use_init = 0;
@@ -542,6 +547,8 @@ extern "C" int cpm_sim_create(cpm_ctx* ctx, const cpm_sim_ops* ops, const cpm_si
   if (n <= 0) return fail(CPM_EINVAL, "cpm_sim_create: n must be positive");
   if (prm->mode != CPM_MODE_FP64 && prm->mode != CPM_MODE_FAST64 && prm->mode != CPM_MODE_MIXED)
     return fail(CPM_EINVAL, "cpm_sim_create: unknown mode");
+  if (prm->warm_start < 0 || prm->warm_start > 2)
+    return fail(CPM_EINVAL, "cpm_sim_create: warm_start must be 0, 1 or 2");
   if (prm->mode != CPM_MODE_FP64) {
     for (const cpm_mat* A : {ops->A_demag, ops->A1, ops->A2})
       if (!A->sell.sptr || A->sell.bs != 1)
@@ -626,6 +633,7 @@ extern "C" int cpm_sim_create(cpm_ctx* ctx, const cpm_sim_ops* ops, const cpm_si
     A(&S->gnint, 3 * n);
     A(&S->mssint, 3 * n);
     if (prm->me) A(&S->uint, 3 * n);
+    if (prm->warm_start == 2) A(&S->xhist, 12 * n);
   }
   if (rc == CPM_OK) {
     hipError_t e = hipMalloc(&S->defect, std::max<int64_t>(n, 1));
@@ -661,6 +669,7 @@ extern "C" int cpm_sim_set_m(cpm_sim* S, const double* m, double* avg_m) {
   cpm_ctx* ctx = S->ctx;
   k_setm<<<grid_for(S->n), kBlock, 0, ctx->stream>>>(S->n, m, S->defect, S->W, S->vol, S->m, S->mprev, S->dscal,
                                                       ctx->partials, ctx->counter);
+  std::fill(S->nwarm, S->nwarm + 10, 0);  // a new trajectory: no history to extrapolate
   CPM_HIP(hipGetLastError());
   CPM_HIP(hipMemcpyAsync(S->hscal, S->dscal + S_AVGM, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   CPM_HIP(hipStreamSynchronize(ctx->stream));
@@ -912,6 +921,35 @@ static inline int use_x0_fast(const cpm_sim* S, int use_init) {
   return (use_init || (S->prm.warm_start && S->stepped)) ? 1 : 0;
 }
 
+// x = 2 x - xp and xp = x (the old x): the solutions of the last two steps
+// extrapolated linearly in time; first: xp = x only (one solution so far).
+__global__ void __launch_bounds__(kBlock) k_extrap(int64_t n, double* __restrict__ x, double* __restrict__ xp,
+                                                   int first) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const double xi = x[i];
+    if (!first) x[i] = 2.0 * xi - xp[i];
+    xp[i] = xi;
+  }
+}
+
+// warm_start 2: the initial guess of a fast-mode solve, formed in place in
+// its solution vector (which holds the last step's solution): the linear
+// extrapolation of the last two solutions once two are kept, the last one
+// before that.
+static void warm_guess(cpm_sim* S, const SolveSpec& sp) {
+  if (S->prm.warm_start != 2 || !S->xhist || !sp.A) return;
+  const int64_t n = S->n;
+  const int64_t len = (sp.K == 3 || sp.A == S->ops.A_el) ? 3 * n : n;
+  // offset of a solve's history (first report slot -> multiple of n)
+  static constexpr int64_t kOff[8] = {0, 3, 4, -1, -1, 7, 8, 9};
+  double* xx = sp.xint ? sp.xint : sp.x;
+  int& h = S->nwarm[sp.slot];
+  if (h >= 1)
+    k_extrap<<<grid_for(len), kBlock, 0, S->ctx->stream>>>(len, xx, S->xhist + kOff[sp.slot] * n, h == 1 ? 1 : 0);
+  h = std::min(h + 1, 2);
+}
+
 // The multigrid hierarchy a solve runs on, or null (plain CG): elasticity
 // and demag (CPM_PRECOND_EL_DEMAG), the Gauss-Seidel solves on A1
 // (CPM_PRECOND_GS).
@@ -956,6 +994,7 @@ static int solve_one(cpm_sim* S, const SolveSpec& sp, int use_init, cpm_step_rep
   int failed = 0;
   int hint = 0;
   for (int j = 0; j < sp.K; ++j) hint = std::max(hint, S->hint[sp.slot + j]);
+  warm_guess(S, sp);
   const int x0 = use_x0_fast(S, use_init);
   cpm_mg* mg = solve_mg(S, sp);
   const double dmg = (p.mode == CPM_MODE_MIXED) ? p.rel_delta_mg : 0.0;
@@ -1149,6 +1188,7 @@ extern "C" int cpm_batch_step(cpm_batch* B, const double* H, const int32_t* use_
       solved[j] = 0;
       if (int rc = stage_pre(S, stage, P[j], sp[j])) return rc;
       if (!sp[j].A) continue;
+      warm_guess(S, sp[j]);
       const cpm_sim_params& p = S->prm;
       cpm_mg* mg = solve_mg(S, sp[j]);
       if (mg) {

@@ -166,8 +166,14 @@ __global__ void k_mg_float_row0(int32_t nst, GridArg ga, double* __restrict__ v,
   for (int t = 0; t < BB; ++t) dg[t] = -dsum[t];
 }
 
+#ifndef CPM_SWEEP_WAVES
+#define CPM_SWEEP_WAVES 1
+#endif
+#ifndef CPM_OUTER_WAVES
+#define CPM_OUTER_WAVES 1
+#endif
 template <int BS, int K, class T, int MODE, bool ST, class VT>
-__global__ void __launch_bounds__(kBlock) k_mg_sweep(int32_t nb, const int64_t* __restrict__ sptr,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CPM_SWEEP_WAVES))) k_mg_sweep(int32_t nb, const int64_t* __restrict__ sptr,
                                                      const int32_t* __restrict__ col, const VT* __restrict__ val,
                                                      const sell_diag_t<VT>* __restrict__ dg, GridArg ga,
                                                      const T* __restrict__ din, T* __restrict__ dout,
@@ -175,7 +181,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_sweep(int32_t nb, const int64_t* 
                                                      const T* __restrict__ dinv, double c1, double c2,
                                                      double vscale, const CgState* st) {
   if (all_done<K>(st)) return;
-  mg_sweep_rows<BS, K, T, MODE, ST, VT>(blockIdx.x, gridDim.x, nb, sptr, col, val, dg, ga, din, dout, x, r, b, dinv,
+  mg_sweep_rows<BS, K, T, MODE, ST, VT>(xcd_perm(blockIdx.x, gridDim.x, ga.xg), gridDim.x, nb, sptr, col, val, dg, ga, din, dout, x, r, b, dinv,
                                         c1, c2, vscale);
 }
 
@@ -360,16 +366,19 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(int32_t nb, const int64_t* 
   double acc[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) acc[j] = 0.0;
-  sell_rows<BS, K, double, !PLAIN, false, ST, VT>(blockIdx.x, gridDim.x, nb, sptr, col, val, dg, p, nullptr, q, ga,
+  sell_rows<BS, K, double, !PLAIN, false, ST, VT>(xcd_perm(blockIdx.x, gridDim.x, ga.xg), gridDim.x, nb, sptr, col, val, dg, p, nullptr, q, ga,
                                                   acc);
   if (!PLAIN) block_partial<K>(acc, partials);
 }
 
 // q = A p of the mixed-precision PCG: the FP32 form with compensated FP64
 // diagonals, or the FP64 form once the solve has switched its products to
-// FP64 (CgState::p64); partials of p_j.q_j
+// FP64 (CgState::p64); partials of p_j.q_j.  The FP64 products walk the
+// stored stencil offsets two slots at a time (the same slots in the same
+// order as the compile-time Kuhn walk, so the same sums): the kernel's
+// registers, and so its occupancy, are those of the FP32 path.
 template <int BS, int K, bool ST>
-__global__ void __launch_bounds__(kBlock) k_pcg_spmv_rel(int32_t nb, const int64_t* __restrict__ sptr,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CPM_OUTER_WAVES))) k_pcg_spmv_rel(int32_t nb, const int64_t* __restrict__ sptr,
                                                          const int32_t* __restrict__ col, const float* __restrict__ v32,
                                                          const double* __restrict__ dgc,
                                                          const double* __restrict__ v64,
@@ -380,11 +389,13 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv_rel(int32_t nb, const int64
   double acc[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) acc[j] = 0.0;
-  if (any_p64<K>(st))
-    sell_rows<BS, K, double, true, false, ST>(blockIdx.x, gridDim.x, nb, sptr, col, v64, dg64, p, nullptr, q, ga,
-                                              acc);
-  else
-    sell_rows<BS, K, double, true, false, ST, float>(blockIdx.x, gridDim.x, nb, sptr, col, v32, dgc, p, nullptr, q,
+  if (any_p64<K>(st)) {
+    GridArg g64 = ga;
+    g64.kuhn = 0;
+    sell_rows<BS, K, double, true, false, ST, double, 2>(xcd_perm(blockIdx.x, gridDim.x, ga.xg), gridDim.x, nb, sptr,
+                                                         col, v64, dg64, p, nullptr, q, g64, acc);
+  } else
+    sell_rows<BS, K, double, true, false, ST, float>(xcd_perm(blockIdx.x, gridDim.x, ga.xg), gridDim.x, nb, sptr, col, v32, dgc, p, nullptr, q,
                                                      ga, acc);
   block_partial<K>(acc, partials);
 }
@@ -401,7 +412,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_true(int32_t nb, const int64_t* 
   double acc[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) acc[j] = 0.0;
-  pcg_true_rows<BS, K, ST>(blockIdx.x, gridDim.x, nb, sptr, col, val, dg, ga, x, b, r, st, acc);
+  pcg_true_rows<BS, K, ST>(xcd_perm(blockIdx.x, gridDim.x, ga.xg), gridDim.x, nb, sptr, col, val, dg, ga, x, b, r, st, acc);
   block_partial<K>(acc, partials);
 }
 
@@ -876,57 +887,63 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
     CPM_HIP(hipGetLastError());
     return CPM_OK;
   };
-  // the captured chunk: `chunk` iterations with this solve's vectors and
-  // states (the kernels read every scalar from the states, so a replay is
-  // the same work as launching them)
-  constexpr int chunk = 2;
-  hipGraphExec_t gexec = nullptr;
-  if (M->graphs) {
+  // the captured chunks: `ch` iterations (2, and 1 for the last iterations
+  // of a solve) with this solve's vectors and states (the kernels read every
+  // scalar from the states, so a replay is the same work as launching them)
+  hipGraphExec_t gexec[3] = {nullptr, nullptr, nullptr};
+  auto capture = [&](int ch) -> int {
     uint64_t key = 1469598103934665603ull;
     uint64_t dbits;
     std::memcpy(&dbits, &delta, sizeof dbits);
     for (uint64_t v : {(uint64_t)(uintptr_t)x, (uint64_t)(uintptr_t)b, (uint64_t)(uintptr_t)M->r,
                        (uint64_t)(uintptr_t)L0.x, (uint64_t)(uintptr_t)st, (uint64_t)(uintptr_t)ctx->partials,
-                       (uint64_t)K, (uint64_t)sizeof(T), (uint64_t)chunk, dbits})
+                       (uint64_t)K, (uint64_t)sizeof(T), (uint64_t)ch, dbits})
       key = (key ^ v) * 1099511628211ull;
     for (auto& c : M->chunks)
-      if (c.key == key) gexec = c.exec;
-    if (!gexec) {
-      if (!M->side) CPM_HIP(hipStreamCreateWithFlags(&M->side, hipStreamNonBlocking));
-      hipGraph_t g = nullptr;
-      CPM_HIP(hipStreamBeginCapture(M->side, hipStreamCaptureModeThreadLocal));
-      s = M->side;
-      M->ls = M->side;
-      const int rc = iterate(chunk, nullptr);
-      s = ctx->stream;
-      M->ls = nullptr;
-      hipError_t e = hipStreamEndCapture(M->side, &g);
-      if (rc) return rc;
-      if (e != hipSuccess) return fail(CPM_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
-      e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      if (e != hipSuccess) return fail(CPM_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e));
-      if (M->chunks.size() >= 4) {
-        (void)hipGraphExecDestroy(M->chunks.front().exec);
-        M->chunks.erase(M->chunks.begin());
-      }
-      M->chunks.push_back({key, gexec, chunk});
+      if (c.key == key) gexec[ch] = c.exec;
+    if (gexec[ch]) return CPM_OK;
+    if (!M->side) CPM_HIP(hipStreamCreateWithFlags(&M->side, hipStreamNonBlocking));
+    hipGraph_t g = nullptr;
+    CPM_HIP(hipStreamBeginCapture(M->side, hipStreamCaptureModeThreadLocal));
+    s = M->side;
+    M->ls = M->side;
+    const int rc = iterate(ch, nullptr);
+    s = ctx->stream;
+    M->ls = nullptr;
+    hipError_t e = hipStreamEndCapture(M->side, &g);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(CPM_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&gexec[ch], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(CPM_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    // a hierarchy serves up to three solves (the Gauss-Seidel columns, g1*,
+    // g2*), each with chunks of 2 and 1 iterations
+    if (M->chunks.size() >= 8) {
+      (void)hipGraphExecDestroy(M->chunks.front().exec);
+      M->chunks.erase(M->chunks.begin());
     }
-  }
-  auto issue = [&](int todo) -> int {
-    int done_direct = 0;
-    if (!gexec || prof) {
-      done_direct = gexec ? 1 : todo;
-      if (int rc = iterate(done_direct, prof ? &ctx->ev[0] : nullptr)) return rc;
-    }
-    if (gexec)
-      for (int k = done_direct; k < todo; k += chunk) CPM_HIP(hipGraphLaunch(gexec, s));
+    M->chunks.push_back({key, gexec[ch], ch});
     return CPM_OK;
   };
-  auto issued_for = [&](int todo) {
-    if (!gexec) return todo;
-    const int direct = prof ? 1 : 0;
-    return direct + ((todo - direct + chunk - 1) / chunk) * chunk;
+  if (M->graphs)
+    if (int rc = capture(2)) return rc;
+  // todo iterations: the first one launched directly when profiling (timed
+  // with events) or when there are no graphs, the rest from the chunks
+  auto issue = [&](int todo) -> int {
+    int done_direct = 0;
+    if (!gexec[2] || prof) {
+      done_direct = gexec[2] ? 1 : todo;
+      if (int rc = iterate(done_direct, prof ? &ctx->ev[0] : nullptr)) return rc;
+    }
+    if (!gexec[2]) return CPM_OK;
+    int k = done_direct;
+    for (; k + 2 <= todo; k += 2) CPM_HIP(hipGraphLaunch(gexec[2], s));
+    if (k < todo) {
+      if (!gexec[1])
+        if (int rc = capture(1)) return rc;
+      CPM_HIP(hipGraphLaunch(gexec[1], s));
+    }
+    return CPM_OK;
   };
   CgState h[K];
   auto read = [&]() -> int {
@@ -946,6 +963,10 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
     return m;
   };
   if (int rc = read()) return rc;
+  // iterations per host check: the previous solve's count less one, then
+  // one at a time up to two past it (an iteration launched after the solve
+  // has converged costs a V-cycle's worth of launches, a check one
+  // round trip), then a quarter of the count so far
   int batch = hint > 2 ? hint - 1 : 4;
   int issued = 0;
   cpm_mat* A0 = L0.A;
@@ -953,7 +974,7 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
     const int todo = std::max(1, std::min(std::min(batch, 1024), maxiter - issued + 1));
     const int it_before = h[0].it;
     if (int rc = issue(todo)) return rc;
-    issued += issued_for(todo);
+    issued += todo;
     if (int rc = read()) return rc;
     if (prof) {
       // the directly launched first iteration of the batch did V-cycle work
@@ -974,7 +995,8 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
         A0->prof2_launches += 1;
       }
     }
-    batch = std::max(2, std::min(16, it_max() / 4 + 2));
+    const int itm = it_max();
+    batch = (hint > 2 && itm < hint + 2) ? 1 : std::max(2, std::min(16, itm / 4 + 2));
   }
   *failed = 0;
   for (int j = 0; j < K; ++j) {

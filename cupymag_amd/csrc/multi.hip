@@ -85,7 +85,7 @@ __global__ void __launch_bounds__(kBlock) kb_spmv(const BSys* __restrict__ sys, 
   double acc[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) acc[j] = 0.0;
-  const int64_t lb = (int64_t)blockIdx.x - d.blk0;
+  const int64_t lb = xcd_perm((int64_t)blockIdx.x - d.blk0, d.nblk, d.ga.xg);
   if (d.stencil)
     sell_rows<BS, K, T, OP == OP_CG, OP == OP_REL, true>(lb, d.nblk, d.nbrows, d.sptr, d.col, val, dg, x, x32, y,
                                                           d.ga, acc);

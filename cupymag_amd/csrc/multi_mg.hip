@@ -107,7 +107,7 @@ __global__ void __launch_bounds__(kBlock) kbm_sweep(const SmoothSeg* __restrict_
   T* dout = dv[k & 1];
   const double c1 = (MODE == 0 || MODE == 4) ? d.c1[k] : 0.0;
   const double c2 = (MODE == 0 || MODE == 4) ? d.c2[k] : (MODE == 3) ? d.c0 : 0.0;
-  const int64_t lb = (int64_t)blockIdx.x - d.blk0;
+  const int64_t lb = xcd_perm((int64_t)blockIdx.x - d.blk0, d.nblk, d.ga.xg);
   if (d.stencil)
     mg_sweep_rows<BS, K, T, MODE, true, VT>(lb, d.nblk, d.nb, d.sptr, d.col, (const VT*)d.val,
                                             (const sell_diag_t<VT>*)d.dg, d.ga, din, dout,
@@ -155,7 +155,7 @@ __global__ void __launch_bounds__(kBlock) kbm_spmv(const OuterSeg* __restrict__ 
   double acc[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) acc[j] = 0.0;
-  const int64_t lb = (int64_t)blockIdx.x - d.sblk0;
+  const int64_t lb = xcd_perm((int64_t)blockIdx.x - d.sblk0, d.snblk, d.ga.xg);
   const double* in = PLAIN ? d.x : d.p;
   if (!PLAIN && d.delta > 0.0 && !any_p64<K>(d.st)) {
     if (d.stencil)
@@ -184,7 +184,7 @@ __global__ void __launch_bounds__(kBlock) kbm_true(const OuterSeg* __restrict__ 
   double acc[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) acc[j] = 0.0;
-  const int64_t lb = (int64_t)blockIdx.x - d.sblk0;
+  const int64_t lb = xcd_perm((int64_t)blockIdx.x - d.sblk0, d.snblk, d.ga.xg);
   if (d.stencil)
     pcg_true_rows<BS, K, true>(lb, d.snblk, d.nb, d.sptr, d.col, d.v64, d.dg64, d.ga, d.x, d.b, d.r, d.st, acc);
   else
@@ -585,6 +585,10 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
     return true;
   };
   if (int rc = read()) return rc;
+  // iterations per host check: the hint (the slowest member's previous
+  // count) less one, then one at a time up to two past it, then a quarter of
+  // the count so far (an iteration after the last live member has converged
+  // costs a V-cycle's worth of launches, a check one round trip)
   int batch = hint > 2 ? hint - 1 : 4;
   int issued = 0;
   while (!all_live_done() && issued <= maxit) {
@@ -595,7 +599,7 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
     int it_max = 0;
     for (int j = 0; j < nst; ++j)
       if (!W.hst[j].done) it_max = std::max(it_max, (int)W.hst[j].it);
-    batch = std::max(2, std::min(16, it_max / 4 + 2));
+    batch = (hint > 2 && it_max < hint + 2) ? 1 : std::max(2, std::min(16, it_max / 4 + 2));
   }
   for (int j = 0; j < ns; ++j) {
     failed[j] = 0;

@@ -31,15 +31,21 @@ struct Sell {
   // GridArg::kuhn = 2 (CPM_GRID_ARG32)
   int half32 = 0;
   int grid_cap = 8192;       // workgroup cap of the SpMV launches (tunable, grid-stride beyond)
+  int xg = 0;                // XCD group of the row launches of a stencil form (GridArg::xg)
+  int shz = 0, shy = 0;      // log2(dims[2]) + 1, log2(dims[1]) + 1 for powers of two, else 0 (GridArg)
 };
 
 // the grid argument of the kernels for a stencil-form operator (FP64 form)
-#define CPM_GRID_ARG(S) (GridArg{(S).dims[0], (S).dims[1], (S).dims[2], (S).nst, (S).soff, (S).kuhn})
+#define CPM_GRID_ARG(S) (GridArg{(S).dims[0], (S).dims[1], (S).dims[2], (S).nst, (S).soff, (S).kuhn, (S).xg, (S).shz, (S).shy})
 // ... for its FP32 form v32 (kuhn = 2 when v32 keeps half of the stencil;
 // the FP64 products of a kernel that reads both ignore the difference)
 #define CPM_GRID_ARG32(S) \
-  (GridArg{(S).dims[0], (S).dims[1], (S).dims[2], (S).nst, (S).soff, (S).kuhn ? ((S).half32 ? 2 : 1) : 0})
+  (GridArg{(S).dims[0], (S).dims[1], (S).dims[2], (S).nst, (S).soff, (S).kuhn ? ((S).half32 ? 2 : 1) : 0, (S).xg, (S).shz, (S).shy})
 
+// XCD group (xcd_perm) of the row launches of a stencil form on a grid of
+// these dims: the largest power of two G <= 64 for which 8 G workgroups of
+// kBlock rows tile one plane (ny nz rows) exactly, else 1 (dispatch order).
+int sell_xcd_group(const int32_t dims[3]);
 void sell_free(Sell& S);
 int sell_grid(const Sell& A);
 int sell_spmv64(cpm_ctx* ctx, const cpm_mat* M, int K, const double* x, double* y);

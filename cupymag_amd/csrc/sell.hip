@@ -168,6 +168,14 @@ int sell_build(cpm_ctx* ctx, Sell& S, int bs, int32_t nbrows, const int64_t* ind
   }
   if (stencil)
     for (int a = 0; a < 3; ++a) S.dims[a] = dims[a];
+  S.xg = stencil ? sell_xcd_group(dims) : 0;
+  auto shift = [](int32_t d) {
+    int k = 0;
+    while (k < 30 && (1 << k) < d) ++k;
+    return (d > 0 && (1 << k) == d) ? k + 1 : 0;
+  };
+  S.shz = stencil ? shift(dims[2]) : 0;
+  S.shy = stencil ? shift(dims[1]) : 0;
   S.nslices = (int32_t)nsl;
   S.nslot = nslot;
   // FP32 form: half of a slot-symmetric Kuhn stencil (every block once, the
@@ -231,7 +239,7 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(int32_t nbrows, const int6
   double acc[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) acc[j] = 0.0;
-  sell_rows<BS, K, T, CG, ADDX32, ST>(blockIdx.x, gridDim.x, nbrows, sptr, col, val, dg, x, x32, y, ga, acc);
+  sell_rows<BS, K, T, CG, ADDX32, ST>(xcd_perm(blockIdx.x, gridDim.x, ga.xg), gridDim.x, nbrows, sptr, col, val, dg, x, x32, y, ga, acc);
   if (CG) block_partial<K>(acc, partials);
 }
 
@@ -280,6 +288,14 @@ __global__ void __launch_bounds__(kBlock) k_rel_r(int64_t n, const double* __res
 
 // one thread per (block) row, up to A.grid_cap (<= kMaxSpmvGrid) workgroups
 // (3 partials each fit the partials buffer)
+int sell_xcd_group(const int32_t dims[3]) {
+  const int64_t plane = (int64_t)dims[1] * dims[2];
+  if (plane % kBlock) return 1;
+  int g = 1;
+  while (g < 64 && (plane / kBlock) % (8LL * 2 * g) == 0) g *= 2;
+  return g;
+}
+
 int sell_grid(const Sell& A) {
   int64_t g = ((((int64_t)A.nbrows + 63) & ~63) + kBlock - 1) / kBlock;
   return (int)std::max<int64_t>(1, std::min<int64_t>(g, A.grid_cap));

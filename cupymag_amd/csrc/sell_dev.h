@@ -33,6 +33,8 @@ struct GridArg {
   int32_t nx, ny, nz, nst;
   const int32_t* soff;
   int32_t kuhn;
+  int32_t xg;        // XCD group of the row launches (xcd_perm); 0 or 1: dispatch order
+  int32_t shz, shy;  // log2(nz) + 1 and log2(ny) + 1 when nz and ny are powers of two, else 0
 };
 
 // Slot codes (di+1) | (dj+1) << 2 | (dk+1) << 4 of the neighbours of a node
@@ -121,14 +123,19 @@ inline void kuhn_half32(int bs, int32_t nbrows, int64_t nsl, const GridArg& ga, 
 template <class VT>
 using sell_diag_t = typename std::conditional<sizeof(VT) == 2, float, double>::type;
 
-// sum = row `row` of A x (ADDX32: A (x + x32)), FP64 accumulation, for
+// sum = row `row` of A x (ADDX32: A (x + x32)) for
 // BS x BS blocks and K interleaved right-hand sides x[(c*BS + e)*K + j];
 // xo = the row's own entries of x (+ x32).  Padding rows (row >= nbrows)
 // only stream their (zero) slots and leave xo unset.  Off-diagonal values
 // of type VT (default T); for VT = _Float16 they are stored times a power of
 // two and their sum is scaled back by vscale, and the diagonal blocks are
-// FP32 (sell_diag_t).
-template <int BS, int K, class T, bool ADDX32, bool ST, class VT = T>
+// FP32 (sell_diag_t).  UF > 0: UF slots per trip through the stored offsets
+// of the stencil (GridArg::kuhn = 0) -- fewer loads in flight and fewer
+// registers, for product paths that rarely run inside a kernel whose
+// occupancy is set by its main path.  The off-diagonal products are added
+// in FP64, or in FP32 with ACC32 (the FP32 V-cycle sweeps, whose vectors
+// are FP32); the diagonal block's product in FP64.
+template <int BS, int K, class T, bool ADDX32, bool ST, class VT = T, int UF = 0, bool ACC32 = false>
 __host__ __device__ __forceinline__ void sell_row(int64_t row, int32_t nbrows, const int64_t* __restrict__ sptr,
                                          const int32_t* __restrict__ col, const VT* __restrict__ val,
                                          const sell_diag_t<VT>* __restrict__ dg, const T* __restrict__ x,
@@ -141,27 +148,37 @@ __host__ __device__ __forceinline__ void sell_row(int64_t row, int32_t nbrows, c
   const int64_t s1 = ST ? s0 + ga.nst : sptr[s + 1];
   int32_t ri = 0, rj = 0, rk = 0;
   if (ST && row < nbrows) {
-    const int32_t r = (int32_t)row, nyz = ga.ny * ga.nz;
-    ri = r / nyz;
-    rj = (r / ga.nz) % ga.ny;
-    rk = r % ga.nz;
+    const int32_t r = (int32_t)row;
+    if (ga.shz && ga.shy) {
+      const int32_t lz = ga.shz - 1, ly = ga.shy - 1;
+      ri = r >> (lz + ly);
+      rj = (r >> lz) & ((1 << ly) - 1);
+      rk = r & ((1 << lz) - 1);
+    } else {
+      const int32_t nyz = ga.ny * ga.nz;
+      ri = r / nyz;
+      rj = (r / ga.nz) % ga.ny;
+      rk = r % ga.nz;
+    }
   }
+  using AT = typename std::conditional<ACC32, float, double>::type;
+  AT acc[NV];
 #pragma unroll
-  for (int v = 0; v < NV; ++v) sum[v] = 0.0;
+  for (int v = 0; v < NV; ++v) acc[v] = (AT)0;
   // matrix planes are read once per launch: streaming (non-temporal) loads
   // keep them from evicting the gathered vector entries; two slots per
   // trip keep more loads in flight
   auto slot_fma_at = [&](int64_t slot, int64_t c) {
-    double xv[NV];
+    AT xv[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
-      xv[v] = (double)x[c * NV + v];
-      if (ADDX32) xv[v] += (double)x32[c * NV + v];
+      xv[v] = (AT)x[c * NV + v];
+      if (ADDX32) xv[v] += (AT)x32[c * NV + v];
     }
     if constexpr (sizeof(VT) == 2 && BB == 9) {
       // FP16 3x3 blocks: entries 0..7 of a lane in one 16-byte word (one
       // 128-bit load), entry 8 on its own (2 loads per slot instead of 9)
-      double av[9];
+      AT av[9];
       typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
       const u32x4* w = reinterpret_cast<const u32x4*>(val + slot * 576);
       const u32x4 w4 = __builtin_nontemporal_load(&w[l]);
@@ -169,24 +186,24 @@ __host__ __device__ __forceinline__ void sell_row(int64_t row, int32_t nbrows, c
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const uint32_t pw = pw4[q];
-        av[2 * q] = (double)__builtin_bit_cast(_Float16, (uint16_t)(pw & 0xffffu));
-        av[2 * q + 1] = (double)__builtin_bit_cast(_Float16, (uint16_t)(pw >> 16));
+        av[2 * q] = (AT)__builtin_bit_cast(_Float16, (uint16_t)(pw & 0xffffu));
+        av[2 * q + 1] = (AT)__builtin_bit_cast(_Float16, (uint16_t)(pw >> 16));
       }
-      av[8] = (double)__builtin_nontemporal_load(&val[slot * 576 + 512 + l]);
+      av[8] = (AT)__builtin_nontemporal_load(&val[slot * 576 + 512 + l]);
 #pragma unroll
       for (int d = 0; d < BS; ++d)
 #pragma unroll
         for (int e = 0; e < BS; ++e)
 #pragma unroll
-          for (int j = 0; j < K; ++j) sum[d * K + j] += av[d * BS + e] * xv[e * K + j];
+          for (int j = 0; j < K; ++j) acc[d * K + j] += av[d * BS + e] * xv[e * K + j];
     } else {
 #pragma unroll
       for (int d = 0; d < BS; ++d)
 #pragma unroll
         for (int e = 0; e < BS; ++e) {
-          const double a = (double)__builtin_nontemporal_load(&val[(slot * BB + d * BS + e) * 64 + l]);
+          const AT a = (AT)__builtin_nontemporal_load(&val[(slot * BB + d * BS + e) * 64 + l]);
 #pragma unroll
-          for (int j = 0; j < K; ++j) sum[d * K + j] += a * xv[e * K + j];
+          for (int j = 0; j < K; ++j) acc[d * K + j] += a * xv[e * K + j];
         }
     }
   };
@@ -211,7 +228,10 @@ __host__ __device__ __forceinline__ void sell_row(int64_t row, int32_t nbrows, c
   // slots per trip: loads of U slots in flight before their FMAs (7 = half
   // the 14 neighbours of a Kuhn row; a sweep of 2..7 at 128^3 favoured 7 for
   // every value type, most for FP16)
-  constexpr int U = (sizeof(VT) == 2) ? SELL_HALF_UNROLL : (sizeof(VT) == 4) ? SELL_FLOAT_UNROLL : SELL_DOUBLE_UNROLL;
+  constexpr int U = UF > 0                ? UF
+                    : (sizeof(VT) == 2) ? SELL_HALF_UNROLL
+                    : (sizeof(VT) == 4) ? SELL_FLOAT_UNROLL
+                                        : SELL_DOUBLE_UNROLL;
   if (ST && ga.kuhn) {
     // Kuhn stencil: slot k's offset is known at compile time, so its
     // neighbour is the row plus one wrap-aware step per dimension (computed
@@ -250,48 +270,48 @@ __host__ __device__ __forceinline__ void sell_row(int64_t row, int32_t nbrows, c
             const bool back = t == 0;
             const int64_t plane = back ? (int64_t)(c >> 6) * 7 + (6 - u) : s * 7 + u;
             const int64_t lane = back ? (int64_t)(c & 63) : l;
-            double xv[NV];
+            AT xv[NV];
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
-              xv[v] = (double)x[(int64_t)c * NV + v];
-              if (ADDX32) xv[v] += (double)x32[(int64_t)c * NV + v];
+              xv[v] = (AT)x[(int64_t)c * NV + v];
+              if (ADDX32) xv[v] += (AT)x32[(int64_t)c * NV + v];
             }
             if constexpr (sizeof(VT) == 4) {
-              double av[BB];
+              AT av[BB];
 #pragma unroll
-              for (int q = 0; q < BB; ++q) av[q] = (double)val[(plane * BB + q) * 64 + lane];
+              for (int q = 0; q < BB; ++q) av[q] = (AT)val[(plane * BB + q) * 64 + lane];
 #pragma unroll
               for (int d = 0; d < BS; ++d)
 #pragma unroll
                 for (int e = 0; e < BS; ++e) {
-                  const double a = back ? av[e * BS + d] : av[d * BS + e];
+                  const AT a = back ? av[e * BS + d] : av[d * BS + e];
 #pragma unroll
-                  for (int j = 0; j < K; ++j) sum[d * K + j] += a * xv[e * K + j];
+                  for (int j = 0; j < K; ++j) acc[d * K + j] += a * xv[e * K + j];
                 }
             } else if constexpr (BB == 9) {
-              double av[9];
+              AT av[9];
               typedef uint32_t u32x4h __attribute__((ext_vector_type(4)));
               const u32x4h w4 = reinterpret_cast<const u32x4h*>(val + plane * 576)[lane];
               const uint32_t pw4[4] = {w4.x, w4.y, w4.z, w4.w};
 #pragma unroll
               for (int q = 0; q < 4; ++q) {
-                av[2 * q] = (double)__builtin_bit_cast(_Float16, (uint16_t)(pw4[q] & 0xffffu));
-                av[2 * q + 1] = (double)__builtin_bit_cast(_Float16, (uint16_t)(pw4[q] >> 16));
+                av[2 * q] = (AT)__builtin_bit_cast(_Float16, (uint16_t)(pw4[q] & 0xffffu));
+                av[2 * q + 1] = (AT)__builtin_bit_cast(_Float16, (uint16_t)(pw4[q] >> 16));
               }
-              av[8] = (double)val[plane * 576 + 512 + lane];
+              av[8] = (AT)val[plane * 576 + 512 + lane];
 #pragma unroll
               for (int d = 0; d < BS; ++d)
 #pragma unroll
                 for (int e = 0; e < BS; ++e) {
-                  const double a = back ? av[e * BS + d] : av[d * BS + e];
+                  const AT a = back ? av[e * BS + d] : av[d * BS + e];
 #pragma unroll
-                  for (int j = 0; j < K; ++j) sum[d * K + j] += a * xv[e * K + j];
+                  for (int j = 0; j < K; ++j) acc[d * K + j] += a * xv[e * K + j];
                 }
             } else {
               static_assert(BB == 1, "1x1 or 3x3 blocks");
-              const double a = (double)val[plane * 64 + lane];
+              const AT a = (AT)val[plane * 64 + lane];
 #pragma unroll
-              for (int j = 0; j < K; ++j) sum[j] += a * xv[j];
+              for (int j = 0; j < K; ++j) acc[j] += a * xv[j];
             }
           }
         }
@@ -309,10 +329,8 @@ __host__ __device__ __forceinline__ void sell_row(int64_t row, int32_t nbrows, c
     }
     for (; slot < s1; ++slot) slot_fma(slot);
   }
-  if constexpr (sizeof(VT) == 2) {
 #pragma unroll
-    for (int v = 0; v < NV; ++v) sum[v] *= vscale;
-  }
+  for (int v = 0; v < NV; ++v) sum[v] = (sizeof(VT) == 2) ? (double)acc[v] * vscale : (double)acc[v];
   if (row < nbrows) {
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
@@ -330,19 +348,27 @@ __host__ __device__ __forceinline__ void sell_row(int64_t row, int32_t nbrows, c
   }
 }
 
+// Calls f(row) for the rows of a workgroup range, padding rows included:
+// workgroup position lb takes rows [256 lb, 256 lb + 256), then every lg
+// workgroups further on.
+template <class F>
+__device__ __forceinline__ void for_rows(int64_t lb, int64_t lg, int32_t nbrows, F&& f) {
+  const int64_t stride = lg * kBlock;
+  for (int64_t row = lb * kBlock + threadIdx.x; row < (int64_t)((nbrows + 63) & ~63); row += stride) f(row);
+}
+
 // y = A x (ADDX32: A (x + x32)) over the block rows of a workgroup range; CG:
 // acc[j] += x.y of column j over the rows this thread owns.
-template <int BS, int K, class T, bool CG, bool ADDX32, bool ST, class VT = T>
+template <int BS, int K, class T, bool CG, bool ADDX32, bool ST, class VT = T, int UF = 0>
 __device__ __forceinline__ void sell_rows(int64_t lb, int64_t lg, int32_t nbrows, const int64_t* __restrict__ sptr,
                                           const int32_t* __restrict__ col, const VT* __restrict__ val,
                                           const sell_diag_t<VT>* __restrict__ dg, const T* __restrict__ x,
                                           const float* __restrict__ x32, T* __restrict__ y, const GridArg& ga,
                                           double (&acc)[K]) {
   constexpr int NV = BS * K;
-  const int64_t stride = lg * kBlock;
-  for (int64_t row = lb * kBlock + threadIdx.x; row < (int64_t)((nbrows + 63) & ~63); row += stride) {
+  for_rows(lb, lg, nbrows, [&](int64_t row) {
     double sum[NV], xo[NV];
-    sell_row<BS, K, T, ADDX32, ST, VT>(row, nbrows, sptr, col, val, dg, x, x32, ga, sum, xo);
+    sell_row<BS, K, T, ADDX32, ST, VT, UF>(row, nbrows, sptr, col, val, dg, x, x32, ga, sum, xo);
     if (row < nbrows) {
 #pragma unroll
       for (int v = 0; v < NV; ++v) {
@@ -351,7 +377,7 @@ __device__ __forceinline__ void sell_rows(int64_t lb, int64_t lg, int32_t nbrows
         if (CG) acc[v % K] += xo[v] * (double)yv;
       }
     }
-  }
+  });
 }
 
 // p = r (first iteration) or r + beta p, on the live columns.

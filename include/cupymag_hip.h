@@ -373,8 +373,10 @@ typedef struct cpm_sim_params {
   double Mv[36];              /* rot_111_matrices.get_M_matrix (row-major) */
   double MvinvT[36];          /* inv(Mv)^T (Micromagnetics.py:54) */
   int32_t mode;               /* CPM_MODE_* */
-  int32_t warm_start;         /* fast modes: start every CG from the previous step's solution
-                                 (the reference does so only when use_init is set) */
+  int32_t warm_start;         /* fast modes: 1 = start every CG from the previous step's solution
+                                 (the reference does so only when use_init is set); 2 = from the
+                                 linear extrapolation in time of the last two steps' solutions
+                                 (2 x_n - x_{n-1}; 12 n doubles more device memory) */
   double rel_delta;           /* CPM_MODE_MIXED: FP64 residual update when the FP32 recurrence
                                  residual drops by this factor (reliable updates) */
   int32_t precond;            /* fast modes: CPM_PRECOND_* bits; the solves named run multigrid-

@@ -89,20 +89,22 @@ def test_batch_matches_oracle(mode):
             assert abs(rep.sumsq - st["sumsq"]) <= 1e-6 * max(st["sumsq"], 1e-30) + 1e-20
 
 
-@pytest.mark.parametrize("mode", ["fast64", "mixed"])
+@pytest.mark.parametrize("mode", ["fast64", "mixed", "mixed+xwarm"])
 def test_batch_member_results_independent_of_the_batch(mode):
     """Solo runs vs one continuously changing batch: member 2 joins after
     the first step, member 0 leaves after the second, member 1 stays to the
     end; every member's state after each of its steps equals its solo run
-    bit for bit."""
+    bit for bit (also with initial guesses extrapolated from each member's
+    own last two solutions)."""
     from cupymag_amd.core.batch import GSPMBatch
     from cupymag_amd.core.gspm import GSPMSimulation
     built = [_member(c) for c in MEMBERS[:3]]
     nsteps = [2, 4, 3]
+    base, _, opt = mode.partition("+")
 
     def fresh(k):
         P, S, _ = built[k]
-        sim = GSPMSimulation(S, P, mode=mode)
+        sim = GSPMSimulation(S, P, mode=base, warm_start="extrapolate" if opt == "xwarm" else False)
         sim.set_m(tilted_m(S.n_dof, seed=30 + k))
         return sim
 

@@ -11,7 +11,8 @@ GPU.
   256 x those bounds, CG iterations +-1; then 100 steps of the mixed and
   mixed+multigrid paths against plain FP64, every field within 1e-7.
 * 128^3 Kuhn mesh, rotated ellipsoid, rot111, dt 10 ps, alpha 0.1: 50
-  steps of mixed+multigrid against fast64 (FP64 CG with the reference's
+  steps of mixed+multigrid, cold-started and started from the extrapolated
+  last two solutions, against fast64 (FP64 CG with the reference's
   recurrence, itself checked against plain FP64 after two steps), m within
   1e-7, at CG tolerance 1e-12 (see workloads.headline_config for why).
 * 64 simulations, Kuhn n^3 with n log-uniform in [16, 96], autotuned SpMV
@@ -130,18 +131,21 @@ def test_cfg3_mixed_multigrid_within_1e7_of_fp64_after_50_steps():
     assert float((a.get("M") - b.get("M")).abs().max()) <= 1e-12
     del a
     ref = GSPMSimulation(S, P, mode="fast64")
-    fast = GSPMSimulation(S, P, mode="mixed", precond="mg")
-    for s in (ref, fast):
+    fast = {"mixed+mg": GSPMSimulation(S, P, mode="mixed", precond="mg"),
+            "mixed+mg+xwarm": GSPMSimulation(S, P, mode="mixed", precond="mg", warm_start="extrapolate")}
+    for s in (ref, *fast.values()):
         s.set_m(m0)
     m_prev = torch.as_tensor(m0, device="cuda")
     for step in range(1, 51):
         ref.step(H, False)
-        rep = fast.step(H, False)
-        assert max(rep.iters[:7]) < 60, rep.iters   # multigrid solves stay short at this tolerance
+        for name, sim in fast.items():
+            rep = sim.step(H, False)
+            assert max(rep.iters[:7]) < 60, (name, rep.iters)   # multigrid solves stay short at this tolerance
         if step % 10 == 0:
-            err = float((fast.get("M") - ref.get("M")).abs().max())
-            print(f"[cfg3] step {step}: max |m_mixed+mg - m_fast64| = {err:.3e}", flush=True)
-            assert err <= 1e-7, (step, err)
+            for name, sim in fast.items():
+                err = float((sim.get("M") - ref.get("M")).abs().max())
+                print(f"[cfg3] step {step}: max |m_{name} - m_fast64| = {err:.3e}", flush=True)
+                assert err <= 1e-7, (name, step, err)
     moved = float((ref.get("M") - m_prev).abs().max())
     assert moved > 0.5    # the state really evolves (m reverses in places)
 

@@ -316,14 +316,16 @@ def test_sell_cg_matches_oracle(which):
         assert _rel(x32.cpu().numpy(), _interleave([xr[:n], xr[n:2 * n], xr[2 * n:]])) < 1e-8
 
 
-@pytest.mark.parametrize("mode", ["fast64", "mixed", "mixed+warm"])
+@pytest.mark.parametrize("mode", ["fast64", "mixed", "mixed+warm", "mixed+xwarm"])
 @pytest.mark.parametrize("case", [dict(grid="Tet", n=6, me=True, rot=True, dt=5e-12),
                                   dict(grid="Hex", n=6, me=True, rot=False, sigma11=1e8)],
                          ids=["tet-me-rot", "hex-me"])
 def test_fast_modes_match_oracle(mode, case):
     """fast64: same tolerances as the plain FP64 path (m 1e-10, CG iterations
-    +-1).  mixed (and mixed with warm starts): magnetization within 1e-7 of
-    the FP64 oracle."""
+    +-1).  mixed (and mixed with warm starts from the last solution or from
+    the extrapolation of the last two): magnetization within 1e-7 of the
+    FP64 oracle over four steps (the extrapolation first applies at the
+    third)."""
     from cupymag_amd.core.gspm import GSPMSimulation
     from oracle import gspm as og
     case = dict(case)
@@ -334,12 +336,12 @@ def test_fast_modes_match_oracle(mode, case):
     p, ops, Avg, _ = oracle_setup(cfg, arr)
     m0 = tilted_m(ops["n"], seed=12)
     base, _, opt = mode.partition("+")
-    sim = GSPMSimulation(S, P, mode=base, warm_start=(opt == "warm"))
+    sim = GSPMSimulation(S, P, mode=base, warm_start={"warm": True, "xwarm": "extrapolate"}.get(opt, False))
     sim.set_m(m0)
     st = og.init_state(ops, Avg, m0)
     p["H_now"] = p["H"]
     H = [P.Hext1, P.Hext2, P.Hext3]
-    for k in range(3):
+    for k in range(4):
         log = []
         ui = st["use_init"]
         og.step(st, ops, Avg, p, log)

@@ -408,3 +408,22 @@ def test_host_assembler_under_address_and_ub_sanitizers(_asan_assembler, grid, e
             continue
         np.testing.assert_array_equal(got, np.asarray(H.array(name)), err_msg=name)
     assert pos < len(out)  # the element matrices follow
+
+
+def test_xcd_order_of_the_stencil_launches_is_a_permutation(tmp_path):
+    """The XCD order of the workgroups of the stencil launches (device.h:
+    xcd_perm) is a bijection of the workgroup positions for every grid size
+    and group, and gives every XCD consecutive workgroups."""
+    import shutil
+    import subprocess
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    root = Path(__file__).resolve().parent.parent
+    exe = str(tmp_path / "row_order_host")
+    subprocess.run([hipcc, "--offload-host-only", "-O2", "-std=c++17", f"-I{root}/cupymag_amd/csrc",
+                    f"-I{root}/include", str(root / "tests" / "row_order_host.hip"), "-o", exe],
+                   check=True, capture_output=True)
+    out = subprocess.run([exe, "300"], check=True, capture_output=True, text=True).stdout.split("\n")
+    perms = [line.split() for line in out if line]
+    assert len(perms) == 300 * 8
+    assert all(p[3] == "1" for p in perms)
+    assert all(p[4] == "1" for p in perms if int(p[2]) > 1 or int(p[2]) < 0)
