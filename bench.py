@@ -117,7 +117,7 @@ def outer_bytes(nnz, rows, half32=False):
 
 
 def run_gpu(n, steps, warmup, steps64, modes, tol, ref_steps=REF_STEPS, rel_delta=0.3, autotune=True,
-            rel_delta_gs=None, mg_degree=2, mg_ratio=10.0):
+            rel_delta_gs=None, mg_degree=2, mg_ratio=10.0, matrix_free_deriv=False):
     """Run the plain FP64 path and the fast paths on the same operators and
     initial state; returns per-mode timings, CG iteration counts, the timed
     kernels and the magnetization after the steps the errors are taken at:
@@ -138,7 +138,8 @@ def run_gpu(n, steps, warmup, steps64, modes, tol, ref_steps=REF_STEPS, rel_delt
     P = from_dict(cfg)
     t0 = time.time()
     arrays = kuhn_mesh(n)
-    S = SimulationOperators(P, mesh_arrays=arrays, keep_host=False, autotune=autotune, matrix_free_fel=True)
+    S = SimulationOperators(P, mesh_arrays=arrays, keep_host=False, autotune=autotune, matrix_free_fel=True,
+                            matrix_free_deriv=matrix_free_deriv)
     log(f"[bench] mesh {n}^3: {S.n_nodes} nodes, {S.n_elements} tets, {S.n_dof} DOFs, "
         f"{int(S.elements_np[:, -1].sum())} defect tets ({time.time() - t0:.1f}s)")
     t0 = time.time()
@@ -577,6 +578,9 @@ def main():
     ap.add_argument("--rel-delta-gs", type=float, default=None,
                     help="reliable-update factor of the Gauss-Seidel solves (default 0.01)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default SpMV launch schedules")
+    ap.add_argument("--matrix-free-deriv", action="store_true",
+                    help="strain, Htilde and the demag right-hand side from the element walk instead of the "
+                         "Fx/Fy/Fz CSR operators")
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--cpu-n", type=int, default=40)
     ap.add_argument("--cpu-steps", type=int, default=3)
@@ -618,7 +622,7 @@ def main():
     steps64 = args.steps_fp64
     g = run_gpu(args.n, args.steps, args.warmup, steps64, modes, args.tol, ref_steps=args.ref_steps,
                 rel_delta=args.rel_delta, autotune=not args.no_autotune, rel_delta_gs=args.rel_delta_gs,
-                mg_degree=args.mg_degree, mg_ratio=args.mg_ratio)
+                mg_degree=args.mg_degree, mg_ratio=args.mg_ratio, matrix_free_deriv=args.matrix_free_deriv)
     iters_full = g["modes"]["fp64"]["iters"][-1] if "fp64" in g["modes"] else None
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n, args.cpu_steps, args.tol, args.n, iters_full, g["nnz"])
     st = g["snap_steps"]

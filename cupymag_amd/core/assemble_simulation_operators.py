@@ -88,7 +88,7 @@ class SimulationOperators(FEMMesh):
             d["F3"] = None
         for k in ("A1", "A2"):
             d[k] = up(k, sell_bs=1)
-        d["F1"] = up("F1")
+        d["F1"] = up("F1", sell_bs=1)
         if P.ME:
             d["A_el"] = up("A_el", sell_bs=3)
             d["F_el"] = mf if (self.matrix_free_fel and mf is not None) else up("F_el")

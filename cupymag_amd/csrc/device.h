@@ -347,10 +347,14 @@ __device__ inline void cg_state_update(CgState* st, const double (&v)[K], double
       // (about eps64 ||A|| ||x||, 1e-13 of ||b|| on the anchored elasticity
       // operator at 64^3): the solve ends there, converged, as the
       // reference's test on the recurrence residual would.
+      // (a replacement taken with another column's, upd = 2, comes before
+      // this column's residual has dropped by delta: only the 10x test
+      // applies to it)
       const double tr = sqrt(v[j]);
-      const bool stalled = tr > 0.5 * c.maxr;
+      const bool own = c.upd == 1;
+      const bool stalled = own && tr > 0.5 * c.maxr;
       const bool floor = c.p64 && stalled && c.resid <= c.atol;
-      if (tr > 10.0 * c.resid || stalled || c.it - c.it_upd >= kRelCycle) c.p64 = 1;
+      if (tr > 10.0 * c.resid || stalled || (own && c.it - c.it_upd >= kRelCycle)) c.p64 = 1;
       c.it_upd = c.it;
       c.upd = 0;
       c.resid = tr;
@@ -373,6 +377,18 @@ __device__ inline void cg_state_update(CgState* st, const double (&v)[K], double
         c.fail = 1;
       }
     }
+  }
+  if constexpr (K > 1 && (MODE == RED_RR_REL || MODE == RED_RR_PCG_REL)) {
+    // one FP64 residual pass serves every column: when a live column asks
+    // for one, the other live columns take it too (upd = 2), so that the
+    // columns' requests do not each cost a pass of the FP64 operator
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < K; ++j) any = any || (!st[j].done && st[j].upd == 1);
+    if (any)
+#pragma unroll
+      for (int j = 0; j < K; ++j)
+        if (!st[j].done && st[j].upd == 0) st[j].upd = 2;
   }
 }
 

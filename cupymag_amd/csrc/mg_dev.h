@@ -155,11 +155,12 @@ __device__ __forceinline__ void mg_sweep_rows(int64_t lb, int64_t lg, int32_t nb
   constexpr int NV = BS * K;
   constexpr bool RES = (MODE == 2 || MODE == 3);  // residual of x (else r -= A d_in)
   const T* src = RES ? x : din;
-  for_rows(lb, lg, nb, [&](int64_t row) {
+  const int64_t stride = lg * kBlock;
+  for (int64_t row = lb * kBlock + threadIdx.x; row < (int64_t)((nb + 63) & ~63); row += stride) {
     double sum[NV], xo[NV];
     sell_row<BS, K, T, false, ST, VT, 0, sizeof(T) == 4>(row, nb, sptr, col, val, dg, src, nullptr, ga, sum, xo,
                                                          vscale);
-    if (row >= nb) return;
+    if (row >= nb) continue;
     double rv[NV];
 #pragma unroll
     for (int c = 0; c < NV; ++c) {
@@ -185,7 +186,7 @@ __device__ __forceinline__ void mg_sweep_rows(int64_t lb, int64_t lg, int32_t nb
         }
       }
     }
-  });
+  }
 }
 
 // Row `row` (< nrows_pad, padding rows included) of the FP16 half form of a
@@ -661,10 +662,11 @@ __device__ __forceinline__ void pcg_true_rows(int64_t lb, int64_t lg, int32_t nb
   bool upd[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) upd[j] = st[j].upd != 0;
-  for_rows(lb, lg, nb, [&](int64_t row) {
+  const int64_t stride = lg * kBlock;
+  for (int64_t row = lb * kBlock + threadIdx.x; row < (int64_t)((nb + 63) & ~63); row += stride) {
     double sum[NV], xo[NV];
     sell_row<BS, K, double, false, ST>(row, nb, sptr, col, val, dg, x, nullptr, ga, sum, xo);
-    if (row >= nb) return;
+    if (row >= nb) continue;
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       const int j = v % K;
@@ -673,7 +675,7 @@ __device__ __forceinline__ void pcg_true_rows(int64_t lb, int64_t lg, int32_t nb
       r[row * NV + v] = ri;
       acc[j] += ri * ri;
     }
-  });
+  }
 }
 
 // some column asked for a reliable update

@@ -348,15 +348,6 @@ __host__ __device__ __forceinline__ void sell_row(int64_t row, int32_t nbrows, c
   }
 }
 
-// Calls f(row) for the rows of a workgroup range, padding rows included:
-// workgroup position lb takes rows [256 lb, 256 lb + 256), then every lg
-// workgroups further on.
-template <class F>
-__device__ __forceinline__ void for_rows(int64_t lb, int64_t lg, int32_t nbrows, F&& f) {
-  const int64_t stride = lg * kBlock;
-  for (int64_t row = lb * kBlock + threadIdx.x; row < (int64_t)((nbrows + 63) & ~63); row += stride) f(row);
-}
-
 // y = A x (ADDX32: A (x + x32)) over the block rows of a workgroup range; CG:
 // acc[j] += x.y of column j over the rows this thread owns.
 template <int BS, int K, class T, bool CG, bool ADDX32, bool ST, class VT = T, int UF = 0>
@@ -366,7 +357,8 @@ __device__ __forceinline__ void sell_rows(int64_t lb, int64_t lg, int32_t nbrows
                                           const float* __restrict__ x32, T* __restrict__ y, const GridArg& ga,
                                           double (&acc)[K]) {
   constexpr int NV = BS * K;
-  for_rows(lb, lg, nbrows, [&](int64_t row) {
+  const int64_t stride = lg * kBlock;
+  for (int64_t row = lb * kBlock + threadIdx.x; row < (int64_t)((nbrows + 63) & ~63); row += stride) {
     double sum[NV], xo[NV];
     sell_row<BS, K, T, ADDX32, ST, VT, UF>(row, nbrows, sptr, col, val, dg, x, x32, ga, sum, xo);
     if (row < nbrows) {
@@ -377,7 +369,7 @@ __device__ __forceinline__ void sell_rows(int64_t lb, int64_t lg, int32_t nbrows
         if (CG) acc[v % K] += xo[v] * (double)yv;
       }
     }
-  });
+  }
 }
 
 // p = r (first iteration) or r + beta p, on the live columns.
