@@ -44,7 +44,7 @@ class cpm_sim_params(C.Structure):
                 ("Nd", C.c_double * 3), ("Ebar_sigma", C.c_double * 6), ("R", C.c_double * 9),
                 ("Mv", C.c_double * 36), ("MvinvT", C.c_double * 36), ("mode", C.c_int32),
                 ("warm_start", C.c_int32), ("rel_delta", C.c_double), ("precond", C.c_int32),
-                ("reserved", C.c_int32 * 3), ("rel_delta_gs", C.c_double), ("rel_delta_mg", C.c_double)]
+                ("overlap", C.c_int32), ("reserved", C.c_int32 * 2), ("rel_delta_gs", C.c_double), ("rel_delta_mg", C.c_double)]
 
 
 class cpm_step_report(C.Structure):

@@ -73,10 +73,35 @@ extern "C" int cpm_ctx_create(int device, void* stream, cpm_ctx** out) {
   return CPM_OK;
 }
 
+namespace cpm {
+
+int ensure_lane2(cpm_ctx* ctx) {
+  if (ctx->stream2) return CPM_OK;
+  CPM_HIP(hipSetDevice(ctx->device));
+  hipError_t e = hipMalloc(&ctx->partials2, sizeof(double) * kMaxPartials);
+  if (e == hipSuccess) e = hipHostMalloc(&ctx->pinned2, sizeof(double) * 256, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming);
+  // created last: a context with stream2 has every lane-2 resource
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking);
+  if (e != hipSuccess) return fail(CPM_EHIP, std::string("second lane: ") + hipGetErrorString(e));
+  return CPM_OK;
+}
+
+}  // namespace cpm
+
 extern "C" int cpm_ctx_destroy(cpm_ctx* c) {
   if (!c) return CPM_OK;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  if (c->stream2) {
+    (void)hipStreamSynchronize(c->stream2);
+    (void)hipStreamDestroy(c->stream2);
+  }
+  if (c->partials2) (void)hipFree(c->partials2);
+  if (c->pinned2) (void)hipHostFree(c->pinned2);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   for (void* p : {(void*)c->partials, (void*)c->counter, (void*)c->scal, (void*)c->cg, (void*)c->r, (void*)c->p,
                   (void*)c->q, (void*)c->z, (void*)c->r32, (void*)c->p32, (void*)c->q32, (void*)c->x32, (void*)c->b32,
                   (void*)c->y64})

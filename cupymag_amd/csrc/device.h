@@ -72,6 +72,13 @@ struct cpm_ctx {
   float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr, *b32 = nullptr;
   double* y64 = nullptr;
   int64_t ws32_len = 0;
+  // second lane of solves (ensure_lane2): a non-blocking stream of default
+  // priority, its partials (kMaxPartials doubles) and pinned state mirror,
+  // and the events that order it against the context's stream
+  hipStream_t stream2 = nullptr;
+  double* partials2 = nullptr;
+  double* pinned2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 struct cpm_mat {
@@ -127,6 +134,9 @@ __host__ __device__ __forceinline__ int64_t xcd_perm(int64_t w, int64_t g, int32
 }
 
 int ensure_workspace(cpm_ctx* ctx, int64_t n);
+// the context's second lane of solves (stream2, partials2, pinned2, the
+// fork and join events), created on first use
+int ensure_lane2(cpm_ctx* ctx);
 int read_cg_state(cpm_ctx* ctx, int slot, CgState* out);
 
 // y = A_valset x on the context stream (no sync).

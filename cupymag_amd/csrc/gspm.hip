@@ -20,6 +20,7 @@
 //   :237-278 m3*, stage-2 field, f*dt2 + m*           -> k_gs3_field2
 //   :292-306 normalise, defect -> 0, sumsq, avg_m     -> k_project
 #include <algorithm>
+#include <thread>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -790,7 +791,10 @@ static FieldPrm field_prm(const cpm_sim* S, const double* H) {
 static inline bool fast_mode(const cpm_sim* S) { return S->prm.mode != CPM_MODE_FP64; }
 
 // Field kernels of stage `stage` (0..5) and the description of its solve.
-static int stage_pre(cpm_sim* S, int stage, const FieldPrm& P, SolveSpec& sp) {
+// ovl (overlap_demag): the demag right-hand side is formed in stage 0,
+// where the demag solve starts beside the elasticity solve, and stage 1
+// forms the strain only
+static int stage_pre(cpm_sim* S, int stage, const FieldPrm& P, SolveSpec& sp, bool ovl = false) {
   cpm_ctx* ctx = S->ctx;
   hipStream_t s = ctx->stream;
   const cpm_sim_params& p = S->prm;
@@ -813,13 +817,22 @@ static int stage_pre(cpm_sim* S, int stage, const FieldPrm& P, SolveSpec& sp) {
       } else if ((rc = launch_spmv(ctx, S->ops.F_el, 0, S->E0, S->bel))) {
         return rc;
       }
+      if (ovl) {  // :166 (b_demag depends on m only)
+        if (S->ops.fder) {
+          if ((rc = fder_strain_demag(S->ops.fder, nullptr, p.rot111, dR, S->m, S->dscal + S_AVGM, nullptr, S->bd)))
+            return rc;
+        } else {
+          CPM_G_DISPATCH(F3->lanes, k_demag_rhs, n, s, (int32_t)n, F3->ptr, F3->idx, F3->val, F3->nnz, S->m,
+                         S->dscal + S_AVGM, S->bd);
+        }
+      }
       sp = SolveSpec{0, 1, S->ops.A_el, S->bel, S->u, fast ? S->bint : nullptr, fast ? S->uint : nullptr};
       break;
     case 1:  // :127-169
       if (S->ops.fder) {
         // strain (when coupled) and the demag right-hand side in one walk
-        if ((rc = fder_strain_demag(S->ops.fder, p.me ? S->u : nullptr, p.rot111, dR, S->m, S->dscal + S_AVGM,
-                                    S->E, S->bd)))
+        if ((rc = fder_strain_demag(S->ops.fder, p.me ? S->u : nullptr, p.rot111, dR, ovl ? nullptr : S->m,
+                                    S->dscal + S_AVGM, S->E, ovl ? nullptr : S->bd)))
           return rc;
       } else if (p.me) {
         CPM_G_DISPATCH(F3->lanes, k_strain, n, s, (int32_t)n, F3->ptr, F3->idx, F3->val, F3->nnz, S->u, p.rot111,
@@ -830,6 +843,7 @@ static int stage_pre(cpm_sim* S, int stage, const FieldPrm& P, SolveSpec& sp) {
                                            dEs, dMv, S->dscal, ctx->partials, ctx->counter);
         k_efinal<<<gv, kBlock, 0, s>>>(n, S->E, p.rot111, dMvinvT, S->dscal);
       }
+      if (ovl) break;  // the demag solve ran in stage 0
       if (!S->ops.fder)
         CPM_G_DISPATCH(F3->lanes, k_demag_rhs, n, s, (int32_t)n, F3->ptr, F3->idx, F3->val, F3->nnz, S->m,
                        S->dscal + S_AVGM, S->bd);
@@ -963,7 +977,8 @@ static cpm_mg* solve_mg(const cpm_sim* S, const SolveSpec& sp) {
 
 // One solve of a single simulation: CSR CG per column (plain FP64 path, the
 // reference's sequence) or the SELL solvers (fast modes).
-static int solve_one(cpm_sim* S, const SolveSpec& sp, int use_init, cpm_step_report* rep) {
+static int solve_one(cpm_sim* S, const SolveSpec& sp, int use_init, cpm_step_report* rep,
+                     const PcgLane* ln = nullptr) {
   cpm_ctx* ctx = S->ctx;
   const cpm_sim_params& p = S->prm;
   const int64_t n = S->n;
@@ -994,11 +1009,11 @@ static int solve_one(cpm_sim* S, const SolveSpec& sp, int use_init, cpm_step_rep
   int failed = 0;
   int hint = 0;
   for (int j = 0; j < sp.K; ++j) hint = std::max(hint, S->hint[sp.slot + j]);
-  warm_guess(S, sp);
   const int x0 = use_x0_fast(S, use_init);
   cpm_mg* mg = solve_mg(S, sp);
   const double dmg = (p.mode == CPM_MODE_MIXED) ? p.rel_delta_mg : 0.0;
-  int rc = mg ? mg_pcg(mg, sp.K, bb, xx, x0, p.tol, p.maxiter, dmg, hint, it, res, &failed)
+  int rc = mg ? (ln ? mg_pcg_lane(mg, *ln, sp.K, bb, xx, x0, p.tol, p.maxiter, dmg, hint, it, res, &failed)
+                    : mg_pcg(mg, sp.K, bb, xx, x0, p.tol, p.maxiter, dmg, hint, it, res, &failed))
            : (p.mode == CPM_MODE_MIXED)
                ? sell_cg_mixed(ctx, Am, sp.K, bb, xx, x0, p.tol, p.maxiter, solve_delta(p, sp.slot), hint, it, res,
                                &failed,
@@ -1023,6 +1038,52 @@ static int solve_one(cpm_sim* S, const SolveSpec& sp, int use_init, cpm_step_rep
 
 }  // namespace cpm
 
+// The demag solve runs beside the elasticity solve: the two are
+// independent (Micromagnetics.py:112-169; the demag right-hand side depends
+// on m only), so with multigrid on both the demag PCG takes the context's
+// second lane, from a host thread of its own, while this thread drives the
+// elasticity PCG on the first; the V-cycles of the scalar operator fill the
+// gaps of the elasticity solve's launches.  Each solve's arithmetic is the
+// same as when they run one after the other.
+static bool overlap_demag(const cpm_sim* S) {
+  const cpm_sim_params& p = S->prm;
+  return p.overlap && fast_mode(S) && p.me && (p.precond & CPM_PRECOND_EL_DEMAG) && S->ops.mg_el &&
+         S->ops.mg_demag;
+}
+
+static int solve_el_and_demag(cpm_sim* S, const SolveSpec& se, int use_init, cpm_step_report* rep) {
+  cpm_ctx* ctx = S->ctx;
+  PcgLane l2;
+  if (int rc = lane2(ctx, &l2)) return rc;
+  const SolveSpec sd{1, 1, S->ops.A_demag, S->bd, S->U};
+  warm_guess(S, sd);
+  CPM_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
+  int drc = CPM_OK;
+  std::string derr;
+  cpm_step_report drep{};
+  std::thread demag([&] {
+    if (hipSetDevice(ctx->device) != hipSuccess || hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0) != hipSuccess) {
+      drc = fail(CPM_EHIP, "demag solve: second lane not ready");
+      derr = cpm_last_error();
+      return;
+    }
+    drc = solve_one(S, sd, use_init, &drep, &l2);
+    if (drc < 0) derr = cpm_last_error();
+    (void)hipEventRecord(ctx->ev_join, ctx->stream2);
+  });
+  const int rc = solve_one(S, se, use_init, rep);
+  demag.join();
+  CPM_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
+  rep->iters[sd.slot] = drep.iters[sd.slot];
+  if (rc) return rc;  // as one after the other: the elasticity solve's failure comes first
+  if (drc < 0) return fail(drc, derr);
+  if (drc > 0) {
+    rep->failed = drep.failed;
+    rep->resid = drep.resid;
+  }
+  return drc;
+}
+
 extern "C" int cpm_sim_step(cpm_sim* S, const double* H, int use_init, cpm_step_report* rep) {
   if (!S || !rep || (!H && !S->ctl_on)) return fail(CPM_EINVAL, "cpm_sim_step: null argument");
   if (S->batch) return fail(CPM_EINVAL, "cpm_sim_step: the simulation belongs to a batch (cpm_batch_step)");
@@ -1030,11 +1091,14 @@ extern "C" int cpm_sim_step(cpm_sim* S, const double* H, int use_init, cpm_step_
   std::memset(rep, 0, sizeof(*rep));
   use_init = step_use_init(S, use_init);
   const FieldPrm P = field_prm(S, S->ctl_on ? nullptr : H);
+  const bool ovl = overlap_demag(S);
   for (int stage = 0; stage < 6; ++stage) {
     SolveSpec sp;
-    if (int rc = stage_pre(S, stage, P, sp)) return rc;
+    if (int rc = stage_pre(S, stage, P, sp, ovl)) return rc;
     if (!sp.A) continue;
-    if (int rc = solve_one(S, sp, use_init, rep)) return rc;
+    warm_guess(S, sp);
+    if (int rc = (ovl && stage == 0) ? solve_el_and_demag(S, sp, use_init, rep) : solve_one(S, sp, use_init, rep))
+      return rc;
     if (int rc = stage_post(S, sp)) return rc;
   }
   if (int rc = stage_final(S, S->hscal)) return rc;

@@ -10,6 +10,20 @@ struct cpm_mat;
 
 namespace cpm {
 
+struct CgState;
+
+// What one PCG in flight uses of its context: the launch stream, the CG
+// states of its columns, the reduction partials (kMaxPartials doubles) and
+// a pinned host mirror of the states.  A context has two lanes, so that two
+// independent solves of one step run at once on two streams.
+struct PcgLane {
+  hipStream_t stream = nullptr;
+  CgState* st = nullptr;
+  double* partials = nullptr;
+  void* pinned = nullptr;
+  int prof = 0;  // time the level-0 kernels of the first iteration of a batch (HIP events of the context)
+};
+
 struct MgLevel {
   cpm_mat* A = nullptr;
   int32_t nb = 0;           // nodes (block rows)
@@ -55,9 +69,11 @@ struct cpm_mg {
   // FP64 PCG vectors (fine level); level and PCG vectors hold kcap columns
   double *r = nullptr, *p = nullptr, *q = nullptr, *rold = nullptr;
   int kcap = 1;
-  // stream the V-cycle kernels go to (null: the context's stream); set to
-  // the side stream while a chunk of iterations is captured
+  // stream the V-cycle kernels go to (null: the context's stream); the
+  // lane's stream during a solve, the side stream while a chunk of
+  // iterations is captured; lpart: the partials of the solve in progress
   hipStream_t ls = nullptr;
+  double* lpart = nullptr;
   // captured chunks of PCG iterations (hipGraph), replayed on the context
   // stream; keyed by the vectors and states they were captured with
   int graphs = 1;
@@ -82,8 +98,18 @@ namespace cpm {
 // stopping test on the FP64 residual); 0: FP64 products.
 int mg_pcg(cpm_mg* M, int K, const double* b, double* x, int use_x0, double tol, int maxiter, double delta,
            int hint, int32_t* iters, double* resid, int* failed);
-// grow the level and PCG vectors of M to K columns (synchronises)
-int mg_reserve(cpm_mg* M, int K);
+// the same on a lane of the hierarchy's context (its stream, states,
+// partials and pinned mirror)
+int mg_pcg_lane(cpm_mg* M, const PcgLane& ln, int K, const double* b, double* x, int use_x0, double tol,
+                int maxiter, double delta, int hint, int32_t* iters, double* resid, int* failed);
+// the context's first lane (its stream, CG states 0.., partials, pinned
+// mirror; profiling as the context has it) and its second (stream2, CG
+// states 8.., partials2, pinned2, no profiling; created on first use)
+PcgLane lane1(cpm_ctx* ctx);
+int lane2(cpm_ctx* ctx, PcgLane* out);
+// grow the level and PCG vectors of M to K columns (synchronises the
+// context's stream, or `s` when given)
+int mg_reserve(cpm_mg* M, int K, hipStream_t s = nullptr);
 // the fine operator a hierarchy was built on
 const cpm_mat* mg_operator(const cpm_mg* M);
 

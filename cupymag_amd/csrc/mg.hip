@@ -616,7 +616,7 @@ static void vcycle(cpm_mg* M, size_t l, const double* b64, const CgState* st, hi
     const int gf = grid_for(Lv.n * K);
     k_mg_first<BS, K, T, true, true><<<gn, kBlock, 0, s>>>(Lv.nb, b64, (T*)Lv.b, (T*)Lv.r, (T*)d[0], (T*)Lv.x,
                                                            (const T*)Lv.dinv, c0, st,
-                                                           M->floating ? M->ctx->partials : nullptr, gf, gf);
+                                                           M->floating ? M->lpart : nullptr, gf, gf);
   }
   else if (!first_done)
     k_mg_first<BS, K, T, true, false><<<gn, kBlock, 0, s>>>(Lv.nb, nullptr, (T*)Lv.b, (T*)Lv.r, (T*)d[0],
@@ -746,11 +746,12 @@ static int mg_alloc(cpm_mg* M, void** p, size_t bytes) {
 
 // Level and outer vectors for K columns (allocated for one column at
 // creation, grown on the first solve with more).
-int mg_reserve(cpm_mg* M, int K) {
+int mg_reserve(cpm_mg* M, int K, hipStream_t s) {
   if (K <= M->kcap) return CPM_OK;
   if (M->kcap == 0) return fail(CPM_ENOMEM, "multigrid: hierarchy unusable after a failed allocation");
   cpm_ctx* ctx = M->ctx;
-  CPM_HIP(hipStreamSynchronize(ctx->stream));
+  if (!s) s = ctx->stream;
+  CPM_HIP(hipStreamSynchronize(s));
   // captured chunks refer to the vectors replaced below
   for (auto& c : M->chunks) (void)hipGraphExecDestroy(c.exec);
   M->chunks.clear();
@@ -771,28 +772,39 @@ int mg_reserve(cpm_mg* M, int K) {
   const int64_t n0 = M->L[0].n;
   for (void** p : {(void**)&M->r, (void**)&M->p, (void**)&M->q, (void**)&M->rold})
     if (int rc = regrow(p, 8 * n0 * K)) return rc;
-  CPM_HIP(hipMemsetAsync(M->rold, 0, 8 * n0 * K, ctx->stream));
-  CPM_HIP(hipStreamSynchronize(ctx->stream));
+  CPM_HIP(hipMemsetAsync(M->rold, 0, 8 * n0 * K, s));
+  CPM_HIP(hipStreamSynchronize(s));
   M->kcap = K;
   return CPM_OK;
 }
 
 // Preconditioned CG on level 0 for K columns (see the file comment); column
-// j of b and x is entry i*K + j, with its own state ctx->cg[j].
+// j of b and x is entry i*K + j, with its own state ln.st[j].
 // delta > 0: the outer products use the FP32 form of the operator, with an
 // FP64 residual replacement whenever the recurrence residual has dropped by
 // delta since the last one (and before stopping: the stopping test is on
 // the FP64 residual); delta = 0: FP64 products.
 template <int BS, int K, class T>
-static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol, int maxiter, double delta,
-                   int hint, int32_t* iters, double* resid, int* failed) {
+static int pcg_run(cpm_mg* M, const PcgLane& ln, const double* b, double* x, int use_x0, double tol, int maxiter,
+                   double delta, int hint, int32_t* iters, double* resid, int* failed) {
   cpm_ctx* ctx = M->ctx;
-  hipStream_t s = ctx->stream;  // launch stream of the lambdas below
+  hipStream_t s = ln.stream;  // launch stream of the lambdas below
   MgLevel& L0 = M->L[0];
   const Sell& S = L0.A->sell;
   const int64_t n = L0.n * K;
   const int gs = sell_grid(S), gv = grid_for(n);
-  CgState* st = ctx->cg;
+  CgState* st = ln.st;
+  double* const partials = ln.partials;
+  // the V-cycle launches on the lane's stream and reads its partials
+  M->ls = ln.stream;
+  M->lpart = partials;
+  struct Restore {
+    cpm_mg* M;
+    ~Restore() {
+      M->ls = nullptr;
+      M->lpart = nullptr;
+    }
+  } restore{M};
   const bool rel = delta > 0.0;
   // ev (profiling): the product is timed by its dispatch (mixed, stencil form)
   auto spmv = [&](const double* in, double* out, bool plain, hipEvent_t* ev = nullptr) {
@@ -801,29 +813,29 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
         hipExtLaunchKernelGGL(k_pcg_spmv_rel<BS, K, true>, dim3(gs), dim3(kBlock), 0, s, ev[0], ev[1], 0,
                               (int32_t)S.nbrows, (const int64_t*)S.sptr, (const int32_t*)S.col,
                               (const float*)S.v32, (const double*)S.dgc, (const double*)S.v64,
-                              (const double*)S.dg64, CPM_GRID_ARG32(S), in, out, (const CgState*)st, ctx->partials);
+                              (const double*)S.dg64, CPM_GRID_ARG32(S), in, out, (const CgState*)st, partials);
       else if (S.stencil)
         k_pcg_spmv_rel<BS, K, true><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v32, S.dgc, S.v64, S.dg64,
-                                                          CPM_GRID_ARG32(S), in, out, st, ctx->partials);
+                                                          CPM_GRID_ARG32(S), in, out, st, partials);
       else
         k_pcg_spmv_rel<BS, K, false><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v32, S.dgc, S.v64, S.dg64,
-                                                           CPM_GRID_ARG32(S), in, out, st, ctx->partials);
+                                                           CPM_GRID_ARG32(S), in, out, st, partials);
       return;
     }
     if (plain) {
       if (S.stencil)
         k_pcg_spmv<BS, K, true, true><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), in,
-                                                            out, st, ctx->partials);
+                                                            out, st, partials);
       else
         k_pcg_spmv<BS, K, false, true><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), in,
-                                                             out, st, ctx->partials);
+                                                             out, st, partials);
     } else {
       if (S.stencil)
         k_pcg_spmv<BS, K, true, false><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), in,
-                                                             out, st, ctx->partials);
+                                                             out, st, partials);
       else
         k_pcg_spmv<BS, K, false, false><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S),
-                                                              in, out, st, ctx->partials);
+                                                              in, out, st, partials);
     }
   };
   // floating hierarchy (one column): z = F^T V(F r) + e_0 r_0, the sums of
@@ -832,13 +844,13 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
   if (fl && K != 1) return fail(CPM_EINVAL, "multigrid CG: a floating hierarchy solves one column");
   constexpr int FB = (K == 1) ? BS : 0;
   auto precond = [&](hipEvent_t* ev) {
-    if (fl) k_pcg_rsum<BS * K, K><<<gv, kBlock, 0, s>>>(n, M->r, st, ctx->partials);
+    if (fl) k_pcg_rsum<BS * K, K><<<gv, kBlock, 0, s>>>(n, M->r, st, partials);
     vcycle<BS, K, T>(M, 0, M->r, st, ev);
     if (fl)
-      k_pcg_dot<K, T, FB><<<gv, kBlock, 0, s>>>(n, M->r, M->rold, (const T*)L0.x, st, ctx->partials);
+      k_pcg_dot<K, T, FB><<<gv, kBlock, 0, s>>>(n, M->r, M->rold, (const T*)L0.x, st, partials);
     else
-      k_pcg_dot<K, T, 0><<<gv, kBlock, 0, s>>>(n, M->r, M->rold, (const T*)L0.x, st, ctx->partials);
-    k_pcg_red_rz<K><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st);
+      k_pcg_dot<K, T, 0><<<gv, kBlock, 0, s>>>(n, M->r, M->rold, (const T*)L0.x, st, partials);
+    k_pcg_red_rz<K><<<1, kRedBlock, 0, s>>>(gv, partials, st);
   };
   auto pupd = [&]() {
     if (fl)
@@ -848,11 +860,11 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
   };
   // start: r = b - A x0 (or b), state, z = M r, rho = r.z
   if (use_x0) spmv(x, M->q, true);
-  k_pcg_init<K><<<gv, kBlock, 0, s>>>(n, b, x, M->q, M->r, use_x0, ctx->partials);
+  k_pcg_init<K><<<gv, kBlock, 0, s>>>(n, b, x, M->q, M->r, use_x0, partials);
   if (rel)
-    k_pcg_red_init<K, true><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st, tol, maxiter);
+    k_pcg_red_init<K, true><<<1, kRedBlock, 0, s>>>(gv, partials, st, tol, maxiter);
   else
-    k_pcg_red_init<K, false><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st, tol, maxiter);
+    k_pcg_red_init<K, false><<<1, kRedBlock, 0, s>>>(gv, partials, st, tol, maxiter);
   if (use_x0) k_pcg_zero_b<K><<<gv, kBlock, 0, s>>>(n, x, st);
   precond(nullptr);
   CPM_HIP(hipGetLastError());
@@ -860,27 +872,27 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
   // post-smoothing of the first iteration of every batch (launched
   // directly, the rest of the batch from the captured chunk) is timed with
   // HIP events and accumulated on the level-0 operator
-  const bool prof = ctx->prof && M->degree >= 2 && K == 1;
+  const bool prof = ln.prof && M->degree >= 2 && K == 1;
   auto iterate = [&](int todo, hipEvent_t* ev) -> int {
     for (int k = 0; k < todo; ++k) {
       pupd();
       spmv(M->p, M->q, false, (k == 0 && ev && rel && S.stencil) ? ev + 4 : nullptr);
       if (rel)
-        k_red<K, RED_ALPHA_PCG_REL><<<1, kRedBlock, 0, s>>>(gs, ctx->partials, st, 0.0);
+        k_red<K, RED_ALPHA_PCG_REL><<<1, kRedBlock, 0, s>>>(gs, partials, st, 0.0);
       else
-        k_red<K, RED_ALPHA><<<1, kRedBlock, 0, s>>>(gs, ctx->partials, st, 0.0);
-      k_pcg_xr<K><<<gv, kBlock, 0, s>>>(n, x, M->r, M->rold, M->p, M->q, st, ctx->partials);
+        k_red<K, RED_ALPHA><<<1, kRedBlock, 0, s>>>(gs, partials, st, 0.0);
+      k_pcg_xr<K><<<gv, kBlock, 0, s>>>(n, x, M->r, M->rold, M->p, M->q, st, partials);
       if (rel) {
-        k_red<K, RED_RR_PCG_REL><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st, delta);
+        k_red<K, RED_RR_PCG_REL><<<1, kRedBlock, 0, s>>>(gv, partials, st, delta);
         if (S.stencil)
           k_pcg_true<BS, K, true><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), x, b,
-                                                        M->r, st, ctx->partials);
+                                                        M->r, st, partials);
         else
           k_pcg_true<BS, K, false><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), x, b,
-                                                         M->r, st, ctx->partials);
-        k_red<K, RED_RR_PCG_TRUE><<<1, kRedBlock, 0, s>>>(gs, ctx->partials, st, 0.0);
+                                                         M->r, st, partials);
+        k_red<K, RED_RR_PCG_TRUE><<<1, kRedBlock, 0, s>>>(gs, partials, st, 0.0);
       } else {
-        k_red<K, RED_RR_PCG><<<1, kRedBlock, 0, s>>>(gv, ctx->partials, st, 0.0);
+        k_red<K, RED_RR_PCG><<<1, kRedBlock, 0, s>>>(gv, partials, st, 0.0);
       }
       precond((k == 0) ? ev : nullptr);
     }
@@ -896,7 +908,7 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
     uint64_t dbits;
     std::memcpy(&dbits, &delta, sizeof dbits);
     for (uint64_t v : {(uint64_t)(uintptr_t)x, (uint64_t)(uintptr_t)b, (uint64_t)(uintptr_t)M->r,
-                       (uint64_t)(uintptr_t)L0.x, (uint64_t)(uintptr_t)st, (uint64_t)(uintptr_t)ctx->partials,
+                       (uint64_t)(uintptr_t)L0.x, (uint64_t)(uintptr_t)st, (uint64_t)(uintptr_t)partials,
                        (uint64_t)K, (uint64_t)sizeof(T), (uint64_t)ch, dbits})
       key = (key ^ v) * 1099511628211ull;
     for (auto& c : M->chunks)
@@ -908,8 +920,8 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
     s = M->side;
     M->ls = M->side;
     const int rc = iterate(ch, nullptr);
-    s = ctx->stream;
-    M->ls = nullptr;
+    s = ln.stream;
+    M->ls = ln.stream;
     hipError_t e = hipStreamEndCapture(M->side, &g);
     if (rc) return rc;
     if (e != hipSuccess) return fail(CPM_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
@@ -947,9 +959,9 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
   };
   CgState h[K];
   auto read = [&]() -> int {
-    CPM_HIP(hipMemcpyAsync(ctx->pinned, st, sizeof(CgState) * K, hipMemcpyDeviceToHost, s));
+    CPM_HIP(hipMemcpyAsync(ln.pinned, st, sizeof(CgState) * K, hipMemcpyDeviceToHost, s));
     CPM_HIP(hipStreamSynchronize(s));
-    std::memcpy(h, ctx->pinned, sizeof(CgState) * K);
+    std::memcpy(h, ln.pinned, sizeof(CgState) * K);
     return CPM_OK;
   };
   auto all = [&]() {
@@ -1009,21 +1021,46 @@ static int pcg_run(cpm_mg* M, const double* b, double* x, int use_x0, double tol
 
 const cpm_mat* mg_operator(const cpm_mg* M) { return M && !M->L.empty() ? M->L[0].A : nullptr; }
 
-int mg_pcg(cpm_mg* M, int K, const double* b, double* x, int use_x0, double tol, int maxiter, double delta,
-           int hint, int32_t* iters, double* resid, int* failed) {
+PcgLane lane1(cpm_ctx* ctx) {
+  PcgLane ln;
+  ln.stream = ctx->stream;
+  ln.st = ctx->cg;
+  ln.partials = ctx->partials;
+  ln.pinned = ctx->pinned;
+  ln.prof = ctx->prof;
+  return ln;
+}
+
+int lane2(cpm_ctx* ctx, PcgLane* out) {
+  if (int rc = ensure_lane2(ctx)) return rc;
+  out->stream = ctx->stream2;
+  out->st = ctx->cg + 8;
+  out->partials = ctx->partials2;
+  out->pinned = ctx->pinned2;
+  out->prof = 0;
+  return CPM_OK;
+}
+
+int mg_pcg_lane(cpm_mg* M, const PcgLane& ln, int K, const double* b, double* x, int use_x0, double tol,
+                int maxiter, double delta, int hint, int32_t* iters, double* resid, int* failed) {
   if (K != 1 && !(K == 3 && M->bs == 1))
     return fail(CPM_EINVAL, "multigrid CG: 1 right-hand side, or 3 for a scalar operator");
   if (!(delta >= 0.0 && delta < 1.0)) return fail(CPM_EINVAL, "multigrid CG: delta must be in [0, 1)");
-  if (int rc = mg_reserve(M, K)) return rc;
+  if (int rc = mg_reserve(M, K, ln.stream)) return rc;
   const double d = delta;
   if (M->bs == 3)
-    return M->prec == 32 ? pcg_run<3, 1, float>(M, b, x, use_x0, tol, maxiter, d, hint, iters, resid, failed)
-                         : pcg_run<3, 1, double>(M, b, x, use_x0, tol, maxiter, d, hint, iters, resid, failed);
+    return M->prec == 32 ? pcg_run<3, 1, float>(M, ln, b, x, use_x0, tol, maxiter, d, hint, iters, resid, failed)
+                         : pcg_run<3, 1, double>(M, ln, b, x, use_x0, tol, maxiter, d, hint, iters, resid, failed);
   if (K == 3)
-    return M->prec == 32 ? pcg_run<1, 3, float>(M, b, x, use_x0, tol, maxiter, d, hint, iters, resid, failed)
-                         : pcg_run<1, 3, double>(M, b, x, use_x0, tol, maxiter, d, hint, iters, resid, failed);
-  return M->prec == 32 ? pcg_run<1, 1, float>(M, b, x, use_x0, tol, maxiter, d, hint, iters, resid, failed)
-                       : pcg_run<1, 1, double>(M, b, x, use_x0, tol, maxiter, d, hint, iters, resid, failed);
+    return M->prec == 32 ? pcg_run<1, 3, float>(M, ln, b, x, use_x0, tol, maxiter, d, hint, iters, resid, failed)
+                         : pcg_run<1, 3, double>(M, ln, b, x, use_x0, tol, maxiter, d, hint, iters, resid, failed);
+  return M->prec == 32 ? pcg_run<1, 1, float>(M, ln, b, x, use_x0, tol, maxiter, d, hint, iters, resid, failed)
+                       : pcg_run<1, 1, double>(M, ln, b, x, use_x0, tol, maxiter, d, hint, iters, resid, failed);
+}
+
+int mg_pcg(cpm_mg* M, int K, const double* b, double* x, int use_x0, double tol, int maxiter, double delta,
+           int hint, int32_t* iters, double* resid, int* failed) {
+  return mg_pcg_lane(M, lane1(M->ctx), K, b, x, use_x0, tol, maxiter, delta, hint, iters, resid, failed);
 }
 
 }  // namespace cpm
@@ -1401,7 +1438,9 @@ extern "C" int cpm_mg_apply(cpm_mg* M, const double* r, double* z) {
     else
       k_pcg_rsum<1, 1><<<gv, kBlock, 0, ctx->stream>>>(L0.n, r, st, ctx->partials);
   }
+  M->lpart = ctx->partials;  // the partials k_pcg_rsum just wrote (floating hierarchies)
   vcycle_any(M, r, st);
+  M->lpart = nullptr;
   CPM_HIP(hipGetLastError());
   if (M->floating) {
     if (M->bs == 3) {

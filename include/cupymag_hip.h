@@ -381,7 +381,10 @@ typedef struct cpm_sim_params {
                                  residual drops by this factor (reliable updates) */
   int32_t precond;            /* fast modes: CPM_PRECOND_* bits; the solves named run multigrid-
                                  preconditioned CG (cpm_cg_mg) on the hierarchies in cpm_sim_ops */
-  int32_t reserved[3];
+  int32_t overlap;            /* fast modes with multigrid elasticity and demag solves: the demag solve
+                                 runs beside the elasticity solve, on the context's second stream
+                                 (same results bit for bit; single simulations, not batches) */
+  int32_t reserved[2];
   double rel_delta_gs;        /* CPM_MODE_MIXED: rel_delta of the Gauss-Seidel solves (A1, A2);
                                  0 = rel_delta */
   double rel_delta_mg;        /* CPM_MODE_MIXED with precond: the multigrid solves' outer products use

@@ -233,6 +233,41 @@ def test_gspm_steps_with_multigrid_match_oracle(mode, mg_gs):
         assert np.linalg.norm(sim.get("UEL").cpu().numpy() - st["u"]) <= 1e-6 * np.linalg.norm(st["u"])
 
 
+@pytest.mark.parametrize("mode,warm", [("fast64", False), ("mixed", False), ("mixed", "extrapolate")])
+def test_demag_solve_beside_the_elasticity_solve_changes_nothing(mode, warm):
+    """The demag solve on the context's second stream, beside the
+    elasticity solve (overlap), gives every field, CG iteration count and
+    step scalar of the two solves one after the other, bit for bit, over
+    four steps; so does a failed step: with maxiter too small both report
+    the elasticity solve's failure first."""
+    from cupymag_amd.core.gspm import GSPMSimulation
+    cfg, arr, P, S = _setup(n=16)
+    m0 = tilted_m(S.n_dof, seed=21)
+    H = [P.Hext1, P.Hext2, P.Hext3]
+    sims = [GSPMSimulation(S, P, mode=mode, precond="mg", warm_start=warm, overlap=ov) for ov in (True, False)]
+    for s in sims:
+        s.set_m(m0)
+    for k in range(4):
+        reps = [s.step(H, False) for s in sims]
+        assert reps[0].iters == reps[1].iters and reps[0].sumsq == reps[1].sumsq, (k, reps[0].iters, reps[1].iters)
+        assert reps[0].iters[1] > 0
+        for f in ("M", "U", "UEL", "E", "HTILDE"):
+            assert torch.equal(sims[0].get(f), sims[1].get(f)), (k, f)
+    saved = P.maxiter
+    P.maxiter = 2
+    try:
+        errs = []
+        for ov in (True, False):
+            sim = GSPMSimulation(S, P, mode=mode, precond="mg", overlap=ov)
+            sim.set_m(m0)
+            with pytest.raises(RuntimeError, match="elasticity") as ei:
+                sim.step(H, False)
+            errs.append(str(ei.value))
+        assert errs[0] == errs[1]
+    finally:
+        P.maxiter = saved
+
+
 def test_multigrid_rejections():
     from cupymag_amd.core.batch import GSPMBatch
     from cupymag_amd.core.gspm import GSPMSimulation
