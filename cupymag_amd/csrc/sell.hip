@@ -391,7 +391,9 @@ static int drive(cpm_ctx* ctx, cpm_mat* M, int maxiter, int hint, Issue&& issue,
         M->prof_launches += 1;
       }
     }
-    batch = std::max(2, std::min(64, it_max / 4 + 2));
+    // as the multigrid PCG: one iteration per check up to two past the
+    // previous solve's count, then a quarter of the count so far
+    batch = (hint > 2 && it_max < hint + 2) ? 1 : std::max(2, std::min(64, it_max / 4 + 2));
   }
   *failed = 0;
   for (int j = 0; j < K; ++j) {
