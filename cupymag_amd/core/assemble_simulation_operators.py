@@ -110,16 +110,16 @@ class SimulationOperators(FEMMesh):
             return None
         return MatrixFreeFel(self.node_coords_np, n, Cv, ctx)
 
-    def multigrid(self, precision=32, degree=2, ratio=10.0, floating=True):
+    def multigrid(self, precision=32, degree=2, ratio=10.0, floating=True, coarsest_scalar=8):
         """Multigrid hierarchies of A_demag, A1 and A_el (cached per
         settings), or None when the mesh has none
         (cupymag_amd.solvers.multigrid.build_hierarchies)."""
         cache = self.__dict__.setdefault("_mg", {})
-        key = (precision, degree, ratio, bool(floating))
+        key = (precision, degree, ratio, bool(floating), int(coarsest_scalar))
         if key not in cache:
             from ..solvers.multigrid import build_hierarchies
             cache[key] = build_hierarchies(self, precision, degree, ratio, nthreads=self.nthreads,
-                                           floating=floating)
+                                           floating=floating, coarsest_scalar=coarsest_scalar)
         return cache[key]
 
     def csr_demag_and_deriv(self):

@@ -530,6 +530,31 @@ __device__ __forceinline__ void mg_dense_rows(int64_t lb, int64_t lg, int64_t n,
     double s[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) s[j] = 0.0;
+    if constexpr (sizeof(T) == 4) {
+      if ((n & 3) == 0) {
+        // FP32 rows of up to 16^3 (x3) entries: four consecutive entries per
+        // lane in one 128-bit load (rows start 16-byte aligned), four such
+        // loads in flight per lane
+        typedef float f32x4 __attribute__((ext_vector_type(4)));
+        const f32x4* a4 = reinterpret_cast<const f32x4*>(cinv + row * n);
+        const int64_t n4 = n >> 2;
+#pragma unroll 4
+        for (int64_t c4 = lane; c4 < n4; c4 += 64) {
+          const f32x4 a = a4[c4];
+          const int64_t c = c4 << 2;
+#pragma unroll
+          for (int j = 0; j < K; ++j)
+            s[j] += ((double)a.x * (double)b[c * K + j] + (double)a.y * (double)b[(c + 1) * K + j]) +
+                    ((double)a.z * (double)b[(c + 2) * K + j] + (double)a.w * (double)b[(c + 3) * K + j]);
+        }
+#pragma unroll
+        for (int j = 0; j < K; ++j) s[j] = group_sum<64>(s[j]);
+        if (lane == 0)
+#pragma unroll
+          for (int j = 0; j < K; ++j) x[row * K + j] = (T)s[j];
+        continue;
+      }
+    }
     for (int64_t c = lane; c < n; c += 64) {
       const double a = (double)cinv[row * n + c];
 #pragma unroll

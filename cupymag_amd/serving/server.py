@@ -65,9 +65,12 @@ def estimate_bytes(n_dof, me, multigrid_n=None, mode="mixed"):
     b = 1.15 * _BYTES_PER_DOF[bool(me)] * n_dof
     if multigrid_n:
         from ..mesh.coarsen import hierarchy_sizes
-        nc = hierarchy_sizes(int(multigrid_n))[-1] ** 3
+        from ..solvers.multigrid import scalar_levels
+        sizes = hierarchy_sizes(int(multigrid_n))
+        nc = sizes[-1] ** 3                 # coarsest nodes of the elasticity hierarchy
+        ns = scalar_levels(sizes)[-1] ** 3  # ... of the scalar ones (demag, A1)
         ts = 4.0 if mode == "mixed" else 8.0
-        b += _MG_BYTES_PER_DOF[bool(me)] * n_dof + ts * (2 * nc ** 2 + (9 * nc ** 2 if me else 0))
+        b += _MG_BYTES_PER_DOF[bool(me)] * n_dof + ts * (2 * ns ** 2 + (9 * nc ** 2 if me else 0))
         if mode != "mixed":
             b += _MG64_EXTRA_PER_DOF[bool(me)] * n_dof
     return int(b) + (64 << 20)

@@ -143,7 +143,16 @@ def _pinv_periodic(A, bs):
     return np.linalg.inv(A + sc * QQ) - QQ / sc
 
 
-def build_hierarchies(SimOp, precision=32, degree=2, ratio=10.0, coarsest_max=8, nthreads=None, floating=True):
+def scalar_levels(sizes, coarsest_scalar=8):
+    """Levels of a scalar operator's hierarchy: the first level of at most
+    coarsest_scalar cells per dimension is its coarsest (dense) level, and
+    there are at least two."""
+    k = next((i for i, m in enumerate(sizes) if m <= coarsest_scalar), len(sizes) - 1)
+    return sizes[:max(2, k + 1)]
+
+
+def build_hierarchies(SimOp, precision=32, degree=2, ratio=10.0, coarsest_max=8, nthreads=None, floating=True,
+                      coarsest_scalar=8):
     """{"A_demag": KuhnMultigrid, "A1": KuhnMultigrid (Gauss-Seidel operator
     M + alpha1 K; coarse levels without the defect surgery), "A_el":
     KuhnMultigrid (with coupling)} for a SimulationOperators object on an
@@ -156,7 +165,13 @@ def build_hierarchies(SimOp, precision=32, degree=2, ratio=10.0, coarsest_max=8,
     coarsest solve is its pseudo-inverse; the V-cycle then solves the
     periodic problem and the anchor is applied around it (cpm_mg_create),
     which keeps the iteration counts flat as the mesh grows.  False: anchored
-    coarse levels (node 0 left out of the transfers)."""
+    coarse levels (node 0 left out of the transfers).
+
+    The elasticity hierarchy coarsens to <= coarsest_max cells per
+    dimension; the scalar ones (A_demag, A1) to <= coarsest_scalar
+    (scalar_levels; 16 makes their dense level up to 16^3 unknowns, 64 MB in
+    FP32, in place of the sweeps and transfers of the levels below: no
+    faster at 128^3, and a longer set-up)."""
     from ..physics.host_ops import HostOps, voigt_C
     n = kuhn_cells(SimOp.node_coords_np, SimOp.elements_np)
     if n is None or SimOp.grid_dims != (n, n, n):
@@ -171,9 +186,12 @@ def build_hierarchies(SimOp, precision=32, degree=2, ratio=10.0, coarsest_max=8,
         fine["A_el"] = SimOp.csr_elasticity()[0]
     Cv = voigt_C(P.c11, P.c12, P.c44, bool(P.rot111)) if P.ME else None
     levels = {k: [v] for k, v in fine.items()}
+    nlev = {k: (len(sizes) if k == "A_el" else len(scalar_levels(sizes, coarsest_scalar))) for k in fine}
     dense = {}
     coords = SimOp.node_coords_np
     for li, m in enumerate(sizes[1:]):
+        if all(len(levels[k]) == nlev[k] for k in fine):
+            break
         coords, tets = coarse_kuhn(coords, sizes[li])
         el = np.zeros((tets.shape[0], 5), dtype=np.int32)
         el[:, :4] = tets
@@ -183,10 +201,10 @@ def build_hierarchies(SimOp, precision=32, degree=2, ratio=10.0, coarsest_max=8,
         if dims != (m, m, m):
             return None
         H = HostOps(coords, el, gid, ndof, None, SimOp.alpha1, SimOp.alpha2, Cv, nthreads, anchors=not floating)
-        last = li == len(sizes) - 2
         for name, bs in (("A_demag", 1), ("A_el", 3), ("A1", 1)):
-            if name not in fine:
+            if name not in fine or len(levels[name]) == nlev[name]:
                 continue
+            last = len(levels[name]) == nlev[name] - 1
             ip, ix, v, shape = H.csr(name)
             M = DeviceCSR(ip, ix, v, shape, ctx)
             # the host arrays are views of the assembler's buffers, freed below

@@ -3,7 +3,8 @@ elasticity/demag solves of cpm_batch_step for members created with
 precond="mg").
 
 * Hierarchies of different depths (10^3 -> 5^3, 20^3 -> 10^3 -> 5^3,
-  13^3 -> 7^3 non-nested), scalar (demag, Gauss-Seidel A1) and 3x3-block
+  13^3 -> 7^3 non-nested; the scalar ones also with their dense level at
+  <= 16 cells, 20^3 -> 10^3), scalar (demag, Gauss-Seidel A1) and 3x3-block
   operators, FP32 and FP64 V-cycles, one column or three (A1): every
   system's solution, iteration counts and residuals equal the one-system
   solve (cpm_cg_mg) bit for bit, with and without initial guesses, for a
@@ -40,10 +41,17 @@ def meshes():
 
 @pytest.mark.parametrize("name", ["A_demag", "A_el", "A1"])
 @pytest.mark.parametrize("prec", [16, 32, 64])
-def test_solve_batch_equals_single_solves(meshes, name, prec):
+@pytest.mark.parametrize("coarsest_scalar", [8, 16])
+def test_solve_batch_equals_single_solves(meshes, name, prec, coarsest_scalar):
+    """The scalar hierarchies at the elasticity's depths (20^3 -> 10^3 ->
+    5^3), or with coarsest_scalar = 16 stopping at <= 16 cells (20^3 ->
+    10^3)."""
     from cupymag_amd.solvers.multigrid import solve_batch
-    mgs = [S.multigrid(precision=prec)[name] for _, _, _, S in meshes]
-    assert [m.info()["levels"] for m in mgs] == [2, 3, 2]
+    if name == "A_el" and coarsest_scalar == 8:
+        pytest.skip("the elasticity hierarchy has one depth rule")
+    mgs = [S.multigrid(precision=prec, coarsest_scalar=coarsest_scalar)[name] for _, _, _, S in meshes]
+    want = [2, 3, 2] if (name == "A_el" or coarsest_scalar == 8) else [2, 2, 2]
+    assert [m.info()["levels"] for m in mgs] == want
     bs = 3 if name == "A_el" else 1
     rng = np.random.default_rng(11)
     rhs, x0s = [], []

@@ -64,6 +64,14 @@ def test_kuhn_detection_and_sizes():
     assert hierarchy_sizes(90) == [90, 45, 23, 12, 6]
     assert hierarchy_sizes(87) == [87, 44, 22, 11, 6]
     assert hierarchy_sizes(9) == [9, 5] and hierarchy_sizes(8) == [8]
+    # scalar hierarchies with their dense level at <= 16 cells (two levels at least)
+    from cupymag_amd.solvers.multigrid import scalar_levels
+    assert scalar_levels(hierarchy_sizes(128)) == [128, 64, 32, 16, 8]
+    assert scalar_levels(hierarchy_sizes(128), 16) == [128, 64, 32, 16]
+    assert scalar_levels(hierarchy_sizes(96), 16) == [96, 48, 24, 12]
+    assert scalar_levels(hierarchy_sizes(16), 16) == [16, 8]
+    assert scalar_levels(hierarchy_sizes(18), 16) == [18, 9]
+    assert scalar_levels(hierarchy_sizes(9), 16) == [9, 5]
     C, T = coarse_kuhn(coords, 8)
     assert np.array_equal(T, kuhn_mesh(4)[1])
     np.testing.assert_array_equal(C, kuhn_mesh(4, h=2.0)[0])
