@@ -84,14 +84,19 @@ namespace cpm {
 
 // fpart (floating hierarchy, level 0): partials of the per-component sums
 // of b64, fn of them at stride fstride
+// The elementwise level kernels (first step, prolongation, restriction
+// with the coarse level's first step) run over the level's row grid in the
+// XCD order of its sweeps (xg = the level's GridArg::xg), so that each XCD
+// writes the rows its L2 then serves to the sweep that follows.
 template <int BS, int K, class T, bool ZERO, bool FROM64>
 __global__ void __launch_bounds__(kBlock) k_mg_first(int32_t nb, const double* __restrict__ b64, T* __restrict__ b,
                                                      T* __restrict__ r, T* __restrict__ d, T* __restrict__ x,
                                                      const T* __restrict__ dinv, double c0, const CgState* st,
-                                                     const double* __restrict__ fpart = nullptr,
+                                                     int32_t xg, const double* __restrict__ fpart = nullptr,
                                                      int64_t fstride = 0, int32_t fn = 0) {
   if (all_done<K>(st)) return;
-  mg_first_rows<BS, K, T, ZERO, FROM64>(blockIdx.x, gridDim.x, nb, b64, b, r, d, x, dinv, c0, fpart, fstride, fn);
+  mg_first_rows<BS, K, T, ZERO, FROM64>(xcd_perm(blockIdx.x, gridDim.x, xg), gridDim.x, nb, b64, b, r, d, x, dinv,
+                                        c0, fpart, fstride, fn);
 }
 
 // Periodic stiffness K from the anchored operator A of a floating
@@ -283,17 +288,18 @@ __global__ void __launch_bounds__(kBlock) k_mg_restrict_first(Dims3 fine, Dims3 
                                                               const T* __restrict__ rf, T* __restrict__ bc,
                                                               T* __restrict__ rc, T* __restrict__ dc,
                                                               T* __restrict__ xc, const T* __restrict__ dinvc,
-                                                              double c0, const CgState* st) {
+                                                              double c0, const CgState* st, int32_t xg) {
   if (all_done<K>(st)) return;
-  mg_restrict_first_rows<BS, K, T>(blockIdx.x, gridDim.x, fine, coarse, anchor != 0, rf, bc, rc, dc, xc, dinvc, c0);
+  mg_restrict_first_rows<BS, K, T>(xcd_perm(blockIdx.x, gridDim.x, xg), gridDim.x, fine, coarse, anchor != 0, rf, bc,
+                                   rc, dc, xc, dinvc, c0);
 }
 
 template <int W, int K, class T>
 __global__ void __launch_bounds__(kBlock) k_mg_prolong(Dims3 fine, Dims3 coarse, int anchor,
                                                        const T* __restrict__ ec, T* __restrict__ xf,
-                                                       const CgState* st) {
+                                                       const CgState* st, int32_t xg) {
   if (all_done<K>(st)) return;
-  mg_prolong_rows<W, T>(blockIdx.x, gridDim.x, fine, coarse, anchor != 0, ec, xf);
+  mg_prolong_rows<W, T>(xcd_perm(blockIdx.x, gridDim.x, xg), gridDim.x, fine, coarse, anchor != 0, ec, xf);
 }
 
 template <int K, class T>
@@ -599,7 +605,9 @@ static void vcycle(cpm_mg* M, size_t l, const double* b64, const CgState* st, hi
                    bool first_done = false) {
   MgLevel& Lv = M->L[l];
   hipStream_t s = M->ls ? M->ls : M->ctx->stream;
-  const int gn = grid_for(Lv.nb);
+  // the level's sweep grid and XCD order (the coarsest level has no sweeps)
+  const int gn = l + 1 == M->L.size() ? grid_for(Lv.nb) : sell_grid(Lv.A->sell);
+  const int32_t xg = Lv.A->sell.xg;
   if (l + 1 == M->L.size()) {
     const int g = (int)std::min<int64_t>(kMaxGrid, (Lv.n + 3) / 4);
     k_mg_dense<K, T><<<g, kBlock, 0, s>>>(Lv.n, (const T*)M->cinv, (const T*)Lv.b, (T*)Lv.x, st);
@@ -615,12 +623,12 @@ static void vcycle(cpm_mg* M, size_t l, const double* b64, const CgState* st, hi
   if (l == 0 && b64) {
     const int gf = grid_for(Lv.n * K);
     k_mg_first<BS, K, T, true, true><<<gn, kBlock, 0, s>>>(Lv.nb, b64, (T*)Lv.b, (T*)Lv.r, (T*)d[0], (T*)Lv.x,
-                                                           (const T*)Lv.dinv, c0, st,
+                                                           (const T*)Lv.dinv, c0, st, xg,
                                                            M->floating ? M->lpart : nullptr, gf, gf);
   }
   else if (!first_done)
     k_mg_first<BS, K, T, true, false><<<gn, kBlock, 0, s>>>(Lv.nb, nullptr, (T*)Lv.b, (T*)Lv.r, (T*)d[0],
-                                                            (T*)Lv.x, (const T*)Lv.dinv, c0, st);
+                                                            (T*)Lv.x, (const T*)Lv.dinv, c0, st, xg);
   for (int k = 1; k < m; ++k)
     sweep<BS, K, T, 0>(M, Lv, d[(k - 1) & 1], d[k & 1], c1[k], c2[k], st, (k == 1 && ev) ? ev : nullptr);
   sweep<BS, K, T, 1>(M, Lv, d[(m - 1) & 1], nullptr, 0.0, 0.0, st);
@@ -629,16 +637,16 @@ static void vcycle(cpm_mg* M, size_t l, const double* b64, const CgState* st, hi
   const Dims3 fd{{Lv.dims[0], Lv.dims[1], Lv.dims[2]}}, cd{{Lc.dims[0], Lc.dims[1], Lc.dims[2]}};
   if (l + 2 < M->L.size()) {
     // restriction fused with the coarse level's first step
-    k_mg_restrict_first<BS, K, T><<<grid_for(Lc.nb), kBlock, 0, s>>>(fd, cd, M->anchor, (const T*)Lv.r, (T*)Lc.b,
-                                                                    (T*)Lc.r, (T*)Lc.d0, (T*)Lc.x,
-                                                                    (const T*)Lc.dinv, M->c0[l + 1], st);
+    k_mg_restrict_first<BS, K, T><<<sell_grid(Lc.A->sell), kBlock, 0, s>>>(
+        fd, cd, M->anchor, (const T*)Lv.r, (T*)Lc.b, (T*)Lc.r, (T*)Lc.d0, (T*)Lc.x, (const T*)Lc.dinv, M->c0[l + 1],
+        st, Lc.A->sell.xg);
     vcycle<BS, K, T>(M, l + 1, nullptr, st, nullptr, true);
   } else {
     k_mg_restrict<BS * K, K, T><<<grid_for(Lc.nb), kBlock, 0, s>>>(fd, cd, M->anchor, (const T*)Lv.r, (T*)Lc.b,
                                                                   st);
     vcycle<BS, K, T>(M, l + 1, nullptr, st);
   }
-  k_mg_prolong<BS * K, K, T><<<gn, kBlock, 0, s>>>(fd, cd, M->anchor, (const T*)Lc.x, (T*)Lv.x, st);
+  k_mg_prolong<BS * K, K, T><<<gn, kBlock, 0, s>>>(fd, cd, M->anchor, (const T*)Lc.x, (T*)Lv.x, st, xg);
   // post-smoothing: the same polynomial on the new residual (degree >= 2:
   // the residual and the first step's direction in one pass, its x update
   // in the second step's pass)
@@ -649,7 +657,7 @@ static void vcycle(cpm_mg* M, size_t l, const double* b64, const CgState* st, hi
   } else {
     sweep<BS, K, T, 2>(M, Lv, nullptr, nullptr, 0.0, 0.0, st);
     k_mg_first<BS, K, T, false, false><<<gn, kBlock, 0, s>>>(Lv.nb, nullptr, (T*)Lv.b, (T*)Lv.r, (T*)d[0],
-                                                             (T*)Lv.x, (const T*)Lv.dinv, c0, st);
+                                                             (T*)Lv.x, (const T*)Lv.dinv, c0, st, xg);
   }
 }
 
