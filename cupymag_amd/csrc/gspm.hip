@@ -1071,7 +1071,9 @@ static int solve_el_and_demag(cpm_sim* S, const SolveSpec& se, int use_init, cpm
     if (drc < 0) derr = cpm_last_error();
     (void)hipEventRecord(ctx->ev_join, ctx->stream2);
   });
-  const int rc = solve_one(S, se, use_init, rep);
+  PcgLane l1 = lane1(ctx);
+  l1.shared = 1;
+  const int rc = solve_one(S, se, use_init, rep, &l1);
   demag.join();
   CPM_HIP(hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
   rep->iters[sd.slot] = drep.iters[sd.slot];

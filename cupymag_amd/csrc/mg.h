@@ -22,6 +22,10 @@ struct PcgLane {
   double* partials = nullptr;
   void* pinned = nullptr;
   int prof = 0;  // time the level-0 kernels of the first iteration of a batch (HIP events of the context)
+  // another solve runs beside this one while its first batch runs: the
+  // timed iteration of that batch is its last, so that the timings are of
+  // the kernels alone
+  int shared = 0;
 };
 
 struct MgLevel {

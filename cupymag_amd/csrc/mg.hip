@@ -877,9 +877,9 @@ static int pcg_run(cpm_mg* M, const PcgLane& ln, const double* b, double* x, int
   precond(nullptr);
   CPM_HIP(hipGetLastError());
   // profiling: the first Chebyshev step of level 0 in the pre- and in the
-  // post-smoothing of the first iteration of every batch (launched
-  // directly, the rest of the batch from the captured chunk) is timed with
-  // HIP events and accumulated on the level-0 operator
+  // post-smoothing of one iteration of every batch (launched directly, the
+  // rest of the batch from the captured chunks) is timed with HIP events and
+  // accumulated on the level-0 operator
   const bool prof = ln.prof && M->degree >= 2 && K == 1;
   auto iterate = [&](int todo, hipEvent_t* ev) -> int {
     for (int k = 0; k < todo; ++k) {
@@ -947,23 +947,26 @@ static int pcg_run(cpm_mg* M, const PcgLane& ln, const double* b, double* x, int
   };
   if (M->graphs)
     if (int rc = capture(2)) return rc;
-  // todo iterations: the first one launched directly when profiling (timed
-  // with events) or when there are no graphs, the rest from the chunks
-  auto issue = [&](int todo) -> int {
-    int done_direct = 0;
-    if (!gexec[2] || prof) {
-      done_direct = gexec[2] ? 1 : todo;
-      if (int rc = iterate(done_direct, prof ? &ctx->ev[0] : nullptr)) return rc;
-    }
-    if (!gexec[2]) return CPM_OK;
-    int k = done_direct;
-    for (; k + 2 <= todo; k += 2) CPM_HIP(hipGraphLaunch(gexec[2], s));
-    if (k < todo) {
+  // cnt iterations from the chunks (launched directly when there are none)
+  auto chunked = [&](int cnt) -> int {
+    if (cnt <= 0) return CPM_OK;
+    if (!gexec[2]) return iterate(cnt, nullptr);
+    int k = 0;
+    for (; k + 2 <= cnt; k += 2) CPM_HIP(hipGraphLaunch(gexec[2], s));
+    if (k < cnt) {
       if (!gexec[1])
         if (int rc = capture(1)) return rc;
       CPM_HIP(hipGraphLaunch(gexec[1], s));
     }
     return CPM_OK;
+  };
+  // todo iterations; with tk >= 0 the iteration tk of them is launched
+  // directly and timed with events
+  auto issue = [&](int todo, int tk) -> int {
+    if (tk < 0) return chunked(todo);
+    if (int rc = chunked(tk)) return rc;
+    if (int rc = iterate(1, &ctx->ev[0])) return rc;
+    return chunked(todo - tk - 1);
   };
   CgState h[K];
   auto read = [&]() -> int {
@@ -993,13 +996,18 @@ static int pcg_run(cpm_mg* M, const PcgLane& ln, const double* b, double* x, int
   while (!all() && issued <= maxiter) {
     const int todo = std::max(1, std::min(std::min(batch, 1024), maxiter - issued + 1));
     const int it_before = h[0].it;
-    if (int rc = issue(todo)) return rc;
+    // the timed iteration: the first of the batch, or the last of the
+    // first batch when another solve runs beside this one (its kernels
+    // then share the GPU with the other solve's early iterations)
+    const int tk = !prof ? -1 : (ln.shared && issued == 0) ? todo - 1 : 0;
+    if (int rc = issue(todo, tk)) return rc;
     issued += todo;
     if (int rc = read()) return rc;
-    if (prof) {
-      // the directly launched first iteration of the batch did V-cycle work
-      // unless the solve converged in it
-      const bool worked = h[0].it > it_before + 1 || (h[0].it == it_before + 1 && (!h[0].done || h[0].fail));
+    if (tk >= 0) {
+      // the timed iteration did V-cycle work unless the solve converged
+      // before or in it
+      const int base = it_before + tk;
+      const bool worked = h[0].it > base + 1 || (h[0].it == base + 1 && (!h[0].done || h[0].fail));
       if (worked)
         for (int h2 = 0; h2 < 2; ++h2) {
           float ms = 0.f;
@@ -1008,7 +1016,7 @@ static int pcg_run(cpm_mg* M, const PcgLane& ln, const double* b, double* x, int
           A0->prof_launches += 1;
         }
       // the outer product of that iteration ran whenever the iteration did
-      if (rel && S.stencil && h[0].it > it_before) {
+      if (rel && S.stencil && h[0].it > base) {
         float ms = 0.f;
         CPM_HIP(hipEventElapsedTime(&ms, ctx->ev[4], ctx->ev[5]));
         A0->prof2_ms += ms;
