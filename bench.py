@@ -54,21 +54,23 @@ def mode_kwargs(base, opts, mg_degree=2, mg_ratio=10.0):
     (warm start from the extrapolation of the last two solutions), '+mg'
     (multigrid for the elasticity, demag and Gauss-Seidel A1 solves; FP16
     level values in the mixed mode), '+mged' (multigrid for elasticity and
-    demag only), '+mg32' (FP32 level values)."""
+    demag only), '+mg32' (FP32 level values), '+seq' (the demag solve after
+    the elasticity solve instead of beside it)."""
     mg = "mg" in opts or "mged" in opts or "mg32" in opts
     mgp = (32 if "mg32" in opts else 16) if base == "mixed" else 64
     warm = "extrapolate" if "xwarm" in opts else ("warm" in opts)
     return dict(warm_start=warm, precond="mg" if mg else None, mg_gs="mged" not in opts,
-                mg_degree=mg_degree, mg_ratio=mg_ratio, mg_precision=mgp if mg else None)
+                mg_degree=mg_degree, mg_ratio=mg_ratio, mg_precision=mgp if mg else None,
+                overlap="seq" not in opts)
 
 
-def kernel_bytes(base, mg, mgp, nnz, rows, stencil=True, half=False, half32=False):
+def kernel_bytes(base, mg, mgp, nnz, rows, stencil=True, half=False, half32=False, half64=False):
     """Algorithmic bytes per launch of a mode's roofline kernel and its name
     (rocprofv3 spelling).  nnz, rows: of the elasticity operator A_el (3x3
     blocks, nb = rows / 3 block rows, nnz / 9 blocks of which nb diagonal).
     half: the level-0 FP16 form keeps half of the symmetric stencil (each
-    off-diagonal block stored once, for the row and its mirror); half32: the
-    FP32 form of A_el does."""
+    off-diagonal block stored once, for the row and its mirror); half32 and
+    half64: the FP32 and the FP64 forms of A_el do."""
     nb = rows // 3
     off = nnz / 9.0 - nb
     if base == "fp64":
@@ -98,7 +100,7 @@ def kernel_bytes(base, mg, mgp, nnz, rows, stencil=True, half=False, half32=Fals
     # values, p read, q written
     s_ = 8 if base == "fast64" else 4
     idx = 0 if stencil else 4
-    hs = half32 and s_ == 4
+    hs = half32 if s_ == 4 else half64
     form = ("half-stored symmetric stencil" if hs else "stencil") if stencil else "indexed"
     name = f"k_sell_spmv<3,1,{'double' if s_ == 8 else 'float'}> (A_el, SELL-64 3x3 blocks, {form})"
     roc = (f"k_sell_spmv<3, 1, {'double' if s_ == 8 else 'float'}, true, false, false, "
@@ -214,7 +216,7 @@ def run_gpu(n, steps, warmup, steps64, modes, tol, ref_steps=REF_STEPS, rel_delt
         half = bool(mg and mgp == 16 and S.multigrid(precision=mgp, degree=mg_degree,
                                                       ratio=mg_ratio)["A_el"].info()["half_stencil"][0])
         kname, rocname, bytes_launch = kernel_bytes(base, mg, mgp, nnz, rows, info["stencil"], half,
-                                                    info["half32"])
+                                                    info["half32"], info["half64"])
         avg_s = ms.value / 1e3 / max(nl.value, 1)
         d = dict(elapsed=elapsed, steps=k_timed, lat=lat, iters=iters, snaps=snaps,
                  spmv=dict(kernel=kname, rocname=rocname, launches=nl.value, avg_us=avg_s * 1e6,

@@ -155,13 +155,13 @@ static int tune_sell_t(cpm_ctx* ctx, cpm_mat* A, hipEvent_t e0, hipEvent_t e1, d
   const T* val;
   const double* dg;
   if constexpr (sizeof(T) == 8) {
-    val = S.v64;
+    val = sell_v64(S);
     dg = S.dg64;
   } else {
     val = S.v32;
     dg = S.dgc;
   }
-  const GridArg ga = sizeof(T) == 8 ? CPM_GRID_ARG(S) : CPM_GRID_ARG32(S);
+  const GridArg ga = sizeof(T) == 8 ? CPM_GRID_ARG64(S) : CPM_GRID_ARG32(S);
   const int saved = S.grid_cap;
   *best_us = 1e30f;
   *best_cap = saved;

@@ -559,6 +559,12 @@ static void launch_sweep(hipStream_t s, int g, const Sell& S, const VT* val, con
   // the level's own FP32 form
   GridArg ga = grid_arg(S);
   if (sizeof(VT) == 2 ? Lv.khalf != 0 : ((const void*)val == (const void*)S.v32 && S.half32)) ga.kuhn = 2;
+  // the level's own FP64 form: its half copy when there is one
+  if constexpr (sizeof(VT) == 8)
+    if (val == S.v64 && S.half64) {
+      val = S.v64h;
+      ga = CPM_GRID_ARG64(S);
+    }
   if (ev)
     hipExtLaunchKernelGGL(k_mg_sweep<BS, K, T, MODE, ST, VT>, dim3(g), dim3(kBlock), 0, s, ev[0], ev[1], 0,
                           (int32_t)S.nbrows, (const int64_t*)S.sptr, (const int32_t*)S.col, val, dg, ga,
@@ -832,18 +838,18 @@ static int pcg_run(cpm_mg* M, const PcgLane& ln, const double* b, double* x, int
     }
     if (plain) {
       if (S.stencil)
-        k_pcg_spmv<BS, K, true, true><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), in,
-                                                            out, st, partials);
+        k_pcg_spmv<BS, K, true, true><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, sell_v64(S), S.dg64,
+                                                            CPM_GRID_ARG64(S), in, out, st, partials);
       else
-        k_pcg_spmv<BS, K, false, true><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), in,
-                                                             out, st, partials);
+        k_pcg_spmv<BS, K, false, true><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, sell_v64(S), S.dg64,
+                                                             CPM_GRID_ARG64(S), in, out, st, partials);
     } else {
       if (S.stencil)
-        k_pcg_spmv<BS, K, true, false><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), in,
-                                                             out, st, partials);
+        k_pcg_spmv<BS, K, true, false><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, sell_v64(S), S.dg64,
+                                                             CPM_GRID_ARG64(S), in, out, st, partials);
       else
-        k_pcg_spmv<BS, K, false, false><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S),
-                                                              in, out, st, partials);
+        k_pcg_spmv<BS, K, false, false><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, sell_v64(S), S.dg64,
+                                                              CPM_GRID_ARG64(S), in, out, st, partials);
     }
   };
   // floating hierarchy (one column): z = F^T V(F r) + e_0 r_0, the sums of
@@ -893,11 +899,11 @@ static int pcg_run(cpm_mg* M, const PcgLane& ln, const double* b, double* x, int
       if (rel) {
         k_red<K, RED_RR_PCG_REL><<<1, kRedBlock, 0, s>>>(gv, partials, st, delta);
         if (S.stencil)
-          k_pcg_true<BS, K, true><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), x, b,
-                                                        M->r, st, partials);
+          k_pcg_true<BS, K, true><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, sell_v64(S), S.dg64,
+                                                        CPM_GRID_ARG64(S), x, b, M->r, st, partials);
         else
-          k_pcg_true<BS, K, false><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v64, S.dg64, grid_arg(S), x, b,
-                                                         M->r, st, partials);
+          k_pcg_true<BS, K, false><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, sell_v64(S), S.dg64,
+                                                         CPM_GRID_ARG64(S), x, b, M->r, st, partials);
         k_red<K, RED_RR_PCG_TRUE><<<1, kRedBlock, 0, s>>>(gs, partials, st, 0.0);
       } else {
         k_red<K, RED_RR_PCG><<<1, kRedBlock, 0, s>>>(gv, partials, st, 0.0);

@@ -29,11 +29,12 @@ namespace cpm {
 struct BSys {
   const int64_t* sptr;
   const int32_t* col;
-  const double* v64;
+  const double* v64;  // the FP64 products' values (sell_v64)
   const float* v32;
   const double* dg64;
   const double* dgc;
-  GridArg ga;
+  GridArg ga;    // for v32
+  GridArg ga64;  // for v64
   int32_t nbrows, stencil;
   int64_t n;
   const double* b;
@@ -86,12 +87,13 @@ __global__ void __launch_bounds__(kBlock) kb_spmv(const BSys* __restrict__ sys, 
 #pragma unroll
   for (int j = 0; j < K; ++j) acc[j] = 0.0;
   const int64_t lb = xcd_perm((int64_t)blockIdx.x - d.blk0, d.nblk, d.ga.xg);
+  const GridArg& ga = sizeof(T) == 8 ? d.ga64 : d.ga;
   if (d.stencil)
     sell_rows<BS, K, T, OP == OP_CG, OP == OP_REL, true>(lb, d.nblk, d.nbrows, d.sptr, d.col, val, dg, x, x32, y,
-                                                          d.ga, acc);
+                                                          ga, acc);
   else
     sell_rows<BS, K, T, OP == OP_CG, OP == OP_REL, false>(lb, d.nblk, d.nbrows, d.sptr, d.col, val, dg, x, x32, y,
-                                                           d.ga, acc);
+                                                           ga, acc);
   if (OP == OP_CG) block_partial<K>(acc, partials);
 }
 
@@ -260,11 +262,12 @@ static int multi_loop(cpm_ctx* ctx, MultiSolver& M, const std::vector<MultiSys>&
     BSys& d = M.h_sys[j];
     d.sptr = A.sptr;
     d.col = A.col;
-    d.v64 = A.v64;
+    d.v64 = sell_v64(A);
     d.v32 = A.v32;
     d.dg64 = A.dg64;
     d.dgc = A.dgc;
-    d.ga = CPM_GRID_ARG32(A);  // for v32; the FP64 products ignore the half-storage flag
+    d.ga = CPM_GRID_ARG32(A);
+    d.ga64 = CPM_GRID_ARG64(A);
     d.nbrows = A.nbrows;
     d.stencil = A.stencil;
     d.n = m.n;

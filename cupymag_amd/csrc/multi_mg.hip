@@ -26,11 +26,13 @@ namespace cpm {
 struct OuterSeg {
   const int64_t* sptr;
   const int32_t* col;
-  const double* v64;
+  const double* v64;      // every slot (the FP64 fallback of the FP32 products)
+  const double* v64h;     // the other FP64 products' values (sell_v64)
   const double* dg64;
-  const float* v32;      // FP32 form (reliable-update systems)
+  const float* v32;       // FP32 form (reliable-update systems)
   const double* dgc;
-  GridArg ga;
+  GridArg ga;             // for v32 and v64
+  GridArg ga64;           // for v64h
   int32_t nb, stencil;
   int64_t n;
   const double* b;
@@ -164,12 +166,18 @@ __global__ void __launch_bounds__(kBlock) kbm_spmv(const OuterSeg* __restrict__ 
     else
       sell_rows<BS, K, double, true, false, false, float>(lb, d.snblk, d.nb, d.sptr, d.col, d.v32, d.dgc, in,
                                                           nullptr, d.q, d.ga, acc);
-  } else if (d.stencil) {
-    sell_rows<BS, K, double, !PLAIN, false, true>(lb, d.snblk, d.nb, d.sptr, d.col, d.v64, d.dg64, in, nullptr, d.q,
-                                                  d.ga, acc);
   } else {
-    sell_rows<BS, K, double, !PLAIN, false, false>(lb, d.snblk, d.nb, d.sptr, d.col, d.v64, d.dg64, in, nullptr, d.q,
-                                                   d.ga, acc);
+    // FP64 products: the half copy, except for the FP64 fallback of the
+    // reliable-update systems (every slot, as the one-system kernel reads)
+    const bool h = PLAIN || d.delta <= 0.0;
+    const double* v = h ? d.v64h : d.v64;
+    const GridArg& ga = h ? d.ga64 : d.ga;
+    if (d.stencil)
+      sell_rows<BS, K, double, !PLAIN, false, true>(lb, d.snblk, d.nb, d.sptr, d.col, v, d.dg64, in, nullptr, d.q,
+                                                    ga, acc);
+    else
+      sell_rows<BS, K, double, !PLAIN, false, false>(lb, d.snblk, d.nb, d.sptr, d.col, v, d.dg64, in, nullptr, d.q,
+                                                     ga, acc);
   }
   if (!PLAIN) block_partial<K>(acc, partials);
 }
@@ -186,9 +194,9 @@ __global__ void __launch_bounds__(kBlock) kbm_true(const OuterSeg* __restrict__ 
   for (int j = 0; j < K; ++j) acc[j] = 0.0;
   const int64_t lb = xcd_perm((int64_t)blockIdx.x - d.sblk0, d.snblk, d.ga.xg);
   if (d.stencil)
-    pcg_true_rows<BS, K, true>(lb, d.snblk, d.nb, d.sptr, d.col, d.v64, d.dg64, d.ga, d.x, d.b, d.r, d.st, acc);
+    pcg_true_rows<BS, K, true>(lb, d.snblk, d.nb, d.sptr, d.col, d.v64h, d.dg64, d.ga64, d.x, d.b, d.r, d.st, acc);
   else
-    pcg_true_rows<BS, K, false>(lb, d.snblk, d.nb, d.sptr, d.col, d.v64, d.dg64, d.ga, d.x, d.b, d.r, d.st, acc);
+    pcg_true_rows<BS, K, false>(lb, d.snblk, d.nb, d.sptr, d.col, d.v64h, d.dg64, d.ga64, d.x, d.b, d.r, d.st, acc);
   block_partial<K>(acc, partials);
 }
 
@@ -410,9 +418,11 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
     cpm_mg* M = y.M;
     const Sell& S = M->L[0].A->sell;
     OuterSeg o{};
-    o.sptr = S.sptr, o.col = S.col, o.v64 = S.v64, o.dg64 = S.dg64, o.v32 = S.v32, o.dgc = S.dgc;
+    o.sptr = S.sptr, o.col = S.col, o.v64 = S.v64, o.v64h = sell_v64(S), o.dg64 = S.dg64, o.v32 = S.v32;
+    o.dgc = S.dgc;
     o.delta = y.delta;
-    o.ga = CPM_GRID_ARG32(S);  // for v32; the FP64 products ignore the half-storage flag
+    o.ga = CPM_GRID_ARG32(S);  // v64 holds every slot: the FP64 products read it in full (h64 = 0)
+    o.ga64 = CPM_GRID_ARG64(S);
     o.nb = S.nbrows, o.stencil = S.stencil, o.n = M->L[0].n * K;
     o.b = y.b, o.x = y.x, o.r = M->r, o.p = M->p, o.q = M->q, o.rold = M->rold, o.z = M->L[0].x;
     o.st = W.st + j * K;
@@ -466,6 +476,11 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
         // values keeping half of the stencil: the FP16 form (k_mg_make_half),
         // or the level's own FP32 form
         if (M->half ? Lv.khalf != 0 : (g.val == (const void*)S.v32 && S.half32)) g.ga.kuhn = 2;
+        // the level's own FP64 form: its half copy when there is one
+        if (g.val == (const void*)S.v64 && S.half64) {
+          g.val = S.v64h;
+          g.ga = CPM_GRID_ARG64(S);
+        }
         g.nb = S.nbrows, g.stencil = S.stencil;
         g.x = Lv.x, g.r = Lv.r, g.b = Lv.b, g.d0 = Lv.d0, g.d1 = Lv.d1, g.dinv = Lv.dinv;
         g.b64 = (d == 0) ? M->r : nullptr;

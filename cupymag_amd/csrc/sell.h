@@ -30,6 +30,13 @@ struct Sell {
   // only, 7 planes per slice ([slice*7 + u][bs*bs][64]), read with
   // GridArg::kuhn = 2 (CPM_GRID_ARG32)
   int half32 = 0;
+  // the same for the FP64 products of the solvers when the stencil is
+  // symmetric to rounding (1e-14 of its largest entry): v64h holds the
+  // forward slots of v64 ([slice*7 + u][bs*bs][64]), read with
+  // GridArg::kuhn = 2 and h64 = 1 (CPM_GRID_ARG64); v64 keeps every slot
+  // for the multigrid set-up and the FP64 fallback of the FP32 products
+  double* v64h = nullptr;
+  int half64 = 0;
   int grid_cap = 8192;       // workgroup cap of the SpMV launches (tunable, grid-stride beyond)
   int xg = 0;                // XCD group of the row launches of a stencil form (GridArg::xg)
   int shz = 0, shy = 0;      // log2(dims[2]) + 1, log2(dims[1]) + 1 for powers of two, else 0 (GridArg)
@@ -41,6 +48,13 @@ struct Sell {
 // the FP64 products of a kernel that reads both ignore the difference)
 #define CPM_GRID_ARG32(S) \
   (GridArg{(S).dims[0], (S).dims[1], (S).dims[2], (S).nst, (S).soff, (S).kuhn ? ((S).half32 ? 2 : 1) : 0, (S).xg, (S).shz, (S).shy})
+
+// ... for the FP64 products of the solvers (sell_v64: the half form when
+// there is one)
+#define CPM_GRID_ARG64(S)                                                                                           \
+  (GridArg{(S).dims[0], (S).dims[1], (S).dims[2], (S).nst, (S).soff, (S).kuhn ? ((S).half64 ? 2 : 1) : 0, (S).xg, \
+           (S).shz, (S).shy, (S).half64})
+inline const double* sell_v64(const Sell& S) { return S.half64 ? S.v64h : S.v64; }
 
 // XCD group (xcd_perm) of the row launches of a stencil form on a grid of
 // these dims: the largest power of two G <= 64 for which 8 G workgroups of

@@ -190,6 +190,21 @@ int sell_build(cpm_ctx* ctx, Sell& S, int bs, int32_t nbrows, const int64_t* ind
       kuhn_half32(bs, nbrows, nsl, ga, val.data(), dg.data(), v32.data(), dgc.data());
     }
   }
+  // FP64 half form when the mirrored blocks agree to rounding (the
+  // elasticity blocks K_ab and K_ba^T are sums of the same products in
+  // different orders: they differ in the last bits; scalar operators are
+  // symmetric exactly, and their half products equal the full ones)
+  S.half64 = 0;
+  std::vector<double> v64h;
+  if (S.half32) {
+    const GridArg ga{dims[0], dims[1], dims[2], kKuhnSlots, nullptr, 1};
+    if (kuhn_slot_symmetric(bs, nbrows, ga, val.data(), 1e-14)) {
+      S.half64 = 1;
+      v64h.resize(nsl * 7 * bb * 64);
+      for (int64_t s = 0; s < nsl; ++s)
+        std::memcpy(&v64h[s * 7 * bb * 64], &val[(s * kKuhnSlots + 7) * bb * 64], sizeof(double) * 7 * bb * 64);
+    }
+  }
   if (!S.half32) {
     v32.resize(val.size());
     for (size_t i = 0; i < val.size(); ++i) v32[i] = (float)val[i];
@@ -211,12 +226,17 @@ int sell_build(cpm_ctx* ctx, Sell& S, int bs, int32_t nbrows, const int64_t* ind
   if (!stencil) CPM_HIP(hipMemcpyAsync(S.col, col.data(), sizeof(int32_t) * col.size(), hipMemcpyHostToDevice, st));
   CPM_HIP(hipMemcpyAsync(S.v64, val.data(), sizeof(double) * val.size(), hipMemcpyHostToDevice, st));
   CPM_HIP(hipMemcpyAsync(S.v32, v32.data(), sizeof(float) * v32.size(), hipMemcpyHostToDevice, st));
+  if (S.half64) {
+    CPM_HIP(hipMalloc(&S.v64h, sizeof(double) * v64h.size()));
+    CPM_HIP(hipMemcpyAsync(S.v64h, v64h.data(), sizeof(double) * v64h.size(), hipMemcpyHostToDevice, st));
+  }
   CPM_HIP(hipStreamSynchronize(st));
   return CPM_OK;
 }
 
 void sell_free(Sell& S) {
-  for (void* p : {(void*)S.sptr, (void*)S.col, (void*)S.v64, (void*)S.v32, (void*)S.dg64, (void*)S.dgc, (void*)S.soff})
+  for (void* p : {(void*)S.sptr, (void*)S.col, (void*)S.v64, (void*)S.v64h, (void*)S.v32, (void*)S.dg64, (void*)S.dgc,
+                  (void*)S.soff})
     if (p) (void)hipFree(p);
   S = Sell{};
 }
@@ -309,13 +329,13 @@ static int spmv_t(cpm_ctx* ctx, const Sell& A, const T* x, const float* x32, T* 
   const T* val;
   const double* dg;
   if constexpr (sizeof(T) == 8) {
-    val = A.v64;
+    val = sell_v64(A);
     dg = A.dg64;
   } else {
     val = A.v32;
     dg = A.dgc;
   }
-  const GridArg ga = sizeof(T) == 8 ? CPM_GRID_ARG(A) : CPM_GRID_ARG32(A);
+  const GridArg ga = sizeof(T) == 8 ? CPM_GRID_ARG64(A) : CPM_GRID_ARG32(A);
   const dim3 grid(sell_grid(A)), block(kBlock);
   if (ev) {
     if (A.stencil)
@@ -559,7 +579,8 @@ extern "C" int cpm_mat_add_sell(cpm_mat* A, int bs, const int64_t* indptr, const
 extern "C" int cpm_mat_sell_info(const cpm_mat* A, int* bs, int* stencil, int64_t* slots) {
   if (!A) return cpm::fail(CPM_EINVAL, "cpm_mat_sell_info: null matrix");
   if (bs) *bs = A->sell.bs;
-  if (stencil) *stencil = A->sell.stencil ? (A->sell.kuhn ? (A->sell.half32 ? 3 : 2) : 1) : 0;
+  if (stencil)
+    *stencil = A->sell.stencil ? (A->sell.kuhn ? (A->sell.half64 ? 4 : A->sell.half32 ? 3 : 2) : 1) : 0;
   if (slots) *slots = A->sell.nslot;
   return CPM_OK;
 }

@@ -28,13 +28,16 @@ namespace cpm {
 // kKuhnCodes (the host checks soff), so the kernels decode them at compile
 // time; 2: besides, the FP32 or FP16 values passed with it keep half of the
 // stencil (the forward slots 7..13, 7 planes per slice: Sell::half32,
-// k_mg_make_half); FP64 values always hold every slot.
+// k_mg_make_half).  FP64 values hold every slot unless h64 is set as well
+// (Sell::half64), so one grid argument serves a kernel that reads both an
+// FP32 half form and the full FP64 form.
 struct GridArg {
   int32_t nx, ny, nz, nst;
   const int32_t* soff;
   int32_t kuhn;
   int32_t xg;        // XCD group of the row launches (xcd_perm); 0 or 1: dispatch order
   int32_t shz, shy;  // log2(nz) + 1 and log2(ny) + 1 when nz and ny are powers of two, else 0
+  int32_t h64;       // with kuhn = 2: the FP64 values passed with it keep half of the stencil too
 };
 
 // Slot codes (di+1) | (dj+1) << 2 | (dk+1) << 4 of the neighbours of a node
@@ -63,7 +66,7 @@ __host__ __device__ __forceinline__ int64_t kuhn_neighbour(int64_t row, const Gr
 // Host, at build time, for a Kuhn stencil form with FP64 values val
 // ([slot][bs*bs][64], slot = slice*14 + t): whether A(i, i-d) = A(i-d, i)^T
 // for every row and backward slot, to 1e-12 of the largest entry.
-inline bool kuhn_slot_symmetric(int bs, int32_t nbrows, const GridArg& ga, const double* val) {
+inline bool kuhn_slot_symmetric(int bs, int32_t nbrows, const GridArg& ga, const double* val, double tol = 1e-12) {
   const int bb = bs * bs;
   double amax = 0.0, dmax = 0.0;
   for (int64_t b = 0; b < nbrows; ++b) {
@@ -80,7 +83,7 @@ inline bool kuhn_slot_symmetric(int bs, int32_t nbrows, const GridArg& ga, const
         }
     }
   }
-  return dmax <= 1e-12 * amax;
+  return dmax <= tol * amax;
 }
 
 // Host: the FP32 half form of a slot-symmetric Kuhn stencil operator,
@@ -253,9 +256,10 @@ __host__ __device__ __forceinline__ void sell_row(int64_t row, int32_t nbrows, c
         }
       }
     };
-    if constexpr (sizeof(VT) <= 4) {
-      if (ga.kuhn == 2) {
-        // half storage (FP32 and FP16 forms of slot-symmetric operators):
+    {
+      if (ga.kuhn == 2 && (sizeof(VT) <= 4 || ga.h64)) {
+        // half storage (FP32 and FP16 forms of slot-symmetric operators, the
+        // FP64 form of those symmetric to rounding):
         // slice s holds planes s*7 + u with the forward blocks of slots 7 + u
         // (offsets +d); the backward block of slot t < 7 (offset -d) is
         // A(i, i-d) = A(i-d, i)^T, the forward block of slot 13 - t (plane
@@ -276,7 +280,7 @@ __host__ __device__ __forceinline__ void sell_row(int64_t row, int32_t nbrows, c
               xv[v] = (AT)x[(int64_t)c * NV + v];
               if (ADDX32) xv[v] += (AT)x32[(int64_t)c * NV + v];
             }
-            if constexpr (sizeof(VT) == 4) {
+            if constexpr (sizeof(VT) >= 4) {
               AT av[BB];
 #pragma unroll
               for (int q = 0; q < BB; ++q) av[q] = (AT)val[(plane * BB + q) * 64 + lane];
@@ -318,8 +322,6 @@ __host__ __device__ __forceinline__ void sell_row(int64_t row, int32_t nbrows, c
       } else {
         kuhn_full();
       }
-    } else {
-      kuhn_full();
     }
   } else {
     int64_t slot = s0;

@@ -115,7 +115,8 @@ class DeviceCSR:
     def sell_info(self):
         bs, st, sl = C.c_int(), C.c_int(), C.c_int64()
         check(lib().cpm_mat_sell_info(self.h, C.byref(bs), C.byref(st), C.byref(sl)), "cpm_mat_sell_info")
-        return dict(bs=bs.value, stencil=bool(st.value), kuhn=st.value >= 2, half32=st.value == 3, slots=sl.value)
+        return dict(bs=bs.value, stencil=bool(st.value), kuhn=st.value >= 2, half32=st.value >= 3,
+                    half64=st.value == 4, slots=sl.value)
 
     def matvec(self, x, out=None, valset=0):
         x = as_device(x, self.ctx)

@@ -42,12 +42,15 @@ from .workloads import initial_m
 
 # device bytes per DOF of a member (operators in CSR and SELL-64 forms with
 # the matrix-free F_el, the step's state vectors, the batched-CG workspace
-# of the mixed mode)
-_BYTES_PER_DOF = {False: 2200.0, True: 5700.0}
+# of the mixed mode; on Kuhn meshes the half FP64 copies of the symmetric
+# SELL forms: 7 off-diagonal values of A_demag, A1 and A2, 7 3x3 blocks of
+# A_el)
+_BYTES_PER_DOF = {False: 2200.0 + 170.0, True: 5700.0 + 680.0}
 # FP32 multigrid hierarchies per fine DOF (demag, Gauss-Seidel A1 with three
 # columns, elasticity): coarse operators (1/8 + 1/64 + ... of the fine SELL
-# forms), level vectors, the FP64 outer CG vectors
-_MG_BYTES_PER_DOF = {False: 500.0, True: 1200.0}
+# forms, with their half FP64 copies), level vectors, the FP64 outer CG
+# vectors
+_MG_BYTES_PER_DOF = {False: 500.0 + 20.0, True: 1200.0 + 90.0}
 
 
 # FP64 hierarchies (the fast64 mode) add per fine DOF: the FP64 level vectors

@@ -150,8 +150,10 @@ int cpm_mat_add_sell(cpm_mat* A, int bs, const int64_t* indptr, const int32_t* i
  * if those offsets are the 14 neighbours of a Kuhn mesh (decoded at compile
  * time by the kernels), 3 if besides the operator is symmetric slot by slot
  * and its FP32 form keeps half of the stencil (each off-diagonal block once,
- * read transposed for the mirrored neighbour), 0 if indexed; and the number
- * of 64-row slots (of the FP64 form). */
+ * read transposed for the mirrored neighbour), 4 if moreover it is
+ * symmetric to rounding (1e-14 of its largest entry) and the FP64 products
+ * of the solvers read a half copy of the FP64 form the same way, 0 if
+ * indexed; and the number of 64-row slots (of the full FP64 form). */
 int cpm_mat_sell_info(const cpm_mat* A, int* bs, int* stencil, int64_t* slots);
 /* CG on the SELL form for nrhs interleaved right-hand sides (b[i*nrhs + j];
  * nrhs = 3 only for bs = 1; bs = 3 vectors are node-major interleaved).

@@ -439,6 +439,7 @@ def test_stencil_and_indexed_sell_agree(grid):
     assert A_el.sell_info()["stencil"] and A1.sell_info()["stencil"]
     assert A_el.sell_info()["kuhn"] == A1.sell_info()["kuhn"] == (grid == "Tet")
     assert A_el.sell_info()["half32"] == A1.sell_info()["half32"] == (grid == "Tet")
+    assert A_el.sell_info()["half64"] == A1.sell_info()["half64"] == (grid == "Tet")
     rng = np.random.default_rng(3)
     for A, bs, K in ((A_el, 3, 1), (A1, 1, 3)):
         ip, ix, v = A.to_arrays()
