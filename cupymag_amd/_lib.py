@@ -86,6 +86,7 @@ _SIGS = {
     "cpm_cg_precond": (C.c_int, [_P, _P, _P, _P, _P, C.c_int, _D, C.c_int, C.POINTER(_I32), C.POINTER(_D)]),
     "cpm_cg_noalloc": (C.c_int, [_P, _P, _P, _P, C.c_int, _D, C.c_int, C.POINTER(_I32), C.POINTER(_D)]),
     "cpm_prof_enable": (C.c_int, [_P, C.c_int]),
+    "cpm_prof_only": (C.c_int, [_P, _P]),
     "cpm_mat_prof": (C.c_int, [_P, C.POINTER(_D), C.POINTER(_I64), C.c_int]),
     "cpm_mat_prof_outer": (C.c_int, [_P, C.POINTER(_D), C.POINTER(_I64), C.c_int]),
     "cpm_fel_strain": (C.c_int, [_P, _P, _P, _P]),

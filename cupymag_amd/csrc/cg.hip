@@ -278,7 +278,8 @@ int run_cg(cpm_ctx* ctx, const cpm_mat* A, const double* b, double* x, int use_x
   CPM_HIP(hipGetLastError());
   CgState h{};
   int batch = hint > 2 ? hint - 1 : 8;
-  const int max_batch = ctx->prof ? 256 : 4096;
+  const bool prof = prof_on(ctx, A);
+  const int max_batch = prof ? 256 : 4096;
   int issued = 0;
   for (;;) {
     const int todo = std::max(1, std::min(std::min(batch, max_batch), maxiter - issued));
@@ -292,10 +293,10 @@ int run_cg(cpm_ctx* ctx, const cpm_mat* A, const double* b, double* x, int use_x
       } else {
         k_cg_pupd<<<gv, kBlock, 0, s>>>(n, w.r, w.p, st);
       }
-      if (ctx->prof) CPM_HIP(hipEventRecord(ctx->ev[2 * k], s));
+      if (prof) CPM_HIP(hipEventRecord(ctx->ev[2 * k], s));
       int gsp = 0;
       CPM_LANES_DISPATCH(A->lanes, launch_cg_spmv_g, ctx, A, w, &gsp);
-      if (ctx->prof) CPM_HIP(hipEventRecord(ctx->ev[2 * k + 1], s));
+      if (prof) CPM_HIP(hipEventRecord(ctx->ev[2 * k + 1], s));
       k_red<1, RED_ALPHA><<<1, kRedBlock, 0, s>>>(gsp, w.partials, st, 0.0);
       k_cg_xr<<<gv, kBlock, 0, s>>>(n, x, w.r, w.p, w.q, st, w.partials);
       if (Mp)
@@ -308,7 +309,7 @@ int run_cg(cpm_ctx* ctx, const cpm_mat* A, const double* b, double* x, int use_x
     CPM_HIP(hipMemcpyAsync(ctx->pinned, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
     CPM_HIP(hipStreamSynchronize(s));
     std::memcpy(&h, ctx->pinned, sizeof(CgState));
-    if (ctx->prof) {
+    if (prof) {
       // only launches that did work (iterations it_before .. h.it-1 of this batch)
       cpm_mat* Am = const_cast<cpm_mat*>(A);
       for (int k = 0; k < h.it - it_before && k < todo; ++k) {
@@ -377,6 +378,13 @@ extern "C" int cpm_prof_enable(cpm_ctx* ctx, int on) {
   if (on && !ctx->ev[0])
     for (auto& e : ctx->ev) CPM_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
   ctx->prof = on ? 1 : 0;
+  return CPM_OK;
+}
+
+extern "C" int cpm_prof_only(cpm_ctx* ctx, const cpm_mat* A) {
+  if (!ctx) return cpm::fail(CPM_EINVAL, "cpm_prof_only: null context");
+  if (A && A->ctx != ctx) return cpm::fail(CPM_EINVAL, "cpm_prof_only: matrix of another context");
+  ctx->prof_mat = A;
   return CPM_OK;
 }
 

@@ -65,8 +65,10 @@ struct cpm_ctx {
   double* noalloc = nullptr;  // device address of the no-allocation CG scratch (cg.hip)
   int64_t ws_len = 0;
   int ncu = 256;
-  // optional timing of every CG SpMV launch with HIP events (cpm_prof_enable)
+  // optional timing of every CG SpMV launch with HIP events (cpm_prof_enable),
+  // of one operator's launches only when prof_mat is set (cpm_prof_only)
   int prof = 0;
+  const cpm_mat* prof_mat = nullptr;
   hipEvent_t ev[2 * 256] = {};
   // FP32 workspace of the mixed-precision solver (+ one FP64 scratch vector)
   float *r32 = nullptr, *p32 = nullptr, *q32 = nullptr, *x32 = nullptr, *b32 = nullptr;
@@ -100,6 +102,11 @@ struct cpm_mat {
 };
 
 namespace cpm {
+
+// timing of the launches on operator A requested (cpm_prof_enable, cpm_prof_only)
+inline bool prof_on(const cpm_ctx* ctx, const cpm_mat* A) {
+  return ctx->prof && (!ctx->prof_mat || ctx->prof_mat == A);
+}
 
 inline int grid_for(int64_t work_items, int items_per_block = kBlock) {
   int64_t g = (work_items + items_per_block - 1) / items_per_block;

@@ -886,7 +886,7 @@ static int pcg_run(cpm_mg* M, const PcgLane& ln, const double* b, double* x, int
   // post-smoothing of one iteration of every batch (launched directly, the
   // rest of the batch from the captured chunks) is timed with HIP events and
   // accumulated on the level-0 operator
-  const bool prof = ln.prof && M->degree >= 2 && K == 1;
+  const bool prof = ln.prof && prof_on(ctx, L0.A) && M->degree >= 2 && K == 1;
   auto iterate = [&](int todo, hipEvent_t* ev) -> int {
     for (int k = 0; k < todo; ++k) {
       pupd();

@@ -386,7 +386,8 @@ static int drive(cpm_ctx* ctx, cpm_mat* M, int maxiter, int hint, Issue&& issue,
   CgState h[K] = {};
   if (int rc = read_states(ctx, K, h)) return rc;
   int batch = hint > 2 ? hint - 1 : 8;
-  const int max_batch = ctx->prof ? 256 : 4096;
+  const bool prof = M && prof_on(ctx, M);
+  const int max_batch = prof ? 256 : 4096;
   int issued = 0;
   bool all_done = true;
   for (int j = 0; j < K; ++j) all_done &= h[j].done != 0;
@@ -403,7 +404,7 @@ static int drive(cpm_ctx* ctx, cpm_mat* M, int maxiter, int hint, Issue&& issue,
       it_max = std::max(it_max, h[j].it);
       all_done &= h[j].done != 0;
     }
-    if (ctx->prof && M) {
+    if (prof) {
       for (int k = 0; k < it_max - it_before && k < todo; ++k) {
         float ms = 0.f;
         CPM_HIP(hipEventElapsedTime(&ms, ctx->ev[2 * k], ctx->ev[2 * k + 1]));
@@ -442,7 +443,8 @@ static int cg64_loop(cpm_ctx* ctx, cpm_mat* M, const double* b, double* x, int u
   auto issue = [&](int todo) -> int {
     for (int k = 0; k < todo; ++k) {
       k_mcg_pupd<K, double><<<g, kBlock, 0, s>>>(n, ctx->r, ctx->p, st);
-      if (int rc = spmv_t<BS, K, double, true>(ctx, A, ctx->p, nullptr, ctx->q, ctx->prof ? &ctx->ev[2 * k] : nullptr))
+      if (int rc = spmv_t<BS, K, double, true>(ctx, A, ctx->p, nullptr, ctx->q,
+                                               prof_on(ctx, M) ? &ctx->ev[2 * k] : nullptr))
         return rc;
       reduce_t<K, RED_ALPHA>(ctx, gs);
       k_mcg_xr<K, double><<<g, kBlock, 0, s>>>(n, x, ctx->r, ctx->p, ctx->q, st, ctx->partials);
@@ -511,7 +513,8 @@ static int rel_loop(cpm_ctx* ctx, cpm_mat* M, const double* b, double* x, int us
   auto issue = [&](int todo) -> int {
     for (int k = 0; k < todo; ++k) {
       k_mcg_pupd<K, float><<<g, kBlock, 0, s>>>(n, ctx->r32, ctx->p32, st);
-      if (int rc = spmv_t<BS, K, float, true>(ctx, A, ctx->p32, nullptr, ctx->q32, ctx->prof ? &ctx->ev[2 * k] : nullptr))
+      if (int rc = spmv_t<BS, K, float, true>(ctx, A, ctx->p32, nullptr, ctx->q32,
+                                              prof_on(ctx, M) ? &ctx->ev[2 * k] : nullptr))
         return rc;
       reduce_t<K, RED_ALPHA>(ctx, gs);
       k_mcg_xr<K, float><<<g, kBlock, 0, s>>>(n, ctx->x32, ctx->r32, ctx->p32, ctx->q32, st, ctx->partials);

@@ -129,6 +129,10 @@ int cpm_cg_noalloc(cpm_ctx* ctx, const cpm_mat* A, const double* b, double* x, i
  * total milliseconds and the number of launches that did work. */
 int cpm_prof_enable(cpm_ctx* ctx, int on);
 int cpm_mat_prof(cpm_mat* A, double* total_ms, int64_t* launches, int reset);
+/* No counterpart in the reference (measurement).  Restrict the timing of
+ * cpm_prof_enable to the launches on operator A (NULL: every operator), so
+ * that the other solves run as they do without profiling. */
+int cpm_prof_only(cpm_ctx* ctx, const cpm_mat* A);
 /* No counterpart in the reference (measurement).  With profiling on, the
  * outer FP32 product of the mixed-precision multigrid PCG on this (level-0)
  * operator is timed by its dispatch in the first iteration of every

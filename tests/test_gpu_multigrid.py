@@ -269,6 +269,49 @@ def test_demag_solve_beside_the_elasticity_solve_changes_nothing(mode, warm):
         P.maxiter = saved
 
 
+def test_timing_of_one_operator_leaves_the_others_untimed():
+    """cpm_prof_only restricts the kernel timing to one operator: with it
+    set to the elasticity operator, the mixed multigrid step times that
+    operator's level-0 sweeps and outer products, and no launch of the
+    Gauss-Seidel or demag operators; the timing changes no result bit."""
+    import ctypes as C
+    from cupymag_amd._lib import check, lib
+    from cupymag_amd.core.gspm import GSPMSimulation
+    cfg, arr, P, S = _setup(n=16)
+    m0 = tilted_m(S.n_dof, seed=5)
+    H = [P.Hext1, P.Hext2, P.Hext3]
+    A_el, A1, Ad = S.csr_elasticity()[0], S.csr_GS()[0], S.csr_demag_and_deriv()[0]
+    sims = [GSPMSimulation(S, P, mode="mixed", precond="mg", warm_start="extrapolate") for _ in range(2)]
+    ctx = sims[0].ctx
+    for s in sims:
+        s.set_m(m0)
+    for A in (A_el, A1, Ad):
+        check(lib().cpm_mat_prof(A.h, None, None, 1))
+        check(lib().cpm_mat_prof_outer(A.h, None, None, 1))
+
+    def counts(A):
+        n1, n2 = C.c_int64(), C.c_int64()
+        check(lib().cpm_mat_prof(A.h, None, C.byref(n1), 0))
+        check(lib().cpm_mat_prof_outer(A.h, None, C.byref(n2), 0))
+        return n1.value, n2.value
+
+    try:
+        for k in range(3):
+            check(lib().cpm_prof_enable(ctx.h, 1))
+            check(lib().cpm_prof_only(ctx.h, A_el.h))
+            r0 = sims[0].step(H, False)
+            check(lib().cpm_prof_enable(ctx.h, 0))
+            r1 = sims[1].step(H, False)
+            assert r0.iters == r1.iters and r0.sumsq == r1.sumsq, (k, r0.iters, r1.iters)
+            assert torch.equal(sims[0].get("M"), sims[1].get("M"))
+    finally:
+        check(lib().cpm_prof_enable(ctx.h, 0))
+        check(lib().cpm_prof_only(ctx.h, None))
+    n_el = counts(A_el)
+    assert n_el[0] > 0 and n_el[1] > 0, n_el
+    assert counts(A1) == (0, 0) and counts(Ad) == (0, 0)
+
+
 def test_multigrid_rejections():
     from cupymag_amd.core.batch import GSPMBatch
     from cupymag_amd.core.gspm import GSPMSimulation
