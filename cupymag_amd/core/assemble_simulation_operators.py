@@ -110,16 +110,16 @@ class SimulationOperators(FEMMesh):
             return None
         return MatrixFreeFel(self.node_coords_np, n, Cv, ctx)
 
-    def multigrid(self, precision=32, degree=2, ratio=10.0, floating=True, coarsest_scalar=8):
+    def multigrid(self, precision=32, degree=2, ratio=10.0, floating=True, coarsest_scalar=8, a1_coarse="auto"):
         """Multigrid hierarchies of A_demag, A1 and A_el (cached per
         settings), or None when the mesh has none
         (cupymag_amd.solvers.multigrid.build_hierarchies)."""
         cache = self.__dict__.setdefault("_mg", {})
-        key = (precision, degree, ratio, bool(floating), int(coarsest_scalar))
+        key = (precision, degree, ratio, bool(floating), int(coarsest_scalar), a1_coarse)
         if key not in cache:
             from ..solvers.multigrid import build_hierarchies
             cache[key] = build_hierarchies(self, precision, degree, ratio, nthreads=self.nthreads,
-                                           floating=floating, coarsest_scalar=coarsest_scalar)
+                                           floating=floating, coarsest_scalar=coarsest_scalar, a1_coarse=a1_coarse)
         return cache[key]
 
     def csr_demag_and_deriv(self):

@@ -67,6 +67,11 @@ struct cpm_mg {
   std::vector<cpm::MgLevel> L;
   void* cinv = nullptr;     // coarsest dense inverse (T), ncoarse x ncoarse
   int64_t ncoarse = 0;
+  // csmooth > 0: no dense inverse; the coarsest level is solved by csmooth
+  // Chebyshev steps on D^-1 A over [lmax / cratio, 1.1 lmax] (its first
+  // step fused with the restriction, as on the other levels)
+  int csmooth = 0;
+  double cratio = 0.0;
   // Chebyshev coefficients: c0[l] = 1/theta; for step k of level l,
   // c1[8l + k] = rho_k rho_(k-1) and c2[8l + k] = 2 rho_k / delta
   std::vector<double> c0, c1, c2;

@@ -614,16 +614,22 @@ static void vcycle(cpm_mg* M, size_t l, const double* b64, const CgState* st, hi
   // the level's sweep grid and XCD order (the coarsest level has no sweeps)
   const int gn = l + 1 == M->L.size() ? grid_for(Lv.nb) : sell_grid(Lv.A->sell);
   const int32_t xg = Lv.A->sell.xg;
+  const double* c1 = &M->c1[l * 8];
+  const double* c2 = &M->c2[l * 8];
+  void* d[2] = {Lv.d0, Lv.d1};
   if (l + 1 == M->L.size()) {
+    if (M->csmooth) {
+      // Chebyshev solve: the first step ran with the restriction
+      for (int k = 1; k < M->csmooth; ++k)
+        sweep<BS, K, T, 0>(M, Lv, d[(k - 1) & 1], d[k & 1], c1[k], c2[k], st);
+      return;
+    }
     const int g = (int)std::min<int64_t>(kMaxGrid, (Lv.n + 3) / 4);
     k_mg_dense<K, T><<<g, kBlock, 0, s>>>(Lv.n, (const T*)M->cinv, (const T*)Lv.b, (T*)Lv.x, st);
     return;
   }
   const int m = M->degree;
   const double c0 = M->c0[l];
-  const double* c1 = &M->c1[l * 8];
-  const double* c2 = &M->c2[l * 8];
-  void* d[2] = {Lv.d0, Lv.d1};
   // pre-smoothing from x = 0 (floating: b64's component sums are in the
   // partials of k_pcg_rsum, launched just before over grid_for(n) workgroups)
   if (l == 0 && b64) {
@@ -641,7 +647,7 @@ static void vcycle(cpm_mg* M, size_t l, const double* b64, const CgState* st, hi
   // coarse-grid correction
   MgLevel& Lc = M->L[l + 1];
   const Dims3 fd{{Lv.dims[0], Lv.dims[1], Lv.dims[2]}}, cd{{Lc.dims[0], Lc.dims[1], Lc.dims[2]}};
-  if (l + 2 < M->L.size()) {
+  if (l + 2 < M->L.size() || M->csmooth) {
     // restriction fused with the coarse level's first step
     k_mg_restrict_first<BS, K, T><<<sell_grid(Lc.A->sell), kBlock, 0, s>>>(
         fd, cd, M->anchor, (const T*)Lv.r, (T*)Lc.b, (T*)Lc.r, (T*)Lc.d0, (T*)Lc.x, (const T*)Lc.dinv, M->c0[l + 1],
@@ -1102,9 +1108,13 @@ extern "C" int cpm_mg_destroy(cpm_mg* M) {
   return CPM_OK;
 }
 
-extern "C" int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, const double* coarse_inv,
-                             int precision, int degree, double ratio, cpm_mg** out) {
-  if (!ctx || !levels || !coarse_inv || !out) return fail(CPM_EINVAL, "cpm_mg_create: null argument");
+// cdeg > 0: the coarsest level is solved by cdeg Chebyshev steps over
+// [lmax / cratio, 1.1 lmax] instead of the dense inverse coarse_inv
+static int mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, const double* coarse_inv, int precision,
+                     int degree, double ratio, int cdeg, double cratio, cpm_mg** out) {
+  if (!ctx || !levels || (!coarse_inv && !cdeg) || !out) return fail(CPM_EINVAL, "cpm_mg_create: null argument");
+  if (cdeg && (cdeg < 1 || cdeg > 8 || !(cratio > 1.0)))
+    return fail(CPM_EINVAL, "cpm_mg_create: coarsest Chebyshev degree must be 1..8 and its ratio > 1");
   if (nlevels < 2 || nlevels > 16) return fail(CPM_EINVAL, "cpm_mg_create: 2..16 levels");
   if (precision != 16 && precision != 32 && precision != 64)
     return fail(CPM_EINVAL, "cpm_mg_create: precision must be 16, 32 or 64");
@@ -1171,6 +1181,12 @@ extern "C" int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, 
     }
     M->floating = (a[0] && !a[1]) ? 1 : 0;
     M->anchor = (a[0] && a[1]) ? 1 : 0;
+    if (cdeg && a[0]) {
+      delete M;
+      return fail(CPM_EINVAL, "cpm_mg_create: an anchored operator needs the dense coarsest solve");
+    }
+    M->csmooth = cdeg;
+    M->cratio = cratio;
     const Sell& F = levels[0]->sell;
     if (M->floating && (F.dims[0] < 3 || F.dims[1] < 3 || F.dims[2] < 3 || !F.kuhn)) {
       delete M;
@@ -1348,7 +1364,7 @@ extern "C" int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, 
       rc = fail(CPM_EHIP, hipGetErrorString(e));
       break;
     }
-    if (l + 1 < nlevels) {
+    if (l + 1 < nlevels || cdeg) {
       // lambda_max(D^-1 A) by power iteration on the FP64 form
       void *vx = nullptr, *vy = nullptr, *vd = nullptr;
       A(&vx, 8 * Lv.n);
@@ -1386,12 +1402,14 @@ extern "C" int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, 
     M->c0.assign(nlevels, 0.0);
     M->c1.assign(nlevels * 8, 0.0);
     M->c2.assign(nlevels * 8, 0.0);
-    for (int l = 0; l + 1 < nlevels; ++l) {
-      const double hi = 1.1 * M->L[l].lmax, lo = hi / ratio;
+    for (int l = 0; l < nlevels; ++l) {
+      const bool last = l + 1 == nlevels;
+      if (last && !cdeg) break;
+      const double hi = 1.1 * M->L[l].lmax, lo = hi / (last ? cratio : ratio);
       const double th = 0.5 * (hi + lo), de = 0.5 * (hi - lo), sg = th / de;
       M->c0[l] = 1.0 / th;
       double rho = 1.0 / sg;
-      for (int k = 1; k < degree; ++k) {
+      for (int k = 1; k < (last ? cdeg : degree); ++k) {
         const double rn = 1.0 / (2.0 * sg - rho);
         M->c1[l * 8 + k] = rn * rho;
         M->c2[l * 8 + k] = 2.0 * rn / de;
@@ -1399,10 +1417,10 @@ extern "C" int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, 
       }
     }
     // coarsest dense inverse
-    const int64_t nc = M->L.back().n;
+    const int64_t nc = cdeg ? 0 : M->L.back().n;
     M->ncoarse = nc;
-    A(&M->cinv, ts * nc * nc);
-    if (rc == CPM_OK) {
+    if (!cdeg) A(&M->cinv, ts * nc * nc);
+    if (rc == CPM_OK && !cdeg) {
       hipError_t e;
       if (precision == 32) {
         std::vector<float> f(coarse_inv, coarse_inv + nc * nc);
@@ -1426,6 +1444,18 @@ extern "C" int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, 
   }
   *out = M;
   return CPM_OK;
+}
+
+extern "C" int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, const double* coarse_inv,
+                             int precision, int degree, double ratio, cpm_mg** out) {
+  if (!coarse_inv) return fail(CPM_EINVAL, "cpm_mg_create: null argument");
+  return mg_create(ctx, nlevels, levels, coarse_inv, precision, degree, ratio, 0, 0.0, out);
+}
+
+extern "C" int cpm_mg_create_smoothed(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, int precision, int degree,
+                                      double ratio, int cdegree, double cratio, cpm_mg** out) {
+  if (cdegree < 1) return fail(CPM_EINVAL, "cpm_mg_create_smoothed: coarsest Chebyshev degree must be 1..8");
+  return mg_create(ctx, nlevels, levels, nullptr, precision, degree, ratio, cdegree, cratio, out);
 }
 
 extern "C" int cpm_mg_info(const cpm_mg* M, int* nlevels, double* lmax, int max_levels) {

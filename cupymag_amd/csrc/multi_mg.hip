@@ -66,7 +66,8 @@ struct SmoothSeg {
   void *rc, *dc;      // next level's r and d0 (fused first step)
   const void* dinvc;  // next level's D^-1
   double c0c;         // next level's first Chebyshev coefficient
-  int32_t fuse;       // the next level is not the coarsest: restriction + its first step
+  int32_t fuse;       // the next level has a first step (not a dense coarsest level): fused with the restriction
+  int32_t nsteps;     // Chebyshev steps of this level's smoothing (the coarsest of a smoothed hierarchy: its solve)
   const double* fpart;  // depth 0 of a floating hierarchy: partials of the sums of b64
   int64_t fstride;
   int32_t fn;
@@ -103,7 +104,7 @@ template <int BS, int K, class T, int MODE, class VT>
 __global__ void __launch_bounds__(kBlock) kbm_sweep(const SmoothSeg* __restrict__ seg, const int32_t* __restrict__ map,
                                                     int k) {
   const SmoothSeg& d = seg[map[blockIdx.x]];
-  if (all_done<K>(d.st)) return;
+  if (k >= d.nsteps || all_done<K>(d.st)) return;
   T* dv[2] = {(T*)d.d0, (T*)d.d1};
   const T* din = (MODE == 0 || MODE == 4) ? dv[(k - 1) & 1] : dv[k & 1];
   T* dout = dv[k & 1];
@@ -375,6 +376,7 @@ struct MultiMgTables {
   size_t capsmap = 0;
   std::vector<Table<SmoothSeg>> smooth;  // per depth
   std::vector<Table<DenseSeg>> dense;    // per depth
+  std::vector<Table<SmoothSeg>> csm;     // per depth: coarsest levels solved by Chebyshev steps
   CgState* st = nullptr;
   CgState* hst = nullptr;
   double* partials = nullptr;
@@ -383,6 +385,7 @@ struct MultiMgTables {
     free_table(outer);
     for (auto& t : smooth) free_table(t);
     for (auto& t : dense) free_table(t);
+    for (auto& t : csm) free_table(t);
     if (st) (void)hipFree(st);
     if (hst) (void)hipHostFree(hst);
     if (partials) (void)hipFree(partials);
@@ -450,10 +453,43 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
   // V-cycle segments per depth
   if ((int)W.smooth.size() < depth + 1) W.smooth.resize(depth + 1);
   if ((int)W.dense.size() < depth + 1) W.dense.resize(depth + 1);
+  if ((int)W.csm.size() < depth + 1) W.csm.resize(depth + 1);
+  // the smoothing fields of level d of system j (operator values, diagonal
+  // blocks, level vectors, Chebyshev coefficients)
+  auto level_fields = [&](SmoothSeg& g, int j, int d) {
+    cpm_mg* M = sys[j].M;
+    const MgLevel& Lv = M->L[d];
+    const Sell& S = Lv.A->sell;
+    g.sptr = S.sptr, g.col = S.col;
+    // smoothing values: FP16 form, or (recovered stiffness of a
+    // floating level 0, else the level's operator) FP32 / FP64
+    g.val = M->half            ? (const void*)Lv.v16
+            : (M->prec == 32) ? (Lv.kv32 ? (const void*)Lv.kv32 : (const void*)S.v32)
+                              : (Lv.kv64 ? (const void*)Lv.kv64 : (const void*)S.v64);
+    g.dg = M->half            ? (const void*)Lv.dg32
+           : (M->prec == 32) ? (const void*)(Lv.kdgc ? Lv.kdgc : S.dgc)
+                             : (const void*)(Lv.kdg64 ? Lv.kdg64 : S.dg64);
+    g.vscale = M->half ? Lv.vscale : 1.0;
+    g.ga = CPM_GRID_ARG(S);
+    // values keeping half of the stencil: the FP16 form (k_mg_make_half),
+    // or the level's own FP32 form
+    if (M->half ? Lv.khalf != 0 : (g.val == (const void*)S.v32 && S.half32)) g.ga.kuhn = 2;
+    // the level's own FP64 form: its half copy when there is one
+    if (g.val == (const void*)S.v64 && S.half64) {
+      g.val = S.v64h;
+      g.ga = CPM_GRID_ARG64(S);
+    }
+    g.nb = S.nbrows, g.stencil = S.stencil;
+    g.x = Lv.x, g.r = Lv.r, g.b = Lv.b, g.d0 = Lv.d0, g.d1 = Lv.d1, g.dinv = Lv.dinv;
+    g.c0 = M->c0[d];
+    for (int k = 0; k < 8; ++k) g.c1[k] = M->c1[d * 8 + k], g.c2[k] = M->c2[d * 8 + k];
+    g.st = W.st + j * K;
+  };
   for (int d = 0; d <= depth; ++d) {
     auto& Sm = W.smooth[d];
     auto& Dn = W.dense[d];
-    Sm.h.clear(), Sm.map.clear(), Dn.h.clear(), Dn.map.clear();
+    auto& Cs = W.csm[d];
+    Sm.h.clear(), Sm.map.clear(), Dn.h.clear(), Dn.map.clear(), Cs.h.clear(), Cs.map.clear();
     for (int j = 0; j < ns; ++j) {
       cpm_mg* M = sys[j].M;
       const int last = (int)M->L.size() - 1;
@@ -462,48 +498,39 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
         const MgLevel& Lc = M->L[d + 1];
         const Sell& S = Lv.A->sell;
         SmoothSeg g{};
-        g.sptr = S.sptr, g.col = S.col;
-        // smoothing values: FP16 form, or (recovered stiffness of a
-        // floating level 0, else the level's operator) FP32 / FP64
-        g.val = M->half            ? (const void*)Lv.v16
-                : (M->prec == 32) ? (Lv.kv32 ? (const void*)Lv.kv32 : (const void*)S.v32)
-                                  : (Lv.kv64 ? (const void*)Lv.kv64 : (const void*)S.v64);
-        g.dg = M->half            ? (const void*)Lv.dg32
-               : (M->prec == 32) ? (const void*)(Lv.kdgc ? Lv.kdgc : S.dgc)
-                                 : (const void*)(Lv.kdg64 ? Lv.kdg64 : S.dg64);
-        g.vscale = M->half ? Lv.vscale : 1.0;
-        g.ga = CPM_GRID_ARG(S);
-        // values keeping half of the stencil: the FP16 form (k_mg_make_half),
-        // or the level's own FP32 form
-        if (M->half ? Lv.khalf != 0 : (g.val == (const void*)S.v32 && S.half32)) g.ga.kuhn = 2;
-        // the level's own FP64 form: its half copy when there is one
-        if (g.val == (const void*)S.v64 && S.half64) {
-          g.val = S.v64h;
-          g.ga = CPM_GRID_ARG64(S);
-        }
-        g.nb = S.nbrows, g.stencil = S.stencil;
-        g.x = Lv.x, g.r = Lv.r, g.b = Lv.b, g.d0 = Lv.d0, g.d1 = Lv.d1, g.dinv = Lv.dinv;
+        level_fields(g, j, d);
+        g.nsteps = M->degree;
         g.b64 = (d == 0) ? M->r : nullptr;
-        g.c0 = M->c0[d];
-        for (int k = 0; k < 8; ++k) g.c1[k] = M->c1[d * 8 + k], g.c2[k] = M->c2[d * 8 + k];
         g.fd = Dims3{{Lv.dims[0], Lv.dims[1], Lv.dims[2]}};
         g.cd = Dims3{{Lc.dims[0], Lc.dims[1], Lc.dims[2]}};
         g.bc = Lc.b, g.xc = Lc.x;
         g.anchor = M->anchor;
-        g.fuse = (d + 1 < last) ? 1 : 0;
+        // the next level's first step with the restriction, unless it is a
+        // dense coarsest level
+        const bool fuse = d + 1 < last || M->csmooth;
+        g.fuse = fuse ? 1 : 0;
         g.rc = Lc.r, g.dc = Lc.d0, g.dinvc = Lc.dinv;
-        g.c0c = (d + 1 < last) ? M->c0[d + 1] : 0.0;
+        g.c0c = fuse ? M->c0[d + 1] : 0.0;
         if (d == 0 && M->floating) {
           // the partials kbm_rsum leaves for this system's vector workgroups
           g.fpart = pt + O.h[j].blk0;
           g.fstride = (int64_t)O.map.size();
           g.fn = O.h[j].nblk;
         }
-        g.st = W.st + j * K;
         g.blk0 = (int32_t)Sm.map.size();
         g.nblk = sell_grid(S);
         for (int b = 0; b < g.nblk; ++b) Sm.map.push_back((int32_t)Sm.h.size());
         Sm.h.push_back(g);
+      } else if (d == last && M->csmooth) {
+        // coarsest level solved by Chebyshev steps 1 .. csmooth-1 (step 0
+        // ran with the restriction)
+        SmoothSeg g{};
+        level_fields(g, j, d);
+        g.nsteps = M->csmooth;
+        g.blk0 = (int32_t)Cs.map.size();
+        g.nblk = sell_grid(M->L[d].A->sell);
+        for (int b = 0; b < g.nblk; ++b) Cs.map.push_back((int32_t)Cs.h.size());
+        Cs.h.push_back(g);
       } else if (d == last) {
         DenseSeg e{};
         e.n = M->L[d].n, e.cinv = M->cinv, e.b = M->L[d].b, e.x = M->L[d].x, e.st = W.st + j * K;
@@ -515,7 +542,10 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
     }
     if (int rc = upload(Sm, s)) return rc;
     if (int rc = upload(Dn, s)) return rc;
+    if (int rc = upload(Cs, s)) return rc;
   }
+  int csteps = 0;  // most Chebyshev steps of a smoothed coarsest level in the batch
+  for (const auto& y : sys) csteps = std::max(csteps, y.M->csmooth);
   if (int rc = upload(O, s)) return rc;
   if (W.smap.size() > W.capsmap) {
     if (W.dsmap) CPM_HIP(hipFree(W.dsmap));
@@ -544,6 +574,9 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
       }
       auto& Dn = W.dense[d + 1];
       if (Dn.G) kbm_dense<K, T><<<Dn.G, kBlock, 0, s>>>(Dn.d, Dn.dmap);
+      auto& Cs = W.csm[d + 1];
+      if (Cs.G)
+        for (int k = 1; k < csteps; ++k) kbm_sweep<BS, K, T, 0, VT><<<Cs.G, kBlock, 0, s>>>(Cs.d, Cs.dmap, k);
     }
     for (int d = depth - 1; d >= 0; --d) {
       auto& Sm = W.smooth[d];

@@ -28,22 +28,32 @@ from ..mesh.periodic import periodic_node_map, structured_dims
 class KuhnMultigrid:
     """A multigrid hierarchy (levels[0] is the operator being solved)."""
 
-    def __init__(self, levels, coarse_dense, precision=32, degree=2, ratio=10.0, floating=False):
+    def __init__(self, levels, coarse_dense, precision=32, degree=2, ratio=10.0, floating=False,
+                 coarse_smooth=None):
         """floating: levels[0] is anchored, levels[1:] are the periodic
         stiffness and coarse_dense its pseudo-inverse (cpm_mg_create tells
         the kinds apart from the operators themselves; the flag is kept for
-        info())."""
+        info()).  coarse_smooth = (degree, ratio), with coarse_dense None:
+        the last level is solved by that many Chebyshev steps over
+        [lmax / ratio, lmax] (cpm_mg_create_smoothed; operators without an
+        anchor)."""
         self.floating = bool(floating)
         self.levels = list(levels)
         self.ctx = levels[0].ctx
         self.precision = int(precision)
         self.degree = int(degree)
         self.ratio = float(ratio)
+        self.coarse_smooth = None if coarse_smooth is None else (int(coarse_smooth[0]), float(coarse_smooth[1]))
         arr = (C.c_void_p * len(levels))(*[L.h.value for L in levels])
-        cinv = np.ascontiguousarray(coarse_dense, dtype=np.float64)
         h = C.c_void_p()
-        check(lib().cpm_mg_create(self.ctx.h, len(levels), arr, ptr(cinv), self.precision, self.degree,
-                                  self.ratio, C.byref(h)), "cpm_mg_create")
+        if self.coarse_smooth is not None:
+            check(lib().cpm_mg_create_smoothed(self.ctx.h, len(levels), arr, self.precision, self.degree, self.ratio,
+                                               self.coarse_smooth[0], self.coarse_smooth[1], C.byref(h)),
+                  "cpm_mg_create_smoothed")
+        else:
+            cinv = np.ascontiguousarray(coarse_dense, dtype=np.float64)
+            check(lib().cpm_mg_create(self.ctx.h, len(levels), arr, ptr(cinv), self.precision, self.degree,
+                                      self.ratio, C.byref(h)), "cpm_mg_create")
         self.h = h
 
     def info(self):
@@ -53,7 +63,8 @@ class KuhnMultigrid:
         hs = (C.c_int * 16)()
         check(lib().cpm_mg_forms(self.h, hs, 16), "cpm_mg_forms")
         return dict(levels=n.value, lmax=list(lm[:n.value]), dims=[getattr(L, "grid_dims", None) for L in self.levels],
-                    floating=self.floating, half_stencil=[bool(v) for v in hs[:n.value]])
+                    floating=self.floating, half_stencil=[bool(v) for v in hs[:n.value]],
+                    coarsest="dense" if self.coarse_smooth is None else ("chebyshev",) + self.coarse_smooth)
 
     def set_graphs(self, on):
         """Replay the PCG iterations from captured hipGraph chunks (default)
@@ -143,6 +154,37 @@ def _pinv_periodic(A, bs):
     return np.linalg.inv(A + sc * QQ) - QQ / sc
 
 
+# A1 = M + alpha1 K: the stiffness part shrinks against the mass part as the
+# mesh coarsens (alpha1 K / M ~ 1/h^2), so a coarse level of A1 is well
+# conditioned and a few Chebyshev steps solve it as well as the levels and
+# the dense solve below it would.  The first coarse level whose Jacobi-scaled
+# condition number is at most A1_KAPPA_MAX becomes the coarsest, solved by
+# A1_CHEB Chebyshev steps over [lmax / A1_CHEB_RATIO, lmax].  At the 128^3
+# headline (kappa 131 on level 0, 33 on the 64^3 level, 8.4 on 32^3) this
+# keeps two levels; a CPU probe of the PCG (SciPy, same smoother) took 15
+# iterations with it against 16 with the five levels down to the dense 8^3
+# solve.
+A1_KAPPA_MAX = 40.0
+A1_CHEB = 6
+A1_CHEB_RATIO = 20.0
+
+
+def jacobi_kappa(ip, ix, v, n):
+    """Condition number of D^-1/2 A D^-1/2 for a host CSR operator (Lanczos,
+    loose tolerance), or inf when the estimate does not converge."""
+    import scipy.sparse as sps
+    import scipy.sparse.linalg as sla
+    A = sps.csr_matrix((np.asarray(v, dtype=np.float64), np.asarray(ix), np.asarray(ip)), shape=(n, n))
+    d = 1.0 / np.sqrt(A.diagonal())
+    B = sps.diags(d) @ A @ sps.diags(d)
+    try:
+        lo = sla.eigsh(B, k=1, which="SA", tol=1e-2, maxiter=20 * n, return_eigenvectors=False)[0]
+        hi = sla.eigsh(B, k=1, which="LA", tol=1e-2, maxiter=20 * n, return_eigenvectors=False)[0]
+    except sla.ArpackNoConvergence:
+        return float("inf")
+    return float(hi / lo) if lo > 0 else float("inf")
+
+
 def scalar_levels(sizes, coarsest_scalar=8):
     """Levels of a scalar operator's hierarchy: the first level of at most
     coarsest_scalar cells per dimension is its coarsest (dense) level, and
@@ -152,7 +194,7 @@ def scalar_levels(sizes, coarsest_scalar=8):
 
 
 def build_hierarchies(SimOp, precision=32, degree=2, ratio=10.0, coarsest_max=8, nthreads=None, floating=True,
-                      coarsest_scalar=8):
+                      coarsest_scalar=8, a1_coarse="auto"):
     """{"A_demag": KuhnMultigrid, "A1": KuhnMultigrid (Gauss-Seidel operator
     M + alpha1 K; coarse levels without the defect surgery), "A_el":
     KuhnMultigrid (with coupling)} for a SimulationOperators object on an
@@ -171,7 +213,14 @@ def build_hierarchies(SimOp, precision=32, degree=2, ratio=10.0, coarsest_max=8,
     dimension; the scalar ones (A_demag, A1) to <= coarsest_scalar
     (scalar_levels; 16 makes their dense level up to 16^3 unknowns, 64 MB in
     FP32, in place of the sweeps and transfers of the levels below: no
-    faster at 128^3, and a longer set-up)."""
+    faster at 128^3, and a longer set-up).
+
+    a1_coarse: "auto" stops the A1 hierarchy at its first coarse level whose
+    Jacobi-scaled condition number is at most A1_KAPPA_MAX and solves that
+    level with A1_CHEB Chebyshev steps (cpm_mg_create_smoothed); "dense"
+    keeps the levels down to the dense solve."""
+    if a1_coarse not in ("auto", "dense"):
+        raise ValueError(f"a1_coarse must be 'auto' or 'dense', not {a1_coarse!r}")
     from ..physics.host_ops import HostOps, voigt_C
     n = kuhn_cells(SimOp.node_coords_np, SimOp.elements_np)
     if n is None or SimOp.grid_dims != (n, n, n):
@@ -188,6 +237,7 @@ def build_hierarchies(SimOp, precision=32, degree=2, ratio=10.0, coarsest_max=8,
     levels = {k: [v] for k, v in fine.items()}
     nlev = {k: (len(sizes) if k == "A_el" else len(scalar_levels(sizes, coarsest_scalar))) for k in fine}
     dense = {}
+    smoothed = set()
     coords = SimOp.node_coords_np
     for li, m in enumerate(sizes[1:]):
         if all(len(levels[k]) == nlev[k] for k in fine):
@@ -206,18 +256,25 @@ def build_hierarchies(SimOp, precision=32, degree=2, ratio=10.0, coarsest_max=8,
                 continue
             last = len(levels[name]) == nlev[name] - 1
             ip, ix, v, shape = H.csr(name)
+            if name == "A1" and a1_coarse == "auto" and jacobi_kappa(ip, ix, v, shape[0]) <= A1_KAPPA_MAX:
+                # a well-conditioned coarse level: the last one, smoothed
+                last = True
+                nlev[name] = len(levels[name]) + 1
+                smoothed.add(name)
             M = DeviceCSR(ip, ix, v, shape, ctx)
             # the host arrays are views of the assembler's buffers, freed below
             M._host = (ip.copy(), ix.copy(), np.array(v, copy=True))
             M.grid_dims = dims
             M.add_sell(bs, ip, ix, v, dims)
             levels[name].append(M)
-            if last:
+            if last and name not in smoothed:
                 import scipy.sparse as sps
                 A = sps.csr_matrix((np.array(v), np.array(ix), np.array(ip)), shape=shape).toarray()
                 perm = _interleave_perm(ndof, bs)
                 Ai = A[np.ix_(perm, perm)]
                 dense[name] = _pinv_periodic(Ai, bs) if floating and name != "A1" else np.linalg.inv(Ai)
         H.free()
-    return {name: KuhnMultigrid(levels[name], dense[name], precision, degree, ratio,
-                                floating=floating and name != "A1") for name in fine}
+    return {name: KuhnMultigrid(levels[name], dense.get(name), precision, degree, ratio,
+                                floating=floating and name != "A1",
+                                coarse_smooth=(A1_CHEB, A1_CHEB_RATIO) if name in smoothed else None)
+            for name in fine}

@@ -267,6 +267,14 @@ int cpm_weighted_sum(cpm_ctx* ctx, int32_t n, int ncomp, const double* W, const 
 typedef struct cpm_mg cpm_mg;
 int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, const double* coarse_inv, int precision,
                   int degree, double ratio, cpm_mg** out);
+/* No counterpart in the reference (as cpm_mg_create).  The same hierarchy
+ * for an operator without an anchor (the Gauss-Seidel operator A1 = M +
+ * alpha1 K, whose coarse levels are well conditioned): no dense inverse; the
+ * last level is solved by `cdegree` (1..8) Chebyshev steps on D^-1 A over
+ * [lmax/cratio, lmax], the first fused with the restriction.  An anchored
+ * levels[0] is refused (CPM_EINVAL). */
+int cpm_mg_create_smoothed(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, int precision, int degree,
+                           double ratio, int cdegree, double cratio, cpm_mg** out);
 int cpm_mg_destroy(cpm_mg* mg);
 /* levels and the lambda_max estimate of each level (host, up to max_levels) */
 int cpm_mg_info(const cpm_mg* mg, int* nlevels, double* lmax, int max_levels);

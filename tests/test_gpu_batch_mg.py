@@ -39,18 +39,29 @@ def meshes():
     return [_ops(n) for n in (10, 20, 13)]
 
 
-@pytest.mark.parametrize("name", ["A_demag", "A_el", "A1"])
+@pytest.mark.parametrize("name,a1", [("A_demag", "dense"), ("A_el", "dense"), ("A1", "dense"), ("A1", "auto"),
+                                     ("A1", "mix")])
 @pytest.mark.parametrize("prec", [16, 32, 64])
 @pytest.mark.parametrize("coarsest_scalar", [8, 16])
-def test_solve_batch_equals_single_solves(meshes, name, prec, coarsest_scalar):
+def test_solve_batch_equals_single_solves(meshes, name, a1, prec, coarsest_scalar):
     """The scalar hierarchies at the elasticity's depths (20^3 -> 10^3 ->
     5^3), or with coarsest_scalar = 16 stopping at <= 16 cells (20^3 ->
-    10^3)."""
+    10^3); A1 also with its first coarse level solved by Chebyshev steps
+    (a1_coarse "auto" on these meshes), alone and mixed with dense-coarsest
+    hierarchies in one batch."""
     from cupymag_amd.solvers.multigrid import solve_batch
     if name == "A_el" and coarsest_scalar == 8:
         pytest.skip("the elasticity hierarchy has one depth rule")
-    mgs = [S.multigrid(precision=prec, coarsest_scalar=coarsest_scalar)[name] for _, _, _, S in meshes]
-    want = [2, 3, 2] if (name == "A_el" or coarsest_scalar == 8) else [2, 2, 2]
+    if a1 != "dense" and coarsest_scalar == 16:
+        pytest.skip("a smoothed coarsest level does not depend on coarsest_scalar")
+    kinds = {"dense": ["dense"] * 3, "auto": ["auto"] * 3, "mix": ["auto", "dense", "auto"]}[a1]
+    mgs = [S.multigrid(precision=prec, coarsest_scalar=coarsest_scalar, a1_coarse=k)[name]
+           for (_, _, _, S), k in zip(meshes, kinds)]
+    if a1 == "dense":
+        want = [2, 3, 2] if (name == "A_el" or coarsest_scalar == 8) else [2, 2, 2]
+    else:
+        want = [2 if k == "auto" else l for k, l in zip(kinds, [2, 3, 2])]
+        assert [m.info()["coarsest"] != "dense" for m in mgs] == [k == "auto" for k in kinds]
     assert [m.info()["levels"] for m in mgs] == want
     bs = 3 if name == "A_el" else 1
     rng = np.random.default_rng(11)

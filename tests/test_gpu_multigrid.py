@@ -43,7 +43,9 @@ def _interleaved(M, bs):
     return A[perm][:, perm].tocsr()
 
 
-def _host_vcycle(mats, dims, lmax, cinv, bs, degree, ratio, b, anchor=True):
+def _host_vcycle(mats, dims, lmax, cinv, bs, degree, ratio, b, anchor=True, coarse_smooth=None):
+    """The V-cycle in SciPy: cinv solves the coarsest level, or (cinv None)
+    coarse_smooth = (degree, ratio) Chebyshev steps on it from zero."""
     from cupymag_amd.mesh.coarsen import prolongation
 
     def dinv_of(A):
@@ -56,28 +58,28 @@ def _host_vcycle(mats, dims, lmax, cinv, bs, degree, ratio, b, anchor=True):
             D[i] = np.linalg.inv(blk)
         return lambda v: np.einsum("nij,nj->ni", D, v.reshape(n, bs)).ravel()
 
-    Dinv = [dinv_of(A) for A in mats[:-1]]
+    Dinv = [dinv_of(A) for A in (mats if cinv is None else mats[:-1])]
 
-    def coef(l):
+    def coef(l, deg, rat):
         hi = 1.1 * lmax[l]
-        lo = hi / ratio
+        lo = hi / rat
         th, de = 0.5 * (hi + lo), 0.5 * (hi - lo)
         sg = th / de
         rho = 1.0 / sg
         c1, c2 = [0.0], [0.0]
-        for k in range(1, degree):
+        for k in range(1, deg):
             rn = 1.0 / (2 * sg - rho)
             c1.append(rn * rho)
             c2.append(2 * rn / de)
             rho = rn
         return 1.0 / th, c1, c2
 
-    def smooth(l, b, x, r):
+    def smooth(l, b, x, r, deg=degree, rat=ratio):
         A = mats[l]
-        c0, c1, c2 = coef(l)
+        c0, c1, c2 = coef(l, deg, rat)
         d = c0 * Dinv[l](r)
         x = x + d
-        for k in range(1, degree):
+        for k in range(1, deg):
             r = r - A @ d
             d = c1[k] * d + c2[k] * Dinv[l](r)
             x = x + d
@@ -85,6 +87,8 @@ def _host_vcycle(mats, dims, lmax, cinv, bs, degree, ratio, b, anchor=True):
 
     def V(l, b):
         if l == len(mats) - 1:
+            if cinv is None:
+                return smooth(l, b, np.zeros_like(b), b.copy(), *coarse_smooth)[0]
             return cinv @ b
         A = mats[l]
         x, r, d = smooth(l, b, np.zeros_like(b), b.copy())
@@ -116,10 +120,25 @@ def test_vcycle_matches_host_reference(name, bs, n, floating):
         pytest.skip("A1 has no anchor: one kind of hierarchy")
     _, _, _, S = _setup(n=n)
     sizes = hierarchy_sizes(n) if bs == 3 else scalar_levels(hierarchy_sizes(n))
-    for prec, tol in ((64, 1e-11), (32, 1e-5), (16, 3e-3)):
-        mg = S.multigrid(precision=prec, floating=floating)[name]
+    # A1 both ways: down to the dense solve, and its default on these small
+    # meshes, the first coarse level solved by Chebyshev steps
+    for prec, tol, a1 in ((64, 1e-11, "dense"), (32, 1e-5, "dense"), (16, 3e-3, "dense"), (64, 1e-11, "auto"),
+                          (16, 3e-3, "auto")):
+        if a1 == "auto" and name != "A1":
+            continue
+        mg = S.multigrid(precision=prec, floating=floating, a1_coarse=a1)[name]
         info = mg.info()
+        if a1 == "auto":
+            assert info["levels"] == 2 and info["coarsest"] == ("chebyshev", 6, 20.0), info
+            mats = [_interleaved(L, bs) for L in mg.levels]
+            r = np.random.default_rng(5).standard_normal(mats[0].shape[0])
+            z = mg.apply(torch.as_tensor(r, device="cuda")).cpu().numpy()
+            zr = _host_vcycle(mats, [(m, m, m) for m in sizes[:2]], info["lmax"], None, bs, mg.degree, mg.ratio, r,
+                              anchor=False, coarse_smooth=(6, 20.0))
+            assert np.abs(z - zr).max() / np.abs(zr).max() < tol, prec
+            continue
         assert info["levels"] == len(sizes) and info["dims"][-1] == (sizes[-1],) * 3
+        assert info["coarsest"] == "dense"
         assert all(l > 0 for l in info["lmax"][:-1])
         # FP16 forms of the (slot-symmetric) periodic Kuhn levels keep half
         # of the stencil; the other precisions have no FP16 form
@@ -326,6 +345,14 @@ def test_multigrid_rejections():
     _, _, P7, S7 = _setup(n=8, me=False)
     with pytest.raises(ValueError, match="multigrid needs"):
         GSPMSimulation(S7, P7, mode="mixed", precond="mg")
+    # a coarsest level solved by Chebyshev steps is for operators without an
+    # anchor (A1): the anchored demag operator is refused
+    from cupymag_amd.solvers.multigrid import KuhnMultigrid
+    demag = S.multigrid(precision=32, floating=False)["A_demag"]
+    with pytest.raises(ValueError, match="dense coarsest"):
+        KuhnMultigrid(demag.levels, None, 32, coarse_smooth=(6, 20.0))
+    with pytest.raises(ValueError, match="a1_coarse"):
+        S.multigrid(precision=32, a1_coarse="sometimes")
 
 
 @pytest.mark.parametrize("prec", [16, 64])
