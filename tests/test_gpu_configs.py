@@ -224,7 +224,9 @@ def test_cfg4_batch_of_64_equals_one_at_a_time():
 def test_cfg5_requests_up_to_128_cubed_same_alone_and_in_the_mix():
     from cupymag_amd.serving.server import SweepServer
     from cupymag_amd.serving.workloads import sweep_requests
-    reqs = sweep_requests(4, rate=20.0, seed=7, nmin=20, nmax=128, points=2, steps_per_point=3)
+    # 20 steps per request: the first large request (prepared in ~16 s) is
+    # still stepping when the small ones, prepared after it, join
+    reqs = sweep_requests(4, rate=20.0, seed=7, nmin=20, nmax=128, points=2, steps_per_point=10)
     ns = sorted(r["n"] for r in reqs)
     assert ns == [20, 26, 106, 125], ns
     for k, r in enumerate(reqs):
