@@ -112,6 +112,7 @@ _SIGS = {
     "cpm_mat_get_schedule": (C.c_int, [_P, C.POINTER(cpm_schedule)]),
     "cpm_mg_create": (C.c_int, [_P, C.c_int, _P, _P, C.c_int, C.c_int, _D, C.POINTER(_P)]),
     "cpm_mg_create_smoothed": (C.c_int, [_P, C.c_int, _P, C.c_int, C.c_int, _D, C.c_int, _D, C.POINTER(_P)]),
+    "cpm_mg_set_coarse_degree": (C.c_int, [_P, C.c_int]),
     "cpm_mg_destroy": (C.c_int, [_P]),
     "cpm_mg_info": (C.c_int, [_P, C.POINTER(C.c_int), _P, C.c_int]),
     "cpm_mg_forms": (C.c_int, [_P, _P, C.c_int]),

@@ -110,16 +110,19 @@ class SimulationOperators(FEMMesh):
             return None
         return MatrixFreeFel(self.node_coords_np, n, Cv, ctx)
 
-    def multigrid(self, precision=32, degree=2, ratio=10.0, floating=True, coarsest_scalar=8, a1_coarse="auto"):
+    def multigrid(self, precision=32, degree=2, ratio=10.0, floating=True, coarsest_scalar=8, a1_coarse="auto",
+                  el_coarse_degree=None):
         """Multigrid hierarchies of A_demag, A1 and A_el (cached per
         settings), or None when the mesh has none
         (cupymag_amd.solvers.multigrid.build_hierarchies)."""
+        from ..solvers.multigrid import EL_COARSE_DEGREE, build_hierarchies
         cache = self.__dict__.setdefault("_mg", {})
-        key = (precision, degree, ratio, bool(floating), int(coarsest_scalar), a1_coarse)
+        ecd = EL_COARSE_DEGREE if el_coarse_degree is None else int(el_coarse_degree)
+        key = (precision, degree, ratio, bool(floating), int(coarsest_scalar), a1_coarse, ecd)
         if key not in cache:
-            from ..solvers.multigrid import build_hierarchies
             cache[key] = build_hierarchies(self, precision, degree, ratio, nthreads=self.nthreads,
-                                           floating=floating, coarsest_scalar=coarsest_scalar, a1_coarse=a1_coarse)
+                                           floating=floating, coarsest_scalar=coarsest_scalar, a1_coarse=a1_coarse,
+                                           el_coarse_degree=ecd)
         return cache[key]
 
     def csr_demag_and_deriv(self):

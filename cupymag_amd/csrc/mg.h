@@ -58,6 +58,9 @@ struct MgLevel {
 struct cpm_mg {
   cpm_ctx* ctx = nullptr;
   int bs = 1, prec = 32, degree = 2;
+  // Chebyshev degree of the levels between the finest and the coarsest
+  // (cpm_mg_set_coarse_degree); 0: `degree` on every level
+  int degree1 = 0;
   int anchor = 1;           // row 0 of the fine operator is an identity row: node 0 left out of the transfers
   // anchored fine operator over singular periodic levels: the V-cycle
   // solves the periodic problem, with the anchor applied around it
@@ -75,6 +78,11 @@ struct cpm_mg {
   // Chebyshev coefficients: c0[l] = 1/theta; for step k of level l,
   // c1[8l + k] = rho_k rho_(k-1) and c2[8l + k] = 2 rho_k / delta
   std::vector<double> c0, c1, c2;
+  // smoothing degree of level l (the coarsest: its Chebyshev solve, or 0)
+  int level_degree(size_t l) const {
+    if (l + 1 == L.size()) return csmooth;
+    return (l == 0 || !degree1) ? degree : degree1;
+  }
   // FP64 PCG vectors (fine level); level and PCG vectors hold kcap columns
   double *r = nullptr, *p = nullptr, *q = nullptr, *rold = nullptr;
   int kcap = 1;

@@ -628,7 +628,7 @@ static void vcycle(cpm_mg* M, size_t l, const double* b64, const CgState* st, hi
     k_mg_dense<K, T><<<g, kBlock, 0, s>>>(Lv.n, (const T*)M->cinv, (const T*)Lv.b, (T*)Lv.x, st);
     return;
   }
-  const int m = M->degree;
+  const int m = M->level_degree(l);
   const double c0 = M->c0[l];
   // pre-smoothing from x = 0 (floating: b64's component sums are in the
   // partials of k_pcg_rsum, launched just before over grid_for(n) workgroups)
@@ -1108,6 +1108,30 @@ extern "C" int cpm_mg_destroy(cpm_mg* M) {
   return CPM_OK;
 }
 
+// Chebyshev coefficients of every level for its degree (level_degree) over
+// [lmax / ratio, 1.1 lmax] (the smoothed coarsest level: over [lmax / cratio,
+// 1.1 lmax])
+static void chebyshev_coefficients(cpm_mg* M) {
+  const size_t nl = M->L.size();
+  M->c0.assign(nl, 0.0);
+  M->c1.assign(nl * 8, 0.0);
+  M->c2.assign(nl * 8, 0.0);
+  for (size_t l = 0; l < nl; ++l) {
+    const bool last = l + 1 == nl;
+    if (last && !M->csmooth) break;
+    const double hi = 1.1 * M->L[l].lmax, lo = hi / (last ? M->cratio : M->ratio);
+    const double th = 0.5 * (hi + lo), de = 0.5 * (hi - lo), sg = th / de;
+    M->c0[l] = 1.0 / th;
+    double rho = 1.0 / sg;
+    for (int k = 1; k < M->level_degree(l); ++k) {
+      const double rn = 1.0 / (2.0 * sg - rho);
+      M->c1[l * 8 + k] = rn * rho;
+      M->c2[l * 8 + k] = 2.0 * rn / de;
+      rho = rn;
+    }
+  }
+}
+
 // cdeg > 0: the coarsest level is solved by cdeg Chebyshev steps over
 // [lmax / cratio, 1.1 lmax] instead of the dense inverse coarse_inv
 static int mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, const double* coarse_inv, int precision,
@@ -1398,24 +1422,7 @@ static int mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, const do
     M->L.push_back(Lv);
   }
   if (rc == CPM_OK) {
-    // Chebyshev coefficients per level
-    M->c0.assign(nlevels, 0.0);
-    M->c1.assign(nlevels * 8, 0.0);
-    M->c2.assign(nlevels * 8, 0.0);
-    for (int l = 0; l < nlevels; ++l) {
-      const bool last = l + 1 == nlevels;
-      if (last && !cdeg) break;
-      const double hi = 1.1 * M->L[l].lmax, lo = hi / (last ? cratio : ratio);
-      const double th = 0.5 * (hi + lo), de = 0.5 * (hi - lo), sg = th / de;
-      M->c0[l] = 1.0 / th;
-      double rho = 1.0 / sg;
-      for (int k = 1; k < (last ? cdeg : degree); ++k) {
-        const double rn = 1.0 / (2.0 * sg - rho);
-        M->c1[l * 8 + k] = rn * rho;
-        M->c2[l * 8 + k] = 2.0 * rn / de;
-        rho = rn;
-      }
-    }
+    chebyshev_coefficients(M);
     // coarsest dense inverse
     const int64_t nc = cdeg ? 0 : M->L.back().n;
     M->ncoarse = nc;
@@ -1456,6 +1463,19 @@ extern "C" int cpm_mg_create_smoothed(cpm_ctx* ctx, int nlevels, cpm_mat* const*
                                       double ratio, int cdegree, double cratio, cpm_mg** out) {
   if (cdegree < 1) return fail(CPM_EINVAL, "cpm_mg_create_smoothed: coarsest Chebyshev degree must be 1..8");
   return mg_create(ctx, nlevels, levels, nullptr, precision, degree, ratio, cdegree, cratio, out);
+}
+
+extern "C" int cpm_mg_set_coarse_degree(cpm_mg* M, int degree) {
+  if (!M) return fail(CPM_EINVAL, "cpm_mg_set_coarse_degree: null hierarchy");
+  if (degree < 0 || degree > 8 || (degree == 1) != (M->degree == 1))
+    return fail(CPM_EINVAL, "cpm_mg_set_coarse_degree: degree must be 0 (the finest level's) or 2..8 (1 only with "
+                            "a finest degree of 1)");
+  M->degree1 = degree;
+  chebyshev_coefficients(M);
+  // the captured chunks hold the old sequence of sweeps
+  for (auto& c : M->chunks) (void)hipGraphExecDestroy(c.exec);
+  M->chunks.clear();
+  return CPM_OK;
 }
 
 extern "C" int cpm_mg_info(const cpm_mg* M, int* nlevels, double* lmax, int max_levels) {

@@ -401,7 +401,10 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
                double* resid, int* failed) {
   hipStream_t s = ctx->stream;
   const int ns = (int)sys.size();
-  const int m = sys[0].M->degree;
+  // smoothing degree at depth d: the finest level's, or the coarser levels'
+  // (the same for every system of the batch)
+  const int m0 = sys[0].M->degree, m1 = sys[0].M->degree1 ? sys[0].M->degree1 : m0;
+  auto mdeg = [&](int d) { return d == 0 ? m0 : m1; };
   int depth = 0;  // deepest level index over the batch
   for (const auto& y : sys) depth = std::max(depth, (int)y.M->L.size() - 1);
   if ((size_t)(ns * K) > W.cap_sys) {
@@ -499,7 +502,7 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
         const Sell& S = Lv.A->sell;
         SmoothSeg g{};
         level_fields(g, j, d);
-        g.nsteps = M->degree;
+        g.nsteps = M->level_degree(d);
         g.b64 = (d == 0) ? M->r : nullptr;
         g.fd = Dims3{{Lv.dims[0], Lv.dims[1], Lv.dims[2]}};
         g.cd = Dims3{{Lc.dims[0], Lc.dims[1], Lc.dims[2]}};
@@ -568,6 +571,7 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
       if (Sm.G) {
         // depth > 0: the first step ran fused with the restriction
         if (d == 0) kbm_first<BS, K, T, true, true><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap);
+        const int m = mdeg(d);
         for (int k = 1; k < m; ++k) kbm_sweep<BS, K, T, 0, VT><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap, k);
         kbm_sweep<BS, K, T, 1, VT><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap, m - 1);
         kbm_restrict<BS, K, T><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap);
@@ -582,6 +586,7 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
       auto& Sm = W.smooth[d];
       if (!Sm.G) continue;
       kbm_prolong<BS * K, K, T><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap);
+      const int m = mdeg(d);
       if (m >= 2) {
         // the residual with the first step's direction, then x += d_0 with the second step
         kbm_sweep<BS, K, T, 3, VT><<<Sm.G, kBlock, 0, s>>>(Sm.d, Sm.dmap, 0);
@@ -668,7 +673,8 @@ int multi_mg_pcg(cpm_ctx* ctx, MultiMgSolver& W, const std::vector<MultiMgSys>& 
   for (const auto& y : sys) {
     if (!y.M || !y.b || !y.x) return fail(CPM_EINVAL, "batched multigrid CG: null system");
     if (y.M->ctx != ctx) return fail(CPM_EINVAL, "batched multigrid CG: a hierarchy uses another context");
-    if (y.M->bs != M0->bs || y.M->prec != M0->prec || y.M->half != M0->half || y.M->degree != M0->degree)
+    if (y.M->bs != M0->bs || y.M->prec != M0->prec || y.M->half != M0->half || y.M->degree != M0->degree ||
+        y.M->degree1 != M0->degree1)
       return fail(CPM_EINVAL, "batched multigrid CG: hierarchies differ in block size, precision or degree");
     if (y.M->L.size() > 16) return fail(CPM_EINVAL, "batched multigrid CG: too many levels");
     if (!(y.delta >= 0.0 && y.delta < 1.0)) return fail(CPM_EINVAL, "batched multigrid CG: delta must be in [0, 1)");

@@ -29,14 +29,15 @@ class KuhnMultigrid:
     """A multigrid hierarchy (levels[0] is the operator being solved)."""
 
     def __init__(self, levels, coarse_dense, precision=32, degree=2, ratio=10.0, floating=False,
-                 coarse_smooth=None):
+                 coarse_smooth=None, coarse_degree=0):
         """floating: levels[0] is anchored, levels[1:] are the periodic
         stiffness and coarse_dense its pseudo-inverse (cpm_mg_create tells
         the kinds apart from the operators themselves; the flag is kept for
         info()).  coarse_smooth = (degree, ratio), with coarse_dense None:
         the last level is solved by that many Chebyshev steps over
         [lmax / ratio, lmax] (cpm_mg_create_smoothed; operators without an
-        anchor)."""
+        anchor).  coarse_degree: the Chebyshev degree of the levels between
+        the finest and the coarsest (0: ``degree``; cpm_mg_set_coarse_degree)."""
         self.floating = bool(floating)
         self.levels = list(levels)
         self.ctx = levels[0].ctx
@@ -55,6 +56,9 @@ class KuhnMultigrid:
             check(lib().cpm_mg_create(self.ctx.h, len(levels), arr, ptr(cinv), self.precision, self.degree,
                                       self.ratio, C.byref(h)), "cpm_mg_create")
         self.h = h
+        self.coarse_degree = int(coarse_degree)
+        if self.coarse_degree:
+            check(lib().cpm_mg_set_coarse_degree(h, self.coarse_degree), "cpm_mg_set_coarse_degree")
 
     def info(self):
         n = C.c_int()
@@ -64,7 +68,8 @@ class KuhnMultigrid:
         check(lib().cpm_mg_forms(self.h, hs, 16), "cpm_mg_forms")
         return dict(levels=n.value, lmax=list(lm[:n.value]), dims=[getattr(L, "grid_dims", None) for L in self.levels],
                     floating=self.floating, half_stencil=[bool(v) for v in hs[:n.value]],
-                    coarsest="dense" if self.coarse_smooth is None else ("chebyshev",) + self.coarse_smooth)
+                    coarsest="dense" if self.coarse_smooth is None else ("chebyshev",) + self.coarse_smooth,
+                    coarse_degree=self.coarse_degree or self.degree)
 
     def set_graphs(self, on):
         """Replay the PCG iterations from captured hipGraph chunks (default)
@@ -165,6 +170,12 @@ def _pinv_periodic(A, bs):
 # iterations with it against 16 with the five levels down to the dense 8^3
 # solve.
 A1_KAPPA_MAX = 40.0
+# The elasticity V-cycle's levels below the finest smooth with Chebyshev
+# degree 3 (the finest keeps `degree`): a CPU probe of the periodic
+# elasticity PCG at 128^3 took 18 iterations against 20 with degree 2 on
+# every level (16 with degree 3 on every level, at 2 more level-0 passes
+# per iteration), for about 5% more V-cycle work per iteration.
+EL_COARSE_DEGREE = 3
 A1_CHEB = 6
 A1_CHEB_RATIO = 20.0
 
@@ -194,7 +205,7 @@ def scalar_levels(sizes, coarsest_scalar=8):
 
 
 def build_hierarchies(SimOp, precision=32, degree=2, ratio=10.0, coarsest_max=8, nthreads=None, floating=True,
-                      coarsest_scalar=8, a1_coarse="auto"):
+                      coarsest_scalar=8, a1_coarse="auto", el_coarse_degree=EL_COARSE_DEGREE):
     """{"A_demag": KuhnMultigrid, "A1": KuhnMultigrid (Gauss-Seidel operator
     M + alpha1 K; coarse levels without the defect surgery), "A_el":
     KuhnMultigrid (with coupling)} for a SimulationOperators object on an
@@ -218,7 +229,10 @@ def build_hierarchies(SimOp, precision=32, degree=2, ratio=10.0, coarsest_max=8,
     a1_coarse: "auto" stops the A1 hierarchy at its first coarse level whose
     Jacobi-scaled condition number is at most A1_KAPPA_MAX and solves that
     level with A1_CHEB Chebyshev steps (cpm_mg_create_smoothed); "dense"
-    keeps the levels down to the dense solve."""
+    keeps the levels down to the dense solve.
+
+    el_coarse_degree: the Chebyshev degree of the elasticity hierarchy's
+    levels below the finest (0: ``degree``)."""
     if a1_coarse not in ("auto", "dense"):
         raise ValueError(f"a1_coarse must be 'auto' or 'dense', not {a1_coarse!r}")
     from ..physics.host_ops import HostOps, voigt_C
@@ -276,5 +290,6 @@ def build_hierarchies(SimOp, precision=32, degree=2, ratio=10.0, coarsest_max=8,
         H.free()
     return {name: KuhnMultigrid(levels[name], dense.get(name), precision, degree, ratio,
                                 floating=floating and name != "A1",
-                                coarse_smooth=(A1_CHEB, A1_CHEB_RATIO) if name in smoothed else None)
+                                coarse_smooth=(A1_CHEB, A1_CHEB_RATIO) if name in smoothed else None,
+                                coarse_degree=el_coarse_degree if (name == "A_el" and degree >= 2) else 0)
             for name in fine}

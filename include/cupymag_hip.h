@@ -275,6 +275,13 @@ int cpm_mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, const doubl
  * levels[0] is refused (CPM_EINVAL). */
 int cpm_mg_create_smoothed(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, int precision, int degree,
                            double ratio, int cdegree, double cratio, cpm_mg** out);
+/* No counterpart in the reference (as cpm_mg_create).  Chebyshev degree of
+ * the levels between the finest and the coarsest (0: the finest level's
+ * degree on every level; 2..8, or 1 when the finest level's is 1): the
+ * coarse levels cost a fraction of the finest, and a higher degree there
+ * can save outer iterations.  Recomputes the coefficients and drops the
+ * captured iteration chunks; batched solves need the same degrees. */
+int cpm_mg_set_coarse_degree(cpm_mg* mg, int degree);
 int cpm_mg_destroy(cpm_mg* mg);
 /* levels and the lambda_max estimate of each level (host, up to max_levels) */
 int cpm_mg_info(const cpm_mg* mg, int* nlevels, double* lmax, int max_levels);

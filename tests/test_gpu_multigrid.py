@@ -45,7 +45,9 @@ def _interleaved(M, bs):
 
 def _host_vcycle(mats, dims, lmax, cinv, bs, degree, ratio, b, anchor=True, coarse_smooth=None):
     """The V-cycle in SciPy: cinv solves the coarsest level, or (cinv None)
-    coarse_smooth = (degree, ratio) Chebyshev steps on it from zero."""
+    coarse_smooth = (degree, ratio) Chebyshev steps on it from zero; degree:
+    one for every level, or a list per level."""
+    degs = list(degree) if isinstance(degree, (list, tuple)) else [degree] * len(mats)
     from cupymag_amd.mesh.coarsen import prolongation
 
     def dinv_of(A):
@@ -74,7 +76,8 @@ def _host_vcycle(mats, dims, lmax, cinv, bs, degree, ratio, b, anchor=True, coar
             rho = rn
         return 1.0 / th, c1, c2
 
-    def smooth(l, b, x, r, deg=degree, rat=ratio):
+    def smooth(l, b, x, r, deg=None, rat=ratio):
+        deg = degs[l] if deg is None else deg
         A = mats[l]
         c0, c1, c2 = coef(l, deg, rat)
         d = c0 * Dinv[l](r)
@@ -148,6 +151,9 @@ def test_vcycle_matches_host_reference(name, bs, n, floating):
             assert not any(info["half_stencil"])
         mats = [_interleaved(L, bs) for L in mg.levels]
         dims = [(m, m, m) for m in sizes]
+        # the elasticity hierarchy smooths its coarser levels with a higher degree
+        degs = [mg.degree] + [info["coarse_degree"]] * (len(mats) - 1)
+        assert info["coarse_degree"] == (3 if name == "A_el" else mg.degree)
         from cupymag_amd.solvers.multigrid import _interleave_perm
         Ac = mg.levels[-1].to_scipy().toarray()
         perm = _interleave_perm(Ac.shape[0] // bs, bs)
@@ -166,14 +172,14 @@ def test_vcycle_matches_host_reference(name, bs, n, floating):
             Fr = r.copy()
             for d in range(bs):
                 Fr[d] = r[d] - r[d::bs].sum()
-            y = _host_vcycle(mats, dims, info["lmax"], cinv, bs, mg.degree, mg.ratio, Fr, anchor=False)
+            y = _host_vcycle(mats, dims, info["lmax"], cinv, bs, degs, mg.ratio, Fr, anchor=False)
             zr = y.copy()
             for d in range(bs):
                 zr[d::bs] -= y[d]
             zr[:bs] = r[:bs]
         else:
             cinv = np.linalg.inv(Ac[np.ix_(perm, perm)])
-            zr = _host_vcycle(mats, dims, info["lmax"], cinv, bs, mg.degree, mg.ratio, r, anchor=anchored)
+            zr = _host_vcycle(mats, dims, info["lmax"], cinv, bs, degs, mg.ratio, r, anchor=anchored)
         err = np.abs(z - zr).max() / np.abs(zr).max()
         assert err < tol, (prec, err)
         if fl:
@@ -183,7 +189,7 @@ def test_vcycle_matches_host_reference(name, bs, n, floating):
         elif prec == 64:
             # with node 0 left out of the transfers the V-cycle differs (so
             # the check above tells the two apart)
-            zw = _host_vcycle(mats, dims, info["lmax"], cinv, bs, mg.degree, mg.ratio, r, anchor=True)
+            zw = _host_vcycle(mats, dims, info["lmax"], cinv, bs, degs, mg.ratio, r, anchor=True)
             assert np.abs(zw - zr).max() > 1e-6 * np.abs(zr).max()
 
 
@@ -353,6 +359,11 @@ def test_multigrid_rejections():
         KuhnMultigrid(demag.levels, None, 32, coarse_smooth=(6, 20.0))
     with pytest.raises(ValueError, match="a1_coarse"):
         S.multigrid(precision=32, a1_coarse="sometimes")
+    # coarse-level degrees: 2..8 under a finest degree >= 2
+    a1 = S.multigrid(precision=32)["A1"]
+    for bad in (1, 9):
+        with pytest.raises(ValueError, match="degree must be"):
+            KuhnMultigrid(a1.levels, None, 32, coarse_smooth=(6, 20.0), coarse_degree=bad)
 
 
 @pytest.mark.parametrize("prec", [16, 64])
