@@ -31,6 +31,9 @@ struct cpm_fel {
   double* coords = nullptr;  // (n+1)^3 x 3
   double* dR = nullptr;      // 9 doubles: the rotation of cpm_fel_strain
   double C[36];
+  // regular lattice (every node at origin + (i hx, j hy, k hz)): F_el E0 is
+  // one constant 27-point stencil, S[offset][d][v] (27 x 3 x 6), device copy
+  double* S = nullptr;
 };
 
 namespace cpm {
@@ -157,6 +160,103 @@ __global__ void __launch_bounds__(kBlock) k_fel_mf(FelArgs fa, KuhnTets kt, cons
       }
     }
     for (int d = 0; d < 3; ++d) b[d * N + dof] = (dof == 0) ? 0.0 : acc[d];
+  }
+}
+
+// b = F_el E0 on a regular lattice: every DOF row is the same 27-point
+// stencil of 3 x 6 blocks (periodic), S[(o*3 + d)*6 + v] for the neighbour at
+// offset o = (di+1)*9 + (dj+1)*3 + (dk+1); rows 0, n, 2n stay zero
+__global__ void __launch_bounds__(kBlock) k_fel_stencil(int32_t n, const double* __restrict__ S,
+                                                        const double* __restrict__ E0, double* __restrict__ b) {
+  const int64_t N = (int64_t)n * n * n;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t dof = (int64_t)blockIdx.x * kBlock + threadIdx.x; dof < N; dof += stride) {
+    const int32_t i = (int32_t)(dof / ((int64_t)n * n)), j = (int32_t)((dof / n) % n), k = (int32_t)(dof % n);
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int di = -1; di <= 1; ++di) {
+      int32_t ii = i + di;
+      ii += (ii < 0) ? n : 0;
+      ii -= (ii >= n) ? n : 0;
+      for (int dj = -1; dj <= 1; ++dj) {
+        int32_t jj = j + dj;
+        jj += (jj < 0) ? n : 0;
+        jj -= (jj >= n) ? n : 0;
+#pragma unroll
+        for (int dk = -1; dk <= 1; ++dk) {
+          int32_t kk = k + dk;
+          kk += (kk < 0) ? n : 0;
+          kk -= (kk >= n) ? n : 0;
+          const int o = (di + 1) * 9 + (dj + 1) * 3 + (dk + 1);
+          const double* e = E0 + (((int64_t)ii * n + jj) * n + kk) * 6;
+          double ev[6];
+#pragma unroll
+          for (int v = 0; v < 6; ++v) ev[v] = e[v];
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int v = 0; v < 6; ++v) acc[d] += S[(o * 3 + d) * 6 + v] * ev[v];
+        }
+      }
+    }
+    for (int d = 0; d < 3; ++d) b[d * N + dof] = (dof == 0) ? 0.0 : acc[d];
+  }
+}
+
+// Host: the stencil of k_fel_stencil for a regular lattice of spacings h,
+// from the element terms k_fel_mf forms per tetrahedron (B_a^T C times
+// detJ / 24, and the weights of the 4-point rule summed per node).
+static void fel_stencil(const double h[3], const double C[36], const KuhnTets& kt, std::vector<double>& S) {
+  const double a7 = 0.5854102, b7 = 0.1381966;
+  S.assign(27 * 3 * 6, 0.0);
+  for (int o = 0; o < 8; ++o) {
+    const int lx = 1 - (o & 1), ly = 1 - ((o >> 1) & 1), lz = 1 - ((o >> 2) & 1);
+    const int lcode = lx | (ly << 1) | (lz << 2);
+    for (int t = 0; t < 6; ++t) {
+      int a = -1;
+      for (int q = 0; q < 4; ++q)
+        if (kt.v[t][q] == lcode) a = q;
+      if (a < 0) continue;
+      double x[4][3];
+      for (int q = 0; q < 4; ++q) {
+        const int c = kt.v[t][q];
+        x[q][0] = (c & 1) * h[0], x[q][1] = ((c >> 1) & 1) * h[1], x[q][2] = ((c >> 2) & 1) * h[2];
+      }
+      double J[3][3];
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) J[r][c] = x[r + 1][c] - x[0][c];
+      const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+      const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+      const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+      const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+      if (std::fabs(det) < 1e-14) continue;
+      const double id = 1.0 / det;
+      const double inv[3][3] = {
+          {c00 * id, (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id, (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id},
+          {c01 * id, (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id, (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id},
+          {c02 * id, (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id, (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id}};
+      const double dref[3] = {a == 0 ? -1.0 : (a == 1 ? 1.0 : 0.0), a == 0 ? -1.0 : (a == 2 ? 1.0 : 0.0),
+                              a == 0 ? -1.0 : (a == 3 ? 1.0 : 0.0)};
+      double g[3];
+      for (int r = 0; r < 3; ++r) g[r] = inv[r][0] * dref[0] + inv[r][1] * dref[1] + inv[r][2] * dref[2];
+      const double dx = g[0], dy = g[1], dz = g[2];
+      double BtC[3][6];
+      for (int v = 0; v < 6; ++v) {
+        BtC[0][v] = dx * C[0 * 6 + v] + dy * C[3 * 6 + v] + dz * C[5 * 6 + v];
+        BtC[1][v] = dy * C[1 * 6 + v] + dx * C[3 * 6 + v] + dz * C[4 * 6 + v];
+        BtC[2][v] = dz * C[2 * 6 + v] + dy * C[4 * 6 + v] + dx * C[5 * 6 + v];
+      }
+      const double dw = det * (1.0 / 24.0);
+      for (int bb = 0; bb < 4; ++bb) {
+        double wsum = 0.0;
+        for (int q = 0; q < 4; ++q) wsum += (q == bb) ? a7 : b7;
+        // node bb of the tetrahedron, relative to the row's vertex
+        const int c = kt.v[t][bb];
+        const int oi = (c & 1) - lx, oj = ((c >> 1) & 1) - ly, ok = ((c >> 2) & 1) - lz;
+        const int off = (oi + 1) * 9 + (oj + 1) * 3 + (ok + 1);
+        for (int d = 0; d < 3; ++d)
+          for (int v = 0; v < 6; ++v) S[(off * 3 + d) * 6 + v] += BtC[d][v] * dw * wsum;
+      }
+    }
   }
 }
 
@@ -350,12 +450,17 @@ int fder_hd(const cpm_fel* F, const double* U, double* H) {
 }
 
 int fel_apply(const cpm_fel* F, const double* E0, double* b) {
+  const int64_t N = (int64_t)F->n * F->n * F->n;
+  if (F->S) {
+    k_fel_stencil<<<grid_for(N), kBlock, 0, F->ctx->stream>>>(F->n, F->S, E0, b);
+    CPM_HIP(hipGetLastError());
+    return CPM_OK;
+  }
   FelArgs fa;
   fa.n = F->n;
   fa.coords = F->coords;
   std::copy(F->C, F->C + 36, fa.C);
   static const KuhnTets kt = kuhn_tets();
-  const int64_t N = (int64_t)F->n * F->n * F->n;
   k_fel_mf<<<grid_for(N), kBlock, 0, F->ctx->stream>>>(fa, kt, E0, b);
   CPM_HIP(hipGetLastError());
   return CPM_OK;
@@ -382,9 +487,32 @@ extern "C" int cpm_fel_create(cpm_ctx* ctx, int32_t n, const double* coords, con
   hipError_t e = hipMalloc(&F->coords, bytes);
   if (e == hipSuccess) e = hipMalloc(&F->dR, sizeof(double) * 9);
   if (e == hipSuccess) e = hipMemcpy(F->coords, coords, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess && C) {
+    // a regular lattice (nodes at origin + (i hx, j hy, k hz) to 1e-12 of
+    // the spacing): the F_el product as one stencil
+    const int64_t n1 = n + 1;
+    double h[3];
+    for (int a = 0; a < 3; ++a) h[a] = coords[((a == 0 ? n1 * n1 : a == 1 ? n1 : 1)) * 3 + a] - coords[a];
+    bool regular = h[0] > 0 && h[1] > 0 && h[2] > 0;
+    for (int64_t i = 0; i < n1 && regular; ++i)
+      for (int64_t j = 0; j < n1 && regular; ++j)
+        for (int64_t k = 0; k < n1 && regular; ++k) {
+          const double* x = coords + ((i * n1 + j) * n1 + k) * 3;
+          const int64_t idx[3] = {i, j, k};
+          for (int a = 0; a < 3; ++a)
+            if (std::fabs(x[a] - (coords[a] + idx[a] * h[a])) > 1e-12 * h[a]) regular = false;
+        }
+    if (regular) {
+      std::vector<double> S;
+      fel_stencil(h, F->C, kuhn_tets(), S);
+      e = hipMalloc(&F->S, sizeof(double) * S.size());
+      if (e == hipSuccess) e = hipMemcpy(F->S, S.data(), sizeof(double) * S.size(), hipMemcpyHostToDevice);
+    }
+  }
   if (e != hipSuccess) {
     if (F->coords) (void)hipFree(F->coords);
     if (F->dR) (void)hipFree(F->dR);
+    if (F->S) (void)hipFree(F->S);
     delete F;
     return fail(e == hipErrorOutOfMemory ? CPM_ENOMEM : CPM_EHIP, hipGetErrorString(e));
   }
@@ -398,6 +526,7 @@ extern "C" int cpm_fel_destroy(cpm_fel* F) {
   (void)hipStreamSynchronize(F->ctx->stream);
   if (F->coords) (void)hipFree(F->coords);
   if (F->dR) (void)hipFree(F->dR);
+  if (F->S) (void)hipFree(F->S);
   delete F;
   return CPM_OK;
 }

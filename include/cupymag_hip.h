@@ -335,7 +335,10 @@ int cpm_cg_mg_batch(cpm_ctx* ctx, int count, int nrhs, cpm_mg* const* mgs, const
  * |detJ| < 1e-14).  C: the 6x6 stiffness (rotated when rot111), row-major.
  * E0: 6n interleaved; b: 3n [bx; by; bz], rows 0, n, 2n zero.  Results equal
  * the CSR product up to summation order.  Saves the 270 entries per row
- * triple of F_el (6.8 GB at 128^3). */
+ * triple of F_el (6.8 GB at 128^3).  On a regular lattice (every node at
+ * origin + (i hx, j hy, k hz)) the element terms are the same in every cube,
+ * so the product is one 27-point stencil of 3 x 6 blocks, summed once on the
+ * host at creation from the same element formulas. */
 typedef struct cpm_fel cpm_fel;
 int cpm_fel_create(cpm_ctx* ctx, int32_t n, const double* coords, const double* C, cpm_fel** out);
 int cpm_fel_destroy(cpm_fel* fel);

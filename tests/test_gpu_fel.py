@@ -30,8 +30,10 @@ def _ops(n, jitter, rot, matrix_free, perm=None, me=True, deriv=False):
     return cfg, (coords, tets), P, S
 
 
-@pytest.mark.parametrize("n,jitter,rot", [(8, 0.0, False), (8, 0.15, True), (12, 0.15, False)])
+@pytest.mark.parametrize("n,jitter,rot", [(8, 0.0, False), (10, 0.0, True), (8, 0.15, True), (12, 0.15, False)])
 def test_matrix_free_product_equals_csr(n, jitter, rot):
+    """Regular meshes (the stencil form of the product) and jittered ones
+    (the element walk), cubic and [111]-rotated C."""
     from cupymag_amd.physics.fel import MatrixFreeFel
     _, _, _, Sc = _ops(n, jitter, rot, False)
     _, _, _, Sm = _ops(n, jitter, rot, True)
@@ -56,11 +58,11 @@ def test_unordered_mesh_keeps_csr():
     assert isinstance(S.csr_elasticity()[1], DeviceCSR)
 
 
-@pytest.mark.parametrize("rot", [False, True])
-def test_steps_with_matrix_free_fel_match_oracle(rot):
+@pytest.mark.parametrize("rot,jitter", [(False, 0.15), (True, 0.15), (True, 0.0)])
+def test_steps_with_matrix_free_fel_match_oracle(rot, jitter):
     from cupymag_amd.core.gspm import GSPMSimulation
     from oracle import gspm as og
-    cfg, arr, P, S = _ops(6, 0.15, rot, True)
+    cfg, arr, P, S = _ops(6, jitter, rot, True)
     p, ops, Avg, _ = oracle_setup(cfg, arr)
     m0 = tilted_m(ops["n"], seed=9)
     sim = GSPMSimulation(S, P, mode="fp64")
