@@ -583,9 +583,9 @@ def main():
     ap.add_argument("--rel-delta-gs", type=float, default=None,
                     help="reliable-update factor of the Gauss-Seidel solves (default 0.01)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default SpMV launch schedules")
-    ap.add_argument("--matrix-free-deriv", action="store_true",
-                    help="strain, Htilde and the demag right-hand side from the element walk instead of the "
-                         "Fx/Fy/Fz CSR operators")
+    ap.add_argument("--csr-deriv", action="store_true",
+                    help="strain, Htilde and the demag right-hand side from the Fx/Fy/Fz CSR matrices instead of "
+                         "the matrix-free element terms (a 27-point stencil on the regular 128^3 lattice)")
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--cpu-n", type=int, default=40)
     ap.add_argument("--cpu-steps", type=int, default=3)
@@ -627,7 +627,7 @@ def main():
     steps64 = args.steps_fp64
     g = run_gpu(args.n, args.steps, args.warmup, steps64, modes, args.tol, ref_steps=args.ref_steps,
                 rel_delta=args.rel_delta, autotune=not args.no_autotune, rel_delta_gs=args.rel_delta_gs,
-                mg_degree=args.mg_degree, mg_ratio=args.mg_ratio, matrix_free_deriv=args.matrix_free_deriv)
+                mg_degree=args.mg_degree, mg_ratio=args.mg_ratio, matrix_free_deriv=not args.csr_deriv)
     iters_full = g["modes"]["fp64"]["iters"][-1] if "fp64" in g["modes"] else None
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n, args.cpu_steps, args.tol, args.n, iters_full, g["nnz"])
     st = g["snap_steps"]

@@ -32,8 +32,10 @@ struct cpm_fel {
   double* dR = nullptr;      // 9 doubles: the rotation of cpm_fel_strain
   double C[36];
   // regular lattice (every node at origin + (i hx, j hy, k hz)): F_el E0 is
-  // one constant 27-point stencil, S[offset][d][v] (27 x 3 x 6), device copy
+  // one constant 27-point stencil, S[offset][d][v] (27 x 3 x 6), and so are
+  // Fx, Fy, Fz, FS[offset][k] (27 x 3); device copies
   double* S = nullptr;
+  double* FS = nullptr;
 };
 
 namespace cpm {
@@ -260,6 +262,59 @@ static void fel_stencil(const double h[3], const double C[36], const KuhnTets& k
   }
 }
 
+// Host: the stencil of k_fder_stencil for a regular lattice of spacings h:
+// FS[o*3 + k] = sum over the tetrahedra at the row's vertex a and their
+// nodes b at offset o of sum_q N_q[a] (dN_b/dx_k) detJ / 24 (the element
+// terms of k_fder).
+static void fder_stencil(const double h[3], const KuhnTets& kt, std::vector<double>& FS) {
+  const double a7 = 0.5854102, b7 = 0.1381966;
+  FS.assign(27 * 3, 0.0);
+  for (int o = 0; o < 8; ++o) {
+    const int lx = 1 - (o & 1), ly = 1 - ((o >> 1) & 1), lz = 1 - ((o >> 2) & 1);
+    const int lcode = lx | (ly << 1) | (lz << 2);
+    for (int t = 0; t < 6; ++t) {
+      int a = -1;
+      for (int q = 0; q < 4; ++q)
+        if (kt.v[t][q] == lcode) a = q;
+      if (a < 0) continue;
+      double x[4][3];
+      for (int q = 0; q < 4; ++q) {
+        const int c = kt.v[t][q];
+        x[q][0] = (c & 1) * h[0], x[q][1] = ((c >> 1) & 1) * h[1], x[q][2] = ((c >> 2) & 1) * h[2];
+      }
+      double J[3][3];
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) J[r][c] = x[r + 1][c] - x[0][c];
+      const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+      const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+      const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+      const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+      if (std::fabs(det) < 1e-14) continue;
+      const double id = 1.0 / det;
+      const double inv[3][3] = {
+          {c00 * id, (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id, (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id},
+          {c01 * id, (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id, (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id},
+          {c02 * id, (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id, (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id}};
+      const double dw = det * (1.0 / 24.0);
+      for (int b = 0; b < 4; ++b) {
+        const double dref[3] = {b == 0 ? -1.0 : (b == 1 ? 1.0 : 0.0), b == 0 ? -1.0 : (b == 2 ? 1.0 : 0.0),
+                                b == 0 ? -1.0 : (b == 3 ? 1.0 : 0.0)};
+        double g[3];
+        for (int r = 0; r < 3; ++r) g[r] = inv[r][0] * dref[0] + inv[r][1] * dref[1] + inv[r][2] * dref[2];
+        double f[3] = {0.0, 0.0, 0.0};
+        for (int q = 0; q < 4; ++q) {
+          const double nq = (q == a) ? a7 : b7;
+          for (int c = 0; c < 3; ++c) f[c] += nq * g[c] * dw;
+        }
+        const int cb = kt.v[t][b];
+        const int oi = (cb & 1) - lx, oj = ((cb >> 1) & 1) - ly, ok = ((cb >> 2) & 1) - lz;
+        const int off = (oi + 1) * 9 + (oj + 1) * 3 + (ok + 1);
+        for (int c = 0; c < 3; ++c) FS[off * 3 + c] += f[c];
+      }
+    }
+  }
+}
+
 // ---- matrix-free weak-form derivatives Fx, Fy, Fz -------------------------
 // F_k[a][b] = sum_q N_q[a] (dN_b/dx_k) detJ w_q (assemble_demag.py:235-305,
 // the host assembler's scalar_row with the same products in the same
@@ -324,6 +379,42 @@ struct FderOut {
   double* y;   // (n) F_k x
 };
 
+// The outputs of the derivative products at one DOF (k_fder, k_fder_stencil):
+// D[k][j] = (F_k u_j), sm[k] = (F_k m~_k), sh[k] = (F_k U), sx = (F_k x).
+__device__ inline void fder_finish(int64_t dof, int64_t N, const FderIn& in, const FderOut& out, double (&D)[3][3],
+                                   const double (&sm)[3], const double (&sh)[3], double sx) {
+  if (in.u) {
+    double e[6];
+    if (!in.rot) {
+      e[0] = D[0][0];
+      e[1] = D[1][1];
+      e[2] = D[2][2];
+      e[3] = D[1][0] + D[0][1];
+      e[4] = D[2][1] + D[1][2];
+      e[5] = D[2][0] + D[0][2];
+    } else {
+      // compute_derivatives.py:94-123; D[k][j]: derivative k of component j
+      const double* R = in.R;
+      auto mix = [&](int col, int jj) { return R[col] * D[0][jj] + R[3 + col] * D[1][jj] + R[6 + col] * D[2][jj]; };
+      e[0] = mix(0, 0);
+      e[1] = mix(1, 1);
+      e[2] = mix(2, 2);
+      e[3] = mix(1, 0) + mix(0, 1);
+      e[4] = mix(2, 1) + mix(1, 2);
+      e[5] = mix(2, 0) + mix(0, 2);
+    }
+#pragma unroll
+    for (int v = 0; v < 6; ++v) out.E[6 * dof + v] = e[v];
+  }
+  if (in.m) out.bd[dof] = (dof == 0) ? 0.0 : -((sm[0] + sm[1]) + sm[2]);
+  if (in.U) {
+    out.H[dof] = -sh[0];
+    out.H[N + dof] = -sh[1];
+    out.H[2 * N + dof] = -sh[2];
+  }
+  if (in.x) out.y[dof] = sx;
+}
+
 // One thread per DOF walks the 24 tetrahedra of its vertex and forms the
 // products requested in `in` (each output written once: deterministic).
 __global__ void __launch_bounds__(kBlock) k_fder(int32_t n, const double* __restrict__ coords, KuhnTets kt,
@@ -386,43 +477,72 @@ __global__ void __launch_bounds__(kBlock) k_fder(int32_t n, const double* __rest
         }
       }
     }
-    if (in.u) {
-      double e[6];
-      if (!in.rot) {
-        e[0] = D[0][0];
-        e[1] = D[1][1];
-        e[2] = D[2][2];
-        e[3] = D[1][0] + D[0][1];
-        e[4] = D[2][1] + D[1][2];
-        e[5] = D[2][0] + D[0][2];
-      } else {
-        // compute_derivatives.py:94-123; D[k][j]: derivative k of component j
-        const double* R = in.R;
-        auto mix = [&](int col, int jj) { return R[col] * D[0][jj] + R[3 + col] * D[1][jj] + R[6 + col] * D[2][jj]; };
-        e[0] = mix(0, 0);
-        e[1] = mix(1, 1);
-        e[2] = mix(2, 2);
-        e[3] = mix(1, 0) + mix(0, 1);
-        e[4] = mix(2, 1) + mix(1, 2);
-        e[5] = mix(2, 0) + mix(0, 2);
-      }
+    fder_finish(dof, N, in, out, D, sm, sh, sx);
+  }
+}
+
+// b = F x for the products of k_fder on a regular lattice: every row is the
+// same 27-point stencil, FS[o*3 + k] the coefficient of F_k at neighbour
+// offset o = (di+1)*9 + (dj+1)*3 + (dk+1)
+__global__ void __launch_bounds__(kBlock) k_fder_stencil(int32_t n, const double* __restrict__ FS, FderIn in,
+                                                         FderOut out) {
+  const int64_t N = (int64_t)n * n * n;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  const double a0 = in.m ? in.avg[0] : 0.0, a1 = in.m ? in.avg[1] : 0.0, a2 = in.m ? in.avg[2] : 0.0;
+  for (int64_t dof = (int64_t)blockIdx.x * kBlock + threadIdx.x; dof < N; dof += stride) {
+    const int32_t i = (int32_t)(dof / ((int64_t)n * n)), j = (int32_t)((dof / n) % n), k = (int32_t)(dof % n);
+    double D[3][3] = {};
+    double sm[3] = {0, 0, 0};
+    double sh[3] = {0, 0, 0};
+    double sx = 0.0;
+    for (int di = -1; di <= 1; ++di) {
+      int32_t ii = i + di;
+      ii += (ii < 0) ? n : 0;
+      ii -= (ii >= n) ? n : 0;
+      for (int dj = -1; dj <= 1; ++dj) {
+        int32_t jj = j + dj;
+        jj += (jj < 0) ? n : 0;
+        jj -= (jj >= n) ? n : 0;
 #pragma unroll
-      for (int v = 0; v < 6; ++v) out.E[6 * dof + v] = e[v];
+        for (int dk = -1; dk <= 1; ++dk) {
+          int32_t kk = k + dk;
+          kk += (kk < 0) ? n : 0;
+          kk -= (kk >= n) ? n : 0;
+          const int o = (di + 1) * 9 + (dj + 1) * 3 + (dk + 1);
+          const double f[3] = {FS[o * 3], FS[o * 3 + 1], FS[o * 3 + 2]};
+          const int64_t cb = ((int64_t)ii * n + jj) * n + kk;
+          if (in.u) {
+            const double uj[3] = {in.u[cb], in.u[N + cb], in.u[2 * N + cb]};
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+              for (int q = 0; q < 3; ++q) D[c][q] += f[c] * uj[q];
+          }
+          if (in.m) {
+            sm[0] += f[0] * (in.m[3 * cb] - a0);
+            sm[1] += f[1] * (in.m[3 * cb + 1] - a1);
+            sm[2] += f[2] * (in.m[3 * cb + 2] - a2);
+          }
+          if (in.U) {
+            const double uc = in.U[cb];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) sh[c] += f[c] * uc;
+          }
+          if (in.x) sx += f[in.k] * in.x[cb];
+        }
+      }
     }
-    if (in.m) out.bd[dof] = (dof == 0) ? 0.0 : -((sm[0] + sm[1]) + sm[2]);
-    if (in.U) {
-      out.H[dof] = -sh[0];
-      out.H[N + dof] = -sh[1];
-      out.H[2 * N + dof] = -sh[2];
-    }
-    if (in.x) out.y[dof] = sx;
+    fder_finish(dof, N, in, out, D, sm, sh, sx);
   }
 }
 
 int fder_apply(const cpm_fel* F, const FderIn& in, const FderOut& out) {
   static const KuhnTets kt = kuhn_tets();
   const int64_t N = (int64_t)F->n * F->n * F->n;
-  k_fder<<<grid_for(N), kBlock, 0, F->ctx->stream>>>(F->n, F->coords, kt, in, out);
+  if (F->FS)
+    k_fder_stencil<<<grid_for(N), kBlock, 0, F->ctx->stream>>>(F->n, F->FS, in, out);
+  else
+    k_fder<<<grid_for(N), kBlock, 0, F->ctx->stream>>>(F->n, F->coords, kt, in, out);
   CPM_HIP(hipGetLastError());
   return CPM_OK;
 }
@@ -487,9 +607,9 @@ extern "C" int cpm_fel_create(cpm_ctx* ctx, int32_t n, const double* coords, con
   hipError_t e = hipMalloc(&F->coords, bytes);
   if (e == hipSuccess) e = hipMalloc(&F->dR, sizeof(double) * 9);
   if (e == hipSuccess) e = hipMemcpy(F->coords, coords, bytes, hipMemcpyHostToDevice);
-  if (e == hipSuccess && C) {
+  if (e == hipSuccess) {
     // a regular lattice (nodes at origin + (i hx, j hy, k hz) to 1e-12 of
-    // the spacing): the F_el product as one stencil
+    // the spacing): the F_el product and the derivatives as stencils
     const int64_t n1 = n + 1;
     double h[3];
     for (int a = 0; a < 3; ++a) h[a] = coords[((a == 0 ? n1 * n1 : a == 1 ? n1 : 1)) * 3 + a] - coords[a];
@@ -502,17 +622,24 @@ extern "C" int cpm_fel_create(cpm_ctx* ctx, int32_t n, const double* coords, con
           for (int a = 0; a < 3; ++a)
             if (std::fabs(x[a] - (coords[a] + idx[a] * h[a])) > 1e-12 * h[a]) regular = false;
         }
-    if (regular) {
+    if (regular && C) {
       std::vector<double> S;
       fel_stencil(h, F->C, kuhn_tets(), S);
       e = hipMalloc(&F->S, sizeof(double) * S.size());
       if (e == hipSuccess) e = hipMemcpy(F->S, S.data(), sizeof(double) * S.size(), hipMemcpyHostToDevice);
+    }
+    if (regular && e == hipSuccess) {
+      std::vector<double> FS;
+      fder_stencil(h, kuhn_tets(), FS);
+      e = hipMalloc(&F->FS, sizeof(double) * FS.size());
+      if (e == hipSuccess) e = hipMemcpy(F->FS, FS.data(), sizeof(double) * FS.size(), hipMemcpyHostToDevice);
     }
   }
   if (e != hipSuccess) {
     if (F->coords) (void)hipFree(F->coords);
     if (F->dR) (void)hipFree(F->dR);
     if (F->S) (void)hipFree(F->S);
+    if (F->FS) (void)hipFree(F->FS);
     delete F;
     return fail(e == hipErrorOutOfMemory ? CPM_ENOMEM : CPM_EHIP, hipGetErrorString(e));
   }
@@ -527,6 +654,7 @@ extern "C" int cpm_fel_destroy(cpm_fel* F) {
   if (F->coords) (void)hipFree(F->coords);
   if (F->dR) (void)hipFree(F->dR);
   if (F->S) (void)hipFree(F->S);
+  if (F->FS) (void)hipFree(F->FS);
   delete F;
   return CPM_OK;
 }

@@ -82,8 +82,10 @@ def test_steps_with_matrix_free_fel_match_oracle(rot, jitter):
         assert all(abs(a - b) <= 1 for a, b in zip(rep.iters, ref_it)), (rep.iters, ref_it)
 
 
-@pytest.mark.parametrize("n,jitter", [(8, 0.0), (8, 0.15), (12, 0.15)])
+@pytest.mark.parametrize("n,jitter", [(8, 0.0), (11, 0.0), (8, 0.15), (12, 0.15)])
 def test_matrix_free_derivatives_equal_csr(n, jitter):
+    """Regular meshes (the 27-point stencil form) and jittered ones (the
+    element walk)."""
     from cupymag_amd.physics.fel import FelDerivView
     from oracle.params import rot111_R
     _, _, _, Sc = _ops(n, jitter, False, False)
@@ -109,15 +111,16 @@ def test_matrix_free_derivatives_equal_csr(n, jitter):
         assert np.abs(hm - hc).max() <= 1e-13 * np.abs(hc).max()
 
 
-@pytest.mark.parametrize("me,rot,mode", [(True, False, "fp64"), (True, True, "fp64"), (False, False, "fp64"),
-                                         (True, True, "mixed")])
-def test_steps_with_matrix_free_derivatives_match_oracle(me, rot, mode):
+@pytest.mark.parametrize("me,rot,mode,jitter", [(True, False, "fp64", 0.15), (True, True, "fp64", 0.15),
+                                                (False, False, "fp64", 0.15), (True, True, "mixed", 0.15),
+                                                (True, True, "fp64", 0.0)])
+def test_steps_with_matrix_free_derivatives_match_oracle(me, rot, mode, jitter):
     """Strain, Htilde and the demag right-hand side from the element walk
     (no Fx/Fy/Fz CSR on the device): steps match the oracle (FP64: m 1e-10,
     u 1e-10, iterations +-1; mixed: m 1e-7)."""
     from cupymag_amd.core.gspm import GSPMSimulation
     from oracle import gspm as og
-    cfg, arr, P, S = _ops(6, 0.15, rot, True, me=me, deriv=True)
+    cfg, arr, P, S = _ops(6, jitter, rot, True, me=me, deriv=True)
     assert S._upload()["F3"] is None
     p, ops, Avg, _ = oracle_setup(cfg, arr)
     m0 = tilted_m(ops["n"], seed=10)
