@@ -581,7 +581,7 @@ def main():
     ap.add_argument("--mg-degree", type=int, default=2, help="multigrid: Chebyshev smoother degree")
     ap.add_argument("--mg-ratio", type=float, default=10.0, help="multigrid: Chebyshev eigenvalue ratio")
     ap.add_argument("--rel-delta-gs", type=float, default=None,
-                    help="reliable-update factor of the Gauss-Seidel solves (default 0.01)")
+                    help="reliable-update factor of the Gauss-Seidel solves (default 1e-4)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default SpMV launch schedules")
     ap.add_argument("--csr-deriv", action="store_true",
                     help="strain, Htilde and the demag right-hand side from the Fx/Fy/Fz CSR matrices instead of "
