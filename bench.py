@@ -112,10 +112,10 @@ def outer_bytes(nnz, rows, half32=False):
     """Algorithmic bytes of the mixed multigrid PCG's outer product q = A p
     (k_pcg_spmv_rel, FP32 form): FP32 off-diagonal 3x3 blocks (each stored
     once when the FP32 form keeps half of the symmetric stencil), FP64
-    compensated diagonal blocks, p read and q written in FP64."""
+    compensated diagonal blocks, p read and q written in FP32."""
     nb = rows // 3
     off = nnz / 9.0 - nb
-    return (off / 2 if half32 else off) * 36.0 + nb * 72.0 + 2.0 * rows * 8.0
+    return (off / 2 if half32 else off) * 36.0 + nb * 72.0 + 2.0 * rows * 4.0
 
 
 def run_gpu(n, steps, warmup, steps64, modes, tol, ref_steps=REF_STEPS, rel_delta=0.3, autotune=True,
@@ -229,7 +229,7 @@ def run_gpu(n, steps, warmup, steps64, modes, tol, ref_steps=REF_STEPS, rel_delt
             avg2 = ms2.value / 1e3 / nl2.value
             d["outer"] = dict(kernel="k_pcg_spmv_rel<3,1,true> (A_el level 0, outer PCG product, FP32 3x3 blocks"
                                      + (", half-stored symmetric stencil" if info["half32"] else "")
-                                     + ", FP64 compensated diagonal blocks and vectors)",
+                                     + ", FP64 compensated diagonal blocks, FP32 p and q)",
                               rocname="k_pcg_spmv_rel<3, 1, true>", launches=nl2.value, avg_us=avg2 * 1e6,
                               bytes=ob, gbs=ob / avg2 / 1e9)
         out["modes"][mode] = d
@@ -676,8 +676,8 @@ def main():
                       "traffic_source": tr["source"] if tr else None, "avg_launch_us": o["avg_us"],
                       "bytes_per_launch": o["bytes"], "launches": o["launches"]}
     dtypes = {"fp64": "f64", "fast64": "f64", "mixed": "f32 CG in f64 refinement",
-              "mixed+mg": "f16 level-operator values and f32 vectors in the V-cycle, f32 outer products, "
-                          "f64 PCG recurrence and residual replacements",
+              "mixed+mg": "f16 level-operator values and f32 vectors in the V-cycle, f32 outer products "
+                          "and search directions, f64 iterate, residual and residual replacements",
               "fast64+mg": "f64"}
     dtypes["mixed+mg+xwarm"] = dtypes["mixed+mg"]
     out = {

@@ -379,7 +379,9 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(int32_t nb, const int64_t* 
 
 // q = A p of the mixed-precision PCG: the FP32 form with compensated FP64
 // diagonals, or the FP64 form once the solve has switched its products to
-// FP64 (CgState::p64); partials of p_j.q_j.  The FP64 products walk the
+// FP64 (CgState::p64); partials of p_j.q_j.  p and q are FP32 (the sums in
+// FP64, q rounded once; its p.q partials use the rounded q that r -= alpha q
+// reads).  The FP64 products walk the
 // stored stencil offsets two slots at a time (the same slots in the same
 // order as the compile-time Kuhn walk, so the same sums): the kernel's
 // registers, and so its occupancy, are those of the FP32 path.
@@ -389,7 +391,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CPM
                                                          const double* __restrict__ dgc,
                                                          const double* __restrict__ v64,
                                                          const double* __restrict__ dg64, GridArg ga,
-                                                         const double* __restrict__ p, double* __restrict__ q,
+                                                         const float* __restrict__ p, float* __restrict__ q,
                                                          const CgState* st, double* partials) {
   if (all_done<K>(st)) return;
   double acc[K];
@@ -398,11 +400,11 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CPM
   if (any_p64<K>(st)) {
     GridArg g64 = ga;
     g64.kuhn = 0;
-    sell_rows<BS, K, double, true, false, ST, double, 2>(xcd_perm(blockIdx.x, gridDim.x, ga.xg), gridDim.x, nb, sptr,
-                                                         col, v64, dg64, p, nullptr, q, g64, acc);
+    sell_rows<BS, K, float, true, false, ST, double, 2>(xcd_perm(blockIdx.x, gridDim.x, ga.xg), gridDim.x, nb, sptr,
+                                                        col, v64, dg64, p, nullptr, q, g64, acc);
   } else
-    sell_rows<BS, K, double, true, false, ST, float>(xcd_perm(blockIdx.x, gridDim.x, ga.xg), gridDim.x, nb, sptr, col, v32, dgc, p, nullptr, q,
-                                                     ga, acc);
+    sell_rows<BS, K, float, true, false, ST, float>(xcd_perm(blockIdx.x, gridDim.x, ga.xg), gridDim.x, nb, sptr, col,
+                                                    v32, dgc, p, nullptr, q, ga, acc);
   block_partial<K>(acc, partials);
 }
 
@@ -454,16 +456,16 @@ __global__ void __launch_bounds__(kBlock) k_pcg_zero_b(int64_t n, double* __rest
 }
 
 // live columns: x += alpha p, r_old = r, r -= alpha q; partials of r_j.r_j.
-template <int K>
+template <int K, class PQ>
 __global__ void __launch_bounds__(kBlock) k_pcg_xr(int64_t n, double* __restrict__ x, double* __restrict__ r,
-                                                   double* __restrict__ rold, const double* __restrict__ p,
-                                                   const double* __restrict__ q, const CgState* st,
+                                                   double* __restrict__ rold, const PQ* __restrict__ p,
+                                                   const PQ* __restrict__ q, const CgState* st,
                                                    double* partials) {
   if (all_done<K>(st)) return;
   double acc[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) acc[j] = 0.0;
-  pcg_xr_rows<K>(blockIdx.x, gridDim.x, n, st, x, r, rold, p, q, acc);
+  pcg_xr_rows<K, PQ>(blockIdx.x, gridDim.x, n, st, x, r, rold, p, q, acc);
   block_partial<K>(acc, partials);
 }
 
@@ -521,11 +523,11 @@ __global__ void __launch_bounds__(kRedBlock) k_pcg_red_rz(int nblk, const double
       if (!st[j].done) pcg_state_rz(st[j], v[j], v[K + j]);
 }
 
-template <int K, class T, int FB>
-__global__ void __launch_bounds__(kBlock) k_pcg_pupd(int64_t n, const T* __restrict__ z, double* __restrict__ p,
+template <int K, class T, int FB, class PQ>
+__global__ void __launch_bounds__(kBlock) k_pcg_pupd(int64_t n, const T* __restrict__ z, PQ* __restrict__ p,
                                                      const double* __restrict__ r, const CgState* st) {
   if (all_done<K>(st)) return;
-  pcg_pupd_rows<K, T, FB>(blockIdx.x, gridDim.x, n, st, z, p, r);
+  pcg_pupd_rows<K, T, FB, PQ>(blockIdx.x, gridDim.x, n, st, z, p, r);
 }
 
 // ---------------------------------------------------------------- host
@@ -827,35 +829,40 @@ static int pcg_run(cpm_mg* M, const PcgLane& ln, const double* b, double* x, int
   } restore{M};
   const bool rel = delta > 0.0;
   // ev (profiling): the product is timed by its dispatch (mixed, stencil form)
-  auto spmv = [&](const double* in, double* out, bool plain, hipEvent_t* ev = nullptr) {
+  // p and q: FP32 in the mixed-precision solves (rel), FP64 otherwise; in
+  // and out are FP64 for the start's product (plain)
+  auto spmv = [&](const void* in, void* out, bool plain, hipEvent_t* ev = nullptr) {
     if (!plain && rel) {
       if (S.stencil && ev)
         hipExtLaunchKernelGGL(k_pcg_spmv_rel<BS, K, true>, dim3(gs), dim3(kBlock), 0, s, ev[0], ev[1], 0,
                               (int32_t)S.nbrows, (const int64_t*)S.sptr, (const int32_t*)S.col,
                               (const float*)S.v32, (const double*)S.dgc, (const double*)S.v64,
-                              (const double*)S.dg64, CPM_GRID_ARG32(S), in, out, (const CgState*)st, partials);
+                              (const double*)S.dg64, CPM_GRID_ARG32(S), (const float*)in, (float*)out,
+                              (const CgState*)st, partials);
       else if (S.stencil)
         k_pcg_spmv_rel<BS, K, true><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v32, S.dgc, S.v64, S.dg64,
-                                                          CPM_GRID_ARG32(S), in, out, st, partials);
+                                                          CPM_GRID_ARG32(S), (const float*)in, (float*)out, st,
+                                                          partials);
       else
         k_pcg_spmv_rel<BS, K, false><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, S.v32, S.dgc, S.v64, S.dg64,
-                                                           CPM_GRID_ARG32(S), in, out, st, partials);
+                                                           CPM_GRID_ARG32(S), (const float*)in, (float*)out, st,
+                                                           partials);
       return;
     }
     if (plain) {
       if (S.stencil)
         k_pcg_spmv<BS, K, true, true><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, sell_v64(S), S.dg64,
-                                                            CPM_GRID_ARG64(S), in, out, st, partials);
+                                                            CPM_GRID_ARG64(S), (const double*)in, (double*)out, st, partials);
       else
         k_pcg_spmv<BS, K, false, true><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, sell_v64(S), S.dg64,
-                                                             CPM_GRID_ARG64(S), in, out, st, partials);
+                                                             CPM_GRID_ARG64(S), (const double*)in, (double*)out, st, partials);
     } else {
       if (S.stencil)
         k_pcg_spmv<BS, K, true, false><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, sell_v64(S), S.dg64,
-                                                             CPM_GRID_ARG64(S), in, out, st, partials);
+                                                             CPM_GRID_ARG64(S), (const double*)in, (double*)out, st, partials);
       else
         k_pcg_spmv<BS, K, false, false><<<gs, kBlock, 0, s>>>(S.nbrows, S.sptr, S.col, sell_v64(S), S.dg64,
-                                                              CPM_GRID_ARG64(S), in, out, st, partials);
+                                                              CPM_GRID_ARG64(S), (const double*)in, (double*)out, st, partials);
     }
   };
   // floating hierarchy (one column): z = F^T V(F r) + e_0 r_0, the sums of
@@ -873,10 +880,14 @@ static int pcg_run(cpm_mg* M, const PcgLane& ln, const double* b, double* x, int
     k_pcg_red_rz<K><<<1, kRedBlock, 0, s>>>(gv, partials, st);
   };
   auto pupd = [&]() {
-    if (fl)
-      k_pcg_pupd<K, T, FB><<<gv, kBlock, 0, s>>>(n, (const T*)L0.x, M->p, M->r, st);
+    if (fl && rel)
+      k_pcg_pupd<K, T, FB, float><<<gv, kBlock, 0, s>>>(n, (const T*)L0.x, (float*)M->p, M->r, st);
+    else if (fl)
+      k_pcg_pupd<K, T, FB, double><<<gv, kBlock, 0, s>>>(n, (const T*)L0.x, M->p, M->r, st);
+    else if (rel)
+      k_pcg_pupd<K, T, 0, float><<<gv, kBlock, 0, s>>>(n, (const T*)L0.x, (float*)M->p, M->r, st);
     else
-      k_pcg_pupd<K, T, 0><<<gv, kBlock, 0, s>>>(n, (const T*)L0.x, M->p, M->r, st);
+      k_pcg_pupd<K, T, 0, double><<<gv, kBlock, 0, s>>>(n, (const T*)L0.x, M->p, M->r, st);
   };
   // start: r = b - A x0 (or b), state, z = M r, rho = r.z
   if (use_x0) spmv(x, M->q, true);
@@ -901,7 +912,11 @@ static int pcg_run(cpm_mg* M, const PcgLane& ln, const double* b, double* x, int
         k_red<K, RED_ALPHA_PCG_REL><<<1, kRedBlock, 0, s>>>(gs, partials, st, 0.0);
       else
         k_red<K, RED_ALPHA><<<1, kRedBlock, 0, s>>>(gs, partials, st, 0.0);
-      k_pcg_xr<K><<<gv, kBlock, 0, s>>>(n, x, M->r, M->rold, M->p, M->q, st, partials);
+      if (rel)
+        k_pcg_xr<K, float><<<gv, kBlock, 0, s>>>(n, x, M->r, M->rold, (const float*)M->p, (const float*)M->q, st,
+                                                 partials);
+      else
+        k_pcg_xr<K, double><<<gv, kBlock, 0, s>>>(n, x, M->r, M->rold, M->p, M->q, st, partials);
       if (rel) {
         k_red<K, RED_RR_PCG_REL><<<1, kRedBlock, 0, s>>>(gv, partials, st, delta);
         if (S.stencil)

@@ -594,11 +594,13 @@ __device__ __forceinline__ void pcg_init_rows(int64_t lb, int64_t lg, int64_t n,
 }
 
 // live columns: x += alpha p, r_old = r, r -= alpha q; acc[j] += r_j.r_j.
-template <int K>
+// p and q of type PQ: FP64, or FP32 in the mixed-precision solves (whose
+// products are FP32-accurate; x and r stay FP64).
+template <int K, class PQ = double>
 __device__ __forceinline__ void pcg_xr_rows(int64_t lb, int64_t lg, int64_t n, const CgState* st,
                                             double* __restrict__ x, double* __restrict__ r,
-                                            double* __restrict__ rold, const double* __restrict__ p,
-                                            const double* __restrict__ q, double (&acc)[K]) {
+                                            double* __restrict__ rold, const PQ* __restrict__ p,
+                                            const PQ* __restrict__ q, double (&acc)[K]) {
   double alpha[K];
   bool live[K];
 #pragma unroll
@@ -607,10 +609,10 @@ __device__ __forceinline__ void pcg_xr_rows(int64_t lb, int64_t lg, int64_t n, c
   for (int64_t i = lb * kBlock + threadIdx.x; i < n; i += stride) {
     const int j = (int)(i % K);
     if (!live[j]) continue;
-    x[i] += alpha[j] * p[i];
+    x[i] += alpha[j] * (double)p[i];
     const double ro = r[i];
     rold[i] = ro;
-    const double rn = ro - alpha[j] * q[i];
+    const double rn = ro - alpha[j] * (double)q[i];
     r[i] = rn;
     acc[j] += rn * rn;
   }
@@ -655,10 +657,11 @@ __device__ __forceinline__ void pcg_dot_rows(int64_t lb, int64_t lg, int64_t n, 
   }
 }
 
-// live columns: p = z (first iteration) or z + beta p
-template <int K, class T, int FB = 0>
+// live columns: p = z (first iteration) or z + beta p, formed in FP64 and
+// stored as PQ
+template <int K, class T, int FB = 0, class PQ = double>
 __device__ __forceinline__ void pcg_pupd_rows(int64_t lb, int64_t lg, int64_t n, const CgState* st,
-                                              const T* __restrict__ z, double* __restrict__ p,
+                                              const T* __restrict__ z, PQ* __restrict__ p,
                                               const double* __restrict__ r) {
   static_assert(FB == 0 || K == 1, "floating hierarchies solve one column");
   double beta[K];
@@ -670,7 +673,7 @@ __device__ __forceinline__ void pcg_pupd_rows(int64_t lb, int64_t lg, int64_t n,
     const int j = (int)(i % K);
     if (!live[j]) continue;
     const double zi = pcg_z<FB, T>(i, z, r);
-    p[i] = first[j] ? zi : zi + beta[j] * p[i];
+    p[i] = (PQ)(first[j] ? zi : zi + beta[j] * (double)p[i]);
   }
 }
 

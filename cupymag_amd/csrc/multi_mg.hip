@@ -37,7 +37,8 @@ struct OuterSeg {
   int64_t n;
   const double* b;
   double* x;
-  double *r, *p, *q, *rold;
+  double *r, *rold;
+  void *p, *q;            // FP32 for the reliable-update systems (delta > 0), else FP64
   const void* z;
   CgState* st;
   int32_t blk0, nblk;    // vector phases: grid_for(n) workgroups
@@ -159,26 +160,39 @@ __global__ void __launch_bounds__(kBlock) kbm_spmv(const OuterSeg* __restrict__ 
 #pragma unroll
   for (int j = 0; j < K; ++j) acc[j] = 0.0;
   const int64_t lb = xcd_perm((int64_t)blockIdx.x - d.sblk0, d.snblk, d.ga.xg);
-  const double* in = PLAIN ? d.x : d.p;
-  if (!PLAIN && d.delta > 0.0 && !any_p64<K>(d.st)) {
-    if (d.stencil)
-      sell_rows<BS, K, double, true, false, true, float>(lb, d.snblk, d.nb, d.sptr, d.col, d.v32, d.dgc, in, nullptr,
-                                                         d.q, d.ga, acc);
-    else
-      sell_rows<BS, K, double, true, false, false, float>(lb, d.snblk, d.nb, d.sptr, d.col, d.v32, d.dgc, in,
-                                                          nullptr, d.q, d.ga, acc);
+  if (!PLAIN && d.delta > 0.0) {
+    // reliable-update systems: FP32 p and q, the FP32 form, or every FP64
+    // slot two at a time once the system's products went to FP64 (as the
+    // one-system kernel reads)
+    const float* in = (const float*)d.p;
+    float* out = (float*)d.q;
+    if (!any_p64<K>(d.st)) {
+      if (d.stencil)
+        sell_rows<BS, K, float, true, false, true, float>(lb, d.snblk, d.nb, d.sptr, d.col, d.v32, d.dgc, in,
+                                                          nullptr, out, d.ga, acc);
+      else
+        sell_rows<BS, K, float, true, false, false, float>(lb, d.snblk, d.nb, d.sptr, d.col, d.v32, d.dgc, in,
+                                                           nullptr, out, d.ga, acc);
+    } else {
+      GridArg g64 = d.ga;
+      g64.kuhn = 0;
+      if (d.stencil)
+        sell_rows<BS, K, float, true, false, true, double, 2>(lb, d.snblk, d.nb, d.sptr, d.col, d.v64, d.dg64, in,
+                                                              nullptr, out, g64, acc);
+      else
+        sell_rows<BS, K, float, true, false, false, double, 2>(lb, d.snblk, d.nb, d.sptr, d.col, d.v64, d.dg64, in,
+                                                               nullptr, out, g64, acc);
+    }
   } else {
-    // FP64 products: the half copy, except for the FP64 fallback of the
-    // reliable-update systems (every slot, as the one-system kernel reads)
-    const bool h = PLAIN || d.delta <= 0.0;
-    const double* v = h ? d.v64h : d.v64;
-    const GridArg& ga = h ? d.ga64 : d.ga;
+    // FP64 products with the half copy of the values
+    const double* in = PLAIN ? d.x : (const double*)d.p;
+    double* out = (double*)d.q;
     if (d.stencil)
-      sell_rows<BS, K, double, !PLAIN, false, true>(lb, d.snblk, d.nb, d.sptr, d.col, v, d.dg64, in, nullptr, d.q,
-                                                    ga, acc);
+      sell_rows<BS, K, double, !PLAIN, false, true>(lb, d.snblk, d.nb, d.sptr, d.col, d.v64h, d.dg64, in, nullptr,
+                                                    out, d.ga64, acc);
     else
-      sell_rows<BS, K, double, !PLAIN, false, false>(lb, d.snblk, d.nb, d.sptr, d.col, v, d.dg64, in, nullptr, d.q,
-                                                     ga, acc);
+      sell_rows<BS, K, double, !PLAIN, false, false>(lb, d.snblk, d.nb, d.sptr, d.col, d.v64h, d.dg64, in, nullptr,
+                                                     out, d.ga64, acc);
   }
   if (!PLAIN) block_partial<K>(acc, partials);
 }
@@ -208,7 +222,7 @@ __global__ void __launch_bounds__(kBlock) kbm_init(const OuterSeg* __restrict__ 
   double acc[2 * K];
 #pragma unroll
   for (int j = 0; j < 2 * K; ++j) acc[j] = 0.0;
-  pcg_init_rows<K>((int64_t)blockIdx.x - d.blk0, d.nblk, d.n, d.b, d.x, d.q, d.r, d.use_x0, acc);
+  pcg_init_rows<K>((int64_t)blockIdx.x - d.blk0, d.nblk, d.n, d.b, d.x, (const double*)d.q, d.r, d.use_x0, acc);
   block_partial<2 * K>(acc, partials);
 }
 
@@ -228,7 +242,11 @@ __global__ void __launch_bounds__(kBlock) kbm_xr(const OuterSeg* __restrict__ se
   double acc[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) acc[j] = 0.0;
-  pcg_xr_rows<K>((int64_t)blockIdx.x - d.blk0, d.nblk, d.n, d.st, d.x, d.r, d.rold, d.p, d.q, acc);
+  const int64_t lb = (int64_t)blockIdx.x - d.blk0;
+  if (d.delta > 0.0)
+    pcg_xr_rows<K, float>(lb, d.nblk, d.n, d.st, d.x, d.r, d.rold, (const float*)d.p, (const float*)d.q, acc);
+  else
+    pcg_xr_rows<K, double>(lb, d.nblk, d.n, d.st, d.x, d.r, d.rold, (const double*)d.p, (const double*)d.q, acc);
   block_partial<K>(acc, partials);
 }
 
@@ -271,13 +289,20 @@ __global__ void __launch_bounds__(kBlock) kbm_pupd(const OuterSeg* __restrict__ 
   const OuterSeg& d = seg[map[blockIdx.x]];
   if (all_done<K>(d.st)) return;
   const int64_t lb = (int64_t)blockIdx.x - d.blk0;
+  const bool rel = d.delta > 0.0;
   if constexpr (K == 1) {
     if (d.floating) {
-      pcg_pupd_rows<K, T, BS>(lb, d.nblk, d.n, d.st, (const T*)d.z, d.p, d.r);
+      if (rel)
+        pcg_pupd_rows<K, T, BS, float>(lb, d.nblk, d.n, d.st, (const T*)d.z, (float*)d.p, d.r);
+      else
+        pcg_pupd_rows<K, T, BS, double>(lb, d.nblk, d.n, d.st, (const T*)d.z, (double*)d.p, d.r);
       return;
     }
   }
-  pcg_pupd_rows<K, T, 0>(lb, d.nblk, d.n, d.st, (const T*)d.z, d.p, d.r);
+  if (rel)
+    pcg_pupd_rows<K, T, 0, float>(lb, d.nblk, d.n, d.st, (const T*)d.z, (float*)d.p, d.r);
+  else
+    pcg_pupd_rows<K, T, 0, double>(lb, d.nblk, d.n, d.st, (const T*)d.z, (double*)d.p, d.r);
 }
 
 enum : int { MRED_INIT = 0, MRED_ALPHA = 1, MRED_RR = 2, MRED_RZ = 3, MRED_TRUE = 4 };
