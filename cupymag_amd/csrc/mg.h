@@ -84,7 +84,7 @@ struct cpm_mg {
     return (l == 0 || !degree1) ? degree : degree1;
   }
   // FP64 PCG vectors (fine level); level and PCG vectors hold kcap columns
-  double *r = nullptr, *p = nullptr, *q = nullptr, *rold = nullptr;
+  double *r = nullptr, *p = nullptr, *q = nullptr;
   int kcap = 1;
   // stream the V-cycle kernels go to (null: the context's stream); the
   // lane's stream during a solve, the side stream while a chunk of

@@ -455,30 +455,29 @@ __global__ void __launch_bounds__(kBlock) k_pcg_zero_b(int64_t n, double* __rest
   zero_b_rows<K>(blockIdx.x, gridDim.x, n, x, st);
 }
 
-// live columns: x += alpha p, r_old = r, r -= alpha q; partials of r_j.r_j.
+// live columns: x += alpha p, r -= alpha q; partials of r_j.r_j.
 template <int K, class PQ>
 __global__ void __launch_bounds__(kBlock) k_pcg_xr(int64_t n, double* __restrict__ x, double* __restrict__ r,
-                                                   double* __restrict__ rold, const PQ* __restrict__ p,
-                                                   const PQ* __restrict__ q, const CgState* st,
-                                                   double* partials) {
+                                                   const PQ* __restrict__ p, const PQ* __restrict__ q,
+                                                   const CgState* st, double* partials) {
   if (all_done<K>(st)) return;
   double acc[K];
 #pragma unroll
   for (int j = 0; j < K; ++j) acc[j] = 0.0;
-  pcg_xr_rows<K, PQ>(blockIdx.x, gridDim.x, n, st, x, r, rold, p, q, acc);
+  pcg_xr_rows<K, PQ>(blockIdx.x, gridDim.x, n, st, x, r, p, q, acc);
   block_partial<K>(acc, partials);
 }
 
-// partials of r_j.z_j and r_old_j.z_j (FB: see pcg_z)
-template <int K, class T, int FB>
+// partials of r_j.z_j and q_j.z_j (FB: see pcg_z)
+template <int K, class T, int FB, class PQ>
 __global__ void __launch_bounds__(kBlock) k_pcg_dot(int64_t n, const double* __restrict__ r,
-                                                    const double* __restrict__ rold, const T* __restrict__ z,
+                                                    const PQ* __restrict__ q, const T* __restrict__ z,
                                                     const CgState* st, double* partials) {
   if (all_done<K>(st)) return;
   double acc[2 * K];
 #pragma unroll
   for (int j = 0; j < 2 * K; ++j) acc[j] = 0.0;
-  pcg_dot_rows<K, T, FB>(blockIdx.x, gridDim.x, n, st, r, rold, z, acc);
+  pcg_dot_rows<K, T, FB, PQ>(blockIdx.x, gridDim.x, n, st, r, q, z, acc);
   block_partial<2 * K>(acc, partials);
 }
 
@@ -792,10 +791,8 @@ int mg_reserve(cpm_mg* M, int K, hipStream_t s) {
     for (void** p : {&Lv.x, &Lv.r, &Lv.b, &Lv.d0, &Lv.d1})
       if (int rc = regrow(p, ts * Lv.n * K)) return rc;
   const int64_t n0 = M->L[0].n;
-  for (void** p : {(void**)&M->r, (void**)&M->p, (void**)&M->q, (void**)&M->rold})
+  for (void** p : {(void**)&M->r, (void**)&M->p, (void**)&M->q})
     if (int rc = regrow(p, 8 * n0 * K)) return rc;
-  CPM_HIP(hipMemsetAsync(M->rold, 0, 8 * n0 * K, s));
-  CPM_HIP(hipStreamSynchronize(s));
   M->kcap = K;
   return CPM_OK;
 }
@@ -873,10 +870,14 @@ static int pcg_run(cpm_mg* M, const PcgLane& ln, const double* b, double* x, int
   auto precond = [&](hipEvent_t* ev) {
     if (fl) k_pcg_rsum<BS * K, K><<<gv, kBlock, 0, s>>>(n, M->r, st, partials);
     vcycle<BS, K, T>(M, 0, M->r, st, ev);
-    if (fl)
-      k_pcg_dot<K, T, FB><<<gv, kBlock, 0, s>>>(n, M->r, M->rold, (const T*)L0.x, st, partials);
+    if (fl && rel)
+      k_pcg_dot<K, T, FB, float><<<gv, kBlock, 0, s>>>(n, M->r, (const float*)M->q, (const T*)L0.x, st, partials);
+    else if (fl)
+      k_pcg_dot<K, T, FB, double><<<gv, kBlock, 0, s>>>(n, M->r, M->q, (const T*)L0.x, st, partials);
+    else if (rel)
+      k_pcg_dot<K, T, 0, float><<<gv, kBlock, 0, s>>>(n, M->r, (const float*)M->q, (const T*)L0.x, st, partials);
     else
-      k_pcg_dot<K, T, 0><<<gv, kBlock, 0, s>>>(n, M->r, M->rold, (const T*)L0.x, st, partials);
+      k_pcg_dot<K, T, 0, double><<<gv, kBlock, 0, s>>>(n, M->r, M->q, (const T*)L0.x, st, partials);
     k_pcg_red_rz<K><<<1, kRedBlock, 0, s>>>(gv, partials, st);
   };
   auto pupd = [&]() {
@@ -913,10 +914,9 @@ static int pcg_run(cpm_mg* M, const PcgLane& ln, const double* b, double* x, int
       else
         k_red<K, RED_ALPHA><<<1, kRedBlock, 0, s>>>(gs, partials, st, 0.0);
       if (rel)
-        k_pcg_xr<K, float><<<gv, kBlock, 0, s>>>(n, x, M->r, M->rold, (const float*)M->p, (const float*)M->q, st,
-                                                 partials);
+        k_pcg_xr<K, float><<<gv, kBlock, 0, s>>>(n, x, M->r, (const float*)M->p, (const float*)M->q, st, partials);
       else
-        k_pcg_xr<K, double><<<gv, kBlock, 0, s>>>(n, x, M->r, M->rold, M->p, M->q, st, partials);
+        k_pcg_xr<K, double><<<gv, kBlock, 0, s>>>(n, x, M->r, M->p, M->q, st, partials);
       if (rel) {
         k_red<K, RED_RR_PCG_REL><<<1, kRedBlock, 0, s>>>(gv, partials, st, delta);
         if (S.stencil)
@@ -1453,12 +1453,7 @@ static int mg_create(cpm_ctx* ctx, int nlevels, cpm_mat* const* levels, const do
       if (e != hipSuccess) rc = fail(CPM_EHIP, hipGetErrorString(e));
     }
     const int64_t n0 = M->L[0].n;
-    for (void** p : {(void**)&M->r, (void**)&M->p, (void**)&M->q, (void**)&M->rold}) A(p, 8 * n0);
-    if (rc == CPM_OK) {
-      hipError_t e = hipMemsetAsync(M->rold, 0, 8 * n0, s);
-      if (e == hipSuccess) e = hipStreamSynchronize(s);
-      if (e != hipSuccess) rc = fail(CPM_EHIP, hipGetErrorString(e));
-    }
+    for (void** p : {(void**)&M->r, (void**)&M->p, (void**)&M->q}) A(p, 8 * n0);
   }
   if (rc != CPM_OK) {
     cpm_mg_destroy(M);

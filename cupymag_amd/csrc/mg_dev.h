@@ -593,14 +593,13 @@ __device__ __forceinline__ void pcg_init_rows(int64_t lb, int64_t lg, int64_t n,
   }
 }
 
-// live columns: x += alpha p, r_old = r, r -= alpha q; acc[j] += r_j.r_j.
-// p and q of type PQ: FP64, or FP32 in the mixed-precision solves (whose
-// products are FP32-accurate; x and r stay FP64).
+// live columns: x += alpha p, r -= alpha q; acc[j] += r_j.r_j.  p and q of
+// type PQ: FP64, or FP32 in the mixed-precision solves (whose products are
+// FP32-accurate; x and r stay FP64).
 template <int K, class PQ = double>
 __device__ __forceinline__ void pcg_xr_rows(int64_t lb, int64_t lg, int64_t n, const CgState* st,
                                             double* __restrict__ x, double* __restrict__ r,
-                                            double* __restrict__ rold, const PQ* __restrict__ p,
-                                            const PQ* __restrict__ q, double (&acc)[K]) {
+                                            const PQ* __restrict__ p, const PQ* __restrict__ q, double (&acc)[K]) {
   double alpha[K];
   bool live[K];
 #pragma unroll
@@ -610,9 +609,7 @@ __device__ __forceinline__ void pcg_xr_rows(int64_t lb, int64_t lg, int64_t n, c
     const int j = (int)(i % K);
     if (!live[j]) continue;
     x[i] += alpha[j] * (double)p[i];
-    const double ro = r[i];
-    rold[i] = ro;
-    const double rn = ro - alpha[j] * (double)q[i];
+    const double rn = r[i] - alpha[j] * (double)q[i];
     r[i] = rn;
     acc[j] += rn * rn;
   }
@@ -639,10 +636,11 @@ __device__ __forceinline__ double pcg_z(int64_t i, const T* __restrict__ z, cons
   }
 }
 
-// acc[j] = r_j.z_j, acc[K + j] = r_old_j.z_j (the latter after the first iteration)
-template <int K, class T, int FB = 0>
+// acc[j] = r_j.z_j, acc[K + j] = q_j.z_j (after the first iteration; q = A p
+// of the iteration, type PQ)
+template <int K, class T, int FB = 0, class PQ = double>
 __device__ __forceinline__ void pcg_dot_rows(int64_t lb, int64_t lg, int64_t n, const CgState* st,
-                                             const double* __restrict__ r, const double* __restrict__ rold,
+                                             const double* __restrict__ r, const PQ* __restrict__ q,
                                              const T* __restrict__ z, double (&acc)[2 * K]) {
   static_assert(FB == 0 || K == 1, "floating hierarchies solve one column");
   bool first[K];
@@ -653,7 +651,7 @@ __device__ __forceinline__ void pcg_dot_rows(int64_t lb, int64_t lg, int64_t n, 
     const int j = (int)(i % K);
     const double zi = pcg_z<FB, T>(i, z, r);
     acc[j] += r[i] * zi;
-    if (!first[j]) acc[K + j] += rold[i] * zi;
+    if (!first[j]) acc[K + j] += (double)q[i] * zi;
   }
 }
 
@@ -715,14 +713,15 @@ __device__ __forceinline__ bool any_upd(const CgState* st) {
   return u;
 }
 
-// rho = r.z, beta = (r.z - r_old.z) / rho_old (flexible CG), from the sums v.
+// rho = r.z, beta = (r - r_old).z / rho_old = -alpha q.z / rho_old (flexible
+// CG: r - r_old = -alpha q), from the sums r.z and q.z.
 // r.z of an SPD preconditioner is positive; a preconditioner applied in
 // reduced precision to a residual at its rounding level can lose that, and
 // then the direction restarts (beta = 0) and the products go to FP64.
-__device__ inline void pcg_state_rz(CgState& c, double rz, double rold_z) {
+__device__ inline void pcg_state_rz(CgState& c, double rz, double qz) {
   c.rho_old = c.rho;
   c.rho = rz;
-  c.beta = (c.it == 0) ? 0.0 : (rz - rold_z) / c.rho_old;
+  c.beta = (c.it == 0) ? 0.0 : -c.alpha * qz / c.rho_old;
   if (!isfinite(c.beta) || !(rz > 0.0)) {
     c.beta = 0.0;
     c.p64 = 1;

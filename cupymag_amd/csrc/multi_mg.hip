@@ -37,7 +37,7 @@ struct OuterSeg {
   int64_t n;
   const double* b;
   double* x;
-  double *r, *rold;
+  double* r;
   void *p, *q;            // FP32 for the reliable-update systems (delta > 0), else FP64
   const void* z;
   CgState* st;
@@ -244,9 +244,9 @@ __global__ void __launch_bounds__(kBlock) kbm_xr(const OuterSeg* __restrict__ se
   for (int j = 0; j < K; ++j) acc[j] = 0.0;
   const int64_t lb = (int64_t)blockIdx.x - d.blk0;
   if (d.delta > 0.0)
-    pcg_xr_rows<K, float>(lb, d.nblk, d.n, d.st, d.x, d.r, d.rold, (const float*)d.p, (const float*)d.q, acc);
+    pcg_xr_rows<K, float>(lb, d.nblk, d.n, d.st, d.x, d.r, (const float*)d.p, (const float*)d.q, acc);
   else
-    pcg_xr_rows<K, double>(lb, d.nblk, d.n, d.st, d.x, d.r, d.rold, (const double*)d.p, (const double*)d.q, acc);
+    pcg_xr_rows<K, double>(lb, d.nblk, d.n, d.st, d.x, d.r, (const double*)d.p, (const double*)d.q, acc);
   block_partial<K>(acc, partials);
 }
 
@@ -273,14 +273,23 @@ __global__ void __launch_bounds__(kBlock) kbm_dot(const OuterSeg* __restrict__ s
 #pragma unroll
   for (int j = 0; j < 2 * K; ++j) acc[j] = 0.0;
   const int64_t lb = (int64_t)blockIdx.x - d.blk0;
+  const bool rel = d.delta > 0.0;
   bool done = false;
   if constexpr (K == 1) {
     if (d.floating) {
-      pcg_dot_rows<K, T, BS>(lb, d.nblk, d.n, d.st, d.r, d.rold, (const T*)d.z, acc);
+      if (rel)
+        pcg_dot_rows<K, T, BS, float>(lb, d.nblk, d.n, d.st, d.r, (const float*)d.q, (const T*)d.z, acc);
+      else
+        pcg_dot_rows<K, T, BS, double>(lb, d.nblk, d.n, d.st, d.r, (const double*)d.q, (const T*)d.z, acc);
       done = true;
     }
   }
-  if (!done) pcg_dot_rows<K, T, 0>(lb, d.nblk, d.n, d.st, d.r, d.rold, (const T*)d.z, acc);
+  if (!done) {
+    if (rel)
+      pcg_dot_rows<K, T, 0, float>(lb, d.nblk, d.n, d.st, d.r, (const float*)d.q, (const T*)d.z, acc);
+    else
+      pcg_dot_rows<K, T, 0, double>(lb, d.nblk, d.n, d.st, d.r, (const double*)d.q, (const T*)d.z, acc);
+  }
   block_partial<2 * K>(acc, partials);
 }
 
@@ -455,7 +464,7 @@ static int run(cpm_ctx* ctx, MultiMgTables& W, const std::vector<MultiMgSys>& sy
     o.ga = CPM_GRID_ARG32(S);  // v64 holds every slot: the FP64 products read it in full (h64 = 0)
     o.ga64 = CPM_GRID_ARG64(S);
     o.nb = S.nbrows, o.stencil = S.stencil, o.n = M->L[0].n * K;
-    o.b = y.b, o.x = y.x, o.r = M->r, o.p = M->p, o.q = M->q, o.rold = M->rold, o.z = M->L[0].x;
+    o.b = y.b, o.x = y.x, o.r = M->r, o.p = M->p, o.q = M->q, o.z = M->L[0].x;
     o.st = W.st + j * K;
     o.blk0 = (int32_t)O.map.size();
     o.nblk = grid_for(o.n);
