@@ -47,9 +47,10 @@
 // into a level that is not the coarsest is fused with that level's first
 // Chebyshev step.
 //
-// The outer solver is CG preconditioned with one V-cycle per iteration: x,
-// r, p, q in FP64, z from the V-cycle; beta = (r - r_old).z / rho_old
-// (flexible CG, robust to a preconditioner applied in lower precision), and
+// The outer solver is CG preconditioned with one V-cycle per iteration: x
+// and r in FP64, p and q in FP64 (FP32 when delta > 0), z from the V-cycle;
+// beta = (r - r_old).z / rho_old = -alpha q.z / rho_old (flexible CG, robust
+// to a preconditioner applied in lower precision), and
 // CuPy's stopping rule on the residual: ||r|| <= tol ||b|| after the update,
 // at least one iteration, b = 0 gives 0.  The products A p use the FP64
 // form, or (delta > 0) the FP32 form with an FP64 residual replacement
@@ -503,7 +504,7 @@ __global__ void __launch_bounds__(kBlock) k_mg_float_out(int64_t n, const T* __r
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) z[i] = pcg_z<BS, T>(i, y, r);
 }
 
-// live columns: rho = r.z; beta = (r.z - r_old.z) / rho_old (flexible CG)
+// live columns: rho = r.z; beta = -alpha q.z / rho_old (flexible CG)
 template <int K>
 __global__ void __launch_bounds__(kRedBlock) k_pcg_red_rz(int nblk, const double* __restrict__ partials,
                                                           CgState* st) {

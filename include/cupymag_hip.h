@@ -300,7 +300,8 @@ int cpm_mg_apply(cpm_mg* mg, const double* r, double* z);
  * scalar operator: b and x hold nrhs interleaved columns (entry i*nrhs + j),
  * each with its own scalars and stopping test; iters and resid hold nrhs
  * entries.  delta > 0: the outer products use the FP32 form of the
- * operator, with an FP64 residual replacement whenever the recurrence
+ * operator on FP32 search directions (x and r stay FP64), with an FP64
+ * residual replacement whenever the recurrence
  * residual drops by delta and before stopping (the stopping test is then
  * on the FP64 residual); 0: FP64 products.  Returns 0, or a positive
  * iteration count when maxiter was reached. */
