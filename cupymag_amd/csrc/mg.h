@@ -83,7 +83,9 @@ struct cpm_mg {
     if (l + 1 == L.size()) return csmooth;
     return (l == 0 || !degree1) ? degree : degree1;
   }
-  // FP64 PCG vectors (fine level); level and PCG vectors hold kcap columns
+  // PCG vectors of the fine level, 8 bytes per entry (p and q hold FP32
+  // entries in the mixed-precision solves); level and PCG vectors hold kcap
+  // columns
   double *r = nullptr, *p = nullptr, *q = nullptr;
   int kcap = 1;
   // stream the V-cycle kernels go to (null: the context's stream); the

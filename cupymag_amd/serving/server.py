@@ -48,8 +48,8 @@ from .workloads import initial_m
 _BYTES_PER_DOF = {False: 2200.0 + 170.0, True: 5700.0 + 680.0}
 # FP32 multigrid hierarchies per fine DOF (demag, Gauss-Seidel A1 with three
 # columns, elasticity): coarse operators (1/8 + 1/64 + ... of the fine SELL
-# forms, with their half FP64 copies), level vectors, the FP64 outer CG
-# vectors
+# forms, with their half FP64 copies), level vectors, the outer CG vectors
+# (r, p, q)
 _MG_BYTES_PER_DOF = {False: 500.0 + 20.0, True: 1200.0 + 90.0}
 
 
