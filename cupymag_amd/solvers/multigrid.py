@@ -168,7 +168,9 @@ def _pinv_periodic(A, bs):
 # headline (kappa 131 on level 0, 33 on the 64^3 level, 8.4 on 32^3) this
 # keeps two levels; a CPU probe of the PCG (SciPy, same smoother) took 15
 # iterations with it against 16 with the five levels down to the dense 8^3
-# solve.
+# solve.  On the GPU at 128^3 (20 steps) each A1 solve took 19-20
+# iterations with 4 coarsest steps, 16 with 6 and 15 with 8 (44.8, 42.6 and
+# 42.3 ms per step): 8, the most the library takes.
 A1_KAPPA_MAX = 40.0
 # The elasticity V-cycle's levels below the finest smooth with Chebyshev
 # degree 3 (the finest keeps `degree`): a CPU probe of the periodic
@@ -176,7 +178,7 @@ A1_KAPPA_MAX = 40.0
 # every level (16 with degree 3 on every level, at 2 more level-0 passes
 # per iteration), for about 5% more V-cycle work per iteration.
 EL_COARSE_DEGREE = 3
-A1_CHEB = 6
+A1_CHEB = 8
 A1_CHEB_RATIO = 20.0
 
 

@@ -132,12 +132,13 @@ def test_vcycle_matches_host_reference(name, bs, n, floating):
         mg = S.multigrid(precision=prec, floating=floating, a1_coarse=a1)[name]
         info = mg.info()
         if a1 == "auto":
-            assert info["levels"] == 2 and info["coarsest"] == ("chebyshev", 6, 20.0), info
+            from cupymag_amd.solvers.multigrid import A1_CHEB, A1_CHEB_RATIO
+            assert info["levels"] == 2 and info["coarsest"] == ("chebyshev", A1_CHEB, A1_CHEB_RATIO), info
             mats = [_interleaved(L, bs) for L in mg.levels]
             r = np.random.default_rng(5).standard_normal(mats[0].shape[0])
             z = mg.apply(torch.as_tensor(r, device="cuda")).cpu().numpy()
             zr = _host_vcycle(mats, [(m, m, m) for m in sizes[:2]], info["lmax"], None, bs, mg.degree, mg.ratio, r,
-                              anchor=False, coarse_smooth=(6, 20.0))
+                              anchor=False, coarse_smooth=info["coarsest"][1:])
             assert np.abs(z - zr).max() / np.abs(zr).max() < tol, prec
             continue
         assert info["levels"] == len(sizes) and info["dims"][-1] == (sizes[-1],) * 3
